@@ -1,0 +1,192 @@
+/* swm_mi355x.h -- C ABI of the MI355X (gfx950) dynamical core of the CSU
+ * geodesic-grid shallow-water model.
+ *
+ * The library replaces the reference's hot path, the time step
+ *   step::step_dt, step::wrap_prognostics   (swm/source/step.F:114-212, :438-493)
+ * and the operator / halo / solver modules it calls, behind entry points that
+ * a Fortran ISO_C_BINDING shim binds one for one.  Each declaration names the
+ * reference interface it replaces (file:line, paths relative to the
+ * reference's doe_ca/ directory).
+ *
+ * Conventions
+ *  - Host arrays use the reference's Fortran layout: column-major, first index
+ *    fastest, the 10 rhombic panels ("subdomains") last, a one-cell halo on
+ *    every panel: cell fields (iim,jjm,10), corner scalars (2,iim,jjm,10),
+ *    corner vectors (2,2,iim,jjm,10) = (space, corner, i, j, panel), edge
+ *    fields (3,iim,jjm,10), with iim = jjm = 2 + 2^g.
+ *  - Device data lives in the context; it is laid out as one plane per
+ *    component (structure of arrays) and never crosses PCIe inside a step.
+ *  - Every function returns swm_status; SWM_OK is 0.  The reference has no
+ *    return codes and stops the program on fatal errors (utilities/layer_1/
+ *    abort.F:26-49); a Fortran shim maps a non-zero status to that behaviour.
+ *  - One host thread per context; calls on one context are serialised on its
+ *    stream.  Contexts are independent.
+ */
+#ifndef SWM_MI355X_H
+#define SWM_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int swm_status;
+enum {
+  SWM_OK = 0,
+  SWM_ERR_ARG = 1,            /* bad argument or unknown name */
+  SWM_ERR_DIMS = 2,           /* array extents do not match the grid */
+  SWM_ERR_HIP = 3,            /* HIP runtime error (message in swm_last_error) */
+  SWM_ERR_NOT_CONVERGED = 4,  /* elliptic solve failed: elliptic_solver.F:176-178 */
+  SWM_ERR_NAN = 5,            /* non-finite value detected in the state */
+  SWM_ERR_NO_DEVICE = 6,      /* no usable gfx950 device */
+  SWM_ERR_STATE = 7           /* call out of order (e.g. step before init) */
+};
+
+/* Text of the last error on this thread (never NULL). */
+const char* swm_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Host-side grid set-up (needs no device)
+ * ------------------------------------------------------------------------ */
+typedef struct swm_grid swm_grid;
+
+/* Cell centres of the recursively bisected icosahedron, level g (1..10),
+ * written as (3, 10*4^g+2) in the reference's global order (NP, SP, rows).
+ * Replaces utilities/grid_generation/generate_grid.F:69-130,327-463. */
+swm_status swm_grid_points(int g, double* p_list);
+
+/* Builds the grid: panel index maps, halo tables, masks, metrics, Coriolis
+ * parameter and the multigrid hierarchy.  p_list (3, max_ig) may be NULL to
+ * use swm_grid_points.  radius and omega are the sphere's radius (m) and
+ * rotation rate (1/s).
+ * Replaces initialize_wrap (utilities/layer_2/wrap_data.F:34-157),
+ * set_grid_center_mask / set_grid_corner_mask (utilities/layer_2/grid_mask.F:
+ * 60-333), generate_grid_metrics (utilities/layer_3/grid_metrics.F:55-219),
+ * initialize_multigrid (utilities/layer_3/multigrid_solver.F:61-474) and
+ * set_coriolis_parameter (swm/source/initialize.F:286-321). */
+swm_status swm_grid_create(int g, const double* p_list, double radius, double omega,
+                           swm_grid** out);
+void swm_grid_destroy(swm_grid* grid);
+
+/* Grid extents: level, panel extents, number of cells. */
+swm_status swm_grid_dims(const swm_grid* grid, int* g, int* iim, int* jjm, int* max_ig);
+
+/* Named set-up arrays, copied out in the reference layout.  Names (element
+ * type): "p","c","area","area_inv","corner_area","corner_area_inv",
+ * "sub_area","dc","normal","tangent","v_weights","s_weights","l_weights","f",
+ * "grid_center_mask","grid_corner_mask","polygon_type","tot_area" (double);
+ * "grid_center_index","grid_corner_index" (int32);
+ * halo tables "face_dst","face_src","vrtx_dst","vrtx_src","edge_dst",
+ * "edge_src" (int32, flat (component, slot) indices, source -1 = zero);
+ * multigrid weights "lap_w<l>" for level l = 1..g (double, (7, 2+2^l, 2+2^l, 10)).
+ * swm_grid_array_len returns the element count, or -1 for an unknown name. */
+int64_t swm_grid_array_len(const swm_grid* grid, const char* name);
+swm_status swm_grid_array(const swm_grid* grid, const char* name, void* out);
+
+/* ------------------------------------------------------------------------
+ * Device model (one context = one model state on one GPU)
+ * ------------------------------------------------------------------------ */
+typedef struct swm_ctx swm_ctx;
+
+enum { SWM_FP64 = 0, SWM_MIXED = 1 };
+
+typedef struct swm_config {
+  int ntracer;       /* passive tracers (the reference compiles 5: swm_parameters.F:66) */
+  double dt;         /* time step (s) */
+  double grav;       /* gravity (m/s^2) */
+  int precision;     /* SWM_FP64 or SWM_MIXED */
+  int device;        /* HIP device ordinal */
+} swm_config;
+
+/* Allocates the device state for a grid (the grid may be destroyed after). */
+swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out);
+void swm_destroy(swm_ctx* ctx);
+
+/* Initial state from thickness h and surface height hs (iim,jjm,10), corner
+ * velocity u (2,2,iim,jjm,10) and tracers (iim,jjm,ntracer,10); vorticity,
+ * divergence and kinetic energy are derived from u on the device as the
+ * reference does after its initial conditions (swm/source/initialize.F:
+ * 418-468, utilities/layer_4/operators_hl.F:153-266), then the halos are
+ * filled as by the first wrap_prognostics (swm/source/driver.F:78). */
+swm_status swm_init_state(swm_ctx* ctx, const double* h, const double* hs, const double* u,
+                          const double* tracer);
+
+/* Advances nsteps time steps: for each, the time-level pointers and ABM3
+ * weights (swm/source/model_control.F:28-89), step_dt and wrap_prognostics
+ * (swm/source/step.F:114-212, :438-493; loop of swm/source/driver.F:90-130).
+ * Returns SWM_ERR_NOT_CONVERGED if an elliptic solve fails. */
+swm_status swm_step(swm_ctx* ctx, int nsteps);
+
+/* Copies a field of the current time level to the host (reference layout):
+ * "h","eta","div","relative","ke","psi","chi" (iim,jjm,10); "u"
+ * (2,2,iim,jjm,10); "tracer" (iim,jjm,ntracer,10). */
+swm_status swm_get_field(swm_ctx* ctx, const char* name, double* out);
+
+/* Waits for all work queued on the context's stream. */
+swm_status swm_synchronize(swm_ctx* ctx);
+
+/* Steps taken and V-cycles run by the last elliptic solve. */
+int swm_step_count(const swm_ctx* ctx);
+int swm_last_vcycles(const swm_ctx* ctx);
+
+/* Device time of the last swm_step call and of its hottest kernel, measured
+ * with HIP events on the context's stream (ms), and that kernel's mean
+ * duration and algorithmic bytes per launch. */
+swm_status swm_timing(const swm_ctx* ctx, double* step_ms, double* kernel_ms_mean,
+                      double* kernel_bytes, int* kernel_launches);
+
+/* Compensated (Neumaier) FP64 global mass sum of the current thickness,
+ * sum(h * area) over real cells (swm/source/budgets.F:108-109). */
+swm_status swm_total_mass(swm_ctx* ctx, double* mass);
+
+/* ------------------------------------------------------------------------
+ * Reference operators on all 10 panels of host arrays (copy in, run on the
+ * device, copy out).  Metric arguments are those of the context's grid.
+ * ------------------------------------------------------------------------ */
+/* utilities/layer_2/operators.F:270-392 */
+swm_status swm_op_div_of_an_edge_flux(swm_ctx* ctx, const double* xin, double* xout);
+/* utilities/layer_2/operators.F:402-518 */
+swm_status swm_op_laplacian_of_a_scalar(swm_ctx* ctx, const double* xin, double* xout);
+/* utilities/layer_2/operators.F:47-147 */
+swm_status swm_op_curl_of_a_vector(swm_ctx* ctx, const double* xin, double* xout);
+/* utilities/layer_2/operators.F:155-261 */
+swm_status swm_op_div_of_a_vector(swm_ctx* ctx, const double* xin, double* xout);
+/* utilities/layer_2/operators.F:1316-1393 (ke_c at corners -> ke at centres) */
+swm_status swm_op_calculate_center_kinetic_energy(swm_ctx* ctx, const double* ke_c, double* ke);
+/* utilities/layer_2/operators.F:982-1112 (flag = .true.) */
+swm_status swm_op_average_to_cell_corners(swm_ctx* ctx, const double* xin, double* xout);
+/* utilities/layer_2/operators.F:1402-1514 */
+swm_status swm_op_calculate_velocity(swm_ctx* ctx, const double* psi, const double* chi, double* u);
+/* utilities/layer_2/operators.F:1626-1722 */
+swm_status swm_op_centered_mass_flux(swm_ctx* ctx, const double* mass_c, const double* u,
+                                     double* flux);
+/* utilities/layer_2/operators.F:1731-1832 */
+swm_status swm_op_upstream_mass_flux(swm_ctx* ctx, const double* u, const double* mass,
+                                     double* flux);
+/* utilities/layer_2/operators.F:2000-2074 */
+swm_status swm_op_centered_vorticity_flux(swm_ctx* ctx, const double* mass_flux,
+                                          const double* mass, const double* eta, double* flux);
+/* utilities/layer_2/operators.F:2083-2186 */
+swm_status swm_op_centered_divergence_curl(swm_ctx* ctx, const double* mass_c, const double* eta,
+                                           const double* u, const double* mass, double* flux);
+
+/* Halo fill in place, wrap_data::wrap (utilities/layer_2/wrap_data.F:1790-1974):
+ * kind 0 = face (cell data), 1 = vrtx (corner data, nspace components first),
+ * 2 = edge; nlayer layers of (iim,jjm,10) panels each (face), or
+ * (nspace,2,iim,jjm) resp. (1,3,iim,jjm) per layer. */
+swm_status swm_wrap(swm_ctx* ctx, int kind, int nspace, int nlayer, double* x);
+
+/* elliptic_solver_xxx::elliptic (utilities/layer_3/elliptic_solver.F:40-195):
+ * rhs and xout are (iim,jjm,nvar,10); rhs loses its area-weighted mean, xout
+ * is the warm start and the result. */
+swm_status swm_elliptic(swm_ctx* ctx, int nvar, double* rhs, double* xout);
+
+/* multigrid_solver_xxx::multigrid (utilities/layer_3/multigrid_solver.F:476-591):
+ * one V-cycle on (iim,jjm,nvar,10) arrays. */
+swm_status swm_multigrid(swm_ctx* ctx, int nvar, const double* beta, double* alpha);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SWM_MI355X_H */

@@ -1,0 +1,181 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the oracle (swm_oracle.c).
+
+The oracle is the CPU FP64 restatement of the reference step used to check
+the device path and timed as the CPU baseline; it is never part of the
+shipped library.  Build with ``make -C oracle`` (writes oracle/liboracle.so).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "liboracle.so"], check=True)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    lib.orc_create.restype = vp
+    lib.orc_create.argtypes = [vp]
+    lib.orc_destroy.argtypes = [vp]
+    lib.orc_init.argtypes = [vp, vp, vp, vp]
+    lib.orc_step.restype = ctypes.c_int
+    lib.orc_step.argtypes = [vp, ctypes.c_int]
+    lib.orc_get.restype = ctypes.c_int
+    lib.orc_get.argtypes = [vp, ctypes.c_char_p, vp]
+    lib.orc_last_vcycles.restype = ctypes.c_int
+    lib.orc_last_vcycles.argtypes = [vp]
+    for name in ("orc_div_of_an_edge_flux", "orc_laplacian", "orc_curl_of_a_vector", "orc_div_of_a_vector",
+                 "orc_center_ke", "orc_average_to_corners"):
+        getattr(lib, name).argtypes = [vp, vp, vp]
+    lib.orc_velocity.argtypes = [vp, vp, vp, vp]
+    lib.orc_centered_mass_flux.argtypes = [vp, vp, vp, vp]
+    lib.orc_upstream_mass_flux.argtypes = [vp, vp, vp, vp]
+    lib.orc_vorticity_flux.argtypes = [vp, vp, vp, vp, vp]
+    lib.orc_divergence_curl.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.orc_multigrid.argtypes = [vp, ctypes.c_int, vp, vp]
+    lib.orc_elliptic.restype = ctypes.c_int
+    lib.orc_elliptic.argtypes = [vp, ctypes.c_int, vp, vp]
+    lib.orc_wrap.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+class _OrcGrid(ctypes.Structure):
+    _fields_ = [("g", ctypes.c_int), ("ntracer", ctypes.c_int), ("dt", ctypes.c_double), ("grav", ctypes.c_double)] + [
+        (n, ctypes.c_void_p) for n in ("area", "area_inv", "corner_area_inv", "sub_area", "dc", "normal", "tangent",
+                                       "v_weights", "s_weights", "l_weights", "f", "hs", "center_mask")] + [
+        ("nface", ctypes.c_int), ("nvrtx", ctypes.c_int), ("nedge", ctypes.c_int)] + [
+        (n, ctypes.c_void_p) for n in ("face_dst", "face_src", "vrtx_dst", "vrtx_src", "edge_dst", "edge_src")] + [
+        ("lap_w", ctypes.c_void_p)]
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+GRID_FIELDS = ["area", "area_inv", "corner_area_inv", "sub_area", "dc", "normal", "tangent", "v_weights",
+               "s_weights", "l_weights", "f", "grid_center_mask", "face_dst", "face_src", "vrtx_dst", "vrtx_src",
+               "edge_dst", "edge_src"]
+
+
+def grid_arrays_from(grid):
+    """Set-up arrays of a swm_mi355x.Grid (or any object with .array and .g)."""
+    d = {n: grid.array(n) for n in GRID_FIELDS}
+    for l in range(1, grid.g + 1):
+        d[f"lap_w{l}"] = grid.array(f"lap_w{l}")
+    d["g"] = grid.g
+    return d
+
+
+class Oracle:
+    """One FP64 model state on the CPU; arrays as in the reference layout."""
+
+    def __init__(self, arrays, ntracer, dt, grav, hs=None):
+        L = lib()
+        self.g = int(arrays["g"])
+        self.iim = self.jjm = 2 + 2 ** self.g
+        self.ntracer = ntracer
+        ns = self.iim * self.jjm * 10
+        keep = {}
+
+        def f64(name, a):
+            keep[name] = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1, order="F"))
+            return _p(keep[name])
+
+        def i32(name, a):
+            keep[name] = np.ascontiguousarray(np.asarray(a, dtype=np.int32).reshape(-1))
+            return _p(keep[name])
+
+        gr = _OrcGrid()
+        gr.g, gr.ntracer, gr.dt, gr.grav = self.g, ntracer, dt, grav
+        for n in ("area", "area_inv", "corner_area_inv", "sub_area", "dc", "normal", "tangent", "v_weights",
+                  "s_weights", "l_weights", "f"):
+            setattr(gr, n, f64(n, arrays[n]))
+        gr.center_mask = f64("mask", arrays["grid_center_mask"])
+        gr.hs = f64("hs", np.zeros(ns) if hs is None else hs)
+        for n in ("face", "vrtx", "edge"):
+            setattr(gr, "n" + n, len(arrays[n + "_dst"]))
+            setattr(gr, n + "_dst", i32(n + "_dst", arrays[n + "_dst"]))
+            setattr(gr, n + "_src", i32(n + "_src", arrays[n + "_src"]))
+        ptrs = (ctypes.c_void_p * self.g)()
+        for l in range(1, self.g + 1):
+            ptrs[l - 1] = f64(f"lap_w{l}", arrays[f"lap_w{l}"])
+        gr.lap_w = ctypes.cast(ptrs, ctypes.c_void_p)
+        self._keep, self._ptrs = keep, ptrs
+        self._h = L.orc_create(ctypes.byref(gr))
+
+    def init_state(self, h, u, tracer):
+        h = np.asfortranarray(h, dtype=np.float64)
+        u = np.asfortranarray(u, dtype=np.float64)
+        t = np.asfortranarray(tracer, dtype=np.float64)
+        lib().orc_init(self._h, _p(h), _p(u), _p(t))
+
+    def step(self, n=1):
+        if lib().orc_step(self._h, n) != 0:
+            raise RuntimeError("oracle: elliptic solve did not converge")
+
+    def field(self, name):
+        s = (self.iim, self.jjm, 10)
+        shape = {"u": (2, 2) + s, "tracer": (self.iim, self.jjm, self.ntracer, 10)}.get(name, s)
+        out = np.zeros(shape, order="F")
+        if lib().orc_get(self._h, name.encode(), _p(out)) != 0:
+            raise KeyError(name)
+        return out
+
+    @property
+    def last_vcycles(self):
+        return lib().orc_last_vcycles(self._h)
+
+    def op(self, name, *args, out_shape):
+        out = np.zeros(out_shape, order="F")
+        arrs = [np.asfortranarray(a, dtype=np.float64) for a in args]
+        getattr(lib(), name)(self._h, *[_p(a) for a in arrs], _p(out))
+        return out
+
+    def multigrid(self, beta, alpha):
+        nvar = beta.shape[2]
+        b = np.ascontiguousarray(np.moveaxis(np.asarray(beta, dtype=np.float64), 2, 0))  # (nvar, iim, jjm, 10)
+        a = np.ascontiguousarray(np.moveaxis(np.asarray(alpha, dtype=np.float64), 2, 0))
+        bf = np.stack([x.reshape(-1, order="F") for x in b])
+        af = np.stack([x.reshape(-1, order="F") for x in a])
+        lib().orc_multigrid(self._h, nvar, _p(bf), _p(af))
+        return np.stack([x.reshape((self.iim, self.jjm, 10), order="F") for x in af], axis=2)
+
+    def elliptic(self, rhs, xout):
+        nvar = rhs.shape[2]
+        rf = np.stack([np.asarray(rhs[:, :, v, :], dtype=np.float64).reshape(-1, order="F") for v in range(nvar)])
+        xf = np.stack([np.asarray(xout[:, :, v, :], dtype=np.float64).reshape(-1, order="F") for v in range(nvar)])
+        st = lib().orc_elliptic(self._h, nvar, _p(rf), _p(xf))
+        shp = (self.iim, self.jjm, 10)
+        r = np.stack([x.reshape(shp, order="F") for x in rf], axis=2)
+        x = np.stack([x.reshape(shp, order="F") for x in xf], axis=2)
+        return st, r, x
+
+    def close(self):
+        if self._h:
+            lib().orc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,83 @@
+/* swm_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU FP64 restatement of the reference shallow-water dynamical-core step
+ * (swm/source/step.F step_dt + wrap_prognostics and everything they call),
+ * used as the checker of the device path and as the timed CPU baseline.
+ * Arrays are in the reference's Fortran layout (column-major, 1-cell halo,
+ * 10 panels); time levels are kept as separate arrays.  Nothing in the shipped
+ * library links or calls this code.
+ */
+#ifndef SWM_ORACLE_H
+#define SWM_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_model orc_model;
+
+/* Grid description handed to the oracle (arrays are copied).
+ * Layouts: cell (iim,jjm,10); corner (2,iim,jjm,10); (3,2,iim,jjm,10) for
+ * dc/sub_area/s_weights; (3,3,2,iim,jjm,10) normal/tangent;
+ * (3,2,2,iim,jjm,10) v_weights; (7,iim,jjm,10) l_weights.
+ * Halo tables: flat (component, slot) destination/source pairs, source -1
+ * meaning zero; ncomp 1 (cells), 2 (corners), 3 (edges).
+ * lap_w[l-1]: (7, 2+2^l, 2+2^l, 10) multigrid Laplacian weights of level l. */
+typedef struct {
+  int g, ntracer;
+  double dt, grav;
+  const double *area, *area_inv, *corner_area_inv, *sub_area, *dc, *normal, *tangent,
+      *v_weights, *s_weights, *l_weights, *f, *hs, *center_mask;
+  int nface, nvrtx, nedge;
+  const int32_t *face_dst, *face_src, *vrtx_dst, *vrtx_src, *edge_dst, *edge_src;
+  const double* const* lap_w;
+} orc_grid;
+
+orc_model* orc_create(const orc_grid* grid);
+void orc_destroy(orc_model* m);
+
+/* Initial state from thickness h, surface height hs is part of the grid,
+ * corner velocity u (2,2,iim,jjm,10) and tracers (iim,jjm,ntracer,10); the
+ * derived vorticity, divergence and kinetic energy follow the reference IC
+ * flow (initialize.F:418-468) and the first wrap_prognostics. */
+void orc_init(orc_model* m, const double* h, const double* u, const double* tracer);
+
+/* Advances nsteps steps (time_manager, get_pointers, get_weights, step_dt,
+ * wrap_prognostics).  Returns 0, or 1 if the elliptic solve did not converge. */
+int orc_step(orc_model* m, int nsteps);
+
+/* Current-level fields: "h","eta","div","relative","ke","psi","chi" (cell),
+ * "u" (2,2,iim,jjm,10), "tracer" (iim,jjm,ntracer,10). */
+int orc_get(const orc_model* m, const char* name, double* out);
+
+/* Number of V-cycles run by the last elliptic solve (2 + extra). */
+int orc_last_vcycles(const orc_model* m);
+
+/* Single operators on all 10 panels (reference argument meaning). */
+void orc_div_of_an_edge_flux(const orc_model* m, const double* xin, double* xout);
+void orc_laplacian(const orc_model* m, const double* xin, double* xout);
+void orc_curl_of_a_vector(const orc_model* m, const double* xin, double* xout);
+void orc_div_of_a_vector(const orc_model* m, const double* xin, double* xout);
+void orc_center_ke(const orc_model* m, const double* ke_c, double* ke);
+void orc_average_to_corners(const orc_model* m, const double* xin, double* xout);
+void orc_velocity(const orc_model* m, const double* psi, const double* chi, double* u);
+void orc_centered_mass_flux(const orc_model* m, const double* mass_c, const double* u,
+                            double* flux);
+void orc_upstream_mass_flux(const orc_model* m, const double* u, const double* mass,
+                            double* flux);
+void orc_vorticity_flux(const orc_model* m, const double* mass_flux, const double* mass,
+                        const double* eta, double* flux);
+void orc_divergence_curl(const orc_model* m, const double* mass_c, const double* eta,
+                         const double* u, const double* mass, double* flux);
+/* One multigrid V-cycle and one full elliptic solve on (iim,jjm,nvar,10). */
+void orc_multigrid(orc_model* m, int nvar, const double* beta, double* alpha);
+int orc_elliptic(orc_model* m, int nvar, double* rhs, double* xout);
+/* Halo fills in place: kind 0 face (cell), 1 corner (nspace components
+ * interleaved first), 2 edge. */
+void orc_wrap(const orc_model* m, int kind, int nspace, double* x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

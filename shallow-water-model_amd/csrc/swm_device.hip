@@ -1,0 +1,633 @@
+// swm_device.hip -- device model context, time step driver and the C ABI of
+// the device path.  The kernels live in swm_kernels.hip (same translation
+// unit, included below).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/swm_mi355x.h"
+#include "swm_internal.h"
+#include "swm_kernels.h"
+#include "swm_kernels.hip"
+
+namespace {
+
+using swm::dev::FieldSet;
+using swm::dev::Geo;
+using swm::dev::Lev;
+
+constexpr int BLK = 256;
+
+#define HIP_TRY(expr)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      swm::set_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr);    \
+      return SWM_ERR_HIP;                                                                   \
+    }                                                                                       \
+  } while (0)
+
+inline unsigned nblk(size_t n) { return (unsigned)((n + BLK - 1) / BLK); }
+
+struct DevTable {
+  int n = 0;
+  int* dst = nullptr;
+  int* src = nullptr;
+};
+
+// one 256-byte aligned device arena for all buffers of a context
+struct Arena {
+  char* base = nullptr;
+  size_t used = 0, cap = 0;
+  template <class T>
+  T* take(size_t n) {
+    const size_t bytes = ((n * sizeof(T) + 255) / 256) * 256;
+    T* p = reinterpret_cast<T*>(base + used);
+    used += bytes;
+    return p;
+  }
+};
+
+}  // namespace
+
+struct swm_ctx {
+  int g = 0, iim = 0, jjm = 0, ns = 0, nt = 0, max_ig = 0;
+  double dt = 0, grav = 0;
+  int precision = SWM_FP64, device = 0;
+  hipStream_t stream = nullptr;
+  Arena arena;
+  size_t arena_bytes = 0;
+  // metrics
+  double *area, *area_inv, *cai, *sub_area, *dc, *nrm, *tng, *vw, *sw, *lw, *f, *hs, *mask, *area_mask;
+  double total_area = 0.0;
+  Geo geo;
+  DevTable face, vrtx, edge;
+  struct Level {
+    int iim, jjm, ns;
+    double *lw, *lwt, *alpha, *beta, *temp;
+    int* src;
+    Lev L;
+  };
+  std::vector<Level> lev;
+  // state
+  double *h[2], *eta[2], *div[2], *u[2];
+  std::vector<double*> tr[2], told;
+  double *eta_f[3], *div_f[3], *msh_f[3];
+  double *relative, *ke, *xout, *inv_h, *mass_flux;
+  // scratch
+  double *hc, *etac, *flx1, *anti, *mss_low, *beta, *f1, *f2, *ml, *bdn, *bup, *vf, *dcurl, *rhs, *r2;
+  double *part, *sums;
+  double* host_sums = nullptr;   // pinned
+  int npart = 0;
+  // time stepping
+  int step_count = 0, n3 = 1, n4 = 2, n1f = 1, n2f = 2, n3f = 3;
+  double w1 = 0, w2 = 0, w3 = 0;
+  int last_vcycles = 0;
+  bool initialized = false;
+  // timing
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  double last_step_ms = 0.0;
+  std::vector<hipEvent_t> kev;   // pairs around the profiled kernel
+  size_t kev_used = 0;
+  bool profile = false;
+  double kernel_bytes = 0.0;
+};
+
+namespace {
+
+swm_status upload(double* d, const std::vector<double>& h, hipStream_t s) {
+  HIP_TRY(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  return SWM_OK;
+}
+
+// reference layout (ncomp, slots) -> planes (ncomp, slots) with plane index
+// remapped by perm(component)
+std::vector<double> to_planes(const std::vector<double>& a, int ncomp, int ns, const std::vector<int>& perm,
+                              int nplanes) {
+  std::vector<double> out((size_t)nplanes * ns, 0.0);
+  for (int s = 0; s < ns; ++s)
+    for (int c = 0; c < ncomp; ++c) {
+      const int p = perm[c];
+      if (p >= 0) out[(size_t)p * ns + s] = a[(size_t)ncomp * s + c];
+    }
+  return out;
+}
+
+swm_status make_table(swm_ctx* C, const swm::HaloTable& t, int ns, DevTable& d) {
+  // table entries are (component, slot) with the component fastest; the
+  // device index is component*ns + slot
+  std::vector<int> dst(t.dst.size()), src(t.src.size());
+  for (size_t k = 0; k < t.dst.size(); ++k) {
+    dst[k] = (t.dst[k] % t.ncomp) * ns + t.dst[k] / t.ncomp;
+    src[k] = t.src[k] < 0 ? -1 : (t.src[k] % t.ncomp) * ns + t.src[k] / t.ncomp;
+  }
+  d.n = (int)dst.size();
+  d.dst = C->arena.take<int>(dst.size() + 1);
+  d.src = C->arena.take<int>(src.size() + 1);
+  if (d.n) {
+    HIP_TRY(hipMemcpyAsync(d.dst, dst.data(), dst.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
+    HIP_TRY(hipMemcpyAsync(d.src, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
+  }
+  return SWM_OK;
+}
+
+void gather(swm_ctx* C, const DevTable& t, std::initializer_list<double*> fields) {
+  FieldSet fs{};
+  int nf = 0;
+  for (double* p : fields) {
+    fs.p[nf++] = p;
+    if (nf == 8) {
+      hipLaunchKernelGGL(swm::dev::k_gather, dim3(nblk(t.n), nf), dim3(BLK), 0, C->stream, t.dst, t.src, t.n, fs);
+      nf = 0;
+    }
+  }
+  if (nf && t.n)
+    hipLaunchKernelGGL(swm::dev::k_gather, dim3(nblk(t.n), nf), dim3(BLK), 0, C->stream, t.dst, t.src, t.n, fs);
+}
+void gather_list(swm_ctx* C, const DevTable& t, const std::vector<double*>& fields) {
+  for (size_t a = 0; a < fields.size(); a += 8) {
+    FieldSet fs{};
+    int nf = 0;
+    for (size_t b = a; b < fields.size() && b < a + 8; ++b) fs.p[nf++] = fields[b];
+    if (t.n) hipLaunchKernelGGL(swm::dev::k_gather, dim3(nblk(t.n), nf), dim3(BLK), 0, C->stream, t.dst, t.src, t.n, fs);
+  }
+}
+
+void fill(swm_ctx* C, double* x, size_t n, double v) {
+  hipLaunchKernelGGL(swm::dev::k_fill, dim3(nblk(n)), dim3(BLK), 0, C->stream, x, n, v);
+}
+
+// ------------------------------------------------------------- multigrid --
+void relax(swm_ctx* C, int l, int iters, double* alpha, const double* beta, int nvar) {
+  swm_ctx::Level& L = C->lev[l - 1];
+  double* a = alpha;
+  double* t = L.temp;
+  for (int it = 0; it < iters; ++it) {
+    const bool prof = C->profile && l == C->g && C->kev_used + 2 <= C->kev.size();
+    if (prof) hipEventRecord(C->kev[C->kev_used++], C->stream);
+    hipLaunchKernelGGL(swm::dev::k_relax, dim3(nblk(L.ns), nvar), dim3(BLK), 0, C->stream, L.L, beta, a, t);
+    if (prof) hipEventRecord(C->kev[C->kev_used++], C->stream);
+    std::swap(a, t);
+  }
+  // an even sweep count leaves the result in alpha
+}
+
+void vcycle(swm_ctx* C, int nvar, double* alpha, const double* beta) {
+  const int G = C->g;
+  for (int l = 1; l < G; ++l) {
+    fill(C, C->lev[l - 1].alpha, (size_t)nvar * C->lev[l - 1].ns, 0.0);
+    fill(C, C->lev[l - 1].beta, (size_t)nvar * C->lev[l - 1].ns, 0.0);
+  }
+  auto A = [&](int l) { return l == G ? alpha : C->lev[l - 1].alpha; };
+  auto B = [&](int l) { return l == G ? const_cast<double*>(beta) : C->lev[l - 1].beta; };
+  for (int l = G; l >= 2; --l) {
+    relax(C, l, 2, A(l), B(l), nvar);
+    hipLaunchKernelGGL(swm::dev::k_inject, dim3(nblk(C->lev[l - 2].ns), nvar), dim3(BLK), 0, C->stream,
+                       C->lev[l - 1].L, C->lev[l - 2].L, A(l), B(l), B(l - 1));
+  }
+  relax(C, 1, 24, A(1), B(1), nvar);
+  for (int l = 1; l <= G - 1; ++l) {
+    hipLaunchKernelGGL(swm::dev::k_interp, dim3(nblk(C->lev[l].ns), nvar), dim3(BLK), 0, C->stream,
+                       C->lev[l - 1].L, C->lev[l].L, A(l), A(l + 1));
+    relax(C, l + 1, 2, A(l + 1), B(l + 1), nvar);
+  }
+}
+
+// sums of x0*w and x1*w into C->sums[0..1] (Neumaier over block partials)
+void sum2(swm_ctx* C, const double* x0, const double* x1, const double* w, int n) {
+  hipLaunchKernelGGL(swm::dev::k_sum_partial, dim3(C->npart, 2), dim3(BLK), 0, C->stream, x0, x1, w, n, C->part);
+  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(1), dim3(2), 0, C->stream, C->part, C->npart, C->sums);
+}
+
+// elliptic_solver.F:40-195 on the two fields rhs / xout of the context
+swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
+  const int ns = C->ns;
+  for (int v = 0; v < nvar; ++v) gather(C, C->face, {xout + (size_t)v * ns, rhs + (size_t)v * ns});
+  sum2(C, rhs, nvar > 1 ? rhs + ns : rhs, C->area_mask, ns);
+  hipLaunchKernelGGL(swm::dev::k_sub_mean, dim3(nblk(ns)), dim3(BLK), 0, C->stream, ns, C->sums, C->total_area,
+                     rhs, nvar > 1 ? rhs + ns : C->r2);
+  for (int it = 0; it < 2; ++it) vcycle(C, nvar, xout, rhs);
+  for (int v = 0; v < nvar; ++v) gather(C, C->face, {xout + (size_t)v * ns});
+  auto norms = [&](const std::vector<int>& vars, double* out) -> swm_status {
+    for (size_t b = 0; b < vars.size(); ++b) {
+      const int v = vars[b];
+      hipLaunchKernelGGL(swm::dev::k_resid2, dim3(nblk(ns)), dim3(BLK), 0, C->stream, C->geo,
+                         xout + (size_t)v * ns, rhs + (size_t)v * ns, C->r2);
+      hipLaunchKernelGGL(swm::dev::k_sum_partial, dim3(C->npart, 1), dim3(BLK), 0, C->stream, C->r2, C->r2,
+                         (const double*)nullptr, ns, C->part);
+      hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(1), dim3(1), 0, C->stream, C->part, C->npart, C->sums + b);
+    }
+    HIP_TRY(hipMemcpyAsync(C->host_sums, C->sums, vars.size() * sizeof(double), hipMemcpyDeviceToHost, C->stream));
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    for (size_t b = 0; b < vars.size(); ++b) out[b] = C->host_sums[b] / (double)C->max_ig;
+    return SWM_OK;
+  };
+  std::vector<int> all;
+  for (int v = 0; v < nvar; ++v) all.push_back(v);
+  double nrm[2] = {0, 0};
+  swm_status st = norms(all, nrm);
+  if (st) return st;
+  std::vector<int> bad;
+  for (int v = 0; v < nvar; ++v)
+    if (nrm[v] > 1.0e-20) bad.push_back(v);
+  int iter = 0;
+  bool any = !bad.empty();
+  while (any && iter < 10) {
+    ++iter;
+    if (bad.size() == 2) vcycle(C, 2, xout, rhs);
+    else vcycle(C, 1, xout + (size_t)bad[0] * ns, rhs + (size_t)bad[0] * ns);
+    for (int v : bad) gather(C, C->face, {xout + (size_t)v * ns});
+    double nb[2];
+    st = norms(bad, nb);
+    if (st) return st;
+    any = false;
+    for (size_t b = 0; b < bad.size(); ++b)
+      if (nb[b] > 1.0e-20) any = true;
+  }
+  C->last_vcycles = 2 + iter;
+  if (any) {
+    swm::set_error("elliptic solve did not converge (mean squared residual above 1e-20 after 12 V-cycles)");
+    return SWM_ERR_NOT_CONVERGED;
+  }
+  return SWM_OK;
+}
+
+void pointers(swm_ctx* C) {   // model_control.F:28-48
+  const int sc = C->step_count;
+  C->n4 = (sc + 1) % 2 + 1; C->n3 = sc % 2 + 1;
+  C->n3f = (sc + 2) % 3 + 1; C->n2f = (sc + 1) % 3 + 1; C->n1f = sc % 3 + 1;
+}
+void weights(swm_ctx* C) {    // model_control.F:58-89
+  const int sc = C->step_count;
+  if (sc <= 1) { C->w3 = 1.0; C->w2 = 0.0; C->w1 = 0.0; }
+  if (sc == 2) { C->w3 = 1.5; C->w2 = -0.5; C->w1 = 0.0; }
+  if (sc >= 3) { C->w3 = 1.916666666666666666666; C->w2 = -1.333333333333333333333; C->w1 = 0.416666666666666666666; }
+}
+
+void wrap_prognostics(swm_ctx* C) {   // step.F:438-493
+  const int n4 = C->n4 - 1;
+  const int ns = C->ns;
+  hipLaunchKernelGGL(swm::dev::k_inv_h, dim3(nblk(ns)), dim3(BLK), 0, C->stream, C->geo, C->h[n4], C->inv_h);
+  std::vector<double*> f = {C->h[n4], C->eta[n4], C->div[n4], C->relative, C->ke, C->inv_h};
+  for (int q = 0; q < C->nt; ++q) f.push_back(C->tr[n4][q]);
+  gather_list(C, C->face, f);
+  gather(C, C->vrtx, {C->u[0], C->u[0] + 2 * (size_t)ns, C->u[1], C->u[1] + 2 * (size_t)ns});
+}
+
+// step.F:114-212 step_dt (FCT scheme, get_fct_flux step.F:611-678)
+swm_status step_dt(swm_ctx* C) {
+  const int ns = C->ns;
+  const unsigned nb = nblk(ns);
+  hipStream_t S = C->stream;
+  const Geo& G = C->geo;
+  const int n3 = C->n3 - 1, n4 = C->n4 - 1;
+  const double dt = C->dt;
+  double* m3 = C->msh_f[C->n3f - 1];
+  double* m2 = C->msh_f[C->n2f - 1];
+  double* m1 = C->msh_f[C->n1f - 1];
+
+  hipLaunchKernelGGL(swm::dev::k_avg_corners, dim3(nb), dim3(BLK), 0, S, G, C->sw, C->h[n3], C->eta[n3], C->hc, C->etac);
+  gather(C, C->vrtx, {C->hc, C->etac});
+  // mass FCT (mass_fct.F:23-539)
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb), dim3(BLK), 0, S, G, C->u[n3], C->h[n3], C->hc, C->flx1, m3);
+  gather(C, C->edge, {C->flx1, m3});
+  hipLaunchKernelGGL(swm::dev::k_mass_low_beta, dim3(nb), dim3(BLK), 0, S, G, dt, C->w3, C->w2, C->w1, C->flx1, m3, m2,
+                     m1, C->h[n3], C->anti, C->mss_low, C->beta);
+  gather(C, C->face, {C->beta});
+  hipLaunchKernelGGL(swm::dev::k_mass_anti, dim3(nb), dim3(BLK), 0, S, G, C->beta, C->anti);
+  gather(C, C->edge, {C->anti});
+  hipLaunchKernelGGL(swm::dev::k_mass_final, dim3(nb), dim3(BLK), 0, S, G, dt, C->flx1, C->anti, C->mss_low,
+                     C->mass_flux, C->h[n4]);
+  // the tracer limiter fills the halo of the new mass in place (tracer_fct.F:344)
+  gather(C, C->face, {C->h[n4]});
+  for (int q = 0; q < C->nt; ++q) {   // tracer FCT (tracer_fct.F:22-670)
+    double* t3 = C->tr[n3][q];
+    double* t4 = C->tr[n4][q];
+    hipLaunchKernelGGL(swm::dev::k_tr_fluxes, dim3(nb), dim3(BLK), 0, S, G, C->w3, C->w2, C->w1, t3, t4, C->told[q],
+                       C->mass_flux, C->f1, C->f2);
+    gather(C, C->edge, {C->f1, C->f2});
+    hipLaunchKernelGGL(swm::dev::k_tr_low, dim3(nb), dim3(BLK), 0, S, G, dt, C->h[n3], t3, C->f1, C->ml);
+    gather(C, C->face, {C->ml});
+    hipLaunchKernelGGL(swm::dev::k_tr_limiter, dim3(nb), dim3(BLK), 0, S, G, dt, t3, C->ml, C->h[n4], C->mss_low,
+                       C->f1, C->f2, C->anti, C->bdn, C->bup);
+    gather(C, C->face, {C->bdn, C->bup});
+    hipLaunchKernelGGL(swm::dev::k_tr_anti, dim3(nb), dim3(BLK), 0, S, G, C->bdn, C->bup, C->anti);
+    gather(C, C->edge, {C->anti});
+    hipLaunchKernelGGL(swm::dev::k_tr_final, dim3(nb), dim3(BLK), 0, S, G, dt, C->ml, C->anti, C->h[n4], t4, C->told[q]);
+  }
+  // vorticity and divergence (step.F:143-163, :336-429)
+  hipLaunchKernelGGL(swm::dev::k_vort_curl, dim3(nb), dim3(BLK), 0, S, G, C->mass_flux, C->h[n3], C->eta[n3], C->hc,
+                     C->u[n3], C->vf, C->dcurl);
+  gather(C, C->edge, {C->vf, C->dcurl});
+  hipLaunchKernelGGL(swm::dev::k_tendencies, dim3(nb), dim3(BLK), 0, S, G, dt, C->grav, C->w3, C->w2, C->w1, C->vf,
+                     C->dcurl, C->ke, C->h[n3], C->eta[n3], C->div[n3], C->eta_f[C->n3f - 1], C->eta_f[C->n2f - 1],
+                     C->eta_f[C->n1f - 1], C->div_f[C->n3f - 1], C->div_f[C->n2f - 1], C->div_f[C->n1f - 1],
+                     C->eta[n4], C->div[n4], C->relative, C->rhs, C->rhs + ns);
+  // streamfunction and velocity potential (step.F:170-184)
+  swm_status st = elliptic(C, C->rhs, C->xout, 2);
+  if (st) return st;
+  hipLaunchKernelGGL(swm::dev::k_post_elliptic, dim3(nb), dim3(BLK), 0, S, G, C->rhs, C->eta[n4]);
+  hipLaunchKernelGGL(swm::dev::k_velocity, dim3(nb), dim3(BLK), 0, S, G, C->xout, C->xout + ns, C->u[n4]);
+  gather(C, C->vrtx, {C->u[n4], C->u[n4] + 2 * (size_t)ns});
+  hipLaunchKernelGGL(swm::dev::k_center_ke, dim3(nb), dim3(BLK), 0, S, G, C->u[n4], C->ke);
+  gather(C, C->face, {C->ke});
+  return SWM_OK;
+}
+
+// reference layout <-> planes for corner vectors (space, corner, slot)
+void u_to_planes(const double* ref, double* planes, int ns) {
+  for (int s = 0; s < ns; ++s)
+    for (int ic = 0; ic < 2; ++ic)
+      for (int sp = 0; sp < 2; ++sp) planes[(size_t)(ic + 2 * sp) * ns + s] = ref[sp + 2 * (ic + 2 * (size_t)s)];
+}
+void u_from_planes(const double* planes, double* ref, int ns) {
+  for (int s = 0; s < ns; ++s)
+    for (int ic = 0; ic < 2; ++ic)
+      for (int sp = 0; sp < 2; ++sp) ref[sp + 2 * (ic + 2 * (size_t)s)] = planes[(size_t)(ic + 2 * sp) * ns + s];
+}
+
+}  // namespace
+
+extern "C" {
+
+swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out) {
+  if (!grid || !cfg || !out || cfg->ntracer < 0 || cfg->ntracer > 64 || !(cfg->dt > 0.0)) {
+    swm::set_error("swm_create: bad argument");
+    return SWM_ERR_ARG;
+  }
+  if (cfg->precision != SWM_FP64) {
+    swm::set_error("swm_create: only SWM_FP64 is available in this build");
+    return SWM_ERR_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device) {
+    swm::set_error("swm_create: no HIP device");
+    return SWM_ERR_NO_DEVICE;
+  }
+  const swm::Grid& Gh = swm::grid_of(grid);
+  swm_ctx* C = new swm_ctx;
+  C->g = Gh.g; C->iim = Gh.iim; C->jjm = Gh.jjm; C->ns = Gh.iim * Gh.jjm * 10; C->nt = cfg->ntracer;
+  C->max_ig = Gh.max_ig;
+  C->dt = cfg->dt; C->grav = cfg->grav; C->precision = cfg->precision; C->device = cfg->device;
+  HIP_TRY(hipSetDevice(C->device));
+  HIP_TRY(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&C->ev_a));
+  HIP_TRY(hipEventCreate(&C->ev_b));
+  const size_t ns = C->ns;
+  // size of the arena
+  size_t words = 0;
+  auto W = [&](size_t n) { words += ((n * 8 + 255) / 256) * 32; };
+  const int planes_metrics = 1 + 1 + 2 + 6 + 6 + 12 + 12 + 12 + 6 + 7 + 1 + 1 + 1 + 1;
+  W(ns); W(ns); W(2 * ns); W(6 * ns); W(6 * ns); W(12 * ns); W(12 * ns); W(12 * ns); W(6 * ns); W(7 * ns);
+  W(ns); W(ns); W(ns); W(ns);
+  (void)planes_metrics;
+  for (int l = 1; l <= C->g; ++l) {
+    const size_t nl = (size_t)(2 + (1 << l)) * (2 + (1 << l)) * 10;
+    W(7 * nl); W(nl); W(2 * nl); W(2 * nl); W(2 * nl); words += ((nl * 4 + 255) / 256) * 32;
+  }
+  const size_t state = 2 * (3 + 4) + 2 * C->nt + C->nt + 3 + 3 + 9 + 1 + 1 + 2 + 1 + 3;
+  for (size_t k = 0; k < state; ++k) W(ns);
+  const size_t scratch = 2 + 2 + 3 + 3 + 1 + 1 + 3 + 3 + 1 + 1 + 1 + 3 + 3 + 2 + 1;
+  for (size_t k = 0; k < scratch; ++k) W(ns);
+  C->npart = 1024;
+  W(2 * C->npart); W(64);
+  const size_t tabs = Gh.face.dst.size() + Gh.vrtx.dst.size() + Gh.edge.dst.size() + 16;
+  words += (2 * tabs * 4 + 6 * 256) / 8 + 64;
+  C->arena_bytes = words * 8 + (1 << 20);
+  HIP_TRY(hipMalloc(&C->arena.base, C->arena_bytes));
+  C->arena.cap = C->arena_bytes;
+  HIP_TRY(hipHostMalloc(&C->host_sums, 64 * sizeof(double), hipHostMallocDefault));
+  Arena& A = C->arena;
+  auto take = [&](size_t n) { return A.take<double>(n); };
+
+  // metrics as planes
+  auto id = [](int n) { std::vector<int> v(n); for (int k = 0; k < n; ++k) v[k] = k; return v; };
+  C->area = take(ns); C->area_inv = take(ns); C->cai = take(2 * ns); C->sub_area = take(6 * ns); C->dc = take(6 * ns);
+  C->nrm = take(12 * ns); C->tng = take(12 * ns); C->vw = take(12 * ns); C->sw = take(6 * ns); C->lw = take(7 * ns);
+  C->f = take(ns); C->hs = take(ns); C->mask = take(ns); C->area_mask = take(ns);
+  swm_status st;
+  if ((st = upload(C->area, Gh.area, C->stream))) return st;
+  if ((st = upload(C->area_inv, Gh.area_inv, C->stream))) return st;
+  std::vector<std::vector<double>> hold;   // keep host buffers alive until the copies finish
+  auto up = [&](double* d, std::vector<double> v) {
+    hold.push_back(std::move(v));
+    return upload(d, hold.back(), C->stream);
+  };
+  if ((st = up(C->cai, to_planes(Gh.corner_area_inv, 2, ns, id(2), 2)))) return st;
+  if ((st = up(C->sub_area, to_planes(Gh.sub_area, 6, ns, id(6), 6)))) return st;
+  if ((st = up(C->dc, to_planes(Gh.dc, 6, ns, id(6), 6)))) return st;
+  if ((st = up(C->sw, to_planes(Gh.s_weights, 6, ns, id(6), 6)))) return st;
+  if ((st = up(C->vw, to_planes(Gh.v_weights, 12, ns, id(12), 12)))) return st;
+  if ((st = up(C->lw, to_planes(Gh.l_weights, 7, ns, id(7), 7)))) return st;
+  {
+    // normal/tangent (kk, a, ic) with a = 0..2: keep a = 0, 1
+    std::vector<int> perm(18, -1);
+    for (int ic = 0; ic < 2; ++ic)
+      for (int a = 0; a < 2; ++a)
+        for (int kk = 0; kk < 3; ++kk) perm[kk + 3 * (a + 3 * ic)] = kk + 3 * (a + 2 * ic);
+    if ((st = up(C->nrm, to_planes(Gh.normal, 18, ns, perm, 12)))) return st;
+    if ((st = up(C->tng, to_planes(Gh.tangent, 18, ns, perm, 12)))) return st;
+  }
+  if ((st = upload(C->f, Gh.f, C->stream))) return st;
+  HIP_TRY(hipMemsetAsync(C->hs, 0, ns * sizeof(double), C->stream));
+  if ((st = upload(C->mask, Gh.center_mask, C->stream))) return st;
+  {
+    std::vector<double> am(ns);
+    double tot = 0.0;
+    for (size_t n = 0; n < ns; ++n) { am[n] = Gh.area[n] * Gh.center_mask[n]; tot = tot + am[n] * 1.0; }
+    C->total_area = tot;
+    if ((st = up(C->area_mask, am))) return st;
+  }
+  C->geo = Geo{C->iim, C->jjm, C->ns, C->area, C->area_inv, C->cai, C->sub_area, C->dc, C->nrm, C->tng,
+               C->vw, C->sw, C->lw, C->f, C->hs, C->mask};
+  if ((st = make_table(C, Gh.face, C->ns, C->face))) return st;
+  if ((st = make_table(C, Gh.vrtx, C->ns, C->vrtx))) return st;
+  if ((st = make_table(C, Gh.edge, C->ns, C->edge))) return st;
+  // multigrid levels
+  C->lev.resize(C->g);
+  std::vector<std::vector<int>> hold_i;
+  for (int l = 1; l <= C->g; ++l) {
+    swm_ctx::Level& L = C->lev[l - 1];
+    L.iim = L.jjm = 2 + (1 << l);
+    L.ns = L.iim * L.jjm * 10;
+    const size_t nl = L.ns;
+    L.lw = take(7 * nl); L.lwt = take(nl); L.alpha = take(2 * nl); L.beta = take(2 * nl); L.temp = take(2 * nl);
+    L.src = A.take<int>(nl);
+    const std::vector<double>& lw = Gh.levels[l - 1].lap_w;
+    if ((st = up(L.lw, to_planes(lw, 7, (int)nl, id(7), 7)))) return st;
+    std::vector<double> lwt(nl);
+    for (size_t n = 0; n < nl; ++n) lwt[n] = lw[6 + 7 * n] != 0.0 ? 1.0 / (lw[6 + 7 * n] - 0.0) : 0.0;
+    if ((st = up(L.lwt, lwt))) return st;
+    swm::HaloTable t = swm::build_simple_face_table(l);
+    std::vector<int> src(nl);
+    for (size_t n = 0; n < nl; ++n) src[n] = (int)n;
+    for (size_t k = 0; k < t.dst.size(); ++k) src[t.dst[k]] = t.src[k];
+    hold_i.push_back(std::move(src));
+    HIP_TRY(hipMemcpyAsync(L.src, hold_i.back().data(), nl * sizeof(int), hipMemcpyHostToDevice, C->stream));
+    L.L = Lev{L.iim, L.jjm, L.ns, L.lw, L.lwt, L.src};
+  }
+  // state and scratch
+  for (int t = 0; t < 2; ++t) {
+    C->h[t] = take(ns); C->eta[t] = take(ns); C->div[t] = take(ns); C->u[t] = take(4 * ns);
+    for (int q = 0; q < C->nt; ++q) C->tr[t].push_back(take(ns));
+  }
+  for (int q = 0; q < C->nt; ++q) C->told.push_back(take(ns));
+  for (int t = 0; t < 3; ++t) { C->eta_f[t] = take(ns); C->div_f[t] = take(ns); C->msh_f[t] = take(3 * ns); }
+  C->relative = take(ns); C->ke = take(ns); C->xout = take(2 * ns); C->inv_h = take(ns); C->mass_flux = take(3 * ns);
+  C->hc = take(2 * ns); C->etac = take(2 * ns); C->flx1 = take(3 * ns); C->anti = take(3 * ns); C->mss_low = take(ns);
+  C->beta = take(ns); C->f1 = take(3 * ns); C->f2 = take(3 * ns); C->ml = take(ns); C->bdn = take(ns); C->bup = take(ns);
+  C->vf = take(3 * ns); C->dcurl = take(3 * ns); C->rhs = take(2 * ns); C->r2 = take(ns);
+  C->part = take(2 * (size_t)C->npart); C->sums = take(64);
+  if (A.used > A.cap) {
+    swm::set_error("swm_create: arena overflow");
+    return SWM_ERR_HIP;
+  }
+  // zero every state buffer (the reference initialises its module arrays to 0)
+  fill(C, reinterpret_cast<double*>(C->h[0]), (A.base + A.used - reinterpret_cast<char*>(C->h[0])) / 8, 0.0);
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  *out = C;
+  return SWM_OK;
+}
+
+void swm_destroy(swm_ctx* C) {
+  if (!C) return;
+  hipSetDevice(C->device);
+  if (C->stream) hipStreamSynchronize(C->stream);
+  for (hipEvent_t e : C->kev) hipEventDestroy(e);
+  if (C->ev_a) hipEventDestroy(C->ev_a);
+  if (C->ev_b) hipEventDestroy(C->ev_b);
+  if (C->arena.base) hipFree(C->arena.base);
+  if (C->host_sums) hipHostFree(C->host_sums);
+  if (C->stream) hipStreamDestroy(C->stream);
+  delete C;
+}
+
+swm_status swm_init_state(swm_ctx* C, const double* h, const double* hs, const double* u, const double* tracer) {
+  if (!C || !h || !u || (C->nt > 0 && !tracer)) { swm::set_error("swm_init_state: bad argument"); return SWM_ERR_ARG; }
+  HIP_TRY(hipSetDevice(C->device));
+  const int ns = C->ns;
+  hipStream_t S = C->stream;
+  const unsigned nb = nblk(ns);
+  std::vector<double> up(4 * (size_t)ns), tq((size_t)ns);
+  u_to_planes(u, up.data(), ns);
+  // fresh state: zero everything the reference zeroes at start-up
+  fill(C, C->h[0], (C->arena.base + C->arena.used - reinterpret_cast<char*>(C->h[0])) / 8, 0.0);
+  C->step_count = 0;
+  pointers(C);
+  for (int t = 0; t < 2; ++t) {
+    HIP_TRY(hipMemcpyAsync(C->h[t], h, ns * sizeof(double), hipMemcpyHostToDevice, S));
+    HIP_TRY(hipMemcpyAsync(C->u[t], up.data(), 4 * (size_t)ns * sizeof(double), hipMemcpyHostToDevice, S));
+    gather(C, C->face, {C->h[t]});
+    gather(C, C->vrtx, {C->u[t], C->u[t] + 2 * (size_t)ns});
+  }
+  if (hs) HIP_TRY(hipMemcpyAsync(C->hs, hs, ns * sizeof(double), hipMemcpyHostToDevice, S));
+  else fill(C, C->hs, ns, 0.0);
+  gather(C, C->face, {C->hs});
+  // calculate_rel_div_and_ke (operators_hl.F:153-266)
+  gather(C, C->vrtx, {C->u[0], C->u[0] + 2 * (size_t)ns});
+  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb), dim3(BLK), 0, S, C->geo, C->u[0], C->relative, C->div[0]);
+  hipLaunchKernelGGL(swm::dev::k_center_ke, dim3(nb), dim3(BLK), 0, S, C->geo, C->u[0], C->ke);
+  gather(C, C->face, {C->relative, C->div[0], C->ke});
+  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, C->div[1], C->div[0], (size_t)ns);
+  hipLaunchKernelGGL(swm::dev::k_post_elliptic, dim3(nb), dim3(BLK), 0, S, C->geo, C->relative, C->eta[0]);
+  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, C->eta[1], C->eta[0], (size_t)ns);
+  for (int q = 0; q < C->nt; ++q) {
+    for (int k = 0; k < 10; ++k)
+      std::memcpy(&tq[(size_t)k * C->iim * C->jjm], tracer + ((size_t)q + (size_t)C->nt * k) * C->iim * C->jjm,
+                  (size_t)C->iim * C->jjm * sizeof(double));
+    HIP_TRY(hipMemcpyAsync(C->tr[0][q], tq.data(), ns * sizeof(double), hipMemcpyHostToDevice, S));
+    HIP_TRY(hipStreamSynchronize(S));   // tq is reused
+    hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, C->tr[1][q], C->tr[0][q], (size_t)ns);
+  }
+  wrap_prognostics(C);
+  HIP_TRY(hipStreamSynchronize(S));
+  C->initialized = true;
+  return SWM_OK;
+}
+
+swm_status swm_step(swm_ctx* C, int nsteps) {
+  if (!C || nsteps < 0) { swm::set_error("swm_step: bad argument"); return SWM_ERR_ARG; }
+  if (!C->initialized) { swm::set_error("swm_step: state not initialised"); return SWM_ERR_STATE; }
+  HIP_TRY(hipSetDevice(C->device));
+  C->kev_used = 0;
+  HIP_TRY(hipEventRecord(C->ev_a, C->stream));
+  for (int n = 0; n < nsteps; ++n) {
+    C->step_count += 1;
+    pointers(C);
+    weights(C);
+    swm_status st = step_dt(C);
+    if (st) return st;
+    wrap_prognostics(C);
+  }
+  HIP_TRY(hipEventRecord(C->ev_b, C->stream));
+  HIP_TRY(hipEventSynchronize(C->ev_b));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, C->ev_a, C->ev_b));
+  C->last_step_ms = ms;
+  HIP_TRY(hipGetLastError());
+  return SWM_OK;
+}
+
+swm_status swm_get_field(swm_ctx* C, const char* name, double* out) {
+  if (!C || !name || !out) { swm::set_error("swm_get_field: bad argument"); return SWM_ERR_ARG; }
+  HIP_TRY(hipSetDevice(C->device));
+  const int ns = C->ns;
+  const int n4 = C->n4 - 1;
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  const std::string s(name);
+  const double* src = nullptr;
+  if (s == "h") src = C->h[n4];
+  else if (s == "eta") src = C->eta[n4];
+  else if (s == "div") src = C->div[n4];
+  else if (s == "relative") src = C->relative;
+  else if (s == "ke") src = C->ke;
+  else if (s == "psi") src = C->xout;
+  else if (s == "chi") src = C->xout + ns;
+  if (src) {
+    HIP_TRY(hipMemcpy(out, src, ns * sizeof(double), hipMemcpyDeviceToHost));
+    return SWM_OK;
+  }
+  if (s == "u") {
+    std::vector<double> p(4 * (size_t)ns);
+    HIP_TRY(hipMemcpy(p.data(), C->u[n4], p.size() * sizeof(double), hipMemcpyDeviceToHost));
+    u_from_planes(p.data(), out, ns);
+    return SWM_OK;
+  }
+  if (s == "tracer") {
+    std::vector<double> p((size_t)ns);
+    const size_t pl = (size_t)C->iim * C->jjm;
+    for (int q = 0; q < C->nt; ++q) {
+      HIP_TRY(hipMemcpy(p.data(), C->tr[n4][q], ns * sizeof(double), hipMemcpyDeviceToHost));
+      for (int k = 0; k < 10; ++k) std::memcpy(out + ((size_t)q + (size_t)C->nt * k) * pl, &p[k * pl], pl * sizeof(double));
+    }
+    return SWM_OK;
+  }
+  swm::set_error("swm_get_field: unknown field " + s);
+  return SWM_ERR_ARG;
+}
+
+swm_status swm_synchronize(swm_ctx* C) {
+  if (!C) return SWM_ERR_ARG;
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  return SWM_OK;
+}
+
+int swm_step_count(const swm_ctx* C) { return C ? C->step_count : -1; }
+int swm_last_vcycles(const swm_ctx* C) { return C ? C->last_vcycles : -1; }
+
+swm_status swm_total_mass(swm_ctx* C, double* mass) {
+  if (!C || !mass) return SWM_ERR_ARG;
+  HIP_TRY(hipSetDevice(C->device));
+  const int n4 = C->n4 - 1;
+  sum2(C, C->h[n4], C->h[n4], C->area_mask, C->ns);
+  HIP_TRY(hipMemcpyAsync(C->host_sums, C->sums, sizeof(double), hipMemcpyDeviceToHost, C->stream));
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  *mass = C->host_sums[0];
+  return SWM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,708 @@
+// swm_kernels.hip -- gfx950 kernels of the shallow-water dynamical core (FP64).
+//
+// Data layout: every field is a set of planes; one plane is the reference's
+// (iim, jjm, 10) panel array of one component, i fastest (256-byte aligned
+// allocations, one thread per slot, consecutive threads on consecutive i, so
+// every plane access is coalesced).  Multi-component fields use the plane
+// orders
+//   corner scalar   (ic)            plane ic
+//   corner vector   (space, ic)     plane ic + 2*space
+//   edge field      (e)             plane e
+//   dc, sub_area, s_weights (kk, ic)        plane kk + 3*ic
+//   normal, tangent, v_weights (kk, a, ic)  plane kk + 3*(a + 2*ic)
+//   l_weights (n)                           plane n
+// so that one halo table indexes (component, slot) as plane*ns + slot for all
+// of them.  All stencils are HBM-bandwidth bound (a few flops per byte), so
+// the kernels are written for coalescing and few passes, not for MFMA.
+//
+// Each kernel restates the reference routine named in its comment with the
+// same floating-point operation order (compiled without contraction), so
+// the results agree with the reference to the last bit or to round-off.
+#include "swm_kernels.h"
+
+namespace swm {
+namespace dev {
+
+#define SL(i, j, k) ((i) - 1 + iim * ((j) - 1 + jjm * ((k) - 1)))
+
+struct IJK {
+  int i, j, k;
+};
+__device__ __forceinline__ IJK ijk_of(int s, int iim, int jjm) {
+  IJK r;
+  r.i = s % iim + 1;
+  const int q = s / iim;
+  r.j = q % jjm + 1;
+  r.k = q / jjm + 1;
+  return r;
+}
+__device__ __forceinline__ bool interior(const IJK& c, int iim, int jjm) {
+  return c.i >= 2 && c.i <= iim - 1 && c.j >= 2 && c.j <= jjm - 1;
+}
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+
+// ------------------------------------------------------------------ halo --
+// x[dst] = (src < 0 ? 0 : x[src]) for every table entry, on up to 8 fields
+// (blockIdx.y); no source is a destination, so it runs in place.
+__global__ void k_gather(const int* __restrict__ dst, const int* __restrict__ src, int n, FieldSet fs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double* x = fs.p[blockIdx.y];
+  const int s = src[t];
+  x[dst[t]] = s < 0 ? 0.0 : x[s];
+}
+
+// ---------------------------------------------------- corner averages ----
+// operators.F:982-1112 average_to_cell_corners (flag = .true.) of two fields
+__global__ void k_avg_corners(Geo G, const double* __restrict__ sw, const double* __restrict__ x1,
+                              const double* __restrict__ x2, double* __restrict__ o1, double* __restrict__ o2) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  if (i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1) {
+    const int a = SL(i, j, k), b = SL(i - 1, j, k), d = SL(i - 1, j - 1, k), e = SL(i, j - 1, k);
+    const double w10 = sw[0 * ns + s], w20 = sw[1 * ns + s], w30 = sw[2 * ns + s];
+    const double w11 = sw[3 * ns + s], w21 = sw[4 * ns + s], w31 = sw[5 * ns + s];
+    o1[s] = +w10 * x1[a] + w20 * x1[b] + w30 * x1[d];
+    o1[ns + s] = +w11 * x1[a] + w21 * x1[d] + w31 * x1[e];
+    o2[s] = +w10 * x2[a] + w20 * x2[b] + w30 * x2[d];
+    o2[ns + s] = +w11 * x2[a] + w21 * x2[d] + w31 * x2[e];
+  } else {
+    const double f1 = x1[SL(2, 2, k)], f2 = x2[SL(2, 2, k)];
+    o1[s] = f1; o1[ns + s] = f1;
+    o2[s] = f2; o2[ns + s] = f2;
+  }
+}
+
+// half-edge fluxes of cell (i,j) as combined by centered_mass_flux
+// (operators.F:1626-1722); with mc == nullptr the corner-mass factor is
+// dropped (upstream_mass_flux, operators.F:1731-1832).  nrm is either the
+// normal or the tangent planes.
+struct Half {
+  double r[2][3];
+};
+__device__ __forceinline__ Half half_edges(int iim, int jjm, int ns, int i, int j, int k, const double* __restrict__ dc,
+                                           const double* __restrict__ nrm, const double* __restrict__ mc,
+                                           const double* __restrict__ u) {
+  // u planes: (ic + 2*sp); nrm planes: kk + 3*(a + 2*ic); dc planes kk + 3*ic
+#define U(sp, ic, s_) u[((ic) + 2 * (sp)) * ns + (s_)]
+#define N(kk, a, ic, s_) nrm[((kk) + 3 * ((a) + 2 * (ic))) * ns + (s_)]
+#define D(kk, ic, s_) dc[((kk) + 3 * (ic)) * ns + (s_)]
+#define HN(s_, kk, ic) (U(0, ic, s_) * N(kk, 0, ic, s_) + U(1, ic, s_) * N(kk, 1, ic, s_))
+  Half h;
+  const int s0 = SL(i, j, k), sn = SL(i, j + 1, k), se = SL(i + 1, j, k);
+  if (mc) {
+    h.r[0][0] = HN(sn, 1, 1) * mc[ns + sn]; h.r[0][0] = h.r[0][0] * D(1, 1, sn);
+    h.r[1][0] = HN(s0, 0, 0) * mc[s0];      h.r[1][0] = h.r[1][0] * D(0, 0, s0);
+    h.r[0][1] = HN(s0, 2, 0) * mc[s0];      h.r[0][1] = h.r[0][1] * D(2, 0, s0);
+    h.r[1][1] = HN(s0, 0, 1) * mc[ns + s0]; h.r[1][1] = h.r[1][1] * D(0, 1, s0);
+    h.r[0][2] = HN(s0, 2, 1) * mc[ns + s0]; h.r[0][2] = h.r[0][2] * D(2, 1, s0);
+    h.r[1][2] = HN(se, 1, 0) * mc[se];      h.r[1][2] = h.r[1][2] * D(1, 0, se);
+  } else {
+    h.r[0][0] = HN(sn, 1, 1); h.r[0][0] = h.r[0][0] * D(1, 1, sn);
+    h.r[1][0] = HN(s0, 0, 0); h.r[1][0] = h.r[1][0] * D(0, 0, s0);
+    h.r[0][1] = HN(s0, 2, 0); h.r[0][1] = h.r[0][1] * D(2, 0, s0);
+    h.r[1][1] = HN(s0, 0, 1); h.r[1][1] = h.r[1][1] * D(0, 1, s0);
+    h.r[0][2] = HN(s0, 2, 1); h.r[0][2] = h.r[0][2] * D(2, 1, s0);
+    h.r[1][2] = HN(se, 1, 0); h.r[1][2] = h.r[1][2] * D(1, 0, se);
+  }
+#undef U
+#undef N
+#undef D
+#undef HN
+  return h;
+}
+
+// upstream (first-order) and centred (high-order tendency) edge mass fluxes
+// (mass_fct.F:151-214 via operators.F:1626-1832)
+__global__ void k_mass_fluxes(Geo G, const double* __restrict__ u, const double* __restrict__ h3,
+                              const double* __restrict__ hc, double* __restrict__ flx1, double* __restrict__ mshf) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) { flx1[e * ns + s] = 0.0; mshf[e * ns + s] = 0.0; }
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  const Half hc_ = half_edges(iim, jjm, ns, i, j, k, G.dc, G.nrm, hc, u);
+  const Half hu = half_edges(iim, jjm, ns, i, j, k, G.dc, G.nrm, nullptr, u);
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  for (int e = 0; e < 3; ++e) {
+    mshf[e * ns + s] = -hc_.r[0][e] + hc_.r[1][e];
+    const double v = hu.r[1][e] - hu.r[0][e];
+    const double mx = v > 0.0 ? v : 0.0, mn = v < 0.0 ? v : 0.0;
+    flx1[e * ns + s] = mx * h3[s] + mn * h3[up[e]];
+  }
+}
+
+// update of a cell field by the divergence of edge fluxes, pole forms
+// included (mass_fct.F:350-445, :478-532; tracer_fct.F:251-321, :601-665)
+__device__ __forceinline__ double edge_update(int iim, int jjm, int ns, const IJK& c, double base, double dt_ai,
+                                              const double* __restrict__ F, bool& set) {
+  const int i = c.i, j = c.j, k = c.k;
+  set = true;
+  if (i == 1 && j == jjm - 1)
+    return base + dt_ai * (F[1 * ns + SL(i, j, k)] + F[2 * ns + SL(i, j, k)] + F[0 * ns + SL(i + 1, j, k)] +
+                           F[1 * ns + SL(i + 1, j + 1, k)] + F[2 * ns + SL(i, j + 1, k)]);
+  if (i == iim - 1 && j == 1)
+    return base + dt_ai * (F[0 * ns + SL(i, j, k)] + F[1 * ns + SL(i, j, k)] + F[0 * ns + SL(i + 1, j, k)] +
+                           F[1 * ns + SL(i + 1, j + 1, k)] + F[2 * ns + SL(i, j + 1, k)]);
+  if (i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1)
+    return base - dt_ai * ((F[0 * ns + SL(i, j, k)] + F[1 * ns + SL(i, j, k)] + F[2 * ns + SL(i, j, k)]) -
+                           (F[0 * ns + SL(i + 1, j, k)] + F[1 * ns + SL(i + 1, j + 1, k)] + F[2 * ns + SL(i, j + 1, k)]));
+  set = false;
+  return 0.0;
+}
+
+// antidiffusive flux (all slots), low-order mass and the limiter beta_down
+// (mass_fct.F:205-214 and :342-402; the beta loop starts at i,j = 1 and so
+// overwrites its own pole values, which is reproduced)
+__global__ void k_mass_low_beta(Geo G, double dt, double w3, double w2, double w1, const double* __restrict__ flx1,
+                                const double* __restrict__ m3, const double* __restrict__ m2,
+                                const double* __restrict__ m1, const double* __restrict__ h3,
+                                double* __restrict__ anti, double* __restrict__ mss_low, double* __restrict__ beta) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  auto A = [&](int e, int sl) {
+    double f2 = 0.0;
+    f2 = f2 + w3 * m3[e * ns + sl];
+    f2 = f2 + w2 * m2[e * ns + sl];
+    f2 = f2 + w1 * m1[e * ns + sl];
+    return f2 - flx1[e * ns + sl];
+  };
+  for (int e = 0; e < 3; ++e) anti[e * ns + s] = A(e, s);
+  bool set;
+  const double ml = edge_update(iim, jjm, ns, c, h3[s], dt * G.area_inv[s], flx1, set);
+  mss_low[s] = ml;
+  if (i <= iim - 1 && j <= jjm - 1) {
+    const double a_out = (dmax(0.0, A(0, s)) + dmax(0.0, A(1, s)) + dmax(0.0, A(2, s))) -
+                         (dmin(0.0, A(0, SL(i + 1, j, k))) + dmin(0.0, A(1, SL(i + 1, j + 1, k))) +
+                          dmin(0.0, A(2, SL(i, j + 1, k))));
+    beta[s] = G.area[s] * (ml - 1.0e-6) / (dt * (a_out + 1.0e-18));
+  } else {
+    beta[s] = 0.0;
+  }
+}
+
+// mass_fct.F:410-470 set_corrected_mss_anti (in place; non-interior slots
+// keep the uncorrected value until the edge halo fill)
+__global__ void k_mass_anti(Geo G, const double* __restrict__ beta, double* __restrict__ anti) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) return;
+  const int i = c.i, j = c.j, k = c.k;
+  const double b = dmin(1.0, beta[s]);
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  for (int e = 0; e < 3; ++e) {
+    const double a = anti[e * ns + s];
+    anti[e * ns + s] = b * dmax(0.0, a) + dmin(1.0, beta[up[e]]) * dmin(0.0, a);
+  }
+}
+
+// flux-corrected mass and the total mass flux (mass_fct.F:121-128, :478-532)
+__global__ void k_mass_final(Geo G, double dt, const double* __restrict__ flx1, const double* __restrict__ anti,
+                             const double* __restrict__ mss_low, double* __restrict__ mflux, double* __restrict__ h4) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  for (int e = 0; e < 3; ++e) mflux[e * ns + s] = flx1[e * ns + s] + anti[e * ns + s];
+  bool set;
+  h4[s] = edge_update(iim, jjm, ns, c, mss_low[s], dt * G.area_inv[s], anti, set);
+}
+
+// tracer FCT, first- and high-order edge fluxes (tracer_fct.F:166-243); the
+// high-order tracer is the AB3 extrapolation of step.F:657-660
+__global__ void k_tr_fluxes(Geo G, double w3, double w2, double w1, const double* __restrict__ t3,
+                            const double* __restrict__ t4, const double* __restrict__ told,
+                            const double* __restrict__ mflux, double* __restrict__ f1, double* __restrict__ f2) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) { f1[e * ns + s] = 0.0; f2[e * ns + s] = 0.0; }
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  auto TH = [&](int sl) { return w3 * t3[sl] + w2 * t4[sl] + w1 * told[sl]; };
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  const double ths = TH(s);
+  for (int e = 0; e < 3; ++e) {
+    const double F = mflux[e * ns + s];
+    f1[e * ns + s] = t3[s] * dmax(0.0, F) + t3[up[e]] * dmin(0.0, F);
+    f2[e * ns + s] = 0.5 * (ths + TH(up[e])) * F;
+  }
+}
+
+// tracer_fct.F:251-321 set_msstrc_low
+__global__ void k_tr_low(Geo G, double dt, const double* __restrict__ h3, const double* __restrict__ t3,
+                         const double* __restrict__ f1, double* __restrict__ ml) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  bool set;
+  ml[s] = edge_update(iim, jjm, ns, c, h3[s] * t3[s], dt * G.area_inv[s], f1, set);
+}
+
+// tracer_fct.F:329-531: local extrema over the cell and its neighbours of the
+// tracer and its low-order update, and the limiter weights beta_dn / beta_up
+__global__ void k_tr_limiter(Geo G, double dt, const double* __restrict__ t3, const double* __restrict__ ml,
+                             const double* __restrict__ h4, const double* __restrict__ mss_low,
+                             const double* __restrict__ f1, const double* __restrict__ f2,
+                             double* __restrict__ anti, double* __restrict__ bdn, double* __restrict__ bup) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  auto A = [&](int e, int sl) { return f2[e * ns + sl] - f1[e * ns + sl]; };
+  for (int e = 0; e < 3; ++e) anti[e * ns + s] = A(e, s);
+  auto TL = [&](int sl) { return h4[sl] > 0.0 ? ml[sl] / h4[sl] : 0.0; };
+  const int p = iim - 1, q = jjm - 1;
+  const bool np = (i == 1 && j == q), sp = (i == p && j == 1);
+  const bool in = interior(c, iim, jjm);
+  if (!(np || sp || in)) { bdn[s] = 0.0; bup[s] = 0.0; return; }
+  double mn, mx, a_out, a_in;
+  if (in) {
+    const int nb[7] = {s, SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k), SL(i + 1, j, k),
+                       SL(i + 1, j + 1, k), SL(i, j + 1, k)};
+    mn = t3[s]; mx = t3[s];
+    const double tl0 = TL(s);
+    mn = dmin(mn, tl0); mx = dmax(mx, tl0);
+    for (int n = 1; n < 7; ++n) { mn = dmin(mn, t3[nb[n]]); mx = dmax(mx, t3[nb[n]]); }
+    for (int n = 1; n < 7; ++n) { const double v = TL(nb[n]); mn = dmin(mn, v); mx = dmax(mx, v); }
+    const double o1 = A(0, s), o2 = A(1, s), o3 = A(2, s);
+    const double n1 = A(0, SL(i + 1, j, k)), n2 = A(1, SL(i + 1, j + 1, k)), n3 = A(2, SL(i, j + 1, k));
+    a_out = (dmax(0.0, o1) + dmax(0.0, o2) + dmax(0.0, o3)) - (dmin(0.0, n1) + dmin(0.0, n2) + dmin(0.0, n3));
+    a_in = -((dmin(0.0, o1) + dmin(0.0, o2) + dmin(0.0, o3)) - (dmax(0.0, n1) + dmax(0.0, n2) + dmax(0.0, n3)));
+  } else {
+    int nb[5];
+    double ee[5];
+    if (np) {
+      const int t[5] = {SL(p, 1, k), SL(1, q - 1, k), SL(2, q, k), SL(2, q + 1, k), SL(1, q + 1, k)};
+      for (int n = 0; n < 5; ++n) nb[n] = t[n];
+      ee[0] = A(1, s); ee[1] = A(2, s); ee[2] = A(0, SL(i + 1, j, k)); ee[3] = A(1, SL(i + 1, j + 1, k));
+      ee[4] = A(2, SL(i, j + 1, k));
+    } else {
+      const int t[5] = {SL(p - 1, 1, k), SL(1, q, k), SL(p + 1, 1, k), SL(p + 1, 2, k), SL(p, 2, k)};
+      for (int n = 0; n < 5; ++n) nb[n] = t[n];
+      ee[0] = A(0, s); ee[1] = A(1, s); ee[2] = A(0, SL(i + 1, j, k)); ee[3] = A(1, SL(i + 1, j + 1, k));
+      ee[4] = A(2, SL(i, j + 1, k));
+    }
+    mn = t3[s]; mx = t3[s];
+    const double tl0 = TL(s);
+    mn = dmin(mn, tl0); mx = dmax(mx, tl0);
+    for (int n = 0; n < 5; ++n) { mn = dmin(mn, t3[nb[n]]); mx = dmax(mx, t3[nb[n]]); }
+    for (int n = 0; n < 5; ++n) { const double v = TL(nb[n]); mn = dmin(mn, v); mx = dmax(mx, v); }
+    a_out = -(dmin(0.0, ee[0]) + dmin(0.0, ee[1]) + dmin(0.0, ee[2]) + dmin(0.0, ee[3]) + dmin(0.0, ee[4]));
+    a_in = (dmax(0.0, ee[0]) + dmax(0.0, ee[1]) + dmax(0.0, ee[2]) + dmax(0.0, ee[3]) + dmax(0.0, ee[4]));
+  }
+  const double ar = G.area[s] / dt;
+  bdn[s] = ar * (ml[s] - mss_low[s] * mn) / (a_out + 1.0e-18);
+  bup[s] = ar * (mss_low[s] * mx - ml[s]) / (a_in + 1.0e-18);
+}
+
+// tracer_fct.F:539-593 set_corrected_trc_anti (in place)
+__global__ void k_tr_anti(Geo G, const double* __restrict__ bdn, const double* __restrict__ bup,
+                          double* __restrict__ anti) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) return;
+  const int i = c.i, j = c.j, k = c.k;
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  for (int e = 0; e < 3; ++e) {
+    const double a = anti[e * ns + s];
+    anti[e * ns + s] = dmin(dmin(1.0, bdn[s]), bup[up[e]]) * dmax(0.0, a) +
+                       dmin(dmin(1.0, bup[s]), bdn[up[e]]) * dmin(0.0, a);
+  }
+}
+
+// tracer_fct.F:601-665 set_msstrc_fct and the division by the new mass; the
+// previous tracer level is kept for the next AB3 extrapolation (step.F:661)
+__global__ void k_tr_final(Geo G, double dt, const double* __restrict__ ml, const double* __restrict__ anti,
+                           const double* __restrict__ h4, double* __restrict__ t4, double* __restrict__ told) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  bool set;
+  const double mf = edge_update(iim, jjm, ns, c, ml[s], dt * G.area_inv[s], anti, set);
+  told[s] = t4[s];
+  t4[s] = h4[s] != 0.0 ? mf / h4[s] : 0.0;
+}
+
+// operators.F:2000-2186 centered_vorticity_flux and centered_divergence_curl
+__global__ void k_vort_curl(Geo G, const double* __restrict__ mflux, const double* __restrict__ h3,
+                            const double* __restrict__ eta3, const double* __restrict__ hc,
+                            const double* __restrict__ u, double* __restrict__ vf, double* __restrict__ dcurl) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) { vf[e * ns + s] = 0.0; dcurl[e * ns + s] = 0.0; }
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  auto PV = [&](int sl) {
+    const double w = h3[sl] > 0.0 ? 1.0 / h3[sl] : 0.0;
+    return eta3[sl] * w;
+  };
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  const double pv0 = PV(s);
+  const Half ht = half_edges(iim, jjm, ns, i, j, k, G.dc, G.tng, hc, u);
+  for (int e = 0; e < 3; ++e) {
+    const double pvu = PV(up[e]);
+    vf[e * ns + s] = 0.5 * (mflux[e * ns + s]) * (pv0 + pvu);
+    dcurl[e * ns + s] = (-ht.r[0][e] + ht.r[1][e]) * (pvu + pv0) * 0.5;
+  }
+}
+
+// operators.F:270-392 div_of_an_edge_flux at one slot (0 where not defined)
+__device__ __forceinline__ double div_edge_at(int iim, int jjm, int ns, const IJK& c, const double* __restrict__ F,
+                                              double ai) {
+  const int i = c.i, j = c.j, k = c.k;
+  if (i == 1 && j == jjm - 1)
+    return (-F[1 * ns + SL(i, j, k)] - F[2 * ns + SL(i, j, k)] - F[0 * ns + SL(i + 1, j, k)] -
+            F[1 * ns + SL(i + 1, j + 1, k)] - F[2 * ns + SL(i, j + 1, k)]) * ai;
+  if (i == iim - 1 && j == 1)
+    return (-F[0 * ns + SL(i, j, k)] - F[1 * ns + SL(i, j, k)] - F[0 * ns + SL(i + 1, j, k)] -
+            F[1 * ns + SL(i + 1, j + 1, k)] - F[2 * ns + SL(i, j + 1, k)]) * ai;
+  if (i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1)
+    return (F[0 * ns + SL(i, j, k)] + F[1 * ns + SL(i, j, k)] + F[2 * ns + SL(i, j, k)] - F[0 * ns + SL(i + 1, j, k)] -
+            F[1 * ns + SL(i + 1, j + 1, k)] - F[2 * ns + SL(i, j + 1, k)]) * ai;
+  return 0.0;
+}
+
+// operators.F:402-518 laplacian_of_a_scalar at one slot; X(sl) gives the field
+template <class FX>
+__device__ __forceinline__ double lap_at(int iim, int jjm, int ns, const IJK& c, const double* __restrict__ lw,
+                                         FX X) {
+  const int i = c.i, j = c.j, k = c.k;
+  const int s = SL(i, j, k);
+#define LW(n) lw[((n) - 1) * ns + s]
+  if (i == 1 && j == jjm - 1)
+    return +LW(7) * X(s) + LW(2) * X(SL(iim, 1, k)) + LW(3) * X(SL(i, j - 1, k)) + LW(4) * X(SL(i + 1, j, k)) +
+           LW(5) * X(SL(i + 1, j + 1, k)) + LW(6) * X(SL(i, j + 1, k));
+  if (i == iim - 1 && j == 1)
+    return +LW(7) * X(s) + LW(1) * X(SL(i - 1, j, k)) + LW(2) * X(SL(1, jjm, k)) + LW(4) * X(SL(i + 1, j, k)) +
+           LW(5) * X(SL(i + 1, j + 1, k)) + LW(6) * X(SL(i, j + 1, k));
+  if (i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1)
+    return +LW(7) * X(s) + LW(1) * X(SL(i - 1, j, k)) + LW(2) * X(SL(i - 1, j - 1, k)) + LW(3) * X(SL(i, j - 1, k)) +
+           LW(4) * X(SL(i + 1, j, k)) + LW(5) * X(SL(i + 1, j + 1, k)) + LW(6) * X(SL(i, j + 1, k));
+#undef LW
+  return 0.0;
+}
+
+// step.F:336-429 step_vorticity / step_divergence (ABM3) and the elliptic
+// right-hand sides of step.F:170-173
+__global__ void k_tendencies(Geo G, double dt, double grav, double w3, double w2, double w1,
+                             const double* __restrict__ vf, const double* __restrict__ dcurl,
+                             const double* __restrict__ ke, const double* __restrict__ h3,
+                             const double* __restrict__ eta3, const double* __restrict__ div3,
+                             double* __restrict__ etaf_a, const double* __restrict__ etaf_b,
+                             const double* __restrict__ etaf_c, double* __restrict__ divf_a,
+                             const double* __restrict__ divf_b, const double* __restrict__ divf_c,
+                             double* __restrict__ eta4, double* __restrict__ div4, double* __restrict__ relative,
+                             double* __restrict__ rhs1, double* __restrict__ rhs2) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const double ai = G.area_inv[s];
+  const double wv = div_edge_at(iim, jjm, ns, c, vf, ai);
+  const double ef = -wv;
+  etaf_a[s] = ef;
+  const double e4 = eta3[s] + dt * (w3 * ef + w2 * etaf_b[s] + w1 * etaf_c[s]);
+  const double wc = div_edge_at(iim, jjm, ns, c, dcurl, ai);
+  const double* hs = G.hs;
+  auto X = [&](int sl) { return ke[sl] + grav * (h3[sl] + hs[sl]); };
+  const double wl = lap_at(iim, jjm, ns, c, G.lw, X);
+  const double df = -wl + wc;
+  divf_a[s] = df;
+  const double d4 = div3[s] + dt * (w3 * df + w2 * divf_b[s] + w1 * divf_c[s]);
+  eta4[s] = e4;
+  div4[s] = d4;
+  const double rel = e4 - G.f[s];
+  relative[s] = rel;
+  rhs1[s] = rel;
+  rhs2[s] = d4;
+}
+
+// ------------------------------------------------------- reductions -------
+// per-block partial sums of x*w (w may be null) for nf fields (blockIdx.y)
+__global__ void k_sum_partial(const double* __restrict__ x0, const double* __restrict__ x1,
+                              const double* __restrict__ w, int n, double* __restrict__ part) {
+  __shared__ double sh[256];
+  const double* x = blockIdx.y == 0 ? x0 : x1;
+  double acc = 0.0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    acc += w ? x[t] * w[t] : x[t];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+}
+// Neumaier-compensated sum of the partials, one thread per field
+__global__ void k_sum_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  const int f = threadIdx.x;
+  if (f >= (int)blockDim.x) return;
+  double s = 0.0, comp = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    const double v = part[f * nb + b];
+    const double t = s + v;
+    if (fabs(s) >= fabs(v)) comp += (s - t) + v;
+    else comp += (v - t) + s;
+    s = t;
+  }
+  out[f] = s + comp;
+}
+
+// rhs -= mean (elliptic_solver.F:80-92); sums[v] / total gives the mean
+__global__ void k_sub_mean(int n, const double* __restrict__ sums, double total, double* __restrict__ r1,
+                           double* __restrict__ r2) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double m1 = sums[0] / total, m2 = sums[1] / total;
+  r1[t] = r1[t] - m1;
+  r2[t] = r2[t] - m2;
+}
+
+// squared residual of the Poisson problem at real cells
+// (elliptic_solver.F:203-239 check_convergence, :341-358 l2_norm)
+__global__ void k_resid2(Geo G, const double* __restrict__ x, const double* __restrict__ rhs, double* __restrict__ r2) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  auto X = [&](int sl) { return x[sl]; };
+  const double w = lap_at(iim, jjm, ns, c, G.lw, X);
+  const double r = (w - 0.0 * x[s]) - rhs[s];
+  r2[s] = (r * r) * G.mask[s];
+}
+
+// ------------------------------------------------------------- multigrid --
+// Halo reads go through the level's closed-form map (multigrid_solver.F:
+// 1105-1158 wrap_simple): src[slot] is the slot whose value the halo holds.
+
+// weighted Jacobi sweep (multigrid_solver.F:593-682); halo slots become 0
+__global__ void k_relax(Lev L, const double* __restrict__ beta, const double* __restrict__ a,
+                        double* __restrict__ out) {
+  const int iim = L.iim, jjm = L.jjm, ns = L.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const int v = blockIdx.y;
+  beta += (size_t)v * ns; a += (size_t)v * ns; out += (size_t)v * ns;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  const int* __restrict__ m = L.src;
+  auto A = [&](int sl) { return a[m[sl]]; };
+#define LW(n) L.lw[((n) - 1) * ns + s]
+  const double omega = 0.8;
+  double tt;
+  if (i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1) {
+    tt = L.lwt[s] * (beta[s] - LW(1) * A(SL(i - 1, j, k)) - LW(2) * A(SL(i - 1, j - 1, k)) -
+                     LW(3) * A(SL(i, j - 1, k)) - LW(4) * A(SL(i + 1, j, k)) - LW(5) * A(SL(i + 1, j + 1, k)) -
+                     LW(6) * A(SL(i, j + 1, k)));
+  } else if (i == 1 && j == jjm - 1) {
+    tt = L.lwt[s] * (beta[s] - LW(2) * A(SL(iim, 1, k)) - LW(3) * A(SL(i, j - 1, k)) - LW(4) * A(SL(i + 1, j, k)) -
+                     LW(5) * A(SL(i + 1, j + 1, k)) - LW(6) * A(SL(i, j + 1, k)));
+  } else if (i == iim - 1 && j == 1) {
+    tt = L.lwt[s] * (beta[s] - LW(1) * A(SL(i - 1, j, k)) - LW(2) * A(SL(1, jjm, k)) - LW(4) * A(SL(i + 1, j, k)) -
+                     LW(5) * A(SL(i + 1, j + 1, k)) - LW(6) * A(SL(i, j + 1, k)));
+  } else {
+    out[s] = 0.0;
+    return;
+  }
+#undef LW
+  out[s] = (1.0 - omega) * A(s) + omega * tt;
+}
+
+// residual at the odd-odd fine points becomes the coarse right-hand side
+// (multigrid_solver.F:684-886 inject_residual); one thread per coarse slot
+__global__ void k_inject(Lev F, Lev C, const double* __restrict__ xa, const double* __restrict__ xb,
+                         double* __restrict__ x1) {
+  int iim = C.iim, jjm = C.jjm;
+  const int s1 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s1 >= C.ns) return;
+  const int v = blockIdx.y;
+  xa += (size_t)v * F.ns; xb += (size_t)v * F.ns; x1 += (size_t)v * C.ns;
+  const IJK c = ijk_of(s1, iim, jjm);
+  const bool in = c.i >= 2 && c.i <= iim - 1 && c.j >= 2 && c.j <= jjm - 1;
+  const bool np = c.k == 1 && c.i == 1 && c.j == jjm - 1;
+  const bool sp = c.k == 10 && c.i == iim - 1 && c.j == 1;
+  if (!(in || np || sp)) return;
+  const int k = c.k;
+  int i2, j2;
+  if (in) { i2 = 2 * c.i - 1; j2 = 2 * c.j - 1; }
+  else if (np) { i2 = 1; j2 = F.jjm - 1; }
+  else { i2 = F.iim - 1; j2 = 1; }
+  iim = F.iim; jjm = F.jjm;
+  const int ns = F.ns;
+  const int s = SL(i2, j2, k);
+  const int* __restrict__ m = F.src;
+  auto A = [&](int sl) { return xa[m[sl]]; };
+#define LW(n) F.lw[((n) - 1) * ns + s]
+  double r;
+  if (in)
+    r = xb[s] - ((LW(7) - 0.0) * A(s) + LW(1) * A(SL(i2 - 1, j2, k)) + LW(2) * A(SL(i2 - 1, j2 - 1, k)) +
+                 LW(3) * A(SL(i2, j2 - 1, k)) + LW(4) * A(SL(i2 + 1, j2, k)) + LW(5) * A(SL(i2 + 1, j2 + 1, k)) +
+                 LW(6) * A(SL(i2, j2 + 1, k)));
+  else if (np)
+    r = xb[s] - ((LW(7) - 0.0) * A(s) + LW(2) * A(SL(iim, 1, k)) + LW(3) * A(SL(i2, j2 - 1, k)) +
+                 LW(4) * A(SL(i2 + 1, j2, k)) + LW(5) * A(SL(i2 + 1, j2 + 1, k)) + LW(6) * A(SL(i2, j2 + 1, k)));
+  else
+    r = xb[s] - ((LW(7) - 0.0) * A(s) + LW(1) * A(SL(i2 - 1, j2, k)) + LW(2) * A(SL(1, jjm, k)) +
+                 LW(4) * A(SL(i2 + 1, j2, k)) + LW(5) * A(SL(i2 + 1, j2 + 1, k)) + LW(6) * A(SL(i2, j2 + 1, k)));
+#undef LW
+  x1[s1] = r;
+}
+
+// coarse correction added at coincident points, two-point averages between
+// (multigrid_solver.F:888-1054 interpolate); one thread per fine slot
+__global__ void k_interp(Lev C, Lev F, const double* __restrict__ x1, double* __restrict__ x2) {
+  int iim = F.iim, jjm = F.jjm;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= F.ns) return;
+  const int v = blockIdx.y;
+  x1 += (size_t)v * C.ns; x2 += (size_t)v * F.ns;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  const int p2 = iim - 1, q2 = jjm - 1;
+  // temp(2a-1, 2b-1) = coarse (a, b) for a, b = 1..iim1-1 (halo reads mapped)
+  const int ci = C.iim, cj = C.jjm;
+  auto T = [&](int ii, int jj) {
+    const int a = (ii + 1) / 2, b = (jj + 1) / 2;
+    const int sl = (a - 1) + ci * ((b - 1) + cj * (k - 1));
+    return x1[C.src[sl]];
+  };
+  const bool io = (i & 1), jo = (j & 1);
+  double add;
+  bool hit = true;
+  if (i == 1 && j == q2) add = T(1, q2);
+  else if (i == p2 && j == 1) add = T(p2, 1);
+  else if (io && jo && i >= 3 && i <= iim - 1 && j >= 3 && j <= jjm - 1) add = T(i, j);
+  else if (!io && jo && i >= 2 && i <= iim - 2 && j >= 3 && j <= jjm - 1) add = 0.5 * (T(i + 1, j) + T(i - 1, j));
+  else if (!io && !jo && i >= 2 && i <= iim - 2 && j >= 2 && j <= jjm - 2) add = 0.5 * (T(i + 1, j + 1) + T(i - 1, j - 1));
+  else if (io && !jo && i >= 3 && i <= iim - 1 && j >= 2 && j <= jjm - 2) add = 0.5 * (T(i, j + 1) + T(i, j - 1));
+  else hit = false;
+  if (hit) x2[s] = x2[s] + add;
+}
+
+__global__ void k_fill(double* __restrict__ x, size_t n, double v) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] = v;
+}
+__global__ void k_copy(double* __restrict__ y, const double* __restrict__ x, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = x[t];
+}
+
+// ----------------------------------------------------- velocity and KE ----
+// operators.F:1402-1514 calculate_velocity: grad(chi) + k x grad(psi)
+__global__ void k_velocity(Geo G, const double* __restrict__ psi, const double* __restrict__ chi,
+                           double* __restrict__ u) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int p = 0; p < 4; ++p) u[p * ns + s] = 0.0;
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  const int nb[2][3] = {{s, SL(i - 1, j, k), SL(i - 1, j - 1, k)}, {s, SL(i - 1, j - 1, k), SL(i, j - 1, k)}};
+#define VW(kk, sp, ic) G.vw[((kk) + 3 * ((sp) + 2 * (ic))) * ns + s]
+  for (int ic = 0; ic < 2; ++ic) {
+    const double cai = G.cai[ic * ns + s];
+    double gc[2], gp[2];
+    for (int sp = 0; sp < 2; ++sp) {
+      gc[sp] = (+VW(0, sp, ic) * chi[nb[ic][0]] + VW(1, sp, ic) * chi[nb[ic][1]] + VW(2, sp, ic) * chi[nb[ic][2]]) * cai;
+      gp[sp] = (+VW(0, sp, ic) * psi[nb[ic][0]] + VW(1, sp, ic) * psi[nb[ic][1]] + VW(2, sp, ic) * psi[nb[ic][2]]) * cai;
+    }
+    u[(ic + 0) * ns + s] = gc[0] + (-gp[1]);
+    u[(ic + 2) * ns + s] = gc[1] + gp[0];
+  }
+#undef VW
+}
+
+// operators.F:1316-1393 with the corner kinetic energy 0.5 |u|^2 (step.F:192-203)
+__global__ void k_center_ke(Geo G, const double* __restrict__ u, double* __restrict__ ke) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  if (i > iim - 1 || j > jjm - 1) { ke[s] = 0.0; return; }
+  auto KC = [&](int ic, int sl) {
+    const double a = u[ic * ns + sl], b = u[(ic + 2) * ns + sl];
+    return 0.5 * (a * a + b * b);
+  };
+#define SA(kk, ic, sl) G.sub_area[((kk) + 3 * (ic)) * ns + (sl)]
+  const int se = SL(i + 1, j, k), sne = SL(i + 1, j + 1, k), sn = SL(i, j + 1, k);
+  ke[s] = (+SA(0, 0, s) * KC(0, s) + SA(0, 1, s) * KC(1, s) + SA(1, 0, se) * KC(0, se) + SA(1, 1, sne) * KC(1, sne) +
+           SA(2, 0, sne) * KC(0, sne) + SA(2, 1, sn) * KC(1, sn)) * G.area_inv[s];
+#undef SA
+}
+
+// operators.F:47-261 curl and divergence of a corner vector (initial state)
+__global__ void k_curl_div(Geo G, const double* __restrict__ u, double* __restrict__ curl, double* __restrict__ dv) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  if (i > iim - 1 || j > jjm - 1) { if (curl) curl[s] = 0.0; if (dv) dv[s] = 0.0; return; }
+#define VW(kk, sp, ic, sl) G.vw[((kk) + 3 * ((sp) + 2 * (ic))) * ns + (sl)]
+#define UU(sp, ic, sl) u[((ic) + 2 * (sp)) * ns + (sl)]
+  const int se = SL(i + 1, j, k), sne = SL(i + 1, j + 1, k), sn = SL(i, j + 1, k);
+  const double ai = G.area_inv[s];
+  if (curl)
+    curl[s] = (+VW(0, 1, 0, s) * UU(0, 0, s) - VW(0, 0, 0, s) * UU(1, 0, s) + VW(0, 1, 1, s) * UU(0, 1, s) -
+               VW(0, 0, 1, s) * UU(1, 1, s) + VW(1, 1, 0, se) * UU(0, 0, se) - VW(1, 0, 0, se) * UU(1, 0, se) +
+               VW(1, 1, 1, sne) * UU(0, 1, sne) - VW(1, 0, 1, sne) * UU(1, 1, sne) + VW(2, 1, 0, sne) * UU(0, 0, sne) -
+               VW(2, 0, 0, sne) * UU(1, 0, sne) + VW(2, 1, 1, sn) * UU(0, 1, sn) - VW(2, 0, 1, sn) * UU(1, 1, sn)) *
+              ai;
+  if (dv)
+    dv[s] = (-VW(0, 0, 0, s) * UU(0, 0, s) - VW(0, 1, 0, s) * UU(1, 0, s) - VW(0, 0, 1, s) * UU(0, 1, s) -
+             VW(0, 1, 1, s) * UU(1, 1, s) - VW(1, 0, 0, se) * UU(0, 0, se) - VW(1, 1, 0, se) * UU(1, 0, se) -
+             VW(1, 0, 1, sne) * UU(0, 1, sne) - VW(1, 1, 1, sne) * UU(1, 1, sne) - VW(2, 0, 0, sne) * UU(0, 0, sne) -
+             VW(2, 1, 0, sne) * UU(1, 0, sne) - VW(2, 0, 1, sn) * UU(0, 1, sn) - VW(2, 1, 1, sn) * UU(1, 1, sn)) *
+            ai;
+#undef VW
+#undef UU
+}
+
+// eta = relative + f ; inv_h ; psi/chi copies (step.F:177-184, :448-452)
+__global__ void k_post_elliptic(Geo G, const double* __restrict__ rhs1, double* __restrict__ eta4) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= G.ns) return;
+  eta4[s] = rhs1[s] + G.f[s];
+}
+__global__ void k_inv_h(Geo G, const double* __restrict__ h4, double* __restrict__ inv_h) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= G.ns) return;
+  inv_h[s] = G.mask[s] == 1.0 ? 1.0 / h4[s] : 0.0;
+}
+
+#undef SL
+
+}  // namespace dev
+}  // namespace swm

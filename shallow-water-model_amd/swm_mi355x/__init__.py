@@ -1,0 +1,223 @@
+"""Python binding of the MI355X shallow-water dynamical core (ctypes over the
+C ABI declared in include/swm_mi355x.h).
+
+The classes mirror the reference's module interfaces:
+
+* :class:`Grid` -- the set-up of ``initialize_model`` (grid points, halo
+  tables, masks, metrics, multigrid hierarchy), computed on the host.
+* :class:`Model` -- the device state and the time step ``step::step_dt`` +
+  ``step::wrap_prognostics``; the reference operators, ``wrap_data::wrap``,
+  ``elliptic_solver::elliptic`` and ``multigrid_solver::multigrid`` on host
+  arrays in the reference layout.
+
+Arrays are numpy float64 in Fortran order with the reference's shapes, e.g.
+cell fields (iim, jjm, 10).  The library is built in-tree by
+``build_lib.py``; importing this module raises if it is missing, there is
+no fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libswm_mi355x.so")
+
+SWM_FP64 = 0
+SWM_MIXED = 1
+
+# physical constants of the reference build (utilities/layer_1/params_physical.F:45-52)
+REF_RADIUS = 6.37122e6
+REF_GRAV = 9.80616
+REF_OMEGA = 7.292e-5
+
+_STATUS = {0: "SWM_OK", 1: "SWM_ERR_ARG", 2: "SWM_ERR_DIMS", 3: "SWM_ERR_HIP", 4: "SWM_ERR_NOT_CONVERGED",
+           5: "SWM_ERR_NAN", 6: "SWM_ERR_NO_DEVICE", 7: "SWM_ERR_STATE"}
+
+
+class SwmError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libswm_mi355x.so not built ({LIB_PATH}); run shallow-water-model_amd/build_lib.py")
+    return ctypes.CDLL(LIB_PATH)
+
+
+lib = _load()
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("ntracer", ctypes.c_int), ("dt", ctypes.c_double), ("grav", ctypes.c_double),
+                ("precision", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+def _sig(name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_sig("swm_last_error", ctypes.c_char_p, [])
+_sig("swm_grid_points", ctypes.c_int, [ctypes.c_int, ctypes.c_void_p])
+_sig("swm_grid_create", ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                       ctypes.POINTER(ctypes.c_void_p)])
+_sig("swm_grid_destroy", None, [ctypes.c_void_p])
+_sig("swm_grid_dims", ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p, _c_int_p, _c_int_p])
+_sig("swm_grid_array_len", ctypes.c_int64, [ctypes.c_void_p, ctypes.c_char_p])
+_sig("swm_grid_array", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p])
+_sig("swm_create", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Config), ctypes.POINTER(ctypes.c_void_p)])
+_sig("swm_destroy", None, [ctypes.c_void_p])
+_sig("swm_init_state", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4)
+_sig("swm_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
+_sig("swm_get_field", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p])
+_sig("swm_synchronize", ctypes.c_int, [ctypes.c_void_p])
+_sig("swm_step_count", ctypes.c_int, [ctypes.c_void_p])
+_sig("swm_last_vcycles", ctypes.c_int, [ctypes.c_void_p])
+_sig("swm_total_mass", ctypes.c_int, [ctypes.c_void_p, _dp])
+
+
+def _check(status):
+    if status != 0:
+        raise SwmError(status, lib.swm_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a, shape=None):
+    a = np.asfortranarray(np.asarray(a, dtype=np.float64))
+    if shape is not None and a.shape != tuple(shape):
+        raise ValueError(f"expected shape {tuple(shape)}, got {a.shape}")
+    return a
+
+
+def grid_points(g):
+    """Cell centres (3, 10*4^g+2) of the level-g icosahedral grid, global order."""
+    n = 10 * 4 ** g + 2
+    out = np.zeros((3, n), dtype=np.float64, order="F")
+    _check(lib.swm_grid_points(g, _ptr(out)))
+    return out
+
+
+# shapes of the named grid arrays, in units of (iim, jjm, 10) slots
+_GRID_SHAPES = {
+    "p": (3,), "c": (3, 2), "area": (), "area_inv": (), "corner_area": (2,), "corner_area_inv": (2,),
+    "sub_area": (3, 2), "dc": (3, 2), "normal": (3, 3, 2), "tangent": (3, 3, 2), "v_weights": (3, 2, 2),
+    "s_weights": (3, 2), "l_weights": (7,), "f": (), "grid_center_mask": (), "grid_corner_mask": (2,),
+    "polygon_type": (), "grid_center_index": (), "grid_corner_index": (2,),
+}
+_INT_ARRAYS = {"grid_center_index", "grid_corner_index", "face_dst", "face_src", "vrtx_dst", "vrtx_src",
+               "edge_dst", "edge_src"}
+
+
+class Grid:
+    """Host-side grid set-up at icosahedral level g (10 panels of 2^g x 2^g cells
+    plus the two poles, one-cell halos)."""
+
+    def __init__(self, g, p_list=None, radius=REF_RADIUS, omega=REF_OMEGA):
+        h = ctypes.c_void_p()
+        pl = None if p_list is None else _f64(p_list, (3, 10 * 4 ** g + 2))
+        _check(lib.swm_grid_create(g, _ptr(pl), radius, omega, ctypes.byref(h)))
+        self._h = h
+        gi, iim, jjm, mig = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib.swm_grid_dims(h, ctypes.byref(gi), ctypes.byref(iim), ctypes.byref(jjm), ctypes.byref(mig)))
+        self.g, self.iim, self.jjm, self.max_ig = gi.value, iim.value, jjm.value, mig.value
+        self.radius, self.omega = radius, omega
+
+    @property
+    def handle(self):
+        return self._h
+
+    def array(self, name):
+        n = lib.swm_grid_array_len(self._h, name.encode())
+        if n < 0:
+            raise KeyError(name)
+        dt = np.int32 if name in _INT_ARRAYS else np.float64
+        out = np.zeros(n, dtype=dt)
+        _check(lib.swm_grid_array(self._h, name.encode(), _ptr(out)))
+        if name in _GRID_SHAPES:
+            return out.reshape(_GRID_SHAPES[name] + (self.iim, self.jjm, 10), order="F")
+        if name.startswith("lap_w"):
+            m = 2 + 2 ** int(name[5:])
+            return out.reshape((7, m, m, 10), order="F")
+        if name == "tot_area":
+            return float(out[0])
+        return out
+
+    def close(self):
+        if self._h:
+            lib.swm_grid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Model:
+    """Device-resident model state on one GPU (the reference's prognostics
+    module plus the saved state of step.F and the elliptic warm start)."""
+
+    def __init__(self, grid, ntracer=1, dt=450.0, grav=REF_GRAV, precision=SWM_FP64, device=0):
+        cfg = _Config(ntracer, dt, grav, precision, device)
+        h = ctypes.c_void_p()
+        _check(lib.swm_create(grid.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.grid, self.ntracer, self.dt = grid, ntracer, dt
+        self.iim, self.jjm = grid.iim, grid.jjm
+
+    def init_state(self, h, u, tracer, hs=None):
+        s = (self.iim, self.jjm, 10)
+        h = _f64(h, s)
+        u = _f64(u, (2, 2) + s)
+        tr = _f64(tracer, (self.iim, self.jjm, self.ntracer, 10))
+        hs = None if hs is None else _f64(hs, s)
+        _check(lib.swm_init_state(self._h, _ptr(h), _ptr(hs), _ptr(u), _ptr(tr)))
+
+    def step(self, nsteps=1):
+        _check(lib.swm_step(self._h, nsteps))
+
+    def field(self, name):
+        s = (self.iim, self.jjm, 10)
+        shape = {"u": (2, 2) + s, "tracer": (self.iim, self.jjm, self.ntracer, 10)}.get(name, s)
+        out = np.zeros(shape, dtype=np.float64, order="F")
+        _check(lib.swm_get_field(self._h, name.encode(), _ptr(out)))
+        return out
+
+    def synchronize(self):
+        _check(lib.swm_synchronize(self._h))
+
+    @property
+    def step_count(self):
+        return lib.swm_step_count(self._h)
+
+    @property
+    def last_vcycles(self):
+        return lib.swm_last_vcycles(self._h)
+
+    def total_mass(self):
+        m = ctypes.c_double()
+        _check(lib.swm_total_mass(self._h, ctypes.byref(m)))
+        return m.value
+
+    def close(self):
+        if self._h:
+            lib.swm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
