@@ -136,6 +136,21 @@ int swm_last_vcycles(const swm_ctx* ctx);
 swm_status swm_timing(const swm_ctx* ctx, double* step_ms, double* kernel_ms_mean,
                       double* kernel_bytes, int* kernel_launches);
 
+/* Enables (1) or disables (0) HIP-event timing of every launch of the
+ * finest-level multigrid smoother inside swm_step; read back by swm_timing. */
+swm_status swm_profile(swm_ctx* ctx, int enable);
+
+/* Seeded synthetic initial state built on the device: harmonic sums of
+ * degree 1..lmax (coefficients N(0,1) l^-1.5, splitmix64 streams seeded
+ * 20261203 + 100*seed_index + 1..4) for vorticity, divergence (10 % of its
+ * RMS) and thickness (h0 plus a perturbation of RMS h_rms), corner velocity
+ * from the model's discrete gradient scaled to max |u| = umax, tracers as
+ * sums of nbumps Gaussian bumps of width bump_radius (m).  Then derives the
+ * remaining fields as swm_init_state does.  No counterpart in the reference
+ * (it runs fixed test cases, swm/source/initialize.F:330-696). */
+swm_status swm_init_synthetic(swm_ctx* ctx, int seed_index, int lmax, double umax, double h0,
+                              double h_rms, int nbumps, double bump_radius);
+
 /* Compensated (Neumaier) FP64 global mass sum of the current thickness,
  * sum(h * area) over real cells (swm/source/budgets.F:108-109). */
 swm_status swm_total_mass(swm_ctx* ctx, double* mass);

@@ -3,6 +3,7 @@
 // unit, included below).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -62,8 +63,8 @@ struct swm_ctx {
   Arena arena;
   size_t arena_bytes = 0;
   // metrics
-  double *area, *area_inv, *cai, *sub_area, *dc, *nrm, *tng, *vw, *sw, *lw, *f, *hs, *mask, *area_mask;
-  double total_area = 0.0;
+  double *area, *area_inv, *cai, *sub_area, *dc, *nrm, *tng, *vw, *sw, *lw, *f, *hs, *mask, *area_mask, *pxyz;
+  double total_area = 0.0, radius = 0.0;
   Geo geo;
   DevTable face, vrtx, edge;
   struct Level {
@@ -168,9 +169,15 @@ void relax(swm_ctx* C, int l, int iters, double* alpha, const double* beta, int 
   double* t = L.temp;
   for (int it = 0; it < iters; ++it) {
     const bool prof = C->profile && l == C->g && C->kev_used + 2 <= C->kev.size();
-    if (prof) hipEventRecord(C->kev[C->kev_used++], C->stream);
+    if (prof) {
+      (void)hipEventRecord(C->kev[C->kev_used++], C->stream);
+      // algorithmic bytes of one finest-level sweep per real cell: Laplacian
+      // weights 7 + reciprocal centre weight 1, and per field beta, alpha and
+      // the result (neighbour values come from cache), all FP64
+      C->kernel_bytes += (double)C->max_ig * 8.0 * (8.0 + 3.0 * nvar);
+    }
     hipLaunchKernelGGL(swm::dev::k_relax, dim3(nblk(L.ns), nvar), dim3(BLK), 0, C->stream, L.L, beta, a, t);
-    if (prof) hipEventRecord(C->kev[C->kev_used++], C->stream);
+    if (prof) (void)hipEventRecord(C->kev[C->kev_used++], C->stream);
     std::swap(a, t);
   }
   // an even sweep count leaves the result in alpha
@@ -350,6 +357,64 @@ void u_from_planes(const double* planes, double* ref, int ns) {
       for (int sp = 0; sp < 2; ++sp) ref[sp + 2 * (ic + 2 * (size_t)s)] = planes[(size_t)(ic + 2 * sp) * ns + s];
 }
 
+
+// splitmix64 counter-based generator (same sequence as swm_mi355x/ics.py)
+struct SplitMix {
+  uint64_t state;
+  explicit SplitMix(uint64_t s) : state(s) {}
+  uint64_t next() {
+    state += 0x9E3779B97F4A7C15ull;
+    uint64_t z = state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+  double normal() {
+    const double u1 = std::max(uniform(), 1e-300), u2 = uniform();
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+  }
+};
+
+// zero the whole state (the reference's module arrays start at zero)
+void reset_state(swm_ctx* C) {
+  fill(C, C->h[0], (C->arena.base + C->arena.used - reinterpret_cast<char*>(C->h[0])) / 8, 0.0);
+  fill(C, C->hs, C->ns, 0.0);
+  C->step_count = 0;
+  C->initialized = false;
+  pointers(C);
+}
+
+// h, hs, u (planes) and tracers of time level 1 are on the device: derive the
+// rest as the reference initial conditions do (initialize.F:418-468 and
+// operators_hl.F:153-266), then fill the halos as wrap_prognostics
+swm_status finish_init(swm_ctx* C) {
+  const int ns = C->ns;
+  hipStream_t S = C->stream;
+  const unsigned nb = nblk(ns);
+  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, C->h[1], C->h[0], (size_t)ns);
+  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nblk(4 * (size_t)ns)), dim3(BLK), 0, S, C->u[1], C->u[0], 4 * (size_t)ns);
+  for (int t = 0; t < 2; ++t) {
+    gather(C, C->face, {C->h[t]});
+    gather(C, C->vrtx, {C->u[t], C->u[t] + 2 * (size_t)ns});
+  }
+  gather(C, C->face, {C->hs});
+  gather(C, C->vrtx, {C->u[0], C->u[0] + 2 * (size_t)ns});
+  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb), dim3(BLK), 0, S, C->geo, C->u[0], C->relative, C->div[0]);
+  hipLaunchKernelGGL(swm::dev::k_center_ke, dim3(nb), dim3(BLK), 0, S, C->geo, C->u[0], C->ke);
+  gather(C, C->face, {C->relative, C->div[0], C->ke});
+  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, C->div[1], C->div[0], (size_t)ns);
+  hipLaunchKernelGGL(swm::dev::k_post_elliptic, dim3(nb), dim3(BLK), 0, S, C->geo, C->relative, C->eta[0]);
+  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, C->eta[1], C->eta[0], (size_t)ns);
+  for (int q = 0; q < C->nt; ++q)
+    hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, C->tr[1][q], C->tr[0][q], (size_t)ns);
+  wrap_prognostics(C);
+  HIP_TRY(hipStreamSynchronize(S));
+  HIP_TRY(hipGetLastError());
+  C->initialized = true;
+  return SWM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -383,7 +448,7 @@ swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out
   auto W = [&](size_t n) { words += ((n * 8 + 255) / 256) * 32; };
   const int planes_metrics = 1 + 1 + 2 + 6 + 6 + 12 + 12 + 12 + 6 + 7 + 1 + 1 + 1 + 1;
   W(ns); W(ns); W(2 * ns); W(6 * ns); W(6 * ns); W(12 * ns); W(12 * ns); W(12 * ns); W(6 * ns); W(7 * ns);
-  W(ns); W(ns); W(ns); W(ns);
+  W(ns); W(ns); W(ns); W(ns); W(3 * ns);
   (void)planes_metrics;
   for (int l = 1; l <= C->g; ++l) {
     const size_t nl = (size_t)(2 + (1 << l)) * (2 + (1 << l)) * 10;
@@ -406,17 +471,19 @@ swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out
 
   // metrics as planes
   auto id = [](int n) { std::vector<int> v(n); for (int k = 0; k < n; ++k) v[k] = k; return v; };
-  C->area = take(ns); C->area_inv = take(ns); C->cai = take(2 * ns); C->sub_area = take(6 * ns); C->dc = take(6 * ns);
-  C->nrm = take(12 * ns); C->tng = take(12 * ns); C->vw = take(12 * ns); C->sw = take(6 * ns); C->lw = take(7 * ns);
-  C->f = take(ns); C->hs = take(ns); C->mask = take(ns); C->area_mask = take(ns);
   swm_status st;
-  if ((st = upload(C->area, Gh.area, C->stream))) return st;
-  if ((st = upload(C->area_inv, Gh.area_inv, C->stream))) return st;
   std::vector<std::vector<double>> hold;   // keep host buffers alive until the copies finish
   auto up = [&](double* d, std::vector<double> v) {
     hold.push_back(std::move(v));
     return upload(d, hold.back(), C->stream);
   };
+  C->area = take(ns); C->area_inv = take(ns); C->cai = take(2 * ns); C->sub_area = take(6 * ns); C->dc = take(6 * ns);
+  C->nrm = take(12 * ns); C->tng = take(12 * ns); C->vw = take(12 * ns); C->sw = take(6 * ns); C->lw = take(7 * ns);
+  C->f = take(ns); C->hs = take(ns); C->mask = take(ns); C->area_mask = take(ns); C->pxyz = take(3 * ns);
+  C->radius = Gh.radius;
+  if ((st = up(C->pxyz, to_planes(Gh.p, 3, ns, id(3), 3)))) return st;
+  if ((st = upload(C->area, Gh.area, C->stream))) return st;
+  if ((st = upload(C->area_inv, Gh.area_inv, C->stream))) return st;
   if ((st = up(C->cai, to_planes(Gh.corner_area_inv, 2, ns, id(2), 2)))) return st;
   if ((st = up(C->sub_area, to_planes(Gh.sub_area, 6, ns, id(6), 6)))) return st;
   if ((st = up(C->dc, to_planes(Gh.dc, 6, ns, id(6), 6)))) return st;
@@ -511,42 +578,94 @@ swm_status swm_init_state(swm_ctx* C, const double* h, const double* hs, const d
   HIP_TRY(hipSetDevice(C->device));
   const int ns = C->ns;
   hipStream_t S = C->stream;
-  const unsigned nb = nblk(ns);
-  std::vector<double> up(4 * (size_t)ns), tq((size_t)ns);
+  std::vector<double> up(4 * (size_t)ns), tq((size_t)C->nt * ns);
   u_to_planes(u, up.data(), ns);
-  // fresh state: zero everything the reference zeroes at start-up
-  fill(C, C->h[0], (C->arena.base + C->arena.used - reinterpret_cast<char*>(C->h[0])) / 8, 0.0);
-  C->step_count = 0;
-  pointers(C);
-  for (int t = 0; t < 2; ++t) {
-    HIP_TRY(hipMemcpyAsync(C->h[t], h, ns * sizeof(double), hipMemcpyHostToDevice, S));
-    HIP_TRY(hipMemcpyAsync(C->u[t], up.data(), 4 * (size_t)ns * sizeof(double), hipMemcpyHostToDevice, S));
-    gather(C, C->face, {C->h[t]});
-    gather(C, C->vrtx, {C->u[t], C->u[t] + 2 * (size_t)ns});
-  }
-  if (hs) HIP_TRY(hipMemcpyAsync(C->hs, hs, ns * sizeof(double), hipMemcpyHostToDevice, S));
-  else fill(C, C->hs, ns, 0.0);
-  gather(C, C->face, {C->hs});
-  // calculate_rel_div_and_ke (operators_hl.F:153-266)
-  gather(C, C->vrtx, {C->u[0], C->u[0] + 2 * (size_t)ns});
-  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb), dim3(BLK), 0, S, C->geo, C->u[0], C->relative, C->div[0]);
-  hipLaunchKernelGGL(swm::dev::k_center_ke, dim3(nb), dim3(BLK), 0, S, C->geo, C->u[0], C->ke);
-  gather(C, C->face, {C->relative, C->div[0], C->ke});
-  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, C->div[1], C->div[0], (size_t)ns);
-  hipLaunchKernelGGL(swm::dev::k_post_elliptic, dim3(nb), dim3(BLK), 0, S, C->geo, C->relative, C->eta[0]);
-  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, C->eta[1], C->eta[0], (size_t)ns);
-  for (int q = 0; q < C->nt; ++q) {
+  for (int q = 0; q < C->nt; ++q)
     for (int k = 0; k < 10; ++k)
-      std::memcpy(&tq[(size_t)k * C->iim * C->jjm], tracer + ((size_t)q + (size_t)C->nt * k) * C->iim * C->jjm,
-                  (size_t)C->iim * C->jjm * sizeof(double));
-    HIP_TRY(hipMemcpyAsync(C->tr[0][q], tq.data(), ns * sizeof(double), hipMemcpyHostToDevice, S));
-    HIP_TRY(hipStreamSynchronize(S));   // tq is reused
-    hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, C->tr[1][q], C->tr[0][q], (size_t)ns);
-  }
-  wrap_prognostics(C);
+      std::memcpy(&tq[(size_t)q * ns + (size_t)k * C->iim * C->jjm],
+                  tracer + ((size_t)q + (size_t)C->nt * k) * C->iim * C->jjm, (size_t)C->iim * C->jjm * sizeof(double));
+  reset_state(C);
+  HIP_TRY(hipMemcpyAsync(C->h[0], h, ns * sizeof(double), hipMemcpyHostToDevice, S));
+  HIP_TRY(hipMemcpyAsync(C->u[0], up.data(), 4 * (size_t)ns * sizeof(double), hipMemcpyHostToDevice, S));
+  if (hs) HIP_TRY(hipMemcpyAsync(C->hs, hs, ns * sizeof(double), hipMemcpyHostToDevice, S));
+  for (int q = 0; q < C->nt; ++q)
+    HIP_TRY(hipMemcpyAsync(C->tr[0][q], &tq[(size_t)q * ns], ns * sizeof(double), hipMemcpyHostToDevice, S));
   HIP_TRY(hipStreamSynchronize(S));
-  C->initialized = true;
-  return SWM_OK;
+  return finish_init(C);
+}
+
+// Seeded synthetic state generated on the device (same construction as the
+// host version in swm_mi355x/ics.py): harmonic sums of degree 1..lmax for
+// vorticity, divergence (10 % RMS) and thickness perturbation, corner velocity
+// from the model's own gradient scaled to max |u| = umax, Gaussian tracer bumps.
+swm_status swm_init_synthetic(swm_ctx* C, int seed_index, int lmax, double umax, double h0, double h_rms,
+                              int nbumps, double bump_radius) {
+  if (!C || lmax < 1 || lmax > 400 || nbumps < 0 || nbumps > 64) { swm::set_error("swm_init_synthetic: bad argument"); return SWM_ERR_ARG; }
+  HIP_TRY(hipSetDevice(C->device));
+  const int ns = C->ns;
+  hipStream_t S = C->stream;
+  const unsigned nb = nblk(ns);
+  const uint64_t s0 = 20261203ull + 100ull * (uint64_t)seed_index;
+  // coefficients in the order of the host version: stream s0+1 (vorticity),
+  // s0+2 (divergence), s0+3 (thickness); l = 1..lmax, m = -l..l
+  const int nlm = (lmax + 1) * (lmax + 1) - 1;
+  std::vector<double> coef(3 * (size_t)nlm);
+  for (int set = 0; set < 3; ++set) {
+    SplitMix r(s0 + 1 + set);
+    for (int l = 1; l <= lmax; ++l)
+      for (int m = -l; m <= l; ++m) coef[3 * (size_t)(l * l - 1 + l + m) + set] = r.normal();
+  }
+  reset_state(C);
+  // scratch: psi -> xout, zeta -> f1, chi -> xout+ns, delta -> f2, hp -> ml
+  double* dcoef = C->flx1;   // 3*ns >= 3*nlm for every supported level
+  if ((size_t)3 * nlm > (size_t)3 * ns) { swm::set_error("swm_init_synthetic: lmax too large for the grid"); return SWM_ERR_ARG; }
+  HIP_TRY(hipMemcpyAsync(dcoef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice, S));
+  double *psi = C->xout, *chi = C->xout + ns, *zeta = C->f1, *delta = C->f2, *hp = C->ml;
+  hipLaunchKernelGGL(swm::dev::k_harmonics, dim3(nb), dim3(BLK), 0, S, ns, C->pxyz, lmax, dcoef, psi, zeta, chi,
+                     delta, hp);
+  const double a2 = C->radius * C->radius;
+  hipLaunchKernelGGL(swm::dev::k_scale, dim3(nb), dim3(BLK), 0, S, psi, (size_t)ns, a2);
+  hipLaunchKernelGGL(swm::dev::k_scale, dim3(nb), dim3(BLK), 0, S, chi, (size_t)ns, a2);
+  auto sumsq = [&](const double* x, double* out) -> swm_status {
+    hipLaunchKernelGGL(swm::dev::k_sumsq_partial, dim3(C->npart), dim3(BLK), 0, S, x, C->mask, ns, C->part);
+    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(1), dim3(1), 0, S, C->part, C->npart, C->sums);
+    HIP_TRY(hipMemcpyAsync(C->host_sums, C->sums, sizeof(double), hipMemcpyDeviceToHost, S));
+    HIP_TRY(hipStreamSynchronize(S));
+    *out = C->host_sums[0];
+    return SWM_OK;
+  };
+  double sz, sd, sh;
+  swm_status st;
+  if ((st = sumsq(zeta, &sz)) || (st = sumsq(delta, &sd)) || (st = sumsq(hp, &sh))) return st;
+  const double nreal = (double)C->max_ig;
+  const double rz = std::sqrt(sz / nreal), rd = std::sqrt(sd / nreal), rh = std::sqrt(sh / nreal);
+  hipLaunchKernelGGL(swm::dev::k_synth_finish, dim3(nb), dim3(BLK), 0, S, C->geo, h0, h_rms / rh, hp, C->h[0],
+                     0.1 * rz / rd, chi);
+  hipLaunchKernelGGL(swm::dev::k_velocity, dim3(nb), dim3(BLK), 0, S, C->geo, psi, chi, C->u[0]);
+  hipLaunchKernelGGL(swm::dev::k_speedmax_partial, dim3(C->npart), dim3(BLK), 0, S, C->geo, C->u[0], C->part);
+  std::vector<double> pm(C->npart);
+  HIP_TRY(hipMemcpyAsync(pm.data(), C->part, C->npart * sizeof(double), hipMemcpyDeviceToHost, S));
+  HIP_TRY(hipStreamSynchronize(S));
+  double vmax = 0.0;
+  for (double v : pm) vmax = std::max(vmax, v);
+  hipLaunchKernelGGL(swm::dev::k_scale, dim3(nblk(4 * (size_t)ns)), dim3(BLK), 0, S, C->u[0], 4 * (size_t)ns, umax / vmax);
+  // tracer bumps: stream s0+4, one set of centres per tracer
+  SplitMix rt(s0 + 4);
+  for (int q = 0; q < C->nt; ++q) {
+    std::vector<double> cen(3 * (size_t)std::max(1, nbumps));
+    for (int b = 0; b < nbumps; ++b) {
+      const double zc = 2.0 * rt.uniform() - 1.0, lc = 2.0 * M_PI * rt.uniform();
+      const double rc = std::sqrt(std::max(0.0, 1.0 - zc * zc));
+      cen[3 * b] = rc * std::cos(lc); cen[3 * b + 1] = rc * std::sin(lc); cen[3 * b + 2] = zc;
+    }
+    HIP_TRY(hipMemcpyAsync(C->sums, cen.data(), cen.size() * sizeof(double), hipMemcpyHostToDevice, S));
+    hipLaunchKernelGGL(swm::dev::k_bumps, dim3(nb), dim3(BLK), 0, S, ns, C->pxyz, nbumps, C->sums, C->radius,
+                       bump_radius, C->tr[0][q]);
+    HIP_TRY(hipStreamSynchronize(S));
+  }
+  fill(C, C->xout, 2 * (size_t)ns, 0.0);   // the solver's warm start begins at zero
+  HIP_TRY(hipStreamSynchronize(S));
+  return finish_init(C);
 }
 
 swm_status swm_step(swm_ctx* C, int nsteps) {
@@ -554,6 +673,7 @@ swm_status swm_step(swm_ctx* C, int nsteps) {
   if (!C->initialized) { swm::set_error("swm_step: state not initialised"); return SWM_ERR_STATE; }
   HIP_TRY(hipSetDevice(C->device));
   C->kev_used = 0;
+  C->kernel_bytes = 0.0;
   HIP_TRY(hipEventRecord(C->ev_a, C->stream));
   for (int n = 0; n < nsteps; ++n) {
     C->step_count += 1;
@@ -618,6 +738,35 @@ swm_status swm_synchronize(swm_ctx* C) {
 
 int swm_step_count(const swm_ctx* C) { return C ? C->step_count : -1; }
 int swm_last_vcycles(const swm_ctx* C) { return C ? C->last_vcycles : -1; }
+
+swm_status swm_profile(swm_ctx* C, int enable) {
+  if (!C) return SWM_ERR_ARG;
+  HIP_TRY(hipSetDevice(C->device));
+  C->profile = enable != 0;
+  if (C->profile && C->kev.empty()) {
+    C->kev.resize(4096);
+    for (hipEvent_t& e : C->kev) HIP_TRY(hipEventCreate(&e));
+  }
+  return SWM_OK;
+}
+
+swm_status swm_timing(const swm_ctx* C, double* step_ms, double* kernel_ms_mean, double* kernel_bytes,
+                      int* kernel_launches) {
+  if (!C) return SWM_ERR_ARG;
+  if (step_ms) *step_ms = C->last_step_ms;
+  double tot = 0.0;
+  int n = 0;
+  for (size_t k = 0; k + 1 < C->kev_used; k += 2) {
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, C->kev[k], C->kev[k + 1]));
+    tot += ms;
+    ++n;
+  }
+  if (kernel_ms_mean) *kernel_ms_mean = n ? tot / n : 0.0;
+  if (kernel_bytes) *kernel_bytes = n ? C->kernel_bytes / n : 0.0;
+  if (kernel_launches) *kernel_launches = n;
+  return SWM_OK;
+}
 
 swm_status swm_total_mass(swm_ctx* C, double* mass) {
   if (!C || !mass) return SWM_ERR_ARG;

@@ -706,3 +706,123 @@ __global__ void k_inv_h(Geo G, const double* __restrict__ h4, double* __restrict
 
 }  // namespace dev
 }  // namespace swm
+
+namespace swm {
+namespace dev {
+
+// ------------------------------------------------- synthetic initial state --
+// Real 4-pi normalised spherical-harmonic sums at the cell centres p (3
+// planes): out[0] = sum c1 l^-1.5 Y / (-l(l+1)) (streamfunction shape),
+// out[1] = sum c1 l^-1.5 Y (vorticity shape), out[2], out[3] the same with c2
+// (velocity potential / divergence), out[4] = sum c3 l^-1.5 Y (thickness).
+// coef holds, for l = 1..L and m = -l..l, the three coefficient sets at
+// index 3*(l*l - 1 + l + m) + set.
+__global__ void k_harmonics(int ns, const double* __restrict__ p, int lmax, const double* __restrict__ coef,
+                            double* __restrict__ o0, double* __restrict__ o1, double* __restrict__ o2,
+                            double* __restrict__ o3, double* __restrict__ o4) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const double x = p[s], y = p[ns + s], z = p[2 * ns + s];
+  const double mu = fmin(1.0, fmax(-1.0, z));
+  const double cphi = sqrt(fmax(0.0, 1.0 - mu * mu));
+  const double lon = atan2(y, x);
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+  double pmm = 1.0;
+  for (int m = 0; m <= lmax; ++m) {
+    if (m == 1) pmm = sqrt(3.0) * cphi;
+    else if (m > 1) pmm = sqrt((2.0 * m + 1.0) / (2.0 * m)) * cphi * pmm;
+    double sm, cm;
+    sincos(m * lon, &sm, &cm);
+    double pl2 = 0.0, pl1 = 0.0;
+    for (int l = m; l <= lmax; ++l) {
+      double pl;
+      if (l == m) pl = pmm;
+      else if (l == m + 1) pl = sqrt(2.0 * m + 3.0) * mu * pmm;
+      else {
+        const double a = sqrt((2.0 * l - 1.0) * (2.0 * l + 1.0) / ((double)(l - m) * (l + m)));
+        const double b = sqrt((2.0 * l + 1.0) * (l + m - 1.0) * (l - m - 1.0) / ((double)(l - m) * (l + m) * (2.0 * l - 3.0)));
+        pl = a * mu * pl1 - b * pl2;
+      }
+      pl2 = pl1; pl1 = pl;
+      if (l < 1) continue;
+      const double w = pow((double)l, -1.5);
+      const double inv = -1.0 / (l * (l + 1.0));
+      const int ip = 3 * (l * l - 1 + l + m), in_ = 3 * (l * l - 1 + l - m);
+      double t1, t2, t3;
+      if (m == 0) {
+        t1 = coef[ip] * pl; t2 = coef[ip + 1] * pl; t3 = coef[ip + 2] * pl;
+      } else {
+        t1 = pl * (coef[ip] * cm + coef[in_] * sm);
+        t2 = pl * (coef[ip + 1] * cm + coef[in_ + 1] * sm);
+        t3 = pl * (coef[ip + 2] * cm + coef[in_ + 2] * sm);
+      }
+      a0 += w * inv * t1; a1 += w * t1; a2 += w * inv * t2; a3 += w * t2; a4 += w * t3;
+    }
+  }
+  o0[s] = a0; o1[s] = a1; o2[s] = a2; o3[s] = a3; o4[s] = a4;
+}
+
+// per-block partial sums of squares (masked) and maxima of |u| at corners
+__global__ void k_sumsq_partial(const double* __restrict__ x, const double* __restrict__ w, int n,
+                                double* __restrict__ part) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) acc += x[t] * x[t] * w[t];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ void k_speedmax_partial(Geo G, const double* __restrict__ u, double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  double acc = 0.0;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
+    const IJK c = ijk_of(s, iim, jjm);
+    if (!interior(c, iim, jjm)) continue;
+    for (int ic = 0; ic < 2; ++ic) {
+      const double a = u[ic * ns + s], b = u[(ic + 2) * ns + s];
+      acc = fmax(acc, sqrt(a * a + b * b));
+    }
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// thickness h0 + scale*hp at real cells (0 elsewhere); chi scaled in place;
+// u scaled; tracer = sum of Gaussian bumps (all slots)
+__global__ void k_synth_finish(Geo G, double h0, double hscale, const double* __restrict__ hp, double* __restrict__ h,
+                               double cscale, double* __restrict__ chi) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= G.ns) return;
+  h[s] = G.mask[s] == 1.0 ? h0 + hscale * hp[s] : 0.0;
+  chi[s] = chi[s] * cscale;
+}
+__global__ void k_scale(double* __restrict__ x, size_t n, double a) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] = x[t] * a;
+}
+__global__ void k_bumps(int ns, const double* __restrict__ p, int nb, const double* __restrict__ centres,
+                        double radius, double width, double* __restrict__ t) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const double x = p[s], y = p[ns + s], z = p[2 * ns + s];
+  double acc = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    const double d = radius * acos(fmin(1.0, fmax(-1.0, centres[3 * b] * x + centres[3 * b + 1] * y + centres[3 * b + 2] * z)));
+    const double r = d / width;
+    acc += exp(-(r * r));
+  }
+  t[s] = acc;
+}
+
+}  // namespace dev
+}  // namespace swm
