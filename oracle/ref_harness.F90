@@ -52,7 +52,8 @@ program ref_harness
   call get_command_argument(1, mode)
   call get_command_argument(2, outdir)
 
-  call harness_setup()
+  call setup_wrap()
+  if (trim(mode) /= 'wrap') call harness_setup()
 
   select case (trim(mode))
   case ('setup')
@@ -78,11 +79,9 @@ contains
          form='unformatted', status='replace')
   end subroutine open_out
 
-  ! setup order of initialize_model (initialize.F:82-140), minus output
-  subroutine harness_setup()
-    integer :: i, j, ksdm
-    real(kind=dbl_kind) :: s(2)
-
+  ! setup order of initialize_model (initialize.F:82-140), minus output;
+  ! the halo tables need no grid file
+  subroutine setup_wrap()
     call init_swm_parameters()
     call initialize_component(swm_name)
     call initialize_params_horizontal_xxx()
@@ -92,6 +91,12 @@ contains
     call set_assign_owner(isdm, jsdm, max_level, max_level, &
          partition_factor, swm_name, './assign_file')
     call initialize_wrap(swm_name)
+  end subroutine setup_wrap
+
+  subroutine harness_setup()
+    integer :: i, j, ksdm
+    real(kind=dbl_kind) :: s(2)
+
     call set_grid_center_mask(swm_name, im, jm, iim, jjm, nsdm, &
          grid_center_mask, grid_center_index, polygon_type)
     call set_grid_corner_mask(swm_name, ncorners, iim, jjm, nsdm, &

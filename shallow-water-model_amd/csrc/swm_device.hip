@@ -415,6 +415,50 @@ swm_status finish_init(swm_ctx* C) {
   return SWM_OK;
 }
 
+
+// ---------------------------------------- host-array operator entry points --
+// (ncomp, iim, jjm, 10) reference layout <-> ncomp device planes
+void ref_to_planes(const double* ref, int ncomp, int ns, double* planes) {
+  for (int s = 0; s < ns; ++s)
+    for (int c = 0; c < ncomp; ++c) planes[(size_t)c * ns + s] = ref[(size_t)ncomp * s + c];
+}
+void planes_to_ref(const double* planes, int ncomp, int ns, double* ref) {
+  for (int s = 0; s < ns; ++s)
+    for (int c = 0; c < ncomp; ++c) ref[(size_t)ncomp * s + c] = planes[(size_t)c * ns + s];
+}
+
+struct OpIO {
+  swm_ctx* C;
+  std::vector<double> buf;
+  swm_status put(double* d, const double* ref, int ncomp, bool corner_vector = false) {
+    buf.assign((size_t)ncomp * C->ns, 0.0);
+    if (corner_vector) u_to_planes(ref, buf.data(), C->ns);
+    else ref_to_planes(ref, ncomp, C->ns, buf.data());
+    HIP_TRY(hipMemcpy(d, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
+    return SWM_OK;
+  }
+  swm_status get(double* ref, const double* d, int ncomp, bool corner_vector = false) {
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    HIP_TRY(hipGetLastError());
+    buf.assign((size_t)ncomp * C->ns, 0.0);
+    HIP_TRY(hipMemcpy(buf.data(), d, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (corner_vector) u_from_planes(buf.data(), ref, C->ns);
+    else planes_to_ref(buf.data(), ncomp, C->ns, ref);
+    return SWM_OK;
+  }
+};
+
+// (iim, jjm, nvar, 10) <-> nvar planes
+void vars_to_planes(const double* ref, int nvar, int pl, double* planes) {
+  for (int k = 0; k < 10; ++k)
+    for (int v = 0; v < nvar; ++v)
+      std::memcpy(planes + (size_t)v * pl * 10 + (size_t)k * pl, ref + ((size_t)v + (size_t)nvar * k) * pl, pl * sizeof(double));
+}
+void planes_to_vars(const double* planes, int nvar, int pl, double* ref) {
+  for (int k = 0; k < 10; ++k)
+    for (int v = 0; v < nvar; ++v)
+      std::memcpy(ref + ((size_t)v + (size_t)nvar * k) * pl, planes + (size_t)v * pl * 10 + (size_t)k * pl, pl * sizeof(double));
+}
 }  // namespace
 
 extern "C" {
@@ -776,6 +820,190 @@ swm_status swm_total_mass(swm_ctx* C, double* mass) {
   HIP_TRY(hipMemcpyAsync(C->host_sums, C->sums, sizeof(double), hipMemcpyDeviceToHost, C->stream));
   HIP_TRY(hipStreamSynchronize(C->stream));
   *mass = C->host_sums[0];
+  return SWM_OK;
+}
+
+#define OP_BEGIN(C)                                                              \
+  if (!(C)) { swm::set_error("null context"); return SWM_ERR_ARG; }             \
+  HIP_TRY(hipSetDevice((C)->device));                                           \
+  HIP_TRY(hipStreamSynchronize((C)->stream));                                   \
+  OpIO io{(C), {}};                                                             \
+  swm_status st_;                                                               \
+  const unsigned nb_ = nblk((C)->ns);                                           \
+  (void)nb_;
+#define OP_TRY(x) if ((st_ = (x))) return st_
+
+swm_status swm_op_div_of_an_edge_flux(swm_ctx* C, const double* xin, double* xout) {
+  OP_BEGIN(C);
+  if (!xin || !xout) return SWM_ERR_ARG;
+  OP_TRY(io.put(C->vf, xin, 3));
+  hipLaunchKernelGGL(swm::dev::k_op_div_edge, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->vf, C->r2);
+  return io.get(xout, C->r2, 1);
+}
+
+swm_status swm_op_laplacian_of_a_scalar(swm_ctx* C, const double* xin, double* xout) {
+  OP_BEGIN(C);
+  if (!xin || !xout) return SWM_ERR_ARG;
+  OP_TRY(io.put(C->ml, xin, 1));
+  hipLaunchKernelGGL(swm::dev::k_op_lap, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->ml, C->r2);
+  return io.get(xout, C->r2, 1);
+}
+
+swm_status swm_op_curl_of_a_vector(swm_ctx* C, const double* xin, double* xout) {
+  OP_BEGIN(C);
+  if (!xin || !xout) return SWM_ERR_ARG;
+  OP_TRY(io.put(C->f1, xin, 4, true));
+  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->f1, C->r2, (double*)nullptr);
+  return io.get(xout, C->r2, 1);
+}
+
+swm_status swm_op_div_of_a_vector(swm_ctx* C, const double* xin, double* xout) {
+  OP_BEGIN(C);
+  if (!xin || !xout) return SWM_ERR_ARG;
+  OP_TRY(io.put(C->f1, xin, 4, true));
+  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->f1, (double*)nullptr, C->r2);
+  return io.get(xout, C->r2, 1);
+}
+
+swm_status swm_op_calculate_center_kinetic_energy(swm_ctx* C, const double* ke_c, double* ke) {
+  OP_BEGIN(C);
+  if (!ke_c || !ke) return SWM_ERR_ARG;
+  OP_TRY(io.put(C->f1, ke_c, 2));
+  hipLaunchKernelGGL(swm::dev::k_op_center_ke, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->f1, C->r2);
+  return io.get(ke, C->r2, 1);
+}
+
+swm_status swm_op_average_to_cell_corners(swm_ctx* C, const double* xin, double* xout) {
+  OP_BEGIN(C);
+  if (!xin || !xout) return SWM_ERR_ARG;
+  OP_TRY(io.put(C->ml, xin, 1));
+  hipLaunchKernelGGL(swm::dev::k_avg_corners, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->sw, C->ml, C->ml, C->f1,
+                     C->f2);
+  return io.get(xout, C->f1, 2);
+}
+
+swm_status swm_op_calculate_velocity(swm_ctx* C, const double* psi, const double* chi, double* u) {
+  OP_BEGIN(C);
+  if (!psi || !chi || !u) return SWM_ERR_ARG;
+  OP_TRY(io.put(C->ml, psi, 1));
+  OP_TRY(io.put(C->r2, chi, 1));
+  hipLaunchKernelGGL(swm::dev::k_velocity, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->ml, C->r2, C->f1);
+  return io.get(u, C->f1, 4, true);
+}
+
+swm_status swm_op_centered_mass_flux(swm_ctx* C, const double* mass_c, const double* u, double* flux) {
+  OP_BEGIN(C);
+  if (!mass_c || !u || !flux) return SWM_ERR_ARG;
+  OP_TRY(io.put(C->hc, mass_c, 2));
+  OP_TRY(io.put(C->f1, u, 4, true));
+  fill(C, C->ml, C->ns, 0.0);
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->f1, C->ml, C->hc, C->f2, C->vf);
+  return io.get(flux, C->vf, 3);
+}
+
+swm_status swm_op_upstream_mass_flux(swm_ctx* C, const double* u, const double* mass, double* flux) {
+  OP_BEGIN(C);
+  if (!u || !mass || !flux) return SWM_ERR_ARG;
+  OP_TRY(io.put(C->f1, u, 4, true));
+  OP_TRY(io.put(C->ml, mass, 1));
+  fill(C, C->hc, 2 * (size_t)C->ns, 0.0);
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->f1, C->ml, C->hc, C->f2, C->vf);
+  return io.get(flux, C->f2, 3);
+}
+
+swm_status swm_op_centered_vorticity_flux(swm_ctx* C, const double* mass_flux, const double* mass, const double* eta,
+                                          double* flux) {
+  OP_BEGIN(C);
+  if (!mass_flux || !mass || !eta || !flux) return SWM_ERR_ARG;
+  OP_TRY(io.put(C->f2, mass_flux, 3));
+  OP_TRY(io.put(C->ml, mass, 1));
+  OP_TRY(io.put(C->r2, eta, 1));
+  fill(C, C->hc, 2 * (size_t)C->ns, 0.0);
+  fill(C, C->f1, 4 * (size_t)C->ns, 0.0);
+  hipLaunchKernelGGL(swm::dev::k_vort_curl, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->f2, C->ml, C->r2, C->hc, C->f1,
+                     C->vf, C->dcurl);
+  return io.get(flux, C->vf, 3);
+}
+
+swm_status swm_op_centered_divergence_curl(swm_ctx* C, const double* mass_c, const double* eta, const double* u,
+                                           const double* mass, double* flux) {
+  OP_BEGIN(C);
+  if (!mass_c || !eta || !u || !mass || !flux) return SWM_ERR_ARG;
+  OP_TRY(io.put(C->hc, mass_c, 2));
+  OP_TRY(io.put(C->r2, eta, 1));
+  OP_TRY(io.put(C->f1, u, 4, true));
+  OP_TRY(io.put(C->ml, mass, 1));
+  fill(C, C->f2, 3 * (size_t)C->ns, 0.0);
+  hipLaunchKernelGGL(swm::dev::k_vort_curl, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->f2, C->ml, C->r2, C->hc, C->f1,
+                     C->vf, C->dcurl);
+  return io.get(flux, C->dcurl, 3);
+}
+
+swm_status swm_wrap(swm_ctx* C, int kind, int nspace, int nlayer, double* x) {
+  OP_BEGIN(C);
+  if (!x || kind < 0 || kind > 2 || nlayer < 1 || nspace < 1 || (kind == 0 && nspace != 1) ||
+      (kind == 2 && nspace != 1) || nspace > 4) {
+    swm::set_error("swm_wrap: bad argument");
+    return SWM_ERR_ARG;
+  }
+  const int ncomp = kind == 0 ? 1 : (kind == 1 ? 2 : 3);
+  const DevTable& t = kind == 0 ? C->face : (kind == 1 ? C->vrtx : C->edge);
+  const int ns = C->ns, pl = C->iim * C->jjm;
+  // reference layout (nspace, ncomp, iim, jjm, nlayer, 10)
+  const size_t per = (size_t)nspace * ncomp;
+  std::vector<double> planes((size_t)ncomp * ns);
+  for (int l = 0; l < nlayer; ++l)
+    for (int sp = 0; sp < nspace; ++sp) {
+      for (int k = 0; k < 10; ++k)
+        for (int q = 0; q < pl; ++q)
+          for (int c = 0; c < ncomp; ++c)
+            planes[(size_t)c * ns + (size_t)k * pl + q] = x[sp + nspace * (c + ncomp * ((size_t)q + (size_t)pl * (l + (size_t)nlayer * k)))];
+      HIP_TRY(hipMemcpy(C->f1, planes.data(), planes.size() * sizeof(double), hipMemcpyHostToDevice));
+      gather(C, t, {C->f1});
+      HIP_TRY(hipStreamSynchronize(C->stream));
+      HIP_TRY(hipMemcpy(planes.data(), C->f1, planes.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (int k = 0; k < 10; ++k)
+        for (int q = 0; q < pl; ++q)
+          for (int c = 0; c < ncomp; ++c)
+            x[sp + nspace * (c + ncomp * ((size_t)q + (size_t)pl * (l + (size_t)nlayer * k)))] = planes[(size_t)c * ns + (size_t)k * pl + q];
+    }
+  (void)per;
+  return SWM_OK;
+}
+
+swm_status swm_elliptic(swm_ctx* C, int nvar, double* rhs, double* xout) {
+  OP_BEGIN(C);
+  if (!rhs || !xout || nvar < 1 || nvar > 2) { swm::set_error("swm_elliptic: nvar must be 1 or 2"); return SWM_ERR_ARG; }
+  const int pl = C->iim * C->jjm;
+  std::vector<double> b((size_t)nvar * C->ns);
+  // rhs -> f1 planes, xout -> f2 planes
+  vars_to_planes(rhs, nvar, pl, b.data());
+  HIP_TRY(hipMemcpy(C->f1, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
+  vars_to_planes(xout, nvar, pl, b.data());
+  HIP_TRY(hipMemcpy(C->f2, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
+  swm_status st = elliptic(C, C->f1, C->f2, nvar);
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  HIP_TRY(hipMemcpy(b.data(), C->f1, b.size() * sizeof(double), hipMemcpyDeviceToHost));
+  planes_to_vars(b.data(), nvar, pl, rhs);
+  HIP_TRY(hipMemcpy(b.data(), C->f2, b.size() * sizeof(double), hipMemcpyDeviceToHost));
+  planes_to_vars(b.data(), nvar, pl, xout);
+  return st;
+}
+
+swm_status swm_multigrid(swm_ctx* C, int nvar, const double* beta, double* alpha) {
+  OP_BEGIN(C);
+  if (!beta || !alpha || nvar < 1 || nvar > 2) { swm::set_error("swm_multigrid: nvar must be 1 or 2"); return SWM_ERR_ARG; }
+  const int pl = C->iim * C->jjm;
+  std::vector<double> b((size_t)nvar * C->ns);
+  vars_to_planes(beta, nvar, pl, b.data());
+  HIP_TRY(hipMemcpy(C->f1, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
+  vars_to_planes(alpha, nvar, pl, b.data());
+  HIP_TRY(hipMemcpy(C->f2, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
+  vcycle(C, nvar, C->f2, C->f1);
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(b.data(), C->f2, b.size() * sizeof(double), hipMemcpyDeviceToHost));
+  planes_to_vars(b.data(), nvar, pl, alpha);
   return SWM_OK;
 }
 

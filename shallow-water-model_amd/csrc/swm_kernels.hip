@@ -826,3 +826,39 @@ __global__ void k_bumps(int ns, const double* __restrict__ p, int nb, const doub
 
 }  // namespace dev
 }  // namespace swm
+
+namespace swm {
+namespace dev {
+
+// single reference operators on every slot (for the operator entry points)
+__global__ void k_op_div_edge(Geo G, const double* __restrict__ F, double* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= G.ns) return;
+  const IJK c = ijk_of(s, G.iim, G.jjm);
+  out[s] = div_edge_at(G.iim, G.jjm, G.ns, c, F, G.area_inv[s]);
+}
+__global__ void k_op_lap(Geo G, const double* __restrict__ x, double* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= G.ns) return;
+  const IJK c = ijk_of(s, G.iim, G.jjm);
+  auto X = [&](int sl) { return x[sl]; };
+  out[s] = lap_at(G.iim, G.jjm, G.ns, c, G.lw, X);
+}
+// operators.F:1316-1393 with corner values given
+__global__ void k_op_center_ke(Geo G, const double* __restrict__ kec, double* __restrict__ ke) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  if (i > iim - 1 || j > jjm - 1) { ke[s] = 0.0; return; }
+  const int se = (i + 1) - 1 + iim * ((j) - 1 + jjm * (k - 1)), sne = (i + 1) - 1 + iim * ((j + 1) - 1 + jjm * (k - 1)),
+            sn = (i) - 1 + iim * ((j + 1) - 1 + jjm * (k - 1));
+#define SA(kk, ic, sl) G.sub_area[((kk) + 3 * (ic)) * ns + (sl)]
+  ke[s] = (+SA(0, 0, s) * kec[s] + SA(0, 1, s) * kec[ns + s] + SA(1, 0, se) * kec[se] + SA(1, 1, sne) * kec[ns + sne] +
+           SA(2, 0, sne) * kec[sne] + SA(2, 1, sn) * kec[ns + sn]) * G.area_inv[s];
+#undef SA
+}
+
+}  // namespace dev
+}  // namespace swm

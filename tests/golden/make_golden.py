@@ -1,0 +1,190 @@
+"""Writes the golden vectors under tests/golden/ from the compiled reference.
+
+Runs only where the reference sources are present (the development
+container): it builds oracle/_ref/g<g>/ref_harness (oracle/Makefile.ref),
+the unmodified reference Fortran modules driven by our own harness, runs it
+and stores its outputs as compressed npz fixtures:
+
+  gridgen_g5.npz   cell centres of the reference grid generator (its source
+                   is fixed at level 5), 20-decimal text as printed
+  wrap_g<g>.npz    halo gather tables probed from the reference wrap
+                   (g = 1..8): every slot holds a unique id before the wrap
+  setup_g<g>.npz   metrics, masks and index maps (g = 3 generated grid,
+                   g = 4 the shipped optimised grid, whose points are stored)
+  ops_g3.npz       one application of each operator on pseudo-random input
+  mg_g4.npz        one V-cycle and one elliptic solve
+  run_g<g>.npz     initial state and the state after selected steps of the
+                   full model (g = 3, dt = 1800 s; g = 4, dt = 900 s), 5 tracers
+
+    python tests/golden/make_golden.py
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "shallow-water-model_amd"))
+sys.path.insert(0, ROOT)
+
+REF = "/root/reference/doe_ca"
+ORACLE = os.path.join(ROOT, "oracle")
+RES = {1: "0000042", 2: "0000162", 3: "0000642", 4: "0002562", 5: "0010242", 6: "0040962", 7: "0163842",
+       8: "0655362", 9: "2621442"}
+
+
+def harness(g):
+    subprocess.run(["make", "-C", ORACLE, "-f", "Makefile.ref", f"G={g}"], check=True, stdout=subprocess.DEVNULL)
+    return os.path.join(ORACLE, "_ref", f"g{g}", "ref_harness")
+
+
+def rundir(g, points):
+    d = os.path.join(ORACLE, "_ref", f"run_g{g}")
+    os.makedirs(os.path.join(d, "data"), exist_ok=True)
+    os.makedirs(os.path.join(d, "out"), exist_ok=True)
+    with open(os.path.join(d, "data", f"grid_centers_{RES[g]}"), "w") as f:
+        for k in range(points.shape[1]):
+            f.write("%.17e %.17e %.17e\n" % tuple(points[:, k]))
+    return d
+
+
+def run(exe, d, *args, check=True):
+    return subprocess.run([exe] + list(args), cwd=d, check=check, stdout=subprocess.DEVNULL,
+                          stderr=subprocess.DEVNULL).returncode
+
+
+def read_setup(path, g):
+    iim = 2 + 2 ** g
+    ns = iim * iim * 10
+    raw = open(path, "rb").read()
+    off = 32
+    out = {}
+
+    def take(n, dt="<f8"):
+        nonlocal off
+        sz = np.dtype(dt).itemsize * n
+        a = np.frombuffer(raw[off:off + sz], dtype=dt).copy()
+        off += sz
+        return a
+
+    out["tot_area"] = take(1)
+    for name, k in [("p", 3), ("c", 6), ("area", 1), ("area_inv", 1), ("corner_area", 2), ("corner_area_inv", 2),
+                    ("sub_area", 6), ("dc", 6), ("normal", 18), ("tangent", 18), ("v_weights", 12),
+                    ("s_weights", 6), ("l_weights", 7), ("f", 1), ("grid_center_mask", 1),
+                    ("grid_corner_mask", 2), ("polygon_type", 1)]:
+        out[name] = take(k * ns)
+    out["grid_center_index"] = take(ns, "<i4")
+    out["grid_corner_index"] = take(2 * ns, "<i4")
+    return out
+
+
+def probe_tables(path, g):
+    """Gather tables from the probe: flat (component, slot) destination and
+    source, source -1 meaning zero (same convention as the library)."""
+    iim = 2 + 2 ** g
+    ns = iim * iim * 10
+    w = np.fromfile(path, dtype="<f8")
+    xf, xv, xe = w[:ns], w[ns:5 * ns], w[5 * ns:]
+    out = {}
+    ids = np.arange(1, ns + 1, dtype=float)
+    d = np.nonzero(xf != ids)[0]
+    out["face_dst"], out["face_src"] = d.astype(np.int32), (xf[d] - 1).astype(np.int32)
+    # corner probe has 2 space components per (corner, slot) entry; check they move together
+    v0, v1 = xv[0::2], xv[1::2]
+    idv = np.arange(1, 4 * ns + 1, dtype=float)
+    e0 = np.where(v0 == 0, 0, (v0 - 1) // 2 + 1)
+    e1 = np.where(v1 == 0, 0, (v1 - 2) // 2 + 1)
+    assert np.array_equal(e0, e1), "vertex wrap mixed the space components"
+    ent = np.arange(1, 2 * ns + 1, dtype=float)
+    d = np.nonzero(e0 != ent)[0]
+    out["vrtx_dst"], out["vrtx_src"] = d.astype(np.int32), (e0[d] - 1).astype(np.int32)
+    ide = np.arange(1, 3 * ns + 1, dtype=float)
+    d = np.nonzero(xe != ide)[0]
+    out["edge_dst"], out["edge_src"] = d.astype(np.int32), (xe[d] - 1).astype(np.int32)
+    del idv
+    return out
+
+
+def read_run(path, g, nt):
+    iim = 2 + 2 ** g
+    ns = iim * iim * 10
+    rec = 8 + 8 * (3 * ns + nt * ns + 4 * ns + 4 * ns)
+    raw = open(path, "rb").read()
+    states = {}
+    for k in range(len(raw) // rec):
+        off = k * rec
+        st = int(np.frombuffer(raw[off:off + 4], dtype="<i4")[0])
+        a = np.frombuffer(raw[off + 8:off + rec], dtype="<f8")
+        o = 0
+        d = {}
+        for name, n in [("h", ns), ("eta", ns), ("div", ns), ("tracer", nt * ns), ("u", 4 * ns), ("psi", ns),
+                        ("chi", ns), ("ke", ns), ("relative", ns)]:
+            d[name] = a[o:o + n].copy()
+            o += n
+        states[st] = d
+    return states
+
+
+def main():
+    if not os.path.isdir(REF):
+        sys.exit("make_golden.py needs the reference sources at " + REF)
+    import swm_mi355x as S
+    from swm_mi355x import ics
+
+    # reference grid generator (fixed at level 5 in its source)
+    subprocess.run(["make", "-C", ORACLE, "-f", "Makefile.ref", "gridgen"], check=True, stdout=subprocess.DEVNULL)
+    gd = os.path.join(ORACLE, "_ref", "gridgen")
+    subprocess.run(["./generate_grid"], cwd=gd, check=True, stdout=subprocess.DEVNULL)
+    pts = np.loadtxt(os.path.join(gd, "grid_centers_0010242")).T
+    np.savez_compressed(os.path.join(HERE, "gridgen_g5.npz"), p_list=pts)
+
+    shipped4 = np.loadtxt(os.path.join(REF, "swm", "run.0002562", "data", "grid_centers_0002562")).T
+    for g in range(1, 9):
+        exe = harness(g)
+        points = shipped4 if g == 4 else S.grid_points(g)
+        d = rundir(g, points)
+        run(exe, d, "wrap", "out")
+        np.savez_compressed(os.path.join(HERE, f"wrap_g{g}.npz"), **probe_tables(os.path.join(d, "out", "wrap.bin"), g))
+        print("wrap", g, flush=True)
+        if g in (3, 4):
+            run(exe, d, "setup", "out")
+            st = read_setup(os.path.join(d, "out", "setup.bin"), g)
+            if g == 4:
+                st["p_list"] = points
+            np.savez_compressed(os.path.join(HERE, f"setup_g{g}.npz"), **st)
+        if g == 3:
+            run(exe, d, "ops", "out")
+            raw = np.fromfile(os.path.join(d, "out", "ops.bin"), dtype="<f8")
+            np.savez_compressed(os.path.join(HERE, "ops_g3.npz"), raw=raw)
+        if g == 4:
+            run(exe, d, "mg", "out")
+            raw = np.fromfile(os.path.join(d, "out", "mg.bin"), dtype="<f8")
+            np.savez_compressed(os.path.join(HERE, "mg_g4.npz"), raw=raw)
+        if g in (3, 4):
+            grid = S.Grid(g, p_list=points)
+            nt = 5
+            ic = ics.synthetic_state(grid, ntracer=nt, lmax=8, seed_index=g)
+            with open(os.path.join(d, "ic.bin"), "wb") as f:
+                for k in ("h", "hs", "u", "tracer"):
+                    f.write(np.asfortranarray(ic[k]).tobytes(order="F"))
+            # g = 3 runs until the reference's elliptic solver gives up (it
+            # stops the program, elliptic_solver.F:176-178); the step it
+            # stops in is stored for the failure-path tests
+            dt, nsteps, nmax = (1800.0, 12, 40) if g == 3 else (900.0, 12, 12)
+            rc = run(exe, d, "run", "out", "ic.bin", repr(dt), str(nmax), "1", check=False)
+            states = read_run(os.path.join(d, "out", "run.bin"), g, nt)
+            keep = {f"ic_{k}": np.asarray(v) for k, v in ic.items()}
+            keep["dt"] = np.array(dt)
+            keep["nsteps"] = np.array(nsteps)
+            keep["abort_step"] = np.array(max(states) + 1 if (rc != 0 or max(states) < nmax) else -1)
+            for s in (1, 2, 3, nsteps):
+                for k, v in states[s].items():
+                    keep[f"s{s}_{k}"] = v
+            np.savez_compressed(os.path.join(HERE, f"run_g{g}.npz"), **keep)
+            print("run", g, flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,161 @@
+"""Host set-up and oracle against the golden vectors of the compiled reference
+(tests/golden/, written by tests/golden/make_golden.py).
+
+Bars: bit for bit everywhere -- grid points of our generator (to the
+reference generator's 20 printed decimals), metrics, masks and index maps,
+halo gather tables, single operators, one multigrid V-cycle, one elliptic
+solve and whole-model trajectories over 12 steps, on every slot including the
+halos -- because the oracle and the host set-up keep the reference's
+floating-point operation order.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import swm_mi355x as S
+from oracle.oracle import Oracle, grid_arrays_from
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+def grid_for(g):
+    if g == 4:
+        return S.Grid(4, p_list=gold("setup_g4.npz")["p_list"])
+    return S.Grid(g)
+
+
+def test_generator_matches_reference_generator():
+    # the reference generator is fixed at level 5 and prints 20 decimals
+    ref = gold("gridgen_g5.npz")["p_list"]
+    mine = S.grid_points(5)
+    assert mine.shape == ref.shape
+    assert np.abs(mine - ref).max() <= 1e-19
+
+
+@pytest.mark.parametrize("g", [3, 4])
+def test_setup_bit_exact(g):
+    ref = gold(f"setup_g{g}.npz")
+    grid = grid_for(g)
+    assert grid.array("tot_area") == ref["tot_area"][0]
+    for name in ("p", "c", "area", "area_inv", "corner_area", "corner_area_inv", "sub_area", "dc", "normal",
+                 "tangent", "v_weights", "s_weights", "l_weights", "f", "grid_center_mask", "grid_corner_mask",
+                 "polygon_type", "grid_center_index", "grid_corner_index"):
+        mine = grid.array(name).reshape(-1, order="F")
+        assert np.array_equal(mine, ref[name]), name
+
+
+@pytest.mark.parametrize("g", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_halo_tables_bit_exact(g):
+    ref = gold(f"wrap_g{g}.npz")
+    grid = S.Grid(g)
+    for kind in ("face", "vrtx", "edge"):
+        d, s = grid.array(kind + "_dst"), grid.array(kind + "_src")
+        o = np.argsort(d)
+        assert np.array_equal(d[o], ref[kind + "_dst"]), kind
+        assert np.array_equal(s[o], ref[kind + "_src"]), kind
+
+
+def test_halo_sources_are_never_destinations():
+    # the device fills halos with an in-place gather; this must be race free
+    grid = S.Grid(6)
+    for kind in ("face", "vrtx", "edge"):
+        d, s = grid.array(kind + "_dst"), grid.array(kind + "_src")
+        assert not np.isin(s[s >= 0], d).any()
+
+
+def _oracle(g, nt=1, dt=900.0):
+    grid = grid_for(g)
+    return grid, Oracle(grid_arrays_from(grid), nt, dt, S.REF_GRAV)
+
+
+def test_oracle_operators_bit_exact():
+    g = 3
+    iim = 2 + 2 ** g
+    ns = iim * iim * 10
+    raw = gold("ops_g3.npz")["raw"]
+    sizes = [("s1", ns), ("s2", ns), ("e1", 3 * ns), ("vc", 4 * ns), ("cc", 2 * ns), ("o1", ns), ("o2", ns),
+             ("o3", ns), ("o4", ns), ("o5", ns), ("oc", 2 * ns), ("ov", 4 * ns), ("oe", 3 * ns), ("oe2", 3 * ns),
+             ("oe3", 3 * ns), ("oe4", 3 * ns)]
+    a, o = {}, 0
+    for k, n in sizes:
+        a[k] = raw[o:o + n]
+        o += n
+    assert o == raw.size
+    _, orc = _oracle(g)
+    c = (iim, iim, 10)
+    e = (3,) + c
+    checks = [
+        ("orc_div_of_an_edge_flux", (a["e1"],), c, "o1"),
+        ("orc_laplacian", (a["s1"],), c, "o2"),
+        ("orc_curl_of_a_vector", (a["vc"],), c, "o3"),
+        ("orc_div_of_a_vector", (a["vc"],), c, "o4"),
+        ("orc_center_ke", (a["cc"],), c, "o5"),
+        ("orc_average_to_corners", (a["s1"],), (2,) + c, "oc"),
+        ("orc_velocity", (a["s1"], a["s2"]), (2, 2) + c, "ov"),
+        ("orc_centered_mass_flux", (a["cc"], a["vc"]), e, "oe"),
+        ("orc_upstream_mass_flux", (a["vc"], a["s2"]), e, "oe2"),
+        ("orc_vorticity_flux", (a["e1"], a["s2"], a["s1"]), e, "oe3"),
+        ("orc_divergence_curl", (a["cc"], a["s1"], a["vc"], a["s2"]), e, "oe4"),
+    ]
+    for fn, args, shape, ref in checks:
+        out = orc.op(fn, *args, out_shape=shape)
+        assert np.array_equal(out.reshape(-1, order="F"), a[ref]), fn
+
+
+def _mg_gold():
+    g = 4
+    iim = 2 + 2 ** g
+    n = iim * iim * 2 * 10
+    raw = gold("mg_g4.npz")["raw"]
+    shp = (iim, iim, 2, 10)
+    return [raw[k * n:(k + 1) * n].reshape(shp, order="F") for k in range(4)]
+
+
+def test_oracle_multigrid_vcycle_bit_exact():
+    beta, alpha, _, _ = _mg_gold()
+    _, orc = _oracle(4)
+    out = orc.multigrid(beta, np.zeros_like(beta))
+    assert np.array_equal(out, alpha)
+
+
+def test_oracle_elliptic_bit_exact():
+    beta, _, rhs, xo = _mg_gold()
+    _, orc = _oracle(4)
+    st, r, x = orc.elliptic(beta.copy(), np.zeros_like(beta))
+    assert st == 0
+    assert np.array_equal(r, rhs)
+    assert np.array_equal(x, xo)
+
+
+@pytest.mark.parametrize("g", [3, 4])
+def test_oracle_trajectory_bit_exact(g):
+    ref = gold(f"run_g{g}.npz")
+    nt = 5
+    dt = float(ref["dt"])
+    grid, orc = _oracle(g, nt, dt)
+    orc.init_state(ref["ic_h"], ref["ic_u"], ref["ic_tracer"])
+    done = 0
+    for s in (1, 2, 3, int(ref["nsteps"])):
+        orc.step(s - done)
+        done = s
+        for name in ("h", "eta", "div", "psi", "chi", "ke", "relative", "u", "tracer"):
+            mine = orc.field(name).reshape(-1, order="F")
+            assert np.array_equal(mine, ref[f"s{s}_{name}"]), f"step {s} {name}"
+
+
+def test_oracle_stops_where_the_reference_stops():
+    # with this state the reference's elliptic solver fails to reach its
+    # criterion in 12 V-cycles and stops the program (elliptic_solver.F:176-178)
+    ref = gold("run_g3.npz")
+    stop = int(ref["abort_step"])
+    assert stop > 0
+    _, orc = _oracle(3, 5, float(ref["dt"]))
+    orc.init_state(ref["ic_h"], ref["ic_u"], ref["ic_tracer"])
+    orc.step(stop - 1)
+    with pytest.raises(RuntimeError):
+        orc.step(1)
