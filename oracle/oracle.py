@@ -150,6 +150,13 @@ class Oracle:
         getattr(lib(), name)(self._h, *[_p(a) for a in arrs], _p(out))
         return out
 
+    def wrap(self, kind, x, nspace=1):
+        """Halo fill of one layer in place: kind "face" (iim,jjm,10), "vrtx"
+        (nspace,2,iim,jjm,10), "edge" (3,iim,jjm,10)."""
+        assert x.dtype == np.float64 and x.flags.f_contiguous
+        lib().orc_wrap(self._h, {"face": 0, "vrtx": 1, "edge": 2}[kind], nspace, _p(x))
+        return x
+
     def multigrid(self, beta, alpha):
         nvar = beta.shape[2]
         b = np.ascontiguousarray(np.moveaxis(np.asarray(beta, dtype=np.float64), 2, 0))  # (nvar, iim, jjm, 10)

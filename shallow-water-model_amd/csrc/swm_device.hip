@@ -427,23 +427,48 @@ void planes_to_ref(const double* planes, int ncomp, int ns, double* ref) {
     for (int c = 0; c < ncomp; ++c) ref[(size_t)ncomp * s + c] = planes[(size_t)c * ns + s];
 }
 
+template <class... T>
+bool all_set(const T*... p) {
+  return ((p != nullptr) && ...);
+}
+
+// device buffers of one operator call (allocated per call, freed on return;
+// the model state and the step's scratch are not touched)
 struct OpIO {
   swm_ctx* C;
-  std::vector<double> buf;
-  swm_status put(double* d, const double* ref, int ncomp, bool corner_vector = false) {
-    buf.assign((size_t)ncomp * C->ns, 0.0);
-    if (corner_vector) u_to_planes(ref, buf.data(), C->ns);
-    else ref_to_planes(ref, ncomp, C->ns, buf.data());
-    HIP_TRY(hipMemcpy(d, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice));
-    return SWM_OK;
+  std::vector<double*> bufs;
+  std::vector<double> host;
+  ~OpIO() {
+    for (double* p : bufs) (void)hipFree(p);
+  }
+  double* dev(int ncomp) {
+    double* p = nullptr;
+    if (hipMalloc(&p, (size_t)ncomp * C->ns * sizeof(double)) != hipSuccess) return nullptr;
+    bufs.push_back(p);
+    return p;
+  }
+  // host reference layout -> fresh device planes (nullptr on failure)
+  double* put(const double* ref, int ncomp, bool corner_vector = false) {
+    double* d = dev(ncomp);
+    if (!d) return nullptr;
+    host.assign((size_t)ncomp * C->ns, 0.0);
+    if (corner_vector) u_to_planes(ref, host.data(), C->ns);
+    else ref_to_planes(ref, ncomp, C->ns, host.data());
+    if (hipMemcpy(d, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return d;
+  }
+  double* zeros(int ncomp) {
+    double* d = dev(ncomp);
+    if (d) fill(C, d, (size_t)ncomp * C->ns, 0.0);
+    return d;
   }
   swm_status get(double* ref, const double* d, int ncomp, bool corner_vector = false) {
-    HIP_TRY(hipStreamSynchronize(C->stream));
     HIP_TRY(hipGetLastError());
-    buf.assign((size_t)ncomp * C->ns, 0.0);
-    HIP_TRY(hipMemcpy(buf.data(), d, buf.size() * sizeof(double), hipMemcpyDeviceToHost));
-    if (corner_vector) u_from_planes(buf.data(), ref, C->ns);
-    else planes_to_ref(buf.data(), ncomp, C->ns, ref);
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    host.assign((size_t)ncomp * C->ns, 0.0);
+    HIP_TRY(hipMemcpy(host.data(), d, host.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (corner_vector) u_from_planes(host.data(), ref, C->ns);
+    else planes_to_ref(host.data(), ncomp, C->ns, ref);
     return SWM_OK;
   }
 };
@@ -606,14 +631,15 @@ swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out
 
 void swm_destroy(swm_ctx* C) {
   if (!C) return;
-  hipSetDevice(C->device);
-  if (C->stream) hipStreamSynchronize(C->stream);
-  for (hipEvent_t e : C->kev) hipEventDestroy(e);
-  if (C->ev_a) hipEventDestroy(C->ev_a);
-  if (C->ev_b) hipEventDestroy(C->ev_b);
-  if (C->arena.base) hipFree(C->arena.base);
-  if (C->host_sums) hipHostFree(C->host_sums);
-  if (C->stream) hipStreamDestroy(C->stream);
+  // teardown: errors here have no caller to report to
+  (void)hipSetDevice(C->device);
+  if (C->stream) (void)hipStreamSynchronize(C->stream);
+  for (hipEvent_t e : C->kev) (void)hipEventDestroy(e);
+  if (C->ev_a) (void)hipEventDestroy(C->ev_a);
+  if (C->ev_b) (void)hipEventDestroy(C->ev_b);
+  if (C->arena.base) (void)hipFree(C->arena.base);
+  if (C->host_sums) (void)hipHostFree(C->host_sums);
+  if (C->stream) (void)hipStreamDestroy(C->stream);
   delete C;
 }
 
@@ -827,182 +853,193 @@ swm_status swm_total_mass(swm_ctx* C, double* mass) {
   if (!(C)) { swm::set_error("null context"); return SWM_ERR_ARG; }             \
   HIP_TRY(hipSetDevice((C)->device));                                           \
   HIP_TRY(hipStreamSynchronize((C)->stream));                                   \
-  OpIO io{(C), {}};                                                             \
-  swm_status st_;                                                               \
+  OpIO io{(C), {}, {}};                                                         \
   const unsigned nb_ = nblk((C)->ns);                                           \
   (void)nb_;
-#define OP_TRY(x) if ((st_ = (x))) return st_
+#define OP_ARGS(...)                                                             \
+  if (!all_set(__VA_ARGS__)) { swm::set_error("null array argument"); return SWM_ERR_ARG; }
+#define OP_BUF(x)                                                                \
+  if (!(x)) { swm::set_error("operator buffers: device allocation or copy failed"); return SWM_ERR_HIP; }
 
 swm_status swm_op_div_of_an_edge_flux(swm_ctx* C, const double* xin, double* xout) {
   OP_BEGIN(C);
-  if (!xin || !xout) return SWM_ERR_ARG;
-  OP_TRY(io.put(C->vf, xin, 3));
-  hipLaunchKernelGGL(swm::dev::k_op_div_edge, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->vf, C->r2);
-  return io.get(xout, C->r2, 1);
+  OP_ARGS(xin, xout);
+  double *x = io.put(xin, 3), *y = io.dev(1);
+  OP_BUF(x && y);
+  hipLaunchKernelGGL(swm::dev::k_op_div_edge, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, x, y);
+  return io.get(xout, y, 1);
 }
 
 swm_status swm_op_laplacian_of_a_scalar(swm_ctx* C, const double* xin, double* xout) {
   OP_BEGIN(C);
-  if (!xin || !xout) return SWM_ERR_ARG;
-  OP_TRY(io.put(C->ml, xin, 1));
-  hipLaunchKernelGGL(swm::dev::k_op_lap, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->ml, C->r2);
-  return io.get(xout, C->r2, 1);
+  OP_ARGS(xin, xout);
+  double *x = io.put(xin, 1), *y = io.dev(1);
+  OP_BUF(x && y);
+  hipLaunchKernelGGL(swm::dev::k_op_lap, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, x, y);
+  return io.get(xout, y, 1);
 }
 
 swm_status swm_op_curl_of_a_vector(swm_ctx* C, const double* xin, double* xout) {
   OP_BEGIN(C);
-  if (!xin || !xout) return SWM_ERR_ARG;
-  OP_TRY(io.put(C->f1, xin, 4, true));
-  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->f1, C->r2, (double*)nullptr);
-  return io.get(xout, C->r2, 1);
+  OP_ARGS(xin, xout);
+  double *x = io.put(xin, 4, true), *y = io.dev(1);
+  OP_BUF(x && y);
+  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, x, y, (double*)nullptr);
+  return io.get(xout, y, 1);
 }
 
 swm_status swm_op_div_of_a_vector(swm_ctx* C, const double* xin, double* xout) {
   OP_BEGIN(C);
-  if (!xin || !xout) return SWM_ERR_ARG;
-  OP_TRY(io.put(C->f1, xin, 4, true));
-  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->f1, (double*)nullptr, C->r2);
-  return io.get(xout, C->r2, 1);
+  OP_ARGS(xin, xout);
+  double *x = io.put(xin, 4, true), *y = io.dev(1);
+  OP_BUF(x && y);
+  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, x, (double*)nullptr, y);
+  return io.get(xout, y, 1);
 }
 
 swm_status swm_op_calculate_center_kinetic_energy(swm_ctx* C, const double* ke_c, double* ke) {
   OP_BEGIN(C);
-  if (!ke_c || !ke) return SWM_ERR_ARG;
-  OP_TRY(io.put(C->f1, ke_c, 2));
-  hipLaunchKernelGGL(swm::dev::k_op_center_ke, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->f1, C->r2);
-  return io.get(ke, C->r2, 1);
+  OP_ARGS(ke_c, ke);
+  double *x = io.put(ke_c, 2), *y = io.dev(1);
+  OP_BUF(x && y);
+  hipLaunchKernelGGL(swm::dev::k_op_center_ke, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, x, y);
+  return io.get(ke, y, 1);
 }
 
 swm_status swm_op_average_to_cell_corners(swm_ctx* C, const double* xin, double* xout) {
   OP_BEGIN(C);
-  if (!xin || !xout) return SWM_ERR_ARG;
-  OP_TRY(io.put(C->ml, xin, 1));
-  hipLaunchKernelGGL(swm::dev::k_avg_corners, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->sw, C->ml, C->ml, C->f1,
-                     C->f2);
-  return io.get(xout, C->f1, 2);
+  OP_ARGS(xin, xout);
+  double *x = io.put(xin, 1), *y = io.dev(2), *y2 = io.dev(2);
+  OP_BUF(x && y && y2);
+  hipLaunchKernelGGL(swm::dev::k_avg_corners, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->sw, x, x, y, y2);
+  return io.get(xout, y, 2);
 }
 
 swm_status swm_op_calculate_velocity(swm_ctx* C, const double* psi, const double* chi, double* u) {
   OP_BEGIN(C);
-  if (!psi || !chi || !u) return SWM_ERR_ARG;
-  OP_TRY(io.put(C->ml, psi, 1));
-  OP_TRY(io.put(C->r2, chi, 1));
-  hipLaunchKernelGGL(swm::dev::k_velocity, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->ml, C->r2, C->f1);
-  return io.get(u, C->f1, 4, true);
+  OP_ARGS(psi, chi, u);
+  double *p = io.put(psi, 1), *c = io.put(chi, 1), *y = io.dev(4);
+  OP_BUF(p && c && y);
+  hipLaunchKernelGGL(swm::dev::k_velocity, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, p, c, y);
+  return io.get(u, y, 4, true);
 }
 
 swm_status swm_op_centered_mass_flux(swm_ctx* C, const double* mass_c, const double* u, double* flux) {
   OP_BEGIN(C);
-  if (!mass_c || !u || !flux) return SWM_ERR_ARG;
-  OP_TRY(io.put(C->hc, mass_c, 2));
-  OP_TRY(io.put(C->f1, u, 4, true));
-  fill(C, C->ml, C->ns, 0.0);
-  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->f1, C->ml, C->hc, C->f2, C->vf);
-  return io.get(flux, C->vf, 3);
+  OP_ARGS(mass_c, u, flux);
+  double *mc = io.put(mass_c, 2), *uu = io.put(u, 4, true), *m = io.zeros(1), *up = io.dev(3), *y = io.dev(3);
+  OP_BUF(mc && uu && m && up && y);
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, uu, m, mc, up, y);
+  return io.get(flux, y, 3);
 }
 
 swm_status swm_op_upstream_mass_flux(swm_ctx* C, const double* u, const double* mass, double* flux) {
   OP_BEGIN(C);
-  if (!u || !mass || !flux) return SWM_ERR_ARG;
-  OP_TRY(io.put(C->f1, u, 4, true));
-  OP_TRY(io.put(C->ml, mass, 1));
-  fill(C, C->hc, 2 * (size_t)C->ns, 0.0);
-  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->f1, C->ml, C->hc, C->f2, C->vf);
-  return io.get(flux, C->f2, 3);
+  OP_ARGS(u, mass, flux);
+  double *uu = io.put(u, 4, true), *m = io.put(mass, 1), *mc = io.zeros(2), *y = io.dev(3), *cen = io.dev(3);
+  OP_BUF(uu && m && mc && y && cen);
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, uu, m, mc, y, cen);
+  return io.get(flux, y, 3);
 }
 
 swm_status swm_op_centered_vorticity_flux(swm_ctx* C, const double* mass_flux, const double* mass, const double* eta,
                                           double* flux) {
   OP_BEGIN(C);
-  if (!mass_flux || !mass || !eta || !flux) return SWM_ERR_ARG;
-  OP_TRY(io.put(C->f2, mass_flux, 3));
-  OP_TRY(io.put(C->ml, mass, 1));
-  OP_TRY(io.put(C->r2, eta, 1));
-  fill(C, C->hc, 2 * (size_t)C->ns, 0.0);
-  fill(C, C->f1, 4 * (size_t)C->ns, 0.0);
-  hipLaunchKernelGGL(swm::dev::k_vort_curl, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->f2, C->ml, C->r2, C->hc, C->f1,
-                     C->vf, C->dcurl);
-  return io.get(flux, C->vf, 3);
+  OP_ARGS(mass_flux, mass, eta, flux);
+  double *mf = io.put(mass_flux, 3), *m = io.put(mass, 1), *e = io.put(eta, 1), *mc = io.zeros(2),
+         *uu = io.zeros(4), *y = io.dev(3), *dc = io.dev(3);
+  OP_BUF(mf && m && e && mc && uu && y && dc);
+  hipLaunchKernelGGL(swm::dev::k_vort_curl, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, mf, m, e, mc, uu, y, dc);
+  return io.get(flux, y, 3);
 }
 
 swm_status swm_op_centered_divergence_curl(swm_ctx* C, const double* mass_c, const double* eta, const double* u,
                                            const double* mass, double* flux) {
   OP_BEGIN(C);
-  if (!mass_c || !eta || !u || !mass || !flux) return SWM_ERR_ARG;
-  OP_TRY(io.put(C->hc, mass_c, 2));
-  OP_TRY(io.put(C->r2, eta, 1));
-  OP_TRY(io.put(C->f1, u, 4, true));
-  OP_TRY(io.put(C->ml, mass, 1));
-  fill(C, C->f2, 3 * (size_t)C->ns, 0.0);
-  hipLaunchKernelGGL(swm::dev::k_vort_curl, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->f2, C->ml, C->r2, C->hc, C->f1,
-                     C->vf, C->dcurl);
-  return io.get(flux, C->dcurl, 3);
+  OP_ARGS(mass_c, eta, u, mass, flux);
+  double *mc = io.put(mass_c, 2), *e = io.put(eta, 1), *uu = io.put(u, 4, true), *m = io.put(mass, 1),
+         *mf = io.zeros(3), *vf = io.dev(3), *y = io.dev(3);
+  OP_BUF(mc && e && uu && m && mf && vf && y);
+  hipLaunchKernelGGL(swm::dev::k_vort_curl, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, mf, m, e, mc, uu, vf, y);
+  return io.get(flux, y, 3);
 }
 
 swm_status swm_wrap(swm_ctx* C, int kind, int nspace, int nlayer, double* x) {
   OP_BEGIN(C);
-  if (!x || kind < 0 || kind > 2 || nlayer < 1 || nspace < 1 || (kind == 0 && nspace != 1) ||
-      (kind == 2 && nspace != 1) || nspace > 4) {
-    swm::set_error("swm_wrap: bad argument");
+  if (!x || kind < 0 || kind > 2 || nlayer < 1 || nspace < 1 || nspace > 4 || (kind != 1 && nspace != 1)) {
+    swm::set_error("swm_wrap: kind must be 0 (face), 1 (vrtx) or 2 (edge); nspace > 1 only for vrtx");
     return SWM_ERR_ARG;
   }
   const int ncomp = kind == 0 ? 1 : (kind == 1 ? 2 : 3);
   const DevTable& t = kind == 0 ? C->face : (kind == 1 ? C->vrtx : C->edge);
   const int ns = C->ns, pl = C->iim * C->jjm;
-  // reference layout (nspace, ncomp, iim, jjm, nlayer, 10)
-  const size_t per = (size_t)nspace * ncomp;
+  double* d = io.dev(ncomp);
+  OP_BUF(d);
+  // reference layout (nspace, ncomp, iim, jjm, nlayer, 10); the gather table
+  // moves every space component of an entry together
   std::vector<double> planes((size_t)ncomp * ns);
+  auto at = [&](int sp, int c, int q, int l, int k) -> double& {
+    return x[sp + (size_t)nspace * (c + (size_t)ncomp * (q + (size_t)pl * (l + (size_t)nlayer * k)))];
+  };
   for (int l = 0; l < nlayer; ++l)
     for (int sp = 0; sp < nspace; ++sp) {
       for (int k = 0; k < 10; ++k)
         for (int q = 0; q < pl; ++q)
-          for (int c = 0; c < ncomp; ++c)
-            planes[(size_t)c * ns + (size_t)k * pl + q] = x[sp + nspace * (c + ncomp * ((size_t)q + (size_t)pl * (l + (size_t)nlayer * k)))];
-      HIP_TRY(hipMemcpy(C->f1, planes.data(), planes.size() * sizeof(double), hipMemcpyHostToDevice));
-      gather(C, t, {C->f1});
+          for (int c = 0; c < ncomp; ++c) planes[(size_t)c * ns + (size_t)k * pl + q] = at(sp, c, q, l, k);
+      HIP_TRY(hipMemcpy(d, planes.data(), planes.size() * sizeof(double), hipMemcpyHostToDevice));
+      gather(C, t, {d});
+      HIP_TRY(hipGetLastError());
       HIP_TRY(hipStreamSynchronize(C->stream));
-      HIP_TRY(hipMemcpy(planes.data(), C->f1, planes.size() * sizeof(double), hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(planes.data(), d, planes.size() * sizeof(double), hipMemcpyDeviceToHost));
       for (int k = 0; k < 10; ++k)
         for (int q = 0; q < pl; ++q)
-          for (int c = 0; c < ncomp; ++c)
-            x[sp + nspace * (c + ncomp * ((size_t)q + (size_t)pl * (l + (size_t)nlayer * k)))] = planes[(size_t)c * ns + (size_t)k * pl + q];
+          for (int c = 0; c < ncomp; ++c) at(sp, c, q, l, k) = planes[(size_t)c * ns + (size_t)k * pl + q];
     }
-  (void)per;
   return SWM_OK;
 }
 
 swm_status swm_elliptic(swm_ctx* C, int nvar, double* rhs, double* xout) {
   OP_BEGIN(C);
-  if (!rhs || !xout || nvar < 1 || nvar > 2) { swm::set_error("swm_elliptic: nvar must be 1 or 2"); return SWM_ERR_ARG; }
+  if (!rhs || !xout || nvar < 1 || nvar > 2) {
+    swm::set_error("swm_elliptic: nvar must be 1 or 2");
+    return SWM_ERR_ARG;
+  }
   const int pl = C->iim * C->jjm;
   std::vector<double> b((size_t)nvar * C->ns);
-  // rhs -> f1 planes, xout -> f2 planes
+  double *r = io.dev(nvar), *x = io.dev(nvar);
+  OP_BUF(r && x);
   vars_to_planes(rhs, nvar, pl, b.data());
-  HIP_TRY(hipMemcpy(C->f1, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(r, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
   vars_to_planes(xout, nvar, pl, b.data());
-  HIP_TRY(hipMemcpy(C->f2, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
-  swm_status st = elliptic(C, C->f1, C->f2, nvar);
+  HIP_TRY(hipMemcpy(x, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
+  const swm_status st = elliptic(C, r, x, nvar);
+  HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(C->stream));
-  HIP_TRY(hipMemcpy(b.data(), C->f1, b.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(b.data(), r, b.size() * sizeof(double), hipMemcpyDeviceToHost));
   planes_to_vars(b.data(), nvar, pl, rhs);
-  HIP_TRY(hipMemcpy(b.data(), C->f2, b.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(b.data(), x, b.size() * sizeof(double), hipMemcpyDeviceToHost));
   planes_to_vars(b.data(), nvar, pl, xout);
   return st;
 }
 
 swm_status swm_multigrid(swm_ctx* C, int nvar, const double* beta, double* alpha) {
   OP_BEGIN(C);
-  if (!beta || !alpha || nvar < 1 || nvar > 2) { swm::set_error("swm_multigrid: nvar must be 1 or 2"); return SWM_ERR_ARG; }
+  if (!beta || !alpha || nvar < 1 || nvar > 2) {
+    swm::set_error("swm_multigrid: nvar must be 1 or 2");
+    return SWM_ERR_ARG;
+  }
   const int pl = C->iim * C->jjm;
   std::vector<double> b((size_t)nvar * C->ns);
+  double *be = io.dev(nvar), *al = io.dev(nvar);
+  OP_BUF(be && al);
   vars_to_planes(beta, nvar, pl, b.data());
-  HIP_TRY(hipMemcpy(C->f1, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(be, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
   vars_to_planes(alpha, nvar, pl, b.data());
-  HIP_TRY(hipMemcpy(C->f2, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
-  vcycle(C, nvar, C->f2, C->f1);
-  HIP_TRY(hipStreamSynchronize(C->stream));
+  HIP_TRY(hipMemcpy(al, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
+  vcycle(C, nvar, al, be);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpy(b.data(), C->f2, b.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  HIP_TRY(hipMemcpy(b.data(), al, b.size() * sizeof(double), hipMemcpyDeviceToHost));
   planes_to_vars(b.data(), nvar, pl, alpha);
   return SWM_OK;
 }

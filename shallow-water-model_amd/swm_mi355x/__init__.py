@@ -82,6 +82,28 @@ _sig("swm_synchronize", ctypes.c_int, [ctypes.c_void_p])
 _sig("swm_step_count", ctypes.c_int, [ctypes.c_void_p])
 _sig("swm_last_vcycles", ctypes.c_int, [ctypes.c_void_p])
 _sig("swm_total_mass", ctypes.c_int, [ctypes.c_void_p, _dp])
+_sig("swm_wrap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p])
+_sig("swm_elliptic", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
+_sig("swm_multigrid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
+
+# reference operators: name -> (input kinds, output kind); kinds are cell "c",
+# corner scalar "cc" (2,...), corner vector "v" (2,2,...), edge "e" (3,...)
+OPERATORS = {
+    "div_of_an_edge_flux": (("e",), "c"),
+    "laplacian_of_a_scalar": (("c",), "c"),
+    "curl_of_a_vector": (("v",), "c"),
+    "div_of_a_vector": (("v",), "c"),
+    "calculate_center_kinetic_energy": (("cc",), "c"),
+    "average_to_cell_corners": (("c",), "cc"),
+    "calculate_velocity": (("c", "c"), "v"),
+    "centered_mass_flux": (("cc", "v"), "e"),
+    "upstream_mass_flux": (("v", "c"), "e"),
+    "centered_vorticity_flux": (("e", "c", "c"), "e"),
+    "centered_divergence_curl": (("cc", "c", "v", "c"), "e"),
+}
+for _n, (_ins, _o) in OPERATORS.items():
+    _sig("swm_op_" + _n, ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * (len(_ins) + 1))
+_KIND_LEAD = {"c": (), "cc": (2,), "v": (2, 2), "e": (3,)}
 
 
 def _check(status):
@@ -205,6 +227,54 @@ class Model:
     @property
     def last_vcycles(self):
         return lib.swm_last_vcycles(self._h)
+
+    def _shape(self, kind):
+        return _KIND_LEAD[kind] + (self.iim, self.jjm, 10)
+
+    def op(self, name, *args):
+        """One reference operator (``operators::<name>``) on host arrays of
+        the reference layout; metrics are the model's grid."""
+        ins, okind = OPERATORS[name]
+        if len(args) != len(ins):
+            raise TypeError(f"{name} takes {len(ins)} arrays")
+        arrs = [_f64(a, self._shape(k)) for a, k in zip(args, ins)]
+        out = np.zeros(self._shape(okind), dtype=np.float64, order="F")
+        _check(getattr(lib, "swm_op_" + name)(self._h, *[_ptr(a) for a in arrs], _ptr(out)))
+        return out
+
+    def wrap(self, kind, x, nspace=1):
+        """Halo fill (wrap_data::wrap) of x in place; kind "face" x is
+        (iim,jjm[,nlayer],10), "vrtx" (nspace,2,iim,jjm[,nlayer],10), "edge"
+        (3,iim,jjm[,nlayer],10).  Returns x."""
+        k = {"face": 0, "vrtx": 1, "edge": 2}[kind]
+        lead = {0: 1, 1: 2 * nspace, 2: 3}[k]
+        if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.f_contiguous):
+            raise TypeError("wrap works in place on a Fortran-ordered float64 array")
+        nl = x.size // (lead * self.iim * self.jjm * 10)
+        if nl * lead * self.iim * self.jjm * 10 != x.size:
+            raise ValueError("array size does not match the grid")
+        _check(lib.swm_wrap(self._h, k, nspace, nl, _ptr(x)))
+        return x
+
+    def elliptic(self, rhs, x0):
+        """elliptic_solver::elliptic on (iim,jjm,nvar,10) arrays; returns
+        (rhs less its area-weighted mean, solution)."""
+        r = np.array(rhs, dtype=np.float64, order="F", copy=True)
+        x = np.array(x0, dtype=np.float64, order="F", copy=True)
+        if r.shape != x.shape or r.shape[:2] != (self.iim, self.jjm) or r.shape[3] != 10:
+            raise ValueError("expected (iim, jjm, nvar, 10) arrays")
+        _check(lib.swm_elliptic(self._h, r.shape[2], _ptr(r), _ptr(x)))
+        return r, x
+
+    def multigrid(self, beta, alpha):
+        """One V-cycle (multigrid_solver::multigrid) on (iim,jjm,nvar,10)
+        arrays; returns the new alpha."""
+        b = _f64(beta)
+        a = np.array(alpha, dtype=np.float64, order="F", copy=True)
+        if b.shape != a.shape or b.shape[:2] != (self.iim, self.jjm) or b.shape[3] != 10:
+            raise ValueError("expected (iim, jjm, nvar, 10) arrays")
+        _check(lib.swm_multigrid(self._h, b.shape[2], _ptr(b), _ptr(a)))
+        return a
 
     def total_mass(self):
         m = ctypes.c_double()
