@@ -163,7 +163,16 @@ void fill(swm_ctx* C, double* x, size_t n, double v) {
 }
 
 // ------------------------------------------------------------- multigrid --
-void relax(swm_ctx* C, int l, int iters, double* alpha, const double* beta, int nvar) {
+template <int NV>
+void launch_relax(swm_ctx* C, const swm_ctx::Level& L, bool zero, const double* beta, const double* a, double* t) {
+  const dim3 grid(nblk(L.ns)), blk(BLK);
+  if (zero) hipLaunchKernelGGL((swm::dev::k_relax<NV, true>), grid, blk, 0, C->stream, L.L, beta, a, t);
+  else hipLaunchKernelGGL((swm::dev::k_relax<NV, false>), grid, blk, 0, C->stream, L.L, beta, a, t);
+}
+
+// iters Jacobi sweeps at level l (an even count leaves the result in alpha);
+// zero_start: alpha is known to be zero on entry
+void relax(swm_ctx* C, int l, int iters, double* alpha, const double* beta, int nvar, bool zero_start) {
   swm_ctx::Level& L = C->lev[l - 1];
   double* a = alpha;
   double* t = L.temp;
@@ -171,43 +180,53 @@ void relax(swm_ctx* C, int l, int iters, double* alpha, const double* beta, int 
     const bool prof = C->profile && l == C->g && C->kev_used + 2 <= C->kev.size();
     if (prof) {
       (void)hipEventRecord(C->kev[C->kev_used++], C->stream);
-      // algorithmic bytes of one finest-level sweep per real cell: Laplacian
-      // weights 7 + reciprocal centre weight 1, and per field beta, alpha and
-      // the result (neighbour values come from cache), all FP64
-      C->kernel_bytes += (double)C->max_ig * 8.0 * (8.0 + 3.0 * nvar);
+      // algorithmic bytes of one finest-level sweep per real cell: six
+      // Laplacian weights and the reciprocal centre weight, and per field
+      // beta, alpha and the result, all FP64 (neighbour values hit in cache)
+      C->kernel_bytes += (double)C->max_ig * 8.0 * (7.0 + 3.0 * nvar);
     }
-    hipLaunchKernelGGL(swm::dev::k_relax, dim3(nblk(L.ns), nvar), dim3(BLK), 0, C->stream, L.L, beta, a, t);
+    const bool zero = zero_start && it == 0;
+    if (nvar == 2) launch_relax<2>(C, L, zero, beta, a, t);
+    else launch_relax<1>(C, L, zero, beta, a, t);
     if (prof) (void)hipEventRecord(C->kev[C->kev_used++], C->stream);
     std::swap(a, t);
   }
-  // an even sweep count leaves the result in alpha
 }
 
+// multigrid_solver.F:476-591.  The coarse levels' alpha is zero at the start
+// of a cycle (the reference clears it); their first sweep is told so instead
+// of clearing memory.  Coarse beta is written by the injection at every slot
+// the smoother reads.
 void vcycle(swm_ctx* C, int nvar, double* alpha, const double* beta) {
   const int G = C->g;
-  for (int l = 1; l < G; ++l) {
-    fill(C, C->lev[l - 1].alpha, (size_t)nvar * C->lev[l - 1].ns, 0.0);
-    fill(C, C->lev[l - 1].beta, (size_t)nvar * C->lev[l - 1].ns, 0.0);
-  }
+  const int top = G < swm::dev::CO_MAXLEV ? G : swm::dev::CO_MAXLEV;   // levels 1..top in one workgroup
   auto A = [&](int l) { return l == G ? alpha : C->lev[l - 1].alpha; };
   auto B = [&](int l) { return l == G ? const_cast<double*>(beta) : C->lev[l - 1].beta; };
-  for (int l = G; l >= 2; --l) {
-    relax(C, l, 2, A(l), B(l), nvar);
+  for (int l = G; l > top; --l) {
+    relax(C, l, 2, A(l), B(l), nvar, l < G);
     hipLaunchKernelGGL(swm::dev::k_inject, dim3(nblk(C->lev[l - 2].ns), nvar), dim3(BLK), 0, C->stream,
                        C->lev[l - 1].L, C->lev[l - 2].L, A(l), B(l), B(l - 1));
   }
-  relax(C, 1, 24, A(1), B(1), nvar);
-  for (int l = 1; l <= G - 1; ++l) {
+  swm::dev::CoarseArgs P{};
+  for (int l = 1; l <= top; ++l) P.lev[l - 1] = C->lev[l - 1].L;
+  P.nlev = top;
+  P.nvar = nvar;
+  P.beta = B(top);
+  P.alpha = A(top);
+  P.zero = top < G ? 1 : 0;
+  hipLaunchKernelGGL(swm::dev::k_coarse_cycle, dim3(1), dim3(swm::dev::CO_THREADS), 0, C->stream, P);
+  for (int l = top; l <= G - 1; ++l) {
     hipLaunchKernelGGL(swm::dev::k_interp, dim3(nblk(C->lev[l].ns), nvar), dim3(BLK), 0, C->stream,
                        C->lev[l - 1].L, C->lev[l].L, A(l), A(l + 1));
-    relax(C, l + 1, 2, A(l + 1), B(l + 1), nvar);
+    relax(C, l + 1, 2, A(l + 1), B(l + 1), nvar, false);
   }
 }
 
-// sums of x0*w and x1*w into C->sums[0..1] (Neumaier over block partials)
+// sums of x0*w and x1*w into C->sums[0..1] (block partials, then a
+// compensated fixed-order sum)
 void sum2(swm_ctx* C, const double* x0, const double* x1, const double* w, int n) {
   hipLaunchKernelGGL(swm::dev::k_sum_partial, dim3(C->npart, 2), dim3(BLK), 0, C->stream, x0, x1, w, n, C->part);
-  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(1), dim3(2), 0, C->stream, C->part, C->npart, C->sums);
+  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(256), 0, C->stream, C->part, C->npart, C->sums);
 }
 
 // elliptic_solver.F:40-195 on the two fields rhs / xout of the context
@@ -220,14 +239,12 @@ swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
   for (int it = 0; it < 2; ++it) vcycle(C, nvar, xout, rhs);
   for (int v = 0; v < nvar; ++v) gather(C, C->face, {xout + (size_t)v * ns});
   auto norms = [&](const std::vector<int>& vars, double* out) -> swm_status {
-    for (size_t b = 0; b < vars.size(); ++b) {
-      const int v = vars[b];
-      hipLaunchKernelGGL(swm::dev::k_resid2, dim3(nblk(ns)), dim3(BLK), 0, C->stream, C->geo,
-                         xout + (size_t)v * ns, rhs + (size_t)v * ns, C->r2);
-      hipLaunchKernelGGL(swm::dev::k_sum_partial, dim3(C->npart, 1), dim3(BLK), 0, C->stream, C->r2, C->r2,
-                         (const double*)nullptr, ns, C->part);
-      hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(1), dim3(1), 0, C->stream, C->part, C->npart, C->sums + b);
-    }
+    const int v0 = vars[0], v1 = vars.back();
+    hipLaunchKernelGGL(swm::dev::k_resid2_partial, dim3(C->npart, (unsigned)vars.size()), dim3(BLK), 0, C->stream,
+                       C->geo, xout + (size_t)v0 * ns, rhs + (size_t)v0 * ns, xout + (size_t)v1 * ns,
+                       rhs + (size_t)v1 * ns, C->part);
+    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3((unsigned)vars.size()), dim3(256), 0, C->stream, C->part, C->npart,
+                       C->sums);
     HIP_TRY(hipMemcpyAsync(C->host_sums, C->sums, vars.size() * sizeof(double), hipMemcpyDeviceToHost, C->stream));
     HIP_TRY(hipStreamSynchronize(C->stream));
     for (size_t b = 0; b < vars.size(); ++b) out[b] = C->host_sums[b] / (double)C->max_ig;
@@ -698,7 +715,7 @@ swm_status swm_init_synthetic(swm_ctx* C, int seed_index, int lmax, double umax,
   hipLaunchKernelGGL(swm::dev::k_scale, dim3(nb), dim3(BLK), 0, S, chi, (size_t)ns, a2);
   auto sumsq = [&](const double* x, double* out) -> swm_status {
     hipLaunchKernelGGL(swm::dev::k_sumsq_partial, dim3(C->npart), dim3(BLK), 0, S, x, C->mask, ns, C->part);
-    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(1), dim3(1), 0, S, C->part, C->npart, C->sums);
+    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(1), dim3(256), 0, S, C->part, C->npart, C->sums);
     HIP_TRY(hipMemcpyAsync(C->host_sums, C->sums, sizeof(double), hipMemcpyDeviceToHost, S));
     HIP_TRY(hipStreamSynchronize(S));
     *out = C->host_sums[0];

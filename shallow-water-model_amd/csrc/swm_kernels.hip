@@ -460,19 +460,33 @@ __global__ void k_sum_partial(const double* __restrict__ x0, const double* __res
   }
   if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
 }
-// Neumaier-compensated sum of the partials, one thread per field
-__global__ void k_sum_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
-  const int f = threadIdx.x;
-  if (f >= (int)blockDim.x) return;
-  double s = 0.0, comp = 0.0;
-  for (int b = 0; b < nb; ++b) {
-    const double v = part[f * nb + b];
-    const double t = s + v;
-    if (fabs(s) >= fabs(v)) comp += (s - t) + v;
-    else comp += (v - t) + s;
-    s = t;
+// compensated sum of the partials of field blockIdx.x (256 threads, fixed
+// order: strided Neumaier sums per thread, then a pairwise tree of
+// (sum, error) pairs combined with Knuth's two-sum); deterministic
+__device__ __forceinline__ void two_sum_add(double& s, double& c, double v) {
+  const double t = s + v;
+  const double bp = t - s;
+  c += (s - (t - bp)) + (v - bp);
+  s = t;
+}
+__global__ void __launch_bounds__(256) k_sum_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ double ss[256], cc[256];
+  const int f = blockIdx.x, t = threadIdx.x;
+  double s = 0.0, c = 0.0;
+  for (int b = t; b < nb; b += 256) two_sum_add(s, c, part[f * nb + b]);
+  ss[t] = s;
+  cc[t] = c;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) {
+      double s1 = ss[t], c1 = cc[t] + cc[t + o];
+      two_sum_add(s1, c1, ss[t + o]);
+      ss[t] = s1;
+      cc[t] = c1;
+    }
+    __syncthreads();
   }
-  out[f] = s + comp;
+  if (t == 0) out[f] = ss[0] + cc[0];
 }
 
 // rhs -= mean (elliptic_solver.F:80-92); sums[v] / total gives the mean
@@ -485,54 +499,98 @@ __global__ void k_sub_mean(int n, const double* __restrict__ sums, double total,
   r2[t] = r2[t] - m2;
 }
 
-// squared residual of the Poisson problem at real cells
-// (elliptic_solver.F:203-239 check_convergence, :341-358 l2_norm)
-__global__ void k_resid2(Geo G, const double* __restrict__ x, const double* __restrict__ rhs, double* __restrict__ r2) {
+// per-block partial sums of the squared Poisson residual at real cells for
+// field blockIdx.y of (x0, r0), (x1, r1) (elliptic_solver.F:203-239
+// check_convergence, :341-358 l2_norm)
+__global__ void __launch_bounds__(256) k_resid2_partial(Geo G, const double* __restrict__ x0,
+                                                        const double* __restrict__ r0, const double* __restrict__ x1,
+                                                        const double* __restrict__ r1, double* __restrict__ part) {
+  __shared__ double sh[256];
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= ns) return;
-  const IJK c = ijk_of(s, iim, jjm);
-  auto X = [&](int sl) { return x[sl]; };
-  const double w = lap_at(iim, jjm, ns, c, G.lw, X);
-  const double r = (w - 0.0 * x[s]) - rhs[s];
-  r2[s] = (r * r) * G.mask[s];
+  const double* x = blockIdx.y == 0 ? x0 : x1;
+  const double* rhs = blockIdx.y == 0 ? r0 : r1;
+  double acc = 0.0;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
+    const IJK c = ijk_of(s, iim, jjm);
+    auto X = [&](int sl) { return x[sl]; };
+    const double w = lap_at(iim, jjm, ns, c, G.lw, X);
+    const double r = (w - 0.0 * x[s]) - rhs[s];
+    acc += (r * r) * G.mask[s];
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
 }
 
 // ------------------------------------------------------------- multigrid --
 // Halo reads go through the level's closed-form map (multigrid_solver.F:
 // 1105-1158 wrap_simple): src[slot] is the slot whose value the halo holds.
 
-// weighted Jacobi sweep (multigrid_solver.F:593-682); halo slots become 0
-__global__ void k_relax(Lev L, const double* __restrict__ beta, const double* __restrict__ a,
-                        double* __restrict__ out) {
+// weighted Jacobi sweep (multigrid_solver.F:593-682) of NV fields (planes
+// ns apart) with one thread per slot, so the weights are read once; halo
+// slots become 0.  ZERO: the input is identically zero (the coarse levels'
+// first sweep after multigrid_solver.F:520-524 clears them), so it is not
+// read; the arithmetic is the same, so the result is the same to the bit.
+// Slots at least two away from the panel edge have no halo neighbour and
+// read their neighbours without the map.
+template <int NV, bool ZERO>
+__global__ void __launch_bounds__(256) k_relax(Lev L, const double* __restrict__ beta, const double* __restrict__ a,
+                                               double* __restrict__ out) {
   const int iim = L.iim, jjm = L.jjm, ns = L.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
-  const int v = blockIdx.y;
-  beta += (size_t)v * ns; a += (size_t)v * ns; out += (size_t)v * ns;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
-  const int* __restrict__ m = L.src;
-  auto A = [&](int sl) { return a[m[sl]]; };
-#define LW(n) L.lw[((n) - 1) * ns + s]
-  const double omega = 0.8;
-  double tt;
-  if (i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1) {
-    tt = L.lwt[s] * (beta[s] - LW(1) * A(SL(i - 1, j, k)) - LW(2) * A(SL(i - 1, j - 1, k)) -
-                     LW(3) * A(SL(i, j - 1, k)) - LW(4) * A(SL(i + 1, j, k)) - LW(5) * A(SL(i + 1, j + 1, k)) -
-                     LW(6) * A(SL(i, j + 1, k)));
-  } else if (i == 1 && j == jjm - 1) {
-    tt = L.lwt[s] * (beta[s] - LW(2) * A(SL(iim, 1, k)) - LW(3) * A(SL(i, j - 1, k)) - LW(4) * A(SL(i + 1, j, k)) -
-                     LW(5) * A(SL(i + 1, j + 1, k)) - LW(6) * A(SL(i, j + 1, k)));
-  } else if (i == iim - 1 && j == 1) {
-    tt = L.lwt[s] * (beta[s] - LW(1) * A(SL(i - 1, j, k)) - LW(2) * A(SL(1, jjm, k)) - LW(4) * A(SL(i + 1, j, k)) -
-                     LW(5) * A(SL(i + 1, j + 1, k)) - LW(6) * A(SL(i, j + 1, k)));
-  } else {
-    out[s] = 0.0;
+  const bool in = i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1;
+  const bool np = i == 1 && j == jjm - 1, sp = i == iim - 1 && j == 1;
+  if (!(in || np || sp)) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) out[(size_t)v * ns + s] = 0.0;
     return;
   }
-#undef LW
-  out[s] = (1.0 - omega) * A(s) + omega * tt;
+  const bool deep = i >= 3 && i <= iim - 2 && j >= 3 && j <= jjm - 2;
+  const int* __restrict__ m = L.src;
+  int nb[6];
+  if (in) {
+    nb[0] = SL(i - 1, j, k); nb[1] = SL(i - 1, j - 1, k); nb[2] = SL(i, j - 1, k);
+    nb[3] = SL(i + 1, j, k); nb[4] = SL(i + 1, j + 1, k); nb[5] = SL(i, j + 1, k);
+  } else if (np) {
+    nb[0] = -1; nb[1] = SL(iim, 1, k); nb[2] = SL(i, j - 1, k);
+    nb[3] = SL(i + 1, j, k); nb[4] = SL(i + 1, j + 1, k); nb[5] = SL(i, j + 1, k);
+  } else {
+    nb[0] = SL(i - 1, j, k); nb[1] = SL(1, jjm, k); nb[2] = -1;
+    nb[3] = SL(i + 1, j, k); nb[4] = SL(i + 1, j + 1, k); nb[5] = SL(i, j + 1, k);
+  }
+  int me = s;   // the slot's own value is read through the map as well
+  if (!ZERO && !deep) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) nb[q] = nb[q] < 0 ? -1 : m[nb[q]];
+    me = m[s];
+  }
+  double w[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) w[q] = L.lw[q * ns + s];
+  const double wt = L.lwt[s];
+  const double omega = 0.8;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double* __restrict__ av = a + (size_t)v * ns;
+    auto A = [&](int q) { return ZERO ? 0.0 : av[nb[q]]; };
+    const double bv = beta[(size_t)v * ns + s];
+    double tt;
+    if (in)
+      tt = wt * (bv - w[0] * A(0) - w[1] * A(1) - w[2] * A(2) - w[3] * A(3) - w[4] * A(4) - w[5] * A(5));
+    else if (np)
+      tt = wt * (bv - w[1] * A(1) - w[2] * A(2) - w[3] * A(3) - w[4] * A(4) - w[5] * A(5));
+    else
+      tt = wt * (bv - w[0] * A(0) - w[1] * A(1) - w[3] * A(3) - w[4] * A(4) - w[5] * A(5));
+    const double self = ZERO ? 0.0 : av[me];
+    out[(size_t)v * ns + s] = (1.0 - omega) * self + omega * tt;
+  }
 }
 
 // residual at the odd-odd fine points becomes the coarse right-hand side
@@ -702,8 +760,6 @@ __global__ void k_inv_h(Geo G, const double* __restrict__ h4, double* __restrict
   inv_h[s] = G.mask[s] == 1.0 ? 1.0 / h4[s] : 0.0;
 }
 
-#undef SL
-
 }  // namespace dev
 }  // namespace swm
 
@@ -862,3 +918,212 @@ __global__ void k_op_center_ke(Geo G, const double* __restrict__ kec, double* __
 
 }  // namespace dev
 }  // namespace swm
+
+namespace swm {
+namespace dev {
+
+// ------------------------------------------- coarse end of the V-cycle ----
+// Levels 1..nlev (nlev <= 3) of one V-cycle (multigrid_solver.F:476-591) in
+// one workgroup: the levels' alpha, the coarse right-hand sides and the
+// smoother's second buffer live in LDS; weights and halo maps are read from
+// global memory (L2) into registers once per visit of a level, so each
+// Jacobi sweep is LDS traffic and one barrier.  Same arithmetic, in the same
+// order, as k_relax / k_inject / k_interp.
+constexpr int CO_THREADS = 1024;
+constexpr int CO_MAXI = 2;   // items (field, slot) per thread: nvar * ns(level 3) <= 2048
+constexpr int CO_MAXLEV = 3;
+
+struct CoarseArgs {
+  Lev lev[CO_MAXLEV];   // levels 1..nlev
+  int nlev, nvar;
+  const double* beta;   // right-hand side at level nlev (nvar planes of ns)
+  double* alpha;        // solution at level nlev (in: warm start unless zero)
+  int zero;             // alpha at level nlev is zero on entry
+};
+
+__device__ __forceinline__ int co_ns(int l) { return (2 + (1 << l)) * (2 + (1 << l)) * 10; }
+
+// relax nsw sweeps at level L: x (LDS, result) and t (LDS, scratch); b is the
+// right-hand side (LDS or global)
+__device__ void co_relax(const Lev& L, int nvar, int nsw, double* x, double* t, const double* b) {
+  const int iim = L.iim, jjm = L.jjm, ns = L.ns;
+  const int nitem = nvar * ns;
+  int kind[CO_MAXI], self[CO_MAXI], out_i[CO_MAXI], nb[CO_MAXI][6];
+  double w[CO_MAXI][6], wt[CO_MAXI], bv[CO_MAXI];
+#pragma unroll
+  for (int r = 0; r < CO_MAXI; ++r) {
+    const int it = threadIdx.x + r * CO_THREADS;
+    kind[r] = -1;
+    if (it >= nitem) continue;
+    const int v = it / ns, s = it - v * ns;
+    const IJK c = ijk_of(s, iim, jjm);
+    const int i = c.i, j = c.j, k = c.k;
+    const bool in = i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1;
+    const bool np = i == 1 && j == jjm - 1, sp = i == iim - 1 && j == 1;
+    out_i[r] = v * ns + s;
+    kind[r] = in ? 0 : (np ? 1 : (sp ? 2 : 3));
+    if (kind[r] == 3) continue;
+    int q6[6];
+    if (in) {
+      q6[0] = SL(i - 1, j, k); q6[1] = SL(i - 1, j - 1, k); q6[2] = SL(i, j - 1, k);
+    } else if (np) {
+      q6[0] = s; q6[1] = SL(iim, 1, k); q6[2] = SL(i, j - 1, k);
+    } else {
+      q6[0] = SL(i - 1, j, k); q6[1] = SL(1, jjm, k); q6[2] = s;
+    }
+    q6[3] = SL(i + 1, j, k); q6[4] = SL(i + 1, j + 1, k); q6[5] = SL(i, j + 1, k);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      nb[r][q] = v * ns + L.src[q6[q]];
+      w[r][q] = L.lw[q * ns + s];
+    }
+    self[r] = v * ns + L.src[s];
+    wt[r] = L.lwt[s];
+    bv[r] = b[v * ns + s];
+  }
+  const double omega = 0.8;
+  double* a = x;
+  double* o = t;
+  for (int sw = 0; sw < nsw; ++sw) {
+#pragma unroll
+    for (int r = 0; r < CO_MAXI; ++r) {
+      if (kind[r] < 0) continue;
+      if (kind[r] == 3) { o[out_i[r]] = 0.0; continue; }
+      double tt;
+      if (kind[r] == 0)
+        tt = wt[r] * (bv[r] - w[r][0] * a[nb[r][0]] - w[r][1] * a[nb[r][1]] - w[r][2] * a[nb[r][2]] -
+                      w[r][3] * a[nb[r][3]] - w[r][4] * a[nb[r][4]] - w[r][5] * a[nb[r][5]]);
+      else if (kind[r] == 1)
+        tt = wt[r] * (bv[r] - w[r][1] * a[nb[r][1]] - w[r][2] * a[nb[r][2]] - w[r][3] * a[nb[r][3]] -
+                      w[r][4] * a[nb[r][4]] - w[r][5] * a[nb[r][5]]);
+      else
+        tt = wt[r] * (bv[r] - w[r][0] * a[nb[r][0]] - w[r][1] * a[nb[r][1]] - w[r][3] * a[nb[r][3]] -
+                      w[r][4] * a[nb[r][4]] - w[r][5] * a[nb[r][5]]);
+      o[out_i[r]] = (1.0 - omega) * a[self[r]] + omega * tt;
+    }
+    __syncthreads();
+    double* sw_ = a; a = o; o = sw_;
+  }
+  // nsw is even: the result is in x
+}
+
+// residual at odd-odd fine points of level F -> right-hand side of level Cc
+__device__ void co_inject(const Lev& F, const Lev& Cc, int nvar, const double* xa, const double* xb, double* x1) {
+  const int nitem = nvar * Cc.ns;
+  for (int it = threadIdx.x; it < nitem; it += CO_THREADS) {
+    const int v = it / Cc.ns, s1 = it - v * Cc.ns;
+    int iim = Cc.iim, jjm = Cc.jjm;
+    const IJK c = ijk_of(s1, iim, jjm);
+    const bool in = c.i >= 2 && c.i <= iim - 1 && c.j >= 2 && c.j <= jjm - 1;
+    const bool np = c.k == 1 && c.i == 1 && c.j == jjm - 1;
+    const bool sp = c.k == 10 && c.i == iim - 1 && c.j == 1;
+    if (!(in || np || sp)) continue;
+    const int k = c.k;
+    int i2, j2;
+    if (in) { i2 = 2 * c.i - 1; j2 = 2 * c.j - 1; }
+    else if (np) { i2 = 1; j2 = F.jjm - 1; }
+    else { i2 = F.iim - 1; j2 = 1; }
+    iim = F.iim; jjm = F.jjm;
+    const int ns = F.ns;
+    const int s = SL(i2, j2, k);
+    const int* __restrict__ m = F.src;
+    const double* xav = xa + v * ns;
+    auto A = [&](int sl) { return xav[m[sl]]; };
+#define LW(n) F.lw[((n) - 1) * ns + s]
+    double r;
+    if (in)
+      r = xb[v * ns + s] - ((LW(7) - 0.0) * A(s) + LW(1) * A(SL(i2 - 1, j2, k)) + LW(2) * A(SL(i2 - 1, j2 - 1, k)) +
+                            LW(3) * A(SL(i2, j2 - 1, k)) + LW(4) * A(SL(i2 + 1, j2, k)) +
+                            LW(5) * A(SL(i2 + 1, j2 + 1, k)) + LW(6) * A(SL(i2, j2 + 1, k)));
+    else if (np)
+      r = xb[v * ns + s] - ((LW(7) - 0.0) * A(s) + LW(2) * A(SL(iim, 1, k)) + LW(3) * A(SL(i2, j2 - 1, k)) +
+                            LW(4) * A(SL(i2 + 1, j2, k)) + LW(5) * A(SL(i2 + 1, j2 + 1, k)) +
+                            LW(6) * A(SL(i2, j2 + 1, k)));
+    else
+      r = xb[v * ns + s] - ((LW(7) - 0.0) * A(s) + LW(1) * A(SL(i2 - 1, j2, k)) + LW(2) * A(SL(1, jjm, k)) +
+                            LW(4) * A(SL(i2 + 1, j2, k)) + LW(5) * A(SL(i2 + 1, j2 + 1, k)) +
+                            LW(6) * A(SL(i2, j2 + 1, k)));
+#undef LW
+    x1[v * Cc.ns + s1] = r;
+  }
+  __syncthreads();
+}
+
+// coarse correction of level Cc added into level F
+__device__ void co_interp(const Lev& Cc, const Lev& F, int nvar, const double* x1, double* x2) {
+  const int nitem = nvar * F.ns;
+  for (int it = threadIdx.x; it < nitem; it += CO_THREADS) {
+    const int v = it / F.ns, s = it - v * F.ns;
+    const int iim = F.iim, jjm = F.jjm;
+    const IJK c = ijk_of(s, iim, jjm);
+    const int i = c.i, j = c.j, k = c.k;
+    const int p2 = iim - 1, q2 = jjm - 1;
+    const int ci = Cc.iim, cj = Cc.jjm;
+    const double* x1v = x1 + v * Cc.ns;
+    auto T = [&](int ii, int jj) {
+      const int a = (ii + 1) / 2, b = (jj + 1) / 2;
+      return x1v[Cc.src[(a - 1) + ci * ((b - 1) + cj * (k - 1))]];
+    };
+    const bool io = (i & 1), jo = (j & 1);
+    double add;
+    bool hit = true;
+    if (i == 1 && j == q2) add = T(1, q2);
+    else if (i == p2 && j == 1) add = T(p2, 1);
+    else if (io && jo && i >= 3 && i <= iim - 1 && j >= 3 && j <= jjm - 1) add = T(i, j);
+    else if (!io && jo && i >= 2 && i <= iim - 2 && j >= 3 && j <= jjm - 1) add = 0.5 * (T(i + 1, j) + T(i - 1, j));
+    else if (!io && !jo && i >= 2 && i <= iim - 2 && j >= 2 && j <= jjm - 2)
+      add = 0.5 * (T(i + 1, j + 1) + T(i - 1, j - 1));
+    else if (io && !jo && i >= 3 && i <= iim - 1 && j >= 2 && j <= jjm - 2) add = 0.5 * (T(i, j + 1) + T(i, j - 1));
+    else hit = false;
+    if (hit) x2[v * F.ns + s] = x2[v * F.ns + s] + add;
+  }
+  __syncthreads();
+}
+
+// level l of the bundle with constant indices (no dynamic indexing of the
+// kernel argument, which would go through scratch memory)
+__device__ __forceinline__ Lev co_lev(const CoarseArgs& P, int l) {
+  return l == 1 ? P.lev[0] : (l == 2 ? P.lev[1] : P.lev[2]);
+}
+
+__global__ void __launch_bounds__(CO_THREADS) k_coarse_cycle(CoarseArgs P) {
+  // LDS: alpha of levels 1..nlev, rhs of levels 1..nlev-1, one scratch level
+  __shared__ double lds[2 * (160 + 360 + 1000) + 2 * (160 + 360) + 2 * 1000];
+  const int nv = P.nvar, top = P.nlev;
+  // offsets: alpha(l) at nv * (ns(1)+..+ns(l-1)), rhs(l) after all alphas
+  auto cum = [&](int l) {   // ns(1) + .. + ns(l)
+    int c = 0;
+    for (int m = 1; m <= l; ++m) c += co_ns(m);
+    return c;
+  };
+  auto AL = [&](int l) { return lds + nv * cum(l - 1); };
+  auto BE = [&](int l) { return lds + nv * (cum(top) + cum(l - 1)); };
+  double* tmp = lds + nv * (cum(top) + cum(top - 1));
+  // coarse levels start from zero (multigrid_solver.F:520-524); the top
+  // level from the caller's alpha unless that is zero as well
+  {
+    const int nz = nv * cum(top - 1), n = nv * co_ns(top);
+    for (int t = threadIdx.x; t < nz; t += CO_THREADS) lds[t] = 0.0;
+    double* at = AL(top);
+    for (int t = threadIdx.x; t < n; t += CO_THREADS) at[t] = P.zero ? 0.0 : P.alpha[t];
+  }
+  __syncthreads();
+  for (int l = top; l >= 2; --l) {
+    const double* b = l == top ? P.beta : BE(l);
+    co_relax(co_lev(P, l), nv, 2, AL(l), tmp, b);
+    co_inject(co_lev(P, l), co_lev(P, l - 1), nv, AL(l), b, BE(l - 1));
+  }
+  co_relax(co_lev(P, 1), nv, 24, AL(1), tmp, top == 1 ? P.beta : BE(1));
+  for (int l = 1; l <= top - 1; ++l) {
+    co_interp(co_lev(P, l), co_lev(P, l + 1), nv, AL(l), AL(l + 1));
+    co_relax(co_lev(P, l + 1), nv, 2, AL(l + 1), tmp, l + 1 == top ? P.beta : BE(l + 1));
+  }
+  const int n = nv * co_ns(top);
+  const double* at = AL(top);
+  for (int t = threadIdx.x; t < n; t += CO_THREADS) P.alpha[t] = at[t];
+}
+
+}  // namespace dev
+}  // namespace swm
+
+#undef SL
