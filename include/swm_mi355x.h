@@ -156,34 +156,70 @@ swm_status swm_init_synthetic(swm_ctx* ctx, int seed_index, int lmax, double uma
 swm_status swm_total_mass(swm_ctx* ctx, double* mass);
 
 /* ------------------------------------------------------------------------
- * Reference operators on all 10 panels of host arrays (copy in, run on the
- * device, copy out).  Metric arguments are those of the context's grid.
+ * Reference operators (utilities/layer_2/operators.F), one entry point per
+ * subroutine with the subroutine's own argument list, in its order, plus
+ * nsdm: the arrays hold nsdm panels (the reference calls the subroutines
+ * once per panel inside its ksdm loops, i.e. nsdm = 1; the loop may be
+ * replaced by one call with nsdm = 10).  Metric arrays are arguments, as in
+ * the reference; every array is (leading dims, nx, ny, nsdm) in the
+ * reference layout.  Host arrays are copied in, the operator runs on the
+ * device, the result is copied out.  Results are set on i = 2..nx-1,
+ * j = 2..ny-1 and are zero (ghost fill for average_to_cell_corners)
+ * elsewhere, as in the reference.  The fixed counts must be those of the
+ * model: npad 1, ndata 3, nspace 2, ncorners 2, nedges 3, nweights 7, r3 3
+ * (else SWM_ERR_DIMS).
  * ------------------------------------------------------------------------ */
-/* utilities/layer_2/operators.F:270-392 */
-swm_status swm_op_div_of_an_edge_flux(swm_ctx* ctx, const double* xin, double* xout);
-/* utilities/layer_2/operators.F:402-518 */
-swm_status swm_op_laplacian_of_a_scalar(swm_ctx* ctx, const double* xin, double* xout);
-/* utilities/layer_2/operators.F:47-147 */
-swm_status swm_op_curl_of_a_vector(swm_ctx* ctx, const double* xin, double* xout);
-/* utilities/layer_2/operators.F:155-261 */
-swm_status swm_op_div_of_a_vector(swm_ctx* ctx, const double* xin, double* xout);
-/* utilities/layer_2/operators.F:1316-1393 (ke_c at corners -> ke at centres) */
-swm_status swm_op_calculate_center_kinetic_energy(swm_ctx* ctx, const double* ke_c, double* ke);
-/* utilities/layer_2/operators.F:982-1112 (flag = .true.) */
-swm_status swm_op_average_to_cell_corners(swm_ctx* ctx, const double* xin, double* xout);
-/* utilities/layer_2/operators.F:1402-1514 */
-swm_status swm_op_calculate_velocity(swm_ctx* ctx, const double* psi, const double* chi, double* u);
-/* utilities/layer_2/operators.F:1626-1722 */
-swm_status swm_op_centered_mass_flux(swm_ctx* ctx, const double* mass_c, const double* u,
-                                     double* flux);
-/* utilities/layer_2/operators.F:1731-1832 */
-swm_status swm_op_upstream_mass_flux(swm_ctx* ctx, const double* u, const double* mass,
-                                     double* flux);
-/* utilities/layer_2/operators.F:2000-2074 */
-swm_status swm_op_centered_vorticity_flux(swm_ctx* ctx, const double* mass_flux,
-                                          const double* mass, const double* eta, double* flux);
-/* utilities/layer_2/operators.F:2083-2186 */
-swm_status swm_op_centered_divergence_curl(swm_ctx* ctx, const double* mass_c, const double* eta,
+/* operators.F:47-147 */
+swm_status swm_op_curl_of_a_vector(swm_ctx* ctx, int npad, int nx, int ny, int nsdm, int ndata, int nspace,
+                                   int ncorners, const double* area_inv, const double* v_weights,
+                                   const double* xin, double* xout);
+/* operators.F:155-261 */
+swm_status swm_op_div_of_a_vector(swm_ctx* ctx, int npad, int nx, int ny, int nsdm, int ndata, int nspace,
+                                  int ncorners, const double* area_inv, const double* v_weights,
+                                  const double* xin, double* xout);
+/* operators.F:270-392 */
+swm_status swm_op_div_of_an_edge_flux(swm_ctx* ctx, int npad, int nx, int ny, int nsdm, int nedges,
+                                      const double* area_inv, const double* xin, double* xout);
+/* operators.F:402-518 */
+swm_status swm_op_laplacian_of_a_scalar(swm_ctx* ctx, int npad, int nx, int ny, int nsdm, int nweights,
+                                        const double* l_weights, const double* xin, double* xout);
+/* operators.F:528-582 (every slot) */
+swm_status swm_op_k_cross_v(swm_ctx* ctx, int nx, int ny, int nsdm, int nspace, int ncorners,
+                            const double* xin, double* xout);
+/* operators.F:592-682 */
+swm_status swm_op_gradient(swm_ctx* ctx, int npad, int nx, int ny, int nsdm, int ndata, int nspace,
+                           int ncorners, const double* corner_area_inv, const double* v_weights,
+                           const double* xin, double* xout);
+/* operators.F:982-1112; flag is the logical flag (non-zero = .true.);
+ * min_value: -1 = absent, 1 = present and .true. (0, present and .false.,
+ * leaves the reference's result undefined and is refused) */
+swm_status swm_op_average_to_cell_corners(swm_ctx* ctx, int npad, int nx, int ny, int nsdm, int ndata,
+                                          int ncorners, const double* s_weights, const double* xin,
+                                          double* xout, int flag, int min_value);
+/* operators.F:1316-1393 */
+swm_status swm_op_calculate_center_kinetic_energy(swm_ctx* ctx, int npad, int nx, int ny, int nsdm, int ndata,
+                                                  int nspace, int ncorners, const double* sub_area,
+                                                  const double* area_inv, const double* ke_c, double* ke);
+/* operators.F:1402-1514 */
+swm_status swm_op_calculate_velocity(swm_ctx* ctx, int npad, int nx, int ny, int nsdm, int ndata, int nspace,
+                                     int ncorners, const double* corner_area_inv, const double* v_weights,
+                                     const double* psi, const double* chi, double* xout);
+/* operators.F:1626-1722 */
+swm_status swm_op_centered_mass_flux(swm_ctx* ctx, int npad, int nx, int ny, int nsdm, int ndata, int ncorners,
+                                     int nedges, int nspace, int r3, const double* dc, const double* normal,
+                                     const double* mass_c, const double* u, const double* mass, double* flux);
+/* operators.F:1731-1832 */
+swm_status swm_op_upstream_mass_flux(swm_ctx* ctx, int npad, int nx, int ny, int nsdm, int ndata, int ncorners,
+                                     int nedges, int nspace, int r3, const double* dc, const double* normal,
+                                     const double* u, const double* mass, double* flux);
+/* operators.F:2000-2074 */
+swm_status swm_op_centered_vorticity_flux(swm_ctx* ctx, int npad, int nx, int ny, int nsdm, int nedges,
+                                          const double* mass_flux, const double* mass, const double* xin,
+                                          double* xout);
+/* operators.F:2083-2186 */
+swm_status swm_op_centered_divergence_curl(swm_ctx* ctx, int npad, int nx, int ny, int nsdm, int ndata,
+                                           int ncorners, int nedges, int nspace, int r3, const double* dc,
+                                           const double* tangent, const double* mass_c, const double* eta,
                                            const double* u, const double* mass, double* flux);
 
 /* Halo fill in place, wrap_data::wrap (utilities/layer_2/wrap_data.F:1790-1974):

@@ -4,8 +4,10 @@
 
 Host set-up code is compiled with g++ and the device code with hipcc for
 gfx950; both without floating-point contraction, so that the FP64 path keeps
-the reference's operation order.  Output: shallow-water-model_amd/lib/
-libswm_mi355x.so.
+the reference's operation order.  The Fortran modules (fortran/swm_mi355x.F90,
+ISO_C_BINDING over the C ABI) and the Fortran driver check program are
+compiled with ROCm's flang.  Output under shallow-water-model_amd/lib/:
+libswm_mi355x.so, swm_fortran_check and the Fortran module files in obj/.
 """
 import os
 import shutil
@@ -16,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(OUT_DIR, "libswm_mi355x.so")
+FORTRAN_CHECK = os.path.join(OUT_DIR, "swm_fortran_check")
 
 HOST_SOURCES = ["swm_setup.cpp", "swm_metrics.cpp", "swm_capi_grid.cpp"]
 DEVICE_SOURCES = ["swm_device.hip"]
@@ -26,6 +29,13 @@ def _hipcc():
         if cand and os.path.exists(cand):
             return cand
     raise RuntimeError("hipcc not found: the library needs ROCm's hipcc")
+
+
+def _flang():
+    for cand in (os.environ.get("FLANG"), "/opt/rocm/lib/llvm/bin/flang", shutil.which("flang")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("flang not found: the Fortran modules need ROCm's flang")
 
 
 def _run(cmd):
@@ -53,6 +63,16 @@ def build(verbose=True):
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # Fortran modules and the driver check program (runs against the .so
+    # beside it)
+    flang = _flang()
+    fdir = os.path.join(HERE, "fortran")
+    fobj = os.path.join(obj_dir, "swm_mi355x.F90.o")
+    _run([flang, "-O2", "-module-dir", obj_dir, "-c", os.path.join(fdir, "swm_mi355x.F90"), "-o", fobj])
+    tmp = FORTRAN_CHECK + ".tmp"
+    _run([flang, "-O2", "-module-dir", obj_dir, os.path.join(fdir, "swm_fortran_check.F90"), fobj,
+          "-L" + OUT_DIR, "-lswm_mi355x", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, FORTRAN_CHECK)
     return LIB
 
 

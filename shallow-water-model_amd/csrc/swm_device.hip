@@ -450,45 +450,53 @@ bool all_set(const T*... p) {
 }
 
 // device buffers of one operator call (allocated per call, freed on return;
-// the model state and the step's scratch are not touched)
+// the model state and the step's scratch are not touched).  ns is the
+// number of slots of the call's arrays (nx * ny * panels).
 struct OpIO {
   swm_ctx* C;
+  size_t ns;
   std::vector<double*> bufs;
   std::vector<double> host;
   ~OpIO() {
     for (double* p : bufs) (void)hipFree(p);
   }
-  double* dev(int ncomp) {
+  double* dev(int nplanes) {
     double* p = nullptr;
-    if (hipMalloc(&p, (size_t)ncomp * C->ns * sizeof(double)) != hipSuccess) return nullptr;
+    if (hipMalloc(&p, (size_t)nplanes * ns * sizeof(double)) != hipSuccess) return nullptr;
     bufs.push_back(p);
     return p;
   }
-  // host reference layout -> fresh device planes (nullptr on failure)
-  double* put(const double* ref, int ncomp, bool corner_vector = false) {
-    double* d = dev(ncomp);
+  // reference layout (lead, slots), lead leading entries per slot -> planes;
+  // plane p takes leading entry perm[p] (identity when perm is empty)
+  double* put(const double* ref, int lead, int nplanes, const std::vector<int>& perm = {}) {
+    double* d = dev(nplanes);
     if (!d) return nullptr;
-    host.assign((size_t)ncomp * C->ns, 0.0);
-    if (corner_vector) u_to_planes(ref, host.data(), C->ns);
-    else ref_to_planes(ref, ncomp, C->ns, host.data());
+    host.assign((size_t)nplanes * ns, 0.0);
+    for (size_t s = 0; s < ns; ++s)
+      for (int p = 0; p < nplanes; ++p) host[(size_t)p * ns + s] = ref[(size_t)lead * s + (perm.empty() ? p : perm[p])];
     if (hipMemcpy(d, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
     return d;
   }
-  double* zeros(int ncomp) {
-    double* d = dev(ncomp);
-    if (d) fill(C, d, (size_t)ncomp * C->ns, 0.0);
+  double* zeros(int nplanes) {
+    double* d = dev(nplanes);
+    if (d) fill(C, d, (size_t)nplanes * ns, 0.0);
     return d;
   }
-  swm_status get(double* ref, const double* d, int ncomp, bool corner_vector = false) {
+  swm_status get(double* ref, const double* d, int lead, const std::vector<int>& perm = {}) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(C->stream));
-    host.assign((size_t)ncomp * C->ns, 0.0);
+    host.assign((size_t)lead * ns, 0.0);
     HIP_TRY(hipMemcpy(host.data(), d, host.size() * sizeof(double), hipMemcpyDeviceToHost));
-    if (corner_vector) u_from_planes(host.data(), ref, C->ns);
-    else planes_to_ref(host.data(), ncomp, C->ns, ref);
+    for (size_t s = 0; s < ns; ++s)
+      for (int p = 0; p < lead; ++p) ref[(size_t)lead * s + (perm.empty() ? p : perm[p])] = host[(size_t)p * ns + s];
     return SWM_OK;
   }
 };
+
+// plane order ic + 2*space of a corner vector (space, corner): leading index
+const std::vector<int> kCornerVec = {0, 2, 1, 3};
+// planes kk + 3*(a + 2*ic), a = 0, 1, of normal / tangent (ndata=3, r3=3, corner)
+const std::vector<int> kFrame = {0, 1, 2, 3, 4, 5, 9, 10, 11, 12, 13, 14};
 
 // (iim, jjm, nvar, 10) <-> nvar planes
 void vars_to_planes(const double* ref, int nvar, int pl, double* planes) {
@@ -869,120 +877,207 @@ swm_status swm_total_mass(swm_ctx* C, double* mass) {
 #define OP_BEGIN(C)                                                              \
   if (!(C)) { swm::set_error("null context"); return SWM_ERR_ARG; }             \
   HIP_TRY(hipSetDevice((C)->device));                                           \
-  HIP_TRY(hipStreamSynchronize((C)->stream));                                   \
-  OpIO io{(C), {}, {}};                                                         \
-  const unsigned nb_ = nblk((C)->ns);                                           \
-  (void)nb_;
+  HIP_TRY(hipStreamSynchronize((C)->stream));
 #define OP_ARGS(...)                                                             \
   if (!all_set(__VA_ARGS__)) { swm::set_error("null array argument"); return SWM_ERR_ARG; }
 #define OP_BUF(x)                                                                \
   if (!(x)) { swm::set_error("operator buffers: device allocation or copy failed"); return SWM_ERR_HIP; }
+// extents of a single-operator call: one-cell padding, at least one interior
+// cell per panel, and the fixed counts of this grid (3 data, 2 corners,
+// 2 space components, 3 edges, 7 Laplacian weights, 3 frame components)
+#define OP_DIMS(npad, nx, ny, nsdm, ok)                                           \
+  if ((npad) != 1 || (nx) < 3 || (ny) < 3 || (nsdm) < 1 || !(ok)) {              \
+    swm::set_error("operator extents: need npad = 1, nx, ny >= 3, nsdm >= 1 and the grid's fixed counts"); \
+    return SWM_ERR_DIMS;                                                        \
+  }                                                                             \
+  OpIO io{C, (size_t)(nx) * (ny) * (nsdm), {}, {}};                             \
+  swm::dev::Geo G{};                                                            \
+  G.iim = (nx); G.jjm = (ny); G.ns = (int)io.ns;                                \
+  const unsigned nb_ = nblk(io.ns);
 
-swm_status swm_op_div_of_an_edge_flux(swm_ctx* C, const double* xin, double* xout) {
+swm_status swm_op_curl_of_a_vector(swm_ctx* C, int npad, int nx, int ny, int nsdm, int ndata, int nspace, int ncorners,
+                                   const double* area_inv, const double* v_weights, const double* xin, double* xout) {
   OP_BEGIN(C);
-  OP_ARGS(xin, xout);
-  double *x = io.put(xin, 3), *y = io.dev(1);
-  OP_BUF(x && y);
-  hipLaunchKernelGGL(swm::dev::k_op_div_edge, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, x, y);
+  OP_ARGS(area_inv, v_weights, xin, xout);
+  OP_DIMS(npad, nx, ny, nsdm, ndata == 3 && nspace == 2 && ncorners == 2);
+  G.area_inv = io.put(area_inv, 1, 1);
+  G.vw = io.put(v_weights, 12, 12);
+  double *x = io.put(xin, 4, 4, kCornerVec), *y = io.dev(1);
+  OP_BUF(G.area_inv && G.vw && x && y);
+  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb_), dim3(BLK), 0, C->stream, G, x, y, (double*)nullptr);
   return io.get(xout, y, 1);
 }
 
-swm_status swm_op_laplacian_of_a_scalar(swm_ctx* C, const double* xin, double* xout) {
+swm_status swm_op_div_of_a_vector(swm_ctx* C, int npad, int nx, int ny, int nsdm, int ndata, int nspace, int ncorners,
+                                  const double* area_inv, const double* v_weights, const double* xin, double* xout) {
   OP_BEGIN(C);
-  OP_ARGS(xin, xout);
-  double *x = io.put(xin, 1), *y = io.dev(1);
-  OP_BUF(x && y);
-  hipLaunchKernelGGL(swm::dev::k_op_lap, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, x, y);
+  OP_ARGS(area_inv, v_weights, xin, xout);
+  OP_DIMS(npad, nx, ny, nsdm, ndata == 3 && nspace == 2 && ncorners == 2);
+  G.area_inv = io.put(area_inv, 1, 1);
+  G.vw = io.put(v_weights, 12, 12);
+  double *x = io.put(xin, 4, 4, kCornerVec), *y = io.dev(1);
+  OP_BUF(G.area_inv && G.vw && x && y);
+  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb_), dim3(BLK), 0, C->stream, G, x, (double*)nullptr, y);
   return io.get(xout, y, 1);
 }
 
-swm_status swm_op_curl_of_a_vector(swm_ctx* C, const double* xin, double* xout) {
+swm_status swm_op_div_of_an_edge_flux(swm_ctx* C, int npad, int nx, int ny, int nsdm, int nedges,
+                                      const double* area_inv, const double* xin, double* xout) {
   OP_BEGIN(C);
-  OP_ARGS(xin, xout);
-  double *x = io.put(xin, 4, true), *y = io.dev(1);
-  OP_BUF(x && y);
-  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, x, y, (double*)nullptr);
+  OP_ARGS(area_inv, xin, xout);
+  OP_DIMS(npad, nx, ny, nsdm, nedges == 3);
+  G.area_inv = io.put(area_inv, 1, 1);
+  double *x = io.put(xin, 3, 3), *y = io.dev(1);
+  OP_BUF(G.area_inv && x && y);
+  hipLaunchKernelGGL(swm::dev::k_op_div_edge, dim3(nb_), dim3(BLK), 0, C->stream, G, x, y);
   return io.get(xout, y, 1);
 }
 
-swm_status swm_op_div_of_a_vector(swm_ctx* C, const double* xin, double* xout) {
+swm_status swm_op_laplacian_of_a_scalar(swm_ctx* C, int npad, int nx, int ny, int nsdm, int nweights,
+                                        const double* l_weights, const double* xin, double* xout) {
   OP_BEGIN(C);
-  OP_ARGS(xin, xout);
-  double *x = io.put(xin, 4, true), *y = io.dev(1);
-  OP_BUF(x && y);
-  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, x, (double*)nullptr, y);
+  OP_ARGS(l_weights, xin, xout);
+  OP_DIMS(npad, nx, ny, nsdm, nweights == 7);
+  G.lw = io.put(l_weights, 7, 7);
+  double *x = io.put(xin, 1, 1), *y = io.dev(1);
+  OP_BUF(G.lw && x && y);
+  hipLaunchKernelGGL(swm::dev::k_op_lap, dim3(nb_), dim3(BLK), 0, C->stream, G, x, y);
   return io.get(xout, y, 1);
 }
 
-swm_status swm_op_calculate_center_kinetic_energy(swm_ctx* C, const double* ke_c, double* ke) {
-  OP_BEGIN(C);
-  OP_ARGS(ke_c, ke);
-  double *x = io.put(ke_c, 2), *y = io.dev(1);
-  OP_BUF(x && y);
-  hipLaunchKernelGGL(swm::dev::k_op_center_ke, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, x, y);
-  return io.get(ke, y, 1);
-}
-
-swm_status swm_op_average_to_cell_corners(swm_ctx* C, const double* xin, double* xout) {
+swm_status swm_op_k_cross_v(swm_ctx* C, int nx, int ny, int nsdm, int nspace, int ncorners, const double* xin,
+                            double* xout) {
   OP_BEGIN(C);
   OP_ARGS(xin, xout);
-  double *x = io.put(xin, 1), *y = io.dev(2), *y2 = io.dev(2);
-  OP_BUF(x && y && y2);
-  hipLaunchKernelGGL(swm::dev::k_avg_corners, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, C->sw, x, x, y, y2);
+  OP_DIMS(1, nx, ny, nsdm, nspace == 2 && ncorners == 2);
+  (void)G;
+  double *x = io.put(xin, 4, 4, kCornerVec), *y = io.dev(4);
+  OP_BUF(x && y);
+  hipLaunchKernelGGL(swm::dev::k_op_cross, dim3(nb_), dim3(BLK), 0, C->stream, (int)io.ns, x, y);
+  return io.get(xout, y, 4, kCornerVec);
+}
+
+swm_status swm_op_gradient(swm_ctx* C, int npad, int nx, int ny, int nsdm, int ndata, int nspace, int ncorners,
+                           const double* corner_area_inv, const double* v_weights, const double* xin, double* xout) {
+  OP_BEGIN(C);
+  OP_ARGS(corner_area_inv, v_weights, xin, xout);
+  OP_DIMS(npad, nx, ny, nsdm, ndata == 3 && nspace == 2 && ncorners == 2);
+  G.cai = io.put(corner_area_inv, 2, 2);
+  G.vw = io.put(v_weights, 12, 12);
+  double *x = io.put(xin, 1, 1), *y = io.dev(4);
+  OP_BUF(G.cai && G.vw && x && y);
+  hipLaunchKernelGGL(swm::dev::k_op_gradient, dim3(nb_), dim3(BLK), 0, C->stream, G, x, y);
+  return io.get(xout, y, 4, kCornerVec);
+}
+
+swm_status swm_op_average_to_cell_corners(swm_ctx* C, int npad, int nx, int ny, int nsdm, int ndata, int ncorners,
+                                          const double* s_weights, const double* xin, double* xout, int flag,
+                                          int min_value) {
+  OP_BEGIN(C);
+  OP_ARGS(s_weights, xin, xout);
+  OP_DIMS(npad, nx, ny, nsdm, ndata == 3 && ncorners == 2 && min_value >= -1 && min_value <= 1);
+  if (min_value == 0) {
+    // present and .false.: the reference leaves the interior undefined
+    swm::set_error("average_to_cell_corners: min_value = .false. leaves the result undefined");
+    return SWM_ERR_ARG;
+  }
+  G.sw = io.put(s_weights, 6, 6);
+  double *x = io.put(xin, 1, 1), *y = io.dev(2);
+  OP_BUF(G.sw && x && y);
+  const int mode = min_value == 1 ? 2 : (flag ? 0 : 1);
+  hipLaunchKernelGGL(swm::dev::k_op_avg_corners, dim3(nb_), dim3(BLK), 0, C->stream, G, mode, x, y);
   return io.get(xout, y, 2);
 }
 
-swm_status swm_op_calculate_velocity(swm_ctx* C, const double* psi, const double* chi, double* u) {
+swm_status swm_op_calculate_center_kinetic_energy(swm_ctx* C, int npad, int nx, int ny, int nsdm, int ndata,
+                                                  int nspace, int ncorners, const double* sub_area,
+                                                  const double* area_inv, const double* ke_c, double* ke) {
   OP_BEGIN(C);
-  OP_ARGS(psi, chi, u);
-  double *p = io.put(psi, 1), *c = io.put(chi, 1), *y = io.dev(4);
-  OP_BUF(p && c && y);
-  hipLaunchKernelGGL(swm::dev::k_velocity, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, p, c, y);
-  return io.get(u, y, 4, true);
+  OP_ARGS(sub_area, area_inv, ke_c, ke);
+  OP_DIMS(npad, nx, ny, nsdm, ndata == 3 && nspace == 2 && ncorners == 2);
+  G.sub_area = io.put(sub_area, 6, 6);
+  G.area_inv = io.put(area_inv, 1, 1);
+  double *x = io.put(ke_c, 2, 2), *y = io.dev(1);
+  OP_BUF(G.sub_area && G.area_inv && x && y);
+  hipLaunchKernelGGL(swm::dev::k_op_center_ke, dim3(nb_), dim3(BLK), 0, C->stream, G, x, y);
+  return io.get(ke, y, 1);
 }
 
-swm_status swm_op_centered_mass_flux(swm_ctx* C, const double* mass_c, const double* u, double* flux) {
+swm_status swm_op_calculate_velocity(swm_ctx* C, int npad, int nx, int ny, int nsdm, int ndata, int nspace,
+                                     int ncorners, const double* corner_area_inv, const double* v_weights,
+                                     const double* psi, const double* chi, double* xout) {
   OP_BEGIN(C);
-  OP_ARGS(mass_c, u, flux);
-  double *mc = io.put(mass_c, 2), *uu = io.put(u, 4, true), *m = io.zeros(1), *up = io.dev(3), *y = io.dev(3);
-  OP_BUF(mc && uu && m && up && y);
-  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, uu, m, mc, up, y);
+  OP_ARGS(corner_area_inv, v_weights, psi, chi, xout);
+  OP_DIMS(npad, nx, ny, nsdm, ndata == 3 && nspace == 2 && ncorners == 2);
+  G.cai = io.put(corner_area_inv, 2, 2);
+  G.vw = io.put(v_weights, 12, 12);
+  double *p = io.put(psi, 1, 1), *c = io.put(chi, 1, 1), *y = io.dev(4);
+  OP_BUF(G.cai && G.vw && p && c && y);
+  hipLaunchKernelGGL(swm::dev::k_velocity, dim3(nb_), dim3(BLK), 0, C->stream, G, p, c, y);
+  return io.get(xout, y, 4, kCornerVec);
+}
+
+swm_status swm_op_centered_mass_flux(swm_ctx* C, int npad, int nx, int ny, int nsdm, int ndata, int ncorners,
+                                     int nedges, int nspace, int r3, const double* dc, const double* normal,
+                                     const double* mass_c, const double* u, const double* mass, double* flux) {
+  OP_BEGIN(C);
+  OP_ARGS(dc, normal, mass_c, u, mass, flux);
+  OP_DIMS(npad, nx, ny, nsdm, ndata == 3 && ncorners == 2 && nedges == 3 && nspace == 2 && r3 == 3);
+  G.dc = io.put(dc, 6, 6);
+  G.nrm = io.put(normal, 18, 12, kFrame);
+  double *mc = io.put(mass_c, 2, 2), *uu = io.put(u, 4, 4, kCornerVec), *m = io.put(mass, 1, 1), *up = io.dev(3),
+         *y = io.dev(3);
+  OP_BUF(G.dc && G.nrm && mc && uu && m && up && y);
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, G, uu, m, mc, up, y);
   return io.get(flux, y, 3);
 }
 
-swm_status swm_op_upstream_mass_flux(swm_ctx* C, const double* u, const double* mass, double* flux) {
+swm_status swm_op_upstream_mass_flux(swm_ctx* C, int npad, int nx, int ny, int nsdm, int ndata, int ncorners,
+                                     int nedges, int nspace, int r3, const double* dc, const double* normal,
+                                     const double* u, const double* mass, double* flux) {
   OP_BEGIN(C);
-  OP_ARGS(u, mass, flux);
-  double *uu = io.put(u, 4, true), *m = io.put(mass, 1), *mc = io.zeros(2), *y = io.dev(3), *cen = io.dev(3);
-  OP_BUF(uu && m && mc && y && cen);
-  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, uu, m, mc, y, cen);
+  OP_ARGS(dc, normal, u, mass, flux);
+  OP_DIMS(npad, nx, ny, nsdm, ndata == 3 && ncorners == 2 && nedges == 3 && nspace == 2 && r3 == 3);
+  G.dc = io.put(dc, 6, 6);
+  G.nrm = io.put(normal, 18, 12, kFrame);
+  double *uu = io.put(u, 4, 4, kCornerVec), *m = io.put(mass, 1, 1), *mc = io.zeros(2), *y = io.dev(3),
+         *cen = io.dev(3);
+  OP_BUF(G.dc && G.nrm && uu && m && mc && y && cen);
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, G, uu, m, mc, y, cen);
   return io.get(flux, y, 3);
 }
 
-swm_status swm_op_centered_vorticity_flux(swm_ctx* C, const double* mass_flux, const double* mass, const double* eta,
-                                          double* flux) {
+swm_status swm_op_centered_vorticity_flux(swm_ctx* C, int npad, int nx, int ny, int nsdm, int nedges,
+                                          const double* mass_flux, const double* mass, const double* xin,
+                                          double* xout) {
   OP_BEGIN(C);
-  OP_ARGS(mass_flux, mass, eta, flux);
-  double *mf = io.put(mass_flux, 3), *m = io.put(mass, 1), *e = io.put(eta, 1), *mc = io.zeros(2),
-         *uu = io.zeros(4), *y = io.dev(3), *dc = io.dev(3);
-  OP_BUF(mf && m && e && mc && uu && y && dc);
-  hipLaunchKernelGGL(swm::dev::k_vort_curl, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, mf, m, e, mc, uu, y, dc);
-  return io.get(flux, y, 3);
+  OP_ARGS(mass_flux, mass, xin, xout);
+  OP_DIMS(npad, nx, ny, nsdm, nedges == 3);
+  double *mf = io.put(mass_flux, 3, 3), *m = io.put(mass, 1, 1), *e = io.put(xin, 1, 1), *y = io.dev(3);
+  OP_BUF(mf && m && e && y);
+  hipLaunchKernelGGL(swm::dev::k_op_vort_flux, dim3(nb_), dim3(BLK), 0, C->stream, G, mf, m, e, y);
+  return io.get(xout, y, 3);
 }
 
-swm_status swm_op_centered_divergence_curl(swm_ctx* C, const double* mass_c, const double* eta, const double* u,
+swm_status swm_op_centered_divergence_curl(swm_ctx* C, int npad, int nx, int ny, int nsdm, int ndata, int ncorners,
+                                           int nedges, int nspace, int r3, const double* dc, const double* tangent,
+                                           const double* mass_c, const double* eta, const double* u,
                                            const double* mass, double* flux) {
   OP_BEGIN(C);
-  OP_ARGS(mass_c, eta, u, mass, flux);
-  double *mc = io.put(mass_c, 2), *e = io.put(eta, 1), *uu = io.put(u, 4, true), *m = io.put(mass, 1),
-         *mf = io.zeros(3), *vf = io.dev(3), *y = io.dev(3);
-  OP_BUF(mc && e && uu && m && mf && vf && y);
-  hipLaunchKernelGGL(swm::dev::k_vort_curl, dim3(nb_), dim3(BLK), 0, C->stream, C->geo, mf, m, e, mc, uu, vf, y);
+  OP_ARGS(dc, tangent, mass_c, eta, u, mass, flux);
+  OP_DIMS(npad, nx, ny, nsdm, ndata == 3 && ncorners == 2 && nedges == 3 && nspace == 2 && r3 == 3);
+  G.dc = io.put(dc, 6, 6);
+  G.tng = io.put(tangent, 18, 12, kFrame);
+  double *mc = io.put(mass_c, 2, 2), *e = io.put(eta, 1, 1), *uu = io.put(u, 4, 4, kCornerVec),
+         *m = io.put(mass, 1, 1), *y = io.dev(3);
+  OP_BUF(G.dc && G.tng && mc && e && uu && m && y);
+  hipLaunchKernelGGL(swm::dev::k_op_div_curl, dim3(nb_), dim3(BLK), 0, C->stream, G, mc, e, uu, m, y);
   return io.get(flux, y, 3);
 }
 
 swm_status swm_wrap(swm_ctx* C, int kind, int nspace, int nlayer, double* x) {
   OP_BEGIN(C);
+  OpIO io{C, (size_t)C->ns, {}, {}};
   if (!x || kind < 0 || kind > 2 || nlayer < 1 || nspace < 1 || nspace > 4 || (kind != 1 && nspace != 1)) {
     swm::set_error("swm_wrap: kind must be 0 (face), 1 (vrtx) or 2 (edge); nspace > 1 only for vrtx");
     return SWM_ERR_ARG;
@@ -1017,6 +1112,7 @@ swm_status swm_wrap(swm_ctx* C, int kind, int nspace, int nlayer, double* x) {
 
 swm_status swm_elliptic(swm_ctx* C, int nvar, double* rhs, double* xout) {
   OP_BEGIN(C);
+  OpIO io{C, (size_t)C->ns, {}, {}};
   if (!rhs || !xout || nvar < 1 || nvar > 2) {
     swm::set_error("swm_elliptic: nvar must be 1 or 2");
     return SWM_ERR_ARG;
@@ -1041,6 +1137,7 @@ swm_status swm_elliptic(swm_ctx* C, int nvar, double* rhs, double* xout) {
 
 swm_status swm_multigrid(swm_ctx* C, int nvar, const double* beta, double* alpha) {
   OP_BEGIN(C);
+  OpIO io{C, (size_t)C->ns, {}, {}};
   if (!beta || !alpha || nvar < 1 || nvar > 2) {
     swm::set_error("swm_multigrid: nvar must be 1 or 2");
     return SWM_ERR_ARG;

@@ -1127,3 +1127,116 @@ __global__ void __launch_bounds__(CO_THREADS) k_coarse_cycle(CoarseArgs P) {
 }  // namespace swm
 
 #undef SL
+
+namespace swm {
+namespace dev {
+
+// ------------------------------------------------ single operators --------
+// Kernels of the operator entry points that the time step does not use in
+// this form.  Same slot rules as the reference loops: results on
+// i = 2..nx-1, j = 2..ny-1 of every panel (npad = 1), zero elsewhere.
+#define SLP(i, j, k) ((i) - 1 + iim * ((j) - 1 + jjm * ((k) - 1)))
+
+// operators.F:592-682 gradient at the two corners; out planes ic + 2*space
+__global__ void k_op_gradient(Geo G, const double* __restrict__ x, double* __restrict__ out) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int p = 0; p < 4; ++p) out[p * ns + s] = 0.0;
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  const int nb[2][3] = {{s, SLP(i - 1, j, k), SLP(i - 1, j - 1, k)}, {s, SLP(i - 1, j - 1, k), SLP(i, j - 1, k)}};
+#define VW(kk, sp, ic) G.vw[((kk) + 3 * ((sp) + 2 * (ic))) * ns + s]
+  for (int ic = 0; ic < 2; ++ic)
+    for (int sp = 0; sp < 2; ++sp)
+      out[(ic + 2 * sp) * ns + s] =
+          (+VW(0, sp, ic) * x[nb[ic][0]] + VW(1, sp, ic) * x[nb[ic][1]] + VW(2, sp, ic) * x[nb[ic][2]]) * G.cai[ic * ns + s];
+#undef VW
+}
+
+// operators.F:528-582 k_cross_v on every slot; planes ic + 2*space
+__global__ void k_op_cross(int ns, const double* __restrict__ x, double* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  for (int ic = 0; ic < 2; ++ic) {
+    out[ic * ns + s] = -x[(ic + 2) * ns + s];
+    out[(ic + 2) * ns + s] = +x[ic * ns + s];
+  }
+}
+
+// operators.F:982-1112 average_to_cell_corners; mode 0: flag = .true.
+// (s_weights), 1: flag = .false. (weights 0.3333333), 2: min_value = .true.
+__global__ void k_op_avg_corners(Geo G, int mode, const double* __restrict__ x, double* __restrict__ out) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  if (!interior(c, iim, jjm)) {
+    const double f = x[SLP(2, 2, k)];
+    out[s] = f;
+    out[ns + s] = f;
+    return;
+  }
+  const int a = s, b = SLP(i - 1, j, k), d = SLP(i - 1, j - 1, k), e = SLP(i, j - 1, k);
+  const double* sw = G.sw;
+  if (mode == 0) {
+    out[s] = +sw[0 * ns + s] * x[a] + sw[1 * ns + s] * x[b] + sw[2 * ns + s] * x[d];
+    out[ns + s] = +sw[3 * ns + s] * x[a] + sw[4 * ns + s] * x[d] + sw[5 * ns + s] * x[e];
+  } else if (mode == 1) {
+    const double f = 0.3333333;
+    out[s] = +f * x[a] + f * x[b] + f * x[d];
+    out[ns + s] = +f * x[a] + f * x[d] + f * x[e];
+  } else {
+    out[s] = dmin(dmin(sw[0 * ns + s] * x[a], sw[1 * ns + s] * x[b]), sw[2 * ns + s] * x[d]);
+    out[ns + s] = dmin(dmin(sw[3 * ns + s] * x[a], sw[4 * ns + s] * x[d]), sw[5 * ns + s] * x[e]);
+  }
+}
+
+__device__ __forceinline__ double pv_of(const double* __restrict__ h, const double* __restrict__ eta, int sl) {
+  const double w = h[sl] > 0.0 ? 1.0 / h[sl] : 0.0;
+  return eta[sl] * w;
+}
+
+// operators.F:2000-2074 centered_vorticity_flux
+__global__ void k_op_vort_flux(Geo G, const double* __restrict__ mflux, const double* __restrict__ h,
+                               const double* __restrict__ eta, double* __restrict__ out) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) out[e * ns + s] = 0.0;
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  const int up[3] = {SLP(i - 1, j, k), SLP(i - 1, j - 1, k), SLP(i, j - 1, k)};
+  const double pv0 = pv_of(h, eta, s);
+  for (int e = 0; e < 3; ++e) out[e * ns + s] = 0.5 * (mflux[e * ns + s]) * (pv0 + pv_of(h, eta, up[e]));
+}
+
+// operators.F:2083-2186 centered_divergence_curl
+__global__ void k_op_div_curl(Geo G, const double* __restrict__ hc, const double* __restrict__ eta,
+                              const double* __restrict__ u, const double* __restrict__ h, double* __restrict__ out) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) out[e * ns + s] = 0.0;
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  const int up[3] = {SLP(i - 1, j, k), SLP(i - 1, j - 1, k), SLP(i, j - 1, k)};
+  const double pv0 = pv_of(h, eta, s);
+  const Half ht = half_edges(iim, jjm, ns, i, j, k, G.dc, G.tng, hc, u);
+  for (int e = 0; e < 3; ++e) out[e * ns + s] = (-ht.r[0][e] + ht.r[1][e]) * (pv_of(h, eta, up[e]) + pv0) * 0.5;
+}
+
+#undef SLP
+
+}  // namespace dev
+}  // namespace swm

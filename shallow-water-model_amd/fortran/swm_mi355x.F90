@@ -1,0 +1,535 @@
+!=======================================================================
+! swm_mi355x.F90 -- Fortran side of the MI355X dynamical core.
+!
+!   module swm_mi355x_c     ISO_C_BINDING interfaces of include/swm_mi355x.h,
+!                           one for one
+!   module swm_mi355x       the device context of a run and the drop-in
+!                           replacements of the reference's hot path:
+!                             step_dt, wrap_prognostics (swm/source/step.F:
+!                             114-212, 438-493), elliptic (utilities/layer_3/
+!                             elliptic_solver.F:40-195), multigrid
+!                             (utilities/layer_3/multigrid_solver.F:476-591),
+!                             wrap (utilities/layer_2/wrap_data.F:1790-1974)
+!   module operators_mi355x the operators of utilities/layer_2/operators.F
+!                           under their own names and argument lists
+!
+! A non-zero status from the library stops the program, as the reference's
+! abort_code does (utilities/layer_1/abort.F:26-49).
+!=======================================================================
+module swm_mi355x_c
+  use iso_c_binding
+  implicit none
+  public
+
+  integer(c_int), parameter :: SWM_OK = 0, SWM_FP64 = 0, SWM_MIXED = 1
+
+  type, bind(C) :: swm_config
+    integer(c_int) :: ntracer
+    real(c_double) :: dt
+    real(c_double) :: grav
+    integer(c_int) :: precision
+    integer(c_int) :: device
+  end type swm_config
+
+  interface
+    function swm_last_error() bind(C, name="swm_last_error") result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function
+    function swm_grid_points(g, p_list) bind(C, name="swm_grid_points") result(st)
+      import :: c_int, c_double
+      integer(c_int), value :: g
+      real(c_double) :: p_list(*)
+      integer(c_int) :: st
+    end function
+    function swm_grid_create(g, p_list, radius, omega, grid) bind(C, name="swm_grid_create") result(st)
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: g
+      type(c_ptr), value :: p_list
+      real(c_double), value :: radius, omega
+      type(c_ptr) :: grid
+      integer(c_int) :: st
+    end function
+    subroutine swm_grid_destroy(grid) bind(C, name="swm_grid_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: grid
+    end subroutine
+    function swm_grid_dims(grid, g, iim, jjm, max_ig) bind(C, name="swm_grid_dims") result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: grid
+      integer(c_int) :: g, iim, jjm, max_ig
+      integer(c_int) :: st
+    end function
+    function swm_grid_array(grid, name, out) bind(C, name="swm_grid_array") result(st)
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: grid
+      character(kind=c_char) :: name(*)
+      type(c_ptr), value :: out
+      integer(c_int) :: st
+    end function
+    function swm_create(grid, cfg, ctx) bind(C, name="swm_create") result(st)
+      import :: c_int, c_ptr, swm_config
+      type(c_ptr), value :: grid
+      type(swm_config) :: cfg
+      type(c_ptr) :: ctx
+      integer(c_int) :: st
+    end function
+    subroutine swm_destroy(ctx) bind(C, name="swm_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine
+    function swm_init_state(ctx, h, hs, u, tracer) bind(C, name="swm_init_state") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: h(*), u(*), tracer(*)
+      type(c_ptr), value :: hs
+      integer(c_int) :: st
+    end function
+    function swm_init_synthetic(ctx, seed_index, lmax, umax, h0, h_rms, nbumps, bump_radius) &
+        bind(C, name="swm_init_synthetic") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: seed_index, lmax, nbumps
+      real(c_double), value :: umax, h0, h_rms, bump_radius
+      integer(c_int) :: st
+    end function
+    function swm_step(ctx, nsteps) bind(C, name="swm_step") result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps
+      integer(c_int) :: st
+    end function
+    function swm_get_field(ctx, name, out) bind(C, name="swm_get_field") result(st)
+      import :: c_int, c_ptr, c_char, c_double
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: name(*)
+      real(c_double) :: out(*)
+      integer(c_int) :: st
+    end function
+    function swm_synchronize(ctx) bind(C, name="swm_synchronize") result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: st
+    end function
+    function swm_step_count(ctx) bind(C, name="swm_step_count") result(n)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: n
+    end function
+    function swm_last_vcycles(ctx) bind(C, name="swm_last_vcycles") result(n)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: n
+    end function
+    function swm_total_mass(ctx, mass) bind(C, name="swm_total_mass") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: mass
+      integer(c_int) :: st
+    end function
+    function swm_wrap(ctx, kind, nspace, nlayer, x) bind(C, name="swm_wrap") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kind, nspace, nlayer
+      real(c_double) :: x(*)
+      integer(c_int) :: st
+    end function
+    function swm_elliptic(ctx, nvar, rhs, xout) bind(C, name="swm_elliptic") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nvar
+      real(c_double) :: rhs(*), xout(*)
+      integer(c_int) :: st
+    end function
+    function swm_multigrid(ctx, nvar, beta, alpha) bind(C, name="swm_multigrid") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nvar
+      real(c_double) :: beta(*), alpha(*)
+      integer(c_int) :: st
+    end function
+    ! ---- operators.F, argument lists of the reference plus nsdm ----
+    function swm_op_curl_of_a_vector(ctx, npad, nx, ny, nsdm, ndata, nspace, ncorners, area_inv, &
+        v_weights, xin, xout) bind(C, name="swm_op_curl_of_a_vector") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npad, nx, ny, nsdm, ndata, nspace, ncorners
+      real(c_double) :: area_inv(*), v_weights(*), xin(*), xout(*)
+      integer(c_int) :: st
+    end function
+    function swm_op_div_of_a_vector(ctx, npad, nx, ny, nsdm, ndata, nspace, ncorners, area_inv, &
+        v_weights, xin, xout) bind(C, name="swm_op_div_of_a_vector") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npad, nx, ny, nsdm, ndata, nspace, ncorners
+      real(c_double) :: area_inv(*), v_weights(*), xin(*), xout(*)
+      integer(c_int) :: st
+    end function
+    function swm_op_div_of_an_edge_flux(ctx, npad, nx, ny, nsdm, nedges, area_inv, xin, xout) &
+        bind(C, name="swm_op_div_of_an_edge_flux") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npad, nx, ny, nsdm, nedges
+      real(c_double) :: area_inv(*), xin(*), xout(*)
+      integer(c_int) :: st
+    end function
+    function swm_op_laplacian_of_a_scalar(ctx, npad, nx, ny, nsdm, nweights, l_weights, xin, xout) &
+        bind(C, name="swm_op_laplacian_of_a_scalar") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npad, nx, ny, nsdm, nweights
+      real(c_double) :: l_weights(*), xin(*), xout(*)
+      integer(c_int) :: st
+    end function
+    function swm_op_k_cross_v(ctx, nx, ny, nsdm, nspace, ncorners, xin, xout) &
+        bind(C, name="swm_op_k_cross_v") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nx, ny, nsdm, nspace, ncorners
+      real(c_double) :: xin(*), xout(*)
+      integer(c_int) :: st
+    end function
+    function swm_op_gradient(ctx, npad, nx, ny, nsdm, ndata, nspace, ncorners, corner_area_inv, &
+        v_weights, xin, xout) bind(C, name="swm_op_gradient") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npad, nx, ny, nsdm, ndata, nspace, ncorners
+      real(c_double) :: corner_area_inv(*), v_weights(*), xin(*), xout(*)
+      integer(c_int) :: st
+    end function
+    function swm_op_average_to_cell_corners(ctx, npad, nx, ny, nsdm, ndata, ncorners, s_weights, &
+        xin, xout, flag, min_value) bind(C, name="swm_op_average_to_cell_corners") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npad, nx, ny, nsdm, ndata, ncorners, flag, min_value
+      real(c_double) :: s_weights(*), xin(*), xout(*)
+      integer(c_int) :: st
+    end function
+    function swm_op_calculate_center_kinetic_energy(ctx, npad, nx, ny, nsdm, ndata, nspace, &
+        ncorners, sub_area, area_inv, ke_c, ke) bind(C, name="swm_op_calculate_center_kinetic_energy") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npad, nx, ny, nsdm, ndata, nspace, ncorners
+      real(c_double) :: sub_area(*), area_inv(*), ke_c(*), ke(*)
+      integer(c_int) :: st
+    end function
+    function swm_op_calculate_velocity(ctx, npad, nx, ny, nsdm, ndata, nspace, ncorners, &
+        corner_area_inv, v_weights, psi, chi, xout) bind(C, name="swm_op_calculate_velocity") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npad, nx, ny, nsdm, ndata, nspace, ncorners
+      real(c_double) :: corner_area_inv(*), v_weights(*), psi(*), chi(*), xout(*)
+      integer(c_int) :: st
+    end function
+    function swm_op_centered_mass_flux(ctx, npad, nx, ny, nsdm, ndata, ncorners, nedges, nspace, r3, &
+        dc, normal, mass_c, u, mass, flux) bind(C, name="swm_op_centered_mass_flux") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npad, nx, ny, nsdm, ndata, ncorners, nedges, nspace, r3
+      real(c_double) :: dc(*), normal(*), mass_c(*), u(*), mass(*), flux(*)
+      integer(c_int) :: st
+    end function
+    function swm_op_upstream_mass_flux(ctx, npad, nx, ny, nsdm, ndata, ncorners, nedges, nspace, r3, &
+        dc, normal, u, mass, flux) bind(C, name="swm_op_upstream_mass_flux") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npad, nx, ny, nsdm, ndata, ncorners, nedges, nspace, r3
+      real(c_double) :: dc(*), normal(*), u(*), mass(*), flux(*)
+      integer(c_int) :: st
+    end function
+    function swm_op_centered_vorticity_flux(ctx, npad, nx, ny, nsdm, nedges, mass_flux, mass, xin, &
+        xout) bind(C, name="swm_op_centered_vorticity_flux") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npad, nx, ny, nsdm, nedges
+      real(c_double) :: mass_flux(*), mass(*), xin(*), xout(*)
+      integer(c_int) :: st
+    end function
+    function swm_op_centered_divergence_curl(ctx, npad, nx, ny, nsdm, ndata, ncorners, nedges, &
+        nspace, r3, dc, tangent, mass_c, eta, u, mass, flux) &
+        bind(C, name="swm_op_centered_divergence_curl") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npad, nx, ny, nsdm, ndata, ncorners, nedges, nspace, r3
+      real(c_double) :: dc(*), tangent(*), mass_c(*), eta(*), u(*), mass(*), flux(*)
+      integer(c_int) :: st
+    end function
+  end interface
+end module swm_mi355x_c
+
+!=======================================================================
+module swm_mi355x
+  use iso_c_binding
+  use swm_mi355x_c
+  implicit none
+  private
+  public :: swm_gpu_initialize, swm_gpu_finalize, swm_gpu_check, swm_gpu_context, swm_gpu_grid_array, &
+            step_dt, wrap_prognostics, sync_prognostics_to_host, elliptic, multigrid, wrap_face, &
+            wrap_vrtx, wrap_edge
+
+  type(c_ptr), save :: ctx = c_null_ptr
+  type(c_ptr), save :: grid = c_null_ptr
+  integer, save :: iim_ = 0, jjm_ = 0, ntracer_ = 0
+
+contains
+
+  ! status check: a failure stops the program, as abort_code does
+  subroutine swm_gpu_check(st, where)
+    integer(c_int), intent(in) :: st
+    character(len=*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    character(len=512) :: text
+    integer :: k
+    if (st == SWM_OK) return
+    call c_f_pointer(swm_last_error(), msg, [512])
+    text = ''
+    do k = 1, 512
+      if (msg(k) == c_null_char) exit
+      text(k:k) = msg(k)
+    end do
+    write(*, '(a,a,a,i0,a,a)') 'swm_mi355x: ', where, ' failed, status ', st, ': ', trim(text)
+    stop 1
+  end subroutine swm_gpu_check
+
+  function swm_gpu_context() result(p)
+    type(c_ptr) :: p
+    p = ctx
+  end function
+
+  ! Replaces the device-relevant part of initialize_model (swm/source/
+  ! initialize.F:82-140): builds the level-g grid (the reference's own cell
+  ! centres may be passed in p_list(3, 10*4**g+2)), the device state and the
+  ! initial fields h, hs (iim,jjm,10), u (2,2,iim,jjm,10), tracer
+  ! (iim,jjm,ntracer,10).
+  subroutine swm_gpu_initialize(g, ntracer, dt, grav, radius, omega, h, hs, u, tracer, p_list, device)
+    integer, intent(in) :: g, ntracer
+    real(c_double), intent(in) :: dt, grav, radius, omega
+    real(c_double), intent(in), target :: h(:,:,:), hs(:,:,:), u(:,:,:,:,:), tracer(:,:,:,:)
+    real(c_double), intent(in), optional, target :: p_list(:,:)
+    integer, intent(in), optional :: device
+    type(swm_config) :: cfg
+    type(c_ptr) :: pl
+    integer(c_int) :: gg, mig
+    pl = c_null_ptr
+    if (present(p_list)) pl = c_loc(p_list)
+    call swm_gpu_check(swm_grid_create(int(g, c_int), pl, radius, omega, grid), 'swm_grid_create')
+    call swm_gpu_check(swm_grid_dims(grid, gg, iim_, jjm_, mig), 'swm_grid_dims')
+    cfg%ntracer = ntracer
+    cfg%dt = dt
+    cfg%grav = grav
+    cfg%precision = SWM_FP64
+    cfg%device = 0
+    if (present(device)) cfg%device = device
+    ntracer_ = ntracer
+    call swm_gpu_check(swm_create(grid, cfg, ctx), 'swm_create')
+    call swm_gpu_check(swm_init_state(ctx, h, c_loc(hs), u, tracer), 'swm_init_state')
+  end subroutine swm_gpu_initialize
+
+  ! a named set-up array of the run's grid (swm_grid_array), e.g. the
+  ! metrics the operator subroutines take as arguments
+  subroutine swm_gpu_grid_array(name, out)
+    character(len=*), intent(in) :: name
+    real(c_double), intent(out), target, contiguous :: out(..)
+    call swm_gpu_check(swm_grid_array(grid, trim(name)//c_null_char, c_loc(out)), 'swm_grid_array '//name)
+  end subroutine
+
+  subroutine swm_gpu_finalize()
+    if (c_associated(ctx)) call swm_destroy(ctx)
+    if (c_associated(grid)) call swm_grid_destroy(grid)
+    ctx = c_null_ptr
+    grid = c_null_ptr
+  end subroutine
+
+  ! step::step_dt with get_pointers/get_weights of the driver loop
+  ! (swm/source/driver.F:103-121): one time step on the device
+  subroutine step_dt()
+    call swm_gpu_check(swm_step(ctx, 1_c_int), 'step_dt')
+  end subroutine step_dt
+
+  ! step::wrap_prognostics: the device step leaves every halo filled
+  subroutine wrap_prognostics()
+  end subroutine wrap_prognostics
+
+  ! current time level of the prognostic fields to host arrays (before
+  ! output or restart writes of the driver)
+  subroutine sync_prognostics_to_host(h, u, tracer, eta, div, psi, chi, ke)
+    real(c_double), intent(out) :: h(:,:,:), u(:,:,:,:,:), tracer(:,:,:,:)
+    real(c_double), intent(out), optional :: eta(:,:,:), div(:,:,:), psi(:,:,:), chi(:,:,:), ke(:,:,:)
+    call swm_gpu_check(swm_get_field(ctx, 'h'//c_null_char, h), 'get h')
+    call swm_gpu_check(swm_get_field(ctx, 'u'//c_null_char, u), 'get u')
+    call swm_gpu_check(swm_get_field(ctx, 'tracer'//c_null_char, tracer), 'get tracer')
+    if (present(eta)) call swm_gpu_check(swm_get_field(ctx, 'eta'//c_null_char, eta), 'get eta')
+    if (present(div)) call swm_gpu_check(swm_get_field(ctx, 'div'//c_null_char, div), 'get div')
+    if (present(psi)) call swm_gpu_check(swm_get_field(ctx, 'psi'//c_null_char, psi), 'get psi')
+    if (present(chi)) call swm_gpu_check(swm_get_field(ctx, 'chi'//c_null_char, chi), 'get chi')
+    if (present(ke)) call swm_gpu_check(swm_get_field(ctx, 'ke'//c_null_char, ke), 'get ke')
+  end subroutine sync_prognostics_to_host
+
+  ! elliptic_solver_xxx::elliptic (component_name is not needed)
+  subroutine elliptic(component_name, nvar, rhs, xout)
+    character(len=*), intent(in) :: component_name
+    integer, intent(in) :: nvar
+    real(c_double), intent(inout) :: rhs(:,:,:,:), xout(:,:,:,:)
+    call swm_gpu_check(swm_elliptic(ctx, int(nvar, c_int), rhs, xout), 'elliptic '//component_name)
+  end subroutine elliptic
+
+  ! multigrid_solver_xxx::multigrid: one V-cycle
+  subroutine multigrid(nvar, beta, alpha)
+    integer, intent(in) :: nvar
+    real(c_double), intent(in) :: beta(:,:,:,:)
+    real(c_double), intent(inout) :: alpha(:,:,:,:)
+    call swm_gpu_check(swm_multigrid(ctx, int(nvar, c_int), beta, alpha), 'multigrid')
+  end subroutine multigrid
+
+  ! wrap_data::wrap, one keyword at a time: face (iim,jjm,nlayer,10)
+  subroutine wrap_face(x, nlayer)
+    real(c_double), intent(inout) :: x(*)
+    integer, intent(in) :: nlayer
+    call swm_gpu_check(swm_wrap(ctx, 0_c_int, 1_c_int, int(nlayer, c_int), x), 'wrap face')
+  end subroutine
+  ! vrtx (nspace,2,iim,jjm,nlayer,10)
+  subroutine wrap_vrtx(x, nspace, nlayer)
+    real(c_double), intent(inout) :: x(*)
+    integer, intent(in) :: nspace, nlayer
+    call swm_gpu_check(swm_wrap(ctx, 1_c_int, int(nspace, c_int), int(nlayer, c_int), x), 'wrap vrtx')
+  end subroutine
+  ! edge (3,iim,jjm,nlayer,10)
+  subroutine wrap_edge(x, nlayer)
+    real(c_double), intent(inout) :: x(*)
+    integer, intent(in) :: nlayer
+    call swm_gpu_check(swm_wrap(ctx, 2_c_int, 1_c_int, int(nlayer, c_int), x), 'wrap edge')
+  end subroutine
+end module swm_mi355x
+
+!=======================================================================
+! operators.F under the reference's names and argument lists (one panel per
+! call, as the reference calls them); the device context is the run's.
+module operators_mi355x
+  use iso_c_binding
+  use swm_mi355x_c
+  use swm_mi355x, only: swm_gpu_check, swm_gpu_context
+  implicit none
+  private
+  public :: curl_of_a_vector, div_of_a_vector, div_of_an_edge_flux, laplacian_of_a_scalar, k_cross_v, &
+            gradient, average_to_cell_corners, calculate_center_kinetic_energy, calculate_velocity, &
+            centered_mass_flux, upstream_mass_flux, centered_vorticity_flux, centered_divergence_curl
+
+contains
+
+  subroutine curl_of_a_vector(npad, nx, ny, ndata, nspace, ncorners, area_inv, v_weights, xin, xout)
+    integer, intent(in) :: npad, nx, ny, ndata, nspace, ncorners
+    real(c_double), intent(in) :: area_inv(nx,ny), v_weights(ndata,nspace,ncorners,nx,ny), &
+                                  xin(nspace,ncorners,nx,ny)
+    real(c_double), intent(out) :: xout(nx,ny)
+    call swm_gpu_check(swm_op_curl_of_a_vector(swm_gpu_context(), npad, nx, ny, 1, ndata, nspace, ncorners, &
+                       area_inv, v_weights, xin, xout), 'curl_of_a_vector')
+  end subroutine
+
+  subroutine div_of_a_vector(npad, nx, ny, ndata, nspace, ncorners, area_inv, v_weights, xin, xout)
+    integer, intent(in) :: npad, nx, ny, ndata, nspace, ncorners
+    real(c_double), intent(in) :: area_inv(nx,ny), v_weights(ndata,nspace,ncorners,nx,ny), &
+                                  xin(nspace,ncorners,nx,ny)
+    real(c_double), intent(out) :: xout(nx,ny)
+    call swm_gpu_check(swm_op_div_of_a_vector(swm_gpu_context(), npad, nx, ny, 1, ndata, nspace, ncorners, &
+                       area_inv, v_weights, xin, xout), 'div_of_a_vector')
+  end subroutine
+
+  subroutine div_of_an_edge_flux(npad, nx, ny, nedges, area_inv, xin, xout)
+    integer, intent(in) :: npad, nx, ny, nedges
+    real(c_double), intent(in) :: area_inv(nx,ny), xin(nedges,nx,ny)
+    real(c_double), intent(out) :: xout(nx,ny)
+    call swm_gpu_check(swm_op_div_of_an_edge_flux(swm_gpu_context(), npad, nx, ny, 1, nedges, area_inv, xin, &
+                       xout), 'div_of_an_edge_flux')
+  end subroutine
+
+  subroutine laplacian_of_a_scalar(npad, nx, ny, nweights, l_weights, xin, xout)
+    integer, intent(in) :: npad, nx, ny, nweights
+    real(c_double), intent(in) :: l_weights(nweights,nx,ny), xin(nx,ny)
+    real(c_double), intent(out) :: xout(nx,ny)
+    call swm_gpu_check(swm_op_laplacian_of_a_scalar(swm_gpu_context(), npad, nx, ny, 1, nweights, l_weights, &
+                       xin, xout), 'laplacian_of_a_scalar')
+  end subroutine
+
+  subroutine k_cross_v(nx, ny, nspace, ncorners, xin, xout)
+    integer, intent(in) :: nx, ny, nspace, ncorners
+    real(c_double), intent(in) :: xin(nspace,ncorners,nx,ny)
+    real(c_double), intent(out) :: xout(nspace,ncorners,nx,ny)
+    call swm_gpu_check(swm_op_k_cross_v(swm_gpu_context(), nx, ny, 1, nspace, ncorners, xin, xout), 'k_cross_v')
+  end subroutine
+
+  subroutine gradient(npad, nx, ny, ndata, nspace, ncorners, corner_area_inv, v_weights, xin, xout)
+    integer, intent(in) :: npad, nx, ny, ndata, nspace, ncorners
+    real(c_double), intent(in) :: corner_area_inv(ncorners,nx,ny), v_weights(ndata,nspace,ncorners,nx,ny), &
+                                  xin(nx,ny)
+    real(c_double), intent(out) :: xout(nspace,ncorners,nx,ny)
+    call swm_gpu_check(swm_op_gradient(swm_gpu_context(), npad, nx, ny, 1, ndata, nspace, ncorners, &
+                       corner_area_inv, v_weights, xin, xout), 'gradient')
+  end subroutine
+
+  subroutine average_to_cell_corners(npad, nx, ny, ndata, ncorners, s_weights, xin, xout, flag, min_value)
+    integer, intent(in) :: npad, nx, ny, ndata, ncorners
+    real(c_double), intent(in) :: s_weights(ndata,ncorners,nx,ny), xin(nx,ny)
+    real(c_double), intent(out) :: xout(ncorners,nx,ny)
+    logical :: flag
+    logical, optional :: min_value
+    integer(c_int) :: f, mv
+    f = merge(1, 0, flag)
+    mv = -1
+    if (present(min_value)) mv = merge(1, 0, min_value)
+    call swm_gpu_check(swm_op_average_to_cell_corners(swm_gpu_context(), npad, nx, ny, 1, ndata, ncorners, &
+                       s_weights, xin, xout, f, mv), 'average_to_cell_corners')
+  end subroutine
+
+  subroutine calculate_center_kinetic_energy(npad, nx, ny, ndata, nspace, ncorners, sub_area, area_inv, ke_c, ke)
+    integer, intent(in) :: npad, nx, ny, ndata, nspace, ncorners
+    real(c_double), intent(in) :: sub_area(ndata,ncorners,nx,ny), area_inv(nx,ny), ke_c(ncorners,nx,ny)
+    real(c_double), intent(out) :: ke(nx,ny)
+    call swm_gpu_check(swm_op_calculate_center_kinetic_energy(swm_gpu_context(), npad, nx, ny, 1, ndata, nspace, &
+                       ncorners, sub_area, area_inv, ke_c, ke), 'calculate_center_kinetic_energy')
+  end subroutine
+
+  subroutine calculate_velocity(npad, nx, ny, ndata, nspace, ncorners, corner_area_inv, v_weights, psi, chi, xout)
+    integer, intent(in) :: npad, nx, ny, ndata, nspace, ncorners
+    real(c_double), intent(in) :: corner_area_inv(ncorners,nx,ny), v_weights(ndata,nspace,ncorners,nx,ny), &
+                                  psi(nx,ny), chi(nx,ny)
+    real(c_double), intent(out) :: xout(nspace,ncorners,nx,ny)
+    call swm_gpu_check(swm_op_calculate_velocity(swm_gpu_context(), npad, nx, ny, 1, ndata, nspace, ncorners, &
+                       corner_area_inv, v_weights, psi, chi, xout), 'calculate_velocity')
+  end subroutine
+
+  subroutine centered_mass_flux(npad, nx, ny, ndata, ncorners, nedges, nspace, r3, dc, normal, mass_c, u, mass, flux)
+    integer, intent(in) :: npad, nx, ny, ndata, ncorners, nedges, nspace, r3
+    real(c_double), intent(in) :: dc(ndata,ncorners,nx,ny), normal(ndata,r3,ncorners,nx,ny), &
+                                  mass_c(ncorners,nx,ny), u(nspace,ncorners,nx,ny), mass(nx,ny)
+    real(c_double), intent(out) :: flux(nedges,nx,ny)
+    call swm_gpu_check(swm_op_centered_mass_flux(swm_gpu_context(), npad, nx, ny, 1, ndata, ncorners, nedges, &
+                       nspace, r3, dc, normal, mass_c, u, mass, flux), 'centered_mass_flux')
+  end subroutine
+
+  subroutine upstream_mass_flux(npad, nx, ny, ndata, ncorners, nedges, nspace, r3, dc, normal, u, mass, flux)
+    integer, intent(in) :: npad, nx, ny, ndata, ncorners, nedges, nspace, r3
+    real(c_double), intent(in) :: dc(ndata,ncorners,nx,ny), normal(ndata,r3,ncorners,nx,ny), &
+                                  u(nspace,ncorners,nx,ny), mass(nx,ny)
+    real(c_double), intent(out) :: flux(nedges,nx,ny)
+    call swm_gpu_check(swm_op_upstream_mass_flux(swm_gpu_context(), npad, nx, ny, 1, ndata, ncorners, nedges, &
+                       nspace, r3, dc, normal, u, mass, flux), 'upstream_mass_flux')
+  end subroutine
+
+  subroutine centered_vorticity_flux(npad, nx, ny, nedges, mass_flux, mass, xin, xout)
+    integer, intent(in) :: npad, nx, ny, nedges
+    real(c_double), intent(in) :: mass_flux(nedges,nx,ny), mass(nx,ny), xin(nx,ny)
+    real(c_double), intent(out) :: xout(nedges,nx,ny)
+    call swm_gpu_check(swm_op_centered_vorticity_flux(swm_gpu_context(), npad, nx, ny, 1, nedges, mass_flux, &
+                       mass, xin, xout), 'centered_vorticity_flux')
+  end subroutine
+
+  subroutine centered_divergence_curl(npad, nx, ny, ndata, ncorners, nedges, nspace, r3, dc, tangent, mass_c, &
+                                      eta, u, mass, flux)
+    integer, intent(in) :: npad, nx, ny, ndata, ncorners, nedges, nspace, r3
+    real(c_double), intent(in) :: dc(ndata,ncorners,nx,ny), tangent(ndata,r3,ncorners,nx,ny), &
+                                  mass_c(ncorners,nx,ny), eta(nx,ny), u(nspace,ncorners,nx,ny), mass(nx,ny)
+    real(c_double), intent(out) :: flux(nedges,nx,ny)
+    call swm_gpu_check(swm_op_centered_divergence_curl(swm_gpu_context(), npad, nx, ny, 1, ndata, ncorners, &
+                       nedges, nspace, r3, dc, tangent, mass_c, eta, u, mass, flux), 'centered_divergence_curl')
+  end subroutine
+end module operators_mi355x

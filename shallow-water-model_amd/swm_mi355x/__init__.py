@@ -86,24 +86,43 @@ _sig("swm_wrap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, cty
 _sig("swm_elliptic", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
 _sig("swm_multigrid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
 
-# reference operators: name -> (input kinds, output kind); kinds are cell "c",
-# corner scalar "cc" (2,...), corner vector "v" (2,2,...), edge "e" (3,...)
+# reference operators (operators.F): name -> (integer arguments after ctx,
+# array arguments as (name, leading shape), output (name, leading shape));
+# the C entry points take the subroutine's argument list plus nsdm
+_FIXED = {"npad": 1, "ndata": 3, "nspace": 2, "ncorners": 2, "nedges": 3, "nweights": 7, "r3": 3}
 OPERATORS = {
-    "div_of_an_edge_flux": (("e",), "c"),
-    "laplacian_of_a_scalar": (("c",), "c"),
-    "curl_of_a_vector": (("v",), "c"),
-    "div_of_a_vector": (("v",), "c"),
-    "calculate_center_kinetic_energy": (("cc",), "c"),
-    "average_to_cell_corners": (("c",), "cc"),
-    "calculate_velocity": (("c", "c"), "v"),
-    "centered_mass_flux": (("cc", "v"), "e"),
-    "upstream_mass_flux": (("v", "c"), "e"),
-    "centered_vorticity_flux": (("e", "c", "c"), "e"),
-    "centered_divergence_curl": (("cc", "c", "v", "c"), "e"),
+    "curl_of_a_vector": (("npad", "nx", "ny", "nsdm", "ndata", "nspace", "ncorners"),
+                         (("area_inv", ()), ("v_weights", (3, 2, 2)), ("xin", (2, 2))), ("xout", ())),
+    "div_of_a_vector": (("npad", "nx", "ny", "nsdm", "ndata", "nspace", "ncorners"),
+                        (("area_inv", ()), ("v_weights", (3, 2, 2)), ("xin", (2, 2))), ("xout", ())),
+    "div_of_an_edge_flux": (("npad", "nx", "ny", "nsdm", "nedges"), (("area_inv", ()), ("xin", (3,))), ("xout", ())),
+    "laplacian_of_a_scalar": (("npad", "nx", "ny", "nsdm", "nweights"), (("l_weights", (7,)), ("xin", ())),
+                              ("xout", ())),
+    "k_cross_v": (("nx", "ny", "nsdm", "nspace", "ncorners"), (("xin", (2, 2)),), ("xout", (2, 2))),
+    "gradient": (("npad", "nx", "ny", "nsdm", "ndata", "nspace", "ncorners"),
+                 (("corner_area_inv", (2,)), ("v_weights", (3, 2, 2)), ("xin", ())), ("xout", (2, 2))),
+    "average_to_cell_corners": (("npad", "nx", "ny", "nsdm", "ndata", "ncorners"),
+                                (("s_weights", (3, 2)), ("xin", ())), ("xout", (2,))),
+    "calculate_center_kinetic_energy": (("npad", "nx", "ny", "nsdm", "ndata", "nspace", "ncorners"),
+                                        (("sub_area", (3, 2)), ("area_inv", ()), ("ke_c", (2,))), ("ke", ())),
+    "calculate_velocity": (("npad", "nx", "ny", "nsdm", "ndata", "nspace", "ncorners"),
+                           (("corner_area_inv", (2,)), ("v_weights", (3, 2, 2)), ("psi", ()), ("chi", ())),
+                           ("xout", (2, 2))),
+    "centered_mass_flux": (("npad", "nx", "ny", "nsdm", "ndata", "ncorners", "nedges", "nspace", "r3"),
+                           (("dc", (3, 2)), ("normal", (3, 3, 2)), ("mass_c", (2,)), ("u", (2, 2)), ("mass", ())),
+                           ("flux", (3,))),
+    "upstream_mass_flux": (("npad", "nx", "ny", "nsdm", "ndata", "ncorners", "nedges", "nspace", "r3"),
+                           (("dc", (3, 2)), ("normal", (3, 3, 2)), ("u", (2, 2)), ("mass", ())), ("flux", (3,))),
+    "centered_vorticity_flux": (("npad", "nx", "ny", "nsdm", "nedges"),
+                                (("mass_flux", (3,)), ("mass", ()), ("xin", ())), ("xout", (3,))),
+    "centered_divergence_curl": (("npad", "nx", "ny", "nsdm", "ndata", "ncorners", "nedges", "nspace", "r3"),
+                                 (("dc", (3, 2)), ("tangent", (3, 3, 2)), ("mass_c", (2,)), ("eta", ()),
+                                  ("u", (2, 2)), ("mass", ())), ("flux", (3,))),
 }
-for _n, (_ins, _o) in OPERATORS.items():
-    _sig("swm_op_" + _n, ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * (len(_ins) + 1))
-_KIND_LEAD = {"c": (), "cc": (2,), "v": (2, 2), "e": (3,)}
+for _n, (_ints, _ins, _o) in OPERATORS.items():
+    _extra = [ctypes.c_int, ctypes.c_int] if _n == "average_to_cell_corners" else []
+    _sig("swm_op_" + _n, ctypes.c_int,
+         [ctypes.c_void_p] + [ctypes.c_int] * len(_ints) + [ctypes.c_void_p] * (len(_ins) + 1) + _extra)
 
 
 def _check(status):
@@ -228,18 +247,27 @@ class Model:
     def last_vcycles(self):
         return lib.swm_last_vcycles(self._h)
 
-    def _shape(self, kind):
-        return _KIND_LEAD[kind] + (self.iim, self.jjm, 10)
-
-    def op(self, name, *args):
-        """One reference operator (``operators::<name>``) on host arrays of
-        the reference layout; metrics are the model's grid."""
-        ins, okind = OPERATORS[name]
-        if len(args) != len(ins):
-            raise TypeError(f"{name} takes {len(ins)} arrays")
-        arrs = [_f64(a, self._shape(k)) for a, k in zip(args, ins)]
-        out = np.zeros(self._shape(okind), dtype=np.float64, order="F")
-        _check(getattr(lib, "swm_op_" + name)(self._h, *[_ptr(a) for a in arrs], _ptr(out)))
+    def op(self, name, *arrays, flag=True, min_value=None):
+        """One reference operator (``operators::<name>``) on host arrays in the
+        reference layout, metric arrays included, in the subroutine's
+        argument order: (leading dims, nx, ny) for one panel or (leading
+        dims, nx, ny, nsdm) for nsdm panels at once."""
+        ints, ins, (_, olead) = OPERATORS[name]
+        if len(arrays) != len(ins):
+            raise TypeError(f"{name} takes {len(ins)} arrays: {[n for n, _ in ins]}")
+        lead0 = ins[0][1]
+        a0 = np.asarray(arrays[0])
+        nx, ny = a0.shape[len(lead0)], a0.shape[len(lead0) + 1]
+        nsdm = a0.shape[len(lead0) + 2] if a0.ndim == len(lead0) + 3 else 1
+        tail = (nx, ny) if a0.ndim == len(lead0) + 2 else (nx, ny, nsdm)
+        arrs = [_f64(a, lead + tail) for a, (_, lead) in zip(arrays, ins)]
+        vals = dict(_FIXED, nx=nx, ny=ny, nsdm=nsdm)
+        out = np.zeros(olead + tail, dtype=np.float64, order="F")
+        extra = []
+        if name == "average_to_cell_corners":
+            extra = [1 if flag else 0, -1 if min_value is None else (1 if min_value else 0)]
+        _check(getattr(lib, "swm_op_" + name)(self._h, *[vals[k] for k in ints], *[_ptr(a) for a in arrs],
+                                              _ptr(out), *extra))
         return out
 
     def wrap(self, kind, x, nspace=1):

@@ -38,33 +38,86 @@ def _ops_gold():
     return a
 
 
+def _op_cases(grid, a):
+    m = {n: grid.array(n) for n in ("area_inv", "v_weights", "l_weights", "corner_area_inv", "s_weights", "sub_area",
+                                    "dc", "normal", "tangent")}
+    return [
+        ("div_of_an_edge_flux", (m["area_inv"], a["e1"]), "o1"),
+        ("laplacian_of_a_scalar", (m["l_weights"], a["s1"]), "o2"),
+        ("curl_of_a_vector", (m["area_inv"], m["v_weights"], a["vc"]), "o3"),
+        ("div_of_a_vector", (m["area_inv"], m["v_weights"], a["vc"]), "o4"),
+        ("calculate_center_kinetic_energy", (m["sub_area"], m["area_inv"], a["cc"]), "o5"),
+        ("average_to_cell_corners", (m["s_weights"], a["s1"]), "oc"),
+        ("calculate_velocity", (m["corner_area_inv"], m["v_weights"], a["s1"], a["s2"]), "ov"),
+        ("centered_mass_flux", (m["dc"], m["normal"], a["cc"], a["vc"], a["s2"]), "oe"),
+        ("upstream_mass_flux", (m["dc"], m["normal"], a["vc"], a["s2"]), "oe2"),
+        ("centered_vorticity_flux", (a["e1"], a["s2"], a["s1"]), "oe3"),
+        ("centered_divergence_curl", (m["dc"], m["tangent"], a["cc"], a["s1"], a["vc"], a["s2"]), "oe4"),
+    ], m
+
+
 def test_operators_match_reference_bit_for_bit():
+    """All ten panels in one call, and panel by panel as the reference calls
+    them (nsdm = 1), against the reference's outputs on every slot."""
     print("case operators g=3 golden", flush=True)
     a = _ops_gold()
     grid = S.Grid(3)
     model = S.Model(grid, ntracer=1, dt=1800.0, grav=S.REF_GRAV)
-    cases = [
-        ("div_of_an_edge_flux", (a["e1"],), "o1"),
-        ("laplacian_of_a_scalar", (a["s1"],), "o2"),
-        ("curl_of_a_vector", (a["vc"],), "o3"),
-        ("div_of_a_vector", (a["vc"],), "o4"),
-        ("calculate_center_kinetic_energy", (a["cc"],), "o5"),
-        ("average_to_cell_corners", (a["s1"],), "oc"),
-        ("calculate_velocity", (a["s1"], a["s2"]), "ov"),
-        ("centered_mass_flux", (a["cc"], a["vc"]), "oe"),
-        ("upstream_mass_flux", (a["vc"], a["s2"]), "oe2"),
-        ("centered_vorticity_flux", (a["e1"], a["s2"], a["s1"]), "oe3"),
-        ("centered_divergence_curl", (a["cc"], a["s1"], a["vc"], a["s2"]), "oe4"),
-    ]
+    cases, _ = _op_cases(grid, a)
     report = {}
     for name, args, ref in cases:
         out = model.op(name, *args)
         bad = out != a[ref]
         if bad.any():
-            idx = np.argwhere(bad)
-            report[name] = (int(bad.sum()), idx[:4].tolist(),
-                            float(np.abs(out - a[ref]).max()), float(np.abs(a[ref]).max()))
-    assert not report, f"operators differing from the reference (count, first slots, max diff, scale): {report}"
+            report[name] = (int(bad.sum()), np.argwhere(bad)[:4].tolist(), float(np.abs(out - a[ref]).max()))
+        for k in (0, 9):
+            one = model.op(name, *[np.asfortranarray(x[..., k]) for x in args])
+            if not np.array_equal(one, a[ref][..., k]):
+                report[f"{name} panel {k + 1}"] = "differs"
+    assert not report, f"operators differing from the reference (count, first slots, max diff): {report}"
+
+
+def test_gradient_and_k_cross_v_compose_to_reference_velocity():
+    # calculate_velocity = gradient(chi) + k_cross_v(gradient(psi)) (operators.F:1402-1514)
+    a = _ops_gold()
+    grid = S.Grid(3)
+    model = S.Model(grid, ntracer=1, dt=1800.0, grav=S.REF_GRAV)
+    _, m = _op_cases(grid, a)
+    gchi = model.op("gradient", m["corner_area_inv"], m["v_weights"], a["s2"])
+    gpsi = model.op("gradient", m["corner_area_inv"], m["v_weights"], a["s1"])
+    kx = model.op("k_cross_v", gpsi)
+    assert np.array_equal(kx[0], -gpsi[1]) and np.array_equal(kx[1], gpsi[0])
+    assert np.array_equal(gchi + kx, a["ov"])
+
+
+def test_average_to_cell_corners_other_modes():
+    # flag = .false. (fixed weights 0.3333333) and min_value = .true.
+    # (operators.F:1052-1106), written out in numpy on the interior; ghost
+    # rows and columns carry xin(2,2) of the panel
+    a = _ops_gold()
+    grid = S.Grid(3)
+    model = S.Model(grid, ntracer=1, dt=1800.0, grav=S.REF_GRAV)
+    sw = grid.array("s_weights")
+    x = a["s1"]
+    n = grid.iim
+    I = slice(1, n - 1)
+    Im = slice(0, n - 2)
+    f = 0.3333333
+    out = model.op("average_to_cell_corners", sw, x, flag=False)
+    assert np.array_equal(out[0, I, I], f * x[I, I] + f * x[Im, I] + f * x[Im, Im])
+    assert np.array_equal(out[1, I, I], f * x[I, I] + f * x[Im, Im] + f * x[I, Im])
+    mn = model.op("average_to_cell_corners", sw, x, min_value=True)
+    assert np.array_equal(mn[0, I, I], np.minimum(np.minimum(sw[0, 0, I, I] * x[I, I], sw[1, 0, I, I] * x[Im, I]),
+                                                  sw[2, 0, I, I] * x[Im, Im]))
+    assert np.array_equal(mn[1, I, I], np.minimum(np.minimum(sw[0, 1, I, I] * x[I, I], sw[1, 1, I, I] * x[Im, Im]),
+                                                  sw[2, 1, I, I] * x[I, Im]))
+    for o in (out, mn):
+        ghost = np.ones((n, n), bool)
+        ghost[I, I] = False
+        for k in range(10):
+            assert np.all(o[:, ghost, k] == x[1, 1, k])
+    with pytest.raises(S.SwmError):
+        model.op("average_to_cell_corners", sw, x, min_value=False)
 
 
 @pytest.mark.parametrize("g", [3, 5])
@@ -135,4 +188,9 @@ def test_entry_point_errors():
     with pytest.raises(S.SwmError):
         S._check(S.lib.swm_elliptic(model._h, 3, None, None))
     with pytest.raises(S.SwmError):
-        S._check(S.lib.swm_op_laplacian_of_a_scalar(model._h, None, None))
+        S._check(S.lib.swm_op_laplacian_of_a_scalar(model._h, 1, grid.iim, grid.iim, 10, 7, None, None, None))
+    lw = grid.array("l_weights")
+    x = np.zeros((grid.iim, grid.iim, 10), order="F")
+    with pytest.raises(S.SwmError):   # npad other than 1
+        S._check(S.lib.swm_op_laplacian_of_a_scalar(model._h, 2, grid.iim, grid.iim, 10, 7, S._ptr(lw), S._ptr(x),
+                                                    S._ptr(x)))
