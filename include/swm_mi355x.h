@@ -130,6 +130,18 @@ swm_status swm_synchronize(swm_ctx* ctx);
 int swm_step_count(const swm_ctx* ctx);
 int swm_last_vcycles(const swm_ctx* ctx);
 
+/* Convergence settings of the elliptic solve.  The reference's are the
+ * defaults: mean(r^2) over real cells <= 1e-20 (relative = 0, tol = 1e-20)
+ * within 12 V-cycles, else SWM_ERR_NOT_CONVERGED where the reference stops
+ * (elliptic_solver.F:120-180).  relative = 1 tests sqrt(mean(r^2) /
+ * mean(rhs^2)) <= tol instead.  max_cycles counts all cycles (>= 2).  No
+ * counterpart in the reference, where these are compile-time constants. */
+swm_status swm_set_solver(swm_ctx* ctx, double tol, int relative, int max_cycles);
+
+/* Mean squared residuals at the end of the last elliptic solve, and mean
+ * squared right-hand sides (relative criterion only), per field (psi, chi). */
+swm_status swm_last_residual(const swm_ctx* ctx, double resid_ms[2], double rhs_ms[2]);
+
 /* Device time of the last swm_step call and of its hottest kernel, measured
  * with HIP events on the context's stream (ms), and that kernel's mean
  * duration and algorithmic bytes per launch. */

@@ -55,6 +55,37 @@ struct Arena {
 
 }  // namespace
 
+namespace {
+
+// one multigrid level of a view: weights, reciprocal centre weight, halo
+// map and the level's alpha / beta / smoother scratch (two fields each)
+template <class R>
+struct LevelT {
+  int iim = 0, jjm = 0, ns = 0;
+  R *lw = nullptr, *lwt = nullptr, *alpha = nullptr, *beta = nullptr, *temp = nullptr;
+  int* src = nullptr;
+  swm::dev::LevT<R> L{};
+};
+
+// the device model in one storage type: metrics, multigrid hierarchy, state
+// and scratch of the time step.  R = double: the FP64 path; R = float: the
+// mixed path (thickness-like fields stored less h0, vorticity less f).
+template <class R>
+struct View {
+  swm::dev::GeoT<R> geo{};
+  std::vector<LevelT<R>> lev;
+  R *h[2] = {}, *eta[2] = {}, *div[2] = {}, *u[2] = {};
+  std::vector<R*> tr[2], told;
+  R *eta_f[3] = {}, *div_f[3] = {}, *msh_f[3] = {};
+  R *relative = nullptr, *ke = nullptr, *inv_h = nullptr, *mass_flux = nullptr;
+  R *hc = nullptr, *flx1 = nullptr, *anti = nullptr, *mss_low = nullptr, *beta = nullptr, *f1 = nullptr,
+    *f2 = nullptr, *ml = nullptr, *bdn = nullptr, *bup = nullptr, *vf = nullptr, *dcurl = nullptr, *rhs = nullptr;
+  R* state_begin = nullptr;   // state and scratch are contiguous from here
+  size_t state_words = 0;
+};
+
+}  // namespace
+
 struct swm_ctx {
   int g = 0, iim = 0, jjm = 0, ns = 0, nt = 0, max_ig = 0;
   double dt = 0, grav = 0;
@@ -62,26 +93,18 @@ struct swm_ctx {
   hipStream_t stream = nullptr;
   Arena arena;
   size_t arena_bytes = 0;
-  // metrics
+  // FP64 metrics (both paths: set-up, FP64 residual, sums)
   double *area, *area_inv, *cai, *sub_area, *dc, *nrm, *tng, *vw, *sw, *lw, *f, *hs, *mask, *area_mask, *pxyz;
   double total_area = 0.0, radius = 0.0;
-  Geo geo;
   DevTable face, vrtx, edge;
-  struct Level {
-    int iim, jjm, ns;
-    double *lw, *lwt, *alpha, *beta, *temp;
-    int* src;
-    Lev L;
-  };
-  std::vector<Level> lev;
-  // state
-  double *h[2], *eta[2], *div[2], *u[2];
-  std::vector<double*> tr[2], told;
-  double *eta_f[3], *div_f[3], *msh_f[3];
-  double *relative, *ke, *xout, *inv_h, *mass_flux;
-  // scratch
-  double *hc, *etac, *flx1, *anti, *mss_low, *beta, *f1, *f2, *ml, *bdn, *bup, *vf, *dcurl, *rhs, *r2;
-  double *part, *sums;
+  View<double> d;       // FP64 model (the FP64 path; initial state of the mixed path)
+  View<float> m;        // mixed path
+  float* mx_r = nullptr;   // mixed elliptic: FP32 defect (2 planes)
+  float* mx_e = nullptr;   // mixed elliptic: FP32 correction (2 planes)
+  double h0 = 0.0;         // mixed path: thickness offset
+  double* xout = nullptr;  // streamfunction and velocity potential (FP64, both paths)
+  double* r2 = nullptr;
+  double *part = nullptr, *sums = nullptr;
   double* host_sums = nullptr;   // pinned
   int npart = 0;
   // time stepping
@@ -89,6 +112,13 @@ struct swm_ctx {
   double w1 = 0, w2 = 0, w3 = 0;
   int last_vcycles = 0;
   bool initialized = false;
+  // elliptic solve settings (reference: mean(r^2) <= 1e-20, 12 cycles) and
+  // the mean squared residuals / right-hand sides of the last solve
+  double solver_tol = 1.0e-20;
+  int solver_relative = 0;
+  int solver_max_cycles = 12;
+  double last_resid[2] = {0.0, 0.0};
+  double last_rhs_ms[2] = {0.0, 0.0};
   // timing
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   double last_step_ms = 0.0;
@@ -129,65 +159,61 @@ swm_status make_table(swm_ctx* C, const swm::HaloTable& t, int ns, DevTable& d) 
   d.n = (int)dst.size();
   d.dst = C->arena.take<int>(dst.size() + 1);
   d.src = C->arena.take<int>(src.size() + 1);
-  if (d.n) {
-    HIP_TRY(hipMemcpyAsync(d.dst, dst.data(), dst.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
-    HIP_TRY(hipMemcpyAsync(d.src, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice, C->stream));
+  if (d.n && C->arena.base) {
+    HIP_TRY(hipMemcpy(d.dst, dst.data(), dst.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.src, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   return SWM_OK;
 }
 
-void gather(swm_ctx* C, const DevTable& t, std::initializer_list<double*> fields) {
-  FieldSet fs{};
-  int nf = 0;
-  for (double* p : fields) {
-    fs.p[nf++] = p;
-    if (nf == 8) {
-      hipLaunchKernelGGL(swm::dev::k_gather, dim3(nblk(t.n), nf), dim3(BLK), 0, C->stream, t.dst, t.src, t.n, fs);
-      nf = 0;
-    }
-  }
-  if (nf && t.n)
-    hipLaunchKernelGGL(swm::dev::k_gather, dim3(nblk(t.n), nf), dim3(BLK), 0, C->stream, t.dst, t.src, t.n, fs);
-}
-void gather_list(swm_ctx* C, const DevTable& t, const std::vector<double*>& fields) {
+template <class R>
+void gather_list(swm_ctx* C, const DevTable& t, const std::vector<R*>& fields) {
   for (size_t a = 0; a < fields.size(); a += 8) {
-    FieldSet fs{};
+    swm::dev::FieldSetT<R> fs{};
     int nf = 0;
     for (size_t b = a; b < fields.size() && b < a + 8; ++b) fs.p[nf++] = fields[b];
-    if (t.n) hipLaunchKernelGGL(swm::dev::k_gather, dim3(nblk(t.n), nf), dim3(BLK), 0, C->stream, t.dst, t.src, t.n, fs);
+    if (t.n) hipLaunchKernelGGL(swm::dev::k_gather<R>, dim3(nblk(t.n), nf), dim3(BLK), 0, C->stream, t.dst, t.src, t.n, fs);
   }
 }
+template <class R>
+void gather(swm_ctx* C, const DevTable& t, std::initializer_list<R*> fields) {
+  gather_list<R>(C, t, std::vector<R*>(fields));
+}
 
-void fill(swm_ctx* C, double* x, size_t n, double v) {
-  hipLaunchKernelGGL(swm::dev::k_fill, dim3(nblk(n)), dim3(BLK), 0, C->stream, x, n, v);
+template <class R>
+void fill(swm_ctx* C, R* x, size_t n, R v) {
+  hipLaunchKernelGGL(swm::dev::k_fill<R>, dim3(nblk(n)), dim3(BLK), 0, C->stream, x, n, v);
 }
 
 // ------------------------------------------------------------- multigrid --
-template <int NV>
-void launch_relax(swm_ctx* C, const swm_ctx::Level& L, bool zero, const double* beta, const double* a, double* t) {
+template <class R, int NV>
+void launch_relax(swm_ctx* C, const LevelT<R>& L, bool zero, const R* beta, const R* a, R* t) {
   const dim3 grid(nblk(L.ns)), blk(BLK);
-  if (zero) hipLaunchKernelGGL((swm::dev::k_relax<NV, true>), grid, blk, 0, C->stream, L.L, beta, a, t);
-  else hipLaunchKernelGGL((swm::dev::k_relax<NV, false>), grid, blk, 0, C->stream, L.L, beta, a, t);
+  if (zero) hipLaunchKernelGGL((swm::dev::k_relax<R, NV, true>), grid, blk, 0, C->stream, L.L, beta, a, t);
+  else hipLaunchKernelGGL((swm::dev::k_relax<R, NV, false>), grid, blk, 0, C->stream, L.L, beta, a, t);
 }
 
 // iters Jacobi sweeps at level l (an even count leaves the result in alpha);
 // zero_start: alpha is known to be zero on entry
-void relax(swm_ctx* C, int l, int iters, double* alpha, const double* beta, int nvar, bool zero_start) {
-  swm_ctx::Level& L = C->lev[l - 1];
-  double* a = alpha;
-  double* t = L.temp;
+template <class R>
+void relax(swm_ctx* C, View<R>& V, int l, int iters, R* alpha, const R* beta, int nvar, bool zero_start) {
+  LevelT<R>& L = V.lev[l - 1];
+  R* a = alpha;
+  R* t = L.temp;
   for (int it = 0; it < iters; ++it) {
     const bool prof = C->profile && l == C->g && C->kev_used + 2 <= C->kev.size();
     if (prof) {
       (void)hipEventRecord(C->kev[C->kev_used++], C->stream);
       // algorithmic bytes of one finest-level sweep per real cell: six
       // Laplacian weights and the reciprocal centre weight, and per field
-      // beta, alpha and the result, all FP64 (neighbour values hit in cache)
-      C->kernel_bytes += (double)C->max_ig * 8.0 * (7.0 + 3.0 * nvar);
+      // beta, alpha and the result (neighbour values hit in cache); alpha is
+      // not read by a zero-start sweep
+      const bool z = zero_start && it == 0;
+      C->kernel_bytes += (double)C->max_ig * sizeof(R) * (7.0 + (z ? 2.0 : 3.0) * nvar);
     }
     const bool zero = zero_start && it == 0;
-    if (nvar == 2) launch_relax<2>(C, L, zero, beta, a, t);
-    else launch_relax<1>(C, L, zero, beta, a, t);
+    if (nvar == 2) launch_relax<R, 2>(C, L, zero, beta, a, t);
+    else launch_relax<R, 1>(C, L, zero, beta, a, t);
     if (prof) (void)hipEventRecord(C->kev[C->kev_used++], C->stream);
     std::swap(a, t);
   }
@@ -196,58 +222,95 @@ void relax(swm_ctx* C, int l, int iters, double* alpha, const double* beta, int 
 // multigrid_solver.F:476-591.  The coarse levels' alpha is zero at the start
 // of a cycle (the reference clears it); their first sweep is told so instead
 // of clearing memory.  Coarse beta is written by the injection at every slot
-// the smoother reads.
-void vcycle(swm_ctx* C, int nvar, double* alpha, const double* beta) {
+// the smoother reads.  top_zero: the finest alpha is zero on entry as well
+// (a correction cycle of the mixed path).
+template <class R>
+void vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_zero = false) {
   const int G = C->g;
   const int top = G < swm::dev::CO_MAXLEV ? G : swm::dev::CO_MAXLEV;   // levels 1..top in one workgroup
-  auto A = [&](int l) { return l == G ? alpha : C->lev[l - 1].alpha; };
-  auto B = [&](int l) { return l == G ? const_cast<double*>(beta) : C->lev[l - 1].beta; };
+  auto A = [&](int l) { return l == G ? alpha : V.lev[l - 1].alpha; };
+  auto B = [&](int l) { return l == G ? const_cast<R*>(beta) : V.lev[l - 1].beta; };
   for (int l = G; l > top; --l) {
-    relax(C, l, 2, A(l), B(l), nvar, l < G);
-    hipLaunchKernelGGL(swm::dev::k_inject, dim3(nblk(C->lev[l - 2].ns), nvar), dim3(BLK), 0, C->stream,
-                       C->lev[l - 1].L, C->lev[l - 2].L, A(l), B(l), B(l - 1));
+    relax<R>(C, V, l, 2, A(l), B(l), nvar, l < G || top_zero);
+    hipLaunchKernelGGL(swm::dev::k_inject<R>, dim3(nblk(V.lev[l - 2].ns), nvar), dim3(BLK), 0, C->stream,
+                       V.lev[l - 1].L, V.lev[l - 2].L, A(l), B(l), B(l - 1));
   }
-  swm::dev::CoarseArgs P{};
-  for (int l = 1; l <= top; ++l) P.lev[l - 1] = C->lev[l - 1].L;
+  swm::dev::CoarseArgs<R> P{};
+  for (int l = 1; l <= top; ++l) P.lev[l - 1] = V.lev[l - 1].L;
   P.nlev = top;
   P.nvar = nvar;
   P.beta = B(top);
   P.alpha = A(top);
-  P.zero = top < G ? 1 : 0;
-  hipLaunchKernelGGL(swm::dev::k_coarse_cycle, dim3(1), dim3(swm::dev::CO_THREADS), 0, C->stream, P);
+  P.zero = (top < G || top_zero) ? 1 : 0;
+  hipLaunchKernelGGL(swm::dev::k_coarse_cycle<R>, dim3(1), dim3(swm::dev::CO_THREADS), 0, C->stream, P);
   for (int l = top; l <= G - 1; ++l) {
-    hipLaunchKernelGGL(swm::dev::k_interp, dim3(nblk(C->lev[l].ns), nvar), dim3(BLK), 0, C->stream,
-                       C->lev[l - 1].L, C->lev[l].L, A(l), A(l + 1));
-    relax(C, l + 1, 2, A(l + 1), B(l + 1), nvar, false);
+    hipLaunchKernelGGL(swm::dev::k_interp<R>, dim3(nblk(V.lev[l].ns), nvar), dim3(BLK), 0, C->stream,
+                       V.lev[l - 1].L, V.lev[l].L, A(l), A(l + 1));
+    relax<R>(C, V, l + 1, 2, A(l + 1), B(l + 1), nvar, false);
   }
 }
 
 // sums of x0*w and x1*w into C->sums[0..1] (block partials, then a
 // compensated fixed-order sum)
-void sum2(swm_ctx* C, const double* x0, const double* x1, const double* w, int n) {
-  hipLaunchKernelGGL(swm::dev::k_sum_partial, dim3(C->npart, 2), dim3(BLK), 0, C->stream, x0, x1, w, n, C->part);
+template <class R>
+void sum2(swm_ctx* C, const R* x0, const R* x1, const R* w, int n) {
+  hipLaunchKernelGGL(swm::dev::k_sum_partial<R>, dim3(C->npart, 2), dim3(BLK), 0, C->stream, x0, x1, w, n, C->part);
   hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(256), 0, C->stream, C->part, C->npart, C->sums);
 }
 
-// elliptic_solver.F:40-195 on the two fields rhs / xout of the context
+swm_status read_sums(swm_ctx* C, int n, double* out) {
+  HIP_TRY(hipMemcpyAsync(C->host_sums, C->sums, n * sizeof(double), hipMemcpyDeviceToHost, C->stream));
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  for (int b = 0; b < n; ++b) out[b] = C->host_sums[b];
+  return SWM_OK;
+}
+
+// convergence test of elliptic_solver.F:203-239 on the mean squared
+// residual of field v: mean(r^2) <= tol (the reference's absolute form), or,
+// when set relative, sqrt(mean(r^2) / mean(rhs^2)) <= tol
+bool converged(const swm_ctx* C, int v, double ms) {
+  if (C->solver_relative) return ms <= C->solver_tol * C->solver_tol * C->last_rhs_ms[v];
+  return ms <= C->solver_tol;
+}
+
+// mean squares of the right-hand sides over real cells (relative criterion)
+template <class R>
+swm_status rhs_mean_squares(swm_ctx* C, const R* rhs, int nvar) {
+  if (!C->solver_relative) return SWM_OK;
+  const int ns = C->ns;
+  hipLaunchKernelGGL(swm::dev::k_sumsq_mixed<R>, dim3(C->npart, nvar), dim3(BLK), 0, C->stream, rhs,
+                     rhs + (size_t)(nvar - 1) * ns, C->mask, ns, C->part);
+  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(nvar), dim3(256), 0, C->stream, C->part, C->npart, C->sums);
+  double ms[2];
+  swm_status st = read_sums(C, nvar, ms);
+  if (st) return st;
+  for (int v = 0; v < nvar; ++v) C->last_rhs_ms[v] = ms[v] / (double)C->max_ig;
+  return SWM_OK;
+}
+
+// elliptic_solver.F:40-195 on the FP64 fields rhs / xout
 swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
   const int ns = C->ns;
-  for (int v = 0; v < nvar; ++v) gather(C, C->face, {xout + (size_t)v * ns, rhs + (size_t)v * ns});
-  sum2(C, rhs, nvar > 1 ? rhs + ns : rhs, C->area_mask, ns);
-  hipLaunchKernelGGL(swm::dev::k_sub_mean, dim3(nblk(ns)), dim3(BLK), 0, C->stream, ns, C->sums, C->total_area,
-                     rhs, nvar > 1 ? rhs + ns : C->r2);
-  for (int it = 0; it < 2; ++it) vcycle(C, nvar, xout, rhs);
-  for (int v = 0; v < nvar; ++v) gather(C, C->face, {xout + (size_t)v * ns});
+  View<double>& V = C->d;
+  for (int v = 0; v < nvar; ++v) gather<double>(C, C->face, {xout + (size_t)v * ns, rhs + (size_t)v * ns});
+  sum2<double>(C, rhs, nvar > 1 ? rhs + ns : rhs, C->area_mask, ns);
+  hipLaunchKernelGGL(swm::dev::k_sub_mean<double>, dim3(nblk(ns)), dim3(BLK), 0, C->stream, ns, C->sums,
+                     C->total_area, rhs, nvar > 1 ? rhs + ns : C->r2);
+  swm_status st0 = rhs_mean_squares<double>(C, rhs, nvar);
+  if (st0) return st0;
+  for (int it = 0; it < 2; ++it) vcycle<double>(C, V, nvar, xout, rhs);
+  for (int v = 0; v < nvar; ++v) gather<double>(C, C->face, {xout + (size_t)v * ns});
   auto norms = [&](const std::vector<int>& vars, double* out) -> swm_status {
     const int v0 = vars[0], v1 = vars.back();
     hipLaunchKernelGGL(swm::dev::k_resid2_partial, dim3(C->npart, (unsigned)vars.size()), dim3(BLK), 0, C->stream,
-                       C->geo, xout + (size_t)v0 * ns, rhs + (size_t)v0 * ns, xout + (size_t)v1 * ns,
+                       V.geo, xout + (size_t)v0 * ns, rhs + (size_t)v0 * ns, xout + (size_t)v1 * ns,
                        rhs + (size_t)v1 * ns, C->part);
     hipLaunchKernelGGL(swm::dev::k_sum_final, dim3((unsigned)vars.size()), dim3(256), 0, C->stream, C->part, C->npart,
                        C->sums);
-    HIP_TRY(hipMemcpyAsync(C->host_sums, C->sums, vars.size() * sizeof(double), hipMemcpyDeviceToHost, C->stream));
-    HIP_TRY(hipStreamSynchronize(C->stream));
-    for (size_t b = 0; b < vars.size(); ++b) out[b] = C->host_sums[b] / (double)C->max_ig;
+    double raw[2];
+    swm_status st = read_sums(C, (int)vars.size(), raw);
+    if (st) return st;
+    for (size_t b = 0; b < vars.size(); ++b) out[b] = raw[b] / (double)C->max_ig;
     return SWM_OK;
   };
   std::vector<int> all;
@@ -256,25 +319,112 @@ swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
   swm_status st = norms(all, nrm);
   if (st) return st;
   std::vector<int> bad;
-  for (int v = 0; v < nvar; ++v)
-    if (nrm[v] > 1.0e-20) bad.push_back(v);
+  for (int v = 0; v < nvar; ++v) {
+    C->last_resid[v] = nrm[v];
+    if (!converged(C, v, nrm[v])) bad.push_back(v);
+  }
   int iter = 0;
   bool any = !bad.empty();
-  while (any && iter < 10) {
+  while (any && iter < C->solver_max_cycles - 2) {
     ++iter;
-    if (bad.size() == 2) vcycle(C, 2, xout, rhs);
-    else vcycle(C, 1, xout + (size_t)bad[0] * ns, rhs + (size_t)bad[0] * ns);
-    for (int v : bad) gather(C, C->face, {xout + (size_t)v * ns});
+    if (bad.size() == 2) vcycle<double>(C, V, 2, xout, rhs);
+    else vcycle<double>(C, V, 1, xout + (size_t)bad[0] * ns, rhs + (size_t)bad[0] * ns);
+    for (int v : bad) gather<double>(C, C->face, {xout + (size_t)v * ns});
     double nb[2];
     st = norms(bad, nb);
     if (st) return st;
     any = false;
-    for (size_t b = 0; b < bad.size(); ++b)
-      if (nb[b] > 1.0e-20) any = true;
+    for (size_t b = 0; b < bad.size(); ++b) {
+      C->last_resid[bad[b]] = nb[b];
+      if (!converged(C, bad[b], nb[b])) any = true;
+    }
   }
   C->last_vcycles = 2 + iter;
   if (any) {
-    swm::set_error("elliptic solve did not converge (mean squared residual above 1e-20 after 12 V-cycles)");
+    swm::set_error("elliptic solve did not converge (residual above the criterion after " +
+                   std::to_string(C->solver_max_cycles) + " V-cycles; the reference stops here, "
+                   "elliptic_solver.F:176-178)");
+    return SWM_ERR_NOT_CONVERGED;
+  }
+  return SWM_OK;
+}
+
+// Mixed path: the same solve as FP64 defect correction.  The FP32 right-hand
+// side (mean removed with FP64 sums) is the problem; the FP64 solution xout is
+// the iterate.  Each cycle forms the defect r = rhs - L(xout) with the FP64
+// weights (k_defect, which also sums r^2 for the convergence test), runs one
+// FP32 V-cycle on L(e) = r from e = 0, and adds e to xout.  As in the FP64
+// path: two cycles, then the reference's criterion mean(r^2) <= 1e-20 checked
+// before each further cycle, at most 12 cycles in all.
+swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
+  const int ns = C->ns;
+  View<float>& V = C->m;
+  const swm::dev::Geo& G64 = C->d.geo;
+  gather<double>(C, C->face, {xout, xout + ns});
+  gather<float>(C, C->face, {rhs, rhs + ns});
+  // area-weighted means in FP64 (elliptic_solver.F:80-92)
+  hipLaunchKernelGGL((swm::dev::k_sum_partial<float, double>), dim3(C->npart, 2), dim3(BLK), 0, C->stream, rhs,
+                     rhs + ns, C->area_mask, ns, C->part);
+  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(256), 0, C->stream, C->part, C->npart, C->sums);
+  hipLaunchKernelGGL(swm::dev::k_sub_mean<float>, dim3(nblk(ns)), dim3(BLK), 0, C->stream, ns, C->sums,
+                     C->total_area, rhs, rhs + ns);
+  // defect of fields v0 (and v1 when nv = 2) into mx_r, with partial sums
+  auto defect = [&](int nv, int v0, int v1) {
+    hipLaunchKernelGGL(swm::dev::k_defect, dim3(C->npart, nv), dim3(BLK), 0, C->stream, G64, xout + (size_t)v0 * ns,
+                       rhs + (size_t)v0 * ns, C->mx_r + (size_t)v0 * ns, xout + (size_t)v1 * ns,
+                       rhs + (size_t)v1 * ns, C->mx_r + (size_t)v1 * ns, C->part);
+  };
+  // one FP32 correction cycle on fields v0.. (nv of them, consecutive or one)
+  auto correct = [&](int nv, int v0) {
+    vcycle<float>(C, V, nv, C->mx_e + (size_t)v0 * ns, C->mx_r + (size_t)v0 * ns, true);
+    hipLaunchKernelGGL(swm::dev::k_correct, dim3(nblk((size_t)nv * ns)), dim3(BLK), 0, C->stream, (size_t)nv * ns,
+                       C->mx_e + (size_t)v0 * ns, xout + (size_t)v0 * ns);
+    for (int v = v0; v < v0 + nv; ++v) gather<double>(C, C->face, {xout + (size_t)v * ns});
+  };
+  auto norms = [&](int nv, double* out) -> swm_status {
+    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(nv), dim3(256), 0, C->stream, C->part, C->npart, C->sums);
+    double raw[2];
+    swm_status st = read_sums(C, nv, raw);
+    if (st) return st;
+    for (int b = 0; b < nv; ++b) out[b] = raw[b] / (double)C->max_ig;
+    return SWM_OK;
+  };
+  swm_status st0 = rhs_mean_squares<float>(C, rhs, 2);
+  if (st0) return st0;
+  for (int it = 0; it < 2; ++it) {
+    defect(2, 0, 1);
+    correct(2, 0);
+  }
+  defect(2, 0, 1);
+  double nrm[2];
+  swm_status st = norms(2, nrm);
+  if (st) return st;
+  std::vector<int> bad;
+  for (int v = 0; v < 2; ++v) {
+    C->last_resid[v] = nrm[v];
+    if (!converged(C, v, nrm[v])) bad.push_back(v);
+  }
+  int iter = 0;
+  bool any = !bad.empty();
+  while (any && iter < C->solver_max_cycles - 2) {
+    ++iter;
+    // mx_r holds the defect of every field still in bad
+    if (bad.size() == 2) correct(2, 0);
+    else correct(1, bad[0]);
+    defect((int)bad.size(), bad[0], bad.back());
+    double nb[2];
+    st = norms((int)bad.size(), nb);
+    if (st) return st;
+    any = false;
+    for (size_t b = 0; b < bad.size(); ++b) {
+      C->last_resid[bad[b]] = nb[b];
+      if (!converged(C, bad[b], nb[b])) any = true;
+    }
+  }
+  C->last_vcycles = 2 + iter;
+  if (any) {
+    swm::set_error("elliptic solve did not converge (residual above the criterion after " +
+                   std::to_string(C->solver_max_cycles) + " correction cycles)");
     return SWM_ERR_NOT_CONVERGED;
   }
   return SWM_OK;
@@ -292,73 +442,78 @@ void weights(swm_ctx* C) {    // model_control.F:58-89
   if (sc >= 3) { C->w3 = 1.916666666666666666666; C->w2 = -1.333333333333333333333; C->w1 = 0.416666666666666666666; }
 }
 
-void wrap_prognostics(swm_ctx* C) {   // step.F:438-493
+template <class R>
+void wrap_prognostics(swm_ctx* C, View<R>& V) {   // step.F:438-493
   const int n4 = C->n4 - 1;
   const int ns = C->ns;
-  hipLaunchKernelGGL(swm::dev::k_inv_h, dim3(nblk(ns)), dim3(BLK), 0, C->stream, C->geo, C->h[n4], C->inv_h);
-  std::vector<double*> f = {C->h[n4], C->eta[n4], C->div[n4], C->relative, C->ke, C->inv_h};
-  for (int q = 0; q < C->nt; ++q) f.push_back(C->tr[n4][q]);
-  gather_list(C, C->face, f);
-  gather(C, C->vrtx, {C->u[0], C->u[0] + 2 * (size_t)ns, C->u[1], C->u[1] + 2 * (size_t)ns});
+  hipLaunchKernelGGL(swm::dev::k_inv_h<R>, dim3(nblk(ns)), dim3(BLK), 0, C->stream, V.geo, V.h[n4], V.inv_h);
+  std::vector<R*> f = {V.h[n4], V.eta[n4], V.div[n4], V.relative, V.ke, V.inv_h};
+  for (int q = 0; q < C->nt; ++q) f.push_back(V.tr[n4][q]);
+  gather_list<R>(C, C->face, f);
+  gather<R>(C, C->vrtx, {V.u[0], V.u[0] + 2 * (size_t)ns, V.u[1], V.u[1] + 2 * (size_t)ns});
 }
 
 // step.F:114-212 step_dt (FCT scheme, get_fct_flux step.F:611-678)
-swm_status step_dt(swm_ctx* C) {
+template <class R>
+swm_status step_dt(swm_ctx* C, View<R>& V) {
   const int ns = C->ns;
   const unsigned nb = nblk(ns);
   hipStream_t S = C->stream;
-  const Geo& G = C->geo;
+  const swm::dev::GeoT<R>& G = V.geo;
   const int n3 = C->n3 - 1, n4 = C->n4 - 1;
-  const double dt = C->dt;
-  double* m3 = C->msh_f[C->n3f - 1];
-  double* m2 = C->msh_f[C->n2f - 1];
-  double* m1 = C->msh_f[C->n1f - 1];
+  const R dt = (R)C->dt, w3 = (R)C->w3, w2 = (R)C->w2, w1 = (R)C->w1, grav = (R)C->grav;
+  R* m3 = V.msh_f[C->n3f - 1];
+  R* m2 = V.msh_f[C->n2f - 1];
+  R* m1 = V.msh_f[C->n1f - 1];
 
-  hipLaunchKernelGGL(swm::dev::k_avg_corners, dim3(nb), dim3(BLK), 0, S, G, C->sw, C->h[n3], C->eta[n3], C->hc, C->etac);
-  gather(C, C->vrtx, {C->hc, C->etac});
+  hipLaunchKernelGGL(swm::dev::k_avg_corners<R>, dim3(nb), dim3(BLK), 0, S, G, G.sw, V.h[n3], V.hc);
+  gather<R>(C, C->vrtx, {V.hc});
   // mass FCT (mass_fct.F:23-539)
-  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb), dim3(BLK), 0, S, G, C->u[n3], C->h[n3], C->hc, C->flx1, m3);
-  gather(C, C->edge, {C->flx1, m3});
-  hipLaunchKernelGGL(swm::dev::k_mass_low_beta, dim3(nb), dim3(BLK), 0, S, G, dt, C->w3, C->w2, C->w1, C->flx1, m3, m2,
-                     m1, C->h[n3], C->anti, C->mss_low, C->beta);
-  gather(C, C->face, {C->beta});
-  hipLaunchKernelGGL(swm::dev::k_mass_anti, dim3(nb), dim3(BLK), 0, S, G, C->beta, C->anti);
-  gather(C, C->edge, {C->anti});
-  hipLaunchKernelGGL(swm::dev::k_mass_final, dim3(nb), dim3(BLK), 0, S, G, dt, C->flx1, C->anti, C->mss_low,
-                     C->mass_flux, C->h[n4]);
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes<R>, dim3(nb), dim3(BLK), 0, S, G, V.u[n3], V.h[n3], V.hc, V.flx1, m3);
+  gather<R>(C, C->edge, {V.flx1, m3});
+  hipLaunchKernelGGL(swm::dev::k_mass_low_beta<R>, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, V.flx1, m3, m2, m1,
+                     V.h[n3], V.anti, V.mss_low, V.beta);
+  gather<R>(C, C->face, {V.beta});
+  hipLaunchKernelGGL(swm::dev::k_mass_anti<R>, dim3(nb), dim3(BLK), 0, S, G, V.beta, V.anti);
+  gather<R>(C, C->edge, {V.anti});
+  hipLaunchKernelGGL(swm::dev::k_mass_final<R>, dim3(nb), dim3(BLK), 0, S, G, dt, V.flx1, V.anti, V.mss_low,
+                     V.mass_flux, V.h[n4]);
   // the tracer limiter fills the halo of the new mass in place (tracer_fct.F:344)
-  gather(C, C->face, {C->h[n4]});
+  gather<R>(C, C->face, {V.h[n4]});
   for (int q = 0; q < C->nt; ++q) {   // tracer FCT (tracer_fct.F:22-670)
-    double* t3 = C->tr[n3][q];
-    double* t4 = C->tr[n4][q];
-    hipLaunchKernelGGL(swm::dev::k_tr_fluxes, dim3(nb), dim3(BLK), 0, S, G, C->w3, C->w2, C->w1, t3, t4, C->told[q],
-                       C->mass_flux, C->f1, C->f2);
-    gather(C, C->edge, {C->f1, C->f2});
-    hipLaunchKernelGGL(swm::dev::k_tr_low, dim3(nb), dim3(BLK), 0, S, G, dt, C->h[n3], t3, C->f1, C->ml);
-    gather(C, C->face, {C->ml});
-    hipLaunchKernelGGL(swm::dev::k_tr_limiter, dim3(nb), dim3(BLK), 0, S, G, dt, t3, C->ml, C->h[n4], C->mss_low,
-                       C->f1, C->f2, C->anti, C->bdn, C->bup);
-    gather(C, C->face, {C->bdn, C->bup});
-    hipLaunchKernelGGL(swm::dev::k_tr_anti, dim3(nb), dim3(BLK), 0, S, G, C->bdn, C->bup, C->anti);
-    gather(C, C->edge, {C->anti});
-    hipLaunchKernelGGL(swm::dev::k_tr_final, dim3(nb), dim3(BLK), 0, S, G, dt, C->ml, C->anti, C->h[n4], t4, C->told[q]);
+    R* t3 = V.tr[n3][q];
+    R* t4 = V.tr[n4][q];
+    hipLaunchKernelGGL(swm::dev::k_tr_fluxes<R>, dim3(nb), dim3(BLK), 0, S, G, w3, w2, w1, t3, t4, V.told[q],
+                       V.mass_flux, V.f1, V.f2);
+    gather<R>(C, C->edge, {V.f1, V.f2});
+    hipLaunchKernelGGL(swm::dev::k_tr_low<R>, dim3(nb), dim3(BLK), 0, S, G, dt, V.h[n3], t3, V.f1, V.ml);
+    gather<R>(C, C->face, {V.ml});
+    hipLaunchKernelGGL(swm::dev::k_tr_limiter<R>, dim3(nb), dim3(BLK), 0, S, G, dt, t3, V.ml, V.h[n4], V.mss_low,
+                       V.f1, V.f2, V.anti, V.bdn, V.bup);
+    gather<R>(C, C->face, {V.bdn, V.bup});
+    hipLaunchKernelGGL(swm::dev::k_tr_anti<R>, dim3(nb), dim3(BLK), 0, S, G, V.bdn, V.bup, V.anti);
+    gather<R>(C, C->edge, {V.anti});
+    hipLaunchKernelGGL(swm::dev::k_tr_final<R>, dim3(nb), dim3(BLK), 0, S, G, dt, V.ml, V.anti, V.h[n4], t4,
+                       V.told[q]);
   }
   // vorticity and divergence (step.F:143-163, :336-429)
-  hipLaunchKernelGGL(swm::dev::k_vort_curl, dim3(nb), dim3(BLK), 0, S, G, C->mass_flux, C->h[n3], C->eta[n3], C->hc,
-                     C->u[n3], C->vf, C->dcurl);
-  gather(C, C->edge, {C->vf, C->dcurl});
-  hipLaunchKernelGGL(swm::dev::k_tendencies, dim3(nb), dim3(BLK), 0, S, G, dt, C->grav, C->w3, C->w2, C->w1, C->vf,
-                     C->dcurl, C->ke, C->h[n3], C->eta[n3], C->div[n3], C->eta_f[C->n3f - 1], C->eta_f[C->n2f - 1],
-                     C->eta_f[C->n1f - 1], C->div_f[C->n3f - 1], C->div_f[C->n2f - 1], C->div_f[C->n1f - 1],
-                     C->eta[n4], C->div[n4], C->relative, C->rhs, C->rhs + ns);
+  hipLaunchKernelGGL(swm::dev::k_vort_curl<R>, dim3(nb), dim3(BLK), 0, S, G, V.mass_flux, V.h[n3], V.eta[n3], V.hc,
+                     V.u[n3], V.vf, V.dcurl);
+  gather<R>(C, C->edge, {V.vf, V.dcurl});
+  hipLaunchKernelGGL(swm::dev::k_tendencies<R>, dim3(nb), dim3(BLK), 0, S, G, dt, grav, w3, w2, w1, V.vf, V.dcurl,
+                     V.ke, V.h[n3], V.eta[n3], V.div[n3], V.eta_f[C->n3f - 1], V.eta_f[C->n2f - 1],
+                     V.eta_f[C->n1f - 1], V.div_f[C->n3f - 1], V.div_f[C->n2f - 1], V.div_f[C->n1f - 1], V.eta[n4],
+                     V.div[n4], V.relative, V.rhs, V.rhs + ns);
   // streamfunction and velocity potential (step.F:170-184)
-  swm_status st = elliptic(C, C->rhs, C->xout, 2);
+  swm_status st;
+  if constexpr (sizeof(R) == 8) st = elliptic(C, V.rhs, C->xout, 2);
+  else st = elliptic_mixed(C, V.rhs, C->xout);
   if (st) return st;
-  hipLaunchKernelGGL(swm::dev::k_post_elliptic, dim3(nb), dim3(BLK), 0, S, G, C->rhs, C->eta[n4]);
-  hipLaunchKernelGGL(swm::dev::k_velocity, dim3(nb), dim3(BLK), 0, S, G, C->xout, C->xout + ns, C->u[n4]);
-  gather(C, C->vrtx, {C->u[n4], C->u[n4] + 2 * (size_t)ns});
-  hipLaunchKernelGGL(swm::dev::k_center_ke, dim3(nb), dim3(BLK), 0, S, G, C->u[n4], C->ke);
-  gather(C, C->face, {C->ke});
+  hipLaunchKernelGGL(swm::dev::k_post_elliptic<R>, dim3(nb), dim3(BLK), 0, S, G, V.rhs, V.eta[n4]);
+  hipLaunchKernelGGL(swm::dev::k_velocity<R>, dim3(nb), dim3(BLK), 0, S, G, C->xout, C->xout + ns, V.u[n4]);
+  gather<R>(C, C->vrtx, {V.u[n4], V.u[n4] + 2 * (size_t)ns});
+  hipLaunchKernelGGL(swm::dev::k_center_ke<R>, dim3(nb), dim3(BLK), 0, S, G, V.u[n4], V.ke);
+  gather<R>(C, C->face, {V.ke});
   return SWM_OK;
 }
 
@@ -395,37 +550,69 @@ struct SplitMix {
 
 // zero the whole state (the reference's module arrays start at zero)
 void reset_state(swm_ctx* C) {
-  fill(C, C->h[0], (C->arena.base + C->arena.used - reinterpret_cast<char*>(C->h[0])) / 8, 0.0);
-  fill(C, C->hs, C->ns, 0.0);
+  fill<double>(C, C->d.state_begin, C->d.state_words, 0.0);
+  if (C->precision == SWM_MIXED) fill<float>(C, C->m.state_begin, C->m.state_words, 0.0f);
+  fill<double>(C, C->hs, C->ns, 0.0);
   C->step_count = 0;
   C->initialized = false;
   pointers(C);
 }
 
+// FP64 field -> FP32 storage of the mixed path (y = x - off - c), and back
+void to32(swm_ctx* C, const double* x, float* y, size_t n, const double* off = nullptr, double c = 0.0) {
+  hipLaunchKernelGGL(swm::dev::k_to_f32, dim3(nblk(n)), dim3(BLK), 0, C->stream, n, x, off, c, y);
+}
+void to64(swm_ctx* C, const float* x, double* y, size_t n, const double* off = nullptr, double c = 0.0) {
+  hipLaunchKernelGGL(swm::dev::k_to_f64, dim3(nblk(n)), dim3(BLK), 0, C->stream, n, x, off, c, y);
+}
+
 // h, hs, u (planes) and tracers of time level 1 are on the device: derive the
 // rest as the reference initial conditions do (initialize.F:418-468 and
-// operators_hl.F:153-266), then fill the halos as wrap_prognostics
+// operators_hl.F:153-266), then fill the halos as wrap_prognostics.  The
+// mixed path starts from this FP64 state: h less its area-weighted mean h0,
+// eta less f, the rest as is.
 swm_status finish_init(swm_ctx* C) {
   const int ns = C->ns;
   hipStream_t S = C->stream;
   const unsigned nb = nblk(ns);
-  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, C->h[1], C->h[0], (size_t)ns);
-  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nblk(4 * (size_t)ns)), dim3(BLK), 0, S, C->u[1], C->u[0], 4 * (size_t)ns);
+  View<double>& D = C->d;
+  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, D.h[1], D.h[0], (size_t)ns);
+  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nblk(4 * (size_t)ns)), dim3(BLK), 0, S, D.u[1], D.u[0], 4 * (size_t)ns);
   for (int t = 0; t < 2; ++t) {
-    gather(C, C->face, {C->h[t]});
-    gather(C, C->vrtx, {C->u[t], C->u[t] + 2 * (size_t)ns});
+    gather<double>(C, C->face, {D.h[t]});
+    gather<double>(C, C->vrtx, {D.u[t], D.u[t] + 2 * (size_t)ns});
   }
-  gather(C, C->face, {C->hs});
-  gather(C, C->vrtx, {C->u[0], C->u[0] + 2 * (size_t)ns});
-  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb), dim3(BLK), 0, S, C->geo, C->u[0], C->relative, C->div[0]);
-  hipLaunchKernelGGL(swm::dev::k_center_ke, dim3(nb), dim3(BLK), 0, S, C->geo, C->u[0], C->ke);
-  gather(C, C->face, {C->relative, C->div[0], C->ke});
-  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, C->div[1], C->div[0], (size_t)ns);
-  hipLaunchKernelGGL(swm::dev::k_post_elliptic, dim3(nb), dim3(BLK), 0, S, C->geo, C->relative, C->eta[0]);
-  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, C->eta[1], C->eta[0], (size_t)ns);
+  gather<double>(C, C->face, {C->hs});
+  gather<double>(C, C->vrtx, {D.u[0], D.u[0] + 2 * (size_t)ns});
+  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb), dim3(BLK), 0, S, D.geo, D.u[0], D.relative, D.div[0]);
+  hipLaunchKernelGGL(swm::dev::k_center_ke<double>, dim3(nb), dim3(BLK), 0, S, D.geo, D.u[0], D.ke);
+  gather<double>(C, C->face, {D.relative, D.div[0], D.ke});
+  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, D.div[1], D.div[0], (size_t)ns);
+  hipLaunchKernelGGL(swm::dev::k_post_elliptic<double>, dim3(nb), dim3(BLK), 0, S, D.geo, D.relative, D.eta[0]);
+  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, D.eta[1], D.eta[0], (size_t)ns);
   for (int q = 0; q < C->nt; ++q)
-    hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, C->tr[1][q], C->tr[0][q], (size_t)ns);
-  wrap_prognostics(C);
+    hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, D.tr[1][q], D.tr[0][q], (size_t)ns);
+  wrap_prognostics<double>(C, D);
+  if (C->precision == SWM_MIXED) {
+    View<float>& M = C->m;
+    sum2<double>(C, D.h[0], D.h[0], C->area_mask, ns);
+    double sh[2];
+    swm_status st = read_sums(C, 2, sh);
+    if (st) return st;
+    C->h0 = sh[0] / C->total_area;
+    M.geo.h0 = (float)C->h0;
+    to32(C, C->hs, const_cast<float*>(M.geo.hs), ns);
+    for (int t = 0; t < 2; ++t) {
+      to32(C, D.h[t], M.h[t], ns, nullptr, C->h0);
+      to32(C, D.eta[t], M.eta[t], ns, C->f);
+      to32(C, D.div[t], M.div[t], ns);
+      to32(C, D.u[t], M.u[t], 4 * (size_t)ns);
+      for (int q = 0; q < C->nt; ++q) to32(C, D.tr[t][q], M.tr[t][q], ns);
+    }
+    to32(C, D.relative, M.relative, ns);
+    to32(C, D.ke, M.ke, ns);
+    to32(C, D.inv_h, M.inv_h, ns);
+  }
   HIP_TRY(hipStreamSynchronize(S));
   HIP_TRY(hipGetLastError());
   C->initialized = true;
@@ -479,7 +666,7 @@ struct OpIO {
   }
   double* zeros(int nplanes) {
     double* d = dev(nplanes);
-    if (d) fill(C, d, (size_t)nplanes * ns, 0.0);
+    if (d) fill<double>(C, d, (size_t)nplanes * ns, 0.0);
     return d;
   }
   swm_status get(double* ref, const double* d, int lead, const std::vector<int>& perm = {}) {
@@ -511,15 +698,180 @@ void planes_to_vars(const double* planes, int nvar, int pl, double* ref) {
 }
 }  // namespace
 
+namespace swm {
+
+// buffers of a view in the arena: multigrid levels (the halo maps are shared
+// with the FP64 view when given), state, scratch
+template <class R>
+void layout_view(swm_ctx* C, View<R>& V, const View<double>* maps) {
+  Arena& A = C->arena;
+  const size_t ns = C->ns;
+  V.lev.resize(C->g);
+  for (int l = 1; l <= C->g; ++l) {
+    LevelT<R>& L = V.lev[l - 1];
+    L.iim = L.jjm = 2 + (1 << l);
+    L.ns = L.iim * L.jjm * 10;
+    const size_t nl = L.ns;
+    L.lw = A.take<R>(7 * nl); L.lwt = A.take<R>(nl);
+    L.alpha = A.take<R>(2 * nl); L.beta = A.take<R>(2 * nl); L.temp = A.take<R>(2 * nl);
+    L.src = maps ? maps->lev[l - 1].src : A.take<int>(nl);
+    L.L = swm::dev::LevT<R>{L.iim, L.jjm, L.ns, L.lw, L.lwt, L.src};
+  }
+  auto take = [&](size_t n) { return A.take<R>(n); };
+  const size_t before = A.used;
+  for (int t = 0; t < 2; ++t) {
+    V.h[t] = take(ns); V.eta[t] = take(ns); V.div[t] = take(ns); V.u[t] = take(4 * ns);
+    V.tr[t].clear();
+    for (int q = 0; q < C->nt; ++q) V.tr[t].push_back(take(ns));
+  }
+  V.told.clear();
+  for (int q = 0; q < C->nt; ++q) V.told.push_back(take(ns));
+  for (int t = 0; t < 3; ++t) { V.eta_f[t] = take(ns); V.div_f[t] = take(ns); V.msh_f[t] = take(3 * ns); }
+  V.relative = take(ns); V.ke = take(ns); V.inv_h = take(ns); V.mass_flux = take(3 * ns);
+  V.hc = take(2 * ns); V.flx1 = take(3 * ns); V.anti = take(3 * ns); V.mss_low = take(ns);
+  V.beta = take(ns); V.f1 = take(3 * ns); V.f2 = take(3 * ns); V.ml = take(ns); V.bdn = take(ns); V.bup = take(ns);
+  V.vf = take(3 * ns); V.dcurl = take(3 * ns); V.rhs = take(2 * ns);
+  V.state_begin = V.h[0];
+  V.state_words = (A.used - before) / sizeof(R);
+}
+
+// FP64 metric planes (reference layout -> planes), tables, levels and views
+void layout_all(swm_ctx* C, const swm::Grid& Gh) {
+  Arena& A = C->arena;
+  const size_t ns = C->ns;
+  auto take = [&](size_t n) { return A.take<double>(n); };
+  C->area = take(ns); C->area_inv = take(ns); C->cai = take(2 * ns); C->sub_area = take(6 * ns); C->dc = take(6 * ns);
+  C->nrm = take(12 * ns); C->tng = take(12 * ns); C->vw = take(12 * ns); C->sw = take(6 * ns); C->lw = take(7 * ns);
+  C->f = take(ns); C->hs = take(ns); C->mask = take(ns); C->area_mask = take(ns); C->pxyz = take(3 * ns);
+  make_table(C, Gh.face, C->ns, C->face);
+  make_table(C, Gh.vrtx, C->ns, C->vrtx);
+  make_table(C, Gh.edge, C->ns, C->edge);
+  C->xout = take(2 * ns); C->r2 = take(ns);
+  C->part = take(2 * (size_t)C->npart); C->sums = take(64);
+  layout_view<double>(C, C->d, nullptr);
+  C->d.geo = swm::dev::Geo{C->iim, C->jjm, C->ns, C->area, C->area_inv, C->cai, C->sub_area, C->dc, C->nrm, C->tng,
+                           C->vw, C->sw, C->lw, C->f, C->hs, C->mask, 0.0, nullptr};
+  if (C->precision == SWM_MIXED) {
+    // FP32 copies of the metric planes, same order as the FP64 ones
+    auto tf = [&](size_t n) { return A.take<float>(n); };
+    swm::dev::GeoT<float>& g = C->m.geo;
+    float *area = tf(ns), *ai = tf(ns), *cai = tf(2 * ns), *sa = tf(6 * ns), *dc = tf(6 * ns), *nrm = tf(12 * ns),
+          *tng = tf(12 * ns), *vw = tf(12 * ns), *sw = tf(6 * ns), *lw = tf(7 * ns), *f = tf(ns), *hs = tf(ns),
+          *mask = tf(ns);
+    g = swm::dev::GeoT<float>{C->iim, C->jjm, C->ns, area, ai, cai, sa, dc, nrm, tng, vw, sw, lw, f, hs, mask, 0.0f, f};
+    layout_view<float>(C, C->m, &C->d);
+    C->mx_r = tf(2 * ns);
+    C->mx_e = tf(2 * ns);
+  }
+}
+
+swm_status up64(double* d, const std::vector<double>& h) {
+  HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  return SWM_OK;
+}
+
+swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
+  HIP_TRY(hipSetDevice(C->device));
+  HIP_TRY(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&C->ev_a));
+  HIP_TRY(hipEventCreate(&C->ev_b));
+  // sizing pass, then the arena
+  C->arena = Arena{};
+  layout_all(C, Gh);
+  C->arena_bytes = C->arena.used + 4096;
+  C->arena = Arena{};
+  HIP_TRY(hipMalloc(&C->arena.base, C->arena_bytes));
+  C->arena.cap = C->arena_bytes;
+  C->face = C->vrtx = C->edge = DevTable{};
+  layout_all(C, Gh);
+  if (C->arena.used > C->arena.cap) {
+    swm::set_error("swm_create: arena overflow");
+    return SWM_ERR_HIP;
+  }
+  HIP_TRY(hipHostMalloc(&C->host_sums, 64 * sizeof(double), hipHostMallocDefault));
+  const int ns = C->ns;
+  auto id = [](int n) { std::vector<int> v(n); for (int k = 0; k < n; ++k) v[k] = k; return v; };
+  swm_status st;
+  if ((st = up64(C->pxyz, to_planes(Gh.p, 3, ns, id(3), 3)))) return st;
+  if ((st = up64(C->area, Gh.area))) return st;
+  if ((st = up64(C->area_inv, Gh.area_inv))) return st;
+  if ((st = up64(C->cai, to_planes(Gh.corner_area_inv, 2, ns, id(2), 2)))) return st;
+  if ((st = up64(C->sub_area, to_planes(Gh.sub_area, 6, ns, id(6), 6)))) return st;
+  if ((st = up64(C->dc, to_planes(Gh.dc, 6, ns, id(6), 6)))) return st;
+  if ((st = up64(C->sw, to_planes(Gh.s_weights, 6, ns, id(6), 6)))) return st;
+  if ((st = up64(C->vw, to_planes(Gh.v_weights, 12, ns, id(12), 12)))) return st;
+  if ((st = up64(C->lw, to_planes(Gh.l_weights, 7, ns, id(7), 7)))) return st;
+  {
+    // normal/tangent (kk, a, ic) with a = 0..2: keep a = 0, 1
+    std::vector<int> perm(18, -1);
+    for (int ic = 0; ic < 2; ++ic)
+      for (int a = 0; a < 2; ++a)
+        for (int kk = 0; kk < 3; ++kk) perm[kk + 3 * (a + 3 * ic)] = kk + 3 * (a + 2 * ic);
+    if ((st = up64(C->nrm, to_planes(Gh.normal, 18, ns, perm, 12)))) return st;
+    if ((st = up64(C->tng, to_planes(Gh.tangent, 18, ns, perm, 12)))) return st;
+  }
+  if ((st = up64(C->f, Gh.f))) return st;
+  if ((st = up64(C->hs, std::vector<double>(ns, 0.0)))) return st;
+  if ((st = up64(C->mask, Gh.center_mask))) return st;
+  {
+    std::vector<double> am(ns);
+    double tot = 0.0;
+    for (int n = 0; n < ns; ++n) { am[n] = Gh.area[n] * Gh.center_mask[n]; tot = tot + am[n] * 1.0; }
+    C->total_area = tot;
+    if ((st = up64(C->area_mask, am))) return st;
+  }
+  // multigrid levels
+  for (int l = 1; l <= C->g; ++l) {
+    LevelT<double>& L = C->d.lev[l - 1];
+    const size_t nl = L.ns;
+    const std::vector<double>& lw = Gh.levels[l - 1].lap_w;
+    if ((st = up64(L.lw, to_planes(lw, 7, (int)nl, id(7), 7)))) return st;
+    std::vector<double> lwt(nl);
+    for (size_t n = 0; n < nl; ++n) lwt[n] = lw[6 + 7 * n] != 0.0 ? 1.0 / (lw[6 + 7 * n] - 0.0) : 0.0;
+    if ((st = up64(L.lwt, lwt))) return st;
+    swm::HaloTable t = swm::build_simple_face_table(l);
+    std::vector<int> src(nl);
+    for (size_t n = 0; n < nl; ++n) src[n] = (int)n;
+    for (size_t k = 0; k < t.dst.size(); ++k) src[t.dst[k]] = t.src[k];
+    HIP_TRY(hipMemcpy(L.src, src.data(), nl * sizeof(int), hipMemcpyHostToDevice));
+  }
+  const unsigned nb = nblk(ns);
+  hipStream_t S = C->stream;
+  if (C->precision == SWM_MIXED) {
+    // FP32 metrics, level weights and reciprocal centre weights
+    const swm::dev::GeoT<float>& g = C->m.geo;
+    auto cv = [&](const double* x, const float* y, size_t n) {
+      hipLaunchKernelGGL(swm::dev::k_to_f32, dim3(nblk(n)), dim3(BLK), 0, S, n, x, (const double*)nullptr, 0.0,
+                         const_cast<float*>(y));
+    };
+    cv(C->area, g.area, ns); cv(C->area_inv, g.area_inv, ns); cv(C->cai, g.cai, 2 * (size_t)ns);
+    cv(C->sub_area, g.sub_area, 6 * (size_t)ns); cv(C->dc, g.dc, 6 * (size_t)ns); cv(C->nrm, g.nrm, 12 * (size_t)ns);
+    cv(C->tng, g.tng, 12 * (size_t)ns); cv(C->vw, g.vw, 12 * (size_t)ns); cv(C->sw, g.sw, 6 * (size_t)ns);
+    cv(C->lw, g.lw, 7 * (size_t)ns); cv(C->f, g.f, ns); cv(C->mask, g.mask, ns);
+    for (int l = 1; l <= C->g; ++l) {
+      const LevelT<double>& L = C->d.lev[l - 1];
+      cv(L.lw, C->m.lev[l - 1].lw, 7 * (size_t)L.ns);
+      cv(L.lwt, C->m.lev[l - 1].lwt, (size_t)L.ns);
+    }
+    (void)nb;
+  }
+  // zero every state buffer (the reference initialises its module arrays to 0)
+  fill<double>(C, C->d.state_begin, C->d.state_words, 0.0);
+  if (C->precision == SWM_MIXED) fill<float>(C, C->m.state_begin, C->m.state_words, 0.0f);
+  fill<double>(C, C->xout, 2 * (size_t)ns, 0.0);
+  HIP_TRY(hipStreamSynchronize(S));
+  HIP_TRY(hipGetLastError());
+  return SWM_OK;
+}
+
+}  // namespace swm
+
 extern "C" {
 
 swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out) {
-  if (!grid || !cfg || !out || cfg->ntracer < 0 || cfg->ntracer > 64 || !(cfg->dt > 0.0)) {
+  if (!grid || !cfg || !out || cfg->ntracer < 0 || cfg->ntracer > 64 || !(cfg->dt > 0.0) ||
+      (cfg->precision != SWM_FP64 && cfg->precision != SWM_MIXED)) {
     swm::set_error("swm_create: bad argument");
-    return SWM_ERR_ARG;
-  }
-  if (cfg->precision != SWM_FP64) {
-    swm::set_error("swm_create: only SWM_FP64 is available in this build");
     return SWM_ERR_ARG;
   }
   int ndev = 0;
@@ -532,124 +884,13 @@ swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out
   C->g = Gh.g; C->iim = Gh.iim; C->jjm = Gh.jjm; C->ns = Gh.iim * Gh.jjm * 10; C->nt = cfg->ntracer;
   C->max_ig = Gh.max_ig;
   C->dt = cfg->dt; C->grav = cfg->grav; C->precision = cfg->precision; C->device = cfg->device;
-  HIP_TRY(hipSetDevice(C->device));
-  HIP_TRY(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
-  HIP_TRY(hipEventCreate(&C->ev_a));
-  HIP_TRY(hipEventCreate(&C->ev_b));
-  const size_t ns = C->ns;
-  // size of the arena
-  size_t words = 0;
-  auto W = [&](size_t n) { words += ((n * 8 + 255) / 256) * 32; };
-  const int planes_metrics = 1 + 1 + 2 + 6 + 6 + 12 + 12 + 12 + 6 + 7 + 1 + 1 + 1 + 1;
-  W(ns); W(ns); W(2 * ns); W(6 * ns); W(6 * ns); W(12 * ns); W(12 * ns); W(12 * ns); W(6 * ns); W(7 * ns);
-  W(ns); W(ns); W(ns); W(ns); W(3 * ns);
-  (void)planes_metrics;
-  for (int l = 1; l <= C->g; ++l) {
-    const size_t nl = (size_t)(2 + (1 << l)) * (2 + (1 << l)) * 10;
-    W(7 * nl); W(nl); W(2 * nl); W(2 * nl); W(2 * nl); words += ((nl * 4 + 255) / 256) * 32;
-  }
-  const size_t state = 2 * (3 + 4) + 2 * C->nt + C->nt + 3 + 3 + 9 + 1 + 1 + 2 + 1 + 3;
-  for (size_t k = 0; k < state; ++k) W(ns);
-  const size_t scratch = 2 + 2 + 3 + 3 + 1 + 1 + 3 + 3 + 1 + 1 + 1 + 3 + 3 + 2 + 1;
-  for (size_t k = 0; k < scratch; ++k) W(ns);
   C->npart = 1024;
-  W(2 * C->npart); W(64);
-  const size_t tabs = Gh.face.dst.size() + Gh.vrtx.dst.size() + Gh.edge.dst.size() + 16;
-  words += (2 * tabs * 4 + 6 * 256) / 8 + 64;
-  C->arena_bytes = words * 8 + (1 << 20);
-  HIP_TRY(hipMalloc(&C->arena.base, C->arena_bytes));
-  C->arena.cap = C->arena_bytes;
-  HIP_TRY(hipHostMalloc(&C->host_sums, 64 * sizeof(double), hipHostMallocDefault));
-  Arena& A = C->arena;
-  auto take = [&](size_t n) { return A.take<double>(n); };
-
-  // metrics as planes
-  auto id = [](int n) { std::vector<int> v(n); for (int k = 0; k < n; ++k) v[k] = k; return v; };
-  swm_status st;
-  std::vector<std::vector<double>> hold;   // keep host buffers alive until the copies finish
-  auto up = [&](double* d, std::vector<double> v) {
-    hold.push_back(std::move(v));
-    return upload(d, hold.back(), C->stream);
-  };
-  C->area = take(ns); C->area_inv = take(ns); C->cai = take(2 * ns); C->sub_area = take(6 * ns); C->dc = take(6 * ns);
-  C->nrm = take(12 * ns); C->tng = take(12 * ns); C->vw = take(12 * ns); C->sw = take(6 * ns); C->lw = take(7 * ns);
-  C->f = take(ns); C->hs = take(ns); C->mask = take(ns); C->area_mask = take(ns); C->pxyz = take(3 * ns);
   C->radius = Gh.radius;
-  if ((st = up(C->pxyz, to_planes(Gh.p, 3, ns, id(3), 3)))) return st;
-  if ((st = upload(C->area, Gh.area, C->stream))) return st;
-  if ((st = upload(C->area_inv, Gh.area_inv, C->stream))) return st;
-  if ((st = up(C->cai, to_planes(Gh.corner_area_inv, 2, ns, id(2), 2)))) return st;
-  if ((st = up(C->sub_area, to_planes(Gh.sub_area, 6, ns, id(6), 6)))) return st;
-  if ((st = up(C->dc, to_planes(Gh.dc, 6, ns, id(6), 6)))) return st;
-  if ((st = up(C->sw, to_planes(Gh.s_weights, 6, ns, id(6), 6)))) return st;
-  if ((st = up(C->vw, to_planes(Gh.v_weights, 12, ns, id(12), 12)))) return st;
-  if ((st = up(C->lw, to_planes(Gh.l_weights, 7, ns, id(7), 7)))) return st;
-  {
-    // normal/tangent (kk, a, ic) with a = 0..2: keep a = 0, 1
-    std::vector<int> perm(18, -1);
-    for (int ic = 0; ic < 2; ++ic)
-      for (int a = 0; a < 2; ++a)
-        for (int kk = 0; kk < 3; ++kk) perm[kk + 3 * (a + 3 * ic)] = kk + 3 * (a + 2 * ic);
-    if ((st = up(C->nrm, to_planes(Gh.normal, 18, ns, perm, 12)))) return st;
-    if ((st = up(C->tng, to_planes(Gh.tangent, 18, ns, perm, 12)))) return st;
+  swm_status st = swm::create_device(C, Gh);
+  if (st) {
+    swm_destroy(C);
+    return st;
   }
-  if ((st = upload(C->f, Gh.f, C->stream))) return st;
-  HIP_TRY(hipMemsetAsync(C->hs, 0, ns * sizeof(double), C->stream));
-  if ((st = upload(C->mask, Gh.center_mask, C->stream))) return st;
-  {
-    std::vector<double> am(ns);
-    double tot = 0.0;
-    for (size_t n = 0; n < ns; ++n) { am[n] = Gh.area[n] * Gh.center_mask[n]; tot = tot + am[n] * 1.0; }
-    C->total_area = tot;
-    if ((st = up(C->area_mask, am))) return st;
-  }
-  C->geo = Geo{C->iim, C->jjm, C->ns, C->area, C->area_inv, C->cai, C->sub_area, C->dc, C->nrm, C->tng,
-               C->vw, C->sw, C->lw, C->f, C->hs, C->mask};
-  if ((st = make_table(C, Gh.face, C->ns, C->face))) return st;
-  if ((st = make_table(C, Gh.vrtx, C->ns, C->vrtx))) return st;
-  if ((st = make_table(C, Gh.edge, C->ns, C->edge))) return st;
-  // multigrid levels
-  C->lev.resize(C->g);
-  std::vector<std::vector<int>> hold_i;
-  for (int l = 1; l <= C->g; ++l) {
-    swm_ctx::Level& L = C->lev[l - 1];
-    L.iim = L.jjm = 2 + (1 << l);
-    L.ns = L.iim * L.jjm * 10;
-    const size_t nl = L.ns;
-    L.lw = take(7 * nl); L.lwt = take(nl); L.alpha = take(2 * nl); L.beta = take(2 * nl); L.temp = take(2 * nl);
-    L.src = A.take<int>(nl);
-    const std::vector<double>& lw = Gh.levels[l - 1].lap_w;
-    if ((st = up(L.lw, to_planes(lw, 7, (int)nl, id(7), 7)))) return st;
-    std::vector<double> lwt(nl);
-    for (size_t n = 0; n < nl; ++n) lwt[n] = lw[6 + 7 * n] != 0.0 ? 1.0 / (lw[6 + 7 * n] - 0.0) : 0.0;
-    if ((st = up(L.lwt, lwt))) return st;
-    swm::HaloTable t = swm::build_simple_face_table(l);
-    std::vector<int> src(nl);
-    for (size_t n = 0; n < nl; ++n) src[n] = (int)n;
-    for (size_t k = 0; k < t.dst.size(); ++k) src[t.dst[k]] = t.src[k];
-    hold_i.push_back(std::move(src));
-    HIP_TRY(hipMemcpyAsync(L.src, hold_i.back().data(), nl * sizeof(int), hipMemcpyHostToDevice, C->stream));
-    L.L = Lev{L.iim, L.jjm, L.ns, L.lw, L.lwt, L.src};
-  }
-  // state and scratch
-  for (int t = 0; t < 2; ++t) {
-    C->h[t] = take(ns); C->eta[t] = take(ns); C->div[t] = take(ns); C->u[t] = take(4 * ns);
-    for (int q = 0; q < C->nt; ++q) C->tr[t].push_back(take(ns));
-  }
-  for (int q = 0; q < C->nt; ++q) C->told.push_back(take(ns));
-  for (int t = 0; t < 3; ++t) { C->eta_f[t] = take(ns); C->div_f[t] = take(ns); C->msh_f[t] = take(3 * ns); }
-  C->relative = take(ns); C->ke = take(ns); C->xout = take(2 * ns); C->inv_h = take(ns); C->mass_flux = take(3 * ns);
-  C->hc = take(2 * ns); C->etac = take(2 * ns); C->flx1 = take(3 * ns); C->anti = take(3 * ns); C->mss_low = take(ns);
-  C->beta = take(ns); C->f1 = take(3 * ns); C->f2 = take(3 * ns); C->ml = take(ns); C->bdn = take(ns); C->bup = take(ns);
-  C->vf = take(3 * ns); C->dcurl = take(3 * ns); C->rhs = take(2 * ns); C->r2 = take(ns);
-  C->part = take(2 * (size_t)C->npart); C->sums = take(64);
-  if (A.used > A.cap) {
-    swm::set_error("swm_create: arena overflow");
-    return SWM_ERR_HIP;
-  }
-  // zero every state buffer (the reference initialises its module arrays to 0)
-  fill(C, reinterpret_cast<double*>(C->h[0]), (A.base + A.used - reinterpret_cast<char*>(C->h[0])) / 8, 0.0);
-  HIP_TRY(hipStreamSynchronize(C->stream));
   *out = C;
   return SWM_OK;
 }
@@ -680,11 +921,11 @@ swm_status swm_init_state(swm_ctx* C, const double* h, const double* hs, const d
       std::memcpy(&tq[(size_t)q * ns + (size_t)k * C->iim * C->jjm],
                   tracer + ((size_t)q + (size_t)C->nt * k) * C->iim * C->jjm, (size_t)C->iim * C->jjm * sizeof(double));
   reset_state(C);
-  HIP_TRY(hipMemcpyAsync(C->h[0], h, ns * sizeof(double), hipMemcpyHostToDevice, S));
-  HIP_TRY(hipMemcpyAsync(C->u[0], up.data(), 4 * (size_t)ns * sizeof(double), hipMemcpyHostToDevice, S));
+  HIP_TRY(hipMemcpyAsync(C->d.h[0], h, ns * sizeof(double), hipMemcpyHostToDevice, S));
+  HIP_TRY(hipMemcpyAsync(C->d.u[0], up.data(), 4 * (size_t)ns * sizeof(double), hipMemcpyHostToDevice, S));
   if (hs) HIP_TRY(hipMemcpyAsync(C->hs, hs, ns * sizeof(double), hipMemcpyHostToDevice, S));
   for (int q = 0; q < C->nt; ++q)
-    HIP_TRY(hipMemcpyAsync(C->tr[0][q], &tq[(size_t)q * ns], ns * sizeof(double), hipMemcpyHostToDevice, S));
+    HIP_TRY(hipMemcpyAsync(C->d.tr[0][q], &tq[(size_t)q * ns], ns * sizeof(double), hipMemcpyHostToDevice, S));
   HIP_TRY(hipStreamSynchronize(S));
   return finish_init(C);
 }
@@ -712,10 +953,10 @@ swm_status swm_init_synthetic(swm_ctx* C, int seed_index, int lmax, double umax,
   }
   reset_state(C);
   // scratch: psi -> xout, zeta -> f1, chi -> xout+ns, delta -> f2, hp -> ml
-  double* dcoef = C->flx1;   // 3*ns >= 3*nlm for every supported level
+  double* dcoef = C->d.flx1;   // 3*ns >= 3*nlm for every supported level
   if ((size_t)3 * nlm > (size_t)3 * ns) { swm::set_error("swm_init_synthetic: lmax too large for the grid"); return SWM_ERR_ARG; }
   HIP_TRY(hipMemcpyAsync(dcoef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice, S));
-  double *psi = C->xout, *chi = C->xout + ns, *zeta = C->f1, *delta = C->f2, *hp = C->ml;
+  double *psi = C->xout, *chi = C->xout + ns, *zeta = C->d.f1, *delta = C->d.f2, *hp = C->d.ml;
   hipLaunchKernelGGL(swm::dev::k_harmonics, dim3(nb), dim3(BLK), 0, S, ns, C->pxyz, lmax, dcoef, psi, zeta, chi,
                      delta, hp);
   const double a2 = C->radius * C->radius;
@@ -734,16 +975,16 @@ swm_status swm_init_synthetic(swm_ctx* C, int seed_index, int lmax, double umax,
   if ((st = sumsq(zeta, &sz)) || (st = sumsq(delta, &sd)) || (st = sumsq(hp, &sh))) return st;
   const double nreal = (double)C->max_ig;
   const double rz = std::sqrt(sz / nreal), rd = std::sqrt(sd / nreal), rh = std::sqrt(sh / nreal);
-  hipLaunchKernelGGL(swm::dev::k_synth_finish, dim3(nb), dim3(BLK), 0, S, C->geo, h0, h_rms / rh, hp, C->h[0],
+  hipLaunchKernelGGL(swm::dev::k_synth_finish, dim3(nb), dim3(BLK), 0, S, C->d.geo, h0, h_rms / rh, hp, C->d.h[0],
                      0.1 * rz / rd, chi);
-  hipLaunchKernelGGL(swm::dev::k_velocity, dim3(nb), dim3(BLK), 0, S, C->geo, psi, chi, C->u[0]);
-  hipLaunchKernelGGL(swm::dev::k_speedmax_partial, dim3(C->npart), dim3(BLK), 0, S, C->geo, C->u[0], C->part);
+  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, C->d.geo, psi, chi, C->d.u[0]);
+  hipLaunchKernelGGL(swm::dev::k_speedmax_partial, dim3(C->npart), dim3(BLK), 0, S, C->d.geo, C->d.u[0], C->part);
   std::vector<double> pm(C->npart);
   HIP_TRY(hipMemcpyAsync(pm.data(), C->part, C->npart * sizeof(double), hipMemcpyDeviceToHost, S));
   HIP_TRY(hipStreamSynchronize(S));
   double vmax = 0.0;
   for (double v : pm) vmax = std::max(vmax, v);
-  hipLaunchKernelGGL(swm::dev::k_scale, dim3(nblk(4 * (size_t)ns)), dim3(BLK), 0, S, C->u[0], 4 * (size_t)ns, umax / vmax);
+  hipLaunchKernelGGL(swm::dev::k_scale, dim3(nblk(4 * (size_t)ns)), dim3(BLK), 0, S, C->d.u[0], 4 * (size_t)ns, umax / vmax);
   // tracer bumps: stream s0+4, one set of centres per tracer
   SplitMix rt(s0 + 4);
   for (int q = 0; q < C->nt; ++q) {
@@ -755,10 +996,10 @@ swm_status swm_init_synthetic(swm_ctx* C, int seed_index, int lmax, double umax,
     }
     HIP_TRY(hipMemcpyAsync(C->sums, cen.data(), cen.size() * sizeof(double), hipMemcpyHostToDevice, S));
     hipLaunchKernelGGL(swm::dev::k_bumps, dim3(nb), dim3(BLK), 0, S, ns, C->pxyz, nbumps, C->sums, C->radius,
-                       bump_radius, C->tr[0][q]);
+                       bump_radius, C->d.tr[0][q]);
     HIP_TRY(hipStreamSynchronize(S));
   }
-  fill(C, C->xout, 2 * (size_t)ns, 0.0);   // the solver's warm start begins at zero
+  fill<double>(C, C->xout, 2 * (size_t)ns, 0.0);   // the solver's warm start begins at zero
   HIP_TRY(hipStreamSynchronize(S));
   return finish_init(C);
 }
@@ -774,9 +1015,16 @@ swm_status swm_step(swm_ctx* C, int nsteps) {
     C->step_count += 1;
     pointers(C);
     weights(C);
-    swm_status st = step_dt(C);
-    if (st) return st;
-    wrap_prognostics(C);
+    swm_status st;
+    if (C->precision == SWM_MIXED) {
+      st = step_dt<float>(C, C->m);
+      if (st) return st;
+      wrap_prognostics<float>(C, C->m);
+    } else {
+      st = step_dt<double>(C, C->d);
+      if (st) return st;
+      wrap_prognostics<double>(C, C->d);
+    }
   }
   HIP_TRY(hipEventRecord(C->ev_b, C->stream));
   HIP_TRY(hipEventSynchronize(C->ev_b));
@@ -792,23 +1040,35 @@ swm_status swm_get_field(swm_ctx* C, const char* name, double* out) {
   HIP_TRY(hipSetDevice(C->device));
   const int ns = C->ns;
   const int n4 = C->n4 - 1;
-  HIP_TRY(hipStreamSynchronize(C->stream));
   const std::string s(name);
+  const bool mixed = C->precision == SWM_MIXED;
+  // mixed path: FP32 storage back to FP64 values in a scratch plane first
+  double* tmp = C->d.f1;   // 3 planes of FP64 scratch, unused by the mixed step
+  auto plane = [&](const double* d64, const float* d32, const double* off = nullptr, double c = 0.0) -> const double* {
+    if (!mixed) return d64;
+    to64(C, d32, tmp, ns, off, c);
+    return tmp;
+  };
   const double* src = nullptr;
-  if (s == "h") src = C->h[n4];
-  else if (s == "eta") src = C->eta[n4];
-  else if (s == "div") src = C->div[n4];
-  else if (s == "relative") src = C->relative;
-  else if (s == "ke") src = C->ke;
+  if (s == "h") src = plane(C->d.h[n4], C->m.h[n4], nullptr, C->h0);
+  else if (s == "eta") src = plane(C->d.eta[n4], C->m.eta[n4], C->f);
+  else if (s == "div") src = plane(C->d.div[n4], C->m.div[n4]);
+  else if (s == "relative") src = plane(C->d.relative, C->m.relative);
+  else if (s == "ke") src = plane(C->d.ke, C->m.ke);
   else if (s == "psi") src = C->xout;
   else if (s == "chi") src = C->xout + ns;
   if (src) {
+    HIP_TRY(hipStreamSynchronize(C->stream));
     HIP_TRY(hipMemcpy(out, src, ns * sizeof(double), hipMemcpyDeviceToHost));
     return SWM_OK;
   }
   if (s == "u") {
     std::vector<double> p(4 * (size_t)ns);
-    HIP_TRY(hipMemcpy(p.data(), C->u[n4], p.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int c = 0; c < 4; ++c) {
+      const double* q = plane(C->d.u[n4] + (size_t)c * ns, mixed ? C->m.u[n4] + (size_t)c * ns : nullptr);
+      HIP_TRY(hipStreamSynchronize(C->stream));
+      HIP_TRY(hipMemcpy(p.data() + (size_t)c * ns, q, ns * sizeof(double), hipMemcpyDeviceToHost));
+    }
     u_from_planes(p.data(), out, ns);
     return SWM_OK;
   }
@@ -816,7 +1076,9 @@ swm_status swm_get_field(swm_ctx* C, const char* name, double* out) {
     std::vector<double> p((size_t)ns);
     const size_t pl = (size_t)C->iim * C->jjm;
     for (int q = 0; q < C->nt; ++q) {
-      HIP_TRY(hipMemcpy(p.data(), C->tr[n4][q], ns * sizeof(double), hipMemcpyDeviceToHost));
+      const double* t = plane(C->d.tr[n4][q], mixed ? C->m.tr[n4][q] : nullptr);
+      HIP_TRY(hipStreamSynchronize(C->stream));
+      HIP_TRY(hipMemcpy(p.data(), t, ns * sizeof(double), hipMemcpyDeviceToHost));
       for (int k = 0; k < 10; ++k) std::memcpy(out + ((size_t)q + (size_t)C->nt * k) * pl, &p[k * pl], pl * sizeof(double));
     }
     return SWM_OK;
@@ -833,6 +1095,26 @@ swm_status swm_synchronize(swm_ctx* C) {
 
 int swm_step_count(const swm_ctx* C) { return C ? C->step_count : -1; }
 int swm_last_vcycles(const swm_ctx* C) { return C ? C->last_vcycles : -1; }
+
+swm_status swm_set_solver(swm_ctx* C, double tol, int relative, int max_cycles) {
+  if (!C || !(tol > 0.0) || max_cycles < 2 || max_cycles > 1000) {
+    swm::set_error("swm_set_solver: need tol > 0 and 2 <= max_cycles <= 1000");
+    return SWM_ERR_ARG;
+  }
+  C->solver_tol = tol;
+  C->solver_relative = relative ? 1 : 0;
+  C->solver_max_cycles = max_cycles;
+  return SWM_OK;
+}
+
+swm_status swm_last_residual(const swm_ctx* C, double* resid_ms, double* rhs_ms) {
+  if (!C) return SWM_ERR_ARG;
+  for (int v = 0; v < 2; ++v) {
+    if (resid_ms) resid_ms[v] = C->last_resid[v];
+    if (rhs_ms) rhs_ms[v] = C->last_rhs_ms[v];
+  }
+  return SWM_OK;
+}
 
 swm_status swm_profile(swm_ctx* C, int enable) {
   if (!C) return SWM_ERR_ARG;
@@ -867,10 +1149,21 @@ swm_status swm_total_mass(swm_ctx* C, double* mass) {
   if (!C || !mass) return SWM_ERR_ARG;
   HIP_TRY(hipSetDevice(C->device));
   const int n4 = C->n4 - 1;
-  sum2(C, C->h[n4], C->h[n4], C->area_mask, C->ns);
-  HIP_TRY(hipMemcpyAsync(C->host_sums, C->sums, sizeof(double), hipMemcpyDeviceToHost, C->stream));
-  HIP_TRY(hipStreamSynchronize(C->stream));
-  *mass = C->host_sums[0];
+  double m[2];
+  if (C->precision == SWM_MIXED) {
+    // sum of (h' + h0) area = sum h' area + h0 * total area
+    hipLaunchKernelGGL((swm::dev::k_sum_partial<float, double>), dim3(C->npart, 2), dim3(BLK), 0, C->stream,
+                       C->m.h[n4], C->m.h[n4], C->area_mask, C->ns, C->part);
+    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(256), 0, C->stream, C->part, C->npart, C->sums);
+    swm_status st = read_sums(C, 2, m);
+    if (st) return st;
+    *mass = m[0] + C->h0 * C->total_area;
+    return SWM_OK;
+  }
+  sum2<double>(C, C->d.h[n4], C->d.h[n4], C->area_mask, C->ns);
+  swm_status st = read_sums(C, 2, m);
+  if (st) return st;
+  *mass = m[0];
   return SWM_OK;
 }
 
@@ -1013,7 +1306,7 @@ swm_status swm_op_calculate_velocity(swm_ctx* C, int npad, int nx, int ny, int n
   G.vw = io.put(v_weights, 12, 12);
   double *p = io.put(psi, 1, 1), *c = io.put(chi, 1, 1), *y = io.dev(4);
   OP_BUF(G.cai && G.vw && p && c && y);
-  hipLaunchKernelGGL(swm::dev::k_velocity, dim3(nb_), dim3(BLK), 0, C->stream, G, p, c, y);
+  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb_), dim3(BLK), 0, C->stream, G, p, c, y);
   return io.get(xout, y, 4, kCornerVec);
 }
 
@@ -1099,7 +1392,7 @@ swm_status swm_wrap(swm_ctx* C, int kind, int nspace, int nlayer, double* x) {
         for (int q = 0; q < pl; ++q)
           for (int c = 0; c < ncomp; ++c) planes[(size_t)c * ns + (size_t)k * pl + q] = at(sp, c, q, l, k);
       HIP_TRY(hipMemcpy(d, planes.data(), planes.size() * sizeof(double), hipMemcpyHostToDevice));
-      gather(C, t, {d});
+      gather<double>(C, t, {d});
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipStreamSynchronize(C->stream));
       HIP_TRY(hipMemcpy(planes.data(), d, planes.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -1150,7 +1443,7 @@ swm_status swm_multigrid(swm_ctx* C, int nvar, const double* beta, double* alpha
   HIP_TRY(hipMemcpy(be, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
   vars_to_planes(alpha, nvar, pl, b.data());
   HIP_TRY(hipMemcpy(al, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice));
-  vcycle(C, nvar, al, be);
+  vcycle<double>(C, C->d, nvar, al, be);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(C->stream));
   HIP_TRY(hipMemcpy(b.data(), al, b.size() * sizeof(double), hipMemcpyDeviceToHost));

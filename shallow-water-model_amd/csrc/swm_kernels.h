@@ -1,28 +1,44 @@
 // swm_kernels.h -- argument bundles of the device kernels.
+//
+// The step and multigrid kernels are templates on the storage type R of the
+// state and metric planes: R = double is the FP64 path (the reference's
+// arithmetic, operation for operation), R = float the mixed path (FP32
+// storage and stencils; see the kernels for the offset forms it uses).
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace swm {
 namespace dev {
 
-// grid extents and metric planes of the model level
-struct Geo {
+// grid extents and metric planes of the model level.  Mixed path only: h0
+// is subtracted from every stored thickness-like field (h, the low-order
+// mass) and fo (the Coriolis parameter) from the stored vorticity, so that
+// FP32 keeps the small increments of nearly constant fields.
+template <class R>
+struct GeoT {
   int iim, jjm, ns;
-  const double *area, *area_inv, *cai, *sub_area, *dc, *nrm, *tng, *vw, *sw, *lw, *f, *hs, *mask;
+  const R *area, *area_inv, *cai, *sub_area, *dc, *nrm, *tng, *vw, *sw, *lw, *f, *hs, *mask;
+  R h0;
+  const R* fo;
 };
+using Geo = GeoT<double>;
 
 // one multigrid level: Laplacian weights, reciprocal central weight and the
 // closed-form halo map (src[slot] = slot whose value the halo slot holds)
-struct Lev {
+template <class R>
+struct LevT {
   int iim, jjm, ns;
-  const double* lw;
-  const double* lwt;
+  const R* lw;
+  const R* lwt;
   const int* src;
 };
+using Lev = LevT<double>;
 
-struct FieldSet {
-  double* p[8];
+template <class R>
+struct FieldSetT {
+  R* p[8];
 };
+using FieldSet = FieldSetT<double>;
 
 }  // namespace dev
 }  // namespace swm

@@ -42,21 +42,42 @@ __device__ __forceinline__ bool interior(const IJK& c, int iim, int jjm) {
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 
+template <class R>
+__device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
+template <class R>
+__device__ __forceinline__ R rmax(R a, R b) { return a > b ? a : b; }
+
+// the mixed path stores thickness-like fields less h0 and the vorticity less
+// the Coriolis parameter; these give the absolute values (identity in FP64)
+template <class R>
+__device__ __forceinline__ R habs(const GeoT<R>& G, R x) {
+  if constexpr (sizeof(R) == 4) return x + G.h0;
+  else return x;
+}
+template <class R>
+__device__ __forceinline__ R eabs(const GeoT<R>& G, const R* __restrict__ eta, int s) {
+  if constexpr (sizeof(R) == 4) return eta[s] + G.fo[s];
+  else return eta[s];
+}
+
 // ------------------------------------------------------------------ halo --
 // x[dst] = (src < 0 ? 0 : x[src]) for every table entry, on up to 8 fields
 // (blockIdx.y); no source is a destination, so it runs in place.
-__global__ void k_gather(const int* __restrict__ dst, const int* __restrict__ src, int n, FieldSet fs) {
+template <class R>
+__global__ void k_gather(const int* __restrict__ dst, const int* __restrict__ src, int n, FieldSetT<R> fs) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  double* x = fs.p[blockIdx.y];
+  R* x = fs.p[blockIdx.y];
   const int s = src[t];
-  x[dst[t]] = s < 0 ? 0.0 : x[s];
+  x[dst[t]] = s < 0 ? R(0) : x[s];
 }
 
 // ---------------------------------------------------- corner averages ----
-// operators.F:982-1112 average_to_cell_corners (flag = .true.) of two fields
-__global__ void k_avg_corners(Geo G, const double* __restrict__ sw, const double* __restrict__ x1,
-                              const double* __restrict__ x2, double* __restrict__ o1, double* __restrict__ o2) {
+// operators.F:982-1112 average_to_cell_corners (flag = .true.) of the
+// thickness (absolute values); the step's corner vorticity (step.F:132-141)
+// only feeds an unused work array (step.F:391-393) and is not formed
+template <class R>
+__global__ void k_avg_corners(GeoT<R> G, const R* __restrict__ sw, const R* __restrict__ x1, R* __restrict__ o1) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
@@ -64,16 +85,15 @@ __global__ void k_avg_corners(Geo G, const double* __restrict__ sw, const double
   const int i = c.i, j = c.j, k = c.k;
   if (i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1) {
     const int a = SL(i, j, k), b = SL(i - 1, j, k), d = SL(i - 1, j - 1, k), e = SL(i, j - 1, k);
-    const double w10 = sw[0 * ns + s], w20 = sw[1 * ns + s], w30 = sw[2 * ns + s];
-    const double w11 = sw[3 * ns + s], w21 = sw[4 * ns + s], w31 = sw[5 * ns + s];
-    o1[s] = +w10 * x1[a] + w20 * x1[b] + w30 * x1[d];
-    o1[ns + s] = +w11 * x1[a] + w21 * x1[d] + w31 * x1[e];
-    o2[s] = +w10 * x2[a] + w20 * x2[b] + w30 * x2[d];
-    o2[ns + s] = +w11 * x2[a] + w21 * x2[d] + w31 * x2[e];
+    const R w10 = sw[0 * ns + s], w20 = sw[1 * ns + s], w30 = sw[2 * ns + s];
+    const R w11 = sw[3 * ns + s], w21 = sw[4 * ns + s], w31 = sw[5 * ns + s];
+    const R xa = habs(G, x1[a]), xb = habs(G, x1[b]), xd = habs(G, x1[d]), xe = habs(G, x1[e]);
+    o1[s] = +w10 * xa + w20 * xb + w30 * xd;
+    o1[ns + s] = +w11 * xa + w21 * xd + w31 * xe;
   } else {
-    const double f1 = x1[SL(2, 2, k)], f2 = x2[SL(2, 2, k)];
-    o1[s] = f1; o1[ns + s] = f1;
-    o2[s] = f2; o2[ns + s] = f2;
+    const R f1 = habs(G, x1[SL(2, 2, k)]);
+    o1[s] = f1;
+    o1[ns + s] = f1;
   }
 }
 
@@ -81,18 +101,20 @@ __global__ void k_avg_corners(Geo G, const double* __restrict__ sw, const double
 // (operators.F:1626-1722); with mc == nullptr the corner-mass factor is
 // dropped (upstream_mass_flux, operators.F:1731-1832).  nrm is either the
 // normal or the tangent planes.
+template <class R>
 struct Half {
-  double r[2][3];
+  R r[2][3];
 };
-__device__ __forceinline__ Half half_edges(int iim, int jjm, int ns, int i, int j, int k, const double* __restrict__ dc,
-                                           const double* __restrict__ nrm, const double* __restrict__ mc,
-                                           const double* __restrict__ u) {
+template <class R>
+__device__ __forceinline__ Half<R> half_edges(int iim, int jjm, int ns, int i, int j, int k, const R* __restrict__ dc,
+                                              const R* __restrict__ nrm, const R* __restrict__ mc,
+                                              const R* __restrict__ u) {
   // u planes: (ic + 2*sp); nrm planes: kk + 3*(a + 2*ic); dc planes kk + 3*ic
 #define U(sp, ic, s_) u[((ic) + 2 * (sp)) * ns + (s_)]
 #define N(kk, a, ic, s_) nrm[((kk) + 3 * ((a) + 2 * (ic))) * ns + (s_)]
 #define D(kk, ic, s_) dc[((kk) + 3 * (ic)) * ns + (s_)]
 #define HN(s_, kk, ic) (U(0, ic, s_) * N(kk, 0, ic, s_) + U(1, ic, s_) * N(kk, 1, ic, s_))
-  Half h;
+  Half<R> h;
   const int s0 = SL(i, j, k), sn = SL(i, j + 1, k), se = SL(i + 1, j, k);
   if (mc) {
     h.r[0][0] = HN(sn, 1, 1) * mc[ns + sn]; h.r[0][0] = h.r[0][0] * D(1, 1, sn);
@@ -118,32 +140,35 @@ __device__ __forceinline__ Half half_edges(int iim, int jjm, int ns, int i, int 
 
 // upstream (first-order) and centred (high-order tendency) edge mass fluxes
 // (mass_fct.F:151-214 via operators.F:1626-1832)
-__global__ void k_mass_fluxes(Geo G, const double* __restrict__ u, const double* __restrict__ h3,
-                              const double* __restrict__ hc, double* __restrict__ flx1, double* __restrict__ mshf) {
+template <class R>
+__global__ void k_mass_fluxes(GeoT<R> G, const R* __restrict__ u, const R* __restrict__ h3, const R* __restrict__ hc,
+                              R* __restrict__ flx1, R* __restrict__ mshf) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   if (!interior(c, iim, jjm)) {
-    for (int e = 0; e < 3; ++e) { flx1[e * ns + s] = 0.0; mshf[e * ns + s] = 0.0; }
+    for (int e = 0; e < 3; ++e) { flx1[e * ns + s] = R(0); mshf[e * ns + s] = R(0); }
     return;
   }
   const int i = c.i, j = c.j, k = c.k;
-  const Half hc_ = half_edges(iim, jjm, ns, i, j, k, G.dc, G.nrm, hc, u);
-  const Half hu = half_edges(iim, jjm, ns, i, j, k, G.dc, G.nrm, nullptr, u);
+  const Half<R> hc_ = half_edges<R>(iim, jjm, ns, i, j, k, G.dc, G.nrm, hc, u);
+  const Half<R> hu = half_edges<R>(iim, jjm, ns, i, j, k, G.dc, G.nrm, nullptr, u);
   const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  const R hs0 = habs(G, h3[s]);
   for (int e = 0; e < 3; ++e) {
     mshf[e * ns + s] = -hc_.r[0][e] + hc_.r[1][e];
-    const double v = hu.r[1][e] - hu.r[0][e];
-    const double mx = v > 0.0 ? v : 0.0, mn = v < 0.0 ? v : 0.0;
-    flx1[e * ns + s] = mx * h3[s] + mn * h3[up[e]];
+    const R v = hu.r[1][e] - hu.r[0][e];
+    const R mx = v > R(0) ? v : R(0), mn = v < R(0) ? v : R(0);
+    flx1[e * ns + s] = mx * hs0 + mn * habs(G, h3[up[e]]);
   }
 }
 
 // update of a cell field by the divergence of edge fluxes, pole forms
 // included (mass_fct.F:350-445, :478-532; tracer_fct.F:251-321, :601-665)
-__device__ __forceinline__ double edge_update(int iim, int jjm, int ns, const IJK& c, double base, double dt_ai,
-                                              const double* __restrict__ F, bool& set) {
+template <class R>
+__device__ __forceinline__ R edge_update(int iim, int jjm, int ns, const IJK& c, R base, R dt_ai,
+                                         const R* __restrict__ F, bool& set) {
   const int i = c.i, j = c.j, k = c.k;
   set = true;
   if (i == 1 && j == jjm - 1)
@@ -156,23 +181,24 @@ __device__ __forceinline__ double edge_update(int iim, int jjm, int ns, const IJ
     return base - dt_ai * ((F[0 * ns + SL(i, j, k)] + F[1 * ns + SL(i, j, k)] + F[2 * ns + SL(i, j, k)]) -
                            (F[0 * ns + SL(i + 1, j, k)] + F[1 * ns + SL(i + 1, j + 1, k)] + F[2 * ns + SL(i, j + 1, k)]));
   set = false;
-  return 0.0;
+  return R(0);
 }
 
 // antidiffusive flux (all slots), low-order mass and the limiter beta_down
 // (mass_fct.F:205-214 and :342-402; the beta loop starts at i,j = 1 and so
-// overwrites its own pole values, which is reproduced)
-__global__ void k_mass_low_beta(Geo G, double dt, double w3, double w2, double w1, const double* __restrict__ flx1,
-                                const double* __restrict__ m3, const double* __restrict__ m2,
-                                const double* __restrict__ m1, const double* __restrict__ h3,
-                                double* __restrict__ anti, double* __restrict__ mss_low, double* __restrict__ beta) {
+// overwrites its own pole values, which is reproduced).  The low-order mass
+// is stored like h (less h0 in the mixed path).
+template <class R>
+__global__ void k_mass_low_beta(GeoT<R> G, R dt, R w3, R w2, R w1, const R* __restrict__ flx1, const R* __restrict__ m3,
+                                const R* __restrict__ m2, const R* __restrict__ m1, const R* __restrict__ h3,
+                                R* __restrict__ anti, R* __restrict__ mss_low, R* __restrict__ beta) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
   auto A = [&](int e, int sl) {
-    double f2 = 0.0;
+    R f2 = R(0);
     f2 = f2 + w3 * m3[e * ns + sl];
     f2 = f2 + w2 * m2[e * ns + sl];
     f2 = f2 + w1 * m1[e * ns + sl];
@@ -180,88 +206,92 @@ __global__ void k_mass_low_beta(Geo G, double dt, double w3, double w2, double w
   };
   for (int e = 0; e < 3; ++e) anti[e * ns + s] = A(e, s);
   bool set;
-  const double ml = edge_update(iim, jjm, ns, c, h3[s], dt * G.area_inv[s], flx1, set);
+  const R ml = edge_update<R>(iim, jjm, ns, c, h3[s], dt * G.area_inv[s], flx1, set);
   mss_low[s] = ml;
   if (i <= iim - 1 && j <= jjm - 1) {
-    const double a_out = (dmax(0.0, A(0, s)) + dmax(0.0, A(1, s)) + dmax(0.0, A(2, s))) -
-                         (dmin(0.0, A(0, SL(i + 1, j, k))) + dmin(0.0, A(1, SL(i + 1, j + 1, k))) +
-                          dmin(0.0, A(2, SL(i, j + 1, k))));
-    beta[s] = G.area[s] * (ml - 1.0e-6) / (dt * (a_out + 1.0e-18));
+    const R a_out = (rmax(R(0), A(0, s)) + rmax(R(0), A(1, s)) + rmax(R(0), A(2, s))) -
+                    (rmin(R(0), A(0, SL(i + 1, j, k))) + rmin(R(0), A(1, SL(i + 1, j + 1, k))) +
+                     rmin(R(0), A(2, SL(i, j + 1, k))));
+    beta[s] = G.area[s] * (habs(G, ml) - R(1.0e-6)) / (dt * (a_out + R(1.0e-18)));
   } else {
-    beta[s] = 0.0;
+    beta[s] = R(0);
   }
 }
 
 // mass_fct.F:410-470 set_corrected_mss_anti (in place; non-interior slots
 // keep the uncorrected value until the edge halo fill)
-__global__ void k_mass_anti(Geo G, const double* __restrict__ beta, double* __restrict__ anti) {
+template <class R>
+__global__ void k_mass_anti(GeoT<R> G, const R* __restrict__ beta, R* __restrict__ anti) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   if (!interior(c, iim, jjm)) return;
   const int i = c.i, j = c.j, k = c.k;
-  const double b = dmin(1.0, beta[s]);
+  const R b = rmin(R(1), beta[s]);
   const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
   for (int e = 0; e < 3; ++e) {
-    const double a = anti[e * ns + s];
-    anti[e * ns + s] = b * dmax(0.0, a) + dmin(1.0, beta[up[e]]) * dmin(0.0, a);
+    const R a = anti[e * ns + s];
+    anti[e * ns + s] = b * rmax(R(0), a) + rmin(R(1), beta[up[e]]) * rmin(R(0), a);
   }
 }
 
 // flux-corrected mass and the total mass flux (mass_fct.F:121-128, :478-532)
-__global__ void k_mass_final(Geo G, double dt, const double* __restrict__ flx1, const double* __restrict__ anti,
-                             const double* __restrict__ mss_low, double* __restrict__ mflux, double* __restrict__ h4) {
+template <class R>
+__global__ void k_mass_final(GeoT<R> G, R dt, const R* __restrict__ flx1, const R* __restrict__ anti,
+                             const R* __restrict__ mss_low, R* __restrict__ mflux, R* __restrict__ h4) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   for (int e = 0; e < 3; ++e) mflux[e * ns + s] = flx1[e * ns + s] + anti[e * ns + s];
   bool set;
-  h4[s] = edge_update(iim, jjm, ns, c, mss_low[s], dt * G.area_inv[s], anti, set);
+  h4[s] = edge_update<R>(iim, jjm, ns, c, mss_low[s], dt * G.area_inv[s], anti, set);
 }
 
 // tracer FCT, first- and high-order edge fluxes (tracer_fct.F:166-243); the
 // high-order tracer is the AB3 extrapolation of step.F:657-660
-__global__ void k_tr_fluxes(Geo G, double w3, double w2, double w1, const double* __restrict__ t3,
-                            const double* __restrict__ t4, const double* __restrict__ told,
-                            const double* __restrict__ mflux, double* __restrict__ f1, double* __restrict__ f2) {
+template <class R>
+__global__ void k_tr_fluxes(GeoT<R> G, R w3, R w2, R w1, const R* __restrict__ t3, const R* __restrict__ t4,
+                            const R* __restrict__ told, const R* __restrict__ mflux, R* __restrict__ f1,
+                            R* __restrict__ f2) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   if (!interior(c, iim, jjm)) {
-    for (int e = 0; e < 3; ++e) { f1[e * ns + s] = 0.0; f2[e * ns + s] = 0.0; }
+    for (int e = 0; e < 3; ++e) { f1[e * ns + s] = R(0); f2[e * ns + s] = R(0); }
     return;
   }
   const int i = c.i, j = c.j, k = c.k;
   auto TH = [&](int sl) { return w3 * t3[sl] + w2 * t4[sl] + w1 * told[sl]; };
   const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
-  const double ths = TH(s);
+  const R ths = TH(s);
   for (int e = 0; e < 3; ++e) {
-    const double F = mflux[e * ns + s];
-    f1[e * ns + s] = t3[s] * dmax(0.0, F) + t3[up[e]] * dmin(0.0, F);
-    f2[e * ns + s] = 0.5 * (ths + TH(up[e])) * F;
+    const R F = mflux[e * ns + s];
+    f1[e * ns + s] = t3[s] * rmax(R(0), F) + t3[up[e]] * rmin(R(0), F);
+    f2[e * ns + s] = R(0.5) * (ths + TH(up[e])) * F;
   }
 }
 
 // tracer_fct.F:251-321 set_msstrc_low
-__global__ void k_tr_low(Geo G, double dt, const double* __restrict__ h3, const double* __restrict__ t3,
-                         const double* __restrict__ f1, double* __restrict__ ml) {
+template <class R>
+__global__ void k_tr_low(GeoT<R> G, R dt, const R* __restrict__ h3, const R* __restrict__ t3, const R* __restrict__ f1,
+                         R* __restrict__ ml) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   bool set;
-  ml[s] = edge_update(iim, jjm, ns, c, h3[s] * t3[s], dt * G.area_inv[s], f1, set);
+  ml[s] = edge_update<R>(iim, jjm, ns, c, habs(G, h3[s]) * t3[s], dt * G.area_inv[s], f1, set);
 }
 
 // tracer_fct.F:329-531: local extrema over the cell and its neighbours of the
 // tracer and its low-order update, and the limiter weights beta_dn / beta_up
-__global__ void k_tr_limiter(Geo G, double dt, const double* __restrict__ t3, const double* __restrict__ ml,
-                             const double* __restrict__ h4, const double* __restrict__ mss_low,
-                             const double* __restrict__ f1, const double* __restrict__ f2,
-                             double* __restrict__ anti, double* __restrict__ bdn, double* __restrict__ bup) {
+template <class R>
+__global__ void k_tr_limiter(GeoT<R> G, R dt, const R* __restrict__ t3, const R* __restrict__ ml,
+                             const R* __restrict__ h4, const R* __restrict__ mss_low, const R* __restrict__ f1,
+                             const R* __restrict__ f2, R* __restrict__ anti, R* __restrict__ bdn, R* __restrict__ bup) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
@@ -269,27 +299,30 @@ __global__ void k_tr_limiter(Geo G, double dt, const double* __restrict__ t3, co
   const int i = c.i, j = c.j, k = c.k;
   auto A = [&](int e, int sl) { return f2[e * ns + sl] - f1[e * ns + sl]; };
   for (int e = 0; e < 3; ++e) anti[e * ns + s] = A(e, s);
-  auto TL = [&](int sl) { return h4[sl] > 0.0 ? ml[sl] / h4[sl] : 0.0; };
+  auto TL = [&](int sl) {
+    const R hh = habs(G, h4[sl]);
+    return hh > R(0) ? ml[sl] / hh : R(0);
+  };
   const int p = iim - 1, q = jjm - 1;
   const bool np = (i == 1 && j == q), sp = (i == p && j == 1);
   const bool in = interior(c, iim, jjm);
-  if (!(np || sp || in)) { bdn[s] = 0.0; bup[s] = 0.0; return; }
-  double mn, mx, a_out, a_in;
+  if (!(np || sp || in)) { bdn[s] = R(0); bup[s] = R(0); return; }
+  R mn, mx, a_out, a_in;
   if (in) {
     const int nb[7] = {s, SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k), SL(i + 1, j, k),
                        SL(i + 1, j + 1, k), SL(i, j + 1, k)};
     mn = t3[s]; mx = t3[s];
-    const double tl0 = TL(s);
-    mn = dmin(mn, tl0); mx = dmax(mx, tl0);
-    for (int n = 1; n < 7; ++n) { mn = dmin(mn, t3[nb[n]]); mx = dmax(mx, t3[nb[n]]); }
-    for (int n = 1; n < 7; ++n) { const double v = TL(nb[n]); mn = dmin(mn, v); mx = dmax(mx, v); }
-    const double o1 = A(0, s), o2 = A(1, s), o3 = A(2, s);
-    const double n1 = A(0, SL(i + 1, j, k)), n2 = A(1, SL(i + 1, j + 1, k)), n3 = A(2, SL(i, j + 1, k));
-    a_out = (dmax(0.0, o1) + dmax(0.0, o2) + dmax(0.0, o3)) - (dmin(0.0, n1) + dmin(0.0, n2) + dmin(0.0, n3));
-    a_in = -((dmin(0.0, o1) + dmin(0.0, o2) + dmin(0.0, o3)) - (dmax(0.0, n1) + dmax(0.0, n2) + dmax(0.0, n3)));
+    const R tl0 = TL(s);
+    mn = rmin(mn, tl0); mx = rmax(mx, tl0);
+    for (int n = 1; n < 7; ++n) { mn = rmin(mn, t3[nb[n]]); mx = rmax(mx, t3[nb[n]]); }
+    for (int n = 1; n < 7; ++n) { const R v = TL(nb[n]); mn = rmin(mn, v); mx = rmax(mx, v); }
+    const R o1 = A(0, s), o2 = A(1, s), o3 = A(2, s);
+    const R n1 = A(0, SL(i + 1, j, k)), n2 = A(1, SL(i + 1, j + 1, k)), n3 = A(2, SL(i, j + 1, k));
+    a_out = (rmax(R(0), o1) + rmax(R(0), o2) + rmax(R(0), o3)) - (rmin(R(0), n1) + rmin(R(0), n2) + rmin(R(0), n3));
+    a_in = -((rmin(R(0), o1) + rmin(R(0), o2) + rmin(R(0), o3)) - (rmax(R(0), n1) + rmax(R(0), n2) + rmax(R(0), n3)));
   } else {
     int nb[5];
-    double ee[5];
+    R ee[5];
     if (np) {
       const int t[5] = {SL(p, 1, k), SL(1, q - 1, k), SL(2, q, k), SL(2, q + 1, k), SL(1, q + 1, k)};
       for (int n = 0; n < 5; ++n) nb[n] = t[n];
@@ -302,21 +335,22 @@ __global__ void k_tr_limiter(Geo G, double dt, const double* __restrict__ t3, co
       ee[4] = A(2, SL(i, j + 1, k));
     }
     mn = t3[s]; mx = t3[s];
-    const double tl0 = TL(s);
-    mn = dmin(mn, tl0); mx = dmax(mx, tl0);
-    for (int n = 0; n < 5; ++n) { mn = dmin(mn, t3[nb[n]]); mx = dmax(mx, t3[nb[n]]); }
-    for (int n = 0; n < 5; ++n) { const double v = TL(nb[n]); mn = dmin(mn, v); mx = dmax(mx, v); }
-    a_out = -(dmin(0.0, ee[0]) + dmin(0.0, ee[1]) + dmin(0.0, ee[2]) + dmin(0.0, ee[3]) + dmin(0.0, ee[4]));
-    a_in = (dmax(0.0, ee[0]) + dmax(0.0, ee[1]) + dmax(0.0, ee[2]) + dmax(0.0, ee[3]) + dmax(0.0, ee[4]));
+    const R tl0 = TL(s);
+    mn = rmin(mn, tl0); mx = rmax(mx, tl0);
+    for (int n = 0; n < 5; ++n) { mn = rmin(mn, t3[nb[n]]); mx = rmax(mx, t3[nb[n]]); }
+    for (int n = 0; n < 5; ++n) { const R v = TL(nb[n]); mn = rmin(mn, v); mx = rmax(mx, v); }
+    a_out = -(rmin(R(0), ee[0]) + rmin(R(0), ee[1]) + rmin(R(0), ee[2]) + rmin(R(0), ee[3]) + rmin(R(0), ee[4]));
+    a_in = (rmax(R(0), ee[0]) + rmax(R(0), ee[1]) + rmax(R(0), ee[2]) + rmax(R(0), ee[3]) + rmax(R(0), ee[4]));
   }
-  const double ar = G.area[s] / dt;
-  bdn[s] = ar * (ml[s] - mss_low[s] * mn) / (a_out + 1.0e-18);
-  bup[s] = ar * (mss_low[s] * mx - ml[s]) / (a_in + 1.0e-18);
+  const R ar = G.area[s] / dt;
+  const R mlow = habs(G, mss_low[s]);
+  bdn[s] = ar * (ml[s] - mlow * mn) / (a_out + R(1.0e-18));
+  bup[s] = ar * (mlow * mx - ml[s]) / (a_in + R(1.0e-18));
 }
 
 // tracer_fct.F:539-593 set_corrected_trc_anti (in place)
-__global__ void k_tr_anti(Geo G, const double* __restrict__ bdn, const double* __restrict__ bup,
-                          double* __restrict__ anti) {
+template <class R>
+__global__ void k_tr_anti(GeoT<R> G, const R* __restrict__ bdn, const R* __restrict__ bup, R* __restrict__ anti) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
@@ -325,56 +359,60 @@ __global__ void k_tr_anti(Geo G, const double* __restrict__ bdn, const double* _
   const int i = c.i, j = c.j, k = c.k;
   const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
   for (int e = 0; e < 3; ++e) {
-    const double a = anti[e * ns + s];
-    anti[e * ns + s] = dmin(dmin(1.0, bdn[s]), bup[up[e]]) * dmax(0.0, a) +
-                       dmin(dmin(1.0, bup[s]), bdn[up[e]]) * dmin(0.0, a);
+    const R a = anti[e * ns + s];
+    anti[e * ns + s] = rmin(rmin(R(1), bdn[s]), bup[up[e]]) * rmax(R(0), a) +
+                       rmin(rmin(R(1), bup[s]), bdn[up[e]]) * rmin(R(0), a);
   }
 }
 
 // tracer_fct.F:601-665 set_msstrc_fct and the division by the new mass; the
 // previous tracer level is kept for the next AB3 extrapolation (step.F:661)
-__global__ void k_tr_final(Geo G, double dt, const double* __restrict__ ml, const double* __restrict__ anti,
-                           const double* __restrict__ h4, double* __restrict__ t4, double* __restrict__ told) {
+template <class R>
+__global__ void k_tr_final(GeoT<R> G, R dt, const R* __restrict__ ml, const R* __restrict__ anti,
+                           const R* __restrict__ h4, R* __restrict__ t4, R* __restrict__ told) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   bool set;
-  const double mf = edge_update(iim, jjm, ns, c, ml[s], dt * G.area_inv[s], anti, set);
+  const R mf = edge_update<R>(iim, jjm, ns, c, ml[s], dt * G.area_inv[s], anti, set);
   told[s] = t4[s];
-  t4[s] = h4[s] != 0.0 ? mf / h4[s] : 0.0;
+  const R hh = habs(G, h4[s]);
+  t4[s] = hh != R(0) ? mf / hh : R(0);
 }
 
 // operators.F:2000-2186 centered_vorticity_flux and centered_divergence_curl
-__global__ void k_vort_curl(Geo G, const double* __restrict__ mflux, const double* __restrict__ h3,
-                            const double* __restrict__ eta3, const double* __restrict__ hc,
-                            const double* __restrict__ u, double* __restrict__ vf, double* __restrict__ dcurl) {
+template <class R>
+__global__ void k_vort_curl(GeoT<R> G, const R* __restrict__ mflux, const R* __restrict__ h3,
+                            const R* __restrict__ eta3, const R* __restrict__ hc, const R* __restrict__ u,
+                            R* __restrict__ vf, R* __restrict__ dcurl) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   if (!interior(c, iim, jjm)) {
-    for (int e = 0; e < 3; ++e) { vf[e * ns + s] = 0.0; dcurl[e * ns + s] = 0.0; }
+    for (int e = 0; e < 3; ++e) { vf[e * ns + s] = R(0); dcurl[e * ns + s] = R(0); }
     return;
   }
   const int i = c.i, j = c.j, k = c.k;
   auto PV = [&](int sl) {
-    const double w = h3[sl] > 0.0 ? 1.0 / h3[sl] : 0.0;
-    return eta3[sl] * w;
+    const R hh = habs(G, h3[sl]);
+    const R w = hh > R(0) ? R(1) / hh : R(0);
+    return eabs(G, eta3, sl) * w;
   };
   const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
-  const double pv0 = PV(s);
-  const Half ht = half_edges(iim, jjm, ns, i, j, k, G.dc, G.tng, hc, u);
+  const R pv0 = PV(s);
+  const Half<R> ht = half_edges<R>(iim, jjm, ns, i, j, k, G.dc, G.tng, hc, u);
   for (int e = 0; e < 3; ++e) {
-    const double pvu = PV(up[e]);
-    vf[e * ns + s] = 0.5 * (mflux[e * ns + s]) * (pv0 + pvu);
-    dcurl[e * ns + s] = (-ht.r[0][e] + ht.r[1][e]) * (pvu + pv0) * 0.5;
+    const R pvu = PV(up[e]);
+    vf[e * ns + s] = R(0.5) * (mflux[e * ns + s]) * (pv0 + pvu);
+    dcurl[e * ns + s] = (-ht.r[0][e] + ht.r[1][e]) * (pvu + pv0) * R(0.5);
   }
 }
 
 // operators.F:270-392 div_of_an_edge_flux at one slot (0 where not defined)
-__device__ __forceinline__ double div_edge_at(int iim, int jjm, int ns, const IJK& c, const double* __restrict__ F,
-                                              double ai) {
+template <class R>
+__device__ __forceinline__ R div_edge_at(int iim, int jjm, int ns, const IJK& c, const R* __restrict__ F, R ai) {
   const int i = c.i, j = c.j, k = c.k;
   if (i == 1 && j == jjm - 1)
     return (-F[1 * ns + SL(i, j, k)] - F[2 * ns + SL(i, j, k)] - F[0 * ns + SL(i + 1, j, k)] -
@@ -385,13 +423,12 @@ __device__ __forceinline__ double div_edge_at(int iim, int jjm, int ns, const IJ
   if (i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1)
     return (F[0 * ns + SL(i, j, k)] + F[1 * ns + SL(i, j, k)] + F[2 * ns + SL(i, j, k)] - F[0 * ns + SL(i + 1, j, k)] -
             F[1 * ns + SL(i + 1, j + 1, k)] - F[2 * ns + SL(i, j + 1, k)]) * ai;
-  return 0.0;
+  return R(0);
 }
 
 // operators.F:402-518 laplacian_of_a_scalar at one slot; X(sl) gives the field
-template <class FX>
-__device__ __forceinline__ double lap_at(int iim, int jjm, int ns, const IJK& c, const double* __restrict__ lw,
-                                         FX X) {
+template <class R, class FX>
+__device__ __forceinline__ R lap_at(int iim, int jjm, int ns, const IJK& c, const R* __restrict__ lw, FX X) {
   const int i = c.i, j = c.j, k = c.k;
   const int s = SL(i, j, k);
 #define LW(n) lw[((n) - 1) * ns + s]
@@ -405,53 +442,85 @@ __device__ __forceinline__ double lap_at(int iim, int jjm, int ns, const IJK& c,
     return +LW(7) * X(s) + LW(1) * X(SL(i - 1, j, k)) + LW(2) * X(SL(i - 1, j - 1, k)) + LW(3) * X(SL(i, j - 1, k)) +
            LW(4) * X(SL(i + 1, j, k)) + LW(5) * X(SL(i + 1, j + 1, k)) + LW(6) * X(SL(i, j + 1, k));
 #undef LW
-  return 0.0;
+  return R(0);
+}
+
+// The same Laplacian written on differences from the centre value, for the
+// mixed path: the weights sum to zero, so sum_k w_k (x_k - x_0) is the same
+// operator, and rounding the weights to FP32 then errs relative to the
+// differences, not to the (much larger) field values.
+template <class R, class FX>
+__device__ __forceinline__ R lap_diff_at(int iim, int jjm, int ns, const IJK& c, const R* __restrict__ lw, FX X) {
+  const int i = c.i, j = c.j, k = c.k;
+  const int s = SL(i, j, k);
+  const R x0 = X(s);
+  auto D = [&](int sl) { return X(sl) - x0; };
+#define LW(n) lw[((n) - 1) * ns + s]
+  if (i == 1 && j == jjm - 1)
+    return LW(2) * D(SL(iim, 1, k)) + LW(3) * D(SL(i, j - 1, k)) + LW(4) * D(SL(i + 1, j, k)) +
+           LW(5) * D(SL(i + 1, j + 1, k)) + LW(6) * D(SL(i, j + 1, k));
+  if (i == iim - 1 && j == 1)
+    return LW(1) * D(SL(i - 1, j, k)) + LW(2) * D(SL(1, jjm, k)) + LW(4) * D(SL(i + 1, j, k)) +
+           LW(5) * D(SL(i + 1, j + 1, k)) + LW(6) * D(SL(i, j + 1, k));
+  if (i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1)
+    return LW(1) * D(SL(i - 1, j, k)) + LW(2) * D(SL(i - 1, j - 1, k)) + LW(3) * D(SL(i, j - 1, k)) +
+           LW(4) * D(SL(i + 1, j, k)) + LW(5) * D(SL(i + 1, j + 1, k)) + LW(6) * D(SL(i, j + 1, k));
+#undef LW
+  return R(0);
 }
 
 // step.F:336-429 step_vorticity / step_divergence (ABM3) and the elliptic
-// right-hand sides of step.F:170-173
-__global__ void k_tendencies(Geo G, double dt, double grav, double w3, double w2, double w1,
-                             const double* __restrict__ vf, const double* __restrict__ dcurl,
-                             const double* __restrict__ ke, const double* __restrict__ h3,
-                             const double* __restrict__ eta3, const double* __restrict__ div3,
-                             double* __restrict__ etaf_a, const double* __restrict__ etaf_b,
-                             const double* __restrict__ etaf_c, double* __restrict__ divf_a,
-                             const double* __restrict__ divf_b, const double* __restrict__ divf_c,
-                             double* __restrict__ eta4, double* __restrict__ div4, double* __restrict__ relative,
-                             double* __restrict__ rhs1, double* __restrict__ rhs2) {
+// right-hand sides of step.F:170-173.  Mixed path: eta is stored as the
+// relative vorticity (eta - f), which is then also the right-hand side, and
+// the Laplacian of ke + g (h + hs) is formed on differences (h stored less
+// h0, a constant the Laplacian does not see).
+template <class R>
+__global__ void k_tendencies(GeoT<R> G, R dt, R grav, R w3, R w2, R w1, const R* __restrict__ vf,
+                             const R* __restrict__ dcurl, const R* __restrict__ ke, const R* __restrict__ h3,
+                             const R* __restrict__ eta3, const R* __restrict__ div3, R* __restrict__ etaf_a,
+                             const R* __restrict__ etaf_b, const R* __restrict__ etaf_c, R* __restrict__ divf_a,
+                             const R* __restrict__ divf_b, const R* __restrict__ divf_c, R* __restrict__ eta4,
+                             R* __restrict__ div4, R* __restrict__ relative, R* __restrict__ rhs1,
+                             R* __restrict__ rhs2) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
-  const double ai = G.area_inv[s];
-  const double wv = div_edge_at(iim, jjm, ns, c, vf, ai);
-  const double ef = -wv;
+  const R ai = G.area_inv[s];
+  const R wv = div_edge_at<R>(iim, jjm, ns, c, vf, ai);
+  const R ef = -wv;
   etaf_a[s] = ef;
-  const double e4 = eta3[s] + dt * (w3 * ef + w2 * etaf_b[s] + w1 * etaf_c[s]);
-  const double wc = div_edge_at(iim, jjm, ns, c, dcurl, ai);
-  const double* hs = G.hs;
+  const R e4 = eta3[s] + dt * (w3 * ef + w2 * etaf_b[s] + w1 * etaf_c[s]);
+  const R wc = div_edge_at<R>(iim, jjm, ns, c, dcurl, ai);
+  const R* hs = G.hs;
   auto X = [&](int sl) { return ke[sl] + grav * (h3[sl] + hs[sl]); };
-  const double wl = lap_at(iim, jjm, ns, c, G.lw, X);
-  const double df = -wl + wc;
+  R wl;
+  if constexpr (sizeof(R) == 4) wl = lap_diff_at<R>(iim, jjm, ns, c, G.lw, X);
+  else wl = lap_at<R>(iim, jjm, ns, c, G.lw, X);
+  const R df = -wl + wc;
   divf_a[s] = df;
-  const double d4 = div3[s] + dt * (w3 * df + w2 * divf_b[s] + w1 * divf_c[s]);
+  const R d4 = div3[s] + dt * (w3 * df + w2 * divf_b[s] + w1 * divf_c[s]);
   eta4[s] = e4;
   div4[s] = d4;
-  const double rel = e4 - G.f[s];
+  R rel;
+  if constexpr (sizeof(R) == 4) rel = e4;
+  else rel = e4 - G.f[s];
   relative[s] = rel;
   rhs1[s] = rel;
   rhs2[s] = d4;
 }
 
 // ------------------------------------------------------- reductions -------
-// per-block partial sums of x*w (w may be null) for nf fields (blockIdx.y)
-__global__ void k_sum_partial(const double* __restrict__ x0, const double* __restrict__ x1,
-                              const double* __restrict__ w, int n, double* __restrict__ part) {
+// per-block partial sums of x*w (w may be null) for nf fields (blockIdx.y),
+// accumulated in FP64
+template <class R, class W = R>
+__global__ void k_sum_partial(const R* __restrict__ x0, const R* __restrict__ x1, const W* __restrict__ w, int n,
+                              double* __restrict__ part) {
   __shared__ double sh[256];
-  const double* x = blockIdx.y == 0 ? x0 : x1;
+  const R* x = blockIdx.y == 0 ? x0 : x1;
   double acc = 0.0;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
-    acc += w ? x[t] * w[t] : x[t];
+    acc += w ? (double)x[t] * (double)w[t] : (double)x[t];
   sh[threadIdx.x] = acc;
   __syncthreads();
   for (int o = blockDim.x / 2; o > 0; o >>= 1) {
@@ -460,6 +529,26 @@ __global__ void k_sum_partial(const double* __restrict__ x0, const double* __res
   }
   if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
 }
+// per-block partial sums of x^2 * w for field blockIdx.y (x0, x1), FP64
+template <class R>
+__global__ void k_sumsq_mixed(const R* __restrict__ x0, const R* __restrict__ x1, const double* __restrict__ w, int n,
+                              double* __restrict__ part) {
+  __shared__ double sh[256];
+  const R* x = blockIdx.y == 0 ? x0 : x1;
+  double acc = 0.0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const double v = (double)x[t];
+    acc += v * v * w[t];
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+}
+
 // compensated sum of the partials of field blockIdx.x (256 threads, fixed
 // order: strided Neumaier sums per thread, then a pairwise tree of
 // (sum, error) pairs combined with Knuth's two-sum); deterministic
@@ -490,13 +579,14 @@ __global__ void __launch_bounds__(256) k_sum_final(const double* __restrict__ pa
 }
 
 // rhs -= mean (elliptic_solver.F:80-92); sums[v] / total gives the mean
-__global__ void k_sub_mean(int n, const double* __restrict__ sums, double total, double* __restrict__ r1,
-                           double* __restrict__ r2) {
+template <class R>
+__global__ void k_sub_mean(int n, const double* __restrict__ sums, double total, R* __restrict__ r1,
+                           R* __restrict__ r2) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const double m1 = sums[0] / total, m2 = sums[1] / total;
-  r1[t] = r1[t] - m1;
-  r2[t] = r2[t] - m2;
+  r1[t] = r1[t] - (R)m1;
+  r2[t] = r2[t] - (R)m2;
 }
 
 // per-block partial sums of the squared Poisson residual at real cells for
@@ -513,7 +603,7 @@ __global__ void __launch_bounds__(256) k_resid2_partial(Geo G, const double* __r
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
     const IJK c = ijk_of(s, iim, jjm);
     auto X = [&](int sl) { return x[sl]; };
-    const double w = lap_at(iim, jjm, ns, c, G.lw, X);
+    const double w = lap_at<double>(iim, jjm, ns, c, G.lw, X);
     const double r = (w - 0.0 * x[s]) - rhs[s];
     acc += (r * r) * G.mask[s];
   }
@@ -524,6 +614,44 @@ __global__ void __launch_bounds__(256) k_resid2_partial(Geo G, const double* __r
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+}
+
+// Mixed path, FP64 defect of the Poisson problem for field blockIdx.y:
+// r = rhs - L(x) with the FP64 weights and the FP64 solution x (all slots;
+// halo slots get 0), stored in FP32 as the right-hand side of the next
+// correction cycle, and per-block partial sums of r^2 at real cells.
+__global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict__ x0, const float* __restrict__ b0,
+                                                float* __restrict__ r0, const double* __restrict__ x1,
+                                                const float* __restrict__ b1, float* __restrict__ r1,
+                                                double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const double* x = blockIdx.y == 0 ? x0 : x1;
+  const float* b = blockIdx.y == 0 ? b0 : b1;
+  float* r = blockIdx.y == 0 ? r0 : r1;
+  double acc = 0.0;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
+    const IJK c = ijk_of(s, iim, jjm);
+    auto X = [&](int sl) { return x[sl]; };
+    const double w = lap_at<double>(iim, jjm, ns, c, G.lw, X);
+    const double m = G.mask[s];
+    const double d = m != 0.0 ? (double)b[s] - w : 0.0;
+    r[s] = (float)d;
+    acc += (d * d) * m;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+}
+
+// x += e (FP64 solution, FP32 correction), nvar planes of ns
+__global__ void k_correct(size_t n, const float* __restrict__ e, double* __restrict__ x) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] = x[t] + (double)e[t];
 }
 
 // ------------------------------------------------------------- multigrid --
@@ -537,9 +665,9 @@ __global__ void __launch_bounds__(256) k_resid2_partial(Geo G, const double* __r
 // read; the arithmetic is the same, so the result is the same to the bit.
 // Slots at least two away from the panel edge have no halo neighbour and
 // read their neighbours without the map.
-template <int NV, bool ZERO>
-__global__ void __launch_bounds__(256) k_relax(Lev L, const double* __restrict__ beta, const double* __restrict__ a,
-                                               double* __restrict__ out) {
+template <class R, int NV, bool ZERO>
+__global__ void __launch_bounds__(256) k_relax(LevT<R> L, const R* __restrict__ beta, const R* __restrict__ a,
+                                               R* __restrict__ out) {
   const int iim = L.iim, jjm = L.jjm, ns = L.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
@@ -549,7 +677,7 @@ __global__ void __launch_bounds__(256) k_relax(Lev L, const double* __restrict__
   const bool np = i == 1 && j == jjm - 1, sp = i == iim - 1 && j == 1;
   if (!(in || np || sp)) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) out[(size_t)v * ns + s] = 0.0;
+    for (int v = 0; v < NV; ++v) out[(size_t)v * ns + s] = R(0);
     return;
   }
   const bool deep = i >= 3 && i <= iim - 2 && j >= 3 && j <= jjm - 2;
@@ -571,32 +699,32 @@ __global__ void __launch_bounds__(256) k_relax(Lev L, const double* __restrict__
     for (int q = 0; q < 6; ++q) nb[q] = nb[q] < 0 ? -1 : m[nb[q]];
     me = m[s];
   }
-  double w[6];
+  R w[6];
 #pragma unroll
   for (int q = 0; q < 6; ++q) w[q] = L.lw[q * ns + s];
-  const double wt = L.lwt[s];
-  const double omega = 0.8;
+  const R wt = L.lwt[s];
+  const R omega = R(0.8);
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
-    const double* __restrict__ av = a + (size_t)v * ns;
-    auto A = [&](int q) { return ZERO ? 0.0 : av[nb[q]]; };
-    const double bv = beta[(size_t)v * ns + s];
-    double tt;
+    const R* __restrict__ av = a + (size_t)v * ns;
+    auto A = [&](int q) { return ZERO ? R(0) : av[nb[q]]; };
+    const R bv = beta[(size_t)v * ns + s];
+    R tt;
     if (in)
       tt = wt * (bv - w[0] * A(0) - w[1] * A(1) - w[2] * A(2) - w[3] * A(3) - w[4] * A(4) - w[5] * A(5));
     else if (np)
       tt = wt * (bv - w[1] * A(1) - w[2] * A(2) - w[3] * A(3) - w[4] * A(4) - w[5] * A(5));
     else
       tt = wt * (bv - w[0] * A(0) - w[1] * A(1) - w[3] * A(3) - w[4] * A(4) - w[5] * A(5));
-    const double self = ZERO ? 0.0 : av[me];
-    out[(size_t)v * ns + s] = (1.0 - omega) * self + omega * tt;
+    const R self = ZERO ? R(0) : av[me];
+    out[(size_t)v * ns + s] = (R(1) - omega) * self + omega * tt;
   }
 }
 
 // residual at the odd-odd fine points becomes the coarse right-hand side
 // (multigrid_solver.F:684-886 inject_residual); one thread per coarse slot
-__global__ void k_inject(Lev F, Lev C, const double* __restrict__ xa, const double* __restrict__ xb,
-                         double* __restrict__ x1) {
+template <class R>
+__global__ void k_inject(LevT<R> F, LevT<R> C, const R* __restrict__ xa, const R* __restrict__ xb, R* __restrict__ x1) {
   int iim = C.iim, jjm = C.jjm;
   const int s1 = blockIdx.x * blockDim.x + threadIdx.x;
   if (s1 >= C.ns) return;
@@ -618,16 +746,16 @@ __global__ void k_inject(Lev F, Lev C, const double* __restrict__ xa, const doub
   const int* __restrict__ m = F.src;
   auto A = [&](int sl) { return xa[m[sl]]; };
 #define LW(n) F.lw[((n) - 1) * ns + s]
-  double r;
+  R r;
   if (in)
-    r = xb[s] - ((LW(7) - 0.0) * A(s) + LW(1) * A(SL(i2 - 1, j2, k)) + LW(2) * A(SL(i2 - 1, j2 - 1, k)) +
+    r = xb[s] - ((LW(7) - R(0)) * A(s) + LW(1) * A(SL(i2 - 1, j2, k)) + LW(2) * A(SL(i2 - 1, j2 - 1, k)) +
                  LW(3) * A(SL(i2, j2 - 1, k)) + LW(4) * A(SL(i2 + 1, j2, k)) + LW(5) * A(SL(i2 + 1, j2 + 1, k)) +
                  LW(6) * A(SL(i2, j2 + 1, k)));
   else if (np)
-    r = xb[s] - ((LW(7) - 0.0) * A(s) + LW(2) * A(SL(iim, 1, k)) + LW(3) * A(SL(i2, j2 - 1, k)) +
+    r = xb[s] - ((LW(7) - R(0)) * A(s) + LW(2) * A(SL(iim, 1, k)) + LW(3) * A(SL(i2, j2 - 1, k)) +
                  LW(4) * A(SL(i2 + 1, j2, k)) + LW(5) * A(SL(i2 + 1, j2 + 1, k)) + LW(6) * A(SL(i2, j2 + 1, k)));
   else
-    r = xb[s] - ((LW(7) - 0.0) * A(s) + LW(1) * A(SL(i2 - 1, j2, k)) + LW(2) * A(SL(1, jjm, k)) +
+    r = xb[s] - ((LW(7) - R(0)) * A(s) + LW(1) * A(SL(i2 - 1, j2, k)) + LW(2) * A(SL(1, jjm, k)) +
                  LW(4) * A(SL(i2 + 1, j2, k)) + LW(5) * A(SL(i2 + 1, j2 + 1, k)) + LW(6) * A(SL(i2, j2 + 1, k)));
 #undef LW
   x1[s1] = r;
@@ -635,7 +763,8 @@ __global__ void k_inject(Lev F, Lev C, const double* __restrict__ xa, const doub
 
 // coarse correction added at coincident points, two-point averages between
 // (multigrid_solver.F:888-1054 interpolate); one thread per fine slot
-__global__ void k_interp(Lev C, Lev F, const double* __restrict__ x1, double* __restrict__ x2) {
+template <class R>
+__global__ void k_interp(LevT<R> C, LevT<R> F, const R* __restrict__ x1, R* __restrict__ x2) {
   int iim = F.iim, jjm = F.jjm;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= F.ns) return;
@@ -652,19 +781,21 @@ __global__ void k_interp(Lev C, Lev F, const double* __restrict__ x1, double* __
     return x1[C.src[sl]];
   };
   const bool io = (i & 1), jo = (j & 1);
-  double add;
+  R add;
   bool hit = true;
   if (i == 1 && j == q2) add = T(1, q2);
   else if (i == p2 && j == 1) add = T(p2, 1);
   else if (io && jo && i >= 3 && i <= iim - 1 && j >= 3 && j <= jjm - 1) add = T(i, j);
-  else if (!io && jo && i >= 2 && i <= iim - 2 && j >= 3 && j <= jjm - 1) add = 0.5 * (T(i + 1, j) + T(i - 1, j));
-  else if (!io && !jo && i >= 2 && i <= iim - 2 && j >= 2 && j <= jjm - 2) add = 0.5 * (T(i + 1, j + 1) + T(i - 1, j - 1));
-  else if (io && !jo && i >= 3 && i <= iim - 1 && j >= 2 && j <= jjm - 2) add = 0.5 * (T(i, j + 1) + T(i, j - 1));
+  else if (!io && jo && i >= 2 && i <= iim - 2 && j >= 3 && j <= jjm - 1) add = R(0.5) * (T(i + 1, j) + T(i - 1, j));
+  else if (!io && !jo && i >= 2 && i <= iim - 2 && j >= 2 && j <= jjm - 2)
+    add = R(0.5) * (T(i + 1, j + 1) + T(i - 1, j - 1));
+  else if (io && !jo && i >= 3 && i <= iim - 1 && j >= 2 && j <= jjm - 2) add = R(0.5) * (T(i, j + 1) + T(i, j - 1));
   else hit = false;
   if (hit) x2[s] = x2[s] + add;
 }
 
-__global__ void k_fill(double* __restrict__ x, size_t n, double v) {
+template <class R>
+__global__ void k_fill(R* __restrict__ x, size_t n, R v) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) x[t] = v;
 }
@@ -674,44 +805,63 @@ __global__ void k_copy(double* __restrict__ y, const double* __restrict__ x, siz
 }
 
 // ----------------------------------------------------- velocity and KE ----
-// operators.F:1402-1514 calculate_velocity: grad(chi) + k x grad(psi)
-__global__ void k_velocity(Geo G, const double* __restrict__ psi, const double* __restrict__ chi,
-                           double* __restrict__ u) {
+// operators.F:1402-1514 calculate_velocity: grad(chi) + k x grad(psi), from
+// the FP64 streamfunction and velocity potential.  Mixed path: the gradient
+// is formed on differences from the corner's first cell (its weights sum to
+// zero), in FP64 from the FP32 weights.
+template <class R>
+__global__ void k_velocity(GeoT<R> G, const double* __restrict__ psi, const double* __restrict__ chi,
+                           R* __restrict__ u) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   if (!interior(c, iim, jjm)) {
-    for (int p = 0; p < 4; ++p) u[p * ns + s] = 0.0;
+    for (int p = 0; p < 4; ++p) u[p * ns + s] = R(0);
     return;
   }
   const int i = c.i, j = c.j, k = c.k;
   const int nb[2][3] = {{s, SL(i - 1, j, k), SL(i - 1, j - 1, k)}, {s, SL(i - 1, j - 1, k), SL(i, j - 1, k)}};
 #define VW(kk, sp, ic) G.vw[((kk) + 3 * ((sp) + 2 * (ic))) * ns + s]
   for (int ic = 0; ic < 2; ++ic) {
-    const double cai = G.cai[ic * ns + s];
-    double gc[2], gp[2];
-    for (int sp = 0; sp < 2; ++sp) {
-      gc[sp] = (+VW(0, sp, ic) * chi[nb[ic][0]] + VW(1, sp, ic) * chi[nb[ic][1]] + VW(2, sp, ic) * chi[nb[ic][2]]) * cai;
-      gp[sp] = (+VW(0, sp, ic) * psi[nb[ic][0]] + VW(1, sp, ic) * psi[nb[ic][1]] + VW(2, sp, ic) * psi[nb[ic][2]]) * cai;
+    if constexpr (sizeof(R) == 8) {
+      const double cai = G.cai[ic * ns + s];
+      double gc[2], gp[2];
+      for (int sp = 0; sp < 2; ++sp) {
+        gc[sp] = (+VW(0, sp, ic) * chi[nb[ic][0]] + VW(1, sp, ic) * chi[nb[ic][1]] + VW(2, sp, ic) * chi[nb[ic][2]]) * cai;
+        gp[sp] = (+VW(0, sp, ic) * psi[nb[ic][0]] + VW(1, sp, ic) * psi[nb[ic][1]] + VW(2, sp, ic) * psi[nb[ic][2]]) * cai;
+      }
+      u[(ic + 0) * ns + s] = gc[0] + (-gp[1]);
+      u[(ic + 2) * ns + s] = gc[1] + gp[0];
+    } else {
+      const double cai = (double)G.cai[ic * ns + s];
+      const double c1 = chi[nb[ic][1]] - chi[nb[ic][0]], c2 = chi[nb[ic][2]] - chi[nb[ic][0]];
+      const double p1 = psi[nb[ic][1]] - psi[nb[ic][0]], p2 = psi[nb[ic][2]] - psi[nb[ic][0]];
+      double gc[2], gp[2];
+      for (int sp = 0; sp < 2; ++sp) {
+        const double v1 = (double)VW(1, sp, ic), v2 = (double)VW(2, sp, ic);
+        gc[sp] = (v1 * c1 + v2 * c2) * cai;
+        gp[sp] = (v1 * p1 + v2 * p2) * cai;
+      }
+      u[(ic + 0) * ns + s] = (R)(gc[0] - gp[1]);
+      u[(ic + 2) * ns + s] = (R)(gc[1] + gp[0]);
     }
-    u[(ic + 0) * ns + s] = gc[0] + (-gp[1]);
-    u[(ic + 2) * ns + s] = gc[1] + gp[0];
   }
 #undef VW
 }
 
 // operators.F:1316-1393 with the corner kinetic energy 0.5 |u|^2 (step.F:192-203)
-__global__ void k_center_ke(Geo G, const double* __restrict__ u, double* __restrict__ ke) {
+template <class R>
+__global__ void k_center_ke(GeoT<R> G, const R* __restrict__ u, R* __restrict__ ke) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
-  if (i > iim - 1 || j > jjm - 1) { ke[s] = 0.0; return; }
+  if (i > iim - 1 || j > jjm - 1) { ke[s] = R(0); return; }
   auto KC = [&](int ic, int sl) {
-    const double a = u[ic * ns + sl], b = u[(ic + 2) * ns + sl];
-    return 0.5 * (a * a + b * b);
+    const R a = u[ic * ns + sl], b = u[(ic + 2) * ns + sl];
+    return R(0.5) * (a * a + b * b);
   };
 #define SA(kk, ic, sl) G.sub_area[((kk) + 3 * (ic)) * ns + (sl)]
   const int se = SL(i + 1, j, k), sne = SL(i + 1, j + 1, k), sn = SL(i, j + 1, k);
@@ -748,16 +898,33 @@ __global__ void k_curl_div(Geo G, const double* __restrict__ u, double* __restri
 #undef UU
 }
 
-// eta = relative + f ; inv_h ; psi/chi copies (step.F:177-184, :448-452)
-__global__ void k_post_elliptic(Geo G, const double* __restrict__ rhs1, double* __restrict__ eta4) {
+// eta = relative + f (step.F:177-184; mixed path: the stored relative
+// vorticity is the solve's right-hand side itself); inv_h (step.F:448-452)
+template <class R>
+__global__ void k_post_elliptic(GeoT<R> G, const R* __restrict__ rhs1, R* __restrict__ eta4) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= G.ns) return;
-  eta4[s] = rhs1[s] + G.f[s];
+  if constexpr (sizeof(R) == 4) eta4[s] = rhs1[s];
+  else eta4[s] = rhs1[s] + G.f[s];
 }
-__global__ void k_inv_h(Geo G, const double* __restrict__ h4, double* __restrict__ inv_h) {
+template <class R>
+__global__ void k_inv_h(GeoT<R> G, const R* __restrict__ h4, R* __restrict__ inv_h) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= G.ns) return;
-  inv_h[s] = G.mask[s] == 1.0 ? 1.0 / h4[s] : 0.0;
+  inv_h[s] = G.mask[s] == R(1) ? R(1) / habs(G, h4[s]) : R(0);
+}
+
+// FP64 state -> FP32 storage of the mixed path: y = x - off (off may be null)
+__global__ void k_to_f32(size_t n, const double* __restrict__ x, const double* __restrict__ off, double c,
+                         float* __restrict__ y) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = (float)(x[t] - (off ? off[t] : 0.0) - c);
+}
+// and back: y = x + off + c
+__global__ void k_to_f64(size_t n, const float* __restrict__ x, const double* __restrict__ off, double c,
+                         double* __restrict__ y) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) y[t] = (double)x[t] + (off ? off[t] : 0.0) + c;
 }
 
 }  // namespace dev
@@ -933,23 +1100,25 @@ constexpr int CO_THREADS = 1024;
 constexpr int CO_MAXI = 2;   // items (field, slot) per thread: nvar * ns(level 3) <= 2048
 constexpr int CO_MAXLEV = 3;
 
+template <class R>
 struct CoarseArgs {
-  Lev lev[CO_MAXLEV];   // levels 1..nlev
+  LevT<R> lev[CO_MAXLEV];   // levels 1..nlev
   int nlev, nvar;
-  const double* beta;   // right-hand side at level nlev (nvar planes of ns)
-  double* alpha;        // solution at level nlev (in: warm start unless zero)
-  int zero;             // alpha at level nlev is zero on entry
+  const R* beta;   // right-hand side at level nlev (nvar planes of ns)
+  R* alpha;        // solution at level nlev (in: warm start unless zero)
+  int zero;        // alpha at level nlev is zero on entry
 };
 
 __device__ __forceinline__ int co_ns(int l) { return (2 + (1 << l)) * (2 + (1 << l)) * 10; }
 
 // relax nsw sweeps at level L: x (LDS, result) and t (LDS, scratch); b is the
 // right-hand side (LDS or global)
-__device__ void co_relax(const Lev& L, int nvar, int nsw, double* x, double* t, const double* b) {
+template <class R>
+__device__ void co_relax(const LevT<R>& L, int nvar, int nsw, R* x, R* t, const R* b) {
   const int iim = L.iim, jjm = L.jjm, ns = L.ns;
   const int nitem = nvar * ns;
   int kind[CO_MAXI], self[CO_MAXI], out_i[CO_MAXI], nb[CO_MAXI][6];
-  double w[CO_MAXI][6], wt[CO_MAXI], bv[CO_MAXI];
+  R w[CO_MAXI][6], wt[CO_MAXI], bv[CO_MAXI];
 #pragma unroll
   for (int r = 0; r < CO_MAXI; ++r) {
     const int it = threadIdx.x + r * CO_THREADS;
@@ -981,15 +1150,15 @@ __device__ void co_relax(const Lev& L, int nvar, int nsw, double* x, double* t, 
     wt[r] = L.lwt[s];
     bv[r] = b[v * ns + s];
   }
-  const double omega = 0.8;
-  double* a = x;
-  double* o = t;
+  const R omega = R(0.8);
+  R* a = x;
+  R* o = t;
   for (int sw = 0; sw < nsw; ++sw) {
 #pragma unroll
     for (int r = 0; r < CO_MAXI; ++r) {
       if (kind[r] < 0) continue;
-      if (kind[r] == 3) { o[out_i[r]] = 0.0; continue; }
-      double tt;
+      if (kind[r] == 3) { o[out_i[r]] = R(0); continue; }
+      R tt;
       if (kind[r] == 0)
         tt = wt[r] * (bv[r] - w[r][0] * a[nb[r][0]] - w[r][1] * a[nb[r][1]] - w[r][2] * a[nb[r][2]] -
                       w[r][3] * a[nb[r][3]] - w[r][4] * a[nb[r][4]] - w[r][5] * a[nb[r][5]]);
@@ -999,16 +1168,17 @@ __device__ void co_relax(const Lev& L, int nvar, int nsw, double* x, double* t, 
       else
         tt = wt[r] * (bv[r] - w[r][0] * a[nb[r][0]] - w[r][1] * a[nb[r][1]] - w[r][3] * a[nb[r][3]] -
                       w[r][4] * a[nb[r][4]] - w[r][5] * a[nb[r][5]]);
-      o[out_i[r]] = (1.0 - omega) * a[self[r]] + omega * tt;
+      o[out_i[r]] = (R(1) - omega) * a[self[r]] + omega * tt;
     }
     __syncthreads();
-    double* sw_ = a; a = o; o = sw_;
+    R* sw_ = a; a = o; o = sw_;
   }
   // nsw is even: the result is in x
 }
 
 // residual at odd-odd fine points of level F -> right-hand side of level Cc
-__device__ void co_inject(const Lev& F, const Lev& Cc, int nvar, const double* xa, const double* xb, double* x1) {
+template <class R>
+__device__ void co_inject(const LevT<R>& F, const LevT<R>& Cc, int nvar, const R* xa, const R* xb, R* x1) {
   const int nitem = nvar * Cc.ns;
   for (int it = threadIdx.x; it < nitem; it += CO_THREADS) {
     const int v = it / Cc.ns, s1 = it - v * Cc.ns;
@@ -1027,20 +1197,20 @@ __device__ void co_inject(const Lev& F, const Lev& Cc, int nvar, const double* x
     const int ns = F.ns;
     const int s = SL(i2, j2, k);
     const int* __restrict__ m = F.src;
-    const double* xav = xa + v * ns;
+    const R* xav = xa + v * ns;
     auto A = [&](int sl) { return xav[m[sl]]; };
 #define LW(n) F.lw[((n) - 1) * ns + s]
-    double r;
+    R r;
     if (in)
-      r = xb[v * ns + s] - ((LW(7) - 0.0) * A(s) + LW(1) * A(SL(i2 - 1, j2, k)) + LW(2) * A(SL(i2 - 1, j2 - 1, k)) +
+      r = xb[v * ns + s] - ((LW(7) - R(0)) * A(s) + LW(1) * A(SL(i2 - 1, j2, k)) + LW(2) * A(SL(i2 - 1, j2 - 1, k)) +
                             LW(3) * A(SL(i2, j2 - 1, k)) + LW(4) * A(SL(i2 + 1, j2, k)) +
                             LW(5) * A(SL(i2 + 1, j2 + 1, k)) + LW(6) * A(SL(i2, j2 + 1, k)));
     else if (np)
-      r = xb[v * ns + s] - ((LW(7) - 0.0) * A(s) + LW(2) * A(SL(iim, 1, k)) + LW(3) * A(SL(i2, j2 - 1, k)) +
+      r = xb[v * ns + s] - ((LW(7) - R(0)) * A(s) + LW(2) * A(SL(iim, 1, k)) + LW(3) * A(SL(i2, j2 - 1, k)) +
                             LW(4) * A(SL(i2 + 1, j2, k)) + LW(5) * A(SL(i2 + 1, j2 + 1, k)) +
                             LW(6) * A(SL(i2, j2 + 1, k)));
     else
-      r = xb[v * ns + s] - ((LW(7) - 0.0) * A(s) + LW(1) * A(SL(i2 - 1, j2, k)) + LW(2) * A(SL(1, jjm, k)) +
+      r = xb[v * ns + s] - ((LW(7) - R(0)) * A(s) + LW(1) * A(SL(i2 - 1, j2, k)) + LW(2) * A(SL(1, jjm, k)) +
                             LW(4) * A(SL(i2 + 1, j2, k)) + LW(5) * A(SL(i2 + 1, j2 + 1, k)) +
                             LW(6) * A(SL(i2, j2 + 1, k)));
 #undef LW
@@ -1050,7 +1220,8 @@ __device__ void co_inject(const Lev& F, const Lev& Cc, int nvar, const double* x
 }
 
 // coarse correction of level Cc added into level F
-__device__ void co_interp(const Lev& Cc, const Lev& F, int nvar, const double* x1, double* x2) {
+template <class R>
+__device__ void co_interp(const LevT<R>& Cc, const LevT<R>& F, int nvar, const R* x1, R* x2) {
   const int nitem = nvar * F.ns;
   for (int it = threadIdx.x; it < nitem; it += CO_THREADS) {
     const int v = it / F.ns, s = it - v * F.ns;
@@ -1059,21 +1230,21 @@ __device__ void co_interp(const Lev& Cc, const Lev& F, int nvar, const double* x
     const int i = c.i, j = c.j, k = c.k;
     const int p2 = iim - 1, q2 = jjm - 1;
     const int ci = Cc.iim, cj = Cc.jjm;
-    const double* x1v = x1 + v * Cc.ns;
+    const R* x1v = x1 + v * Cc.ns;
     auto T = [&](int ii, int jj) {
       const int a = (ii + 1) / 2, b = (jj + 1) / 2;
       return x1v[Cc.src[(a - 1) + ci * ((b - 1) + cj * (k - 1))]];
     };
     const bool io = (i & 1), jo = (j & 1);
-    double add;
+    R add;
     bool hit = true;
     if (i == 1 && j == q2) add = T(1, q2);
     else if (i == p2 && j == 1) add = T(p2, 1);
     else if (io && jo && i >= 3 && i <= iim - 1 && j >= 3 && j <= jjm - 1) add = T(i, j);
-    else if (!io && jo && i >= 2 && i <= iim - 2 && j >= 3 && j <= jjm - 1) add = 0.5 * (T(i + 1, j) + T(i - 1, j));
+    else if (!io && jo && i >= 2 && i <= iim - 2 && j >= 3 && j <= jjm - 1) add = R(0.5) * (T(i + 1, j) + T(i - 1, j));
     else if (!io && !jo && i >= 2 && i <= iim - 2 && j >= 2 && j <= jjm - 2)
-      add = 0.5 * (T(i + 1, j + 1) + T(i - 1, j - 1));
-    else if (io && !jo && i >= 3 && i <= iim - 1 && j >= 2 && j <= jjm - 2) add = 0.5 * (T(i, j + 1) + T(i, j - 1));
+      add = R(0.5) * (T(i + 1, j + 1) + T(i - 1, j - 1));
+    else if (io && !jo && i >= 3 && i <= iim - 1 && j >= 2 && j <= jjm - 2) add = R(0.5) * (T(i, j + 1) + T(i, j - 1));
     else hit = false;
     if (hit) x2[v * F.ns + s] = x2[v * F.ns + s] + add;
   }
@@ -1082,13 +1253,15 @@ __device__ void co_interp(const Lev& Cc, const Lev& F, int nvar, const double* x
 
 // level l of the bundle with constant indices (no dynamic indexing of the
 // kernel argument, which would go through scratch memory)
-__device__ __forceinline__ Lev co_lev(const CoarseArgs& P, int l) {
+template <class R>
+__device__ __forceinline__ LevT<R> co_lev(const CoarseArgs<R>& P, int l) {
   return l == 1 ? P.lev[0] : (l == 2 ? P.lev[1] : P.lev[2]);
 }
 
-__global__ void __launch_bounds__(CO_THREADS) k_coarse_cycle(CoarseArgs P) {
+template <class R>
+__global__ void __launch_bounds__(CO_THREADS) k_coarse_cycle(CoarseArgs<R> P) {
   // LDS: alpha of levels 1..nlev, rhs of levels 1..nlev-1, one scratch level
-  __shared__ double lds[2 * (160 + 360 + 1000) + 2 * (160 + 360) + 2 * 1000];
+  __shared__ R lds[2 * (160 + 360 + 1000) + 2 * (160 + 360) + 2 * 1000];
   const int nv = P.nvar, top = P.nlev;
   // offsets: alpha(l) at nv * (ns(1)+..+ns(l-1)), rhs(l) after all alphas
   auto cum = [&](int l) {   // ns(1) + .. + ns(l)
@@ -1098,18 +1271,18 @@ __global__ void __launch_bounds__(CO_THREADS) k_coarse_cycle(CoarseArgs P) {
   };
   auto AL = [&](int l) { return lds + nv * cum(l - 1); };
   auto BE = [&](int l) { return lds + nv * (cum(top) + cum(l - 1)); };
-  double* tmp = lds + nv * (cum(top) + cum(top - 1));
+  R* tmp = lds + nv * (cum(top) + cum(top - 1));
   // coarse levels start from zero (multigrid_solver.F:520-524); the top
   // level from the caller's alpha unless that is zero as well
   {
     const int nz = nv * cum(top - 1), n = nv * co_ns(top);
-    for (int t = threadIdx.x; t < nz; t += CO_THREADS) lds[t] = 0.0;
-    double* at = AL(top);
-    for (int t = threadIdx.x; t < n; t += CO_THREADS) at[t] = P.zero ? 0.0 : P.alpha[t];
+    for (int t = threadIdx.x; t < nz; t += CO_THREADS) lds[t] = R(0);
+    R* at = AL(top);
+    for (int t = threadIdx.x; t < n; t += CO_THREADS) at[t] = P.zero ? R(0) : P.alpha[t];
   }
   __syncthreads();
   for (int l = top; l >= 2; --l) {
-    const double* b = l == top ? P.beta : BE(l);
+    const R* b = l == top ? P.beta : BE(l);
     co_relax(co_lev(P, l), nv, 2, AL(l), tmp, b);
     co_inject(co_lev(P, l), co_lev(P, l - 1), nv, AL(l), b, BE(l - 1));
   }
@@ -1119,7 +1292,7 @@ __global__ void __launch_bounds__(CO_THREADS) k_coarse_cycle(CoarseArgs P) {
     co_relax(co_lev(P, l + 1), nv, 2, AL(l + 1), tmp, l + 1 == top ? P.beta : BE(l + 1));
   }
   const int n = nv * co_ns(top);
-  const double* at = AL(top);
+  const R* at = AL(top);
   for (int t = threadIdx.x; t < n; t += CO_THREADS) P.alpha[t] = at[t];
 }
 
@@ -1232,7 +1405,7 @@ __global__ void k_op_div_curl(Geo G, const double* __restrict__ hc, const double
   const int i = c.i, j = c.j, k = c.k;
   const int up[3] = {SLP(i - 1, j, k), SLP(i - 1, j - 1, k), SLP(i, j - 1, k)};
   const double pv0 = pv_of(h, eta, s);
-  const Half ht = half_edges(iim, jjm, ns, i, j, k, G.dc, G.tng, hc, u);
+  const Half<double> ht = half_edges<double>(iim, jjm, ns, i, j, k, G.dc, G.tng, hc, u);
   for (int e = 0; e < 3; ++e) out[e * ns + s] = (-ht.r[0][e] + ht.r[1][e]) * (pv_of(h, eta, up[e]) + pv0) * 0.5;
 }
 

@@ -82,6 +82,8 @@ _sig("swm_synchronize", ctypes.c_int, [ctypes.c_void_p])
 _sig("swm_step_count", ctypes.c_int, [ctypes.c_void_p])
 _sig("swm_last_vcycles", ctypes.c_int, [ctypes.c_void_p])
 _sig("swm_total_mass", ctypes.c_int, [ctypes.c_void_p, _dp])
+_sig("swm_set_solver", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int])
+_sig("swm_last_residual", ctypes.c_int, [ctypes.c_void_p, _dp, _dp])
 _sig("swm_wrap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p])
 _sig("swm_elliptic", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
 _sig("swm_multigrid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
@@ -303,6 +305,19 @@ class Model:
             raise ValueError("expected (iim, jjm, nvar, 10) arrays")
         _check(lib.swm_multigrid(self._h, b.shape[2], _ptr(b), _ptr(a)))
         return a
+
+    def set_solver(self, tol=1.0e-20, relative=False, max_cycles=12):
+        """Convergence settings of the elliptic solve; the defaults are the
+        reference's (mean squared residual <= 1e-20 within 12 V-cycles)."""
+        _check(lib.swm_set_solver(self._h, tol, 1 if relative else 0, max_cycles))
+
+    def last_residual(self):
+        """(mean squared residuals, mean squared right-hand sides) of psi and
+        chi at the end of the last elliptic solve."""
+        r = (ctypes.c_double * 2)()
+        b = (ctypes.c_double * 2)()
+        _check(lib.swm_last_residual(self._h, r, b))
+        return list(r), list(b)
 
     def total_mass(self):
         m = ctypes.c_double()
