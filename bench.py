@@ -1,19 +1,40 @@
 """Benchmark of the MI355X shallow-water dynamical core.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--g 9] [--precision fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--g 9] [--error-steps N]
 
-One "step" is one full model time step (FCT mass and tracer transport,
-vorticity/divergence ABM3 update, multigrid Poisson solve to the reference's
-convergence criterion, velocity and kinetic energy, halo fills) over the whole
-sphere.  The workload is the level-9 icosahedral grid (2,621,442 cells,
-dt = 28.125 s) with seeded synthetic fields of harmonic degree 1..160, built on
-the device; inputs are resident in HBM before the timed region starts.
+One "step" is one full model time step over the whole sphere:
 
-Prints one JSON line: throughput in cell-steps/s, the roofline of the hottest
-kernel (finest-level multigrid smoother, timed with HIP events on the model's
-stream during the timed steps) and the CPU baseline (the oracle, a
-single-threaded C restatement of the reference step, timed on one bounded
-sample of the same workload).
+- FCT transport of mass and one tracer;
+- the ABM3 vorticity/divergence update;
+- the multigrid Poisson solve of streamfunction and velocity potential to the reference's criterion;
+- velocity, kinetic energy and halo fills.
+
+The workload is the level-9 icosahedral grid (2,621,442 cells) with seeded
+synthetic fields of harmonic degree 1..160, built on the device. Inputs are
+resident in HBM before the timed region starts.
+
+**Time step.** dt = 200·2^(5−g) s, i.e. 12.5 s at g=9. That is the
+reference's own choice (200 s at 10242 cells, `swm/run.0010242/data/namelist`),
+scaled with the grid spacing. The larger dt = 450·2^(5−g) s violates the
+scheme's stability limit on these fields: at g=9 the state blows up and the
+elliptic solve fails its criterion at step 25, as the reference does at g=3.
+The work per step does not depend on dt.
+
+**Two forms are timed on the same initial state.** Both do K steps after W
+warm-up steps:
+
+- the mixed-precision step (FP32 state and stencils, FP64 defect correction): the headline `value`;
+- the FP64 step (the reference's arithmetic).
+
+Both then continue, untimed, to N steps in all (default: one simulated day,
+6912 steps at g=9). The mixed path's error against the FP64 path is reported
+per field (area-weighted relative L2 and relative max).
+
+**The JSON line carries:**
+
+- `roofline`: the hottest kernel of the headline path (the finest-level multigrid smoother), timed with HIP events on the model's stream during the timed steps.
+- `step_roofline`: the whole step against the canonical bytes per cell-step.
+- `cpu_baseline`: the oracle, a single-threaded C restatement of the reference FP64 step, timed on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -27,6 +48,57 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "cell-steps/s at g=9, mixed vs FP64 at stated err; HBM roofline %; adj/fwd ratio"
+RADIUS, GRAV, OMEGA = 6.371e6, 9.81, 7.292e-5
+LMAX = {5: 20, 6: 40, 7: 40, 8: 80, 9: 160}
+
+
+def step_bytes(precision, ntracer, vcycles_per_step, checks_per_step):
+    """Canonical algorithmic bytes per cell-step: compulsory traffic of the
+    state (S words: h 2, eta 2, div 2, u 8, ke 2, psi/chi 4, ABM3 histories 6,
+    FCT edge-flux history 9, 4 per tracer) and the metrics (M = 67 words)
+    once, plus per V-cycle 80 words (four finest sweeps of 13 words,
+    restriction 3.25, prolongation 4.5, times 4/3 over the levels) and per
+    residual check 11 words.  Mixed: FP32 state and metrics, FP64
+    psi/chi masters, FP64 defect (88 B) per correction cycle."""
+    S = 35 + 4 * ntracer
+    M = 67
+    if precision == "fp64":
+        return 8 * (S + M + 5) + 8 * (vcycles_per_step * 80 + checks_per_step * 11)
+    n_o = vcycles_per_step
+    return 4 * (S + M) + 16 + n_o * (128 + 320) + 112
+
+
+def rel_errors(grid, a, b):
+    import numpy as np
+    mask = grid.array("grid_center_mask") == 1.0
+    area = grid.array("area")
+    out = {}
+    for k in ("h", "eta", "div", "tracer"):
+        x, y = a[k], b[k]
+        mm, w = mask, area
+        if x.ndim == 4:
+            mm = np.broadcast_to(mask[:, :, None, :], x.shape)
+            w = np.broadcast_to(area[:, :, None, :], x.shape)
+        d, ww, xv = (x - y)[mm], w[mm], x[mm]
+        out[k] = {"rel_l2": float(np.sqrt(np.sum(ww * d * d) / np.sum(ww * xv * xv))),
+                  "rel_linf": float(np.abs(d).max() / np.abs(xv).max())}
+    return out
+
+
+def timed(model, warmup, steps):
+    model.step(warmup)
+    model.synchronize()
+    s0, v0, c0 = model.solver_stats()
+    model.profile(True)
+    t1 = time.perf_counter()
+    model.step(steps)
+    model.synchronize()
+    wall = time.perf_counter() - t1
+    model.profile(False)
+    s1, v1, c1 = model.solver_stats()
+    dev_ms, k_ms, k_bytes, k_n = model.timing()
+    return {"wall": wall, "device_ms": dev_ms, "kernel_ms": k_ms, "kernel_bytes": k_bytes, "kernel_launches": k_n,
+            "vcycles_per_step": (v1 - v0) / max(1, s1 - s0), "checks_per_step": (c1 - c0) / max(1, s1 - s0)}
 
 
 def main():
@@ -35,9 +107,10 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--g", type=int, default=9)
-    ap.add_argument("--precision", choices=["fp64"], default="fp64")
     ap.add_argument("--ntracer", type=int, default=1)
-    ap.add_argument("--lmax", type=int, default=None)
+    ap.add_argument("--error-steps", type=int, default=None,
+                    help="steps of the mixed-vs-FP64 error run (default: one simulated day; 0 skips it)")
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -45,56 +118,74 @@ def main():
               "(the panel-sharded multi-GPU design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    import ctypes
-
     import swm_mi355x as S
 
     g = args.g
-    dt = 450.0 * 2.0 ** (5 - g)
-    lmax = args.lmax or {5: 20, 6: 40, 7: 80, 8: 80, 9: 160}.get(g, 20)
-    radius, grav, omega = 6.371e6, 9.81, 7.292e-5
+    dt = 200.0 * 2.0 ** (5 - g)
+    lmax = LMAX.get(g, 20)
     t0 = time.time()
-    grid = S.Grid(g, radius=radius, omega=omega)
-    print(f"[bench] grid g={g} built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    model = S.Model(grid, ntracer=args.ntracer, dt=dt, grav=grav)
-    S._check(S.lib.swm_init_synthetic(model._h, 1, lmax, ctypes.c_double(40.0), ctypes.c_double(5000.0),
-                                      ctypes.c_double(50.0), 4, ctypes.c_double(500.0e3)))
-    h0 = model.field("h") if not args.no_cpu_baseline else None
-    u0 = model.field("u") if not args.no_cpu_baseline else None
-    t0tr = model.field("tracer") if not args.no_cpu_baseline else None
-    mass0 = model.total_mass()
-
-    model.step(args.warmup)
-    model.synchronize()
-    S._check(S.lib.swm_profile(model._h, 1))
-    t1 = time.perf_counter()
-    model.step(args.steps)
-    model.synchronize()
-    t2 = time.perf_counter()
-    wall = t2 - t1
-    step_ms = ctypes.c_double()
-    kms = ctypes.c_double()
-    kbytes = ctypes.c_double()
-    nlaunch = ctypes.c_int()
-    S._check(S.lib.swm_timing(model._h, ctypes.byref(step_ms), ctypes.byref(kms), ctypes.byref(kbytes),
-                              ctypes.byref(nlaunch)))
-    mass1 = model.total_mass()
+    grid = S.Grid(g, radius=RADIUS, omega=OMEGA)
     cells = grid.max_ig
-    value = cells * args.steps / wall
-    achieved = kbytes.value / (kms.value * 1e-3) / 1e9 if kms.value > 0 else None
+    print(f"[bench] grid g={g} ({cells} cells) built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    def model(precision):
+        m = S.Model(grid, ntracer=args.ntracer, dt=dt, grav=GRAV, precision=precision)
+        m.init_synthetic(seed_index=1, lmax=lmax)
+        return m
+
+    mix = model(S.SWM_MIXED)
+    ref = model(S.SWM_FP64)
+    h0, u0, tr0 = ref.field("h"), ref.field("u"), ref.field("tracer")
+    mass0 = mix.total_mass()
+    tm = timed(mix, args.warmup, args.steps)
+    print(f"[bench] mixed {tm['wall'] * 1e3 / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+    tf = timed(ref, args.warmup, args.steps)
+    print(f"[bench] fp64  {tf['wall'] * 1e3 / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+    value = cells * args.steps / tm["wall"]
+    value64 = cells * args.steps / tf["wall"]
+
+    # error of the mixed path against FP64 after the error window
+    nerr = int(round(86400.0 / dt)) if args.error_steps is None else args.error_steps
+    err = None
+    done = args.warmup + args.steps
+    if nerr > done:
+        chunk = 256
+        while done < nerr:
+            k = min(chunk, nerr - done)
+            mix.step(k)
+            ref.step(k)
+            done += k
+        mix.synchronize()
+        ref.synchronize()
+        names = ("h", "eta", "div", "tracer")
+        err = rel_errors(grid, {n: ref.field(n) for n in names}, {n: mix.field(n) for n in names})
+        err["steps"] = done
+        err["simulated_hours"] = done * dt / 3600.0
+        err["mass_drift_mixed"] = abs(mix.total_mass() - mass0) / mass0
+        print(f"[bench] error after {done} steps: h {err['h']}", file=sys.stderr, flush=True)
+
+    def kroof(t):
+        if not t["kernel_ms"]:
+            return None
+        return t["kernel_bytes"] / (t["kernel_ms"] * 1e-3) / 1e9
+
+    ach = kroof(tm)
+    ach64 = kroof(tf)
+    b_mix = step_bytes("mixed", args.ntracer, tm["vcycles_per_step"], tm["checks_per_step"])
+    b_64 = step_bytes("fp64", args.ntracer, tf["vcycles_per_step"], tf["checks_per_step"])
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.cpu_steps > 0:
         from oracle.oracle import Oracle, grid_arrays_from
-
-        orc = Oracle(grid_arrays_from(grid), args.ntracer, dt, grav)
-        orc.init_state(h0, u0, t0tr)
+        orc = Oracle(grid_arrays_from(grid), args.ntracer, dt, GRAV)
+        orc.init_state(h0, u0, tr0)
         tc = time.perf_counter()
-        orc.step(1)
+        orc.step(args.cpu_steps)
         tcpu = time.perf_counter() - tc
-        cpu = {"value": cells / tcpu, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-               "sample": f"1 step (the first, Euler-weighted) of the same g={g} workload from the same "
-                         f"initial state, {tcpu:.1f} s on one core; oracle/swm_oracle.c, gcc -O2"}
+        cpu = {"value": cells * args.cpu_steps / tcpu, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} FP64 steps (the first ones) of the same g={g} workload from the same "
+                         f"initial state, {tcpu:.1f} s on one host core; oracle/swm_oracle.c (gcc -O2), the C "
+                         f"restatement of the reference step, bit-identical to the compiled reference"}
 
     out = {
         "metric": METRIC,
@@ -103,23 +194,35 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": tm["wall"] * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 (FP64 defect correction, FP64 sums)",
         "data": "synthetic (seeded harmonic fields, degree 1..%d, built on the device)" % lmax,
-        "config": {"workload": f"icosahedral level {g} ({cells} cells), {args.precision.upper()} plain step, "
-                               f"dt={dt} s, FCT mass + {args.ntracer} tracer, multigrid to mean(resid^2)<=1e-20",
-                   "g": g, "cells": cells, "dt": dt, "ntracer": args.ntracer, "precision": args.precision,
-                   "radius": radius, "grav": grav, "omega": omega,
-                   "vcycles_last_step": model.last_vcycles,
-                   "device_ms_per_step": step_ms.value / args.steps,
-                   "mass_drift": abs(mass1 - mass0) / mass0},
-        "roofline": {"bound": "hbm", "kernel": "k_relax (finest level)", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
-                     "launches": nlaunch.value, "mean_ms": kms.value, "bytes_per_launch": kbytes.value},
+        "config": {
+            "workload": f"icosahedral level {g} ({cells} cells), mixed-precision step, dt={dt} s, FCT mass + "
+                        f"{args.ntracer} tracer, multigrid to mean(resid^2)<=1e-20",
+            "g": g, "cells": cells, "dt": dt, "ntracer": args.ntracer,
+            "radius": RADIUS, "grav": GRAV, "omega": OMEGA,
+            "fp64_value": value64,
+            "fp64_ms_per_step": tf["wall"] * 1e3 / args.steps,
+            "mixed_over_fp64": value / value64,
+            "vcycles_per_step": {"mixed": tm["vcycles_per_step"], "fp64": tf["vcycles_per_step"]},
+            "mixed_error_vs_fp64": err,
+            "adjoint": None,
+        },
+        "roofline": {"bound": "hbm", "kernel": "k_relax<float> finest level (mixed path)", "achieved": ach,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS if ach else None,
+                     "traffic": None, "launches": tm["kernel_launches"], "mean_ms": tm["kernel_ms"],
+                     "bytes_per_launch": tm["kernel_bytes"],
+                     "fp64_kernel": {"kernel": "k_relax<double> finest level", "achieved": ach64,
+                                     "frac": ach64 / HBM_PEAK_GBS if ach64 else None, "mean_ms": tf["kernel_ms"],
+                                     "bytes_per_launch": tf["kernel_bytes"]}},
+        "step_roofline": {"bytes_per_cell_step": {"mixed": b_mix, "fp64": b_64},
+                          "achieved_gbs": {"mixed": value * b_mix / 1e9, "fp64": value64 * b_64 / 1e9},
+                          "frac": {"mixed": value * b_mix / 1e9 / HBM_PEAK_GBS,
+                                   "fp64": value64 * b_64 / 1e9 / HBM_PEAK_GBS}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -89,6 +89,13 @@ swm_status swm_grid_array(const swm_grid* grid, const char* name, void* out);
  * ------------------------------------------------------------------------ */
 typedef struct swm_ctx swm_ctx;
 
+/* SWM_FP64: the reference's arithmetic, operation for operation.
+ * SWM_MIXED: FP32 state, metrics and stencils (thickness stored less its
+ * initial area-weighted mean, vorticity less the Coriolis parameter; the
+ * zero-sum gradient and Laplacian stencils formed on differences), FP32
+ * multigrid cycles inside an FP64 defect-correction loop on FP64
+ * streamfunction / velocity potential, FP64 global sums.  No counterpart in
+ * the reference. */
 enum { SWM_FP64 = 0, SWM_MIXED = 1 };
 
 typedef struct swm_config {
@@ -141,6 +148,10 @@ swm_status swm_set_solver(swm_ctx* ctx, double tol, int relative, int max_cycles
 /* Mean squared residuals at the end of the last elliptic solve, and mean
  * squared right-hand sides (relative criterion only), per field (psi, chi). */
 swm_status swm_last_residual(const swm_ctx* ctx, double resid_ms[2], double rhs_ms[2]);
+
+/* Elliptic solves, V-cycles (mixed path: correction cycles) and residual
+ * checks since the last initialisation. */
+swm_status swm_solver_stats(const swm_ctx* ctx, long long* solves, long long* vcycles, long long* checks);
 
 /* Device time of the last swm_step call and of its hottest kernel, measured
  * with HIP events on the context's stream (ms), and that kernel's mean

@@ -119,6 +119,7 @@ struct swm_ctx {
   int solver_max_cycles = 12;
   double last_resid[2] = {0.0, 0.0};
   double last_rhs_ms[2] = {0.0, 0.0};
+  long long solves = 0, vcycles_total = 0, checks_total = 0;   // since the last initialisation
   // timing
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   double last_step_ms = 0.0;
@@ -232,8 +233,12 @@ void vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_
   auto B = [&](int l) { return l == G ? const_cast<R*>(beta) : V.lev[l - 1].beta; };
   for (int l = G; l > top; --l) {
     relax<R>(C, V, l, 2, A(l), B(l), nvar, l < G || top_zero);
-    hipLaunchKernelGGL(swm::dev::k_inject<R>, dim3(nblk(V.lev[l - 2].ns), nvar), dim3(BLK), 0, C->stream,
-                       V.lev[l - 1].L, V.lev[l - 2].L, A(l), B(l), B(l - 1));
+    if (nvar == 2)
+      hipLaunchKernelGGL((swm::dev::k_inject<R, 2>), dim3(nblk(V.lev[l - 2].ns)), dim3(BLK), 0, C->stream,
+                         V.lev[l - 1].L, V.lev[l - 2].L, A(l), B(l), B(l - 1));
+    else
+      hipLaunchKernelGGL((swm::dev::k_inject<R, 1>), dim3(nblk(V.lev[l - 2].ns)), dim3(BLK), 0, C->stream,
+                         V.lev[l - 1].L, V.lev[l - 2].L, A(l), B(l), B(l - 1));
   }
   swm::dev::CoarseArgs<R> P{};
   for (int l = 1; l <= top; ++l) P.lev[l - 1] = V.lev[l - 1].L;
@@ -244,8 +249,12 @@ void vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_
   P.zero = (top < G || top_zero) ? 1 : 0;
   hipLaunchKernelGGL(swm::dev::k_coarse_cycle<R>, dim3(1), dim3(swm::dev::CO_THREADS), 0, C->stream, P);
   for (int l = top; l <= G - 1; ++l) {
-    hipLaunchKernelGGL(swm::dev::k_interp<R>, dim3(nblk(V.lev[l].ns), nvar), dim3(BLK), 0, C->stream,
-                       V.lev[l - 1].L, V.lev[l].L, A(l), A(l + 1));
+    if (nvar == 2)
+      hipLaunchKernelGGL((swm::dev::k_interp<R, 2>), dim3(nblk(V.lev[l].ns)), dim3(BLK), 0, C->stream,
+                         V.lev[l - 1].L, V.lev[l].L, A(l), A(l + 1));
+    else
+      hipLaunchKernelGGL((swm::dev::k_interp<R, 1>), dim3(nblk(V.lev[l].ns)), dim3(BLK), 0, C->stream,
+                         V.lev[l - 1].L, V.lev[l].L, A(l), A(l + 1));
     relax<R>(C, V, l + 1, 2, A(l + 1), B(l + 1), nvar, false);
   }
 }
@@ -340,6 +349,7 @@ swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
     }
   }
   C->last_vcycles = 2 + iter;
+  C->solves += 1; C->vcycles_total += 2 + iter; C->checks_total += 1 + iter;
   if (any) {
     swm::set_error("elliptic solve did not converge (residual above the criterion after " +
                    std::to_string(C->solver_max_cycles) + " V-cycles; the reference stops here, "
@@ -350,16 +360,20 @@ swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
 }
 
 // Mixed path: the same solve as FP64 defect correction.  The FP32 right-hand
-// side (mean removed with FP64 sums) is the problem; the FP64 solution xout is
-// the iterate.  Each cycle forms the defect r = rhs - L(xout) with the FP64
-// weights (k_defect, which also sums r^2 for the convergence test), runs one
-// FP32 V-cycle on L(e) = r from e = 0, and adds e to xout.  As in the FP64
-// path: two cycles, then the reference's criterion mean(r^2) <= 1e-20 checked
-// before each further cycle, at most 12 cycles in all.
+// side (mean removed with FP64 sums) is the problem; the FP64 solution xout
+// is the iterate.  Each cycle forms the FP64 defect r = rhs - L(xout) of all
+// open fields in one pass (k_defect, which also sums r^2 for the convergence
+// test), runs one FP32 V-cycle on L(e) = r from e = 0, and adds e to xout.
+// The defect is formed afresh in FP64 every cycle, so the FP32 cycles only
+// ever see a small, accurately known residual.  As in the FP64 path: two
+// cycles, then the criterion before each further cycle, at most
+// solver_max_cycles cycles.
 swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
   const int ns = C->ns;
   View<float>& V = C->m;
   const swm::dev::Geo& G64 = C->d.geo;
+  float* r = C->mx_r;
+  float* e = C->mx_e;
   gather<double>(C, C->face, {xout, xout + ns});
   gather<float>(C, C->face, {rhs, rhs + ns});
   // area-weighted means in FP64 (elliptic_solver.F:80-92)
@@ -368,17 +382,20 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
   hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(256), 0, C->stream, C->part, C->npart, C->sums);
   hipLaunchKernelGGL(swm::dev::k_sub_mean<float>, dim3(nblk(ns)), dim3(BLK), 0, C->stream, ns, C->sums,
                      C->total_area, rhs, rhs + ns);
-  // defect of fields v0 (and v1 when nv = 2) into mx_r, with partial sums
-  auto defect = [&](int nv, int v0, int v1) {
-    hipLaunchKernelGGL(swm::dev::k_defect, dim3(C->npart, nv), dim3(BLK), 0, C->stream, G64, xout + (size_t)v0 * ns,
-                       rhs + (size_t)v0 * ns, C->mx_r + (size_t)v0 * ns, xout + (size_t)v1 * ns,
-                       rhs + (size_t)v1 * ns, C->mx_r + (size_t)v1 * ns, C->part);
+  swm_status st0 = rhs_mean_squares<float>(C, rhs, 2);
+  if (st0) return st0;
+  const unsigned nbd = (unsigned)C->npart;
+  // defect of fields v0.. (nv of them) into r, with the partial sums of r^2
+  auto defect = [&](int nv, int v0) {
+    hipLaunchKernelGGL((swm::dev::k_defect<false, true>), dim3(nbd), dim3(BLK), 0, C->stream, G64, nv,
+                       xout + (size_t)v0 * ns, e + (size_t)v0 * ns, rhs + (size_t)v0 * ns, r + (size_t)v0 * ns,
+                       C->part);
   };
-  // one FP32 correction cycle on fields v0.. (nv of them, consecutive or one)
+  // one FP32 correction cycle on those fields
   auto correct = [&](int nv, int v0) {
-    vcycle<float>(C, V, nv, C->mx_e + (size_t)v0 * ns, C->mx_r + (size_t)v0 * ns, true);
+    vcycle<float>(C, V, nv, e + (size_t)v0 * ns, r + (size_t)v0 * ns, true);
     hipLaunchKernelGGL(swm::dev::k_correct, dim3(nblk((size_t)nv * ns)), dim3(BLK), 0, C->stream, (size_t)nv * ns,
-                       C->mx_e + (size_t)v0 * ns, xout + (size_t)v0 * ns);
+                       e + (size_t)v0 * ns, xout + (size_t)v0 * ns);
     for (int v = v0; v < v0 + nv; ++v) gather<double>(C, C->face, {xout + (size_t)v * ns});
   };
   auto norms = [&](int nv, double* out) -> swm_status {
@@ -389,13 +406,11 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
     for (int b = 0; b < nv; ++b) out[b] = raw[b] / (double)C->max_ig;
     return SWM_OK;
   };
-  swm_status st0 = rhs_mean_squares<float>(C, rhs, 2);
-  if (st0) return st0;
   for (int it = 0; it < 2; ++it) {
-    defect(2, 0, 1);
+    defect(2, 0);
     correct(2, 0);
   }
-  defect(2, 0, 1);
+  defect(2, 0);
   double nrm[2];
   swm_status st = norms(2, nrm);
   if (st) return st;
@@ -408,20 +423,21 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
   bool any = !bad.empty();
   while (any && iter < C->solver_max_cycles - 2) {
     ++iter;
-    // mx_r holds the defect of every field still in bad
-    if (bad.size() == 2) correct(2, 0);
-    else correct(1, bad[0]);
-    defect((int)bad.size(), bad[0], bad.back());
+    // r holds the defect of every field still open
+    const int nv = (int)bad.size(), v0 = bad[0];
+    correct(nv, v0);
+    defect(nv, v0);
     double nb[2];
-    st = norms((int)bad.size(), nb);
+    st = norms(nv, nb);
     if (st) return st;
     any = false;
-    for (size_t b = 0; b < bad.size(); ++b) {
+    for (int b = 0; b < nv; ++b) {
       C->last_resid[bad[b]] = nb[b];
       if (!converged(C, bad[b], nb[b])) any = true;
     }
   }
   C->last_vcycles = 2 + iter;
+  C->solves += 1; C->vcycles_total += 2 + iter; C->checks_total += 1 + iter;
   if (any) {
     swm::set_error("elliptic solve did not converge (residual above the criterion after " +
                    std::to_string(C->solver_max_cycles) + " correction cycles)");
@@ -550,6 +566,7 @@ struct SplitMix {
 
 // zero the whole state (the reference's module arrays start at zero)
 void reset_state(swm_ctx* C) {
+  C->solves = C->vcycles_total = C->checks_total = 0;
   fill<double>(C, C->d.state_begin, C->d.state_words, 0.0);
   if (C->precision == SWM_MIXED) fill<float>(C, C->m.state_begin, C->m.state_words, 0.0f);
   fill<double>(C, C->hs, C->ns, 0.0);
@@ -1113,6 +1130,14 @@ swm_status swm_last_residual(const swm_ctx* C, double* resid_ms, double* rhs_ms)
     if (resid_ms) resid_ms[v] = C->last_resid[v];
     if (rhs_ms) rhs_ms[v] = C->last_rhs_ms[v];
   }
+  return SWM_OK;
+}
+
+swm_status swm_solver_stats(const swm_ctx* C, long long* solves, long long* vcycles, long long* checks) {
+  if (!C) return SWM_ERR_ARG;
+  if (solves) *solves = C->solves;
+  if (vcycles) *vcycles = C->vcycles_total;
+  if (checks) *checks = C->checks_total;
   return SWM_OK;
 }
 

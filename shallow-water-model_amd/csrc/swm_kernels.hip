@@ -616,36 +616,49 @@ __global__ void __launch_bounds__(256) k_resid2_partial(Geo G, const double* __r
   if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
 }
 
-// Mixed path, FP64 defect of the Poisson problem for field blockIdx.y:
-// r = rhs - L(x) with the FP64 weights and the FP64 solution x (all slots;
-// halo slots get 0), stored in FP32 as the right-hand side of the next
-// correction cycle, and per-block partial sums of r^2 at real cells.
-__global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict__ x0, const float* __restrict__ b0,
-                                                float* __restrict__ r0, const double* __restrict__ x1,
-                                                const float* __restrict__ b1, float* __restrict__ r1,
-                                                double* __restrict__ part) {
-  __shared__ double sh[256];
+// Mixed path, FP64 defect of the Poisson problem, all nv fields (planes ns
+// apart) in one thread so the FP64 weights are read once: d = rhs - L(x + e)
+// at real cells (0 elsewhere) with the FP64 weights, the FP64 iterate x and,
+// WITH_E, the FP32 correction e (halos filled); WRITE_R stores d in FP32 as
+// the right-hand side of the correction cycles.  Per-block partial sums of
+// d^2 for each field (the convergence test, elliptic_solver.F:203-239).
+template <bool WITH_E, bool WRITE_R>
+__global__ void __launch_bounds__(256) k_defect(Geo G, int nv, const double* __restrict__ x,
+                                                const float* __restrict__ e, const float* __restrict__ b,
+                                                float* __restrict__ r, double* __restrict__ part) {
+  __shared__ double sh[2][256];
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const double* x = blockIdx.y == 0 ? x0 : x1;
-  const float* b = blockIdx.y == 0 ? b0 : b1;
-  float* r = blockIdx.y == 0 ? r0 : r1;
-  double acc = 0.0;
+  double acc[2] = {0.0, 0.0};
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
     const IJK c = ijk_of(s, iim, jjm);
-    auto X = [&](int sl) { return x[sl]; };
-    const double w = lap_at<double>(iim, jjm, ns, c, G.lw, X);
     const double m = G.mask[s];
-    const double d = m != 0.0 ? (double)b[s] - w : 0.0;
-    r[s] = (float)d;
-    acc += (d * d) * m;
+    double d[2] = {0.0, 0.0};
+    for (int v = 0; v < 2; ++v) {
+      if (v >= nv) break;
+      const double* xv = x + (size_t)v * ns;
+      const float* ev = e + (size_t)v * ns;
+      auto X = [&](int sl) { return WITH_E ? xv[sl] + (double)ev[sl] : xv[sl]; };
+      const double w = lap_at<double>(iim, jjm, ns, c, G.lw, X);
+      d[v] = m != 0.0 ? (double)b[(size_t)v * ns + s] - w : 0.0;
+    }
+    for (int v = 0; v < 2; ++v) {
+      if (v >= nv) break;
+      if (WRITE_R) r[(size_t)v * ns + s] = (float)d[v];
+      acc[v] += (d[v] * d[v]) * m;
+    }
   }
-  sh[threadIdx.x] = acc;
+  sh[0][threadIdx.x] = acc[0];
+  sh[1][threadIdx.x] = acc[1];
   __syncthreads();
   for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    if ((int)threadIdx.x < o) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+  if (threadIdx.x == 0)
+    for (int v = 0; v < nv; ++v) part[v * gridDim.x + blockIdx.x] = sh[v][0];
 }
 
 // x += e (FP64 solution, FP32 correction), nvar planes of ns
@@ -723,18 +736,20 @@ __global__ void __launch_bounds__(256) k_relax(LevT<R> L, const R* __restrict__ 
 
 // residual at the odd-odd fine points becomes the coarse right-hand side
 // (multigrid_solver.F:684-886 inject_residual); one thread per coarse slot
-template <class R>
+// and NV fields (planes ns apart), so the weights are read once.  Fine
+// neighbours of interior coarse points other than the outermost ring are
+// interior themselves and are read without the halo map.
+template <class R, int NV>
 __global__ void k_inject(LevT<R> F, LevT<R> C, const R* __restrict__ xa, const R* __restrict__ xb, R* __restrict__ x1) {
   int iim = C.iim, jjm = C.jjm;
   const int s1 = blockIdx.x * blockDim.x + threadIdx.x;
   if (s1 >= C.ns) return;
-  const int v = blockIdx.y;
-  xa += (size_t)v * F.ns; xb += (size_t)v * F.ns; x1 += (size_t)v * C.ns;
   const IJK c = ijk_of(s1, iim, jjm);
   const bool in = c.i >= 2 && c.i <= iim - 1 && c.j >= 2 && c.j <= jjm - 1;
   const bool np = c.k == 1 && c.i == 1 && c.j == jjm - 1;
   const bool sp = c.k == 10 && c.i == iim - 1 && c.j == 1;
   if (!(in || np || sp)) return;
+  const bool deep = c.i >= 3 && c.i <= iim - 2 && c.j >= 3 && c.j <= jjm - 2;
   const int k = c.k;
   int i2, j2;
   if (in) { i2 = 2 * c.i - 1; j2 = 2 * c.j - 1; }
@@ -744,54 +759,77 @@ __global__ void k_inject(LevT<R> F, LevT<R> C, const R* __restrict__ xa, const R
   const int ns = F.ns;
   const int s = SL(i2, j2, k);
   const int* __restrict__ m = F.src;
-  auto A = [&](int sl) { return xa[m[sl]]; };
-#define LW(n) F.lw[((n) - 1) * ns + s]
-  R r;
-  if (in)
-    r = xb[s] - ((LW(7) - R(0)) * A(s) + LW(1) * A(SL(i2 - 1, j2, k)) + LW(2) * A(SL(i2 - 1, j2 - 1, k)) +
-                 LW(3) * A(SL(i2, j2 - 1, k)) + LW(4) * A(SL(i2 + 1, j2, k)) + LW(5) * A(SL(i2 + 1, j2 + 1, k)) +
-                 LW(6) * A(SL(i2, j2 + 1, k)));
-  else if (np)
-    r = xb[s] - ((LW(7) - R(0)) * A(s) + LW(2) * A(SL(iim, 1, k)) + LW(3) * A(SL(i2, j2 - 1, k)) +
-                 LW(4) * A(SL(i2 + 1, j2, k)) + LW(5) * A(SL(i2 + 1, j2 + 1, k)) + LW(6) * A(SL(i2, j2 + 1, k)));
-  else
-    r = xb[s] - ((LW(7) - R(0)) * A(s) + LW(1) * A(SL(i2 - 1, j2, k)) + LW(2) * A(SL(1, jjm, k)) +
-                 LW(4) * A(SL(i2 + 1, j2, k)) + LW(5) * A(SL(i2 + 1, j2 + 1, k)) + LW(6) * A(SL(i2, j2 + 1, k)));
-#undef LW
-  x1[s1] = r;
+  // the seven fine slots, mapped (pole forms leave out one neighbour)
+  int q[7];
+  q[0] = s;
+  if (in) {
+    q[1] = SL(i2 - 1, j2, k); q[2] = SL(i2 - 1, j2 - 1, k); q[3] = SL(i2, j2 - 1, k);
+  } else if (np) {
+    q[1] = s; q[2] = SL(iim, 1, k); q[3] = SL(i2, j2 - 1, k);
+  } else {
+    q[1] = SL(i2 - 1, j2, k); q[2] = SL(1, jjm, k); q[3] = s;
+  }
+  q[4] = SL(i2 + 1, j2, k); q[5] = SL(i2 + 1, j2 + 1, k); q[6] = SL(i2, j2 + 1, k);
+  if (!deep) {
+#pragma unroll
+    for (int n = 0; n < 7; ++n) q[n] = m[q[n]];
+  }
+  R w[7];
+#pragma unroll
+  for (int n = 0; n < 7; ++n) w[n] = F.lw[n * ns + s];
+  // w[n] is l_weights(n+1); q[0] is the centre, q[1..6] the neighbours 1..6
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const R* __restrict__ a = xa + (size_t)v * F.ns;
+    const R bv = xb[(size_t)v * F.ns + s];
+    R r;
+    if (in)
+      r = bv - ((w[6] - R(0)) * a[q[0]] + w[0] * a[q[1]] + w[1] * a[q[2]] + w[2] * a[q[3]] + w[3] * a[q[4]] +
+                w[4] * a[q[5]] + w[5] * a[q[6]]);
+    else if (np)
+      r = bv - ((w[6] - R(0)) * a[q[0]] + w[1] * a[q[2]] + w[2] * a[q[3]] + w[3] * a[q[4]] + w[4] * a[q[5]] +
+                w[5] * a[q[6]]);
+    else
+      r = bv - ((w[6] - R(0)) * a[q[0]] + w[0] * a[q[1]] + w[1] * a[q[2]] + w[3] * a[q[4]] + w[4] * a[q[5]] +
+                w[5] * a[q[6]]);
+    x1[(size_t)v * C.ns + s1] = r;
+  }
 }
 
 // coarse correction added at coincident points, two-point averages between
-// (multigrid_solver.F:888-1054 interpolate); one thread per fine slot
-template <class R>
+// (multigrid_solver.F:888-1054 interpolate); one thread per fine slot and NV
+// fields.  Coarse reads go through the map only on the coarse halo ring.
+template <class R, int NV>
 __global__ void k_interp(LevT<R> C, LevT<R> F, const R* __restrict__ x1, R* __restrict__ x2) {
-  int iim = F.iim, jjm = F.jjm;
+  const int iim = F.iim, jjm = F.jjm;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= F.ns) return;
-  const int v = blockIdx.y;
-  x1 += (size_t)v * C.ns; x2 += (size_t)v * F.ns;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
   const int p2 = iim - 1, q2 = jjm - 1;
   // temp(2a-1, 2b-1) = coarse (a, b) for a, b = 1..iim1-1 (halo reads mapped)
   const int ci = C.iim, cj = C.jjm;
-  auto T = [&](int ii, int jj) {
+  auto CS = [&](int ii, int jj) {
     const int a = (ii + 1) / 2, b = (jj + 1) / 2;
     const int sl = (a - 1) + ci * ((b - 1) + cj * (k - 1));
-    return x1[C.src[sl]];
+    return (a == 1 || a == ci || b == 1 || b == cj) ? C.src[sl] : sl;
   };
   const bool io = (i & 1), jo = (j & 1);
-  R add;
-  bool hit = true;
-  if (i == 1 && j == q2) add = T(1, q2);
-  else if (i == p2 && j == 1) add = T(p2, 1);
-  else if (io && jo && i >= 3 && i <= iim - 1 && j >= 3 && j <= jjm - 1) add = T(i, j);
-  else if (!io && jo && i >= 2 && i <= iim - 2 && j >= 3 && j <= jjm - 1) add = R(0.5) * (T(i + 1, j) + T(i - 1, j));
-  else if (!io && !jo && i >= 2 && i <= iim - 2 && j >= 2 && j <= jjm - 2)
-    add = R(0.5) * (T(i + 1, j + 1) + T(i - 1, j - 1));
-  else if (io && !jo && i >= 3 && i <= iim - 1 && j >= 2 && j <= jjm - 2) add = R(0.5) * (T(i, j + 1) + T(i, j - 1));
-  else hit = false;
-  if (hit) x2[s] = x2[s] + add;
+  int c0 = -1, c1 = -1;
+  if (i == 1 && j == q2) c0 = CS(1, q2);
+  else if (i == p2 && j == 1) c0 = CS(p2, 1);
+  else if (io && jo && i >= 3 && i <= iim - 1 && j >= 3 && j <= jjm - 1) c0 = CS(i, j);
+  else if (!io && jo && i >= 2 && i <= iim - 2 && j >= 3 && j <= jjm - 1) { c0 = CS(i + 1, j); c1 = CS(i - 1, j); }
+  else if (!io && !jo && i >= 2 && i <= iim - 2 && j >= 2 && j <= jjm - 2) { c0 = CS(i + 1, j + 1); c1 = CS(i - 1, j - 1); }
+  else if (io && !jo && i >= 3 && i <= iim - 1 && j >= 2 && j <= jjm - 2) { c0 = CS(i, j + 1); c1 = CS(i, j - 1); }
+  if (c0 < 0) return;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const R* __restrict__ xc = x1 + (size_t)v * C.ns;
+    const R add = c1 < 0 ? xc[c0] : R(0.5) * (xc[c0] + xc[c1]);
+    R* __restrict__ xf = x2 + (size_t)v * F.ns;
+    xf[s] = xf[s] + add;
+  }
 }
 
 template <class R>

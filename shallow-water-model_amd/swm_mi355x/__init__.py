@@ -84,6 +84,11 @@ _sig("swm_last_vcycles", ctypes.c_int, [ctypes.c_void_p])
 _sig("swm_total_mass", ctypes.c_int, [ctypes.c_void_p, _dp])
 _sig("swm_set_solver", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int])
 _sig("swm_last_residual", ctypes.c_int, [ctypes.c_void_p, _dp, _dp])
+_sig("swm_solver_stats", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 3)
+_sig("swm_profile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
+_sig("swm_timing", ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _c_int_p])
+_sig("swm_init_synthetic", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double])
 _sig("swm_wrap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p])
 _sig("swm_elliptic", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
 _sig("swm_multigrid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
@@ -318,6 +323,27 @@ class Model:
         b = (ctypes.c_double * 2)()
         _check(lib.swm_last_residual(self._h, r, b))
         return list(r), list(b)
+
+    def solver_stats(self):
+        """(elliptic solves, V-cycles, residual checks) since initialisation."""
+        v = [ctypes.c_longlong() for _ in range(3)]
+        _check(lib.swm_solver_stats(self._h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def init_synthetic(self, seed_index=0, lmax=20, umax=40.0, h0=5000.0, h_rms=50.0, nbumps=4,
+                       bump_radius=500.0e3):
+        """Seeded synthetic initial state built on the device (swm_init_synthetic)."""
+        _check(lib.swm_init_synthetic(self._h, seed_index, lmax, umax, h0, h_rms, nbumps, bump_radius))
+
+    def profile(self, enable=True):
+        _check(lib.swm_profile(self._h, 1 if enable else 0))
+
+    def timing(self):
+        """(device ms of the last step call, mean ms, bytes and launch count of
+        the profiled kernel: the finest-level smoother)."""
+        a, b, c, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        _check(lib.swm_timing(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(n)))
+        return a.value, b.value, c.value, n.value
 
     def total_mass(self):
         m = ctypes.c_double()
