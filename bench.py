@@ -169,6 +169,12 @@ def main():
             return None
         return t["kernel_bytes"] / (t["kernel_ms"] * 1e-3) / 1e9
 
+    # HBM bytes per launch of the roofline kernels, measured with rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes of this workload (profiles/traffic_g9_pmc.md)
+    traffic = {}
+    tpath = os.path.join(ROOT, "profiles", "traffic_g9_pmc.json")
+    if g == 9 and os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("kernels", {})
     ach = kroof(tm)
     ach64 = kroof(tf)
     b_mix = step_bytes("mixed", args.ntracer, tm["vcycles_per_step"], tm["checks_per_step"])
@@ -214,10 +220,14 @@ def main():
         },
         "roofline": {"bound": "hbm", "kernel": "k_relax<float> finest level (mixed path)", "achieved": ach,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS if ach else None,
-                     "traffic": None, "launches": tm["kernel_launches"], "mean_ms": tm["kernel_ms"],
+                     "traffic": traffic.get("k_relax<float, 2, false>", {}).get("traffic_bytes"),
+                     "traffic_note": "HBM bytes per two-field sweep, rocprofv3 FETCH_SIZE x 1.965 + WRITE_SIZE "
+                                     "(scales from a plain copy kernel), profiles/traffic_g9_pmc.md",
+                     "launches": tm["kernel_launches"], "mean_ms": tm["kernel_ms"],
                      "bytes_per_launch": tm["kernel_bytes"],
                      "fp64_kernel": {"kernel": "k_relax<double> finest level", "achieved": ach64,
                                      "frac": ach64 / HBM_PEAK_GBS if ach64 else None, "mean_ms": tf["kernel_ms"],
+                                     "traffic": traffic.get("k_relax<double, 2, false>", {}).get("traffic_bytes"),
                                      "bytes_per_launch": tf["kernel_bytes"]}},
         "step_roofline": {"bytes_per_cell_step": {"mixed": b_mix, "fp64": b_64},
                           "achieved_gbs": {"mixed": value * b_mix / 1e9, "fp64": value64 * b_64 / 1e9},

@@ -103,6 +103,8 @@ struct swm_ctx {
   float* mx_e = nullptr;   // mixed elliptic: FP32 correction (2 planes)
   double h0 = 0.0;         // mixed path: thickness offset
   double* xout = nullptr;  // streamfunction and velocity potential (FP64, both paths)
+  double* xalt = nullptr;  // mixed path: second FP64 iterate of the defect correction
+  double* part_slots = nullptr;   // per-block partial sums, one block per 256 slots, two fields
   double* r2 = nullptr;
   double *part = nullptr, *sums = nullptr;
   double* host_sums = nullptr;   // pinned
@@ -310,12 +312,15 @@ swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
   for (int it = 0; it < 2; ++it) vcycle<double>(C, V, nvar, xout, rhs);
   for (int v = 0; v < nvar; ++v) gather<double>(C, C->face, {xout + (size_t)v * ns});
   auto norms = [&](const std::vector<int>& vars, double* out) -> swm_status {
-    const int v0 = vars[0], v1 = vars.back();
-    hipLaunchKernelGGL(swm::dev::k_resid2_partial, dim3(C->npart, (unsigned)vars.size()), dim3(BLK), 0, C->stream,
-                       V.geo, xout + (size_t)v0 * ns, rhs + (size_t)v0 * ns, xout + (size_t)v1 * ns,
-                       rhs + (size_t)v1 * ns, C->part);
-    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3((unsigned)vars.size()), dim3(256), 0, C->stream, C->part, C->npart,
-                       C->sums);
+    const int v0 = vars[0];
+    const unsigned nb = nblk(ns);
+    if (vars.size() == 2)
+      hipLaunchKernelGGL(swm::dev::k_resid2<2>, dim3(nb), dim3(BLK), 0, C->stream, V.geo, xout, rhs, C->part_slots);
+    else
+      hipLaunchKernelGGL(swm::dev::k_resid2<1>, dim3(nb), dim3(BLK), 0, C->stream, V.geo, xout + (size_t)v0 * ns,
+                         rhs + (size_t)v0 * ns, C->part_slots);
+    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3((unsigned)vars.size()), dim3(256), 0, C->stream, C->part_slots,
+                       (int)nb, C->sums);
     double raw[2];
     swm_status st = read_sums(C, (int)vars.size(), raw);
     if (st) return st;
@@ -360,20 +365,21 @@ swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
 }
 
 // Mixed path: the same solve as FP64 defect correction.  The FP32 right-hand
-// side (mean removed with FP64 sums) is the problem; the FP64 solution xout
-// is the iterate.  Each cycle forms the FP64 defect r = rhs - L(xout) of all
-// open fields in one pass (k_defect, which also sums r^2 for the convergence
-// test), runs one FP32 V-cycle on L(e) = r from e = 0, and adds e to xout.
-// The defect is formed afresh in FP64 every cycle, so the FP32 cycles only
-// ever see a small, accurately known residual.  As in the FP64 path: two
-// cycles, then the criterion before each further cycle, at most
-// solver_max_cycles cycles.
+// side (mean removed with FP64 sums) is the problem; the FP64 iterate lives
+// in xout / xalt by turns.  A cycle runs one FP32 V-cycle on L(e) = r from
+// e = 0, fills the halos of e, and then one pass (k_defect) adds e to the
+// iterate and forms its FP64 defect r = rhs - L(x) with the sums of r^2 for
+// the convergence test.  The defect is formed afresh in FP64 every cycle, so
+// the FP32 cycles only ever see a small, accurately known residual.  As in
+// the FP64 path: two cycles, then the criterion before each further cycle, at
+// most solver_max_cycles cycles.
 swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
   const int ns = C->ns;
   View<float>& V = C->m;
   const swm::dev::Geo& G64 = C->d.geo;
   float* r = C->mx_r;
   float* e = C->mx_e;
+  const unsigned nb = nblk(ns);
   gather<double>(C, C->face, {xout, xout + ns});
   gather<float>(C, C->face, {rhs, rhs + ns});
   // area-weighted means in FP64 (elliptic_solver.F:80-92)
@@ -384,33 +390,47 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
                      C->total_area, rhs, rhs + ns);
   swm_status st0 = rhs_mean_squares<float>(C, rhs, 2);
   if (st0) return st0;
-  const unsigned nbd = (unsigned)C->npart;
-  // defect of fields v0.. (nv of them) into r, with the partial sums of r^2
-  auto defect = [&](int nv, int v0) {
-    hipLaunchKernelGGL((swm::dev::k_defect<false, true>), dim3(nbd), dim3(BLK), 0, C->stream, G64, nv,
-                       xout + (size_t)v0 * ns, e + (size_t)v0 * ns, rhs + (size_t)v0 * ns, r + (size_t)v0 * ns,
-                       C->part);
+  // current iterate of each field (xout or xalt)
+  double* cur[2] = {xout, xout + ns};
+  double* oth[2] = {C->xalt, C->xalt + ns};
+  auto launch_defect = [&](int nv, int v0, bool corr) {
+    const size_t o = (size_t)v0 * ns;
+    if (nv == 2) {
+      if (corr)
+        hipLaunchKernelGGL((swm::dev::k_defect<2, true>), dim3(nb), dim3(BLK), 0, C->stream, G64, cur[0], e, oth[0],
+                           rhs, r, C->part_slots);
+      else
+        hipLaunchKernelGGL((swm::dev::k_defect<2, false>), dim3(nb), dim3(BLK), 0, C->stream, G64, cur[0], e, oth[0],
+                           rhs, r, C->part_slots);
+    } else {
+      if (corr)
+        hipLaunchKernelGGL((swm::dev::k_defect<1, true>), dim3(nb), dim3(BLK), 0, C->stream, G64, cur[v0], e + o,
+                           oth[v0], rhs + o, r + o, C->part_slots);
+      else
+        hipLaunchKernelGGL((swm::dev::k_defect<1, false>), dim3(nb), dim3(BLK), 0, C->stream, G64, cur[v0], e + o,
+                           oth[v0], rhs + o, r + o, C->part_slots);
+    }
+    if (corr)
+      for (int v = v0; v < v0 + nv; ++v) std::swap(cur[v], oth[v]);
   };
-  // one FP32 correction cycle on those fields
-  auto correct = [&](int nv, int v0) {
+  // one FP32 correction cycle on fields v0.. (nv of them: both, or one)
+  auto cycle = [&](int nv, int v0) {
+    // the two fields' planes are contiguous: cur[0] + ns == cur[1] holds while they move together
     vcycle<float>(C, V, nv, e + (size_t)v0 * ns, r + (size_t)v0 * ns, true);
-    hipLaunchKernelGGL(swm::dev::k_correct, dim3(nblk((size_t)nv * ns)), dim3(BLK), 0, C->stream, (size_t)nv * ns,
-                       e + (size_t)v0 * ns, xout + (size_t)v0 * ns);
-    for (int v = v0; v < v0 + nv; ++v) gather<double>(C, C->face, {xout + (size_t)v * ns});
+    if (nv == 2) gather<float>(C, C->face, {e, e + ns});
+    else gather<float>(C, C->face, {e + (size_t)v0 * ns});
+    launch_defect(nv, v0, true);
   };
   auto norms = [&](int nv, double* out) -> swm_status {
-    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(nv), dim3(256), 0, C->stream, C->part, C->npart, C->sums);
+    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(nv), dim3(256), 0, C->stream, C->part_slots, (int)nb, C->sums);
     double raw[2];
     swm_status st = read_sums(C, nv, raw);
     if (st) return st;
     for (int b = 0; b < nv; ++b) out[b] = raw[b] / (double)C->max_ig;
     return SWM_OK;
   };
-  for (int it = 0; it < 2; ++it) {
-    defect(2, 0);
-    correct(2, 0);
-  }
-  defect(2, 0);
+  launch_defect(2, 0, false);
+  for (int it = 0; it < 2; ++it) cycle(2, 0);
   double nrm[2];
   swm_status st = norms(2, nrm);
   if (st) return st;
@@ -423,19 +443,27 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
   bool any = !bad.empty();
   while (any && iter < C->solver_max_cycles - 2) {
     ++iter;
-    // r holds the defect of every field still open
     const int nv = (int)bad.size(), v0 = bad[0];
-    correct(nv, v0);
-    defect(nv, v0);
-    double nb[2];
-    st = norms(nv, nb);
+    if (nv == 2 && cur[1] != cur[0] + ns) {
+      // the fields ended on different buffers: bring field 1 next to field 0
+      hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, C->stream, cur[0] + ns, cur[1], (size_t)ns);
+      oth[1] = cur[1];
+      cur[1] = cur[0] + ns;
+    }
+    cycle(nv, v0);
+    double nbv[2];
+    st = norms(nv, nbv);
     if (st) return st;
     any = false;
     for (int b = 0; b < nv; ++b) {
-      C->last_resid[bad[b]] = nb[b];
-      if (!converged(C, bad[b], nb[b])) any = true;
+      C->last_resid[bad[b]] = nbv[b];
+      if (!converged(C, bad[b], nbv[b])) any = true;
     }
   }
+  // the result belongs in xout
+  for (int v = 0; v < 2; ++v)
+    if (cur[v] != xout + (size_t)v * ns)
+      hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, C->stream, xout + (size_t)v * ns, cur[v], (size_t)ns);
   C->last_vcycles = 2 + iter;
   C->solves += 1; C->vcycles_total += 2 + iter; C->checks_total += 1 + iter;
   if (any) {
@@ -763,7 +791,8 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
   make_table(C, Gh.face, C->ns, C->face);
   make_table(C, Gh.vrtx, C->ns, C->vrtx);
   make_table(C, Gh.edge, C->ns, C->edge);
-  C->xout = take(2 * ns); C->r2 = take(ns);
+  C->xout = take(2 * ns); C->r2 = take(ns); C->xalt = take(2 * ns);
+  C->part_slots = take(2 * (size_t)nblk(ns));
   C->part = take(2 * (size_t)C->npart); C->sums = take(64);
   layout_view<double>(C, C->d, nullptr);
   C->d.geo = swm::dev::Geo{C->iim, C->jjm, C->ns, C->area, C->area_inv, C->cai, C->sub_area, C->dc, C->nrm, C->tng,

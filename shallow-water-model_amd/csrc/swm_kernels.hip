@@ -42,6 +42,23 @@ __device__ __forceinline__ bool interior(const IJK& c, int iim, int jjm) {
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 
+
+// ------------------------------------------------------ XCD-aware slots ---
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Stencil kernels therefore renumber their blocks so that XCD x works on
+// one contiguous range of blocks: neighbouring rows (j-1, j+1) are then
+// mostly fetched into the same L2 instead of into three different ones.
+// Bijective on [0, gridDim.x) for any block count.
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int per = (nb + 7) >> 3;          // blocks of the fullest XCDs
+  const int full = nb - 8 * (per - 1);    // XCDs that get 'per' blocks
+  const int x = b & 7, q = b >> 3;
+  const int start = x <= full ? x * per : full * per + (x - full) * (per - 1);
+  return start + q;
+}
+__device__ __forceinline__ int xcd_slot() { return xcd_block() * blockDim.x + threadIdx.x; }
+
 template <class R>
 __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
 template <class R>
@@ -79,7 +96,7 @@ __global__ void k_gather(const int* __restrict__ dst, const int* __restrict__ sr
 template <class R>
 __global__ void k_avg_corners(GeoT<R> G, const R* __restrict__ sw, const R* __restrict__ x1, R* __restrict__ o1) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
@@ -144,7 +161,7 @@ template <class R>
 __global__ void k_mass_fluxes(GeoT<R> G, const R* __restrict__ u, const R* __restrict__ h3, const R* __restrict__ hc,
                               R* __restrict__ flx1, R* __restrict__ mshf) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   if (!interior(c, iim, jjm)) {
@@ -193,7 +210,7 @@ __global__ void k_mass_low_beta(GeoT<R> G, R dt, R w3, R w2, R w1, const R* __re
                                 const R* __restrict__ m2, const R* __restrict__ m1, const R* __restrict__ h3,
                                 R* __restrict__ anti, R* __restrict__ mss_low, R* __restrict__ beta) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
@@ -223,7 +240,7 @@ __global__ void k_mass_low_beta(GeoT<R> G, R dt, R w3, R w2, R w1, const R* __re
 template <class R>
 __global__ void k_mass_anti(GeoT<R> G, const R* __restrict__ beta, R* __restrict__ anti) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   if (!interior(c, iim, jjm)) return;
@@ -241,7 +258,7 @@ template <class R>
 __global__ void k_mass_final(GeoT<R> G, R dt, const R* __restrict__ flx1, const R* __restrict__ anti,
                              const R* __restrict__ mss_low, R* __restrict__ mflux, R* __restrict__ h4) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   for (int e = 0; e < 3; ++e) mflux[e * ns + s] = flx1[e * ns + s] + anti[e * ns + s];
@@ -256,7 +273,7 @@ __global__ void k_tr_fluxes(GeoT<R> G, R w3, R w2, R w1, const R* __restrict__ t
                             const R* __restrict__ told, const R* __restrict__ mflux, R* __restrict__ f1,
                             R* __restrict__ f2) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   if (!interior(c, iim, jjm)) {
@@ -279,7 +296,7 @@ template <class R>
 __global__ void k_tr_low(GeoT<R> G, R dt, const R* __restrict__ h3, const R* __restrict__ t3, const R* __restrict__ f1,
                          R* __restrict__ ml) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   bool set;
@@ -293,7 +310,7 @@ __global__ void k_tr_limiter(GeoT<R> G, R dt, const R* __restrict__ t3, const R*
                              const R* __restrict__ h4, const R* __restrict__ mss_low, const R* __restrict__ f1,
                              const R* __restrict__ f2, R* __restrict__ anti, R* __restrict__ bdn, R* __restrict__ bup) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
@@ -352,7 +369,7 @@ __global__ void k_tr_limiter(GeoT<R> G, R dt, const R* __restrict__ t3, const R*
 template <class R>
 __global__ void k_tr_anti(GeoT<R> G, const R* __restrict__ bdn, const R* __restrict__ bup, R* __restrict__ anti) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   if (!interior(c, iim, jjm)) return;
@@ -371,7 +388,7 @@ template <class R>
 __global__ void k_tr_final(GeoT<R> G, R dt, const R* __restrict__ ml, const R* __restrict__ anti,
                            const R* __restrict__ h4, R* __restrict__ t4, R* __restrict__ told) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   bool set;
@@ -387,7 +404,7 @@ __global__ void k_vort_curl(GeoT<R> G, const R* __restrict__ mflux, const R* __r
                             const R* __restrict__ eta3, const R* __restrict__ hc, const R* __restrict__ u,
                             R* __restrict__ vf, R* __restrict__ dcurl) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   if (!interior(c, iim, jjm)) {
@@ -483,7 +500,7 @@ __global__ void k_tendencies(GeoT<R> G, R dt, R grav, R w3, R w2, R w1, const R*
                              R* __restrict__ div4, R* __restrict__ relative, R* __restrict__ rhs1,
                              R* __restrict__ rhs2) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const R ai = G.area_inv[s];
@@ -589,76 +606,84 @@ __global__ void k_sub_mean(int n, const double* __restrict__ sums, double total,
   r2[t] = r2[t] - (R)m2;
 }
 
-// per-block partial sums of the squared Poisson residual at real cells for
-// field blockIdx.y of (x0, r0), (x1, r1) (elliptic_solver.F:203-239
-// check_convergence, :341-358 l2_norm)
-__global__ void __launch_bounds__(256) k_resid2_partial(Geo G, const double* __restrict__ x0,
-                                                        const double* __restrict__ r0, const double* __restrict__ x1,
-                                                        const double* __restrict__ r1, double* __restrict__ part) {
-  __shared__ double sh[256];
-  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const double* x = blockIdx.y == 0 ? x0 : x1;
-  const double* rhs = blockIdx.y == 0 ? r0 : r1;
-  double acc = 0.0;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
-    const IJK c = ijk_of(s, iim, jjm);
-    auto X = [&](int sl) { return x[sl]; };
-    const double w = lap_at<double>(iim, jjm, ns, c, G.lw, X);
-    const double r = (w - 0.0 * x[s]) - rhs[s];
-    acc += (r * r) * G.mask[s];
-  }
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
-}
-
-// Mixed path, FP64 defect of the Poisson problem, all nv fields (planes ns
-// apart) in one thread so the FP64 weights are read once: d = rhs - L(x + e)
-// at real cells (0 elsewhere) with the FP64 weights, the FP64 iterate x and,
-// WITH_E, the FP32 correction e (halos filled); WRITE_R stores d in FP32 as
-// the right-hand side of the correction cycles.  Per-block partial sums of
-// d^2 for each field (the convergence test, elliptic_solver.F:203-239).
-template <bool WITH_E, bool WRITE_R>
-__global__ void __launch_bounds__(256) k_defect(Geo G, int nv, const double* __restrict__ x,
-                                                const float* __restrict__ e, const float* __restrict__ b,
-                                                float* __restrict__ r, double* __restrict__ part) {
-  __shared__ double sh[2][256];
-  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  double acc[2] = {0.0, 0.0};
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
-    const IJK c = ijk_of(s, iim, jjm);
-    const double m = G.mask[s];
-    double d[2] = {0.0, 0.0};
-    for (int v = 0; v < 2; ++v) {
-      if (v >= nv) break;
-      const double* xv = x + (size_t)v * ns;
-      const float* ev = e + (size_t)v * ns;
-      auto X = [&](int sl) { return WITH_E ? xv[sl] + (double)ev[sl] : xv[sl]; };
-      const double w = lap_at<double>(iim, jjm, ns, c, G.lw, X);
-      d[v] = m != 0.0 ? (double)b[(size_t)v * ns + s] - w : 0.0;
-    }
-    for (int v = 0; v < 2; ++v) {
-      if (v >= nv) break;
-      if (WRITE_R) r[(size_t)v * ns + s] = (float)d[v];
-      acc[v] += (d[v] * d[v]) * m;
-    }
-  }
-  sh[0][threadIdx.x] = acc[0];
-  sh[1][threadIdx.x] = acc[1];
+// block-wide sums of NV per-thread values, written to part[v * gridDim.x + block]
+template <int NV>
+__device__ __forceinline__ void block_sums(const double (&acc)[NV], double* __restrict__ part) {
+  __shared__ double sh[NV][256];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) sh[v][threadIdx.x] = acc[v];
   __syncthreads();
   for (int o = blockDim.x / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) {
-      sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
-      sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) sh[v][threadIdx.x] += sh[v][threadIdx.x + o];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0)
-    for (int v = 0; v < nv; ++v) part[v * gridDim.x + blockIdx.x] = sh[v][0];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) part[v * gridDim.x + blockIdx.x] = sh[v][0];
+  }
+}
+
+// FP64 path: squared Poisson residual at real cells of NV fields (planes ns
+// apart), one slot per thread and the weights read once, per-block sums
+// (elliptic_solver.F:203-239 check_convergence, :341-358 l2_norm)
+template <int NV>
+__global__ void __launch_bounds__(256) k_resid2(Geo G, const double* __restrict__ x, const double* __restrict__ rhs,
+                                                double* __restrict__ part) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  if (s < ns) {
+    const IJK c = ijk_of(s, iim, jjm);
+    const double m = G.mask[s];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double* xv = x + (size_t)v * ns;
+      auto X = [&](int sl) { return xv[sl]; };
+      const double w = lap_at<double>(iim, jjm, ns, c, G.lw, X);
+      const double r = (w - 0.0 * xv[s]) - rhs[(size_t)v * ns + s];
+      acc[v] = (r * r) * m;
+    }
+  }
+  block_sums<NV>(acc, part);
+}
+
+// Mixed path, FP64 defect of the Poisson problem for NV fields, one slot per
+// thread, the FP64 weights read once: d = rhs - L(x') at real cells (0
+// elsewhere), x' = x + e with CORR (the FP32 correction e, halos filled;
+// x' is stored to xn on every slot, halos included, so the iterate stays
+// halo-consistent) or x' = x.  d is stored in FP32 as the right-hand side of
+// the next correction cycle; per-block sums of d^2 for the convergence test
+// (elliptic_solver.F:203-239).
+template <int NV, bool CORR>
+__global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict__ x, const float* __restrict__ e,
+                                                double* __restrict__ xn, const float* __restrict__ b,
+                                                float* __restrict__ r, double* __restrict__ part) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  if (s < ns) {
+    const IJK c = ijk_of(s, iim, jjm);
+    const double m = G.mask[s];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double* xv = x + (size_t)v * ns;
+      const float* ev = e + (size_t)v * ns;
+      auto X = [&](int sl) { return CORR ? xv[sl] + (double)ev[sl] : xv[sl]; };
+      const double w = lap_at<double>(iim, jjm, ns, c, G.lw, X);
+      const double d = m != 0.0 ? (double)b[(size_t)v * ns + s] - w : 0.0;
+      if (CORR) xn[(size_t)v * ns + s] = X(s);
+      r[(size_t)v * ns + s] = (float)d;
+      acc[v] = (d * d) * m;
+    }
+  }
+  block_sums<NV>(acc, part);
 }
 
 // x += e (FP64 solution, FP32 correction), nvar planes of ns
@@ -682,7 +707,7 @@ template <class R, int NV, bool ZERO>
 __global__ void __launch_bounds__(256) k_relax(LevT<R> L, const R* __restrict__ beta, const R* __restrict__ a,
                                                R* __restrict__ out) {
   const int iim = L.iim, jjm = L.jjm, ns = L.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
@@ -742,7 +767,7 @@ __global__ void __launch_bounds__(256) k_relax(LevT<R> L, const R* __restrict__ 
 template <class R, int NV>
 __global__ void k_inject(LevT<R> F, LevT<R> C, const R* __restrict__ xa, const R* __restrict__ xb, R* __restrict__ x1) {
   int iim = C.iim, jjm = C.jjm;
-  const int s1 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s1 = xcd_slot();
   if (s1 >= C.ns) return;
   const IJK c = ijk_of(s1, iim, jjm);
   const bool in = c.i >= 2 && c.i <= iim - 1 && c.j >= 2 && c.j <= jjm - 1;
@@ -802,7 +827,7 @@ __global__ void k_inject(LevT<R> F, LevT<R> C, const R* __restrict__ xa, const R
 template <class R, int NV>
 __global__ void k_interp(LevT<R> C, LevT<R> F, const R* __restrict__ x1, R* __restrict__ x2) {
   const int iim = F.iim, jjm = F.jjm;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= F.ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
@@ -851,7 +876,7 @@ template <class R>
 __global__ void k_velocity(GeoT<R> G, const double* __restrict__ psi, const double* __restrict__ chi,
                            R* __restrict__ u) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   if (!interior(c, iim, jjm)) {
@@ -892,7 +917,7 @@ __global__ void k_velocity(GeoT<R> G, const double* __restrict__ psi, const doub
 template <class R>
 __global__ void k_center_ke(GeoT<R> G, const R* __restrict__ u, R* __restrict__ ke) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
@@ -911,7 +936,7 @@ __global__ void k_center_ke(GeoT<R> G, const R* __restrict__ u, R* __restrict__ 
 // operators.F:47-261 curl and divergence of a corner vector (initial state)
 __global__ void k_curl_div(Geo G, const double* __restrict__ u, double* __restrict__ curl, double* __restrict__ dv) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
@@ -940,14 +965,14 @@ __global__ void k_curl_div(Geo G, const double* __restrict__ u, double* __restri
 // vorticity is the solve's right-hand side itself); inv_h (step.F:448-452)
 template <class R>
 __global__ void k_post_elliptic(GeoT<R> G, const R* __restrict__ rhs1, R* __restrict__ eta4) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= G.ns) return;
   if constexpr (sizeof(R) == 4) eta4[s] = rhs1[s];
   else eta4[s] = rhs1[s] + G.f[s];
 }
 template <class R>
 __global__ void k_inv_h(GeoT<R> G, const R* __restrict__ h4, R* __restrict__ inv_h) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= G.ns) return;
   inv_h[s] = G.mask[s] == R(1) ? R(1) / habs(G, h4[s]) : R(0);
 }
