@@ -98,12 +98,19 @@ typedef struct swm_ctx swm_ctx;
  * the reference. */
 enum { SWM_FP64 = 0, SWM_MIXED = 1 };
 
+/* Advection of mass and tracers in step_dt (step.F:131-134, namelist flags
+ * l_fct / l_centered / l_upwind): flux-corrected transport (mass_fct.F,
+ * tracer_fct.F), or centred / upstream fluxes with ABM3 stepping of the
+ * thickness and the mass-weighted tracers (step.F:221-327, :495-602). */
+enum { SWM_ADV_FCT = 0, SWM_ADV_CENTERED = 1, SWM_ADV_UPWIND = 2 };
+
 typedef struct swm_config {
   int ntracer;       /* passive tracers (the reference compiles 5: swm_parameters.F:66) */
   double dt;         /* time step (s) */
   double grav;       /* gravity (m/s^2) */
   int precision;     /* SWM_FP64 or SWM_MIXED */
   int device;        /* HIP device ordinal */
+  int advection;     /* SWM_ADV_FCT (0, the default of a zeroed struct), SWM_ADV_CENTERED, SWM_ADV_UPWIND */
 } swm_config;
 
 /* Allocates the device state for a grid (the grid may be destroyed after). */

@@ -59,7 +59,8 @@ def lib():
 
 
 class _OrcGrid(ctypes.Structure):
-    _fields_ = [("g", ctypes.c_int), ("ntracer", ctypes.c_int), ("dt", ctypes.c_double), ("grav", ctypes.c_double)] + [
+    _fields_ = [("g", ctypes.c_int), ("ntracer", ctypes.c_int), ("dt", ctypes.c_double), ("grav", ctypes.c_double),
+                ("scheme", ctypes.c_int)] + [
         (n, ctypes.c_void_p) for n in ("area", "area_inv", "corner_area_inv", "sub_area", "dc", "normal", "tangent",
                                        "v_weights", "s_weights", "l_weights", "f", "hs", "center_mask")] + [
         ("nface", ctypes.c_int), ("nvrtx", ctypes.c_int), ("nedge", ctypes.c_int)] + [
@@ -85,10 +86,15 @@ def grid_arrays_from(grid):
     return d
 
 
-class Oracle:
-    """One FP64 model state on the CPU; arrays as in the reference layout."""
+SCHEMES = {"fct": 0, "centered": 1, "upwind": 2}
 
-    def __init__(self, arrays, ntracer, dt, grav, hs=None):
+
+class Oracle:
+    """One FP64 model state on the CPU; arrays as in the reference layout.
+    scheme: the advection of step_dt, "fct" (l_fct), "centered" (l_centered)
+    or "upwind" (l_upwind)."""
+
+    def __init__(self, arrays, ntracer, dt, grav, hs=None, scheme="fct"):
         L = lib()
         self.g = int(arrays["g"])
         self.iim = self.jjm = 2 + 2 ** self.g
@@ -106,6 +112,7 @@ class Oracle:
 
         gr = _OrcGrid()
         gr.g, gr.ntracer, gr.dt, gr.grav = self.g, ntracer, dt, grav
+        gr.scheme = SCHEMES[scheme]
         for n in ("area", "area_inv", "corner_area_inv", "sub_area", "dc", "normal", "tangent", "v_weights",
                   "s_weights", "l_weights", "f"):
             setattr(gr, n, f64(n, arrays[n]))

@@ -268,7 +268,13 @@ contains
 
     dt = dtin
     invdt = c1/dt
-    l_fct = .true.
+    ! advection scheme (argument 7): fct (default), centered or upwind
+    arg = 'fct'
+    if (command_argument_count() >= 7) call get_command_argument(7, arg)
+    l_fct = trim(arg) == 'fct'
+    l_centered = trim(arg) == 'centered'
+    l_upwind = trim(arg) == 'upwind'
+    l_remap = .false.
     n_restart = 100000000
     n_output = 100000000
     n_diagnostics = 100000000

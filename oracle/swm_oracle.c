@@ -16,6 +16,7 @@ struct orc_model {
   int g, iim, jjm, nt;
   size_t ns;             /* slots = iim*jjm*10 */
   double dt, grav;
+  int scheme;
   double *area, *area_inv, *cai, *sub_area, *dc, *normal, *tangent, *vw, *sw, *lw, *f, *hs, *mask;
   int nface, nvrtx, nedge;
   int32_t *face_dst, *face_src, *vrtx_dst, *vrtx_src, *edge_dst, *edge_src;
@@ -27,6 +28,8 @@ struct orc_model {
   double *relative, *ke, *psi, *chi, *inv_h;
   double *xout;          /* (slots, 2): elliptic warm start */
   double *msh_f[3];      /* (3,slots) edge-flux tendency history */
+  double *h_f[3];        /* thickness tendency history (centred / upwind) */
+  double **tr_f[3];      /* mass-times-tracer tendency history (centred / upwind) */
   double **work_old;     /* previous-previous tracer levels */
   double *mass_flux;     /* (3,slots) */
   int step_count, last_vcycles;
@@ -349,6 +352,36 @@ static void divergence_curl(int iim, int jjm, const double* dc, const double* tn
 #undef UV
 #undef MC
 #undef FL
+
+/* operators.F:1841-1914 upstream_tracer_flux */
+static void upstream_tracer_flux(int iim, int jjm, const double* mf, const double* x, double* out) {
+  const size_t n = (size_t)iim * jjm * 10;
+  for (size_t q = 0; q < 3 * n; ++q) out[q] = 0.0;
+  for (int k = 1; k <= 10; ++k)
+    for (int j = 2; j <= jjm - 1; ++j)
+      for (int i = 2; i <= iim - 1; ++i) {
+        const size_t s = S(i, j, k);
+        const size_t up[3] = {S(i - 1, j, k), S(i - 1, j - 1, k), S(i, j - 1, k)};
+        for (int e = 0; e < 3; ++e) {
+          const double F = mf[e + 3 * s];
+          const double mx = F > 0.0 ? F : 0.0, mn = F < 0.0 ? F : 0.0;   /* maxval / minval of (F, 0) */
+          out[e + 3 * s] = mx * x[s] + mn * x[up[e]];
+        }
+      }
+}
+
+/* operators.F:1923-1991 centered_tracer_flux */
+static void centered_tracer_flux(int iim, int jjm, const double* mf, const double* x, double* out) {
+  const size_t n = (size_t)iim * jjm * 10;
+  for (size_t q = 0; q < 3 * n; ++q) out[q] = 0.0;
+  for (int k = 1; k <= 10; ++k)
+    for (int j = 2; j <= jjm - 1; ++j)
+      for (int i = 2; i <= iim - 1; ++i) {
+        const size_t s = S(i, j, k);
+        const size_t up[3] = {S(i - 1, j, k), S(i - 1, j - 1, k), S(i, j - 1, k)};
+        for (int e = 0; e < 3; ++e) out[e + 3 * s] = 0.5 * (mf[e + 3 * s]) * (x[s] + x[up[e]]);
+      }
+}
 
 /* ------------------------------------------------------------ mass FCT --- */
 static inline double dmin(double a, double b) { return a < b ? a : b; }
@@ -809,6 +842,7 @@ orc_model* orc_create(const orc_grid* gr) {
   m->ns = (size_t)m->iim * m->jjm * 10;
   m->dt = gr->dt;
   m->grav = gr->grav;
+  m->scheme = gr->scheme;
   const size_t ns = m->ns;
   m->area = dup(gr->area, ns);
   m->area_inv = dup(gr->area_inv, ns);
@@ -838,6 +872,11 @@ orc_model* orc_create(const orc_grid* gr) {
     for (int q = 0; q < m->nt; ++q) m->tr[t][q] = zeros(ns);
   }
   for (int t = 0; t < 3; ++t) { m->eta_f[t] = zeros(ns); m->div_f[t] = zeros(ns); m->msh_f[t] = zeros(3 * ns); }
+  for (int t = 0; t < 3; ++t) {
+    m->h_f[t] = zeros(ns);
+    m->tr_f[t] = (double**)calloc(m->nt > 0 ? m->nt : 1, sizeof(double*));
+    for (int q = 0; q < m->nt; ++q) m->tr_f[t][q] = zeros(ns);
+  }
   m->relative = zeros(ns); m->ke = zeros(ns); m->psi = zeros(ns); m->chi = zeros(ns); m->inv_h = zeros(ns);
   m->xout = zeros(2 * ns);
   m->mass_flux = zeros(3 * ns);
@@ -858,7 +897,11 @@ void orc_destroy(orc_model* m) {
     for (int q = 0; q < m->nt; ++q) free(m->tr[t][q]);
     free(m->tr[t]);
   }
-  for (int t = 0; t < 3; ++t) { free(m->eta_f[t]); free(m->div_f[t]); free(m->msh_f[t]); }
+  for (int t = 0; t < 3; ++t) { free(m->eta_f[t]); free(m->div_f[t]); free(m->msh_f[t]); free(m->h_f[t]); }
+  for (int t = 0; t < 3; ++t) {
+    for (int q = 0; q < m->nt; ++q) free(m->tr_f[t][q]);
+    free(m->tr_f[t]);
+  }
   free(m->relative); free(m->ke); free(m->psi); free(m->chi); free(m->inv_h); free(m->xout); free(m->mass_flux);
   for (int q = 0; q < m->nt; ++q) free(m->work_old[q]);
   free(m->work_old);
@@ -936,6 +979,43 @@ static int step_dt(orc_model* m) {
   wrap_vrtx(m, h_c, 1);
   wrap_vrtx(m, eta_c, 1);
 
+  if (m->scheme != ORC_FCT) {
+    /* get_centered_flux / get_upwind_flux (step.F:495-602) */
+    double** tflux = (double**)calloc(m->nt > 0 ? m->nt : 1, sizeof(double*));
+    if (m->scheme == ORC_CENTERED) centered_mass_flux(iim, jjm, m->dc, m->normal, h_c, m->u[n3], m->mass_flux);
+    else upstream_mass_flux(iim, jjm, m->dc, m->normal, m->u[n3], m->h[n3], m->mass_flux);
+    for (int q = 0; q < m->nt; ++q) {
+      tflux[q] = zeros(3 * ns);
+      if (m->scheme == ORC_CENTERED) centered_tracer_flux(iim, jjm, m->mass_flux, m->tr[n3][q], tflux[q]);
+      else upstream_tracer_flux(iim, jjm, m->mass_flux, m->tr[n3][q], tflux[q]);
+    }
+    wrap_edge(m, m->mass_flux);
+    for (int q = 0; q < m->nt; ++q) wrap_edge(m, tflux[q]);
+    const double dt = m->dt;
+    const int a = m->n3f - 1, b = m->n2f - 1, c = m->n1f - 1;
+    double* w = zeros(ns);
+    /* step_surface_height (step.F:221-270) */
+    div_edge(iim, jjm, m->area_inv, m->mass_flux, w);
+    for (size_t n = 0; n < ns; ++n) m->h_f[a][n] = -w[n];
+    for (size_t n = 0; n < ns; ++n) {
+      m->h[n4][n] = m->h[n3][n] + dt * (m->w3 * m->h_f[a][n] + m->w2 * m->h_f[b][n] + m->w1 * m->h_f[c][n]);
+      if (m->h[n4][n] < 1.0e-10) m->h[n4][n] = 1.0e-10;
+    }
+    /* step_tracer (step.F:278-327) */
+    for (int q = 0; q < m->nt; ++q) {
+      div_edge(iim, jjm, m->area_inv, tflux[q], w);
+      for (size_t n = 0; n < ns; ++n) m->tr_f[a][q][n] = -w[n];
+      for (size_t n = 0; n < ns; ++n) {
+        double t = m->tr[n3][q][n] * m->h[n3][n] +
+                   dt * (m->w3 * m->tr_f[a][q][n] + m->w2 * m->tr_f[b][q][n] + m->w1 * m->tr_f[c][q][n]);
+        if (m->h[n4][n] > 1.0e-10) t = t / m->h[n4][n];
+        m->tr[n4][q][n] = t;
+      }
+      free(tflux[q]);
+    }
+    free(tflux);
+    free(w);
+  } else {
   /* get_fct_flux */
   double* wm[2] = {dup(m->h[0], ns), dup(m->h[1], ns)};
   double* wlow = zeros(ns);
@@ -954,6 +1034,7 @@ static int step_dt(orc_model* m) {
     memcpy(m->tr[n4][q], wtn4, ns * sizeof(double));
   }
   free(wtr); free(wtrh); free(wtn4); free(wm[0]); free(wm[1]); free(wlow); free(wflux);
+  }
 
   double *vflux = zeros(3 * ns), *dcurl = zeros(3 * ns);
   vorticity_flux(iim, jjm, m->mass_flux, m->h[n3], m->eta[n3], vflux);

@@ -24,9 +24,13 @@ typedef struct orc_model orc_model;
  * Halo tables: flat (component, slot) destination/source pairs, source -1
  * meaning zero; ncomp 1 (cells), 2 (corners), 3 (edges).
  * lap_w[l-1]: (7, 2+2^l, 2+2^l, 10) multigrid Laplacian weights of level l. */
+/* advection schemes of step_dt (step.F:131-134): l_fct, l_centered, l_upwind */
+enum { ORC_FCT = 0, ORC_CENTERED = 1, ORC_UPWIND = 2 };
+
 typedef struct {
   int g, ntracer;
   double dt, grav;
+  int scheme;   /* ORC_FCT, ORC_CENTERED or ORC_UPWIND */
   const double *area, *area_inv, *corner_area_inv, *sub_area, *dc, *normal, *tangent,
       *v_weights, *s_weights, *l_weights, *f, *hs, *center_mask;
   int nface, nvrtx, nedge;

@@ -77,6 +77,8 @@ struct View {
   R *h[2] = {}, *eta[2] = {}, *div[2] = {}, *u[2] = {};
   std::vector<R*> tr[2], told;
   R *eta_f[3] = {}, *div_f[3] = {}, *msh_f[3] = {};
+  R* h_f[3] = {};                 // thickness tendencies (centred / upwind advection)
+  std::vector<R*> tr_f[3], tflux;  // mass-times-tracer tendencies and tracer edge fluxes
   R *relative = nullptr, *ke = nullptr, *inv_h = nullptr, *mass_flux = nullptr;
   R *hc = nullptr, *flx1 = nullptr, *anti = nullptr, *mss_low = nullptr, *beta = nullptr, *f1 = nullptr,
     *f2 = nullptr, *ml = nullptr, *bdn = nullptr, *bup = nullptr, *vf = nullptr, *dcurl = nullptr, *rhs = nullptr;
@@ -90,6 +92,7 @@ struct swm_ctx {
   int g = 0, iim = 0, jjm = 0, ns = 0, nt = 0, max_ig = 0;
   double dt = 0, grav = 0;
   int precision = SWM_FP64, device = 0;
+  int advection = SWM_ADV_FCT;
   hipStream_t stream = nullptr;
   Arena arena;
   size_t arena_bytes = 0;
@@ -512,6 +515,31 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
 
   hipLaunchKernelGGL(swm::dev::k_avg_corners<R>, dim3(nb), dim3(BLK), 0, S, G, G.sw, V.h[n3], V.hc);
   gather<R>(C, C->vrtx, {V.hc});
+  if (C->advection != SWM_ADV_FCT) {
+    // get_centered_flux / get_upwind_flux (step.F:495-602): the centred or
+    // upstream mass flux (the other one lands in scratch), tracer fluxes,
+    // then step_surface_height and step_tracer (step.F:221-327)
+    const bool cen = C->advection == SWM_ADV_CENTERED;
+    hipLaunchKernelGGL(swm::dev::k_mass_fluxes<R>, dim3(nb), dim3(BLK), 0, S, G, V.u[n3], V.h[n3], V.hc,
+                       cen ? V.flx1 : V.mass_flux, cen ? V.mass_flux : V.flx1);
+    for (int q = 0; q < C->nt; ++q) {
+      if (cen)
+        hipLaunchKernelGGL((swm::dev::k_tr_flux<R, true>), dim3(nb), dim3(BLK), 0, S, G, V.mass_flux, V.tr[n3][q],
+                           V.tflux[q]);
+      else
+        hipLaunchKernelGGL((swm::dev::k_tr_flux<R, false>), dim3(nb), dim3(BLK), 0, S, G, V.mass_flux, V.tr[n3][q],
+                           V.tflux[q]);
+    }
+    std::vector<R*> ef = {V.mass_flux};
+    for (int q = 0; q < C->nt; ++q) ef.push_back(V.tflux[q]);
+    gather_list<R>(C, C->edge, ef);
+    const int a = C->n3f - 1, b = C->n2f - 1, c = C->n1f - 1;
+    hipLaunchKernelGGL(swm::dev::k_height_update<R>, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, V.mass_flux,
+                       V.h[n3], V.h_f[a], V.h_f[b], V.h_f[c], V.h[n4]);
+    for (int q = 0; q < C->nt; ++q)
+      hipLaunchKernelGGL(swm::dev::k_tracer_update<R>, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, V.tflux[q],
+                         V.tr[n3][q], V.h[n3], V.h[n4], V.tr_f[a][q], V.tr_f[b][q], V.tr_f[c][q], V.tr[n4][q]);
+  } else {
   // mass FCT (mass_fct.F:23-539)
   hipLaunchKernelGGL(swm::dev::k_mass_fluxes<R>, dim3(nb), dim3(BLK), 0, S, G, V.u[n3], V.h[n3], V.hc, V.flx1, m3);
   gather<R>(C, C->edge, {V.flx1, m3});
@@ -539,6 +567,7 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
     gather<R>(C, C->edge, {V.anti});
     hipLaunchKernelGGL(swm::dev::k_tr_final<R>, dim3(nb), dim3(BLK), 0, S, G, dt, V.ml, V.anti, V.h[n4], t4,
                        V.told[q]);
+  }
   }
   // vorticity and divergence (step.F:143-163, :336-429)
   hipLaunchKernelGGL(swm::dev::k_vort_curl<R>, dim3(nb), dim3(BLK), 0, S, G, V.mass_flux, V.h[n3], V.eta[n3], V.hc,
@@ -772,6 +801,13 @@ void layout_view(swm_ctx* C, View<R>& V, const View<double>* maps) {
   V.told.clear();
   for (int q = 0; q < C->nt; ++q) V.told.push_back(take(ns));
   for (int t = 0; t < 3; ++t) { V.eta_f[t] = take(ns); V.div_f[t] = take(ns); V.msh_f[t] = take(3 * ns); }
+  for (int t = 0; t < 3; ++t) {
+    V.h_f[t] = take(ns);
+    V.tr_f[t].clear();
+    for (int q = 0; q < C->nt; ++q) V.tr_f[t].push_back(take(ns));
+  }
+  V.tflux.clear();
+  for (int q = 0; q < C->nt; ++q) V.tflux.push_back(take(3 * ns));
   V.relative = take(ns); V.ke = take(ns); V.inv_h = take(ns); V.mass_flux = take(3 * ns);
   V.hc = take(2 * ns); V.flx1 = take(3 * ns); V.anti = take(3 * ns); V.mss_low = take(ns);
   V.beta = take(ns); V.f1 = take(3 * ns); V.f2 = take(3 * ns); V.ml = take(ns); V.bdn = take(ns); V.bup = take(ns);
@@ -916,7 +952,8 @@ extern "C" {
 
 swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out) {
   if (!grid || !cfg || !out || cfg->ntracer < 0 || cfg->ntracer > 64 || !(cfg->dt > 0.0) ||
-      (cfg->precision != SWM_FP64 && cfg->precision != SWM_MIXED)) {
+      (cfg->precision != SWM_FP64 && cfg->precision != SWM_MIXED) || cfg->advection < SWM_ADV_FCT ||
+      cfg->advection > SWM_ADV_UPWIND) {
     swm::set_error("swm_create: bad argument");
     return SWM_ERR_ARG;
   }
@@ -930,6 +967,7 @@ swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out
   C->g = Gh.g; C->iim = Gh.iim; C->jjm = Gh.jjm; C->ns = Gh.iim * Gh.jjm * 10; C->nt = cfg->ntracer;
   C->max_ig = Gh.max_ig;
   C->dt = cfg->dt; C->grav = cfg->grav; C->precision = cfg->precision; C->device = cfg->device;
+  C->advection = cfg->advection;
   C->npart = 1024;
   C->radius = Gh.radius;
   swm_status st = swm::create_device(C, Gh);

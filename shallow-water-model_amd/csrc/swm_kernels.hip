@@ -398,6 +398,69 @@ __global__ void k_tr_final(GeoT<R> G, R dt, const R* __restrict__ ml, const R* _
   t4[s] = hh != R(0) ? mf / hh : R(0);
 }
 
+// centred (operators.F:1923-1991) or upstream (:1841-1914) tracer edge flux
+// of the mass flux, interior slots (0 elsewhere)
+template <class R, bool CENTERED>
+__global__ void k_tr_flux(GeoT<R> G, const R* __restrict__ mflux, const R* __restrict__ t, R* __restrict__ out) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) out[e * ns + s] = R(0);
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  for (int e = 0; e < 3; ++e) {
+    const R F = mflux[e * ns + s];
+    if (CENTERED) {
+      out[e * ns + s] = R(0.5) * (F) * (t[s] + t[up[e]]);
+    } else {
+      const R mx = F > R(0) ? F : R(0), mn = F < R(0) ? F : R(0);   // maxval / minval of (F, 0)
+      out[e * ns + s] = mx * t[s] + mn * t[up[e]];
+    }
+  }
+}
+
+// step.F:221-270 step_surface_height: ABM3 on minus the divergence of the
+// mass flux, then h >= 1e-10 on every slot (stored less h0 in the mixed path)
+template <class R>
+__global__ void k_height_update(GeoT<R> G, R dt, R w3, R w2, R w1, const R* __restrict__ mflux,
+                                const R* __restrict__ h3, R* __restrict__ hf_a, const R* __restrict__ hf_b,
+                                const R* __restrict__ hf_c, R* __restrict__ h4) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const R w = div_edge_at<R>(iim, jjm, ns, c, mflux, G.area_inv[s]);
+  const R hf = -w;
+  hf_a[s] = hf;
+  R h = h3[s] + dt * (w3 * hf + w2 * hf_b[s] + w1 * hf_c[s]);
+  if (habs(G, h) < R(1.0e-10)) h = R(1.0e-10) - habs(G, R(0));
+  h4[s] = h;
+}
+
+// step.F:278-327 step_tracer: ABM3 on mass times tracer, divided by the new
+// thickness where it exceeds 1e-10
+template <class R>
+__global__ void k_tracer_update(GeoT<R> G, R dt, R w3, R w2, R w1, const R* __restrict__ tflux,
+                                const R* __restrict__ t3, const R* __restrict__ h3, const R* __restrict__ h4,
+                                R* __restrict__ tf_a, const R* __restrict__ tf_b, const R* __restrict__ tf_c,
+                                R* __restrict__ t4) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const R w = div_edge_at<R>(iim, jjm, ns, c, tflux, G.area_inv[s]);
+  const R tf = -w;
+  tf_a[s] = tf;
+  R t = t3[s] * habs(G, h3[s]) + dt * (w3 * tf + w2 * tf_b[s] + w1 * tf_c[s]);
+  const R hn = habs(G, h4[s]);
+  if (hn > R(1.0e-10)) t = t / hn;
+  t4[s] = t;
+}
+
 // operators.F:2000-2186 centered_vorticity_flux and centered_divergence_curl
 template <class R>
 __global__ void k_vort_curl(GeoT<R> G, const R* __restrict__ mflux, const R* __restrict__ h3,

@@ -22,6 +22,7 @@ module swm_mi355x_c
   public
 
   integer(c_int), parameter :: SWM_OK = 0, SWM_FP64 = 0, SWM_MIXED = 1
+  integer(c_int), parameter :: SWM_ADV_FCT = 0, SWM_ADV_CENTERED = 1, SWM_ADV_UPWIND = 2
 
   type, bind(C) :: swm_config
     integer(c_int) :: ntracer
@@ -29,6 +30,7 @@ module swm_mi355x_c
     real(c_double) :: grav
     integer(c_int) :: precision
     integer(c_int) :: device
+    integer(c_int) :: advection
   end type swm_config
 
   interface
@@ -301,12 +303,14 @@ contains
   ! centres may be passed in p_list(3, 10*4**g+2)), the device state and the
   ! initial fields h, hs (iim,jjm,10), u (2,2,iim,jjm,10), tracer
   ! (iim,jjm,ntracer,10).
-  subroutine swm_gpu_initialize(g, ntracer, dt, grav, radius, omega, h, hs, u, tracer, p_list, device)
+  subroutine swm_gpu_initialize(g, ntracer, dt, grav, radius, omega, h, hs, u, tracer, p_list, device, &
+                                l_fct, l_centered, l_upwind)
     integer, intent(in) :: g, ntracer
     real(c_double), intent(in) :: dt, grav, radius, omega
     real(c_double), intent(in), target :: h(:,:,:), hs(:,:,:), u(:,:,:,:,:), tracer(:,:,:,:)
     real(c_double), intent(in), optional, target :: p_list(:,:)
     integer, intent(in), optional :: device
+    logical, intent(in), optional :: l_fct, l_centered, l_upwind   ! the namelist flags (default l_fct)
     type(swm_config) :: cfg
     type(c_ptr) :: pl
     integer(c_int) :: gg, mig
@@ -320,6 +324,16 @@ contains
     cfg%precision = SWM_FP64
     cfg%device = 0
     if (present(device)) cfg%device = device
+    cfg%advection = SWM_ADV_FCT
+    if (present(l_centered)) then
+      if (l_centered) cfg%advection = SWM_ADV_CENTERED
+    end if
+    if (present(l_upwind)) then
+      if (l_upwind) cfg%advection = SWM_ADV_UPWIND
+    end if
+    if (present(l_fct)) then
+      if (l_fct) cfg%advection = SWM_ADV_FCT
+    end if
     ntracer_ = ntracer
     call swm_gpu_check(swm_create(grid, cfg, ctx), 'swm_create')
     call swm_gpu_check(swm_init_state(ctx, h, c_loc(hs), u, tracer), 'swm_init_state')

@@ -25,6 +25,8 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libswm_mi355x.so")
 
 SWM_FP64 = 0
 SWM_MIXED = 1
+# advection of step_dt: l_fct, l_centered, l_upwind
+ADVECTION = {"fct": 0, "centered": 1, "upwind": 2}
 
 # physical constants of the reference build (utilities/layer_1/params_physical.F:45-52)
 REF_RADIUS = 6.37122e6
@@ -55,7 +57,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 
 class _Config(ctypes.Structure):
     _fields_ = [("ntracer", ctypes.c_int), ("dt", ctypes.c_double), ("grav", ctypes.c_double),
-                ("precision", ctypes.c_int), ("device", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("device", ctypes.c_int), ("advection", ctypes.c_int)]
 
 
 def _sig(name, res, args):
@@ -217,8 +219,8 @@ class Model:
     """Device-resident model state on one GPU (the reference's prognostics
     module plus the saved state of step.F and the elliptic warm start)."""
 
-    def __init__(self, grid, ntracer=1, dt=450.0, grav=REF_GRAV, precision=SWM_FP64, device=0):
-        cfg = _Config(ntracer, dt, grav, precision, device)
+    def __init__(self, grid, ntracer=1, dt=450.0, grav=REF_GRAV, precision=SWM_FP64, device=0, advection="fct"):
+        cfg = _Config(ntracer, dt, grav, precision, device, ADVECTION[advection])
         h = ctypes.c_void_p()
         _check(lib.swm_create(grid.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

@@ -15,8 +15,11 @@ and stores its outputs as compressed npz fixtures:
   mg_g4.npz        one V-cycle and one elliptic solve
   run_g<g>.npz     initial state and the state after selected steps of the
                    full model (g = 3, dt = 1800 s; g = 4, dt = 900 s), 5 tracers
+  run_g4_<scheme>.npz  the same with the centred (l_centered) and upwind
+                   (l_upwind) advection of step_dt (g = 4, dt = 400 s)
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # everything
+    python tests/golden/make_golden.py schemes    # only run_g4_<scheme>.npz
 """
 import os
 import subprocess
@@ -127,9 +130,39 @@ def read_run(path, g, nt):
     return states
 
 
+def scheme_runs():
+    """Centred and upwind advection runs of the compiled reference at g = 4."""
+    import swm_mi355x as S
+    from swm_mi355x import ics
+    g, nt, dt, nsteps = 4, 5, 400.0, 12
+    exe = harness(g)
+    points = np.loadtxt(os.path.join(REF, "swm", "run.0002562", "data", "grid_centers_0002562")).T
+    d = rundir(g, points)
+    grid = S.Grid(g, p_list=points)
+    ic = ics.synthetic_state(grid, ntracer=nt, lmax=8, seed_index=g)
+    with open(os.path.join(d, "ic.bin"), "wb") as f:
+        for k in ("h", "hs", "u", "tracer"):
+            f.write(np.asfortranarray(ic[k]).tobytes(order="F"))
+    for scheme in ("centered", "upwind"):
+        rc = run(exe, d, "run", "out", "ic.bin", repr(dt), str(nsteps), "1", scheme, check=False)
+        states = read_run(os.path.join(d, "out", "run.bin"), g, nt)
+        assert rc == 0 and max(states) == nsteps, (scheme, rc, max(states))
+        keep = {f"ic_{k}": np.asarray(v) for k, v in ic.items()}
+        keep["dt"] = np.array(dt)
+        keep["nsteps"] = np.array(nsteps)
+        for st in (1, 2, 3, nsteps):
+            for k, v in states[st].items():
+                keep[f"s{st}_{k}"] = v
+        np.savez_compressed(os.path.join(HERE, f"run_g4_{scheme}.npz"), **keep)
+        print("run", g, scheme, flush=True)
+
+
 def main():
     if not os.path.isdir(REF):
         sys.exit("make_golden.py needs the reference sources at " + REF)
+    if sys.argv[1:] == ["schemes"]:
+        scheme_runs()
+        return
     import swm_mi355x as S
     from swm_mi355x import ics
 
@@ -184,6 +217,7 @@ def main():
                     keep[f"s{s}_{k}"] = v
             np.savez_compressed(os.path.join(HERE, f"run_g{g}.npz"), **keep)
             print("run", g, flush=True)
+    scheme_runs()
 
 
 if __name__ == "__main__":

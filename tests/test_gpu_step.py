@@ -68,3 +68,40 @@ def test_mass_conserved():
     model.step(20)
     m1 = model.total_mass()
     assert abs(m1 - m0) / m0 < 1e-13
+
+
+@pytest.mark.parametrize("scheme,g,nsteps", [("centered", 4, 1), ("centered", 4, 12), ("upwind", 4, 1),
+                                             ("upwind", 5, 8), ("centered", 5, 8)])
+def test_advection_schemes_match_oracle(scheme, g, nsteps):
+    # step_dt with l_centered / l_upwind (step.F:221-327, :495-602); the oracle
+    # is pinned bit for bit to the compiled reference for both (test_golden.py)
+    print(f"case {scheme} g={g} nsteps={nsteps}", flush=True)
+    grid = S.Grid(g)
+    dt = 200.0 * 2.0 ** (5 - g)
+    st = ics.synthetic_state(grid, ntracer=2, lmax=10)
+    model = S.Model(grid, ntracer=2, dt=dt, grav=S.REF_GRAV, advection=scheme)
+    model.init_state(st["h"], st["u"], st["tracer"], hs=st["hs"])
+    orc = Oracle(grid_arrays_from(grid), 2, dt, S.REF_GRAV, scheme=scheme)
+    orc.init_state(st["h"], st["u"], st["tracer"])
+    model.step(nsteps)
+    orc.step(nsteps)
+    _compare(model, orc, grid, TOL_1 if nsteps == 1 else TOL_N)
+    assert model.last_vcycles == orc.last_vcycles
+
+
+@pytest.mark.parametrize("scheme", ["centered", "upwind"])
+def test_advection_schemes_mixed_close_to_fp64(scheme):
+    grid = S.Grid(5)
+    st = ics.synthetic_state(grid, ntracer=1, lmax=10)
+    out = {}
+    for prec in (S.SWM_FP64, S.SWM_MIXED):
+        m = S.Model(grid, ntracer=1, dt=200.0, grav=S.REF_GRAV, precision=prec, advection=scheme)
+        m.init_state(st["h"], st["u"], st["tracer"], hs=st["hs"])
+        m.step(20)
+        out[prec] = {n: m.field(n) for n in ("h", "eta", "tracer")}
+    mask = grid.array("grid_center_mask") == 1.0
+    for n in ("h", "eta"):
+        a, b = out[S.SWM_FP64][n], out[S.SWM_MIXED][n]
+        assert _rel(b, a, mask) <= 1e-6, n
+    a, b = out[S.SWM_FP64]["tracer"], out[S.SWM_MIXED]["tracer"]
+    assert _rel(b, a, np.broadcast_to(mask[:, :, None, :], a.shape)) <= 1e-5
