@@ -181,6 +181,18 @@ swm_status swm_profile(swm_ctx* ctx, int enable);
 swm_status swm_init_synthetic(swm_ctx* ctx, int seed_index, int lmax, double umax, double h0,
                               double h_rms, int nbumps, double bump_radius);
 
+/* Global averages of budgets.F:31-247 compute_budgets at the current time
+ * level, summed over real cells in FP64 with a compensated final sum:
+ * out[0] mass, [1] kinetic energy (h ke), [2] potential energy (g h^2 / 2),
+ * [3] topographic term (g h hs), [4] total energy ([1] + [2] + [3]),
+ * [5] potential enstrophy (eta^2 / h), [6] absolute vorticity, [7]
+ * divergence, [8 + q] mass-weighted variance of tracer q (the reference
+ * reports tracers 1 and 2), each per unit area.  nout >= 8 + ntracer
+ * (at most 16 tracers).  The reference's derived outputs (fractional
+ * energy / enstrophy change, variance ratios) are taken against the caller's
+ * t = 0 values. */
+swm_status swm_budgets(swm_ctx* ctx, double* out, int nout);
+
 /* Compensated (Neumaier) FP64 global mass sum of the current thickness,
  * sum(h * area) over real cells (swm/source/budgets.F:108-109). */
 swm_status swm_total_mass(swm_ctx* ctx, double* mass);

@@ -45,6 +45,7 @@ def _load():
     lib.orc_elliptic.restype = ctypes.c_int
     lib.orc_elliptic.argtypes = [vp, ctypes.c_int, vp, vp]
     lib.orc_wrap.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    lib.orc_budgets.argtypes = [vp, vp]
     return lib
 
 
@@ -65,7 +66,7 @@ class _OrcGrid(ctypes.Structure):
                                        "v_weights", "s_weights", "l_weights", "f", "hs", "center_mask")] + [
         ("nface", ctypes.c_int), ("nvrtx", ctypes.c_int), ("nedge", ctypes.c_int)] + [
         (n, ctypes.c_void_p) for n in ("face_dst", "face_src", "vrtx_dst", "vrtx_src", "edge_dst", "edge_src")] + [
-        ("lap_w", ctypes.c_void_p)]
+        ("lap_w", ctypes.c_void_p), ("tot_area", ctypes.c_double)]
 
 
 def _p(a):
@@ -83,6 +84,7 @@ def grid_arrays_from(grid):
     for l in range(1, grid.g + 1):
         d[f"lap_w{l}"] = grid.array(f"lap_w{l}")
     d["g"] = grid.g
+    d["tot_area"] = grid.array("tot_area")
     return d
 
 
@@ -126,6 +128,7 @@ class Oracle:
         for l in range(1, self.g + 1):
             ptrs[l - 1] = f64(f"lap_w{l}", arrays[f"lap_w{l}"])
         gr.lap_w = ctypes.cast(ptrs, ctypes.c_void_p)
+        gr.tot_area = float(arrays["tot_area"])
         self._keep, self._ptrs = keep, ptrs
         self._h = L.orc_create(ctypes.byref(gr))
 
@@ -163,6 +166,14 @@ class Oracle:
         assert x.dtype == np.float64 and x.flags.f_contiguous
         lib().orc_wrap(self._h, {"face": 0, "vrtx": 1, "edge": 2}[kind], nspace, _p(x))
         return x
+
+    def budgets(self):
+        """compute_budgets global averages (budgets.F:31-247): mass, ke, pe,
+        tf, te, potential enstrophy, eta, div, then one mass-weighted
+        variance per tracer."""
+        out = np.zeros(8 + self.ntracer)
+        lib().orc_budgets(self._h, _p(out))
+        return out
 
     def multigrid(self, beta, alpha):
         nvar = beta.shape[2]

@@ -16,13 +16,17 @@
 !   wrap   : halo-exchange probes (unique-id fields)  -> outdir/wrap.bin
 !   ops    : one application of each operator         -> outdir/ops.bin
 !   mg     : one V-cycle and one elliptic solve       -> outdir/mg.bin
-!   run    : <icfile> <dt> <nsteps> <ndump>           -> outdir/run.bin
+!   run    : <icfile> <dt> <nsteps> <ndump> [scheme] [budgets]
 !            integrates nsteps, dumping the state every ndump steps
+!            (outdir/run.bin); scheme fct | centered | upwind; with the
+!            word budgets, the reference's compute_budgets runs at step 0
+!            and at every dump (its records go to unit 31, fort.31)
 !
 ! All files are raw little-endian stream records of the Fortran arrays in
 ! column-major order; the Python side reshapes them with order='F'.
 
 program ref_harness
+  use budgets, only: compute_budgets
   use kinds
   use params_physical
   use params_grid_horizontal_xxx
@@ -255,6 +259,7 @@ contains
     real(kind=dbl_kind) :: dtin
     real(kind=dbl_kind), allocatable :: h0(:,:,:), hs0(:,:,:), u0(:,:,:,:,:), t0(:,:,:,:)
     character(len=512) :: icfile
+    logical :: do_budgets
 
     call get_command_argument(3, icfile)
     call get_command_argument(4, arg); read(arg,*) dtin
@@ -299,15 +304,24 @@ contains
     tracer(:,:,:,2,:) = tracer(:,:,:,1,:)
 
     call wrap_prognostics()
+    do_budgets = .false.
+    if (command_argument_count() >= 8) then
+      call get_command_argument(8, arg)
+      do_budgets = trim(arg) == 'budgets'
+    end if
     call open_out('run.bin')
     call dump_state(0)
+    if (do_budgets) call compute_budgets()
     do istep = 1, nsteps
       call time_manager()
       call get_pointers()
       call get_weights()
       call step_dt()
       call wrap_prognostics()
-      if (mod(istep, ndump) == 0) call dump_state(istep)
+      if (mod(istep, ndump) == 0) then
+        call dump_state(istep)
+        if (do_budgets) call compute_budgets()
+      end if
     enddo
     close(fu)
   end subroutine do_run

@@ -17,6 +17,7 @@ struct orc_model {
   size_t ns;             /* slots = iim*jjm*10 */
   double dt, grav;
   int scheme;
+  double tot_area;
   double *area, *area_inv, *cai, *sub_area, *dc, *normal, *tangent, *vw, *sw, *lw, *f, *hs, *mask;
   int nface, nvrtx, nedge;
   int32_t *face_dst, *face_src, *vrtx_dst, *vrtx_src, *edge_dst, *edge_src;
@@ -843,6 +844,7 @@ orc_model* orc_create(const orc_grid* gr) {
   m->dt = gr->dt;
   m->grav = gr->grav;
   m->scheme = gr->scheme;
+  m->tot_area = gr->tot_area;
   const size_t ns = m->ns;
   m->area = dup(gr->area, ns);
   m->area_inv = dup(gr->area_inv, ns);
@@ -1096,6 +1098,41 @@ int orc_step(orc_model* m, int nsteps) {
 }
 
 int orc_last_vcycles(const orc_model* m) { return m->last_vcycles; }
+
+/* budgets.F:31-247 compute_budgets; global_real_sum (global_reductions.F:
+ * 77-98) is sum(X*MASK) in memory order */
+void orc_budgets(const orc_model* m, double* out) {
+  const size_t ns = m->ns;
+  const int n4 = m->n4 - 1;
+  const double* h = m->h[n4];
+  const double *a = m->area, *mk = m->mask;
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (size_t n = 0; n < ns; ++n) {
+    s[0] += (h[n] * a[n]) * mk[n];
+    s[1] += ((h[n] * m->ke[n]) * a[n]) * mk[n];
+    s[2] += ((((0.5 * m->grav) * h[n]) * h[n]) * a[n]) * mk[n];
+    s[3] += (((m->grav * h[n]) * m->hs[n]) * a[n]) * mk[n];
+    const double pe = mk[n] > 0.0 ? ((m->eta[n4][n] * m->eta[n4][n]) / h[n]) * a[n] : 0.0;
+    s[5] += pe * mk[n];
+    s[6] += (m->eta[n4][n] * a[n]) * mk[n];
+    s[7] += (m->div[n4][n] * a[n]) * mk[n];
+  }
+  const double inv = 1.0 / m->tot_area;
+  out[0] = s[0] * inv;
+  out[1] = s[1] * inv;
+  out[2] = s[2] * inv;
+  out[3] = s[3] * inv;
+  out[4] = out[1] + out[2] + out[3];
+  out[5] = s[5] * inv;
+  out[6] = s[6] * inv;
+  out[7] = s[7] * inv;
+  for (int q = 0; q < m->nt; ++q) {
+    const double* t = m->tr[n4][q];
+    double acc = 0.0;
+    for (size_t n = 0; n < ns; ++n) acc += (((t[n] * t[n]) * h[n]) * a[n]) * mk[n];
+    out[8 + q] = acc * inv;
+  }
+}
 
 int orc_get(const orc_model* m, const char* name, double* out) {
   const size_t ns = m->ns;

@@ -36,6 +36,7 @@ typedef struct {
   int nface, nvrtx, nedge;
   const int32_t *face_dst, *face_src, *vrtx_dst, *vrtx_src, *edge_dst, *edge_src;
   const double* const* lap_w;
+  double tot_area;   /* total area of the real cells (grid_metrics.F) */
 } orc_grid;
 
 orc_model* orc_create(const orc_grid* grid);
@@ -54,6 +55,12 @@ int orc_step(orc_model* m, int nsteps);
 /* Current-level fields: "h","eta","div","relative","ke","psi","chi" (cell),
  * "u" (2,2,iim,jjm,10), "tracer" (iim,jjm,ntracer,10). */
 int orc_get(const orc_model* m, const char* name, double* out);
+
+/* Global averages of budgets.F:31-247 compute_budgets at the current level:
+ * out[0..7] = mass, kinetic, potential, topographic, total energy, potential
+ * enstrophy, absolute vorticity, divergence; out[8+q] = mass-weighted
+ * variance of tracer q (the reference reports tracers 1 and 2). */
+void orc_budgets(const orc_model* m, double* out);
 
 /* Number of V-cycles run by the last elliptic solve (2 + extra). */
 int orc_last_vcycles(const orc_model* m);

@@ -110,6 +110,7 @@ struct swm_ctx {
   double* part_slots = nullptr;   // per-block partial sums, one block per 256 slots, two fields
   double* r2 = nullptr;
   double *part = nullptr, *sums = nullptr;
+  double* part_budget = nullptr;   // partial sums of the budgets, (8 + BUDGET_MAXTR) x npart
   double* host_sums = nullptr;   // pinned
   int npart = 0;
   // time stepping
@@ -830,6 +831,7 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
   C->xout = take(2 * ns); C->r2 = take(ns); C->xalt = take(2 * ns);
   C->part_slots = take(2 * (size_t)nblk(ns));
   C->part = take(2 * (size_t)C->npart); C->sums = take(64);
+  C->part_budget = take((8 + swm::dev::BUDGET_MAXTR) * (size_t)C->npart);
   layout_view<double>(C, C->d, nullptr);
   C->d.geo = swm::dev::Geo{C->iim, C->jjm, C->ns, C->area, C->area_inv, C->cai, C->sub_area, C->dc, C->nrm, C->tng,
                            C->vw, C->sw, C->lw, C->f, C->hs, C->mask, 0.0, nullptr};
@@ -850,6 +852,21 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
 swm_status up64(double* d, const std::vector<double>& h) {
   HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
   return SWM_OK;
+}
+
+template <class R>
+void launch_budgets(swm_ctx* C, View<R>& V, int nq) {
+  const int n4 = C->n4 - 1;
+  swm::dev::BudgetIn<R> B{};
+  B.h = V.h[n4]; B.ke = V.ke; B.eta = V.eta[n4]; B.div = V.div[n4];
+  for (int q = 0; q < C->nt && q < swm::dev::BUDGET_MAXTR; ++q) B.tr[q] = V.tr[n4][q];
+  B.area = C->area; B.mask = C->mask; B.hs = C->hs;
+  B.f = sizeof(R) == 4 ? C->f : nullptr;
+  B.h0 = sizeof(R) == 4 ? C->h0 : 0.0;
+  B.grav = C->grav;
+  B.ns = C->ns;
+  hipLaunchKernelGGL(swm::dev::k_budgets<R>, dim3(C->npart, nq), dim3(BLK), 0, C->stream, B, C->part_budget);
+  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(nq), dim3(256), 0, C->stream, C->part_budget, C->npart, C->sums);
 }
 
 swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
@@ -1197,6 +1214,29 @@ swm_status swm_last_residual(const swm_ctx* C, double* resid_ms, double* rhs_ms)
     if (resid_ms) resid_ms[v] = C->last_resid[v];
     if (rhs_ms) rhs_ms[v] = C->last_rhs_ms[v];
   }
+  return SWM_OK;
+}
+
+
+swm_status swm_budgets(swm_ctx* C, double* out, int nout) {
+  if (!C || !out) return SWM_ERR_ARG;
+  const int nt = std::min(C->nt, (int)swm::dev::BUDGET_MAXTR);
+  const int nq = 8 + nt;
+  if (nout < nq) {
+    swm::set_error("swm_budgets: out needs 8 + ntracer entries (at most 8 + 16)");
+    return SWM_ERR_ARG;
+  }
+  if (!C->initialized) { swm::set_error("swm_budgets: state not initialised"); return SWM_ERR_STATE; }
+  HIP_TRY(hipSetDevice(C->device));
+  if (C->precision == SWM_MIXED) swm::launch_budgets<float>(C, C->m, nq);
+  else swm::launch_budgets<double>(C, C->d, nq);
+  std::vector<double> sums(nq);
+  HIP_TRY(hipMemcpyAsync(sums.data(), C->sums, nq * sizeof(double), hipMemcpyDeviceToHost, C->stream));
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  HIP_TRY(hipGetLastError());
+  const double inv = 1.0 / C->total_area;
+  for (int q = 0; q < nq; ++q) out[q] = sums[q] * inv;
+  out[4] = out[1] + out[2] + out[3];
   return SWM_OK;
 }
 

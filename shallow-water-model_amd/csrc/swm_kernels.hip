@@ -1040,6 +1040,57 @@ __global__ void k_inv_h(GeoT<R> G, const R* __restrict__ h4, R* __restrict__ inv
   inv_h[s] = G.mask[s] == R(1) ? R(1) / habs(G, h4[s]) : R(0);
 }
 
+// ------------------------------------------------------------- budgets ---
+// budgets.F:31-247 compute_budgets integrands summed over real cells, one
+// quantity per blockIdx.y: 0 h, 1 h ke, 2 g h^2 / 2, 3 g h hs, 4 (unused:
+// the total energy is formed from 1..3), 5 eta^2 / h, 6 eta, 7 div, 8 + q the
+// mass-weighted square of tracer q; each times the cell area.  Per-block
+// partial sums in FP64 (stored values converted to absolute FP64 first).
+constexpr int BUDGET_MAXTR = 16;
+template <class R>
+struct BudgetIn {
+  const R *h, *ke, *eta, *div;
+  const R* tr[BUDGET_MAXTR];
+  const double *area, *mask, *hs, *f;   // f: mixed path (eta stored less f), else null
+  double h0, grav;
+  int ns;
+};
+template <class R>
+__global__ void __launch_bounds__(256) k_budgets(BudgetIn<R> B, double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int q = blockIdx.y;
+  double acc = 0.0;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < B.ns; s += gridDim.x * blockDim.x) {
+    const double m = B.mask[s];
+    if (m == 0.0) continue;
+    const double h = (double)B.h[s] + B.h0, a = B.area[s];
+    const double eta = (double)B.eta[s] + (B.f ? B.f[s] : 0.0);
+    double w;
+    switch (q) {
+      case 0: w = h * a; break;
+      case 1: w = (h * (double)B.ke[s]) * a; break;
+      case 2: w = (((0.5 * B.grav) * h) * h) * a; break;
+      case 3: w = ((B.grav * h) * B.hs[s]) * a; break;
+      case 5: w = ((eta * eta) / h) * a; break;
+      case 6: w = eta * a; break;
+      case 7: w = (double)B.div[s] * a; break;
+      case 4: w = 0.0; break;
+      default: {
+        const double t = (double)B.tr[q - 8][s];
+        w = ((t * t) * h) * a;
+      }
+    }
+    acc += w * m;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[q * gridDim.x + blockIdx.x] = sh[0];
+}
+
 // FP64 state -> FP32 storage of the mixed path: y = x - off (off may be null)
 __global__ void k_to_f32(size_t n, const double* __restrict__ x, const double* __restrict__ off, double c,
                          float* __restrict__ y) {

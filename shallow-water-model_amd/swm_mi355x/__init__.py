@@ -86,6 +86,7 @@ _sig("swm_last_vcycles", ctypes.c_int, [ctypes.c_void_p])
 _sig("swm_total_mass", ctypes.c_int, [ctypes.c_void_p, _dp])
 _sig("swm_set_solver", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int])
 _sig("swm_last_residual", ctypes.c_int, [ctypes.c_void_p, _dp, _dp])
+_sig("swm_budgets", ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int])
 _sig("swm_solver_stats", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 3)
 _sig("swm_profile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_timing", ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _c_int_p])
@@ -325,6 +326,15 @@ class Model:
         b = (ctypes.c_double * 2)()
         _check(lib.swm_last_residual(self._h, r, b))
         return list(r), list(b)
+
+    def budgets(self):
+        """compute_budgets global averages (budgets.F:31-247): mass, ke, pe,
+        tf, te, potential enstrophy, eta, div, then one mass-weighted variance
+        per tracer (at most 16)."""
+        n = 8 + min(self.ntracer, 16)
+        out = (ctypes.c_double * n)()
+        _check(lib.swm_budgets(self._h, out, n))
+        return np.array(out)
 
     def solver_stats(self):
         """(elliptic solves, V-cycles, residual checks) since initialisation."""

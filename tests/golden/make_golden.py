@@ -17,9 +17,14 @@ and stores its outputs as compressed npz fixtures:
                    full model (g = 3, dt = 1800 s; g = 4, dt = 900 s), 5 tracers
   run_g4_<scheme>.npz  the same with the centred (l_centered) and upwind
                    (l_upwind) advection of step_dt (g = 4, dt = 400 s)
+  budgets_g4.npz   the reference's compute_budgets (swm/source/budgets.F) at
+                   steps 0..12 of the FCT run at g = 4, dt = 400 s: its 14
+                   real*4 output records (unit 31) and the 9-digit values it
+                   prints (mass, log10 energy / enstrophy change, tracer
+                   variance ratios)
 
     python tests/golden/make_golden.py            # everything
-    python tests/golden/make_golden.py schemes    # only run_g4_<scheme>.npz
+    python tests/golden/make_golden.py schemes    # only run_g4_<scheme>.npz and budgets_g4.npz
 """
 import os
 import subprocess
@@ -155,6 +160,32 @@ def scheme_runs():
                 keep[f"s{st}_{k}"] = v
         np.savez_compressed(os.path.join(HERE, f"run_g4_{scheme}.npz"), **keep)
         print("run", g, scheme, flush=True)
+    # budgets along the FCT run
+    f31 = os.path.join(d, "fort.31")
+    if os.path.exists(f31):
+        os.remove(f31)
+    out = subprocess.run([exe, "run", "out", "ic.bin", repr(dt), str(nsteps), "1", "fct", "budgets"], cwd=d,
+                         check=True, capture_output=True, text=True).stdout
+    raw = open(f31, "rb").read()
+    recs = []
+    o = 0
+    while o < len(raw):
+        n = int(np.frombuffer(raw[o:o + 4], dtype="<i4")[0])
+        recs.append(np.frombuffer(raw[o + 4:o + 4 + n], dtype="<f4").copy())
+        o += n + 8
+    keys = ("Total Mass", "Log10 of Frac Energy Change", "Log10 of Frac Pot. Enstrophy Change",
+            "Fractional mass-weighted (tracer 1)^2", "Fractional mass-weighted (tracer 2)^2")
+    printed = {k: [] for k in keys}
+    for line in out.splitlines():
+        for k in keys:
+            if line.strip().startswith(k):
+                printed[k].append(float(line.split(":")[-1]))
+    np.savez_compressed(os.path.join(HERE, "budgets_g4.npz"), records=np.array(recs),
+                        mass=np.array(printed[keys[0]]), lte_diff=np.array(printed[keys[1]]),
+                        lpenst_diff=np.array(printed[keys[2]]), mtr1sq_diff=np.array(printed[keys[3]]),
+                        mtr2sq_diff=np.array(printed[keys[4]]), dt=np.array(dt), nsteps=np.array(nsteps),
+                        **{f"ic_{k}": np.asarray(v) for k, v in ic.items()})
+    print("budgets", len(recs), flush=True)
 
 
 def main():

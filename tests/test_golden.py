@@ -177,3 +177,37 @@ def test_oracle_advection_schemes_bit_exact(scheme):
         for name in ("h", "eta", "div", "psi", "chi", "ke", "relative", "u", "tracer"):
             mine = orc.field(name).reshape(-1, order="F")
             assert np.array_equal(mine, ref[f"s{s}_{name}"]), f"{scheme} step {s} {name}"
+
+
+def test_oracle_budgets_against_reference():
+    """compute_budgets of the reference (budgets.F:31-247) along 12 FCT steps at
+    g = 4: its real*4 records (relative 1e-6) and the values it prints with 9
+    digits (mass to 1e-8; log10 energy / enstrophy change and tracer variance
+    ratios, which the reference derives from its t = 0 values)."""
+    ref = gold("budgets_g4.npz")
+    rec = ref["records"]
+    nt, dt = 5, float(ref["dt"])
+    grid = grid_for(4)
+    orc = Oracle(grid_arrays_from(grid), nt, dt, S.REF_GRAV)
+    orc.init_state(ref["ic_h"], ref["ic_u"], ref["ic_tracer"])
+    b0 = None
+    for s in range(int(ref["nsteps"]) + 1):
+        if s:
+            orc.step(1)
+        b = orc.budgets()
+        mine = [b[0], b[1], b[2], b[3], b[4], b[5], b[6], b[7]]
+        cols = [0, 1, 2, 3, 4, 7, 10, 11]
+        for v, c in zip(mine, cols):
+            r = float(rec[s, c])
+            assert abs(v - r) <= 1e-6 * abs(r) + 1e-30, (s, c, v, r)
+        assert abs(b[0] - ref["mass"][s]) <= 1e-8 * ref["mass"][s]
+        if b0 is None:
+            b0 = b
+            continue
+        eps = 1.0e-15
+        te_diff = abs(b[4] - b0[4]) / (b0[4] + eps)
+        pe_diff = abs(b[5] - b0[5]) / (b0[5] + eps)
+        assert abs(np.log10(te_diff + eps) - ref["lte_diff"][s]) <= 1e-6 * abs(ref["lte_diff"][s])
+        assert abs(np.log10(pe_diff + eps) - ref["lpenst_diff"][s]) <= 1e-6 * abs(ref["lpenst_diff"][s])
+        assert abs(b[8] / b0[8] - ref["mtr1sq_diff"][s]) <= 1e-8
+        assert abs(b[9] / b0[9] - ref["mtr2sq_diff"][s]) <= 1e-8

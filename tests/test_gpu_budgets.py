@@ -1,0 +1,59 @@
+"""Device budgets (swm_budgets) against the reference's compute_budgets output
+(budgets.F:31-247, golden records of the compiled reference) and the oracle.
+
+Bars: against the oracle, relative 1e-12 (the device sums in another,
+compensated order); against the reference's real*4 records, relative 1e-6;
+the mixed path against the FP64 path after 12 steps, relative 1e-6 for mass
+and energy."""
+import os
+
+import numpy as np
+import pytest
+
+import swm_mi355x as S
+from oracle.oracle import Oracle, grid_arrays_from
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _grid4():
+    return S.Grid(4, p_list=np.load(os.path.join(GOLD, "setup_g4.npz"))["p_list"])
+
+
+def test_budgets_match_reference_and_oracle():
+    print("case budgets g=4 fct 12 steps", flush=True)
+    ref = np.load(os.path.join(GOLD, "budgets_g4.npz"))
+    rec = ref["records"]
+    dt, nt = float(ref["dt"]), 5
+    grid = _grid4()
+    model = S.Model(grid, ntracer=nt, dt=dt, grav=S.REF_GRAV)
+    model.init_state(ref["ic_h"], ref["ic_u"], ref["ic_tracer"], hs=ref["ic_hs"])
+    orc = Oracle(grid_arrays_from(grid), nt, dt, S.REF_GRAV)
+    orc.init_state(ref["ic_h"], ref["ic_u"], ref["ic_tracer"])
+    cols = [0, 1, 2, 3, 4, 7, 10, 11]
+    for s in range(int(ref["nsteps"]) + 1):
+        if s:
+            model.step(1)
+            orc.step(1)
+        b, o = model.budgets(), orc.budgets()
+        for q in range(len(o)):
+            assert abs(b[q] - o[q]) <= 1e-12 * max(abs(o[q]), 1e-300) + 1e-28, (s, q, b[q], o[q])
+        for q, c in enumerate(cols):
+            r = float(rec[s, c])
+            assert abs(b[q] - r) <= 1e-6 * abs(r) + 1e-30, (s, c, b[q], r)
+
+
+def test_mixed_budgets_close_to_fp64():
+    ref = np.load(os.path.join(GOLD, "budgets_g4.npz"))
+    grid = _grid4()
+    out = {}
+    for prec in (S.SWM_FP64, S.SWM_MIXED):
+        m = S.Model(grid, ntracer=5, dt=400.0, grav=S.REF_GRAV, precision=prec)
+        m.init_state(ref["ic_h"], ref["ic_u"], ref["ic_tracer"], hs=ref["ic_hs"])
+        m.step(12)
+        out[prec] = m.budgets()
+    a, b = out[S.SWM_FP64], out[S.SWM_MIXED]
+    for q in (0, 1, 2, 4, 5, 8, 9):
+        assert abs(a[q] - b[q]) <= 1e-6 * abs(a[q]), (q, a[q], b[q])
