@@ -198,6 +198,49 @@ swm_status swm_budgets(swm_ctx* ctx, double* out, int nout);
 swm_status swm_total_mass(swm_ctx* ctx, double* mass);
 
 /* ------------------------------------------------------------------------
+ * Restart files: the direct-access format of utilities/layer_2/io_utilities.F
+ * as swm/source/swm_restart.F writes and reads it (FP64 values, one record
+ * per global cell holding H, TRACER(ntracer) and U(2,2), a time stamp
+ * record), interchangeable with the reference's files in both directions.
+ * ------------------------------------------------------------------------ */
+/* Model time in whole seconds, nsecond_dyn of time_manager_module.F: zero
+ * after swm_init_state, the file's stamp after swm_read_restart, and
+ * advanced by dt, truncated to an integer, every step
+ * (time_manager_module.F:96).  -1 for a null context. */
+long long swm_time_seconds(const swm_ctx* ctx);
+
+/* write_swm_restart (swm/source/swm_restart.F:113-175): the current time
+ * level and the model time.  path NULL writes the reference's file name for
+ * the model time, "restarts/swm_restart.dDDDDDD.hh:mm:ss" (the directory
+ * must exist). */
+swm_status swm_write_restart(swm_ctx* ctx, const char* path);
+
+/* read_swm_restart (swm/source/swm_restart.F:25-81): replaces the state with
+ * the file's and derives vorticity, divergence and kinetic energy as
+ * swm_init_state does; the surface height stays as set before (the
+ * reference sets it from its initial conditions, swm/source/initialize.F:
+ * 222-231), the model time takes the file's stamp and the step counter
+ * restarts at zero, so the next step is a forward Euler step as in the
+ * reference.  path NULL reads "restarts/swm_restart". */
+swm_status swm_read_restart(swm_ctx* ctx, const char* path);
+
+/* write_swm_restart's file name for a model time (swm_restart.F:122-145),
+ * NUL-terminated into buf (len >= 38). */
+swm_status swm_restart_name(long long nsecond, char* buf, int len);
+
+/* The file codec alone, on host arrays in the reference layout (h
+ * (iim,jjm,10), u (2,2,iim,jjm,10), tracer (iim,jjm,ntracer,10)); needs no
+ * device.  The writer takes each global cell's values from its real slot
+ * (grid_center_mask 1); the reader fills every slot, halos included, from
+ * its global cell's record (grid_center_index), and accepts FP32 (real_bytes
+ * 4) and non-restart files (time stamp 0).  A file of another grid or
+ * tracer count is refused with SWM_ERR_DIMS. */
+swm_status swm_restart_file_write(const swm_grid* grid, const char* path, int ntracer, long long nsecond,
+                                  const double* h, const double* u, const double* tracer);
+swm_status swm_restart_file_read(const swm_grid* grid, const char* path, int ntracer, long long* nsecond,
+                                 double* h, double* u, double* tracer);
+
+/* ------------------------------------------------------------------------
  * Reference operators (utilities/layer_2/operators.F), one entry point per
  * subroutine with the subroutine's own argument list, in its order, plus
  * nsdm: the arrays hold nsdm panels (the reference calls the subroutines

@@ -16,17 +16,23 @@
 !   wrap   : halo-exchange probes (unique-id fields)  -> outdir/wrap.bin
 !   ops    : one application of each operator         -> outdir/ops.bin
 !   mg     : one V-cycle and one elliptic solve       -> outdir/mg.bin
-!   run    : <icfile> <dt> <nsteps> <ndump> [scheme] [budgets]
+!   run    : <icfile> <dt> <nsteps> <ndump> [scheme] [budgets] [wrestart] [rrestart]
 !            integrates nsteps, dumping the state every ndump steps
 !            (outdir/run.bin); scheme fct | centered | upwind; with the
 !            word budgets, the reference's compute_budgets runs at step 0
-!            and at every dump (its records go to unit 31, fort.31)
+!            and at every dump (its records go to unit 31, fort.31); with
+!            wrestart, write_swm_restart runs after the last step (file
+!            restarts/swm_restart.dDDDDDD.hh:mm:ss); with rrestart,
+!            read_swm_restart replaces h, u and tracers of icfile by those
+!            of restarts/swm_restart after the initial state is set, as
+!            initialize.F:222-231 does (swm/source/swm_restart.F)
 !
 ! All files are raw little-endian stream records of the Fortran arrays in
 ! column-major order; the Python side reshapes them with order='F'.
 
 program ref_harness
   use budgets, only: compute_budgets
+  use swm_restart, only: read_swm_restart, write_swm_restart
   use kinds
   use params_physical
   use params_grid_horizontal_xxx
@@ -259,7 +265,8 @@ contains
     real(kind=dbl_kind) :: dtin
     real(kind=dbl_kind), allocatable :: h0(:,:,:), hs0(:,:,:), u0(:,:,:,:,:), t0(:,:,:,:)
     character(len=512) :: icfile
-    logical :: do_budgets
+    logical :: do_budgets, do_wrestart
+    integer :: iarg
 
     call get_command_argument(3, icfile)
     call get_command_argument(4, arg); read(arg,*) dtin
@@ -303,12 +310,15 @@ contains
     enddo
     tracer(:,:,:,2,:) = tracer(:,:,:,1,:)
 
-    call wrap_prognostics()
     do_budgets = .false.
-    if (command_argument_count() >= 8) then
-      call get_command_argument(8, arg)
-      do_budgets = trim(arg) == 'budgets'
-    end if
+    do_wrestart = .false.
+    do iarg = 8, command_argument_count()
+      call get_command_argument(iarg, arg)
+      if (trim(arg) == 'budgets') do_budgets = .true.
+      if (trim(arg) == 'wrestart') do_wrestart = .true.
+      if (trim(arg) == 'rrestart') call read_swm_restart()
+    end do
+    call wrap_prognostics()
     call open_out('run.bin')
     call dump_state(0)
     if (do_budgets) call compute_budgets()
@@ -324,6 +334,7 @@ contains
       end if
     enddo
     close(fu)
+    if (do_wrestart) call write_swm_restart()
   end subroutine do_run
 
   subroutine dump_state(istep)

@@ -20,7 +20,7 @@ OUT_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(OUT_DIR, "libswm_mi355x.so")
 FORTRAN_CHECK = os.path.join(OUT_DIR, "swm_fortran_check")
 
-HOST_SOURCES = ["swm_setup.cpp", "swm_metrics.cpp", "swm_capi_grid.cpp"]
+HOST_SOURCES = ["swm_setup.cpp", "swm_metrics.cpp", "swm_capi_grid.cpp", "swm_restart.cpp"]
 DEVICE_SOURCES = ["swm_device.hip"]
 
 

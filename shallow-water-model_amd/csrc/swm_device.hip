@@ -115,6 +115,10 @@ struct swm_ctx {
   int npart = 0;
   // time stepping
   int step_count = 0, n3 = 1, n4 = 2, n1f = 1, n2f = 2, n3f = 3;
+  long long nsecond = 0;   // model time in whole seconds (time_manager_module.F nsecond_dyn)
+  // global cell numbering and real-cell mask of the slots (restart files)
+  std::vector<int32_t> cell_index;
+  std::vector<double> cell_mask;
   double w1 = 0, w2 = 0, w3 = 0;
   int last_vcycles = 0;
   bool initialized = false;
@@ -622,13 +626,18 @@ struct SplitMix {
   }
 };
 
-// zero the whole state (the reference's module arrays start at zero)
-void reset_state(swm_ctx* C) {
+// zero the whole state (the reference's module arrays start at zero), the
+// solver's warm start included, and the clock and step counter
+// (time_manager_module.F:60-72); keep_hs leaves the surface height as it is
+void reset_state(swm_ctx* C, bool keep_hs = false) {
   C->solves = C->vcycles_total = C->checks_total = 0;
   fill<double>(C, C->d.state_begin, C->d.state_words, 0.0);
   if (C->precision == SWM_MIXED) fill<float>(C, C->m.state_begin, C->m.state_words, 0.0f);
-  fill<double>(C, C->hs, C->ns, 0.0);
+  if (!keep_hs) fill<double>(C, C->hs, C->ns, 0.0);
+  fill<double>(C, C->xout, 2 * (size_t)C->ns, 0.0);
+  fill<double>(C, C->xalt, 2 * (size_t)C->ns, 0.0);
   C->step_count = 0;
+  C->nsecond = 0;
   C->initialized = false;
   pointers(C);
 }
@@ -987,6 +996,8 @@ swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out
   C->advection = cfg->advection;
   C->npart = 1024;
   C->radius = Gh.radius;
+  C->cell_index = Gh.center_index;
+  C->cell_mask = Gh.center_mask;
   swm_status st = swm::create_device(C, Gh);
   if (st) {
     swm_destroy(C);
@@ -1010,8 +1021,11 @@ void swm_destroy(swm_ctx* C) {
   delete C;
 }
 
-swm_status swm_init_state(swm_ctx* C, const double* h, const double* hs, const double* u, const double* tracer) {
-  if (!C || !h || !u || (C->nt > 0 && !tracer)) { swm::set_error("swm_init_state: bad argument"); return SWM_ERR_ARG; }
+namespace swm {
+// state from host arrays in the reference layout; hs == nullptr with keep_hs
+// leaves the surface height as it is, without keep_hs sets it to zero
+swm_status init_from_host(swm_ctx* C, const double* h, const double* hs, const double* u, const double* tracer,
+                          bool keep_hs) {
   HIP_TRY(hipSetDevice(C->device));
   const int ns = C->ns;
   hipStream_t S = C->stream;
@@ -1021,7 +1035,7 @@ swm_status swm_init_state(swm_ctx* C, const double* h, const double* hs, const d
     for (int k = 0; k < 10; ++k)
       std::memcpy(&tq[(size_t)q * ns + (size_t)k * C->iim * C->jjm],
                   tracer + ((size_t)q + (size_t)C->nt * k) * C->iim * C->jjm, (size_t)C->iim * C->jjm * sizeof(double));
-  reset_state(C);
+  reset_state(C, keep_hs);
   HIP_TRY(hipMemcpyAsync(C->d.h[0], h, ns * sizeof(double), hipMemcpyHostToDevice, S));
   HIP_TRY(hipMemcpyAsync(C->d.u[0], up.data(), 4 * (size_t)ns * sizeof(double), hipMemcpyHostToDevice, S));
   if (hs) HIP_TRY(hipMemcpyAsync(C->hs, hs, ns * sizeof(double), hipMemcpyHostToDevice, S));
@@ -1030,6 +1044,53 @@ swm_status swm_init_state(swm_ctx* C, const double* h, const double* hs, const d
   HIP_TRY(hipStreamSynchronize(S));
   return finish_init(C);
 }
+}  // namespace swm
+
+swm_status swm_init_state(swm_ctx* C, const double* h, const double* hs, const double* u, const double* tracer) {
+  if (!C || !h || !u || (C->nt > 0 && !tracer)) { swm::set_error("swm_init_state: bad argument"); return SWM_ERR_ARG; }
+  return swm::init_from_host(C, h, hs, u, tracer, false);
+}
+
+// write_swm_restart (swm/source/swm_restart.F:113-175): the current time
+// level's h, tracers and corner velocity of every real cell and the clock
+swm_status swm_write_restart(swm_ctx* C, const char* path) {
+  if (!C) { swm::set_error("swm_write_restart: bad argument"); return SWM_ERR_ARG; }
+  if (!C->initialized) { swm::set_error("swm_write_restart: state not initialised"); return SWM_ERR_STATE; }
+  char name[64];
+  if (!path) {
+    swm_status st = swm::restart_name(C->nsecond, name, (int)sizeof name);
+    if (st) return st;
+    path = name;
+  }
+  const size_t ns = C->ns;
+  std::vector<double> h(ns), u(4 * ns), tr(std::max<size_t>(1, (size_t)C->nt * ns));
+  swm_status st;
+  if ((st = swm_get_field(C, "h", h.data())) || (st = swm_get_field(C, "u", u.data())) ||
+      (C->nt > 0 && (st = swm_get_field(C, "tracer", tr.data()))))
+    return st;
+  const swm::CellIndex ci{C->iim, C->jjm, C->max_ig, C->cell_index.data(), C->cell_mask.data()};
+  return swm::restart_write(ci, path, C->nt, C->nsecond, h.data(), u.data(), tr.data());
+}
+
+// read_swm_restart (swm/source/swm_restart.F:25-81): h, tracers and corner
+// velocity of every slot from its global cell's record, then the state
+// derived as at initialisation; the surface height stays as it is, the clock
+// takes the file's time stamp and the step counter restarts at zero
+swm_status swm_read_restart(swm_ctx* C, const char* path) {
+  if (!C) { swm::set_error("swm_read_restart: bad argument"); return SWM_ERR_ARG; }
+  if (!path) path = "restarts/swm_restart";
+  const size_t ns = C->ns;
+  std::vector<double> h(ns, 0.0), u(4 * ns, 0.0), tr(std::max<size_t>(1, (size_t)C->nt * ns), 0.0);
+  const swm::CellIndex ci{C->iim, C->jjm, C->max_ig, C->cell_index.data(), C->cell_mask.data()};
+  long long ts = 0;
+  swm_status st = swm::restart_read(ci, path, C->nt, &ts, h.data(), u.data(), tr.data());
+  if (st) return st;
+  if ((st = swm::init_from_host(C, h.data(), nullptr, u.data(), tr.data(), true))) return st;
+  C->nsecond = ts;
+  return SWM_OK;
+}
+
+long long swm_time_seconds(const swm_ctx* C) { return C ? C->nsecond : -1; }
 
 // Seeded synthetic state generated on the device (same construction as the
 // host version in swm_mi355x/ics.py): harmonic sums of degree 1..lmax for
@@ -1114,6 +1175,7 @@ swm_status swm_step(swm_ctx* C, int nsteps) {
   HIP_TRY(hipEventRecord(C->ev_a, C->stream));
   for (int n = 0; n < nsteps; ++n) {
     C->step_count += 1;
+    C->nsecond = (long long)((double)C->nsecond + C->dt);   // time_manager_module.F:96, truncated
     pointers(C);
     weights(C);
     swm_status st;

@@ -5,7 +5,7 @@
 ! laplacian_of_a_scalar and centered_vorticity_flux panel by panel as
 ! swm/source/step.F does.
 !
-!   swm_fortran_check <input> <output>
+!   swm_fortran_check <input> <output> [restart]
 !
 ! input  (stream): int32 g, ntracer, nsteps; float64 dt, grav, radius,
 !        omega; h, hs (iim,jjm,10); u (2,2,iim,jjm,10);
@@ -13,13 +13,15 @@
 ! output (stream): after nsteps steps h, eta, div, psi, chi (iim,jjm,10),
 !        u, tracer; then the Laplacian of h and the vorticity flux of
 !        (h, eta) with a unit mass flux, panel by panel
+! restart: write_swm_restart after the steps (into restarts/ of the
+!        working directory)
 !=======================================================================
 program swm_fortran_check
   use iso_c_binding
   use swm_mi355x
   use operators_mi355x
   implicit none
-  character(len=512) :: fin, fout
+  character(len=512) :: fin, fout, opt
   integer(c_int32_t) :: g, ntracer, nsteps
   real(c_double) :: dt, grav, radius, omega
   real(c_double), allocatable, target :: h(:,:,:), hs(:,:,:), u(:,:,:,:,:), tracer(:,:,:,:)
@@ -29,6 +31,8 @@ program swm_fortran_check
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
+  opt = ''
+  if (command_argument_count() >= 3) call get_command_argument(3, opt)
   open(newunit=fu, file=trim(fin), access='stream', form='unformatted', status='old')
   read(fu) g, ntracer, nsteps, dt, grav, radius, omega
   n = 2 + 2**g
@@ -42,6 +46,7 @@ program swm_fortran_check
     call step_dt()
     call wrap_prognostics()
   end do
+  if (trim(opt) == 'restart') call write_swm_restart()
   allocate(eta(n,n,10), div(n,n,10), psi(n,n,10), chi(n,n,10))
   call sync_prognostics_to_host(h, u, tracer, eta=eta, div=div, psi=psi, chi=chi)
 

@@ -129,6 +129,21 @@ module swm_mi355x_c
       real(c_double) :: mass
       integer(c_int) :: st
     end function
+    function swm_time_seconds(ctx) bind(C, name="swm_time_seconds") result(t)
+      import :: c_ptr, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_long_long) :: t
+    end function
+    function swm_write_restart(ctx, path) bind(C, name="swm_write_restart") result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, path
+      integer(c_int) :: st
+    end function
+    function swm_read_restart(ctx, path) bind(C, name="swm_read_restart") result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, path
+      integer(c_int) :: st
+    end function
     function swm_wrap(ctx, kind, nspace, nlayer, x) bind(C, name="swm_wrap") result(st)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
@@ -267,7 +282,7 @@ module swm_mi355x
   private
   public :: swm_gpu_initialize, swm_gpu_finalize, swm_gpu_check, swm_gpu_context, swm_gpu_grid_array, &
             step_dt, wrap_prognostics, sync_prognostics_to_host, elliptic, multigrid, wrap_face, &
-            wrap_vrtx, wrap_edge
+            wrap_vrtx, wrap_edge, write_swm_restart, read_swm_restart, swm_gpu_time
 
   type(c_ptr), save :: ctx = c_null_ptr
   type(c_ptr), save :: grid = c_null_ptr
@@ -378,6 +393,25 @@ contains
     if (present(chi)) call swm_gpu_check(swm_get_field(ctx, 'chi'//c_null_char, chi), 'get chi')
     if (present(ke)) call swm_gpu_check(swm_get_field(ctx, 'ke'//c_null_char, ke), 'get ke')
   end subroutine sync_prognostics_to_host
+
+  ! swm_restart::write_swm_restart (swm/source/swm_restart.F:113-175): the
+  ! device state to restarts/swm_restart.dDDDDDD.hh:mm:ss of the model time
+  subroutine write_swm_restart()
+    call swm_gpu_check(swm_write_restart(ctx, c_null_ptr), 'write_swm_restart')
+  end subroutine write_swm_restart
+
+  ! swm_restart::read_swm_restart (swm/source/swm_restart.F:25-81): the
+  ! device state and model time from restarts/swm_restart; the surface
+  ! height stays that of swm_gpu_initialize
+  subroutine read_swm_restart()
+    call swm_gpu_check(swm_read_restart(ctx, c_null_ptr), 'read_swm_restart')
+  end subroutine read_swm_restart
+
+  ! the model time in seconds (time_manager_module.F nsecond_dyn)
+  function swm_gpu_time() result(t)
+    integer(c_long_long) :: t
+    t = swm_time_seconds(ctx)
+  end function swm_gpu_time
 
   ! elliptic_solver_xxx::elliptic (component_name is not needed)
   subroutine elliptic(component_name, nvar, rhs, xout)

@@ -35,6 +35,7 @@ REF_OMEGA = 7.292e-5
 
 _STATUS = {0: "SWM_OK", 1: "SWM_ERR_ARG", 2: "SWM_ERR_DIMS", 3: "SWM_ERR_HIP", 4: "SWM_ERR_NOT_CONVERGED",
            5: "SWM_ERR_NAN", 6: "SWM_ERR_NO_DEVICE", 7: "SWM_ERR_STATE"}
+globals().update({v: k for k, v in _STATUS.items()})
 
 
 class SwmError(RuntimeError):
@@ -92,6 +93,14 @@ _sig("swm_profile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_timing", ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _c_int_p])
 _sig("swm_init_synthetic", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double])
+_sig("swm_time_seconds", ctypes.c_longlong, [ctypes.c_void_p])
+_sig("swm_write_restart", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p])
+_sig("swm_read_restart", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p])
+_sig("swm_restart_name", ctypes.c_int, [ctypes.c_longlong, ctypes.c_char_p, ctypes.c_int])
+_sig("swm_restart_file_write", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_longlong]
+     + [ctypes.c_void_p] * 3)
+_sig("swm_restart_file_read", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_longlong)] + [ctypes.c_void_p] * 3)
 _sig("swm_wrap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p])
 _sig("swm_elliptic", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
 _sig("swm_multigrid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
@@ -157,6 +166,36 @@ def grid_points(g):
     out = np.zeros((3, n), dtype=np.float64, order="F")
     _check(lib.swm_grid_points(g, _ptr(out)))
     return out
+
+
+def restart_name(nsecond):
+    """write_swm_restart's file name for a model time (swm_restart.F:122-145)."""
+    buf = ctypes.create_string_buffer(64)
+    _check(lib.swm_restart_name(int(nsecond), buf, 64))
+    return buf.value.decode()
+
+
+def write_restart_file(grid, path, h, u, tracer, nsecond=0):
+    """Writes a restart file (io_utilities.F direct-access format, as
+    swm_restart.F:113-175) from host arrays in the reference layout."""
+    s = (grid.iim, grid.jjm, 10)
+    tracer = np.asarray(tracer)
+    nt = tracer.shape[2]
+    h, u, tr = _f64(h, s), _f64(u, (2, 2) + s), _f64(tracer, (grid.iim, grid.jjm, nt, 10))
+    _check(lib.swm_restart_file_write(grid.handle, os.fsencode(path), nt, int(nsecond), _ptr(h), _ptr(u), _ptr(tr)))
+
+
+def read_restart_file(grid, path, ntracer):
+    """Reads a restart file into (nsecond, h, u, tracer), every slot filled from
+    its global cell's record (swm_restart.F:25-45)."""
+    s = (grid.iim, grid.jjm, 10)
+    h = np.zeros(s, order="F")
+    u = np.zeros((2, 2) + s, order="F")
+    tr = np.zeros((grid.iim, grid.jjm, ntracer, 10), order="F")
+    ts = ctypes.c_longlong()
+    _check(lib.swm_restart_file_read(grid.handle, os.fsencode(path), ntracer, ctypes.byref(ts), _ptr(h), _ptr(u),
+                                     _ptr(tr)))
+    return ts.value, h, u, tr
 
 
 # shapes of the named grid arrays, in units of (iim, jjm, 10) slots
@@ -356,6 +395,21 @@ class Model:
         a, b, c, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
         _check(lib.swm_timing(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(n)))
         return a.value, b.value, c.value, n.value
+
+    @property
+    def time_seconds(self):
+        """Model time in whole seconds (nsecond_dyn)."""
+        return lib.swm_time_seconds(self._h)
+
+    def write_restart(self, path=None):
+        """write_swm_restart: the current state and model time; path None uses
+        the reference's name under restarts/."""
+        _check(lib.swm_write_restart(self._h, None if path is None else os.fsencode(path)))
+
+    def read_restart(self, path=None):
+        """read_swm_restart: state and model time from a file (hs unchanged);
+        the next step is a forward step, as after any initialisation."""
+        _check(lib.swm_read_restart(self._h, None if path is None else os.fsencode(path)))
 
     def total_mass(self):
         m = ctypes.c_double()

@@ -22,9 +22,15 @@ and stores its outputs as compressed npz fixtures:
                    real*4 output records (unit 31) and the 9-digit values it
                    prints (mass, log10 energy / enstrophy change, tracer
                    variance ratios)
+  restart_g3.npz   restart files (swm/source/swm_restart.F) at g = 3 over a
+                   mountain: a file the reference wrote after 3 steps and
+                   its 4-step continuation from it; the reference's
+                   2-step continuation from a file the library wrote, and
+                   the file it wrote after those steps
 
     python tests/golden/make_golden.py            # everything
     python tests/golden/make_golden.py schemes    # only run_g4_<scheme>.npz and budgets_g4.npz
+    python tests/golden/make_golden.py restart    # only restart_g3.npz
 """
 import os
 import subprocess
@@ -188,11 +194,89 @@ def scheme_runs():
     print("budgets", len(recs), flush=True)
 
 
+def cone_state(grid, nt, seed):
+    """Synthetic state over a cone-shaped mountain (the shape of the
+    reference's test case 5, initialize.F:380-393): surface height hs, h the
+    synthetic thickness less hs."""
+    from swm_mi355x import ics
+    st = ics.synthetic_state(grid, ntracer=nt, lmax=8, seed_index=seed)
+    p = grid.array("p")
+    c = np.array([np.cos(np.pi / 6) * np.cos(-np.pi / 2), np.cos(np.pi / 6) * np.sin(-np.pi / 2), np.sin(np.pi / 6)])
+    d = np.arccos(np.clip(np.einsum("k...,k->...", p / np.linalg.norm(p, axis=0), c), -1.0, 1.0))
+    r = np.pi / 9.0
+    hs = np.asfortranarray(np.where(d < r, 2000.0 * (1.0 - d / r), 0.0))
+    st["hs"] = hs
+    st["h"] = np.asfortranarray(st["h"] - hs)
+    return st
+
+
+def restart_runs():
+    """Restart files (swm/source/swm_restart.F) at g = 3, dt = 800 s, 5 tracers,
+    over a mountain: the reference writes one after 3 steps and continues 4
+    steps from it; it also reads a file written by the library's own writer
+    (another state, time stamp 2 d 05:01:01) and continues 2 steps, writing
+    its own file at the end."""
+    import shutil
+    import swm_mi355x as S
+    g, nt, dt = 3, 5, 800.0
+    exe = harness(g)
+    points = S.grid_points(g)
+    d = rundir(g, points)
+    grid = S.Grid(g, p_list=points)
+    ic = cone_state(grid, nt, 3)
+    with open(os.path.join(d, "ic.bin"), "wb") as f:
+        for k in ("h", "hs", "u", "tracer"):
+            f.write(np.asfortranarray(ic[k]).tobytes(order="F"))
+    rdir = os.path.join(d, "restarts")
+    shutil.rmtree(rdir, ignore_errors=True)
+    os.makedirs(rdir)
+    keep = {f"ic_{k}": np.asarray(v) for k, v in ic.items()}
+    keep["dt"] = np.array(dt)
+    # A: three steps, then write_swm_restart
+    run(exe, d, "run", "out", "ic.bin", repr(dt), "3", "1", "fct", "wrestart")
+    states = read_run(os.path.join(d, "out", "run.bin"), g, nt)
+    (name_a,) = os.listdir(rdir)
+    keep["a_name"] = np.array("restarts/" + name_a)
+    keep["a_file"] = np.frombuffer(open(os.path.join(rdir, name_a), "rb").read(), dtype=np.uint8)
+    for k in ("h", "u", "tracer"):
+        keep[f"a_s3_{k}"] = states[3][k]
+    # B: read_swm_restart of A's file, four steps
+    shutil.copy(os.path.join(rdir, name_a), os.path.join(rdir, "swm_restart"))
+    os.remove(os.path.join(rdir, name_a))
+    run(exe, d, "run", "out", "ic.bin", repr(dt), "4", "1", "fct", "rrestart")
+    states = read_run(os.path.join(d, "out", "run.bin"), g, nt)
+    for st in (0, 1, 4):
+        for k, v in states[st].items():
+            keep[f"b_s{st}_{k}"] = v
+    # C: the library's writer, another state; read_swm_restart, two steps,
+    # write_swm_restart
+    other = cone_state(grid, nt, 30)
+    ts = 2 * 86400 + 5 * 3600 + 61
+    S.write_restart_file(grid, os.path.join(rdir, "swm_restart"), other["h"], other["u"], other["tracer"], ts)
+    keep["c_ts"] = np.array(ts)
+    for k in ("h", "u", "tracer"):
+        keep[f"c_in_{k}"] = np.asarray(other[k])
+    run(exe, d, "run", "out", "ic.bin", repr(dt), "2", "1", "fct", "rrestart", "wrestart")
+    states = read_run(os.path.join(d, "out", "run.bin"), g, nt)
+    for st in (0, 2):
+        for k, v in states[st].items():
+            keep[f"c_s{st}_{k}"] = v
+    names = sorted(x for x in os.listdir(rdir) if x != "swm_restart")
+    assert len(names) == 1, names
+    keep["c_name"] = np.array("restarts/" + names[0])
+    keep["c_file"] = np.frombuffer(open(os.path.join(rdir, names[0]), "rb").read(), dtype=np.uint8)
+    np.savez_compressed(os.path.join(HERE, "restart_g3.npz"), **keep)
+    print("restart", keep["a_name"], keep["c_name"], flush=True)
+
+
 def main():
     if not os.path.isdir(REF):
         sys.exit("make_golden.py needs the reference sources at " + REF)
     if sys.argv[1:] == ["schemes"]:
         scheme_runs()
+        return
+    if sys.argv[1:] == ["restart"]:
+        restart_runs()
         return
     import swm_mi355x as S
     from swm_mi355x import ics
@@ -249,6 +333,7 @@ def main():
             np.savez_compressed(os.path.join(HERE, f"run_g{g}.npz"), **keep)
             print("run", g, flush=True)
     scheme_runs()
+    restart_runs()
 
 
 if __name__ == "__main__":

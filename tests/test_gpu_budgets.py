@@ -2,9 +2,12 @@
 (budgets.F:31-247, golden records of the compiled reference) and the oracle.
 
 Bars: against the oracle, relative 1e-12 (the device sums in another,
-compensated order); against the reference's real*4 records, relative 1e-6;
-the mixed path against the FP64 path after 12 steps, relative 1e-6 for mass
-and energy."""
+compensated order); against the reference's real*4 records, relative 1e-6.
+The global means of absolute vorticity and divergence vanish analytically and
+are sums of rounding errors (about 1e-16 of the terms' size), so those two
+are held to 1e-12 of the mean magnitude of their field instead.  The mixed
+path against the FP64 path after 12 steps, relative 1e-6 for mass and
+energy."""
 import os
 
 import numpy as np
@@ -30,19 +33,24 @@ def test_budgets_match_reference_and_oracle():
     grid = _grid4()
     model = S.Model(grid, ntracer=nt, dt=dt, grav=S.REF_GRAV)
     model.init_state(ref["ic_h"], ref["ic_u"], ref["ic_tracer"], hs=ref["ic_hs"])
-    orc = Oracle(grid_arrays_from(grid), nt, dt, S.REF_GRAV)
+    orc = Oracle(grid_arrays_from(grid), nt, dt, S.REF_GRAV, hs=ref["ic_hs"])
     orc.init_state(ref["ic_h"], ref["ic_u"], ref["ic_tracer"])
+    mask = grid.array("grid_center_mask") == 1.0
     cols = [0, 1, 2, 3, 4, 7, 10, 11]
     for s in range(int(ref["nsteps"]) + 1):
         if s:
             model.step(1)
             orc.step(1)
         b, o = model.budgets(), orc.budgets()
+        # scale of the vanishing means (q = 6 eta, q = 7 div)
+        scale = {6: np.abs(orc.field("eta")[mask]).mean(), 7: np.abs(orc.field("div")[mask]).mean()}
         for q in range(len(o)):
-            assert abs(b[q] - o[q]) <= 1e-12 * max(abs(o[q]), 1e-300) + 1e-28, (s, q, b[q], o[q])
+            tol = 1e-12 * (scale[q] if q in scale else abs(o[q]))
+            assert abs(b[q] - o[q]) <= tol + 1e-300, (s, q, b[q], o[q])
         for q, c in enumerate(cols):
             r = float(rec[s, c])
-            assert abs(b[q] - r) <= 1e-6 * abs(r) + 1e-30, (s, c, b[q], r)
+            tol = 1e-12 * scale[q] if q in scale else 1e-6 * abs(r)
+            assert abs(b[q] - r) <= tol + 1e-300, (s, c, b[q], r)
 
 
 def test_mixed_budgets_close_to_fp64():

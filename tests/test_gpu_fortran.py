@@ -33,7 +33,9 @@ def test_fortran_driver_matches_library(tmp_path):
         f.write(np.array([dt, S.REF_GRAV, S.REF_RADIUS, S.REF_OMEGA], dtype="<f8").tobytes())
         for k in ("h", "hs", "u", "tracer"):
             f.write(np.asfortranarray(st[k], dtype="<f8").tobytes(order="F"))
-    r = subprocess.run([EXE, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    (tmp_path / "restarts").mkdir()
+    r = subprocess.run([EXE, str(fin), str(fout), "restart"], capture_output=True, text=True, timeout=120,
+                       cwd=str(tmp_path))
     assert r.returncode == 0, r.stdout + r.stderr
     n = grid.iim
     raw = np.fromfile(fout, dtype="<f8")
@@ -52,6 +54,16 @@ def test_fortran_driver_matches_library(tmp_path):
     model.step(nsteps)
     for name in ("h", "eta", "div", "psi", "chi", "u", "tracer"):
         assert np.array_equal(out[name], model.field(name)), name
+
+    # write_swm_restart through the Fortran module: the reference's name for
+    # the model time, the state of the real cells
+    name = S.restart_name(int(nsteps * dt))
+    ts, rh, ru, rt = S.read_restart_file(grid, str(tmp_path / name), nt)
+    assert ts == int(nsteps * dt)
+    m = grid.array("grid_center_mask") == 1.0
+    assert np.array_equal(rh[m], out["h"][m])
+    assert np.array_equal(ru[:, :, m], out["u"][:, :, m])
+    assert np.array_equal(rt.transpose(0, 1, 3, 2)[m], out["tracer"].transpose(0, 1, 3, 2)[m])
 
     orc = Oracle(grid_arrays_from(grid), nt, dt, S.REF_GRAV)
     h = model.field("h")
