@@ -126,6 +126,28 @@ void swm_destroy(swm_ctx* ctx);
 swm_status swm_init_state(swm_ctx* ctx, const double* h, const double* hs, const double* u,
                           const double* tracer);
 
+/* The reference's built-in initial state, shallow-water test case 5 (zonal
+ * flow over a mountain), initial_conditions_swtc5 (swm/source/initialize.F:
+ * 330-513, default build): h, hs (iim,jjm,10), u (2,2,iim,jjm,10), tracer
+ * (iim,jjm,ntracer,10) on the host, halos filled, ready for swm_init_state.
+ * Tracer 1 is hs, tracer 2 a plateau of 1000 over the mountain, the rest the
+ * Coriolis parameter; ntracer >= 2 as the reference requires.  Needs no
+ * device. */
+swm_status swm_swtc5_state(const swm_grid* grid, int ntracer, double grav, double* h, double* hs, double* u,
+                           double* tracer);
+
+/* The reference driver's run settings, data/namelist as model_control.F:
+ * 98-139 (input) reads it, line by line: tau_end_day, tau_end_hour, dt
+ * (F7.2), l_restart, l_upwind, l_centered, l_fct, l_remap (L1), n_restart,
+ * n_output, n_diagnostics (list-directed).  Logicals are 0 / 1.  Needs no
+ * device. */
+typedef struct swm_namelist {
+  double tau_end_day, tau_end_hour, dt;
+  int l_restart, l_upwind, l_centered, l_fct, l_remap;
+  int n_restart, n_output, n_diagnostics;
+} swm_namelist;
+swm_status swm_read_namelist(const char* path, swm_namelist* out);
+
 /* Advances nsteps time steps: for each, the time-level pointers and ABM3
  * weights (swm/source/model_control.F:28-89), step_dt and wrap_prognostics
  * (swm/source/step.F:114-212, :438-493; loop of swm/source/driver.F:90-130).

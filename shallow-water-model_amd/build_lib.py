@@ -7,7 +7,8 @@ gfx950; both without floating-point contraction, so that the FP64 path keeps
 the reference's operation order.  The Fortran modules (fortran/swm_mi355x.F90,
 ISO_C_BINDING over the C ABI) and the Fortran driver check program are
 compiled with ROCm's flang.  Output under shallow-water-model_amd/lib/:
-libswm_mi355x.so, swm_fortran_check and the Fortran module files in obj/.
+libswm_mi355x.so, swm_fortran_check, the stand-alone driver swm_driver
+(driver/swm_driver.cpp) and the Fortran module files in obj/.
 """
 import os
 import shutil
@@ -19,8 +20,9 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(OUT_DIR, "libswm_mi355x.so")
 FORTRAN_CHECK = os.path.join(OUT_DIR, "swm_fortran_check")
+DRIVER = os.path.join(OUT_DIR, "swm_driver")
 
-HOST_SOURCES = ["swm_setup.cpp", "swm_metrics.cpp", "swm_capi_grid.cpp", "swm_restart.cpp"]
+HOST_SOURCES = ["swm_setup.cpp", "swm_metrics.cpp", "swm_capi_grid.cpp", "swm_restart.cpp", "swm_ics.cpp", "swm_namelist.cpp"]
 DEVICE_SOURCES = ["swm_device.hip"]
 
 
@@ -73,6 +75,11 @@ def build(verbose=True):
     _run([flang, "-O2", "-module-dir", obj_dir, os.path.join(fdir, "swm_fortran_check.F90"), fobj,
           "-L" + OUT_DIR, "-lswm_mi355x", "-Wl,-rpath,$ORIGIN", "-o", tmp])
     os.replace(tmp, FORTRAN_CHECK)
+    # the stand-alone driver program
+    tmp = DRIVER + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-Wall", os.path.join(HERE, "driver", "swm_driver.cpp"),
+          "-L" + OUT_DIR, "-lswm_mi355x", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, DRIVER)
     return LIB
 
 

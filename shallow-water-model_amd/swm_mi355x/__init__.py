@@ -61,6 +61,12 @@ class _Config(ctypes.Structure):
                 ("precision", ctypes.c_int), ("device", ctypes.c_int), ("advection", ctypes.c_int)]
 
 
+class _Namelist(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("tau_end_day", "tau_end_hour", "dt")] + \
+               [(n, ctypes.c_int) for n in ("l_restart", "l_upwind", "l_centered", "l_fct", "l_remap", "n_restart",
+                                            "n_output", "n_diagnostics")]
+
+
 def _sig(name, res, args):
     f = getattr(lib, name)
     f.restype = res
@@ -101,6 +107,8 @@ _sig("swm_restart_file_write", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, 
      + [ctypes.c_void_p] * 3)
 _sig("swm_restart_file_read", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int,
                                              ctypes.POINTER(ctypes.c_longlong)] + [ctypes.c_void_p] * 3)
+_sig("swm_swtc5_state", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double] + [ctypes.c_void_p] * 4)
+_sig("swm_read_namelist", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_Namelist)])
 _sig("swm_wrap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p])
 _sig("swm_elliptic", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
 _sig("swm_multigrid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
@@ -173,6 +181,24 @@ def restart_name(nsecond):
     buf = ctypes.create_string_buffer(64)
     _check(lib.swm_restart_name(int(nsecond), buf, 64))
     return buf.value.decode()
+
+
+def read_namelist(path):
+    """The reference driver's data/namelist (model_control.F:98-139) as a dict."""
+    nl = _Namelist()
+    _check(lib.swm_read_namelist(os.fsencode(path), ctypes.byref(nl)))
+    return {n: getattr(nl, n) for n, _ in _Namelist._fields_}
+
+
+def swtc5_state(grid, ntracer=5, grav=REF_GRAV):
+    """Shallow-water test case 5 (initialize.F:330-513): dict(h, hs, u, tracer)
+    in the reference layout, halos filled."""
+    s = (grid.iim, grid.jjm, 10)
+    out = {"h": np.zeros(s, order="F"), "hs": np.zeros(s, order="F"), "u": np.zeros((2, 2) + s, order="F"),
+           "tracer": np.zeros((grid.iim, grid.jjm, ntracer, 10), order="F")}
+    _check(lib.swm_swtc5_state(grid.handle, ntracer, grav, _ptr(out["h"]), _ptr(out["hs"]), _ptr(out["u"]),
+                               _ptr(out["tracer"])))
+    return out
 
 
 def write_restart_file(grid, path, h, u, tracer, nsecond=0):

@@ -27,10 +27,15 @@ and stores its outputs as compressed npz fixtures:
                    its 4-step continuation from it; the reference's
                    2-step continuation from a file the library wrote, and
                    the file it wrote after those steps
+  swtc5_g6.npz     the shipped g = 6 grid and the values of the reference's
+                   log of test case 5 on it (swm/run.0040962/log.baseline):
+                   total area, initial mass, min/max fields after step 1
+  namelists/       the shipped run directories' data/namelist files
 
     python tests/golden/make_golden.py            # everything
     python tests/golden/make_golden.py schemes    # only run_g4_<scheme>.npz and budgets_g4.npz
     python tests/golden/make_golden.py restart    # only restart_g3.npz
+    python tests/golden/make_golden.py swtc5      # only swtc5_g6.npz and namelists/
 """
 import os
 import subprocess
@@ -269,6 +274,34 @@ def restart_runs():
     print("restart", keep["a_name"], keep["c_name"], flush=True)
 
 
+def swtc5_values():
+    """Shallow-water test case 5 at g = 6 on the shipped grid: its cell centres
+    and the values the reference's own log of that run holds
+    (swm/run.0040962/log.baseline, an older variant of the step: SWTC5,
+    dt = 100 s, diagnostics every step), and the shipped namelists."""
+    import re
+    import shutil
+    run = os.path.join(REF, "swm", "run.0040962")
+    pts = np.loadtxt(os.path.join(run, "data", "grid_centers_0040962")).T
+    log = open(os.path.join(run, "log.baseline")).read()
+    num = r"(-?\d+\.\d+E[+-]\d+)"
+    keep = {"p_list": pts}
+    keep["total_area"] = np.array(float(re.search(r"total area at this resolution :\s+(\d+)\.", log).group(1)))
+    keep["mass0"] = np.array(float(re.search(r"Total Mass :\s+" + num, log).group(1)))
+    # min/max block after step 1
+    blk = log.split("GLOBAL MIN AND MAX OF SELECT FIELDS")[1]
+    for key, label in (("div", "divergence"), ("relative", "rel vorticity"), ("ke", "kinetic energy"),
+                       ("h", "mass thickness"), ("u1", "eastward wind"), ("u2", "northward wind")):
+        m = re.search(r"min/max " + label + r"\s*:\s+" + num + r"\s+" + num, blk)
+        keep[f"s1_{key}"] = np.array([float(m.group(1)), float(m.group(2))])
+    np.savez_compressed(os.path.join(HERE, "swtc5_g6.npz"), **keep)
+    nd = os.path.join(HERE, "namelists")
+    os.makedirs(nd, exist_ok=True)
+    for res in ("0001442", "0002562", "0010242", "0040962", "0163842", "0655362"):
+        shutil.copy(os.path.join(REF, "swm", f"run.{res}", "data", "namelist"), os.path.join(nd, f"namelist.{res}"))
+    print("swtc5", keep["mass0"], keep["s1_relative"], flush=True)
+
+
 def main():
     if not os.path.isdir(REF):
         sys.exit("make_golden.py needs the reference sources at " + REF)
@@ -277,6 +310,9 @@ def main():
         return
     if sys.argv[1:] == ["restart"]:
         restart_runs()
+        return
+    if sys.argv[1:] == ["swtc5"]:
+        swtc5_values()
         return
     import swm_mi355x as S
     from swm_mi355x import ics
@@ -334,6 +370,7 @@ def main():
             print("run", g, flush=True)
     scheme_runs()
     restart_runs()
+    swtc5_values()
 
 
 if __name__ == "__main__":
