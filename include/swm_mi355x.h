@@ -174,6 +174,14 @@ int swm_last_vcycles(const swm_ctx* ctx);
  * counterpart in the reference, where these are compile-time constants. */
 swm_status swm_set_solver(swm_ctx* ctx, double tol, int relative, int max_cycles);
 
+/* Mixed path only: the start of each elliptic solve, the linear
+ * extrapolation 2 x_n - x_(n-1) of the last two solutions (extrapolate = 1,
+ * the default) or the last solution (0, the reference's warm start), and
+ * the number of correction cycles before the criterion is first tested
+ * (min_cycles, default 1; the reference runs 2 from its warm start).  No
+ * counterpart in the reference. */
+swm_status swm_set_mixed_start(swm_ctx* ctx, int extrapolate, int min_cycles);
+
 /* Mean squared residuals at the end of the last elliptic solve, and mean
  * squared right-hand sides (relative criterion only), per field (psi, chi). */
 swm_status swm_last_residual(const swm_ctx* ctx, double resid_ms[2], double rhs_ms[2]);

@@ -107,7 +107,10 @@ struct swm_ctx {
   double h0 = 0.0;         // mixed path: thickness offset
   double* xout = nullptr;  // streamfunction and velocity potential (FP64, both paths)
   double* xalt = nullptr;  // mixed path: second FP64 iterate of the defect correction
-  double* part_slots = nullptr;   // per-block partial sums, one block per 256 slots, two fields
+  double* xprev = nullptr;  // mixed path: the solution before xout (extrapolated warm start)
+  bool have_prev = false;   // xprev holds a solution
+  double* part_slots = nullptr;   // per-block partial sums, one block per 256 slots, up to four columns
+  double* gauge = nullptr;        // mixed path: area-weighted change of the solution's mean, two fields
   double* r2 = nullptr;
   double *part = nullptr, *sums = nullptr;
   double* part_budget = nullptr;   // partial sums of the budgets, (8 + BUDGET_MAXTR) x npart
@@ -127,6 +130,8 @@ struct swm_ctx {
   double solver_tol = 1.0e-20;
   int solver_relative = 0;
   int solver_max_cycles = 12;
+  int mixed_extrapolate = 1;   // mixed path: start from 2 x_n - x_(n-1)
+  int mixed_min_cycles = 1;    // mixed path: cycles before the first check
   double last_resid[2] = {0.0, 0.0};
   double last_rhs_ms[2] = {0.0, 0.0};
   long long solves = 0, vcycles_total = 0, checks_total = 0;   // since the last initialisation
@@ -373,14 +378,20 @@ swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
 }
 
 // Mixed path: the same solve as FP64 defect correction.  The FP32 right-hand
-// side (mean removed with FP64 sums) is the problem; the FP64 iterate lives
-// in xout / xalt by turns.  A cycle runs one FP32 V-cycle on L(e) = r from
-// e = 0, fills the halos of e, and then one pass (k_defect) adds e to the
-// iterate and forms its FP64 defect r = rhs - L(x) with the sums of r^2 for
-// the convergence test.  The defect is formed afresh in FP64 every cycle, so
-// the FP32 cycles only ever see a small, accurately known residual.  As in
-// the FP64 path: two cycles, then the criterion before each further cycle, at
-// most solver_max_cycles cycles.
+// side (mean removed with FP64 sums) is the problem.  The solve starts from
+// the linear extrapolation 2 x_n - x_(n-1) of the last two solutions (the
+// last one alone after an initialisation), which is second-order accurate in
+// dt where the reference's warm start is first-order, and forms its FP64
+// defect r = rhs - L(x).  A cycle runs one FP32 V-cycle on L(e) = r from
+// e = 0, fills the halos of e, and one pass (k_defect) adds e to the iterate
+// and forms the new FP64 defect with the sums of r^2.  One cycle, then the
+// criterion before each further cycle, at most solver_max_cycles cycles
+// (the start and the cycles before the first check are settings:
+// swm_set_mixed_start).  The defect is formed afresh in FP64 every
+// cycle, so the FP32 cycles only ever see a small, accurately known
+// residual.  The iterates move between three FP64 buffers (xout, xalt,
+// xprev), which are renamed at the end so that xout holds the solution and
+// xprev the one before.
 swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
   const int ns = C->ns;
   View<float>& V = C->m;
@@ -398,28 +409,27 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
                      C->total_area, rhs, rhs + ns);
   swm_status st0 = rhs_mean_squares<float>(C, rhs, 2);
   if (st0) return st0;
-  // current iterate of each field (xout or xalt)
-  double* cur[2] = {xout, xout + ns};
-  double* oth[2] = {C->xalt, C->xalt + ns};
-  auto launch_defect = [&](int nv, int v0, bool corr) {
+  // the extrapolated start goes to xalt; then the iterates alternate between
+  // xalt and xprev, whose content is no longer needed once it is read
+  double* blk[3] = {xout, C->xalt, C->xprev};
+  double* cur[2] = {blk[1], blk[1] + ns};
+  double* oth[2] = {blk[2], blk[2] + ns};
+  // the mean change of the extrapolated start still to be taken off, per field
+  bool shift[2] = {false, false};
+  const double inv_area = 1.0 / C->total_area;
+  auto launch_defect = [&](int nv, int v0) {
     const size_t o = (size_t)v0 * ns;
-    if (nv == 2) {
-      if (corr)
-        hipLaunchKernelGGL((swm::dev::k_defect<2, true>), dim3(nb), dim3(BLK), 0, C->stream, G64, cur[0], e, oth[0],
-                           rhs, r, C->part_slots);
-      else
-        hipLaunchKernelGGL((swm::dev::k_defect<2, false>), dim3(nb), dim3(BLK), 0, C->stream, G64, cur[0], e, oth[0],
-                           rhs, r, C->part_slots);
-    } else {
-      if (corr)
-        hipLaunchKernelGGL((swm::dev::k_defect<1, true>), dim3(nb), dim3(BLK), 0, C->stream, G64, cur[v0], e + o,
-                           oth[v0], rhs + o, r + o, C->part_slots);
-      else
-        hipLaunchKernelGGL((swm::dev::k_defect<1, false>), dim3(nb), dim3(BLK), 0, C->stream, G64, cur[v0], e + o,
-                           oth[v0], rhs + o, r + o, C->part_slots);
+    const double* gg = shift[v0] ? C->gauge + v0 : nullptr;
+    if (nv == 2)
+      hipLaunchKernelGGL((swm::dev::k_defect<2, true>), dim3(nb), dim3(BLK), 0, C->stream, G64, cur[0], e, oth[0],
+                         rhs, r, C->part_slots, gg, inv_area);
+    else
+      hipLaunchKernelGGL((swm::dev::k_defect<1, true>), dim3(nb), dim3(BLK), 0, C->stream, G64, cur[v0], e + o,
+                         oth[v0], rhs + o, r + o, C->part_slots, gg, inv_area);
+    for (int v = v0; v < v0 + nv; ++v) {
+      std::swap(cur[v], oth[v]);
+      shift[v] = false;
     }
-    if (corr)
-      for (int v = v0; v < v0 + nv; ++v) std::swap(cur[v], oth[v]);
   };
   // one FP32 correction cycle on fields v0.. (nv of them: both, or one)
   auto cycle = [&](int nv, int v0) {
@@ -427,7 +437,7 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
     vcycle<float>(C, V, nv, e + (size_t)v0 * ns, r + (size_t)v0 * ns, true);
     if (nv == 2) gather<float>(C, C->face, {e, e + ns});
     else gather<float>(C, C->face, {e + (size_t)v0 * ns});
-    launch_defect(nv, v0, true);
+    launch_defect(nv, v0);
   };
   auto norms = [&](int nv, double* out) -> swm_status {
     hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(nv), dim3(256), 0, C->stream, C->part_slots, (int)nb, C->sums);
@@ -437,8 +447,15 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
     for (int b = 0; b < nv; ++b) out[b] = raw[b] / (double)C->max_ig;
     return SWM_OK;
   };
-  launch_defect(2, 0, false);
-  for (int it = 0; it < 2; ++it) cycle(2, 0);
+  const bool extrap = C->have_prev && C->mixed_extrapolate;
+  hipLaunchKernelGGL(swm::dev::k_defect_extrap<2>, dim3(nb), dim3(BLK), 0, C->stream, G64, C->area_mask, blk[0],
+                     extrap ? blk[2] : blk[0], blk[1], rhs, r, C->part_slots);
+  if (extrap) {
+    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(256), 0, C->stream, C->part_slots + 2 * (size_t)nb,
+                       (int)nb, C->gauge);
+    shift[0] = shift[1] = true;
+  }
+  for (int it = 0; it < C->mixed_min_cycles; ++it) cycle(2, 0);
   double nrm[2];
   swm_status st = norms(2, nrm);
   if (st) return st;
@@ -447,9 +464,9 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
     C->last_resid[v] = nrm[v];
     if (!converged(C, v, nrm[v])) bad.push_back(v);
   }
-  int iter = 0;
+  int iter = C->mixed_min_cycles;
   bool any = !bad.empty();
-  while (any && iter < C->solver_max_cycles - 2) {
+  while (any && iter < C->solver_max_cycles) {
     ++iter;
     const int nv = (int)bad.size(), v0 = bad[0];
     if (nv == 2 && cur[1] != cur[0] + ns) {
@@ -468,12 +485,23 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
       if (!converged(C, bad[b], nbv[b])) any = true;
     }
   }
-  // the result belongs in xout
+  // a field that converged without a correction keeps the extrapolated
+  // mean change: taken off here
   for (int v = 0; v < 2; ++v)
-    if (cur[v] != xout + (size_t)v * ns)
-      hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, C->stream, xout + (size_t)v * ns, cur[v], (size_t)ns);
-  C->last_vcycles = 2 + iter;
-  C->solves += 1; C->vcycles_total += 2 + iter; C->checks_total += 1 + iter;
+    if (shift[v])
+      hipLaunchKernelGGL(swm::dev::k_add_scaled, dim3(nb), dim3(BLK), 0, C->stream, (size_t)ns, C->gauge + v,
+                         -inv_area, cur[v]);
+  // both fields into one buffer (the other field's plane there is free),
+  // which becomes xout; the previous solution becomes xprev
+  if (cur[1] != cur[0] + ns)
+    hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, C->stream, cur[0] + ns, cur[1], (size_t)ns);
+  double* fin = cur[0];
+  C->xprev = blk[0];
+  C->xout = fin;
+  C->xalt = fin == blk[1] ? blk[2] : blk[1];
+  C->have_prev = true;
+  C->last_vcycles = iter;
+  C->solves += 1; C->vcycles_total += iter; C->checks_total += 1 + iter;
   if (any) {
     swm::set_error("elliptic solve did not converge (residual above the criterion after " +
                    std::to_string(C->solver_max_cycles) + " correction cycles)");
@@ -636,6 +664,8 @@ void reset_state(swm_ctx* C, bool keep_hs = false) {
   if (!keep_hs) fill<double>(C, C->hs, C->ns, 0.0);
   fill<double>(C, C->xout, 2 * (size_t)C->ns, 0.0);
   fill<double>(C, C->xalt, 2 * (size_t)C->ns, 0.0);
+  fill<double>(C, C->xprev, 2 * (size_t)C->ns, 0.0);
+  C->have_prev = false;
   C->step_count = 0;
   C->nsecond = 0;
   C->initialized = false;
@@ -837,8 +867,8 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
   make_table(C, Gh.face, C->ns, C->face);
   make_table(C, Gh.vrtx, C->ns, C->vrtx);
   make_table(C, Gh.edge, C->ns, C->edge);
-  C->xout = take(2 * ns); C->r2 = take(ns); C->xalt = take(2 * ns);
-  C->part_slots = take(2 * (size_t)nblk(ns));
+  C->xout = take(2 * ns); C->r2 = take(ns); C->xalt = take(2 * ns); C->xprev = take(2 * ns); C->gauge = take(2);
+  C->part_slots = take(4 * (size_t)nblk(ns));
   C->part = take(2 * (size_t)C->npart); C->sums = take(64);
   C->part_budget = take((8 + swm::dev::BUDGET_MAXTR) * (size_t)C->npart);
   layout_view<double>(C, C->d, nullptr);
@@ -1267,6 +1297,16 @@ swm_status swm_set_solver(swm_ctx* C, double tol, int relative, int max_cycles) 
   C->solver_tol = tol;
   C->solver_relative = relative ? 1 : 0;
   C->solver_max_cycles = max_cycles;
+  return SWM_OK;
+}
+
+swm_status swm_set_mixed_start(swm_ctx* C, int extrapolate, int min_cycles) {
+  if (!C || min_cycles < 0 || min_cycles > C->solver_max_cycles) {
+    swm::set_error("swm_set_mixed_start: need 0 <= min_cycles <= max_cycles");
+    return SWM_ERR_ARG;
+  }
+  C->mixed_extrapolate = extrapolate ? 1 : 0;
+  C->mixed_min_cycles = min_cycles;
   return SWM_OK;
 }
 

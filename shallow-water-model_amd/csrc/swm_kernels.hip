@@ -525,6 +525,51 @@ __device__ __forceinline__ R lap_at(int iim, int jjm, int ns, const IJK& c, cons
   return R(0);
 }
 
+// lap_at's stencil at one slot with the weights loaded once for several
+// fields: 7 terms in lap_at's summation order (centre, then neighbours 1..6).
+// A term a pole lacks gets weight 0 on the centre slot, which leaves every
+// partial sum unchanged, so apply() returns lap_at's value to the bit; off
+// the real cells the operator is 0 and nothing is loaded.
+template <class R>
+struct Stencil7 {
+  bool real;
+  int sl[7];
+  R w[7];
+  template <class FX>
+  __device__ __forceinline__ R apply(FX X) const {
+    if (!real) return R(0);
+    R v = w[0] * X(sl[0]);
+    v = v + w[1] * X(sl[1]);
+    v = v + w[2] * X(sl[2]);
+    v = v + w[3] * X(sl[3]);
+    v = v + w[4] * X(sl[4]);
+    v = v + w[5] * X(sl[5]);
+    v = v + w[6] * X(sl[6]);
+    return v;
+  }
+};
+
+template <class R>
+__device__ __forceinline__ Stencil7<R> stencil7(int iim, int jjm, int ns, const IJK& c, const R* __restrict__ lw) {
+  const int i = c.i, j = c.j, k = c.k, s = SL(i, j, k);
+  const bool in = i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1;
+  const bool np = i == 1 && j == jjm - 1, sp = i == iim - 1 && j == 1;
+  Stencil7<R> t;
+  t.real = in || np || sp;
+#pragma unroll
+  for (int q = 0; q < 7; ++q) { t.sl[q] = s; t.w[q] = R(0); }
+  if (!t.real) return t;
+  t.w[0] = lw[6 * ns + s];
+  if (!np) { t.sl[1] = SL(i - 1, j, k); t.w[1] = lw[s]; }
+  t.sl[2] = np ? SL(iim, 1, k) : sp ? SL(1, jjm, k) : SL(i - 1, j - 1, k);
+  t.w[2] = lw[ns + s];
+  if (!sp) { t.sl[3] = SL(i, j - 1, k); t.w[3] = lw[2 * ns + s]; }
+  t.sl[4] = SL(i + 1, j, k); t.w[4] = lw[3 * ns + s];
+  t.sl[5] = SL(i + 1, j + 1, k); t.w[5] = lw[4 * ns + s];
+  t.sl[6] = SL(i, j + 1, k); t.w[6] = lw[5 * ns + s];
+  return t;
+}
+
 // The same Laplacian written on differences from the centre value, for the
 // mixed path: the weights sum to zero, so sum_k w_k (x_k - x_0) is the same
 // operator, and rounding the weights to FP32 then errs relative to the
@@ -703,11 +748,12 @@ __global__ void __launch_bounds__(256) k_resid2(Geo G, const double* __restrict_
   if (s < ns) {
     const IJK c = ijk_of(s, iim, jjm);
     const double m = G.mask[s];
+    const Stencil7<double> st = stencil7<double>(iim, jjm, ns, c, G.lw);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const double* xv = x + (size_t)v * ns;
       auto X = [&](int sl) { return xv[sl]; };
-      const double w = lap_at<double>(iim, jjm, ns, c, G.lw, X);
+      const double w = st.apply(X);
       const double r = (w - 0.0 * xv[s]) - rhs[(size_t)v * ns + s];
       acc[v] = (r * r) * m;
     }
@@ -725,7 +771,8 @@ __global__ void __launch_bounds__(256) k_resid2(Geo G, const double* __restrict_
 template <int NV, bool CORR>
 __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict__ x, const float* __restrict__ e,
                                                 double* __restrict__ xn, const float* __restrict__ b,
-                                                float* __restrict__ r, double* __restrict__ part) {
+                                                float* __restrict__ r, double* __restrict__ part,
+                                                const double* __restrict__ gauge, double inv_area) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   double acc[NV];
@@ -734,19 +781,65 @@ __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict_
   if (s < ns) {
     const IJK c = ijk_of(s, iim, jjm);
     const double m = G.mask[s];
+    const Stencil7<double> st = stencil7<double>(iim, jjm, ns, c, G.lw);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const double* xv = x + (size_t)v * ns;
       const float* ev = e + (size_t)v * ns;
       auto X = [&](int sl) { return CORR ? xv[sl] + (double)ev[sl] : xv[sl]; };
-      const double w = lap_at<double>(iim, jjm, ns, c, G.lw, X);
+      const double w = st.apply(X);
       const double d = m != 0.0 ? (double)b[(size_t)v * ns + s] - w : 0.0;
-      if (CORR) xn[(size_t)v * ns + s] = X(s);
+      if (CORR) xn[(size_t)v * ns + s] = gauge ? X(s) - gauge[v] * inv_area : X(s);
       r[(size_t)v * ns + s] = (float)d;
       acc[v] = (d * d) * m;
     }
   }
   block_sums<NV>(acc, part);
+}
+
+// Mixed path, first FP64 defect of a solve from the extrapolated warm start
+// x0 = 2 x - xp (x the last solution, xp the one before; xp = x gives the
+// plain warm start exactly): x0 is stored to xn on every slot and
+// d = rhs - L(x0) as in k_defect.  The solution's constant part is free (L
+// of a constant is zero), so extrapolating it would let it drift; the
+// per-block sums of area * (x - xp) (columns NV.. of part) give the mean
+// change, which the first correction takes off again (k_defect's gauge).
+// With gauge (k_defect) the stored iterate loses gauge[v] * inv_area; the
+// defect, of the unshifted iterate, is the same up to rounding.
+template <int NV>
+__global__ void __launch_bounds__(256) k_defect_extrap(Geo G, const double* __restrict__ am,
+                                                       const double* __restrict__ x, const double* __restrict__ xp,
+                                                       double* __restrict__ xn, const float* __restrict__ b,
+                                                       float* __restrict__ r, double* __restrict__ part) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  double acc[2 * NV];
+#pragma unroll
+  for (int v = 0; v < 2 * NV; ++v) acc[v] = 0.0;
+  if (s < ns) {
+    const IJK c = ijk_of(s, iim, jjm);
+    const double m = G.mask[s];
+    const Stencil7<double> st = stencil7<double>(iim, jjm, ns, c, G.lw);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double* xv = x + (size_t)v * ns;
+      const double* pv = xp + (size_t)v * ns;
+      auto X = [&](int sl) { return 2.0 * xv[sl] - pv[sl]; };
+      const double w = st.apply(X);
+      const double d = m != 0.0 ? (double)b[(size_t)v * ns + s] - w : 0.0;
+      xn[(size_t)v * ns + s] = X(s);
+      r[(size_t)v * ns + s] = (float)d;
+      acc[v] = (d * d) * m;
+      acc[NV + v] = am[s] * (xv[s] - pv[s]);
+    }
+  }
+  block_sums<2 * NV>(acc, part);
+}
+
+// x += scale * (*g) on n values
+__global__ void k_add_scaled(size_t n, const double* __restrict__ g, double scale, double* __restrict__ x) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] = x[t] + scale * g[0];
 }
 
 // x += e (FP64 solution, FP32 correction), nvar planes of ns

@@ -93,6 +93,7 @@ _sig("swm_last_vcycles", ctypes.c_int, [ctypes.c_void_p])
 _sig("swm_total_mass", ctypes.c_int, [ctypes.c_void_p, _dp])
 _sig("swm_set_solver", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int])
 _sig("swm_last_residual", ctypes.c_int, [ctypes.c_void_p, _dp, _dp])
+_sig("swm_set_mixed_start", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
 _sig("swm_budgets", ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int])
 _sig("swm_solver_stats", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 3)
 _sig("swm_profile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
@@ -383,6 +384,11 @@ class Model:
         """Convergence settings of the elliptic solve; the defaults are the
         reference's (mean squared residual <= 1e-20 within 12 V-cycles)."""
         _check(lib.swm_set_solver(self._h, tol, 1 if relative else 0, max_cycles))
+
+    def set_mixed_start(self, extrapolate=True, min_cycles=1):
+        """Mixed path: extrapolated (2 x_n - x_(n-1)) or plain warm start, and
+        the correction cycles before the first convergence test."""
+        _check(lib.swm_set_mixed_start(self._h, 1 if extrapolate else 0, min_cycles))
 
     def last_residual(self):
         """(mean squared residuals, mean squared right-hand sides) of psi and
