@@ -279,7 +279,7 @@ void vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_
 template <class R>
 void sum2(swm_ctx* C, const R* x0, const R* x1, const R* w, int n) {
   hipLaunchKernelGGL(swm::dev::k_sum_partial<R>, dim3(C->npart, 2), dim3(BLK), 0, C->stream, x0, x1, w, n, C->part);
-  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(256), 0, C->stream, C->part, C->npart, C->sums);
+  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part, C->npart, C->sums);
 }
 
 swm_status read_sums(swm_ctx* C, int n, double* out) {
@@ -304,7 +304,7 @@ swm_status rhs_mean_squares(swm_ctx* C, const R* rhs, int nvar) {
   const int ns = C->ns;
   hipLaunchKernelGGL(swm::dev::k_sumsq_mixed<R>, dim3(C->npart, nvar), dim3(BLK), 0, C->stream, rhs,
                      rhs + (size_t)(nvar - 1) * ns, C->mask, ns, C->part);
-  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(nvar), dim3(256), 0, C->stream, C->part, C->npart, C->sums);
+  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(nvar), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part, C->npart, C->sums);
   double ms[2];
   swm_status st = read_sums(C, nvar, ms);
   if (st) return st;
@@ -332,7 +332,7 @@ swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
     else
       hipLaunchKernelGGL(swm::dev::k_resid2<1>, dim3(nb), dim3(BLK), 0, C->stream, V.geo, xout + (size_t)v0 * ns,
                          rhs + (size_t)v0 * ns, C->part_slots);
-    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3((unsigned)vars.size()), dim3(256), 0, C->stream, C->part_slots,
+    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3((unsigned)vars.size()), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part_slots,
                        (int)nb, C->sums);
     double raw[2];
     swm_status st = read_sums(C, (int)vars.size(), raw);
@@ -404,7 +404,7 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
   // area-weighted means in FP64 (elliptic_solver.F:80-92)
   hipLaunchKernelGGL((swm::dev::k_sum_partial<float, double>), dim3(C->npart, 2), dim3(BLK), 0, C->stream, rhs,
                      rhs + ns, C->area_mask, ns, C->part);
-  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(256), 0, C->stream, C->part, C->npart, C->sums);
+  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part, C->npart, C->sums);
   hipLaunchKernelGGL(swm::dev::k_sub_mean<float>, dim3(nblk(ns)), dim3(BLK), 0, C->stream, ns, C->sums,
                      C->total_area, rhs, rhs + ns);
   swm_status st0 = rhs_mean_squares<float>(C, rhs, 2);
@@ -440,7 +440,7 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
     launch_defect(nv, v0);
   };
   auto norms = [&](int nv, double* out) -> swm_status {
-    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(nv), dim3(256), 0, C->stream, C->part_slots, (int)nb, C->sums);
+    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(nv), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part_slots, (int)nb, C->sums);
     double raw[2];
     swm_status st = read_sums(C, nv, raw);
     if (st) return st;
@@ -451,7 +451,7 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
   hipLaunchKernelGGL(swm::dev::k_defect_extrap<2>, dim3(nb), dim3(BLK), 0, C->stream, G64, C->area_mask, blk[0],
                      extrap ? blk[2] : blk[0], blk[1], rhs, r, C->part_slots);
   if (extrap) {
-    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(256), 0, C->stream, C->part_slots + 2 * (size_t)nb,
+    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part_slots + 2 * (size_t)nb,
                        (int)nb, C->gauge);
     shift[0] = shift[1] = true;
   }
@@ -905,7 +905,7 @@ void launch_budgets(swm_ctx* C, View<R>& V, int nq) {
   B.grav = C->grav;
   B.ns = C->ns;
   hipLaunchKernelGGL(swm::dev::k_budgets<R>, dim3(C->npart, nq), dim3(BLK), 0, C->stream, B, C->part_budget);
-  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(nq), dim3(256), 0, C->stream, C->part_budget, C->npart, C->sums);
+  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(nq), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part_budget, C->npart, C->sums);
 }
 
 swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
@@ -1156,7 +1156,7 @@ swm_status swm_init_synthetic(swm_ctx* C, int seed_index, int lmax, double umax,
   hipLaunchKernelGGL(swm::dev::k_scale, dim3(nb), dim3(BLK), 0, S, chi, (size_t)ns, a2);
   auto sumsq = [&](const double* x, double* out) -> swm_status {
     hipLaunchKernelGGL(swm::dev::k_sumsq_partial, dim3(C->npart), dim3(BLK), 0, S, x, C->mask, ns, C->part);
-    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(1), dim3(256), 0, S, C->part, C->npart, C->sums);
+    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(1), dim3(swm::dev::SUM_THREADS), 0, S, C->part, C->npart, C->sums);
     HIP_TRY(hipMemcpyAsync(C->host_sums, C->sums, sizeof(double), hipMemcpyDeviceToHost, S));
     HIP_TRY(hipStreamSynchronize(S));
     *out = C->host_sums[0];
@@ -1388,7 +1388,7 @@ swm_status swm_total_mass(swm_ctx* C, double* mass) {
     // sum of (h' + h0) area = sum h' area + h0 * total area
     hipLaunchKernelGGL((swm::dev::k_sum_partial<float, double>), dim3(C->npart, 2), dim3(BLK), 0, C->stream,
                        C->m.h[n4], C->m.h[n4], C->area_mask, C->ns, C->part);
-    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(256), 0, C->stream, C->part, C->npart, C->sums);
+    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part, C->npart, C->sums);
     swm_status st = read_sums(C, 2, m);
     if (st) return st;
     *mass = m[0] + C->h0 * C->total_area;

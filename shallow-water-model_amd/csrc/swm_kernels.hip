@@ -683,15 +683,17 @@ __device__ __forceinline__ void two_sum_add(double& s, double& c, double v) {
   c += (s - (t - bp)) + (v - bp);
   s = t;
 }
-__global__ void __launch_bounds__(256) k_sum_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
-  __shared__ double ss[256], cc[256];
+constexpr int SUM_THREADS = 1024;
+__global__ void __launch_bounds__(SUM_THREADS) k_sum_final(const double* __restrict__ part, int nb,
+                                                           double* __restrict__ out) {
+  __shared__ double ss[SUM_THREADS], cc[SUM_THREADS];
   const int f = blockIdx.x, t = threadIdx.x;
   double s = 0.0, c = 0.0;
-  for (int b = t; b < nb; b += 256) two_sum_add(s, c, part[f * nb + b]);
+  for (int b = t; b < nb; b += SUM_THREADS) two_sum_add(s, c, part[f * nb + b]);
   ss[t] = s;
   cc[t] = c;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
+  for (int o = SUM_THREADS / 2; o > 0; o >>= 1) {
     if (t < o) {
       double s1 = ss[t], c1 = cc[t] + cc[t + o];
       two_sum_add(s1, c1, ss[t + o]);
@@ -1384,7 +1386,7 @@ __device__ __forceinline__ int co_ns(int l) { return (2 + (1 << l)) * (2 + (1 <<
 // relax nsw sweeps at level L: x (LDS, result) and t (LDS, scratch); b is the
 // right-hand side (LDS or global)
 template <class R>
-__device__ void co_relax(const LevT<R>& L, int nvar, int nsw, R* x, R* t, const R* b) {
+__device__ __forceinline__ void co_relax(const LevT<R>& L, int nvar, int nsw, R* x, R* t, const R* b) {
   const int iim = L.iim, jjm = L.jjm, ns = L.ns;
   const int nitem = nvar * ns;
   int kind[CO_MAXI], self[CO_MAXI], out_i[CO_MAXI], nb[CO_MAXI][6];
@@ -1448,7 +1450,7 @@ __device__ void co_relax(const LevT<R>& L, int nvar, int nsw, R* x, R* t, const 
 
 // residual at odd-odd fine points of level F -> right-hand side of level Cc
 template <class R>
-__device__ void co_inject(const LevT<R>& F, const LevT<R>& Cc, int nvar, const R* xa, const R* xb, R* x1) {
+__device__ __forceinline__ void co_inject(const LevT<R>& F, const LevT<R>& Cc, int nvar, const R* xa, const R* xb, R* x1) {
   const int nitem = nvar * Cc.ns;
   for (int it = threadIdx.x; it < nitem; it += CO_THREADS) {
     const int v = it / Cc.ns, s1 = it - v * Cc.ns;
@@ -1491,7 +1493,7 @@ __device__ void co_inject(const LevT<R>& F, const LevT<R>& Cc, int nvar, const R
 
 // coarse correction of level Cc added into level F
 template <class R>
-__device__ void co_interp(const LevT<R>& Cc, const LevT<R>& F, int nvar, const R* x1, R* x2) {
+__device__ __forceinline__ void co_interp(const LevT<R>& Cc, const LevT<R>& F, int nvar, const R* x1, R* x2) {
   const int nitem = nvar * F.ns;
   for (int it = threadIdx.x; it < nitem; it += CO_THREADS) {
     const int v = it / F.ns, s = it - v * F.ns;
