@@ -105,6 +105,85 @@ def timed(model, warmup, steps):
             "vcycles_per_step": (v1 - v0) / max(1, s1 - s0), "checks_per_step": (c1 - c0) / max(1, s1 - s0)}
 
 
+def adjoint_bench(S, g=7, hours=6.0, every=16, ndir=8):
+    """The adjoint form: FP64, centred advection, a window of `hours` at
+    level g (dt = 200 * 2^(5-g) s), snapshots every `every` steps.  Control:
+    relative vorticity, divergence and thickness of the seeded synthetic
+    state (degrees 1..40); objective: the area-weighted sum of (h - 5000)^2
+    over a cap of radius 1500 km.  Times the window's forward run, the
+    adjoint (segment recomputation + adjoint steps + initial-state map) and
+    the tangent-linear run, and checks the gradient by central differences
+    along ndir seeded directions and by a dot-product test."""
+    import numpy as np
+    from swm_mi355x import adjoint as A
+    dt = 200.0 * 2.0 ** (5 - g)
+    nsteps = int(round(hours * 3600.0 / dt))
+    grid = S.Grid(g, radius=RADIUS, omega=OMEGA)
+
+    def model(tight=True):
+        m = S.Model(grid, ntracer=1, dt=dt, grav=GRAV, advection="centered")
+        if tight:   # solves to the FP64 floor, so that the model is differentiable to round-off
+            m.set_solver(tol=0.0, relative=True, max_cycles=60)
+        return m
+
+    syn = model(False)
+    syn.init_synthetic(seed_index=3, lmax=40)
+    control = [np.asfortranarray(syn.field(n)) for n in ("relative", "div", "h")]
+    mask = grid.array("grid_center_mask") == 1.0
+    area = grid.array("area")
+    cap = A.cap_mask(grid, np.pi / 6, -np.pi / 2, 1500e3, RADIUS)
+
+    def run_ms(m):
+        m.synchronize()
+        t = time.perf_counter()
+        m.step(nsteps)
+        m.synchronize()
+        return (time.perf_counter() - t) * 1e3
+
+    ref = model(False)
+    ref.init_vorticity(*control)
+    t_ref = run_ms(ref)
+    m = model()
+    m.init_vorticity(*control)
+    m.synchronize()
+    t0 = time.perf_counter()
+    m.adj_forward(nsteps, every)
+    m.synchronize()
+    t_fwd = (time.perf_counter() - t0) * 1e3
+    hN = m.field("h")
+    lh = np.where(cap, 2.0 * area * (hN - 5000.0), 0.0)
+    t0 = time.perf_counter()
+    grad = m.adj_run(lh, control=True)
+    t_adj = (time.perf_counter() - t0) * 1e3
+    rng = np.random.default_rng(7)
+    s = (grid.iim, grid.jjm, 10)
+    dc = [np.where(mask, np.sqrt(np.mean(c[mask] ** 2)) * rng.standard_normal(s), 0.0) for c in control]
+    t0 = time.perf_counter()
+    tl = m.tl_run(control=dc)
+    t_tl = (time.perf_counter() - t0) * 1e3
+    y = [rng.standard_normal(s), 1e6 * rng.standard_normal(s), 1e7 * rng.standard_normal(s),
+         10 * rng.standard_normal((2, 2) + s)]
+    a = sum(float(np.sum(u * v)) for u, v in zip(tl, y))
+    b = sum(float(np.sum(u * v)) for u, v in zip(dc, m.adj_run(*y, control=True)))
+    fd = A.finite_difference_check(model, control, grad, lambda h: A.cap_objective(h, area, cap, 5000.0),
+                                   nsteps, ndir, 8, mask)
+    return {"workload": f"icosahedral level {g} ({grid.max_ig} cells), FP64 centred advection, dt={dt} s, "
+                        f"{hours:g} h window ({nsteps} steps), snapshots every {every} steps with recomputation; "
+                        f"J = sum over a 1500 km cap of area (h - 5000)^2, gradient with respect to initial "
+                        f"vorticity, divergence and thickness",
+            "g": g, "steps": nsteps, "checkpoint_every": every,
+            "forward_ms": t_fwd, "adjoint_ms": t_adj, "tl_ms": t_tl, "adj_over_fwd": t_adj / t_fwd,
+            "tl_over_fwd": t_tl / t_fwd,
+            "forward_ms_reference_criterion": t_ref, "adj_over_fwd_reference_criterion": t_adj / t_ref,
+            "vcycles_per_solve": {"tight": m.solver_stats()[1] / max(1, m.solver_stats()[0]),
+                                  "reference_criterion": ref.solver_stats()[1] / max(1, ref.solver_stats()[0])},
+            "dot_product_rel": abs(a - b) / abs(a),
+            "fd_rel_errors": [r[2] for r in fd],
+            "note": "forward and adjoint solve to the FP64 floor (relative residual ~1e-12, a cycle no longer "
+                    "halves it), so the gradient is the model's to round-off; the reference criterion "
+                    "(mean(r^2) <= 1e-20) takes ~2 V-cycles per step instead of ~9"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +195,7 @@ def main():
                     help="steps of the mixed-vs-FP64 error run (default: one simulated day; 0 skips it)")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint form (level 7, 6 h window)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -209,6 +289,13 @@ def main():
                          f"initial state, {tcpu:.1f} s on one host core; oracle/swm_oracle.c (gcc -O2), the C "
                          f"restatement of the reference step, bit-identical to the compiled reference"}
 
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_bench(S)
+        print(f"[bench] adjoint {adj['adjoint_ms']:.0f} ms vs forward {adj['forward_ms']:.0f} ms "
+              f"(ratio {adj['adj_over_fwd']:.2f}), dot {adj['dot_product_rel']:.1e}, "
+              f"fd max {max(adj['fd_rel_errors']):.1e}", file=sys.stderr, flush=True)
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -232,7 +319,7 @@ def main():
             "mixed_over_fp64": value / value64,
             "vcycles_per_step": {"mixed": tm["vcycles_per_step"], "fp64": tf["vcycles_per_step"]},
             "mixed_error_vs_fp64": err,
-            "adjoint": None,
+            "adjoint": adj,
         },
         "roofline": {"bound": "hbm", "kernel": "k_relax<float> finest level (mixed path)", "achieved": ach,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS if ach else None,

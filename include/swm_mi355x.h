@@ -170,8 +170,10 @@ int swm_last_vcycles(const swm_ctx* ctx);
  * defaults: mean(r^2) over real cells <= 1e-20 (relative = 0, tol = 1e-20)
  * within 12 V-cycles, else SWM_ERR_NOT_CONVERGED where the reference stops
  * (elliptic_solver.F:120-180).  relative = 1 tests sqrt(mean(r^2) /
- * mean(rhs^2)) <= tol instead.  max_cycles counts all cycles (>= 2).  No
- * counterpart in the reference, where these are compile-time constants. */
+ * mean(rhs^2)) <= tol instead; relative = 1 with tol = 0 solves to the
+ * FP64 floor: until a cycle no longer halves the residual.  max_cycles
+ * counts all cycles (>= 2).  No counterpart in the reference, where these
+ * are compile-time constants. */
 swm_status swm_set_solver(swm_ctx* ctx, double tol, int relative, int max_cycles);
 
 /* Mixed path only: the start of each elliptic solve, the linear
@@ -226,6 +228,44 @@ swm_status swm_budgets(swm_ctx* ctx, double* out, int nout);
 /* Compensated (Neumaier) FP64 global mass sum of the current thickness,
  * sum(h * area) over real cells (swm/source/budgets.F:108-109). */
 swm_status swm_total_mass(swm_ctx* ctx, double* mass);
+
+/* ------------------------------------------------------------------------
+ * Tangent-linear and adjoint model of a window of FP64 steps with centred
+ * advection (step.F:114-212 with get_centered_flux).  Hand-written TL and
+ * AD kernels cover every operator of the step; halo fills transpose to
+ * accumulations into their sources; the elliptic solve's transpose reuses
+ * the solver through the symmetry of area times Laplacian.  The trajectory
+ * is kept as snapshots every `every` steps and recomputed one segment at a
+ * time.  No counterpart in the reference.
+ * ------------------------------------------------------------------------ */
+/* Initial state from relative vorticity zeta, divergence delta and
+ * thickness h (iim,jjm,10): (psi, chi) solve L psi = zeta, L chi = delta
+ * (means removed), u = velocity(psi, chi) (operators.F:1402-1514), then the
+ * state as swm_init_state derives it; hs stays as set.  The solves (and
+ * those of the linearised steps) go to a relative residual of 1e-13 or to
+ * the FP64 floor where that is higher (a cycle no longer halves the
+ * residual). */
+swm_status swm_init_vorticity(swm_ctx* ctx, const double* zeta, const double* delta, const double* h);
+
+/* Runs nsteps steps from a freshly initialised state (step count 0),
+ * keeping snapshots every `every` steps: the base of swm_tl_run /
+ * swm_adj_run.  FP64 with centred advection only. */
+swm_status swm_adj_forward(swm_ctx* ctx, int nsteps, int every);
+
+/* Tangent-linear model of the window.  mode 0: in0..in3 the perturbation
+ * (h, eta, div (iim,jjm,10), u (2,2,iim,jjm,10)) of the state read by the
+ * first step; mode 1: in0..in2 the perturbation of the control (zeta,
+ * delta, h) of swm_init_vorticity, in3 unused.  Out: the perturbation of the
+ * state after the last step. */
+swm_status swm_tl_run(swm_ctx* ctx, int mode, const double* in0, const double* in1, const double* in2,
+                      const double* in3, double* out_h, double* out_eta, double* out_div, double* out_u);
+
+/* Adjoint of the window: lh, leta, ldiv, lu, the adjoint (gradient) of the
+ * state after the last step; mode 0: out0..out3 the adjoint of the state
+ * read by the first step (h, eta, div, u); mode 1: out0..out2 the gradient
+ * with respect to the control (zeta, delta, h), out3 unused. */
+swm_status swm_adj_run(swm_ctx* ctx, int mode, const double* lh, const double* leta, const double* ldiv,
+                       const double* lu, double* out0, double* out1, double* out2, double* out3);
 
 /* ------------------------------------------------------------------------
  * Restart files: the direct-access format of utilities/layer_2/io_utilities.F

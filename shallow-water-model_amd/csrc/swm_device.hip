@@ -14,6 +14,7 @@
 #include "swm_internal.h"
 #include "swm_kernels.h"
 #include "swm_kernels.hip"
+#include "swm_tlad.hip"
 
 namespace {
 
@@ -130,6 +131,7 @@ struct swm_ctx {
   double solver_tol = 1.0e-20;
   int solver_relative = 0;
   int solver_max_cycles = 12;
+  int solver_stall = 0;        // also stop where a cycle no longer halves the residual (FP64 floor)
   int mixed_extrapolate = 1;   // mixed path: start from 2 x_n - x_(n-1)
   int mixed_min_cycles = 1;    // mixed path: cycles before the first check
   double last_resid[2] = {0.0, 0.0};
@@ -142,6 +144,8 @@ struct swm_ctx {
   size_t kev_used = 0;
   bool profile = false;
   double kernel_bytes = 0.0;
+  // tangent-linear / adjoint window (allocated on first use)
+  struct TlAd* tl = nullptr;
 };
 
 namespace {
@@ -362,8 +366,12 @@ swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
     if (st) return st;
     any = false;
     for (size_t b = 0; b < bad.size(); ++b) {
+      const double prev = C->last_resid[bad[b]];
       C->last_resid[bad[b]] = nb[b];
-      if (!converged(C, bad[b], nb[b])) any = true;
+      // with solver_stall, a cycle that no longer halves the residual ends
+      // the solve: the FP64 floor of the residual is reached
+      const bool stalled = C->solver_stall && nb[b] > 0.25 * prev;
+      if (!converged(C, bad[b], nb[b]) && !stalled) any = true;
     }
   }
   C->last_vcycles = 2 + iter;
@@ -1004,6 +1012,10 @@ swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
 
 }  // namespace swm
 
+namespace swm {
+void tlad_free(swm_ctx* C);
+}
+
 extern "C" {
 
 swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out) {
@@ -1047,6 +1059,7 @@ void swm_destroy(swm_ctx* C) {
   if (C->ev_b) (void)hipEventDestroy(C->ev_b);
   if (C->arena.base) (void)hipFree(C->arena.base);
   if (C->host_sums) (void)hipHostFree(C->host_sums);
+  swm::tlad_free(C);
   if (C->stream) (void)hipStreamDestroy(C->stream);
   delete C;
 }
@@ -1290,13 +1303,14 @@ int swm_step_count(const swm_ctx* C) { return C ? C->step_count : -1; }
 int swm_last_vcycles(const swm_ctx* C) { return C ? C->last_vcycles : -1; }
 
 swm_status swm_set_solver(swm_ctx* C, double tol, int relative, int max_cycles) {
-  if (!C || !(tol > 0.0) || max_cycles < 2 || max_cycles > 1000) {
-    swm::set_error("swm_set_solver: need tol > 0 and 2 <= max_cycles <= 1000");
+  if (!C || !(tol >= 0.0) || (tol == 0.0 && !relative) || max_cycles < 2 || max_cycles > 1000) {
+    swm::set_error("swm_set_solver: need tol > 0 (or 0 with relative) and 2 <= max_cycles <= 1000");
     return SWM_ERR_ARG;
   }
   C->solver_tol = tol;
   C->solver_relative = relative ? 1 : 0;
   C->solver_max_cycles = max_cycles;
+  C->solver_stall = tol == 0.0 ? 1 : 0;
   return SWM_OK;
 }
 
@@ -1682,6 +1696,615 @@ swm_status swm_multigrid(swm_ctx* C, int nvar, const double* beta, double* alpha
   HIP_TRY(hipStreamSynchronize(C->stream));
   HIP_TRY(hipMemcpy(b.data(), al, b.size() * sizeof(double), hipMemcpyDeviceToHost));
   planes_to_vars(b.data(), nvar, pl, alpha);
+  return SWM_OK;
+}
+
+}  // extern "C"
+
+// ===================================================================
+// Tangent-linear and adjoint model of a window of FP64 steps with centred
+// advection (the kernels are in swm_tlad.hip).
+//
+// swm_adj_forward runs the window from the initial state, keeping a
+// snapshot of the model state every `every` steps (and at the end).  The
+// TL and AD sweeps then recompute one segment of base states at a time
+// from its snapshot (h, eta and u at the start of each step: the
+// coefficients of the linearised step) and run the linear steps over it,
+// forward for TL, backward for AD.  Elliptic solves of the linear steps go
+// to a relative residual of 1e-13, so that the TL and AD of the solve are
+// the inverse Laplacian and its transpose to that accuracy; the transpose
+// of the solve uses the symmetry of area times the Laplacian, L^T = A L A^-1.
+// ===================================================================
+struct TlAd {
+  // TL or AD state: two time levels of h, eta, div, u (4 planes), three
+  // ABM3 history slots of the h, eta and div tendencies
+  double *h[2], *eta[2], *div[2], *u[2], *hf[3], *ef[3], *df[3];
+  // scratch planes
+  double *hc, *F, *vf, *dc, *ke, *X, *rhs, *xs, *psia, *bhc, *bF, *zeros, *am_inv;
+  double* mem = nullptr;
+  // trajectory
+  int every = 0, nsteps = 0;
+  size_t snap_words = 0;
+  std::vector<double*> ckpt;          // snapshots at window steps 0, every, 2 every, ..., and the end
+  std::vector<long long> ckpt_meta;   // step_count, nsecond per snapshot
+  double* seg = nullptr;              // (every + 1) states of [h, eta, u(4)]
+  bool have_window = false;
+  swm::dev::Geo lin;                  // metrics with hs = 0 and f = 0
+};
+
+namespace swm {
+
+void tlad_free(swm_ctx* C) {
+  if (!C->tl) return;
+  TlAd* T = C->tl;
+  for (double* p : T->ckpt) (void)hipFree(p);
+  if (T->seg) (void)hipFree(T->seg);
+  if (T->mem) (void)hipFree(T->mem);
+  delete T;
+  C->tl = nullptr;
+}
+
+namespace {
+
+constexpr double kLinTol = 1.0e-13;   // relative residual of the linear solves
+constexpr int kLinMaxCycles = 80;
+
+swm_status tlad_alloc(swm_ctx* C) {
+  if (C->tl) return SWM_OK;
+  const size_t ns = C->ns;
+  TlAd* T = new TlAd;
+  C->tl = T;
+  const size_t words = ns * (2 * (1 + 1 + 1 + 4) + 9 + 2 + 3 + 3 + 3 + 1 + 1 + 2 + 2 + 2 + 2 + 3 + 1 + 1);
+  HIP_TRY(hipMalloc(&T->mem, words * sizeof(double)));
+  double* p = T->mem;
+  auto take = [&](size_t n) { double* q = p; p += n; return q; };
+  for (int t = 0; t < 2; ++t) { T->h[t] = take(ns); T->eta[t] = take(ns); T->div[t] = take(ns); T->u[t] = take(4 * ns); }
+  for (int t = 0; t < 3; ++t) { T->hf[t] = take(ns); T->ef[t] = take(ns); T->df[t] = take(ns); }
+  T->hc = take(2 * ns); T->F = take(3 * ns); T->vf = take(3 * ns); T->dc = take(3 * ns); T->ke = take(ns);
+  T->X = take(ns); T->rhs = take(2 * ns); T->xs = take(2 * ns); T->psia = take(2 * ns); T->bhc = take(2 * ns);
+  T->bF = take(3 * ns); T->zeros = take(ns); T->am_inv = take(ns);
+  fill<double>(C, T->mem, words, 0.0);
+  T->lin = C->d.geo;
+  T->lin.hs = T->zeros;
+  T->lin.f = T->zeros;
+  // 1 / area at real cells (the transpose of the solve scales by A^-1)
+  hipLaunchKernelGGL(swm::dev::k_mul_mask, dim3(nblk(ns)), dim3(BLK), 0, C->stream, (int)ns, C->mask, C->area_inv,
+                     C->mask, T->am_inv);
+  T->snap_words = C->d.state_words + 2 * ns;
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  return SWM_OK;
+}
+
+void clear_state(swm_ctx* C) {
+  TlAd* T = C->tl;
+  const size_t ns = C->ns;
+  fill<double>(C, T->h[0], (size_t)ns * 2 * (1 + 1 + 1 + 4) + 9 * ns, 0.0);   // the state planes are contiguous
+}
+
+swm_status snapshot(swm_ctx* C, double* dst) {
+  HIP_TRY(hipMemcpyAsync(dst, C->d.state_begin, C->d.state_words * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
+  HIP_TRY(hipMemcpyAsync(dst + C->d.state_words, C->xout, 2 * (size_t)C->ns * sizeof(double), hipMemcpyDeviceToDevice,
+                         C->stream));
+  return SWM_OK;
+}
+swm_status restore(swm_ctx* C, const double* src, long long sc, long long nsec) {
+  HIP_TRY(hipMemcpyAsync(C->d.state_begin, src, C->d.state_words * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
+  HIP_TRY(hipMemcpyAsync(C->xout, src + C->d.state_words, 2 * (size_t)C->ns * sizeof(double), hipMemcpyDeviceToDevice,
+                         C->stream));
+  C->step_count = (int)sc;
+  C->nsecond = nsec;
+  pointers(C);
+  return SWM_OK;
+}
+
+// one forward step of the model (the nonlinear base)
+swm_status base_step(swm_ctx* C) {
+  C->step_count += 1;
+  C->nsecond = (long long)((double)C->nsecond + C->dt);
+  pointers(C);
+  weights(C);
+  swm_status st = step_dt<double>(C, C->d);
+  if (st) return st;
+  wrap_prognostics<double>(C, C->d);
+  return SWM_OK;
+}
+
+// base state slot q of the segment buffer: h, eta, u (4 planes)
+double* seg_h(swm_ctx* C, int q) { return C->tl->seg + (size_t)q * 6 * C->ns; }
+double* seg_e(swm_ctx* C, int q) { return seg_h(C, q) + C->ns; }
+double* seg_u(swm_ctx* C, int q) { return seg_h(C, q) + 2 * (size_t)C->ns; }
+
+swm_status record_base(swm_ctx* C, int q) {
+  const int n3 = C->n4 - 1;   // the level the next step reads (after pointers of the last step: n4)
+  const size_t ns = C->ns;
+  HIP_TRY(hipMemcpyAsync(seg_h(C, q), C->d.h[n3], ns * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
+  HIP_TRY(hipMemcpyAsync(seg_e(C, q), C->d.eta[n3], ns * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
+  HIP_TRY(hipMemcpyAsync(seg_u(C, q), C->d.u[n3], 4 * ns * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
+  return SWM_OK;
+}
+
+// base states of segment sidx: from its snapshot, len steps; slot q holds
+// the state before window step sidx*every + q + 1 (q = len: after the last)
+swm_status recompute_segment(swm_ctx* C, int sidx, int len) {
+  TlAd* T = C->tl;
+  swm_status st;
+  if ((st = restore(C, T->ckpt[sidx], T->ckpt_meta[2 * sidx], T->ckpt_meta[2 * sidx + 1]))) return st;
+  if ((st = record_base(C, 0))) return st;
+  for (int q = 0; q < len; ++q) {
+    if ((st = base_step(C))) return st;
+    if ((st = record_base(C, q + 1))) return st;
+  }
+  return SWM_OK;
+}
+
+// linear elliptic solve (FP64 solver, relative residual kLinTol) of two
+// fields: rhs (gathered and mean-removed in place), x warm start and result
+swm_status lin_solve(swm_ctx* C, double* rhs, double* x) {
+  const double tol = C->solver_tol;
+  const int rel = C->solver_relative, mc = C->solver_max_cycles, stall = C->solver_stall;
+  const int lv = C->last_vcycles;
+  C->solver_tol = kLinTol; C->solver_relative = 1; C->solver_max_cycles = kLinMaxCycles; C->solver_stall = 1;
+  swm_status st = elliptic(C, rhs, x, 2);
+  C->solver_tol = tol; C->solver_relative = rel; C->solver_max_cycles = mc; C->solver_stall = stall;
+  C->last_vcycles = lv;
+  return st;
+}
+
+// the step's coefficients from base state q: corner thickness and mass flux
+void base_coeffs(swm_ctx* C, int q) {
+  TlAd* T = C->tl;
+  const unsigned nb = nblk(C->ns);
+  const swm::dev::Geo& G = C->d.geo;
+  hipLaunchKernelGGL(swm::dev::k_avg_corners<double>, dim3(nb), dim3(BLK), 0, C->stream, G, G.sw, seg_h(C, q), T->bhc);
+  gather<double>(C, C->vrtx, {T->bhc});
+  // k_mass_fluxes writes the upstream flux to its first output (scratch here)
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes<double>, dim3(nb), dim3(BLK), 0, C->stream, G, seg_u(C, q), seg_h(C, q),
+                     T->bhc, T->vf, T->bF);
+  gather<double>(C, C->edge, {T->bF});
+}
+
+// halo transposes of a set of fields
+void gather_adj(swm_ctx* C, const DevTable& t, std::initializer_list<double*> fields) {
+  std::vector<double*> f(fields);
+  for (size_t a = 0; a < f.size(); a += 8) {
+    swm::dev::FieldSet fs{};
+    int nf = 0;
+    for (size_t b = a; b < f.size() && b < a + 8; ++b) fs.p[nf++] = f[b];
+    if (!t.n) continue;
+    hipLaunchKernelGGL(swm::dev::k_gather_adj, dim3(nblk(t.n), nf), dim3(BLK), 0, C->stream, t.dst, t.src, t.n, fs);
+    hipLaunchKernelGGL(swm::dev::k_gather_adj_zero, dim3(nblk(t.n), nf), dim3(BLK), 0, C->stream, t.dst, t.n, fs);
+  }
+}
+
+// y -= am * sum(y) / A over all slots: the transpose of the mean removal
+void mean_removal_adj(swm_ctx* C, double* y) {
+  const int ns = C->ns;
+  hipLaunchKernelGGL(swm::dev::k_sum_partial<double>, dim3(C->npart, 1), dim3(BLK), 0, C->stream, y, y,
+                     (const double*)nullptr, ns, C->part);
+  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(1), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part, C->npart,
+                     C->sums);
+  hipLaunchKernelGGL(swm::dev::k_sub_am, dim3(nblk(ns)), dim3(BLK), 0, C->stream, ns, C->area_mask, C->sums,
+                     1.0 / C->total_area, y);
+}
+
+// ---------------------------------------------------------------- TL step
+// window step with base state q -> q+1; pointers / weights of the model step
+swm_status tl_step(swm_ctx* C, int q) {
+  TlAd* T = C->tl;
+  const size_t ns = C->ns;
+  const unsigned nb = nblk(ns);
+  hipStream_t S = C->stream;
+  const swm::dev::Geo& G = C->d.geo;
+  const int n3 = C->n3 - 1, n4 = C->n4 - 1;
+  const int a = C->n3f - 1, b = C->n2f - 1, c = C->n1f - 1;
+  const double dt = C->dt, w3 = C->w3, w2 = C->w2, w1 = C->w1;
+  base_coeffs(C, q);
+  double *dh3 = T->h[n3], *de3 = T->eta[n3], *du3 = T->u[n3];
+  // kinetic energy of u3 (computed at the end of the previous step)
+  hipLaunchKernelGGL(swm::dev::k_tl_center_ke, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), du3, T->ke);
+  gather<double>(C, C->face, {T->ke});
+  hipLaunchKernelGGL(swm::dev::k_avg_corners<double>, dim3(nb), dim3(BLK), 0, S, G, G.sw, dh3, T->hc);
+  gather<double>(C, C->vrtx, {T->hc});
+  hipLaunchKernelGGL(swm::dev::k_tl_mass_flux, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), T->bhc, du3, T->hc, T->F);
+  gather<double>(C, C->edge, {T->F});
+  hipLaunchKernelGGL(swm::dev::k_tl_height, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, T->F, dh3, T->hf[a],
+                     T->hf[b], T->hf[c], seg_h(C, q + 1), T->h[n4]);
+  hipLaunchKernelGGL(swm::dev::k_tl_vort_curl, dim3(nb), dim3(BLK), 0, S, G, T->bF, seg_h(C, q), seg_e(C, q), T->bhc,
+                     seg_u(C, q), T->F, dh3, de3, T->hc, du3, T->vf, T->dc);
+  gather<double>(C, C->edge, {T->vf, T->dc});
+  hipLaunchKernelGGL(swm::dev::k_tendencies<double>, dim3(nb), dim3(BLK), 0, S, T->lin, dt, C->grav, w3, w2, w1, T->vf,
+                     T->dc, T->ke, dh3, de3, T->div[n3], T->ef[a], T->ef[b], T->ef[c], T->df[a], T->df[b], T->df[c],
+                     T->eta[n4], T->div[n4], T->X, T->rhs, T->rhs + ns);
+  swm_status st = lin_solve(C, T->rhs, T->xs);
+  if (st) return st;
+  hipLaunchKernelGGL(swm::dev::k_post_elliptic<double>, dim3(nb), dim3(BLK), 0, S, T->lin, T->rhs, T->eta[n4]);
+  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, G, T->xs, T->xs + ns, T->u[n4]);
+  gather<double>(C, C->vrtx, {T->u[n4], T->u[n4] + 2 * ns});
+  // wrap_prognostics
+  gather<double>(C, C->face, {T->h[n4], T->eta[n4], T->div[n4]});
+  gather<double>(C, C->vrtx, {T->u[0], T->u[0] + 2 * ns, T->u[1], T->u[1] + 2 * ns});
+  return SWM_OK;
+}
+
+// ---------------------------------------------------------------- AD step
+// transpose of tl_step, same pointers / weights
+swm_status ad_step(swm_ctx* C, int q) {
+  TlAd* T = C->tl;
+  const size_t ns = C->ns;
+  const unsigned nb = nblk(ns);
+  hipStream_t S = C->stream;
+  const swm::dev::Geo& G = C->d.geo;
+  const int n3 = C->n3 - 1, n4 = C->n4 - 1;
+  const int a = C->n3f - 1, b = C->n2f - 1, c = C->n1f - 1;
+  const double dt = C->dt, w3 = C->w3, w2 = C->w2, w1 = C->w1;
+  base_coeffs(C, q);
+  // wrap_prognostics
+  gather_adj(C, C->vrtx, {T->u[0], T->u[0] + 2 * ns, T->u[1], T->u[1] + 2 * ns});
+  gather_adj(C, C->face, {T->h[n4], T->eta[n4], T->div[n4]});
+  // u4 = W velocity(psi, chi)
+  gather_adj(C, C->vrtx, {T->u[n4], T->u[n4] + 2 * ns});
+  fill<double>(C, T->psia, 2 * ns, 0.0);
+  hipLaunchKernelGGL(swm::dev::k_ad_velocity, dim3(nb), dim3(BLK), 0, S, G, T->u[n4], T->psia, T->psia + ns);
+  // the solve's final halo fill, then psi_a / A, solved: rhs_a = A z
+  gather_adj(C, C->face, {T->psia, T->psia + ns});
+  hipLaunchKernelGGL(swm::dev::k_mul_mask, dim3(nb), dim3(BLK), 0, S, (int)ns, T->psia, T->am_inv, C->mask, T->psia);
+  hipLaunchKernelGGL(swm::dev::k_mul_mask, dim3(nb), dim3(BLK), 0, S, (int)ns, T->psia + ns, T->am_inv, C->mask,
+                     T->psia + ns);
+  swm_status st = lin_solve(C, T->psia, T->xs);
+  if (st) return st;
+  // rhs_a = A z (real cells) + the adjoint of eta4 = rhs1 (+ f)
+  hipLaunchKernelGGL(swm::dev::k_mul_mask, dim3(nb), dim3(BLK), 0, S, (int)ns, T->xs, C->area, C->mask, T->rhs);
+  hipLaunchKernelGGL(swm::dev::k_mul_mask, dim3(nb), dim3(BLK), 0, S, (int)ns, T->xs + ns, C->area, C->mask,
+                     T->rhs + ns);
+  hipLaunchKernelGGL(swm::dev::k_axpy, dim3(nb), dim3(BLK), 0, S, ns, 1.0, T->eta[n4], T->rhs);
+  fill<double>(C, T->eta[n4], ns, 0.0);
+  // the solve's mean removal and its first halo fill of rhs
+  mean_removal_adj(C, T->rhs);
+  mean_removal_adj(C, T->rhs + ns);
+  gather_adj(C, C->face, {T->rhs, T->rhs + ns});
+  // tendencies: rhs2 = d4 joins the new level's adjoint
+  hipLaunchKernelGGL(swm::dev::k_axpy, dim3(nb), dim3(BLK), 0, S, ns, 1.0, T->rhs + ns, T->div[n4]);
+  fill<double>(C, T->vf, 3 * ns, 0.0);
+  fill<double>(C, T->dc, 3 * ns, 0.0);
+  fill<double>(C, T->X, ns, 0.0);
+  hipLaunchKernelGGL(swm::dev::k_ad_tendencies, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, T->rhs, T->div[n4],
+                     T->ef[a], T->ef[b], T->ef[c], T->df[a], T->df[b], T->df[c], T->eta[n3], T->div[n3], T->vf, T->dc,
+                     T->X);
+  fill<double>(C, T->ke, ns, 0.0);
+  hipLaunchKernelGGL(swm::dev::k_ad_fold_x, dim3(nb), dim3(BLK), 0, S, (int)ns, C->grav, T->X, T->ke, T->h[n3]);
+  // vorticity flux and divergence curl
+  gather_adj(C, C->edge, {T->vf, T->dc});
+  fill<double>(C, T->F, 3 * ns, 0.0);
+  fill<double>(C, T->hc, 2 * ns, 0.0);
+  hipLaunchKernelGGL(swm::dev::k_ad_vort_curl, dim3(nb), dim3(BLK), 0, S, G, T->bF, seg_h(C, q), seg_e(C, q), T->bhc,
+                     seg_u(C, q), T->vf, T->dc, T->F, T->h[n3], T->eta[n3], T->hc, T->u[n3]);
+  // height update
+  hipLaunchKernelGGL(swm::dev::k_ad_height, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, T->h[n4], T->hf[a], T->hf[b],
+                     T->hf[c], seg_h(C, q + 1), T->h[n3], T->F);
+  // mass flux
+  gather_adj(C, C->edge, {T->F});
+  hipLaunchKernelGGL(swm::dev::k_ad_mass_flux, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), T->bhc, T->F, T->u[n3], T->hc);
+  // corner thickness
+  gather_adj(C, C->vrtx, {T->hc});
+  hipLaunchKernelGGL(swm::dev::k_ad_avg_corners, dim3(nb), dim3(BLK), 0, S, G, T->hc, T->h[n3]);
+  // kinetic energy of u3
+  gather_adj(C, C->face, {T->ke});
+  hipLaunchKernelGGL(swm::dev::k_ad_center_ke, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), T->ke, T->u[n3]);
+  return SWM_OK;
+}
+
+// pointers / weights of window step n (1-based) of a window started at
+// step_count 0
+void set_step(swm_ctx* C, int n) {
+  C->step_count = n;
+  pointers(C);
+  weights(C);
+}
+
+// ------------------------------------------------------ initial state map
+// control (zeta, delta, h) -> state: (psi, chi) = L^-1 P (zeta, delta),
+// u = W velocity(psi, chi), the model's initialisation from h and u
+// (finish_init): both levels, eta = f + W curl(u), div = W div(u)
+swm_status init_tl(swm_ctx* C, const double* dz, const double* dd, const double* dh) {
+  TlAd* T = C->tl;
+  const size_t ns = C->ns;
+  const unsigned nb = nblk(ns);
+  hipStream_t S = C->stream;
+  const swm::dev::Geo& G = C->d.geo;
+  HIP_TRY(hipMemcpyAsync(T->rhs, dz, ns * sizeof(double), hipMemcpyDeviceToDevice, S));
+  HIP_TRY(hipMemcpyAsync(T->rhs + ns, dd, ns * sizeof(double), hipMemcpyDeviceToDevice, S));
+  fill<double>(C, T->xs, 2 * ns, 0.0);
+  swm_status st = lin_solve(C, T->rhs, T->xs);
+  if (st) return st;
+  clear_state(C);
+  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, G, T->xs, T->xs + ns, T->u[0]);
+  HIP_TRY(hipMemcpyAsync(T->h[0], dh, ns * sizeof(double), hipMemcpyDeviceToDevice, S));
+  // finish_init: both levels, wrapped; curl / div of level 0 (wrapped again)
+  HIP_TRY(hipMemcpyAsync(T->h[1], T->h[0], ns * sizeof(double), hipMemcpyDeviceToDevice, S));
+  HIP_TRY(hipMemcpyAsync(T->u[1], T->u[0], 4 * ns * sizeof(double), hipMemcpyDeviceToDevice, S));
+  for (int t = 0; t < 2; ++t) {
+    gather<double>(C, C->face, {T->h[t]});
+    gather<double>(C, C->vrtx, {T->u[t], T->u[t] + 2 * ns});
+  }
+  gather<double>(C, C->vrtx, {T->u[0], T->u[0] + 2 * ns});
+  hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb), dim3(BLK), 0, S, G, T->u[0], T->eta[0], T->div[0]);
+  gather<double>(C, C->face, {T->eta[0], T->div[0]});
+  HIP_TRY(hipMemcpyAsync(T->div[1], T->div[0], ns * sizeof(double), hipMemcpyDeviceToDevice, S));
+  HIP_TRY(hipMemcpyAsync(T->eta[1], T->eta[0], ns * sizeof(double), hipMemcpyDeviceToDevice, S));
+  // wrap_prognostics of the initial state (level 1 is the new level after
+  // initialisation: step_count 0)
+  gather<double>(C, C->face, {T->h[1], T->eta[1], T->div[1]});
+  gather<double>(C, C->vrtx, {T->u[0], T->u[0] + 2 * ns, T->u[1], T->u[1] + 2 * ns});
+  return SWM_OK;
+}
+
+// transpose of init_tl into gz, gd, gh (device planes)
+swm_status init_ad(swm_ctx* C, double* gz, double* gd, double* gh) {
+  TlAd* T = C->tl;
+  const size_t ns = C->ns;
+  const unsigned nb = nblk(ns);
+  hipStream_t S = C->stream;
+  const swm::dev::Geo& G = C->d.geo;
+  gather_adj(C, C->vrtx, {T->u[0], T->u[0] + 2 * ns, T->u[1], T->u[1] + 2 * ns});
+  gather_adj(C, C->face, {T->h[1], T->eta[1], T->div[1]});
+  hipLaunchKernelGGL(swm::dev::k_axpy, dim3(nb), dim3(BLK), 0, S, ns, 1.0, T->eta[1], T->eta[0]);
+  hipLaunchKernelGGL(swm::dev::k_axpy, dim3(nb), dim3(BLK), 0, S, ns, 1.0, T->div[1], T->div[0]);
+  gather_adj(C, C->face, {T->eta[0], T->div[0]});
+  hipLaunchKernelGGL(swm::dev::k_ad_curl_div, dim3(nb), dim3(BLK), 0, S, G, T->eta[0], T->div[0], T->u[0]);
+  gather_adj(C, C->vrtx, {T->u[0], T->u[0] + 2 * ns});
+  for (int t = 1; t >= 0; --t) {
+    gather_adj(C, C->vrtx, {T->u[t], T->u[t] + 2 * ns});
+    gather_adj(C, C->face, {T->h[t]});
+  }
+  hipLaunchKernelGGL(swm::dev::k_axpy, dim3(nblk(4 * ns)), dim3(BLK), 0, S, 4 * ns, 1.0, T->u[1], T->u[0]);
+  HIP_TRY(hipMemcpyAsync(gh, T->h[0], ns * sizeof(double), hipMemcpyDeviceToDevice, S));
+  hipLaunchKernelGGL(swm::dev::k_axpy, dim3(nb), dim3(BLK), 0, S, ns, 1.0, T->h[1], gh);
+  // u = velocity(psi, chi)
+  fill<double>(C, T->psia, 2 * ns, 0.0);
+  hipLaunchKernelGGL(swm::dev::k_ad_velocity, dim3(nb), dim3(BLK), 0, S, G, T->u[0], T->psia, T->psia + ns);
+  gather_adj(C, C->face, {T->psia, T->psia + ns});
+  hipLaunchKernelGGL(swm::dev::k_mul_mask, dim3(nb), dim3(BLK), 0, S, (int)ns, T->psia, T->am_inv, C->mask, T->psia);
+  hipLaunchKernelGGL(swm::dev::k_mul_mask, dim3(nb), dim3(BLK), 0, S, (int)ns, T->psia + ns, T->am_inv, C->mask,
+                     T->psia + ns);
+  fill<double>(C, T->xs, 2 * ns, 0.0);
+  swm_status st = lin_solve(C, T->psia, T->xs);
+  if (st) return st;
+  hipLaunchKernelGGL(swm::dev::k_mul_mask, dim3(nb), dim3(BLK), 0, S, (int)ns, T->xs, C->area, C->mask, gz);
+  hipLaunchKernelGGL(swm::dev::k_mul_mask, dim3(nb), dim3(BLK), 0, S, (int)ns, T->xs + ns, C->area, C->mask, gd);
+  mean_removal_adj(C, gz);
+  mean_removal_adj(C, gd);
+  gather_adj(C, C->face, {gz, gd});
+  clear_state(C);
+  return SWM_OK;
+}
+
+// nonlinear initial state from the control (the base of init_tl)
+swm_status init_from_vorticity(swm_ctx* C, const double* z, const double* d, const double* h) {
+  TlAd* T = C->tl;
+  const size_t ns = C->ns;
+  hipStream_t S = C->stream;
+  reset_state(C, true);
+  HIP_TRY(hipMemcpyAsync(T->rhs, z, ns * sizeof(double), hipMemcpyDeviceToDevice, S));
+  HIP_TRY(hipMemcpyAsync(T->rhs + ns, d, ns * sizeof(double), hipMemcpyDeviceToDevice, S));
+  fill<double>(C, T->xs, 2 * ns, 0.0);
+  swm_status st = lin_solve(C, T->rhs, T->xs);
+  if (st) return st;
+  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nblk(ns)), dim3(BLK), 0, S, C->d.geo, T->xs, T->xs + ns,
+                     C->d.u[0]);
+  HIP_TRY(hipMemcpyAsync(C->d.h[0], h, ns * sizeof(double), hipMemcpyDeviceToDevice, S));
+  // the model's first solve starts from this streamfunction / velocity potential
+  HIP_TRY(hipMemcpyAsync(C->xout, T->xs, 2 * ns * sizeof(double), hipMemcpyDeviceToDevice, S));
+  return finish_init(C);
+}
+
+// host (reference layout) <-> device planes of the TL / AD state
+swm_status state_in(swm_ctx* C, int lev, const double* h, const double* e, const double* d, const double* u) {
+  TlAd* T = C->tl;
+  const size_t ns = C->ns;
+  std::vector<double> up(4 * ns);
+  u_to_planes(u, up.data(), (int)ns);
+  HIP_TRY(hipMemcpy(T->h[lev], h, ns * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(T->eta[lev], e, ns * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(T->div[lev], d, ns * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(T->u[lev], up.data(), 4 * ns * sizeof(double), hipMemcpyHostToDevice));
+  return SWM_OK;
+}
+swm_status state_out(swm_ctx* C, int lev, double* h, double* e, double* d, double* u) {
+  TlAd* T = C->tl;
+  const size_t ns = C->ns;
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  std::vector<double> up(4 * ns);
+  HIP_TRY(hipMemcpy(h, T->h[lev], ns * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(e, T->eta[lev], ns * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(d, T->div[lev], ns * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(up.data(), T->u[lev], 4 * ns * sizeof(double), hipMemcpyDeviceToHost));
+  u_from_planes(up.data(), u, (int)ns);
+  return SWM_OK;
+}
+
+swm_status window_check(swm_ctx* C, const char* who) {
+  if (C->precision != SWM_FP64 || C->advection != SWM_ADV_CENTERED) {
+    swm::set_error(std::string(who) + ": the linearised model is the FP64 step with centred advection");
+    return SWM_ERR_ARG;
+  }
+  return SWM_OK;
+}
+
+// level read by window step 1 (step_count 1) and written by the last step
+int first_level() { return 1 % 2 + 1 - 1; }
+int last_level(int nsteps) { return (nsteps + 1) % 2 + 1 - 1; }
+
+}  // namespace
+}  // namespace swm
+
+extern "C" {
+
+swm_status swm_init_vorticity(swm_ctx* C, const double* zeta, const double* delta, const double* h) {
+  if (!C || !zeta || !delta || !h) { swm::set_error("swm_init_vorticity: bad argument"); return SWM_ERR_ARG; }
+  HIP_TRY(hipSetDevice(C->device));
+  swm_status st;
+  if ((st = swm::tlad_alloc(C))) return st;
+  const size_t ns = C->ns;
+  double* buf = nullptr;
+  HIP_TRY(hipMalloc(&buf, 3 * ns * sizeof(double)));
+  HIP_TRY(hipMemcpy(buf, zeta, ns * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(buf + ns, delta, ns * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(buf + 2 * ns, h, ns * sizeof(double), hipMemcpyHostToDevice));
+  st = swm::init_from_vorticity(C, buf, buf + ns, buf + 2 * ns);
+  (void)hipStreamSynchronize(C->stream);
+  (void)hipFree(buf);
+  return st;
+}
+
+swm_status swm_adj_forward(swm_ctx* C, int nsteps, int every) {
+  if (!C || nsteps < 1 || every < 1) { swm::set_error("swm_adj_forward: bad argument"); return SWM_ERR_ARG; }
+  swm_status st;
+  if ((st = swm::window_check(C, "swm_adj_forward"))) return st;
+  if (!C->initialized || C->step_count != 0) {
+    swm::set_error("swm_adj_forward: the window starts from a freshly initialised state (step count 0)");
+    return SWM_ERR_STATE;
+  }
+  HIP_TRY(hipSetDevice(C->device));
+  if ((st = swm::tlad_alloc(C))) return st;
+  TlAd* T = C->tl;
+  for (double* p : T->ckpt) (void)hipFree(p);
+  T->ckpt.clear();
+  T->ckpt_meta.clear();
+  if (T->seg) { (void)hipFree(T->seg); T->seg = nullptr; }
+  T->have_window = false;
+  const int nck = (nsteps + every - 1) / every + 1;
+  for (int k = 0; k < nck; ++k) {
+    double* p = nullptr;
+    HIP_TRY(hipMalloc(&p, T->snap_words * sizeof(double)));
+    T->ckpt.push_back(p);
+  }
+  HIP_TRY(hipMalloc(&T->seg, (size_t)(every + 1) * 6 * C->ns * sizeof(double)));
+  T->every = every;
+  T->nsteps = nsteps;
+  HIP_TRY(hipEventRecord(C->ev_a, C->stream));
+  for (int n = 0; n < nsteps; ++n) {
+    if (n % every == 0) {
+      if ((st = swm::snapshot(C, T->ckpt[n / every]))) return st;
+      T->ckpt_meta.push_back(C->step_count);
+      T->ckpt_meta.push_back(C->nsecond);
+    }
+    if ((st = swm::base_step(C))) return st;
+  }
+  // the final state, restored after the sweeps
+  if ((st = swm::snapshot(C, T->ckpt[nck - 1]))) return st;
+  T->ckpt_meta.push_back(C->step_count);
+  T->ckpt_meta.push_back(C->nsecond);
+  HIP_TRY(hipEventRecord(C->ev_b, C->stream));
+  HIP_TRY(hipEventSynchronize(C->ev_b));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, C->ev_a, C->ev_b));
+  C->last_step_ms = ms;
+  T->have_window = true;
+  return SWM_OK;
+}
+
+// TL of the window: mode 0, inputs the state perturbation (h, eta, div, u)
+// read by the first step; mode 1, inputs the control perturbation (zeta,
+// delta, h; in3 unused) through the initial state map.  Outputs the state
+// perturbation after the last step.
+swm_status swm_tl_run(swm_ctx* C, int mode, const double* in0, const double* in1, const double* in2,
+                      const double* in3, double* out_h, double* out_eta, double* out_div, double* out_u) {
+  if (!C || mode < 0 || mode > 2 || !in0 || !in1 || !in2 || (mode == 0 && !in3) || !out_h || !out_eta ||
+      !out_div || !out_u) {
+    swm::set_error("swm_tl_run: bad argument");
+    return SWM_ERR_ARG;
+  }
+  if (!C->tl || !C->tl->have_window) { swm::set_error("swm_tl_run: no window (swm_adj_forward first)"); return SWM_ERR_STATE; }
+  HIP_TRY(hipSetDevice(C->device));
+  TlAd* T = C->tl;
+  const size_t ns = C->ns;
+  swm_status st;
+  swm::clear_state(C);
+  fill<double>(C, T->xs, 2 * ns, 0.0);
+  if (mode == 0) {
+    if ((st = swm::state_in(C, swm::first_level(), in0, in1, in2, in3))) return st;
+  } else {   // 1, or 2: the initial state map alone
+    double* buf = nullptr;
+    HIP_TRY(hipMalloc(&buf, 3 * ns * sizeof(double)));
+    HIP_TRY(hipMemcpy(buf, in0, ns * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(buf + ns, in1, ns * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(buf + 2 * ns, in2, ns * sizeof(double), hipMemcpyHostToDevice));
+    st = swm::init_tl(C, buf, buf + ns, buf + 2 * ns);
+    (void)hipStreamSynchronize(C->stream);
+    (void)hipFree(buf);
+    if (st) return st;
+    fill<double>(C, T->xs, 2 * ns, 0.0);
+    if (mode == 2) return swm::state_out(C, swm::first_level(), out_h, out_eta, out_div, out_u);
+  }
+  const int nseg = (T->nsteps + T->every - 1) / T->every;
+  for (int sg = 0; sg < nseg; ++sg) {
+    const int len = std::min(T->every, T->nsteps - sg * T->every);
+    if ((st = swm::recompute_segment(C, sg, len))) return st;
+    for (int q = 0; q < len; ++q) {
+      swm::set_step(C, sg * T->every + q + 1);
+      if ((st = swm::tl_step(C, q))) return st;
+    }
+  }
+  const int lev = swm::last_level(T->nsteps);
+  if ((st = swm::state_out(C, lev, out_h, out_eta, out_div, out_u))) return st;
+  // the model back at the window's end
+  const int nck = (int)T->ckpt.size();
+  if ((st = swm::restore(C, T->ckpt[nck - 1], T->ckpt_meta[2 * nck - 2], T->ckpt_meta[2 * nck - 1]))) return st;
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  return SWM_OK;
+}
+
+// AD of the window: the adjoint of the state after the last step (lh, leta,
+// ldiv, lu) back to the first step (mode 0: out the adjoint state (h, eta,
+// div, u)) or to the control (mode 1: out0..2 the gradient with respect to
+// zeta, delta, h; out3 unused)
+swm_status swm_adj_run(swm_ctx* C, int mode, const double* lh, const double* leta, const double* ldiv,
+                       const double* lu, double* out0, double* out1, double* out2, double* out3) {
+  if (!C || mode < 0 || mode > 2 || !lh || !leta || !ldiv || !lu || !out0 || !out1 || !out2 ||
+      (mode == 0 && !out3)) {
+    swm::set_error("swm_adj_run: bad argument");
+    return SWM_ERR_ARG;
+  }
+  if (!C->tl || !C->tl->have_window) { swm::set_error("swm_adj_run: no window (swm_adj_forward first)"); return SWM_ERR_STATE; }
+  HIP_TRY(hipSetDevice(C->device));
+  TlAd* T = C->tl;
+  const size_t ns = C->ns;
+  swm_status st;
+  swm::clear_state(C);
+  fill<double>(C, T->xs, 2 * ns, 0.0);
+  // mode 2: the initial state map alone, from the adjoint of its output
+  if ((st = swm::state_in(C, mode == 2 ? swm::first_level() : swm::last_level(T->nsteps), lh, leta, ldiv, lu)))
+    return st;
+  const int nseg = mode == 2 ? 0 : (T->nsteps + T->every - 1) / T->every;
+  for (int sg = nseg - 1; sg >= 0; --sg) {
+    const int len = std::min(T->every, T->nsteps - sg * T->every);
+    if ((st = swm::recompute_segment(C, sg, len))) return st;
+    for (int q = len - 1; q >= 0; --q) {
+      swm::set_step(C, sg * T->every + q + 1);
+      if ((st = swm::ad_step(C, q))) return st;
+    }
+  }
+  if (mode == 0) {
+    if ((st = swm::state_out(C, swm::first_level(), out0, out1, out2, out3))) return st;
+  } else {
+    double* buf = nullptr;
+    HIP_TRY(hipMalloc(&buf, 3 * ns * sizeof(double)));
+    st = swm::init_ad(C, buf, buf + ns, buf + 2 * ns);
+    if (!st) {
+      (void)hipStreamSynchronize(C->stream);
+      if (hipMemcpy(out0, buf, ns * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(out1, buf + ns, ns * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(out2, buf + 2 * ns, ns * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        swm::set_error("swm_adj_run: copy of the gradient failed");
+        st = SWM_ERR_HIP;
+      }
+    }
+    (void)hipFree(buf);
+    if (st) return st;
+  }
+  const int nck = (int)T->ckpt.size();
+  if ((st = swm::restore(C, T->ckpt[nck - 1], T->ckpt_meta[2 * nck - 2], T->ckpt_meta[2 * nck - 1]))) return st;
+  HIP_TRY(hipStreamSynchronize(C->stream));
   return SWM_OK;
 }
 

@@ -1,0 +1,467 @@
+// swm_tlad.hip -- tangent-linear (TL) and adjoint (AD) kernels of the FP64
+// time step with centred advection (step.F:114-212 with get_centered_flux,
+// step.F:495-550; operators.F as named per kernel).
+//
+// Every forward operation y = f(x; base) of the step gets a TL kernel
+// (dy = f'(x) dx, gather form, one thread per slot like the forward kernel)
+// and an AD kernel (dx += f'(x)^T dy).  AD kernels run one thread per output
+// slot, as the forward does, and scatter into the input adjoints with FP64
+// atomic adds, so each one is the exact transpose of the forward stencil,
+// pole forms and panel edges included.  Halo fills (x[dst] = x[src]) have the
+// transpose x[src] += x[dst], x[dst] = 0 (k_gather_adj).  Operations that are
+// linear in the state (ABM3 updates, divergence of edge fluxes, Laplacian,
+// velocity from streamfunction and velocity potential) reuse the forward
+// kernels for TL with a metric bundle whose constant terms (surface height,
+// Coriolis parameter) are zero.
+#include "swm_kernels.h"
+
+namespace swm {
+namespace dev {
+
+#define SL(i, j, k) ((i) - 1 + iim * ((j) - 1 + jjm * ((k) - 1)))
+
+__device__ __forceinline__ void aadd(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// ------------------------------------------------------------------ halo --
+// transpose of k_gather: adjoint of the sources += adjoint of the halo
+// slots; then (k_gather_adj_zero) the halo slots' adjoint is cleared.  No
+// source is a destination, so the two passes commute with nothing else.
+__global__ void k_gather_adj(const int* __restrict__ dst, const int* __restrict__ src, int n, FieldSet fs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double* x = fs.p[blockIdx.y];
+  const int s = src[t];
+  if (s >= 0) aadd(&x[s], x[dst[t]]);
+}
+__global__ void k_gather_adj_zero(const int* __restrict__ dst, int n, FieldSet fs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  fs.p[blockIdx.y][dst[t]] = 0.0;
+}
+
+// ------------------------------------------------- average to corners -----
+// AD of k_avg_corners (operators.F:982-1112): corner values back to the
+// three cells with the scalar weights; ghost corners to cell (2,2)
+__global__ void k_ad_avg_corners(Geo G, const double* __restrict__ ob, double* __restrict__ xb) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  const double o0 = ob[s], o1 = ob[ns + s];
+  if (i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1) {
+    const double* sw = G.sw;
+    const int a = SL(i, j, k), b = SL(i - 1, j, k), d = SL(i - 1, j - 1, k), e = SL(i, j - 1, k);
+    aadd(&xb[a], sw[0 * ns + s] * o0 + sw[3 * ns + s] * o1);
+    aadd(&xb[b], sw[1 * ns + s] * o0);
+    aadd(&xb[d], sw[2 * ns + s] * o0 + sw[4 * ns + s] * o1);
+    aadd(&xb[e], sw[5 * ns + s] * o1);
+  } else {
+    aadd(&xb[SL(2, 2, k)], o0 + o1);
+  }
+}
+
+// -------------------------------------------------------- mass flux -------
+// The centred edge mass flux of k_mass_fluxes (operators.F:1626-1722):
+// F_e = -r0_e + r1_e, r = (u . n) * hc * dc at the half-edges
+//   r0_0 (sn,kk=1,ic=1)  r1_0 (s0,0,0)  r0_1 (s0,2,0)  r1_1 (s0,0,1)
+//   r0_2 (s0,2,1)        r1_2 (se,1,0)
+struct HalfTerm {
+  int sl, kk, ic;
+};
+__device__ __forceinline__ void half_terms(int iim, int jjm, int i, int j, int k, HalfTerm (&t)[2][3]) {
+  const int s0 = SL(i, j, k), sn = SL(i, j + 1, k), se = SL(i + 1, j, k);
+  t[0][0] = {sn, 1, 1}; t[1][0] = {s0, 0, 0};
+  t[0][1] = {s0, 2, 0}; t[1][1] = {s0, 0, 1};
+  t[0][2] = {s0, 2, 1}; t[1][2] = {se, 1, 0};
+}
+#define UPL(u, sp, ic, s_) (u)[((ic) + 2 * (sp)) * ns + (s_)]
+#define NPL(n, kk, a, ic, s_) (n)[((kk) + 3 * ((a) + 2 * (ic))) * ns + (s_)]
+#define DPL(kk, ic, s_) G.dc[((kk) + 3 * (ic)) * ns + (s_)]
+
+// TL: dF = d(-r0 + r1) with d r = ((du . n) hc + (u . n) dhc) dc; nrm is the
+// normal (mass flux) or the tangent (divergence-curl) planes
+__device__ __forceinline__ double half_tl(const Geo& G, int ns, const double* __restrict__ nrm,
+                                          const double* __restrict__ u, const double* __restrict__ hc,
+                                          const double* __restrict__ du, const double* __restrict__ dhc,
+                                          const HalfTerm& t) {
+  const double n0 = NPL(nrm, t.kk, 0, t.ic, t.sl), n1 = NPL(nrm, t.kk, 1, t.ic, t.sl);
+  const double un = UPL(u, 0, t.ic, t.sl) * n0 + UPL(u, 1, t.ic, t.sl) * n1;
+  const double dun = UPL(du, 0, t.ic, t.sl) * n0 + UPL(du, 1, t.ic, t.sl) * n1;
+  return (dun * hc[t.ic * ns + t.sl] + un * dhc[t.ic * ns + t.sl]) * DPL(t.kk, t.ic, t.sl);
+}
+// base half-edge value (u . n) hc dc
+__device__ __forceinline__ double half_val(const Geo& G, int ns, const double* __restrict__ nrm,
+                                           const double* __restrict__ u, const double* __restrict__ hc,
+                                           const HalfTerm& t) {
+  const double un = UPL(u, 0, t.ic, t.sl) * NPL(nrm, t.kk, 0, t.ic, t.sl) +
+                    UPL(u, 1, t.ic, t.sl) * NPL(nrm, t.kk, 1, t.ic, t.sl);
+  return un * hc[t.ic * ns + t.sl] * DPL(t.kk, t.ic, t.sl);
+}
+// AD of one half-edge value with adjoint g
+__device__ __forceinline__ void half_ad(const Geo& G, int ns, const double* __restrict__ nrm,
+                                        const double* __restrict__ u, const double* __restrict__ hc, double g,
+                                        const HalfTerm& t, double* __restrict__ ub, double* __restrict__ hcb) {
+  const double n0 = NPL(nrm, t.kk, 0, t.ic, t.sl), n1 = NPL(nrm, t.kk, 1, t.ic, t.sl);
+  const double d = DPL(t.kk, t.ic, t.sl);
+  const double un = UPL(u, 0, t.ic, t.sl) * n0 + UPL(u, 1, t.ic, t.sl) * n1;
+  const double gu = g * hc[t.ic * ns + t.sl] * d;
+  aadd(&UPL(ub, 0, t.ic, t.sl), gu * n0);
+  aadd(&UPL(ub, 1, t.ic, t.sl), gu * n1);
+  aadd(&hcb[t.ic * ns + t.sl], g * un * d);
+}
+
+__global__ void k_tl_mass_flux(Geo G, const double* __restrict__ u, const double* __restrict__ hc,
+                               const double* __restrict__ du, const double* __restrict__ dhc, double* __restrict__ dF) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) dF[e * ns + s] = 0.0;
+    return;
+  }
+  HalfTerm t[2][3];
+  half_terms(iim, jjm, c.i, c.j, c.k, t);
+  for (int e = 0; e < 3; ++e)
+    dF[e * ns + s] = -half_tl(G, ns, G.nrm, u, hc, du, dhc, t[0][e]) + half_tl(G, ns, G.nrm, u, hc, du, dhc, t[1][e]);
+}
+
+__global__ void k_ad_mass_flux(Geo G, const double* __restrict__ u, const double* __restrict__ hc,
+                               const double* __restrict__ Fb, double* __restrict__ ub, double* __restrict__ hcb) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) return;
+  HalfTerm t[2][3];
+  half_terms(iim, jjm, c.i, c.j, c.k, t);
+  for (int e = 0; e < 3; ++e) {
+    const double g = Fb[e * ns + s];
+    half_ad(G, ns, G.nrm, u, hc, -g, t[0][e], ub, hcb);
+    half_ad(G, ns, G.nrm, u, hc, g, t[1][e], ub, hcb);
+  }
+}
+
+// ---------------------------------------------------- height update -------
+// step.F:221-270 step_surface_height: hf_a = -div(F), h4 = h3 + dt (w3 hf_a
+// + w2 hf_b + w1 hf_c); the clip at 1e-10 passes no perturbation where the
+// base thickness was clipped (h4b == 1e-10)
+__global__ void k_tl_height(Geo G, double dt, double w3, double w2, double w1, const double* __restrict__ dF,
+                            const double* __restrict__ dh3, double* __restrict__ dhf_a,
+                            const double* __restrict__ dhf_b, const double* __restrict__ dhf_c,
+                            const double* __restrict__ h4b, double* __restrict__ dh4) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const double hf = -div_edge_at<double>(iim, jjm, ns, c, dF, G.area_inv[s]);
+  dhf_a[s] = hf;
+  const double h = dh3[s] + dt * (w3 * hf + w2 * dhf_b[s] + w1 * dhf_c[s]);
+  dh4[s] = h4b[s] == 1.0e-10 ? 0.0 : h;
+}
+
+// AD of the divergence of edge fluxes at one slot (div_edge_at): the slot's
+// value y = ai * (sum of +- F terms); adds g * dy/dF into Fb
+__device__ __forceinline__ void div_edge_ad(int iim, int jjm, int ns, const IJK& c, double ai, double g,
+                                            double* __restrict__ Fb) {
+  const int i = c.i, j = c.j, k = c.k;
+  const double v = g * ai;
+  if (i == 1 && j == jjm - 1) {
+    aadd(&Fb[1 * ns + SL(i, j, k)], -v); aadd(&Fb[2 * ns + SL(i, j, k)], -v); aadd(&Fb[0 * ns + SL(i + 1, j, k)], -v);
+    aadd(&Fb[1 * ns + SL(i + 1, j + 1, k)], -v); aadd(&Fb[2 * ns + SL(i, j + 1, k)], -v);
+  } else if (i == iim - 1 && j == 1) {
+    aadd(&Fb[0 * ns + SL(i, j, k)], -v); aadd(&Fb[1 * ns + SL(i, j, k)], -v); aadd(&Fb[0 * ns + SL(i + 1, j, k)], -v);
+    aadd(&Fb[1 * ns + SL(i + 1, j + 1, k)], -v); aadd(&Fb[2 * ns + SL(i, j + 1, k)], -v);
+  } else if (i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1) {
+    aadd(&Fb[0 * ns + SL(i, j, k)], v); aadd(&Fb[1 * ns + SL(i, j, k)], v); aadd(&Fb[2 * ns + SL(i, j, k)], v);
+    aadd(&Fb[0 * ns + SL(i + 1, j, k)], -v); aadd(&Fb[1 * ns + SL(i + 1, j + 1, k)], -v);
+    aadd(&Fb[2 * ns + SL(i, j + 1, k)], -v);
+  }
+}
+
+// AD of k_tl_height: consumes h4b_adj (the new level's adjoint) and the
+// adjoint of the slot hf_a (both cleared), adds into h3, hf_b, hf_c, F
+__global__ void k_ad_height(Geo G, double dt, double w3, double w2, double w1, double* __restrict__ h4a,
+                            double* __restrict__ hfa_a, double* __restrict__ hfb_a, double* __restrict__ hfc_a,
+                            const double* __restrict__ h4b, double* __restrict__ h3a, double* __restrict__ Fa) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const double g = h4b[s] == 1.0e-10 ? 0.0 : h4a[s];
+  h4a[s] = 0.0;
+  h3a[s] += g;
+  const double ga = hfa_a[s] + dt * w3 * g;
+  hfa_a[s] = 0.0;
+  hfb_a[s] += dt * w2 * g;
+  hfc_a[s] += dt * w1 * g;
+  div_edge_ad(iim, jjm, ns, c, G.area_inv[s], -ga, Fa);
+}
+
+// --------------------------------------- vorticity flux, divergence curl --
+// k_vort_curl (operators.F:2000-2186): pv = eta / h (0 where h <= 0),
+// vf_e = F_e (pv0 + pv_up) / 2, dcurl_e = (-t0_e + t1_e) (pv_up + pv0) / 2 with
+// the tangential half-edge values t
+__global__ void k_tl_vort_curl(Geo G, const double* __restrict__ F, const double* __restrict__ h3,
+                               const double* __restrict__ e3, const double* __restrict__ hc,
+                               const double* __restrict__ u, const double* __restrict__ dF,
+                               const double* __restrict__ dh3, const double* __restrict__ de3,
+                               const double* __restrict__ dhc, const double* __restrict__ du,
+                               double* __restrict__ dvf, double* __restrict__ ddc) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) { dvf[e * ns + s] = 0.0; ddc[e * ns + s] = 0.0; }
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  auto PV = [&](int sl, double& p, double& dp) {
+    const double hh = h3[sl];
+    const double w = hh > 0.0 ? 1.0 / hh : 0.0;
+    p = e3[sl] * w;
+    dp = hh > 0.0 ? de3[sl] * w - e3[sl] * w * w * dh3[sl] : 0.0;
+  };
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  double p0, dp0;
+  PV(s, p0, dp0);
+  HalfTerm t[2][3];
+  half_terms(iim, jjm, i, j, k, t);
+  for (int e = 0; e < 3; ++e) {
+    double pu, dpu;
+    PV(up[e], pu, dpu);
+    const double f = F[e * ns + s], df = dF[e * ns + s];
+    dvf[e * ns + s] = 0.5 * (df * (p0 + pu) + f * (dp0 + dpu));
+    const double r = -half_val(G, ns, G.tng, u, hc, t[0][e]) + half_val(G, ns, G.tng, u, hc, t[1][e]);
+    const double dr = -half_tl(G, ns, G.tng, u, hc, du, dhc, t[0][e]) + half_tl(G, ns, G.tng, u, hc, du, dhc, t[1][e]);
+    ddc[e * ns + s] = 0.5 * (dr * (pu + p0) + r * (dpu + dp0));
+  }
+}
+
+__global__ void k_ad_vort_curl(Geo G, const double* __restrict__ F, const double* __restrict__ h3,
+                               const double* __restrict__ e3, const double* __restrict__ hc,
+                               const double* __restrict__ u, const double* __restrict__ vfa,
+                               const double* __restrict__ dca, double* __restrict__ Fa, double* __restrict__ h3a,
+                               double* __restrict__ e3a, double* __restrict__ hca, double* __restrict__ ua) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) return;
+  const int i = c.i, j = c.j, k = c.k;
+  auto PV = [&](int sl) {
+    const double hh = h3[sl];
+    return e3[sl] * (hh > 0.0 ? 1.0 / hh : 0.0);
+  };
+  auto PV_AD = [&](int sl, double g) {   // pv = e / h
+    const double hh = h3[sl];
+    if (!(hh > 0.0)) return;
+    const double w = 1.0 / hh;
+    aadd(&e3a[sl], g * w);
+    aadd(&h3a[sl], -g * e3[sl] * w * w);
+  };
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  const double p0 = PV(s);
+  HalfTerm t[2][3];
+  half_terms(iim, jjm, i, j, k, t);
+  double gp0 = 0.0;
+  for (int e = 0; e < 3; ++e) {
+    const double pu = PV(up[e]);
+    const double gv = vfa[e * ns + s], gd = dca[e * ns + s];
+    const double f = F[e * ns + s];
+    const double r = -half_val(G, ns, G.tng, u, hc, t[0][e]) + half_val(G, ns, G.tng, u, hc, t[1][e]);
+    aadd(&Fa[e * ns + s], 0.5 * gv * (p0 + pu));
+    const double gpv = 0.5 * (gv * f + gd * r);
+    gp0 += gpv;
+    PV_AD(up[e], gpv);
+    const double gr = 0.5 * gd * (pu + p0);
+    half_ad(G, ns, G.tng, u, hc, -gr, t[0][e], ua, hca);
+    half_ad(G, ns, G.tng, u, hc, gr, t[1][e], ua, hca);
+  }
+  PV_AD(s, gp0);
+}
+
+// ------------------------------------------------------ kinetic energy ----
+// k_center_ke (operators.F:1316-1393 of 0.5 |u|^2 at the corners)
+#define SAP(kk, ic, sl) G.sub_area[((kk) + 3 * (ic)) * ns + (sl)]
+__global__ void k_tl_center_ke(Geo G, const double* __restrict__ u, const double* __restrict__ du,
+                               double* __restrict__ dke) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  if (i > iim - 1 || j > jjm - 1) { dke[s] = 0.0; return; }
+  auto DK = [&](int ic, int sl) { return u[ic * ns + sl] * du[ic * ns + sl] + u[(ic + 2) * ns + sl] * du[(ic + 2) * ns + sl]; };
+  const int se = SL(i + 1, j, k), sne = SL(i + 1, j + 1, k), sn = SL(i, j + 1, k);
+  dke[s] = (SAP(0, 0, s) * DK(0, s) + SAP(0, 1, s) * DK(1, s) + SAP(1, 0, se) * DK(0, se) + SAP(1, 1, sne) * DK(1, sne) +
+            SAP(2, 0, sne) * DK(0, sne) + SAP(2, 1, sn) * DK(1, sn)) * G.area_inv[s];
+}
+__global__ void k_ad_center_ke(Geo G, const double* __restrict__ u, const double* __restrict__ kea,
+                               double* __restrict__ ua) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  if (i > iim - 1 || j > jjm - 1) return;
+  const double g = kea[s] * G.area_inv[s];
+  auto KA = [&](int ic, int sl, double w) {
+    aadd(&ua[ic * ns + sl], g * w * u[ic * ns + sl]);
+    aadd(&ua[(ic + 2) * ns + sl], g * w * u[(ic + 2) * ns + sl]);
+  };
+  const int se = SL(i + 1, j, k), sne = SL(i + 1, j + 1, k), sn = SL(i, j + 1, k);
+  KA(0, s, SAP(0, 0, s)); KA(1, s, SAP(0, 1, s)); KA(0, se, SAP(1, 0, se)); KA(1, sne, SAP(1, 1, sne));
+  KA(0, sne, SAP(2, 0, sne)); KA(1, sn, SAP(2, 1, sn));
+}
+#undef SAP
+
+// ------------------------------------------------------- tendencies -------
+// AD of k_tendencies (step.F:336-429): e4 = e3 + dt (w3 ef_a + w2 ef_b +
+// w1 ef_c), ef_a = -div(vf); d4 = d3 + dt (w3 df_a + ...), df_a = -lap(X) +
+// div(dcurl), X = ke + g (h3 + hs); rhs1 = e4 - f, rhs2 = d4.  Inputs: the
+// adjoints of rhs1 (rhs1a) and of d4 (d4a, the new level's adjoint plus
+// that of rhs2), both consumed, and of the history slots; outputs added
+// into e3, d3, the b and c slots, vf, dcurl and X (Xa, then folded into ke
+// and h3 by the caller).
+__global__ void k_ad_tendencies(Geo G, double dt, double w3, double w2, double w1, double* __restrict__ rhs1a,
+                                double* __restrict__ d4a, double* __restrict__ efa_a, double* __restrict__ efb_a,
+                                double* __restrict__ efc_a, double* __restrict__ dfa_a, double* __restrict__ dfb_a,
+                                double* __restrict__ dfc_a, double* __restrict__ e3a, double* __restrict__ d3a,
+                                double* __restrict__ vfa, double* __restrict__ dca, double* __restrict__ Xa) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const double ai = G.area_inv[s];
+  const double ge = rhs1a[s];
+  rhs1a[s] = 0.0;
+  const double gd = d4a[s];
+  d4a[s] = 0.0;
+  e3a[s] += ge;
+  const double gefa = efa_a[s] + dt * w3 * ge;
+  efa_a[s] = 0.0;
+  efb_a[s] += dt * w2 * ge;
+  efc_a[s] += dt * w1 * ge;
+  div_edge_ad(iim, jjm, ns, c, ai, -gefa, vfa);
+  d3a[s] += gd;
+  const double gdfa = dfa_a[s] + dt * w3 * gd;
+  dfa_a[s] = 0.0;
+  dfb_a[s] += dt * w2 * gd;
+  dfc_a[s] += dt * w1 * gd;
+  div_edge_ad(iim, jjm, ns, c, ai, gdfa, dca);
+  // -lap(X): the transposed stencil of lap_at at this slot
+  const Stencil7<double> st = stencil7<double>(iim, jjm, ns, c, G.lw);
+  if (st.real) {
+    const double gl = -gdfa;
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+      if (st.w[q] != 0.0) aadd(&Xa[st.sl[q]], st.w[q] * gl);
+  }
+}
+
+// X = ke + g (h3 + hs): Xa folded into ke and h3, then cleared
+__global__ void k_ad_fold_x(int ns, double grav, double* __restrict__ Xa, double* __restrict__ kea,
+                            double* __restrict__ h3a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const double g = Xa[s];
+  Xa[s] = 0.0;
+  kea[s] += g;
+  h3a[s] += grav * g;
+}
+
+// ---------------------------------------------------------- velocity ------
+// AD of k_velocity (operators.F:1402-1514): u(0,ic) = gc0 - gp1, u(1,ic) =
+// gc1 + gp0 with g = cai * sum_k vw(k,sp,ic) x(nb_k); consumes ua (cleared
+// on every slot, as the forward writes every slot)
+__global__ void k_ad_velocity(Geo G, double* __restrict__ ua, double* __restrict__ psia, double* __restrict__ chia) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  double g[4];
+  for (int p = 0; p < 4; ++p) { g[p] = ua[p * ns + s]; ua[p * ns + s] = 0.0; }
+  if (!interior(c, iim, jjm)) return;
+  const int i = c.i, j = c.j, k = c.k;
+  const int nb[2][3] = {{s, SL(i - 1, j, k), SL(i - 1, j - 1, k)}, {s, SL(i - 1, j - 1, k), SL(i, j - 1, k)}};
+#define VW(kk, sp, ic) G.vw[((kk) + 3 * ((sp) + 2 * (ic))) * ns + s]
+  for (int ic = 0; ic < 2; ++ic) {
+    const double cai = G.cai[ic * ns + s];
+    const double gc0 = g[ic], gp1 = -g[ic], gc1 = g[ic + 2], gp0 = g[ic + 2];
+    for (int q = 0; q < 3; ++q) {
+      aadd(&chia[nb[ic][q]], cai * (VW(q, 0, ic) * gc0 + VW(q, 1, ic) * gc1));
+      aadd(&psia[nb[ic][q]], cai * (VW(q, 0, ic) * gp0 + VW(q, 1, ic) * gp1));
+    }
+  }
+#undef VW
+}
+
+// --------------------------------------------------- curl, divergence -----
+// AD of k_curl_div (operators.F:47-261), either output may be null
+__global__ void k_ad_curl_div(Geo G, const double* __restrict__ curla, const double* __restrict__ dva,
+                              double* __restrict__ ua) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  if (i > iim - 1 || j > jjm - 1) return;
+#define VW(kk, sp, ic, sl) G.vw[((kk) + 3 * ((sp) + 2 * (ic))) * ns + (sl)]
+#define UA(sp, ic, sl) ua[((ic) + 2 * (sp)) * ns + (sl)]
+  const int se = SL(i + 1, j, k), sne = SL(i + 1, j + 1, k), sn = SL(i, j + 1, k);
+  const double ai = G.area_inv[s];
+  const double gc = curla ? curla[s] * ai : 0.0, gv = dva ? dva[s] * ai : 0.0;
+  // (kk, ic, slot) of the 6 corner terms
+  const int kk6[6] = {0, 0, 1, 1, 2, 2}, ic6[6] = {0, 1, 0, 1, 0, 1};
+  const int sl6[6] = {s, s, se, sne, sne, sn};
+  for (int q = 0; q < 6; ++q) {
+    const int kk = kk6[q], ic = ic6[q], sl = sl6[q];
+    // curl: + vw(kk,1,ic) u(0,ic) - vw(kk,0,ic) u(1,ic); div: - vw(kk,0,ic) u(0,ic) - vw(kk,1,ic) u(1,ic)
+    aadd(&UA(0, ic, sl), gc * VW(kk, 1, ic, sl) - gv * VW(kk, 0, ic, sl));
+    aadd(&UA(1, ic, sl), -gc * VW(kk, 0, ic, sl) - gv * VW(kk, 1, ic, sl));
+  }
+#undef VW
+#undef UA
+}
+
+// ------------------------------------------------------------ helpers -----
+// y -= am * sums[0] / total (the transpose of the mean removal k_sub_mean,
+// whose mean is sum(am * x) / total, is y -= am * sum(y) / total)
+__global__ void k_sub_am(int n, const double* __restrict__ am, const double* __restrict__ sums, double inv_total,
+                         double* __restrict__ y) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  y[s] = y[s] - am[s] * (sums[0] * inv_total);
+}
+// y = x * w (w may be the reciprocal area; 0 where mask is 0)
+__global__ void k_mul_mask(int n, const double* __restrict__ x, const double* __restrict__ w,
+                           const double* __restrict__ mask, double* __restrict__ y) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  y[s] = mask[s] != 0.0 ? x[s] * w[s] : 0.0;
+}
+// y += x
+__global__ void k_axpy(size_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  y[s] = y[s] + a * x[s];
+}
+// per-block partial sums of x * y over n values (dot products)
+__global__ void __launch_bounds__(256) k_dot_partial(size_t n, const double* __restrict__ x,
+                                                     const double* __restrict__ y, double* __restrict__ part) {
+  double acc[1] = {0.0};
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    acc[0] += x[t] * y[t];
+  block_sums<1>(acc, part);
+}
+
+#undef UPL
+#undef NPL
+#undef DPL
+#undef SL
+
+}  // namespace dev
+}  // namespace swm

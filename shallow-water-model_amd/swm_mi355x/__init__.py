@@ -110,6 +110,10 @@ _sig("swm_restart_file_read", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, c
                                              ctypes.POINTER(ctypes.c_longlong)] + [ctypes.c_void_p] * 3)
 _sig("swm_swtc5_state", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double] + [ctypes.c_void_p] * 4)
 _sig("swm_read_namelist", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_Namelist)])
+_sig("swm_init_vorticity", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 3)
+_sig("swm_adj_forward", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
+_sig("swm_tl_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 8)
+_sig("swm_adj_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 8)
 _sig("swm_wrap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p])
 _sig("swm_elliptic", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
 _sig("swm_multigrid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
@@ -427,6 +431,57 @@ class Model:
         a, b, c, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
         _check(lib.swm_timing(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(n)))
         return a.value, b.value, c.value, n.value
+
+    # ---------------------------------------------------------- TL / AD
+    def init_vorticity(self, zeta, delta, h):
+        """Initial state from relative vorticity, divergence and thickness
+        (iim,jjm,10): streamfunction and velocity potential solved, u from
+        them, the rest derived as by init_state."""
+        s = (self.iim, self.jjm, 10)
+        z, d, hh = _f64(zeta, s), _f64(delta, s), _f64(h, s)
+        _check(lib.swm_init_vorticity(self._h, _ptr(z), _ptr(d), _ptr(hh)))
+
+    def adj_forward(self, nsteps, every=16):
+        """The window of the linearised model: nsteps steps from the freshly
+        initialised state, snapshots every `every` steps."""
+        _check(lib.swm_adj_forward(self._h, nsteps, every))
+
+    def _state_arrays(self):
+        s = (self.iim, self.jjm, 10)
+        return [np.zeros(s, order="F"), np.zeros(s, order="F"), np.zeros(s, order="F"),
+                np.zeros((2, 2) + s, order="F")]
+
+    def tl_run(self, dh=None, deta=None, ddiv=None, du=None, control=None, init_only=False):
+        """Tangent-linear model of the window: the perturbation of the state
+        read by the first step (dh, deta, ddiv, du) or of the control
+        (control=(dzeta, ddelta, dh)); returns (h, eta, div, u) after the
+        last step."""
+        s = (self.iim, self.jjm, 10)
+        out = self._state_arrays()
+        if control is not None:
+            ins = [_f64(x, s) for x in control] + [np.zeros((2, 2) + s, order="F")]
+            mode = 2 if init_only else 1
+        else:
+            ins = [_f64(dh, s), _f64(deta, s), _f64(ddiv, s), _f64(du, (2, 2) + s)]
+            mode = 0
+        _check(lib.swm_tl_run(self._h, mode, *[_ptr(x) for x in ins], *[_ptr(x) for x in out]))
+        return tuple(out)
+
+    def adj_run(self, lh, leta=None, ldiv=None, lu=None, control=False, init_only=False):
+        """Adjoint of the window for the adjoint (gradient) of the state
+        after the last step; returns the adjoint of the state read by the
+        first step (h, eta, div, u), or with control=True the gradient with
+        respect to (zeta, delta, h)."""
+        s = (self.iim, self.jjm, 10)
+        z = np.zeros(s, order="F")
+        ins = [_f64(lh, s), _f64(z if leta is None else leta, s), _f64(z if ldiv is None else ldiv, s),
+               _f64(np.zeros((2, 2) + s) if lu is None else lu, (2, 2) + s)]
+        out = self._state_arrays()
+        mode = 2 if init_only else (1 if control else 0)
+        _check(lib.swm_adj_run(self._h, mode, *[_ptr(x) for x in ins], *[_ptr(x) for x in out]))
+        if init_only:
+            return tuple(out[:3])
+        return tuple(out[:3]) if control else tuple(out)
 
     @property
     def time_seconds(self):
