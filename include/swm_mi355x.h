@@ -117,6 +117,66 @@ typedef struct swm_config {
 swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out);
 void swm_destroy(swm_ctx* ctx);
 
+/* ------------------------------------------------------------------------
+ * Ensembles (no counterpart in the reference, which runs one model per
+ * process).  A context of nmem members holds nmem independent model states
+ * on the same grid with the same settings; swm_step advances every member in
+ * the same launches (one grid row of workgroups per member; the metrics are
+ * read once for all).  Host-side calls (initialisation, fields, budgets,
+ * restart files, the single operators) address the member chosen by
+ * swm_select_member (member 0 by default).  Every member must be initialised
+ * before the first step; initialising a member restarts the step counter of
+ * the context (the next step is a forward Euler step for every member).  In
+ * the mixed path the members share the thickness offset of the member
+ * initialised first, and in both paths the surface height.  The linearised
+ * model is for single-member contexts.
+ * ------------------------------------------------------------------------ */
+swm_status swm_create_members(const swm_grid* grid, const swm_config* cfg, int nmem, swm_ctx** out);
+int swm_members(const swm_ctx* ctx);
+swm_status swm_select_member(swm_ctx* ctx, int member);
+
+/* Fixed schedule of the mixed solve: ncycles > 0 correction cycles per solve
+ * with no host-side convergence test, so that steps can be captured in a HIP
+ * graph; 0 restores the test.  The first two solves after an initialisation
+ * (from a zero, then a first-order warm start) take max(ncycles, nstart).  The largest mean squared residuals of the two
+ * fields and the number of solves that ended above the (absolute) criterion
+ * are recorded per member on the device: swm_solver_track gives them for the
+ * selected member (out[0..2]), counted from its last initialisation. */
+swm_status swm_set_fixed_cycles(swm_ctx* ctx, int ncycles, int nstart);
+swm_status swm_solver_track(swm_ctx* ctx, double* out);
+
+/* An ensemble: groups (contexts, each of one or more members, e.g. at
+ * different grid levels) advanced together.  One ensemble step advances
+ * group k by substeps[k] of its own time steps, so a level g+1 group with
+ * half the time step takes 2.  swm_ensemble_step captures the launches of
+ * steps_per_graph ensemble steps of every group into a HIP graph once per
+ * phase of the groups (ABM3 start and time levels, solver buffers, launch
+ * choice) and replays it with one graph launch after that; every group must
+ * run the mixed step with a fixed schedule (swm_set_fixed_cycles).  The
+ * groups' streams are synchronised before and the ensemble's stream after
+ * the steps, so the contexts' own calls see the results.
+ * swm_ensemble_tune times candidate launch choices (one stream, or one per
+ * group forked inside the graph; 1, 3 or 6 ensemble steps per graph;
+ * workgroups of 64, 128 or 256 for the stencil and for the multigrid
+ * kernels), each over `rounds` x 6 ensemble steps, keeps the fastest, and
+ * restores the groups' state and clocks; trials[0..max_trials) receive the
+ * candidates with their time per ensemble step. */
+typedef struct swm_ensemble swm_ensemble;
+typedef struct swm_ensemble_choice {
+  int concurrent;        /* 0: groups one after another on one stream; 1: a stream per group */
+  int steps_per_graph;   /* ensemble steps per captured graph */
+  int blk_step;          /* workgroup size of the step's stencil kernels and halo fills */
+  int blk_mg;            /* workgroup size of the multigrid smoother and transfers */
+  double ms_per_step;    /* measured ms per ensemble step (output) */
+} swm_ensemble_choice;
+swm_status swm_ensemble_create(swm_ctx* const* groups, const int* substeps, int ngroups, swm_ensemble** out);
+void swm_ensemble_destroy(swm_ensemble* ens);
+swm_status swm_ensemble_set(swm_ensemble* ens, const swm_ensemble_choice* choice);
+swm_status swm_ensemble_get(const swm_ensemble* ens, swm_ensemble_choice* choice);
+swm_status swm_ensemble_step(swm_ensemble* ens, int nsteps);
+swm_status swm_ensemble_tune(swm_ensemble* ens, int rounds, swm_ensemble_choice* trials, int max_trials,
+                             int* ntrials);
+
 /* Initial state from thickness h and surface height hs (iim,jjm,10), corner
  * velocity u (2,2,iim,jjm,10) and tracers (iim,jjm,ntracer,10); vorticity,
  * divergence and kinetic energy are derived from u on the device as the
@@ -212,6 +272,14 @@ swm_status swm_profile(swm_ctx* ctx, int enable);
  * (it runs fixed test cases, swm/source/initialize.F:330-696). */
 swm_status swm_init_synthetic(swm_ctx* ctx, int seed_index, int lmax, double umax, double h0,
                               double h_rms, int nbumps, double bump_radius);
+
+/* swm_init_synthetic plus a member perturbation: vorticity harmonics of the
+ * same degrees from stream 20261203 + 100*seed_index + 1000 + member_seed,
+ * scaled to vort_rel times the RMS of the vorticity and added through the
+ * streamfunction before the velocity is formed and scaled to umax. */
+swm_status swm_init_synthetic_member(swm_ctx* ctx, int seed_index, int lmax, double umax, double h0,
+                                     double h_rms, int nbumps, double bump_radius, int member_seed,
+                                     double vort_rel);
 
 /* Global averages of budgets.F:31-247 compute_budgets at the current time
  * level, summed over real cells in FP64 with a compensated final sum:

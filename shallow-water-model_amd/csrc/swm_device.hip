@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -34,6 +35,7 @@ constexpr int BLK = 256;
   } while (0)
 
 inline unsigned nblk(size_t n) { return (unsigned)((n + BLK - 1) / BLK); }
+inline unsigned nblk_b(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 struct DevTable {
   int n = 0;
@@ -95,8 +97,28 @@ struct swm_ctx {
   int precision = SWM_FP64, device = 0;
   int advection = SWM_ADV_FCT;
   hipStream_t stream = nullptr;
-  Arena arena;
+  Arena arena;             // shared: metrics, halo tables, multigrid weights and maps
   size_t arena_bytes = 0;
+  // ensemble members (same grid and settings).  Member m's state and scratch
+  // is one block at marena.base + m * mb; the pointers below address member
+  // 0's block (or, while one member is selected for host-side calls, that
+  // member's), and launches over the state cover nz members.
+  Arena marena;
+  int nmem = 1, nz = 1, sel = 0;
+  size_t mb = 0;
+  std::vector<char> member_init;   // member m has been initialised
+  bool h0_set = false;             // mixed path: thickness offset fixed by a first member (ensembles)
+  // fixed schedule (> 0): mixed solves run exactly this many correction
+  // cycles with no host-side test, so that a step can be captured in a
+  // graph; the residuals are tracked on the device instead
+  int fixed_cycles = 0;
+  int fixed_start = 0;             // cycles of the fixed schedule's solves before two solutions are held
+  // workgroup sizes of the step's stencil kernels and halo fills, and of the
+  // multigrid smoother / transfer kernels (64, 128 or 256; an autotuner's
+  // choice for ensembles)
+  int blk_step = 256, blk_mg = 256;
+  long long epoch = 0;             // changes whenever captured launches would differ (initialisation, settings)
+  double* track = nullptr;         // per member: max mean(r^2) of the two fields, count of solves above the criterion
   // FP64 metrics (both paths: set-up, FP64 residual, sums)
   double *area, *area_inv, *cai, *sub_area, *dc, *nrm, *tng, *vw, *sw, *lw, *f, *hs, *mask, *area_mask, *pxyz;
   double total_area = 0.0, radius = 0.0;
@@ -109,13 +131,13 @@ struct swm_ctx {
   double* xout = nullptr;  // streamfunction and velocity potential (FP64, both paths)
   double* xalt = nullptr;  // mixed path: second FP64 iterate of the defect correction
   double* xprev = nullptr;  // mixed path: the solution before xout (extrapolated warm start)
-  bool have_prev = false;   // xprev holds a solution
+  int nsol = 0;             // mixed path: solutions held since the initialisation (xout, then xprev too), at most 2
   double* part_slots = nullptr;   // per-block partial sums, one block per 256 slots, up to four columns
   double* gauge = nullptr;        // mixed path: area-weighted change of the solution's mean, two fields
   double* r2 = nullptr;
   double *part = nullptr, *sums = nullptr;
   double* part_budget = nullptr;   // partial sums of the budgets, (8 + BUDGET_MAXTR) x npart
-  double* host_sums = nullptr;   // pinned
+  double* host_sums = nullptr;   // pinned, 64 values per member
   int npart = 0;
   // time stepping
   int step_count = 0, n3 = 1, n4 = 2, n1f = 1, n2f = 2, n3f = 3;
@@ -136,6 +158,7 @@ struct swm_ctx {
   int mixed_min_cycles = 1;    // mixed path: cycles before the first check
   double last_resid[2] = {0.0, 0.0};
   double last_rhs_ms[2] = {0.0, 0.0};
+  std::vector<double> rhs_ms;   // per member and field (relative criterion)
   long long solves = 0, vcycles_total = 0, checks_total = 0;   // since the last initialisation
   // timing
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
@@ -186,13 +209,20 @@ swm_status make_table(swm_ctx* C, const swm::HaloTable& t, int ns, DevTable& d) 
   return SWM_OK;
 }
 
+// launch grid over the state of the nz members in play (z = member)
+inline dim3 zgrid(const swm_ctx* C, unsigned x, unsigned y = 1) { return dim3(x, y, (unsigned)C->nz); }
+
+// halo fills of member state fields
 template <class R>
 void gather_list(swm_ctx* C, const DevTable& t, const std::vector<R*>& fields) {
   for (size_t a = 0; a < fields.size(); a += 8) {
     swm::dev::FieldSetT<R> fs{};
+    fs.mb = C->mb;
     int nf = 0;
     for (size_t b = a; b < fields.size() && b < a + 8; ++b) fs.p[nf++] = fields[b];
-    if (t.n) hipLaunchKernelGGL(swm::dev::k_gather<R>, dim3(nblk(t.n), nf), dim3(BLK), 0, C->stream, t.dst, t.src, t.n, fs);
+    if (t.n)
+      hipLaunchKernelGGL(swm::dev::k_gather<R>, zgrid(C, nblk_b(t.n, C->blk_step), nf), dim3(C->blk_step), 0, C->stream,
+                         t.dst, t.src, t.n, fs);
   }
 }
 template <class R>
@@ -200,15 +230,29 @@ void gather(swm_ctx* C, const DevTable& t, std::initializer_list<R*> fields) {
   gather_list<R>(C, t, std::vector<R*>(fields));
 }
 
+// x = v on n values of member state, for the members in play
 template <class R>
 void fill(swm_ctx* C, R* x, size_t n, R v) {
-  hipLaunchKernelGGL(swm::dev::k_fill<R>, dim3(nblk(n)), dim3(BLK), 0, C->stream, x, n, v);
+  hipLaunchKernelGGL(swm::dev::k_fill<R>, zgrid(C, nblk(n)), dim3(BLK), 0, C->stream, x, n, v, C->mb);
+}
+// x = v on n values of one buffer outside the member blocks
+template <class R>
+void fill_plain(swm_ctx* C, R* x, size_t n, R v) {
+  hipLaunchKernelGGL(swm::dev::k_fill<R>, dim3(nblk(n)), dim3(BLK), 0, C->stream, x, n, v, (size_t)0);
+}
+void copy_state(swm_ctx* C, double* y, const double* x, size_t n) {
+  hipLaunchKernelGGL(swm::dev::k_copy, zgrid(C, nblk(n)), dim3(BLK), 0, C->stream, y, x, n, C->mb);
+}
+// compensated sums of nf columns of block partials, per member in play
+void sum_final(swm_ctx* C, const double* part, int nb, double* out, int nf) {
+  hipLaunchKernelGGL(swm::dev::k_sum_final, zgrid(C, (unsigned)nf), dim3(swm::dev::SUM_THREADS), 0, C->stream, part,
+                     nb, out, C->mb);
 }
 
 // ------------------------------------------------------------- multigrid --
 template <class R, int NV>
 void launch_relax(swm_ctx* C, const LevelT<R>& L, bool zero, const R* beta, const R* a, R* t) {
-  const dim3 grid(nblk(L.ns)), blk(BLK);
+  const dim3 grid = zgrid(C, nblk_b(L.ns, C->blk_mg)), blk(C->blk_mg);
   if (zero) hipLaunchKernelGGL((swm::dev::k_relax<R, NV, true>), grid, blk, 0, C->stream, L.L, beta, a, t);
   else hipLaunchKernelGGL((swm::dev::k_relax<R, NV, false>), grid, blk, 0, C->stream, L.L, beta, a, t);
 }
@@ -253,10 +297,10 @@ void vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_
   for (int l = G; l > top; --l) {
     relax<R>(C, V, l, 2, A(l), B(l), nvar, l < G || top_zero);
     if (nvar == 2)
-      hipLaunchKernelGGL((swm::dev::k_inject<R, 2>), dim3(nblk(V.lev[l - 2].ns)), dim3(BLK), 0, C->stream,
+      hipLaunchKernelGGL((swm::dev::k_inject<R, 2>), zgrid(C, nblk_b(V.lev[l - 2].ns, C->blk_mg)), dim3(C->blk_mg), 0, C->stream,
                          V.lev[l - 1].L, V.lev[l - 2].L, A(l), B(l), B(l - 1));
     else
-      hipLaunchKernelGGL((swm::dev::k_inject<R, 1>), dim3(nblk(V.lev[l - 2].ns)), dim3(BLK), 0, C->stream,
+      hipLaunchKernelGGL((swm::dev::k_inject<R, 1>), zgrid(C, nblk_b(V.lev[l - 2].ns, C->blk_mg)), dim3(C->blk_mg), 0, C->stream,
                          V.lev[l - 1].L, V.lev[l - 2].L, A(l), B(l), B(l - 1));
   }
   swm::dev::CoarseArgs<R> P{};
@@ -266,13 +310,14 @@ void vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_
   P.beta = B(top);
   P.alpha = A(top);
   P.zero = (top < G || top_zero) ? 1 : 0;
-  hipLaunchKernelGGL(swm::dev::k_coarse_cycle<R>, dim3(1), dim3(swm::dev::CO_THREADS), 0, C->stream, P);
+  P.mb = C->mb;
+  hipLaunchKernelGGL(swm::dev::k_coarse_cycle<R>, zgrid(C, 1), dim3(swm::dev::CO_THREADS), 0, C->stream, P);
   for (int l = top; l <= G - 1; ++l) {
     if (nvar == 2)
-      hipLaunchKernelGGL((swm::dev::k_interp<R, 2>), dim3(nblk(V.lev[l].ns)), dim3(BLK), 0, C->stream,
+      hipLaunchKernelGGL((swm::dev::k_interp<R, 2>), zgrid(C, nblk_b(V.lev[l].ns, C->blk_mg)), dim3(C->blk_mg), 0, C->stream,
                          V.lev[l - 1].L, V.lev[l].L, A(l), A(l + 1));
     else
-      hipLaunchKernelGGL((swm::dev::k_interp<R, 1>), dim3(nblk(V.lev[l].ns)), dim3(BLK), 0, C->stream,
+      hipLaunchKernelGGL((swm::dev::k_interp<R, 1>), zgrid(C, nblk_b(V.lev[l].ns, C->blk_mg)), dim3(C->blk_mg), 0, C->stream,
                          V.lev[l - 1].L, V.lev[l].L, A(l), A(l + 1));
     relax<R>(C, V, l + 1, 2, A(l + 1), B(l + 1), nvar, false);
   }
@@ -282,23 +327,49 @@ void vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_
 // compensated fixed-order sum)
 template <class R>
 void sum2(swm_ctx* C, const R* x0, const R* x1, const R* w, int n) {
-  hipLaunchKernelGGL(swm::dev::k_sum_partial<R>, dim3(C->npart, 2), dim3(BLK), 0, C->stream, x0, x1, w, n, C->part);
-  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part, C->npart, C->sums);
+  hipLaunchKernelGGL(swm::dev::k_sum_partial<R>, zgrid(C, C->npart, 2), dim3(BLK), 0, C->stream, x0, x1, w, n, C->part,
+                     C->mb);
+  sum_final(C, C->part, C->npart, C->sums, 2);
 }
 
+// sums[0..n) of each member in play to out[m * n + b]
 swm_status read_sums(swm_ctx* C, int n, double* out) {
-  HIP_TRY(hipMemcpyAsync(C->host_sums, C->sums, n * sizeof(double), hipMemcpyDeviceToHost, C->stream));
+  if (C->nz == 1) {
+    HIP_TRY(hipMemcpyAsync(C->host_sums, C->sums, n * sizeof(double), hipMemcpyDeviceToHost, C->stream));
+  } else {
+    HIP_TRY(hipMemcpy2DAsync(C->host_sums, n * sizeof(double), C->sums, C->mb, n * sizeof(double), C->nz,
+                             hipMemcpyDeviceToHost, C->stream));
+  }
   HIP_TRY(hipStreamSynchronize(C->stream));
-  for (int b = 0; b < n; ++b) out[b] = C->host_sums[b];
+  for (int b = 0; b < n * C->nz; ++b) out[b] = C->host_sums[b];
   return SWM_OK;
 }
 
 // convergence test of elliptic_solver.F:203-239 on the mean squared
-// residual of field v: mean(r^2) <= tol (the reference's absolute form), or,
-// when set relative, sqrt(mean(r^2) / mean(rhs^2)) <= tol
-bool converged(const swm_ctx* C, int v, double ms) {
-  if (C->solver_relative) return ms <= C->solver_tol * C->solver_tol * C->last_rhs_ms[v];
+// residual of field v of member m: mean(r^2) <= tol (the reference's
+// absolute form), or, when set relative, sqrt(mean(r^2) / mean(rhs^2)) <= tol
+bool converged(const swm_ctx* C, int m, int v, double ms) {
+  if (C->solver_relative) return ms <= C->solver_tol * C->solver_tol * C->rhs_ms[2 * m + v];
   return ms <= C->solver_tol;
+}
+
+// the mean squared residuals of fields vars of every member in play (sums
+// of the per-block partials in part, nb blocks): ms[m * 2 + b] for vars[b];
+// last_resid takes the largest over the members
+swm_status member_norms(swm_ctx* C, const double* part, int nb, const std::vector<int>& vars, double* ms) {
+  const int nv = (int)vars.size();
+  sum_final(C, part, nb, C->sums, nv);
+  std::vector<double> raw((size_t)nv * C->nz);
+  swm_status st = read_sums(C, nv, raw.data());
+  if (st) return st;
+  for (int b = 0; b < nv; ++b) C->last_resid[vars[b]] = 0.0;
+  for (int m = 0; m < C->nz; ++m)
+    for (int b = 0; b < nv; ++b) {
+      const double v = raw[(size_t)m * nv + b] / (double)C->max_ig;
+      ms[2 * m + b] = v;
+      C->last_resid[vars[b]] = std::max(C->last_resid[vars[b]], v);
+    }
+  return SWM_OK;
 }
 
 // mean squares of the right-hand sides over real cells (relative criterion)
@@ -306,53 +377,57 @@ template <class R>
 swm_status rhs_mean_squares(swm_ctx* C, const R* rhs, int nvar) {
   if (!C->solver_relative) return SWM_OK;
   const int ns = C->ns;
-  hipLaunchKernelGGL(swm::dev::k_sumsq_mixed<R>, dim3(C->npart, nvar), dim3(BLK), 0, C->stream, rhs,
-                     rhs + (size_t)(nvar - 1) * ns, C->mask, ns, C->part);
-  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(nvar), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part, C->npart, C->sums);
-  double ms[2];
-  swm_status st = read_sums(C, nvar, ms);
+  hipLaunchKernelGGL(swm::dev::k_sumsq_mixed<R>, zgrid(C, C->npart, nvar), dim3(BLK), 0, C->stream, rhs,
+                     rhs + (size_t)(nvar - 1) * ns, C->mask, ns, C->part, C->mb);
+  sum_final(C, C->part, C->npart, C->sums, nvar);
+  std::vector<double> ms((size_t)nvar * C->nz);
+  swm_status st = read_sums(C, nvar, ms.data());
   if (st) return st;
-  for (int v = 0; v < nvar; ++v) C->last_rhs_ms[v] = ms[v] / (double)C->max_ig;
+  C->rhs_ms.assign(2 * (size_t)C->nz, 0.0);
+  for (int m = 0; m < C->nz; ++m)
+    for (int v = 0; v < nvar; ++v) C->rhs_ms[2 * m + v] = ms[(size_t)m * nvar + v] / (double)C->max_ig;
+  for (int v = 0; v < nvar; ++v) C->last_rhs_ms[v] = C->rhs_ms[v];
   return SWM_OK;
 }
 
-// elliptic_solver.F:40-195 on the FP64 fields rhs / xout
+// elliptic_solver.F:40-195 on the FP64 fields rhs / xout.  With several
+// members, a field takes further cycles (for every member) while any member
+// misses the criterion.
 swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
   const int ns = C->ns;
   View<double>& V = C->d;
   for (int v = 0; v < nvar; ++v) gather<double>(C, C->face, {xout + (size_t)v * ns, rhs + (size_t)v * ns});
   sum2<double>(C, rhs, nvar > 1 ? rhs + ns : rhs, C->area_mask, ns);
-  hipLaunchKernelGGL(swm::dev::k_sub_mean<double>, dim3(nblk(ns)), dim3(BLK), 0, C->stream, ns, C->sums,
-                     C->total_area, rhs, nvar > 1 ? rhs + ns : C->r2);
+  hipLaunchKernelGGL(swm::dev::k_sub_mean<double>, zgrid(C, nblk(ns)), dim3(BLK), 0, C->stream, ns, C->sums,
+                     C->total_area, rhs, nvar > 1 ? rhs + ns : C->r2, C->mb);
   swm_status st0 = rhs_mean_squares<double>(C, rhs, nvar);
   if (st0) return st0;
   for (int it = 0; it < 2; ++it) vcycle<double>(C, V, nvar, xout, rhs);
   for (int v = 0; v < nvar; ++v) gather<double>(C, C->face, {xout + (size_t)v * ns});
-  auto norms = [&](const std::vector<int>& vars, double* out) -> swm_status {
+  const unsigned nb = nblk(ns);
+  auto norms = [&](const std::vector<int>& vars, double* ms) -> swm_status {
     const int v0 = vars[0];
-    const unsigned nb = nblk(ns);
     if (vars.size() == 2)
-      hipLaunchKernelGGL(swm::dev::k_resid2<2>, dim3(nb), dim3(BLK), 0, C->stream, V.geo, xout, rhs, C->part_slots);
+      hipLaunchKernelGGL(swm::dev::k_resid2<2>, zgrid(C, nb), dim3(BLK), 0, C->stream, V.geo, xout, rhs, C->part_slots);
     else
-      hipLaunchKernelGGL(swm::dev::k_resid2<1>, dim3(nb), dim3(BLK), 0, C->stream, V.geo, xout + (size_t)v0 * ns,
+      hipLaunchKernelGGL(swm::dev::k_resid2<1>, zgrid(C, nb), dim3(BLK), 0, C->stream, V.geo, xout + (size_t)v0 * ns,
                          rhs + (size_t)v0 * ns, C->part_slots);
-    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3((unsigned)vars.size()), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part_slots,
-                       (int)nb, C->sums);
-    double raw[2];
-    swm_status st = read_sums(C, (int)vars.size(), raw);
-    if (st) return st;
-    for (size_t b = 0; b < vars.size(); ++b) out[b] = raw[b] / (double)C->max_ig;
-    return SWM_OK;
+    return member_norms(C, C->part_slots, (int)nb, vars, ms);
   };
   std::vector<int> all;
   for (int v = 0; v < nvar; ++v) all.push_back(v);
-  double nrm[2] = {0, 0};
-  swm_status st = norms(all, nrm);
+  const int M = C->nz;
+  std::vector<double> ms(2 * (size_t)M), prev(2 * (size_t)M);
+  swm_status st = norms(all, ms.data());
   if (st) return st;
   std::vector<int> bad;
   for (int v = 0; v < nvar; ++v) {
-    C->last_resid[v] = nrm[v];
-    if (!converged(C, v, nrm[v])) bad.push_back(v);
+    bool ok = true;
+    for (int m = 0; m < M; ++m) {
+      prev[2 * m + v] = ms[2 * m + v];
+      if (!converged(C, m, v, ms[2 * m + v])) ok = false;
+    }
+    if (!ok) bad.push_back(v);
   }
   int iter = 0;
   bool any = !bad.empty();
@@ -361,18 +436,19 @@ swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
     if (bad.size() == 2) vcycle<double>(C, V, 2, xout, rhs);
     else vcycle<double>(C, V, 1, xout + (size_t)bad[0] * ns, rhs + (size_t)bad[0] * ns);
     for (int v : bad) gather<double>(C, C->face, {xout + (size_t)v * ns});
-    double nb[2];
-    st = norms(bad, nb);
+    st = norms(bad, ms.data());
     if (st) return st;
     any = false;
-    for (size_t b = 0; b < bad.size(); ++b) {
-      const double prev = C->last_resid[bad[b]];
-      C->last_resid[bad[b]] = nb[b];
-      // with solver_stall, a cycle that no longer halves the residual ends
-      // the solve: the FP64 floor of the residual is reached
-      const bool stalled = C->solver_stall && nb[b] > 0.25 * prev;
-      if (!converged(C, bad[b], nb[b]) && !stalled) any = true;
-    }
+    for (int m = 0; m < M; ++m)
+      for (size_t b = 0; b < bad.size(); ++b) {
+        const int v = bad[b];
+        const double now = ms[2 * m + b];
+        // with solver_stall, a cycle that no longer halves the residual ends
+        // the solve: the FP64 floor of the residual is reached
+        const bool stalled = C->solver_stall && now > 0.25 * prev[2 * m + v];
+        prev[2 * m + v] = now;
+        if (!converged(C, m, v, now) && !stalled) any = true;
+      }
   }
   C->last_vcycles = 2 + iter;
   C->solves += 1; C->vcycles_total += 2 + iter; C->checks_total += 1 + iter;
@@ -407,16 +483,20 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
   float* r = C->mx_r;
   float* e = C->mx_e;
   const unsigned nb = nblk(ns);
+  const size_t mb = C->mb;
   gather<double>(C, C->face, {xout, xout + ns});
   gather<float>(C, C->face, {rhs, rhs + ns});
   // area-weighted means in FP64 (elliptic_solver.F:80-92)
-  hipLaunchKernelGGL((swm::dev::k_sum_partial<float, double>), dim3(C->npart, 2), dim3(BLK), 0, C->stream, rhs,
-                     rhs + ns, C->area_mask, ns, C->part);
-  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part, C->npart, C->sums);
-  hipLaunchKernelGGL(swm::dev::k_sub_mean<float>, dim3(nblk(ns)), dim3(BLK), 0, C->stream, ns, C->sums,
-                     C->total_area, rhs, rhs + ns);
-  swm_status st0 = rhs_mean_squares<float>(C, rhs, 2);
-  if (st0) return st0;
+  hipLaunchKernelGGL((swm::dev::k_sum_partial<float, double>), zgrid(C, C->npart, 2), dim3(BLK), 0, C->stream, rhs,
+                     rhs + ns, C->area_mask, ns, C->part, mb);
+  sum_final(C, C->part, C->npart, C->sums, 2);
+  hipLaunchKernelGGL(swm::dev::k_sub_mean<float>, zgrid(C, nblk(ns)), dim3(BLK), 0, C->stream, ns, C->sums,
+                     C->total_area, rhs, rhs + ns, mb);
+  const bool fixed = C->fixed_cycles > 0;
+  if (!fixed) {
+    swm_status st0 = rhs_mean_squares<float>(C, rhs, 2);
+    if (st0) return st0;
+  }
   // the extrapolated start goes to xalt; then the iterates alternate between
   // xalt and xprev, whose content is no longer needed once it is read
   double* blk[3] = {xout, C->xalt, C->xprev};
@@ -429,10 +509,10 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
     const size_t o = (size_t)v0 * ns;
     const double* gg = shift[v0] ? C->gauge + v0 : nullptr;
     if (nv == 2)
-      hipLaunchKernelGGL((swm::dev::k_defect<2, true>), dim3(nb), dim3(BLK), 0, C->stream, G64, cur[0], e, oth[0],
+      hipLaunchKernelGGL((swm::dev::k_defect<2, true>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, cur[0], e, oth[0],
                          rhs, r, C->part_slots, gg, inv_area);
     else
-      hipLaunchKernelGGL((swm::dev::k_defect<1, true>), dim3(nb), dim3(BLK), 0, C->stream, G64, cur[v0], e + o,
+      hipLaunchKernelGGL((swm::dev::k_defect<1, true>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, cur[v0], e + o,
                          oth[v0], rhs + o, r + o, C->part_slots, gg, inv_area);
     for (int v = v0; v < v0 + nv; ++v) {
       std::swap(cur[v], oth[v]);
@@ -447,69 +527,72 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
     else gather<float>(C, C->face, {e + (size_t)v0 * ns});
     launch_defect(nv, v0);
   };
-  auto norms = [&](int nv, double* out) -> swm_status {
-    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(nv), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part_slots, (int)nb, C->sums);
-    double raw[2];
-    swm_status st = read_sums(C, nv, raw);
-    if (st) return st;
-    for (int b = 0; b < nv; ++b) out[b] = raw[b] / (double)C->max_ig;
-    return SWM_OK;
-  };
-  const bool extrap = C->have_prev && C->mixed_extrapolate;
-  hipLaunchKernelGGL(swm::dev::k_defect_extrap<2>, dim3(nb), dim3(BLK), 0, C->stream, G64, C->area_mask, blk[0],
+  const bool extrap = C->nsol >= 2 && C->mixed_extrapolate;
+  hipLaunchKernelGGL(swm::dev::k_defect_extrap<2>, zgrid(C, nb), dim3(BLK), 0, C->stream, G64, C->area_mask, blk[0],
                      extrap ? blk[2] : blk[0], blk[1], rhs, r, C->part_slots);
   if (extrap) {
-    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part_slots + 2 * (size_t)nb,
-                       (int)nb, C->gauge);
+    sum_final(C, C->part_slots + 2 * (size_t)nb, (int)nb, C->gauge, 2);
     shift[0] = shift[1] = true;
   }
-  for (int it = 0; it < C->mixed_min_cycles; ++it) cycle(2, 0);
-  double nrm[2];
-  swm_status st = norms(2, nrm);
-  if (st) return st;
-  std::vector<int> bad;
-  for (int v = 0; v < 2; ++v) {
-    C->last_resid[v] = nrm[v];
-    if (!converged(C, v, nrm[v])) bad.push_back(v);
-  }
-  int iter = C->mixed_min_cycles;
-  bool any = !bad.empty();
-  while (any && iter < C->solver_max_cycles) {
-    ++iter;
-    const int nv = (int)bad.size(), v0 = bad[0];
-    if (nv == 2 && cur[1] != cur[0] + ns) {
-      // the fields ended on different buffers: bring field 1 next to field 0
-      hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, C->stream, cur[0] + ns, cur[1], (size_t)ns);
-      oth[1] = cur[1];
-      cur[1] = cur[0] + ns;
-    }
-    cycle(nv, v0);
-    double nbv[2];
-    st = norms(nv, nbv);
+  int iter;
+  bool any = false;
+  if (fixed) {
+    // the fixed schedule: no host-side test; the last defect's mean squares
+    // go to the member's residual record on the device
+    iter = C->nsol >= 2 ? C->fixed_cycles : std::max(C->fixed_cycles, C->fixed_start);
+    for (int it = 0; it < iter; ++it) cycle(2, 0);
+    sum_final(C, C->part_slots, (int)nb, C->sums, 2);
+    hipLaunchKernelGGL(swm::dev::k_track_resid, zgrid(C, 1), dim3(64), 0, C->stream, C->sums, 1.0 / (double)C->max_ig,
+                       C->solver_tol, C->track, mb);
+  } else {
+    for (int it = 0; it < C->mixed_min_cycles; ++it) cycle(2, 0);
+    const int M = C->nz;
+    std::vector<double> ms(2 * (size_t)M);
+    swm_status st = member_norms(C, C->part_slots, (int)nb, {0, 1}, ms.data());
     if (st) return st;
-    any = false;
-    for (int b = 0; b < nv; ++b) {
-      C->last_resid[bad[b]] = nbv[b];
-      if (!converged(C, bad[b], nbv[b])) any = true;
+    std::vector<int> bad;
+    for (int v = 0; v < 2; ++v) {
+      bool ok = true;
+      for (int m = 0; m < M; ++m)
+        if (!converged(C, m, v, ms[2 * m + v])) ok = false;
+      if (!ok) bad.push_back(v);
+    }
+    iter = C->mixed_min_cycles;
+    any = !bad.empty();
+    while (any && iter < C->solver_max_cycles) {
+      ++iter;
+      const int nv = (int)bad.size(), v0 = bad[0];
+      if (nv == 2 && cur[1] != cur[0] + ns) {
+        // the fields ended on different buffers: bring field 1 next to field 0
+        copy_state(C, cur[0] + ns, cur[1], (size_t)ns);
+        oth[1] = cur[1];
+        cur[1] = cur[0] + ns;
+      }
+      cycle(nv, v0);
+      st = member_norms(C, C->part_slots, (int)nb, bad, ms.data());
+      if (st) return st;
+      any = false;
+      for (int m = 0; m < M; ++m)
+        for (int b = 0; b < nv; ++b)
+          if (!converged(C, m, bad[b], ms[2 * m + b])) any = true;
     }
   }
   // a field that converged without a correction keeps the extrapolated
   // mean change: taken off here
   for (int v = 0; v < 2; ++v)
     if (shift[v])
-      hipLaunchKernelGGL(swm::dev::k_add_scaled, dim3(nb), dim3(BLK), 0, C->stream, (size_t)ns, C->gauge + v,
-                         -inv_area, cur[v]);
+      hipLaunchKernelGGL(swm::dev::k_add_scaled, zgrid(C, nb), dim3(BLK), 0, C->stream, (size_t)ns, C->gauge + v,
+                         -inv_area, cur[v], mb);
   // both fields into one buffer (the other field's plane there is free),
   // which becomes xout; the previous solution becomes xprev
-  if (cur[1] != cur[0] + ns)
-    hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, C->stream, cur[0] + ns, cur[1], (size_t)ns);
+  if (cur[1] != cur[0] + ns) copy_state(C, cur[0] + ns, cur[1], (size_t)ns);
   double* fin = cur[0];
   C->xprev = blk[0];
   C->xout = fin;
   C->xalt = fin == blk[1] ? blk[2] : blk[1];
-  C->have_prev = true;
+  C->nsol = std::min(C->nsol + 1, 2);
   C->last_vcycles = iter;
-  C->solves += 1; C->vcycles_total += iter; C->checks_total += 1 + iter;
+  C->solves += 1; C->vcycles_total += iter; C->checks_total += fixed ? 0 : 1 + iter;
   if (any) {
     swm::set_error("elliptic solve did not converge (residual above the criterion after " +
                    std::to_string(C->solver_max_cycles) + " correction cycles)");
@@ -534,7 +617,7 @@ template <class R>
 void wrap_prognostics(swm_ctx* C, View<R>& V) {   // step.F:438-493
   const int n4 = C->n4 - 1;
   const int ns = C->ns;
-  hipLaunchKernelGGL(swm::dev::k_inv_h<R>, dim3(nblk(ns)), dim3(BLK), 0, C->stream, V.geo, V.h[n4], V.inv_h);
+  hipLaunchKernelGGL(swm::dev::k_inv_h<R>, zgrid(C, nblk_b(ns, C->blk_step)), dim3(C->blk_step), 0, C->stream, V.geo, V.h[n4], V.inv_h);
   std::vector<R*> f = {V.h[n4], V.eta[n4], V.div[n4], V.relative, V.ke, V.inv_h};
   for (int q = 0; q < C->nt; ++q) f.push_back(V.tr[n4][q]);
   gather_list<R>(C, C->face, f);
@@ -545,7 +628,7 @@ void wrap_prognostics(swm_ctx* C, View<R>& V) {   // step.F:438-493
 template <class R>
 swm_status step_dt(swm_ctx* C, View<R>& V) {
   const int ns = C->ns;
-  const unsigned nb = nblk(ns);
+  const unsigned nb = nblk_b(ns, C->blk_step);
   hipStream_t S = C->stream;
   const swm::dev::GeoT<R>& G = V.geo;
   const int n3 = C->n3 - 1, n4 = C->n4 - 1;
@@ -554,67 +637,67 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   R* m2 = V.msh_f[C->n2f - 1];
   R* m1 = V.msh_f[C->n1f - 1];
 
-  hipLaunchKernelGGL(swm::dev::k_avg_corners<R>, dim3(nb), dim3(BLK), 0, S, G, G.sw, V.h[n3], V.hc);
+  hipLaunchKernelGGL(swm::dev::k_avg_corners<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, G.sw, V.h[n3], V.hc);
   gather<R>(C, C->vrtx, {V.hc});
   if (C->advection != SWM_ADV_FCT) {
     // get_centered_flux / get_upwind_flux (step.F:495-602): the centred or
     // upstream mass flux (the other one lands in scratch), tracer fluxes,
     // then step_surface_height and step_tracer (step.F:221-327)
     const bool cen = C->advection == SWM_ADV_CENTERED;
-    hipLaunchKernelGGL(swm::dev::k_mass_fluxes<R>, dim3(nb), dim3(BLK), 0, S, G, V.u[n3], V.h[n3], V.hc,
+    hipLaunchKernelGGL(swm::dev::k_mass_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.u[n3], V.h[n3], V.hc,
                        cen ? V.flx1 : V.mass_flux, cen ? V.mass_flux : V.flx1);
     for (int q = 0; q < C->nt; ++q) {
       if (cen)
-        hipLaunchKernelGGL((swm::dev::k_tr_flux<R, true>), dim3(nb), dim3(BLK), 0, S, G, V.mass_flux, V.tr[n3][q],
+        hipLaunchKernelGGL((swm::dev::k_tr_flux<R, true>), zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.mass_flux, V.tr[n3][q],
                            V.tflux[q]);
       else
-        hipLaunchKernelGGL((swm::dev::k_tr_flux<R, false>), dim3(nb), dim3(BLK), 0, S, G, V.mass_flux, V.tr[n3][q],
+        hipLaunchKernelGGL((swm::dev::k_tr_flux<R, false>), zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.mass_flux, V.tr[n3][q],
                            V.tflux[q]);
     }
     std::vector<R*> ef = {V.mass_flux};
     for (int q = 0; q < C->nt; ++q) ef.push_back(V.tflux[q]);
     gather_list<R>(C, C->edge, ef);
     const int a = C->n3f - 1, b = C->n2f - 1, c = C->n1f - 1;
-    hipLaunchKernelGGL(swm::dev::k_height_update<R>, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, V.mass_flux,
+    hipLaunchKernelGGL(swm::dev::k_height_update<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, w3, w2, w1, V.mass_flux,
                        V.h[n3], V.h_f[a], V.h_f[b], V.h_f[c], V.h[n4]);
     for (int q = 0; q < C->nt; ++q)
-      hipLaunchKernelGGL(swm::dev::k_tracer_update<R>, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, V.tflux[q],
+      hipLaunchKernelGGL(swm::dev::k_tracer_update<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, w3, w2, w1, V.tflux[q],
                          V.tr[n3][q], V.h[n3], V.h[n4], V.tr_f[a][q], V.tr_f[b][q], V.tr_f[c][q], V.tr[n4][q]);
   } else {
   // mass FCT (mass_fct.F:23-539)
-  hipLaunchKernelGGL(swm::dev::k_mass_fluxes<R>, dim3(nb), dim3(BLK), 0, S, G, V.u[n3], V.h[n3], V.hc, V.flx1, m3);
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.u[n3], V.h[n3], V.hc, V.flx1, m3);
   gather<R>(C, C->edge, {V.flx1, m3});
-  hipLaunchKernelGGL(swm::dev::k_mass_low_beta<R>, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, V.flx1, m3, m2, m1,
+  hipLaunchKernelGGL(swm::dev::k_mass_low_beta<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, w3, w2, w1, V.flx1, m3, m2, m1,
                      V.h[n3], V.anti, V.mss_low, V.beta);
   gather<R>(C, C->face, {V.beta});
-  hipLaunchKernelGGL(swm::dev::k_mass_anti<R>, dim3(nb), dim3(BLK), 0, S, G, V.beta, V.anti);
+  hipLaunchKernelGGL(swm::dev::k_mass_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.beta, V.anti);
   gather<R>(C, C->edge, {V.anti});
-  hipLaunchKernelGGL(swm::dev::k_mass_final<R>, dim3(nb), dim3(BLK), 0, S, G, dt, V.flx1, V.anti, V.mss_low,
+  hipLaunchKernelGGL(swm::dev::k_mass_final<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, V.flx1, V.anti, V.mss_low,
                      V.mass_flux, V.h[n4]);
   // the tracer limiter fills the halo of the new mass in place (tracer_fct.F:344)
   gather<R>(C, C->face, {V.h[n4]});
   for (int q = 0; q < C->nt; ++q) {   // tracer FCT (tracer_fct.F:22-670)
     R* t3 = V.tr[n3][q];
     R* t4 = V.tr[n4][q];
-    hipLaunchKernelGGL(swm::dev::k_tr_fluxes<R>, dim3(nb), dim3(BLK), 0, S, G, w3, w2, w1, t3, t4, V.told[q],
+    hipLaunchKernelGGL(swm::dev::k_tr_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, w3, w2, w1, t3, t4, V.told[q],
                        V.mass_flux, V.f1, V.f2);
     gather<R>(C, C->edge, {V.f1, V.f2});
-    hipLaunchKernelGGL(swm::dev::k_tr_low<R>, dim3(nb), dim3(BLK), 0, S, G, dt, V.h[n3], t3, V.f1, V.ml);
+    hipLaunchKernelGGL(swm::dev::k_tr_low<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, V.h[n3], t3, V.f1, V.ml);
     gather<R>(C, C->face, {V.ml});
-    hipLaunchKernelGGL(swm::dev::k_tr_limiter<R>, dim3(nb), dim3(BLK), 0, S, G, dt, t3, V.ml, V.h[n4], V.mss_low,
+    hipLaunchKernelGGL(swm::dev::k_tr_limiter<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, t3, V.ml, V.h[n4], V.mss_low,
                        V.f1, V.f2, V.anti, V.bdn, V.bup);
     gather<R>(C, C->face, {V.bdn, V.bup});
-    hipLaunchKernelGGL(swm::dev::k_tr_anti<R>, dim3(nb), dim3(BLK), 0, S, G, V.bdn, V.bup, V.anti);
+    hipLaunchKernelGGL(swm::dev::k_tr_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.bdn, V.bup, V.anti);
     gather<R>(C, C->edge, {V.anti});
-    hipLaunchKernelGGL(swm::dev::k_tr_final<R>, dim3(nb), dim3(BLK), 0, S, G, dt, V.ml, V.anti, V.h[n4], t4,
+    hipLaunchKernelGGL(swm::dev::k_tr_final<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, V.ml, V.anti, V.h[n4], t4,
                        V.told[q]);
   }
   }
   // vorticity and divergence (step.F:143-163, :336-429)
-  hipLaunchKernelGGL(swm::dev::k_vort_curl<R>, dim3(nb), dim3(BLK), 0, S, G, V.mass_flux, V.h[n3], V.eta[n3], V.hc,
+  hipLaunchKernelGGL(swm::dev::k_vort_curl<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.mass_flux, V.h[n3], V.eta[n3], V.hc,
                      V.u[n3], V.vf, V.dcurl);
   gather<R>(C, C->edge, {V.vf, V.dcurl});
-  hipLaunchKernelGGL(swm::dev::k_tendencies<R>, dim3(nb), dim3(BLK), 0, S, G, dt, grav, w3, w2, w1, V.vf, V.dcurl,
+  hipLaunchKernelGGL(swm::dev::k_tendencies<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, grav, w3, w2, w1, V.vf, V.dcurl,
                      V.ke, V.h[n3], V.eta[n3], V.div[n3], V.eta_f[C->n3f - 1], V.eta_f[C->n2f - 1],
                      V.eta_f[C->n1f - 1], V.div_f[C->n3f - 1], V.div_f[C->n2f - 1], V.div_f[C->n1f - 1], V.eta[n4],
                      V.div[n4], V.relative, V.rhs, V.rhs + ns);
@@ -623,10 +706,10 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   if constexpr (sizeof(R) == 8) st = elliptic(C, V.rhs, C->xout, 2);
   else st = elliptic_mixed(C, V.rhs, C->xout);
   if (st) return st;
-  hipLaunchKernelGGL(swm::dev::k_post_elliptic<R>, dim3(nb), dim3(BLK), 0, S, G, V.rhs, V.eta[n4]);
-  hipLaunchKernelGGL(swm::dev::k_velocity<R>, dim3(nb), dim3(BLK), 0, S, G, C->xout, C->xout + ns, V.u[n4]);
+  hipLaunchKernelGGL(swm::dev::k_post_elliptic<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.rhs, V.eta[n4]);
+  hipLaunchKernelGGL(swm::dev::k_velocity<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, C->xout, C->xout + ns, V.u[n4]);
   gather<R>(C, C->vrtx, {V.u[n4], V.u[n4] + 2 * (size_t)ns});
-  hipLaunchKernelGGL(swm::dev::k_center_ke<R>, dim3(nb), dim3(BLK), 0, S, G, V.u[n4], V.ke);
+  hipLaunchKernelGGL(swm::dev::k_center_ke<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.u[n4], V.ke);
   gather<R>(C, C->face, {V.ke});
   return SWM_OK;
 }
@@ -669,14 +752,17 @@ void reset_state(swm_ctx* C, bool keep_hs = false) {
   C->solves = C->vcycles_total = C->checks_total = 0;
   fill<double>(C, C->d.state_begin, C->d.state_words, 0.0);
   if (C->precision == SWM_MIXED) fill<float>(C, C->m.state_begin, C->m.state_words, 0.0f);
-  if (!keep_hs) fill<double>(C, C->hs, C->ns, 0.0);
+  if (!keep_hs) fill_plain<double>(C, C->hs, C->ns, 0.0);
   fill<double>(C, C->xout, 2 * (size_t)C->ns, 0.0);
   fill<double>(C, C->xalt, 2 * (size_t)C->ns, 0.0);
   fill<double>(C, C->xprev, 2 * (size_t)C->ns, 0.0);
-  C->have_prev = false;
+  C->nsol = 0;
   C->step_count = 0;
   C->nsecond = 0;
   C->initialized = false;
+  C->member_init[C->sel] = 0;
+  C->epoch += 1;
+  fill<double>(C, C->track, 4, 0.0);
   pointers(C);
 }
 
@@ -698,8 +784,8 @@ swm_status finish_init(swm_ctx* C) {
   hipStream_t S = C->stream;
   const unsigned nb = nblk(ns);
   View<double>& D = C->d;
-  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, D.h[1], D.h[0], (size_t)ns);
-  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nblk(4 * (size_t)ns)), dim3(BLK), 0, S, D.u[1], D.u[0], 4 * (size_t)ns);
+  copy_state(C, D.h[1], D.h[0], (size_t)ns);
+  copy_state(C, D.u[1], D.u[0], 4 * (size_t)ns);
   for (int t = 0; t < 2; ++t) {
     gather<double>(C, C->face, {D.h[t]});
     gather<double>(C, C->vrtx, {D.u[t], D.u[t] + 2 * (size_t)ns});
@@ -709,19 +795,24 @@ swm_status finish_init(swm_ctx* C) {
   hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb), dim3(BLK), 0, S, D.geo, D.u[0], D.relative, D.div[0]);
   hipLaunchKernelGGL(swm::dev::k_center_ke<double>, dim3(nb), dim3(BLK), 0, S, D.geo, D.u[0], D.ke);
   gather<double>(C, C->face, {D.relative, D.div[0], D.ke});
-  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, D.div[1], D.div[0], (size_t)ns);
+  copy_state(C, D.div[1], D.div[0], (size_t)ns);
   hipLaunchKernelGGL(swm::dev::k_post_elliptic<double>, dim3(nb), dim3(BLK), 0, S, D.geo, D.relative, D.eta[0]);
-  hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, D.eta[1], D.eta[0], (size_t)ns);
+  copy_state(C, D.eta[1], D.eta[0], (size_t)ns);
   for (int q = 0; q < C->nt; ++q)
-    hipLaunchKernelGGL(swm::dev::k_copy, dim3(nb), dim3(BLK), 0, S, D.tr[1][q], D.tr[0][q], (size_t)ns);
+    copy_state(C, D.tr[1][q], D.tr[0][q], (size_t)ns);
   wrap_prognostics<double>(C, D);
   if (C->precision == SWM_MIXED) {
     View<float>& M = C->m;
-    sum2<double>(C, D.h[0], D.h[0], C->area_mask, ns);
-    double sh[2];
-    swm_status st = read_sums(C, 2, sh);
-    if (st) return st;
-    C->h0 = sh[0] / C->total_area;
+    // the thickness offset: the area-weighted mean thickness; the members
+    // of an ensemble share the one of the member initialised first
+    if (C->nmem == 1 || !C->h0_set) {
+      sum2<double>(C, D.h[0], D.h[0], C->area_mask, ns);
+      double sh[2];
+      swm_status st = read_sums(C, 2, sh);
+      if (st) return st;
+      C->h0 = sh[0] / C->total_area;
+      C->h0_set = true;
+    }
     M.geo.h0 = (float)C->h0;
     to32(C, C->hs, const_cast<float*>(M.geo.hs), ns);
     for (int t = 0; t < 2; ++t) {
@@ -737,7 +828,8 @@ swm_status finish_init(swm_ctx* C) {
   }
   HIP_TRY(hipStreamSynchronize(S));
   HIP_TRY(hipGetLastError());
-  C->initialized = true;
+  C->member_init[C->sel] = 1;
+  C->initialized = std::all_of(C->member_init.begin(), C->member_init.end(), [](char c) { return c != 0; });
   return SWM_OK;
 }
 
@@ -788,7 +880,7 @@ struct OpIO {
   }
   double* zeros(int nplanes) {
     double* d = dev(nplanes);
-    if (d) fill<double>(C, d, (size_t)nplanes * ns, 0.0);
+    if (d) fill_plain<double>(C, d, (size_t)nplanes * ns, 0.0);
     return d;
   }
   swm_status get(double* ref, const double* d, int lead, const std::vector<int>& perm = {}) {
@@ -822,11 +914,13 @@ void planes_to_vars(const double* planes, int nvar, int pl, double* ref) {
 
 namespace swm {
 
-// buffers of a view in the arena: multigrid levels (the halo maps are shared
-// with the FP64 view when given), state, scratch
+// buffers of a view: multigrid level weights and maps in the shared arena
+// (the maps shared with the FP64 view when given); the levels' fields, the
+// state and scratch in the member block
 template <class R>
 void layout_view(swm_ctx* C, View<R>& V, const View<double>* maps) {
   Arena& A = C->arena;
+  Arena& MA = C->marena;
   const size_t ns = C->ns;
   V.lev.resize(C->g);
   for (int l = 1; l <= C->g; ++l) {
@@ -835,12 +929,12 @@ void layout_view(swm_ctx* C, View<R>& V, const View<double>* maps) {
     L.ns = L.iim * L.jjm * 10;
     const size_t nl = L.ns;
     L.lw = A.take<R>(7 * nl); L.lwt = A.take<R>(nl);
-    L.alpha = A.take<R>(2 * nl); L.beta = A.take<R>(2 * nl); L.temp = A.take<R>(2 * nl);
+    L.alpha = MA.take<R>(2 * nl); L.beta = MA.take<R>(2 * nl); L.temp = MA.take<R>(2 * nl);
     L.src = maps ? maps->lev[l - 1].src : A.take<int>(nl);
-    L.L = swm::dev::LevT<R>{L.iim, L.jjm, L.ns, L.lw, L.lwt, L.src};
+    L.L = swm::dev::LevT<R>{L.iim, L.jjm, L.ns, L.lw, L.lwt, L.src, 0};
   }
-  auto take = [&](size_t n) { return A.take<R>(n); };
-  const size_t before = A.used;
+  auto take = [&](size_t n) { return MA.take<R>(n); };
+  const size_t before = MA.used;
   for (int t = 0; t < 2; ++t) {
     V.h[t] = take(ns); V.eta[t] = take(ns); V.div[t] = take(ns); V.u[t] = take(4 * ns);
     V.tr[t].clear();
@@ -861,7 +955,7 @@ void layout_view(swm_ctx* C, View<R>& V, const View<double>* maps) {
   V.beta = take(ns); V.f1 = take(3 * ns); V.f2 = take(3 * ns); V.ml = take(ns); V.bdn = take(ns); V.bup = take(ns);
   V.vf = take(3 * ns); V.dcurl = take(3 * ns); V.rhs = take(2 * ns);
   V.state_begin = V.h[0];
-  V.state_words = (A.used - before) / sizeof(R);
+  V.state_words = (MA.used - before) / sizeof(R);
 }
 
 // FP64 metric planes (reference layout -> planes), tables, levels and views
@@ -875,10 +969,13 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
   make_table(C, Gh.face, C->ns, C->face);
   make_table(C, Gh.vrtx, C->ns, C->vrtx);
   make_table(C, Gh.edge, C->ns, C->edge);
-  C->xout = take(2 * ns); C->r2 = take(ns); C->xalt = take(2 * ns); C->xprev = take(2 * ns); C->gauge = take(2);
-  C->part_slots = take(4 * (size_t)nblk(ns));
-  C->part = take(2 * (size_t)C->npart); C->sums = take(64);
-  C->part_budget = take((8 + swm::dev::BUDGET_MAXTR) * (size_t)C->npart);
+  // the member block: solver iterates, partial sums, then the views' state
+  auto mt = [&](size_t n) { return C->marena.take<double>(n); };
+  C->xout = mt(2 * ns); C->r2 = mt(ns); C->xalt = mt(2 * ns); C->xprev = mt(2 * ns); C->gauge = mt(2);
+  C->part_slots = mt(4 * (size_t)nblk(ns));
+  C->part = mt(2 * (size_t)C->npart); C->sums = mt(64);
+  C->part_budget = mt((8 + swm::dev::BUDGET_MAXTR) * (size_t)C->npart);
+  C->track = mt(4);
   layout_view<double>(C, C->d, nullptr);
   C->d.geo = swm::dev::Geo{C->iim, C->jjm, C->ns, C->area, C->area_inv, C->cai, C->sub_area, C->dc, C->nrm, C->tng,
                            C->vw, C->sw, C->lw, C->f, C->hs, C->mask, 0.0, nullptr};
@@ -891,10 +988,59 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
           *mask = tf(ns);
     g = swm::dev::GeoT<float>{C->iim, C->jjm, C->ns, area, ai, cai, sa, dc, nrm, tng, vw, sw, lw, f, hs, mask, 0.0f, f};
     layout_view<float>(C, C->m, &C->d);
-    C->mx_r = tf(2 * ns);
-    C->mx_e = tf(2 * ns);
+    C->mx_r = C->marena.take<float>(2 * ns);
+    C->mx_e = C->marena.take<float>(2 * ns);
   }
 }
+
+// every pointer into the member block moves by d bytes (selects another
+// member's block for host-side calls, and back)
+template <class T>
+void shift_ptr(T*& p, ptrdiff_t d) {
+  if (p) p = reinterpret_cast<T*>(reinterpret_cast<char*>(p) + d);
+}
+template <class R>
+void shift_view(View<R>& V, ptrdiff_t d) {
+  for (LevelT<R>& L : V.lev) { shift_ptr(L.alpha, d); shift_ptr(L.beta, d); shift_ptr(L.temp, d); }
+  for (int t = 0; t < 2; ++t) {
+    shift_ptr(V.h[t], d); shift_ptr(V.eta[t], d); shift_ptr(V.div[t], d); shift_ptr(V.u[t], d);
+    for (R*& p : V.tr[t]) shift_ptr(p, d);
+  }
+  for (R*& p : V.told) shift_ptr(p, d);
+  for (int t = 0; t < 3; ++t) {
+    shift_ptr(V.eta_f[t], d); shift_ptr(V.div_f[t], d); shift_ptr(V.msh_f[t], d); shift_ptr(V.h_f[t], d);
+    for (R*& p : V.tr_f[t]) shift_ptr(p, d);
+  }
+  for (R*& p : V.tflux) shift_ptr(p, d);
+  for (R** p : {&V.relative, &V.ke, &V.inv_h, &V.mass_flux, &V.hc, &V.flx1, &V.anti, &V.mss_low, &V.beta, &V.f1,
+                &V.f2, &V.ml, &V.bdn, &V.bup, &V.vf, &V.dcurl, &V.rhs, &V.state_begin})
+    shift_ptr(*p, d);
+}
+void shift_members(swm_ctx* C, ptrdiff_t d) {
+  if (d == 0) return;
+  shift_view(C->d, d);
+  if (C->precision == SWM_MIXED) shift_view(C->m, d);
+  for (double** p : {&C->xout, &C->r2, &C->xalt, &C->xprev, &C->gauge, &C->part_slots, &C->part, &C->sums,
+                     &C->part_budget, &C->track})
+    shift_ptr(*p, d);
+  shift_ptr(C->mx_r, d);
+  shift_ptr(C->mx_e, d);
+}
+
+// host-side calls address one member: its block's pointers and launches over
+// that member alone, for the scope's lifetime
+struct MemberScope {
+  swm_ctx* C;
+  ptrdiff_t d;
+  explicit MemberScope(swm_ctx* c) : C(c), d((ptrdiff_t)c->sel * (ptrdiff_t)c->mb) {
+    shift_members(C, d);
+    C->nz = 1;
+  }
+  ~MemberScope() {
+    shift_members(C, -d);
+    C->nz = C->nmem;
+  }
+};
 
 swm_status up64(double* d, const std::vector<double>& h) {
   HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -913,7 +1059,7 @@ void launch_budgets(swm_ctx* C, View<R>& V, int nq) {
   B.grav = C->grav;
   B.ns = C->ns;
   hipLaunchKernelGGL(swm::dev::k_budgets<R>, dim3(C->npart, nq), dim3(BLK), 0, C->stream, B, C->part_budget);
-  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(nq), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part_budget, C->npart, C->sums);
+  sum_final(C, C->part_budget, C->npart, C->sums, nq);
 }
 
 swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
@@ -921,20 +1067,32 @@ swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
   HIP_TRY(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&C->ev_a));
   HIP_TRY(hipEventCreate(&C->ev_b));
-  // sizing pass, then the arena
+  // sizing pass, then the shared arena and the member blocks
   C->arena = Arena{};
+  C->marena = Arena{};
   layout_all(C, Gh);
   C->arena_bytes = C->arena.used + 4096;
+  C->mb = ((C->marena.used + 4095) / 4096) * 4096;
+  const size_t member_bytes = C->mb;
   C->arena = Arena{};
+  C->marena = Arena{};
   HIP_TRY(hipMalloc(&C->arena.base, C->arena_bytes));
   C->arena.cap = C->arena_bytes;
+  HIP_TRY(hipMalloc(&C->marena.base, member_bytes * (size_t)C->nmem));
+  C->marena.cap = member_bytes;
   C->face = C->vrtx = C->edge = DevTable{};
   layout_all(C, Gh);
-  if (C->arena.used > C->arena.cap) {
+  if (C->arena.used > C->arena.cap || C->marena.used > C->marena.cap) {
     swm::set_error("swm_create: arena overflow");
     return SWM_ERR_HIP;
   }
-  HIP_TRY(hipHostMalloc(&C->host_sums, 64 * sizeof(double), hipHostMallocDefault));
+  C->d.geo.mb = C->mb;
+  C->m.geo.mb = C->mb;
+  for (LevelT<double>& L : C->d.lev) L.L.mb = C->mb;
+  for (LevelT<float>& L : C->m.lev) L.L.mb = C->mb;
+  C->nz = C->nmem;
+  C->member_init.assign(C->nmem, 0);
+  HIP_TRY(hipHostMalloc(&C->host_sums, 64 * (size_t)C->nmem * sizeof(double), hipHostMallocDefault));
   const int ns = C->ns;
   auto id = [](int n) { std::vector<int> v(n); for (int k = 0; k < n; ++k) v[k] = k; return v; };
   swm_status st;
@@ -1001,10 +1159,8 @@ swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
     }
     (void)nb;
   }
-  // zero every state buffer (the reference initialises its module arrays to 0)
-  fill<double>(C, C->d.state_begin, C->d.state_words, 0.0);
-  if (C->precision == SWM_MIXED) fill<float>(C, C->m.state_begin, C->m.state_words, 0.0f);
-  fill<double>(C, C->xout, 2 * (size_t)ns, 0.0);
+  // zero every member block (the reference initialises its module arrays to 0)
+  fill_plain<double>(C, reinterpret_cast<double*>(C->marena.base), member_bytes / 8 * (size_t)C->nmem, 0.0);
   HIP_TRY(hipStreamSynchronize(S));
   HIP_TRY(hipGetLastError());
   return SWM_OK;
@@ -1019,7 +1175,11 @@ void tlad_free(swm_ctx* C);
 extern "C" {
 
 swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out) {
-  if (!grid || !cfg || !out || cfg->ntracer < 0 || cfg->ntracer > 64 || !(cfg->dt > 0.0) ||
+  return swm_create_members(grid, cfg, 1, out);
+}
+
+swm_status swm_create_members(const swm_grid* grid, const swm_config* cfg, int nmem, swm_ctx** out) {
+  if (!grid || !cfg || !out || nmem < 1 || nmem > 4096 || cfg->ntracer < 0 || cfg->ntracer > 64 || !(cfg->dt > 0.0) ||
       (cfg->precision != SWM_FP64 && cfg->precision != SWM_MIXED) || cfg->advection < SWM_ADV_FCT ||
       cfg->advection > SWM_ADV_UPWIND) {
     swm::set_error("swm_create: bad argument");
@@ -1040,6 +1200,7 @@ swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out
   C->radius = Gh.radius;
   C->cell_index = Gh.center_index;
   C->cell_mask = Gh.center_mask;
+  C->nmem = nmem;
   swm_status st = swm::create_device(C, Gh);
   if (st) {
     swm_destroy(C);
@@ -1058,6 +1219,7 @@ void swm_destroy(swm_ctx* C) {
   if (C->ev_a) (void)hipEventDestroy(C->ev_a);
   if (C->ev_b) (void)hipEventDestroy(C->ev_b);
   if (C->arena.base) (void)hipFree(C->arena.base);
+  if (C->marena.base) (void)hipFree(C->marena.base);
   if (C->host_sums) (void)hipHostFree(C->host_sums);
   swm::tlad_free(C);
   if (C->stream) (void)hipStreamDestroy(C->stream);
@@ -1070,6 +1232,7 @@ namespace swm {
 swm_status init_from_host(swm_ctx* C, const double* h, const double* hs, const double* u, const double* tracer,
                           bool keep_hs) {
   HIP_TRY(hipSetDevice(C->device));
+  MemberScope scope(C);
   const int ns = C->ns;
   hipStream_t S = C->stream;
   std::vector<double> up(4 * (size_t)ns), tq((size_t)C->nt * ns);
@@ -1098,7 +1261,7 @@ swm_status swm_init_state(swm_ctx* C, const double* h, const double* hs, const d
 // level's h, tracers and corner velocity of every real cell and the clock
 swm_status swm_write_restart(swm_ctx* C, const char* path) {
   if (!C) { swm::set_error("swm_write_restart: bad argument"); return SWM_ERR_ARG; }
-  if (!C->initialized) { swm::set_error("swm_write_restart: state not initialised"); return SWM_ERR_STATE; }
+  if (!C->member_init[C->sel]) { swm::set_error("swm_write_restart: state not initialised"); return SWM_ERR_STATE; }
   char name[64];
   if (!path) {
     swm_status st = swm::restart_name(C->nsecond, name, (int)sizeof name);
@@ -1139,10 +1302,11 @@ long long swm_time_seconds(const swm_ctx* C) { return C ? C->nsecond : -1; }
 // host version in swm_mi355x/ics.py): harmonic sums of degree 1..lmax for
 // vorticity, divergence (10 % RMS) and thickness perturbation, corner velocity
 // from the model's own gradient scaled to max |u| = umax, Gaussian tracer bumps.
-swm_status swm_init_synthetic(swm_ctx* C, int seed_index, int lmax, double umax, double h0, double h_rms,
-                              int nbumps, double bump_radius) {
-  if (!C || lmax < 1 || lmax > 400 || nbumps < 0 || nbumps > 64) { swm::set_error("swm_init_synthetic: bad argument"); return SWM_ERR_ARG; }
+namespace swm {
+swm_status init_synthetic(swm_ctx* C, int seed_index, int lmax, double umax, double h0, double h_rms, int nbumps,
+                          double bump_radius, int pert_index, double pert_rel) {
   HIP_TRY(hipSetDevice(C->device));
+  MemberScope scope(C);
   const int ns = C->ns;
   hipStream_t S = C->stream;
   const unsigned nb = nblk(ns);
@@ -1169,7 +1333,7 @@ swm_status swm_init_synthetic(swm_ctx* C, int seed_index, int lmax, double umax,
   hipLaunchKernelGGL(swm::dev::k_scale, dim3(nb), dim3(BLK), 0, S, chi, (size_t)ns, a2);
   auto sumsq = [&](const double* x, double* out) -> swm_status {
     hipLaunchKernelGGL(swm::dev::k_sumsq_partial, dim3(C->npart), dim3(BLK), 0, S, x, C->mask, ns, C->part);
-    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(1), dim3(swm::dev::SUM_THREADS), 0, S, C->part, C->npart, C->sums);
+    sum_final(C, C->part, C->npart, C->sums, 1);
     HIP_TRY(hipMemcpyAsync(C->host_sums, C->sums, sizeof(double), hipMemcpyDeviceToHost, S));
     HIP_TRY(hipStreamSynchronize(S));
     *out = C->host_sums[0];
@@ -1179,6 +1343,24 @@ swm_status swm_init_synthetic(swm_ctx* C, int seed_index, int lmax, double umax,
   swm_status st;
   if ((st = sumsq(zeta, &sz)) || (st = sumsq(delta, &sd)) || (st = sumsq(hp, &sh))) return st;
   const double nreal = (double)C->max_ig;
+  if (pert_index >= 0 && pert_rel != 0.0) {
+    // a member perturbation: vorticity harmonics of the same degrees from
+    // stream s0 + 1000 + pert_index, scaled to pert_rel times the RMS of the
+    // vorticity, added through the streamfunction
+    std::vector<double> pc(3 * (size_t)nlm, 0.0);
+    SplitMix rp(s0 + 1000 + (uint64_t)pert_index);
+    for (int l = 1; l <= lmax; ++l)
+      for (int m = -l; m <= l; ++m) pc[3 * (size_t)(l * l - 1 + l + m)] = rp.normal();
+    HIP_TRY(hipMemcpyAsync(dcoef, pc.data(), pc.size() * sizeof(double), hipMemcpyHostToDevice, S));
+    double *ppsi = C->d.vf, *pzeta = C->d.vf + ns, *pchi = C->d.vf + 2 * (size_t)ns, *pdel = C->d.dcurl,
+           *php = C->d.dcurl + ns;
+    hipLaunchKernelGGL(swm::dev::k_harmonics, dim3(nb), dim3(BLK), 0, S, ns, C->pxyz, lmax, dcoef, ppsi, pzeta, pchi,
+                       pdel, php);
+    double sp;
+    if ((st = sumsq(pzeta, &sp))) return st;
+    const double c = pert_rel * std::sqrt(sz / sp);
+    hipLaunchKernelGGL(swm::dev::k_axpy, dim3(nb), dim3(BLK), 0, S, (size_t)ns, c * a2, ppsi, psi);
+  }
   const double rz = std::sqrt(sz / nreal), rd = std::sqrt(sd / nreal), rh = std::sqrt(sh / nreal);
   hipLaunchKernelGGL(swm::dev::k_synth_finish, dim3(nb), dim3(BLK), 0, S, C->d.geo, h0, h_rms / rh, hp, C->d.h[0],
                      0.1 * rz / rd, chi);
@@ -1208,29 +1390,56 @@ swm_status swm_init_synthetic(swm_ctx* C, int seed_index, int lmax, double umax,
   HIP_TRY(hipStreamSynchronize(S));
   return finish_init(C);
 }
+}  // namespace swm
+
+swm_status swm_init_synthetic(swm_ctx* C, int seed_index, int lmax, double umax, double h0, double h_rms,
+                              int nbumps, double bump_radius) {
+  if (!C || lmax < 1 || lmax > 400 || nbumps < 0 || nbumps > 64) { swm::set_error("swm_init_synthetic: bad argument"); return SWM_ERR_ARG; }
+  return swm::init_synthetic(C, seed_index, lmax, umax, h0, h_rms, nbumps, bump_radius, -1, 0.0);
+}
+
+swm_status swm_init_synthetic_member(swm_ctx* C, int seed_index, int lmax, double umax, double h0, double h_rms,
+                                     int nbumps, double bump_radius, int member_seed, double vort_rel) {
+  if (!C || lmax < 1 || lmax > 400 || nbumps < 0 || nbumps > 64 || member_seed < 0 || !(vort_rel >= 0.0) ||
+      vort_rel > 1.0) {
+    swm::set_error("swm_init_synthetic_member: bad argument");
+    return SWM_ERR_ARG;
+  }
+  return swm::init_synthetic(C, seed_index, lmax, umax, h0, h_rms, nbumps, bump_radius, member_seed, vort_rel);
+}
+
+namespace swm {
+// one time step of every member, queued on the context's stream (no host
+// synchronisation unless the solver's convergence test needs one)
+swm_status step_once(swm_ctx* C) {
+  C->step_count += 1;
+  C->nsecond = (long long)((double)C->nsecond + C->dt);   // time_manager_module.F:96, truncated
+  pointers(C);
+  weights(C);
+  swm_status st;
+  if (C->precision == SWM_MIXED) {
+    st = step_dt<float>(C, C->m);
+    if (st) return st;
+    wrap_prognostics<float>(C, C->m);
+  } else {
+    st = step_dt<double>(C, C->d);
+    if (st) return st;
+    wrap_prognostics<double>(C, C->d);
+  }
+  return SWM_OK;
+}
+}  // namespace swm
 
 swm_status swm_step(swm_ctx* C, int nsteps) {
   if (!C || nsteps < 0) { swm::set_error("swm_step: bad argument"); return SWM_ERR_ARG; }
-  if (!C->initialized) { swm::set_error("swm_step: state not initialised"); return SWM_ERR_STATE; }
+  if (!C->initialized) { swm::set_error("swm_step: state not initialised (every member)"); return SWM_ERR_STATE; }
   HIP_TRY(hipSetDevice(C->device));
   C->kev_used = 0;
   C->kernel_bytes = 0.0;
   HIP_TRY(hipEventRecord(C->ev_a, C->stream));
   for (int n = 0; n < nsteps; ++n) {
-    C->step_count += 1;
-    C->nsecond = (long long)((double)C->nsecond + C->dt);   // time_manager_module.F:96, truncated
-    pointers(C);
-    weights(C);
-    swm_status st;
-    if (C->precision == SWM_MIXED) {
-      st = step_dt<float>(C, C->m);
-      if (st) return st;
-      wrap_prognostics<float>(C, C->m);
-    } else {
-      st = step_dt<double>(C, C->d);
-      if (st) return st;
-      wrap_prognostics<double>(C, C->d);
-    }
+    swm_status st = swm::step_once(C);
+    if (st) return st;
   }
   HIP_TRY(hipEventRecord(C->ev_b, C->stream));
   HIP_TRY(hipEventSynchronize(C->ev_b));
@@ -1244,6 +1453,7 @@ swm_status swm_step(swm_ctx* C, int nsteps) {
 swm_status swm_get_field(swm_ctx* C, const char* name, double* out) {
   if (!C || !name || !out) { swm::set_error("swm_get_field: bad argument"); return SWM_ERR_ARG; }
   HIP_TRY(hipSetDevice(C->device));
+  swm::MemberScope scope(C);
   const int ns = C->ns;
   const int n4 = C->n4 - 1;
   const std::string s(name);
@@ -1293,6 +1503,38 @@ swm_status swm_get_field(swm_ctx* C, const char* name, double* out) {
   return SWM_ERR_ARG;
 }
 
+int swm_members(const swm_ctx* C) { return C ? C->nmem : 0; }
+
+swm_status swm_select_member(swm_ctx* C, int member) {
+  if (!C || member < 0 || member >= C->nmem) { swm::set_error("swm_select_member: no such member"); return SWM_ERR_ARG; }
+  C->sel = member;
+  return SWM_OK;
+}
+
+swm_status swm_set_fixed_cycles(swm_ctx* C, int ncycles, int nstart) {
+  if (!C || ncycles < 0 || ncycles > 12 || nstart < 0 || nstart > 24) {
+    swm::set_error("swm_set_fixed_cycles: bad argument");
+    return SWM_ERR_ARG;
+  }
+  if (ncycles > 0 && (C->precision != SWM_MIXED || C->solver_relative)) {
+    swm::set_error("swm_set_fixed_cycles: the fixed schedule is a mode of the mixed solve with the absolute criterion");
+    return SWM_ERR_ARG;
+  }
+  C->fixed_cycles = ncycles;
+  C->fixed_start = nstart;
+  C->epoch += 1;
+  return SWM_OK;
+}
+
+swm_status swm_solver_track(swm_ctx* C, double* out) {
+  if (!C || !out) { swm::set_error("swm_solver_track: bad argument"); return SWM_ERR_ARG; }
+  HIP_TRY(hipSetDevice(C->device));
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  swm::MemberScope scope(C);
+  HIP_TRY(hipMemcpy(out, C->track, 3 * sizeof(double), hipMemcpyDeviceToHost));
+  return SWM_OK;
+}
+
 swm_status swm_synchronize(swm_ctx* C) {
   if (!C) return SWM_ERR_ARG;
   HIP_TRY(hipStreamSynchronize(C->stream));
@@ -1307,10 +1549,15 @@ swm_status swm_set_solver(swm_ctx* C, double tol, int relative, int max_cycles) 
     swm::set_error("swm_set_solver: need tol > 0 (or 0 with relative) and 2 <= max_cycles <= 1000");
     return SWM_ERR_ARG;
   }
+  if (relative && C->fixed_cycles > 0) {
+    swm::set_error("swm_set_solver: the fixed schedule (swm_set_fixed_cycles) takes the absolute criterion");
+    return SWM_ERR_ARG;
+  }
   C->solver_tol = tol;
   C->solver_relative = relative ? 1 : 0;
   C->solver_max_cycles = max_cycles;
   C->solver_stall = tol == 0.0 ? 1 : 0;
+  C->epoch += 1;
   return SWM_OK;
 }
 
@@ -1321,6 +1568,7 @@ swm_status swm_set_mixed_start(swm_ctx* C, int extrapolate, int min_cycles) {
   }
   C->mixed_extrapolate = extrapolate ? 1 : 0;
   C->mixed_min_cycles = min_cycles;
+  C->epoch += 1;
   return SWM_OK;
 }
 
@@ -1342,8 +1590,9 @@ swm_status swm_budgets(swm_ctx* C, double* out, int nout) {
     swm::set_error("swm_budgets: out needs 8 + ntracer entries (at most 8 + 16)");
     return SWM_ERR_ARG;
   }
-  if (!C->initialized) { swm::set_error("swm_budgets: state not initialised"); return SWM_ERR_STATE; }
+  if (!C->member_init[C->sel]) { swm::set_error("swm_budgets: state not initialised"); return SWM_ERR_STATE; }
   HIP_TRY(hipSetDevice(C->device));
+  swm::MemberScope scope(C);
   if (C->precision == SWM_MIXED) swm::launch_budgets<float>(C, C->m, nq);
   else swm::launch_budgets<double>(C, C->d, nq);
   std::vector<double> sums(nq);
@@ -1396,13 +1645,14 @@ swm_status swm_timing(const swm_ctx* C, double* step_ms, double* kernel_ms_mean,
 swm_status swm_total_mass(swm_ctx* C, double* mass) {
   if (!C || !mass) return SWM_ERR_ARG;
   HIP_TRY(hipSetDevice(C->device));
+  swm::MemberScope scope(C);
   const int n4 = C->n4 - 1;
   double m[2];
   if (C->precision == SWM_MIXED) {
     // sum of (h' + h0) area = sum h' area + h0 * total area
-    hipLaunchKernelGGL((swm::dev::k_sum_partial<float, double>), dim3(C->npart, 2), dim3(BLK), 0, C->stream,
-                       C->m.h[n4], C->m.h[n4], C->area_mask, C->ns, C->part);
-    hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(2), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part, C->npart, C->sums);
+    hipLaunchKernelGGL((swm::dev::k_sum_partial<float, double>), zgrid(C, C->npart, 2), dim3(BLK), 0, C->stream,
+                       C->m.h[n4], C->m.h[n4], C->area_mask, C->ns, C->part, C->mb);
+    sum_final(C, C->part, C->npart, C->sums, 2);
     swm_status st = read_sums(C, 2, m);
     if (st) return st;
     *mass = m[0] + C->h0 * C->total_area;
@@ -1418,7 +1668,8 @@ swm_status swm_total_mass(swm_ctx* C, double* mass) {
 #define OP_BEGIN(C)                                                              \
   if (!(C)) { swm::set_error("null context"); return SWM_ERR_ARG; }             \
   HIP_TRY(hipSetDevice((C)->device));                                           \
-  HIP_TRY(hipStreamSynchronize((C)->stream));
+  HIP_TRY(hipStreamSynchronize((C)->stream));                                   \
+  swm::MemberScope member_scope_(C);
 #define OP_ARGS(...)                                                             \
   if (!all_set(__VA_ARGS__)) { swm::set_error("null array argument"); return SWM_ERR_ARG; }
 #define OP_BUF(x)                                                                \
@@ -1880,9 +2131,8 @@ void gather_adj(swm_ctx* C, const DevTable& t, std::initializer_list<double*> fi
 void mean_removal_adj(swm_ctx* C, double* y) {
   const int ns = C->ns;
   hipLaunchKernelGGL(swm::dev::k_sum_partial<double>, dim3(C->npart, 1), dim3(BLK), 0, C->stream, y, y,
-                     (const double*)nullptr, ns, C->part);
-  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(1), dim3(swm::dev::SUM_THREADS), 0, C->stream, C->part, C->npart,
-                     C->sums);
+                     (const double*)nullptr, ns, C->part, C->mb);
+  sum_final(C, C->part, C->npart, C->sums, 1);
   hipLaunchKernelGGL(swm::dev::k_sub_am, dim3(nblk(ns)), dim3(BLK), 0, C->stream, ns, C->area_mask, C->sums,
                      1.0 / C->total_area, y);
 }
@@ -2123,6 +2373,10 @@ swm_status state_out(swm_ctx* C, int lev, double* h, double* e, double* d, doubl
 }
 
 swm_status window_check(swm_ctx* C, const char* who) {
+  if (C->nmem != 1) {
+    swm::set_error(std::string(who) + ": the linearised model runs on single-member contexts");
+    return SWM_ERR_ARG;
+  }
   if (C->precision != SWM_FP64 || C->advection != SWM_ADV_CENTERED) {
     swm::set_error(std::string(who) + ": the linearised model is the FP64 step with centred advection");
     return SWM_ERR_ARG;
@@ -2140,7 +2394,7 @@ int last_level(int nsteps) { return (nsteps + 1) % 2 + 1 - 1; }
 extern "C" {
 
 swm_status swm_init_vorticity(swm_ctx* C, const double* zeta, const double* delta, const double* h) {
-  if (!C || !zeta || !delta || !h) { swm::set_error("swm_init_vorticity: bad argument"); return SWM_ERR_ARG; }
+  if (!C || !zeta || !delta || !h || C->nmem != 1) { swm::set_error("swm_init_vorticity: bad argument"); return SWM_ERR_ARG; }
   HIP_TRY(hipSetDevice(C->device));
   swm_status st;
   if ((st = swm::tlad_alloc(C))) return st;
@@ -2309,3 +2563,5 @@ swm_status swm_adj_run(swm_ctx* C, int mode, const double* lh, const double* let
 }
 
 }  // extern "C"
+
+#include "swm_ensemble.hip"

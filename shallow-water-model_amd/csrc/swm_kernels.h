@@ -14,12 +14,18 @@ namespace dev {
 // is subtracted from every stored thickness-like field (h, the low-order
 // mass) and fo (the Coriolis parameter) from the stored vorticity, so that
 // FP32 keeps the small increments of nearly constant fields.
+//
+// Ensembles: the members of a context share the grid, so the metric planes
+// here are shared, while each member's state and scratch form one block, the
+// blocks mb bytes apart.  A launch's z index is the member; the kernels add
+// z * mb to every state pointer they are given (mb = 0 with one member).
 template <class R>
 struct GeoT {
   int iim, jjm, ns;
   const R *area, *area_inv, *cai, *sub_area, *dc, *nrm, *tng, *vw, *sw, *lw, *f, *hs, *mask;
   R h0;
   const R* fo;
+  size_t mb;
 };
 using Geo = GeoT<double>;
 
@@ -31,13 +37,22 @@ struct LevT {
   const R* lw;
   const R* lwt;
   const int* src;
+  size_t mb;   // member stride of the state (see GeoT)
 };
 using Lev = LevT<double>;
 
 template <class R>
 struct FieldSetT {
   R* p[8];
+  size_t mb;   // member stride (see GeoT)
 };
+
+// state pointers of member blockIdx.z: p + z * mb bytes
+template <class... P>
+__device__ __forceinline__ void at_member(size_t mb, P&... p) {
+  const size_t o = (size_t)blockIdx.z * mb;
+  ((p = (P)((char*)(p) + o)), ...);
+}
 using FieldSet = FieldSetT<double>;
 
 }  // namespace dev

@@ -85,6 +85,7 @@ __global__ void k_gather(const int* __restrict__ dst, const int* __restrict__ sr
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   R* x = fs.p[blockIdx.y];
+  at_member(fs.mb, x);
   const int s = src[t];
   x[dst[t]] = s < 0 ? R(0) : x[s];
 }
@@ -95,6 +96,7 @@ __global__ void k_gather(const int* __restrict__ dst, const int* __restrict__ sr
 // only feeds an unused work array (step.F:391-393) and is not formed
 template <class R>
 __global__ void k_avg_corners(GeoT<R> G, const R* __restrict__ sw, const R* __restrict__ x1, R* __restrict__ o1) {
+  at_member(G.mb, x1, o1);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -160,6 +162,7 @@ __device__ __forceinline__ Half<R> half_edges(int iim, int jjm, int ns, int i, i
 template <class R>
 __global__ void k_mass_fluxes(GeoT<R> G, const R* __restrict__ u, const R* __restrict__ h3, const R* __restrict__ hc,
                               R* __restrict__ flx1, R* __restrict__ mshf) {
+  at_member(G.mb, u, h3, hc, flx1, mshf);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -209,6 +212,7 @@ template <class R>
 __global__ void k_mass_low_beta(GeoT<R> G, R dt, R w3, R w2, R w1, const R* __restrict__ flx1, const R* __restrict__ m3,
                                 const R* __restrict__ m2, const R* __restrict__ m1, const R* __restrict__ h3,
                                 R* __restrict__ anti, R* __restrict__ mss_low, R* __restrict__ beta) {
+  at_member(G.mb, flx1, m3, m2, m1, h3, anti, mss_low, beta);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -239,6 +243,7 @@ __global__ void k_mass_low_beta(GeoT<R> G, R dt, R w3, R w2, R w1, const R* __re
 // keep the uncorrected value until the edge halo fill)
 template <class R>
 __global__ void k_mass_anti(GeoT<R> G, const R* __restrict__ beta, R* __restrict__ anti) {
+  at_member(G.mb, beta, anti);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -257,6 +262,7 @@ __global__ void k_mass_anti(GeoT<R> G, const R* __restrict__ beta, R* __restrict
 template <class R>
 __global__ void k_mass_final(GeoT<R> G, R dt, const R* __restrict__ flx1, const R* __restrict__ anti,
                              const R* __restrict__ mss_low, R* __restrict__ mflux, R* __restrict__ h4) {
+  at_member(G.mb, flx1, anti, mss_low, mflux, h4);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -272,6 +278,7 @@ template <class R>
 __global__ void k_tr_fluxes(GeoT<R> G, R w3, R w2, R w1, const R* __restrict__ t3, const R* __restrict__ t4,
                             const R* __restrict__ told, const R* __restrict__ mflux, R* __restrict__ f1,
                             R* __restrict__ f2) {
+  at_member(G.mb, t3, t4, told, mflux, f1, f2);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -295,6 +302,7 @@ __global__ void k_tr_fluxes(GeoT<R> G, R w3, R w2, R w1, const R* __restrict__ t
 template <class R>
 __global__ void k_tr_low(GeoT<R> G, R dt, const R* __restrict__ h3, const R* __restrict__ t3, const R* __restrict__ f1,
                          R* __restrict__ ml) {
+  at_member(G.mb, h3, t3, f1, ml);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -309,6 +317,7 @@ template <class R>
 __global__ void k_tr_limiter(GeoT<R> G, R dt, const R* __restrict__ t3, const R* __restrict__ ml,
                              const R* __restrict__ h4, const R* __restrict__ mss_low, const R* __restrict__ f1,
                              const R* __restrict__ f2, R* __restrict__ anti, R* __restrict__ bdn, R* __restrict__ bup) {
+  at_member(G.mb, t3, ml, h4, mss_low, f1, f2, anti, bdn, bup);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -368,6 +377,7 @@ __global__ void k_tr_limiter(GeoT<R> G, R dt, const R* __restrict__ t3, const R*
 // tracer_fct.F:539-593 set_corrected_trc_anti (in place)
 template <class R>
 __global__ void k_tr_anti(GeoT<R> G, const R* __restrict__ bdn, const R* __restrict__ bup, R* __restrict__ anti) {
+  at_member(G.mb, bdn, bup, anti);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -387,6 +397,7 @@ __global__ void k_tr_anti(GeoT<R> G, const R* __restrict__ bdn, const R* __restr
 template <class R>
 __global__ void k_tr_final(GeoT<R> G, R dt, const R* __restrict__ ml, const R* __restrict__ anti,
                            const R* __restrict__ h4, R* __restrict__ t4, R* __restrict__ told) {
+  at_member(G.mb, ml, anti, h4, t4, told);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -402,6 +413,7 @@ __global__ void k_tr_final(GeoT<R> G, R dt, const R* __restrict__ ml, const R* _
 // of the mass flux, interior slots (0 elsewhere)
 template <class R, bool CENTERED>
 __global__ void k_tr_flux(GeoT<R> G, const R* __restrict__ mflux, const R* __restrict__ t, R* __restrict__ out) {
+  at_member(G.mb, mflux, t, out);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -429,6 +441,7 @@ template <class R>
 __global__ void k_height_update(GeoT<R> G, R dt, R w3, R w2, R w1, const R* __restrict__ mflux,
                                 const R* __restrict__ h3, R* __restrict__ hf_a, const R* __restrict__ hf_b,
                                 const R* __restrict__ hf_c, R* __restrict__ h4) {
+  at_member(G.mb, mflux, h3, hf_a, hf_b, hf_c, h4);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -448,6 +461,7 @@ __global__ void k_tracer_update(GeoT<R> G, R dt, R w3, R w2, R w1, const R* __re
                                 const R* __restrict__ t3, const R* __restrict__ h3, const R* __restrict__ h4,
                                 R* __restrict__ tf_a, const R* __restrict__ tf_b, const R* __restrict__ tf_c,
                                 R* __restrict__ t4) {
+  at_member(G.mb, tflux, t3, h3, h4, tf_a, tf_b, tf_c, t4);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -466,6 +480,7 @@ template <class R>
 __global__ void k_vort_curl(GeoT<R> G, const R* __restrict__ mflux, const R* __restrict__ h3,
                             const R* __restrict__ eta3, const R* __restrict__ hc, const R* __restrict__ u,
                             R* __restrict__ vf, R* __restrict__ dcurl) {
+  at_member(G.mb, mflux, h3, eta3, hc, u, vf, dcurl);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -607,6 +622,7 @@ __global__ void k_tendencies(GeoT<R> G, R dt, R grav, R w3, R w2, R w1, const R*
                              const R* __restrict__ divf_b, const R* __restrict__ divf_c, R* __restrict__ eta4,
                              R* __restrict__ div4, R* __restrict__ relative, R* __restrict__ rhs1,
                              R* __restrict__ rhs2) {
+  at_member(G.mb, vf, dcurl, ke, h3, eta3, div3, etaf_a, etaf_b, etaf_c, divf_a, divf_b, divf_c, eta4, div4, relative, rhs1, rhs2);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -640,7 +656,8 @@ __global__ void k_tendencies(GeoT<R> G, R dt, R grav, R w3, R w2, R w1, const R*
 // accumulated in FP64
 template <class R, class W = R>
 __global__ void k_sum_partial(const R* __restrict__ x0, const R* __restrict__ x1, const W* __restrict__ w, int n,
-                              double* __restrict__ part) {
+                              double* __restrict__ part, size_t mb) {
+  at_member(mb, x0, x1, part);
   __shared__ double sh[256];
   const R* x = blockIdx.y == 0 ? x0 : x1;
   double acc = 0.0;
@@ -657,7 +674,8 @@ __global__ void k_sum_partial(const R* __restrict__ x0, const R* __restrict__ x1
 // per-block partial sums of x^2 * w for field blockIdx.y (x0, x1), FP64
 template <class R>
 __global__ void k_sumsq_mixed(const R* __restrict__ x0, const R* __restrict__ x1, const double* __restrict__ w, int n,
-                              double* __restrict__ part) {
+                              double* __restrict__ part, size_t mb) {
+  at_member(mb, x0, x1, part);
   __shared__ double sh[256];
   const R* x = blockIdx.y == 0 ? x0 : x1;
   double acc = 0.0;
@@ -685,7 +703,8 @@ __device__ __forceinline__ void two_sum_add(double& s, double& c, double v) {
 }
 constexpr int SUM_THREADS = 1024;
 __global__ void __launch_bounds__(SUM_THREADS) k_sum_final(const double* __restrict__ part, int nb,
-                                                           double* __restrict__ out) {
+                                                           double* __restrict__ out, size_t mb) {
+  at_member(mb, part, out);
   __shared__ double ss[SUM_THREADS], cc[SUM_THREADS];
   const int f = blockIdx.x, t = threadIdx.x;
   double s = 0.0, c = 0.0;
@@ -708,7 +727,8 @@ __global__ void __launch_bounds__(SUM_THREADS) k_sum_final(const double* __restr
 // rhs -= mean (elliptic_solver.F:80-92); sums[v] / total gives the mean
 template <class R>
 __global__ void k_sub_mean(int n, const double* __restrict__ sums, double total, R* __restrict__ r1,
-                           R* __restrict__ r2) {
+                           R* __restrict__ r2, size_t mb) {
+  at_member(mb, sums, r1, r2);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const double m1 = sums[0] / total, m2 = sums[1] / total;
@@ -742,6 +762,7 @@ __device__ __forceinline__ void block_sums(const double (&acc)[NV], double* __re
 template <int NV>
 __global__ void __launch_bounds__(256) k_resid2(Geo G, const double* __restrict__ x, const double* __restrict__ rhs,
                                                 double* __restrict__ part) {
+  at_member(G.mb, x, rhs, part);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   double acc[NV];
@@ -775,6 +796,8 @@ __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict_
                                                 double* __restrict__ xn, const float* __restrict__ b,
                                                 float* __restrict__ r, double* __restrict__ part,
                                                 const double* __restrict__ gauge, double inv_area) {
+  at_member(G.mb, x, e, xn, b, r, part);
+  if (gauge) at_member(G.mb, gauge);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   double acc[NV];
@@ -813,6 +836,7 @@ __global__ void __launch_bounds__(256) k_defect_extrap(Geo G, const double* __re
                                                        const double* __restrict__ x, const double* __restrict__ xp,
                                                        double* __restrict__ xn, const float* __restrict__ b,
                                                        float* __restrict__ r, double* __restrict__ part) {
+  at_member(G.mb, x, xp, xn, b, r, part);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   double acc[2 * NV];
@@ -838,8 +862,23 @@ __global__ void __launch_bounds__(256) k_defect_extrap(Geo G, const double* __re
   block_sums<2 * NV>(acc, part);
 }
 
+// fixed-schedule solves: per member (blockIdx.z), the largest mean squared
+// residual of the two fields so far (track[0], track[1]) and the number of
+// solves that ended above tol (track[2]); a NaN counts as above
+__global__ void k_track_resid(const double* __restrict__ sums, double inv_n, double tol, double* __restrict__ track,
+                              size_t mb) {
+  at_member(mb, sums, track);
+  if (threadIdx.x != 0) return;
+  const double a = sums[0] * inv_n, b = sums[1] * inv_n;
+  if (!(a <= track[0])) track[0] = a;
+  if (!(b <= track[1])) track[1] = b;
+  if (!(a <= tol) || !(b <= tol)) track[2] = track[2] + 1.0;
+}
+
 // x += scale * (*g) on n values
-__global__ void k_add_scaled(size_t n, const double* __restrict__ g, double scale, double* __restrict__ x) {
+__global__ void k_add_scaled(size_t n, const double* __restrict__ g, double scale, double* __restrict__ x,
+                             size_t mb) {
+  at_member(mb, g, x);
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) x[t] = x[t] + scale * g[0];
 }
@@ -864,6 +903,7 @@ __global__ void k_correct(size_t n, const float* __restrict__ e, double* __restr
 template <class R, int NV, bool ZERO>
 __global__ void __launch_bounds__(256) k_relax(LevT<R> L, const R* __restrict__ beta, const R* __restrict__ a,
                                                R* __restrict__ out) {
+  at_member(L.mb, beta, a, out);
   const int iim = L.iim, jjm = L.jjm, ns = L.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -924,6 +964,7 @@ __global__ void __launch_bounds__(256) k_relax(LevT<R> L, const R* __restrict__ 
 // interior themselves and are read without the halo map.
 template <class R, int NV>
 __global__ void k_inject(LevT<R> F, LevT<R> C, const R* __restrict__ xa, const R* __restrict__ xb, R* __restrict__ x1) {
+  at_member(F.mb, xa, xb, x1);
   int iim = C.iim, jjm = C.jjm;
   const int s1 = xcd_slot();
   if (s1 >= C.ns) return;
@@ -984,6 +1025,7 @@ __global__ void k_inject(LevT<R> F, LevT<R> C, const R* __restrict__ xa, const R
 // fields.  Coarse reads go through the map only on the coarse halo ring.
 template <class R, int NV>
 __global__ void k_interp(LevT<R> C, LevT<R> F, const R* __restrict__ x1, R* __restrict__ x2) {
+  at_member(F.mb, x1, x2);
   const int iim = F.iim, jjm = F.jjm;
   const int s = xcd_slot();
   if (s >= F.ns) return;
@@ -1016,11 +1058,13 @@ __global__ void k_interp(LevT<R> C, LevT<R> F, const R* __restrict__ x1, R* __re
 }
 
 template <class R>
-__global__ void k_fill(R* __restrict__ x, size_t n, R v) {
+__global__ void k_fill(R* __restrict__ x, size_t n, R v, size_t mb) {
+  at_member(mb, x);
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) x[t] = v;
 }
-__global__ void k_copy(double* __restrict__ y, const double* __restrict__ x, size_t n) {
+__global__ void k_copy(double* __restrict__ y, const double* __restrict__ x, size_t n, size_t mb) {
+  at_member(mb, y, x);
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) y[t] = x[t];
 }
@@ -1033,6 +1077,7 @@ __global__ void k_copy(double* __restrict__ y, const double* __restrict__ x, siz
 template <class R>
 __global__ void k_velocity(GeoT<R> G, const double* __restrict__ psi, const double* __restrict__ chi,
                            R* __restrict__ u) {
+  at_member(G.mb, psi, chi, u);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -1074,6 +1119,7 @@ __global__ void k_velocity(GeoT<R> G, const double* __restrict__ psi, const doub
 // operators.F:1316-1393 with the corner kinetic energy 0.5 |u|^2 (step.F:192-203)
 template <class R>
 __global__ void k_center_ke(GeoT<R> G, const R* __restrict__ u, R* __restrict__ ke) {
+  at_member(G.mb, u, ke);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -1123,6 +1169,7 @@ __global__ void k_curl_div(Geo G, const double* __restrict__ u, double* __restri
 // vorticity is the solve's right-hand side itself); inv_h (step.F:448-452)
 template <class R>
 __global__ void k_post_elliptic(GeoT<R> G, const R* __restrict__ rhs1, R* __restrict__ eta4) {
+  at_member(G.mb, rhs1, eta4);
   const int s = xcd_slot();
   if (s >= G.ns) return;
   if constexpr (sizeof(R) == 4) eta4[s] = rhs1[s];
@@ -1130,6 +1177,7 @@ __global__ void k_post_elliptic(GeoT<R> G, const R* __restrict__ rhs1, R* __rest
 }
 template <class R>
 __global__ void k_inv_h(GeoT<R> G, const R* __restrict__ h4, R* __restrict__ inv_h) {
+  at_member(G.mb, h4, inv_h);
   const int s = xcd_slot();
   if (s >= G.ns) return;
   inv_h[s] = G.mask[s] == R(1) ? R(1) / habs(G, h4[s]) : R(0);
@@ -1379,6 +1427,7 @@ struct CoarseArgs {
   const R* beta;   // right-hand side at level nlev (nvar planes of ns)
   R* alpha;        // solution at level nlev (in: warm start unless zero)
   int zero;        // alpha at level nlev is zero on entry
+  size_t mb;       // member stride of beta / alpha (one workgroup per member, blockIdx.z)
 };
 
 __device__ __forceinline__ int co_ns(int l) { return (2 + (1 << l)) * (2 + (1 << l)) * 10; }
@@ -1532,6 +1581,7 @@ __device__ __forceinline__ LevT<R> co_lev(const CoarseArgs<R>& P, int l) {
 
 template <class R>
 __global__ void __launch_bounds__(CO_THREADS) k_coarse_cycle(CoarseArgs<R> P) {
+  at_member(P.mb, P.beta, P.alpha);
   // LDS: alpha of levels 1..nlev, rhs of levels 1..nlev-1, one scratch level
   __shared__ R lds[2 * (160 + 360 + 1000) + 2 * (160 + 360) + 2 * 1000];
   const int nv = P.nvar, top = P.nlev;

@@ -117,6 +117,30 @@ _sig("swm_adj_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_vo
 _sig("swm_wrap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p])
 _sig("swm_elliptic", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
 _sig("swm_multigrid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
+_sig("swm_create_members", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Config), ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_void_p)])
+_sig("swm_members", ctypes.c_int, [ctypes.c_void_p])
+_sig("swm_select_member", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
+_sig("swm_set_fixed_cycles", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
+_sig("swm_solver_track", ctypes.c_int, [ctypes.c_void_p, _dp])
+_sig("swm_init_synthetic_member", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                 ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                                 ctypes.c_int, ctypes.c_double])
+
+
+class _Choice(ctypes.Structure):
+    _fields_ = [("concurrent", ctypes.c_int), ("steps_per_graph", ctypes.c_int), ("blk_step", ctypes.c_int),
+                ("blk_mg", ctypes.c_int), ("ms_per_step", ctypes.c_double)]
+
+
+_sig("swm_ensemble_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), _c_int_p, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_void_p)])
+_sig("swm_ensemble_destroy", None, [ctypes.c_void_p])
+_sig("swm_ensemble_set", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Choice)])
+_sig("swm_ensemble_get", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Choice)])
+_sig("swm_ensemble_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
+_sig("swm_ensemble_tune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(_Choice), ctypes.c_int,
+                                         _c_int_p])
 
 # reference operators (operators.F): name -> (integer arguments after ctx,
 # array arguments as (name, leading shape), output (name, leading shape));
@@ -286,17 +310,94 @@ class Grid:
             pass
 
 
+class Ensemble:
+    """Groups of members (Models made with members=n, mixed precision with a
+    fixed schedule) advanced together in captured HIP graphs; group k takes
+    substeps[k] of its own steps per ensemble step."""
+
+    def __init__(self, groups, substeps):
+        self.groups = list(groups)
+        arr = (ctypes.c_void_p * len(self.groups))(*[g._h.value for g in self.groups])
+        sub = (ctypes.c_int * len(self.groups))(*substeps)
+        h = ctypes.c_void_p()
+        _check(lib.swm_ensemble_create(arr, sub, len(self.groups), ctypes.byref(h)))
+        self._h = h
+
+    def step(self, nsteps=1):
+        _check(lib.swm_ensemble_step(self._h, nsteps))
+
+    def choice(self):
+        """Launch choice and the last measured ms per ensemble step."""
+        c = _Choice()
+        _check(lib.swm_ensemble_get(self._h, ctypes.byref(c)))
+        return {k: getattr(c, k) for k, _ in _Choice._fields_}
+
+    def set_choice(self, concurrent=0, steps_per_graph=1, blk_step=256, blk_mg=256):
+        c = _Choice(concurrent, steps_per_graph, blk_step, blk_mg, 0.0)
+        _check(lib.swm_ensemble_set(self._h, ctypes.byref(c)))
+
+    def tune(self, rounds=2, max_trials=32):
+        """Autotune the launch choice; returns the trials (dicts with
+        ms_per_step)."""
+        tr = (_Choice * max_trials)()
+        n = ctypes.c_int()
+        _check(lib.swm_ensemble_tune(self._h, rounds, tr, max_trials, ctypes.byref(n)))
+        return [{k: getattr(t, k) for k, _ in _Choice._fields_} for t in tr[:min(n.value, max_trials)]]
+
+    def close(self):
+        if self._h:
+            lib.swm_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Model:
     """Device-resident model state on one GPU (the reference's prognostics
     module plus the saved state of step.F and the elliptic warm start)."""
 
-    def __init__(self, grid, ntracer=1, dt=450.0, grav=REF_GRAV, precision=SWM_FP64, device=0, advection="fct"):
+    def __init__(self, grid, ntracer=1, dt=450.0, grav=REF_GRAV, precision=SWM_FP64, device=0, advection="fct",
+                 members=1):
         cfg = _Config(ntracer, dt, grav, precision, device, ADVECTION[advection])
         h = ctypes.c_void_p()
-        _check(lib.swm_create(grid.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        if members == 1:
+            _check(lib.swm_create(grid.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        else:
+            _check(lib.swm_create_members(grid.handle, ctypes.byref(cfg), members, ctypes.byref(h)))
         self._h = h
         self.grid, self.ntracer, self.dt = grid, ntracer, dt
         self.iim, self.jjm = grid.iim, grid.jjm
+        self.members = members
+
+    def select(self, member):
+        """Member addressed by the host-side calls (init, field, budgets,
+        restart files, operators) from now on."""
+        _check(lib.swm_select_member(self._h, member))
+        return self
+
+    def set_fixed_cycles(self, ncycles, nstart=8):
+        """Mixed solve: exactly ncycles correction cycles per solve (the first
+        two after an initialisation: max(ncycles, nstart)), no host test (0:
+        the convergence test)."""
+        _check(lib.swm_set_fixed_cycles(self._h, ncycles, nstart))
+
+    def solver_track(self):
+        """(max mean r^2 of psi, of chi, solves above the criterion) of the
+        selected member under the fixed schedule."""
+        out = (ctypes.c_double * 3)()
+        _check(lib.swm_solver_track(self._h, out))
+        return out[0], out[1], int(out[2])
+
+    def init_synthetic_member(self, member_seed, vort_rel=0.01, seed_index=0, lmax=20, umax=40.0, h0=5000.0,
+                              h_rms=50.0, nbumps=4, bump_radius=500.0e3):
+        """Seeded synthetic state with a member perturbation of the vorticity
+        (swm_init_synthetic_member), for the selected member."""
+        _check(lib.swm_init_synthetic_member(self._h, seed_index, lmax, umax, h0, h_rms, nbumps, bump_radius,
+                                             member_seed, vort_rel))
 
     def init_state(self, h, u, tracer, hs=None):
         s = (self.iim, self.jjm, 10)
