@@ -118,7 +118,9 @@ struct swm_ctx {
   // choice for ensembles)
   int blk_step = 256, blk_mg = 256;
   long long epoch = 0;             // changes whenever captured launches would differ (initialisation, settings)
-  double* track = nullptr;         // per member: max mean(r^2) of the two fields, count of solves above the criterion
+  double* track = nullptr;
+  double* rmean = nullptr;         // mixed path: sums of area * rhs of the two fields (mean removal)
+  double* part_rhs = nullptr;      // mixed path: their per-block partial sums, from the tendencies kernel         // per member: max mean(r^2) of the two fields, count of solves above the criterion
   // FP64 metrics (both paths: set-up, FP64 residual, sums)
   double *area, *area_inv, *cai, *sub_area, *dc, *nrm, *tng, *vw, *sw, *lw, *f, *hs, *mask, *area_mask, *pxyz;
   double total_area = 0.0, radius = 0.0;
@@ -476,7 +478,7 @@ swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
 // residual.  The iterates move between three FP64 buffers (xout, xalt,
 // xprev), which are renamed at the end so that xout holds the solution and
 // xprev the one before.
-swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
+swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, const double** bsum_out) {
   const int ns = C->ns;
   View<float>& V = C->m;
   const swm::dev::Geo& G64 = C->d.geo;
@@ -484,19 +486,24 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
   float* e = C->mx_e;
   const unsigned nb = nblk(ns);
   const size_t mb = C->mb;
-  gather<double>(C, C->face, {xout, xout + ns});
-  gather<float>(C, C->face, {rhs, rhs + ns});
-  // area-weighted means in FP64 (elliptic_solver.F:80-92)
-  hipLaunchKernelGGL((swm::dev::k_sum_partial<float, double>), zgrid(C, C->npart, 2), dim3(BLK), 0, C->stream, rhs,
-                     rhs + ns, C->area_mask, ns, C->part, mb);
-  sum_final(C, C->part, C->npart, C->sums, 2);
-  hipLaunchKernelGGL(swm::dev::k_sub_mean<float>, zgrid(C, nblk(ns)), dim3(BLK), 0, C->stream, ns, C->sums,
-                     C->total_area, rhs, rhs + ns, mb);
+  // The iterate's halos are consistent (every defect pass stores the whole
+  // iterate, halos included) and the solve reads the right-hand side at real
+  // cells only, so neither is filled here.  The area-weighted means
+  // (elliptic_solver.F:80-92): FP64 sums of the tendencies kernel's block
+  // partials; the consumers take the mean off on the fly.  The relative
+  // criterion needs the mean-free right-hand side in memory for its norms.
   const bool fixed = C->fixed_cycles > 0;
-  if (!fixed) {
+  sum_final(C, C->part_rhs, nb_rhs, C->rmean, 2);
+  const double* bsum = C->rmean;
+  if (C->solver_relative && !fixed) {
+    hipLaunchKernelGGL(swm::dev::k_sub_mean<float>, zgrid(C, nblk(ns)), dim3(BLK), 0, C->stream, ns, C->rmean,
+                       C->total_area, rhs, rhs + ns, mb);
+    bsum = nullptr;
     swm_status st0 = rhs_mean_squares<float>(C, rhs, 2);
     if (st0) return st0;
   }
+  *bsum_out = bsum;
+  const double total = C->total_area;
   // the extrapolated start goes to xalt; then the iterates alternate between
   // xalt and xprev, whose content is no longer needed once it is read
   double* blk[3] = {xout, C->xalt, C->xprev};
@@ -510,10 +517,10 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
     const double* gg = shift[v0] ? C->gauge + v0 : nullptr;
     if (nv == 2)
       hipLaunchKernelGGL((swm::dev::k_defect<2, true>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, cur[0], e, oth[0],
-                         rhs, r, C->part_slots, gg, inv_area);
+                         rhs, r, C->part_slots, gg, inv_area, bsum, total);
     else
       hipLaunchKernelGGL((swm::dev::k_defect<1, true>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, cur[v0], e + o,
-                         oth[v0], rhs + o, r + o, C->part_slots, gg, inv_area);
+                         oth[v0], rhs + o, r + o, C->part_slots, gg, inv_area, bsum ? bsum + v0 : nullptr, total);
     for (int v = v0; v < v0 + nv; ++v) {
       std::swap(cur[v], oth[v]);
       shift[v] = false;
@@ -529,7 +536,7 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout) {
   };
   const bool extrap = C->nsol >= 2 && C->mixed_extrapolate;
   hipLaunchKernelGGL(swm::dev::k_defect_extrap<2>, zgrid(C, nb), dim3(BLK), 0, C->stream, G64, C->area_mask, blk[0],
-                     extrap ? blk[2] : blk[0], blk[1], rhs, r, C->part_slots);
+                     extrap ? blk[2] : blk[0], blk[1], rhs, r, C->part_slots, bsum, total);
   if (extrap) {
     sum_final(C, C->part_slots + 2 * (size_t)nb, (int)nb, C->gauge, 2);
     shift[0] = shift[1] = true;
@@ -617,8 +624,8 @@ template <class R>
 void wrap_prognostics(swm_ctx* C, View<R>& V) {   // step.F:438-493
   const int n4 = C->n4 - 1;
   const int ns = C->ns;
-  hipLaunchKernelGGL(swm::dev::k_inv_h<R>, zgrid(C, nblk_b(ns, C->blk_step)), dim3(C->blk_step), 0, C->stream, V.geo, V.h[n4], V.inv_h);
-  std::vector<R*> f = {V.h[n4], V.eta[n4], V.div[n4], V.relative, V.ke, V.inv_h};
+  // inv_h (step.F:448-452) is not formed: nothing on the path reads it
+  std::vector<R*> f = {V.h[n4], V.eta[n4], V.div[n4], V.relative, V.ke};
   for (int q = 0; q < C->nt; ++q) f.push_back(V.tr[n4][q]);
   gather_list<R>(C, C->face, f);
   gather<R>(C, C->vrtx, {V.u[0], V.u[0] + 2 * (size_t)ns, V.u[1], V.u[1] + 2 * (size_t)ns});
@@ -697,17 +704,29 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   hipLaunchKernelGGL(swm::dev::k_vort_curl<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.mass_flux, V.h[n3], V.eta[n3], V.hc,
                      V.u[n3], V.vf, V.dcurl);
   gather<R>(C, C->edge, {V.vf, V.dcurl});
-  hipLaunchKernelGGL(swm::dev::k_tendencies<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, grav, w3, w2, w1, V.vf, V.dcurl,
-                     V.ke, V.h[n3], V.eta[n3], V.div[n3], V.eta_f[C->n3f - 1], V.eta_f[C->n2f - 1],
+  // mixed path: the kernel also forms the block sums of the right-hand
+  // sides' area-weighted means, which the solve takes off on the fly
+  constexpr bool mixed = sizeof(R) == 4;
+  hipLaunchKernelGGL((swm::dev::k_tendencies<R, mixed>), zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, grav, w3, w2, w1,
+                     V.vf, V.dcurl, V.ke, V.h[n3], V.eta[n3], V.div[n3], V.eta_f[C->n3f - 1], V.eta_f[C->n2f - 1],
                      V.eta_f[C->n1f - 1], V.div_f[C->n3f - 1], V.div_f[C->n2f - 1], V.div_f[C->n1f - 1], V.eta[n4],
-                     V.div[n4], V.relative, V.rhs, V.rhs + ns);
+                     V.div[n4], V.relative, V.rhs, V.rhs + ns, mixed ? C->area_mask : nullptr,
+                     mixed ? C->part_rhs : nullptr);
   // streamfunction and velocity potential (step.F:170-184)
   swm_status st;
+  const double* bsum = nullptr;
   if constexpr (sizeof(R) == 8) st = elliptic(C, V.rhs, C->xout, 2);
-  else st = elliptic_mixed(C, V.rhs, C->xout);
+  else st = elliptic_mixed(C, V.rhs, C->xout, (int)nb, &bsum);
   if (st) return st;
-  hipLaunchKernelGGL(swm::dev::k_post_elliptic<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.rhs, V.eta[n4]);
-  hipLaunchKernelGGL(swm::dev::k_velocity<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, C->xout, C->xout + ns, V.u[n4]);
+  if constexpr (mixed) {
+    // the new vorticity (the right-hand side less its mean) with the velocity
+    hipLaunchKernelGGL(swm::dev::k_velocity<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, C->xout, C->xout + ns,
+                       V.u[n4], V.rhs, bsum, C->total_area, V.eta[n4]);
+  } else {
+    hipLaunchKernelGGL(swm::dev::k_post_elliptic<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.rhs, V.eta[n4]);
+    hipLaunchKernelGGL(swm::dev::k_velocity<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, C->xout, C->xout + ns,
+                       V.u[n4], (const R*)nullptr, (const double*)nullptr, 0.0, (R*)nullptr);
+  }
   gather<R>(C, C->vrtx, {V.u[n4], V.u[n4] + 2 * (size_t)ns});
   hipLaunchKernelGGL(swm::dev::k_center_ke<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.u[n4], V.ke);
   gather<R>(C, C->face, {V.ke});
@@ -976,6 +995,8 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
   C->part = mt(2 * (size_t)C->npart); C->sums = mt(64);
   C->part_budget = mt((8 + swm::dev::BUDGET_MAXTR) * (size_t)C->npart);
   C->track = mt(4);
+  C->rmean = mt(4);
+  C->part_rhs = mt(2 * (size_t)nblk_b(ns, 64));
   layout_view<double>(C, C->d, nullptr);
   C->d.geo = swm::dev::Geo{C->iim, C->jjm, C->ns, C->area, C->area_inv, C->cai, C->sub_area, C->dc, C->nrm, C->tng,
                            C->vw, C->sw, C->lw, C->f, C->hs, C->mask, 0.0, nullptr};
@@ -1021,7 +1042,7 @@ void shift_members(swm_ctx* C, ptrdiff_t d) {
   shift_view(C->d, d);
   if (C->precision == SWM_MIXED) shift_view(C->m, d);
   for (double** p : {&C->xout, &C->r2, &C->xalt, &C->xprev, &C->gauge, &C->part_slots, &C->part, &C->sums,
-                     &C->part_budget, &C->track})
+                     &C->part_budget, &C->track, &C->rmean, &C->part_rhs})
     shift_ptr(*p, d);
   shift_ptr(C->mx_r, d);
   shift_ptr(C->mx_e, d);
@@ -1364,7 +1385,7 @@ swm_status init_synthetic(swm_ctx* C, int seed_index, int lmax, double umax, dou
   const double rz = std::sqrt(sz / nreal), rd = std::sqrt(sd / nreal), rh = std::sqrt(sh / nreal);
   hipLaunchKernelGGL(swm::dev::k_synth_finish, dim3(nb), dim3(BLK), 0, S, C->d.geo, h0, h_rms / rh, hp, C->d.h[0],
                      0.1 * rz / rd, chi);
-  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, C->d.geo, psi, chi, C->d.u[0]);
+  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, C->d.geo, psi, chi, C->d.u[0], (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr);
   hipLaunchKernelGGL(swm::dev::k_speedmax_partial, dim3(C->npart), dim3(BLK), 0, S, C->d.geo, C->d.u[0], C->part);
   std::vector<double> pm(C->npart);
   HIP_TRY(hipMemcpyAsync(pm.data(), C->part, C->npart * sizeof(double), hipMemcpyDeviceToHost, S));
@@ -1805,7 +1826,7 @@ swm_status swm_op_calculate_velocity(swm_ctx* C, int npad, int nx, int ny, int n
   G.vw = io.put(v_weights, 12, 12);
   double *p = io.put(psi, 1, 1), *c = io.put(chi, 1, 1), *y = io.dev(4);
   OP_BUF(G.cai && G.vw && p && c && y);
-  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb_), dim3(BLK), 0, C->stream, G, p, c, y);
+  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb_), dim3(BLK), 0, C->stream, G, p, c, y, (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr);
   return io.get(xout, y, 4, kCornerVec);
 }
 
@@ -2164,11 +2185,11 @@ swm_status tl_step(swm_ctx* C, int q) {
   gather<double>(C, C->edge, {T->vf, T->dc});
   hipLaunchKernelGGL(swm::dev::k_tendencies<double>, dim3(nb), dim3(BLK), 0, S, T->lin, dt, C->grav, w3, w2, w1, T->vf,
                      T->dc, T->ke, dh3, de3, T->div[n3], T->ef[a], T->ef[b], T->ef[c], T->df[a], T->df[b], T->df[c],
-                     T->eta[n4], T->div[n4], T->X, T->rhs, T->rhs + ns);
+                     T->eta[n4], T->div[n4], T->X, T->rhs, T->rhs + ns, (const double*)nullptr, (double*)nullptr);
   swm_status st = lin_solve(C, T->rhs, T->xs);
   if (st) return st;
   hipLaunchKernelGGL(swm::dev::k_post_elliptic<double>, dim3(nb), dim3(BLK), 0, S, T->lin, T->rhs, T->eta[n4]);
-  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, G, T->xs, T->xs + ns, T->u[n4]);
+  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, G, T->xs, T->xs + ns, T->u[n4], (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr);
   gather<double>(C, C->vrtx, {T->u[n4], T->u[n4] + 2 * ns});
   // wrap_prognostics
   gather<double>(C, C->face, {T->h[n4], T->eta[n4], T->div[n4]});
@@ -2267,7 +2288,7 @@ swm_status init_tl(swm_ctx* C, const double* dz, const double* dd, const double*
   swm_status st = lin_solve(C, T->rhs, T->xs);
   if (st) return st;
   clear_state(C);
-  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, G, T->xs, T->xs + ns, T->u[0]);
+  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, G, T->xs, T->xs + ns, T->u[0], (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr);
   HIP_TRY(hipMemcpyAsync(T->h[0], dh, ns * sizeof(double), hipMemcpyDeviceToDevice, S));
   // finish_init: both levels, wrapped; curl / div of level 0 (wrapped again)
   HIP_TRY(hipMemcpyAsync(T->h[1], T->h[0], ns * sizeof(double), hipMemcpyDeviceToDevice, S));
@@ -2340,7 +2361,7 @@ swm_status init_from_vorticity(swm_ctx* C, const double* z, const double* d, con
   swm_status st = lin_solve(C, T->rhs, T->xs);
   if (st) return st;
   hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nblk(ns)), dim3(BLK), 0, S, C->d.geo, T->xs, T->xs + ns,
-                     C->d.u[0]);
+                     C->d.u[0], (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr);
   HIP_TRY(hipMemcpyAsync(C->d.h[0], h, ns * sizeof(double), hipMemcpyDeviceToDevice, S));
   // the model's first solve starts from this streamfunction / velocity potential
   HIP_TRY(hipMemcpyAsync(C->xout, T->xs, 2 * ns * sizeof(double), hipMemcpyDeviceToDevice, S));

@@ -613,19 +613,19 @@ __device__ __forceinline__ R lap_diff_at(int iim, int jjm, int ns, const IJK& c,
 // right-hand sides of step.F:170-173.  Mixed path: eta is stored as the
 // relative vorticity (eta - f), which is then also the right-hand side, and
 // the Laplacian of ke + g (h + hs) is formed on differences (h stored less
-// h0, a constant the Laplacian does not see).
+// h0, a constant the Laplacian does not see).  One slot; k_tendencies runs
+// it over the grid.
 template <class R>
-__global__ void k_tendencies(GeoT<R> G, R dt, R grav, R w3, R w2, R w1, const R* __restrict__ vf,
-                             const R* __restrict__ dcurl, const R* __restrict__ ke, const R* __restrict__ h3,
-                             const R* __restrict__ eta3, const R* __restrict__ div3, R* __restrict__ etaf_a,
-                             const R* __restrict__ etaf_b, const R* __restrict__ etaf_c, R* __restrict__ divf_a,
-                             const R* __restrict__ divf_b, const R* __restrict__ divf_c, R* __restrict__ eta4,
-                             R* __restrict__ div4, R* __restrict__ relative, R* __restrict__ rhs1,
-                             R* __restrict__ rhs2) {
-  at_member(G.mb, vf, dcurl, ke, h3, eta3, div3, etaf_a, etaf_b, etaf_c, divf_a, divf_b, divf_c, eta4, div4, relative, rhs1, rhs2);
+__device__ __forceinline__ void tendencies_at(const GeoT<R>& G, int s, R dt, R grav, R w3, R w2, R w1,
+                                              const R* __restrict__ vf, const R* __restrict__ dcurl,
+                                              const R* __restrict__ ke, const R* __restrict__ h3,
+                                              const R* __restrict__ eta3, const R* __restrict__ div3,
+                                              R* __restrict__ etaf_a, const R* __restrict__ etaf_b,
+                                              const R* __restrict__ etaf_c, R* __restrict__ divf_a,
+                                              const R* __restrict__ divf_b, const R* __restrict__ divf_c,
+                                              R* __restrict__ eta4, R* __restrict__ div4, R* __restrict__ relative,
+                                              R& rhs1, R& rhs2) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = xcd_slot();
-  if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const R ai = G.area_inv[s];
   const R wv = div_edge_at<R>(iim, jjm, ns, c, vf, ai);
@@ -647,8 +647,8 @@ __global__ void k_tendencies(GeoT<R> G, R dt, R grav, R w3, R w2, R w1, const R*
   if constexpr (sizeof(R) == 4) rel = e4;
   else rel = e4 - G.f[s];
   relative[s] = rel;
-  rhs1[s] = rel;
-  rhs2[s] = d4;
+  rhs1 = rel;
+  rhs2 = d4;
 }
 
 // ------------------------------------------------------- reductions -------
@@ -756,6 +756,44 @@ __device__ __forceinline__ void block_sums(const double (&acc)[NV], double* __re
   }
 }
 
+// k_tendencies over all slots; SUMS (mixed path): also the per-block sums of
+// am * rhs of both fields, which the solve's mean removal takes
+template <class R, bool SUMS = false>
+__global__ void k_tendencies(GeoT<R> G, R dt, R grav, R w3, R w2, R w1, const R* __restrict__ vf,
+                             const R* __restrict__ dcurl, const R* __restrict__ ke, const R* __restrict__ h3,
+                             const R* __restrict__ eta3, const R* __restrict__ div3, R* __restrict__ etaf_a,
+                             const R* __restrict__ etaf_b, const R* __restrict__ etaf_c, R* __restrict__ divf_a,
+                             const R* __restrict__ divf_b, const R* __restrict__ divf_c, R* __restrict__ eta4,
+                             R* __restrict__ div4, R* __restrict__ relative, R* __restrict__ rhs1,
+                             R* __restrict__ rhs2, const double* __restrict__ am, double* __restrict__ part) {
+  at_member(G.mb, vf, dcurl, ke, h3, eta3, div3, etaf_a, etaf_b, etaf_c, divf_a, divf_b, divf_c, eta4, div4, relative, rhs1, rhs2);
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if constexpr (SUMS) {
+    // with the per-block sums of am * rhs (the solve's mean removal) every
+    // thread reaches the block reduction
+    at_member(G.mb, part);
+    double acc[2] = {0.0, 0.0};
+    if (s < ns) {
+      R r1, r2;
+      tendencies_at<R>(G, s, dt, grav, w3, w2, w1, vf, dcurl, ke, h3, eta3, div3, etaf_a, etaf_b, etaf_c, divf_a,
+                       divf_b, divf_c, eta4, div4, relative, r1, r2);
+      rhs1[s] = r1;
+      rhs2[s] = r2;
+      acc[0] = (double)r1 * am[s];
+      acc[1] = (double)r2 * am[s];
+    }
+    block_sums<2>(acc, part);
+    return;
+  }
+  if (s >= ns) return;
+  R r1, r2;
+  tendencies_at<R>(G, s, dt, grav, w3, w2, w1, vf, dcurl, ke, h3, eta3, div3, etaf_a, etaf_b, etaf_c, divf_a, divf_b,
+                   divf_c, eta4, div4, relative, r1, r2);
+  rhs1[s] = r1;
+  rhs2[s] = r2;
+}
+
 // FP64 path: squared Poisson residual at real cells of NV fields (planes ns
 // apart), one slot per thread and the weights read once, per-block sums
 // (elliptic_solver.F:203-239 check_convergence, :341-358 l2_norm)
@@ -784,6 +822,13 @@ __global__ void __launch_bounds__(256) k_resid2(Geo G, const double* __restrict_
   block_sums<NV>(acc, part);
 }
 
+// the mixed solve's FP32 right-hand side of field v less its area-weighted
+// mean (elliptic_solver.F:80-92), bsum[v] / total, as k_sub_mean would store
+// it; bsum == nullptr: b already has its mean removed
+__device__ __forceinline__ float rhs_less_mean(float b, const double* __restrict__ bsum, int v, double total) {
+  return bsum ? b - (float)(bsum[v] / total) : b;
+}
+
 // Mixed path, FP64 defect of the Poisson problem for NV fields, one slot per
 // thread, the FP64 weights read once: d = rhs - L(x') at real cells (0
 // elsewhere), x' = x + e with CORR (the FP32 correction e, halos filled;
@@ -795,9 +840,11 @@ template <int NV, bool CORR>
 __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict__ x, const float* __restrict__ e,
                                                 double* __restrict__ xn, const float* __restrict__ b,
                                                 float* __restrict__ r, double* __restrict__ part,
-                                                const double* __restrict__ gauge, double inv_area) {
+                                                const double* __restrict__ gauge, double inv_area,
+                                                const double* __restrict__ bsum, double total) {
   at_member(G.mb, x, e, xn, b, r, part);
   if (gauge) at_member(G.mb, gauge);
+  if (bsum) at_member(G.mb, bsum);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   double acc[NV];
@@ -813,7 +860,7 @@ __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict_
       const float* ev = e + (size_t)v * ns;
       auto X = [&](int sl) { return CORR ? xv[sl] + (double)ev[sl] : xv[sl]; };
       const double w = st.apply(X);
-      const double d = m != 0.0 ? (double)b[(size_t)v * ns + s] - w : 0.0;
+      const double d = m != 0.0 ? (double)rhs_less_mean(b[(size_t)v * ns + s], bsum, v, total) - w : 0.0;
       if (CORR) xn[(size_t)v * ns + s] = gauge ? X(s) - gauge[v] * inv_area : X(s);
       r[(size_t)v * ns + s] = (float)d;
       acc[v] = (d * d) * m;
@@ -835,8 +882,10 @@ template <int NV>
 __global__ void __launch_bounds__(256) k_defect_extrap(Geo G, const double* __restrict__ am,
                                                        const double* __restrict__ x, const double* __restrict__ xp,
                                                        double* __restrict__ xn, const float* __restrict__ b,
-                                                       float* __restrict__ r, double* __restrict__ part) {
+                                                       float* __restrict__ r, double* __restrict__ part,
+                                                       const double* __restrict__ bsum, double total) {
   at_member(G.mb, x, xp, xn, b, r, part);
+  if (bsum) at_member(G.mb, bsum);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   double acc[2 * NV];
@@ -852,7 +901,7 @@ __global__ void __launch_bounds__(256) k_defect_extrap(Geo G, const double* __re
       const double* pv = xp + (size_t)v * ns;
       auto X = [&](int sl) { return 2.0 * xv[sl] - pv[sl]; };
       const double w = st.apply(X);
-      const double d = m != 0.0 ? (double)b[(size_t)v * ns + s] - w : 0.0;
+      const double d = m != 0.0 ? (double)rhs_less_mean(b[(size_t)v * ns + s], bsum, v, total) - w : 0.0;
       xn[(size_t)v * ns + s] = X(s);
       r[(size_t)v * ns + s] = (float)d;
       acc[v] = (d * d) * m;
@@ -1076,11 +1125,19 @@ __global__ void k_copy(double* __restrict__ y, const double* __restrict__ x, siz
 // zero), in FP64 from the FP32 weights.
 template <class R>
 __global__ void k_velocity(GeoT<R> G, const double* __restrict__ psi, const double* __restrict__ chi,
-                           R* __restrict__ u) {
+                           R* __restrict__ u, const R* __restrict__ rhs1, const double* __restrict__ bsum,
+                           double total, R* __restrict__ eta4) {
   at_member(G.mb, psi, chi, u);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
+  if (eta4) {
+    // mixed path: the new level's vorticity is the solve's right-hand side
+    // less its mean (k_post_elliptic fused here)
+    at_member(G.mb, rhs1, eta4);
+    if (bsum) at_member(G.mb, bsum);
+    if constexpr (sizeof(R) == 4) eta4[s] = rhs_less_mean(rhs1[s], bsum, 0, total);
+  }
   const IJK c = ijk_of(s, iim, jjm);
   if (!interior(c, iim, jjm)) {
     for (int p = 0; p < 4; ++p) u[p * ns + s] = R(0);
@@ -1581,7 +1638,11 @@ __device__ __forceinline__ LevT<R> co_lev(const CoarseArgs<R>& P, int l) {
 
 template <class R>
 __global__ void __launch_bounds__(CO_THREADS) k_coarse_cycle(CoarseArgs<R> P) {
-  at_member(P.mb, P.beta, P.alpha);
+  // the member's right-hand side and solution (local copies: writing to the
+  // argument bundle itself would move it to scratch memory)
+  const R* p_beta = P.beta;
+  R* p_alpha = P.alpha;
+  at_member(P.mb, p_beta, p_alpha);
   // LDS: alpha of levels 1..nlev, rhs of levels 1..nlev-1, one scratch level
   __shared__ R lds[2 * (160 + 360 + 1000) + 2 * (160 + 360) + 2 * 1000];
   const int nv = P.nvar, top = P.nlev;
@@ -1600,22 +1661,22 @@ __global__ void __launch_bounds__(CO_THREADS) k_coarse_cycle(CoarseArgs<R> P) {
     const int nz = nv * cum(top - 1), n = nv * co_ns(top);
     for (int t = threadIdx.x; t < nz; t += CO_THREADS) lds[t] = R(0);
     R* at = AL(top);
-    for (int t = threadIdx.x; t < n; t += CO_THREADS) at[t] = P.zero ? R(0) : P.alpha[t];
+    for (int t = threadIdx.x; t < n; t += CO_THREADS) at[t] = P.zero ? R(0) : p_alpha[t];
   }
   __syncthreads();
   for (int l = top; l >= 2; --l) {
-    const R* b = l == top ? P.beta : BE(l);
+    const R* b = l == top ? p_beta : BE(l);
     co_relax(co_lev(P, l), nv, 2, AL(l), tmp, b);
     co_inject(co_lev(P, l), co_lev(P, l - 1), nv, AL(l), b, BE(l - 1));
   }
-  co_relax(co_lev(P, 1), nv, 24, AL(1), tmp, top == 1 ? P.beta : BE(1));
+  co_relax(co_lev(P, 1), nv, 24, AL(1), tmp, top == 1 ? p_beta : BE(1));
   for (int l = 1; l <= top - 1; ++l) {
     co_interp(co_lev(P, l), co_lev(P, l + 1), nv, AL(l), AL(l + 1));
-    co_relax(co_lev(P, l + 1), nv, 2, AL(l + 1), tmp, l + 1 == top ? P.beta : BE(l + 1));
+    co_relax(co_lev(P, l + 1), nv, 2, AL(l + 1), tmp, l + 1 == top ? p_beta : BE(l + 1));
   }
   const int n = nv * co_ns(top);
   const R* at = AL(top);
-  for (int t = threadIdx.x; t < n; t += CO_THREADS) P.alpha[t] = at[t];
+  for (int t = threadIdx.x; t < n; t += CO_THREADS) p_alpha[t] = at[t];
 }
 
 }  // namespace dev
