@@ -244,6 +244,21 @@ swm_status swm_set_solver(swm_ctx* ctx, double tol, int relative, int max_cycles
  * counterpart in the reference. */
 swm_status swm_set_mixed_start(swm_ctx* ctx, int extrapolate, int min_cycles);
 
+/* Mixed path: smoothing sweeps before and after the coarse correction on the
+ * finest level of the FP32 V-cycles (1 to 4; 2 is the reference's count).
+ * The V-cycle only preconditions the FP64 defect correction, so the solution
+ * meets the same criterion with any count; fewer sweeps make a cycle cheaper
+ * and its residual reduction smaller.  No counterpart in the reference. */
+swm_status swm_set_mixed_smoother(swm_ctx* ctx, int fine_sweeps);
+
+/* Stream overlap in the FCT step (default off): the tracer transport, which
+ * reads only the new mass and the mass flux, runs on a second stream of the
+ * context beside the vorticity, divergence and elliptic solve, and the step
+ * waits for it before the final halo fill.  The arithmetic is the same
+ * either way; at level 9 with one tracer the FP64 step gains about 1 % and
+ * the mixed step nothing, as both streams compete for HBM bandwidth. */
+swm_status swm_set_overlap(swm_ctx* ctx, int enable);
+
 /* Mean squared residuals at the end of the last elliptic solve, and mean
  * squared right-hand sides (relative criterion only), per field (psi, chi). */
 swm_status swm_last_residual(const swm_ctx* ctx, double resid_ms[2], double rhs_ms[2]);

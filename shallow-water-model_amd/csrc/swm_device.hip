@@ -97,6 +97,9 @@ struct swm_ctx {
   int precision = SWM_FP64, device = 0;
   int advection = SWM_ADV_FCT;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;   // the tracer transport beside the dynamics (overlap)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int overlap = 0;
   Arena arena;             // shared: metrics, halo tables, multigrid weights and maps
   size_t arena_bytes = 0;
   // ensemble members (same grid and settings).  Member m's state and scratch
@@ -158,6 +161,7 @@ struct swm_ctx {
   int solver_stall = 0;        // also stop where a cycle no longer halves the residual (FP64 floor)
   int mixed_extrapolate = 1;   // mixed path: start from 2 x_n - x_(n-1)
   int mixed_min_cycles = 1;    // mixed path: cycles before the first check
+  int mixed_fine_sweeps = 2;   // mixed path: smoothing sweeps before / after the coarse correction on the finest level
   double last_resid[2] = {0.0, 0.0};
   double last_rhs_ms[2] = {0.0, 0.0};
   std::vector<double> rhs_ms;   // per member and field (relative criterion)
@@ -285,18 +289,62 @@ void relax(swm_ctx* C, View<R>& V, int l, int iters, R* alpha, const R* beta, in
   }
 }
 
+// one sweep at level l from a into out (zero: a is zero and is not read, so
+// out may be a itself)
+template <class R>
+void relax_once(swm_ctx* C, View<R>& V, int l, const R* a, R* out, const R* beta, int nvar, bool zero) {
+  LevelT<R>& L = V.lev[l - 1];
+  const bool prof = C->profile && l == C->g && C->kev_used + 2 <= C->kev.size();
+  if (prof) {
+    (void)hipEventRecord(C->kev[C->kev_used++], C->stream);
+    C->kernel_bytes += (double)C->max_ig * sizeof(R) * (7.0 + (zero ? 2.0 : 3.0) * nvar);
+  }
+  if (nvar == 2) launch_relax<R, 2>(C, L, zero, beta, a, out);
+  else launch_relax<R, 1>(C, L, zero, beta, a, out);
+  if (prof) (void)hipEventRecord(C->kev[C->kev_used++], C->stream);
+}
+
 // multigrid_solver.F:476-591.  The coarse levels' alpha is zero at the start
 // of a cycle (the reference clears it); their first sweep is told so instead
 // of clearing memory.  Coarse beta is written by the injection at every slot
 // the smoother reads.  top_zero: the finest alpha is zero on entry as well
-// (a correction cycle of the mixed path).
+// (a correction cycle of the mixed path).  fine_sweeps = 1 (with top_zero
+// only): one sweep before and one after the coarse correction on the finest
+// level instead of two; the first, from zero, writes alpha in place, the
+// second leaves the result in the level's scratch.  Returns the finest
+// level's result (alpha, or that scratch).
 template <class R>
-void vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_zero = false) {
+R* vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_zero = false, int fine_sweeps = 2) {
   const int G = C->g;
   const int top = G < swm::dev::CO_MAXLEV ? G : swm::dev::CO_MAXLEV;   // levels 1..top in one workgroup
   auto A = [&](int l) { return l == G ? alpha : V.lev[l - 1].alpha; };
   auto B = [&](int l) { return l == G ? const_cast<R*>(beta) : V.lev[l - 1].beta; };
+  // finest level with another sweep count (mixed correction cycles): the
+  // iterate moves between alpha and the level's scratch
+  const bool own = fine_sweeps != 2 && top_zero && G > top;
+  R* fa = alpha;
+  R* fo = V.lev[G - 1].temp;
+  auto fine = [&](int n, bool zero) {
+    for (int i = 0; i < n; ++i) {
+      if (zero && i == 0) {
+        relax_once<R>(C, V, G, fa, fa, B(G), nvar, true);   // from zero: in place
+      } else {
+        relax_once<R>(C, V, G, fa, fo, B(G), nvar, false);
+        std::swap(fa, fo);
+      }
+    }
+  };
   for (int l = G; l > top; --l) {
+    if (l == G && own) {
+      fine(fine_sweeps, true);
+      if (nvar == 2)
+        hipLaunchKernelGGL((swm::dev::k_inject<R, 2>), zgrid(C, nblk_b(V.lev[l - 2].ns, C->blk_mg)), dim3(C->blk_mg), 0,
+                           C->stream, V.lev[l - 1].L, V.lev[l - 2].L, fa, B(l), B(l - 1));
+      else
+        hipLaunchKernelGGL((swm::dev::k_inject<R, 1>), zgrid(C, nblk_b(V.lev[l - 2].ns, C->blk_mg)), dim3(C->blk_mg), 0,
+                           C->stream, V.lev[l - 1].L, V.lev[l - 2].L, fa, B(l), B(l - 1));
+      continue;
+    }
     relax<R>(C, V, l, 2, A(l), B(l), nvar, l < G || top_zero);
     if (nvar == 2)
       hipLaunchKernelGGL((swm::dev::k_inject<R, 2>), zgrid(C, nblk_b(V.lev[l - 2].ns, C->blk_mg)), dim3(C->blk_mg), 0, C->stream,
@@ -315,14 +363,21 @@ void vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_
   P.mb = C->mb;
   hipLaunchKernelGGL(swm::dev::k_coarse_cycle<R>, zgrid(C, 1), dim3(swm::dev::CO_THREADS), 0, C->stream, P);
   for (int l = top; l <= G - 1; ++l) {
+    const bool f = l + 1 == G && own;
+    R* up = f ? fa : A(l + 1);
     if (nvar == 2)
       hipLaunchKernelGGL((swm::dev::k_interp<R, 2>), zgrid(C, nblk_b(V.lev[l].ns, C->blk_mg)), dim3(C->blk_mg), 0, C->stream,
-                         V.lev[l - 1].L, V.lev[l].L, A(l), A(l + 1));
+                         V.lev[l - 1].L, V.lev[l].L, A(l), up);
     else
       hipLaunchKernelGGL((swm::dev::k_interp<R, 1>), zgrid(C, nblk_b(V.lev[l].ns, C->blk_mg)), dim3(C->blk_mg), 0, C->stream,
-                         V.lev[l - 1].L, V.lev[l].L, A(l), A(l + 1));
+                         V.lev[l - 1].L, V.lev[l].L, A(l), up);
+    if (f) {
+      fine(fine_sweeps, false);
+      return fa;
+    }
     relax<R>(C, V, l + 1, 2, A(l + 1), B(l + 1), nvar, false);
   }
+  return alpha;
 }
 
 // sums of x0*w and x1*w into C->sums[0..1] (block partials, then a
@@ -526,13 +581,17 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
       shift[v] = false;
     }
   };
-  // one FP32 correction cycle on fields v0.. (nv of them: both, or one)
+  // one FP32 correction cycle on fields v0.. (nv of them: both, or one);
+  // the correction lands in e or, with one finest sweep, in that level's
+  // scratch (same plane layout)
   auto cycle = [&](int nv, int v0) {
     // the two fields' planes are contiguous: cur[0] + ns == cur[1] holds while they move together
-    vcycle<float>(C, V, nv, e + (size_t)v0 * ns, r + (size_t)v0 * ns, true);
+    float* res = vcycle<float>(C, V, nv, e + (size_t)v0 * ns, r + (size_t)v0 * ns, true, C->mixed_fine_sweeps);
+    e = res - (size_t)v0 * ns;
     if (nv == 2) gather<float>(C, C->face, {e, e + ns});
     else gather<float>(C, C->face, {e + (size_t)v0 * ns});
     launch_defect(nv, v0);
+    e = C->mx_e;
   };
   const bool extrap = C->nsol >= 2 && C->mixed_extrapolate;
   hipLaunchKernelGGL(swm::dev::k_defect_extrap<2>, zgrid(C, nb), dim3(BLK), 0, C->stream, G64, C->area_mask, blk[0],
@@ -643,6 +702,8 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   R* m3 = V.msh_f[C->n3f - 1];
   R* m2 = V.msh_f[C->n2f - 1];
   R* m1 = V.msh_f[C->n1f - 1];
+  const bool fct = C->advection == SWM_ADV_FCT;
+  const bool overlap = C->overlap && C->nt > 0 && fct;
 
   hipLaunchKernelGGL(swm::dev::k_avg_corners<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, G.sw, V.h[n3], V.hc);
   gather<R>(C, C->vrtx, {V.hc});
@@ -683,22 +744,36 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
                      V.mass_flux, V.h[n4]);
   // the tracer limiter fills the halo of the new mass in place (tracer_fct.F:344)
   gather<R>(C, C->face, {V.h[n4]});
-  for (int q = 0; q < C->nt; ++q) {   // tracer FCT (tracer_fct.F:22-670)
+  }
+  // Tracer FCT (tracer_fct.F:22-670).  The tracers depend on the new mass
+  // and the mass flux only, and nothing after them but the final halo fill
+  // reads them: with overlap they run on the second stream beside the
+  // vorticity, divergence and elliptic solve.
+  if (overlap) {
+    HIP_TRY(hipEventRecord(C->ev_fork, S));
+    HIP_TRY(hipStreamWaitEvent(C->stream2, C->ev_fork, 0));
+    C->stream = C->stream2;
+  }
+  const hipStream_t St = C->stream;
+  for (int q = 0; q < (fct ? C->nt : 0); ++q) {
     R* t3 = V.tr[n3][q];
     R* t4 = V.tr[n4][q];
-    hipLaunchKernelGGL(swm::dev::k_tr_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, w3, w2, w1, t3, t4, V.told[q],
+    hipLaunchKernelGGL(swm::dev::k_tr_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, w3, w2, w1, t3, t4, V.told[q],
                        V.mass_flux, V.f1, V.f2);
     gather<R>(C, C->edge, {V.f1, V.f2});
-    hipLaunchKernelGGL(swm::dev::k_tr_low<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, V.h[n3], t3, V.f1, V.ml);
+    hipLaunchKernelGGL(swm::dev::k_tr_low<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, V.h[n3], t3, V.f1, V.ml);
     gather<R>(C, C->face, {V.ml});
-    hipLaunchKernelGGL(swm::dev::k_tr_limiter<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, t3, V.ml, V.h[n4], V.mss_low,
+    hipLaunchKernelGGL(swm::dev::k_tr_limiter<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, t3, V.ml, V.h[n4], V.mss_low,
                        V.f1, V.f2, V.anti, V.bdn, V.bup);
     gather<R>(C, C->face, {V.bdn, V.bup});
-    hipLaunchKernelGGL(swm::dev::k_tr_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.bdn, V.bup, V.anti);
+    hipLaunchKernelGGL(swm::dev::k_tr_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, V.bdn, V.bup, V.anti);
     gather<R>(C, C->edge, {V.anti});
-    hipLaunchKernelGGL(swm::dev::k_tr_final<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, V.ml, V.anti, V.h[n4], t4,
+    hipLaunchKernelGGL(swm::dev::k_tr_final<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, V.ml, V.anti, V.h[n4], t4,
                        V.told[q]);
   }
+  if (overlap) {
+    HIP_TRY(hipEventRecord(C->ev_join, C->stream2));
+    C->stream = S;
   }
   // vorticity and divergence (step.F:143-163, :336-429)
   hipLaunchKernelGGL(swm::dev::k_vort_curl<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.mass_flux, V.h[n3], V.eta[n3], V.hc,
@@ -730,6 +805,7 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   gather<R>(C, C->vrtx, {V.u[n4], V.u[n4] + 2 * (size_t)ns});
   hipLaunchKernelGGL(swm::dev::k_center_ke<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.u[n4], V.ke);
   gather<R>(C, C->face, {V.ke});
+  if (overlap) HIP_TRY(hipStreamWaitEvent(S, C->ev_join, 0));   // the tracers before the halo fill
   return SWM_OK;
 }
 
@@ -1086,6 +1162,9 @@ void launch_budgets(swm_ctx* C, View<R>& V, int nq) {
 swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
   HIP_TRY(hipSetDevice(C->device));
   HIP_TRY(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&C->stream2, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&C->ev_fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&C->ev_join, hipEventDisableTiming));
   HIP_TRY(hipEventCreate(&C->ev_a));
   HIP_TRY(hipEventCreate(&C->ev_b));
   // sizing pass, then the shared arena and the member blocks
@@ -1237,6 +1316,10 @@ void swm_destroy(swm_ctx* C) {
   (void)hipSetDevice(C->device);
   if (C->stream) (void)hipStreamSynchronize(C->stream);
   for (hipEvent_t e : C->kev) (void)hipEventDestroy(e);
+  if (C->stream2) (void)hipStreamSynchronize(C->stream2);
+  if (C->ev_fork) (void)hipEventDestroy(C->ev_fork);
+  if (C->ev_join) (void)hipEventDestroy(C->ev_join);
+  if (C->stream2) (void)hipStreamDestroy(C->stream2);
   if (C->ev_a) (void)hipEventDestroy(C->ev_a);
   if (C->ev_b) (void)hipEventDestroy(C->ev_b);
   if (C->arena.base) (void)hipFree(C->arena.base);
@@ -1522,6 +1605,23 @@ swm_status swm_get_field(swm_ctx* C, const char* name, double* out) {
   }
   swm::set_error("swm_get_field: unknown field " + s);
   return SWM_ERR_ARG;
+}
+
+swm_status swm_set_mixed_smoother(swm_ctx* C, int fine_sweeps) {
+  if (!C || fine_sweeps < 1 || fine_sweeps > 4) {
+    swm::set_error("swm_set_mixed_smoother: 1 to 4 sweeps");
+    return SWM_ERR_ARG;
+  }
+  C->mixed_fine_sweeps = fine_sweeps;
+  C->epoch += 1;
+  return SWM_OK;
+}
+
+swm_status swm_set_overlap(swm_ctx* C, int enable) {
+  if (!C) { swm::set_error("swm_set_overlap: bad argument"); return SWM_ERR_ARG; }
+  C->overlap = enable ? 1 : 0;
+  C->epoch += 1;
+  return SWM_OK;
 }
 
 int swm_members(const swm_ctx* C) { return C ? C->nmem : 0; }
