@@ -273,8 +273,9 @@ swm_status swm_solver_stats(const swm_ctx* ctx, long long* solves, long long* vc
 swm_status swm_timing(const swm_ctx* ctx, double* step_ms, double* kernel_ms_mean,
                       double* kernel_bytes, int* kernel_launches);
 
-/* Enables (1) or disables (0) HIP-event timing of every launch of the
- * finest-level multigrid smoother inside swm_step; read back by swm_timing. */
+/* HIP-event timing of the finest-level multigrid smoother inside swm_step,
+ * read back by swm_timing: every launch (1), every n-th (n > 1: the event
+ * records cost the step a few microseconds each), or off (0). */
 swm_status swm_profile(swm_ctx* ctx, int enable);
 
 /* Seeded synthetic initial state built on the device: harmonic sums of

@@ -171,7 +171,8 @@ struct swm_ctx {
   double last_step_ms = 0.0;
   std::vector<hipEvent_t> kev;   // pairs around the profiled kernel
   size_t kev_used = 0;
-  bool profile = false;
+  int profile = 0;                 // > 0: every profile-th finest-level smoother launch is timed
+  long long prof_seq = 0;
   double kernel_bytes = 0.0;
   // tangent-linear / adjoint window (allocated on first use)
   struct TlAd* tl = nullptr;
@@ -271,7 +272,8 @@ void relax(swm_ctx* C, View<R>& V, int l, int iters, R* alpha, const R* beta, in
   R* a = alpha;
   R* t = L.temp;
   for (int it = 0; it < iters; ++it) {
-    const bool prof = C->profile && l == C->g && C->kev_used + 2 <= C->kev.size();
+    const bool prof = C->profile && l == C->g && C->kev_used + 2 <= C->kev.size() &&
+                      C->prof_seq++ % C->profile == 0;
     if (prof) {
       (void)hipEventRecord(C->kev[C->kev_used++], C->stream);
       // algorithmic bytes of one finest-level sweep per real cell: six
@@ -294,7 +296,8 @@ void relax(swm_ctx* C, View<R>& V, int l, int iters, R* alpha, const R* beta, in
 template <class R>
 void relax_once(swm_ctx* C, View<R>& V, int l, const R* a, R* out, const R* beta, int nvar, bool zero) {
   LevelT<R>& L = V.lev[l - 1];
-  const bool prof = C->profile && l == C->g && C->kev_used + 2 <= C->kev.size();
+  const bool prof = C->profile && l == C->g && C->kev_used + 2 <= C->kev.size() &&
+                    C->prof_seq++ % C->profile == 0;
   if (prof) {
     (void)hipEventRecord(C->kev[C->kev_used++], C->stream);
     C->kernel_bytes += (double)C->max_ig * sizeof(R) * (7.0 + (zero ? 2.0 : 3.0) * nvar);
@@ -1737,7 +1740,8 @@ swm_status swm_solver_stats(const swm_ctx* C, long long* solves, long long* vcyc
 swm_status swm_profile(swm_ctx* C, int enable) {
   if (!C) return SWM_ERR_ARG;
   HIP_TRY(hipSetDevice(C->device));
-  C->profile = enable != 0;
+  C->profile = enable > 0 ? enable : 0;
+  C->prof_seq = 0;
   if (C->profile && C->kev.empty()) {
     C->kev.resize(4096);
     for (hipEvent_t& e : C->kev) HIP_TRY(hipEventCreate(&e));

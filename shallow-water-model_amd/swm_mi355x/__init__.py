@@ -534,8 +534,10 @@ class Model:
         """Seeded synthetic initial state built on the device (swm_init_synthetic)."""
         _check(lib.swm_init_synthetic(self._h, seed_index, lmax, umax, h0, h_rms, nbumps, bump_radius))
 
-    def profile(self, enable=True):
-        _check(lib.swm_profile(self._h, 1 if enable else 0))
+    def profile(self, enable=True, every=1):
+        """HIP-event timing of the finest-level smoother: every `every`-th
+        launch while enabled."""
+        _check(lib.swm_profile(self._h, max(1, int(every)) if enable else 0))
 
     def timing(self):
         """(device ms of the last step call, mean ms, bytes and launch count of
