@@ -93,9 +93,11 @@ def timed(model, warmup, steps):
     model.step(warmup)
     model.synchronize()
     s0, v0, c0 = model.solver_stats()
-    # every 8th finest-level smoother launch is timed with HIP events (the
-    # event records would cost the step ~3 us each if every launch had them)
-    model.profile(True, every=8)
+    # every 7th finest-level smoother launch is timed with HIP events (the
+    # event records would cost the step ~3 us each if every launch had them;
+    # 7 is prime to the 8 or 12 finest sweeps of a step, so the sample
+    # covers every position in the V-cycles)
+    model.profile(True, every=7)
     t1 = time.perf_counter()
     model.step(steps)
     model.synchronize()
