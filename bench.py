@@ -109,8 +109,10 @@ def timed(model, warmup, steps):
             "vcycles_per_step": (v1 - v0) / max(1, s1 - s0), "checks_per_step": (c1 - c0) / max(1, s1 - s0)}
 
 
-def adjoint_bench(S, g=7, hours=6.0, every=16, ndir=8):
-    """The adjoint form: FP64, centred advection, a window of `hours` at
+def adjoint_bench(S, g=7, hours=6.0, every=16, ndir=8, advection="fct"):
+    """The adjoint form: FP64, the reference's configured advection (FCT,
+    linearised with the limiter's coefficients frozen at the base
+    trajectory), a window of `hours` at
     level g (dt = 200 * 2^(5-g) s), snapshots every `every` steps.  Control:
     relative vorticity, divergence and thickness of the seeded synthetic
     state (degrees 1..40); objective: the area-weighted sum of (h - 5000)^2
@@ -125,7 +127,7 @@ def adjoint_bench(S, g=7, hours=6.0, every=16, ndir=8):
     grid = S.Grid(g, radius=RADIUS, omega=OMEGA)
 
     def model(tight=True):
-        m = S.Model(grid, ntracer=1, dt=dt, grav=GRAV, advection="centered")
+        m = S.Model(grid, ntracer=1, dt=dt, grav=GRAV, advection=advection)
         if tight:   # solves to the FP64 floor, so that the model is differentiable to round-off
             m.set_solver(tol=0.0, relative=True, max_cycles=60)
         return m
@@ -171,7 +173,8 @@ def adjoint_bench(S, g=7, hours=6.0, every=16, ndir=8):
     b = sum(float(np.sum(u * v)) for u, v in zip(dc, m.adj_run(*y, control=True)))
     fd = A.finite_difference_check(model, control, grad, lambda h: A.cap_objective(h, area, cap, 5000.0),
                                    nsteps, ndir, 8, mask)
-    return {"workload": f"icosahedral level {g} ({grid.max_ig} cells), FP64 centred advection, dt={dt} s, "
+    scheme = {"fct": "FCT (frozen limiter)", "centered": "centred", "upwind": "upstream"}[advection]
+    return {"workload": f"icosahedral level {g} ({grid.max_ig} cells), FP64 {scheme} advection, dt={dt} s, "
                         f"{hours:g} h window ({nsteps} steps), snapshots every {every} steps with recomputation; "
                         f"J = sum over a 1500 km cap of area (h - 5000)^2, gradient with respect to initial "
                         f"vorticity, divergence and thickness",

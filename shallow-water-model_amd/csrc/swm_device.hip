@@ -122,6 +122,7 @@ struct swm_ctx {
   int blk_step = 256, blk_mg = 256;
   long long epoch = 0;             // changes whenever captured launches would differ (initialisation, settings)
   double* track = nullptr;
+  double* fct_rec = nullptr;       // FP64 FCT step: the limiter coefficients go here (linearised model's base steps)
   double* rmean = nullptr;         // mixed path: sums of area * rhs of the two fields (mean removal)
   double* part_rhs = nullptr;      // mixed path: their per-block partial sums, from the tendencies kernel         // per member: max mean(r^2) of the two fields, count of solves above the criterion
   // FP64 metrics (both paths: set-up, FP64 residual, sums)
@@ -741,6 +742,10 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   hipLaunchKernelGGL(swm::dev::k_mass_low_beta<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, w3, w2, w1, V.flx1, m3, m2, m1,
                      V.h[n3], V.anti, V.mss_low, V.beta);
   gather<R>(C, C->face, {V.beta});
+  if constexpr (sizeof(R) == 8) {
+    if (C->fct_rec)
+      hipLaunchKernelGGL(swm::dev::k_fct_coef, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.beta, V.anti, C->fct_rec);
+  }
   hipLaunchKernelGGL(swm::dev::k_mass_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.beta, V.anti);
   gather<R>(C, C->edge, {V.anti});
   hipLaunchKernelGGL(swm::dev::k_mass_final<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, V.flx1, V.anti, V.mss_low,
@@ -2093,17 +2098,20 @@ swm_status swm_multigrid(swm_ctx* C, int nvar, const double* beta, double* alpha
 // ===================================================================
 struct TlAd {
   // TL or AD state: two time levels of h, eta, div, u (4 planes), three
-  // ABM3 history slots of the h, eta and div tendencies
-  double *h[2], *eta[2], *div[2], *u[2], *hf[3], *ef[3], *df[3];
+  // ABM3 history slots of the h, eta and div tendencies, and (FCT) three of
+  // the centred edge mass fluxes
+  double *h[2], *eta[2], *div[2], *u[2], *hf[3], *ef[3], *df[3], *mf[3];
   // scratch planes
   double *hc, *F, *vf, *dc, *ke, *X, *rhs, *xs, *psia, *bhc, *bF, *zeros, *am_inv;
+  double *fl, *an, *ml, *bsc;         // FCT: upstream flux, antidiffusive flux, low-order mass, limiter scratch
+  int seg_planes = 6;                 // per base state: h, eta, u (4); FCT: + limiter coefficients (3), mass flux (3)
   double* mem = nullptr;
   // trajectory
   int every = 0, nsteps = 0;
   size_t snap_words = 0;
   std::vector<double*> ckpt;          // snapshots at window steps 0, every, 2 every, ..., and the end
   std::vector<long long> ckpt_meta;   // step_count, nsecond per snapshot
-  double* seg = nullptr;              // (every + 1) states of [h, eta, u(4)]
+  double* seg = nullptr;              // (every + 1) base states (seg_planes each)
   bool have_window = false;
   swm::dev::Geo lin;                  // metrics with hs = 0 and f = 0
 };
@@ -2130,15 +2138,17 @@ swm_status tlad_alloc(swm_ctx* C) {
   const size_t ns = C->ns;
   TlAd* T = new TlAd;
   C->tl = T;
-  const size_t words = ns * (2 * (1 + 1 + 1 + 4) + 9 + 2 + 3 + 3 + 3 + 1 + 1 + 2 + 2 + 2 + 2 + 3 + 1 + 1);
+  const size_t words = ns * (2 * (1 + 1 + 1 + 4) + 9 + 9 + 2 + 3 + 3 + 3 + 1 + 1 + 2 + 2 + 2 + 2 + 3 + 1 + 1 + 3 + 3 + 1 + 1);
   HIP_TRY(hipMalloc(&T->mem, words * sizeof(double)));
   double* p = T->mem;
   auto take = [&](size_t n) { double* q = p; p += n; return q; };
   for (int t = 0; t < 2; ++t) { T->h[t] = take(ns); T->eta[t] = take(ns); T->div[t] = take(ns); T->u[t] = take(4 * ns); }
   for (int t = 0; t < 3; ++t) { T->hf[t] = take(ns); T->ef[t] = take(ns); T->df[t] = take(ns); }
+  for (int t = 0; t < 3; ++t) T->mf[t] = take(3 * ns);
   T->hc = take(2 * ns); T->F = take(3 * ns); T->vf = take(3 * ns); T->dc = take(3 * ns); T->ke = take(ns);
   T->X = take(ns); T->rhs = take(2 * ns); T->xs = take(2 * ns); T->psia = take(2 * ns); T->bhc = take(2 * ns);
   T->bF = take(3 * ns); T->zeros = take(ns); T->am_inv = take(ns);
+  T->fl = take(3 * ns); T->an = take(3 * ns); T->ml = take(ns); T->bsc = take(ns);
   fill<double>(C, T->mem, words, 0.0);
   T->lin = C->d.geo;
   T->lin.hs = T->zeros;
@@ -2154,7 +2164,7 @@ swm_status tlad_alloc(swm_ctx* C) {
 void clear_state(swm_ctx* C) {
   TlAd* T = C->tl;
   const size_t ns = C->ns;
-  fill<double>(C, T->h[0], (size_t)ns * 2 * (1 + 1 + 1 + 4) + 9 * ns, 0.0);   // the state planes are contiguous
+  fill<double>(C, T->h[0], (size_t)ns * 2 * (1 + 1 + 1 + 4) + 18 * ns, 0.0);   // the state planes are contiguous
 }
 
 swm_status snapshot(swm_ctx* C, double* dst) {
@@ -2185,10 +2195,13 @@ swm_status base_step(swm_ctx* C) {
   return SWM_OK;
 }
 
-// base state slot q of the segment buffer: h, eta, u (4 planes)
-double* seg_h(swm_ctx* C, int q) { return C->tl->seg + (size_t)q * 6 * C->ns; }
+// base state slot q of the segment buffer: h, eta, u (4 planes); FCT: the
+// step's frozen limiter coefficients (3 planes) and total mass flux (3)
+double* seg_h(swm_ctx* C, int q) { return C->tl->seg + (size_t)q * C->tl->seg_planes * C->ns; }
 double* seg_e(swm_ctx* C, int q) { return seg_h(C, q) + C->ns; }
 double* seg_u(swm_ctx* C, int q) { return seg_h(C, q) + 2 * (size_t)C->ns; }
+double* seg_c(swm_ctx* C, int q) { return seg_h(C, q) + 6 * (size_t)C->ns; }
+double* seg_F(swm_ctx* C, int q) { return seg_h(C, q) + 9 * (size_t)C->ns; }
 
 swm_status record_base(swm_ctx* C, int q) {
   const int n3 = C->n4 - 1;   // the level the next step reads (after pointers of the last step: n4)
@@ -2206,8 +2219,17 @@ swm_status recompute_segment(swm_ctx* C, int sidx, int len) {
   swm_status st;
   if ((st = restore(C, T->ckpt[sidx], T->ckpt_meta[2 * sidx], T->ckpt_meta[2 * sidx + 1]))) return st;
   if ((st = record_base(C, 0))) return st;
+  const bool fct = C->advection == SWM_ADV_FCT;
   for (int q = 0; q < len; ++q) {
-    if ((st = base_step(C))) return st;
+    // FCT: the step records its limiter coefficients (step_dt), then its
+    // total mass flux is kept
+    if (fct) C->fct_rec = seg_c(C, q);
+    st = base_step(C);
+    C->fct_rec = nullptr;
+    if (st) return st;
+    if (fct)
+      HIP_TRY(hipMemcpyAsync(seg_F(C, q), C->d.mass_flux, 3 * (size_t)C->ns * sizeof(double), hipMemcpyDeviceToDevice,
+                             C->stream));
     if ((st = record_base(C, q + 1))) return st;
   }
   return SWM_OK;
@@ -2226,17 +2248,22 @@ swm_status lin_solve(swm_ctx* C, double* rhs, double* x) {
   return st;
 }
 
-// the step's coefficients from base state q: corner thickness and mass flux
-void base_coeffs(swm_ctx* C, int q) {
+// the step's coefficients from base state q: corner thickness, and the
+// base mass flux the vorticity flux takes (centred or upstream, recomputed;
+// FCT: the recorded total flux); returns the mass flux
+const double* base_coeffs(swm_ctx* C, int q) {
   TlAd* T = C->tl;
   const unsigned nb = nblk(C->ns);
   const swm::dev::Geo& G = C->d.geo;
   hipLaunchKernelGGL(swm::dev::k_avg_corners<double>, dim3(nb), dim3(BLK), 0, C->stream, G, G.sw, seg_h(C, q), T->bhc);
   gather<double>(C, C->vrtx, {T->bhc});
-  // k_mass_fluxes writes the upstream flux to its first output (scratch here)
+  if (C->advection == SWM_ADV_FCT) return seg_F(C, q);
+  // k_mass_fluxes: the upstream flux first, the centred one second
+  const bool cen = C->advection == SWM_ADV_CENTERED;
   hipLaunchKernelGGL(swm::dev::k_mass_fluxes<double>, dim3(nb), dim3(BLK), 0, C->stream, G, seg_u(C, q), seg_h(C, q),
-                     T->bhc, T->vf, T->bF);
+                     T->bhc, cen ? T->vf : T->bF, cen ? T->bF : T->vf);
   gather<double>(C, C->edge, {T->bF});
+  return T->bF;
 }
 
 // halo transposes of a set of fields
@@ -2273,18 +2300,38 @@ swm_status tl_step(swm_ctx* C, int q) {
   const int n3 = C->n3 - 1, n4 = C->n4 - 1;
   const int a = C->n3f - 1, b = C->n2f - 1, c = C->n1f - 1;
   const double dt = C->dt, w3 = C->w3, w2 = C->w2, w1 = C->w1;
-  base_coeffs(C, q);
+  const double* bF = base_coeffs(C, q);
   double *dh3 = T->h[n3], *de3 = T->eta[n3], *du3 = T->u[n3];
   // kinetic energy of u3 (computed at the end of the previous step)
   hipLaunchKernelGGL(swm::dev::k_tl_center_ke, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), du3, T->ke);
   gather<double>(C, C->face, {T->ke});
   hipLaunchKernelGGL(swm::dev::k_avg_corners<double>, dim3(nb), dim3(BLK), 0, S, G, G.sw, dh3, T->hc);
   gather<double>(C, C->vrtx, {T->hc});
-  hipLaunchKernelGGL(swm::dev::k_tl_mass_flux, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), T->bhc, du3, T->hc, T->F);
-  gather<double>(C, C->edge, {T->F});
-  hipLaunchKernelGGL(swm::dev::k_tl_height, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, T->F, dh3, T->hf[a],
-                     T->hf[b], T->hf[c], seg_h(C, q + 1), T->h[n4]);
-  hipLaunchKernelGGL(swm::dev::k_tl_vort_curl, dim3(nb), dim3(BLK), 0, S, G, T->bF, seg_h(C, q), seg_e(C, q), T->bhc,
+  if (C->advection == SWM_ADV_FCT) {
+    // mass FCT with the base step's limiter coefficients: upstream and
+    // centred fluxes, antidiffusive flux and low-order mass, the limited
+    // flux, then the total flux and the new thickness
+    hipLaunchKernelGGL(swm::dev::k_tl_upwind_flux, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), seg_h(C, q), du3, dh3, T->fl);
+    hipLaunchKernelGGL(swm::dev::k_tl_mass_flux, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), T->bhc, du3, T->hc, T->mf[a]);
+    gather<double>(C, C->edge, {T->fl, T->mf[a]});
+    hipLaunchKernelGGL(swm::dev::k_mass_low_beta<double>, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, T->fl, T->mf[a],
+                       T->mf[b], T->mf[c], dh3, T->an, T->ml, T->bsc);
+    hipLaunchKernelGGL(swm::dev::k_fct_scale, dim3(nblk(3 * ns)), dim3(BLK), 0, S, 3 * ns, seg_c(C, q), T->an);
+    gather<double>(C, C->edge, {T->an});
+    hipLaunchKernelGGL(swm::dev::k_mass_final<double>, dim3(nb), dim3(BLK), 0, S, G, dt, T->fl, T->an, T->ml, T->F,
+                       T->h[n4]);
+    gather<double>(C, C->face, {T->h[n4]});
+  } else {
+    if (C->advection == SWM_ADV_CENTERED)
+      hipLaunchKernelGGL(swm::dev::k_tl_mass_flux, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), T->bhc, du3, T->hc, T->F);
+    else
+      hipLaunchKernelGGL(swm::dev::k_tl_upwind_flux, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), seg_h(C, q), du3, dh3,
+                         T->F);
+    gather<double>(C, C->edge, {T->F});
+    hipLaunchKernelGGL(swm::dev::k_tl_height, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, T->F, dh3, T->hf[a],
+                       T->hf[b], T->hf[c], seg_h(C, q + 1), T->h[n4]);
+  }
+  hipLaunchKernelGGL(swm::dev::k_tl_vort_curl, dim3(nb), dim3(BLK), 0, S, G, bF, seg_h(C, q), seg_e(C, q), T->bhc,
                      seg_u(C, q), T->F, dh3, de3, T->hc, du3, T->vf, T->dc);
   gather<double>(C, C->edge, {T->vf, T->dc});
   hipLaunchKernelGGL(swm::dev::k_tendencies<double>, dim3(nb), dim3(BLK), 0, S, T->lin, dt, C->grav, w3, w2, w1, T->vf,
@@ -2312,7 +2359,7 @@ swm_status ad_step(swm_ctx* C, int q) {
   const int n3 = C->n3 - 1, n4 = C->n4 - 1;
   const int a = C->n3f - 1, b = C->n2f - 1, c = C->n1f - 1;
   const double dt = C->dt, w3 = C->w3, w2 = C->w2, w1 = C->w1;
-  base_coeffs(C, q);
+  const double* bF = base_coeffs(C, q);
   // wrap_prognostics
   gather_adj(C, C->vrtx, {T->u[0], T->u[0] + 2 * ns, T->u[1], T->u[1] + 2 * ns});
   gather_adj(C, C->face, {T->h[n4], T->eta[n4], T->div[n4]});
@@ -2351,14 +2398,39 @@ swm_status ad_step(swm_ctx* C, int q) {
   gather_adj(C, C->edge, {T->vf, T->dc});
   fill<double>(C, T->F, 3 * ns, 0.0);
   fill<double>(C, T->hc, 2 * ns, 0.0);
-  hipLaunchKernelGGL(swm::dev::k_ad_vort_curl, dim3(nb), dim3(BLK), 0, S, G, T->bF, seg_h(C, q), seg_e(C, q), T->bhc,
+  hipLaunchKernelGGL(swm::dev::k_ad_vort_curl, dim3(nb), dim3(BLK), 0, S, G, bF, seg_h(C, q), seg_e(C, q), T->bhc,
                      seg_u(C, q), T->vf, T->dc, T->F, T->h[n3], T->eta[n3], T->hc, T->u[n3]);
-  // height update
-  hipLaunchKernelGGL(swm::dev::k_ad_height, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, T->h[n4], T->hf[a], T->hf[b],
-                     T->hf[c], seg_h(C, q + 1), T->h[n3], T->F);
-  // mass flux
-  gather_adj(C, C->edge, {T->F});
-  hipLaunchKernelGGL(swm::dev::k_ad_mass_flux, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), T->bhc, T->F, T->u[n3], T->hc);
+  if (C->advection == SWM_ADV_FCT) {
+    // transpose of the mass FCT with the frozen limiter (tl_step)
+    gather_adj(C, C->face, {T->h[n4]});
+    fill<double>(C, T->fl, 3 * ns, 0.0);
+    fill<double>(C, T->an, 3 * ns, 0.0);
+    fill<double>(C, T->ml, ns, 0.0);
+    hipLaunchKernelGGL(swm::dev::k_ad_fct_final, dim3(nb), dim3(BLK), 0, S, G, dt, T->F, T->h[n4], T->fl, T->an, T->ml);
+    gather_adj(C, C->edge, {T->an});
+    hipLaunchKernelGGL(swm::dev::k_fct_scale, dim3(nblk(3 * ns)), dim3(BLK), 0, S, 3 * ns, seg_c(C, q), T->an);
+    hipLaunchKernelGGL(swm::dev::k_ad_fct_low, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, T->an, T->ml, T->mf[a],
+                       T->mf[b], T->mf[c], T->fl, T->h[n3]);
+    gather_adj(C, C->edge, {T->fl, T->mf[a]});
+    hipLaunchKernelGGL(swm::dev::k_ad_upwind_flux, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), seg_h(C, q), T->fl,
+                       T->u[n3], T->h[n3]);
+    hipLaunchKernelGGL(swm::dev::k_ad_mass_flux, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), T->bhc, T->mf[a], T->u[n3],
+                       T->hc);
+    // the slot's value before this step is read by no later step
+    fill<double>(C, T->mf[a], 3 * ns, 0.0);
+  } else {
+    // height update
+    hipLaunchKernelGGL(swm::dev::k_ad_height, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, T->h[n4], T->hf[a],
+                       T->hf[b], T->hf[c], seg_h(C, q + 1), T->h[n3], T->F);
+    // mass flux
+    gather_adj(C, C->edge, {T->F});
+    if (C->advection == SWM_ADV_CENTERED)
+      hipLaunchKernelGGL(swm::dev::k_ad_mass_flux, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), T->bhc, T->F, T->u[n3],
+                         T->hc);
+    else
+      hipLaunchKernelGGL(swm::dev::k_ad_upwind_flux, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), seg_h(C, q), T->F,
+                         T->u[n3], T->h[n3]);
+  }
   // corner thickness
   gather_adj(C, C->vrtx, {T->hc});
   hipLaunchKernelGGL(swm::dev::k_ad_avg_corners, dim3(nb), dim3(BLK), 0, S, G, T->hc, T->h[n3]);
@@ -2502,8 +2574,8 @@ swm_status window_check(swm_ctx* C, const char* who) {
     swm::set_error(std::string(who) + ": the linearised model runs on single-member contexts");
     return SWM_ERR_ARG;
   }
-  if (C->precision != SWM_FP64 || C->advection != SWM_ADV_CENTERED) {
-    swm::set_error(std::string(who) + ": the linearised model is the FP64 step with centred advection");
+  if (C->precision != SWM_FP64) {
+    swm::set_error(std::string(who) + ": the linearised model is the FP64 step");
     return SWM_ERR_ARG;
   }
   return SWM_OK;
@@ -2557,7 +2629,8 @@ swm_status swm_adj_forward(swm_ctx* C, int nsteps, int every) {
     HIP_TRY(hipMalloc(&p, T->snap_words * sizeof(double)));
     T->ckpt.push_back(p);
   }
-  HIP_TRY(hipMalloc(&T->seg, (size_t)(every + 1) * 6 * C->ns * sizeof(double)));
+  T->seg_planes = C->advection == SWM_ADV_FCT ? 12 : 6;
+  HIP_TRY(hipMalloc(&T->seg, (size_t)(every + 1) * T->seg_planes * C->ns * sizeof(double)));
   T->every = every;
   T->nsteps = nsteps;
   HIP_TRY(hipEventRecord(C->ev_a, C->stream));

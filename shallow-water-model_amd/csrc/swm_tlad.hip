@@ -458,6 +458,157 @@ __global__ void __launch_bounds__(256) k_dot_partial(size_t n, const double* __r
   block_sums<1>(acc, part);
 }
 
+// ------------------------------------- mass FCT with a frozen limiter ----
+// mass_fct.F:23-539 linearised about the base trajectory with the limiter's
+// coefficients and the upstream direction of every edge taken from the base
+// step: the low-order (upstream) flux, the antidiffusive flux from the AB3
+// history of centred fluxes, the low-order mass and the corrected mass are
+// then linear, and the limited flux is the antidiffusive one scaled by a
+// fixed coefficient per edge.  The linear parts reuse the forward kernels
+// (k_mass_low_beta for the antidiffusive flux and low-order mass, its limiter
+// output going to scratch; k_mass_final).
+
+// the frozen limiter coefficient of every edge (base step, before the
+// limiting of k_mass_anti): at interior slots min(1, beta(s)) where the base
+// antidiffusive flux is >= 0, else min(1, beta(up)); 1 on the other slots,
+// which keep the unlimited flux until the edge halo fill
+__global__ void k_fct_coef(Geo G, const double* __restrict__ beta, const double* __restrict__ anti,
+                           double* __restrict__ coef) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) coef[e * ns + s] = 1.0;
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  const double b = dmin(1.0, beta[s]);
+  for (int e = 0; e < 3; ++e) coef[e * ns + s] = anti[e * ns + s] >= 0.0 ? b : dmin(1.0, beta[up[e]]);
+}
+
+// the limited flux, TL and AD alike (a diagonal map): x *= coef on n values
+__global__ void k_fct_scale(size_t n, const double* __restrict__ coef, double* __restrict__ x) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] = x[t] * coef[t];
+}
+
+// half-edge normal velocity flux (u . n) dc (the upstream flux drops the
+// corner mass, operators.F:1731-1832) and its AD
+__device__ __forceinline__ double half_un(const Geo& G, int ns, const double* __restrict__ u, const HalfTerm& t) {
+  const double un = UPL(u, 0, t.ic, t.sl) * NPL(G.nrm, t.kk, 0, t.ic, t.sl) +
+                    UPL(u, 1, t.ic, t.sl) * NPL(G.nrm, t.kk, 1, t.ic, t.sl);
+  return un * DPL(t.kk, t.ic, t.sl);
+}
+__device__ __forceinline__ void half_un_ad(const Geo& G, int ns, double g, const HalfTerm& t, double* __restrict__ ub) {
+  const double d = g * DPL(t.kk, t.ic, t.sl);
+  aadd(&UPL(ub, 0, t.ic, t.sl), d * NPL(G.nrm, t.kk, 0, t.ic, t.sl));
+  aadd(&UPL(ub, 1, t.ic, t.sl), d * NPL(G.nrm, t.kk, 1, t.ic, t.sl));
+}
+
+// TL of the upstream flux of k_mass_fluxes (mass_fct.F:151-214): with the
+// base edge velocity flux v = r1 - r0, flx = v h(s) where v >= 0, else
+// v h(up); interior slots (0 elsewhere)
+__global__ void k_tl_upwind_flux(Geo G, const double* __restrict__ u, const double* __restrict__ h3,
+                                 const double* __restrict__ du, const double* __restrict__ dh3,
+                                 double* __restrict__ dflx) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) dflx[e * ns + s] = 0.0;
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  HalfTerm t[2][3];
+  half_terms(iim, jjm, i, j, k, t);
+  for (int e = 0; e < 3; ++e) {
+    const double v = half_un(G, ns, u, t[1][e]) - half_un(G, ns, u, t[0][e]);
+    const double dv = half_un(G, ns, du, t[1][e]) - half_un(G, ns, du, t[0][e]);
+    const int sl = v >= 0.0 ? s : up[e];
+    dflx[e * ns + s] = dv * h3[sl] + v * dh3[sl];
+  }
+}
+
+// AD of k_tl_upwind_flux: consumes the flux adjoint (cleared on every slot)
+__global__ void k_ad_upwind_flux(Geo G, const double* __restrict__ u, const double* __restrict__ h3,
+                                 double* __restrict__ flxa, double* __restrict__ ua, double* __restrict__ h3a) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) flxa[e * ns + s] = 0.0;
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  HalfTerm t[2][3];
+  half_terms(iim, jjm, i, j, k, t);
+  for (int e = 0; e < 3; ++e) {
+    const double g = flxa[e * ns + s];
+    flxa[e * ns + s] = 0.0;
+    const double v = half_un(G, ns, u, t[1][e]) - half_un(G, ns, u, t[0][e]);
+    const int sl = v >= 0.0 ? s : up[e];
+    aadd(&h3a[sl], g * v);
+    const double gv = g * h3[sl];
+    half_un_ad(G, ns, gv, t[1][e], ua);
+    half_un_ad(G, ns, -gv, t[0][e], ua);
+  }
+}
+
+// AD of k_mass_final (mflux = flx + anti, h4 = mss_low - dt div(anti) at the
+// real cells, 0 elsewhere): consumes the adjoints of mflux and h4
+__global__ void k_ad_fct_final(Geo G, double dt, double* __restrict__ Fa, double* __restrict__ h4a,
+                               double* __restrict__ flxa, double* __restrict__ antia, double* __restrict__ mla) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  for (int e = 0; e < 3; ++e) {
+    const double g = Fa[e * ns + s];
+    Fa[e * ns + s] = 0.0;
+    flxa[e * ns + s] += g;
+    aadd(&antia[e * ns + s], g);
+  }
+  const double g = h4a[s];
+  h4a[s] = 0.0;
+  const bool real = interior(c, iim, jjm) || (c.i == 1 && c.j == jjm - 1) || (c.i == iim - 1 && c.j == 1);
+  if (!real) return;
+  mla[s] += g;
+  div_edge_ad(iim, jjm, ns, c, G.area_inv[s], -dt * g, antia);
+}
+
+// AD of the antidiffusive flux and low-order mass of k_mass_low_beta
+// (anti = w3 m3 + w2 m2 + w1 m1 - flx on every slot; mss_low = h3 - dt
+// div(flx) at the real cells): consumes the adjoints of anti and mss_low,
+// adds into the three flux-history slots, flx and h3
+__global__ void k_ad_fct_low(Geo G, double dt, double w3, double w2, double w1, double* __restrict__ antia,
+                             double* __restrict__ mla, double* __restrict__ m3a, double* __restrict__ m2a,
+                             double* __restrict__ m1a, double* __restrict__ flxa, double* __restrict__ h3a) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  for (int e = 0; e < 3; ++e) {
+    const double g = antia[e * ns + s];
+    antia[e * ns + s] = 0.0;
+    m3a[e * ns + s] += w3 * g;
+    m2a[e * ns + s] += w2 * g;
+    m1a[e * ns + s] += w1 * g;
+    aadd(&flxa[e * ns + s], -g);
+  }
+  const double g = mla[s];
+  mla[s] = 0.0;
+  const bool real = interior(c, iim, jjm) || (c.i == 1 && c.j == jjm - 1) || (c.i == iim - 1 && c.j == 1);
+  if (!real) return;
+  aadd(&h3a[s], g);
+  div_edge_ad(iim, jjm, ns, c, G.area_inv[s], -dt * g, flxa);
+}
+
 #undef UPL
 #undef NPL
 #undef DPL
