@@ -314,8 +314,11 @@ swm_status swm_budgets(swm_ctx* ctx, double* out, int nout);
 swm_status swm_total_mass(swm_ctx* ctx, double* mass);
 
 /* ------------------------------------------------------------------------
- * Tangent-linear and adjoint model of a window of FP64 steps with centred
- * advection (step.F:114-212 with get_centered_flux).  Hand-written TL and
+ * Tangent-linear and adjoint model of a window of FP64 steps (step.F:
+ * 114-212) with any of the three advection schemes: centred, upstream, or
+ * flux-corrected transport (mass_fct.F) linearised with the limiter's
+ * coefficients and the upstream directions frozen at the base trajectory.
+ * Tracers are not part of it (they do not feed back).  Hand-written TL and
  * AD kernels cover every operator of the step; halo fills transpose to
  * accumulations into their sources; the elliptic solve's transpose reuses
  * the solver through the symmetry of area times Laplacian.  The trajectory
@@ -333,7 +336,7 @@ swm_status swm_init_vorticity(swm_ctx* ctx, const double* zeta, const double* de
 
 /* Runs nsteps steps from a freshly initialised state (step count 0),
  * keeping snapshots every `every` steps: the base of swm_tl_run /
- * swm_adj_run.  FP64 with centred advection only. */
+ * swm_adj_run.  FP64, single-member contexts. */
 swm_status swm_adj_forward(swm_ctx* ctx, int nsteps, int every);
 
 /* Tangent-linear model of the window.  mode 0: in0..in3 the perturbation
