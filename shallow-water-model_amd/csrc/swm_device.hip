@@ -318,7 +318,8 @@ void relax_once(swm_ctx* C, View<R>& V, int l, const R* a, R* out, const R* beta
 // second leaves the result in the level's scratch.  Returns the finest
 // level's result (alpha, or that scratch).
 template <class R>
-R* vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_zero = false, int fine_sweeps = 2) {
+R* vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_zero = false, int fine_sweeps = 2,
+          bool pre_done = false, size_t pre_off = 0) {
   const int G = C->g;
   const int top = G < swm::dev::CO_MAXLEV ? G : swm::dev::CO_MAXLEV;   // levels 1..top in one workgroup
   auto A = [&](int l) { return l == G ? alpha : V.lev[l - 1].alpha; };
@@ -349,7 +350,12 @@ R* vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_ze
                            C->stream, V.lev[l - 1].L, V.lev[l - 2].L, fa, B(l), B(l - 1));
       continue;
     }
-    relax<R>(C, V, l, 2, A(l), B(l), nvar, l < G || top_zero);
+    if (l == G && top_zero && pre_done)
+      // the first sweep from zero is in the level's scratch already (the
+      // defect kernel wrote it): the second one
+      relax_once<R>(C, V, G, V.lev[G - 1].temp + pre_off, alpha, B(G), nvar, false);
+    else
+      relax<R>(C, V, l, 2, A(l), B(l), nvar, l < G || top_zero);
     if (nvar == 2)
       hipLaunchKernelGGL((swm::dev::k_inject<R, 2>), zgrid(C, nblk_b(V.lev[l - 2].ns, C->blk_mg)), dim3(C->blk_mg), 0, C->stream,
                          V.lev[l - 1].L, V.lev[l - 2].L, A(l), B(l), B(l - 1));
@@ -563,6 +569,11 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
   }
   *bsum_out = bsum;
   const double total = C->total_area;
+  // With the reference's two finest sweeps, each defect pass also forms the
+  // next correction cycle's first sweep (from zero) in the finest level's
+  // scratch, field v in plane v, and the cycle starts at its second sweep.
+  const float* fine_lwt = (C->mixed_fine_sweeps == 2 && C->g > swm::dev::CO_MAXLEV) ? V.lev[C->g - 1].lwt : nullptr;
+  auto first_e = [&](int, size_t o) -> float* { return fine_lwt ? V.lev[C->g - 1].temp + o : nullptr; };
   // the extrapolated start goes to xalt; then the iterates alternate between
   // xalt and xprev, whose content is no longer needed once it is read
   double* blk[3] = {xout, C->xalt, C->xprev};
@@ -571,35 +582,40 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
   // the mean change of the extrapolated start still to be taken off, per field
   bool shift[2] = {false, false};
   const double inv_area = 1.0 / C->total_area;
-  auto launch_defect = [&](int nv, int v0) {
+  auto launch_defect = [&](int nv, int v0, bool last) {
     const size_t o = (size_t)v0 * ns;
     const double* gg = shift[v0] ? C->gauge + v0 : nullptr;
+    float* e0 = last ? nullptr : first_e(v0, o);
     if (nv == 2)
       hipLaunchKernelGGL((swm::dev::k_defect<2, true>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, cur[0], e, oth[0],
-                         rhs, r, C->part_slots, gg, inv_area, bsum, total);
+                         rhs, r, C->part_slots, gg, inv_area, bsum, total, fine_lwt, e0);
     else
       hipLaunchKernelGGL((swm::dev::k_defect<1, true>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, cur[v0], e + o,
-                         oth[v0], rhs + o, r + o, C->part_slots, gg, inv_area, bsum ? bsum + v0 : nullptr, total);
+                         oth[v0], rhs + o, r + o, C->part_slots, gg, inv_area, bsum ? bsum + v0 : nullptr, total,
+                         fine_lwt, e0);
     for (int v = v0; v < v0 + nv; ++v) {
       std::swap(cur[v], oth[v]);
       shift[v] = false;
     }
   };
   // one FP32 correction cycle on fields v0.. (nv of them: both, or one);
-  // the correction lands in e or, with one finest sweep, in that level's
-  // scratch (same plane layout)
-  auto cycle = [&](int nv, int v0) {
+  // the correction lands in e or, with another finest sweep count, in that
+  // level's scratch (same plane layout).  last: no further cycle follows
+  // (the fixed schedule's last one), so the defect need not form the next
+  // cycle's first sweep
+  auto cycle = [&](int nv, int v0, bool last) {
     // the two fields' planes are contiguous: cur[0] + ns == cur[1] holds while they move together
-    float* res = vcycle<float>(C, V, nv, e + (size_t)v0 * ns, r + (size_t)v0 * ns, true, C->mixed_fine_sweeps);
+    float* res = vcycle<float>(C, V, nv, e + (size_t)v0 * ns, r + (size_t)v0 * ns, true, C->mixed_fine_sweeps,
+                               fine_lwt != nullptr, (size_t)v0 * ns);
     e = res - (size_t)v0 * ns;
     if (nv == 2) gather<float>(C, C->face, {e, e + ns});
     else gather<float>(C, C->face, {e + (size_t)v0 * ns});
-    launch_defect(nv, v0);
+    launch_defect(nv, v0, last);
     e = C->mx_e;
   };
   const bool extrap = C->nsol >= 2 && C->mixed_extrapolate;
   hipLaunchKernelGGL(swm::dev::k_defect_extrap<2>, zgrid(C, nb), dim3(BLK), 0, C->stream, G64, C->area_mask, blk[0],
-                     extrap ? blk[2] : blk[0], blk[1], rhs, r, C->part_slots, bsum, total);
+                     extrap ? blk[2] : blk[0], blk[1], rhs, r, C->part_slots, bsum, total, fine_lwt, first_e(0, 0));
   if (extrap) {
     sum_final(C, C->part_slots + 2 * (size_t)nb, (int)nb, C->gauge, 2);
     shift[0] = shift[1] = true;
@@ -610,12 +626,12 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
     // the fixed schedule: no host-side test; the last defect's mean squares
     // go to the member's residual record on the device
     iter = C->nsol >= 2 ? C->fixed_cycles : std::max(C->fixed_cycles, C->fixed_start);
-    for (int it = 0; it < iter; ++it) cycle(2, 0);
+    for (int it = 0; it < iter; ++it) cycle(2, 0, it == iter - 1);
     sum_final(C, C->part_slots, (int)nb, C->sums, 2);
     hipLaunchKernelGGL(swm::dev::k_track_resid, zgrid(C, 1), dim3(64), 0, C->stream, C->sums, 1.0 / (double)C->max_ig,
                        C->solver_tol, C->track, mb);
   } else {
-    for (int it = 0; it < C->mixed_min_cycles; ++it) cycle(2, 0);
+    for (int it = 0; it < C->mixed_min_cycles; ++it) cycle(2, 0, false);
     const int M = C->nz;
     std::vector<double> ms(2 * (size_t)M);
     swm_status st = member_norms(C, C->part_slots, (int)nb, {0, 1}, ms.data());
@@ -638,7 +654,7 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
         oth[1] = cur[1];
         cur[1] = cur[0] + ns;
       }
-      cycle(nv, v0);
+      cycle(nv, v0, false);
       st = member_norms(C, C->part_slots, (int)nb, bad, ms.data());
       if (st) return st;
       any = false;

@@ -829,6 +829,15 @@ __device__ __forceinline__ float rhs_less_mean(float b, const double* __restrict
   return bsum ? b - (float)(bsum[v] / total) : b;
 }
 
+// the first smoothing sweep of the next correction cycle, from zero, on the
+// defect d just formed: k_relax<float, NV, true> at the finest level gives
+// 0.8 * (wt * d) at the real slots and 0 elsewhere, bit for bit the same
+__device__ __forceinline__ float first_sweep(bool real, const float* __restrict__ lwt, int s, float d) {
+  if (!real) return 0.0f;
+  const float tt = lwt[s] * d;
+  return (1.0f - 0.8f) * 0.0f + 0.8f * tt;
+}
+
 // Mixed path, FP64 defect of the Poisson problem for NV fields, one slot per
 // thread, the FP64 weights read once: d = rhs - L(x') at real cells (0
 // elsewhere), x' = x + e with CORR (the FP32 correction e, halos filled;
@@ -841,10 +850,12 @@ __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict_
                                                 double* __restrict__ xn, const float* __restrict__ b,
                                                 float* __restrict__ r, double* __restrict__ part,
                                                 const double* __restrict__ gauge, double inv_area,
-                                                const double* __restrict__ bsum, double total) {
+                                                const double* __restrict__ bsum, double total,
+                                                const float* __restrict__ lwt, float* __restrict__ e0) {
   at_member(G.mb, x, e, xn, b, r, part);
   if (gauge) at_member(G.mb, gauge);
   if (bsum) at_member(G.mb, bsum);
+  if (e0) at_member(G.mb, e0);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   double acc[NV];
@@ -863,6 +874,7 @@ __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict_
       const double d = m != 0.0 ? (double)rhs_less_mean(b[(size_t)v * ns + s], bsum, v, total) - w : 0.0;
       if (CORR) xn[(size_t)v * ns + s] = gauge ? X(s) - gauge[v] * inv_area : X(s);
       r[(size_t)v * ns + s] = (float)d;
+      if (e0) e0[(size_t)v * ns + s] = first_sweep(st.real, lwt, s, (float)d);
       acc[v] = (d * d) * m;
     }
   }
@@ -883,9 +895,11 @@ __global__ void __launch_bounds__(256) k_defect_extrap(Geo G, const double* __re
                                                        const double* __restrict__ x, const double* __restrict__ xp,
                                                        double* __restrict__ xn, const float* __restrict__ b,
                                                        float* __restrict__ r, double* __restrict__ part,
-                                                       const double* __restrict__ bsum, double total) {
+                                                       const double* __restrict__ bsum, double total,
+                                                       const float* __restrict__ lwt, float* __restrict__ e0) {
   at_member(G.mb, x, xp, xn, b, r, part);
   if (bsum) at_member(G.mb, bsum);
+  if (e0) at_member(G.mb, e0);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   double acc[2 * NV];
@@ -904,6 +918,7 @@ __global__ void __launch_bounds__(256) k_defect_extrap(Geo G, const double* __re
       const double d = m != 0.0 ? (double)rhs_less_mean(b[(size_t)v * ns + s], bsum, v, total) - w : 0.0;
       xn[(size_t)v * ns + s] = X(s);
       r[(size_t)v * ns + s] = (float)d;
+      if (e0) e0[(size_t)v * ns + s] = first_sweep(st.real, lwt, s, (float)d);
       acc[v] = (d * d) * m;
       acc[NV + v] = am[s] * (xv[s] - pv[s]);
     }
