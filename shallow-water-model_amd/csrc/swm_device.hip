@@ -700,14 +700,20 @@ void weights(swm_ctx* C) {    // model_control.F:58-89
 }
 
 template <class R>
-void wrap_prognostics(swm_ctx* C, View<R>& V) {   // step.F:438-493
+// after_step: the step has just filled the halos of the new velocity, of
+// the kinetic energy and (FCT) of the new thickness, and left the old
+// velocity level as it was; a halo fill copies values, so filling those
+// again changes nothing and is left out
+void wrap_prognostics(swm_ctx* C, View<R>& V, bool after_step = false) {   // step.F:438-493
   const int n4 = C->n4 - 1;
   const int ns = C->ns;
   // inv_h (step.F:448-452) is not formed: nothing on the path reads it
-  std::vector<R*> f = {V.h[n4], V.eta[n4], V.div[n4], V.relative, V.ke};
+  std::vector<R*> f = {V.eta[n4], V.div[n4], V.relative};
+  if (!after_step || C->advection != SWM_ADV_FCT) f.push_back(V.h[n4]);
+  if (!after_step) f.push_back(V.ke);
   for (int q = 0; q < C->nt; ++q) f.push_back(V.tr[n4][q]);
   gather_list<R>(C, C->face, f);
-  gather<R>(C, C->vrtx, {V.u[0], V.u[0] + 2 * (size_t)ns, V.u[1], V.u[1] + 2 * (size_t)ns});
+  if (!after_step) gather<R>(C, C->vrtx, {V.u[0], V.u[0] + 2 * (size_t)ns, V.u[1], V.u[1] + 2 * (size_t)ns});
 }
 
 // step.F:114-212 step_dt (FCT scheme, get_fct_flux step.F:611-678)
@@ -1548,11 +1554,11 @@ swm_status step_once(swm_ctx* C) {
   if (C->precision == SWM_MIXED) {
     st = step_dt<float>(C, C->m);
     if (st) return st;
-    wrap_prognostics<float>(C, C->m);
+    wrap_prognostics<float>(C, C->m, true);
   } else {
     st = step_dt<double>(C, C->d);
     if (st) return st;
-    wrap_prognostics<double>(C, C->d);
+    wrap_prognostics<double>(C, C->d, true);
   }
   return SWM_OK;
 }
