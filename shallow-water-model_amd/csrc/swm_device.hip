@@ -1114,6 +1114,8 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
           *tng = tf(12 * ns), *vw = tf(12 * ns), *sw = tf(6 * ns), *lw = tf(7 * ns), *f = tf(ns), *hs = tf(ns),
           *mask = tf(ns);
     g = swm::dev::GeoT<float>{C->iim, C->jjm, C->ns, area, ai, cai, sa, dc, nrm, tng, vw, sw, lw, f, hs, mask, 0.0f, f};
+    g.ndc = tf(12 * ns);
+    g.tdc = tf(12 * ns);
     layout_view<float>(C, C->m, &C->d);
     C->mx_r = C->marena.take<float>(2 * ns);
     C->mx_e = C->marena.take<float>(2 * ns);
@@ -1282,6 +1284,8 @@ swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
     cv(C->sub_area, g.sub_area, 6 * (size_t)ns); cv(C->dc, g.dc, 6 * (size_t)ns); cv(C->nrm, g.nrm, 12 * (size_t)ns);
     cv(C->tng, g.tng, 12 * (size_t)ns); cv(C->vw, g.vw, 12 * (size_t)ns); cv(C->sw, g.sw, 6 * (size_t)ns);
     cv(C->lw, g.lw, 7 * (size_t)ns); cv(C->f, g.f, ns); cv(C->mask, g.mask, ns);
+    hipLaunchKernelGGL(swm::dev::k_premul_dc, dim3(nblk(ns)), dim3(BLK), 0, S, ns, C->nrm, C->dc, const_cast<float*>(g.ndc));
+    hipLaunchKernelGGL(swm::dev::k_premul_dc, dim3(nblk(ns)), dim3(BLK), 0, S, ns, C->tng, C->dc, const_cast<float*>(g.tdc));
     for (int l = 1; l <= C->g; ++l) {
       const LevelT<double>& L = C->d.lev[l - 1];
       cv(L.lw, C->m.lev[l - 1].lw, 7 * (size_t)L.ns);

@@ -26,6 +26,9 @@ struct GeoT {
   R h0;
   const R* fo;
   size_t mb;
+  // mixed path: normal and tangent planes times the half-edge lengths,
+  // (kk, a, ic) planes as nrm / tng (formed in FP64, stored in R)
+  const R *ndc, *tdc;
 };
 using Geo = GeoT<double>;
 

@@ -157,6 +157,43 @@ __device__ __forceinline__ Half<R> half_edges(int iim, int jjm, int ns, int i, i
   return h;
 }
 
+// half_edges for the mixed path from premultiplied planes nd = n * dc (or
+// t * dc): r = (u . nd) * mc, or (u . nd) without the corner mass
+template <class R>
+__device__ __forceinline__ Half<R> half_edges_pre(int iim, int jjm, int ns, int i, int j, int k,
+                                                  const R* __restrict__ nd, const R* __restrict__ mc,
+                                                  const R* __restrict__ u) {
+#define U(sp, ic, s_) u[((ic) + 2 * (sp)) * ns + (s_)]
+#define N(kk, a, ic, s_) nd[((kk) + 3 * ((a) + 2 * (ic))) * ns + (s_)]
+#define HN(s_, kk, ic) (U(0, ic, s_) * N(kk, 0, ic, s_) + U(1, ic, s_) * N(kk, 1, ic, s_))
+  Half<R> h;
+  const int s0 = SL(i, j, k), sn = SL(i, j + 1, k), se = SL(i + 1, j, k);
+  h.r[0][0] = HN(sn, 1, 1); h.r[1][0] = HN(s0, 0, 0); h.r[0][1] = HN(s0, 2, 0);
+  h.r[1][1] = HN(s0, 0, 1); h.r[0][2] = HN(s0, 2, 1); h.r[1][2] = HN(se, 1, 0);
+  if (mc) {
+    h.r[0][0] = h.r[0][0] * mc[ns + sn]; h.r[1][0] = h.r[1][0] * mc[s0]; h.r[0][1] = h.r[0][1] * mc[s0];
+    h.r[1][1] = h.r[1][1] * mc[ns + s0]; h.r[0][2] = h.r[0][2] * mc[ns + s0]; h.r[1][2] = h.r[1][2] * mc[se];
+  }
+#undef U
+#undef N
+#undef HN
+  return h;
+}
+
+// premultiplied metric planes of the mixed path: out(p) = nrm(p) * dc(kk, ic)
+// for the plane p = kk + 3 (a + 2 ic), formed in FP64
+__global__ void k_premul_dc(int ns, const double* __restrict__ nrm, const double* __restrict__ dc,
+                            float* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  for (int ic = 0; ic < 2; ++ic)
+    for (int a = 0; a < 2; ++a)
+      for (int kk = 0; kk < 3; ++kk) {
+        const int p = kk + 3 * (a + 2 * ic);
+        out[(size_t)p * ns + s] = (float)(nrm[(size_t)p * ns + s] * dc[(size_t)(kk + 3 * ic) * ns + s]);
+      }
+}
+
 // upstream (first-order) and centred (high-order tendency) edge mass fluxes
 // (mass_fct.F:151-214 via operators.F:1626-1832)
 template <class R>
@@ -172,8 +209,14 @@ __global__ void k_mass_fluxes(GeoT<R> G, const R* __restrict__ u, const R* __res
     return;
   }
   const int i = c.i, j = c.j, k = c.k;
-  const Half<R> hc_ = half_edges<R>(iim, jjm, ns, i, j, k, G.dc, G.nrm, hc, u);
-  const Half<R> hu = half_edges<R>(iim, jjm, ns, i, j, k, G.dc, G.nrm, nullptr, u);
+  Half<R> hc_, hu;
+  if constexpr (sizeof(R) == 4) {
+    hc_ = half_edges_pre<R>(iim, jjm, ns, i, j, k, G.ndc, hc, u);
+    hu = half_edges_pre<R>(iim, jjm, ns, i, j, k, G.ndc, nullptr, u);
+  } else {
+    hc_ = half_edges<R>(iim, jjm, ns, i, j, k, G.dc, G.nrm, hc, u);
+    hu = half_edges<R>(iim, jjm, ns, i, j, k, G.dc, G.nrm, nullptr, u);
+  }
   const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
   const R hs0 = habs(G, h3[s]);
   for (int e = 0; e < 3; ++e) {
@@ -497,7 +540,9 @@ __global__ void k_vort_curl(GeoT<R> G, const R* __restrict__ mflux, const R* __r
   };
   const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
   const R pv0 = PV(s);
-  const Half<R> ht = half_edges<R>(iim, jjm, ns, i, j, k, G.dc, G.tng, hc, u);
+  Half<R> ht;
+  if constexpr (sizeof(R) == 4) ht = half_edges_pre<R>(iim, jjm, ns, i, j, k, G.tdc, hc, u);
+  else ht = half_edges<R>(iim, jjm, ns, i, j, k, G.dc, G.tng, hc, u);
   for (int e = 0; e < 3; ++e) {
     const R pvu = PV(up[e]);
     vf[e * ns + s] = R(0.5) * (mflux[e * ns + s]) * (pv0 + pvu);
