@@ -770,8 +770,11 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   }
   hipLaunchKernelGGL(swm::dev::k_mass_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.beta, V.anti);
   gather<R>(C, C->edge, {V.anti});
+  // with the first tracer's edge fluxes (k_tr_fluxes fused)
+  swm::dev::TrFluxArgs<R> tf0{};
+  if (C->nt > 0) tf0 = {V.tr[n3][0], V.tr[n4][0], V.told[0], V.f1, V.f2, w3, w2, w1};
   hipLaunchKernelGGL(swm::dev::k_mass_final<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, V.flx1, V.anti, V.mss_low,
-                     V.mass_flux, V.h[n4]);
+                     V.mass_flux, V.h[n4], tf0);
   // the tracer limiter fills the halo of the new mass in place (tracer_fct.F:344)
   gather<R>(C, C->face, {V.h[n4]});
   }
@@ -788,8 +791,9 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   for (int q = 0; q < (fct ? C->nt : 0); ++q) {
     R* t3 = V.tr[n3][q];
     R* t4 = V.tr[n4][q];
-    hipLaunchKernelGGL(swm::dev::k_tr_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, w3, w2, w1, t3, t4, V.told[q],
-                       V.mass_flux, V.f1, V.f2);
+    if (q > 0)
+      hipLaunchKernelGGL(swm::dev::k_tr_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, w3, w2, w1, t3, t4,
+                         V.told[q], V.mass_flux, V.f1, V.f2);
     gather<R>(C, C->edge, {V.f1, V.f2});
     hipLaunchKernelGGL(swm::dev::k_tr_low<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, V.h[n3], t3, V.f1, V.ml);
     gather<R>(C, C->face, {V.ml});
@@ -2345,7 +2349,7 @@ swm_status tl_step(swm_ctx* C, int q) {
     hipLaunchKernelGGL(swm::dev::k_fct_scale, dim3(nblk(3 * ns)), dim3(BLK), 0, S, 3 * ns, seg_c(C, q), T->an);
     gather<double>(C, C->edge, {T->an});
     hipLaunchKernelGGL(swm::dev::k_mass_final<double>, dim3(nb), dim3(BLK), 0, S, G, dt, T->fl, T->an, T->ml, T->F,
-                       T->h[n4]);
+                       T->h[n4], swm::dev::TrFluxArgs<double>{});
     gather<double>(C, C->face, {T->h[n4]});
   } else {
     if (C->advection == SWM_ADV_CENTERED)

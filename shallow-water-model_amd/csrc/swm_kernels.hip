@@ -301,21 +301,68 @@ __global__ void k_mass_anti(GeoT<R> G, const R* __restrict__ beta, R* __restrict
   }
 }
 
+// tracer FCT, first- and high-order edge fluxes (tracer_fct.F:166-243) of
+// one slot from the total mass flux F of its edges; the high-order tracer is
+// the AB3 extrapolation of step.F:657-660
+template <class R>
+__device__ __forceinline__ void tr_fluxes_at(int iim, int jjm, int ns, const IJK& c, int s, R w3, R w2, R w1,
+                                             const R* __restrict__ t3, const R* __restrict__ t4,
+                                             const R* __restrict__ told, const R (&F)[3], R* __restrict__ f1,
+                                             R* __restrict__ f2) {
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) { f1[e * ns + s] = R(0); f2[e * ns + s] = R(0); }
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  auto TH = [&](int sl) { return w3 * t3[sl] + w2 * t4[sl] + w1 * told[sl]; };
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  const R ths = TH(s);
+  for (int e = 0; e < 3; ++e) {
+    f1[e * ns + s] = t3[s] * rmax(R(0), F[e]) + t3[up[e]] * rmin(R(0), F[e]);
+    f2[e * ns + s] = R(0.5) * (ths + TH(up[e])) * F[e];
+  }
+}
+
+// the first tracer's fluxes, fused into k_mass_final (t3 == nullptr: none)
+template <class R>
+struct TrFluxArgs {
+  const R *t3, *t4, *told;
+  R *f1, *f2;
+  R w3, w2, w1;
+};
+
 // flux-corrected mass and the total mass flux (mass_fct.F:121-128, :478-532)
 template <class R>
 __global__ void k_mass_final(GeoT<R> G, R dt, const R* __restrict__ flx1, const R* __restrict__ anti,
-                             const R* __restrict__ mss_low, R* __restrict__ mflux, R* __restrict__ h4) {
+                             const R* __restrict__ mss_low, R* __restrict__ mflux, R* __restrict__ h4,
+                             TrFluxArgs<R> T) {
   at_member(G.mb, flx1, anti, mss_low, mflux, h4);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
-  for (int e = 0; e < 3; ++e) mflux[e * ns + s] = flx1[e * ns + s] + anti[e * ns + s];
+  R F[3];
+  for (int e = 0; e < 3; ++e) {
+    F[e] = flx1[e * ns + s] + anti[e * ns + s];
+    mflux[e * ns + s] = F[e];
+  }
   bool set;
   h4[s] = edge_update<R>(iim, jjm, ns, c, mss_low[s], dt * G.area_inv[s], anti, set);
+  // the first tracer's edge fluxes take the total flux of the slot's own
+  // edges only, which this slot has just formed (k_tr_fluxes fused)
+  if (T.t3) {
+    const R* t3 = T.t3;
+    const R* t4 = T.t4;
+    const R* told = T.told;
+    R* f1 = T.f1;
+    R* f2 = T.f2;
+    at_member(G.mb, t3, t4, told, f1, f2);
+    tr_fluxes_at<R>(iim, jjm, ns, c, s, T.w3, T.w2, T.w1, t3, t4, told, F, f1, f2);
+  }
 }
 
-// tracer FCT, first- and high-order edge fluxes (tracer_fct.F:166-243); the
+// tracer FCT fluxes of a tracer over all slots (k_mass_final forms the
+// first tracer's itself); tracer_fct.F:166-243, the
 // high-order tracer is the AB3 extrapolation of step.F:657-660
 template <class R>
 __global__ void k_tr_fluxes(GeoT<R> G, R w3, R w2, R w1, const R* __restrict__ t3, const R* __restrict__ t4,
@@ -326,19 +373,8 @@ __global__ void k_tr_fluxes(GeoT<R> G, R w3, R w2, R w1, const R* __restrict__ t
   const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
-  if (!interior(c, iim, jjm)) {
-    for (int e = 0; e < 3; ++e) { f1[e * ns + s] = R(0); f2[e * ns + s] = R(0); }
-    return;
-  }
-  const int i = c.i, j = c.j, k = c.k;
-  auto TH = [&](int sl) { return w3 * t3[sl] + w2 * t4[sl] + w1 * told[sl]; };
-  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
-  const R ths = TH(s);
-  for (int e = 0; e < 3; ++e) {
-    const R F = mflux[e * ns + s];
-    f1[e * ns + s] = t3[s] * rmax(R(0), F) + t3[up[e]] * rmin(R(0), F);
-    f2[e * ns + s] = R(0.5) * (ths + TH(up[e])) * F;
-  }
+  const R F[3] = {mflux[s], mflux[ns + s], mflux[2 * ns + s]};
+  tr_fluxes_at<R>(iim, jjm, ns, c, s, w3, w2, w1, t3, t4, told, F, f1, f2);
 }
 
 // tracer_fct.F:251-321 set_msstrc_low
