@@ -82,7 +82,7 @@ struct View {
   R *eta_f[3] = {}, *div_f[3] = {}, *msh_f[3] = {};
   R* h_f[3] = {};                 // thickness tendencies (centred / upwind advection)
   std::vector<R*> tr_f[3], tflux;  // mass-times-tracer tendencies and tracer edge fluxes
-  R *relative = nullptr, *ke = nullptr, *inv_h = nullptr, *mass_flux = nullptr;
+  R *relative = nullptr, *ke = nullptr, *tlow = nullptr, *mass_flux = nullptr;   // tlow: low-order tracer (FCT)
   R *hc = nullptr, *flx1 = nullptr, *anti = nullptr, *mss_low = nullptr, *beta = nullptr, *f1 = nullptr,
     *f2 = nullptr, *ml = nullptr, *bdn = nullptr, *bup = nullptr, *vf = nullptr, *dcurl = nullptr, *rhs = nullptr;
   R* state_begin = nullptr;   // state and scratch are contiguous from here
@@ -795,10 +795,11 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
       hipLaunchKernelGGL(swm::dev::k_tr_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, w3, w2, w1, t3, t4,
                          V.told[q], V.mass_flux, V.f1, V.f2);
     gather<R>(C, C->edge, {V.f1, V.f2});
-    hipLaunchKernelGGL(swm::dev::k_tr_low<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, V.h[n3], t3, V.f1, V.ml);
-    gather<R>(C, C->face, {V.ml});
-    hipLaunchKernelGGL(swm::dev::k_tr_limiter<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, t3, V.ml, V.h[n4], V.mss_low,
-                       V.f1, V.f2, V.anti, V.bdn, V.bup);
+    hipLaunchKernelGGL(swm::dev::k_tr_low<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, V.h[n3], t3, V.f1,
+                       V.h[n4], V.ml, V.tlow);
+    gather<R>(C, C->face, {V.ml, V.tlow});
+    hipLaunchKernelGGL(swm::dev::k_tr_limiter<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, t3, V.ml, V.tlow,
+                       V.mss_low, V.f1, V.f2, V.anti, V.bdn, V.bup);
     gather<R>(C, C->face, {V.bdn, V.bup});
     hipLaunchKernelGGL(swm::dev::k_tr_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, V.bdn, V.bup, V.anti);
     gather<R>(C, C->edge, {V.anti});
@@ -953,7 +954,6 @@ swm_status finish_init(swm_ctx* C) {
     }
     to32(C, D.relative, M.relative, ns);
     to32(C, D.ke, M.ke, ns);
-    to32(C, D.inv_h, M.inv_h, ns);
   }
   HIP_TRY(hipStreamSynchronize(S));
   HIP_TRY(hipGetLastError());
@@ -1079,7 +1079,7 @@ void layout_view(swm_ctx* C, View<R>& V, const View<double>* maps) {
   }
   V.tflux.clear();
   for (int q = 0; q < C->nt; ++q) V.tflux.push_back(take(3 * ns));
-  V.relative = take(ns); V.ke = take(ns); V.inv_h = take(ns); V.mass_flux = take(3 * ns);
+  V.relative = take(ns); V.ke = take(ns); V.tlow = take(ns); V.mass_flux = take(3 * ns);
   V.hc = take(2 * ns); V.flx1 = take(3 * ns); V.anti = take(3 * ns); V.mss_low = take(ns);
   V.beta = take(ns); V.f1 = take(3 * ns); V.f2 = take(3 * ns); V.ml = take(ns); V.bdn = take(ns); V.bup = take(ns);
   V.vf = take(3 * ns); V.dcurl = take(3 * ns); V.rhs = take(2 * ns);
@@ -1145,7 +1145,7 @@ void shift_view(View<R>& V, ptrdiff_t d) {
     for (R*& p : V.tr_f[t]) shift_ptr(p, d);
   }
   for (R*& p : V.tflux) shift_ptr(p, d);
-  for (R** p : {&V.relative, &V.ke, &V.inv_h, &V.mass_flux, &V.hc, &V.flx1, &V.anti, &V.mss_low, &V.beta, &V.f1,
+  for (R** p : {&V.relative, &V.ke, &V.tlow, &V.mass_flux, &V.hc, &V.flx1, &V.anti, &V.mss_low, &V.beta, &V.f1,
                 &V.f2, &V.ml, &V.bdn, &V.bup, &V.vf, &V.dcurl, &V.rhs, &V.state_begin})
     shift_ptr(*p, d);
 }

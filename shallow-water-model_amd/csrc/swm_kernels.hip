@@ -377,26 +377,30 @@ __global__ void k_tr_fluxes(GeoT<R> G, R w3, R w2, R w1, const R* __restrict__ t
   tr_fluxes_at<R>(iim, jjm, ns, c, s, w3, w2, w1, t3, t4, told, F, f1, f2);
 }
 
-// tracer_fct.F:251-321 set_msstrc_low
+// tracer_fct.F:251-321 set_msstrc_low, and the low-order tracer ml / h4 (0
+// where h4 <= 0), which the limiter takes at the cell and its neighbours
 template <class R>
 __global__ void k_tr_low(GeoT<R> G, R dt, const R* __restrict__ h3, const R* __restrict__ t3, const R* __restrict__ f1,
-                         R* __restrict__ ml) {
-  at_member(G.mb, h3, t3, f1, ml);
+                         const R* __restrict__ h4, R* __restrict__ ml, R* __restrict__ tlow) {
+  at_member(G.mb, h3, t3, f1, h4, ml, tlow);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   bool set;
-  ml[s] = edge_update<R>(iim, jjm, ns, c, habs(G, h3[s]) * t3[s], dt * G.area_inv[s], f1, set);
+  const R m = edge_update<R>(iim, jjm, ns, c, habs(G, h3[s]) * t3[s], dt * G.area_inv[s], f1, set);
+  ml[s] = m;
+  const R hh = habs(G, h4[s]);
+  tlow[s] = hh > R(0) ? m / hh : R(0);
 }
 
 // tracer_fct.F:329-531: local extrema over the cell and its neighbours of the
 // tracer and its low-order update, and the limiter weights beta_dn / beta_up
 template <class R>
 __global__ void k_tr_limiter(GeoT<R> G, R dt, const R* __restrict__ t3, const R* __restrict__ ml,
-                             const R* __restrict__ h4, const R* __restrict__ mss_low, const R* __restrict__ f1,
+                             const R* __restrict__ tlow, const R* __restrict__ mss_low, const R* __restrict__ f1,
                              const R* __restrict__ f2, R* __restrict__ anti, R* __restrict__ bdn, R* __restrict__ bup) {
-  at_member(G.mb, t3, ml, h4, mss_low, f1, f2, anti, bdn, bup);
+  at_member(G.mb, t3, ml, tlow, mss_low, f1, f2, anti, bdn, bup);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -404,10 +408,7 @@ __global__ void k_tr_limiter(GeoT<R> G, R dt, const R* __restrict__ t3, const R*
   const int i = c.i, j = c.j, k = c.k;
   auto A = [&](int e, int sl) { return f2[e * ns + sl] - f1[e * ns + sl]; };
   for (int e = 0; e < 3; ++e) anti[e * ns + s] = A(e, s);
-  auto TL = [&](int sl) {
-    const R hh = habs(G, h4[sl]);
-    return hh > R(0) ? ml[sl] / hh : R(0);
-  };
+  auto TL = [&](int sl) { return tlow[sl]; };
   const int p = iim - 1, q = jjm - 1;
   const bool np = (i == 1 && j == q), sp = (i == p && j == 1);
   const bool in = interior(c, iim, jjm);
