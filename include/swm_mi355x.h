@@ -244,12 +244,14 @@ swm_status swm_set_solver(swm_ctx* ctx, double tol, int relative, int max_cycles
  * counterpart in the reference. */
 swm_status swm_set_mixed_start(swm_ctx* ctx, int extrapolate, int min_cycles);
 
-/* Mixed path: smoothing sweeps before and after the coarse correction on the
- * finest level of the FP32 V-cycles (1 to 4; 2 is the reference's count).
- * The V-cycle only preconditions the FP64 defect correction, so the solution
- * meets the same criterion with any count; fewer sweeps make a cycle cheaper
- * and its residual reduction smaller.  No counterpart in the reference. */
-swm_status swm_set_mixed_smoother(swm_ctx* ctx, int fine_sweeps);
+/* Mixed path: the shape of the FP32 V-cycles, which only precondition the
+ * FP64 defect correction (the solution meets the same criterion with any
+ * shape; a cheaper cycle reduces the residual less).  fine_sweeps: sweeps
+ * before and after the coarse correction on the finest level (1 to 4; 2 is
+ * the reference's count).  half_weights (default on): the finest level's
+ * smoother and restriction read its Laplacian weights in half precision,
+ * scaled, in difference form.  No counterpart in the reference. */
+swm_status swm_set_mixed_smoother(swm_ctx* ctx, int fine_sweeps, int half_weights);
 
 /* Stream overlap in the FCT step (default off): the tracer transport, which
  * reads only the new mass and the mass flux, runs on a second stream of the

@@ -163,6 +163,7 @@ struct swm_ctx {
   int mixed_extrapolate = 1;   // mixed path: start from 2 x_n - x_(n-1)
   int mixed_min_cycles = 1;    // mixed path: cycles before the first check
   int mixed_fine_sweeps = 2;   // mixed path: smoothing sweeps before / after the coarse correction on the finest level
+  int mixed_half = 1;          // mixed path: half-precision weights in the finest level's smoother and restriction
   double last_resid[2] = {0.0, 0.0};
   double last_rhs_ms[2] = {0.0, 0.0};
   std::vector<double> rhs_ms;   // per member and field (relative criterion)
@@ -261,6 +262,12 @@ void sum_final(swm_ctx* C, const double* part, int nb, double* out, int nf) {
 template <class R, int NV>
 void launch_relax(swm_ctx* C, const LevelT<R>& L, bool zero, const R* beta, const R* a, R* t) {
   const dim3 grid = zgrid(C, nblk_b(L.ns, C->blk_mg)), blk(C->blk_mg);
+  if constexpr (sizeof(R) == 4) {
+    if (!zero && L.L.hw && C->mixed_half) {
+      hipLaunchKernelGGL((swm::dev::k_relax_h<NV>), grid, blk, 0, C->stream, L.L, beta, a, t);
+      return;
+    }
+  }
   if (zero) hipLaunchKernelGGL((swm::dev::k_relax<R, NV, true>), grid, blk, 0, C->stream, L.L, beta, a, t);
   else hipLaunchKernelGGL((swm::dev::k_relax<R, NV, false>), grid, blk, 0, C->stream, L.L, beta, a, t);
 }
@@ -356,6 +363,19 @@ R* vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_ze
       relax_once<R>(C, V, G, V.lev[G - 1].temp + pre_off, alpha, B(G), nvar, false);
     else
       relax<R>(C, V, l, 2, A(l), B(l), nvar, l < G || top_zero);
+    bool done = false;
+    if constexpr (sizeof(R) == 4) {
+      if (V.lev[l - 1].L.hw && C->mixed_half && top_zero) {
+        if (nvar == 2)
+          hipLaunchKernelGGL((swm::dev::k_inject_h<2>), zgrid(C, nblk_b(V.lev[l - 2].ns, C->blk_mg)), dim3(C->blk_mg), 0,
+                             C->stream, V.lev[l - 1].L, V.lev[l - 2].L, A(l), B(l), B(l - 1));
+        else
+          hipLaunchKernelGGL((swm::dev::k_inject_h<1>), zgrid(C, nblk_b(V.lev[l - 2].ns, C->blk_mg)), dim3(C->blk_mg), 0,
+                             C->stream, V.lev[l - 1].L, V.lev[l - 2].L, A(l), B(l), B(l - 1));
+        done = true;
+      }
+    }
+    if (done) continue;
     if (nvar == 2)
       hipLaunchKernelGGL((swm::dev::k_inject<R, 2>), zgrid(C, nblk_b(V.lev[l - 2].ns, C->blk_mg)), dim3(C->blk_mg), 0, C->stream,
                          V.lev[l - 1].L, V.lev[l - 2].L, A(l), B(l), B(l - 1));
@@ -1121,6 +1141,11 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
     g.ndc = tf(12 * ns);
     g.tdc = tf(12 * ns);
     layout_view<float>(C, C->m, &C->d);
+    if (C->g > swm::dev::CO_MAXLEV) {
+      LevelT<float>& L = C->m.lev[C->g - 1];
+      L.L.hw = A.take<_Float16>(6 * (size_t)L.ns);
+      L.L.hcw = A.take<_Float16>((size_t)L.ns);
+    }
     C->mx_r = C->marena.take<float>(2 * ns);
     C->mx_e = C->marena.take<float>(2 * ns);
   }
@@ -1294,6 +1319,27 @@ swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
       const LevelT<double>& L = C->d.lev[l - 1];
       cv(L.lw, C->m.lev[l - 1].lw, 7 * (size_t)L.ns);
       cv(L.lwt, C->m.lev[l - 1].lwt, (size_t)L.ns);
+    }
+    if (C->g > swm::dev::CO_MAXLEV) {
+      // half-precision weights of the finest level: l_weights(1..6) scaled
+      // by 1 / their largest magnitude, and 0.8 / (S l_weights(7))
+      LevelT<float>& L = C->m.lev[C->g - 1];
+      const std::vector<double>& lw = Gh.levels[C->g - 1].lap_w;
+      const size_t nl = L.ns;
+      double mx = 0.0;
+      for (size_t n = 0; n < nl; ++n)
+        for (int q = 0; q < 6; ++q) mx = std::max(mx, std::fabs(lw[7 * n + q]));
+      const double S = mx > 0.0 ? 1.0 / mx : 1.0;
+      std::vector<_Float16> hw(6 * nl), hc(nl);
+      for (size_t n = 0; n < nl; ++n) {
+        for (int q = 0; q < 6; ++q) hw[(size_t)q * nl + n] = (_Float16)(S * lw[7 * n + q]);
+        const double w7 = lw[7 * n + 6];
+        hc[n] = (_Float16)(w7 != 0.0 ? 0.8 / (S * w7) : 0.0);
+      }
+      HIP_TRY(hipMemcpy(const_cast<_Float16*>(L.L.hw), hw.data(), hw.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(const_cast<_Float16*>(L.L.hcw), hc.data(), hc.size() * sizeof(_Float16), hipMemcpyHostToDevice));
+      L.L.hs = (float)S;
+      L.L.his = (float)(1.0 / S);
     }
     (void)nb;
   }
@@ -1645,12 +1691,13 @@ swm_status swm_get_field(swm_ctx* C, const char* name, double* out) {
   return SWM_ERR_ARG;
 }
 
-swm_status swm_set_mixed_smoother(swm_ctx* C, int fine_sweeps) {
+swm_status swm_set_mixed_smoother(swm_ctx* C, int fine_sweeps, int half_weights) {
   if (!C || fine_sweeps < 1 || fine_sweeps > 4) {
     swm::set_error("swm_set_mixed_smoother: 1 to 4 sweeps");
     return SWM_ERR_ARG;
   }
   C->mixed_fine_sweeps = fine_sweeps;
+  C->mixed_half = half_weights ? 1 : 0;
   C->epoch += 1;
   return SWM_OK;
 }

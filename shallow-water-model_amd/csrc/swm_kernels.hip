@@ -1103,6 +1103,117 @@ __global__ void __launch_bounds__(256) k_relax(LevT<R> L, const R* __restrict__ 
   }
 }
 
+// Mixed path, finest level: a non-zero weighted Jacobi sweep of k_relax with
+// the half-precision weights of LevT, in difference form (the weights sum to
+// zero): out = x0 + hcw (S beta - sum_k hw_k (x_k - x0)), the same operator
+// as k_relax's to the weights' rounding.  It only smooths inside the FP64
+// defect correction, so that rounding sets no accuracy limit.
+template <int NV>
+__global__ void __launch_bounds__(256) k_relax_h(LevT<float> L, const float* __restrict__ beta,
+                                                 const float* __restrict__ a, float* __restrict__ out) {
+  at_member(L.mb, beta, a, out);
+  const int iim = L.iim, jjm = L.jjm, ns = L.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  const bool in = i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1;
+  const bool np = i == 1 && j == jjm - 1, sp = i == iim - 1 && j == 1;
+  if (!(in || np || sp)) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) out[(size_t)v * ns + s] = 0.0f;
+    return;
+  }
+  const bool deep = i >= 3 && i <= iim - 2 && j >= 3 && j <= jjm - 2;
+  const int* __restrict__ m = L.src;
+  int nb[6];
+  if (in) {
+    nb[0] = SL(i - 1, j, k); nb[1] = SL(i - 1, j - 1, k); nb[2] = SL(i, j - 1, k);
+  } else if (np) {
+    nb[0] = s; nb[1] = SL(iim, 1, k); nb[2] = SL(i, j - 1, k);
+  } else {
+    nb[0] = SL(i - 1, j, k); nb[1] = SL(1, jjm, k); nb[2] = s;
+  }
+  nb[3] = SL(i + 1, j, k); nb[4] = SL(i + 1, j + 1, k); nb[5] = SL(i, j + 1, k);
+  int me = s;
+  if (!deep) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) nb[q] = m[nb[q]];
+    me = m[s];
+  }
+  float w[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) w[q] = (float)L.hw[q * ns + s];
+  // the term a pole lacks: weight 0 (its neighbour index is the slot itself)
+  if (np) w[0] = 0.0f;
+  if (sp) w[2] = 0.0f;
+  const float cw = (float)L.hcw[s];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const float* __restrict__ av = a + (size_t)v * ns;
+    const float x0 = av[me];
+    float acc = L.hs * beta[(size_t)v * ns + s];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc -= w[q] * (av[nb[q]] - x0);
+    out[(size_t)v * ns + s] = x0 + cw * acc;
+  }
+}
+
+// Mixed path, finest level: k_inject with the half-precision weights in
+// difference form, r = beta - (1/S) sum_k hw_k (x_k - x0) at the coincident
+// fine points
+template <int NV>
+__global__ void k_inject_h(LevT<float> F, LevT<float> C, const float* __restrict__ xa, const float* __restrict__ xb,
+                           float* __restrict__ x1) {
+  at_member(F.mb, xa, xb, x1);
+  int iim = C.iim, jjm = C.jjm;
+  const int s1 = xcd_slot();
+  if (s1 >= C.ns) return;
+  const IJK c = ijk_of(s1, iim, jjm);
+  const bool in = c.i >= 2 && c.i <= iim - 1 && c.j >= 2 && c.j <= jjm - 1;
+  const bool np = c.k == 1 && c.i == 1 && c.j == jjm - 1;
+  const bool sp = c.k == 10 && c.i == iim - 1 && c.j == 1;
+  if (!(in || np || sp)) return;
+  const bool deep = c.i >= 3 && c.i <= iim - 2 && c.j >= 3 && c.j <= jjm - 2;
+  const int k = c.k;
+  int i2, j2;
+  if (in) { i2 = 2 * c.i - 1; j2 = 2 * c.j - 1; }
+  else if (np) { i2 = 1; j2 = F.jjm - 1; }
+  else { i2 = F.iim - 1; j2 = 1; }
+  iim = F.iim; jjm = F.jjm;
+  const int ns = F.ns;
+  const int s = SL(i2, j2, k);
+  const int* __restrict__ m = F.src;
+  int q[7];
+  q[0] = s;
+  if (in) {
+    q[1] = SL(i2 - 1, j2, k); q[2] = SL(i2 - 1, j2 - 1, k); q[3] = SL(i2, j2 - 1, k);
+  } else if (np) {
+    q[1] = s; q[2] = SL(iim, 1, k); q[3] = SL(i2, j2 - 1, k);
+  } else {
+    q[1] = SL(i2 - 1, j2, k); q[2] = SL(1, jjm, k); q[3] = s;
+  }
+  q[4] = SL(i2 + 1, j2, k); q[5] = SL(i2 + 1, j2 + 1, k); q[6] = SL(i2, j2 + 1, k);
+  if (!deep) {
+#pragma unroll
+    for (int n = 0; n < 7; ++n) q[n] = m[q[n]];
+  }
+  float w[6];
+#pragma unroll
+  for (int n = 0; n < 6; ++n) w[n] = (float)F.hw[n * ns + s];
+  if (np) w[0] = 0.0f;
+  if (sp) w[2] = 0.0f;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const float* __restrict__ a = xa + (size_t)v * ns;
+    const float x0 = a[q[0]];
+    float acc = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 6; ++n) acc += w[n] * (a[q[n + 1]] - x0);
+    x1[(size_t)v * C.ns + s1] = xb[(size_t)v * ns + s] - F.his * acc;
+  }
+}
+
 // residual at the odd-odd fine points becomes the coarse right-hand side
 // (multigrid_solver.F:684-886 inject_residual); one thread per coarse slot
 // and NV fields (planes ns apart), so the weights are read once.  Fine

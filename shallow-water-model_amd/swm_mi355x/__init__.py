@@ -120,7 +120,7 @@ _sig("swm_multigrid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_voi
 _sig("swm_create_members", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Config), ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_void_p)])
 _sig("swm_members", ctypes.c_int, [ctypes.c_void_p])
-_sig("swm_set_mixed_smoother", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
+_sig("swm_set_mixed_smoother", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
 _sig("swm_set_overlap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_select_member", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_set_fixed_cycles", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
@@ -385,10 +385,10 @@ class Model:
         """Tracer transport on a second stream beside the dynamics (FCT)."""
         _check(lib.swm_set_overlap(self._h, 1 if enable else 0))
 
-    def set_mixed_smoother(self, fine_sweeps=2):
+    def set_mixed_smoother(self, fine_sweeps=2, half_weights=True):
         """Mixed path: finest-level sweeps before / after the coarse
-        correction (2 or 1)."""
-        _check(lib.swm_set_mixed_smoother(self._h, fine_sweeps))
+        correction (1 to 4), and half-precision finest-level weights."""
+        _check(lib.swm_set_mixed_smoother(self._h, fine_sweeps, 1 if half_weights else 0))
 
     def set_fixed_cycles(self, ncycles, nstart=8):
         """Mixed solve: exactly ncycles correction cycles per solve (the first
