@@ -1131,6 +1131,9 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
   C->d.geo = swm::dev::Geo{C->iim, C->jjm, C->ns, C->area, C->area_inv, C->cai, C->sub_area, C->dc, C->nrm, C->tng,
                            C->vw, C->sw, C->lw, C->f, C->hs, C->mask, 0.0, nullptr};
   if (C->precision == SWM_MIXED) {
+    // the FP64 defects' symmetric Laplacian coefficients and masked area_inv
+    C->d.geo.sym = take(3 * ns);
+    C->d.geo.aim = take(ns);
     // FP32 copies of the metric planes, same order as the FP64 ones
     auto tf = [&](size_t n) { return A.take<float>(n); };
     swm::dev::GeoT<float>& g = C->m.geo;
@@ -1319,6 +1322,27 @@ swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
       const LevelT<double>& L = C->d.lev[l - 1];
       cv(L.lw, C->m.lev[l - 1].lw, 7 * (size_t)L.ns);
       cv(L.lwt, C->m.lev[l - 1].lwt, (size_t)L.ns);
+    }
+    {
+      // symmetric edge coefficients of the FP64 defects: area * l_weights(e+1)
+      // at interior slots, halos by the edge table (components re-indexed
+      // across panel seams), and area_inv at real cells
+      const std::vector<double>& lw = Gh.l_weights;   // the model grid's, as G.lw
+      std::vector<double> sym(3 * (size_t)ns, 0.0), aim(ns, 0.0);
+      for (int k = 0; k < 10; ++k)
+        for (int j = 2; j <= C->jjm - 1; ++j)
+          for (int i = 2; i <= C->iim - 1; ++i) {
+            const size_t sl = (size_t)(i - 1) + (size_t)C->iim * ((j - 1) + (size_t)C->jjm * k);
+            for (int e = 0; e < 3; ++e) sym[(size_t)e * ns + sl] = Gh.area[sl] * lw[7 * sl + e];
+          }
+      const swm::HaloTable& t = Gh.edge;
+      for (size_t q = 0; q < t.dst.size(); ++q) {
+        const size_t d = (size_t)(t.dst[q] % t.ncomp) * ns + t.dst[q] / t.ncomp;
+        sym[d] = t.src[q] < 0 ? 0.0 : sym[(size_t)(t.src[q] % t.ncomp) * ns + t.src[q] / t.ncomp];
+      }
+      for (int n = 0; n < ns; ++n) aim[n] = Gh.center_mask[n] == 1.0 ? Gh.area_inv[n] : 0.0;
+      HIP_TRY(hipMemcpy(const_cast<double*>(C->d.geo.sym), sym.data(), sym.size() * sizeof(double), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(const_cast<double*>(C->d.geo.aim), aim.data(), aim.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     if (C->g > swm::dev::CO_MAXLEV) {
       // half-precision weights of the finest level: l_weights(1..6) scaled

@@ -29,6 +29,10 @@ struct GeoT {
   // mixed path: normal and tangent planes times the half-edge lengths,
   // (kk, a, ic) planes as nrm / tng (formed in FP64, stored in R)
   const R *ndc, *tdc;
+  // mixed path (FP64 defects): the Laplacian's symmetric edge coefficients
+  // sym(e) = area * l_weights(e+1) of the edge towards up_e (an edge field,
+  // halos filled) and area_inv at real cells, 0 elsewhere
+  const double *sym, *aim;
 };
 using Geo = GeoT<double>;
 

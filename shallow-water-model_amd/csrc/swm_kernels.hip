@@ -911,6 +911,44 @@ __device__ __forceinline__ float rhs_less_mean(float b, const double* __restrict
   return bsum ? b - (float)(bsum[v] / total) : b;
 }
 
+// The mixed path's FP64 defects form the Laplacian from its symmetric edge
+// coefficients: area * L is symmetric, so the coefficient of the edge from s
+// to a forward neighbour is the one that neighbour holds for the edge back,
+// L(x)_s = area_inv(s) sum_e sym_e (x_e - x_s) over the six edges (3 held at
+// s, 3 at the forward neighbours; the weights sum to zero, so the centre term
+// is the differences').  Three coefficients and one masked area_inv per slot
+// replace seven weights and the mask; the operator is the reference's to
+// rounding (1e-14 of the terms).  The poles keep lap_at's form.
+struct LapSym {
+  bool real, pole;   // real: a cell of the mask (pole: the pole slot)
+  int nb[6];
+  double S[6], ai;
+};
+__device__ __forceinline__ LapSym lap_sym(const Geo& G, int iim, int jjm, int ns, const IJK& c) {
+  LapSym t;
+  const int i = c.i, j = c.j, k = c.k, s = SL(i, j, k);
+  t.ai = G.aim[s];
+  t.real = t.ai != 0.0;
+  t.pole = (i == 1 && j == jjm - 1) || (i == iim - 1 && j == 1);
+  if (!t.real || t.pole) return t;
+  t.nb[0] = SL(i - 1, j, k); t.nb[1] = SL(i - 1, j - 1, k); t.nb[2] = SL(i, j - 1, k);
+  t.nb[3] = SL(i + 1, j, k); t.nb[4] = SL(i + 1, j + 1, k); t.nb[5] = SL(i, j + 1, k);
+  t.S[0] = G.sym[s]; t.S[1] = G.sym[ns + s]; t.S[2] = G.sym[2 * ns + s];
+  t.S[3] = G.sym[t.nb[3]]; t.S[4] = G.sym[ns + t.nb[4]]; t.S[5] = G.sym[2 * ns + t.nb[5]];
+  return t;
+}
+template <class FX>
+__device__ __forceinline__ double lap_sym_apply(const LapSym& t, const Geo& G, int iim, int jjm, int ns, const IJK& c,
+                                                FX X) {
+  if (!t.real) return 0.0;
+  if (t.pole) return lap_at<double>(iim, jjm, ns, c, G.lw, X);
+  const double x0 = X(SL(c.i, c.j, c.k));
+  double v = t.S[0] * (X(t.nb[0]) - x0);
+#pragma unroll
+  for (int q = 1; q < 6; ++q) v = v + t.S[q] * (X(t.nb[q]) - x0);
+  return v * t.ai;
+}
+
 // the first smoothing sweep of the next correction cycle, from zero, on the
 // defect d just formed: k_relax<float, NV, true> at the finest level gives
 // 0.8 * (wt * d) at the real slots and 0 elsewhere, bit for bit the same
@@ -945,18 +983,20 @@ __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict_
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
   if (s < ns) {
     const IJK c = ijk_of(s, iim, jjm);
-    const double m = G.mask[s];
-    const Stencil7<double> st = stencil7<double>(iim, jjm, ns, c, G.lw);
+    const LapSym st = lap_sym(G, iim, jjm, ns, c);
+    const double m = st.real ? 1.0 : 0.0;
+    // the first sweep's slots are k_relax's (interior and pole slots)
+    const bool rslot = interior(c, iim, jjm) || (c.i == 1 && c.j == jjm - 1) || (c.i == iim - 1 && c.j == 1);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const double* xv = x + (size_t)v * ns;
       const float* ev = e + (size_t)v * ns;
       auto X = [&](int sl) { return CORR ? xv[sl] + (double)ev[sl] : xv[sl]; };
-      const double w = st.apply(X);
+      const double w = lap_sym_apply(st, G, iim, jjm, ns, c, X);
       const double d = m != 0.0 ? (double)rhs_less_mean(b[(size_t)v * ns + s], bsum, v, total) - w : 0.0;
       if (CORR) xn[(size_t)v * ns + s] = gauge ? X(s) - gauge[v] * inv_area : X(s);
       r[(size_t)v * ns + s] = (float)d;
-      if (e0) e0[(size_t)v * ns + s] = first_sweep(st.real, lwt, s, (float)d);
+      if (e0) e0[(size_t)v * ns + s] = first_sweep(rslot, lwt, s, (float)d);
       acc[v] = (d * d) * m;
     }
   }
@@ -989,18 +1029,19 @@ __global__ void __launch_bounds__(256) k_defect_extrap(Geo G, const double* __re
   for (int v = 0; v < 2 * NV; ++v) acc[v] = 0.0;
   if (s < ns) {
     const IJK c = ijk_of(s, iim, jjm);
-    const double m = G.mask[s];
-    const Stencil7<double> st = stencil7<double>(iim, jjm, ns, c, G.lw);
+    const LapSym st = lap_sym(G, iim, jjm, ns, c);
+    const double m = st.real ? 1.0 : 0.0;
+    const bool rslot = interior(c, iim, jjm) || (c.i == 1 && c.j == jjm - 1) || (c.i == iim - 1 && c.j == 1);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const double* xv = x + (size_t)v * ns;
       const double* pv = xp + (size_t)v * ns;
       auto X = [&](int sl) { return 2.0 * xv[sl] - pv[sl]; };
-      const double w = st.apply(X);
+      const double w = lap_sym_apply(st, G, iim, jjm, ns, c, X);
       const double d = m != 0.0 ? (double)rhs_less_mean(b[(size_t)v * ns + s], bsum, v, total) - w : 0.0;
       xn[(size_t)v * ns + s] = X(s);
       r[(size_t)v * ns + s] = (float)d;
-      if (e0) e0[(size_t)v * ns + s] = first_sweep(st.real, lwt, s, (float)d);
+      if (e0) e0[(size_t)v * ns + s] = first_sweep(rslot, lwt, s, (float)d);
       acc[v] = (d * d) * m;
       acc[NV + v] = am[s] * (xv[s] - pv[s]);
     }
