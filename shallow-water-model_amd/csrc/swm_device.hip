@@ -720,17 +720,16 @@ void weights(swm_ctx* C) {    // model_control.F:58-89
 }
 
 template <class R>
-// after_step: the step has just filled the halos of the new velocity, of
-// the kinetic energy and (FCT) of the new thickness, and left the old
-// velocity level as it was; a halo fill copies values, so filling those
-// again changes nothing and is left out
+// after_step: the step has just filled the halos of the new velocity and
+// (FCT) of the new thickness, and left the old velocity level as it was; a
+// halo fill copies values, so filling those again changes nothing and is
+// left out
 void wrap_prognostics(swm_ctx* C, View<R>& V, bool after_step = false) {   // step.F:438-493
   const int n4 = C->n4 - 1;
   const int ns = C->ns;
   // inv_h (step.F:448-452) is not formed: nothing on the path reads it
-  std::vector<R*> f = {V.eta[n4], V.div[n4], V.relative};
+  std::vector<R*> f = {V.eta[n4], V.div[n4], V.relative, V.ke};
   if (!after_step || C->advection != SWM_ADV_FCT) f.push_back(V.h[n4]);
-  if (!after_step) f.push_back(V.ke);
   for (int q = 0; q < C->nt; ++q) f.push_back(V.tr[n4][q]);
   gather_list<R>(C, C->face, f);
   if (!after_step) gather<R>(C, C->vrtx, {V.u[0], V.u[0] + 2 * (size_t)ns, V.u[1], V.u[1] + 2 * (size_t)ns});
@@ -858,8 +857,9 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
                        V.u[n4], (const R*)nullptr, (const double*)nullptr, 0.0, (R*)nullptr);
   }
   gather<R>(C, C->vrtx, {V.u[n4], V.u[n4] + 2 * (size_t)ns});
+  // the kinetic energy's halo is filled with the other fields' by
+  // wrap_prognostics, which follows the step directly
   hipLaunchKernelGGL(swm::dev::k_center_ke<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.u[n4], V.ke);
-  gather<R>(C, C->face, {V.ke});
   if (overlap) HIP_TRY(hipStreamWaitEvent(S, C->ev_join, 0));   // the tracers before the halo fill
   return SWM_OK;
 }
@@ -1144,8 +1144,8 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
     g.ndc = tf(12 * ns);
     g.tdc = tf(12 * ns);
     layout_view<float>(C, C->m, &C->d);
-    if (C->g > swm::dev::CO_MAXLEV) {
-      LevelT<float>& L = C->m.lev[C->g - 1];
+    for (int l = swm::dev::CO_MAXLEV + 1; l <= C->g; ++l) {
+      LevelT<float>& L = C->m.lev[l - 1];
       L.L.hw = A.take<_Float16>(6 * (size_t)L.ns);
       L.L.hcw = A.take<_Float16>((size_t)L.ns);
     }
@@ -1344,11 +1344,12 @@ swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
       HIP_TRY(hipMemcpy(const_cast<double*>(C->d.geo.sym), sym.data(), sym.size() * sizeof(double), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(const_cast<double*>(C->d.geo.aim), aim.data(), aim.size() * sizeof(double), hipMemcpyHostToDevice));
     }
-    if (C->g > swm::dev::CO_MAXLEV) {
-      // half-precision weights of the finest level: l_weights(1..6) scaled
-      // by 1 / their largest magnitude, and 0.8 / (S l_weights(7))
-      LevelT<float>& L = C->m.lev[C->g - 1];
-      const std::vector<double>& lw = Gh.levels[C->g - 1].lap_w;
+    for (int l = swm::dev::CO_MAXLEV + 1; l <= C->g; ++l) {
+      // half-precision weights of the levels above the coarse three:
+      // l_weights(1..6) scaled by 1 / their largest magnitude, and
+      // 0.8 / (S l_weights(7))
+      LevelT<float>& L = C->m.lev[l - 1];
+      const std::vector<double>& lw = Gh.levels[l - 1].lap_w;
       const size_t nl = L.ns;
       double mx = 0.0;
       for (size_t n = 0; n < nl; ++n)

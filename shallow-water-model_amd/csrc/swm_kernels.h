@@ -45,9 +45,10 @@ struct LevT {
   const R* lwt;
   const int* src;
   size_t mb;   // member stride of the state (see GeoT)
-  // mixed path, finest level: half-precision weights for the smoother and
-  // restriction in difference form, hw(k) = S l_weights(k) (k = 1..6) and
-  // hcw = 0.8 / (S l_weights(7)); scale S and 1 / S (hw == nullptr: none)
+  // mixed path, levels above the coarse three: half-precision weights for
+  // the smoother and restriction in difference form, hw(k) = S
+  // l_weights(k) (k = 1..6) and hcw = 0.8 / (S l_weights(7)); scale S and
+  // 1 / S (hw == nullptr: none)
   const _Float16* hw;
   const _Float16* hcw;
   float hs, his;

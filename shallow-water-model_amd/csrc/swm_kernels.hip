@@ -1144,7 +1144,7 @@ __global__ void __launch_bounds__(256) k_relax(LevT<R> L, const R* __restrict__ 
   }
 }
 
-// Mixed path, finest level: a non-zero weighted Jacobi sweep of k_relax with
+// Mixed path, levels above the coarse three: a non-zero weighted Jacobi sweep of k_relax with
 // the half-precision weights of LevT, in difference form (the weights sum to
 // zero): out = x0 + hcw (S beta - sum_k hw_k (x_k - x0)), the same operator
 // as k_relax's to the weights' rounding.  It only smooths inside the FP64
@@ -1200,7 +1200,7 @@ __global__ void __launch_bounds__(256) k_relax_h(LevT<float> L, const float* __r
   }
 }
 
-// Mixed path, finest level: k_inject with the half-precision weights in
+// Mixed path, levels above the coarse three: k_inject with the half-precision weights in
 // difference form, r = beta - (1/S) sum_k hw_k (x_k - x0) at the coincident
 // fine points
 template <int NV>
