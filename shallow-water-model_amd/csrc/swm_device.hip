@@ -272,6 +272,15 @@ void launch_relax(swm_ctx* C, const LevelT<R>& L, bool zero, const R* beta, cons
   else hipLaunchKernelGGL((swm::dev::k_relax<R, NV, false>), grid, blk, 0, C->stream, L.L, beta, a, t);
 }
 
+// algorithmic bytes of one sweep at a level per real cell of the model
+// grid: the weights (seven words, or seven halves for the half-precision
+// sweep) and per field beta, alpha (not read from zero) and the result
+template <class R>
+double sweep_bytes(const swm_ctx* C, const LevelT<R>& L, bool zero, int nvar) {
+  const bool half = sizeof(R) == 4 && L.L.hw && C->mixed_half && !zero;
+  return (double)C->max_ig * ((half ? 7.0 * 2.0 : 7.0 * sizeof(R)) + (zero ? 2.0 : 3.0) * nvar * sizeof(R));
+}
+
 // iters Jacobi sweeps at level l (an even count leaves the result in alpha);
 // zero_start: alpha is known to be zero on entry
 template <class R>
@@ -289,7 +298,7 @@ void relax(swm_ctx* C, View<R>& V, int l, int iters, R* alpha, const R* beta, in
       // beta, alpha and the result (neighbour values hit in cache); alpha is
       // not read by a zero-start sweep
       const bool z = zero_start && it == 0;
-      C->kernel_bytes += (double)C->max_ig * sizeof(R) * (7.0 + (z ? 2.0 : 3.0) * nvar);
+      C->kernel_bytes += sweep_bytes(C, L, z, nvar);
     }
     const bool zero = zero_start && it == 0;
     if (nvar == 2) launch_relax<R, 2>(C, L, zero, beta, a, t);
@@ -308,7 +317,7 @@ void relax_once(swm_ctx* C, View<R>& V, int l, const R* a, R* out, const R* beta
                     C->prof_seq++ % C->profile == 0;
   if (prof) {
     (void)hipEventRecord(C->kev[C->kev_used++], C->stream);
-    C->kernel_bytes += (double)C->max_ig * sizeof(R) * (7.0 + (zero ? 2.0 : 3.0) * nvar);
+    C->kernel_bytes += sweep_bytes(C, L, zero, nvar);
   }
   if (nvar == 2) launch_relax<R, 2>(C, L, zero, beta, a, out);
   else launch_relax<R, 1>(C, L, zero, beta, a, out);
