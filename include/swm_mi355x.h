@@ -261,6 +261,18 @@ swm_status swm_set_mixed_smoother(swm_ctx* ctx, int fine_sweeps, int half_weight
  * the mixed step nothing, as both streams compete for HBM bandwidth. */
 swm_status swm_set_overlap(swm_ctx* ctx, int enable);
 
+/* Kernel fusions of the multigrid cycle, both paths (default SWM_FUSE_ALL).
+ * SWM_FUSE_INJECT_SWEEP: the restriction of the residual to a coarser level
+ * that runs its own sweeps also performs that level's first smoothing sweep,
+ * which starts from zero and so needs the injected right-hand side at its
+ * own slot only; one launch per level and cycle fewer.  The results are the
+ * same bit for bit with or without; the switch is there to show it.  No
+ * counterpart in the reference (multigrid_solver.F:476-591 clears alpha,
+ * sweeps, and injects, level by level). */
+#define SWM_FUSE_INJECT_SWEEP 1
+#define SWM_FUSE_ALL 1
+swm_status swm_set_fusion(swm_ctx* ctx, int flags);
+
 /* Mean squared residuals at the end of the last elliptic solve, and mean
  * squared right-hand sides (relative criterion only), per field (psi, chi). */
 swm_status swm_last_residual(const swm_ctx* ctx, double resid_ms[2], double rhs_ms[2]);

@@ -163,7 +163,8 @@ struct swm_ctx {
   int mixed_extrapolate = 1;   // mixed path: start from 2 x_n - x_(n-1)
   int mixed_min_cycles = 1;    // mixed path: cycles before the first check
   int mixed_fine_sweeps = 2;   // mixed path: smoothing sweeps before / after the coarse correction on the finest level
-  int mixed_half = 1;          // mixed path: half-precision weights in the finest level's smoother and restriction
+  int mixed_half = 1;          // mixed path: half-precision weights in the smoother and restriction above the coarse levels
+  int fuse_mg = 1;             // multigrid: an injection also performs the coarse level's first sweep (SWM_FUSE_INJECT_SWEEP)
   double last_resid[2] = {0.0, 0.0};
   double last_rhs_ms[2] = {0.0, 0.0};
   std::vector<double> rhs_ms;   // per member and field (relative criterion)
@@ -355,42 +356,46 @@ R* vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_ze
       }
     }
   };
+  // the injection into a level that runs its own sweeps (above the coarse
+  // kernel's) also performs that level's first sweep from zero, into the
+  // level's scratch (swm_set_fusion)
+  const bool fuse = C->fuse_mg;
+  auto inject = [&](int l, const R* xa, bool half_ok) {
+    LevelT<R>& Fl = V.lev[l - 1];
+    LevelT<R>& Cl = V.lev[l - 2];
+    R* z0 = fuse && l - 1 > top ? Cl.temp : nullptr;
+    const dim3 grid = zgrid(C, nblk_b(Cl.ns, C->blk_mg)), blk(C->blk_mg);
+    if constexpr (sizeof(R) == 4) {
+      if (half_ok && Fl.L.hw && C->mixed_half && top_zero) {
+        if (nvar == 2)
+          hipLaunchKernelGGL((swm::dev::k_inject_h<2>), grid, blk, 0, C->stream, Fl.L, Cl.L, xa, B(l), B(l - 1), z0);
+        else
+          hipLaunchKernelGGL((swm::dev::k_inject_h<1>), grid, blk, 0, C->stream, Fl.L, Cl.L, xa, B(l), B(l - 1), z0);
+        return;
+      }
+    }
+    if (nvar == 2)
+      hipLaunchKernelGGL((swm::dev::k_inject<R, 2>), grid, blk, 0, C->stream, Fl.L, Cl.L, xa, B(l), B(l - 1), z0);
+    else
+      hipLaunchKernelGGL((swm::dev::k_inject<R, 1>), grid, blk, 0, C->stream, Fl.L, Cl.L, xa, B(l), B(l - 1), z0);
+  };
   for (int l = G; l > top; --l) {
     if (l == G && own) {
       fine(fine_sweeps, true);
-      if (nvar == 2)
-        hipLaunchKernelGGL((swm::dev::k_inject<R, 2>), zgrid(C, nblk_b(V.lev[l - 2].ns, C->blk_mg)), dim3(C->blk_mg), 0,
-                           C->stream, V.lev[l - 1].L, V.lev[l - 2].L, fa, B(l), B(l - 1));
-      else
-        hipLaunchKernelGGL((swm::dev::k_inject<R, 1>), zgrid(C, nblk_b(V.lev[l - 2].ns, C->blk_mg)), dim3(C->blk_mg), 0,
-                           C->stream, V.lev[l - 1].L, V.lev[l - 2].L, fa, B(l), B(l - 1));
+      inject(l, fa, false);
       continue;
     }
     if (l == G && top_zero && pre_done)
       // the first sweep from zero is in the level's scratch already (the
       // defect kernel wrote it): the second one
       relax_once<R>(C, V, G, V.lev[G - 1].temp + pre_off, alpha, B(G), nvar, false);
+    else if (l < G && fuse)
+      // the first sweep from zero is in the level's scratch (the injection
+      // wrote it): the second one
+      relax_once<R>(C, V, l, V.lev[l - 1].temp, A(l), B(l), nvar, false);
     else
       relax<R>(C, V, l, 2, A(l), B(l), nvar, l < G || top_zero);
-    bool done = false;
-    if constexpr (sizeof(R) == 4) {
-      if (V.lev[l - 1].L.hw && C->mixed_half && top_zero) {
-        if (nvar == 2)
-          hipLaunchKernelGGL((swm::dev::k_inject_h<2>), zgrid(C, nblk_b(V.lev[l - 2].ns, C->blk_mg)), dim3(C->blk_mg), 0,
-                             C->stream, V.lev[l - 1].L, V.lev[l - 2].L, A(l), B(l), B(l - 1));
-        else
-          hipLaunchKernelGGL((swm::dev::k_inject_h<1>), zgrid(C, nblk_b(V.lev[l - 2].ns, C->blk_mg)), dim3(C->blk_mg), 0,
-                             C->stream, V.lev[l - 1].L, V.lev[l - 2].L, A(l), B(l), B(l - 1));
-        done = true;
-      }
-    }
-    if (done) continue;
-    if (nvar == 2)
-      hipLaunchKernelGGL((swm::dev::k_inject<R, 2>), zgrid(C, nblk_b(V.lev[l - 2].ns, C->blk_mg)), dim3(C->blk_mg), 0, C->stream,
-                         V.lev[l - 1].L, V.lev[l - 2].L, A(l), B(l), B(l - 1));
-    else
-      hipLaunchKernelGGL((swm::dev::k_inject<R, 1>), zgrid(C, nblk_b(V.lev[l - 2].ns, C->blk_mg)), dim3(C->blk_mg), 0, C->stream,
-                         V.lev[l - 1].L, V.lev[l - 2].L, A(l), B(l), B(l - 1));
+    inject(l, A(l), true);
   }
   swm::dev::CoarseArgs<R> P{};
   for (int l = 1; l <= top; ++l) P.lev[l - 1] = V.lev[l - 1].L;
@@ -1739,6 +1744,13 @@ swm_status swm_set_mixed_smoother(swm_ctx* C, int fine_sweeps, int half_weights)
 swm_status swm_set_overlap(swm_ctx* C, int enable) {
   if (!C) { swm::set_error("swm_set_overlap: bad argument"); return SWM_ERR_ARG; }
   C->overlap = enable ? 1 : 0;
+  C->epoch += 1;
+  return SWM_OK;
+}
+
+swm_status swm_set_fusion(swm_ctx* C, int flags) {
+  if (!C || (flags & ~SWM_FUSE_ALL)) { swm::set_error("swm_set_fusion: bad argument"); return SWM_ERR_ARG; }
+  C->fuse_mg = (flags & SWM_FUSE_INJECT_SWEEP) ? 1 : 0;
   C->epoch += 1;
   return SWM_OK;
 }

@@ -818,23 +818,34 @@ __global__ void k_sub_mean(int n, const double* __restrict__ sums, double total,
   r2[t] = r2[t] - (R)m2;
 }
 
-// block-wide sums of NV per-thread values, written to part[v * gridDim.x + block]
+// block-wide sums of NV per-thread values, written to part[v * gridDim.x + block]:
+// a butterfly over the 64 lanes of each wave, then the waves' sums (at most
+// four, blocks of 64..256 threads) in wave order; one barrier
 template <int NV>
 __device__ __forceinline__ void block_sums(const double (&acc)[NV], double* __restrict__ part) {
-  __shared__ double sh[NV][256];
+  __shared__ double sh[NV][4];
+  double a[NV];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) sh[v][threadIdx.x] = acc[v];
-  __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) {
+  for (int v = 0; v < NV; ++v) a[v] = acc[v];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) sh[v][threadIdx.x] += sh[v][threadIdx.x + o];
-    }
-    __syncthreads();
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) a[v] = a[v] + __shfl_xor(a[v], o, 64);
   }
-  if (threadIdx.x == 0) {
+  const int w = (int)threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) part[v * gridDim.x + blockIdx.x] = sh[v][0];
+    for (int v = 0; v < NV; ++v) sh[v][w] = a[v];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (int)blockDim.x >> 6;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      double s = sh[v][0];
+      for (int q = 1; q < nw; ++q) s = s + sh[v][q];
+      part[v * gridDim.x + blockIdx.x] = s;
+    }
   }
 }
 
@@ -949,13 +960,23 @@ __device__ __forceinline__ double lap_sym_apply(const LapSym& t, const Geo& G, i
   return v * t.ai;
 }
 
+// the weighted Jacobi sweep from zero, k_relax<R, NV, true>, at a slot it
+// computes: 0.2 * 0 + 0.8 * (wt * b).  k_relax subtracts the neighbours'
+// terms w * 0 from b first; that changes b at most in the sign of a zero,
+// and 0.2 * 0 + 0.8 * (+-0) is +0 either way, so the two agree bit for bit.
+template <class R>
+__device__ __forceinline__ R zero_sweep(R wt, R b) {
+  const R omega = R(0.8);
+  const R tt = wt * b;
+  return (R(1) - omega) * R(0) + omega * tt;
+}
+
 // the first smoothing sweep of the next correction cycle, from zero, on the
 // defect d just formed: k_relax<float, NV, true> at the finest level gives
-// 0.8 * (wt * d) at the real slots and 0 elsewhere, bit for bit the same
+// zero_sweep at the real slots and 0 elsewhere
 __device__ __forceinline__ float first_sweep(bool real, const float* __restrict__ lwt, int s, float d) {
   if (!real) return 0.0f;
-  const float tt = lwt[s] * d;
-  return (1.0f - 0.8f) * 0.0f + 0.8f * tt;
+  return zero_sweep<float>(lwt[s], d);
 }
 
 // Mixed path, FP64 defect of the Poisson problem for NV fields, one slot per
@@ -987,7 +1008,7 @@ __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict_
     const double m = st.real ? 1.0 : 0.0;
     // the first sweep's slots are k_relax's (interior and pole slots)
     const bool rslot = interior(c, iim, jjm) || (c.i == 1 && c.j == jjm - 1) || (c.i == iim - 1 && c.j == 1);
-#pragma unroll
+#pragma unroll 1
     for (int v = 0; v < NV; ++v) {
       const double* xv = x + (size_t)v * ns;
       const float* ev = e + (size_t)v * ns;
@@ -997,7 +1018,8 @@ __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict_
       if (CORR) xn[(size_t)v * ns + s] = gauge ? X(s) - gauge[v] * inv_area : X(s);
       r[(size_t)v * ns + s] = (float)d;
       if (e0) e0[(size_t)v * ns + s] = first_sweep(rslot, lwt, s, (float)d);
-      acc[v] = (d * d) * m;
+#pragma unroll
+      for (int q = 0; q < NV; ++q) acc[q] = q == v ? (d * d) * m : acc[q];
     }
   }
   block_sums<NV>(acc, part);
@@ -1200,13 +1222,29 @@ __global__ void __launch_bounds__(256) k_relax_h(LevT<float> L, const float* __r
   }
 }
 
+// With z0 (the coarse level's scratch) an injection also performs the coarse
+// level's first smoothing sweep, from zero, which needs the right-hand side
+// just injected at its own slot only: z0 = zero_sweep there, at the slots
+// k_relax<R, NV, true> computes.  This covers those outside the injection:
+// the pole form's slots of the panels that do not hold the pole, from the
+// right-hand side in memory as k_relax reads it, and zero at the halo slots.
+template <class R, int NV>
+__device__ __forceinline__ void coarse_zero_sweep_rest(const LevT<R>& C, const IJK& c, int s1,
+                                                       const R* __restrict__ x1, R* __restrict__ z0) {
+  const bool pole_form = (c.i == 1 && c.j == C.jjm - 1) || (c.i == C.iim - 1 && c.j == 1);
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+    z0[(size_t)v * C.ns + s1] = pole_form ? zero_sweep<R>(C.lwt[s1], x1[(size_t)v * C.ns + s1]) : R(0);
+}
+
 // Mixed path, levels above the coarse three: k_inject with the half-precision weights in
 // difference form, r = beta - (1/S) sum_k hw_k (x_k - x0) at the coincident
 // fine points
 template <int NV>
 __global__ void k_inject_h(LevT<float> F, LevT<float> C, const float* __restrict__ xa, const float* __restrict__ xb,
-                           float* __restrict__ x1) {
+                           float* __restrict__ x1, float* __restrict__ z0) {
   at_member(F.mb, xa, xb, x1);
+  if (z0) at_member(F.mb, z0);
   int iim = C.iim, jjm = C.jjm;
   const int s1 = xcd_slot();
   if (s1 >= C.ns) return;
@@ -1214,7 +1252,10 @@ __global__ void k_inject_h(LevT<float> F, LevT<float> C, const float* __restrict
   const bool in = c.i >= 2 && c.i <= iim - 1 && c.j >= 2 && c.j <= jjm - 1;
   const bool np = c.k == 1 && c.i == 1 && c.j == jjm - 1;
   const bool sp = c.k == 10 && c.i == iim - 1 && c.j == 1;
-  if (!(in || np || sp)) return;
+  if (!(in || np || sp)) {
+    if (z0) coarse_zero_sweep_rest<float, NV>(C, c, s1, x1, z0);
+    return;
+  }
   const bool deep = c.i >= 3 && c.i <= iim - 2 && c.j >= 3 && c.j <= jjm - 2;
   const int k = c.k;
   int i2, j2;
@@ -1251,7 +1292,9 @@ __global__ void k_inject_h(LevT<float> F, LevT<float> C, const float* __restrict
     float acc = 0.0f;
 #pragma unroll
     for (int n = 0; n < 6; ++n) acc += w[n] * (a[q[n + 1]] - x0);
-    x1[(size_t)v * C.ns + s1] = xb[(size_t)v * ns + s] - F.his * acc;
+    const float r = xb[(size_t)v * ns + s] - F.his * acc;
+    x1[(size_t)v * C.ns + s1] = r;
+    if (z0) z0[(size_t)v * C.ns + s1] = zero_sweep<float>(C.lwt[s1], r);
   }
 }
 
@@ -1261,8 +1304,10 @@ __global__ void k_inject_h(LevT<float> F, LevT<float> C, const float* __restrict
 // neighbours of interior coarse points other than the outermost ring are
 // interior themselves and are read without the halo map.
 template <class R, int NV>
-__global__ void k_inject(LevT<R> F, LevT<R> C, const R* __restrict__ xa, const R* __restrict__ xb, R* __restrict__ x1) {
+__global__ void k_inject(LevT<R> F, LevT<R> C, const R* __restrict__ xa, const R* __restrict__ xb, R* __restrict__ x1,
+                         R* __restrict__ z0) {
   at_member(F.mb, xa, xb, x1);
+  if (z0) at_member(F.mb, z0);
   int iim = C.iim, jjm = C.jjm;
   const int s1 = xcd_slot();
   if (s1 >= C.ns) return;
@@ -1270,7 +1315,10 @@ __global__ void k_inject(LevT<R> F, LevT<R> C, const R* __restrict__ xa, const R
   const bool in = c.i >= 2 && c.i <= iim - 1 && c.j >= 2 && c.j <= jjm - 1;
   const bool np = c.k == 1 && c.i == 1 && c.j == jjm - 1;
   const bool sp = c.k == 10 && c.i == iim - 1 && c.j == 1;
-  if (!(in || np || sp)) return;
+  if (!(in || np || sp)) {
+    if (z0) coarse_zero_sweep_rest<R, NV>(C, c, s1, x1, z0);
+    return;
+  }
   const bool deep = c.i >= 3 && c.i <= iim - 2 && c.j >= 3 && c.j <= jjm - 2;
   const int k = c.k;
   int i2, j2;
@@ -1315,6 +1363,7 @@ __global__ void k_inject(LevT<R> F, LevT<R> C, const R* __restrict__ xa, const R
       r = bv - ((w[6] - R(0)) * a[q[0]] + w[0] * a[q[1]] + w[1] * a[q[2]] + w[3] * a[q[4]] + w[4] * a[q[5]] +
                 w[5] * a[q[6]]);
     x1[(size_t)v * C.ns + s1] = r;
+    if (z0) z0[(size_t)v * C.ns + s1] = zero_sweep<R>(C.lwt[s1], r);
   }
 }
 

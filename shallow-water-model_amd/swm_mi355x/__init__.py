@@ -25,6 +25,9 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libswm_mi355x.so")
 
 SWM_FP64 = 0
 SWM_MIXED = 1
+# swm_set_fusion flags
+FUSE_INJECT_SWEEP = 1
+FUSE_ALL = 1
 # advection of step_dt: l_fct, l_centered, l_upwind
 ADVECTION = {"fct": 0, "centered": 1, "upwind": 2}
 
@@ -122,6 +125,7 @@ _sig("swm_create_members", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Confi
 _sig("swm_members", ctypes.c_int, [ctypes.c_void_p])
 _sig("swm_set_mixed_smoother", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
 _sig("swm_set_overlap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
+_sig("swm_set_fusion", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_select_member", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_set_fixed_cycles", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
 _sig("swm_solver_track", ctypes.c_int, [ctypes.c_void_p, _dp])
@@ -385,9 +389,15 @@ class Model:
         """Tracer transport on a second stream beside the dynamics (FCT)."""
         _check(lib.swm_set_overlap(self._h, 1 if enable else 0))
 
+    def set_fusion(self, flags=None):
+        """Kernel fusions of the multigrid cycle (FUSE_* flags; None: all).
+        The results are the same bit for bit either way."""
+        _check(lib.swm_set_fusion(self._h, FUSE_ALL if flags is None else int(flags)))
+
     def set_mixed_smoother(self, fine_sweeps=2, half_weights=True):
         """Mixed path: finest-level sweeps before / after the coarse
-        correction (1 to 4), and half-precision finest-level weights."""
+        correction (1 to 4), and half-precision weights in the smoother and
+        restriction above the three coarsest levels."""
         _check(lib.swm_set_mixed_smoother(self._h, fine_sweeps, 1 if half_weights else 0))
 
     def set_fixed_cycles(self, ncycles, nstart=8):

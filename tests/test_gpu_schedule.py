@@ -3,6 +3,9 @@ change how the mixed solve converges:
 
   - stream overlap (swm_set_overlap): the tracer transport on a second
     stream gives the same fields bit for bit, FP64 and mixed;
+  - the multigrid kernel fusions (swm_set_fusion) give the same fields bit
+    for bit as the unfused cycle, FP64 and mixed, for two finest-level sweep
+    counts, over levels 6..4 with their own sweeps;
   - the mixed solve's finest-level sweep count (swm_set_mixed_smoother):
     every count meets the convergence criterion (the V-cycle only
     preconditions the FP64 defect correction), and its error against the
@@ -25,6 +28,8 @@ def _model(grid, precision, **kw):
         m.set_overlap(kw["overlap"])
     if "sweeps" in kw:
         m.set_mixed_smoother(kw["sweeps"])
+    if "fusion" in kw:
+        m.set_fusion(kw["fusion"])
     m.init_synthetic(seed_index=2, lmax=12)
     return m
 
@@ -39,6 +44,22 @@ def test_overlap_gives_the_same_fields(precision):
     b.step(7)
     for n in FIELDS:
         assert np.array_equal(a.field(n), b.field(n)), n
+
+
+@pytest.mark.parametrize("precision,sweeps", [(S.SWM_FP64, 2), (S.SWM_MIXED, 2), (S.SWM_MIXED, 1)])
+def test_fusion_gives_the_same_fields(precision, sweeps):
+    print(f"case multigrid fusion precision={precision} sweeps={sweeps}", flush=True)
+    grid = S.Grid(6)
+    kw = {"sweeps": sweeps} if precision == S.SWM_MIXED else {}
+    a = _model(grid, precision, fusion=0, **kw)
+    b = _model(grid, precision, **kw)
+    a.step(5)
+    b.step(5)
+    assert a.solver_stats() == b.solver_stats()
+    for n in FIELDS:
+        assert np.array_equal(a.field(n), b.field(n)), n
+    with pytest.raises(S.SwmError):
+        b.set_fusion(S.FUSE_ALL + 8)
 
 
 @pytest.mark.parametrize("sweeps", [1, 3, 4])
