@@ -261,6 +261,15 @@ swm_status swm_set_mixed_smoother(swm_ctx* ctx, int fine_sweeps, int half_weight
  * the mixed step nothing, as both streams compete for HBM bandwidth. */
 swm_status swm_set_overlap(swm_ctx* ctx, int enable);
 
+/* Mixed path: the coarse correction below level 4 (default direct = 1) as
+ * one dense FP32 solve of level 3, alpha = M beta with M the inverse of the
+ * level's Laplacian bordered by ones, formed once in FP64 on the host (1.6
+ * MB), instead of the reference's V-cycle over levels 3..1
+ * (multigrid_solver.F:476-591), which direct = 0 restores.  It only
+ * preconditions the FP64 defect correction, whose criterion is unchanged.
+ * Refused for FP64 contexts, whose multigrid cycle is the reference's. */
+swm_status swm_set_mixed_coarse(swm_ctx* ctx, int direct);
+
 /* Kernel fusions of the multigrid cycle, both paths (default SWM_FUSE_ALL).
  * SWM_FUSE_INJECT_SWEEP: the restriction of the residual to a coarser level
  * that runs its own sweeps also performs that level's first smoothing sweep,

@@ -165,6 +165,10 @@ struct swm_ctx {
   int mixed_fine_sweeps = 2;   // mixed path: smoothing sweeps before / after the coarse correction on the finest level
   int mixed_half = 1;          // mixed path: half-precision weights in the smoother and restriction above the coarse levels
   int fuse_mg = 1;             // multigrid: an injection also performs the coarse level's first sweep (SWM_FUSE_INJECT_SWEEP)
+  int mixed_coarse_direct = 1; // mixed path: the three coarsest levels as one dense solve (k_coarse_direct)
+  float* cd_m = nullptr;       // its FP32 matrix (cd_n x cd_n) and the real cells of level 3
+  int* cd_cell = nullptr;
+  int cd_n = 0;
   double last_resid[2] = {0.0, 0.0};
   double last_rhs_ms[2] = {0.0, 0.0};
   std::vector<double> rhs_ms;   // per member and field (relative criterion)
@@ -397,6 +401,20 @@ R* vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_ze
       relax<R>(C, V, l, 2, A(l), B(l), nvar, l < G || top_zero);
     inject(l, A(l), true);
   }
+  bool direct = false;
+  if constexpr (sizeof(R) == 4) {
+    if (C->cd_n && C->mixed_coarse_direct && top < G) {
+      const dim3 grid = zgrid(C, (unsigned)((C->cd_n + 3) / 4)), blk(256);
+      const int n3 = V.lev[top - 1].ns;
+      if (nvar == 2)
+        hipLaunchKernelGGL(swm::dev::k_coarse_direct<2>, grid, blk, 0, C->stream, C->cd_m, C->cd_cell, C->cd_n, n3, B(top),
+                           A(top), C->mb);
+      else
+        hipLaunchKernelGGL(swm::dev::k_coarse_direct<1>, grid, blk, 0, C->stream, C->cd_m, C->cd_cell, C->cd_n, n3, B(top),
+                           A(top), C->mb);
+      direct = true;
+    }
+  }
   swm::dev::CoarseArgs<R> P{};
   for (int l = 1; l <= top; ++l) P.lev[l - 1] = V.lev[l - 1].L;
   P.nlev = top;
@@ -405,7 +423,7 @@ R* vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_ze
   P.alpha = A(top);
   P.zero = (top < G || top_zero) ? 1 : 0;
   P.mb = C->mb;
-  hipLaunchKernelGGL(swm::dev::k_coarse_cycle<R>, zgrid(C, 1), dim3(swm::dev::CO_THREADS), 0, C->stream, P);
+  if (!direct) hipLaunchKernelGGL(swm::dev::k_coarse_cycle<R>, zgrid(C, 1), dim3(swm::dev::CO_THREADS), 0, C->stream, P);
   for (int l = top; l <= G - 1; ++l) {
     const bool f = l + 1 == G && own;
     R* up = f ? fa : A(l + 1);
@@ -1077,6 +1095,72 @@ void planes_to_vars(const double* planes, int nvar, int pl, double* ref) {
 
 namespace swm {
 
+// The dense coarse solve of the mixed path (k_coarse_direct): the real cells
+// of level l (interior slots panel by panel, then the two poles) and the
+// FP32 n x n block of the inverse of the level's Laplacian (the smoother's
+// operator: l_weights(7) at the cell, l_weights(1..6) at its neighbours
+// through the level's halo map; the poles' forms leave one out, as k_relax),
+// bordered by ones so that it is regular.  Gauss-Jordan elimination with
+// partial pivoting in FP64 on the host, once per context.
+void coarse_direct_matrix(const std::vector<double>& lw, int l, std::vector<float>& M, std::vector<int>& cell) {
+  const int iim = 2 + (1 << l), jjm = iim, ns = iim * jjm * 10;
+  auto SL = [&](int i, int j, int k) { return (i - 1) + iim * ((j - 1) + jjm * (k - 1)); };
+  std::vector<int> src(ns);
+  for (int n = 0; n < ns; ++n) src[n] = n;
+  const swm::HaloTable t = swm::build_simple_face_table(l);
+  for (size_t q = 0; q < t.dst.size(); ++q) src[t.dst[q]] = t.src[q];
+  cell.clear();
+  for (int k = 1; k <= 10; ++k)
+    for (int j = 2; j <= jjm - 1; ++j)
+      for (int i = 2; i <= iim - 1; ++i) cell.push_back(SL(i, j, k));
+  cell.push_back(SL(1, jjm - 1, 1));
+  cell.push_back(SL(iim - 1, 1, 10));
+  const int n = (int)cell.size(), m = n + 1;
+  std::vector<int> idx(ns, -1);
+  for (int r = 0; r < n; ++r) idx[cell[r]] = r;
+  // bordered matrix [[A, 1], [1^T, 0]], row-major m x m, and its inverse
+  std::vector<double> B((size_t)m * m, 0.0), X((size_t)m * m, 0.0);
+  for (int r = 0; r < n; ++r) {
+    const int s = cell[r];
+    const int i = s % iim + 1, j = (s / iim) % jjm + 1, k = s / (iim * jjm) + 1;
+    const bool np = i == 1 && j == jjm - 1, sp = i == iim - 1 && j == 1;
+    int nb[6] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k), SL(i + 1, j, k), SL(i + 1, j + 1, k),
+                 SL(i, j + 1, k)};
+    if (np) { nb[0] = -1; nb[1] = SL(iim, 1, k); }
+    if (sp) { nb[1] = SL(1, jjm, k); nb[2] = -1; }
+    B[(size_t)r * m + r] += lw[7 * (size_t)s + 6];
+    for (int q = 0; q < 6; ++q) {
+      if (nb[q] < 0) continue;
+      const int c = idx[src[nb[q]]];
+      if (c >= 0) B[(size_t)r * m + c] += lw[7 * (size_t)s + q];
+    }
+    B[(size_t)r * m + n] = 1.0;
+    B[(size_t)n * m + r] = 1.0;
+  }
+  for (int r = 0; r < m; ++r) X[(size_t)r * m + r] = 1.0;
+  for (int c = 0; c < m; ++c) {
+    int p = c;
+    for (int r = c + 1; r < m; ++r)
+      if (std::fabs(B[(size_t)r * m + c]) > std::fabs(B[(size_t)p * m + c])) p = r;
+    if (p != c)
+      for (int q = 0; q < m; ++q) {
+        std::swap(B[(size_t)p * m + q], B[(size_t)c * m + q]);
+        std::swap(X[(size_t)p * m + q], X[(size_t)c * m + q]);
+      }
+    const double d = 1.0 / B[(size_t)c * m + c];
+    for (int q = 0; q < m; ++q) { B[(size_t)c * m + q] *= d; X[(size_t)c * m + q] *= d; }
+    for (int r = 0; r < m; ++r) {
+      if (r == c) continue;
+      const double f = B[(size_t)r * m + c];
+      if (f == 0.0) continue;
+      for (int q = 0; q < m; ++q) { B[(size_t)r * m + q] -= f * B[(size_t)c * m + q]; X[(size_t)r * m + q] -= f * X[(size_t)c * m + q]; }
+    }
+  }
+  M.assign((size_t)n * n, 0.0f);
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < n; ++c) M[(size_t)r * n + c] = (float)X[(size_t)r * m + c];
+}
+
 // buffers of a view: multigrid level weights and maps in the shared arena
 // (the maps shared with the FP64 view when given); the levels' fields, the
 // state and scratch in the member block
@@ -1162,6 +1246,12 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
       LevelT<float>& L = C->m.lev[l - 1];
       L.L.hw = A.take<_Float16>(6 * (size_t)L.ns);
       L.L.hcw = A.take<_Float16>((size_t)L.ns);
+    }
+    if (C->g > swm::dev::CO_MAXLEV) {
+      const int e = 1 << swm::dev::CO_MAXLEV;
+      C->cd_n = 10 * e * e + 2;
+      C->cd_m = A.take<float>((size_t)C->cd_n * C->cd_n);
+      C->cd_cell = A.take<int>((size_t)C->cd_n);
     }
     C->mx_r = C->marena.take<float>(2 * ns);
     C->mx_e = C->marena.take<float>(2 * ns);
@@ -1379,6 +1469,14 @@ swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
       HIP_TRY(hipMemcpy(const_cast<_Float16*>(L.L.hcw), hc.data(), hc.size() * sizeof(_Float16), hipMemcpyHostToDevice));
       L.L.hs = (float)S;
       L.L.his = (float)(1.0 / S);
+    }
+    if (C->cd_n) {
+      std::vector<float> M;
+      std::vector<int> cell;
+      coarse_direct_matrix(Gh.levels[swm::dev::CO_MAXLEV - 1].lap_w, swm::dev::CO_MAXLEV, M, cell);
+      if ((int)cell.size() != C->cd_n) { swm::set_error("coarse solve: level size"); return SWM_ERR_DIMS; }
+      HIP_TRY(hipMemcpy(C->cd_m, M.data(), M.size() * sizeof(float), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(C->cd_cell, cell.data(), cell.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     (void)nb;
   }
@@ -1744,6 +1842,13 @@ swm_status swm_set_mixed_smoother(swm_ctx* C, int fine_sweeps, int half_weights)
 swm_status swm_set_overlap(swm_ctx* C, int enable) {
   if (!C) { swm::set_error("swm_set_overlap: bad argument"); return SWM_ERR_ARG; }
   C->overlap = enable ? 1 : 0;
+  C->epoch += 1;
+  return SWM_OK;
+}
+
+swm_status swm_set_mixed_coarse(swm_ctx* C, int direct) {
+  if (!C || C->precision != SWM_MIXED) { swm::set_error("swm_set_mixed_coarse: needs a mixed-precision context"); return SWM_ERR_ARG; }
+  C->mixed_coarse_direct = direct ? 1 : 0;
   C->epoch += 1;
   return SWM_OK;
 }

@@ -1977,6 +1977,44 @@ __global__ void __launch_bounds__(CO_THREADS) k_coarse_cycle(CoarseArgs<R> P) {
   for (int t = threadIdx.x; t < n; t += CO_THREADS) p_alpha[t] = at[t];
 }
 
+// Mixed path: the coarse correction of the three coarsest levels as one
+// dense solve.  alpha = M beta at the level's real cells (cell[0..n)), with M
+// the FP32 n x n block of the inverse of the level's Laplacian bordered by
+// ones (A alpha + lambda 1 = beta, sum alpha = 0: the solution without the
+// constant the singular operator leaves free, for beta less its component
+// along 1 outside A's range).  One wave per row, NV fields; the lanes take
+// the columns in a fixed order and a butterfly sums them, so the result does
+// not depend on the launch.  Halo slots of alpha are not written (the
+// interpolation reads the level through its map).
+template <int NV>
+__global__ void __launch_bounds__(256) k_coarse_direct(const float* __restrict__ M, const int* __restrict__ cell, int n,
+                                                       int ns, const float* __restrict__ beta,
+                                                       float* __restrict__ alpha, size_t mb) {
+  at_member(mb, beta, alpha);
+  const int r = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int lane = (int)(threadIdx.x & 63);
+  if (r >= n) return;   // whole waves leave together
+  const float* __restrict__ row = M + (size_t)r * n;
+  float acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0f;
+  for (int c = lane; c < n; c += 64) {
+    const float m = row[c];
+    const int sl = cell[c];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = acc[v] + m * beta[(size_t)v * ns + sl];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = acc[v] + __shfl_xor(acc[v], o, 64);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) alpha[(size_t)v * ns + cell[r]] = acc[v];
+  }
+}
+
 }  // namespace dev
 }  // namespace swm
 

@@ -126,6 +126,7 @@ _sig("swm_members", ctypes.c_int, [ctypes.c_void_p])
 _sig("swm_set_mixed_smoother", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
 _sig("swm_set_overlap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_set_fusion", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
+_sig("swm_set_mixed_coarse", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_select_member", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_set_fixed_cycles", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
 _sig("swm_solver_track", ctypes.c_int, [ctypes.c_void_p, _dp])
@@ -388,6 +389,11 @@ class Model:
     def set_overlap(self, enable=True):
         """Tracer transport on a second stream beside the dynamics (FCT)."""
         _check(lib.swm_set_overlap(self._h, 1 if enable else 0))
+
+    def set_mixed_coarse(self, direct=True):
+        """Mixed path: the three coarsest levels as one dense solve (True)
+        or the reference's V-cycle over them (False)."""
+        _check(lib.swm_set_mixed_coarse(self._h, 1 if direct else 0))
 
     def set_fusion(self, flags=None):
         """Kernel fusions of the multigrid cycle (FUSE_* flags; None: all).

@@ -6,6 +6,10 @@ change how the mixed solve converges:
   - the multigrid kernel fusions (swm_set_fusion) give the same fields bit
     for bit as the unfused cycle, FP64 and mixed, for two finest-level sweep
     counts, over levels 6..4 with their own sweeps;
+  - the mixed solve's coarse correction (swm_set_mixed_coarse): the dense
+    solve of level 3 and the reference's V-cycle over levels 3..1 both meet
+    the criterion, with errors against the FP64 form within 3x of each
+    other, and the dense solve takes no more cycles;
   - the mixed solve's finest-level sweep count (swm_set_mixed_smoother):
     every count meets the convergence criterion (the V-cycle only
     preconditions the FP64 defect correction), and its error against the
@@ -30,6 +34,8 @@ def _model(grid, precision, **kw):
         m.set_mixed_smoother(kw["sweeps"])
     if "fusion" in kw:
         m.set_fusion(kw["fusion"])
+    if "coarse_direct" in kw:
+        m.set_mixed_coarse(kw["coarse_direct"])
     m.init_synthetic(seed_index=2, lmax=12)
     return m
 
@@ -60,6 +66,33 @@ def test_fusion_gives_the_same_fields(precision, sweeps):
         assert np.array_equal(a.field(n), b.field(n)), n
     with pytest.raises(S.SwmError):
         b.set_fusion(S.FUSE_ALL + 8)
+
+
+def test_mixed_coarse_solve_meets_the_criterion():
+    print("case mixed coarse solve dense vs V-cycle", flush=True)
+    grid = S.Grid(6)
+    fp = _model(grid, S.SWM_FP64)
+    vc = _model(grid, S.SWM_MIXED, coarse_direct=False)
+    de = _model(grid, S.SWM_MIXED, coarse_direct=True)
+    for _ in range(8):
+        for m in (fp, vc, de):
+            m.step(1)
+        for m in (vc, de):
+            r, _ = m.last_residual()
+            assert max(r) <= 1e-20, r
+    assert de.solver_stats()[1] <= vc.solver_stats()[1], (de.solver_stats(), vc.solver_stats())
+    mask = grid.array("grid_center_mask") == 1.0
+    area = grid.array("area")[mask]
+
+    def err(a, n):
+        x, y = fp.field(n)[mask], a.field(n)[mask]
+        return np.sqrt(np.sum(area * (y - x) ** 2) / np.sum(area * x * x))
+
+    for n in ("h", "eta", "div"):
+        ev, ed = err(vc, n), err(de, n)
+        assert ed <= 3.0 * ev + 1e-9 and ev <= 3.0 * ed + 1e-9, (n, ev, ed)
+    with pytest.raises(S.SwmError):
+        fp.set_mixed_coarse(True)
 
 
 @pytest.mark.parametrize("sweeps", [1, 3, 4])
