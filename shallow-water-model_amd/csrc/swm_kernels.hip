@@ -28,12 +28,27 @@ namespace dev {
 struct IJK {
   int i, j, k;
 };
+// q = s / d and r = s % d for 0 <= s < 2^24 and 4 <= d < 2^12 in a dozen
+// instructions instead of the integer division's ~25: float(s) is exact and
+// the product with the approximate reciprocal of d is within 3/4 of s / d,
+// so its truncation is the quotient or one off, which the remainder's sign
+// and size correct.
+__device__ __forceinline__ void divmod_small(int s, int d, int& q, int& r) {
+  const float rd = __builtin_amdgcn_rcpf((float)d);
+  q = (int)((float)s * rd);
+  r = s - q * d;
+  if (r < 0) { q -= 1; r += d; }
+  if (r >= d) { q += 1; r -= d; }
+}
 __device__ __forceinline__ IJK ijk_of(int s, int iim, int jjm) {
   IJK r;
-  r.i = s % iim + 1;
-  const int q = s / iim;
-  r.j = q % jjm + 1;
-  r.k = q / jjm + 1;
+  int q, t;
+  divmod_small(s, iim, q, t);
+  r.i = t + 1;
+  int k1, j1;
+  divmod_small(q, jjm, k1, j1);
+  r.j = j1 + 1;
+  r.k = k1 + 1;
   return r;
 }
 __device__ __forceinline__ bool interior(const IJK& c, int iim, int jjm) {
@@ -57,7 +72,17 @@ __device__ __forceinline__ int xcd_block() {
   const int start = x <= full ? x * per : full * per + (x - full) * (per - 1);
   return start + q;
 }
-__device__ __forceinline__ int xcd_slot() { return xcd_block() * blockDim.x + threadIdx.x; }
+// the workgroup's size with one scalar load: hidden_group_size_x, the 16
+// bits at byte 12 of the implicit kernel arguments (code object v5), which
+// hold for the whole dispatch.  blockDim.x reads the same field with a
+// vector load at the start of every wave, choosing the remainder field for
+// a partial last workgroup; every launch here uses whole workgroups.
+__device__ __forceinline__ int bdim() {
+  const __attribute__((address_space(4))) unsigned* p =
+      (const __attribute__((address_space(4))) unsigned*)__builtin_amdgcn_implicitarg_ptr();
+  return (int)(p[3] & 0xffffu);
+}
+__device__ __forceinline__ int xcd_slot() { return xcd_block() * bdim() + threadIdx.x; }
 
 template <class R>
 __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
@@ -82,7 +107,7 @@ __device__ __forceinline__ R eabs(const GeoT<R>& G, const R* __restrict__ eta, i
 // (blockIdx.y); no source is a destination, so it runs in place.
 template <class R>
 __global__ void k_gather(const int* __restrict__ dst, const int* __restrict__ src, int n, FieldSetT<R> fs) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * bdim() + threadIdx.x;
   if (t >= n) return;
   R* x = fs.p[blockIdx.y];
   at_member(fs.mb, x);
@@ -184,7 +209,7 @@ __device__ __forceinline__ Half<R> half_edges_pre(int iim, int jjm, int ns, int 
 // for the plane p = kk + 3 (a + 2 ic), formed in FP64
 __global__ void k_premul_dc(int ns, const double* __restrict__ nrm, const double* __restrict__ dc,
                             float* __restrict__ out) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * bdim() + threadIdx.x;
   if (s >= ns) return;
   for (int ic = 0; ic < 2; ++ic)
     for (int a = 0; a < 2; ++a)
@@ -743,11 +768,11 @@ __global__ void k_sum_partial(const R* __restrict__ x0, const R* __restrict__ x1
   __shared__ double sh[256];
   const R* x = blockIdx.y == 0 ? x0 : x1;
   double acc = 0.0;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+  for (int t = blockIdx.x * bdim() + threadIdx.x; t < n; t += gridDim.x * bdim())
     acc += w ? (double)x[t] * (double)w[t] : (double)x[t];
   sh[threadIdx.x] = acc;
   __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+  for (int o = bdim() / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
     __syncthreads();
   }
@@ -761,13 +786,13 @@ __global__ void k_sumsq_mixed(const R* __restrict__ x0, const R* __restrict__ x1
   __shared__ double sh[256];
   const R* x = blockIdx.y == 0 ? x0 : x1;
   double acc = 0.0;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+  for (int t = blockIdx.x * bdim() + threadIdx.x; t < n; t += gridDim.x * bdim()) {
     const double v = (double)x[t];
     acc += v * v * w[t];
   }
   sh[threadIdx.x] = acc;
   __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+  for (int o = bdim() / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
     __syncthreads();
   }
@@ -811,7 +836,7 @@ template <class R>
 __global__ void k_sub_mean(int n, const double* __restrict__ sums, double total, R* __restrict__ r1,
                            R* __restrict__ r2, size_t mb) {
   at_member(mb, sums, r1, r2);
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * bdim() + threadIdx.x;
   if (t >= n) return;
   const double m1 = sums[0] / total, m2 = sums[1] / total;
   r1[t] = r1[t] - (R)m1;
@@ -839,7 +864,7 @@ __device__ __forceinline__ void block_sums(const double (&acc)[NV], double* __re
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int nw = (int)blockDim.x >> 6;
+    const int nw = bdim() >> 6;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       double s = sh[v][0];
@@ -1088,13 +1113,13 @@ __global__ void k_track_resid(const double* __restrict__ sums, double inv_n, dou
 __global__ void k_add_scaled(size_t n, const double* __restrict__ g, double scale, double* __restrict__ x,
                              size_t mb) {
   at_member(mb, g, x);
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blockIdx.x * bdim() + threadIdx.x;
   if (t < n) x[t] = x[t] + scale * g[0];
 }
 
 // x += e (FP64 solution, FP32 correction), nvar planes of ns
 __global__ void k_correct(size_t n, const float* __restrict__ e, double* __restrict__ x) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blockIdx.x * bdim() + threadIdx.x;
   if (t < n) x[t] = x[t] + (double)e[t];
 }
 
@@ -1407,12 +1432,12 @@ __global__ void k_interp(LevT<R> C, LevT<R> F, const R* __restrict__ x1, R* __re
 template <class R>
 __global__ void k_fill(R* __restrict__ x, size_t n, R v, size_t mb) {
   at_member(mb, x);
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blockIdx.x * bdim() + threadIdx.x;
   if (t < n) x[t] = v;
 }
 __global__ void k_copy(double* __restrict__ y, const double* __restrict__ x, size_t n, size_t mb) {
   at_member(mb, y, x);
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blockIdx.x * bdim() + threadIdx.x;
   if (t < n) y[t] = x[t];
 }
 
@@ -1558,7 +1583,7 @@ __global__ void __launch_bounds__(256) k_budgets(BudgetIn<R> B, double* __restri
   __shared__ double sh[256];
   const int q = blockIdx.y;
   double acc = 0.0;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < B.ns; s += gridDim.x * blockDim.x) {
+  for (int s = blockIdx.x * bdim() + threadIdx.x; s < B.ns; s += gridDim.x * bdim()) {
     const double m = B.mask[s];
     if (m == 0.0) continue;
     const double h = (double)B.h[s] + B.h0, a = B.area[s];
@@ -1582,7 +1607,7 @@ __global__ void __launch_bounds__(256) k_budgets(BudgetIn<R> B, double* __restri
   }
   sh[threadIdx.x] = acc;
   __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+  for (int o = bdim() / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
     __syncthreads();
   }
@@ -1592,13 +1617,13 @@ __global__ void __launch_bounds__(256) k_budgets(BudgetIn<R> B, double* __restri
 // FP64 state -> FP32 storage of the mixed path: y = x - off (off may be null)
 __global__ void k_to_f32(size_t n, const double* __restrict__ x, const double* __restrict__ off, double c,
                          float* __restrict__ y) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blockIdx.x * bdim() + threadIdx.x;
   if (t < n) y[t] = (float)(x[t] - (off ? off[t] : 0.0) - c);
 }
 // and back: y = x + off + c
 __global__ void k_to_f64(size_t n, const float* __restrict__ x, const double* __restrict__ off, double c,
                          double* __restrict__ y) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blockIdx.x * bdim() + threadIdx.x;
   if (t < n) y[t] = (double)x[t] + (off ? off[t] : 0.0) + c;
 }
 
@@ -1618,7 +1643,7 @@ namespace dev {
 __global__ void k_harmonics(int ns, const double* __restrict__ p, int lmax, const double* __restrict__ coef,
                             double* __restrict__ o0, double* __restrict__ o1, double* __restrict__ o2,
                             double* __restrict__ o3, double* __restrict__ o4) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * bdim() + threadIdx.x;
   if (s >= ns) return;
   const double x = p[s], y = p[ns + s], z = p[2 * ns + s];
   const double mu = fmin(1.0, fmax(-1.0, z));
@@ -1665,10 +1690,10 @@ __global__ void k_sumsq_partial(const double* __restrict__ x, const double* __re
                                 double* __restrict__ part) {
   __shared__ double sh[256];
   double acc = 0.0;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) acc += x[t] * x[t] * w[t];
+  for (int t = blockIdx.x * bdim() + threadIdx.x; t < n; t += gridDim.x * bdim()) acc += x[t] * x[t] * w[t];
   sh[threadIdx.x] = acc;
   __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+  for (int o = bdim() / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
     __syncthreads();
   }
@@ -1678,7 +1703,7 @@ __global__ void k_speedmax_partial(Geo G, const double* __restrict__ u, double* 
   __shared__ double sh[256];
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   double acc = 0.0;
-  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
+  for (int s = blockIdx.x * bdim() + threadIdx.x; s < ns; s += gridDim.x * bdim()) {
     const IJK c = ijk_of(s, iim, jjm);
     if (!interior(c, iim, jjm)) continue;
     for (int ic = 0; ic < 2; ++ic) {
@@ -1688,7 +1713,7 @@ __global__ void k_speedmax_partial(Geo G, const double* __restrict__ u, double* 
   }
   sh[threadIdx.x] = acc;
   __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+  for (int o = bdim() / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
     __syncthreads();
   }
@@ -1699,18 +1724,18 @@ __global__ void k_speedmax_partial(Geo G, const double* __restrict__ u, double* 
 // u scaled; tracer = sum of Gaussian bumps (all slots)
 __global__ void k_synth_finish(Geo G, double h0, double hscale, const double* __restrict__ hp, double* __restrict__ h,
                                double cscale, double* __restrict__ chi) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * bdim() + threadIdx.x;
   if (s >= G.ns) return;
   h[s] = G.mask[s] == 1.0 ? h0 + hscale * hp[s] : 0.0;
   chi[s] = chi[s] * cscale;
 }
 __global__ void k_scale(double* __restrict__ x, size_t n, double a) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blockIdx.x * bdim() + threadIdx.x;
   if (t < n) x[t] = x[t] * a;
 }
 __global__ void k_bumps(int ns, const double* __restrict__ p, int nb, const double* __restrict__ centres,
                         double radius, double width, double* __restrict__ t) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * bdim() + threadIdx.x;
   if (s >= ns) return;
   const double x = p[s], y = p[ns + s], z = p[2 * ns + s];
   double acc = 0.0;
@@ -1730,13 +1755,13 @@ namespace dev {
 
 // single reference operators on every slot (for the operator entry points)
 __global__ void k_op_div_edge(Geo G, const double* __restrict__ F, double* __restrict__ out) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * bdim() + threadIdx.x;
   if (s >= G.ns) return;
   const IJK c = ijk_of(s, G.iim, G.jjm);
   out[s] = div_edge_at(G.iim, G.jjm, G.ns, c, F, G.area_inv[s]);
 }
 __global__ void k_op_lap(Geo G, const double* __restrict__ x, double* __restrict__ out) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * bdim() + threadIdx.x;
   if (s >= G.ns) return;
   const IJK c = ijk_of(s, G.iim, G.jjm);
   auto X = [&](int sl) { return x[sl]; };
@@ -1745,7 +1770,7 @@ __global__ void k_op_lap(Geo G, const double* __restrict__ x, double* __restrict
 // operators.F:1316-1393 with corner values given
 __global__ void k_op_center_ke(Geo G, const double* __restrict__ kec, double* __restrict__ ke) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * bdim() + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
@@ -1991,7 +2016,7 @@ __global__ void __launch_bounds__(256) k_coarse_direct(const float* __restrict__
                                                        int ns, const float* __restrict__ beta,
                                                        float* __restrict__ alpha, size_t mb) {
   at_member(mb, beta, alpha);
-  const int r = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int r = (int)(blockIdx.x * (bdim() >> 6) + (threadIdx.x >> 6));
   const int lane = (int)(threadIdx.x & 63);
   if (r >= n) return;   // whole waves leave together
   const float* __restrict__ row = M + (size_t)r * n;
@@ -2032,7 +2057,7 @@ namespace dev {
 // operators.F:592-682 gradient at the two corners; out planes ic + 2*space
 __global__ void k_op_gradient(Geo G, const double* __restrict__ x, double* __restrict__ out) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * bdim() + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   if (!interior(c, iim, jjm)) {
@@ -2051,7 +2076,7 @@ __global__ void k_op_gradient(Geo G, const double* __restrict__ x, double* __res
 
 // operators.F:528-582 k_cross_v on every slot; planes ic + 2*space
 __global__ void k_op_cross(int ns, const double* __restrict__ x, double* __restrict__ out) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * bdim() + threadIdx.x;
   if (s >= ns) return;
   for (int ic = 0; ic < 2; ++ic) {
     out[ic * ns + s] = -x[(ic + 2) * ns + s];
@@ -2063,7 +2088,7 @@ __global__ void k_op_cross(int ns, const double* __restrict__ x, double* __restr
 // (s_weights), 1: flag = .false. (weights 0.3333333), 2: min_value = .true.
 __global__ void k_op_avg_corners(Geo G, int mode, const double* __restrict__ x, double* __restrict__ out) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * bdim() + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
@@ -2097,7 +2122,7 @@ __device__ __forceinline__ double pv_of(const double* __restrict__ h, const doub
 __global__ void k_op_vort_flux(Geo G, const double* __restrict__ mflux, const double* __restrict__ h,
                                const double* __restrict__ eta, double* __restrict__ out) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * bdim() + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   if (!interior(c, iim, jjm)) {
@@ -2114,7 +2139,7 @@ __global__ void k_op_vort_flux(Geo G, const double* __restrict__ mflux, const do
 __global__ void k_op_div_curl(Geo G, const double* __restrict__ hc, const double* __restrict__ eta,
                               const double* __restrict__ u, const double* __restrict__ h, double* __restrict__ out) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * bdim() + threadIdx.x;
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   if (!interior(c, iim, jjm)) {

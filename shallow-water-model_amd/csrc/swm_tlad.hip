@@ -27,14 +27,14 @@ __device__ __forceinline__ void aadd(double* p, double v) { unsafeAtomicAdd(p, v
 // slots; then (k_gather_adj_zero) the halo slots' adjoint is cleared.  No
 // source is a destination, so the two passes commute with nothing else.
 __global__ void k_gather_adj(const int* __restrict__ dst, const int* __restrict__ src, int n, FieldSet fs) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * bdim() + threadIdx.x;
   if (t >= n) return;
   double* x = fs.p[blockIdx.y];
   const int s = src[t];
   if (s >= 0) aadd(&x[s], x[dst[t]]);
 }
 __global__ void k_gather_adj_zero(const int* __restrict__ dst, int n, FieldSet fs) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * bdim() + threadIdx.x;
   if (t >= n) return;
   fs.p[blockIdx.y][dst[t]] = 0.0;
 }
@@ -365,7 +365,7 @@ __global__ void k_ad_tendencies(Geo G, double dt, double w3, double w2, double w
 // X = ke + g (h3 + hs): Xa folded into ke and h3, then cleared
 __global__ void k_ad_fold_x(int ns, double grav, double* __restrict__ Xa, double* __restrict__ kea,
                             double* __restrict__ h3a) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * bdim() + threadIdx.x;
   if (s >= ns) return;
   const double g = Xa[s];
   Xa[s] = 0.0;
@@ -432,20 +432,20 @@ __global__ void k_ad_curl_div(Geo G, const double* __restrict__ curla, const dou
 // whose mean is sum(am * x) / total, is y -= am * sum(y) / total)
 __global__ void k_sub_am(int n, const double* __restrict__ am, const double* __restrict__ sums, double inv_total,
                          double* __restrict__ y) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * bdim() + threadIdx.x;
   if (s >= n) return;
   y[s] = y[s] - am[s] * (sums[0] * inv_total);
 }
 // y = x * w (w may be the reciprocal area; 0 where mask is 0)
 __global__ void k_mul_mask(int n, const double* __restrict__ x, const double* __restrict__ w,
                            const double* __restrict__ mask, double* __restrict__ y) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * bdim() + threadIdx.x;
   if (s >= n) return;
   y[s] = mask[s] != 0.0 ? x[s] * w[s] : 0.0;
 }
 // y += x
 __global__ void k_axpy(size_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
-  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t s = (size_t)blockIdx.x * bdim() + threadIdx.x;
   if (s >= n) return;
   y[s] = y[s] + a * x[s];
 }
@@ -453,7 +453,7 @@ __global__ void k_axpy(size_t n, double a, const double* __restrict__ x, double*
 __global__ void __launch_bounds__(256) k_dot_partial(size_t n, const double* __restrict__ x,
                                                      const double* __restrict__ y, double* __restrict__ part) {
   double acc[1] = {0.0};
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+  for (size_t t = (size_t)blockIdx.x * bdim() + threadIdx.x; t < n; t += (size_t)gridDim.x * bdim())
     acc[0] += x[t] * y[t];
   block_sums<1>(acc, part);
 }
@@ -490,7 +490,7 @@ __global__ void k_fct_coef(Geo G, const double* __restrict__ beta, const double*
 
 // the limited flux, TL and AD alike (a diagonal map): x *= coef on n values
 __global__ void k_fct_scale(size_t n, const double* __restrict__ coef, double* __restrict__ x) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t t = (size_t)blockIdx.x * bdim() + threadIdx.x;
   if (t < n) x[t] = x[t] * coef[t];
 }
 
