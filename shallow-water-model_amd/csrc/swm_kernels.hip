@@ -51,6 +51,18 @@ __device__ __forceinline__ IJK ijk_of(int s, int iim, int jjm) {
   r.k = k1 + 1;
   return r;
 }
+// load and store of element i (0 <= i, i * sizeof(T) < 2^32) of a buffer
+// whose base is the same for the whole wave: the byte offset is formed in 32
+// bits, so the access takes the scalar base plus one 32-bit offset register
+// instead of a 64-bit address computed per lane
+template <class T>
+__device__ __forceinline__ T ldg(const T* __restrict__ p, int i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + (uint32_t)i * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void stg(T* __restrict__ p, int i, T x) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + (uint32_t)i * (uint32_t)sizeof(T)) = x;
+}
 __device__ __forceinline__ bool interior(const IJK& c, int iim, int jjm) {
   return c.i >= 2 && c.i <= iim - 1 && c.j >= 2 && c.j <= jjm - 1;
 }
@@ -963,21 +975,35 @@ struct LapSym {
 __device__ __forceinline__ LapSym lap_sym(const Geo& G, int iim, int jjm, int ns, const IJK& c) {
   LapSym t;
   const int i = c.i, j = c.j, k = c.k, s = SL(i, j, k);
-  t.ai = G.aim[s];
+  t.ai = ldg(G.aim, s);
   t.real = t.ai != 0.0;
   t.pole = (i == 1 && j == jjm - 1) || (i == iim - 1 && j == 1);
-  if (!t.real || t.pole) return t;
+  if (!t.real) return t;
   t.nb[0] = SL(i - 1, j, k); t.nb[1] = SL(i - 1, j - 1, k); t.nb[2] = SL(i, j - 1, k);
   t.nb[3] = SL(i + 1, j, k); t.nb[4] = SL(i + 1, j + 1, k); t.nb[5] = SL(i, j + 1, k);
-  t.S[0] = G.sym[s]; t.S[1] = G.sym[ns + s]; t.S[2] = G.sym[2 * ns + s];
-  t.S[3] = G.sym[t.nb[3]]; t.S[4] = G.sym[ns + t.nb[4]]; t.S[5] = G.sym[2 * ns + t.nb[5]];
+  if (t.pole) {
+    // the pole's own weights l_weights(1..6) in the same difference form
+    // (they sum to minus l_weights(7)); the neighbour its form leaves out
+    // gets weight 0 and the slot itself
+    const bool np = i == 1;
+    if (np) { t.nb[0] = s; t.nb[1] = SL(iim, 1, k); }
+    else { t.nb[1] = SL(1, jjm, k); t.nb[2] = s; }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) t.S[q] = ldg(G.lw + (size_t)q * ns, s);
+    if (np) t.S[0] = 0.0;
+    else t.S[2] = 0.0;
+    t.ai = 1.0;
+    return t;
+  }
+  const double *s0 = G.sym, *s1 = G.sym + ns, *s2 = G.sym + 2 * (size_t)ns;
+  t.S[0] = ldg(s0, s); t.S[1] = ldg(s1, s); t.S[2] = ldg(s2, s);
+  t.S[3] = ldg(s0, t.nb[3]); t.S[4] = ldg(s1, t.nb[4]); t.S[5] = ldg(s2, t.nb[5]);
   return t;
 }
 template <class FX>
 __device__ __forceinline__ double lap_sym_apply(const LapSym& t, const Geo& G, int iim, int jjm, int ns, const IJK& c,
                                                 FX X) {
   if (!t.real) return 0.0;
-  if (t.pole) return lap_at<double>(iim, jjm, ns, c, G.lw, X);
   const double x0 = X(SL(c.i, c.j, c.k));
   double v = t.S[0] * (X(t.nb[0]) - x0);
 #pragma unroll
@@ -1001,7 +1027,7 @@ __device__ __forceinline__ R zero_sweep(R wt, R b) {
 // zero_sweep at the real slots and 0 elsewhere
 __device__ __forceinline__ float first_sweep(bool real, const float* __restrict__ lwt, int s, float d) {
   if (!real) return 0.0f;
-  return zero_sweep<float>(lwt[s], d);
+  return zero_sweep<float>(ldg(lwt, s), d);
 }
 
 // Mixed path, FP64 defect of the Poisson problem for NV fields, one slot per
@@ -1033,18 +1059,17 @@ __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict_
     const double m = st.real ? 1.0 : 0.0;
     // the first sweep's slots are k_relax's (interior and pole slots)
     const bool rslot = interior(c, iim, jjm) || (c.i == 1 && c.j == jjm - 1) || (c.i == iim - 1 && c.j == 1);
-#pragma unroll 1
+#pragma unroll
     for (int v = 0; v < NV; ++v) {
       const double* xv = x + (size_t)v * ns;
       const float* ev = e + (size_t)v * ns;
-      auto X = [&](int sl) { return CORR ? xv[sl] + (double)ev[sl] : xv[sl]; };
+      auto X = [&](int sl) { return CORR ? ldg(xv, sl) + (double)ldg(ev, sl) : ldg(xv, sl); };
       const double w = lap_sym_apply(st, G, iim, jjm, ns, c, X);
-      const double d = m != 0.0 ? (double)rhs_less_mean(b[(size_t)v * ns + s], bsum, v, total) - w : 0.0;
-      if (CORR) xn[(size_t)v * ns + s] = gauge ? X(s) - gauge[v] * inv_area : X(s);
-      r[(size_t)v * ns + s] = (float)d;
-      if (e0) e0[(size_t)v * ns + s] = first_sweep(rslot, lwt, s, (float)d);
-#pragma unroll
-      for (int q = 0; q < NV; ++q) acc[q] = q == v ? (d * d) * m : acc[q];
+      const double d = m != 0.0 ? (double)rhs_less_mean(ldg(b + (size_t)v * ns, s), bsum, v, total) - w : 0.0;
+      if (CORR) stg(xn + (size_t)v * ns, s, gauge ? X(s) - gauge[v] * inv_area : X(s));
+      stg(r + (size_t)v * ns, s, (float)d);
+      if (e0) stg(e0 + (size_t)v * ns, s, first_sweep(rslot, lwt, s, (float)d));
+      acc[v] = (d * d) * m;
     }
   }
   block_sums<NV>(acc, part);
@@ -1083,14 +1108,14 @@ __global__ void __launch_bounds__(256) k_defect_extrap(Geo G, const double* __re
     for (int v = 0; v < NV; ++v) {
       const double* xv = x + (size_t)v * ns;
       const double* pv = xp + (size_t)v * ns;
-      auto X = [&](int sl) { return 2.0 * xv[sl] - pv[sl]; };
+      auto X = [&](int sl) { return 2.0 * ldg(xv, sl) - ldg(pv, sl); };
       const double w = lap_sym_apply(st, G, iim, jjm, ns, c, X);
-      const double d = m != 0.0 ? (double)rhs_less_mean(b[(size_t)v * ns + s], bsum, v, total) - w : 0.0;
-      xn[(size_t)v * ns + s] = X(s);
-      r[(size_t)v * ns + s] = (float)d;
-      if (e0) e0[(size_t)v * ns + s] = first_sweep(rslot, lwt, s, (float)d);
+      const double d = m != 0.0 ? (double)rhs_less_mean(ldg(b + (size_t)v * ns, s), bsum, v, total) - w : 0.0;
+      stg(xn + (size_t)v * ns, s, X(s));
+      stg(r + (size_t)v * ns, s, (float)d);
+      if (e0) stg(e0 + (size_t)v * ns, s, first_sweep(rslot, lwt, s, (float)d));
       acc[v] = (d * d) * m;
-      acc[NV + v] = am[s] * (xv[s] - pv[s]);
+      acc[NV + v] = ldg(am, s) * (ldg(xv, s) - ldg(pv, s));
     }
   }
   block_sums<2 * NV>(acc, part);
@@ -1209,7 +1234,7 @@ __global__ void __launch_bounds__(256) k_relax_h(LevT<float> L, const float* __r
   const bool np = i == 1 && j == jjm - 1, sp = i == iim - 1 && j == 1;
   if (!(in || np || sp)) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) out[(size_t)v * ns + s] = 0.0f;
+    for (int v = 0; v < NV; ++v) stg(out + (size_t)v * ns, s, 0.0f);
     return;
   }
   const bool deep = i >= 3 && i <= iim - 2 && j >= 3 && j <= jjm - 2;
@@ -1226,24 +1251,24 @@ __global__ void __launch_bounds__(256) k_relax_h(LevT<float> L, const float* __r
   int me = s;
   if (!deep) {
 #pragma unroll
-    for (int q = 0; q < 6; ++q) nb[q] = m[nb[q]];
-    me = m[s];
+    for (int q = 0; q < 6; ++q) nb[q] = ldg(m, nb[q]);
+    me = ldg(m, s);
   }
   float w[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) w[q] = (float)L.hw[q * ns + s];
+  for (int q = 0; q < 6; ++q) w[q] = (float)ldg(L.hw + (size_t)q * ns, s);
   // the term a pole lacks: weight 0 (its neighbour index is the slot itself)
   if (np) w[0] = 0.0f;
   if (sp) w[2] = 0.0f;
-  const float cw = (float)L.hcw[s];
+  const float cw = (float)ldg(L.hcw, s);
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     const float* __restrict__ av = a + (size_t)v * ns;
-    const float x0 = av[me];
-    float acc = L.hs * beta[(size_t)v * ns + s];
+    const float x0 = ldg(av, me);
+    float acc = L.hs * ldg(beta + (size_t)v * ns, s);
 #pragma unroll
-    for (int q = 0; q < 6; ++q) acc -= w[q] * (av[nb[q]] - x0);
-    out[(size_t)v * ns + s] = x0 + cw * acc;
+    for (int q = 0; q < 6; ++q) acc -= w[q] * (ldg(av, nb[q]) - x0);
+    stg(out + (size_t)v * ns, s, x0 + cw * acc);
   }
 }
 
