@@ -236,12 +236,13 @@ int swm_last_vcycles(const swm_ctx* ctx);
  * are compile-time constants. */
 swm_status swm_set_solver(swm_ctx* ctx, double tol, int relative, int max_cycles);
 
-/* Mixed path only: the start of each elliptic solve, the linear
- * extrapolation 2 x_n - x_(n-1) of the last two solutions (extrapolate = 1,
- * the default) or the last solution (0, the reference's warm start), and
- * the number of correction cycles before the criterion is first tested
- * (min_cycles, default 1; the reference runs 2 from its warm start).  No
- * counterpart in the reference. */
+/* Mixed path only: the start of each elliptic solve, the quadratic
+ * extrapolation 3 x_n - 3 x_(n-1) + x_(n-2) of the last three solutions
+ * (extrapolate = 2, the default; the linear one while fewer are held), the
+ * linear 2 x_n - x_(n-1) (1), or the last solution (0, the reference's warm
+ * start), and the number of correction cycles before the criterion is
+ * first tested (min_cycles, default 1; the reference runs 2 from its warm
+ * start).  No counterpart in the reference. */
 swm_status swm_set_mixed_start(swm_ctx* ctx, int extrapolate, int min_cycles);
 
 /* Mixed path: the shape of the FP32 V-cycles, which only precondition the

@@ -137,7 +137,8 @@ struct swm_ctx {
   double* xout = nullptr;  // streamfunction and velocity potential (FP64, both paths)
   double* xalt = nullptr;  // mixed path: second FP64 iterate of the defect correction
   double* xprev = nullptr;  // mixed path: the solution before xout (extrapolated warm start)
-  int nsol = 0;             // mixed path: solutions held since the initialisation (xout, then xprev too), at most 2
+  double* xprev2 = nullptr; // mixed path: the solution before xprev (quadratic extrapolation)
+  int nsol = 0;             // mixed path: solutions held since the initialisation (xout, xprev, xprev2), at most 3
   double* part_slots = nullptr;   // per-block partial sums, one block per 256 slots, up to four columns
   double* gauge = nullptr;        // mixed path: area-weighted change of the solution's mean, two fields
   double* r2 = nullptr;
@@ -160,7 +161,7 @@ struct swm_ctx {
   int solver_relative = 0;
   int solver_max_cycles = 12;
   int solver_stall = 0;        // also stop where a cycle no longer halves the residual (FP64 floor)
-  int mixed_extrapolate = 1;   // mixed path: start from 2 x_n - x_(n-1)
+  int mixed_extrapolate = 2;   // mixed path: start from the quadratic (2) or linear (1) extrapolation, or x_n (0)
   int mixed_min_cycles = 1;    // mixed path: cycles before the first check
   int mixed_fine_sweeps = 2;   // mixed path: smoothing sweeps before / after the coarse correction on the finest level
   int mixed_half = 1;          // mixed path: half-precision weights in the smoother and restriction above the coarse levels
@@ -627,8 +628,9 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
   const float* fine_lwt = (C->mixed_fine_sweeps == 2 && C->g > swm::dev::CO_MAXLEV) ? V.lev[C->g - 1].lwt : nullptr;
   auto first_e = [&](int, size_t o) -> float* { return fine_lwt ? V.lev[C->g - 1].temp + o : nullptr; };
   // the extrapolated start goes to xalt; then the iterates alternate between
-  // xalt and xprev, whose content is no longer needed once it is read
-  double* blk[3] = {xout, C->xalt, C->xprev};
+  // xalt and xprev2, whose content is no longer needed once the start is
+  // formed (xprev is kept: it is the next solve's xprev2)
+  double* blk[3] = {xout, C->xalt, C->xprev2};
   double* cur[2] = {blk[1], blk[1] + ns};
   double* oth[2] = {blk[2], blk[2] + ns};
   // the mean change of the extrapolated start still to be taken off, per field
@@ -665,9 +667,17 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
     launch_defect(nv, v0, last);
     e = C->mx_e;
   };
-  const bool extrap = C->nsol >= 2 && C->mixed_extrapolate;
-  hipLaunchKernelGGL(swm::dev::k_defect_extrap<2>, zgrid(C, nb), dim3(BLK), 0, C->stream, G64, C->area_mask, blk[0],
-                     extrap ? blk[2] : blk[0], blk[1], rhs, r, C->part_slots, bsum, total, fine_lwt, first_e(0, 0));
+  // the start: 3 x_n - 3 x_(n-1) + x_(n-2) with three solutions held, the
+  // linear 2 x_n - x_(n-1) with two, x_n with one (or as set)
+  const int order = std::min(C->mixed_extrapolate, C->nsol - 1);
+  const bool extrap = order >= 1;
+  if (order >= 2)
+    hipLaunchKernelGGL((swm::dev::k_defect_extrap<2, true>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, C->area_mask,
+                       blk[0], C->xprev, blk[2], blk[1], rhs, r, C->part_slots, bsum, total, fine_lwt, first_e(0, 0));
+  else
+    hipLaunchKernelGGL((swm::dev::k_defect_extrap<2, false>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, C->area_mask,
+                       blk[0], extrap ? C->xprev : blk[0], (const double*)nullptr, blk[1], rhs, r, C->part_slots, bsum,
+                       total, fine_lwt, first_e(0, 0));
   if (extrap) {
     sum_final(C, C->part_slots + 2 * (size_t)nb, (int)nb, C->gauge, 2);
     shift[0] = shift[1] = true;
@@ -722,13 +732,14 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
       hipLaunchKernelGGL(swm::dev::k_add_scaled, zgrid(C, nb), dim3(BLK), 0, C->stream, (size_t)ns, C->gauge + v,
                          -inv_area, cur[v], mb);
   // both fields into one buffer (the other field's plane there is free),
-  // which becomes xout; the previous solution becomes xprev
+  // which becomes xout; the previous solutions become xprev and xprev2
   if (cur[1] != cur[0] + ns) copy_state(C, cur[0] + ns, cur[1], (size_t)ns);
   double* fin = cur[0];
+  C->xprev2 = C->xprev;
   C->xprev = blk[0];
   C->xout = fin;
   C->xalt = fin == blk[1] ? blk[2] : blk[1];
-  C->nsol = std::min(C->nsol + 1, 2);
+  C->nsol = std::min(C->nsol + 1, 3);
   C->last_vcycles = iter;
   C->solves += 1; C->vcycles_total += iter; C->checks_total += fixed ? 0 : 1 + iter;
   if (any) {
@@ -938,6 +949,7 @@ void reset_state(swm_ctx* C, bool keep_hs = false) {
   fill<double>(C, C->xout, 2 * (size_t)C->ns, 0.0);
   fill<double>(C, C->xalt, 2 * (size_t)C->ns, 0.0);
   fill<double>(C, C->xprev, 2 * (size_t)C->ns, 0.0);
+  fill<double>(C, C->xprev2, 2 * (size_t)C->ns, 0.0);
   C->nsol = 0;
   C->step_count = 0;
   C->nsecond = 0;
@@ -1218,7 +1230,7 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
   make_table(C, Gh.edge, C->ns, C->edge);
   // the member block: solver iterates, partial sums, then the views' state
   auto mt = [&](size_t n) { return C->marena.take<double>(n); };
-  C->xout = mt(2 * ns); C->r2 = mt(ns); C->xalt = mt(2 * ns); C->xprev = mt(2 * ns); C->gauge = mt(2);
+  C->xout = mt(2 * ns); C->r2 = mt(ns); C->xalt = mt(2 * ns); C->xprev = mt(2 * ns); C->xprev2 = mt(2 * ns); C->gauge = mt(2);
   C->part_slots = mt(4 * (size_t)nblk(ns));
   C->part = mt(2 * (size_t)C->npart); C->sums = mt(64);
   C->part_budget = mt((8 + swm::dev::BUDGET_MAXTR) * (size_t)C->npart);
@@ -1285,7 +1297,7 @@ void shift_members(swm_ctx* C, ptrdiff_t d) {
   if (d == 0) return;
   shift_view(C->d, d);
   if (C->precision == SWM_MIXED) shift_view(C->m, d);
-  for (double** p : {&C->xout, &C->r2, &C->xalt, &C->xprev, &C->gauge, &C->part_slots, &C->part, &C->sums,
+  for (double** p : {&C->xout, &C->r2, &C->xalt, &C->xprev, &C->xprev2, &C->gauge, &C->part_slots, &C->part, &C->sums,
                      &C->part_budget, &C->track, &C->rmean, &C->part_rhs})
     shift_ptr(*p, d);
   shift_ptr(C->mx_r, d);
@@ -1919,11 +1931,11 @@ swm_status swm_set_solver(swm_ctx* C, double tol, int relative, int max_cycles) 
 }
 
 swm_status swm_set_mixed_start(swm_ctx* C, int extrapolate, int min_cycles) {
-  if (!C || min_cycles < 0 || min_cycles > C->solver_max_cycles) {
-    swm::set_error("swm_set_mixed_start: need 0 <= min_cycles <= max_cycles");
+  if (!C || min_cycles < 0 || min_cycles > C->solver_max_cycles || extrapolate < 0 || extrapolate > 2) {
+    swm::set_error("swm_set_mixed_start: need 0 <= extrapolate <= 2 and 0 <= min_cycles <= max_cycles");
     return SWM_ERR_ARG;
   }
-  C->mixed_extrapolate = extrapolate ? 1 : 0;
+  C->mixed_extrapolate = extrapolate;
   C->mixed_min_cycles = min_cycles;
   C->epoch += 1;
   return SWM_OK;

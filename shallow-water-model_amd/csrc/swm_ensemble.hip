@@ -33,7 +33,7 @@ struct swm_ensemble {
     hipGraphExec_t exec = nullptr;
     size_t nodes = 0;
     // the groups' solver buffers and flags after the captured steps
-    std::vector<std::array<double*, 3>> bufs;
+    std::vector<std::array<double*, 4>> bufs;
     std::vector<int> nsol;
     std::vector<int> vcycles;
     std::vector<long long> vtotal;   // V-cycles of the captured steps, per group
@@ -60,6 +60,7 @@ std::vector<long long> ens_key(const swm_ensemble* e, int K) {
     k.push_back((long long)(intptr_t)C->xout);
     k.push_back((long long)(intptr_t)C->xalt);
     k.push_back((long long)(intptr_t)C->xprev);
+    k.push_back((long long)(intptr_t)C->xprev2);
   }
   return k;
 }
@@ -128,7 +129,7 @@ swm_status capture(swm_ensemble* e, int K, swm_ensemble::Entry& en) {
   en.nodes = nn;
   for (int k = 0; k < ng; ++k) en.vtotal.push_back(e->g[k]->vcycles_total - v0[k]);
   for (swm_ctx* C : e->g) {
-    en.bufs.push_back({C->xout, C->xalt, C->xprev});
+    en.bufs.push_back({C->xout, C->xalt, C->xprev, C->xprev2});
     en.nsol.push_back(C->nsol);
     en.vcycles.push_back(C->last_vcycles);
   }
@@ -147,6 +148,7 @@ void advance_host(swm_ensemble* e, int K, const swm_ensemble::Entry& en) {
     C->xout = en.bufs[k][0];
     C->xalt = en.bufs[k][1];
     C->xprev = en.bufs[k][2];
+    C->xprev2 = en.bufs[k][3];
     C->nsol = en.nsol[k];
     C->last_vcycles = en.vcycles[k];
     C->solves += n;
@@ -195,7 +197,7 @@ struct EnsSnapshot {
     int step_count, last_vcycles;
     long long nsecond, solves, vcycles_total, checks_total;
     int nsol;
-    double *xout, *xalt, *xprev;
+    double *xout, *xalt, *xprev, *xprev2;
   };
   std::vector<Host> host;
 };
@@ -208,7 +210,7 @@ swm_status snapshot(swm_ensemble* e, EnsSnapshot& sn) {
     sn.mem.push_back(p);
     HIP_TRY(hipMemcpyAsync(p, C->marena.base, bytes, hipMemcpyDeviceToDevice, e->s0));
     sn.host.push_back({C->step_count, C->last_vcycles, C->nsecond, C->solves, C->vcycles_total, C->checks_total,
-                       C->nsol, C->xout, C->xalt, C->xprev});
+                       C->nsol, C->xout, C->xalt, C->xprev, C->xprev2});
   }
   HIP_TRY(hipStreamSynchronize(e->s0));
   return SWM_OK;
@@ -221,7 +223,7 @@ swm_status restore(swm_ensemble* e, EnsSnapshot& sn) {
     const EnsSnapshot::Host& h = sn.host[k];
     C->step_count = h.step_count; C->last_vcycles = h.last_vcycles; C->nsecond = h.nsecond;
     C->solves = h.solves; C->vcycles_total = h.vcycles_total; C->checks_total = h.checks_total;
-    C->nsol = h.nsol; C->xout = h.xout; C->xalt = h.xalt; C->xprev = h.xprev;
+    C->nsol = h.nsol; C->xout = h.xout; C->xalt = h.xalt; C->xprev = h.xprev; C->xprev2 = h.xprev2;
     pointers(C);
     weights(C);
   }

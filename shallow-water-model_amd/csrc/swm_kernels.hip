@@ -1075,23 +1075,26 @@ __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict_
   block_sums<NV>(acc, part);
 }
 
-// Mixed path, first FP64 defect of a solve from the extrapolated warm start
+// Mixed path, first FP64 defect of a solve from the extrapolated warm start:
 // x0 = 2 x - xp (x the last solution, xp the one before; xp = x gives the
-// plain warm start exactly): x0 is stored to xn on every slot and
-// d = rhs - L(x0) as in k_defect.  The solution's constant part is free (L
-// of a constant is zero), so extrapolating it would let it drift; the
-// per-block sums of area * (x - xp) (columns NV.. of part) give the mean
-// change, which the first correction takes off again (k_defect's gauge).
-// With gauge (k_defect) the stored iterate loses gauge[v] * inv_area; the
-// defect, of the unshifted iterate, is the same up to rounding.
-template <int NV>
+// plain warm start exactly) or, Q, x0 = 3 x - 3 xp + xpp (xpp the one
+// before xp).  x0 is stored to xn on every slot and d = rhs - L(x0) as in
+// k_defect.  The solution's constant part is free (L of a constant is zero),
+// so extrapolating it would let it drift; the per-block sums of
+// area * (x0 - x) (columns NV.. of part) give the mean change, which the
+// first correction takes off again (k_defect's gauge).  With gauge
+// (k_defect) the stored iterate loses gauge[v] * inv_area; the defect, of
+// the unshifted iterate, is the same up to rounding.
+template <int NV, bool Q>
 __global__ void __launch_bounds__(256) k_defect_extrap(Geo G, const double* __restrict__ am,
                                                        const double* __restrict__ x, const double* __restrict__ xp,
-                                                       double* __restrict__ xn, const float* __restrict__ b,
-                                                       float* __restrict__ r, double* __restrict__ part,
-                                                       const double* __restrict__ bsum, double total,
-                                                       const float* __restrict__ lwt, float* __restrict__ e0) {
+                                                       const double* __restrict__ xpp, double* __restrict__ xn,
+                                                       const float* __restrict__ b, float* __restrict__ r,
+                                                       double* __restrict__ part, const double* __restrict__ bsum,
+                                                       double total, const float* __restrict__ lwt,
+                                                       float* __restrict__ e0) {
   at_member(G.mb, x, xp, xn, b, r, part);
+  if (Q) at_member(G.mb, xpp);
   if (bsum) at_member(G.mb, bsum);
   if (e0) at_member(G.mb, e0);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
@@ -1108,14 +1111,18 @@ __global__ void __launch_bounds__(256) k_defect_extrap(Geo G, const double* __re
     for (int v = 0; v < NV; ++v) {
       const double* xv = x + (size_t)v * ns;
       const double* pv = xp + (size_t)v * ns;
-      auto X = [&](int sl) { return 2.0 * ldg(xv, sl) - ldg(pv, sl); };
+      const double* qv = Q ? xpp + (size_t)v * ns : nullptr;
+      auto X = [&](int sl) {
+        return Q ? 3.0 * (ldg(xv, sl) - ldg(pv, sl)) + ldg(qv, sl) : 2.0 * ldg(xv, sl) - ldg(pv, sl);
+      };
       const double w = lap_sym_apply(st, G, iim, jjm, ns, c, X);
       const double d = m != 0.0 ? (double)rhs_less_mean(ldg(b + (size_t)v * ns, s), bsum, v, total) - w : 0.0;
-      stg(xn + (size_t)v * ns, s, X(s));
+      const double x0 = X(s);
+      stg(xn + (size_t)v * ns, s, x0);
       stg(r + (size_t)v * ns, s, (float)d);
       if (e0) stg(e0 + (size_t)v * ns, s, first_sweep(rslot, lwt, s, (float)d));
       acc[v] = (d * d) * m;
-      acc[NV + v] = ldg(am, s) * (ldg(xv, s) - ldg(pv, s));
+      acc[NV + v] = ldg(am, s) * (Q ? x0 - ldg(xv, s) : ldg(xv, s) - ldg(pv, s));
     }
   }
   block_sums<2 * NV>(acc, part);
@@ -1172,7 +1179,7 @@ __global__ void __launch_bounds__(256) k_relax(LevT<R> L, const R* __restrict__ 
   const bool np = i == 1 && j == jjm - 1, sp = i == iim - 1 && j == 1;
   if (!(in || np || sp)) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) out[(size_t)v * ns + s] = R(0);
+    for (int v = 0; v < NV; ++v) stg(out + (size_t)v * ns, s, R(0));
     return;
   }
   const bool deep = i >= 3 && i <= iim - 2 && j >= 3 && j <= jjm - 2;
@@ -1191,19 +1198,19 @@ __global__ void __launch_bounds__(256) k_relax(LevT<R> L, const R* __restrict__ 
   int me = s;   // the slot's own value is read through the map as well
   if (!ZERO && !deep) {
 #pragma unroll
-    for (int q = 0; q < 6; ++q) nb[q] = nb[q] < 0 ? -1 : m[nb[q]];
-    me = m[s];
+    for (int q = 0; q < 6; ++q) nb[q] = nb[q] < 0 ? -1 : ldg(m, nb[q]);
+    me = ldg(m, s);
   }
   R w[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) w[q] = L.lw[q * ns + s];
-  const R wt = L.lwt[s];
+  for (int q = 0; q < 6; ++q) w[q] = ldg(L.lw + (size_t)q * ns, s);
+  const R wt = ldg(L.lwt, s);
   const R omega = R(0.8);
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     const R* __restrict__ av = a + (size_t)v * ns;
-    auto A = [&](int q) { return ZERO ? R(0) : av[nb[q]]; };
-    const R bv = beta[(size_t)v * ns + s];
+    auto A = [&](int q) { return ZERO ? R(0) : ldg(av, nb[q]); };
+    const R bv = ldg(beta + (size_t)v * ns, s);
     R tt;
     if (in)
       tt = wt * (bv - w[0] * A(0) - w[1] * A(1) - w[2] * A(2) - w[3] * A(3) - w[4] * A(4) - w[5] * A(5));
@@ -1211,8 +1218,8 @@ __global__ void __launch_bounds__(256) k_relax(LevT<R> L, const R* __restrict__ 
       tt = wt * (bv - w[1] * A(1) - w[2] * A(2) - w[3] * A(3) - w[4] * A(4) - w[5] * A(5));
     else
       tt = wt * (bv - w[0] * A(0) - w[1] * A(1) - w[3] * A(3) - w[4] * A(4) - w[5] * A(5));
-    const R self = ZERO ? R(0) : av[me];
-    out[(size_t)v * ns + s] = (R(1) - omega) * self + omega * tt;
+    const R self = ZERO ? R(0) : ldg(av, me);
+    stg(out + (size_t)v * ns, s, (R(1) - omega) * self + omega * tt);
   }
 }
 
@@ -1328,23 +1335,23 @@ __global__ void k_inject_h(LevT<float> F, LevT<float> C, const float* __restrict
   q[4] = SL(i2 + 1, j2, k); q[5] = SL(i2 + 1, j2 + 1, k); q[6] = SL(i2, j2 + 1, k);
   if (!deep) {
 #pragma unroll
-    for (int n = 0; n < 7; ++n) q[n] = m[q[n]];
+    for (int n = 0; n < 7; ++n) q[n] = ldg(m, q[n]);
   }
   float w[6];
 #pragma unroll
-  for (int n = 0; n < 6; ++n) w[n] = (float)F.hw[n * ns + s];
+  for (int n = 0; n < 6; ++n) w[n] = (float)ldg(F.hw + (size_t)n * ns, s);
   if (np) w[0] = 0.0f;
   if (sp) w[2] = 0.0f;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     const float* __restrict__ a = xa + (size_t)v * ns;
-    const float x0 = a[q[0]];
+    const float x0 = ldg(a, q[0]);
     float acc = 0.0f;
 #pragma unroll
-    for (int n = 0; n < 6; ++n) acc += w[n] * (a[q[n + 1]] - x0);
-    const float r = xb[(size_t)v * ns + s] - F.his * acc;
-    x1[(size_t)v * C.ns + s1] = r;
-    if (z0) z0[(size_t)v * C.ns + s1] = zero_sweep<float>(C.lwt[s1], r);
+    for (int n = 0; n < 6; ++n) acc += w[n] * (ldg(a, q[n + 1]) - x0);
+    const float r = ldg(xb + (size_t)v * ns, s) - F.his * acc;
+    stg(x1 + (size_t)v * C.ns, s1, r);
+    if (z0) stg(z0 + (size_t)v * C.ns, s1, zero_sweep<float>(ldg(C.lwt, s1), r));
   }
 }
 
@@ -1392,28 +1399,29 @@ __global__ void k_inject(LevT<R> F, LevT<R> C, const R* __restrict__ xa, const R
   q[4] = SL(i2 + 1, j2, k); q[5] = SL(i2 + 1, j2 + 1, k); q[6] = SL(i2, j2 + 1, k);
   if (!deep) {
 #pragma unroll
-    for (int n = 0; n < 7; ++n) q[n] = m[q[n]];
+    for (int n = 0; n < 7; ++n) q[n] = ldg(m, q[n]);
   }
   R w[7];
 #pragma unroll
-  for (int n = 0; n < 7; ++n) w[n] = F.lw[n * ns + s];
+  for (int n = 0; n < 7; ++n) w[n] = ldg(F.lw + (size_t)n * ns, s);
   // w[n] is l_weights(n+1); q[0] is the centre, q[1..6] the neighbours 1..6
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     const R* __restrict__ a = xa + (size_t)v * F.ns;
-    const R bv = xb[(size_t)v * F.ns + s];
+    const R bv = ldg(xb + (size_t)v * F.ns, s);
+    auto X = [&](int n) { return ldg(a, q[n]); };
     R r;
     if (in)
-      r = bv - ((w[6] - R(0)) * a[q[0]] + w[0] * a[q[1]] + w[1] * a[q[2]] + w[2] * a[q[3]] + w[3] * a[q[4]] +
-                w[4] * a[q[5]] + w[5] * a[q[6]]);
+      r = bv - ((w[6] - R(0)) * X(0) + w[0] * X(1) + w[1] * X(2) + w[2] * X(3) + w[3] * X(4) +
+                w[4] * X(5) + w[5] * X(6));
     else if (np)
-      r = bv - ((w[6] - R(0)) * a[q[0]] + w[1] * a[q[2]] + w[2] * a[q[3]] + w[3] * a[q[4]] + w[4] * a[q[5]] +
-                w[5] * a[q[6]]);
+      r = bv - ((w[6] - R(0)) * X(0) + w[1] * X(2) + w[2] * X(3) + w[3] * X(4) + w[4] * X(5) +
+                w[5] * X(6));
     else
-      r = bv - ((w[6] - R(0)) * a[q[0]] + w[0] * a[q[1]] + w[1] * a[q[2]] + w[3] * a[q[4]] + w[4] * a[q[5]] +
-                w[5] * a[q[6]]);
-    x1[(size_t)v * C.ns + s1] = r;
-    if (z0) z0[(size_t)v * C.ns + s1] = zero_sweep<R>(C.lwt[s1], r);
+      r = bv - ((w[6] - R(0)) * X(0) + w[0] * X(1) + w[1] * X(2) + w[3] * X(4) + w[4] * X(5) +
+                w[5] * X(6));
+    stg(x1 + (size_t)v * C.ns, s1, r);
+    if (z0) stg(z0 + (size_t)v * C.ns, s1, zero_sweep<R>(ldg(C.lwt, s1), r));
   }
 }
 
@@ -1434,7 +1442,7 @@ __global__ void k_interp(LevT<R> C, LevT<R> F, const R* __restrict__ x1, R* __re
   auto CS = [&](int ii, int jj) {
     const int a = (ii + 1) / 2, b = (jj + 1) / 2;
     const int sl = (a - 1) + ci * ((b - 1) + cj * (k - 1));
-    return (a == 1 || a == ci || b == 1 || b == cj) ? C.src[sl] : sl;
+    return (a == 1 || a == ci || b == 1 || b == cj) ? ldg(C.src, sl) : sl;
   };
   const bool io = (i & 1), jo = (j & 1);
   int c0 = -1, c1 = -1;
@@ -1448,9 +1456,9 @@ __global__ void k_interp(LevT<R> C, LevT<R> F, const R* __restrict__ x1, R* __re
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     const R* __restrict__ xc = x1 + (size_t)v * C.ns;
-    const R add = c1 < 0 ? xc[c0] : R(0.5) * (xc[c0] + xc[c1]);
+    const R add = c1 < 0 ? ldg(xc, c0) : R(0.5) * (ldg(xc, c0) + ldg(xc, c1));
     R* __restrict__ xf = x2 + (size_t)v * F.ns;
-    xf[s] = xf[s] + add;
+    stg(xf, s, ldg((const R*)xf, s) + add);
   }
 }
 

@@ -517,10 +517,11 @@ class Model:
         reference's (mean squared residual <= 1e-20 within 12 V-cycles)."""
         _check(lib.swm_set_solver(self._h, tol, 1 if relative else 0, max_cycles))
 
-    def set_mixed_start(self, extrapolate=True, min_cycles=1):
-        """Mixed path: extrapolated (2 x_n - x_(n-1)) or plain warm start, and
-        the correction cycles before the first convergence test."""
-        _check(lib.swm_set_mixed_start(self._h, 1 if extrapolate else 0, min_cycles))
+    def set_mixed_start(self, extrapolate=2, min_cycles=1):
+        """Mixed path: the warm start, extrapolated from the last three
+        solutions (2), the last two (1) or the last one (0, False), and the
+        correction cycles before the first convergence test."""
+        _check(lib.swm_set_mixed_start(self._h, int(extrapolate), min_cycles))
 
     def last_residual(self):
         """(mean squared residuals, mean squared right-hand sides) of psi and

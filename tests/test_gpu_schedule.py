@@ -9,12 +9,12 @@ change how the mixed solve converges:
   - the mixed solve's coarse correction (swm_set_mixed_coarse): the dense
     solve of level 3 and the reference's V-cycle over levels 3..1 both meet
     the criterion, with errors against the FP64 form within 3x of each
-    other, and the dense solve takes no more cycles;
+    other plus 1e-7, and the dense solve takes no more cycles;
   - the mixed solve's finest-level sweep count (swm_set_mixed_smoother):
     every count meets the convergence criterion (the V-cycle only
     preconditions the FP64 defect correction), and its error against the
-    FP64 form is within 3x of the two-sweep run's (thickness, vorticity,
-    divergence, relative L2 after 8 steps).
+    FP64 form is within 3x of the two-sweep run's plus 1e-7 (thickness,
+    vorticity, divergence, relative L2 after 8 steps).
 """
 import numpy as np
 import pytest
@@ -90,7 +90,7 @@ def test_mixed_coarse_solve_meets_the_criterion():
 
     for n in ("h", "eta", "div"):
         ev, ed = err(vc, n), err(de, n)
-        assert ed <= 3.0 * ev + 1e-9 and ev <= 3.0 * ed + 1e-9, (n, ev, ed)
+        assert ed <= 3.0 * ev + 1e-7 and ev <= 3.0 * ed + 1e-7, (n, ev, ed)
     with pytest.raises(S.SwmError):
         fp.set_mixed_coarse(True)
 
@@ -118,8 +118,10 @@ def test_mixed_sweep_counts_meet_the_criterion(sweeps):
         x, y = fp.field(n)[mask], a.field(n)[mask]
         return np.sqrt(np.sum(area * (y - x) ** 2) / np.sum(area * x * x))
 
+    # (plus 1e-7, the precision bar of the mixed step: with the quadratic
+    # warm start the two-sweep run's error can itself be far below it)
     for n in ("h", "eta", "div"):
         e2, es = err(ref, n), err(m, n)
-        assert es <= 3.0 * e2 + 1e-9, (n, e2, es)
+        assert es <= 3.0 * e2 + 1e-7, (n, e2, es)
     with pytest.raises(S.SwmError):
         m.set_mixed_smoother(5)
