@@ -137,8 +137,10 @@ swm_status swm_select_member(swm_ctx* ctx, int member);
 
 /* Fixed schedule of the mixed solve: ncycles > 0 correction cycles per solve
  * with no host-side convergence test, so that steps can be captured in a HIP
- * graph; 0 restores the test.  The first two solves after an initialisation
- * (from a zero, then a first-order warm start) take max(ncycles, nstart).  The largest mean squared residuals of the two
+ * graph; 0 restores the test.  The solves after an initialisation whose
+ * warm start has not yet its full order (from zero, then the linear
+ * extrapolation while only two solutions are held) take max(ncycles,
+ * nstart).  The largest mean squared residuals of the two
  * fields and the number of solves that ended above the (absolute) criterion
  * are recorded per member on the device: swm_solver_track gives them for the
  * selected member (out[0..2]), counted from its last initialisation. */

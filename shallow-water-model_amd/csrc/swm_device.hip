@@ -687,7 +687,8 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
   if (fixed) {
     // the fixed schedule: no host-side test; the last defect's mean squares
     // go to the member's residual record on the device
-    iter = C->nsol >= 2 ? C->fixed_cycles : std::max(C->fixed_cycles, C->fixed_start);
+    // (the start-up cycles until the warm start has its full order)
+    iter = C->nsol > C->mixed_extrapolate ? C->fixed_cycles : std::max(C->fixed_cycles, C->fixed_start);
     for (int it = 0; it < iter; ++it) cycle(2, 0, it == iter - 1);
     sum_final(C, C->part_slots, (int)nb, C->sums, 2);
     hipLaunchKernelGGL(swm::dev::k_track_resid, zgrid(C, 1), dim3(64), 0, C->stream, C->sums, 1.0 / (double)C->max_ig,
