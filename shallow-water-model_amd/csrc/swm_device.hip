@@ -636,27 +636,40 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
   // the mean change of the extrapolated start still to be taken off, per field
   bool shift[2] = {false, false};
   const double inv_area = 1.0 / C->total_area;
+  // last: no further cycle follows (the fixed schedule's last one), or none
+  // is expected (the criterion is tested next): the defect's FP32 copy and
+  // the next cycle's first sweep are not stored; redefect forms them if a
+  // cycle does follow
   auto launch_defect = [&](int nv, int v0, bool last) {
     const size_t o = (size_t)v0 * ns;
     const double* gg = shift[v0] ? C->gauge + v0 : nullptr;
     float* e0 = last ? nullptr : first_e(v0, o);
+    float* rr = last ? nullptr : r + o;
     if (nv == 2)
       hipLaunchKernelGGL((swm::dev::k_defect<2, true>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, cur[0], e, oth[0],
-                         rhs, r, C->part_slots, gg, inv_area, bsum, total, fine_lwt, e0);
+                         rhs, rr, C->part_slots, gg, inv_area, bsum, total, fine_lwt, e0);
     else
       hipLaunchKernelGGL((swm::dev::k_defect<1, true>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, cur[v0], e + o,
-                         oth[v0], rhs + o, r + o, C->part_slots, gg, inv_area, bsum ? bsum + v0 : nullptr, total,
+                         oth[v0], rhs + o, rr, C->part_slots, gg, inv_area, bsum ? bsum + v0 : nullptr, total,
                          fine_lwt, e0);
     for (int v = v0; v < v0 + nv; ++v) {
       std::swap(cur[v], oth[v]);
       shift[v] = false;
     }
   };
+  auto redefect = [&](int nv, int v0) {
+    const size_t o = (size_t)v0 * ns;
+    if (nv == 2)
+      hipLaunchKernelGGL((swm::dev::k_defect<2, false>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, cur[0], e, cur[0],
+                         rhs, r, C->part_slots, (const double*)nullptr, inv_area, bsum, total, fine_lwt, first_e(0, 0));
+    else
+      hipLaunchKernelGGL((swm::dev::k_defect<1, false>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, cur[v0], e + o,
+                         cur[v0], rhs + o, r + o, C->part_slots, (const double*)nullptr, inv_area,
+                         bsum ? bsum + v0 : nullptr, total, fine_lwt, first_e(v0, o));
+  };
   // one FP32 correction cycle on fields v0.. (nv of them: both, or one);
   // the correction lands in e or, with another finest sweep count, in that
-  // level's scratch (same plane layout).  last: no further cycle follows
-  // (the fixed schedule's last one), so the defect need not form the next
-  // cycle's first sweep
+  // level's scratch (same plane layout)
   auto cycle = [&](int nv, int v0, bool last) {
     // the two fields' planes are contiguous: cur[0] + ns == cur[1] holds while they move together
     float* res = vcycle<float>(C, V, nv, e + (size_t)v0 * ns, r + (size_t)v0 * ns, true, C->mixed_fine_sweeps,
@@ -694,7 +707,10 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
     hipLaunchKernelGGL(swm::dev::k_track_resid, zgrid(C, 1), dim3(64), 0, C->stream, C->sums, 1.0 / (double)C->max_ig,
                        C->solver_tol, C->track, mb);
   } else {
-    for (int it = 0; it < C->mixed_min_cycles; ++it) cycle(2, 0, false);
+    // the defect before the first test is stored lazily (a converged solve
+    // needs neither its FP32 copy nor the next first sweep)
+    for (int it = 0; it < C->mixed_min_cycles; ++it) cycle(2, 0, it == C->mixed_min_cycles - 1);
+    bool lazy = C->mixed_min_cycles > 0;
     const int M = C->nz;
     std::vector<double> ms(2 * (size_t)M);
     swm_status st = member_norms(C, C->part_slots, (int)nb, {0, 1}, ms.data());
@@ -717,7 +733,9 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
         oth[1] = cur[1];
         cur[1] = cur[0] + ns;
       }
-      cycle(nv, v0, false);
+      if (lazy) redefect(nv, v0);
+      cycle(nv, v0, true);
+      lazy = true;
       st = member_norms(C, C->part_slots, (int)nb, bad, ms.data());
       if (st) return st;
       any = false;

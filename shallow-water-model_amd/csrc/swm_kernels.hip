@@ -1035,7 +1035,8 @@ __device__ __forceinline__ float first_sweep(bool real, const float* __restrict_
 // elsewhere), x' = x + e with CORR (the FP32 correction e, halos filled;
 // x' is stored to xn on every slot, halos included, so the iterate stays
 // halo-consistent) or x' = x.  d is stored in FP32 as the right-hand side of
-// the next correction cycle; per-block sums of d^2 for the convergence test
+// the next correction cycle (r == nullptr: not stored, where no cycle is
+// expected to follow); per-block sums of d^2 for the convergence test
 // (elliptic_solver.F:203-239).
 template <int NV, bool CORR>
 __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict__ x, const float* __restrict__ e,
@@ -1067,7 +1068,7 @@ __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict_
       const double w = lap_sym_apply(st, G, iim, jjm, ns, c, X);
       const double d = m != 0.0 ? (double)rhs_less_mean(ldg(b + (size_t)v * ns, s), bsum, v, total) - w : 0.0;
       if (CORR) stg(xn + (size_t)v * ns, s, gauge ? X(s) - gauge[v] * inv_area : X(s));
-      stg(r + (size_t)v * ns, s, (float)d);
+      if (r) stg(r + (size_t)v * ns, s, (float)d);
       if (e0) stg(e0 + (size_t)v * ns, s, first_sweep(rslot, lwt, s, (float)d));
       acc[v] = (d * d) * m;
     }
