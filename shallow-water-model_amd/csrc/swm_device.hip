@@ -684,13 +684,18 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
   // linear 2 x_n - x_(n-1) with two, x_n with one (or as set)
   const int order = std::min(C->mixed_extrapolate, C->nsol - 1);
   const bool extrap = order >= 1;
-  if (order >= 2)
-    hipLaunchKernelGGL((swm::dev::k_defect_extrap<2, true>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, C->area_mask,
-                       blk[0], C->xprev, blk[2], blk[1], rhs, r, C->part_slots, bsum, total, fine_lwt, first_e(0, 0));
-  else
-    hipLaunchKernelGGL((swm::dev::k_defect_extrap<2, false>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, C->area_mask,
-                       blk[0], extrap ? C->xprev : blk[0], (const double*)nullptr, blk[1], rhs, r, C->part_slots, bsum,
-                       total, fine_lwt, first_e(0, 0));
+  if (order >= 2) {
+    // the quadratic start pointwise, then its defect with one read per
+    // stencil point (measured: 25 us faster at g=9 than forming the start at
+    // the seven points of every stencil, three reads each)
+    hipLaunchKernelGGL(swm::dev::k_extrap_start, zgrid(C, nb), dim3(BLK), 0, C->stream, ns, C->area_mask, blk[0],
+                       C->xprev, blk[2], blk[1], C->part_slots, mb);
+    hipLaunchKernelGGL((swm::dev::k_defect<2, false>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, blk[1], e, blk[1],
+                       rhs, r, C->part_slots, (const double*)nullptr, inv_area, bsum, total, fine_lwt, first_e(0, 0));
+  } else {
+    hipLaunchKernelGGL(swm::dev::k_defect_extrap<2>, zgrid(C, nb), dim3(BLK), 0, C->stream, G64, C->area_mask, blk[0],
+                       extrap ? C->xprev : blk[0], blk[1], rhs, r, C->part_slots, bsum, total, fine_lwt, first_e(0, 0));
+  }
   if (extrap) {
     sum_final(C, C->part_slots + 2 * (size_t)nb, (int)nb, C->gauge, 2);
     shift[0] = shift[1] = true;

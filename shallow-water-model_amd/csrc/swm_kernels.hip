@@ -1076,26 +1076,23 @@ __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict_
   block_sums<NV>(acc, part);
 }
 
-// Mixed path, first FP64 defect of a solve from the extrapolated warm start:
+// Mixed path, first FP64 defect of a solve from the linear start
 // x0 = 2 x - xp (x the last solution, xp the one before; xp = x gives the
-// plain warm start exactly) or, Q, x0 = 3 x - 3 xp + xpp (xpp the one
-// before xp).  x0 is stored to xn on every slot and d = rhs - L(x0) as in
-// k_defect.  The solution's constant part is free (L of a constant is zero),
-// so extrapolating it would let it drift; the per-block sums of
-// area * (x0 - x) (columns NV.. of part) give the mean change, which the
-// first correction takes off again (k_defect's gauge).  With gauge
-// (k_defect) the stored iterate loses gauge[v] * inv_area; the defect, of
-// the unshifted iterate, is the same up to rounding.
-template <int NV, bool Q>
+// plain warm start exactly): x0 is stored to xn on every slot and
+// d = rhs - L(x0) as in k_defect.  The solution's constant part is free (L
+// of a constant is zero), so extrapolating it would let it drift; the
+// per-block sums of area * (x - xp) (columns NV.. of part) give the mean
+// change, which the first correction takes off again (k_defect's gauge).
+// With gauge (k_defect) the stored iterate loses gauge[v] * inv_area; the
+// defect, of the unshifted iterate, is the same up to rounding.
+template <int NV>
 __global__ void __launch_bounds__(256) k_defect_extrap(Geo G, const double* __restrict__ am,
                                                        const double* __restrict__ x, const double* __restrict__ xp,
-                                                       const double* __restrict__ xpp, double* __restrict__ xn,
-                                                       const float* __restrict__ b, float* __restrict__ r,
-                                                       double* __restrict__ part, const double* __restrict__ bsum,
-                                                       double total, const float* __restrict__ lwt,
-                                                       float* __restrict__ e0) {
+                                                       double* __restrict__ xn, const float* __restrict__ b,
+                                                       float* __restrict__ r, double* __restrict__ part,
+                                                       const double* __restrict__ bsum, double total,
+                                                       const float* __restrict__ lwt, float* __restrict__ e0) {
   at_member(G.mb, x, xp, xn, b, r, part);
-  if (Q) at_member(G.mb, xpp);
   if (bsum) at_member(G.mb, bsum);
   if (e0) at_member(G.mb, e0);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
@@ -1112,21 +1109,44 @@ __global__ void __launch_bounds__(256) k_defect_extrap(Geo G, const double* __re
     for (int v = 0; v < NV; ++v) {
       const double* xv = x + (size_t)v * ns;
       const double* pv = xp + (size_t)v * ns;
-      const double* qv = Q ? xpp + (size_t)v * ns : nullptr;
-      auto X = [&](int sl) {
-        return Q ? 3.0 * (ldg(xv, sl) - ldg(pv, sl)) + ldg(qv, sl) : 2.0 * ldg(xv, sl) - ldg(pv, sl);
-      };
+      auto X = [&](int sl) { return 2.0 * ldg(xv, sl) - ldg(pv, sl); };
       const double w = lap_sym_apply(st, G, iim, jjm, ns, c, X);
       const double d = m != 0.0 ? (double)rhs_less_mean(ldg(b + (size_t)v * ns, s), bsum, v, total) - w : 0.0;
-      const double x0 = X(s);
-      stg(xn + (size_t)v * ns, s, x0);
+      stg(xn + (size_t)v * ns, s, X(s));
       stg(r + (size_t)v * ns, s, (float)d);
       if (e0) stg(e0 + (size_t)v * ns, s, first_sweep(rslot, lwt, s, (float)d));
       acc[v] = (d * d) * m;
-      acc[NV + v] = ldg(am, s) * (Q ? x0 - ldg(xv, s) : ldg(xv, s) - ldg(pv, s));
+      acc[NV + v] = ldg(am, s) * (ldg(xv, s) - ldg(pv, s));
     }
   }
   block_sums<2 * NV>(acc, part);
+}
+
+// Mixed path, the quadratic start of a solve, pointwise: x0 = 3 (x - xp) +
+// xpp (x the last solution, xp and xpp the two before) stored to xn on every
+// slot, and the per-block sums of area * (x0 - x) of both fields in columns
+// 2 and 3 of part (the mean change, which the first correction takes off);
+// k_defect<2, false> then forms the defect of x0 with one read per stencil
+// point instead of three.
+__global__ void __launch_bounds__(256) k_extrap_start(int ns, const double* __restrict__ am,
+                                                      const double* __restrict__ x, const double* __restrict__ xp,
+                                                      const double* __restrict__ xpp, double* __restrict__ xn,
+                                                      double* __restrict__ part, size_t mb) {
+  at_member(mb, x, xp, xpp, xn, part);
+  const int s = xcd_slot();
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (s < ns) {
+    const double a = ldg(am, s);
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const size_t o = (size_t)v * ns;
+      const double xv = ldg(x + o, s);
+      const double x0 = 3.0 * (xv - ldg(xp + o, s)) + ldg(xpp + o, s);
+      stg(xn + o, s, x0);
+      acc[2 + v] = a * (x0 - xv);
+    }
+  }
+  block_sums<4>(acc, part);
 }
 
 // fixed-schedule solves: per member (blockIdx.z), the largest mean squared
