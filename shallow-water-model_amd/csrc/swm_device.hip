@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -99,7 +100,12 @@ struct swm_ctx {
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;   // the tracer transport beside the dynamics (overlap)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  int overlap = 0;
+  int overlap = 3;             // tracer transport on stream2: 0 off, 1 from the mass update, n >= 2 from level g - n + 1 of the first V-cycle
+  // overlap: work issued by the first V-cycle of a step when it reaches
+  // level hook_level on the way down (the tracer transport, started where
+  // the cycle's launches become latency-bound)
+  std::function<void()> hook;
+  int hook_level = 0;
   Arena arena;             // shared: metrics, halo tables, multigrid weights and maps
   size_t arena_bytes = 0;
   // ensemble members (same grid and settings).  Member m's state and scratch
@@ -385,6 +391,11 @@ R* vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_ze
       hipLaunchKernelGGL((swm::dev::k_inject<R, 1>), grid, blk, 0, C->stream, Fl.L, Cl.L, xa, B(l), B(l - 1), z0);
   };
   for (int l = G; l > top; --l) {
+    if (C->hook && l == C->hook_level) {
+      std::function<void()> h = std::move(C->hook);
+      C->hook = nullptr;
+      h();
+    }
     if (l == G && own) {
       fine(fine_sweeps, true);
       inject(l, fa, false);
@@ -867,34 +878,56 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   // Tracer FCT (tracer_fct.F:22-670).  The tracers depend on the new mass
   // and the mass flux only, and nothing after them but the final halo fill
   // reads them: with overlap they run on the second stream beside the
-  // vorticity, divergence and elliptic solve.
-  if (overlap) {
-    HIP_TRY(hipEventRecord(C->ev_fork, S));
-    HIP_TRY(hipStreamWaitEvent(C->stream2, C->ev_fork, 0));
-    C->stream = C->stream2;
-  }
-  const hipStream_t St = C->stream;
-  for (int q = 0; q < (fct ? C->nt : 0); ++q) {
-    R* t3 = V.tr[n3][q];
-    R* t4 = V.tr[n4][q];
-    if (q > 0)
-      hipLaunchKernelGGL(swm::dev::k_tr_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, w3, w2, w1, t3, t4,
-                         V.told[q], V.mass_flux, V.f1, V.f2);
-    gather<R>(C, C->edge, {V.f1, V.f2});
-    hipLaunchKernelGGL(swm::dev::k_tr_low<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, V.h[n3], t3, V.f1,
-                       V.h[n4], V.ml, V.tlow);
-    gather<R>(C, C->face, {V.ml, V.tlow});
-    hipLaunchKernelGGL(swm::dev::k_tr_limiter<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, t3, V.ml, V.tlow,
-                       V.mss_low, V.f1, V.f2, V.anti, V.bdn, V.bup);
-    gather<R>(C, C->face, {V.bdn, V.bup});
-    hipLaunchKernelGGL(swm::dev::k_tr_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, V.bdn, V.bup, V.anti);
-    gather<R>(C, C->edge, {V.anti});
-    hipLaunchKernelGGL(swm::dev::k_tr_final<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, V.ml, V.anti, V.h[n4], t4,
-                       V.told[q]);
-  }
-  if (overlap) {
-    HIP_TRY(hipEventRecord(C->ev_join, C->stream2));
-    C->stream = S;
+  // vorticity, divergence and elliptic solve, started here (mode 1) or by
+  // the first V-cycle when it reaches a level of latency-bound launches
+  // (mode >= 2: level g - mode + 1, through C->hook).
+  swm_status tr_st = SWM_OK;
+  auto tracers = [&]() {
+    const hipStream_t main_stream = C->stream;
+    if (overlap) {
+      hipError_t he = hipEventRecord(C->ev_fork, main_stream);
+      if (he == hipSuccess) he = hipStreamWaitEvent(C->stream2, C->ev_fork, 0);
+      if (he != hipSuccess) {
+        swm::set_error(std::string("tracer overlap: ") + hipGetErrorString(he));
+        tr_st = SWM_ERR_HIP;
+        return;
+      }
+      C->stream = C->stream2;
+    }
+    const hipStream_t St = C->stream;
+    for (int q = 0; q < (fct ? C->nt : 0); ++q) {
+      R* t3 = V.tr[n3][q];
+      R* t4 = V.tr[n4][q];
+      if (q > 0)
+        hipLaunchKernelGGL(swm::dev::k_tr_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, w3, w2, w1, t3, t4,
+                           V.told[q], V.mass_flux, V.f1, V.f2);
+      gather<R>(C, C->edge, {V.f1, V.f2});
+      hipLaunchKernelGGL(swm::dev::k_tr_low<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, V.h[n3], t3, V.f1,
+                         V.h[n4], V.ml, V.tlow);
+      gather<R>(C, C->face, {V.ml, V.tlow});
+      hipLaunchKernelGGL(swm::dev::k_tr_limiter<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, t3, V.ml, V.tlow,
+                         V.mss_low, V.f1, V.f2, V.anti, V.bdn, V.bup);
+      gather<R>(C, C->face, {V.bdn, V.bup});
+      hipLaunchKernelGGL(swm::dev::k_tr_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, V.bdn, V.bup, V.anti);
+      gather<R>(C, C->edge, {V.anti});
+      hipLaunchKernelGGL(swm::dev::k_tr_final<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, V.ml, V.anti, V.h[n4],
+                         t4, V.told[q]);
+    }
+    if (overlap) {
+      const hipError_t he = hipEventRecord(C->ev_join, C->stream2);
+      if (he != hipSuccess) {
+        swm::set_error(std::string("tracer overlap: ") + hipGetErrorString(he));
+        tr_st = SWM_ERR_HIP;
+      }
+      C->stream = main_stream;
+    }
+  };
+  const int hook_level = C->g - C->overlap + 1;
+  if (overlap && C->overlap >= 2 && hook_level > swm::dev::CO_MAXLEV) {
+    C->hook = tracers;
+    C->hook_level = hook_level;
+  } else {
+    tracers();
   }
   // vorticity and divergence (step.F:143-163, :336-429)
   hipLaunchKernelGGL(swm::dev::k_vort_curl<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.mass_flux, V.h[n3], V.eta[n3], V.hc,
@@ -913,7 +946,13 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   const double* bsum = nullptr;
   if constexpr (sizeof(R) == 8) st = elliptic(C, V.rhs, C->xout, 2);
   else st = elliptic_mixed(C, V.rhs, C->xout, (int)nb, &bsum);
+  if (C->hook) {   // a solve that did not reach the hook's level
+    std::function<void()> h = std::move(C->hook);
+    C->hook = nullptr;
+    h();
+  }
   if (st) return st;
+  if (tr_st) return tr_st;
   if constexpr (mixed) {
     // the new vorticity (the right-hand side less its mean) with the velocity
     hipLaunchKernelGGL(swm::dev::k_velocity<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, C->xout, C->xout + ns,
@@ -1875,9 +1914,9 @@ swm_status swm_set_mixed_smoother(swm_ctx* C, int fine_sweeps, int half_weights)
   return SWM_OK;
 }
 
-swm_status swm_set_overlap(swm_ctx* C, int enable) {
-  if (!C) { swm::set_error("swm_set_overlap: bad argument"); return SWM_ERR_ARG; }
-  C->overlap = enable ? 1 : 0;
+swm_status swm_set_overlap(swm_ctx* C, int mode) {
+  if (!C || mode < 0) { swm::set_error("swm_set_overlap: bad argument"); return SWM_ERR_ARG; }
+  C->overlap = mode;
   C->epoch += 1;
   return SWM_OK;
 }

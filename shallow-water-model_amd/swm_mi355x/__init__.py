@@ -387,8 +387,10 @@ class Model:
         return self
 
     def set_overlap(self, enable=True):
-        """Tracer transport on a second stream beside the dynamics (FCT)."""
-        _check(lib.swm_set_overlap(self._h, 1 if enable else 0))
+        """Tracer transport on a second stream beside the dynamics (FCT):
+        False/0 off, True/1 from the mass update on, n >= 2 started by the
+        first V-cycle at level g - n + 1."""
+        _check(lib.swm_set_overlap(self._h, int(enable)))
 
     def set_mixed_coarse(self, direct=True):
         """Mixed path: the three coarsest levels as one dense solve (True)

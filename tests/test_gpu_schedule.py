@@ -2,7 +2,8 @@
 change how the mixed solve converges:
 
   - stream overlap (swm_set_overlap): the tracer transport on a second
-    stream gives the same fields bit for bit, FP64 and mixed;
+    stream, started at once or by the first V-cycle, gives the same fields
+    bit for bit, FP64 and mixed;
   - the multigrid kernel fusions (swm_set_fusion) give the same fields bit
     for bit as the unfused cycle, FP64 and mixed, for two finest-level sweep
     counts, over levels 6..4 with their own sweeps;
@@ -40,12 +41,13 @@ def _model(grid, precision, **kw):
     return m
 
 
-@pytest.mark.parametrize("precision", [S.SWM_FP64, S.SWM_MIXED])
-def test_overlap_gives_the_same_fields(precision):
-    print(f"case overlap precision={precision}", flush=True)
-    grid = S.Grid(4)
-    a = _model(grid, precision, overlap=False)
-    b = _model(grid, precision, overlap=True)
+@pytest.mark.parametrize("precision,mode", [(S.SWM_FP64, 1), (S.SWM_MIXED, 1), (S.SWM_FP64, 3), (S.SWM_MIXED, 2),
+                                            (S.SWM_MIXED, 3)])
+def test_overlap_gives_the_same_fields(precision, mode):
+    print(f"case overlap precision={precision} mode={mode}", flush=True)
+    grid = S.Grid(6)
+    a = _model(grid, precision, overlap=0)
+    b = _model(grid, precision, overlap=mode)
     a.step(7)
     b.step(7)
     for n in FIELDS:
