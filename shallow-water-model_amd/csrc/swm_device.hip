@@ -1549,7 +1549,10 @@ swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
       std::vector<float> M;
       std::vector<int> cell;
       coarse_direct_matrix(Gh.levels[swm::dev::CO_MAXLEV - 1].lap_w, swm::dev::CO_MAXLEV, M, cell);
-      if ((int)cell.size() != C->cd_n) { swm::set_error("coarse solve: level size"); return SWM_ERR_DIMS; }
+      if ((int)cell.size() != C->cd_n || C->cd_n > swm::dev::CD_MAXN) {
+        swm::set_error("coarse solve: level size");
+        return SWM_ERR_DIMS;
+      }
       HIP_TRY(hipMemcpy(C->cd_m, M.data(), M.size() * sizeof(float), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(C->cd_cell, cell.data(), cell.size() * sizeof(int), hipMemcpyHostToDevice));
     }

@@ -827,7 +827,18 @@ __global__ void __launch_bounds__(SUM_THREADS) k_sum_final(const double* __restr
   __shared__ double ss[SUM_THREADS], cc[SUM_THREADS];
   const int f = blockIdx.x, t = threadIdx.x;
   double s = 0.0, c = 0.0;
-  for (int b = t; b < nb; b += SUM_THREADS) two_sum_add(s, c, part[f * nb + b]);
+  // the partials in batches of eight independent loads, added in the same
+  // order as one at a time
+  const double* __restrict__ pf = part + (size_t)f * nb;
+  int b = t;
+  for (; b + 7 * SUM_THREADS < nb; b += 8 * SUM_THREADS) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = pf[b + q * SUM_THREADS];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) two_sum_add(s, c, v[q]);
+  }
+  for (; b < nb; b += SUM_THREADS) two_sum_add(s, c, pf[b]);
   ss[t] = s;
   cc[t] = c;
   __syncthreads();
@@ -2056,6 +2067,7 @@ __global__ void __launch_bounds__(CO_THREADS) k_coarse_cycle(CoarseArgs<R> P) {
   for (int t = threadIdx.x; t < n; t += CO_THREADS) p_alpha[t] = at[t];
 }
 
+constexpr int CD_MAXN = 10 * 64 + 2;   // real cells of level 3
 // Mixed path: the coarse correction of the three coarsest levels as one
 // dense solve.  alpha = M beta at the level's real cells (cell[0..n)), with M
 // the FP32 n x n block of the inverse of the level's Laplacian bordered by
@@ -2070,6 +2082,15 @@ __global__ void __launch_bounds__(256) k_coarse_direct(const float* __restrict__
                                                        int ns, const float* __restrict__ beta,
                                                        float* __restrict__ alpha, size_t mb) {
   at_member(mb, beta, alpha);
+  // beta at the real cells, gathered once per workgroup into LDS (at most
+  // CD_MAXN cells per field), so the rows' dot products read M and LDS only
+  __shared__ float sb[NV][CD_MAXN];
+  for (int c = threadIdx.x; c < n; c += bdim()) {
+    const int sl = ldg(cell, c);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) sb[v][c] = ldg(beta + (size_t)v * ns, sl);
+  }
+  __syncthreads();
   const int r = (int)(blockIdx.x * (bdim() >> 6) + (threadIdx.x >> 6));
   const int lane = (int)(threadIdx.x & 63);
   if (r >= n) return;   // whole waves leave together
@@ -2077,11 +2098,20 @@ __global__ void __launch_bounds__(256) k_coarse_direct(const float* __restrict__
   float acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = 0.0f;
-  for (int c = lane; c < n; c += 64) {
-    const float m = row[c];
-    const int sl = cell[c];
+  // all of a lane's (at most 11) loads of the row issued together
+  float mr[(CD_MAXN + 63) / 64];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] = acc[v] + m * beta[(size_t)v * ns + sl];
+  for (int q = 0; q < (CD_MAXN + 63) / 64; ++q) {
+    const int c = lane + 64 * q;
+    mr[q] = c < n ? row[c] : 0.0f;
+  }
+#pragma unroll
+  for (int q = 0; q < (CD_MAXN + 63) / 64; ++q) {
+    const int c = lane + 64 * q;
+    if (c < n) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) acc[v] = acc[v] + mr[q] * sb[v][c];
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -2090,7 +2120,7 @@ __global__ void __launch_bounds__(256) k_coarse_direct(const float* __restrict__
   }
   if (lane == 0) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) alpha[(size_t)v * ns + cell[r]] = acc[v];
+    for (int v = 0; v < NV; ++v) alpha[(size_t)v * ns + ldg(cell, r)] = acc[v];
   }
 }
 
