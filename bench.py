@@ -93,10 +93,10 @@ def timed(model, warmup, steps):
     model.step(warmup)
     model.synchronize()
     s0, v0, c0 = model.solver_stats()
-    # every 7th finest-level smoother launch is timed with HIP events (the
-    # event records would cost the step ~3 us each if every launch had them;
-    # 7 is prime to the 8 or 12 finest sweeps of a step, so the sample
-    # covers every position in the V-cycles)
+    # every 7th pair of back-to-back finest-level smoother launches is timed
+    # with HIP events (the event records would cost the step ~3 us each if
+    # every pair had them; 7 is prime to the pairs of a step, 1 mixed and 6
+    # FP64, so the sample covers every position in the V-cycles)
     model.profile(True, every=7)
     t1 = time.perf_counter()
     model.step(steps)

@@ -303,8 +303,10 @@ swm_status swm_timing(const swm_ctx* ctx, double* step_ms, double* kernel_ms_mea
                       double* kernel_bytes, int* kernel_launches);
 
 /* HIP-event timing of the finest-level multigrid smoother inside swm_step,
- * read back by swm_timing: every launch (1), every n-th (n > 1: the event
- * records cost the step a few microseconds each), or off (0). */
+ * read back by swm_timing: two back-to-back sweeps (a level's smoothing
+ * pair, neither from zero) between one pair of events, every such pair (1),
+ * every n-th (n > 1: the event records cost the step a few microseconds
+ * each), or off (0); the mean is per launch. */
 swm_status swm_profile(swm_ctx* ctx, int enable);
 
 /* Seeded synthetic initial state built on the device: harmonic sums of
