@@ -989,7 +989,11 @@ __device__ __forceinline__ LapSym lap_sym(const Geo& G, int iim, int jjm, int ns
   t.ai = ldg(G.aim, s);
   t.real = t.ai != 0.0;
   t.pole = (i == 1 && j == jjm - 1) || (i == iim - 1 && j == 1);
-  if (!t.real) return t;
+  if (!t.real) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) t.nb[q] = s;   // in-range indices; the result is 0 there
+    return t;
+  }
   t.nb[0] = SL(i - 1, j, k); t.nb[1] = SL(i - 1, j - 1, k); t.nb[2] = SL(i, j - 1, k);
   t.nb[3] = SL(i + 1, j, k); t.nb[4] = SL(i + 1, j + 1, k); t.nb[5] = SL(i, j + 1, k);
   if (t.pole) {
@@ -1071,16 +1075,41 @@ __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict_
     const double m = st.real ? 1.0 : 0.0;
     // the first sweep's slots are k_relax's (interior and pole slots)
     const bool rslot = interior(c, iim, jjm) || (c.i == 1 && c.j == jjm - 1) || (c.i == iim - 1 && c.j == 1);
+    // every value of both fields loaded before any is used or stored, so
+    // that the loads of all seven points overlap
+    double xq[NV][7];
+    float eq[NV][7], bq[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const double* xv = x + (size_t)v * ns;
       const float* ev = e + (size_t)v * ns;
-      auto X = [&](int sl) { return CORR ? ldg(xv, sl) + (double)ldg(ev, sl) : ldg(xv, sl); };
-      const double w = lap_sym_apply(st, G, iim, jjm, ns, c, X);
-      const double d = m != 0.0 ? (double)rhs_less_mean(ldg(b + (size_t)v * ns, s), bsum, v, total) - w : 0.0;
-      if (CORR) stg(xn + (size_t)v * ns, s, gauge ? X(s) - gauge[v] * inv_area : X(s));
+#pragma unroll
+      for (int q = 0; q < 7; ++q) {
+        const int sl = q < 6 ? st.nb[q] : s;
+        xq[v][q] = ldg(xv, sl);
+        if (CORR) eq[v][q] = ldg(ev, sl);
+      }
+      bq[v] = ldg(b + (size_t)v * ns, s);
+    }
+    const float wt = e0 && rslot ? ldg(lwt, s) : 0.0f;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      double X[7];
+#pragma unroll
+      for (int q = 0; q < 7; ++q) X[q] = CORR ? xq[v][q] + (double)eq[v][q] : xq[v][q];
+      // lap_sym_apply's terms in its order
+      double w = 0.0;
+      if (st.real) {
+        const double x0 = X[6];
+        w = st.S[0] * (X[0] - x0);
+#pragma unroll
+        for (int q = 1; q < 6; ++q) w = w + st.S[q] * (X[q] - x0);
+        w = w * st.ai;
+      }
+      const double d = m != 0.0 ? (double)rhs_less_mean(bq[v], bsum, v, total) - w : 0.0;
+      if (CORR) stg(xn + (size_t)v * ns, s, gauge ? X[6] - gauge[v] * inv_area : X[6]);
       if (r) stg(r + (size_t)v * ns, s, (float)d);
-      if (e0) stg(e0 + (size_t)v * ns, s, first_sweep(rslot, lwt, s, (float)d));
+      if (e0) stg(e0 + (size_t)v * ns, s, rslot ? zero_sweep<float>(wt, (float)d) : 0.0f);
       acc[v] = (d * d) * m;
     }
   }
