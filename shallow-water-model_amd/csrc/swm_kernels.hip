@@ -298,24 +298,50 @@ __global__ void k_mass_low_beta(GeoT<R> G, R dt, R w3, R w2, R w1, const R* __re
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
-  auto A = [&](int e, int sl) {
+  // every value first (the loads overlap): the three edges at the slot and
+  // the one edge each at (i+1, j), (i+1, j+1), (i, j+1), those clamped into
+  // the array where they fall outside it (their values are then not used)
+  const int nbs[3] = {min(SL(i + 1, j, k), ns - 1), min(SL(i + 1, j + 1, k), ns - 1), min(SL(i, j + 1, k), ns - 1)};
+  R fo[3], ho3[3], ho2[3], ho1[3], fn[3], hn3[3], hn2[3], hn1[3];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const size_t o = (size_t)e * ns;
+    fo[e] = ldg(flx1 + o, s); ho3[e] = ldg(m3 + o, s); ho2[e] = ldg(m2 + o, s); ho1[e] = ldg(m1 + o, s);
+    fn[e] = ldg(flx1 + o, nbs[e]); hn3[e] = ldg(m3 + o, nbs[e]); hn2[e] = ldg(m2 + o, nbs[e]); hn1[e] = ldg(m1 + o, nbs[e]);
+  }
+  const R hs3 = ldg(h3, s), ai = ldg(G.area_inv, s), ar = ldg(G.area, s);
+  auto AB = [&](R x3, R x2, R x1, R f) {
     R f2 = R(0);
-    f2 = f2 + w3 * m3[e * ns + sl];
-    f2 = f2 + w2 * m2[e * ns + sl];
-    f2 = f2 + w1 * m1[e * ns + sl];
-    return f2 - flx1[e * ns + sl];
+    f2 = f2 + w3 * x3;
+    f2 = f2 + w2 * x2;
+    f2 = f2 + w1 * x1;
+    return f2 - f;
   };
-  for (int e = 0; e < 3; ++e) anti[e * ns + s] = A(e, s);
-  bool set;
-  const R ml = edge_update<R>(iim, jjm, ns, c, h3[s], dt * G.area_inv[s], flx1, set);
-  mss_low[s] = ml;
+  R ao[3];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    ao[e] = AB(ho3[e], ho2[e], ho1[e], fo[e]);
+    stg(anti + (size_t)e * ns, s, ao[e]);
+  }
+  // edge_update's forms, on the loaded fluxes
+  const R dt_ai = dt * ai;
+  R ml;
+  if (i == 1 && j == jjm - 1)
+    ml = hs3 + dt_ai * (fo[1] + fo[2] + fn[0] + fn[1] + fn[2]);
+  else if (i == iim - 1 && j == 1)
+    ml = hs3 + dt_ai * (fo[0] + fo[1] + fn[0] + fn[1] + fn[2]);
+  else if (i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1)
+    ml = hs3 - dt_ai * ((fo[0] + fo[1] + fo[2]) - (fn[0] + fn[1] + fn[2]));
+  else
+    ml = R(0);
+  stg(mss_low, s, ml);
   if (i <= iim - 1 && j <= jjm - 1) {
-    const R a_out = (rmax(R(0), A(0, s)) + rmax(R(0), A(1, s)) + rmax(R(0), A(2, s))) -
-                    (rmin(R(0), A(0, SL(i + 1, j, k))) + rmin(R(0), A(1, SL(i + 1, j + 1, k))) +
-                     rmin(R(0), A(2, SL(i, j + 1, k))));
-    beta[s] = G.area[s] * (habs(G, ml) - R(1.0e-6)) / (dt * (a_out + R(1.0e-18)));
+    const R a_out = (rmax(R(0), ao[0]) + rmax(R(0), ao[1]) + rmax(R(0), ao[2])) -
+                    (rmin(R(0), AB(hn3[0], hn2[0], hn1[0], fn[0])) + rmin(R(0), AB(hn3[1], hn2[1], hn1[1], fn[1])) +
+                     rmin(R(0), AB(hn3[2], hn2[2], hn1[2], fn[2])));
+    stg(beta, s, ar * (habs(G, ml) - R(1.0e-6)) / (dt * (a_out + R(1.0e-18))));
   } else {
-    beta[s] = R(0);
+    stg(beta, s, R(0));
   }
 }
 
