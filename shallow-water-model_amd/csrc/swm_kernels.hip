@@ -404,23 +404,77 @@ __global__ void k_mass_final(GeoT<R> G, R dt, const R* __restrict__ flx1, const 
   const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
-  R F[3];
+  const int i = c.i, j = c.j, k = c.k;
+  auto cl = [&](int sl) { return min(max(sl, 0), ns - 1); };
+  // every value first, so that the loads overlap (indices clamped into the
+  // array; values a form does not use are not used): the slot's three edges
+  // of both fluxes, the one edge each of anti at (i+1, j), (i+1, j+1),
+  // (i, j+1) (edge_update), and the first tracer at the slot and at the
+  // three upstream neighbours (tr_fluxes_at)
+  const int en[3] = {cl(SL(i + 1, j, k)), cl(SL(i + 1, j + 1, k)), cl(SL(i, j + 1, k))};
+  R fl[3], an[6];
+#pragma unroll
   for (int e = 0; e < 3; ++e) {
-    F[e] = flx1[e * ns + s] + anti[e * ns + s];
-    mflux[e * ns + s] = F[e];
+    const size_t o = (size_t)e * ns;
+    fl[e] = ldg(flx1 + o, s);
+    an[e] = ldg(anti + o, s);
+    an[3 + e] = ldg(anti + o, en[e]);
   }
-  bool set;
-  h4[s] = edge_update<R>(iim, jjm, ns, c, mss_low[s], dt * G.area_inv[s], anti, set);
+  const R base = ldg(mss_low, s), ai = ldg(G.area_inv, s);
+  const bool tr = T.t3 != nullptr;
+  const R* t3 = T.t3;
+  const R* t4 = T.t4;
+  const R* told = T.told;
+  R* f1 = T.f1;
+  R* f2 = T.f2;
+  if (tr) at_member(G.mb, t3, t4, told, f1, f2);
+  const bool in = interior(c, iim, jjm);
+  const int up[3] = {cl(SL(i - 1, j, k)), cl(SL(i - 1, j - 1, k)), cl(SL(i, j - 1, k))};
+  R a3[4], a4[4], ao[4];
+  if (tr) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int sl = q == 0 ? s : up[q - 1];
+      a3[q] = ldg(t3, sl);
+      a4[q] = ldg(t4, sl);
+      ao[q] = ldg(told, sl);
+    }
+  }
+  R F[3];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    F[e] = fl[e] + an[e];
+    stg(mflux + (size_t)e * ns, s, F[e]);
+  }
+  // edge_update's forms, on the loaded antidiffusive fluxes
+  const R dt_ai = dt * ai;
+  R hn;
+  if (i == 1 && j == jjm - 1)
+    hn = base + dt_ai * (an[1] + an[2] + an[3] + an[4] + an[5]);
+  else if (i == iim - 1 && j == 1)
+    hn = base + dt_ai * (an[0] + an[1] + an[3] + an[4] + an[5]);
+  else if (i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1)
+    hn = base - dt_ai * ((an[0] + an[1] + an[2]) - (an[3] + an[4] + an[5]));
+  else
+    hn = R(0);
+  stg(h4, s, hn);
   // the first tracer's edge fluxes take the total flux of the slot's own
-  // edges only, which this slot has just formed (k_tr_fluxes fused)
-  if (T.t3) {
-    const R* t3 = T.t3;
-    const R* t4 = T.t4;
-    const R* told = T.told;
-    R* f1 = T.f1;
-    R* f2 = T.f2;
-    at_member(G.mb, t3, t4, told, f1, f2);
-    tr_fluxes_at<R>(iim, jjm, ns, c, s, T.w3, T.w2, T.w1, t3, t4, told, F, f1, f2);
+  // edges only, which this slot has just formed (k_tr_fluxes fused;
+  // tr_fluxes_at's arithmetic)
+  if (tr) {
+    const R w3 = T.w3, w2 = T.w2, w1 = T.w1;
+    if (!in) {
+#pragma unroll
+      for (int e = 0; e < 3; ++e) { stg(f1 + (size_t)e * ns, s, R(0)); stg(f2 + (size_t)e * ns, s, R(0)); }
+    } else {
+      auto TH = [&](int q) { return w3 * a3[q] + w2 * a4[q] + w1 * ao[q]; };
+      const R ths = TH(0);
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        stg(f1 + (size_t)e * ns, s, a3[0] * rmax(R(0), F[e]) + a3[1 + e] * rmin(R(0), F[e]));
+        stg(f2 + (size_t)e * ns, s, R(0.5) * (ths + TH(1 + e)) * F[e]);
+      }
+    }
   }
 }
 
