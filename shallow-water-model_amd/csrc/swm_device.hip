@@ -589,19 +589,19 @@ swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
 
 // Mixed path: the same solve as FP64 defect correction.  The FP32 right-hand
 // side (mean removed with FP64 sums) is the problem.  The solve starts from
-// the linear extrapolation 2 x_n - x_(n-1) of the last two solutions (the
-// last one alone after an initialisation), which is second-order accurate in
-// dt where the reference's warm start is first-order, and forms its FP64
-// defect r = rhs - L(x).  A cycle runs one FP32 V-cycle on L(e) = r from
-// e = 0, fills the halos of e, and one pass (k_defect) adds e to the iterate
-// and forms the new FP64 defect with the sums of r^2.  One cycle, then the
-// criterion before each further cycle, at most solver_max_cycles cycles
-// (the start and the cycles before the first check are settings:
-// swm_set_mixed_start).  The defect is formed afresh in FP64 every
-// cycle, so the FP32 cycles only ever see a small, accurately known
-// residual.  The iterates move between three FP64 buffers (xout, xalt,
-// xprev), which are renamed at the end so that xout holds the solution and
-// xprev the one before.
+// the quadratic extrapolation 3 x_n - 3 x_(n-1) + x_(n-2) of the last three
+// solutions (the linear 2 x_n - x_(n-1) with two, the last one alone after
+// an initialisation), third-order accurate in dt where the reference's warm
+// start is first-order, and forms its FP64 defect r = rhs - L(x).  A cycle
+// runs one FP32 V-cycle on L(e) = r from e = 0, fills the halos of e, and
+// one pass (k_defect) adds e to the iterate and forms the new FP64 defect
+// with the sums of r^2.  One cycle, then the criterion before each further
+// cycle, at most solver_max_cycles cycles (the start and the cycles before
+// the first check are settings: swm_set_mixed_start).  The defect is formed
+// afresh in FP64 every cycle, so the FP32 cycles only ever see a small,
+// accurately known residual.  The iterates move between FP64 buffers that
+// are renamed at the end: xout holds the solution, xprev and xprev2 the two
+// before.
 swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, const double** bsum_out) {
   const int ns = C->ns;
   View<float>& V = C->m;
