@@ -268,7 +268,13 @@ void copy_state(swm_ctx* C, double* y, const double* x, size_t n) {
 // compensated sums of nf columns of block partials, per member in play
 void sum_final(swm_ctx* C, const double* part, int nb, double* out, int nf) {
   hipLaunchKernelGGL(swm::dev::k_sum_final, zgrid(C, (unsigned)nf), dim3(swm::dev::SUM_THREADS), 0, C->stream, part,
-                     nb, out, C->mb);
+                     nb, out, C->mb, nf, (const double*)nullptr, 0, (double*)nullptr);
+}
+// two sets of columns in one launch (nf1 of part, nf2 of part2)
+void sum_final2(swm_ctx* C, const double* part, int nb, double* out, int nf1, const double* part2, int nb2,
+                double* out2, int nf2) {
+  hipLaunchKernelGGL(swm::dev::k_sum_final, zgrid(C, (unsigned)(nf1 + nf2)), dim3(swm::dev::SUM_THREADS), 0, C->stream,
+                     part, nb, out, C->mb, nf1, part2, nb2, out2);
 }
 
 // ------------------------------------------------------------- multigrid --
@@ -617,7 +623,13 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
   // partials; the consumers take the mean off on the fly.  The relative
   // criterion needs the mean-free right-hand side in memory for its norms.
   const bool fixed = C->fixed_cycles > 0;
-  sum_final(C, C->part_rhs, nb_rhs, C->rmean, 2);
+  // the start: 3 x_n - 3 x_(n-1) + x_(n-2) with three solutions held, the
+  // linear 2 x_n - x_(n-1) with two, x_n with one (or as set)
+  const int order = std::min(C->mixed_extrapolate, C->nsol - 1);
+  // with the quadratic start (and the absolute criterion) the means' sums
+  // are formed in one launch with the start's gauge sums
+  const bool late_mean = order >= 2 && !(C->solver_relative && !fixed);
+  if (!late_mean) sum_final(C, C->part_rhs, nb_rhs, C->rmean, 2);
   const double* bsum = C->rmean;
   if (C->solver_relative && !fixed) {
     hipLaunchKernelGGL(swm::dev::k_sub_mean<float>, zgrid(C, nblk(ns)), dim3(BLK), 0, C->stream, ns, C->rmean,
@@ -686,9 +698,6 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
     launch_defect(nv, v0, last);
     e = C->mx_e;
   };
-  // the start: 3 x_n - 3 x_(n-1) + x_(n-2) with three solutions held, the
-  // linear 2 x_n - x_(n-1) with two, x_n with one (or as set)
-  const int order = std::min(C->mixed_extrapolate, C->nsol - 1);
   const bool extrap = order >= 1;
   if (order >= 2) {
     // the quadratic start pointwise, then its defect with one read per
@@ -696,6 +705,8 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
     // the seven points of every stencil, three reads each)
     hipLaunchKernelGGL(swm::dev::k_extrap_start, zgrid(C, nb), dim3(BLK), 0, C->stream, ns, C->area_mask, blk[0],
                        C->xprev, blk[2], blk[1], C->part_slots, mb);
+    if (late_mean)
+      sum_final2(C, C->part_rhs, nb_rhs, C->rmean, 2, C->part_slots + 2 * (size_t)nb, (int)nb, C->gauge, 2);
     hipLaunchKernelGGL((swm::dev::k_defect<2, false>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, blk[1], e, blk[1],
                        rhs, r, C->part_slots, (const double*)nullptr, inv_area, bsum, total, fine_lwt, first_e(0, 0));
   } else {
@@ -703,7 +714,7 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
                        extrap ? C->xprev : blk[0], blk[1], rhs, r, C->part_slots, bsum, total, fine_lwt, first_e(0, 0));
   }
   if (extrap) {
-    sum_final(C, C->part_slots + 2 * (size_t)nb, (int)nb, C->gauge, 2);
+    if (!late_mean) sum_final(C, C->part_slots + 2 * (size_t)nb, (int)nb, C->gauge, 2);
     shift[0] = shift[1] = true;
   }
   int iter;

@@ -901,11 +901,21 @@ __device__ __forceinline__ void two_sum_add(double& s, double& c, double v) {
   s = t;
 }
 constexpr int SUM_THREADS = 1024;
+// (blocks nf1.. sum the columns of a second set part2 / nb2 into out2)
 __global__ void __launch_bounds__(SUM_THREADS) k_sum_final(const double* __restrict__ part, int nb,
-                                                           double* __restrict__ out, size_t mb) {
+                                                           double* __restrict__ out, size_t mb, int nf1,
+                                                           const double* __restrict__ part2, int nb2,
+                                                           double* __restrict__ out2) {
+  int f = blockIdx.x;
+  if (part2 && f >= nf1) {
+    part = part2;
+    nb = nb2;
+    out = out2;
+    f -= nf1;
+  }
   at_member(mb, part, out);
   __shared__ double ss[SUM_THREADS], cc[SUM_THREADS];
-  const int f = blockIdx.x, t = threadIdx.x;
+  const int t = threadIdx.x;
   double s = 0.0, c = 0.0;
   // the partials in batches of eight independent loads, added in the same
   // order as one at a time
