@@ -151,6 +151,7 @@ struct swm_ctx {
   double *part = nullptr, *sums = nullptr;
   double* part_budget = nullptr;   // partial sums of the budgets, (8 + BUDGET_MAXTR) x npart
   double* host_sums = nullptr;   // pinned, 64 values per member
+  double* host_sums_dev = nullptr;   // its device address (kernels write there directly)
   int npart = 0;
   // time stepping
   int step_count = 0, n3 = 1, n4 = 2, n1f = 1, n2f = 2, n3f = 3;
@@ -490,10 +491,18 @@ bool converged(const swm_ctx* C, int m, int v, double ms) {
 // last_resid takes the largest over the members
 swm_status member_norms(swm_ctx* C, const double* part, int nb, const std::vector<int>& vars, double* ms) {
   const int nv = (int)vars.size();
-  sum_final(C, part, nb, C->sums, nv);
   std::vector<double> raw((size_t)nv * C->nz);
-  swm_status st = read_sums(C, nv, raw.data());
-  if (st) return st;
+  if (C->nz == 1 && C->host_sums_dev) {
+    // one member: the final sums go straight to pinned host memory (no copy
+    // launch before the host reads them)
+    sum_final(C, part, nb, C->host_sums_dev, nv);
+    HIP_TRY(hipStreamSynchronize(C->stream));
+    for (int b = 0; b < nv; ++b) raw[b] = C->host_sums[b];
+  } else {
+    sum_final(C, part, nb, C->sums, nv);
+    swm_status st = read_sums(C, nv, raw.data());
+    if (st) return st;
+  }
   for (int b = 0; b < nv; ++b) C->last_resid[vars[b]] = 0.0;
   for (int m = 0; m < C->nz; ++m)
     for (int b = 0; b < nv; ++b) {
@@ -1442,6 +1451,7 @@ swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
   C->nz = C->nmem;
   C->member_init.assign(C->nmem, 0);
   HIP_TRY(hipHostMalloc(&C->host_sums, 64 * (size_t)C->nmem * sizeof(double), hipHostMallocDefault));
+  HIP_TRY(hipHostGetDevicePointer((void**)&C->host_sums_dev, C->host_sums, 0));
   const int ns = C->ns;
   auto id = [](int n) { std::vector<int> v(n); for (int k = 0; k < n; ++k) v[k] = k; return v; };
   swm_status st;
