@@ -1587,7 +1587,6 @@ __global__ void k_interp(LevT<R> C, LevT<R> F, const R* __restrict__ x1, R* __re
   if (s >= F.ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
-  const int p2 = iim - 1, q2 = jjm - 1;
   // temp(2a-1, 2b-1) = coarse (a, b) for a, b = 1..iim1-1 (halo reads mapped)
   const int ci = C.iim, cj = C.jjm;
   auto CS = [&](int ii, int jj) {
@@ -1595,19 +1594,28 @@ __global__ void k_interp(LevT<R> C, LevT<R> F, const R* __restrict__ x1, R* __re
     const int sl = (a - 1) + ci * ((b - 1) + cj * (k - 1));
     return (a == 1 || a == ci || b == 1 || b == cj) ? ldg(C.src, sl) : sl;
   };
+  // the cases (pole forms, coincident, between along i, diagonal, between
+  // along j) exclude one another; they are selected, not branched on, so
+  // that neighbouring lanes of the other parity do not diverge: c0 at
+  // (i + di, j + dj), c1 at (i - di, j - dj) for the two-point averages
   const bool io = (i & 1), jo = (j & 1);
-  int c0 = -1, c1 = -1;
-  if (i == 1 && j == q2) c0 = CS(1, q2);
-  else if (i == p2 && j == 1) c0 = CS(p2, 1);
-  else if (io && jo && i >= 3 && i <= iim - 1 && j >= 3 && j <= jjm - 1) c0 = CS(i, j);
-  else if (!io && jo && i >= 2 && i <= iim - 2 && j >= 3 && j <= jjm - 1) { c0 = CS(i + 1, j); c1 = CS(i - 1, j); }
-  else if (!io && !jo && i >= 2 && i <= iim - 2 && j >= 2 && j <= jjm - 2) { c0 = CS(i + 1, j + 1); c1 = CS(i - 1, j - 1); }
-  else if (io && !jo && i >= 3 && i <= iim - 1 && j >= 2 && j <= jjm - 2) { c0 = CS(i, j + 1); c1 = CS(i, j - 1); }
-  if (c0 < 0) return;
+  const bool pn = i == 1 && j == jjm - 1, ps = i == iim - 1 && j == 1;
+  const bool iA = i >= 3 && i <= iim - 1, iB = i >= 2 && i <= iim - 2;
+  const bool jA = j >= 3 && j <= jjm - 1, jB = j >= 2 && j <= jjm - 2;
+  const bool cA = io && jo && iA && jA;
+  const bool cB = !io && jo && iB && jA;
+  const bool cC = !io && !jo && iB && jB;
+  const bool cD = io && !jo && iA && jB;
+  const bool two = cB || cC || cD;
+  if (!(pn || ps || cA || two)) return;
+  const int di = (cB || cC) ? 1 : 0, dj = (cC || cD) ? 1 : 0;
+  const int c0 = CS(i + di, j + dj);
+  const int c1 = two ? CS(i - di, j - dj) : c0;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     const R* __restrict__ xc = x1 + (size_t)v * C.ns;
-    const R add = c1 < 0 ? ldg(xc, c0) : R(0.5) * (ldg(xc, c0) + ldg(xc, c1));
+    const R y0 = ldg(xc, c0), y1 = ldg(xc, c1);
+    const R add = two ? R(0.5) * (y0 + y1) : y0;
     R* __restrict__ xf = x2 + (size_t)v * F.ns;
     stg(xf, s, ldg((const R*)xf, s) + add);
   }
