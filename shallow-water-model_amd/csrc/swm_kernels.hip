@@ -2128,18 +2128,23 @@ __device__ __forceinline__ void co_interp(const LevT<R>& Cc, const LevT<R>& F, i
       const int a = (ii + 1) / 2, b = (jj + 1) / 2;
       return x1v[Cc.src[(a - 1) + ci * ((b - 1) + cj * (k - 1))]];
     };
+    // the cases selected, not branched on (as k_interp)
     const bool io = (i & 1), jo = (j & 1);
-    R add;
-    bool hit = true;
-    if (i == 1 && j == q2) add = T(1, q2);
-    else if (i == p2 && j == 1) add = T(p2, 1);
-    else if (io && jo && i >= 3 && i <= iim - 1 && j >= 3 && j <= jjm - 1) add = T(i, j);
-    else if (!io && jo && i >= 2 && i <= iim - 2 && j >= 3 && j <= jjm - 1) add = R(0.5) * (T(i + 1, j) + T(i - 1, j));
-    else if (!io && !jo && i >= 2 && i <= iim - 2 && j >= 2 && j <= jjm - 2)
-      add = R(0.5) * (T(i + 1, j + 1) + T(i - 1, j - 1));
-    else if (io && !jo && i >= 3 && i <= iim - 1 && j >= 2 && j <= jjm - 2) add = R(0.5) * (T(i, j + 1) + T(i, j - 1));
-    else hit = false;
-    if (hit) x2[v * F.ns + s] = x2[v * F.ns + s] + add;
+    const bool pn = i == 1 && j == q2, ps = i == p2 && j == 1;
+    const bool iA = i >= 3 && i <= iim - 1, iB = i >= 2 && i <= iim - 2;
+    const bool jA = j >= 3 && j <= jjm - 1, jB = j >= 2 && j <= jjm - 2;
+    const bool cA = io && jo && iA && jA;
+    const bool cB = !io && jo && iB && jA;
+    const bool cC = !io && !jo && iB && jB;
+    const bool cD = io && !jo && iA && jB;
+    const bool two = cB || cC || cD;
+    if (pn || ps || cA || two) {
+      const int di = (cB || cC) ? 1 : 0, dj = (cC || cD) ? 1 : 0;
+      const R y0 = T(i + di, j + dj);
+      const R y1 = two ? T(i - di, j - dj) : y0;
+      const R add = two ? R(0.5) * (y0 + y1) : y0;
+      x2[v * F.ns + s] = x2[v * F.ns + s] + add;
+    }
   }
   __syncthreads();
 }
