@@ -125,7 +125,7 @@ struct swm_ctx {
   // workgroup sizes of the step's stencil kernels and halo fills, and of the
   // multigrid smoother / transfer kernels (64, 128 or 256; an autotuner's
   // choice for ensembles)
-  int blk_step = 256, blk_mg = 256;
+  int blk_step = 256, blk_mg = 64;
   long long epoch = 0;             // changes whenever captured launches would differ (initialisation, settings)
   double* track = nullptr;
   double* fct_rec = nullptr;       // FP64 FCT step: the limiter coefficients go here (linearised model's base steps)
