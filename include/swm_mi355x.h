@@ -143,7 +143,12 @@ swm_status swm_select_member(swm_ctx* ctx, int member);
  * nstart).  The largest mean squared residuals of the two
  * fields and the number of solves that ended above the (absolute) criterion
  * are recorded per member on the device: swm_solver_track gives them for the
- * selected member (out[0..2]), counted from its last initialisation. */
+ * selected member (out[0..2]), counted from its last initialisation.
+ * FP64 contexts: every two-field solve of the step runs exactly ncycles
+ * V-cycles from the warm start (nstart unused), which makes the model a
+ * smooth function of its initial state (no cycle count that switches with
+ * the residual), e.g. for finite-difference checks; the linear solves of
+ * the tangent-linear and adjoint model keep their own criterion. */
 swm_status swm_set_fixed_cycles(swm_ctx* ctx, int ncycles, int nstart);
 swm_status swm_solver_track(swm_ctx* ctx, double* out);
 
@@ -384,6 +389,13 @@ swm_status swm_tl_run(swm_ctx* ctx, int mode, const double* in0, const double* i
  * with respect to the control (zeta, delta, h), out3 unused. */
 swm_status swm_adj_run(swm_ctx* ctx, int mode, const double* lh, const double* leta, const double* ldiv,
                        const double* lu, double* out0, double* out1, double* out2, double* out3);
+
+/* FCT windows: how many edges had a frozen limiter coefficient below 1
+ * (the mass limiter of mass_fct.F:403-532 acting, where the frozen
+ * linearisation is not the derivative), summed over the base steps that the
+ * last swm_tl_run / swm_adj_run recomputed, and the number of edge-steps
+ * (three edges per interior slot and step).  Both 0 for other schemes. */
+swm_status swm_adj_limiter_stats(swm_ctx* ctx, long long* below_one, long long* edges);
 
 /* ------------------------------------------------------------------------
  * Restart files: the direct-access format of utilities/layer_2/io_utilities.F

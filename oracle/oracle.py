@@ -46,6 +46,7 @@ def _load():
     lib.orc_elliptic.argtypes = [vp, ctypes.c_int, vp, vp]
     lib.orc_wrap.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     lib.orc_budgets.argtypes = [vp, vp]
+    lib.orc_set_sum_mode.argtypes = [vp, ctypes.c_int]
     return lib
 
 
@@ -94,9 +95,10 @@ SCHEMES = {"fct": 0, "centered": 1, "upwind": 2}
 class Oracle:
     """One FP64 model state on the CPU; arrays as in the reference layout.
     scheme: the advection of step_dt, "fct" (l_fct), "centered" (l_centered)
-    or "upwind" (l_upwind)."""
+    or "upwind" (l_upwind).  sums: the elliptic solve's global sums in the
+    reference's memory order ("reference") or compensated ("compensated")."""
 
-    def __init__(self, arrays, ntracer, dt, grav, hs=None, scheme="fct"):
+    def __init__(self, arrays, ntracer, dt, grav, hs=None, scheme="fct", sums="reference"):
         L = lib()
         self.g = int(arrays["g"])
         self.iim = self.jjm = 2 + 2 ** self.g
@@ -131,6 +133,7 @@ class Oracle:
         gr.tot_area = float(arrays["tot_area"])
         self._keep, self._ptrs = keep, ptrs
         self._h = L.orc_create(ctypes.byref(gr))
+        L.orc_set_sum_mode(self._h, {"reference": 0, "compensated": 1}[sums])
 
     def init_state(self, h, u, tracer):
         h = np.asfortranarray(h, dtype=np.float64)

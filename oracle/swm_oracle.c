@@ -36,6 +36,7 @@ struct orc_model {
   int step_count, last_vcycles;
   double w1, w2, w3;
   int n3, n4, n1f, n2f, n3f;
+  int sum_mode;          /* elliptic sums: 0 memory order (the reference), 1 compensated */
 };
 
 #define S(i, j, k) ((size_t)((i)-1) + (size_t)iim * ((size_t)((j)-1) + (size_t)jjm * ((k)-1)))
@@ -783,16 +784,27 @@ void orc_multigrid(orc_model* m, int nvar, const double* beta, double* alpha) {
   for (int l = 1; l <= G; ++l) { free(aw[l]); free(bw[l]); }
 }
 
+/* Neumaier's compensated running sum (sum_mode 1): s + c is the sum of the
+ * added terms to about one rounding, whatever their count */
+static void neumaier_add(double* s, double* c, double v) {
+  const double t = *s + v;
+  if (fabs(*s) >= fabs(v)) *c += (*s - t) + v;
+  else *c += (v - t) + *s;
+  *s = t;
+}
+
 /* elliptic_solver.F:203-239 check_convergence + :341-358 l2_norm */
 static double l2_resid(orc_model* m, const double* rhs, const double* x) {
   const size_t ns = m->ns;
   double* w = zeros(ns);
   laplacian(m->iim, m->jjm, m->lw, x, w);
-  double num = 0.0, cnt = 0.0;
+  double num = 0.0, cnt = 0.0, comp = 0.0;
   for (size_t n = 0; n < ns; ++n) {
     const double r = (w[n] - 0.0 * x[n]) - rhs[n];
-    num = num + (r * r) * m->mask[n];
+    if (m->sum_mode) neumaier_add(&num, &comp, (r * r) * m->mask[n]);
+    else num = num + (r * r) * m->mask[n];
   }
+  num = num + comp;
   for (size_t n = 0; n < ns; ++n) cnt = cnt + m->mask[n] * m->mask[n];
   free(w);
   const int isum = (int)cnt;
@@ -801,15 +813,25 @@ static double l2_resid(orc_model* m, const double* rhs, const double* x) {
 
 /* elliptic_solver.F:40-195 elliptic: mean removal, two V-cycles, then up to
  * ten more for each field that has not reached mean(resid^2) <= 1e-20 */
+void orc_set_sum_mode(orc_model* m, int mode) { m->sum_mode = mode ? 1 : 0; }
+
 int orc_elliptic(orc_model* m, int nvar, double* rhs, double* xout) {
   const size_t ns = m->ns;
   for (int v = 0; v < nvar; ++v) { wrap_face(m, xout + v * ns); }
   for (int v = 0; v < nvar; ++v) { wrap_face(m, rhs + v * ns); }
-  double total = 0.0;
-  for (size_t n = 0; n < ns; ++n) total = total + (m->area[n] * m->mask[n]) * 1.0;
+  double total = 0.0, ct = 0.0;
+  for (size_t n = 0; n < ns; ++n) {
+    if (m->sum_mode) neumaier_add(&total, &ct, (m->area[n] * m->mask[n]) * 1.0);
+    else total = total + (m->area[n] * m->mask[n]) * 1.0;
+  }
+  total = total + ct;
   for (int v = 0; v < nvar; ++v) {
-    double s = 0.0;
-    for (size_t n = 0; n < ns; ++n) s = s + rhs[v * ns + n] * (m->area[n] * m->mask[n]);
+    double s = 0.0, cs = 0.0;
+    for (size_t n = 0; n < ns; ++n) {
+      if (m->sum_mode) neumaier_add(&s, &cs, rhs[v * ns + n] * (m->area[n] * m->mask[n]));
+      else s = s + rhs[v * ns + n] * (m->area[n] * m->mask[n]);
+    }
+    s = s + cs;
     const double xmean = s / total;
     for (size_t n = 0; n < ns; ++n) rhs[v * ns + n] = rhs[v * ns + n] - xmean;
   }

@@ -62,6 +62,15 @@ int orc_get(const orc_model* m, const char* name, double* out);
  * variance of tracer q (the reference reports tracers 1 and 2). */
 void orc_budgets(const orc_model* m, double* out);
 
+/* Global sums of the elliptic solve (mean removal, residual norm): mode 0
+ * (default) in memory order as the reference's sum(X*MASK)
+ * (elliptic_solver.F:80-92, :341-358, compiled as one sequential loop);
+ * mode 1 compensated (Neumaier), within about one rounding of the exact sum
+ * at any size.  The order's rounding error grows with the number of cells:
+ * at 2.6 million cells it moves the mean by ~1e-13 of the field's scale,
+ * which the solve and the velocity gradient turn into ~1e-12 of u. */
+void orc_set_sum_mode(orc_model* m, int mode);
+
 /* Number of V-cycles run by the last elliptic solve (2 + extra). */
 int orc_last_vcycles(const orc_model* m);
 

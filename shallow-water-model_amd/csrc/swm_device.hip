@@ -543,9 +543,22 @@ swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
                      C->total_area, rhs, nvar > 1 ? rhs + ns : C->r2, C->mb);
   swm_status st0 = rhs_mean_squares<double>(C, rhs, nvar);
   if (st0) return st0;
+  const unsigned nb = nblk(ns);
+  if (C->fixed_cycles > 0 && nvar == 2) {
+    // fixed schedule: exactly fixed_cycles V-cycles, no host-side test; the
+    // final mean squared residuals go to the per-member record
+    for (int it = 0; it < C->fixed_cycles; ++it) vcycle<double>(C, V, 2, xout, rhs);
+    for (int v = 0; v < 2; ++v) gather<double>(C, C->face, {xout + (size_t)v * ns});
+    hipLaunchKernelGGL(swm::dev::k_resid2<2>, zgrid(C, nb), dim3(BLK), 0, C->stream, V.geo, xout, rhs, C->part_slots);
+    sum_final(C, C->part_slots, (int)nb, C->sums, 2);
+    hipLaunchKernelGGL(swm::dev::k_track_resid, zgrid(C, 1), dim3(64), 0, C->stream, C->sums, 1.0 / (double)C->max_ig,
+                       C->solver_tol, C->track, C->mb);
+    C->last_vcycles = C->fixed_cycles;
+    C->solves += 1; C->vcycles_total += C->fixed_cycles; C->checks_total += 1;
+    return SWM_OK;
+  }
   for (int it = 0; it < 2; ++it) vcycle<double>(C, V, nvar, xout, rhs);
   for (int v = 0; v < nvar; ++v) gather<double>(C, C->face, {xout + (size_t)v * ns});
-  const unsigned nb = nblk(ns);
   auto norms = [&](const std::vector<int>& vars, double* ms) -> swm_status {
     const int v0 = vars[0];
     if (vars.size() == 2)
@@ -1968,8 +1981,8 @@ swm_status swm_set_fixed_cycles(swm_ctx* C, int ncycles, int nstart) {
     swm::set_error("swm_set_fixed_cycles: bad argument");
     return SWM_ERR_ARG;
   }
-  if (ncycles > 0 && (C->precision != SWM_MIXED || C->solver_relative)) {
-    swm::set_error("swm_set_fixed_cycles: the fixed schedule is a mode of the mixed solve with the absolute criterion");
+  if (ncycles > 0 && C->solver_relative) {
+    swm::set_error("swm_set_fixed_cycles: the fixed schedule takes the absolute criterion");
     return SWM_ERR_ARG;
   }
   C->fixed_cycles = ncycles;
@@ -2438,6 +2451,10 @@ struct TlAd {
   double* seg = nullptr;              // (every + 1) base states (seg_planes each)
   bool have_window = false;
   swm::dev::Geo lin;                  // metrics with hs = 0 and f = 0
+  // FCT: edges whose frozen limiter coefficient is below 1, counted over the
+  // base steps recomputed by the last TL or AD sweep
+  unsigned long long* lim_count = nullptr;
+  long long lim_steps = 0;
 };
 
 namespace swm {
@@ -2448,6 +2465,7 @@ void tlad_free(swm_ctx* C) {
   for (double* p : T->ckpt) (void)hipFree(p);
   if (T->seg) (void)hipFree(T->seg);
   if (T->mem) (void)hipFree(T->mem);
+  if (T->lim_count) (void)hipFree(T->lim_count);
   delete T;
   C->tl = nullptr;
 }
@@ -2464,6 +2482,8 @@ swm_status tlad_alloc(swm_ctx* C) {
   C->tl = T;
   const size_t words = ns * (2 * (1 + 1 + 1 + 4) + 9 + 9 + 2 + 3 + 3 + 3 + 1 + 1 + 2 + 2 + 2 + 2 + 3 + 1 + 1 + 3 + 3 + 1 + 1);
   HIP_TRY(hipMalloc(&T->mem, words * sizeof(double)));
+  HIP_TRY(hipMalloc(&T->lim_count, sizeof(unsigned long long)));
+  HIP_TRY(hipMemsetAsync(T->lim_count, 0, sizeof(unsigned long long), C->stream));
   double* p = T->mem;
   auto take = [&](size_t n) { double* q = p; p += n; return q; };
   for (int t = 0; t < 2; ++t) { T->h[t] = take(ns); T->eta[t] = take(ns); T->div[t] = take(ns); T->u[t] = take(4 * ns); }
@@ -2551,9 +2571,14 @@ swm_status recompute_segment(swm_ctx* C, int sidx, int len) {
     st = base_step(C);
     C->fct_rec = nullptr;
     if (st) return st;
-    if (fct)
+    if (fct) {
       HIP_TRY(hipMemcpyAsync(seg_F(C, q), C->d.mass_flux, 3 * (size_t)C->ns * sizeof(double), hipMemcpyDeviceToDevice,
                              C->stream));
+      const size_t n3 = 3 * (size_t)C->ns;
+      hipLaunchKernelGGL(swm::dev::k_count_below_one, dim3((unsigned)((n3 + BLK - 1) / BLK)), dim3(BLK), 0, C->stream,
+                         n3, (const double*)seg_c(C, q), T->lim_count);
+      T->lim_steps += 1;
+    }
     if ((st = record_base(C, q + 1))) return st;
   }
   return SWM_OK;
@@ -2564,11 +2589,13 @@ swm_status recompute_segment(swm_ctx* C, int sidx, int len) {
 swm_status lin_solve(swm_ctx* C, double* rhs, double* x) {
   const double tol = C->solver_tol;
   const int rel = C->solver_relative, mc = C->solver_max_cycles, stall = C->solver_stall;
-  const int lv = C->last_vcycles;
+  const int lv = C->last_vcycles, fc = C->fixed_cycles;
   C->solver_tol = kLinTol; C->solver_relative = 1; C->solver_max_cycles = kLinMaxCycles; C->solver_stall = 1;
+  C->fixed_cycles = 0;
   swm_status st = elliptic(C, rhs, x, 2);
   C->solver_tol = tol; C->solver_relative = rel; C->solver_max_cycles = mc; C->solver_stall = stall;
   C->last_vcycles = lv;
+  C->fixed_cycles = fc;
   return st;
 }
 
@@ -2996,6 +3023,8 @@ swm_status swm_tl_run(swm_ctx* C, int mode, const double* in0, const double* in1
   const size_t ns = C->ns;
   swm_status st;
   swm::clear_state(C);
+  HIP_TRY(hipMemsetAsync(T->lim_count, 0, sizeof(unsigned long long), C->stream));
+  T->lim_steps = 0;
   fill<double>(C, T->xs, 2 * ns, 0.0);
   if (mode == 0) {
     if ((st = swm::state_in(C, swm::first_level(), in0, in1, in2, in3))) return st;
@@ -3047,6 +3076,8 @@ swm_status swm_adj_run(swm_ctx* C, int mode, const double* lh, const double* let
   const size_t ns = C->ns;
   swm_status st;
   swm::clear_state(C);
+  HIP_TRY(hipMemsetAsync(T->lim_count, 0, sizeof(unsigned long long), C->stream));
+  T->lim_steps = 0;
   fill<double>(C, T->xs, 2 * ns, 0.0);
   // mode 2: the initial state map alone, from the adjoint of its output
   if ((st = swm::state_in(C, mode == 2 ? swm::first_level() : swm::last_level(T->nsteps), lh, leta, ldiv, lu)))
@@ -3081,6 +3112,23 @@ swm_status swm_adj_run(swm_ctx* C, int mode, const double* lh, const double* let
   const int nck = (int)T->ckpt.size();
   if ((st = swm::restore(C, T->ckpt[nck - 1], T->ckpt_meta[2 * nck - 2], T->ckpt_meta[2 * nck - 1]))) return st;
   HIP_TRY(hipStreamSynchronize(C->stream));
+  return SWM_OK;
+}
+
+swm_status swm_adj_limiter_stats(swm_ctx* C, long long* below_one, long long* edges) {
+  if (!C || !below_one || !edges) { swm::set_error("swm_adj_limiter_stats: bad argument"); return SWM_ERR_ARG; }
+  if (!C->tl || !C->tl->have_window) {
+    swm::set_error("swm_adj_limiter_stats: no window (swm_adj_forward first)");
+    return SWM_ERR_STATE;
+  }
+  HIP_TRY(hipSetDevice(C->device));
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  unsigned long long n = 0;
+  HIP_TRY(hipMemcpy(&n, C->tl->lim_count, sizeof(n), hipMemcpyDeviceToHost));
+  *below_one = (long long)n;
+  // three edges per interior slot (the poles, in halo slots, own none)
+  const long long per = 3LL * (C->iim - 2) * (C->jjm - 2) * 10;
+  *edges = per * C->tl->lim_steps;
   return SWM_OK;
 }
 

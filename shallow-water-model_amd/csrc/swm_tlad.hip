@@ -488,6 +488,15 @@ __global__ void k_fct_coef(Geo G, const double* __restrict__ beta, const double*
   for (int e = 0; e < 3; ++e) coef[e * ns + s] = anti[e * ns + s] >= 0.0 ? b : dmin(1.0, beta[up[e]]);
 }
 
+// number of the n coefficients below 1 (active limiter), added to *count:
+// one atomic per wave
+__global__ void k_count_below_one(size_t n, const double* __restrict__ coef, unsigned long long* __restrict__ count) {
+  const size_t t = (size_t)blockIdx.x * bdim() + threadIdx.x;
+  const bool below = t < n && coef[t] < 1.0;
+  const unsigned long long m = __ballot(below);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (unsigned long long)__popcll(m));
+}
+
 // the limited flux, TL and AD alike (a diagonal map): x *= coef on n values
 __global__ void k_fct_scale(size_t n, const double* __restrict__ coef, double* __restrict__ x) {
   const size_t t = (size_t)blockIdx.x * bdim() + threadIdx.x;

@@ -117,6 +117,7 @@ _sig("swm_init_vorticity", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] *
 _sig("swm_adj_forward", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
 _sig("swm_tl_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 8)
 _sig("swm_adj_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 8)
+_sig("swm_adj_limiter_stats", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 2)
 _sig("swm_wrap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p])
 _sig("swm_elliptic", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
 _sig("swm_multigrid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
@@ -615,6 +616,13 @@ class Model:
         if init_only:
             return tuple(out[:3])
         return tuple(out[:3]) if control else tuple(out)
+
+    def limiter_stats(self):
+        """FCT window: (edge-steps with a frozen limiter coefficient below 1,
+        edge-steps in all) over the base steps of the last TL or AD sweep."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib.swm_adj_limiter_stats(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     @property
     def time_seconds(self):

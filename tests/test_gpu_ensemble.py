@@ -176,10 +176,17 @@ def test_fixed_schedule_track_and_errors():
         r0, r1, above = m.solver_track()
         assert 0.0 < r0 <= 1e-20 and 0.0 < r1 <= 1e-20, (r0, r1)
         assert above == 0
-    # the schedule needs the mixed path and the absolute criterion
+    # the schedule needs the absolute criterion; graphs need the mixed path
     f = _model(grid, S.SWM_FP64, members=2)
+    f.set_solver(tol=1e-12, relative=True)
     with pytest.raises(S.SwmError):
         f.set_fixed_cycles(2)
+    f.set_solver()
+    f.set_fixed_cycles(2)
+    for k in range(2):
+        _init(f, k, k)
+    with pytest.raises(S.SwmError):
+        S.Ensemble([f], [1]).step(1)
     with pytest.raises(S.SwmError):
         m.set_solver(tol=1e-10, relative=True)
     with pytest.raises(S.SwmError):
