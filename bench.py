@@ -15,26 +15,35 @@ resident in HBM before the timed region starts.
 
 **Time step.** dt = 200·2^(5−g) s, i.e. 12.5 s at g=9. That is the
 reference's own choice (200 s at 10242 cells, `swm/run.0010242/data/namelist`),
-scaled with the grid spacing. The larger dt = 450·2^(5−g) s violates the
+scaled with the grid spacing. The configured dt = 450·2^(5−g) s violates the
 scheme's stability limit on these fields: at g=9 the state blows up and the
 elliptic solve fails its criterion at step 25, as the reference does at g=3.
-The work per step does not depend on dt.
+The V-cycles per step depend on dt (the warm start is older by one step), so
+the line reports them, and a scan over dt = 25, 20, 16 s reports the largest
+step at which the g=9 day completes, with its cell-steps/s.
 
-**Two forms are timed on the same initial state.** Both do K steps after W
-warm-up steps:
+**Forms timed on the same initial state.** K steps after W warm-up steps:
 
 - the mixed-precision step (FP32 state and stencils, FP64 defect correction): the headline `value`;
-- the FP64 step (the reference's arithmetic).
+- the FP64 step (the reference's arithmetic);
+- the mixed step on the FP64 step's solver schedule (the last solution as the warm start, two cycles before the first test), which separates the gain of the precision from that of the schedule.
 
-Both then continue, untimed, to N steps in all (default: one simulated day,
-6912 steps at g=9). The mixed path's error against the FP64 path is reported
-per field (area-weighted relative L2 and relative max).
+The mixed and FP64 forms then continue to one simulated day (6912 steps at
+g=9); the whole day is timed and its V-cycles per step reported. The mixed
+path's error against the FP64 path is reported per field (area-weighted
+relative L2 and relative max), also with both forms solved to mean(r^2) <=
+1e-24, where only the precision differs.
+
+**Level 8** (655362 cells, dt = 25 s): the FP64 step over one simulated day
+(3456 steps), and a microbench of single operators on its resident state:
+div_of_an_edge_flux, curl of a corner vector, the Laplacian and the face
+halo fill, in microseconds and GB/s against the 8 TB/s HBM peak.
 
 **The JSON line carries:**
 
 - `roofline`: the hottest kernel of the headline path (the finest-level multigrid smoother), timed with HIP events on the model's stream during the timed steps.
 - `step_roofline`: the whole step against the canonical bytes per cell-step.
-- `cpu_baseline`: the oracle, a single-threaded C restatement of the reference FP64 step, timed on a bounded sample of the same workload.
+- `cpu_baseline`: the oracle, the C restatement of the reference FP64 step, on the host's cores (OpenMP over the panels) and on one core, over a bounded sample of steps of the same workload.
 """
 import argparse
 import json
@@ -90,8 +99,11 @@ def rel_errors(grid, a, b):
 
 
 def timed(model, warmup, steps):
+    model.synchronize()
+    tw = time.perf_counter()
     model.step(warmup)
     model.synchronize()
+    warm = time.perf_counter() - tw
     s0, v0, c0 = model.solver_stats()
     # every 7th pair of back-to-back finest-level smoother launches is timed
     # with HIP events (the event records would cost the step ~3 us each if
@@ -105,7 +117,7 @@ def timed(model, warmup, steps):
     model.profile(False)
     s1, v1, c1 = model.solver_stats()
     dev_ms, k_ms, k_bytes, k_n = model.timing()
-    return {"wall": wall, "device_ms": dev_ms, "kernel_ms": k_ms, "kernel_bytes": k_bytes, "kernel_launches": k_n,
+    return {"wall": wall, "warm_wall": warm, "device_ms": dev_ms, "kernel_ms": k_ms, "kernel_bytes": k_bytes, "kernel_launches": k_n,
             "vcycles_per_step": (v1 - v0) / max(1, s1 - s0), "checks_per_step": (c1 - c0) / max(1, s1 - s0)}
 
 
@@ -317,6 +329,90 @@ def ensemble_bench(S, hours=12.0, n6=96, n7=32, tune_rounds=2, eager_steps=12):
                     "single-model form (the configured 450*2^(5-g) s is past the scheme's stability limit)"}
 
 
+def level8_bench(S, reps=200):
+    """Level 8 (655362 cells), FP64, dt = 25 s (the reference's own time step
+    at this size, swm/run.0655362/data/namelist): one simulated day (3456
+    steps) timed as a whole, then the single-operator microbench on the
+    resident state."""
+    g, dt = 8, 25.0
+    grid = S.Grid(g, radius=RADIUS, omega=OMEGA)
+    m = S.Model(grid, ntracer=1, dt=dt, grav=GRAV)
+    m.init_synthetic(seed_index=1, lmax=LMAX[g])
+    nsteps = int(round(86400.0 / dt))
+    m.synchronize()
+    t0 = time.perf_counter()
+    m.step(nsteps)
+    m.synchronize()
+    wall = time.perf_counter() - t0
+    solves, vcyc, checks = m.solver_stats()
+    ops = {}
+    for op in S.Model.OPERATOR_BENCH:
+        us, nbytes = m.operator_bench(op, reps)
+        gbs = nbytes / (us * 1e-6) / 1e9
+        ops[op] = {"us": us, "bytes_per_launch": nbytes, "gbs": gbs, "frac_of_hbm_peak": gbs / HBM_PEAK_GBS}
+    out = {"workload": f"icosahedral level {g} ({grid.max_ig} cells), FP64 step, dt={dt} s, FCT mass + 1 tracer, "
+                       f"multigrid to mean(resid^2)<=1e-20, one simulated day ({nsteps} steps)",
+           "cells": grid.max_ig, "dt": dt, "steps": nsteps, "wall_s": wall,
+           "cell_steps_per_s": grid.max_ig * nsteps / wall, "ms_per_step": wall * 1e3 / nsteps,
+           "vcycles_per_step": vcyc / max(1, solves),
+           "step_roofline_frac": grid.max_ig * nsteps / wall * step_bytes("fp64", 1, vcyc / max(1, solves),
+                                                                          checks / max(1, solves)) / 1e9
+           / HBM_PEAK_GBS,
+           "operators": ops,
+           "operators_note": "reps back-to-back launches of the kernel each operator entry point runs, between HIP "
+                             "events; algorithmic FP64 bytes per real cell: div_of_an_edge_flux 40 B, curl of a "
+                             "corner vector 144 B, Laplacian 72 B; face halo fill 16 B per ghost slot"}
+    m.close()
+    grid.close()
+    return out
+
+
+def dt_scan(S, grid, lmax, ntracer, dts=(25.0, 20.0, 16.0)):
+    """The mixed step over one simulated day at each dt, largest first, until
+    one completes (a blow-up ends in an elliptic solve that misses its
+    criterion)."""
+    out = []
+    for dt in dts:
+        m = S.Model(grid, ntracer=ntracer, dt=dt, grav=GRAV, precision=S.SWM_MIXED)
+        m.init_synthetic(seed_index=1, lmax=lmax)
+        n = int(round(86400.0 / dt))
+        m.synchronize()
+        t0 = time.perf_counter()
+        done, ok = 0, True
+        try:
+            while done < n:
+                k = min(256, n - done)
+                m.step(k)
+                done += k
+            m.synchronize()
+        except S.SwmError:
+            ok = False
+            done = m.step_count
+        wall = time.perf_counter() - t0
+        solves, vcyc, _ = m.solver_stats()
+        rec = {"dt": dt, "steps": n, "completed": ok, "steps_done": done}
+        if ok:
+            rec.update({"cell_steps_per_s": grid.max_ig * n / wall, "ms_per_step": wall * 1e3 / n,
+                        "vcycles_per_step": vcyc / max(1, solves)})
+        out.append(rec)
+        m.close()
+        if ok:
+            break
+    return out
+
+
+def cpu_baseline_run(grid, ntracer, dt, state, steps, threads):
+    from oracle.oracle import Oracle, grid_arrays_from, set_threads
+    set_threads(threads)
+    orc = Oracle(grid_arrays_from(grid), ntracer, dt, GRAV)
+    orc.init_state(*state)
+    tc = time.perf_counter()
+    orc.step(steps)
+    t = time.perf_counter() - tc
+    orc.close()
+    return grid.max_ig * steps / t, t
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -325,12 +421,14 @@ def main():
     ap.add_argument("--g", type=int, default=9)
     ap.add_argument("--ntracer", type=int, default=1)
     ap.add_argument("--error-steps", type=int, default=None,
-                    help="steps of the mixed-vs-FP64 error run (default: one simulated day; 0 skips it)")
+                    help="steps of the day window and the mixed-vs-FP64 error (default: one simulated day; 0 skips)")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint form (level 7, 6 h window)")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the ensemble form (96 members at level 6 and 32 at level 7, 12 h)")
+    ap.add_argument("--no-level8", action="store_true", help="skip the level-8 FP64 day and operator microbench")
+    ap.add_argument("--no-dt-scan", action="store_true", help="skip the scan over larger time steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -347,8 +445,12 @@ def main():
     cells = grid.max_ig
     print(f"[bench] grid g={g} ({cells} cells) built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
 
-    def model(precision):
+    def model(precision, tol=None, reference_schedule=False):
         m = S.Model(grid, ntracer=args.ntracer, dt=dt, grav=GRAV, precision=precision)
+        if tol:
+            m.set_solver(tol=tol)
+        if reference_schedule:
+            m.set_mixed_start(0, 2)
         m.init_synthetic(seed_index=1, lmax=lmax)
         return m
 
@@ -362,38 +464,78 @@ def main():
     print(f"[bench] fp64  {tf['wall'] * 1e3 / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
     value = cells * args.steps / tm["wall"]
     value64 = cells * args.steps / tf["wall"]
+    # the mixed step on the FP64 step's solver schedule (same warm start)
+    mrs = model(S.SWM_MIXED, reference_schedule=True)
+    trs = timed(mrs, args.warmup, args.steps)
+    mrs.close()
+    value_rs = cells * args.steps / trs["wall"]
+    print(f"[bench] mixed on the FP64 schedule {trs['wall'] * 1e3 / args.steps:.3f} ms/step", file=sys.stderr,
+          flush=True)
 
-    # error of the mixed path against FP64 after the error window
+    # the rest of one simulated day, timed per form; the error against FP64
     nerr = int(round(86400.0 / dt)) if args.error_steps is None else args.error_steps
-    err = None
+    err, day = None, None
     done = args.warmup + args.steps
     if nerr > done:
-        # the FP64 form solved to mean(r^2) <= 1e-24 from the same state: the
-        # converged solution, against which the solver-criterion error of
-        # both forms shows
-        conv = model(S.SWM_FP64)
-        conv.set_solver(tol=1e-24)
-        conv.step(done)
-        chunk = 256
-        while done < nerr:
-            k = min(chunk, nerr - done)
-            mix.step(k)
-            ref.step(k)
-            conv.step(k)
-            done += k
-        for m in (mix, ref, conv):
+        rest = {}
+        for name, m in (("mixed", mix), ("fp64", ref)):
+            m.synchronize()
+            t1 = time.perf_counter()
+            k = done
+            while k < nerr:
+                n = min(256, nerr - k)
+                m.step(n)
+                k += n
+            m.synchronize()
+            rest[name] = time.perf_counter() - t1
+        day = {}
+        for name, m, t in (("mixed", mix, tm), ("fp64", ref, tf)):
+            wall = t["warm_wall"] + t["wall"] + rest[name]
+            solves, vcyc, checks = m.solver_stats()
+            day[name] = {"steps": nerr, "wall_s": wall, "cell_steps_per_s": cells * nerr / wall,
+                         "ms_per_step": wall * 1e3 / nerr, "vcycles_per_step": vcyc / max(1, solves),
+                         "checks_per_step": checks / max(1, solves)}
+        day["mixed_over_fp64"] = day["mixed"]["cell_steps_per_s"] / day["fp64"]["cell_steps_per_s"]
+        print(f"[bench] one day: mixed {day['mixed']['ms_per_step']:.3f} ms/step "
+              f"({day['mixed']['vcycles_per_step']:.2f} cycles), fp64 {day['fp64']['ms_per_step']:.3f} ms/step "
+              f"({day['fp64']['vcycles_per_step']:.2f} V-cycles)", file=sys.stderr, flush=True)
+        # both forms solved to mean(r^2) <= 1e-24 from the same state: the
+        # precision alone, and the converged solution against which the
+        # solver-criterion error of both forms shows
+        conv = model(S.SWM_FP64, tol=1e-24)
+        mixt = model(S.SWM_MIXED, tol=1e-24)
+        for m in (conv, mixt):
+            k = 0
+            while k < nerr:
+                n = min(256, nerr - k)
+                m.step(n)
+                k += n
             m.synchronize()
         names = ("h", "eta", "div", "tracer")
-        fm, fr, fc = ({n: m.field(n) for n in names} for m in (mix, ref, conv))
+        fm, fr, fc, ft = ({n: m.field(n) for n in names} for m in (mix, ref, conv, mixt))
         err = {"vs_fp64": rel_errors(grid, fr, fm), "vs_fp64_converged": rel_errors(grid, fc, fm),
                "fp64_vs_fp64_converged": rel_errors(grid, fc, fr),
-               "steps": done, "simulated_hours": done * dt / 3600.0,
+               "precision_only": rel_errors(grid, fc, ft),
+               "steps": nerr, "simulated_hours": nerr * dt / 3600.0,
                "mass_drift_mixed": abs(mix.total_mass() - mass0) / mass0,
+               "h_target_1e-6": {"rel_l2_vs_fp64": None, "rel_linf_vs_fp64": None},
                "note": "relative L2 (area-weighted) and Linf over real cells; 'converged' is the FP64 form "
-                       "solved to mean(r^2) <= 1e-24, the others use the reference criterion 1e-20"}
-        print(f"[bench] error after {done} steps: h vs fp64 {err['vs_fp64']['h']}, vs converged "
-              f"{err['vs_fp64_converged']['h']}, fp64 vs converged {err['fp64_vs_fp64_converged']['h']}",
+                       "solved to mean(r^2) <= 1e-24, the others use the reference criterion 1e-20; "
+                       "'precision_only' is the mixed form against the FP64 form, both solved to 1e-24"}
+        err["h_target_1e-6"] = {"rel_l2_vs_fp64": err["vs_fp64"]["h"]["rel_l2"],
+                                "rel_linf_vs_fp64": err["vs_fp64"]["h"]["rel_linf"],
+                                "rel_l2_precision_only": err["precision_only"]["h"]["rel_l2"],
+                                "rel_linf_precision_only": err["precision_only"]["h"]["rel_linf"]}
+        conv.close()
+        mixt.close()
+        print(f"[bench] error after {nerr} steps: h vs fp64 {err['vs_fp64']['h']}, precision only "
+              f"{err['precision_only']['h']}, fp64 vs converged {err['fp64_vs_fp64_converged']['h']}",
               file=sys.stderr, flush=True)
+
+    scan = None
+    if not args.no_dt_scan:
+        scan = dt_scan(S, grid, lmax, args.ntracer)
+        print(f"[bench] dt scan {scan}", file=sys.stderr, flush=True)
 
     def kroof(t):
         if not t["kernel_ms"]:
@@ -413,16 +555,28 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline and args.cpu_steps > 0:
-        from oracle.oracle import Oracle, grid_arrays_from
-        orc = Oracle(grid_arrays_from(grid), args.ntracer, dt, GRAV)
-        orc.init_state(h0, u0, tr0)
-        tc = time.perf_counter()
-        orc.step(args.cpu_steps)
-        tcpu = time.perf_counter() - tc
-        cpu = {"value": cells * args.cpu_steps / tcpu, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+        try:
+            ncores = min(16, len(os.sched_getaffinity(0)))
+        except AttributeError:
+            ncores = min(16, os.cpu_count() or 1)
+        state = (h0, u0, tr0)
+        v_all, t_all = cpu_baseline_run(grid, args.ntracer, dt, state, args.cpu_steps, ncores)
+        v_one, t_one = cpu_baseline_run(grid, args.ntracer, dt, state, args.cpu_steps, 1)
+        cpu = {"value": v_all, "unit": "cell-steps/s", "cores": ncores, "kind": "port",
                "sample": f"{args.cpu_steps} FP64 steps (the first ones) of the same g={g} workload from the same "
-                         f"initial state, {tcpu:.1f} s on one host core; oracle/swm_oracle.c (gcc -O2), the C "
-                         f"restatement of the reference step, bit-identical to the compiled reference"}
+                         f"initial state, {t_all:.1f} s on {ncores} host threads (OpenMP over the 10 panels; "
+                         f"global sums sequential); oracle/swm_oracle.c (gcc -O2), the C restatement of the "
+                         f"reference step, bit-identical to the compiled reference at any thread count",
+               "one_core": {"value": v_one, "cores": 1, "seconds": t_one}}
+        print(f"[bench] cpu baseline {v_all:.3e} cell-steps/s on {ncores} threads, {v_one:.3e} on one",
+              file=sys.stderr, flush=True)
+
+    lev8 = None
+    if not args.no_level8:
+        lev8 = level8_bench(S)
+        print(f"[bench] level 8 FP64 day {lev8['ms_per_step']:.3f} ms/step, operators "
+              + ", ".join(f"{k} {v['us']:.1f} us {v['gbs']:.0f} GB/s" for k, v in lev8["operators"].items()),
+              file=sys.stderr, flush=True)
 
     adj = None
     if not args.no_adjoint:
@@ -462,7 +616,15 @@ def main():
             "fp64_ms_per_step": tf["wall"] * 1e3 / args.steps,
             "mixed_over_fp64": value / value64,
             "vcycles_per_step": {"mixed": tm["vcycles_per_step"], "fp64": tf["vcycles_per_step"]},
+            "same_warm_start": {"mixed_value": value_rs, "mixed_ms_per_step": trs["wall"] * 1e3 / args.steps,
+                                "mixed_over_fp64": value_rs / value64,
+                                "vcycles_per_step": trs["vcycles_per_step"],
+                                "note": "the mixed step with the FP64 step's solver schedule: the last solution as "
+                                        "the warm start and two cycles before the first convergence test"},
+            "one_day": day,
             "mixed_error_vs_fp64": err,
+            "dt_scan": scan,
+            "level8": lev8,
             "adjoint": adj,
             "ensemble": ensemble,
         },

@@ -314,6 +314,18 @@ swm_status swm_timing(const swm_ctx* ctx, double* step_ms, double* kernel_ms_mea
  * each), or off (0); the mean is per launch. */
 swm_status swm_profile(swm_ctx* ctx, int enable);
 
+/* Microbench of single reference operators on the device-resident state of
+ * an initialised FP64 context (no host copies): op 0 div_of_an_edge_flux
+ * (operators.F:270-392) of the mass flux, 1 curl_of_a_vector (operators.F:
+ * 47-147) of the corner velocity, 2 laplacian_of_a_scalar (operators.F:
+ * 402-518) of the thickness, 3 the face halo fill (wrap_data.F:1790-2607)
+ * of one cell field.  Runs reps back-to-back launches of the kernel the
+ * operator entry points run, between HIP events on the context's stream;
+ * returns the mean microseconds per launch and the algorithmic bytes per
+ * launch (40, 144 and 72 B per real cell; 16 B per ghost slot).  The state
+ * is not changed.  No counterpart in the reference. */
+swm_status swm_operator_bench(swm_ctx* ctx, int op, int reps, double* us_per_launch, double* bytes_per_launch);
+
 /* Seeded synthetic initial state built on the device: harmonic sums of
  * degree 1..lmax (coefficients N(0,1) l^-1.5, splitmix64 streams seeded
  * 20261203 + 100*seed_index + 1..4) for vorticity, divergence (10 % of its

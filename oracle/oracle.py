@@ -47,6 +47,7 @@ def _load():
     lib.orc_wrap.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     lib.orc_budgets.argtypes = [vp, vp]
     lib.orc_set_sum_mode.argtypes = [vp, ctypes.c_int]
+    lib.orc_set_threads.argtypes = [ctypes.c_int]
     return lib
 
 
@@ -90,6 +91,12 @@ def grid_arrays_from(grid):
 
 
 SCHEMES = {"fct": 0, "centered": 1, "upwind": 2}
+
+
+def set_threads(n):
+    """Host threads of the oracle's panel-parallel loops (results do not
+    depend on the count)."""
+    lib().orc_set_threads(int(n))
 
 
 class Oracle:

@@ -9,6 +9,9 @@
 #include "swm_oracle.h"
 
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -76,6 +79,7 @@ void orc_wrap(const orc_model* m, int kind, int nspace, double* x) {
 /* operators.F:270-392 div_of_an_edge_flux */
 static void div_edge(int iim, int jjm, const double* area_inv, const double* x, double* out) {
 #define X3(e, i, j, k) x[(e)-1 + 3 * S(i, j, k)]
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k) {
     for (int j = 1; j <= jjm; ++j)
       for (int i = 1; i <= iim; ++i) out[S(i, j, k)] = 0.0;
@@ -97,6 +101,7 @@ static void div_edge(int iim, int jjm, const double* area_inv, const double* x, 
 static void laplacian(int iim, int jjm, const double* lw, const double* x, double* out) {
 #define L(n, i, j, k) lw[(n)-1 + 7 * S(i, j, k)]
 #define XX(i, j, k) x[S(i, j, k)]
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k) {
     for (int j = 1; j <= jjm; ++j)
       for (int i = 1; i <= iim; ++i) out[S(i, j, k)] = 0.0;
@@ -125,6 +130,7 @@ static void laplacian(int iim, int jjm, const double* lw, const double* x, doubl
 /* operators.F:47-147 curl_of_a_vector (computed from i,j = 1 as the reference) */
 static void curl_vec(int iim, int jjm, const double* area_inv, const double* vw, const double* xin,
                      double* out) {
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k) {
     for (int j = 1; j <= jjm; ++j)
       for (int i = 1; i <= iim; ++i) out[S(i, j, k)] = 0.0;
@@ -146,6 +152,7 @@ static void curl_vec(int iim, int jjm, const double* area_inv, const double* vw,
 /* operators.F:155-261 div_of_a_vector */
 static void div_vec(int iim, int jjm, const double* area_inv, const double* vw, const double* xin,
                     double* out) {
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k) {
     for (int j = 1; j <= jjm; ++j)
       for (int i = 1; i <= iim; ++i) out[S(i, j, k)] = 0.0;
@@ -170,6 +177,7 @@ static void center_ke(int iim, int jjm, const double* sub_area, const double* ar
                       const double* kec, double* ke) {
 #define SA(kk, ic, i, j, k) sub_area[(kk)-1 + 3 * ((ic)-1 + 2 * S(i, j, k))]
 #define KC(ic, i, j, k) kec[(ic)-1 + 2 * S(i, j, k)]
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k) {
     for (int j = 1; j <= jjm; ++j)
       for (int i = 1; i <= iim; ++i) ke[S(i, j, k)] = 0.0;
@@ -190,6 +198,7 @@ static void center_ke(int iim, int jjm, const double* sub_area, const double* ar
 static void avg_corners(int iim, int jjm, const double* sw, const double* x, double* out) {
 #define SW(kk, ic, i, j, k) sw[(kk)-1 + 3 * ((ic)-1 + 2 * S(i, j, k))]
 #define O2(ic, i, j, k) out[(ic)-1 + 2 * S(i, j, k)]
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k) {
     const double fill = x[S(2, 2, k)];
     for (int j = 1; j <= jjm; ++j)
@@ -211,6 +220,7 @@ static void avg_corners(int iim, int jjm, const double* sw, const double* x, dou
 static void gradient(int iim, int jjm, const double* cai, const double* vw, const double* x, double* out) {
 #define G2(sp, ic, i, j, k) out[(sp)-1 + 2 * ((ic)-1 + 2 * S(i, j, k))]
   memset(out, 0, sizeof(double) * 4 * (size_t)iim * jjm * 10);
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k)
     for (int j = 2; j <= jjm - 1; ++j)
       for (int i = 2; i <= iim - 1; ++i)
@@ -278,6 +288,7 @@ static void half_edges(int iim, int jjm, const double* dc, const double* nrm, co
 static void centered_mass_flux(int iim, int jjm, const double* dc, const double* nrm, const double* mc,
                                const double* u, double* flux) {
   memset(flux, 0, sizeof(double) * 3 * (size_t)iim * jjm * 10);
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k)
     for (int j = 2; j <= jjm - 1; ++j)
       for (int i = 2; i <= iim - 1; ++i) {
@@ -292,6 +303,7 @@ static void centered_mass_flux(int iim, int jjm, const double* dc, const double*
 static void upstream_mass_flux(int iim, int jjm, const double* dc, const double* nrm, const double* u,
                                const double* mass, double* flux) {
   memset(flux, 0, sizeof(double) * 3 * (size_t)iim * jjm * 10);
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k)
     for (int j = 2; j <= jjm - 1; ++j)
       for (int i = 2; i <= iim - 1; ++i) {
@@ -318,6 +330,7 @@ static void vorticity_flux(int iim, int jjm, const double* mf, const double* mas
     pv[n] = eta[n] * w;
   }
   memset(flux, 0, sizeof(double) * 3 * ns);
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k)
     for (int j = 2; j <= jjm - 1; ++j)
       for (int i = 2; i <= iim - 1; ++i) {
@@ -338,6 +351,7 @@ static void divergence_curl(int iim, int jjm, const double* dc, const double* tn
     pv[n] = eta[n] * w;
   }
   memset(flux, 0, sizeof(double) * 3 * ns);
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k)
     for (int j = 2; j <= jjm - 1; ++j)
       for (int i = 2; i <= iim - 1; ++i) {
@@ -359,6 +373,7 @@ static void divergence_curl(int iim, int jjm, const double* dc, const double* tn
 static void upstream_tracer_flux(int iim, int jjm, const double* mf, const double* x, double* out) {
   const size_t n = (size_t)iim * jjm * 10;
   for (size_t q = 0; q < 3 * n; ++q) out[q] = 0.0;
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k)
     for (int j = 2; j <= jjm - 1; ++j)
       for (int i = 2; i <= iim - 1; ++i) {
@@ -376,6 +391,7 @@ static void upstream_tracer_flux(int iim, int jjm, const double* mf, const doubl
 static void centered_tracer_flux(int iim, int jjm, const double* mf, const double* x, double* out) {
   const size_t n = (size_t)iim * jjm * 10;
   for (size_t q = 0; q < 3 * n; ++q) out[q] = 0.0;
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k)
     for (int j = 2; j <= jjm - 1; ++j)
       for (int i = 2; i <= iim - 1; ++i) {
@@ -398,6 +414,7 @@ static void edge_update(const orc_model* m, const double* base, const double* F,
   const int iim = m->iim, jjm = m->jjm;
   const double dt = m->dt;
   memset(out, 0, sizeof(double) * m->ns);
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k) {
     int i = 1, j = jjm - 1;
     out[S(i, j, k)] = base[S(i, j, k)] + dt * m->area_inv[S(i, j, k)] *
@@ -429,12 +446,15 @@ static void mass_fct(orc_model* m, const double* mss_n3, const double* mss_c, co
   centered_mass_flux(iim, jjm, m->dc, m->normal, mss_c, u, m->msh_f[ti[0] - 1]);
   wrap_edge(m, m->msh_f[ti[0] - 1]);
   for (int t = 0; t < 3; ++t)
+    #pragma omp parallel for schedule(static)
     for (size_t q = 0; q < ne; ++q) flx2[q] = flx2[q] + tw[t] * m->msh_f[ti[t] - 1][q];
+  #pragma omp parallel for schedule(static)
   for (size_t q = 0; q < ne; ++q) anti[q] = flx2[q] - flx1[q];
   edge_update(m, mss_n3, flx1, mss_low);
   /* set_beta_dn (mass_fct.F:366-402): the interior loop starts at i,j = 1
    * and so overwrites the pole values computed first */
   const double mss_min = 1.0e-6, small = 1.0e-18, dt = m->dt;
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k) {
     int i = 1, j = jjm - 1;
     double a_out = -(dmin(0.0, E3(anti, 2, i, j, k)) + dmin(0.0, E3(anti, 3, i, j, k)) +
@@ -457,6 +477,7 @@ static void mass_fct(orc_model* m, const double* mss_n3, const double* mss_c, co
   }
   wrap_face(m, beta);
   /* set_corrected_mss_anti (mass_fct.F:410-470) */
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k)
     for (int j = 2; j <= jjm - 1; ++j)
       for (int i = 2; i <= iim - 1; ++i) {
@@ -469,6 +490,7 @@ static void mass_fct(orc_model* m, const double* mss_n3, const double* mss_c, co
         E3(anti, 3, i, j, k) = b * dmax(0.0, a) + dmin(1.0, beta[S(i, j - 1, k)]) * dmin(0.0, a);
       }
   wrap_edge(m, anti);
+  #pragma omp parallel for schedule(static)
   for (size_t q = 0; q < ne; ++q) mss_flux[q] = flx1[q] + anti[q];
   edge_update(m, mss_low, anti, mss_n4);
   free(flx1); free(flx2); free(anti); free(beta);
@@ -486,6 +508,7 @@ static void tracer_fct(orc_model* m, double* trc, const double* trc_high, const 
   double *ml = zeros(ns), *mf = zeros(ns), *tmin = zeros(ns), *tmax = zeros(ns), *tlow = zeros(ns);
   double *bdn = zeros(ns), *bup = zeros(ns);
   /* set_tracer_flux_horizontal (:166-243) */
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k)
     for (int j = 2; j <= jjm - 1; ++j)
       for (int i = 2; i <= iim - 1; ++i) {
@@ -496,6 +519,7 @@ static void tracer_fct(orc_model* m, double* trc, const double* trc_high, const 
         }
       }
   wrap_edge(m, f1);
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k)
     for (int j = 2; j <= jjm - 1; ++j)
       for (int i = 2; i <= iim - 1; ++i) {
@@ -504,10 +528,12 @@ static void tracer_fct(orc_model* m, double* trc, const double* trc_high, const 
           f2[e + 3 * S(i, j, k)] = 0.5 * (trc_high[S(i, j, k)] + trc_high[up[e]]) * mflux[e + 3 * S(i, j, k)];
       }
   wrap_edge(m, f2);
+  #pragma omp parallel for schedule(static)
   for (size_t q = 0; q < ne; ++q) anti[q] = f2[q] - f1[q];
   /* set_msstrc_low (:251-321) */
   {
     double* base = zeros(ns);
+    #pragma omp parallel for schedule(static)
     for (size_t n = 0; n < ns; ++n) base[n] = mss[n] * trc[n];
     edge_update(m, base, f1, ml);
     free(base);
@@ -516,7 +542,9 @@ static void tracer_fct(orc_model* m, double* trc, const double* trc_high, const 
   wrap_face(m, trc);
   wrap_face(m, mss_horz);
   wrap_face(m, ml);
+  #pragma omp parallel for schedule(static)
   for (size_t n = 0; n < ns; ++n) tlow[n] = mss_horz[n] > 0.0 ? ml[n] / mss_horz[n] : 0.0;
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k) {
     const int p = iim - 1, q = jjm - 1;
     double v12[12];
@@ -552,6 +580,7 @@ static void tracer_fct(orc_model* m, double* trc, const double* trc_high, const 
   }
   /* set_beta_weights_trc (:420-531) */
   const double small = 1.0e-18;
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k) {
     int i = 1, j = jjm - 1;
     for (int pole = 0; pole < 2; ++pole) {
@@ -584,6 +613,7 @@ static void tracer_fct(orc_model* m, double* trc, const double* trc_high, const 
   wrap_face(m, bdn);
   wrap_face(m, bup);
   /* set_corrected_trc_anti (:539-593) */
+  #pragma omp parallel for schedule(static)
   for (int k = 1; k <= 10; ++k)
     for (int j = 2; j <= jjm - 1; ++j)
       for (int i = 2; i <= iim - 1; ++i) {
@@ -597,6 +627,7 @@ static void tracer_fct(orc_model* m, double* trc, const double* trc_high, const 
       }
   wrap_edge(m, anti);
   edge_update(m, ml, anti, mf);
+  #pragma omp parallel for schedule(static)
   for (size_t n = 0; n < ns; ++n) trc_fct[n] = mss_horz[n] != 0.0 ? mf[n] / mss_horz[n] : 0.0;
   free(f1); free(f2); free(anti); free(ml); free(mf); free(tmin); free(tmax); free(tlow); free(bdn); free(bup);
 }
@@ -650,6 +681,7 @@ static void relax(int iters, int lev, const double* lw, const double* beta, doub
       const double* b = beta + v * nl;
       double* t = temp + v * nl;
 #define A(i, j, k) a[S(i, j, k)]
+      #pragma omp parallel for schedule(static)
       for (int k = 1; k <= 10; ++k) {
         for (int j = 2; j <= jjm - 1; ++j)
           for (int i = 2; i <= iim - 1; ++i) {
@@ -688,6 +720,7 @@ static void inject_residual(int lev, const double* lw, double* xa2, const double
     double* a = xa2 + v * nl2;
     const double* b = xb2 + v * nl2;
     double* c = x1 + v * nl1;
+    #pragma omp parallel for schedule(static)
     for (int k = 1; k <= 10; ++k) {
       int iim = iim2, jjm = jjm2;
 #define A(i, j) a[S(i, j, k)]
@@ -731,6 +764,7 @@ static void interpolate(int lev, double* x1, double* x2, int nvar) {
     const double* c = x1 + v * nl1;
     double* f = x2 + v * nl2;
     memset(temp, 0, sizeof(double) * nl2);
+    #pragma omp parallel for schedule(static)
     for (int k = 1; k <= 10; ++k) {
       int iim = iim1, jjm = jjm1;
       for (int j1 = 1; j1 <= iim1 - 1; ++j1)
@@ -743,6 +777,7 @@ static void interpolate(int lev, double* x1, double* x2, int nvar) {
     }
     int iim = iim2, jjm = jjm2;
     const int p2 = iim2 - 1, q2 = jjm2 - 1;
+    #pragma omp parallel for schedule(static)
     for (int k = 1; k <= 10; ++k) {
       for (int j = 3; j <= jjm2 - 1; j += 2)
         for (int i = 3; i <= iim2 - 1; i += 2) {
@@ -814,6 +849,14 @@ static double l2_resid(orc_model* m, const double* rhs, const double* x) {
 /* elliptic_solver.F:40-195 elliptic: mean removal, two V-cycles, then up to
  * ten more for each field that has not reached mean(resid^2) <= 1e-20 */
 void orc_set_sum_mode(orc_model* m, int mode) { m->sum_mode = mode ? 1 : 0; }
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
 
 int orc_elliptic(orc_model* m, int nvar, double* rhs, double* xout) {
   const size_t ns = m->ns;
@@ -980,9 +1023,11 @@ void orc_init(orc_model* m, const double* h, const double* u, const double* trac
   free(kec);
   wrap_face(m, m->relative); wrap_face(m, m->div[0]); wrap_face(m, m->ke);
   memcpy(m->div[1], m->div[0], ns * sizeof(double));
+  #pragma omp parallel for schedule(static)
   for (size_t n = 0; n < ns; ++n) m->eta[0][n] = m->f[n] + m->relative[n];
   memcpy(m->eta[1], m->eta[0], ns * sizeof(double));
   for (int q = 0; q < m->nt; ++q) {
+    #pragma omp parallel for schedule(static)
     for (int k = 1; k <= 10; ++k)
       for (int j = 1; j <= jjm; ++j)
         for (int i = 1; i <= iim; ++i)
@@ -1020,6 +1065,7 @@ static int step_dt(orc_model* m) {
     double* w = zeros(ns);
     /* step_surface_height (step.F:221-270) */
     div_edge(iim, jjm, m->area_inv, m->mass_flux, w);
+    #pragma omp parallel for schedule(static)
     for (size_t n = 0; n < ns; ++n) m->h_f[a][n] = -w[n];
     for (size_t n = 0; n < ns; ++n) {
       m->h[n4][n] = m->h[n3][n] + dt * (m->w3 * m->h_f[a][n] + m->w2 * m->h_f[b][n] + m->w1 * m->h_f[c][n]);
@@ -1028,6 +1074,7 @@ static int step_dt(orc_model* m) {
     /* step_tracer (step.F:278-327) */
     for (int q = 0; q < m->nt; ++q) {
       div_edge(iim, jjm, m->area_inv, tflux[q], w);
+      #pragma omp parallel for schedule(static)
       for (size_t n = 0; n < ns; ++n) m->tr_f[a][q][n] = -w[n];
       for (size_t n = 0; n < ns; ++n) {
         double t = m->tr[n3][q][n] * m->h[n3][n] +
@@ -1071,12 +1118,15 @@ static int step_dt(orc_model* m) {
   const double dt = m->dt;
   const int a = m->n3f - 1, b = m->n2f - 1, c = m->n1f - 1;
   div_edge(iim, jjm, m->area_inv, vflux, w1);
+  #pragma omp parallel for schedule(static)
   for (size_t n = 0; n < ns; ++n) m->eta_f[a][n] = -w1[n];
   for (size_t n = 0; n < ns; ++n)
     m->eta[n4][n] = m->eta[n3][n] + dt * (m->w3 * m->eta_f[a][n] + m->w2 * m->eta_f[b][n] + m->w1 * m->eta_f[c][n]);
   div_edge(iim, jjm, m->area_inv, dcurl, w3);
+  #pragma omp parallel for schedule(static)
   for (size_t n = 0; n < ns; ++n) w1[n] = m->ke[n] + m->grav * (m->h[n3][n] + m->hs[n]);
   laplacian(iim, jjm, m->lw, w1, w2);
+  #pragma omp parallel for schedule(static)
   for (size_t n = 0; n < ns; ++n) m->div_f[a][n] = -w2[n] + w3[n];
   for (size_t n = 0; n < ns; ++n)
     m->div[n4][n] = m->div[n3][n] + dt * (m->w3 * m->div_f[a][n] + m->w2 * m->div_f[b][n] + m->w1 * m->div_f[c][n]);
@@ -1170,6 +1220,7 @@ int orc_get(const orc_model* m, const char* name, double* out) {
   else if (!strcmp(name, "u")) memcpy(out, m->u[n4], 4 * ns * 8);
   else if (!strcmp(name, "tracer")) {
     for (int q = 0; q < m->nt; ++q)
+      #pragma omp parallel for schedule(static)
       for (int k = 1; k <= 10; ++k)
         for (int j = 1; j <= jjm; ++j)
           for (int i = 1; i <= iim; ++i)

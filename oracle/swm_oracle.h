@@ -71,6 +71,10 @@ void orc_budgets(const orc_model* m, double* out);
  * which the solve and the velocity gradient turn into ~1e-12 of u. */
 void orc_set_sum_mode(orc_model* m, int mode);
 
+/* Host threads of the panel-parallel loops (OpenMP; results do not depend
+ * on the count). */
+void orc_set_threads(int n);
+
 /* Number of V-cycles run by the last elliptic solve (2 + extra). */
 int orc_last_vcycles(const orc_model* m);
 

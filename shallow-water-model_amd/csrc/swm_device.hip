@@ -2132,6 +2132,55 @@ swm_status swm_total_mass(swm_ctx* C, double* mass) {
   return SWM_OK;
 }
 
+// Microbench of single operators on the context's resident metrics and
+// state: reps back-to-back launches of the kernel the operator entry point
+// runs, timed with HIP events on the context's stream.  The output goes to a
+// scratch plane of the step; the state is not changed.
+swm_status swm_operator_bench(swm_ctx* C, int op, int reps, double* us_per_launch, double* bytes_per_launch) {
+  if (!C || op < 0 || op > 3 || reps < 1 || !us_per_launch || !bytes_per_launch) {
+    swm::set_error("swm_operator_bench: bad argument (op 0..3, reps >= 1)");
+    return SWM_ERR_ARG;
+  }
+  if (C->precision != SWM_FP64 || !C->initialized) {
+    swm::set_error("swm_operator_bench: needs an initialised FP64 context");
+    return SWM_ERR_STATE;
+  }
+  HIP_TRY(hipSetDevice(C->device));
+  swm::MemberScope scope(C);
+  View<double>& V = C->d;
+  const swm::dev::Geo& G = V.geo;
+  const int lev = C->n4 - 1;
+  const unsigned nb = nblk(C->ns);
+  double* out = V.flx1;
+  if (op == 3)   // the halo fill works in place on a copy of the thickness
+    HIP_TRY(hipMemcpyAsync(out, V.h[lev], C->ns * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
+  auto launch = [&]() {
+    switch (op) {
+      case 0: hipLaunchKernelGGL(swm::dev::k_op_div_edge, dim3(nb), dim3(BLK), 0, C->stream, G, V.mass_flux, out); break;
+      case 1: hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb), dim3(BLK), 0, C->stream, G, V.u[lev], out,
+                                 (double*)nullptr); break;
+      case 2: hipLaunchKernelGGL(swm::dev::k_op_lap, dim3(nb), dim3(BLK), 0, C->stream, G, V.h[lev], out); break;
+      default: gather<double>(C, C->face, {out}); break;
+    }
+  };
+  launch();   // warm
+  HIP_TRY(hipEventRecord(C->ev_a, C->stream));
+  for (int r = 0; r < reps; ++r) launch();
+  HIP_TRY(hipEventRecord(C->ev_b, C->stream));
+  HIP_TRY(hipEventSynchronize(C->ev_b));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, C->ev_a, C->ev_b));
+  *us_per_launch = 1e3 * (double)ms / reps;
+  // algorithmic FP64 bytes per real cell: edge-flux divergence 3 own edges
+  // + 3 neighbour edges, area_inv, result (40 B); curl of a corner vector
+  // 12 v_weights, 4 velocity components, area_inv, result (144 B);
+  // Laplacian 7 weights, centre value, result (72 B); halo fill 16 B per
+  // ghost slot (value read and written)
+  static const double per_cell[3] = {40.0, 144.0, 72.0};
+  *bytes_per_launch = op < 3 ? per_cell[op] * (double)C->max_ig : 16.0 * (double)C->face.n;
+  return SWM_OK;
+}
+
 #define OP_BEGIN(C)                                                              \
   if (!(C)) { swm::set_error("null context"); return SWM_ERR_ARG; }             \
   HIP_TRY(hipSetDevice((C)->device));                                           \

@@ -1947,13 +1947,13 @@ namespace dev {
 
 // single reference operators on every slot (for the operator entry points)
 __global__ void k_op_div_edge(Geo G, const double* __restrict__ F, double* __restrict__ out) {
-  const int s = blockIdx.x * bdim() + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= G.ns) return;
   const IJK c = ijk_of(s, G.iim, G.jjm);
   out[s] = div_edge_at(G.iim, G.jjm, G.ns, c, F, G.area_inv[s]);
 }
 __global__ void k_op_lap(Geo G, const double* __restrict__ x, double* __restrict__ out) {
-  const int s = blockIdx.x * bdim() + threadIdx.x;
+  const int s = xcd_slot();
   if (s >= G.ns) return;
   const IJK c = ijk_of(s, G.iim, G.jjm);
   auto X = [&](int sl) { return x[sl]; };

@@ -117,6 +117,7 @@ _sig("swm_init_vorticity", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] *
 _sig("swm_adj_forward", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
 _sig("swm_tl_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 8)
 _sig("swm_adj_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 8)
+_sig("swm_operator_bench", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp])
 _sig("swm_adj_limiter_stats", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 2)
 _sig("swm_wrap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p])
 _sig("swm_elliptic", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
@@ -616,6 +617,17 @@ class Model:
         if init_only:
             return tuple(out[:3])
         return tuple(out[:3]) if control else tuple(out)
+
+    OPERATOR_BENCH = ("div_of_an_edge_flux", "curl_of_a_vector", "laplacian_of_a_scalar", "face_halo_fill")
+
+    def operator_bench(self, op, reps=200):
+        """Microbench of one operator on the resident state (FP64): op is a
+        name of OPERATOR_BENCH; returns (us per launch, algorithmic bytes
+        per launch)."""
+        us, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib.swm_operator_bench(self._h, self.OPERATOR_BENCH.index(op), reps, ctypes.byref(us),
+                                      ctypes.byref(b)))
+        return us.value, b.value
 
     def limiter_stats(self):
         """FCT window: (edge-steps with a frozen limiter coefficient below 1,

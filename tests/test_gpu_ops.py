@@ -194,3 +194,24 @@ def test_entry_point_errors():
     with pytest.raises(S.SwmError):   # npad other than 1
         S._check(S.lib.swm_op_laplacian_of_a_scalar(model._h, 2, grid.iim, grid.iim, 10, 7, S._ptr(lw), S._ptr(x),
                                                     S._ptr(x)))
+
+
+def test_operator_bench_leaves_the_state_and_reports_bytes():
+    print("case operator microbench", flush=True)
+    grid = S.Grid(4)
+    m = S.Model(grid, ntracer=1, dt=400.0, grav=S.REF_GRAV)
+    m.init_synthetic(lmax=8)
+    m.step(2)
+    before = {n: m.field(n) for n in ("h", "eta", "div", "u", "psi")}
+    per = {"div_of_an_edge_flux": 40.0 * grid.max_ig, "curl_of_a_vector": 144.0 * grid.max_ig,
+           "laplacian_of_a_scalar": 72.0 * grid.max_ig,
+           "face_halo_fill": 16.0 * len(grid.array("face_dst"))}
+    for op in S.Model.OPERATOR_BENCH:
+        us, nbytes = m.operator_bench(op, reps=5)
+        assert us > 0.0 and nbytes == per[op], (op, us, nbytes)
+    for n, x in before.items():
+        assert np.array_equal(m.field(n), x), n
+    mix = S.Model(grid, ntracer=1, dt=400.0, grav=S.REF_GRAV, precision=S.SWM_MIXED)
+    mix.init_synthetic(lmax=8)
+    with pytest.raises(S.SwmError):
+        mix.operator_bench("face_halo_fill")
