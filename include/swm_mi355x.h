@@ -402,6 +402,20 @@ swm_status swm_tl_run(swm_ctx* ctx, int mode, const double* in0, const double* i
 swm_status swm_adj_run(swm_ctx* ctx, int mode, const double* lh, const double* leta, const double* ldiv,
                        const double* lu, double* out0, double* out1, double* out2, double* out3);
 
+/* What the linearised model differentiates in the elliptic solves of the
+ * window's steps.  exact = 1 (default): the forward model as it runs, each
+ * solve being the base step's V-cycles (their number as the base step's
+ * convergence test chose it, elliptic_solver.F:120-180) from the previous
+ * step's solution; the tangent-linear runs the same cycles on the
+ * perturbations and carries the warm start's perturbation from step to step,
+ * the adjoint runs the transposed cycle kernels in reverse order.  The
+ * gradient is then the model's own, whatever its criterion (the reference's
+ * takes 2-3 cycles).  exact = 0: the solves are taken as exact inverses of
+ * the Laplacian, solved to a relative residual of 1e-13 (or the FP64 floor)
+ * in both sweeps; the derivative of a forward that solves to that floor.
+ * Levels 1..3 always use exact = 0. */
+swm_status swm_set_linearisation(swm_ctx* ctx, int exact);
+
 /* FCT windows: how many edges had a frozen limiter coefficient below 1
  * (the mass limiter of mass_fct.F:403-532 acting, where the frozen
  * linearisation is not the derivative), summed over the base steps that the

@@ -161,6 +161,11 @@ struct swm_ctx {
   std::vector<double> cell_mask;
   double w1 = 0, w2 = 0, w3 = 0;
   int last_vcycles = 0;
+  // cycles of the last FP64 two-field solve: `first` cycles of both fields
+  // (2, or the fixed count), then `extra` cycles of the fields in `bad_mask`
+  // (bit 0 psi, bit 1 chi); fixed: the fixed schedule (no halo fill between)
+  struct SolveSched { int first = 0, extra = 0, bad_mask = 0, fixed = 0; } last_sched;
+  int lin_exact = 1;           // TL/AD: the derivative of the forward's own cycles (1) or of converged solves (0)
   bool initialized = false;
   // elliptic solve settings (reference: mean(r^2) <= 1e-20, 12 cycles) and
   // the mean squared residuals / right-hand sides of the last solve
@@ -554,6 +559,7 @@ swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
     hipLaunchKernelGGL(swm::dev::k_track_resid, zgrid(C, 1), dim3(64), 0, C->stream, C->sums, 1.0 / (double)C->max_ig,
                        C->solver_tol, C->track, C->mb);
     C->last_vcycles = C->fixed_cycles;
+    C->last_sched = {C->fixed_cycles, 0, 3, 1};
     C->solves += 1; C->vcycles_total += C->fixed_cycles; C->checks_total += 1;
     return SWM_OK;
   }
@@ -605,6 +611,11 @@ swm_status elliptic(swm_ctx* C, double* rhs, double* xout, int nvar) {
       }
   }
   C->last_vcycles = 2 + iter;
+  {
+    int mask = 0;
+    for (int v : bad) mask |= 1 << v;
+    C->last_sched = {2, iter, mask, 0};
+  }
   C->solves += 1; C->vcycles_total += 2 + iter; C->checks_total += 1 + iter;
   if (any) {
     swm::set_error("elliptic solve did not converge (residual above the criterion after " +
@@ -2504,6 +2515,14 @@ struct TlAd {
   // base steps recomputed by the last TL or AD sweep
   unsigned long long* lim_count = nullptr;
   long long lim_steps = 0;
+  // exact linearisation of the solves: the schedule of each base step of the
+  // segment, the adjoint multigrid buffers (per level: iterate, right-hand
+  // side, scratch; two fields) and the coarse cycle's transpose
+  std::vector<swm_ctx::SolveSched> sched;
+  double* mg_mem = nullptr;
+  double *mg_a[24] = {}, *mg_b[24] = {}, *mg_t[24] = {};
+  double* coarse_mt = nullptr;
+  int coarse_n = 0;
 };
 
 namespace swm {
@@ -2515,6 +2534,8 @@ void tlad_free(swm_ctx* C) {
   if (T->seg) (void)hipFree(T->seg);
   if (T->mem) (void)hipFree(T->mem);
   if (T->lim_count) (void)hipFree(T->lim_count);
+  if (T->mg_mem) (void)hipFree(T->mg_mem);
+  if (T->coarse_mt) (void)hipFree(T->coarse_mt);
   delete T;
   C->tl = nullptr;
 }
@@ -2620,6 +2641,7 @@ swm_status recompute_segment(swm_ctx* C, int sidx, int len) {
     st = base_step(C);
     C->fct_rec = nullptr;
     if (st) return st;
+    T->sched[q] = C->last_sched;
     if (fct) {
       HIP_TRY(hipMemcpyAsync(seg_F(C, q), C->d.mass_flux, 3 * (size_t)C->ns * sizeof(double), hipMemcpyDeviceToDevice,
                              C->stream));
@@ -2689,6 +2711,168 @@ void mean_removal_adj(swm_ctx* C, double* y) {
                      1.0 / C->total_area, y);
 }
 
+// ---------------------------------------- exact linearisation of the solve
+// The FP64 step's solve (elliptic) is: halo fills of the warm start and the
+// right-hand side, mean removal of the right-hand side, `first` V-cycles of
+// both fields, then `extra` cycles of the fields that missed the criterion,
+// with halo fills of the iterate between.  Every kernel of a V-cycle is
+// linear with the level's weights as coefficients, so for the cycle counts
+// of the base step the solve is a linear map of (warm start, right-hand
+// side).  Its tangent-linear is the same sequence on the perturbations (the
+// warm start's perturbation carried from the previous step), its adjoint
+// the transposed kernels in reverse order.  This is the derivative of the
+// model as it runs, whatever its criterion; the solves' dependence on the
+// warm start is part of it.  Needs a level above the coarse cycle's.
+bool exact_lin(const swm_ctx* C) { return C->lin_exact && C->g > swm::dev::CO_MAXLEV; }
+
+// adjoint level buffers and the coarse cycle's transpose, once per context:
+// the transpose is probed from the device's own coarse kernel, one unit
+// right-hand side at a time (column j of M is the cycle's answer to e_j,
+// which is row j of M^T)
+swm_status exact_prepare(swm_ctx* C) {
+  TlAd* T = C->tl;
+  if (T->mg_mem) return SWM_OK;
+  View<double>& V = C->d;
+  const int G = C->g, top = swm::dev::CO_MAXLEV;
+  size_t words = 0;
+  for (int l = top; l <= G; ++l) words += 3 * 2 * (size_t)V.lev[l - 1].ns;
+  HIP_TRY(hipMalloc(&T->mg_mem, words * sizeof(double)));
+  fill_plain<double>(C, T->mg_mem, words, 0.0);
+  double* p = T->mg_mem;
+  for (int l = top; l <= G; ++l) {
+    const size_t n2 = 2 * (size_t)V.lev[l - 1].ns;
+    T->mg_a[l] = p; p += n2;
+    T->mg_b[l] = p; p += n2;
+    T->mg_t[l] = p; p += n2;
+  }
+  const int n = V.lev[top - 1].ns;
+  T->coarse_n = n;
+  HIP_TRY(hipMalloc(&T->coarse_mt, (size_t)n * n * sizeof(double)));
+  swm::dev::CoarseArgs<double> P{};
+  for (int l = 1; l <= top; ++l) P.lev[l - 1] = V.lev[l - 1].L;
+  P.nlev = top;
+  P.nvar = 1;
+  P.beta = V.lev[top - 1].beta;
+  P.alpha = V.lev[top - 1].alpha;
+  P.zero = 1;
+  P.mb = C->mb;
+  for (int j = 0; j < n; ++j) {
+    hipLaunchKernelGGL(swm::dev::k_unit, dim3(nblk(n)), dim3(BLK), 0, C->stream, V.lev[top - 1].beta, n, j);
+    hipLaunchKernelGGL(swm::dev::k_coarse_cycle<double>, dim3(1), dim3(swm::dev::CO_THREADS), 0, C->stream, P);
+    HIP_TRY(hipMemcpyAsync(T->coarse_mt + (size_t)j * n, V.lev[top - 1].alpha, n * sizeof(double),
+                           hipMemcpyDeviceToDevice, C->stream));
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  return SWM_OK;
+}
+
+// TL of the step's solve with the base step's schedule: rhs (gathered and
+// mean-removed in place), x the warm start's perturbation and the result
+void solve_sched_tl(swm_ctx* C, double* rhs, double* x, const swm_ctx::SolveSched& sc) {
+  const int ns = C->ns;
+  View<double>& V = C->d;
+  for (int v = 0; v < 2; ++v) gather<double>(C, C->face, {x + (size_t)v * ns, rhs + (size_t)v * ns});
+  sum2<double>(C, rhs, rhs + ns, C->area_mask, ns);
+  hipLaunchKernelGGL(swm::dev::k_sub_mean<double>, zgrid(C, nblk(ns)), dim3(BLK), 0, C->stream, ns, C->sums,
+                     C->total_area, rhs, rhs + ns, C->mb);
+  for (int it = 0; it < sc.first; ++it) vcycle<double>(C, V, 2, x, rhs);
+  for (int v = 0; v < 2; ++v) gather<double>(C, C->face, {x + (size_t)v * ns});
+  if (sc.fixed) return;
+  for (int it = 0; it < sc.extra; ++it) {
+    if (sc.bad_mask == 3) vcycle<double>(C, V, 2, x, rhs);
+    else {
+      const int b = sc.bad_mask == 1 ? 0 : 1;
+      vcycle<double>(C, V, 1, x + (size_t)b * ns, rhs + (size_t)b * ns);
+    }
+    for (int v = 0; v < 2; ++v)
+      if (sc.bad_mask >> v & 1) gather<double>(C, C->face, {x + (size_t)v * ns});
+  }
+}
+
+// transpose of vcycle<double> (the FP64 cycle: top_zero off, two finest
+// sweeps) on nvar fields: xb the adjoint of the finest iterate (out: of its
+// value on entry), bb += the adjoint of the finest right-hand side
+template <int NV>
+void vcycle_ad_nv(swm_ctx* C, double* xb, double* bb) {
+  TlAd* T = C->tl;
+  View<double>& V = C->d;
+  const int G = C->g, top = swm::dev::CO_MAXLEV;
+  auto Lv = [&](int l) -> const Lev& { return V.lev[l - 1].L; };
+  auto Ab = [&](int l) { return l == G ? xb : T->mg_a[l]; };
+  auto Bb = [&](int l) { return l == G ? bb : T->mg_b[l]; };
+  auto Tb = [&](int l) { return T->mg_t[l]; };
+  auto grid_of = [&](int l) { return dim3(nblk_b(V.lev[l - 1].ns, C->blk_mg)); };
+  const dim3 blk(C->blk_mg);
+  hipStream_t S = C->stream;
+  // a sweep out = R(a, b) at level l, transposed: ob consumed, ab and bb added
+  auto relax_ad = [&](int l, double* ob, double* ab, double* b, bool zero) {
+    if (zero) hipLaunchKernelGGL((swm::dev::k_relax_ad<NV, true>), grid_of(l), blk, 0, S, Lv(l), ob, ab, b);
+    else hipLaunchKernelGGL((swm::dev::k_relax_ad<NV, false>), grid_of(l), blk, 0, S, Lv(l), ob, ab, b);
+  };
+  const bool fuse = C->fuse_mg;
+  // the way up, reversed: level l+1's two sweeps (alpha -> temp -> alpha),
+  // then the interpolation from level l
+  for (int l = G - 1; l >= top; --l) {
+    relax_ad(l + 1, Ab(l + 1), Tb(l + 1), Bb(l + 1), false);
+    relax_ad(l + 1, Tb(l + 1), Ab(l + 1), Bb(l + 1), false);
+    hipLaunchKernelGGL((swm::dev::k_interp_ad<NV>), grid_of(l + 1), blk, 0, S, Lv(l), Lv(l + 1), Ab(l), Ab(l + 1));
+  }
+  // the coarse cycle from zero: bb(top) += M^T ab(top), whose iterate's
+  // adjoint is then spent
+  {
+    const int n = T->coarse_n;
+    hipLaunchKernelGGL(swm::dev::k_coarse_ad, dim3((unsigned)((n + 3) / 4), NV), dim3(256), 0, S, T->coarse_mt, n,
+                       Ab(top), Bb(top));
+    fill_plain<double>(C, Ab(top), (size_t)NV * V.lev[top - 1].ns, 0.0);
+  }
+  // the way down, reversed: the injection into level l-1 (with its fused
+  // first sweep of level l-1 where that level runs its own sweeps), then
+  // level l's sweeps
+  for (int l = top + 1; l <= G; ++l) {
+    double* cz = fuse && l - 1 > top ? Tb(l - 1) : nullptr;
+    hipLaunchKernelGGL((swm::dev::k_inject_ad<NV>), grid_of(l - 1), blk, 0, S, Lv(l), Lv(l - 1), Ab(l), Bb(l),
+                       Bb(l - 1), cz);
+    if (l == G) {
+      relax_ad(l, Ab(l), Tb(l), Bb(l), false);
+      relax_ad(l, Tb(l), Ab(l), Bb(l), false);
+    } else if (fuse) {
+      // the second sweep from the injection's first one (in temp)
+      relax_ad(l, Ab(l), Tb(l), Bb(l), false);
+    } else {
+      // two sweeps, the first from zero: alpha -> temp -> alpha
+      relax_ad(l, Ab(l), Tb(l), Bb(l), false);
+      relax_ad(l, Tb(l), Ab(l), Bb(l), true);
+    }
+  }
+}
+void vcycle_ad(swm_ctx* C, int nvar, double* xb, double* bb) {
+  if (nvar == 2) vcycle_ad_nv<2>(C, xb, bb);
+  else vcycle_ad_nv<1>(C, xb, bb);
+}
+
+// AD of solve_sched_tl up to (not including) the mean removal and the first
+// halo fills: xb the adjoint of the solution after the solve's last halo fill
+// (out: of the iterate before the cycles); bb = the adjoint of the
+// mean-removed right-hand side
+void solve_sched_ad(swm_ctx* C, double* xb, double* bb, const swm_ctx::SolveSched& sc) {
+  const int ns = C->ns;
+  fill<double>(C, bb, 2 * (size_t)ns, 0.0);
+  if (!sc.fixed) {
+    for (int it = sc.extra - 1; it >= 0; --it) {
+      for (int v = 0; v < 2; ++v)
+        if (sc.bad_mask >> v & 1) gather_adj(C, C->face, {xb + (size_t)v * ns});
+      if (sc.bad_mask == 3) vcycle_ad(C, 2, xb, bb);
+      else {
+        const int b = sc.bad_mask == 1 ? 0 : 1;
+        vcycle_ad(C, 1, xb + (size_t)b * ns, bb + (size_t)b * ns);
+      }
+    }
+  }
+  gather_adj(C, C->face, {xb, xb + ns});
+  for (int it = 0; it < sc.first; ++it) vcycle_ad(C, 2, xb, bb);
+}
+
 // ---------------------------------------------------------------- TL step
 // window step with base state q -> q+1; pointers / weights of the model step
 swm_status tl_step(swm_ctx* C, int q) {
@@ -2737,8 +2921,12 @@ swm_status tl_step(swm_ctx* C, int q) {
   hipLaunchKernelGGL(swm::dev::k_tendencies<double>, dim3(nb), dim3(BLK), 0, S, T->lin, dt, C->grav, w3, w2, w1, T->vf,
                      T->dc, T->ke, dh3, de3, T->div[n3], T->ef[a], T->ef[b], T->ef[c], T->df[a], T->df[b], T->df[c],
                      T->eta[n4], T->div[n4], T->X, T->rhs, T->rhs + ns, (const double*)nullptr, (double*)nullptr);
-  swm_status st = lin_solve(C, T->rhs, T->xs);
-  if (st) return st;
+  if (exact_lin(C)) {
+    solve_sched_tl(C, T->rhs, T->xs, T->sched[q]);   // xs: the warm start's perturbation, carried
+  } else {
+    swm_status st = lin_solve(C, T->rhs, T->xs);
+    if (st) return st;
+  }
   hipLaunchKernelGGL(swm::dev::k_post_elliptic<double>, dim3(nb), dim3(BLK), 0, S, T->lin, T->rhs, T->eta[n4]);
   hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, G, T->xs, T->xs + ns, T->u[n4], (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr);
   gather<double>(C, C->vrtx, {T->u[n4], T->u[n4] + 2 * ns});
@@ -2767,6 +2955,18 @@ swm_status ad_step(swm_ctx* C, int q) {
   gather_adj(C, C->vrtx, {T->u[n4], T->u[n4] + 2 * ns});
   fill<double>(C, T->psia, 2 * ns, 0.0);
   hipLaunchKernelGGL(swm::dev::k_ad_velocity, dim3(nb), dim3(BLK), 0, S, G, T->u[n4], T->psia, T->psia + ns);
+  if (exact_lin(C)) {
+    // the solution's adjoint: the velocity's and (xs, carried) the next
+    // step's warm start's; through the solve's cycles, transposed
+    hipLaunchKernelGGL(swm::dev::k_axpy, dim3(nblk(2 * ns)), dim3(BLK), 0, S, 2 * ns, 1.0, T->psia, T->xs);
+    solve_sched_ad(C, T->xs, T->rhs, T->sched[q]);
+    hipLaunchKernelGGL(swm::dev::k_axpy, dim3(nb), dim3(BLK), 0, S, ns, 1.0, T->eta[n4], T->rhs);
+    fill<double>(C, T->eta[n4], ns, 0.0);
+    mean_removal_adj(C, T->rhs);
+    mean_removal_adj(C, T->rhs + ns);
+    gather_adj(C, C->face, {T->rhs, T->rhs + ns});
+    gather_adj(C, C->face, {T->xs, T->xs + ns});   // xs: the warm start's adjoint, to the previous step
+  } else {
   // the solve's final halo fill, then psi_a / A, solved: rhs_a = A z
   gather_adj(C, C->face, {T->psia, T->psia + ns});
   hipLaunchKernelGGL(swm::dev::k_mul_mask, dim3(nb), dim3(BLK), 0, S, (int)ns, T->psia, T->am_inv, C->mask, T->psia);
@@ -2784,6 +2984,7 @@ swm_status ad_step(swm_ctx* C, int q) {
   mean_removal_adj(C, T->rhs);
   mean_removal_adj(C, T->rhs + ns);
   gather_adj(C, C->face, {T->rhs, T->rhs + ns});
+  }
   // tendencies: rhs2 = d4 joins the new level's adjoint
   hipLaunchKernelGGL(swm::dev::k_axpy, dim3(nb), dim3(BLK), 0, S, ns, 1.0, T->rhs + ns, T->div[n4]);
   fill<double>(C, T->vf, 3 * ns, 0.0);
@@ -2909,6 +3110,9 @@ swm_status init_ad(swm_ctx* C, double* gz, double* gd, double* gh) {
   // u = velocity(psi, chi)
   fill<double>(C, T->psia, 2 * ns, 0.0);
   hipLaunchKernelGGL(swm::dev::k_ad_velocity, dim3(nb), dim3(BLK), 0, S, G, T->u[0], T->psia, T->psia + ns);
+  // the solution is also the first step's warm start (exact linearisation)
+  if (exact_lin(C))
+    hipLaunchKernelGGL(swm::dev::k_axpy, dim3(nblk(2 * ns)), dim3(BLK), 0, S, 2 * ns, 1.0, T->xs, T->psia);
   gather_adj(C, C->face, {T->psia, T->psia + ns});
   hipLaunchKernelGGL(swm::dev::k_mul_mask, dim3(nb), dim3(BLK), 0, S, (int)ns, T->psia, T->am_inv, C->mask, T->psia);
   hipLaunchKernelGGL(swm::dev::k_mul_mask, dim3(nb), dim3(BLK), 0, S, (int)ns, T->psia + ns, T->am_inv, C->mask,
@@ -3033,6 +3237,7 @@ swm_status swm_adj_forward(swm_ctx* C, int nsteps, int every) {
   HIP_TRY(hipMalloc(&T->seg, (size_t)(every + 1) * T->seg_planes * C->ns * sizeof(double)));
   T->every = every;
   T->nsteps = nsteps;
+  T->sched.assign((size_t)every, swm_ctx::SolveSched{});
   HIP_TRY(hipEventRecord(C->ev_a, C->stream));
   for (int n = 0; n < nsteps; ++n) {
     if (n % every == 0) {
@@ -3087,7 +3292,9 @@ swm_status swm_tl_run(swm_ctx* C, int mode, const double* in0, const double* in1
     (void)hipStreamSynchronize(C->stream);
     (void)hipFree(buf);
     if (st) return st;
-    fill<double>(C, T->xs, 2 * ns, 0.0);
+    // exact linearisation: xs, the initial solve's perturbation, is the
+    // first step's warm start; else the solves do not depend on it
+    if (!swm::exact_lin(C)) fill<double>(C, T->xs, 2 * ns, 0.0);
     if (mode == 2) return swm::state_out(C, swm::first_level(), out_h, out_eta, out_div, out_u);
   }
   const int nseg = (T->nsteps + T->every - 1) / T->every;
@@ -3124,6 +3331,7 @@ swm_status swm_adj_run(swm_ctx* C, int mode, const double* lh, const double* let
   TlAd* T = C->tl;
   const size_t ns = C->ns;
   swm_status st;
+  if (swm::exact_lin(C) && (st = swm::exact_prepare(C))) return st;
   swm::clear_state(C);
   HIP_TRY(hipMemsetAsync(T->lim_count, 0, sizeof(unsigned long long), C->stream));
   T->lim_steps = 0;
@@ -3161,6 +3369,12 @@ swm_status swm_adj_run(swm_ctx* C, int mode, const double* lh, const double* let
   const int nck = (int)T->ckpt.size();
   if ((st = swm::restore(C, T->ckpt[nck - 1], T->ckpt_meta[2 * nck - 2], T->ckpt_meta[2 * nck - 1]))) return st;
   HIP_TRY(hipStreamSynchronize(C->stream));
+  return SWM_OK;
+}
+
+swm_status swm_set_linearisation(swm_ctx* C, int exact) {
+  if (!C || exact < 0 || exact > 1) { swm::set_error("swm_set_linearisation: bad argument"); return SWM_ERR_ARG; }
+  C->lin_exact = exact;
   return SWM_OK;
 }
 

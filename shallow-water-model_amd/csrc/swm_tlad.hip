@@ -618,6 +618,213 @@ __global__ void k_ad_fct_low(Geo G, double dt, double w3, double w2, double w1, 
   div_edge_ad(iim, jjm, ns, c, G.area_inv[s], -dt * g, flxa);
 }
 
+// ------------------------------------------------------- multigrid AD --
+// Transposes of the FP64 V-cycle kernels (k_relax, k_inject, k_interp of
+// swm_kernels.hip, multigrid_solver.F:593-1054).  Each kernel of the cycle is
+// linear in its fields with the level's weights as constant coefficients, so
+// the cycle run n times from a warm start is a linear map of (warm start,
+// right-hand side), and these kernels, applied in reverse order, are its
+// exact transpose.  One thread per slot of the forward kernel's output; the
+// adjoint of that output is read and cleared (it has been consumed), and the
+// adjoints of the inputs are added: at the thread's own slot directly, at
+// neighbour slots (read through the level's halo map as the forward reads
+// them) with atomic adds.
+
+// transpose of k_relax<double, NV, ZERO>: ob the adjoint of the sweep's
+// output (cleared), ab += its transpose onto the input iterate, bb += onto
+// the right-hand side
+template <int NV, bool ZERO>
+__global__ void __launch_bounds__(256) k_relax_ad(Lev L, double* __restrict__ ob, double* __restrict__ ab,
+                                                  double* __restrict__ bb) {
+  const int iim = L.iim, jjm = L.jjm, ns = L.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  const bool in = i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1;
+  const bool np = i == 1 && j == jjm - 1, sp = i == iim - 1 && j == 1;
+  double g[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    g[v] = ob[(size_t)v * ns + s];
+    ob[(size_t)v * ns + s] = 0.0;
+  }
+  if (!(in || np || sp)) return;   // the forward writes 0 there
+  const bool deep = i >= 3 && i <= iim - 2 && j >= 3 && j <= jjm - 2;
+  const int* __restrict__ m = L.src;
+  int nb[6];
+  if (in) {
+    nb[0] = SL(i - 1, j, k); nb[1] = SL(i - 1, j - 1, k); nb[2] = SL(i, j - 1, k);
+    nb[3] = SL(i + 1, j, k); nb[4] = SL(i + 1, j + 1, k); nb[5] = SL(i, j + 1, k);
+  } else if (np) {
+    nb[0] = -1; nb[1] = SL(iim, 1, k); nb[2] = SL(i, j - 1, k);
+    nb[3] = SL(i + 1, j, k); nb[4] = SL(i + 1, j + 1, k); nb[5] = SL(i, j + 1, k);
+  } else {
+    nb[0] = SL(i - 1, j, k); nb[1] = SL(1, jjm, k); nb[2] = -1;
+    nb[3] = SL(i + 1, j, k); nb[4] = SL(i + 1, j + 1, k); nb[5] = SL(i, j + 1, k);
+  }
+  int me = s;
+  if (!ZERO && !deep) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) nb[q] = nb[q] < 0 ? -1 : m[nb[q]];
+    me = m[s];
+  }
+  const double wt = L.lwt[s];
+  const double omega = 0.8;
+  double w[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) w[q] = L.lw[(size_t)q * ns + s];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    if (g[v] == 0.0) continue;
+    // out = (1 - omega) a(me) + omega wt (b - sum_q w_q a(nb_q))
+    bb[(size_t)v * ns + s] += omega * wt * g[v];
+    if (ZERO) continue;
+    double* __restrict__ av = ab + (size_t)v * ns;
+    aadd(&av[me], (1.0 - omega) * g[v]);
+    const double h = -omega * wt * g[v];
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+      if (nb[q] >= 0) aadd(&av[nb[q]], w[q] * h);
+  }
+}
+
+// transpose of k_inject<double, NV> (with its fused zero-start sweep z0 when
+// czb is given): cbb, czb the adjoints of the coarse right-hand side and of
+// that sweep (cleared at every coarse slot); fbb += at the fine point's own
+// slot, fab += onto the seven fine values of the residual
+template <int NV>
+__global__ void k_inject_ad(Lev F, Lev C, double* __restrict__ fab, double* __restrict__ fbb,
+                            double* __restrict__ cbb, double* __restrict__ czb) {
+  int iim = C.iim, jjm = C.jjm;
+  const int s1 = xcd_slot();
+  if (s1 >= C.ns) return;
+  const IJK c = ijk_of(s1, iim, jjm);
+  const bool in = c.i >= 2 && c.i <= iim - 1 && c.j >= 2 && c.j <= jjm - 1;
+  const bool np = c.k == 1 && c.i == 1 && c.j == jjm - 1;
+  const bool sp = c.k == 10 && c.i == iim - 1 && c.j == 1;
+  double r[NV];
+  const double zc = czb ? 0.8 * C.lwt[s1] : 0.0;   // zero_sweep: 0.8 wt r
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    r[v] = cbb[(size_t)v * C.ns + s1];
+    cbb[(size_t)v * C.ns + s1] = 0.0;
+    if (czb) {
+      r[v] += zc * czb[(size_t)v * C.ns + s1];
+      czb[(size_t)v * C.ns + s1] = 0.0;
+    }
+  }
+  // outside the injected slots the coarse values are constants of memory
+  if (!(in || np || sp)) return;
+  const bool deep = c.i >= 3 && c.i <= iim - 2 && c.j >= 3 && c.j <= jjm - 2;
+  const int k = c.k;
+  int i2, j2;
+  if (in) { i2 = 2 * c.i - 1; j2 = 2 * c.j - 1; }
+  else if (np) { i2 = 1; j2 = F.jjm - 1; }
+  else { i2 = F.iim - 1; j2 = 1; }
+  iim = F.iim; jjm = F.jjm;
+  const int ns = F.ns;
+  const int s = SL(i2, j2, k);
+  const int* __restrict__ m = F.src;
+  int q[7];
+  q[0] = s;
+  if (in) {
+    q[1] = SL(i2 - 1, j2, k); q[2] = SL(i2 - 1, j2 - 1, k); q[3] = SL(i2, j2 - 1, k);
+  } else if (np) {
+    q[1] = s; q[2] = SL(iim, 1, k); q[3] = SL(i2, j2 - 1, k);
+  } else {
+    q[1] = SL(i2 - 1, j2, k); q[2] = SL(1, jjm, k); q[3] = s;
+  }
+  q[4] = SL(i2 + 1, j2, k); q[5] = SL(i2 + 1, j2 + 1, k); q[6] = SL(i2, j2 + 1, k);
+  if (!deep) {
+#pragma unroll
+    for (int n = 0; n < 7; ++n) q[n] = m[q[n]];
+  }
+  double w[7];
+#pragma unroll
+  for (int n = 0; n < 7; ++n) w[n] = F.lw[(size_t)n * ns + s];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    if (r[v] == 0.0) continue;
+    // r = b(s) - (w7 x(q0) + sum_n w_n x(q_n)), pole forms without one term
+    fbb[(size_t)v * ns + s] += r[v];
+    double* __restrict__ a = fab + (size_t)v * ns;
+    aadd(&a[q[0]], -w[6] * r[v]);
+    if (!np) aadd(&a[q[1]], -w[0] * r[v]);
+    aadd(&a[q[2]], -w[1] * r[v]);
+    if (!sp) aadd(&a[q[3]], -w[2] * r[v]);
+    aadd(&a[q[4]], -w[3] * r[v]);
+    aadd(&a[q[5]], -w[4] * r[v]);
+    aadd(&a[q[6]], -w[5] * r[v]);
+  }
+}
+
+// transpose of k_interp<double, NV> (x2 += P x1): x1b += P^T x2b; the fine
+// adjoint is unchanged (the identity part of the update)
+template <int NV>
+__global__ void k_interp_ad(Lev C, Lev F, double* __restrict__ x1b, const double* __restrict__ x2b) {
+  const int iim = F.iim, jjm = F.jjm;
+  const int s = xcd_slot();
+  if (s >= F.ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const int i = c.i, j = c.j, k = c.k;
+  const int ci = C.iim, cj = C.jjm;
+  auto CS = [&](int ii, int jj) {
+    const int a = (ii + 1) / 2, b = (jj + 1) / 2;
+    const int sl = (a - 1) + ci * ((b - 1) + cj * (k - 1));
+    return (a == 1 || a == ci || b == 1 || b == cj) ? C.src[sl] : sl;
+  };
+  const bool io = (i & 1), jo = (j & 1);
+  const bool pn = i == 1 && j == jjm - 1, ps = i == iim - 1 && j == 1;
+  const bool iA = i >= 3 && i <= iim - 1, iB = i >= 2 && i <= iim - 2;
+  const bool jA = j >= 3 && j <= jjm - 1, jB = j >= 2 && j <= jjm - 2;
+  const bool cA = io && jo && iA && jA;
+  const bool cB = !io && jo && iB && jA;
+  const bool cC = !io && !jo && iB && jB;
+  const bool cD = io && !jo && iA && jB;
+  const bool two = cB || cC || cD;
+  if (!(pn || ps || cA || two)) return;
+  const int di = (cB || cC) ? 1 : 0, dj = (cC || cD) ? 1 : 0;
+  const int c0 = CS(i + di, j + dj);
+  const int c1 = two ? CS(i - di, j - dj) : c0;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double g = x2b[(size_t)v * F.ns + s];
+    if (g == 0.0) continue;
+    double* __restrict__ xc = x1b + (size_t)v * C.ns;
+    if (two) {
+      aadd(&xc[c0], 0.5 * g);
+      aadd(&xc[c1], 0.5 * g);
+    } else {
+      aadd(&xc[c0], g);
+    }
+  }
+}
+
+// transpose of the coarse-level cycle (k_coarse_cycle from a zero start),
+// alpha = M beta at the n slots of its finest level: bb += M^T ab, with
+// mt = M^T stored by rows (row j: the derivatives of alpha with respect to
+// beta(j)); one wave per row, field blockIdx.y
+__global__ void k_coarse_ad(const double* __restrict__ mt, int n, const double* __restrict__ ab,
+                            double* __restrict__ bb) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (bdim() >> 6) + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const double* __restrict__ r = mt + (size_t)row * n;
+  const double* __restrict__ a = ab + (size_t)blockIdx.y * n;
+  double acc = 0.0;
+  for (int s = lane; s < n; s += 64) acc += r[s] * a[s];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0) bb[(size_t)blockIdx.y * n + row] += acc;
+}
+
+// x = unit vector e_j on n values (the coarse cycle's probes)
+__global__ void k_unit(double* __restrict__ x, int n, int j) {
+  const int t = blockIdx.x * bdim() + threadIdx.x;
+  if (t < n) x[t] = t == j ? 1.0 : 0.0;
+}
+
 #undef UPL
 #undef NPL
 #undef DPL

@@ -117,6 +117,7 @@ _sig("swm_init_vorticity", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] *
 _sig("swm_adj_forward", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
 _sig("swm_tl_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 8)
 _sig("swm_adj_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 8)
+_sig("swm_set_linearisation", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_operator_bench", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp])
 _sig("swm_adj_limiter_stats", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 2)
 _sig("swm_wrap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p])
@@ -617,6 +618,12 @@ class Model:
         if init_only:
             return tuple(out[:3])
         return tuple(out[:3]) if control else tuple(out)
+
+    def set_linearisation(self, exact=True):
+        """TL/AD of the window's solves: the forward's own V-cycles (True, the
+        default: the derivative of the model as it runs) or converged solves
+        (False)."""
+        _check(lib.swm_set_linearisation(self._h, 1 if exact else 0))
 
     OPERATOR_BENCH = ("div_of_an_edge_flux", "curl_of_a_vector", "laplacian_of_a_scalar", "face_halo_fill")
 

@@ -61,10 +61,14 @@ def forecast(model_factory, control, nsteps):
     return m.field("h")
 
 
-def finite_difference_check(model_factory, control, gradient, objective, nsteps, ndir, seed, mask, eps=1e-6):
+def finite_difference_check(model_factory, control, gradient, objective, nsteps, ndir, seed, mask, eps=1e-6,
+                            order=2):
     """Central differences of objective(h after nsteps) along ndir seeded
     random directions in control space (each component scaled to its RMS
     over the real cells, mask, eps relative), against <gradient, direction>.
+    order 2: (J(e) - J(-e)) / 2e; order 4: (8 (J(e) - J(-e)) - (J(2e) -
+    J(-2e))) / 12e, whose truncation error is O(e^4), so that a larger eps
+    (less FP64 rounding noise of J divided by e) can be used.
     Returns a list of (fd, adjoint, relative difference)."""
     rng = np.random.default_rng(seed)
     out = []
@@ -73,9 +77,14 @@ def finite_difference_check(model_factory, control, gradient, objective, nsteps,
         for c in control:
             rms = np.sqrt(np.mean(c[mask] ** 2))
             d.append(np.where(mask, rms * rng.standard_normal(c.shape), 0.0))
-        jp = objective(forecast(model_factory, [a + eps * b for a, b in zip(control, d)], nsteps))
-        jm = objective(forecast(model_factory, [a - eps * b for a, b in zip(control, d)], nsteps))
-        fd = (jp - jm) / (2.0 * eps)
+
+        def J(t):
+            return objective(forecast(model_factory, [a + t * b for a, b in zip(control, d)], nsteps))
+
+        if order == 4:
+            fd = (8.0 * (J(eps) - J(-eps)) - (J(2.0 * eps) - J(-2.0 * eps))) / (12.0 * eps)
+        else:
+            fd = (J(eps) - J(-eps)) / (2.0 * eps)
         ad = float(sum(np.sum(g * b) for g, b in zip(gradient, d)))
         out.append((fd, ad, abs(fd - ad) / max(abs(fd), 1e-300)))
     return out

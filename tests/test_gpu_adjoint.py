@@ -3,11 +3,17 @@
 centred, upstream, and flux-corrected transport with the limiter's
 coefficients frozen at the base trajectory.
 
+The linearised model is the derivative of the forward as it runs (each
+step's solve is its V-cycles from the previous solution, the default of
+swm_set_linearisation); the forward here solves to a relative residual of
+1e-13.  The other choice (the solves taken as converged inverses) is
+checked by a dot-product test of its own.
+
 Checks, as the linearised model's own references:
   - dot-product tests <TL x, y> = <x, AD y> over every slot of every state
     field (h, eta, div, u), for the window alone, for the initial state map
     alone (vorticity, divergence, thickness -> state) and for both, to
-    1e-10 relative (the linear solves go to a relative residual of 1e-13);
+    1e-10 relative;
   - the gradient of J = sum over a spherical cap of area * (h - 5000)^2
     after the window, with respect to initial vorticity, divergence and
     thickness, against central finite differences of the nonlinear model
@@ -197,3 +203,45 @@ def test_dot_product_with_active_limiter():
     a = _dot(m.tl_run(*x), y)
     b = _dot(x, m.adj_run(*y))
     assert abs(a - b) <= 1e-10 * abs(a), (a, b)
+
+
+@pytest.mark.parametrize("scheme", ["fct", "centered"])
+def test_converged_solve_linearisation_dot_product(scheme):
+    print(f"case TL/AD dot product, converged-solve linearisation, {scheme}", flush=True)
+    grid = S.Grid(G)
+    m = _model(grid, scheme)
+    m.set_linearisation(False)
+    m.init_vorticity(*_control(grid))
+    m.adj_forward(NSTEPS, EVERY)
+    rng = np.random.default_rng(31)
+    x, y = _state_vectors(grid, rng)
+    a = _dot(m.tl_run(*x), y)
+    b = _dot(x, m.adj_run(*y))
+    assert abs(a - b) <= 1e-10 * abs(a), (a, b)
+
+
+def test_production_forward_gradient_fourth_order():
+    # the reference criterion (2-3 V-cycles per solve) and the exact
+    # linearisation; with 3 fixed cycles per solve the forward is smooth, and
+    # fourth-order differences at eps = 1e-4 check the gradient to 1e-8
+    print("case TL/AD production forward, fixed cycles, fourth-order differences", flush=True)
+    from swm_mi355x import adjoint as A
+    grid = S.Grid(G)
+    area = grid.array("area")
+    mask = grid.array("grid_center_mask") == 1.0
+    cap = A.cap_mask(grid, 0.6, 1.0, 1500e3, S.REF_RADIUS)
+    c = _control(grid)
+
+    def model():
+        mm = S.Model(grid, ntracer=1, dt=DT, grav=S.REF_GRAV, advection="fct")
+        mm.set_fixed_cycles(3, 0)
+        return mm
+
+    m = model()
+    m.init_vorticity(*c)
+    m.adj_forward(NSTEPS, EVERY)
+    hN = m.field("h")
+    grad = m.adj_run(np.where(cap, 2.0 * area * (hN - 5000.0), 0.0), control=True)
+    res = A.finite_difference_check(model, c, grad, lambda hh: A.cap_objective(hh, area, cap, 5000.0), NSTEPS, 4, 17,
+                                    mask, eps=1e-4, order=4)
+    assert all(r[2] <= 1e-8 for r in res), res

@@ -6,19 +6,19 @@ dt = 50 s (the reference's own 200 s at 10242 cells scaled with the grid
 spacing), a 6 h window (432 steps), snapshots every 16 steps with
 recomputation.  Objective J = sum over a spherical cap of radius 1500 km of
 area * (h - 5000)^2 after the window; control: the initial relative
-vorticity, divergence and thickness of the seeded synthetic state.
-  - dot-product test <TL x, y> = <x, AD y> to 1e-10;
+vorticity, divergence and thickness of the seeded synthetic state.  The
+linearised model is that of the forward as it runs (swm_set_linearisation
+exact): each step's solve is its V-cycles from the previous solution.
+  - the production forward (the reference's criterion, 2-3 V-cycles per
+    solve): dot-product test <TL x, y> = <x, AD y> to 1e-10; central
+    differences at eps = 1e-6 relative to 2e-6, the FP64 rounding noise of J
+    divided by 2 eps (larger eps changes the criterion's cycle counts
+    between the two runs);
+  - the forward with exactly 3 V-cycles per solve (a smooth function of the
+    control): fourth-order central differences at eps = 1e-4 along 8
+    seeded directions to 1e-7;
   - the limiter's activity over the window is recorded (the frozen
-    linearisation is the derivative where no coefficient is below 1);
-  - central differences along 8 seeded directions: at eps = 1e-6 relative
-    they carry the FP64 rounding noise of J divided by 2 eps (up to 1.7e-6 in
-    a direction of small derivative, 10 times less at eps = 1e-5), so they
-    are held to 2e-6; at eps = 1e-4, where that noise is 100 times smaller,
-    to 5e-7.  What remains there (up to 4e-7) is systematic: the forward
-    solves stop at the FP64 floor some ten V-cycles from the previous step's
-    solution, so the forward map keeps a dependence on that warm start (the
-    cycle's contraction to the tenth power) that the linearised model's
-    converged solves leave out.
+    linearisation is the derivative where no coefficient is below 1).
 
 Ensemble: 96 members at level 6 (dt 100 s) and 32 at level 7 (dt 50 s, two
 substeps), the mixed step with one fixed correction cycle per solve, in
@@ -45,9 +45,10 @@ def window7():
     nsteps = int(round(6 * 3600.0 / dt))
     grid = S.Grid(g, radius=RADIUS, omega=OMEGA)
 
-    def model():
+    def model(fixed=0):
         m = S.Model(grid, ntracer=1, dt=dt, grav=GRAV, advection="fct")
-        m.set_solver(tol=0.0, relative=True, max_cycles=60)
+        if fixed:
+            m.set_fixed_cycles(fixed, 0)
         return m
 
     syn = S.Model(grid, ntracer=1, dt=dt, grav=GRAV)
@@ -59,10 +60,14 @@ def window7():
     return grid, m, model, control, nsteps
 
 
+def _cap(grid):
+    return A.cap_mask(grid, np.pi / 6, -np.pi / 2, 1500e3, RADIUS)
+
+
 @pytest.mark.timeout(300)
 def test_adjoint_g7_dot_product(window7):
     grid, m, _, control, nsteps = window7
-    print(f"case adjoint g=7 window {nsteps} steps: dot product", flush=True)
+    print(f"case adjoint g=7 window {nsteps} steps, production forward: dot product", flush=True)
     rng = np.random.default_rng(5)
     mask = grid.array("grid_center_mask") == 1.0
     s = (grid.iim, grid.jjm, 10)
@@ -80,24 +85,40 @@ def test_adjoint_g7_dot_product(window7):
     assert edges == 3 * (grid.iim - 2) * (grid.jjm - 2) * 10 * nsteps
 
 
-@pytest.mark.timeout(600)
-def test_adjoint_g7_gradient_against_finite_differences(window7):
+@pytest.mark.timeout(300)
+def test_adjoint_g7_production_forward_central_differences(window7):
     grid, m, model, control, nsteps = window7
-    print(f"case adjoint g=7 window {nsteps} steps: gradient vs central differences", flush=True)
+    print(f"case adjoint g=7 window {nsteps} steps, production forward: central differences", flush=True)
     area = grid.array("area")
     mask = grid.array("grid_center_mask") == 1.0
-    cap = A.cap_mask(grid, np.pi / 6, -np.pi / 2, 1500e3, RADIUS)
+    cap = _cap(grid)
     hN = m.field("h")
     grad = m.adj_run(np.where(cap, 2.0 * area * (hN - 5000.0), 0.0), control=True)
     below, edges = m.limiter_stats()
     assert below == 0, (below, edges)   # the frozen linearisation is the derivative on this window
     obj = lambda h: A.cap_objective(h, area, cap, 5000.0)
-    fine = A.finite_difference_check(model, control, grad, obj, nsteps, 8, 8, mask, eps=1e-6)
-    coarse = A.finite_difference_check(model, control, grad, obj, nsteps, 8, 8, mask, eps=1e-4)
-    print("  eps 1e-6: " + " ".join(f"{r[2]:.1e}" for r in fine), flush=True)
-    print("  eps 1e-4: " + " ".join(f"{r[2]:.1e}" for r in coarse), flush=True)
-    assert all(r[2] <= 2e-6 for r in fine), fine
-    assert all(r[2] <= 5e-7 for r in coarse), coarse
+    fd = A.finite_difference_check(model, control, grad, obj, nsteps, 8, 8, mask, eps=1e-6)
+    print("  eps 1e-6: " + " ".join(f"{r[2]:.1e}" for r in fd), flush=True)
+    assert all(r[2] <= 2e-6 for r in fd), fd
+
+
+@pytest.mark.timeout(600)
+def test_adjoint_g7_fixed_cycles_fourth_order_differences(window7):
+    grid, _, model, control, nsteps = window7
+    print(f"case adjoint g=7 window {nsteps} steps, 3 V-cycles per solve: fourth-order differences", flush=True)
+    area = grid.array("area")
+    mask = grid.array("grid_center_mask") == 1.0
+    cap = _cap(grid)
+    m = model(fixed=3)
+    m.init_vorticity(*control)
+    m.adj_forward(nsteps, 16)
+    hN = m.field("h")
+    grad = m.adj_run(np.where(cap, 2.0 * area * (hN - 5000.0), 0.0), control=True)
+    obj = lambda h: A.cap_objective(h, area, cap, 5000.0)
+    fd = A.finite_difference_check(lambda: model(fixed=3), control, grad, obj, nsteps, 8, 8, mask, eps=1e-4,
+                                   order=4)
+    print("  eps 1e-4, fourth order: " + " ".join(f"{r[2]:.1e}" for r in fd), flush=True)
+    assert all(r[2] <= 1e-7 for r in fd), fd
 
 
 def _group(grid, n, seed0, fixed=1):
