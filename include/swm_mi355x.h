@@ -533,6 +533,18 @@ swm_status swm_op_centered_divergence_curl(swm_ctx* ctx, int npad, int nx, int n
                                            const double* tangent, const double* mass_c, const double* eta,
                                            const double* u, const double* mass, double* flux);
 
+/* global_reductions::global_real_sum (utilities/layer_2/global_reductions.F:
+ * 77-98): sum(x * mask) over n values (mask NULL: sum(x)).  The reference
+ * sums in memory order and all-reduces over ranks; here FP64 block partials
+ * and a compensated fixed-order final sum give the same result on every
+ * call, within about one rounding of the exact sum.  n <= 2^30.
+ * swm_global_real_sum takes host arrays and returns when *out holds the sum;
+ * swm_global_real_sum_device takes device arrays and writes the sum to the
+ * device double *out in order on the context's stream (block partials in
+ * the context's buffers). */
+swm_status swm_global_real_sum(swm_ctx* ctx, const double* x, const double* mask, int64_t n, double* out);
+swm_status swm_global_real_sum_device(swm_ctx* ctx, const double* x, const double* mask, int64_t n, double* out);
+
 /* Halo fill in place, wrap_data::wrap (utilities/layer_2/wrap_data.F:1790-1974):
  * kind 0 = face (cell data), 1 = vrtx (corner data, nspace components first),
  * 2 = edge; nlayer layers of (iim,jjm,10) panels each (face), or

@@ -2192,6 +2192,63 @@ swm_status swm_operator_bench(swm_ctx* C, int op, int reps, double* us_per_launc
   return SWM_OK;
 }
 
+// global_reductions.F:77-98 global_real_sum: sum(X * MASK) over n values,
+// here as FP64 block partials (1024 blocks, strided, then a tree) and a
+// compensated fixed-order sum of the partials, so the result is the same on
+// every call and within about one rounding of the exact sum.
+constexpr int64_t kSumMaxN = (int64_t)1 << 30;   // the partial-sum kernel indexes with int
+constexpr int kSumParts = 1024;
+
+void launch_global_sum(hipStream_t S, const double* x, const double* mask, int n, double* part, double* out) {
+  hipLaunchKernelGGL((swm::dev::k_sum_partial<double, double>), dim3(kSumParts, 1), dim3(BLK), 0, S, x, x, mask, n,
+                     part, (size_t)0);
+  hipLaunchKernelGGL(swm::dev::k_sum_final, dim3(1), dim3(swm::dev::SUM_THREADS), 0, S, (const double*)part, kSumParts,
+                     out, (size_t)0, 1, (const double*)nullptr, 0, (double*)nullptr);
+}
+
+swm_status swm_global_real_sum_device(swm_ctx* C, const double* x, const double* mask, int64_t n, double* out) {
+  if (!C || (n > 0 && !x) || !out || n < 0 || n > kSumMaxN) {
+    swm::set_error("swm_global_real_sum_device: need x (n > 0), out and 0 <= n <= 2^30");
+    return SWM_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(C->device));
+  // the context's partial-sum buffer (its first member block) holds the partials
+  launch_global_sum(C->stream, x, mask, (int)n, C->part, out);
+  HIP_TRY(hipGetLastError());
+  return SWM_OK;
+}
+
+swm_status swm_global_real_sum(swm_ctx* C, const double* x, const double* mask, int64_t n, double* out) {
+  if (!C || (n > 0 && !x) || !out || n < 0 || n > kSumMaxN) {
+    swm::set_error("swm_global_real_sum: need x (n > 0), out and 0 <= n <= 2^30");
+    return SWM_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(C->device));
+  double *dx = nullptr, *dm = nullptr, *dout = nullptr;
+  const size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(double);
+  swm_status st = SWM_OK;
+  if (hipMalloc(&dx, bytes) != hipSuccess || (mask && hipMalloc(&dm, bytes) != hipSuccess) ||
+      hipMalloc(&dout, sizeof(double)) != hipSuccess) {
+    swm::set_error("swm_global_real_sum: device allocation failed");
+    st = SWM_ERR_HIP;
+  }
+  if (!st && n > 0 && (hipMemcpy(dx, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+                       (mask && hipMemcpy(dm, mask, (size_t)n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))) {
+    swm::set_error("swm_global_real_sum: copy to the device failed");
+    st = SWM_ERR_HIP;
+  }
+  if (!st) st = swm_global_real_sum_device(C, dx, dm, n, dout);
+  if (!st && (hipStreamSynchronize(C->stream) != hipSuccess ||
+              hipMemcpy(out, dout, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)) {
+    swm::set_error("swm_global_real_sum: the sum failed on the device");
+    st = SWM_ERR_HIP;
+  }
+  (void)hipFree(dx);
+  (void)hipFree(dm);
+  (void)hipFree(dout);
+  return st;
+}
+
 #define OP_BEGIN(C)                                                              \
   if (!(C)) { swm::set_error("null context"); return SWM_ERR_ARG; }             \
   HIP_TRY(hipSetDevice((C)->device));                                           \

@@ -117,6 +117,9 @@ _sig("swm_init_vorticity", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] *
 _sig("swm_adj_forward", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
 _sig("swm_tl_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 8)
 _sig("swm_adj_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 8)
+_sig("swm_global_real_sum", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, _dp])
+_sig("swm_global_real_sum_device", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 2 + [ctypes.c_int64,
+                                                                                               ctypes.c_void_p])
 _sig("swm_set_linearisation", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_operator_bench", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp])
 _sig("swm_adj_limiter_stats", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 2)
@@ -618,6 +621,17 @@ class Model:
         if init_only:
             return tuple(out[:3])
         return tuple(out[:3]) if control else tuple(out)
+
+    def global_real_sum(self, x, mask=None):
+        """global_real_sum (global_reductions.F:77-98): sum(x * mask) of host
+        arrays, on the device with a compensated final sum."""
+        xf = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1, order="F"))
+        mf = None if mask is None else np.ascontiguousarray(np.asarray(mask, dtype=np.float64).reshape(-1, order="F"))
+        if mf is not None and mf.size != xf.size:
+            raise ValueError("mask and x differ in size")
+        out = ctypes.c_double()
+        _check(lib.swm_global_real_sum(self._h, _ptr(xf), _ptr(mf), xf.size, ctypes.byref(out)))
+        return out.value
 
     def set_linearisation(self, exact=True):
         """TL/AD of the window's solves: the forward's own V-cycles (True, the

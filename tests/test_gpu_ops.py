@@ -215,3 +215,26 @@ def test_operator_bench_leaves_the_state_and_reports_bytes():
     mix.init_synthetic(lmax=8)
     with pytest.raises(S.SwmError):
         mix.operator_bench("face_halo_fill")
+
+
+def test_global_real_sum():
+    # global_reductions.F:77-98 sum(X * MASK); compensated, so within a few
+    # roundings of the exactly rounded sum (math.fsum) whatever the order
+    import math
+    print("case global_real_sum", flush=True)
+    grid = S.Grid(4)
+    m = S.Model(grid, ntracer=1, dt=400.0, grav=S.REF_GRAV)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(200003) * 10.0 ** rng.integers(-8, 8, 200003)
+    mask = (rng.random(200003) < 0.7).astype(np.float64)
+    exact = math.fsum(x * mask)
+    got = m.global_real_sum(x, mask)
+    assert abs(got - exact) <= 4e-16 * np.sum(np.abs(x * mask)), (got, exact)
+    assert m.global_real_sum(x) == pytest.approx(math.fsum(x), rel=0, abs=4e-16 * np.sum(np.abs(x)))
+    # the area of the real cells, as grid_metrics.F sums it
+    area = grid.array("area")
+    cm = grid.array("grid_center_mask")
+    assert abs(m.global_real_sum(area, cm) - grid.array("tot_area")) <= 1e-13 * grid.array("tot_area")
+    assert m.global_real_sum(np.zeros(0)) == 0.0
+    with pytest.raises(ValueError):
+        m.global_real_sum(x, mask[:10])
