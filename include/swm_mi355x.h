@@ -545,6 +545,19 @@ swm_status swm_op_centered_divergence_curl(swm_ctx* ctx, int npad, int nx, int n
 swm_status swm_global_real_sum(swm_ctx* ctx, const double* x, const double* mask, int64_t n, double* out);
 swm_status swm_global_real_sum_device(swm_ctx* ctx, const double* x, const double* mask, int64_t n, double* out);
 
+/* swm_global_real_sum_device without allocation and without a context: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  x, mask and
+ * out are device arrays on GPU `device`; the sum is written to *out in order
+ * on `stream` (a hipStream_t; NULL: the default stream), with the same
+ * kernels, so the same result bit for bit.  The block partials live in a
+ * file-scope device array of 1024 doubles (8 KB per GPU, held until the
+ * process ends), shared by every call on that GPU: calls that overlap in
+ * time on one GPU race.  n <= 2^30 (larger n is refused with SWM_ERR_ARG).
+ * Allocates nothing (no hipMalloc, hipFree or allocating library call). */
+swm_status swm_global_real_sum_noalloc(const double* x, const double* mask, int64_t n, double* out, int device,
+                                       void* stream);
+
 /* Halo fill in place, wrap_data::wrap (utilities/layer_2/wrap_data.F:1790-1974):
  * kind 0 = face (cell data), 1 = vrtx (corner data, nspace components first),
  * 2 = edge; nlayer layers of (iim,jjm,10) panels each (face), or

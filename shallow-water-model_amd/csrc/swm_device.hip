@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2247,6 +2248,65 @@ swm_status swm_global_real_sum(swm_ctx* C, const double* x, const double* mask, 
   (void)hipFree(dm);
   (void)hipFree(dout);
   return st;
+}
+
+// Allocation-free global sum, for callers that cannot allocate device memory
+// (code under graph capture, serving loops).  The block partials go to a
+// file-scope device array of kSumParts doubles (8 KB per GPU, held until the
+// process ends) instead of a context buffer; every partial is written by its
+// block on each call, so nothing needs clearing.  Calls on one GPU share the
+// array: calls that overlap in time on one GPU race.
+static __device__ __align__(256) double scratch[kSumParts];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};   // the array's address on each device
+}  // namespace
+
+swm_status swm_global_real_sum_noalloc(const double* x, const double* mask, int64_t n, double* out, int device,
+                                       void* stream) {
+  if ((n > 0 && !x) || !out || n < 0) {
+    swm::set_error("swm_global_real_sum_noalloc: need x (n > 0), out and n >= 0");
+    return SWM_ERR_ARG;
+  }
+  if (n > kSumMaxN) {
+    swm::set_error("swm_global_real_sum_noalloc: n = " + std::to_string(n) +
+                   " exceeds the maximum of the allocation-free sum (2^30 values)");
+    return SWM_ERR_ARG;
+  }
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev || device >= kMaxDevices) {
+    swm::set_error("swm_global_real_sum_noalloc: no such device");
+    return SWM_ERR_ARG;
+  }
+  int prev = 0;
+  HIP_TRY(hipGetDevice(&prev));
+  HIP_TRY(hipSetDevice(device));
+  double* part = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) {
+        (void)hipSetDevice(prev);
+        swm::set_error(std::string("swm_global_real_sum_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+        return SWM_ERR_HIP;
+      }
+      g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    part = g_scratch_ptr[device];
+  }
+  launch_global_sum(static_cast<hipStream_t>(stream), x, mask, (int)n, part, out);
+  const hipError_t e = hipGetLastError();
+  (void)hipSetDevice(prev);
+  if (e != hipSuccess) {
+    swm::set_error(std::string("swm_global_real_sum_noalloc: launch failed: ") + hipGetErrorString(e));
+    return SWM_ERR_HIP;
+  }
+  return SWM_OK;
 }
 
 #define OP_BEGIN(C)                                                              \

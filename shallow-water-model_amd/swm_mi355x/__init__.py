@@ -120,6 +120,8 @@ _sig("swm_adj_run", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_vo
 _sig("swm_global_real_sum", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, _dp])
 _sig("swm_global_real_sum_device", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 2 + [ctypes.c_int64,
                                                                                                ctypes.c_void_p])
+_sig("swm_global_real_sum_noalloc", ctypes.c_int, [ctypes.c_void_p] * 2 + [ctypes.c_int64, ctypes.c_void_p,
+                                                                              ctypes.c_int, ctypes.c_void_p])
 _sig("swm_set_linearisation", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_operator_bench", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp])
 _sig("swm_adj_limiter_stats", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 2)
