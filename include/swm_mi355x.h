@@ -558,10 +558,40 @@ swm_status swm_global_real_sum_device(swm_ctx* ctx, const double* x, const doubl
 swm_status swm_global_real_sum_noalloc(const double* x, const double* mask, int64_t n, double* out, int device,
                                        void* stream);
 
+/* mass_fct::set_mass_fct (utilities/layer_2/mass_fct.F:23-539) with its
+ * own argument list: the whole sphere of the context's grid (iim, jjm of the
+ * grid, nsdm 10, nlm 1, ntprog 2, nttend 3, the model's fixed counts), host
+ * arrays in the reference layout.  mss (iim,jjm,1,2,10): level n3 read, level
+ * n4 written; msh_f (3,iim,jjm,1,3,10) the AB3 edge-flux history, slot
+ * tend_index(1) written, the slots combined with tend_weights; outputs
+ * mss_low (iim,jjm,1,10) and mss_flux_horz (3,iim,jjm,1,10).  The optional
+ * mss_flux_forced and mask (NULL when absent; step.F passes neither) are
+ * refused.  Runs the step's mass-FCT kernels and halo fills on the device. */
+swm_status swm_op_set_mass_fct(swm_ctx* ctx, int iim, int jjm, int nlm, int nsdm, int ntprog, int nttend,
+                               int ncorners, int ndata, int nedges, int nspace, int npad, int r3, int n3, int n4,
+                               const int* tend_index, const double* tend_weights, double dt, const double* area_inv,
+                               const double* area, const double* dc, const double* normal, double* mss,
+                               const double* mss_c, const double* u, double* mss_low, double* mss_flux_horz,
+                               double* msh_f, const double* mss_flux_forced, const double* mask);
+
+/* tracer_fct::set_tracer_fct (utilities/layer_2/tracer_fct.F:22-670) with its
+ * own argument list, on the whole sphere (iim, jjm of the grid, nsdm 10, nlm
+ * 1, nedges 3): trc, trc_high, mss, mss_low, mss_horz (iim,jjm,1,10) and
+ * mss_flux_horz (3,iim,jjm,1,10) in, trc_fct (iim,jjm,1,10) out; the
+ * optional outputs trc_flux_horz (3,iim,jjm,1,10) and msstrc_low_order
+ * (iim,jjm,1,10) are written when not NULL; the optional mask (step.F does
+ * not pass it) is refused. */
+swm_status swm_op_set_tracer_fct(swm_ctx* ctx, int iim, int jjm, int nlm, int nsdm, int nedges, double dt,
+                                 const double* area_inv, const double* area, const double* trc, const double* trc_high,
+                                 const double* mss, const double* mss_low, const double* mss_horz,
+                                 const double* mss_flux_horz, double* trc_fct, double* trc_flux_horz,
+                                 double* msstrc_low_order, const double* mask);
+
 /* Halo fill in place, wrap_data::wrap (utilities/layer_2/wrap_data.F:1790-1974):
- * kind 0 = face (cell data), 1 = vrtx (corner data, nspace components first),
- * 2 = edge; nlayer layers of (iim,jjm,10) panels each (face), or
- * (nspace,2,iim,jjm) resp. (1,3,iim,jjm) per layer. */
+ * kind 0 = face (cell data), 1 = vrtx (corner data, nspace components first,
+ * 1..4), 2 = edge (nspace 1 or 2 components first); nlayer layers of
+ * (iim,jjm,10) panels each (face), or (nspace,2,iim,jjm) resp.
+ * (nspace,3,iim,jjm) per layer, as the reference's arrays are laid out. */
 swm_status swm_wrap(swm_ctx* ctx, int kind, int nspace, int nlayer, double* x);
 
 /* elliptic_solver_xxx::elliptic (utilities/layer_3/elliptic_solver.F:40-195):

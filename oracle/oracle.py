@@ -48,6 +48,8 @@ def _load():
     lib.orc_budgets.argtypes = [vp, vp]
     lib.orc_set_sum_mode.argtypes = [vp, ctypes.c_int]
     lib.orc_set_threads.argtypes = [ctypes.c_int]
+    lib.orc_set_mass_fct.argtypes = [vp] * 10
+    lib.orc_set_tracer_fct.argtypes = [vp] * 8
     return lib
 
 
@@ -203,6 +205,30 @@ class Oracle:
         r = np.stack([x.reshape(shp, order="F") for x in rf], axis=2)
         x = np.stack([x.reshape(shp, order="F") for x in xf], axis=2)
         return st, r, x
+
+    def set_mass_fct(self, tend_index, tend_weights, mss_n3, mss_c, u, msh_f):
+        """set_mass_fct (mass_fct.F:23-539): msh_f (3, iim, jjm, 3 slots, 10)
+        in the reference layout, updated in its slot tend_index[0]; returns
+        (new mass, low-order mass, total mass flux (3, iim, jjm, 10))."""
+        s = (self.iim, self.jjm, 10)
+        ti = np.ascontiguousarray(tend_index, dtype=np.int32)
+        tw = np.ascontiguousarray(tend_weights, dtype=np.float64)
+        hist = np.ascontiguousarray(np.stack([np.asarray(msh_f)[:, :, :, t, :].reshape(-1, order="F")
+                                              for t in range(3)]))
+        args = [np.asfortranarray(a, dtype=np.float64) for a in (mss_n3, mss_c, u)]
+        low, new = np.zeros(s, order="F"), np.zeros(s, order="F")
+        flux = np.zeros((3,) + s, order="F")
+        lib().orc_set_mass_fct(self._h, _p(ti), _p(tw), *[_p(a) for a in args], _p(hist), _p(low), _p(flux), _p(new))
+        for t in range(3):
+            msh_f[:, :, :, t, :] = hist[t].reshape((3,) + s, order="F")
+        return new, low, flux
+
+    def set_tracer_fct(self, trc, trc_high, mss, mss_low, mss_horz, mflux):
+        """set_tracer_fct (tracer_fct.F:22-670): the flux-corrected tracer."""
+        args = [np.asfortranarray(a, dtype=np.float64) for a in (trc, trc_high, mss, mss_low, mss_horz, mflux)]
+        out = np.zeros((self.iim, self.jjm, 10), order="F")
+        lib().orc_set_tracer_fct(self._h, *[_p(a) for a in args], _p(out))
+        return out
 
     def close(self):
         if self._h:

@@ -633,6 +633,46 @@ static void tracer_fct(orc_model* m, double* trc, const double* trc_high, const 
 }
 #undef E3
 
+/* mass_fct.F:23-539 set_mass_fct under its own arguments: the AB3 slots and
+ * weights of the edge-flux history given (tend_index, 1-based; tend_weights),
+ * the history msh_f as three (3, slots) arrays one after the other (the slot
+ * tend_index[0] is written), outputs the low-order mass, the total mass flux
+ * and the new mass.  The model's own history and weights are left as they
+ * were. */
+void orc_set_mass_fct(orc_model* m, const int* tend_index, const double* tend_weights, const double* mss_n3,
+                      const double* mss_c, const double* u, double* msh_f, double* mss_low, double* mss_flux,
+                      double* mss_n4) {
+  const size_t ne = 3 * m->ns;
+  double* keep[3];
+  const int f3 = m->n3f, f2 = m->n2f, f1 = m->n1f;
+  const double w3 = m->w3, w2 = m->w2, w1 = m->w1;
+  for (int t = 0; t < 3; ++t) {
+    keep[t] = dup(m->msh_f[t], ne);
+    memcpy(m->msh_f[t], msh_f + t * ne, ne * sizeof(double));
+  }
+  m->n3f = tend_index[0]; m->n2f = tend_index[1]; m->n1f = tend_index[2];
+  m->w3 = tend_weights[0]; m->w2 = tend_weights[1]; m->w1 = tend_weights[2];
+  mass_fct(m, mss_n3, mss_c, u, mss_low, mss_flux, mss_n4);
+  for (int t = 0; t < 3; ++t) {
+    memcpy(msh_f + t * ne, m->msh_f[t], ne * sizeof(double));
+    memcpy(m->msh_f[t], keep[t], ne * sizeof(double));
+    free(keep[t]);
+  }
+  m->n3f = f3; m->n2f = f2; m->n1f = f1;
+  m->w3 = w3; m->w2 = w2; m->w1 = w1;
+}
+
+/* tracer_fct.F:22-670 set_tracer_fct under its own arguments (trc and
+ * mss_horz are read as given; their halo fills act on copies) */
+void orc_set_tracer_fct(orc_model* m, const double* trc, const double* trc_high, const double* mss,
+                        const double* mss_low, const double* mss_horz, const double* mflux, double* trc_fct) {
+  double* t = dup(trc, m->ns);
+  double* h = dup(mss_horz, m->ns);
+  tracer_fct(m, t, trc_high, mss, mss_low, h, mflux, trc_fct);
+  free(t);
+  free(h);
+}
+
 /* ------------------------------------------------------------ multigrid -- */
 /* multigrid_solver.F:1105-1158 wrap_simple, closed form for 10 panels */
 static void wrap_simple(int lev, double* x, int nvar) {

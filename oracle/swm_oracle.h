@@ -94,6 +94,14 @@ void orc_vorticity_flux(const orc_model* m, const double* mass_flux, const doubl
                         const double* eta, double* flux);
 void orc_divergence_curl(const orc_model* m, const double* mass_c, const double* eta,
                          const double* u, const double* mass, double* flux);
+/* set_mass_fct (mass_fct.F:23-539) and set_tracer_fct (tracer_fct.F:22-670)
+ * under their own arguments on the whole sphere; msh_f is the edge-flux
+ * history, three (3, slots) arrays one after the other, tend_index 1-based */
+void orc_set_mass_fct(orc_model* m, const int* tend_index, const double* tend_weights, const double* mss_n3,
+                      const double* mss_c, const double* u, double* msh_f, double* mss_low, double* mss_flux,
+                      double* mss_n4);
+void orc_set_tracer_fct(orc_model* m, const double* trc, const double* trc_high, const double* mss,
+                        const double* mss_low, const double* mss_horz, const double* mflux, double* trc_fct);
 /* One multigrid V-cycle and one full elliptic solve on (iim,jjm,nvar,10). */
 void orc_multigrid(orc_model* m, int nvar, const double* beta, double* alpha);
 int orc_elliptic(orc_model* m, int nvar, double* rhs, double* xout);

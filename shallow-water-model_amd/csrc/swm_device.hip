@@ -2511,11 +2511,155 @@ swm_status swm_op_centered_divergence_curl(swm_ctx* C, int npad, int nx, int ny,
   return io.get(flux, y, 3);
 }
 
+// The two flux-corrected transport routines under their own argument lists,
+// on the whole sphere of the context's grid (iim, jjm and nsdm = 10 of the
+// grid, one layer), with the step's kernels.  Levels of a multi-level host
+// array are moved to and from device planes slot by slot.
+namespace {
+// level t of a host array (lead, iim*jjm, nlev, 10) as (lead, ns) in slot order
+std::vector<double> take_level(const double* a, int lead, int pl, int nlev, int t) {
+  std::vector<double> out((size_t)lead * pl * 10);
+  for (int k = 0; k < 10; ++k)
+    for (int q = 0; q < pl; ++q)
+      for (int e = 0; e < lead; ++e)
+        out[(size_t)e + lead * ((size_t)q + (size_t)pl * k)] = a[(size_t)e + lead * ((size_t)q + (size_t)pl * (t + (size_t)nlev * k))];
+  return out;
+}
+void put_level(double* a, const std::vector<double>& v, int lead, int pl, int nlev, int t) {
+  for (int k = 0; k < 10; ++k)
+    for (int q = 0; q < pl; ++q)
+      for (int e = 0; e < lead; ++e)
+        a[(size_t)e + lead * ((size_t)q + (size_t)pl * (t + (size_t)nlev * k))] = v[(size_t)e + lead * ((size_t)q + (size_t)pl * k)];
+}
+}  // namespace
+
+swm_status swm_op_set_mass_fct(swm_ctx* C, int iim, int jjm, int nlm, int nsdm, int ntprog, int nttend, int ncorners,
+                               int ndata, int nedges, int nspace, int npad, int r3, int n3, int n4,
+                               const int* tend_index, const double* tend_weights, double dt, const double* area_inv,
+                               const double* area, const double* dc, const double* normal, double* mss,
+                               const double* mss_c, const double* u, double* mss_low, double* mss_flux_horz,
+                               double* msh_f, const double* mss_flux_forced, const double* mask) {
+  OP_BEGIN(C);
+  OP_ARGS(tend_index, tend_weights, area_inv, area, dc, normal, mss, mss_c, u, mss_low, mss_flux_horz, msh_f);
+  if (mss_flux_forced || mask) {
+    swm::set_error("swm_op_set_mass_fct: the optional mss_flux_forced and mask are not supported "
+                   "(step.F passes neither)");
+    return SWM_ERR_ARG;
+  }
+  bool ti_ok = true;
+  for (int t = 0; t < 3; ++t) ti_ok = ti_ok && tend_index[t] >= 1 && tend_index[t] <= 3;
+  ti_ok = ti_ok && tend_index[0] != tend_index[1] && tend_index[0] != tend_index[2] && tend_index[1] != tend_index[2];
+  if (iim != C->iim || jjm != C->jjm || nsdm != 10 || nlm != 1 || ntprog != 2 || nttend != 3 || ncorners != 2 ||
+      ndata != 3 || nedges != 3 || nspace != 2 || npad != 1 || r3 != 3 || n3 < 1 || n3 > 2 || n4 < 1 || n4 > 2 ||
+      n3 == n4 || !ti_ok) {
+    swm::set_error("swm_op_set_mass_fct: the whole grid (iim, jjm, 10 panels, one layer), the model's fixed counts, "
+                   "two time levels and a permutation of the three tendency slots");
+    return SWM_ERR_DIMS;
+  }
+  const int pl = iim * jjm;
+  OpIO io{C, (size_t)C->ns, {}, {}};
+  swm::dev::Geo G = C->d.geo;   // the grid's extents and halo maps; metrics from the arguments
+  G.area_inv = io.put(area_inv, 1, 1);
+  G.area = io.put(area, 1, 1);
+  G.dc = io.put(dc, 6, 6);
+  G.nrm = io.put(normal, 18, 12, kFrame);
+  const std::vector<double> m3h = take_level(mss, 1, pl, 2, n3 - 1);
+  double* h3 = io.put(m3h.data(), 1, 1);
+  double *hc = io.put(mss_c, 2, 2), *uu = io.put(u, 4, 4, kCornerVec);
+  double* hist[3];
+  for (int t = 0; t < 3; ++t) {
+    const std::vector<double> v = take_level(msh_f, 3, pl, 3, t);
+    hist[t] = io.put(v.data(), 3, 3);
+  }
+  double *flx1 = io.zeros(3), *anti = io.zeros(3), *low = io.zeros(1), *beta = io.zeros(1), *mflux = io.zeros(3),
+         *h4 = io.zeros(1);
+  OP_BUF(G.area_inv && G.area && G.dc && G.nrm && h3 && hc && uu && hist[0] && hist[1] && hist[2] && flx1 && anti &&
+         low && beta && mflux && h4);
+  double *ma = hist[tend_index[0] - 1], *mb = hist[tend_index[1] - 1], *mc = hist[tend_index[2] - 1];
+  const double w3 = tend_weights[0], w2 = tend_weights[1], w1 = tend_weights[2];
+  const unsigned nb = nblk(C->ns);
+  hipStream_t S = C->stream;
+  // mass_fct.F:98-133: upstream and centred fluxes, low order, beta down,
+  // limited antidiffusive flux, flux-corrected mass and total flux
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes<double>, dim3(nb), dim3(BLK), 0, S, G, uu, h3, hc, flx1, ma);
+  gather<double>(C, C->edge, {flx1, ma});
+  hipLaunchKernelGGL(swm::dev::k_mass_low_beta<double>, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, flx1, ma, mb, mc,
+                     h3, anti, low, beta);
+  gather<double>(C, C->face, {beta});
+  hipLaunchKernelGGL(swm::dev::k_mass_anti<double>, dim3(nb), dim3(BLK), 0, S, G, beta, anti);
+  gather<double>(C, C->edge, {anti});
+  hipLaunchKernelGGL(swm::dev::k_mass_final<double>, dim3(nb), dim3(BLK), 0, S, G, dt, flx1, anti, low, mflux, h4,
+                     swm::dev::TrFluxArgs<double>{});
+  swm_status st;
+  std::vector<double> v((size_t)C->ns);
+  if ((st = io.get(v.data(), h4, 1))) return st;
+  put_level(mss, v, 1, pl, 2, n4 - 1);
+  if ((st = io.get(mss_low, low, 1))) return st;
+  if ((st = io.get(mss_flux_horz, mflux, 3))) return st;
+  v.assign(3 * (size_t)C->ns, 0.0);
+  if ((st = io.get(v.data(), ma, 3))) return st;
+  put_level(msh_f, v, 3, pl, 3, tend_index[0] - 1);
+  return SWM_OK;
+}
+
+swm_status swm_op_set_tracer_fct(swm_ctx* C, int iim, int jjm, int nlm, int nsdm, int nedges, double dt,
+                                 const double* area_inv, const double* area, const double* trc, const double* trc_high,
+                                 const double* mss, const double* mss_low, const double* mss_horz,
+                                 const double* mss_flux_horz, double* trc_fct, double* trc_flux_horz,
+                                 double* msstrc_low_order, const double* mask) {
+  OP_BEGIN(C);
+  OP_ARGS(area_inv, area, trc, trc_high, mss, mss_low, mss_horz, mss_flux_horz, trc_fct);
+  if (mask) {
+    swm::set_error("swm_op_set_tracer_fct: the optional mask is not supported (step.F does not pass it)");
+    return SWM_ERR_ARG;
+  }
+  if (iim != C->iim || jjm != C->jjm || nsdm != 10 || nlm != 1 || nedges != 3) {
+    swm::set_error("swm_op_set_tracer_fct: the whole grid (iim, jjm, 10 panels, one layer) and 3 edges");
+    return SWM_ERR_DIMS;
+  }
+  OpIO io{C, (size_t)C->ns, {}, {}};
+  swm::dev::Geo G = C->d.geo;
+  G.area_inv = io.put(area_inv, 1, 1);
+  G.area = io.put(area, 1, 1);
+  double *t3 = io.put(trc, 1, 1), *th = io.put(trc_high, 1, 1), *h3 = io.put(mss, 1, 1), *low = io.put(mss_low, 1, 1),
+         *h4 = io.put(mss_horz, 1, 1), *mflux = io.put(mss_flux_horz, 3, 3);
+  double *f1 = io.zeros(3), *f2 = io.zeros(3), *anti = io.zeros(3), *ml = io.zeros(1), *tlow = io.zeros(1),
+         *bdn = io.zeros(1), *bup = io.zeros(1), *t4 = io.zeros(1), *told = io.zeros(1);
+  OP_BUF(G.area_inv && G.area && t3 && th && h3 && low && h4 && mflux && f1 && f2 && anti && ml && tlow && bdn && bup &&
+         t4 && told);
+  const unsigned nb = nblk(C->ns);
+  hipStream_t S = C->stream;
+  // tracer_fct.F:98-159: upstream and centred tracer fluxes (the centred one
+  // of trc_high: weights 0, 1, 0 select it exactly), low order, the limiter
+  // after the halo fills of trc and mss_horz, beta up and down, the limited
+  // antidiffusive flux, the flux-corrected tracer
+  hipLaunchKernelGGL(swm::dev::k_tr_fluxes<double>, dim3(nb), dim3(BLK), 0, S, G, 0.0, 1.0, 0.0, t3, th, t3, mflux, f1,
+                     f2);
+  gather<double>(C, C->edge, {f1, f2});
+  gather<double>(C, C->face, {t3, h4});
+  hipLaunchKernelGGL(swm::dev::k_tr_low<double>, dim3(nb), dim3(BLK), 0, S, G, dt, h3, t3, f1, h4, ml, tlow);
+  gather<double>(C, C->face, {ml, tlow});
+  hipLaunchKernelGGL(swm::dev::k_tr_limiter<double>, dim3(nb), dim3(BLK), 0, S, G, dt, t3, ml, tlow, low, f1, f2, anti,
+                     bdn, bup);
+  gather<double>(C, C->face, {bdn, bup});
+  hipLaunchKernelGGL(swm::dev::k_tr_anti<double>, dim3(nb), dim3(BLK), 0, S, G, bdn, bup, anti);
+  gather<double>(C, C->edge, {anti});
+  hipLaunchKernelGGL(swm::dev::k_tr_final<double>, dim3(nb), dim3(BLK), 0, S, G, dt, ml, anti, h4, t4, told);
+  swm_status st;
+  if ((st = io.get(trc_fct, t4, 1))) return st;
+  if (msstrc_low_order && (st = io.get(msstrc_low_order, ml, 1))) return st;
+  if (trc_flux_horz) {
+    hipLaunchKernelGGL(swm::dev::k_axpy, dim3(nblk(3 * (size_t)C->ns)), dim3(BLK), 0, S, 3 * (size_t)C->ns, 1.0, anti, f1);
+    if ((st = io.get(trc_flux_horz, f1, 3))) return st;
+  }
+  return SWM_OK;
+}
+
 swm_status swm_wrap(swm_ctx* C, int kind, int nspace, int nlayer, double* x) {
   OP_BEGIN(C);
   OpIO io{C, (size_t)C->ns, {}, {}};
-  if (!x || kind < 0 || kind > 2 || nlayer < 1 || nspace < 1 || nspace > 4 || (kind != 1 && nspace != 1)) {
-    swm::set_error("swm_wrap: kind must be 0 (face), 1 (vrtx) or 2 (edge); nspace > 1 only for vrtx");
+  if (!x || kind < 0 || kind > 2 || nlayer < 1 || nspace < 1 || nspace > (kind == 0 ? 1 : (kind == 1 ? 4 : 2))) {
+    swm::set_error("swm_wrap: kind must be 0 (face), 1 (vrtx, nspace 1..4) or 2 (edge, nspace 1..2)");
     return SWM_ERR_ARG;
   }
   const int ncomp = kind == 0 ? 1 : (kind == 1 ? 2 : 3);

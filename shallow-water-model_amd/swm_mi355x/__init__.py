@@ -122,6 +122,10 @@ _sig("swm_global_real_sum_device", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_v
                                                                                                ctypes.c_void_p])
 _sig("swm_global_real_sum_noalloc", ctypes.c_int, [ctypes.c_void_p] * 2 + [ctypes.c_int64, ctypes.c_void_p,
                                                                               ctypes.c_int, ctypes.c_void_p])
+_sig("swm_op_set_mass_fct", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 14 + [ctypes.c_void_p] * 2 +
+     [ctypes.c_double] + [ctypes.c_void_p] * 12)
+_sig("swm_op_set_tracer_fct", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 5 + [ctypes.c_double] +
+     [ctypes.c_void_p] * 12)
 _sig("swm_set_linearisation", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_operator_bench", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp])
 _sig("swm_adj_limiter_stats", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 2)
@@ -487,6 +491,39 @@ class Model:
         _check(getattr(lib, "swm_op_" + name)(self._h, *[vals[k] for k in ints], *[_ptr(a) for a in arrs],
                                               _ptr(out), *extra))
         return out
+
+    def set_mass_fct(self, n3, n4, tend_index, tend_weights, area_inv, area, dc, normal, mss, mss_c, u, msh_f):
+        """mass_fct::set_mass_fct (mass_fct.F:23-539) on host arrays in the
+        reference layout, whole sphere: mss (iim,jjm,1,2,10) and msh_f
+        (3,iim,jjm,1,3,10) are updated in place; returns (mss_low, mss_flux_horz)."""
+        s = (self.iim, self.jjm)
+        for a, shp in ((mss, s + (1, 2, 10)), (msh_f, (3,) + s + (1, 3, 10))):
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.f_contiguous and a.shape == shp):
+                raise TypeError(f"expected a Fortran-ordered float64 array of shape {shp}")
+        ins = [_f64(area_inv, s + (10,)), _f64(area, s + (10,)), _f64(dc, (3, 2) + s + (10,)),
+               _f64(normal, (3, 3, 2) + s + (10,))]
+        mc, uu = _f64(mss_c, (2,) + s + (1, 10)), _f64(u, (2, 2) + s + (1, 10))
+        ti = np.ascontiguousarray(tend_index, dtype=np.int32)
+        tw = np.ascontiguousarray(tend_weights, dtype=np.float64)
+        low = np.zeros(s + (1, 10), order="F")
+        flux = np.zeros((3,) + s + (1, 10), order="F")
+        _check(lib.swm_op_set_mass_fct(self._h, self.iim, self.jjm, 1, 10, 2, 3, 2, 3, 3, 2, 1, 3, n3, n4, _ptr(ti),
+                                       _ptr(tw), self.dt, *[_ptr(a) for a in ins], _ptr(mss), _ptr(mc), _ptr(uu),
+                                       _ptr(low), _ptr(flux), _ptr(msh_f), None, None))
+        return low, flux
+
+    def set_tracer_fct(self, area_inv, area, trc, trc_high, mss, mss_low, mss_horz, mss_flux_horz):
+        """tracer_fct::set_tracer_fct (tracer_fct.F:22-670) on host arrays,
+        whole sphere; returns (trc_fct, trc_flux_horz, msstrc_low_order)."""
+        s = (self.iim, self.jjm)
+        ins = [_f64(a, s + (1, 10)) for a in (trc, trc_high, mss, mss_low, mss_horz)]
+        fl = _f64(mss_flux_horz, (3,) + s + (1, 10))
+        out, tf, ml = np.zeros(s + (1, 10), order="F"), np.zeros((3,) + s + (1, 10), order="F"), \
+            np.zeros(s + (1, 10), order="F")
+        _check(lib.swm_op_set_tracer_fct(self._h, self.iim, self.jjm, 1, 10, 3, self.dt, _ptr(_f64(area_inv, s + (10,))),
+                                         _ptr(_f64(area, s + (10,))), *[_ptr(a) for a in ins], _ptr(fl), _ptr(out),
+                                         _ptr(tf), _ptr(ml), None))
+        return out, tf, ml
 
     def wrap(self, kind, x, nspace=1):
         """Halo fill (wrap_data::wrap) of x in place; kind "face" x is

@@ -238,3 +238,43 @@ def test_global_real_sum():
     assert m.global_real_sum(np.zeros(0)) == 0.0
     with pytest.raises(ValueError):
         m.global_real_sum(x, mask[:10])
+
+
+def test_fct_entry_points_match_oracle():
+    # set_mass_fct / set_tracer_fct under their own argument lists, whole
+    # sphere, against the oracle's (pinned to the compiled reference through
+    # the model trajectories); an AB3 history in permuted slots
+    from swm_mi355x import ics
+    print("case set_mass_fct / set_tracer_fct", flush=True)
+    g, dt = 4, 900.0
+    grid = S.Grid(g)
+    n = grid.iim
+    st = ics.synthetic_state(grid, ntracer=1, lmax=8)
+    m = S.Model(grid, ntracer=1, dt=dt, grav=S.REF_GRAV)
+    orc = Oracle(grid_arrays_from(grid), 1, dt, S.REF_GRAV)
+    rng = np.random.default_rng(5)
+    h = st["h"]
+    hc = orc.op("orc_average_to_corners", h, out_shape=(2, n, n, 10))
+    u = st["u"]
+    hist = np.asfortranarray(rng.standard_normal((3, n, n, 3, 10)) * 1e9)
+    ti, tw = [2, 3, 1], [23.0 / 12.0, -16.0 / 12.0, 5.0 / 12.0]
+    mss = np.zeros((n, n, 1, 2, 10), order="F")
+    mss[:, :, 0, 0, :] = h
+    msh = np.zeros((3, n, n, 1, 3, 10), order="F")
+    msh[:, :, :, 0, :, :] = hist
+    args = [grid.array(k) for k in ("area_inv", "area", "dc", "normal")]
+    low, flux = m.set_mass_fct(1, 2, ti, tw, *args, mss, hc[:, :, :, None, :], u[:, :, :, :, None, :], msh)
+    hist_o = hist.copy(order="F")
+    new_o, low_o, flux_o = orc.set_mass_fct(ti, tw, h, hc, u, hist_o)
+    assert np.array_equal(low[:, :, 0, :], low_o)
+    assert np.array_equal(flux[:, :, :, 0, :], flux_o)
+    assert np.array_equal(mss[:, :, 0, 1, :], new_o)
+    assert np.array_equal(msh[:, :, :, 0, :, :], hist_o)
+    assert np.array_equal(mss[:, :, 0, 0, :], h)   # level n3 untouched
+    trc = st["tracer"][:, :, 0, :]
+    trh = trc + 0.01 * rng.standard_normal(trc.shape)
+    tf, tflux, tml = m.set_tracer_fct(grid.array("area_inv"), grid.array("area"), trc[:, :, None, :],
+                                      trh[:, :, None, :], h[:, :, None, :], low, mss[:, :, :, 1, :], flux)
+    assert np.array_equal(tf[:, :, 0, :], orc.set_tracer_fct(trc, trh, h, low_o, new_o, flux_o))
+    with pytest.raises(S.SwmError):   # the whole sphere only
+        S._check(S.lib.swm_op_set_tracer_fct(m._h, n - 1, n, 1, 10, 3, dt, *([None] * 12)))
