@@ -12,6 +12,11 @@
 !                             wrap (utilities/layer_2/wrap_data.F:1790-1974)
 !   module operators_mi355x the operators of utilities/layer_2/operators.F
 !                           under their own names and argument lists
+!   module mass_fct_mi355x, tracer_fct_mi355x, global_reductions_mi355x
+!                           set_mass_fct (utilities/layer_2/mass_fct.F:23),
+!                           set_tracer_fct (utilities/layer_2/tracer_fct.F:
+!                           22), global_real_sum (utilities/layer_2/
+!                           global_reductions.F:77-98)
 !
 ! A non-zero status from the library stops the program, as the reference's
 ! abort_code does (utilities/layer_1/abort.F:26-49).
@@ -165,6 +170,40 @@ module swm_mi355x_c
       real(c_double) :: beta(*), alpha(*)
       integer(c_int) :: st
     end function
+    function swm_global_real_sum(ctx, x, mask, n, out) bind(C, name="swm_global_real_sum") result(st)
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      real(c_double) :: x(*), mask(*)
+      integer(c_int64_t), value :: n
+      real(c_double) :: out
+      integer(c_int) :: st
+    end function
+    function swm_op_set_mass_fct(ctx, iim, jjm, nlm, nsdm, ntprog, nttend, ncorners, ndata, nedges, nspace, &
+        npad, r3, n3, n4, tend_index, tend_weights, dt, area_inv, area, dc, normal, mss, mss_c, u, mss_low, &
+        mss_flux_horz, msh_f, mss_flux_forced, mask) bind(C, name="swm_op_set_mass_fct") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: iim, jjm, nlm, nsdm, ntprog, nttend, ncorners, ndata, nedges, nspace, npad, r3, n3, n4
+      integer(c_int) :: tend_index(*)
+      real(c_double) :: tend_weights(*)
+      real(c_double), value :: dt
+      real(c_double) :: area_inv(*), area(*), dc(*), normal(*), mss(*), mss_c(*), u(*), mss_low(*), &
+                        mss_flux_horz(*), msh_f(*)
+      type(c_ptr), value :: mss_flux_forced, mask
+      integer(c_int) :: st
+    end function
+    function swm_op_set_tracer_fct(ctx, iim, jjm, nlm, nsdm, nedges, dt, area_inv, area, trc, trc_high, mss, &
+        mss_low, mss_horz, mss_flux_horz, trc_fct, trc_flux_horz, msstrc_low_order, mask) &
+        bind(C, name="swm_op_set_tracer_fct") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: iim, jjm, nlm, nsdm, nedges
+      real(c_double), value :: dt
+      real(c_double) :: area_inv(*), area(*), trc(*), trc_high(*), mss(*), mss_low(*), mss_horz(*), &
+                        mss_flux_horz(*), trc_fct(*)
+      type(c_ptr), value :: trc_flux_horz, msstrc_low_order, mask
+      integer(c_int) :: st
+    end function
     ! ---- operators.F, argument lists of the reference plus nsdm ----
     function swm_op_curl_of_a_vector(ctx, npad, nx, ny, nsdm, ndata, nspace, ncorners, area_inv, &
         v_weights, xin, xout) bind(C, name="swm_op_curl_of_a_vector") result(st)
@@ -281,8 +320,15 @@ module swm_mi355x
   implicit none
   private
   public :: swm_gpu_initialize, swm_gpu_finalize, swm_gpu_check, swm_gpu_context, swm_gpu_grid_array, &
-            step_dt, wrap_prognostics, sync_prognostics_to_host, elliptic, multigrid, wrap_face, &
-            wrap_vrtx, wrap_edge, write_swm_restart, read_swm_restart, swm_gpu_time
+            step_dt, wrap_prognostics, sync_prognostics_to_host, elliptic, multigrid, wrap, wrap_face, &
+            wrap_vrtx, wrap_edge, write_swm_restart, read_swm_restart, swm_gpu_time, initialize_wrap, &
+            initialize_multigrid
+
+  ! elliptic_solver.F:30 and multigrid_solver.F:54: the Helmholtz constant of
+  ! del^2 x - helm x = rhs; the device solver is the Poisson solver the
+  ! shallow-water step uses (0), so elliptic and multigrid stop on any other
+  real(c_double), public, save :: helm_const = 0.0_c_double
+  real(c_double), public, save :: helm_const_mg = 0.0_c_double
 
   type(c_ptr), save :: ctx = c_null_ptr
   type(c_ptr), save :: grid = c_null_ptr
@@ -413,11 +459,33 @@ contains
     t = swm_time_seconds(ctx)
   end function swm_gpu_time
 
+  ! wrap_data::initialize_wrap (wrap_data.F:34-157) and multigrid_solver_xxx::
+  ! initialize_multigrid (multigrid_solver.F:61-270): the halo tables and the
+  ! multigrid hierarchy are built by swm_gpu_initialize, with the grid; these
+  ! only check that it ran
+  subroutine initialize_wrap(component_name)
+    character(len=*), intent(in) :: component_name
+    if (.not. c_associated(ctx)) call swm_gpu_check(7_c_int, 'initialize_wrap '//component_name)
+  end subroutine initialize_wrap
+  subroutine initialize_multigrid(component_name)
+    character(len=*), intent(in) :: component_name
+    if (.not. c_associated(ctx)) call swm_gpu_check(7_c_int, 'initialize_multigrid '//component_name)
+  end subroutine initialize_multigrid
+
+  subroutine check_poisson(where)
+    character(len=*), intent(in) :: where
+    if (helm_const /= 0.0_c_double .or. helm_const_mg /= 0.0_c_double) then
+      write(*, '(a,a)') 'swm_mi355x: ', where//': helm_const must be 0 (the device solver is the Poisson solver)'
+      stop 1
+    end if
+  end subroutine check_poisson
+
   ! elliptic_solver_xxx::elliptic (component_name is not needed)
   subroutine elliptic(component_name, nvar, rhs, xout)
     character(len=*), intent(in) :: component_name
     integer, intent(in) :: nvar
     real(c_double), intent(inout) :: rhs(:,:,:,:), xout(:,:,:,:)
+    call check_poisson('elliptic')
     call swm_gpu_check(swm_elliptic(ctx, int(nvar, c_int), rhs, xout), 'elliptic '//component_name)
   end subroutine elliptic
 
@@ -426,8 +494,46 @@ contains
     integer, intent(in) :: nvar
     real(c_double), intent(in) :: beta(:,:,:,:)
     real(c_double), intent(inout) :: alpha(:,:,:,:)
+    call check_poisson('multigrid')
     call swm_gpu_check(swm_multigrid(ctx, int(nvar, c_int), beta, alpha), 'multigrid')
   end subroutine multigrid
+
+  ! wrap_data::wrap (wrap_data.F:1790-1974) with the reference's optional
+  ! keyword arguments: every array present is halo-filled in place; with
+  ! l_HA_stencil, the two edges next to the pentagon of edge_scalar are
+  ! zeroed afterwards (every panel is next to one with 10 panels)
+  subroutine wrap(component_name, l_HA_stencil, face, face_1L, vrtx, vrtx_1L, vrtx_scalar, vrtx_scalar_1L, &
+                  edge, edge_1L, edge_scalar, edge_scalar_1L)
+    character(len=*), intent(in) :: component_name
+    logical, intent(in), optional :: l_HA_stencil
+    real(c_double), optional, contiguous :: face(:,:,:,:), face_1L(:,:,:), vrtx(:,:,:,:,:,:), vrtx_1L(:,:,:,:,:), &
+                                            vrtx_scalar(:,:,:,:,:), vrtx_scalar_1L(:,:,:,:), edge(:,:,:,:,:,:), &
+                                            edge_1L(:,:,:,:,:), edge_scalar(:,:,:,:,:), edge_scalar_1L(:,:,:,:)
+    character(len=64) :: w
+    w = 'wrap '//component_name
+    if (present(face)) call swm_gpu_check(swm_wrap(ctx, 0_c_int, 1_c_int, int(size(face, 3), c_int), face), w)
+    if (present(face_1L)) call swm_gpu_check(swm_wrap(ctx, 0_c_int, 1_c_int, 1_c_int, face_1L), w)
+    if (present(vrtx)) call swm_gpu_check(swm_wrap(ctx, 1_c_int, int(size(vrtx, 1), c_int), &
+                                                   int(size(vrtx, 5), c_int), vrtx), w)
+    if (present(vrtx_1L)) call swm_gpu_check(swm_wrap(ctx, 1_c_int, int(size(vrtx_1L, 1), c_int), 1_c_int, &
+                                                      vrtx_1L), w)
+    if (present(vrtx_scalar)) call swm_gpu_check(swm_wrap(ctx, 1_c_int, 1_c_int, &
+                                                          int(size(vrtx_scalar, 4), c_int), vrtx_scalar), w)
+    if (present(vrtx_scalar_1L)) call swm_gpu_check(swm_wrap(ctx, 1_c_int, 1_c_int, 1_c_int, vrtx_scalar_1L), w)
+    if (present(edge)) call swm_gpu_check(swm_wrap(ctx, 2_c_int, int(size(edge, 1), c_int), &
+                                                   int(size(edge, 5), c_int), edge), w)
+    if (present(edge_1L)) call swm_gpu_check(swm_wrap(ctx, 2_c_int, int(size(edge_1L, 1), c_int), 1_c_int, &
+                                                      edge_1L), w)
+    if (present(edge_scalar)) call swm_gpu_check(swm_wrap(ctx, 2_c_int, 1_c_int, &
+                                                          int(size(edge_scalar, 4), c_int), edge_scalar), w)
+    if (present(edge_scalar_1L)) call swm_gpu_check(swm_wrap(ctx, 2_c_int, 1_c_int, 1_c_int, edge_scalar_1L), w)
+    if (present(edge_scalar) .and. present(l_HA_stencil)) then
+      if (l_HA_stencil) then
+        edge_scalar(1, iim_, jjm_ - 1, :, :) = 0.0_c_double
+        edge_scalar(3, iim_ - 1, jjm_, :, :) = 0.0_c_double
+      end if
+    end if
+  end subroutine wrap
 
   ! wrap_data::wrap, one keyword at a time: face (iim,jjm,nlayer,10)
   subroutine wrap_face(x, nlayer)
@@ -581,3 +687,99 @@ contains
                        nedges, nspace, r3, dc, tangent, mass_c, eta, u, mass, flux), 'centered_divergence_curl')
   end subroutine
 end module operators_mi355x
+
+!=======================================================================
+! global_reductions.F:77-98 global_real_sum under its own name and
+! arguments: sum(X * MASK), on the device with a compensated final sum
+module global_reductions_mi355x
+  use iso_c_binding
+  use swm_mi355x_c
+  use swm_mi355x, only: swm_gpu_check, swm_gpu_context
+  implicit none
+  private
+  public :: global_real_sum
+contains
+  real(c_double) function global_real_sum(X, MASK, size, component_name)
+    integer, intent(in) :: size
+    character(len=*), intent(in) :: component_name
+    real(c_double), intent(in) :: X(size), MASK(size)
+    real(c_double) :: out
+    call swm_gpu_check(swm_global_real_sum(swm_gpu_context(), X, MASK, int(size, c_int64_t), out), &
+                       'global_real_sum '//component_name)
+    global_real_sum = out
+  end function global_real_sum
+end module global_reductions_mi355x
+
+!=======================================================================
+! mass_fct.F:23-539 set_mass_fct under its own name and arguments (the whole
+! sphere of the run's grid; the optional mss_flux_forced and mask, which
+! step.F never passes, stop the program)
+module mass_fct_mi355x
+  use iso_c_binding
+  use swm_mi355x_c
+  use swm_mi355x, only: swm_gpu_check, swm_gpu_context
+  implicit none
+  private
+  public :: set_mass_fct
+contains
+  subroutine set_mass_fct(component_name, iim, jjm, nlm, nsdm, ntprog, nttend, ncorners, ndata, nedges, nspace, &
+                          npad, r3, n3, n4, tend_index, tend_weights, dt, area_inv, area, dc, normal, mss, mss_c, u, &
+                          mss_low, mss_flux_horz, msh_f, mss_flux_forced, mask)
+    character(len=*), intent(in) :: component_name
+    integer, intent(in) :: iim, jjm, nlm, nsdm, ntprog, nttend, ncorners, ndata, nedges, nspace, npad, r3, n3, n4
+    integer, intent(in) :: tend_index(nttend)
+    real(c_double), intent(in) :: dt, tend_weights(nttend)
+    real(c_double), intent(in) :: area_inv(iim,jjm,nsdm), area(iim,jjm,nsdm), &
+                                  normal(ndata,r3,ncorners,iim,jjm,nsdm), dc(ndata,ncorners,iim,jjm,nsdm)
+    real(c_double), intent(inout) :: mss(iim,jjm,nlm,ntprog,nsdm)
+    real(c_double), intent(in) :: mss_c(ncorners,iim,jjm,nlm,nsdm), u(nspace,ncorners,iim,jjm,nlm,nsdm)
+    real(c_double), intent(out) :: mss_low(iim,jjm,nlm,nsdm), mss_flux_horz(nedges,iim,jjm,nlm,nsdm)
+    real(c_double), intent(inout) :: msh_f(nedges,iim,jjm,nlm,nttend,nsdm)
+    real(c_double), intent(in), optional :: mss_flux_forced(nedges,iim,jjm,nlm,nsdm), mask(nedges,iim,jjm,nlm,nsdm)
+    integer(c_int) :: ti(3)
+    if (present(mss_flux_forced) .or. present(mask)) &
+      call swm_gpu_check(1_c_int, 'set_mass_fct '//component_name//' (mss_flux_forced, mask)')
+    ti = int(tend_index(1:3), c_int)
+    call swm_gpu_check(swm_op_set_mass_fct(swm_gpu_context(), iim, jjm, nlm, nsdm, ntprog, nttend, ncorners, &
+                       ndata, nedges, nspace, npad, r3, n3, n4, ti, tend_weights, dt, area_inv, area, dc, normal, &
+                       mss, mss_c, u, mss_low, mss_flux_horz, msh_f, c_null_ptr, c_null_ptr), &
+                       'set_mass_fct '//component_name)
+  end subroutine set_mass_fct
+end module mass_fct_mi355x
+
+!=======================================================================
+! tracer_fct.F:22-670 set_tracer_fct under its own name and arguments (the
+! whole sphere; the optional mask, which step.F never passes, stops the
+! program)
+module tracer_fct_mi355x
+  use iso_c_binding
+  use swm_mi355x_c
+  use swm_mi355x, only: swm_gpu_check, swm_gpu_context
+  implicit none
+  private
+  public :: set_tracer_fct
+contains
+  subroutine set_tracer_fct(component_name, iim, jjm, nlm, nsdm, nedges, dt, area_inv, area, trc, trc_high, mss, &
+                            mss_low, mss_horz, mss_flux_horz, trc_fct, trc_flux_horz, msstrc_low_order, mask)
+    character(len=*), intent(in) :: component_name
+    integer, intent(in) :: iim, jjm, nlm, nsdm, nedges
+    real(c_double), intent(in) :: dt
+    real(c_double), intent(in) :: area_inv(iim,jjm,nsdm), area(iim,jjm,nsdm)
+    real(c_double), intent(in) :: trc(iim,jjm,nlm,nsdm), trc_high(iim,jjm,nlm,nsdm), mss(iim,jjm,nlm,nsdm), &
+                                  mss_low(iim,jjm,nlm,nsdm), mss_horz(iim,jjm,nlm,nsdm), &
+                                  mss_flux_horz(nedges,iim,jjm,nlm,nsdm)
+    real(c_double), intent(out) :: trc_fct(iim,jjm,nlm,nsdm)
+    real(c_double), intent(out), optional, target :: trc_flux_horz(nedges,iim,jjm,nlm,nsdm), &
+                                                     msstrc_low_order(iim,jjm,nlm,nsdm)
+    real(c_double), intent(in), optional :: mask(nedges,iim,jjm,nlm,nsdm)
+    type(c_ptr) :: pf, pl
+    if (present(mask)) call swm_gpu_check(1_c_int, 'set_tracer_fct '//component_name//' (mask)')
+    pf = c_null_ptr
+    pl = c_null_ptr
+    if (present(trc_flux_horz)) pf = c_loc(trc_flux_horz)
+    if (present(msstrc_low_order)) pl = c_loc(msstrc_low_order)
+    call swm_gpu_check(swm_op_set_tracer_fct(swm_gpu_context(), iim, jjm, nlm, nsdm, nedges, dt, area_inv, area, &
+                       trc, trc_high, mss, mss_low, mss_horz, mss_flux_horz, trc_fct, pf, pl, c_null_ptr), &
+                       'set_tracer_fct '//component_name)
+  end subroutine set_tracer_fct
+end module tracer_fct_mi355x
