@@ -113,6 +113,12 @@ typedef struct swm_config {
   int advection;     /* SWM_ADV_FCT (0, the default of a zeroed struct), SWM_ADV_CENTERED, SWM_ADV_UPWIND */
 } swm_config;
 
+/* Checks, once per device, that the kernels read their workgroup size from
+ * the implicit kernel arguments where the code-object v5 layout puts it
+ * (the build pins version 5); swm_create runs it.  SWM_ERR_HIP if not.  No
+ * counterpart in the reference. */
+swm_status swm_self_test(int device);
+
 /* Allocates the device state for a grid (the grid may be destroyed after). */
 swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out);
 void swm_destroy(swm_ctx* ctx);
@@ -129,7 +135,9 @@ void swm_destroy(swm_ctx* ctx);
  * the context (the next step is a forward Euler step for every member).  In
  * the mixed path the members share the thickness offset of the member
  * initialised first, and in both paths the surface height.  The linearised
- * model is for single-member contexts.
+ * model is for single-member contexts.  When a member is initialised again
+ * after steps, the others continue from their latest state: their next step
+ * is a forward Euler step from it, as after swm_restart_clock.
  * ------------------------------------------------------------------------ */
 swm_status swm_create_members(const swm_grid* grid, const swm_config* cfg, int nmem, swm_ctx** out);
 int swm_members(const swm_ctx* ctx);
@@ -218,8 +226,28 @@ swm_status swm_read_namelist(const char* path, swm_namelist* out);
 /* Advances nsteps time steps: for each, the time-level pointers and ABM3
  * weights (swm/source/model_control.F:28-89), step_dt and wrap_prognostics
  * (swm/source/step.F:114-212, :438-493; loop of swm/source/driver.F:90-130).
- * Returns SWM_ERR_NOT_CONVERGED if an elliptic solve fails. */
+ * Returns SWM_ERR_NOT_CONVERGED if an elliptic solve fails, SWM_ERR_NAN if
+ * its residual norms (read back by every solve of the convergence test) are
+ * not finite: the state holds NaN or Inf. */
 swm_status swm_step(swm_ctx* ctx, int nsteps);
+
+/* Halo shadow check (debugging): while enabled, every halo fill of the step
+ * (wrap_data.F:1790-2607 on the device) is checked against the same fill
+ * done on the host from the host copy of the tables: the fields of every
+ * member are copied to the host before and after the device fill and
+ * compared bit for bit.  A step with a mismatch returns SWM_ERR_STATE;
+ * swm_halo_shadow_stats gives the fields checked and the values that
+ * differed since the context was created.  Synchronises at every fill (slow);
+ * refused inside graph launches.  No counterpart in the reference. */
+swm_status swm_set_halo_shadow(swm_ctx* ctx, int enable);
+swm_status swm_halo_shadow_stats(const swm_ctx* ctx, long long* checks, long long* mismatches);
+
+/* Restarts the step counter of every member without changing its state:
+ * the latest time level is copied into both levels and the next step is a
+ * forward Euler step, as after a restart read (swm/source/swm_restart.F:
+ * 76-87 restarts the ABM3 from Euler).  The model time goes on.  No
+ * counterpart as a call in the reference. */
+swm_status swm_restart_clock(swm_ctx* ctx);
 
 /* Copies a field of the current time level to the host (reference layout):
  * "h","eta","div","relative","ke","psi","chi" (iim,jjm,10); "u"

@@ -58,7 +58,10 @@ def build(verbose=True):
     hipcc = _hipcc()
     for src in DEVICE_SOURCES:
         obj = os.path.join(obj_dir, src + ".o")
-        _run([hipcc, "-std=c++17", "-O3", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=off",
+        # code-object v5: the kernels read their workgroup size from the
+        # implicit kernel arguments at the offset of that layout
+        _run([hipcc, "-std=c++17", "-O3", "-fPIC", "--offload-arch=gfx950", "-mcode-object-version=5",
+              "-ffp-contract=off",
               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
               "-Wno-unused-result", "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

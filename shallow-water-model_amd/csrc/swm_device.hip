@@ -43,6 +43,10 @@ struct DevTable {
   int n = 0;
   int* dst = nullptr;
   int* src = nullptr;
+  // host copy of the entries and the span of a field (components x slots),
+  // for the halo shadow check
+  std::vector<int> hdst, hsrc;
+  size_t span = 0;
 };
 
 // one 256-byte aligned device arena for all buffers of a context
@@ -167,6 +171,8 @@ struct swm_ctx {
   // (bit 0 psi, bit 1 chi); fixed: the fixed schedule (no halo fill between)
   struct SolveSched { int first = 0, extra = 0, bad_mask = 0, fixed = 0; } last_sched;
   int lin_exact = 1;           // TL/AD: the derivative of the forward's own cycles (1) or of converged solves (0)
+  int halo_shadow = 0;         // every halo fill checked against the host's (swm_set_halo_shadow)
+  long long shadow_checks = 0, shadow_bad = 0;
   bool initialized = false;
   // elliptic solve settings (reference: mean(r^2) <= 1e-20, 12 cycles) and
   // the mean squared residuals / right-hand sides of the last solve
@@ -229,6 +235,9 @@ swm_status make_table(swm_ctx* C, const swm::HaloTable& t, int ns, DevTable& d) 
     src[k] = t.src[k] < 0 ? -1 : (t.src[k] % t.ncomp) * ns + t.src[k] / t.ncomp;
   }
   d.n = (int)dst.size();
+  d.hdst = dst;
+  d.hsrc = src;
+  d.span = (size_t)t.ncomp * ns;
   d.dst = C->arena.take<int>(dst.size() + 1);
   d.src = C->arena.take<int>(src.size() + 1);
   if (d.n && C->arena.base) {
@@ -243,7 +252,7 @@ inline dim3 zgrid(const swm_ctx* C, unsigned x, unsigned y = 1) { return dim3(x,
 
 // halo fills of member state fields
 template <class R>
-void gather_list(swm_ctx* C, const DevTable& t, const std::vector<R*>& fields) {
+void gather_launch(swm_ctx* C, const DevTable& t, const std::vector<R*>& fields) {
   for (size_t a = 0; a < fields.size(); a += 8) {
     swm::dev::FieldSetT<R> fs{};
     fs.mb = C->mb;
@@ -253,6 +262,37 @@ void gather_list(swm_ctx* C, const DevTable& t, const std::vector<R*>& fields) {
       hipLaunchKernelGGL(swm::dev::k_gather<R>, zgrid(C, nblk_b(t.n, C->blk_step), nf), dim3(C->blk_step), 0, C->stream,
                          t.dst, t.src, t.n, fs);
   }
+}
+// the halo shadow check (swm_set_halo_shadow): each field of every member in
+// play is copied to the host before and after the device fill, the host
+// applies the same table to its copy, and the two are compared bit for bit
+template <class R>
+void gather_shadow(swm_ctx* C, const DevTable& t, const std::vector<R*>& fields) {
+  const size_t span = t.span;
+  std::vector<std::vector<R>> pre(fields.size() * (size_t)C->nz, std::vector<R>(span));
+  auto at = [&](R* f, int z) { return (R*)((char*)f + (size_t)z * C->mb); };
+  bool ok = hipStreamSynchronize(C->stream) == hipSuccess;
+  for (size_t f = 0; ok && f < fields.size(); ++f)
+    for (int z = 0; ok && z < C->nz; ++z)
+      ok = hipMemcpy(pre[f * C->nz + z].data(), at(fields[f], z), span * sizeof(R), hipMemcpyDeviceToHost) == hipSuccess;
+  gather_launch<R>(C, t, fields);
+  ok = ok && hipStreamSynchronize(C->stream) == hipSuccess;
+  std::vector<R> post(span);
+  for (size_t f = 0; ok && f < fields.size(); ++f)
+    for (int z = 0; ok && z < C->nz; ++z) {
+      std::vector<R>& x = pre[f * C->nz + z];
+      ok = hipMemcpy(post.data(), at(fields[f], z), span * sizeof(R), hipMemcpyDeviceToHost) == hipSuccess;
+      for (int k = 0; k < t.n; ++k) x[(size_t)t.hdst[k]] = t.hsrc[k] < 0 ? R(0) : x[(size_t)t.hsrc[k]];
+      for (size_t q = 0; ok && q < span; ++q)
+        if (std::memcmp(&x[q], &post[q], sizeof(R)) != 0) C->shadow_bad += 1;
+      C->shadow_checks += 1;
+    }
+  if (!ok) C->shadow_bad += 1;   // a copy failed: report it through the same count
+}
+template <class R>
+void gather_list(swm_ctx* C, const DevTable& t, const std::vector<R*>& fields) {
+  if (C->halo_shadow && t.n) gather_shadow<R>(C, t, fields);
+  else gather_launch<R>(C, t, fields);
 }
 template <class R>
 void gather(swm_ctx* C, const DevTable& t, std::initializer_list<R*> fields) {
@@ -510,6 +550,13 @@ swm_status member_norms(swm_ctx* C, const double* part, int nb, const std::vecto
     if (st) return st;
   }
   for (int b = 0; b < nv; ++b) C->last_resid[vars[b]] = 0.0;
+  for (int m = 0; m < C->nz; ++m)
+    for (int b = 0; b < nv; ++b)
+      if (!std::isfinite(raw[(size_t)m * nv + b])) {
+        swm::set_error("non-finite residual norm in the elliptic solve (member " + std::to_string(m) +
+                       ", field " + std::to_string(vars[b]) + "): the state holds NaN or Inf");
+        return SWM_ERR_NAN;
+      }
   for (int m = 0; m < C->nz; ++m)
     for (int b = 0; b < nv; ++b) {
       const double v = raw[(size_t)m * nv + b] / (double)C->max_ig;
@@ -1044,7 +1091,51 @@ struct SplitMix {
 // zero the whole state (the reference's module arrays start at zero), the
 // solver's warm start included, and the clock and step counter
 // (time_manager_module.F:60-72); keep_hs leaves the surface height as it is
+}  // namespace
+namespace swm {
+void shift_members(swm_ctx* C, ptrdiff_t d);
+}  // namespace swm
+namespace {
+
+// every member's latest time level into both time levels (h, eta, div, u,
+// tracers; FP64 and FP32 views), so that after the step counter restarts,
+// the next step of each member is a forward Euler step from its latest
+// state.  Works on all members whatever member a scope has selected.
+template <class R>
+swm_status latest_into_both_levels_view(swm_ctx* C, View<R>& V) {
+  const int lv = C->n4 - 1, ot = 1 - lv;
+  const size_t ns = C->ns;
+  auto cp = [&](R* dst, const R* src, size_t n) -> swm_status {
+    if (!dst || !src) return SWM_OK;
+    HIP_TRY(hipMemcpy2DAsync(dst, C->mb, src, C->mb, n * sizeof(R), (size_t)C->nmem, hipMemcpyDeviceToDevice,
+                             C->stream));
+    return SWM_OK;
+  };
+  swm_status st;
+  if ((st = cp(V.h[ot], V.h[lv], ns)) || (st = cp(V.eta[ot], V.eta[lv], ns)) || (st = cp(V.div[ot], V.div[lv], ns)) ||
+      (st = cp(V.u[ot], V.u[lv], 4 * ns)))
+    return st;
+  for (int q = 0; q < C->nt; ++q)
+    if ((st = cp(V.tr[ot][q], V.tr[lv][q], ns))) return st;
+  return SWM_OK;
+}
+swm_status latest_into_both_levels(swm_ctx* C) {
+  if (C->step_count == 0) return SWM_OK;
+  const ptrdiff_t d = C->nz == 1 ? (ptrdiff_t)C->sel * (ptrdiff_t)C->mb : 0;   // undo a member scope
+  const int nz = C->nz;
+  swm::shift_members(C, -d);
+  C->nz = C->nmem;
+  swm_status st = latest_into_both_levels_view<double>(C, C->d);
+  if (!st && C->precision == SWM_MIXED) st = latest_into_both_levels_view<float>(C, C->m);
+  swm::shift_members(C, d);
+  C->nz = nz;
+  return st;
+}
+
 void reset_state(swm_ctx* C, bool keep_hs = false) {
+  // the other members continue from their latest state (the step counter
+  // restarts for all of them)
+  if (C->nmem > 1) (void)latest_into_both_levels(C);
   C->solves = C->vcycles_total = C->checks_total = 0;
   fill<double>(C, C->d.state_begin, C->d.state_words, 0.0);
   if (C->precision == SWM_MIXED) fill<float>(C, C->m.state_begin, C->m.state_words, 0.0f);
@@ -1618,6 +1709,41 @@ swm_status swm_create(const swm_grid* grid, const swm_config* cfg, swm_ctx** out
   return swm_create_members(grid, cfg, 1, out);
 }
 
+namespace {
+std::mutex g_selftest_mutex;
+int g_selftest_done[64] = {};   // 1: passed on that device
+}  // namespace
+
+swm_status swm_self_test(int device) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    swm::set_error("swm_self_test: no such HIP device");
+    return SWM_ERR_NO_DEVICE;
+  }
+  std::lock_guard<std::mutex> lock(g_selftest_mutex);
+  if (device < 64 && g_selftest_done[device]) return SWM_OK;
+  int prev = 0;
+  HIP_TRY(hipGetDevice(&prev));
+  HIP_TRY(hipSetDevice(device));
+  int* d = nullptr;
+  HIP_TRY(hipMalloc(&d, 2 * sizeof(int)));
+  bool ok = true;
+  for (int b : {64, 128, 256, 512, 1024}) {
+    int h[2] = {0, 0};
+    hipLaunchKernelGGL(swm::dev::k_block_size_probe, dim3(2), dim3(b), 0, 0, d);
+    if (hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess || h[0] != 1 || h[1] != 1) ok = false;
+  }
+  (void)hipFree(d);
+  (void)hipSetDevice(prev);
+  if (!ok) {
+    swm::set_error("swm_self_test: the kernels' workgroup size, read from the implicit kernel arguments, differs "
+                   "from blockDim (the code object is not version 5)");
+    return SWM_ERR_HIP;
+  }
+  if (device < 64) g_selftest_done[device] = 1;
+  return SWM_OK;
+}
+
 swm_status swm_create_members(const swm_grid* grid, const swm_config* cfg, int nmem, swm_ctx** out) {
   if (!grid || !cfg || !out || nmem < 1 || nmem > 4096 || cfg->ntracer < 0 || cfg->ntracer > 64 || !(cfg->dt > 0.0) ||
       (cfg->precision != SWM_FP64 && cfg->precision != SWM_MIXED) || cfg->advection < SWM_ADV_FCT ||
@@ -1630,6 +1756,8 @@ swm_status swm_create_members(const swm_grid* grid, const swm_config* cfg, int n
     swm::set_error("swm_create: no HIP device");
     return SWM_ERR_NO_DEVICE;
   }
+  swm_status st0 = swm_self_test(cfg->device);
+  if (st0) return st0;
   const swm::Grid& Gh = swm::grid_of(grid);
   swm_ctx* C = new swm_ctx;
   C->g = Gh.g; C->iim = Gh.iim; C->jjm = Gh.jjm; C->ns = Gh.iim * Gh.jjm * 10; C->nt = cfg->ntracer;
@@ -1882,9 +2010,16 @@ swm_status swm_step(swm_ctx* C, int nsteps) {
   C->kernel_bytes = 0.0;
   C->kernel_pair_launches = 0;
   HIP_TRY(hipEventRecord(C->ev_a, C->stream));
+  const long long bad0 = C->shadow_bad;
   for (int n = 0; n < nsteps; ++n) {
     swm_status st = swm::step_once(C);
     if (st) return st;
+    if (C->shadow_bad != bad0) {
+      swm::set_error("halo shadow check: " + std::to_string(C->shadow_bad - bad0) + " values of the device's halo "
+                     "fills differ from the host's fill with the same tables (step " + std::to_string(C->step_count) +
+                     ")");
+      return SWM_ERR_STATE;
+    }
   }
   HIP_TRY(hipEventRecord(C->ev_b, C->stream));
   HIP_TRY(hipEventSynchronize(C->ev_b));
@@ -2515,6 +2650,7 @@ swm_status swm_op_centered_divergence_curl(swm_ctx* C, int npad, int nx, int ny,
 // on the whole sphere of the context's grid (iim, jjm and nsdm = 10 of the
 // grid, one layer), with the step's kernels.  Levels of a multi-level host
 // array are moved to and from device planes slot by slot.
+}  // extern "C"
 namespace {
 // level t of a host array (lead, iim*jjm, nlev, 10) as (lead, ns) in slot order
 std::vector<double> take_level(const double* a, int lead, int pl, int nlev, int t) {
@@ -2532,6 +2668,7 @@ void put_level(double* a, const std::vector<double>& v, int lead, int pl, int nl
         a[(size_t)e + lead * ((size_t)q + (size_t)pl * (t + (size_t)nlev * k))] = v[(size_t)e + lead * ((size_t)q + (size_t)pl * k)];
 }
 }  // namespace
+extern "C" {
 
 swm_status swm_op_set_mass_fct(swm_ctx* C, int iim, int jjm, int nlm, int nsdm, int ntprog, int nttend, int ncorners,
                                int ndata, int nedges, int nspace, int npad, int r3, int n3, int n4,
@@ -3636,6 +3773,33 @@ swm_status swm_adj_run(swm_ctx* C, int mode, const double* lh, const double* let
 swm_status swm_set_linearisation(swm_ctx* C, int exact) {
   if (!C || exact < 0 || exact > 1) { swm::set_error("swm_set_linearisation: bad argument"); return SWM_ERR_ARG; }
   C->lin_exact = exact;
+  return SWM_OK;
+}
+
+swm_status swm_set_halo_shadow(swm_ctx* C, int enable) {
+  if (!C || enable < 0 || enable > 1) { swm::set_error("swm_set_halo_shadow: bad argument"); return SWM_ERR_ARG; }
+  C->halo_shadow = enable;
+  C->epoch += 1;
+  return SWM_OK;
+}
+
+swm_status swm_halo_shadow_stats(const swm_ctx* C, long long* checks, long long* mismatches) {
+  if (!C || !checks || !mismatches) { swm::set_error("swm_halo_shadow_stats: bad argument"); return SWM_ERR_ARG; }
+  *checks = C->shadow_checks;
+  *mismatches = C->shadow_bad;
+  return SWM_OK;
+}
+
+swm_status swm_restart_clock(swm_ctx* C) {
+  if (!C || !C->initialized) { swm::set_error("swm_restart_clock: needs an initialised context"); return SWM_ERR_STATE; }
+  HIP_TRY(hipSetDevice(C->device));
+  swm_status st = latest_into_both_levels(C);
+  if (st) return st;
+  C->step_count = 0;
+  C->nsol = 0;
+  C->epoch += 1;
+  pointers(C);
+  HIP_TRY(hipStreamSynchronize(C->stream));
   return SWM_OK;
 }
 

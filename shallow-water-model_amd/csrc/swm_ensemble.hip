@@ -320,6 +320,10 @@ swm_status swm_ensemble_step(swm_ensemble* e, int nsteps) {
   if (!e || nsteps < 0) { swm::set_error("swm_ensemble_step: bad argument"); return SWM_ERR_ARG; }
   for (const swm_ctx* C : e->g) {
     if (!C->initialized) { swm::set_error("swm_ensemble_step: a group is not initialised (every member)"); return SWM_ERR_STATE; }
+    if (C->halo_shadow) {
+      swm::set_error("swm_ensemble_step: the halo shadow check cannot run inside a captured graph");
+      return SWM_ERR_STATE;
+    }
     if (C->precision != SWM_MIXED || C->fixed_cycles < 1) {
       swm::set_error("swm_ensemble_step: every group runs the mixed step with a fixed schedule (swm_set_fixed_cycles)");
       return SWM_ERR_STATE;

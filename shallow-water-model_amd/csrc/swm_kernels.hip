@@ -96,6 +96,13 @@ __device__ __forceinline__ int bdim() {
 }
 __device__ __forceinline__ int xcd_slot() { return xcd_block() * bdim() + threadIdx.x; }
 
+// the workgroup size bdim() reads from the implicit kernel arguments
+// (code-object v5 layout, which the build pins) against blockDim.x: 1 per
+// block where they agree
+__global__ void k_block_size_probe(int* out) {
+  if (threadIdx.x == 0) out[blockIdx.x] = bdim() == (int)blockDim.x ? 1 : 0;
+}
+
 template <class R>
 __device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
 template <class R>

@@ -126,6 +126,10 @@ _sig("swm_op_set_mass_fct", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 1
      [ctypes.c_double] + [ctypes.c_void_p] * 12)
 _sig("swm_op_set_tracer_fct", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 5 + [ctypes.c_double] +
      [ctypes.c_void_p] * 12)
+_sig("swm_self_test", ctypes.c_int, [ctypes.c_int])
+_sig("swm_restart_clock", ctypes.c_int, [ctypes.c_void_p])
+_sig("swm_set_halo_shadow", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
+_sig("swm_halo_shadow_stats", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 2)
 _sig("swm_set_linearisation", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_operator_bench", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp])
 _sig("swm_adj_limiter_stats", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 2)
@@ -671,6 +675,21 @@ class Model:
         out = ctypes.c_double()
         _check(lib.swm_global_real_sum(self._h, _ptr(xf), _ptr(mf), xf.size, ctypes.byref(out)))
         return out.value
+
+    def set_halo_shadow(self, enable=True):
+        """Check every halo fill against the host's fill with the same tables."""
+        _check(lib.swm_set_halo_shadow(self._h, 1 if enable else 0))
+
+    def halo_shadow_stats(self):
+        """(fields checked, values that differed) of the halo shadow check."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib.swm_halo_shadow_stats(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def restart_clock(self):
+        """Restart the step counter of every member, keeping its latest state:
+        the next step is a forward Euler step (swm_restart_clock)."""
+        _check(lib.swm_restart_clock(self._h))
 
     def set_linearisation(self, exact=True):
         """TL/AD of the window's solves: the forward's own V-cycles (True, the
