@@ -250,7 +250,8 @@ swm_status swm_halo_shadow_stats(const swm_ctx* ctx, long long* checks, long lon
 swm_status swm_restart_clock(swm_ctx* ctx);
 
 /* Copies a field of the current time level to the host (reference layout):
- * "h","eta","div","relative","ke","psi","chi" (iim,jjm,10); "u"
+ * "h","eta","div","relative","ke","psi","chi" and the surface height "hs"
+ * (iim,jjm,10); "u"
  * (2,2,iim,jjm,10); "tracer" (iim,jjm,ntracer,10). */
 swm_status swm_get_field(swm_ctx* ctx, const char* name, double* out);
 
@@ -493,6 +494,59 @@ swm_status swm_restart_file_write(const swm_grid* grid, const char* path, int nt
                                   const double* h, const double* u, const double* tracer);
 swm_status swm_restart_file_read(const swm_grid* grid, const char* path, int ntracer, long long* nsecond,
                                  double* h, double* u, double* tracer);
+
+/* ------------------------------------------------------------------------
+ * Quick-plot output (swm/source/swm_qp_module.F over
+ * utilities/layer_1/netcdf_geodesic.F and netcdf_general_tools.F): one
+ * netCDF classic file per run holding, per output time, the 11 fields
+ * thickness, height, relative, divergence, absolute, kinetic_energy and
+ * tracer_1..tracer_5 as FP32 at the real cells (grid_center_mask > 0.5,
+ * panel-major), with the cells' centre and six corner coordinates, their
+ * global numbers and the reference's dimensions, variables and attributes.
+ * ------------------------------------------------------------------------ */
+typedef struct swm_qp swm_qp;
+
+/* The context's panel extents (iim, jjm) and tracer count. */
+swm_status swm_ctx_layout(const swm_ctx* ctx, int* iim, int* jjm, int* ntracer);
+
+/* The reference's file name for a run starting at model time nsecond on MPI
+ * task `task` (swm_qp_module.F:130-136, :176):
+ * "qp_output/hswm_dDDDDDDpTTT.g2.nc", NUL-terminated into buf (len >= 32). */
+swm_status swm_qp_name(long long nsecond, int task, char* buf, int len);
+
+/* initialize_swm_qp (swm_qp_module.F:70-214) with ginit_nc
+ * (netcdf_geodesic.F:58-171): creates the file with the grid's coordinates
+ * and no record.  simulation_tag is the "run" attribute (character*40, blanks
+ * trimmed); path NULL uses swm_qp_name(nsecond, 0), whose directory must
+ * exist.  swm_qp_create derives grid_center_lat/lon and grid_corner_lat/lon
+ * from the grid (grid_metrics.F:194-206); swm_qp_create_coords takes them in
+ * the reference layout, (iim, jjm, 10) and (2, iim, jjm, 10) radians. */
+swm_status swm_qp_create(const swm_grid* grid, const char* simulation_tag, long long nsecond, const char* path,
+                         swm_qp** out);
+swm_status swm_qp_create_coords(const swm_grid* grid, const char* simulation_tag, long long nsecond,
+                                const char* path, const double* center_lat, const double* center_lon,
+                                const double* corner_lat, const double* corner_lon, swm_qp** out);
+
+/* pack_swm_qp (swm_qp_module.F:309-370): the context's current time level
+ * into the file's fields (h, h + hs, relative vorticity, divergence, absolute
+ * vorticity, kinetic energy, tracers 1..min(ntracer, 5)), each FP64 value
+ * rounded to FP32.  Fields with no model counterpart stay zero. */
+swm_status swm_qp_pack(swm_qp* qp, swm_ctx* ctx);
+
+/* qpsum in replace mode (qp_utilities.F) of one field by its qp name, from a
+ * host array (iim, jjm, 10). */
+swm_status swm_qp_pack_field(swm_qp* qp, const char* field, const double* x);
+
+/* write_swm_qp (swm_qp_module.F:218-300) with gwrt_nc and set_time
+ * (netcdf_general_tools.F:288-352): appends one record at model time nsecond
+ * (time = the middle of the period since the previous write, start_period,
+ * end_period, period_length in days, char_time "mm/dd/yyyy" in the no-leap
+ * calendar), then zeroes the packed fields. */
+swm_status swm_qp_write(swm_qp* qp, long long nsecond);
+
+int swm_qp_records(const swm_qp* qp);
+const char* swm_qp_path(const swm_qp* qp);
+void swm_qp_destroy(swm_qp* qp);
 
 /* ------------------------------------------------------------------------
  * Reference operators (utilities/layer_2/operators.F), one entry point per

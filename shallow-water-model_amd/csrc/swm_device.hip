@@ -2053,6 +2053,7 @@ swm_status swm_get_field(swm_ctx* C, const char* name, double* out) {
   else if (s == "ke") src = plane(C->d.ke, C->m.ke);
   else if (s == "psi") src = C->xout;
   else if (s == "chi") src = C->xout + ns;
+  else if (s == "hs") src = C->hs;
   if (src) {
     HIP_TRY(hipStreamSynchronize(C->stream));
     HIP_TRY(hipMemcpy(out, src, ns * sizeof(double), hipMemcpyDeviceToHost));
@@ -2116,6 +2117,14 @@ swm_status swm_set_fusion(swm_ctx* C, int flags) {
 }
 
 int swm_members(const swm_ctx* C) { return C ? C->nmem : 0; }
+
+swm_status swm_ctx_layout(const swm_ctx* C, int* iim, int* jjm, int* ntracer) {
+  if (!C) { swm::set_error("swm_ctx_layout: bad argument"); return SWM_ERR_ARG; }
+  if (iim) *iim = C->iim;
+  if (jjm) *jjm = C->jjm;
+  if (ntracer) *ntracer = C->nt;
+  return SWM_OK;
+}
 
 swm_status swm_select_member(swm_ctx* C, int member) {
   if (!C || member < 0 || member >= C->nmem) { swm::set_error("swm_select_member: no such member"); return SWM_ERR_ARG; }

@@ -11,11 +11,16 @@
 //
 //   swm_driver [-g G] [-precision fp64|mixed] [-ntracer N] [-device D]
 //
-// run in a reference run directory (data/, restarts/).  Not supported, as
-// outside the library: incremental remapping (l_remap) and the netCDF / DX
-// output of l_output (reported once and skipped).
+// run in a reference run directory (data/, restarts/).  The quick-plot
+// netCDF file of swm_qp_module.F (qp_output/hswm_d000000p000.g2.nc, the
+// directory made when absent) holds the initial state and every n_output-th
+// step, as initialize.F:153-159, :275-276 and driver.F:133-135 write it.  Not
+// supported, as outside the library: incremental remapping (l_remap) and the
+// OpenDX files of l_dx (dx_write_utility.F), reported once and skipped.
 #include <dirent.h>
+#include <sys/stat.h>
 
+#include <cerrno>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -151,6 +156,12 @@ int main(int argc, char** argv) {
   swm_ctx* ctx = nullptr;
   check(swm_create(grid, &cfg, &ctx), "swm_create");
 
+  // initialize.F:153-159: the qp file, named for model time 0 (the clock is
+  // set from a restart file only afterwards)
+  if (mkdir("qp_output", 0755) != 0 && errno != EEXIST) die("cannot create qp_output/");
+  swm_qp* qp = nullptr;
+  check(swm_qp_create(grid, "SWTC#5", 0, nullptr, &qp), "initialize_swm_qp");
+
   // initialize.F:222-231: test case 5, then the restart file over it
   std::printf("\n  generating initial conditions for SW Test Case 5\n\n");
   std::vector<double> h(ns), hs(ns), u(4 * ns), tr((size_t)ntracer * ns);
@@ -160,6 +171,10 @@ int main(int argc, char** argv) {
     check(swm_read_restart(ctx, "restarts/swm_restart"), "read_swm_restart");
     std::printf("\n finished reading initial conditions\n\n");
   }
+  auto write_qp = [&]() {   // pack_swm_qp, write_swm_qp
+    check(swm_qp_pack(qp, ctx), "pack_swm_qp");
+    check(swm_qp_write(qp, swm_time_seconds(ctx)), "write_swm_qp");
+  };
 
   FILE* f31 = std::fopen("fort.31", "wb");
   if (!f31) die("cannot open fort.31");
@@ -216,8 +231,9 @@ int main(int argc, char** argv) {
     std::printf("\n\n");
   };
   budgets();   // initialize.F:236
+  write_qp();  // initialize.F:275-276
 
-  bool warned_output = false, halt = false;
+  bool warned_dx = false, halt = false;
   int step_count = 0;
   const auto t0 = std::chrono::steady_clock::now();
   while (!halt) {
@@ -238,9 +254,12 @@ int main(int argc, char** argv) {
       check(swm_write_restart(ctx, name), "write_swm_restart");
       std::printf("\n finished writing restart file %s\n\n", name);
     }
-    if (l_output && !warned_output) {
-      std::printf("\n  l_output: netCDF / DX output is not part of the device library; skipped\n\n");
-      warned_output = true;
+    if (l_output) {
+      write_qp();
+      if (!warned_dx) {
+        std::printf("\n  l_output: OpenDX output is not part of the device library; skipped\n\n");
+        warned_dx = true;
+      }
     }
     if (l_diag) budgets();
   }
@@ -248,6 +267,8 @@ int main(int argc, char** argv) {
   std::fclose(f31);
   std::printf("\n swm_driver: %d steps at level %d (%ld cells), %.3f s, %.4e cell-steps/s\n", step_count, g, max_ig,
               wall, (double)max_ig * step_count / wall);
+  std::printf(" qp output: %d records in %s\n", swm_qp_records(qp), swm_qp_path(qp));
+  swm_qp_destroy(qp);
   swm_destroy(ctx);
   swm_grid_destroy(grid);
   return 0;

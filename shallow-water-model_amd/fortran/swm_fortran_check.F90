@@ -14,7 +14,9 @@
 !        u, tracer; then the Laplacian of h and the vorticity flux of
 !        (h, eta) with a unit mass flux, panel by panel
 ! restart: write_swm_restart after the steps (into restarts/ of the
-!        working directory)
+!        working directory), and the quick-plot file of the run
+!        (qp_output/ of the working directory) with the initial state and
+!        the state after the steps
 !
 !   swm_fortran_check ops <input> <output>
 !
@@ -30,6 +32,7 @@
 program swm_fortran_check
   use iso_c_binding
   use swm_mi355x
+  use swm_qp_module_mi355x
   use operators_mi355x
   implicit none
   character(len=512) :: fin, fout, opt
@@ -59,11 +62,16 @@ program swm_fortran_check
 
   call swm_gpu_initialize(int(g), int(ntracer), dt, grav, radius, omega, h, hs, u, tracer)
   call wrap_prognostics()
+  if (trim(opt) == 'restart') call qp_init()
   do k = 1, nsteps
     call step_dt()
     call wrap_prognostics()
   end do
-  if (trim(opt) == 'restart') call write_swm_restart()
+  if (trim(opt) == 'restart') then
+    call write_swm_restart()
+    call pack_swm_qp()
+    call write_swm_qp(swm_gpu_time())
+  end if
   allocate(eta(n,n,10), div(n,n,10), psi(n,n,10), chi(n,n,10))
   call sync_prognostics_to_host(h, u, tracer, eta=eta, div=div, psi=psi, chi=chi)
 
@@ -81,6 +89,42 @@ program swm_fortran_check
   call swm_gpu_finalize()
 
 contains
+
+  ! initialize.F:153-159 and :275-276: the qp file with the grid's
+  ! coordinates (xyz_to_lonlat of the centres and corners, grid_metrics.F:
+  ! 194-206) and the initial state as its first record
+  subroutine qp_init()
+    real(c_double), allocatable :: pxyz(:,:,:,:), cxyz(:,:,:,:,:), clat(:,:,:), clon(:,:,:), crlat(:,:,:,:), &
+                                   crlon(:,:,:,:)
+    integer :: i, j, kk, ic
+    allocate(pxyz(3,n,n,10), cxyz(3,2,n,n,10), clat(n,n,10), clon(n,n,10), crlat(2,n,n,10), crlon(2,n,n,10))
+    call swm_gpu_grid_array('p', pxyz)
+    call swm_gpu_grid_array('c', cxyz)
+    do kk = 1, 10
+      do j = 1, n
+        do i = 1, n
+          call lonlat(pxyz(:,i,j,kk), clon(i,j,kk), clat(i,j,kk))
+          do ic = 1, 2
+            call lonlat(cxyz(:,ic,i,j,kk), crlon(ic,i,j,kk), crlat(ic,i,j,kk))
+          end do
+        end do
+      end do
+    end do
+    call initialize_swm_qp('SWTC#5', 0_c_long_long, clat, clon, crlat, crlon)
+    call pack_swm_qp()
+    call write_swm_qp(swm_gpu_time())
+  end subroutine qp_init
+
+  subroutine lonlat(p, lon, lat)
+    real(c_double), intent(in) :: p(3)
+    real(c_double), intent(out) :: lon, lat
+    lat = asin(p(3))
+    if (p(1) /= 0.0_c_double .or. p(2) /= 0.0_c_double) then
+      lon = atan2(p(2), p(1))
+    else
+      lon = 0.0_c_double
+    end if
+  end subroutine lonlat
 
   subroutine run_ops(fin, fout)
     use mass_fct_mi355x

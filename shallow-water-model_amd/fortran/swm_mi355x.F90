@@ -149,6 +149,31 @@ module swm_mi355x_c
       type(c_ptr), value :: ctx, path
       integer(c_int) :: st
     end function
+    function swm_qp_create_coords(grid, tag, nsecond, path, clat, clon, crlat, crlon, qp) &
+        bind(C, name="swm_qp_create_coords") result(st)
+      import :: c_int, c_ptr, c_char, c_long_long, c_double
+      type(c_ptr), value :: grid, path
+      character(kind=c_char) :: tag(*)
+      integer(c_long_long), value :: nsecond
+      real(c_double) :: clat(*), clon(*), crlat(*), crlon(*)
+      type(c_ptr) :: qp
+      integer(c_int) :: st
+    end function
+    function swm_qp_pack(qp, ctx) bind(C, name="swm_qp_pack") result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: qp, ctx
+      integer(c_int) :: st
+    end function
+    function swm_qp_write(qp, nsecond) bind(C, name="swm_qp_write") result(st)
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: qp
+      integer(c_long_long), value :: nsecond
+      integer(c_int) :: st
+    end function
+    subroutine swm_qp_destroy(qp) bind(C, name="swm_qp_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: qp
+    end subroutine
     function swm_wrap(ctx, kind, nspace, nlayer, x) bind(C, name="swm_wrap") result(st)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
@@ -319,7 +344,7 @@ module swm_mi355x
   use swm_mi355x_c
   implicit none
   private
-  public :: swm_gpu_initialize, swm_gpu_finalize, swm_gpu_check, swm_gpu_context, swm_gpu_grid_array, &
+  public :: swm_gpu_initialize, swm_gpu_finalize, swm_gpu_check, swm_gpu_context, swm_gpu_grid, swm_gpu_grid_array, &
             step_dt, wrap_prognostics, sync_prognostics_to_host, elliptic, multigrid, wrap, wrap_face, &
             wrap_vrtx, wrap_edge, write_swm_restart, read_swm_restart, swm_gpu_time, initialize_wrap, &
             initialize_multigrid
@@ -357,6 +382,11 @@ contains
   function swm_gpu_context() result(p)
     type(c_ptr) :: p
     p = ctx
+  end function
+
+  function swm_gpu_grid() result(p)
+    type(c_ptr) :: p
+    p = grid
   end function
 
   ! Replaces the device-relevant part of initialize_model (swm/source/
@@ -687,6 +717,46 @@ contains
                        nedges, nspace, r3, dc, tangent, mass_c, eta, u, mass, flux), 'centered_divergence_curl')
   end subroutine
 end module operators_mi355x
+
+!=======================================================================
+! swm/source/swm_qp_module.F under its own names and arguments: the
+! quick-plot netCDF file of the run, qp_output/hswm_dDDDDDDp000.g2.nc (the
+! directory must exist), with the coordinates the caller passes, the fields
+! packed from the device state's current time level
+module swm_qp_module_mi355x
+  use iso_c_binding
+  use swm_mi355x_c
+  use swm_mi355x, only: swm_gpu_check, swm_gpu_context, swm_gpu_grid
+  implicit none
+  private
+  public :: initialize_swm_qp, write_swm_qp, pack_swm_qp
+  type(c_ptr), save :: qp = c_null_ptr
+contains
+  ! swm_qp_module.F:70-214
+  subroutine initialize_swm_qp(simulation_tag, nsecond, grid_center_lat, grid_center_lon, grid_corner_lat, &
+                               grid_corner_lon)
+    character(len=*), intent(in) :: simulation_tag
+    integer(c_long_long), intent(in) :: nsecond
+    real(c_double), intent(in) :: grid_center_lat(:,:,:), grid_center_lon(:,:,:)
+    real(c_double), intent(in) :: grid_corner_lat(:,:,:,:), grid_corner_lon(:,:,:,:)
+    if (c_associated(qp)) call swm_qp_destroy(qp)
+    qp = c_null_ptr
+    call swm_gpu_check(swm_qp_create_coords(swm_gpu_grid(), trim(simulation_tag)//c_null_char, nsecond, &
+                                            c_null_ptr, grid_center_lat, grid_center_lon, grid_corner_lat, &
+                                            grid_corner_lon, qp), 'initialize_swm_qp')
+  end subroutine initialize_swm_qp
+
+  ! swm_qp_module.F:309-370
+  subroutine pack_swm_qp()
+    call swm_gpu_check(swm_qp_pack(qp, swm_gpu_context()), 'pack_swm_qp')
+  end subroutine pack_swm_qp
+
+  ! swm_qp_module.F:218-300
+  subroutine write_swm_qp(nsecond)
+    integer(c_long_long), intent(in) :: nsecond
+    call swm_gpu_check(swm_qp_write(qp, nsecond), 'write_swm_qp')
+  end subroutine write_swm_qp
+end module swm_qp_module_mi355x
 
 !=======================================================================
 ! global_reductions.F:77-98 global_real_sum under its own name and

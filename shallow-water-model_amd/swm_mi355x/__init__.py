@@ -139,6 +139,18 @@ _sig("swm_multigrid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_voi
 _sig("swm_create_members", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Config), ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_void_p)])
 _sig("swm_members", ctypes.c_int, [ctypes.c_void_p])
+_sig("swm_ctx_layout", ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p, _c_int_p])
+_sig("swm_qp_name", ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, ctypes.c_char_p, ctypes.c_int])
+_sig("swm_qp_create", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_longlong, ctypes.c_char_p,
+                                     ctypes.POINTER(ctypes.c_void_p)])
+_sig("swm_qp_create_coords", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_longlong, ctypes.c_char_p]
+     + [ctypes.c_void_p] * 4 + [ctypes.POINTER(ctypes.c_void_p)])
+_sig("swm_qp_pack", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p])
+_sig("swm_qp_pack_field", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p])
+_sig("swm_qp_write", ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong])
+_sig("swm_qp_records", ctypes.c_int, [ctypes.c_void_p])
+_sig("swm_qp_path", ctypes.c_char_p, [ctypes.c_void_p])
+_sig("swm_qp_destroy", None, [ctypes.c_void_p])
 _sig("swm_set_mixed_smoother", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
 _sig("swm_set_overlap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_set_fusion", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
@@ -274,6 +286,74 @@ def read_restart_file(grid, path, ntracer):
     _check(lib.swm_restart_file_read(grid.handle, os.fsencode(path), ntracer, ctypes.byref(ts), _ptr(h), _ptr(u),
                                      _ptr(tr)))
     return ts.value, h, u, tr
+
+
+QP_FIELDS = ("thickness", "height", "relative", "divergence", "absolute", "kinetic_energy",
+             "tracer_1", "tracer_2", "tracer_3", "tracer_4", "tracer_5")
+
+
+def qp_name(nsecond=0, task=0):
+    """initialize_swm_qp's file name (swm_qp_module.F:130-136, :176)."""
+    buf = ctypes.create_string_buffer(64)
+    _check(lib.swm_qp_name(int(nsecond), int(task), buf, 64))
+    return buf.value.decode()
+
+
+class QpFile:
+    """The quick-plot netCDF file of swm_qp_module.F: initialize_swm_qp at
+    construction, pack_swm_qp (pack / pack_field) and write_swm_qp (write).
+
+    path None uses the reference's name under qp_output/ (the directory must
+    exist); coords = (center_lat, center_lon, corner_lat, corner_lon) in the
+    reference layout replaces the coordinates derived from the grid."""
+
+    def __init__(self, grid, simulation_tag="SWTC#5", nsecond=0, path=None, coords=None):
+        h = ctypes.c_void_p()
+        p = None if path is None else os.fsencode(path)
+        tag = simulation_tag.encode()
+        if coords is None:
+            _check(lib.swm_qp_create(grid.handle, tag, int(nsecond), p, ctypes.byref(h)))
+        else:
+            s = (grid.iim, grid.jjm, 10)
+            clat, clon = _f64(coords[0], s), _f64(coords[1], s)
+            crlat, crlon = _f64(coords[2], (2,) + s), _f64(coords[3], (2,) + s)
+            _check(lib.swm_qp_create_coords(grid.handle, tag, int(nsecond), p, _ptr(clat), _ptr(clon), _ptr(crlat),
+                                            _ptr(crlon), ctypes.byref(h)))
+        self._h = h
+        self.grid = grid
+
+    @property
+    def path(self):
+        return lib.swm_qp_path(self._h).decode()
+
+    @property
+    def records(self):
+        return lib.swm_qp_records(self._h)
+
+    def pack(self, model):
+        """pack_swm_qp: the model's current time level."""
+        _check(lib.swm_qp_pack(self._h, model._h))
+
+    def pack_field(self, name, x):
+        """qpsum (replace) of one field from a host array (iim, jjm, 10)."""
+        x = _f64(x, (self.grid.iim, self.grid.jjm, 10))
+        _check(lib.swm_qp_pack_field(self._h, name.encode(), _ptr(x)))
+
+    def write(self, nsecond):
+        """write_swm_qp: one record at model time nsecond; the packed fields
+        return to zero."""
+        _check(lib.swm_qp_write(self._h, int(nsecond)))
+
+    def close(self):
+        if self._h:
+            lib.swm_qp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # shapes of the named grid arrays, in units of (iim, jjm, 10) slots

@@ -34,6 +34,7 @@ def test_fortran_driver_matches_library(tmp_path):
         for k in ("h", "hs", "u", "tracer"):
             f.write(np.asfortranarray(st[k], dtype="<f8").tobytes(order="F"))
     (tmp_path / "restarts").mkdir()
+    (tmp_path / "qp_output").mkdir()
     r = subprocess.run([EXE, str(fin), str(fout), "restart"], capture_output=True, text=True, timeout=120,
                        cwd=str(tmp_path))
     assert r.returncode == 0, r.stdout + r.stderr
@@ -64,6 +65,23 @@ def test_fortran_driver_matches_library(tmp_path):
     assert np.array_equal(rh[m], out["h"][m])
     assert np.array_equal(ru[:, :, m], out["u"][:, :, m])
     assert np.array_equal(rt.transpose(0, 1, 3, 2)[m], out["tracer"].transpose(0, 1, 3, 2)[m])
+
+    # initialize_swm_qp / pack_swm_qp / write_swm_qp through the Fortran
+    # module: the same file as the library makes driven from Python (the
+    # creation time in the history attribute aside)
+    mine = tmp_path / "mine_qp.nc"
+    m0 = S.Model(grid, ntracer=nt, dt=dt, grav=S.REF_GRAV)
+    m0.init_state(st["h"], st["u"], st["tracer"], hs=st["hs"])
+    q = S.QpFile(grid, "SWTC#5", 0, str(mine))
+    q.pack(m0)
+    q.write(0)
+    m0.step(nsteps)
+    q.pack(m0)
+    q.write(int(nsteps * dt))
+    q.close()
+    a, b = (tmp_path / S.qp_name(0)).read_bytes(), mine.read_bytes()
+    k = b.index(b"Created: ")
+    assert a[:k] == b[:k] and a[k + 39:] == b[k + 39:]
 
     orc = Oracle(grid_arrays_from(grid), nt, dt, S.REF_GRAV)
     h = model.field("h")

@@ -1,11 +1,13 @@
 """Shallow-water test case 5 on the device and the stand-alone driver
 (lib/swm_driver: data/namelist, test case 5 or a restart file, the time
-manager's restart / diagnostics cadence, fort.31 budget records).
+manager's restart / diagnostics / output cadence, fort.31 budget records,
+the quick-plot netCDF file).
 
 The device run of test case 5 on the shipped g = 6 grid meets the FP64 step
 bars against the oracle (1e-12 after one step, 1e-9 after several).  The
-driver's outputs (budget records, restart files) equal, bit for bit, those of
-the same library driven step by step from Python.
+driver's outputs (budget records, restart files, the quick-plot records)
+equal, bit for bit, those of the same library driven step by step from
+Python.
 """
 import os
 import subprocess
@@ -93,6 +95,8 @@ def test_driver_matches_library(tmp_path):
     m.init_state(st["h"], st["u"], st["tracer"], hs=st["hs"])
     cols = [0, 1, 2, 3, 4, 7, 10, 11]
     names = {2: S.restart_name(200), 4: S.restart_name(400)}
+    qp_path = tmp_path / S.qp_name(0)
+    qp_want = []
     for s in range(6):
         if s:
             m.step(1)
@@ -102,6 +106,9 @@ def test_driver_matches_library(tmp_path):
             mine = tmp_path / f"mine_{s}"
             m.write_restart(str(mine))
             assert (tmp_path / names[s]).read_bytes() == mine.read_bytes(), s
+        if s in (0, 3):   # the initial state and n_output = 3
+            qp_want.append((s * 100, _qp_fields(grid, m, st["hs"])))
+    _check_qp(qp_path, qp_want, 0)
     # restart from the file of step 4 (l_restart): 100 s steps from 400 s
     # until tau > 0.169 h, i.e. 3 steps
     (tmp_path / "restarts" / "swm_restart").write_bytes((tmp_path / names[4]).read_bytes())
@@ -114,6 +121,37 @@ def test_driver_matches_library(tmp_path):
     m2.init_state(st["h"], st["u"], st["tracer"], hs=st["hs"])
     m2.read_restart(str(tmp_path / names[4]))
     assert np.array_equal(rec[0, cols], m2.budgets()[:8].astype(np.float32))
+    qp_want = [(400, _qp_fields(grid, m2, st["hs"]))]
     m2.step(3)
     assert m2.time_seconds == 700
     assert np.array_equal(rec[1, cols], m2.budgets()[:8].astype(np.float32))
+    qp_want.append((700, _qp_fields(grid, m2, st["hs"])))
+    # the file is named for model time 0 and its first period starts there
+    # (initialize.F:153-159 runs before the restart read)
+    _check_qp(qp_path, qp_want, 0)
+
+
+def _qp_fields(grid, m, hs):
+    mask = (grid.array("grid_center_mask") > 0.5).ravel(order="F")
+    pk = lambda x: np.asarray(x).ravel(order="F")[mask].astype(np.float32)
+    h = m.field("h")
+    t = m.field("tracer")
+    out = {"thickness": pk(h), "height": pk(h + hs), "relative": pk(m.field("relative")),
+           "divergence": pk(m.field("div")), "absolute": pk(m.field("eta")), "kinetic_energy": pk(m.field("ke"))}
+    for k in range(5):
+        out[f"tracer_{k + 1}"] = pk(t[:, :, k, :])
+    return out
+
+
+def _check_qp(path, want, t0):
+    from scipy.io import netcdf_file
+    nc = netcdf_file(str(path), "r", mmap=False)
+    v = nc.variables
+    assert v["time"].shape[0] == len(want)
+    start = t0
+    for r, (t, f) in enumerate(want):
+        assert v["start_period"][r] == start / 86400.0 and v["end_period"][r] == t / 86400.0, r
+        for name, x in f.items():
+            assert np.array_equal(v[name][r], x), (r, name)
+        start = t
+    nc.close()
