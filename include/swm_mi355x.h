@@ -98,11 +98,13 @@ typedef struct swm_ctx swm_ctx;
  * the reference. */
 enum { SWM_FP64 = 0, SWM_MIXED = 1 };
 
-/* Advection of mass and tracers in step_dt (step.F:131-134, namelist flags
- * l_fct / l_centered / l_upwind): flux-corrected transport (mass_fct.F,
- * tracer_fct.F), or centred / upstream fluxes with ABM3 stepping of the
- * thickness and the mass-weighted tracers (step.F:221-327, :495-602). */
-enum { SWM_ADV_FCT = 0, SWM_ADV_CENTERED = 1, SWM_ADV_UPWIND = 2 };
+/* Advection of mass and tracers in step_dt (step.F:131-135, namelist flags
+ * l_fct / l_centered / l_upwind / l_remap): flux-corrected transport
+ * (mass_fct.F, tracer_fct.F), centred / upstream fluxes with ABM3 stepping
+ * of the thickness and the mass-weighted tracers (step.F:221-327, :495-602),
+ * or incremental remapping (get_remap_flux, step.F:687-771, over
+ * utilities/layer_2/remap.F; FP64 single-member contexts only). */
+enum { SWM_ADV_FCT = 0, SWM_ADV_CENTERED = 1, SWM_ADV_UPWIND = 2, SWM_ADV_REMAP = 3 };
 
 typedef struct swm_config {
   int ntracer;       /* passive tracers (the reference compiles 5: swm_parameters.F:66) */
@@ -110,7 +112,8 @@ typedef struct swm_config {
   double grav;       /* gravity (m/s^2) */
   int precision;     /* SWM_FP64 or SWM_MIXED */
   int device;        /* HIP device ordinal */
-  int advection;     /* SWM_ADV_FCT (0, the default of a zeroed struct), SWM_ADV_CENTERED, SWM_ADV_UPWIND */
+  int advection;     /* SWM_ADV_FCT (0, the default of a zeroed struct), SWM_ADV_CENTERED, SWM_ADV_UPWIND,
+                        SWM_ADV_REMAP */
 } swm_config;
 
 /* Checks, once per device, that the kernels read their workgroup size from
@@ -248,6 +251,18 @@ swm_status swm_halo_shadow_stats(const swm_ctx* ctx, long long* checks, long lon
  * 76-87 restarts the ABM3 from Euler).  The model time goes on.  No
  * counterpart as a call in the reference. */
 swm_status swm_restart_clock(swm_ctx* ctx);
+
+/* Incremental remapping (SWM_ADV_REMAP): the tracer limiter's switches
+ * l_limit_tracers_min and l_limit_tracers_max of limit_gradient (remap.F:
+ * 2152-2397), both 1 by default (the two-sided van Leer limit that
+ * l_limit_tracers = .true. asks for).  remap_mass_and_tracers (remap.F:
+ * 1072-1604) gives the two switches no value when the caller does not pass
+ * them, and get_remap_flux does not: each build of the reference uses
+ * whatever its stack holds (observed: a level-4 build on in every step, a
+ * level-3 build on in the first step only).  swm_step returns
+ * SWM_ERR_STATE when a thickness below -1e-10 reaches the remapping
+ * (remap_prep stops the reference there). */
+swm_status swm_set_remap_limiter(swm_ctx* ctx, int limit_min, int limit_max);
 
 /* Copies a field of the current time level to the host (reference layout):
  * "h","eta","div","relative","ke","psi","chi" and the surface height "hs"
@@ -668,6 +683,19 @@ swm_status swm_op_set_tracer_fct(swm_ctx* ctx, int iim, int jjm, int nlm, int ns
                                  const double* mss, const double* mss_low, const double* mss_horz,
                                  const double* mss_flux_horz, double* trc_fct, double* trc_flux_horz,
                                  double* msstrc_low_order, const double* mask);
+
+/* remap_mass_and_tracers (utilities/layer_2/remap.F:1072-1604) under its
+ * mandatory arguments, in place on mass (iim,jjm,nmass,nsdm) and trace
+ * (iim,jjm,ntrace,nmass,nsdm) with corner velocities uc (2,2,iim,jjm,nsdm);
+ * edgeflux (1,3,iim,jjm,nsdm, may be NULL) receives the mass transport over
+ * dt.  The optional arguments take the values get_remap_flux gives them
+ * (step.F:726-730): linear integrals, first-order departure points, no
+ * mask, tracers independent, their gradients limited with the switches of
+ * swm_set_remap_limiter.  A context made with SWM_ADV_REMAP, the whole grid
+ * (10 panels) and nmass = 1 (SWM_ERR_DIMS otherwise); SWM_ERR_STATE for a
+ * mass below -1e-10, where remap_prep stops the reference. */
+swm_status swm_op_remap_mass_and_tracers(swm_ctx* ctx, int iim, int jjm, int nmass, int nsdm, int ntrace,
+                                         double* mass, double* trace, double dt, const double* uc, double* edgeflux);
 
 /* Halo fill in place, wrap_data::wrap (utilities/layer_2/wrap_data.F:1790-1974):
  * kind 0 = face (cell data), 1 = vrtx (corner data, nspace components first,

@@ -50,6 +50,10 @@ def _load():
     lib.orc_set_threads.argtypes = [ctypes.c_int]
     lib.orc_set_mass_fct.argtypes = [vp] * 10
     lib.orc_set_tracer_fct.argtypes = [vp] * 8
+    lib.orc_set_remap.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
+    lib.orc_set_remap_limiter.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.orc_remap_mass_and_tracers.restype = ctypes.c_int
+    lib.orc_remap_mass_and_tracers.argtypes = [vp] * 5
     return lib
 
 
@@ -63,6 +67,11 @@ def lib():
     return _lib
 
 
+REMAP_FIELDS = ("c", "w", "trans_to_x1", "trans_x1_to_x4", "trans_x4_to_x3", "cl_x4", "cr_x4", "tl_x4", "tr_x4",
+                "bl_x4", "br_x4", "p_x4", "c_x3", "area_inv_x3", "xav", "yav", "xxav", "xyav", "yyav", "gwgt_x3",
+                "garea_inv")
+
+
 class _OrcGrid(ctypes.Structure):
     _fields_ = [("g", ctypes.c_int), ("ntracer", ctypes.c_int), ("dt", ctypes.c_double), ("grav", ctypes.c_double),
                 ("scheme", ctypes.c_int)] + [
@@ -71,6 +80,11 @@ class _OrcGrid(ctypes.Structure):
         ("nface", ctypes.c_int), ("nvrtx", ctypes.c_int), ("nedge", ctypes.c_int)] + [
         (n, ctypes.c_void_p) for n in ("face_dst", "face_src", "vrtx_dst", "vrtx_src", "edge_dst", "edge_src")] + [
         ("lap_w", ctypes.c_void_p), ("tot_area", ctypes.c_double)]
+
+
+class _OrcRemap(ctypes.Structure):
+    _fields_ = [("radius", ctypes.c_double)] + [
+        (n, ctypes.c_void_p) for n in REMAP_FIELDS]
 
 
 def _p(a):
@@ -92,7 +106,19 @@ def grid_arrays_from(grid):
     return d
 
 
-SCHEMES = {"fct": 0, "centered": 1, "upwind": 2}
+SCHEMES = {"fct": 0, "centered": 1, "upwind": 2, "remap": 3}
+
+
+def remap_arrays_from(grid):
+    """The remapping quantities (init_remap) of a swm_mi355x.Grid, with the
+    corners, edge midpoints and corner frames they use, for scheme "remap"."""
+    d = {"radius": grid.radius, "c": grid.array("c"), "w": grid.array("w"), "trans_to_x1": grid.array("trans_to_X1")}
+    for n in ("cl_x4", "cr_x4", "tl_x4", "tr_x4", "bl_x4", "br_x4", "p_x4", "c_x3", "area_inv_x3", "xav", "yav", "xxav",
+              "xyav", "yyav", "gwgt_x3", "garea_inv"):
+        d[n] = grid.array("remap_" + n)
+    d["trans_x1_to_x4"] = grid.array("remap_trans_X1_to_X4")
+    d["trans_x4_to_x3"] = grid.array("remap_trans_X4_to_X3")
+    return d
 
 
 def set_threads(n):
@@ -103,11 +129,13 @@ def set_threads(n):
 
 class Oracle:
     """One FP64 model state on the CPU; arrays as in the reference layout.
-    scheme: the advection of step_dt, "fct" (l_fct), "centered" (l_centered)
-    or "upwind" (l_upwind).  sums: the elliptic solve's global sums in the
+    scheme: the advection of step_dt, "fct" (l_fct), "centered" (l_centered),
+    "upwind" (l_upwind) or "remap" (l_remap; remap = remap_arrays_from(grid),
+    limiter = the tracer limiter's (l_limit_tracers_min, _max)).  sums: the elliptic solve's global sums in the
     reference's memory order ("reference") or compensated ("compensated")."""
 
-    def __init__(self, arrays, ntracer, dt, grav, hs=None, scheme="fct", sums="reference"):
+    def __init__(self, arrays, ntracer, dt, grav, hs=None, scheme="fct", sums="reference", remap=None,
+                 limiter=(1, 1)):
         L = lib()
         self.g = int(arrays["g"])
         self.iim = self.jjm = 2 + 2 ** self.g
@@ -143,6 +171,12 @@ class Oracle:
         self._keep, self._ptrs = keep, ptrs
         self._h = L.orc_create(ctypes.byref(gr))
         L.orc_set_sum_mode(self._h, {"reference": 0, "compensated": 1}[sums])
+        if remap is not None:
+            rm = _OrcRemap()
+            rm.radius = float(remap["radius"])
+            for n in REMAP_FIELDS:
+                setattr(rm, n, f64("remap_" + n, remap[n]))
+            L.orc_set_remap(self._h, ctypes.byref(rm), int(limiter[0]), int(limiter[1]))
 
     def init_state(self, h, u, tracer):
         h = np.asfortranarray(h, dtype=np.float64)
@@ -151,8 +185,11 @@ class Oracle:
         lib().orc_init(self._h, _p(h), _p(u), _p(t))
 
     def step(self, n=1):
-        if lib().orc_step(self._h, n) != 0:
+        st = lib().orc_step(self._h, n)
+        if st == 1:
             raise RuntimeError("oracle: elliptic solve did not converge")
+        if st == 2:
+            raise RuntimeError("oracle: remapping met a negative thickness (or has no remapping quantities)")
 
     def field(self, name):
         s = (self.iim, self.jjm, 10)
@@ -229,6 +266,20 @@ class Oracle:
         out = np.zeros((self.iim, self.jjm, 10), order="F")
         lib().orc_set_tracer_fct(self._h, *[_p(a) for a in args], _p(out))
         return out
+
+    def set_remap_limiter(self, limit_min, limit_max):
+        lib().orc_set_remap_limiter(self._h, int(limit_min), int(limit_max))
+
+    def remap_mass_and_tracers(self, mass, trace, uc):
+        """remap_mass_and_tracers (remap.F:1072-1604) with the model's dt:
+        returns (mass, trace, edgeflux (3, iim, jjm, 10))."""
+        m = np.array(mass, dtype=np.float64, order="F")
+        t = np.array(trace, dtype=np.float64, order="F")
+        u = np.asfortranarray(uc, dtype=np.float64)
+        ef = np.zeros((3, self.iim, self.jjm, 10), order="F")
+        if lib().orc_remap_mass_and_tracers(self._h, _p(m), _p(t), _p(u), _p(ef)) != 0:
+            raise RuntimeError("oracle: remapping met a negative mass")
+        return m, t, ef
 
     def close(self):
         if self._h:

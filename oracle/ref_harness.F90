@@ -16,9 +16,12 @@
 !   wrap   : halo-exchange probes (unique-id fields)  -> outdir/wrap.bin
 !   ops    : one application of each operator         -> outdir/ops.bin
 !   mg     : one V-cycle and one elliptic solve       -> outdir/mg.bin
+!   diss   : init_diss_op, then dissipation of a pseudo-random corner
+!            vector field with norder 1 and 2         -> outdir/diss.bin
 !   run    : <icfile> <dt> <nsteps> <ndump> [scheme] [budgets] [wrestart] [rrestart]
 !            integrates nsteps, dumping the state every ndump steps
-!            (outdir/run.bin); scheme fct | centered | upwind; with the
+!            (outdir/run.bin); scheme fct | centered | upwind | remap
+!            (remap: l_remap with init_remap first, initialize.F:216); with the
 !            word budgets, the reference's compute_budgets runs at step 0
 !            and at every dump (its records go to unit 31, fort.31); with
 !            wrestart, write_swm_restart runs after the last step (file
@@ -32,6 +35,8 @@
 
 program ref_harness
   use budgets, only: compute_budgets
+  use remap_xxx, only: init_remap
+  use swm_dissipation_operator, only: init_diss_op, dissipation
   use swm_restart, only: read_swm_restart, write_swm_restart
   use kinds
   use params_physical
@@ -72,6 +77,8 @@ program ref_harness
     call dump_wrap()
   case ('ops')
     call dump_ops()
+  case ('diss')
+    call dump_diss()
   case ('mg')
     call dump_mg()
   case ('run')
@@ -144,6 +151,7 @@ contains
     write(fu) sub_area, dc, normal, tangent, v_weights, s_weights, l_weights
     write(fu) f, grid_center_mask, grid_corner_mask, polygon_type
     write(fu) grid_center_index, grid_corner_index
+    write(fu) w, trans_to_X1, trans_to_X2
     close(fu)
   end subroutine dump_setup
 
@@ -233,6 +241,21 @@ contains
     close(fu)
   end subroutine dump_ops
 
+  subroutine dump_diss()
+    real(kind=dbl_kind), allocatable :: vin(:,:,:,:,:), o1(:,:,:,:,:), o2(:,:,:,:,:)
+    integer(kind=8) :: st
+    allocate(vin(2,2,iim,jjm,nsdm), o1(2,2,iim,jjm,nsdm), o2(2,2,iim,jjm,nsdm))
+    st = 20261204_8
+    call fill(vin, size(vin), st)
+    vin = 30.0_dbl_kind * vin
+    call init_diss_op(swm_name)
+    call dissipation(swm_name, 1, 2.5e12_dbl_kind, vin, o1)
+    call dissipation(swm_name, 2, 2.5e12_dbl_kind, vin, o2)
+    call open_out('diss.bin')
+    write(fu) vin, o1, o2
+    close(fu)
+  end subroutine dump_diss
+
   subroutine dump_mg()
     real(kind=dbl_kind), allocatable :: beta(:,:,:,:), alpha(:,:,:,:), rhs(:,:,:,:), xo(:,:,:,:)
     integer :: i, j, k
@@ -286,7 +309,8 @@ contains
     l_fct = trim(arg) == 'fct'
     l_centered = trim(arg) == 'centered'
     l_upwind = trim(arg) == 'upwind'
-    l_remap = .false.
+    l_remap = trim(arg) == 'remap'
+    if (l_remap) call init_remap(swm_name)
     n_restart = 100000000
     n_output = 100000000
     n_diagnostics = 100000000

@@ -40,6 +40,8 @@ struct orc_model {
   double w1, w2, w3;
   int n3, n4, n1f, n2f, n3f;
   int sum_mode;          /* elliptic sums: 0 memory order (the reference), 1 compensated */
+  orc_remap rm;          /* remapping quantities (ORC_REMAP) */
+  int rm_owned, rm_limit_min, rm_limit_max;
 };
 
 #define S(i, j, k) ((size_t)((i)-1) + (size_t)iim * ((size_t)((j)-1) + (size_t)jjm * ((k)-1)))
@@ -939,6 +941,394 @@ int orc_elliptic(orc_model* m, int nvar, double* rhs, double* xout) {
   return any ? 1 : 0;
 }
 
+/* ------------------------------------------------ incremental remapping --- */
+/* utilities/layer_2/remap.F:1072-1604 remap_mass_and_tracers with the
+ * arguments get_remap_flux (swm/source/step.F:687-771) passes: one mass
+ * field, ntracer independent tracers, linear integrals, first-order
+ * departure points, tracer gradients limited, no checks. */
+#define RM_EPS 1.0e-10
+static const int rm_si[3][4] = {{0, 0, -1, -1}, {0, -1, -1, 0}, {0, -1, 0, 1}};   /* ishift_p */
+static const int rm_sj[3][4] = {{0, 1, 0, -1}, {0, 0, -1, -1}, {0, -1, -1, 0}};   /* jshift_p */
+
+/* operators.F:691-835 gradient_at_cell_centers (x and y components) */
+static void rm_gradient(const orc_model* m, int k, const double* x, double* gx, double* gy) {
+  const int iim = m->iim, jjm = m->jjm;
+  const double *gw = m->rm.gwgt_x3, *ga = m->rm.garea_inv;
+#define GW(c, nv, i, j) gw[(c) + 3 * ((nv) + 7 * S(i, j, k))]
+  for (int j = 1; j <= jjm; ++j)
+    for (int i = 1; i <= iim; ++i) { gx[S(i, j, k)] = 0.0; gy[S(i, j, k)] = 0.0; }
+  for (int j = 2; j <= jjm - 1; ++j)
+    for (int i = 2; i <= iim - 1; ++i)
+      for (int c = 0; c < 2; ++c) {
+        double* o = c == 0 ? gx : gy;
+        o[S(i, j, k)] = (x[S(i, j, k)] * GW(c, 0, i, j) + x[S(i - 1, j, k)] * GW(c, 1, i, j) +
+                         x[S(i - 1, j - 1, k)] * GW(c, 2, i, j) + x[S(i, j - 1, k)] * GW(c, 3, i, j) +
+                         x[S(i + 1, j, k)] * GW(c, 4, i, j) + x[S(i + 1, j + 1, k)] * GW(c, 5, i, j) +
+                         x[S(i, j + 1, k)] * GW(c, 6, i, j)) * ga[S(i, j, k)];
+      }
+  {
+    const int i = 1, j = jjm - 1;   /* north pole: side 1 missing, (i-1,j-1) in (iim,1) */
+    for (int c = 0; c < 2; ++c) {
+      double* o = c == 0 ? gx : gy;
+      o[S(i, j, k)] = (x[S(i, j, k)] * GW(c, 0, i, j) + x[S(iim, 1, k)] * GW(c, 2, i, j) +
+                       x[S(i, j - 1, k)] * GW(c, 3, i, j) + x[S(i + 1, j, k)] * GW(c, 4, i, j) +
+                       x[S(i + 1, j + 1, k)] * GW(c, 5, i, j) + x[S(i, j + 1, k)] * GW(c, 6, i, j)) * ga[S(i, j, k)];
+    }
+  }
+  {
+    const int i = iim - 1, j = 1;   /* south pole: side 3 missing, (i-1,j-1) in (1,jjm) */
+    for (int c = 0; c < 2; ++c) {
+      double* o = c == 0 ? gx : gy;
+      o[S(i, j, k)] = (x[S(i, j, k)] * GW(c, 0, i, j) + x[S(i - 1, j, k)] * GW(c, 1, i, j) +
+                       x[S(1, jjm, k)] * GW(c, 2, i, j) + x[S(i + 1, j, k)] * GW(c, 4, i, j) +
+                       x[S(i + 1, j + 1, k)] * GW(c, 5, i, j) + x[S(i, j + 1, k)] * GW(c, 6, i, j)) * ga[S(i, j, k)];
+    }
+  }
+#undef GW
+}
+
+static inline double rm_min(double a, double b) { return b < a ? b : a; }
+static inline double rm_max(double a, double b) { return b > a ? b : a; }
+
+/* remap.F:2152-2397 limit_gradient (l_pd false) */
+static void rm_limit(const orc_model* m, int k, const double* phi, const double* phimask, const double* xs,
+                     const double* ys, double* phix, double* phiy) {
+  const int iim = m->iim, jjm = m->jjm;
+  double* pmn = (double*)calloc((size_t)iim * jjm, sizeof(double));
+  double* pmx = (double*)calloc((size_t)iim * jjm, sizeof(double));
+#define L2(i, j) ((size_t)((i)-1) + (size_t)iim * ((j)-1))
+#define NB(ii, jj) (phimask[S(ii, jj, k)] * phi[S(ii, jj, k)] + (1.0 - phimask[S(ii, jj, k)]) * phi[S(i, j, k)])
+  for (int j = 2; j <= jjm - 1; ++j)
+    for (int i = 2; i <= iim - 1; ++i) {
+      const double w[7] = {NB(i - 1, j), NB(i - 1, j - 1), NB(i, j - 1), NB(i + 1, j), NB(i + 1, j + 1), NB(i, j + 1),
+                           phi[S(i, j, k)]};
+      double lo = w[0], hi = w[0];
+      for (int q = 1; q < 7; ++q) { lo = rm_min(lo, w[q]); hi = rm_max(hi, w[q]); }
+      pmn[L2(i, j)] = lo - phi[S(i, j, k)];
+      pmx[L2(i, j)] = hi - phi[S(i, j, k)];
+    }
+  {
+    const int i = 1, j = jjm - 1;
+    const double w[6] = {NB(iim, 1), NB(i, j - 1), NB(i + 1, j), NB(i + 1, j + 1), NB(i, j + 1), phi[S(i, j, k)]};
+    double lo = w[0], hi = w[0];
+    for (int q = 1; q < 6; ++q) { lo = rm_min(lo, w[q]); hi = rm_max(hi, w[q]); }
+    pmn[L2(i, j)] = lo - phi[S(i, j, k)];
+    pmx[L2(i, j)] = hi - phi[S(i, j, k)];
+  }
+  {
+    const int i = iim - 1, j = 1;
+    const double w[6] = {NB(i - 1, j), NB(1, jjm), NB(i + 1, j), NB(i + 1, j + 1), NB(i, j + 1), phi[S(i, j, k)]};
+    double lo = w[0], hi = w[0];
+    for (int q = 1; q < 6; ++q) { lo = rm_min(lo, w[q]); hi = rm_max(hi, w[q]); }
+    pmn[L2(i, j)] = lo - phi[S(i, j, k)];
+    pmx[L2(i, j)] = hi - phi[S(i, j, k)];
+  }
+#undef NB
+  for (int j = 1; j <= jjm - 1; ++j)
+    for (int i = 1; i <= iim - 1; ++i) {
+      const size_t n = S(i, j, k);
+      if (!(phimask[n] > RM_EPS)) continue;
+      const double w0 = -xs[n] * phix[n] - ys[n] * phiy[n];
+      const double* cx = &m->rm.c_x3[12 * n];
+      double qmn = 0.0, qmx = 0.0;
+      for (int nv = 0; nv < 6; ++nv) {
+        const double v = w0 + phix[n] * cx[2 * nv] + phiy[n] * cx[2 * nv + 1];
+        if (nv == 0) { qmn = v; qmx = v; }
+        else { qmn = rm_min(qmn, v); qmx = rm_max(qmx, v); }
+      }
+      const double wk1 = fabs(qmn) > 0.0 ? rm_max(0.0, pmn[L2(i, j)] / qmn) : 1.0;
+      const double wk2 = fabs(qmx) > 0.0 ? rm_max(0.0, pmx[L2(i, j)] / qmx) : 1.0;
+      double wk3 = rm_min(rm_min(1.0, wk1), wk2);
+      if (!m->rm_limit_min) wk3 = rm_min(1.0, wk2);
+      if (!m->rm_limit_max) wk3 = rm_min(1.0, wk1);
+      if (!m->rm_limit_min && !m->rm_limit_max) wk3 = 1.0;
+      phix[n] = wk3 * phix[n];
+      phiy[n] = wk3 * phiy[n];
+    }
+#undef L2
+  free(pmn);
+  free(pmx);
+}
+
+/* remap_mass_and_tracers; mass (slots) and trace[q] (slots) are updated in
+ * place, mflux (3, slots) receives the mass transport across the edges */
+static int rm_remap(orc_model* m, double* mass, double** trace, const double* uc, double* mflux) {
+  const int iim = m->iim, jjm = m->jjm, nt = m->nt;
+  const size_t ns = m->ns;
+  const double a = m->rm.radius, dt = m->dt;
+  int bad = 0;
+  double *mmask = zeros(ns), *mx = zeros(ns), *my = zeros(ns), *mp = zeros(ns), *xmav = zeros(ns),
+         *ymav = zeros(ns), *dp = zeros(6 * ns);
+  double **tx = (double**)calloc(nt ? nt : 1, sizeof(double*)), **ty = (double**)calloc(nt ? nt : 1, sizeof(double*)),
+         **tp = (double**)calloc(nt ? nt : 1, sizeof(double*)), **mtflux = (double**)calloc(nt ? nt : 1, sizeof(double*));
+  for (int q = 0; q < nt; ++q) { tx[q] = zeros(ns); ty[q] = zeros(ns); tp[q] = zeros(ns); mtflux[q] = zeros(3 * ns); }
+
+  /* remap_prep (remap.F:1613-1794) */
+  for (size_t n = 0; n < ns; ++n) {
+    if (mass[n] < -RM_EPS) bad = 1;
+    else if (mass[n] < RM_EPS) {
+      mass[n] = 0.0;
+      mmask[n] = 0.0;
+      for (int q = 0; q < nt; ++q) trace[q][n] = 0.0;
+    } else mmask[n] = 1.0;
+  }
+  if (bad) goto done;
+
+  /* construct_fields (remap.F:1803-2061) */
+  for (int k = 1; k <= 10; ++k) {
+    rm_gradient(m, k, mass, mx, my);
+    for (int q = 0; q < nt; ++q) rm_gradient(m, k, trace[q], tx[q], ty[q]);
+    for (int j = 1; j <= jjm - 1; ++j)
+      for (int i = 1; i <= iim - 1; ++i) {
+        const size_t n = S(i, j, k);
+        if (mass[n] > RM_EPS) {
+          mp[n] = mass[n] - m->rm.xav[n] * mx[n] - m->rm.yav[n] * my[n];
+          const double wk1 = 1.0 / mass[n];
+          xmav[n] = (mx[n] * m->rm.xxav[n] + my[n] * m->rm.xyav[n] + mp[n] * m->rm.xav[n]) * wk1;
+          ymav[n] = (mx[n] * m->rm.xyav[n] + my[n] * m->rm.yyav[n] + mp[n] * m->rm.yav[n]) * wk1;
+        } else {
+          mp[n] = 0.0; xmav[n] = 0.0; ymav[n] = 0.0;
+        }
+      }
+    for (int q = 0; q < nt; ++q) {
+      rm_limit(m, k, trace[q], mmask, xmav, ymav, tx[q], ty[q]);
+      for (int j = 1; j <= jjm - 1; ++j)
+        for (int i = 1; i <= iim - 1; ++i) {
+          const size_t n = S(i, j, k);
+          tp[q][n] = mmask[n] > RM_EPS ? trace[q][n] - tx[q][n] * xmav[n] - ty[q][n] * ymav[n] : 0.0;
+        }
+    }
+  }
+  wrap_face(m, mx); wrap_face(m, my); wrap_face(m, mp);
+  for (int q = 0; q < nt; ++q) { wrap_face(m, tx[q]); wrap_face(m, ty[q]); wrap_face(m, tp[q]); }
+
+  /* departure_points_1order (remap.F:2406-2537), after the velocity wrap */
+  for (int k = 1; k <= 10; ++k)
+    for (int j = 2; j <= jjm - 1; ++j)
+      for (int i = 2; i <= iim - 1; ++i)
+        for (int nc = 0; nc < 2; ++nc) {
+          const size_t cs = 2 * S(i, j, k) + nc;
+          const double u1 = uc[2 * cs], u2 = uc[2 * cs + 1];
+          const double* T = &m->rm.trans_to_x1[9 * cs];
+          const double* c = &m->rm.c[3 * cs];
+          /* the reference's speed test uses the first component twice */
+          const double wk1 = sqrt(u1 * u1 + u1 * u1);
+          if (wk1 > RM_EPS) {
+            double ut[3];
+            for (int r = 0; r < 3; ++r) ut[r] = T[r] * u1 + T[r + 3] * u2;
+            for (int r = 0; r < 3; ++r) dp[3 * cs + r] = a * c[r] - ut[r] * dt;
+          } else {
+            for (int r = 0; r < 3; ++r) dp[3 * cs + r] = a * c[r];
+          }
+        }
+  wrap_vrtx(m, dp, 3);
+
+  /* locate_triangles (remap.F:2903-3624) and transport_integrals
+   * (remap.F:3633-3963), linear integrals, edge by edge */
+  memset(mflux, 0, 3 * ns * sizeof(double));
+  for (int k = 1; k <= 10; ++k)
+    for (int j = 2; j <= jjm - 1; ++j)
+      for (int i = 2; i <= iim - 1; ++i)
+        for (int ne = 0; ne < 3; ++ne) {
+          const size_t e = ne + 3 * S(i, j, k);
+          double fluxsign[4] = {0.0, 0.0, 0.0, 0.0};
+          double tp_[4][4][2];   /* [group][point 0..3][x,y] */
+          int nds[4] = {0, 0, 0, 0};
+          memset(tp_, 0, sizeof tp_);
+          double cl[2] = {m->rm.cl_x4[2 * e], m->rm.cl_x4[2 * e + 1]}, cr[2] = {m->rm.cr_x4[2 * e], m->rm.cr_x4[2 * e + 1]};
+          const double tl[2] = {m->rm.tl_x4[2 * e], m->rm.tl_x4[2 * e + 1]}, tr[2] = {m->rm.tr_x4[2 * e], m->rm.tr_x4[2 * e + 1]};
+          const double bl[2] = {m->rm.bl_x4[2 * e], m->rm.bl_x4[2 * e + 1]}, br[2] = {m->rm.br_x4[2 * e], m->rm.br_x4[2 * e + 1]};
+          double il[2] = {0.0, 0.0}, ir[2] = {0.0, 0.0}, ic[2] = {0.0, 0.0};
+          /* departure points of the edge's ends, relative to its midpoint */
+          size_t csl, csr;
+          int nd0, nc0;
+          if (ne == 0) { csl = 2 * S(i, j, k) + 0; csr = 2 * S(i, j + 1, k) + 1; nd0 = 0; nc0 = 0; }
+          else if (ne == 1) { csl = 2 * S(i, j, k) + 1; csr = 2 * S(i, j, k) + 0; nd0 = 0; nc0 = 1; }
+          else { csl = 2 * S(i + 1, j, k) + 0; csr = 2 * S(i, j, k) + 1; nd0 = 2; nc0 = 1; }
+          const double* wv = &m->rm.w[9 * (2 * S(i, j, k) + nc0) + nd0];   /* w(nd0, s, nc0): stride 3 */
+          double dlx[3], drx[3];
+          for (int s = 0; s < 3; ++s) {
+            dlx[s] = dp[3 * csl + s] - a * wv[3 * s];
+            drx[s] = dp[3 * csr + s] - a * wv[3 * s];
+          }
+          const double* T4 = &m->rm.trans_x1_to_x4[9 * e];
+          double dl[2], dr[2];
+          for (int r = 0; r < 2; ++r) {
+            dl[r] = T4[r] * dlx[0] + T4[r + 3] * dlx[1] + T4[r + 6] * dlx[2];
+            dr[r] = T4[r] * drx[0] + T4[r + 3] * drx[1] + T4[r + 6] * drx[2];
+          }
+          if (fabs(dr[0] - dl[0]) < RM_EPS) dr[0] = dr[0] > dl[0] ? dr[0] + RM_EPS : dr[0] - RM_EPS;
+          if (fabs(dl[0] - cl[0]) < RM_EPS) dl[0] = dl[0] > cl[0] ? dl[0] + RM_EPS : dl[0] - RM_EPS;
+          if (fabs(dr[0] - cr[0]) < RM_EPS) dr[0] = dr[0] > cr[0] ? dr[0] + RM_EPS : dr[0] - RM_EPS;
+          const double mdlr = (dr[1] - dl[1]) / (dr[0] - dl[0]);
+          const double bdlr = dl[1] - mdlr * dl[0];
+          int dlr_int_clr = 0;
+          if (fabs(mdlr) > RM_EPS) {
+            ic[0] = -bdlr / mdlr;
+            if (ic[0] > cl[0] && ic[0] < cr[0]) dlr_int_clr = 1;
+          }
+          const double mtcl = tl[1] / (tl[0] - cl[0]), btcl = -mtcl * cl[0];
+          const double mbcl = bl[1] / (bl[0] - cl[0]), bbcl = -mbcl * cl[0];
+          const double mdcl = dl[1] / (dl[0] - cl[0]);
+          int dl_in_left = 0;
+          if (dl[0] < cl[0] && mdcl < mbcl && mdcl > mtcl) {
+            dl_in_left = 1;
+            il[0] = (btcl - bdlr) / (mdlr - mtcl);
+            if (il[0] < cl[0]) il[1] = mtcl * il[0] + btcl;
+            else { il[0] = (bbcl - bdlr) / (mdlr - mbcl); il[1] = mbcl * il[0] + bbcl; }
+          }
+          const double mtcr = tr[1] / (tr[0] - cr[0]), btcr = -mtcr * cr[0];
+          const double mbcr = br[1] / (br[0] - cr[0]), bbcr = -mbcr * cr[0];
+          const double mdcr = dr[1] / (dr[0] - cr[0]);
+          int dr_in_right = 0;
+          if (dr[0] > cr[0] && mdcr > mbcr && mdcr < mtcr) {
+            dr_in_right = 1;
+            ir[0] = (btcr - bdlr) / (mdlr - mtcr);
+            if (ir[0] > cr[0]) ir[1] = mtcr * ir[0] + btcr;
+            else { ir[0] = (bbcr - bdlr) / (mdlr - mbcr); ir[1] = mbcr * ir[0] + bbcr; }
+          }
+#define SETV(g, v, p) do { tp_[g][v][0] = (p)[0]; tp_[g][v][1] = (p)[1]; } while (0)
+          if (dl_in_left) {
+            SETV(0, 1, cl); SETV(0, 2, dl); SETV(0, 3, il);
+            nds[0] = 4;
+            fluxsign[0] = il[1] > 0.0 ? -1.0 : 1.0;
+            dl[0] = il[0]; dl[1] = il[1];
+          }
+          if (dr_in_right) {
+            SETV(1, 1, cr); SETV(1, 2, dr); SETV(1, 3, ir);
+            nds[1] = 2;
+            fluxsign[1] = ir[1] > 0.0 ? -1.0 : 1.0;
+            dr[0] = ir[0]; dr[1] = ir[1];
+          }
+          if (!dlr_int_clr) {
+            SETV(2, 1, cl); SETV(2, 2, dl); SETV(2, 3, dr);
+            if (dl[1] > 0.0) { nds[2] = 3; fluxsign[2] = -1.0; } else { nds[2] = 1; fluxsign[2] = 1.0; }
+            SETV(3, 1, cl); SETV(3, 2, cr); SETV(3, 3, dr);
+            if (dl[1] > 0.0) { nds[3] = 3; fluxsign[3] = -1.0; } else { nds[3] = 1; fluxsign[3] = 1.0; }
+          } else {
+            SETV(2, 1, cl); SETV(2, 2, dl); SETV(2, 3, ic);
+            if (dl[1] > 0.0) { nds[2] = 3; fluxsign[2] = -1.0; } else { nds[2] = 1; fluxsign[2] = 1.0; }
+            SETV(3, 1, cr); SETV(3, 2, dr); SETV(3, 3, ic);
+            if (dl[1] < 0.0) { nds[3] = 3; fluxsign[3] = -1.0; } else { nds[3] = 1; fluxsign[3] = 1.0; }
+          }
+#undef SETV
+          for (int ng = 0; ng < 4; ++ng) {
+            if (fluxsign[ng] == 0.0) continue;
+            double (*t)[2] = tp_[ng];
+            const double area = fluxsign[ng] * 0.5 *
+                                fabs((t[2][0] - t[1][0]) * (t[3][1] - t[1][1]) - (t[2][1] - t[1][1]) * (t[3][0] - t[1][0]));
+            const int nd = nds[ng] - 1;
+            const double* px = &m->rm.p_x4[2 * (nd + 4 * e)];
+            const double* T = &m->rm.trans_x4_to_x3[4 * (nd + 4 * e)];
+            for (int nv = 1; nv <= 3; ++nv) {
+              const double u0 = t[nv][0] - px[0], u1 = t[nv][1] - px[1];
+              t[nv][0] = T[0] * u0 + T[2] * u1;
+              t[nv][1] = T[1] * u0 + T[3] * u1;
+            }
+            t[0][0] = (1.0 / 3.0) * (t[1][0] + t[2][0] + t[3][0]);
+            t[0][1] = (1.0 / 3.0) * (t[1][1] + t[2][1] + t[3][1]);
+            /* transport_integrals, linear: the source cell's field at the midpoint */
+            const int ip = i + rm_si[ne][nd], jp = j + rm_sj[ne][nd];
+            const size_t sn = S(ip, jp, k);
+            const double m0 = mp[sn] + t[0][0] * mx[sn] + t[0][1] * my[sn];
+            mflux[e] = mflux[e] + area * m0;
+            for (int q = 0; q < nt; ++q) {
+              const double t0 = tp[q][sn] + t[0][0] * tx[q][sn] + t[0][1] * ty[q][sn];
+              mtflux[q][e] = mtflux[q][e] + area * m0 * t0;
+            }
+          }
+        }
+  wrap_edge(m, mflux);
+  for (int q = 0; q < nt; ++q) wrap_edge(m, mtflux[q]);
+
+  /* update_fields (remap.F:3972-4145) */
+  {
+    double* flx = zeros(ns);
+    double** mtold = (double**)calloc(nt ? nt : 1, sizeof(double*));
+    for (int q = 0; q < nt; ++q) {
+      mtold[q] = zeros(ns);
+      for (size_t n = 0; n < ns; ++n) mtold[q][n] = mass[n] * trace[q][n];
+    }
+    div_edge(iim, jjm, m->rm.area_inv_x3, mflux, flx);
+    for (size_t n = 0; n < ns; ++n) {
+      mass[n] = mass[n] - flx[n];
+      if (mass[n] < -RM_EPS) { /* the reference prints a warning and goes on */ }
+      else if (mass[n] < RM_EPS) mass[n] = 0.0;
+    }
+    for (int q = 0; q < nt; ++q) {
+      div_edge(iim, jjm, m->rm.area_inv_x3, mtflux[q], flx);
+      for (size_t n = 0; n < ns; ++n) trace[q][n] = mass[n] > 0.0 ? (mtold[q][n] - flx[n]) / mass[n] : 0.0;
+      free(mtold[q]);
+    }
+    free(mtold);
+    free(flx);
+  }
+done:
+  for (int q = 0; q < nt; ++q) { free(tx[q]); free(ty[q]); free(tp[q]); free(mtflux[q]); }
+  free(tx); free(ty); free(tp); free(mtflux);
+  free(mmask); free(mx); free(my); free(mp); free(xmav); free(ymav); free(dp);
+  return bad;
+}
+
+int orc_remap_mass_and_tracers(orc_model* m, double* mass, double* trace, const double* uc, double* edgeflux) {
+  const size_t ns = m->ns;
+  const int iim = m->iim, jjm = m->jjm, nt = m->nt;
+  if (!m->rm_owned) return 2;
+  double* u = dup(uc, 4 * ns);
+  wrap_vrtx(m, u, 2);   /* remap.F:1391 */
+  double** tr = (double**)calloc(nt ? nt : 1, sizeof(double*));
+  for (int q = 0; q < nt; ++q) {
+    tr[q] = zeros(ns);
+    for (int k = 1; k <= 10; ++k)
+      for (int j = 1; j <= jjm; ++j)
+        for (int i = 1; i <= iim; ++i)
+          tr[q][S(i, j, k)] = trace[(size_t)(i - 1) + iim * ((size_t)(j - 1) + jjm * ((size_t)q + nt * (k - 1)))];
+  }
+  double* mflux = zeros(3 * ns);
+  const int bad = rm_remap(m, mass, tr, u, mflux);
+  if (!bad) {
+    const double dti = 1.0 / m->dt;
+    if (edgeflux)
+      for (size_t n = 0; n < 3 * ns; ++n) edgeflux[n] = mflux[n] * dti;
+    for (int q = 0; q < nt; ++q)
+      for (int k = 1; k <= 10; ++k)
+        for (int j = 1; j <= jjm; ++j)
+          for (int i = 1; i <= iim; ++i)
+            trace[(size_t)(i - 1) + iim * ((size_t)(j - 1) + jjm * ((size_t)q + nt * (k - 1)))] = tr[q][S(i, j, k)];
+  }
+  for (int q = 0; q < nt; ++q) free(tr[q]);
+  free(tr); free(mflux); free(u);
+  return bad ? 2 : 0;
+}
+
+void orc_set_remap_limiter(orc_model* m, int limit_min, int limit_max) {
+  m->rm_limit_min = limit_min;
+  m->rm_limit_max = limit_max;
+}
+
+void orc_set_remap(orc_model* m, const orc_remap* r, int limit_min, int limit_max) {
+  const size_t ns = m->ns;
+  orc_remap* d = &m->rm;
+  if (m->rm_owned) {
+    free((void*)d->c); free((void*)d->w); free((void*)d->trans_to_x1); free((void*)d->trans_x1_to_x4);
+    free((void*)d->trans_x4_to_x3); free((void*)d->cl_x4); free((void*)d->cr_x4); free((void*)d->tl_x4);
+    free((void*)d->tr_x4); free((void*)d->bl_x4); free((void*)d->br_x4); free((void*)d->p_x4); free((void*)d->c_x3);
+    free((void*)d->area_inv_x3); free((void*)d->xav); free((void*)d->yav); free((void*)d->xxav); free((void*)d->xyav);
+    free((void*)d->yyav); free((void*)d->gwgt_x3); free((void*)d->garea_inv);
+  }
+  d->radius = r->radius;
+  d->c = dup(r->c, 6 * ns); d->w = dup(r->w, 18 * ns); d->trans_to_x1 = dup(r->trans_to_x1, 18 * ns);
+  d->trans_x1_to_x4 = dup(r->trans_x1_to_x4, 27 * ns); d->trans_x4_to_x3 = dup(r->trans_x4_to_x3, 48 * ns);
+  d->cl_x4 = dup(r->cl_x4, 6 * ns); d->cr_x4 = dup(r->cr_x4, 6 * ns); d->tl_x4 = dup(r->tl_x4, 6 * ns);
+  d->tr_x4 = dup(r->tr_x4, 6 * ns); d->bl_x4 = dup(r->bl_x4, 6 * ns); d->br_x4 = dup(r->br_x4, 6 * ns);
+  d->p_x4 = dup(r->p_x4, 24 * ns); d->c_x3 = dup(r->c_x3, 12 * ns); d->area_inv_x3 = dup(r->area_inv_x3, ns);
+  d->xav = dup(r->xav, ns); d->yav = dup(r->yav, ns); d->xxav = dup(r->xxav, ns); d->xyav = dup(r->xyav, ns);
+  d->yyav = dup(r->yyav, ns); d->gwgt_x3 = dup(r->gwgt_x3, 21 * ns); d->garea_inv = dup(r->garea_inv, ns);
+  m->rm_owned = 1;
+  m->rm_limit_min = limit_min;
+  m->rm_limit_max = limit_max;
+}
+
 /* --------------------------------------------------------------- model --- */
 orc_model* orc_create(const orc_grid* gr) {
   orc_model* m = (orc_model*)calloc(1, sizeof(orc_model));
@@ -994,6 +1384,14 @@ orc_model* orc_create(const orc_grid* gr) {
 
 void orc_destroy(orc_model* m) {
   if (!m) return;
+  if (m->rm_owned) {
+    orc_remap* d = &m->rm;
+    free((void*)d->c); free((void*)d->w); free((void*)d->trans_to_x1); free((void*)d->trans_x1_to_x4);
+    free((void*)d->trans_x4_to_x3); free((void*)d->cl_x4); free((void*)d->cr_x4); free((void*)d->tl_x4);
+    free((void*)d->tr_x4); free((void*)d->bl_x4); free((void*)d->br_x4); free((void*)d->p_x4); free((void*)d->c_x3);
+    free((void*)d->area_inv_x3); free((void*)d->xav); free((void*)d->yav); free((void*)d->xxav); free((void*)d->xyav);
+    free((void*)d->yyav); free((void*)d->gwgt_x3); free((void*)d->garea_inv);
+  }
   free(m->area); free(m->area_inv); free(m->cai); free(m->sub_area); free(m->dc); free(m->normal);
   free(m->tangent); free(m->vw); free(m->sw); free(m->lw); free(m->f); free(m->hs); free(m->mask);
   free(m->face_dst); free(m->face_src); free(m->vrtx_dst); free(m->vrtx_src); free(m->edge_dst); free(m->edge_src);
@@ -1088,7 +1486,38 @@ static int step_dt(orc_model* m) {
   wrap_vrtx(m, h_c, 1);
   wrap_vrtx(m, eta_c, 1);
 
-  if (m->scheme != ORC_FCT) {
+  if (m->scheme == ORC_REMAP) {
+    /* get_remap_flux (step.F:687-771) */
+    if (!m->rm_owned) { free(h_c); free(eta_c); return 2; }
+    double* hr = dup(m->h[n3], ns);
+    double** trr = (double**)calloc(m->nt > 0 ? m->nt : 1, sizeof(double*));
+    for (int q = 0; q < m->nt; ++q) trr[q] = dup(m->tr[n3][q], ns);
+    double* mflux = zeros(3 * ns);
+    if (rm_remap(m, hr, trr, m->u[n3], mflux)) {
+      for (int q = 0; q < m->nt; ++q) free(trr[q]);
+      free(trr); free(hr); free(mflux); free(h_c); free(eta_c);
+      return 2;
+    }
+    const double dt = m->dt;
+    const double dti_edge = 1.0 / dt;            /* remap.F: real(dbl_kind) */
+    const double dti = (double)(float)(1.0 / dt);   /* step.F:705: declared real(int_kind), i.e. real*4 */
+    for (size_t n = 0; n < 3 * ns; ++n) m->mass_flux[n] = mflux[n] * dti_edge;
+    const int a = m->n3f - 1, b = m->n2f - 1, c = m->n1f - 1;
+    for (size_t n = 0; n < ns; ++n) {
+      m->h_f[a][n] = (hr[n] - m->h[n3][n]) * dti;
+      m->h[n4][n] = m->h[n3][n] + dt * (m->w3 * m->h_f[a][n] + m->w2 * m->h_f[b][n] + m->w1 * m->h_f[c][n]);
+    }
+    for (int q = 0; q < m->nt; ++q) {
+      for (size_t n = 0; n < ns; ++n) {
+        m->tr_f[a][q][n] = (hr[n] * trr[q][n] - m->h[n3][n] * m->tr[n3][q][n]) * dti;
+        m->tr[n4][q][n] = (m->h[n3][n] * m->tr[n3][q][n] +
+                           dt * (m->w3 * m->tr_f[a][q][n] + m->w2 * m->tr_f[b][q][n] + m->w1 * m->tr_f[c][q][n])) /
+                          m->h[n4][n];
+      }
+      free(trr[q]);
+    }
+    free(trr); free(hr); free(mflux);
+  } else if (m->scheme != ORC_FCT) {
     /* get_centered_flux / get_upwind_flux (step.F:495-602) */
     double** tflux = (double**)calloc(m->nt > 0 ? m->nt : 1, sizeof(double*));
     if (m->scheme == ORC_CENTERED) centered_mass_flux(iim, jjm, m->dc, m->normal, h_c, m->u[n3], m->mass_flux);
@@ -1203,7 +1632,8 @@ int orc_step(orc_model* m, int nsteps) {
     m->step_count += 1;
     pointers(m);
     weights(m);
-    if (step_dt(m)) return 1;
+    const int st = step_dt(m);
+    if (st) return st;
     wrap_prognostics(m);
   }
   return 0;

@@ -24,8 +24,9 @@ typedef struct orc_model orc_model;
  * Halo tables: flat (component, slot) destination/source pairs, source -1
  * meaning zero; ncomp 1 (cells), 2 (corners), 3 (edges).
  * lap_w[l-1]: (7, 2+2^l, 2+2^l, 10) multigrid Laplacian weights of level l. */
-/* advection schemes of step_dt (step.F:131-134): l_fct, l_centered, l_upwind */
-enum { ORC_FCT = 0, ORC_CENTERED = 1, ORC_UPWIND = 2 };
+/* advection schemes of step_dt (step.F:131-135): l_fct, l_centered, l_upwind,
+ * l_remap */
+enum { ORC_FCT = 0, ORC_CENTERED = 1, ORC_UPWIND = 2, ORC_REMAP = 3 };
 
 typedef struct {
   int g, ntracer;
@@ -39,7 +40,34 @@ typedef struct {
   double tot_area;   /* total area of the real cells (grid_metrics.F) */
 } orc_grid;
 
+/* Grid quantities of incremental remapping (init_remap, remap.F:176-1063)
+ * in the reference layout, slot (iim,jjm,10) last: corners c (3,2,slot),
+ * edge midpoints w (3,3,2,slot) and trans_to_X1 (3,3,2,slot) of the grid
+ * metrics; trans_X1_to_X4 (3,3,3,slot); trans_X4_to_X3 (2,2,4,3,slot);
+ * cl/cr/tl/tr/bl/br_x4 (2,3,slot); p_x4 (2,4,3,slot); c_x3 (2,6,slot);
+ * gwgt_x3 (3,7,slot); the rest (slot).  radius: the sphere's radius a. */
+typedef struct {
+  double radius;
+  const double *c, *w, *trans_to_x1;
+  const double *trans_x1_to_x4, *trans_x4_to_x3, *cl_x4, *cr_x4, *tl_x4, *tr_x4, *bl_x4, *br_x4, *p_x4;
+  const double *c_x3, *area_inv_x3, *xav, *yav, *xxav, *xyav, *yyav, *gwgt_x3, *garea_inv;
+} orc_remap;
+
 orc_model* orc_create(const orc_grid* grid);
+
+/* The remapping quantities of an ORC_REMAP model (copied), and the tracer
+ * limiter's switches l_limit_tracers_min / l_limit_tracers_max, which
+ * remap_mass_and_tracers (remap.F:1072-1604) leaves without a value when
+ * get_remap_flux (step.F:687-771) does not pass them: 1, 1 is the
+ * two-sided van Leer limit of limit_gradient (remap.F:2152-2397).
+ * orc_set_remap_limiter changes the switches between steps. */
+void orc_set_remap(orc_model* m, const orc_remap* r, int limit_min, int limit_max);
+void orc_set_remap_limiter(orc_model* m, int limit_min, int limit_max);
+/* remap_mass_and_tracers (remap.F:1072-1604) alone, with get_remap_flux's
+ * arguments and the model's dt: mass (slots) and trace (iim,jjm,ntracer,10)
+ * in place, edgeflux (3,slots, may be NULL) = the mass transport over dt.
+ * Returns 2 for a mass below -1e-10 (nothing written). */
+int orc_remap_mass_and_tracers(orc_model* m, double* mass, double* trace, const double* uc, double* edgeflux);
 void orc_destroy(orc_model* m);
 
 /* Initial state from thickness h, surface height hs is part of the grid,
@@ -49,7 +77,9 @@ void orc_destroy(orc_model* m);
 void orc_init(orc_model* m, const double* h, const double* u, const double* tracer);
 
 /* Advances nsteps steps (time_manager, get_pointers, get_weights, step_dt,
- * wrap_prognostics).  Returns 0, or 1 if the elliptic solve did not converge. */
+ * wrap_prognostics).  Returns 0, 1 if the elliptic solve did not converge,
+ * 2 if remapping met a negative thickness (remap_prep stops there,
+ * remap.F:1730-1734) or an ORC_REMAP model has no remapping quantities. */
 int orc_step(orc_model* m, int nsteps);
 
 /* Current-level fields: "h","eta","div","relative","ke","psi","chi" (cell),

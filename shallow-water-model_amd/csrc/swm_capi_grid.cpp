@@ -2,6 +2,8 @@
 #include <cstring>
 #include <exception>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/swm_mi355x.h"
 #include "swm_internal.h"
@@ -76,6 +78,9 @@ ArrayRef lookup(const swm_grid* grid, const char* name) {
   const std::string s(name);
   if (s == "p") D(G.p);
   else if (s == "c") D(G.c);
+  else if (s == "w") D(G.w);
+  else if (s == "trans_to_X1") D(G.trans_to_x1);
+  else if (s == "trans_to_X2") D(G.trans_to_x2);
   else if (s == "area") D(G.area);
   else if (s == "area_inv") D(G.area_inv);
   else if (s == "corner_area") D(G.corner_area);
@@ -101,7 +106,20 @@ ArrayRef lookup(const swm_grid* grid, const char* name) {
   else if (s == "vrtx_src") I(G.vrtx.src);
   else if (s == "edge_dst") I(G.edge.dst);
   else if (s == "edge_src") I(G.edge.src);
-  else if (s.rfind("lap_w", 0) == 0) {
+  else if (s.rfind("remap_", 0) == 0) {
+    const swm::RemapGeometry& R = swm::remap_geometry(G);
+    const std::string t = s.substr(6);
+    const std::pair<const char*, const std::vector<double>*> names[] = {
+        {"vertex_mask", &R.vertex_mask}, {"num_vertices", &R.num_vertices}, {"trans_X1_to_X3", &R.trans_x1_to_x3},
+        {"trans_X1_to_X4", &R.trans_x1_to_x4}, {"trans_X4_to_X3", &R.trans_x4_to_x3}, {"cl_x4", &R.cl_x4},
+        {"cr_x4", &R.cr_x4}, {"tl_x4", &R.tl_x4}, {"tr_x4", &R.tr_x4}, {"bl_x4", &R.bl_x4}, {"br_x4", &R.br_x4},
+        {"p_x4", &R.p_x4}, {"c_x3", &R.c_x3}, {"area_x3", &R.area_x3}, {"area_inv_x3", &R.area_inv_x3},
+        {"xav", &R.xav}, {"yav", &R.yav}, {"xxav", &R.xxav}, {"xyav", &R.xyav}, {"yyav", &R.yyav},
+        {"xxxav", &R.xxxav}, {"xxyav", &R.xxyav}, {"xyyav", &R.xyyav}, {"yyyav", &R.yyyav},
+        {"gwgt_x3", &R.gwgt_x3}, {"garea_inv", &R.garea_inv}};
+    for (const auto& nv : names)
+      if (t == nv.first) D(*nv.second);
+  } else if (s.rfind("lap_w", 0) == 0) {
     const int l = std::atoi(s.c_str() + 5);
     if (l >= 1 && l <= G.g) D(G.levels[l - 1].lap_w);
   }

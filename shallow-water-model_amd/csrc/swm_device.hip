@@ -18,6 +18,7 @@
 #include "swm_kernels.h"
 #include "swm_kernels.hip"
 #include "swm_tlad.hip"
+#include "swm_remap.hip"
 
 namespace {
 
@@ -97,6 +98,20 @@ struct View {
 
 }  // namespace
 
+// incremental remapping (SWM_ADV_REMAP, FP64): the remapping quantities as
+// device planes, the step's fields and transports, the tracer pointer
+// tables and the pinned flag of a negative thickness
+struct RemapDev {
+  void* mem = nullptr;
+  double *cellp = nullptr, *edgep = nullptr, *cornp = nullptr;
+  double *mx = nullptr, *my = nullptr, *mp = nullptr, *tx = nullptr, *ty = nullptr, *tp = nullptr;
+  double *dp = nullptr, *mflux = nullptr, *mtflux = nullptr;
+  double** ptab = nullptr;   // tracers of levels 1, 2, then tendencies of slots 1..3 (nt each)
+  int* flag_host = nullptr;
+  int* flag_dev = nullptr;
+  int limit_min = 1, limit_max = 1;
+};
+
 struct swm_ctx {
   int g = 0, iim = 0, jjm = 0, ns = 0, nt = 0, max_ig = 0;
   double dt = 0, grav = 0;
@@ -174,6 +189,7 @@ struct swm_ctx {
   int halo_shadow = 0;         // every halo fill checked against the host's (swm_set_halo_shadow)
   long long shadow_checks = 0, shadow_bad = 0;
   bool initialized = false;
+  RemapDev rm;
   // elliptic solve settings (reference: mean(r^2) <= 1e-20, 12 cycles) and
   // the mean squared residuals / right-hand sides of the last solve
   double solver_tol = 1.0e-20;
@@ -900,6 +916,48 @@ void wrap_prognostics(swm_ctx* C, View<R>& V, bool after_step = false) {   // st
   if (!after_step) gather<R>(C, C->vrtx, {V.u[0], V.u[0] + 2 * (size_t)ns, V.u[1], V.u[1] + 2 * (size_t)ns});
 }
 
+// get_remap_flux (step.F:687-771): remap_mass_and_tracers with the halo
+// fills of the reference between its parts, then the ABM3 updates
+void remap_step(swm_ctx* C, View<double>& V) {
+  RemapDev& D = C->rm;
+  const int ns = C->ns, nt = C->nt;
+  const unsigned nb = nblk_b(ns, C->blk_step);
+  hipStream_t S = C->stream;
+  const int n3 = C->n3 - 1, n4 = C->n4 - 1;
+  swm::dev::RemapArgs A{C->iim, C->jjm, ns, nt, D.cellp, D.edgep, D.cornp, C->radius, D.limit_min, D.limit_max,
+                        D.flag_dev};
+  const size_t np = (size_t)ns;
+  double* const* tr3 = nt ? D.ptab + (size_t)n3 * nt : nullptr;
+  double* const* tr4 = nt ? D.ptab + (size_t)n4 * nt : nullptr;
+  hipLaunchKernelGGL(swm::dev::k_rm_construct, zgrid(C, nb), dim3(C->blk_step), 0, S, A, V.h[n3],
+                     (const double* const*)tr3, D.mx, D.my, D.mp, D.tx, D.ty, D.tp);
+  std::vector<double*> ff = {D.mx, D.my, D.mp};
+  for (int q = 0; q < nt; ++q) {
+    ff.push_back(D.tx + q * np);
+    ff.push_back(D.ty + q * np);
+    ff.push_back(D.tp + q * np);
+  }
+  gather_list<double>(C, C->face, ff);
+  hipLaunchKernelGGL(swm::dev::k_rm_departure, zgrid(C, nb), dim3(C->blk_step), 0, S, A, C->dt, V.u[n3],
+                     V.u[n3] + 2 * np, D.dp);
+  gather<double>(C, C->vrtx, {D.dp, D.dp + 2 * np, D.dp + 4 * np});
+  hipLaunchKernelGGL(swm::dev::k_rm_transport, zgrid(C, nb), dim3(C->blk_step), 0, S, A, D.dp, D.mx, D.my, D.mp, D.tx,
+                     D.ty, D.tp, D.mflux, D.mtflux);
+  std::vector<double*> ef = {D.mflux};
+  for (int q = 0; q < nt; ++q) ef.push_back(D.mtflux + 3 * q * np);
+  gather_list<double>(C, C->edge, ef);
+  const int a = C->n3f - 1, b = C->n2f - 1, c = C->n1f - 1;
+  double* const* tfa = nt ? D.ptab + (size_t)(2 + a) * nt : nullptr;
+  double* const* tfb = nt ? D.ptab + (size_t)(2 + b) * nt : nullptr;
+  double* const* tfc = nt ? D.ptab + (size_t)(2 + c) * nt : nullptr;
+  // step.F:705 declares 1/dt real*4 for the tendencies; the edge flux of
+  // remap.F:1578 divides in FP64
+  const double dti_step = (double)(float)(1.0 / C->dt);
+  hipLaunchKernelGGL(swm::dev::k_rm_update, zgrid(C, nb), dim3(C->blk_step), 0, S, A, C->dt, 1.0 / C->dt, dti_step,
+                     C->w3, C->w2, C->w1, V.h[n3], (const double* const*)tr3, D.mflux, D.mtflux, V.h_f[a], V.h_f[b],
+                     V.h_f[c], tfa, (const double* const*)tfb, (const double* const*)tfc, V.h[n4], tr4, V.mass_flux);
+}
+
 // step.F:114-212 step_dt (FCT scheme, get_fct_flux step.F:611-678)
 template <class R>
 swm_status step_dt(swm_ctx* C, View<R>& V) {
@@ -917,7 +975,10 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
 
   hipLaunchKernelGGL(swm::dev::k_avg_corners<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, G.sw, V.h[n3], V.hc);
   gather<R>(C, C->vrtx, {V.hc});
-  if (C->advection != SWM_ADV_FCT) {
+  if (C->advection == SWM_ADV_REMAP) {
+    // get_remap_flux (step.F:687-771), FP64 contexts only
+    if constexpr (sizeof(R) == 8) remap_step(C, V);
+  } else if (C->advection != SWM_ADV_FCT) {
     // get_centered_flux / get_upwind_flux (step.F:495-602): the centred or
     // upstream mass flux (the other one lands in scratch), tracer fluxes,
     // then step_surface_height and step_tracer (step.F:221-327)
@@ -1533,6 +1594,82 @@ void launch_budgets(swm_ctx* C, View<R>& V, int nq) {
   sum_final(C, C->part_budget, C->npart, C->sums, nq);
 }
 
+// init_remap's quantities (built on the host) as device planes, the step's
+// remapping fields and the pinned flag (SWM_ADV_REMAP)
+swm_status remap_setup(swm_ctx* C, const swm::Grid& Gh) {
+  const swm::RemapGeometry& R = swm::remap_geometry(Gh);
+  RemapDev& D = C->rm;
+  const size_t ns = (size_t)C->ns, nt = (size_t)C->nt;
+  const size_t ne = 3 * ns, nc = 2 * ns;
+  std::vector<double> cellp((size_t)swm::dev::RM_CELL * ns), edgep((size_t)swm::dev::RM_EDGE * ne),
+      cornp((size_t)swm::dev::RM_CORN * nc);
+  for (size_t n = 0; n < ns; ++n) {
+    for (int c = 0; c < 2; ++c)
+      for (int nv = 0; nv < 7; ++nv) cellp[(size_t)(c * 7 + nv) * ns + n] = R.gwgt_x3[c + 3 * (nv + 7 * n)];
+    cellp[14 * ns + n] = R.garea_inv[n];
+    cellp[15 * ns + n] = R.xav[n];
+    cellp[16 * ns + n] = R.yav[n];
+    cellp[17 * ns + n] = R.xxav[n];
+    cellp[18 * ns + n] = R.xyav[n];
+    cellp[19 * ns + n] = R.yyav[n];
+    for (int nv = 0; nv < 6; ++nv)
+      for (int c = 0; c < 2; ++c) cellp[(size_t)(20 + 2 * nv + c) * ns + n] = R.c_x3[c + 2 * (nv + 6 * n)];
+    cellp[32 * ns + n] = R.area_inv_x3[n];
+    static const int nd0[3] = {0, 0, 2}, nc0[3] = {0, 1, 1};   // the edge's midpoint w(nd0, :, nc0)
+    for (int e = 0; e < 3; ++e) {
+      const size_t h = e + 3 * n, d = (size_t)e * ns + n;
+      auto put = [&](int plane, double v) { edgep[(size_t)plane * ne + d] = v; };
+      for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 3; ++c) put(r * 3 + c, R.trans_x1_to_x4[r + 3 * (c + 3 * h)]);
+      for (int nd = 0; nd < 4; ++nd)
+        for (int r = 0; r < 2; ++r)
+          for (int c = 0; c < 2; ++c) put(6 + 4 * nd + r + 2 * c, R.trans_x4_to_x3[r + 2 * (c + 2 * (nd + 4 * h))]);
+      put(22, R.cl_x4[2 * h]);
+      put(23, R.cr_x4[2 * h]);
+      put(24, R.tl_x4[2 * h]); put(25, R.tl_x4[2 * h + 1]);
+      put(26, R.tr_x4[2 * h]); put(27, R.tr_x4[2 * h + 1]);
+      put(28, R.bl_x4[2 * h]); put(29, R.bl_x4[2 * h + 1]);
+      put(30, R.br_x4[2 * h]); put(31, R.br_x4[2 * h + 1]);
+      for (int nd = 0; nd < 4; ++nd)
+        for (int c = 0; c < 2; ++c) put(32 + 2 * nd + c, R.p_x4[c + 2 * (nd + 4 * h)]);
+      for (int r = 0; r < 3; ++r) put(40 + r, Gh.w[9 * (2 * n + nc0[e]) + nd0[e] + 3 * r]);
+    }
+    for (int ic = 0; ic < 2; ++ic) {
+      const size_t h = 2 * n + ic, d = (size_t)ic * ns + n;
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 2; ++c) cornp[(size_t)(r + 3 * c) * nc + d] = Gh.trans_to_x1[9 * h + r + 3 * c];
+      for (int r = 0; r < 3; ++r) cornp[(size_t)(6 + r) * nc + d] = Gh.c[3 * h + r];
+    }
+  }
+  auto al = [](size_t bytes) { return (bytes + 255) / 256 * 256; };
+  const size_t b_cell = al(cellp.size() * 8), b_edge = al(edgep.size() * 8), b_corn = al(cornp.size() * 8);
+  const size_t b_cf = al(ns * 8), b_tf = al(std::max<size_t>(nt, 1) * ns * 8), b_dp = al(3 * nc * 8),
+               b_mf = al(ne * 8), b_mtf = al(std::max<size_t>(nt, 1) * ne * 8), b_pt = al(std::max<size_t>(5 * nt, 1) * 8);
+  const size_t total = b_cell + b_edge + b_corn + 3 * b_cf + 3 * b_tf + b_dp + b_mf + b_mtf + b_pt;
+  HIP_TRY(hipMalloc(&D.mem, total));
+  char* p = static_cast<char*>(D.mem);
+  auto take = [&](size_t b) { char* q = p; p += b; return q; };
+  D.cellp = (double*)take(b_cell); D.edgep = (double*)take(b_edge); D.cornp = (double*)take(b_corn);
+  D.mx = (double*)take(b_cf); D.my = (double*)take(b_cf); D.mp = (double*)take(b_cf);
+  D.tx = (double*)take(b_tf); D.ty = (double*)take(b_tf); D.tp = (double*)take(b_tf);
+  D.dp = (double*)take(b_dp); D.mflux = (double*)take(b_mf); D.mtflux = (double*)take(b_mtf);
+  D.ptab = (double**)take(b_pt);
+  HIP_TRY(hipMemcpy(D.cellp, cellp.data(), cellp.size() * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(D.edgep, edgep.data(), edgep.size() * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(D.cornp, cornp.data(), cornp.size() * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(D.mx, 0, 3 * b_cf + 3 * b_tf + b_dp + b_mf + b_mtf));
+  std::vector<double*> tab;
+  for (int t = 0; t < 2; ++t)
+    for (size_t q = 0; q < nt; ++q) tab.push_back(C->d.tr[t][q]);
+  for (int t = 0; t < 3; ++t)
+    for (size_t q = 0; q < nt; ++q) tab.push_back(C->d.tr_f[t][q]);
+  if (nt) HIP_TRY(hipMemcpy(D.ptab, tab.data(), tab.size() * sizeof(double*), hipMemcpyHostToDevice));
+  HIP_TRY(hipHostMalloc(&D.flag_host, 64, hipHostMallocDefault));
+  HIP_TRY(hipHostGetDevicePointer((void**)&D.flag_dev, D.flag_host, 0));
+  *D.flag_host = 0;
+  return SWM_OK;
+}
+
 swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
   HIP_TRY(hipSetDevice(C->device));
   HIP_TRY(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
@@ -1693,6 +1830,7 @@ swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
   // zero every member block (the reference initialises its module arrays to 0)
   fill_plain<double>(C, reinterpret_cast<double*>(C->marena.base), member_bytes / 8 * (size_t)C->nmem, 0.0);
   HIP_TRY(hipStreamSynchronize(S));
+  if (C->advection == SWM_ADV_REMAP && (st = remap_setup(C, Gh))) return st;
   HIP_TRY(hipGetLastError());
   return SWM_OK;
 }
@@ -1747,8 +1885,12 @@ swm_status swm_self_test(int device) {
 swm_status swm_create_members(const swm_grid* grid, const swm_config* cfg, int nmem, swm_ctx** out) {
   if (!grid || !cfg || !out || nmem < 1 || nmem > 4096 || cfg->ntracer < 0 || cfg->ntracer > 64 || !(cfg->dt > 0.0) ||
       (cfg->precision != SWM_FP64 && cfg->precision != SWM_MIXED) || cfg->advection < SWM_ADV_FCT ||
-      cfg->advection > SWM_ADV_UPWIND) {
+      cfg->advection > SWM_ADV_REMAP) {
     swm::set_error("swm_create: bad argument");
+    return SWM_ERR_ARG;
+  }
+  if (cfg->advection == SWM_ADV_REMAP && (cfg->precision != SWM_FP64 || nmem != 1)) {
+    swm::set_error("swm_create: incremental remapping runs in single-member FP64 contexts");
     return SWM_ERR_ARG;
   }
   int ndev = 0;
@@ -1793,6 +1935,8 @@ void swm_destroy(swm_ctx* C) {
   if (C->arena.base) (void)hipFree(C->arena.base);
   if (C->marena.base) (void)hipFree(C->marena.base);
   if (C->host_sums) (void)hipHostFree(C->host_sums);
+  if (C->rm.mem) (void)hipFree(C->rm.mem);
+  if (C->rm.flag_host) (void)hipHostFree(C->rm.flag_host);
   swm::tlad_free(C);
   if (C->stream) (void)hipStreamDestroy(C->stream);
   delete C;
@@ -2011,9 +2155,16 @@ swm_status swm_step(swm_ctx* C, int nsteps) {
   C->kernel_pair_launches = 0;
   HIP_TRY(hipEventRecord(C->ev_a, C->stream));
   const long long bad0 = C->shadow_bad;
+  auto remap_stop = [&]() {
+    swm::set_error("remapping: a thickness below -1e-10 at step " + std::to_string(C->step_count) +
+                   " (remap_prep stops the reference there)");
+    *C->rm.flag_host = 0;
+    return SWM_ERR_STATE;
+  };
   for (int n = 0; n < nsteps; ++n) {
     swm_status st = swm::step_once(C);
     if (st) return st;
+    if (C->rm.flag_host && *(volatile int*)C->rm.flag_host) return remap_stop();
     if (C->shadow_bad != bad0) {
       swm::set_error("halo shadow check: " + std::to_string(C->shadow_bad - bad0) + " values of the device's halo "
                      "fills differ from the host's fill with the same tables (step " + std::to_string(C->step_count) +
@@ -2027,6 +2178,17 @@ swm_status swm_step(swm_ctx* C, int nsteps) {
   HIP_TRY(hipEventElapsedTime(&ms, C->ev_a, C->ev_b));
   C->last_step_ms = ms;
   HIP_TRY(hipGetLastError());
+  if (C->rm.flag_host && *(volatile int*)C->rm.flag_host) return remap_stop();
+  return SWM_OK;
+}
+
+swm_status swm_set_remap_limiter(swm_ctx* C, int limit_min, int limit_max) {
+  if (!C || C->advection != SWM_ADV_REMAP) {
+    swm::set_error("swm_set_remap_limiter: needs a context with incremental remapping");
+    return SWM_ERR_ARG;
+  }
+  C->rm.limit_min = limit_min ? 1 : 0;
+  C->rm.limit_max = limit_max ? 1 : 0;
   return SWM_OK;
 }
 
@@ -2797,6 +2959,87 @@ swm_status swm_op_set_tracer_fct(swm_ctx* C, int iim, int jjm, int nlm, int nsdm
   if (trc_flux_horz) {
     hipLaunchKernelGGL(swm::dev::k_axpy, dim3(nblk(3 * (size_t)C->ns)), dim3(BLK), 0, S, 3 * (size_t)C->ns, 1.0, anti, f1);
     if ((st = io.get(trc_flux_horz, f1, 3))) return st;
+  }
+  return SWM_OK;
+}
+
+swm_status swm_op_remap_mass_and_tracers(swm_ctx* C, int iim, int jjm, int nmass, int nsdm, int ntrace, double* mass,
+                                         double* trace, double dt, const double* uc, double* edgeflux) {
+  OP_BEGIN(C);
+  OP_ARGS(mass, uc);
+  if (ntrace < 0 || (ntrace > 0 && !trace) || !(dt > 0.0)) {
+    swm::set_error("swm_op_remap_mass_and_tracers: bad argument");
+    return SWM_ERR_ARG;
+  }
+  if (C->advection != SWM_ADV_REMAP || !C->rm.mem) {
+    swm::set_error("swm_op_remap_mass_and_tracers: needs a context made with SWM_ADV_REMAP");
+    return SWM_ERR_ARG;
+  }
+  if (iim != C->iim || jjm != C->jjm || nsdm != 10 || nmass != 1) {
+    swm::set_error("swm_op_remap_mass_and_tracers: the whole grid (iim, jjm, 10 panels) and one mass field");
+    return SWM_ERR_DIMS;
+  }
+  const size_t ns = (size_t)C->ns;
+  const int nq = ntrace > 0 ? ntrace : 1;
+  OpIO io{C, ns, {}, {}};
+  double* h = io.put(mass, 1, 1);
+  double* u = io.put(uc, 4, 4, kCornerVec);
+  double *tr = io.dev(nq), *ptab = io.dev(1), *mx = io.zeros(1), *my = io.zeros(1), *mp = io.zeros(1),
+         *tx = io.zeros(nq), *ty = io.zeros(nq), *tp = io.zeros(nq), *dp = io.zeros(6), *mflux = io.zeros(3),
+         *mtflux = io.zeros(3 * nq), *hout = io.zeros(1), *tout = io.zeros(nq), *ef = io.zeros(3);
+  OP_BUF(h && u && tr && ptab && mx && my && mp && tx && ty && tp && dp && mflux && mtflux && hout && tout && ef);
+  // tracers (iim, jjm, ntrace, 1, 10) -> planes; the plane pointer table
+  std::vector<double> planes((size_t)ntrace * ns);
+  const size_t pl = (size_t)iim * jjm;
+  for (int k = 0; k < 10; ++k)
+    for (int q = 0; q < ntrace; ++q)
+      std::memcpy(&planes[(size_t)q * ns + k * pl], &trace[((size_t)q + (size_t)ntrace * k) * pl], pl * sizeof(double));
+  std::vector<double*> tab(nq);
+  for (int q = 0; q < nq; ++q) tab[q] = tr + (size_t)q * ns;
+  if (ntrace) HIP_TRY(hipMemcpy(tr, planes.data(), planes.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ptab, tab.data(), tab.size() * sizeof(double*), hipMemcpyHostToDevice));
+  RemapDev& D = C->rm;
+  *D.flag_host = 0;
+  swm::dev::RemapArgs A{C->iim, C->jjm, (int)ns, ntrace, D.cellp, D.edgep, D.cornp, C->radius, D.limit_min,
+                        D.limit_max, D.flag_dev};
+  const unsigned nb = nblk_b(ns, C->blk_step);
+  hipStream_t S = C->stream;
+  const double* const* t3 = reinterpret_cast<const double* const*>(ptab);
+  // remap.F:1340-1580 as get_remap_flux calls it: prep and fields, halo
+  // fills, departure points (after the velocity's own halo fill), triangles
+  // and transports, their halo fills, the updated fields and the edge flux
+  gather<double>(C, C->vrtx, {u, u + 2 * ns});
+  hipLaunchKernelGGL(swm::dev::k_rm_construct, zgrid(C, nb), dim3(C->blk_step), 0, S, A, h, t3, mx, my, mp, tx, ty, tp);
+  std::vector<double*> ff = {mx, my, mp};
+  for (int q = 0; q < ntrace; ++q) {
+    ff.push_back(tx + q * ns);
+    ff.push_back(ty + q * ns);
+    ff.push_back(tp + q * ns);
+  }
+  gather_list<double>(C, C->face, ff);
+  hipLaunchKernelGGL(swm::dev::k_rm_departure, zgrid(C, nb), dim3(C->blk_step), 0, S, A, dt, u, u + 2 * ns, dp);
+  gather<double>(C, C->vrtx, {dp, dp + 2 * ns, dp + 4 * ns});
+  hipLaunchKernelGGL(swm::dev::k_rm_transport, zgrid(C, nb), dim3(C->blk_step), 0, S, A, dp, mx, my, mp, tx, ty, tp,
+                     mflux, mtflux);
+  std::vector<double*> efl = {mflux};
+  for (int q = 0; q < ntrace; ++q) efl.push_back(mtflux + 3 * q * ns);
+  gather_list<double>(C, C->edge, efl);
+  hipLaunchKernelGGL(swm::dev::k_rm_fields, zgrid(C, nb), dim3(C->blk_step), 0, S, A, 1.0 / dt, h, t3, mflux, mtflux,
+                     hout, tout, ef);
+  HIP_TRY(hipStreamSynchronize(S));
+  if (*(volatile int*)D.flag_host) {
+    *D.flag_host = 0;
+    swm::set_error("swm_op_remap_mass_and_tracers: a mass below -1e-10 (remap_prep stops the reference there)");
+    return SWM_ERR_STATE;
+  }
+  swm_status st;
+  if ((st = io.get(mass, hout, 1))) return st;
+  if (edgeflux && (st = io.get(edgeflux, ef, 3))) return st;
+  if (ntrace) {
+    HIP_TRY(hipMemcpy(planes.data(), tout, planes.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 10; ++k)
+      for (int q = 0; q < ntrace; ++q)
+        std::memcpy(&trace[((size_t)q + (size_t)ntrace * k) * pl], &planes[(size_t)q * ns + k * pl], pl * sizeof(double));
   }
   return SWM_OK;
 }
@@ -3587,6 +3830,10 @@ swm_status window_check(swm_ctx* C, const char* who) {
   }
   if (C->precision != SWM_FP64) {
     swm::set_error(std::string(who) + ": the linearised model is the FP64 step");
+    return SWM_ERR_ARG;
+  }
+  if (C->advection == SWM_ADV_REMAP) {
+    swm::set_error(std::string(who) + ": the linearised model has no incremental remapping");
     return SWM_ERR_ARG;
   }
   return SWM_OK;

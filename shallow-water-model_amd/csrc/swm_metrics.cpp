@@ -139,7 +139,7 @@ std::vector<double> list_to_global(const double* plist, int g, int nv) {
 
 struct LevelMetrics {
   std::vector<double> c, area, area_inv, corner_area, corner_area_inv, sub_area, dc, normal,
-      tangent, v_weights, s_weights, l_weights, polygon_type;
+      tangent, v_weights, s_weights, l_weights, polygon_type, w, trans_to_x2, trans_to_x1;
   double tot_area = 0.0;
 };
 
@@ -156,6 +156,9 @@ void metrics_level(int g, const std::vector<double>& p, double radius, LevelMetr
   M.normal.assign(3 * 3 * 2 * ns, 0.0); M.tangent.assign(3 * 3 * 2 * ns, 0.0);
   M.v_weights.assign(3 * 2 * 2 * ns, 0.0); M.s_weights.assign(3 * 2 * ns, 0.0);
   M.l_weights.assign(7 * ns, 0.0);
+  M.w.assign(3 * 3 * 2 * ns, 0.0);
+  M.trans_to_x2.assign(3 * 3 * 2 * ns, 0.0);
+  M.trans_to_x1.assign(3 * 3 * 2 * ns, 0.0);
   M.polygon_type = polygon_types(g);
   const std::vector<double>& pt = M.polygon_type;
 
@@ -214,10 +217,19 @@ void metrics_level(int g, const std::vector<double>& p, double radius, LevelMetr
           tangent_plane(h3, zhat, h2);
           cross3(h2, h3, h1);
           const double* X2[3] = {h1, h2, h3};   // rows; the transform to X2 is X2 itself
+          // trans_to_X2 = X2 X1^T and trans_to_X1 = X1 X2^T with X1 the
+          // identity: the entries of X2, each a sum of one product and two
+          // zero products (a zero of either sign ends as +0)
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+              M.trans_to_x2[9 * cs + a + 3 * b] = X2[a][b] + 0.0;
+              M.trans_to_x1[9 * cs + a + 3 * b] = X2[b][a] + 0.0;
+            }
           double r[3][3];
           for (int kk = 0; kk < 3; ++kk) {
             const int l = (kk + 1) % 3;
             for (int s = 0; s < 3; ++s) r[kk][s] = 0.5 * (t[kk][s] + t[l][s]);
+            for (int s = 0; s < 3; ++s) M.w[9 * cs + kk + 3 * s] = r[kk][s];   // w(kk, s, corner)
             double gv[3] = {c[0] - r[kk][0], c[1] - r[kk][1], c[2] - r[kk][2]};
             unit3(gv);
             double* tg = &M.tangent[3 * (3 * cs) + 0];
@@ -373,7 +385,16 @@ void build_grid(Grid& G, int g, const double* p_list, double radius, double omeg
   LevelMetrics M;
   metrics_level(g, G.p, radius, M);
   for (int s = 0; s < 3; ++s) apply_halo_strided(G.vrtx, M.c.data(), 3, s);  // corners (grid_metrics.F:151)
+  // the corner frames are wrapped (grid_metrics_generator.F:520-524), the
+  // edge midpoints are not
+  for (int s = 0; s < 9; ++s) {
+    apply_halo_strided(G.vrtx, M.trans_to_x1.data(), 9, s);
+    apply_halo_strided(G.vrtx, M.trans_to_x2.data(), 9, s);
+  }
   G.c.swap(M.c);
+  G.w.swap(M.w);
+  G.trans_to_x1.swap(M.trans_to_x1);
+  G.trans_to_x2.swap(M.trans_to_x2);
   G.area.swap(M.area); G.area_inv.swap(M.area_inv);
   G.corner_area.swap(M.corner_area); G.corner_area_inv.swap(M.corner_area_inv);
   G.sub_area.swap(M.sub_area); G.dc.swap(M.dc);

@@ -7,6 +7,7 @@
 // multi-component ones into separate planes.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -27,6 +28,29 @@ struct Level {   // one multigrid level (the finest level equals the model grid)
   HaloTable face;                   // general face halo table of this level
 };
 
+// Grid quantities of incremental remapping (init_remap, remap.F:176-1063),
+// in the reference layout with the slot (iim, jjm, 10) last: X3 is the local
+// east/north/up frame at cell centres, X4 the tangent/outward-normal/up frame
+// at cell edges (origin at the edge midpoint), distances in metres.
+struct RemapGeometry {
+  std::vector<double> vertex_mask;      // (6, slot) 1 for real vertices
+  std::vector<double> num_vertices;     // (slot)
+  std::vector<double> trans_x1_to_x3;   // (3, 3, slot)
+  std::vector<double> trans_x1_to_x4;   // (3, 3, 3 edges, slot)
+  std::vector<double> trans_x4_to_x3;   // (2, 2, 4 cells, 3 edges, slot)
+  std::vector<double> cl_x4, cr_x4, tl_x4, tr_x4, bl_x4, br_x4;   // (2, 3 edges, slot)
+  std::vector<double> p_x4;             // (2, 4 cells, 3 edges, slot)
+  std::vector<double> c_x3;             // (2, 6, slot) corners relative to the centre
+  std::vector<double> area_x3, area_inv_x3, xav, yav, xxav, xyav, yyav, xxxav, xxyav, xyyav, yyyav;  // (slot)
+  std::vector<double> gwgt_x3;          // (3, 0:6, slot) cell-centre gradient weights
+  std::vector<double> garea_inv;        // (slot)
+};
+
+// the four cells around edge ne (0-based) of cell (i, j): offsets of cell
+// nd = 0..3 (remap.F ishift_p / jshift_p)
+extern const int kRemapShiftI[3][4];
+extern const int kRemapShiftJ[3][4];
+
 struct Grid {
   int g = 0, iim = 0, jjm = 0, nsdm = 10, im = 0, jm = 0, max_ig = 0;
   double radius = 6.37122e6, omega = 7.292e-5;
@@ -34,6 +58,9 @@ struct Grid {
   std::vector<double> p_list;          // (3, max_ig) cell centres, global order
   std::vector<double> p;               // (3, iim, jjm, 10)
   std::vector<double> c;               // (3, 2, iim, jjm, 10) Voronoi corners
+  std::vector<double> w;               // (3, 3, 2, iim, jjm, 10) edge midpoints (ndata, xyz, corner)
+  std::vector<double> trans_to_x2;     // (3, 3, 2, iim, jjm, 10) X1 -> local east/north/up at corners
+  std::vector<double> trans_to_x1;     // (3, 3, 2, iim, jjm, 10) its inverse (transpose)
   // metrics (all in metres after scaling by the radius)
   std::vector<double> area, area_inv;                // (iim, jjm, 10)
   std::vector<double> corner_area, corner_area_inv;  // (2, iim, jjm, 10)
@@ -53,7 +80,14 @@ struct Grid {
   HaloTable face, vrtx, edge;
   // multigrid hierarchy, levels 1..g at index 0..g-1
   std::vector<Level> levels;
+  // remapping quantities, built on first use (remap_geometry)
+  mutable std::shared_ptr<const RemapGeometry> remap;
 };
+
+// init_remap for the grid.
+void build_remap_geometry(const Grid& G, RemapGeometry& R);
+// The grid's remapping quantities, built on the first call (thread-safe).
+const RemapGeometry& remap_geometry(const Grid& G);
 
 // Generates the unit-sphere cell centres of the recursively bisected
 // icosahedron at level g (10*4^g+2 points) in the reference's global order.

@@ -14,9 +14,10 @@
 // run in a reference run directory (data/, restarts/).  The quick-plot
 // netCDF file of swm_qp_module.F (qp_output/hswm_d000000p000.g2.nc, the
 // directory made when absent) holds the initial state and every n_output-th
-// step, as initialize.F:153-159, :275-276 and driver.F:133-135 write it.  Not
-// supported, as outside the library: incremental remapping (l_remap) and the
-// OpenDX files of l_dx (dx_write_utility.F), reported once and skipped.
+// step, as initialize.F:153-159, :275-276 and driver.F:133-135 write it.
+// l_remap selects incremental remapping (FP64).  Not supported, as outside
+// the library: the OpenDX files of l_dx (dx_write_utility.F), reported once
+// and skipped.
 #include <dirent.h>
 #include <sys/stat.h>
 
@@ -108,8 +109,9 @@ int main(int argc, char** argv) {
 
   swm_namelist nl;
   check(swm_read_namelist("data/namelist", &nl), "reading data/namelist");
-  if (nl.l_remap) die("l_remap: incremental remapping is not part of the device library");
-  if (nl.l_upwind + nl.l_centered + nl.l_fct != 1) die("exactly one of l_upwind, l_centered, l_fct must be set");
+  if (nl.l_upwind + nl.l_centered + nl.l_fct + nl.l_remap != 1)
+    die("exactly one of l_upwind, l_centered, l_fct, l_remap must be set");
+  if (nl.l_remap && precision != SWM_FP64) die("l_remap: incremental remapping runs in FP64");
   const double tau_end = 24.0 * nl.tau_end_day + nl.tau_end_hour - 0.001;   // model_control.F:124
   std::printf("\n  input : tau_end        = %.15g\n  input : tau_end_day    = %.15g\n"
               "  input : tau_end_hour   = %.15g\n  input : time step      = %.15g\n\n",
@@ -152,7 +154,7 @@ int main(int argc, char** argv) {
   cfg.grav = grav;
   cfg.precision = precision;
   cfg.device = device;
-  cfg.advection = nl.l_fct ? SWM_ADV_FCT : nl.l_centered ? SWM_ADV_CENTERED : SWM_ADV_UPWIND;
+  cfg.advection = nl.l_fct ? SWM_ADV_FCT : nl.l_centered ? SWM_ADV_CENTERED : nl.l_remap ? SWM_ADV_REMAP : SWM_ADV_UPWIND;
   swm_ctx* ctx = nullptr;
   check(swm_create(grid, &cfg, &ctx), "swm_create");
 

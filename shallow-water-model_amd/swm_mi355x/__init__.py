@@ -28,8 +28,8 @@ SWM_MIXED = 1
 # swm_set_fusion flags
 FUSE_INJECT_SWEEP = 1
 FUSE_ALL = 1
-# advection of step_dt: l_fct, l_centered, l_upwind
-ADVECTION = {"fct": 0, "centered": 1, "upwind": 2}
+# advection of step_dt: l_fct, l_centered, l_upwind, l_remap
+ADVECTION = {"fct": 0, "centered": 1, "upwind": 2, "remap": 3}
 
 # physical constants of the reference build (utilities/layer_1/params_physical.F:45-52)
 REF_RADIUS = 6.37122e6
@@ -140,6 +140,9 @@ _sig("swm_create_members", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Confi
                                           ctypes.POINTER(ctypes.c_void_p)])
 _sig("swm_members", ctypes.c_int, [ctypes.c_void_p])
 _sig("swm_ctx_layout", ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p, _c_int_p])
+_sig("swm_set_remap_limiter", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
+_sig("swm_op_remap_mass_and_tracers", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 5 +
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p])
 _sig("swm_qp_name", ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, ctypes.c_char_p, ctypes.c_int])
 _sig("swm_qp_create", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_longlong, ctypes.c_char_p,
                                      ctypes.POINTER(ctypes.c_void_p)])
@@ -358,10 +361,17 @@ class QpFile:
 
 # shapes of the named grid arrays, in units of (iim, jjm, 10) slots
 _GRID_SHAPES = {
-    "p": (3,), "c": (3, 2), "area": (), "area_inv": (), "corner_area": (2,), "corner_area_inv": (2,),
+    "p": (3,), "c": (3, 2), "w": (3, 3, 2), "trans_to_X1": (3, 3, 2), "trans_to_X2": (3, 3, 2),
+    "area": (), "area_inv": (), "corner_area": (2,), "corner_area_inv": (2,),
     "sub_area": (3, 2), "dc": (3, 2), "normal": (3, 3, 2), "tangent": (3, 3, 2), "v_weights": (3, 2, 2),
     "s_weights": (3, 2), "l_weights": (7,), "f": (), "grid_center_mask": (), "grid_corner_mask": (2,),
     "polygon_type": (), "grid_center_index": (), "grid_corner_index": (2,),
+    "remap_vertex_mask": (6,), "remap_num_vertices": (), "remap_trans_X1_to_X3": (3, 3),
+    "remap_trans_X1_to_X4": (3, 3, 3), "remap_trans_X4_to_X3": (2, 2, 4, 3), "remap_cl_x4": (2, 3),
+    "remap_cr_x4": (2, 3), "remap_tl_x4": (2, 3), "remap_tr_x4": (2, 3), "remap_bl_x4": (2, 3), "remap_br_x4": (2, 3),
+    "remap_p_x4": (2, 4, 3), "remap_c_x3": (2, 6), "remap_area_x3": (), "remap_area_inv_x3": (), "remap_xav": (),
+    "remap_yav": (), "remap_xxav": (), "remap_xyav": (), "remap_yyav": (), "remap_xxxav": (), "remap_xxyav": (),
+    "remap_xyyav": (), "remap_yyyav": (), "remap_gwgt_x3": (3, 7), "remap_garea_inv": (),
 }
 _INT_ARRAYS = {"grid_center_index", "grid_corner_index", "face_dst", "face_src", "vrtx_dst", "vrtx_src",
                "edge_dst", "edge_src"}
@@ -799,6 +809,25 @@ class Model:
     def time_seconds(self):
         """Model time in whole seconds (nsecond_dyn)."""
         return lib.swm_time_seconds(self._h)
+
+    def set_remap_limiter(self, limit_min=True, limit_max=True):
+        """Incremental remapping: the tracer limiter's l_limit_tracers_min /
+        _max switches (swm_set_remap_limiter; both on by default)."""
+        _check(lib.swm_set_remap_limiter(self._h, int(bool(limit_min)), int(bool(limit_max))))
+
+    def remap_mass_and_tracers(self, mass, trace, dt, uc):
+        """remap_mass_and_tracers (remap.F:1072-1604) on host arrays, mass
+        (iim, jjm, 10), trace (iim, jjm, ntrace, 10), uc (2, 2, iim, jjm, 10):
+        returns (mass, trace, edgeflux (3, iim, jjm, 10))."""
+        s = (self.iim, self.jjm, 10)
+        m = np.array(_f64(mass, s), order="F")
+        nt = np.asarray(trace).shape[2]
+        t = np.array(_f64(trace, (self.iim, self.jjm, nt, 10)), order="F")
+        u = _f64(uc, (2, 2) + s)
+        ef = np.zeros((3,) + s, order="F")
+        _check(lib.swm_op_remap_mass_and_tracers(self._h, self.iim, self.jjm, 1, 10, nt, _ptr(m), _ptr(t), float(dt),
+                                                 _ptr(u), _ptr(ef)))
+        return m, t, ef
 
     def write_restart(self, path=None):
         """write_swm_restart: the current state and model time; path None uses

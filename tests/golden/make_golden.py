@@ -9,14 +9,19 @@ and stores its outputs as compressed npz fixtures:
                    is fixed at level 5), 20-decimal text as printed
   wrap_g<g>.npz    halo gather tables probed from the reference wrap
                    (g = 1..8): every slot holds a unique id before the wrap
-  setup_g<g>.npz   metrics, masks and index maps (g = 3 generated grid,
-                   g = 4 the shipped optimised grid, whose points are stored)
+  setup_g<g>.npz   metrics, masks, index maps, edge midpoints and corner
+                   frames (g = 3 generated grid, g = 4 the shipped optimised
+                   grid, whose points are stored)
   ops_g3.npz       one application of each operator on pseudo-random input
   mg_g4.npz        one V-cycle and one elliptic solve
   run_g<g>.npz     initial state and the state after selected steps of the
                    full model (g = 3, dt = 1800 s; g = 4, dt = 900 s), 5 tracers
   run_g4_<scheme>.npz  the same with the centred (l_centered) and upwind
                    (l_upwind) advection of step_dt (g = 4, dt = 400 s)
+  run_g<g>_remap.npz  incremental remapping (l_remap) runs: g = 3 (dt = 900 s)
+                   and g = 4 (the shipped grid, dt = 400 s), 12 steps
+  diss_g4.npz      the dissipation operator (init_diss_op, dissipation with
+                   norder 1 and 2) of a pseudo-random corner vector field
   budgets_g4.npz   the reference's compute_budgets (swm/source/budgets.F) at
                    steps 0..12 of the FCT run at g = 4, dt = 400 s: its 14
                    real*4 output records (unit 31) and the 9-digit values it
@@ -35,6 +40,8 @@ and stores its outputs as compressed npz fixtures:
     python tests/golden/make_golden.py            # everything
     python tests/golden/make_golden.py schemes    # only run_g4_<scheme>.npz and budgets_g4.npz
     python tests/golden/make_golden.py restart    # only restart_g3.npz
+    python tests/golden/make_golden.py setup      # only setup_g3.npz and setup_g4.npz
+    python tests/golden/make_golden.py remap      # only run_g<g>_remap.npz and diss_g4.npz
     python tests/golden/make_golden.py swtc5      # only swtc5_g6.npz and namelists/
 """
 import os
@@ -96,6 +103,8 @@ def read_setup(path, g):
         out[name] = take(k * ns)
     out["grid_center_index"] = take(ns, "<i4")
     out["grid_corner_index"] = take(2 * ns, "<i4")
+    for name in ("w", "trans_to_X1", "trans_to_X2"):
+        out[name] = take(18 * ns)
     return out
 
 
@@ -302,9 +311,69 @@ def swtc5_values():
     print("swtc5", keep["mass0"], keep["s1_relative"], flush=True)
 
 
+def remap_runs():
+    """Incremental remapping (l_remap, init_remap first) runs of the compiled
+    reference: g = 3 (generated grid, dt = 900 s) and g = 4 (the shipped grid,
+    dt = 400 s), 12 steps, 5 tracers; and the dissipation operator
+    (swm_dissipation_operator.F) on the shipped g = 4 grid."""
+    import swm_mi355x as S
+    from swm_mi355x import ics
+    nt, nsteps = 5, 12
+    shipped4 = np.loadtxt(os.path.join(REF, "swm", "run.0002562", "data", "grid_centers_0002562")).T
+    for g, dt in ((3, 900.0), (4, 400.0)):
+        exe = harness(g)
+        points = shipped4 if g == 4 else S.grid_points(g)
+        d = rundir(g, points)
+        grid = S.Grid(g, p_list=points)
+        ic = ics.synthetic_state(grid, ntracer=nt, lmax=8, seed_index=g)
+        with open(os.path.join(d, "ic.bin"), "wb") as f:
+            for k in ("h", "hs", "u", "tracer"):
+                f.write(np.asfortranarray(ic[k]).tobytes(order="F"))
+        rc = run(exe, d, "run", "out", "ic.bin", repr(dt), str(nsteps), "1", "remap", check=False)
+        states = read_run(os.path.join(d, "out", "run.bin"), g, nt)
+        assert rc == 0 and max(states) == nsteps, (g, rc, max(states))
+        keep = {f"ic_{k}": np.asarray(v) for k, v in ic.items()}
+        keep["dt"] = np.array(dt)
+        keep["nsteps"] = np.array(nsteps)
+        if g == 4:
+            keep["p_list"] = points
+        for st in (1, 2, 3, nsteps):
+            for k, v in states[st].items():
+                keep[f"s{st}_{k}"] = v
+        np.savez_compressed(os.path.join(HERE, f"run_g{g}_remap.npz"), **keep)
+        print("remap", g, flush=True)
+        if g == 4:
+            run(exe, d, "diss", "out")
+            raw = np.fromfile(os.path.join(d, "out", "diss.bin"), dtype="<f8")
+            np.savez_compressed(os.path.join(HERE, "diss_g4.npz"), raw=raw, p_list=points)
+            print("diss", g, flush=True)
+
+
+def setup_files():
+    """setup_g3.npz (generated grid) and setup_g4.npz (the shipped grid)."""
+    import swm_mi355x as S
+    shipped4 = np.loadtxt(os.path.join(REF, "swm", "run.0002562", "data", "grid_centers_0002562")).T
+    for g in (3, 4):
+        exe = harness(g)
+        points = shipped4 if g == 4 else S.grid_points(g)
+        d = rundir(g, points)
+        run(exe, d, "setup", "out")
+        st = read_setup(os.path.join(d, "out", "setup.bin"), g)
+        if g == 4:
+            st["p_list"] = points
+        np.savez_compressed(os.path.join(HERE, f"setup_g{g}.npz"), **st)
+        print("setup", g, flush=True)
+
+
 def main():
     if not os.path.isdir(REF):
         sys.exit("make_golden.py needs the reference sources at " + REF)
+    if sys.argv[1:] == ["setup"]:
+        setup_files()
+        return
+    if sys.argv[1:] == ["remap"]:
+        remap_runs()
+        return
     if sys.argv[1:] == ["schemes"]:
         scheme_runs()
         return
@@ -371,6 +440,7 @@ def main():
     scheme_runs()
     restart_runs()
     swtc5_values()
+    remap_runs()
 
 
 if __name__ == "__main__":

@@ -42,6 +42,10 @@ int main(int argc, char** argv) {
     for (const char* n : dnames) (void)arr<double>(grid, n);
     for (const char* n : inames) (void)arr<int32_t>(grid, n);
     for (int l = 1; l <= g; ++l) (void)arr<double>(grid, ("lap_w" + std::to_string(l)).c_str());
+    // the remapping quantities (init_remap), built on first use
+    for (const char* n : {"w", "trans_to_X1", "trans_to_X2", "remap_c_x3", "remap_gwgt_x3", "remap_trans_X4_to_X3",
+                          "remap_p_x4", "remap_area_inv_x3", "remap_garea_inv"})
+      (void)arr<double>(grid, n);
     check(swm_grid_array_len(grid, "no_such_array") == -1, "unknown array");
     swm_grid_destroy(grid);
   }

@@ -14,7 +14,8 @@ import numpy as np
 import pytest
 
 import swm_mi355x as S
-from oracle.oracle import Oracle, grid_arrays_from
+from oracle import oracle as OR
+from oracle.oracle import Oracle, grid_arrays_from, remap_arrays_from
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -44,7 +45,7 @@ def test_setup_bit_exact(g):
     assert grid.array("tot_area") == ref["tot_area"][0]
     for name in ("p", "c", "area", "area_inv", "corner_area", "corner_area_inv", "sub_area", "dc", "normal",
                  "tangent", "v_weights", "s_weights", "l_weights", "f", "grid_center_mask", "grid_corner_mask",
-                 "polygon_type", "grid_center_index", "grid_corner_index"):
+                 "polygon_type", "grid_center_index", "grid_corner_index", "w", "trans_to_X1", "trans_to_X2"):
         mine = grid.array(name).reshape(-1, order="F")
         assert np.array_equal(mine, ref[name]), name
 
@@ -203,6 +204,52 @@ def test_oracle_advection_schemes_bit_exact(scheme):
         for name in ("h", "eta", "div", "psi", "chi", "ke", "relative", "u", "tracer"):
             mine = orc.field(name).reshape(-1, order="F")
             assert np.array_equal(mine, ref[f"s{s}_{name}"]), f"{scheme} step {s} {name}"
+
+
+# the tracer limiter's switches in the reference's runs: remap_mass_and_tracers
+# (remap.F:1072-1604) leaves l_limit_tracers_min / _max without a value when
+# the caller does not pass them, and get_remap_flux does not, so each build
+# of the reference uses what its stack holds.  The g = 4 build applies the
+# two-sided limit (l_limit_tracers = .true.) in every step; the g = 3 build
+# in the first step only, and none after.
+REMAP_LIMITER = {3: lambda step: (1, 1) if step == 1 else (0, 0), 4: lambda step: (1, 1)}
+
+
+@pytest.mark.parametrize("g", [3, 4])
+def test_oracle_remap_bit_exact(g):
+    """step_dt with l_remap (get_remap_flux, step.F:687-771, over
+    remap.F:176-1604) against the compiled reference, 12 steps, every field
+    bit for bit, with the remapping quantities of the library's host set-up
+    (init_remap) and the limiter switches each reference build used."""
+    ref = gold(f"run_g{g}_remap.npz")
+    nt, dt = 5, float(ref["dt"])
+    grid = grid_for(g)
+    orc = Oracle(grid_arrays_from(grid), nt, dt, S.REF_GRAV, hs=ref["ic_hs"], scheme="remap",
+                 remap=remap_arrays_from(grid), limiter=(1, 1))
+    orc.init_state(ref["ic_h"], ref["ic_u"], ref["ic_tracer"])
+    done = 0
+    for s in (1, 2, 3, int(ref["nsteps"])):
+        while done < s:
+            done += 1
+            OR.lib().orc_set_remap_limiter(orc._h, *REMAP_LIMITER[g](done))
+            orc.step(1)
+        for name in ("h", "eta", "div", "psi", "chi", "ke", "relative", "u", "tracer"):
+            mine = orc.field(name).reshape(-1, order="F")
+            assert np.array_equal(mine, ref[f"s{s}_{name}"]), f"g={g} step {s} {name}"
+
+
+def test_oracle_remap_limiter_pinned():
+    """The limiter switches matter: with the two-sided limit in every step the
+    g = 3 tracers leave that build's trajectory at step 2 while the thickness
+    stays on it (the limiter acts on tracers only)."""
+    ref = gold("run_g3_remap.npz")
+    grid = grid_for(3)
+    orc = Oracle(grid_arrays_from(grid), 5, float(ref["dt"]), S.REF_GRAV, hs=ref["ic_hs"], scheme="remap",
+                 remap=remap_arrays_from(grid), limiter=(1, 1))
+    orc.init_state(ref["ic_h"], ref["ic_u"], ref["ic_tracer"])
+    orc.step(2)
+    assert np.array_equal(orc.field("h").reshape(-1, order="F"), ref["s2_h"])
+    assert not np.array_equal(orc.field("tracer").reshape(-1, order="F"), ref["s2_tracer"])
 
 
 def test_oracle_budgets_against_reference():

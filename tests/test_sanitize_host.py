@@ -12,8 +12,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "shallow-water-model_amd", "csrc")
-HOST = ["swm_setup.cpp", "swm_metrics.cpp", "swm_capi_grid.cpp", "swm_restart.cpp", "swm_ics.cpp",
-        "swm_namelist.cpp"]
+HOST = ["swm_setup.cpp", "swm_metrics.cpp", "swm_remap_setup.cpp", "swm_capi_grid.cpp", "swm_restart.cpp",
+        "swm_ics.cpp", "swm_namelist.cpp"]
 
 
 @pytest.mark.timeout(600)
