@@ -684,6 +684,18 @@ swm_status swm_op_set_tracer_fct(swm_ctx* ctx, int iim, int jjm, int nlm, int ns
                                  const double* mss_flux_horz, double* trc_fct, double* trc_flux_horz,
                                  double* msstrc_low_order, const double* mask);
 
+/* init_diss_op (swm/source/swm_dissipation_operator.F:50-203): the
+ * operator's geometry for the context from the grid it was made with. */
+swm_status swm_init_diss_op(swm_ctx* ctx, const swm_grid* grid);
+
+/* dissipation (swm/source/swm_dissipation_operator.F:212-443): norder
+ * applications of the vector Laplacian (gradient of the edge divergence plus
+ * k x gradient of the edge curl) to the corner vectors xin (2,2,iim,jjm,10),
+ * each after a corner halo fill, times -visc for even norder and +visc for
+ * odd, into xout (zero off the interior cells).  The reference computes it
+ * for output only (driver.F:108-115, viscosity builds). */
+swm_status swm_op_dissipation(swm_ctx* ctx, int norder, double visc, const double* xin, double* xout);
+
 /* remap_mass_and_tracers (utilities/layer_2/remap.F:1072-1604) under its
  * mandatory arguments, in place on mass (iim,jjm,nmass,nsdm) and trace
  * (iim,jjm,ntrace,nmass,nsdm) with corner velocities uc (2,2,iim,jjm,nsdm);

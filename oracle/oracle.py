@@ -52,6 +52,9 @@ def _load():
     lib.orc_set_tracer_fct.argtypes = [vp] * 8
     lib.orc_set_remap.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
     lib.orc_set_remap_limiter.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.orc_set_diss.argtypes = [vp] * 5 + [ctypes.c_double]
+    lib.orc_dissipation.restype = ctypes.c_int
+    lib.orc_dissipation.argtypes = [vp, ctypes.c_int, ctypes.c_double, vp, vp]
     lib.orc_remap_mass_and_tracers.restype = ctypes.c_int
     lib.orc_remap_mass_and_tracers.argtypes = [vp] * 5
     return lib
@@ -265,6 +268,21 @@ class Oracle:
         args = [np.asfortranarray(a, dtype=np.float64) for a in (trc, trc_high, mss, mss_low, mss_horz, mflux)]
         out = np.zeros((self.iim, self.jjm, 10), order="F")
         lib().orc_set_tracer_fct(self._h, *[_p(a) for a in args], _p(out))
+        return out
+
+    def set_diss(self, grid):
+        """init_diss_op from a swm_mi355x.Grid's centres, corners, edge
+        midpoints and corner frames."""
+        arrs = [np.asfortranarray(grid.array(n), dtype=np.float64) for n in ("p", "c", "w", "trans_to_X2")]
+        lib().orc_set_diss(self._h, *[_p(a) for a in arrs], float(grid.radius))
+
+    def dissipation(self, norder, visc, xin):
+        """dissipation (swm_dissipation_operator.F:212-443) of corner vectors
+        (2, 2, iim, jjm, 10)."""
+        x = np.asfortranarray(xin, dtype=np.float64)
+        out = np.zeros_like(x, order="F")
+        if lib().orc_dissipation(self._h, int(norder), float(visc), _p(x), _p(out)) != 0:
+            raise RuntimeError("oracle: set_diss first")
         return out
 
     def set_remap_limiter(self, limit_min, limit_max):

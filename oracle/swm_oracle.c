@@ -42,6 +42,7 @@ struct orc_model {
   int sum_mode;          /* elliptic sums: 0 memory order (the reference), 1 compensated */
   orc_remap rm;          /* remapping quantities (ORC_REMAP) */
   int rm_owned, rm_limit_min, rm_limit_max;
+  double *ds_t, *ds_magt, *ds_magn;   /* dissipation operator (orc_set_diss) */
 };
 
 #define S(i, j, k) ((size_t)((i)-1) + (size_t)iim * ((size_t)((j)-1) + (size_t)jjm * ((k)-1)))
@@ -1329,6 +1330,121 @@ void orc_set_remap(orc_model* m, const orc_remap* r, int limit_min, int limit_ma
   m->rm_limit_max = limit_max;
 }
 
+/* ------------------------------------------------- dissipation operator --- */
+/* swm/source/swm_dissipation_operator.F:50-203 init_diss_op: per interior
+ * cell and corner, the unit vectors from the corner to its three edge
+ * midpoints and from the neighbouring corners to the same midpoints, in each
+ * corner's X2 frame (t), their lengths (magt) and the distances between the
+ * cell centres across the three edges (magn), in metres */
+void orc_set_diss(orc_model* m, const double* p, const double* c, const double* w, const double* trans_to_x2,
+                  double radius) {
+  const int iim = m->iim, jjm = m->jjm;
+  const size_t ns = m->ns;
+  free(m->ds_t); free(m->ds_magt); free(m->ds_magn);
+  m->ds_t = zeros(24 * ns);      /* t(2, ndata, nspace, ncorners, slot) */
+  m->ds_magt = zeros(12 * ns);   /* magt(2, ndata, ncorners, slot) */
+  m->ds_magn = zeros(6 * ns);    /* magn(ndata, ncorners, slot) */
+#define P(i, j, k) (&p[3 * S(i, j, k)])
+#define CC(ic, i, j, k) (&c[3 * (2 * S(i, j, k) + (ic))])
+#define TX2(ic, i, j, k) (&trans_to_x2[9 * (2 * S(i, j, k) + (ic))])
+  for (int k = 1; k <= 10; ++k)
+    for (int j = 2; j <= jjm - 1; ++j)
+      for (int i = 2; i <= iim - 1; ++i) {
+        const size_t s = S(i, j, k);
+        const double* d[2][3][2] = {{{P(i - 1, j, k), P(i, j, k)}, {P(i - 1, j - 1, k), P(i - 1, j, k)},
+                                     {P(i, j, k), P(i - 1, j - 1, k)}},
+                                    {{P(i - 1, j - 1, k), P(i, j, k)}, {P(i, j - 1, k), P(i - 1, j - 1, k)},
+                                     {P(i, j, k), P(i, j - 1, k)}}};
+        for (int ic = 0; ic < 2; ++ic)
+          for (int id = 0; id < 3; ++id) {
+            const double t1 = d[ic][id][0][0] - d[ic][id][1][0], t2 = d[ic][id][0][1] - d[ic][id][1][1],
+                         t3 = d[ic][id][0][2] - d[ic][id][1][2];
+            m->ds_magn[id + 3 * (ic + 2 * s)] = sqrt(t1 * t1 + t2 * t2 + t3 * t3);
+          }
+        /* the other corner of each edge and its cell */
+        const int oi[2][3] = {{i, i - 1, i}, {i, i, i + 1}}, oj[2][3] = {{j + 1, j, j}, {j, j - 1, j}};
+        for (int ic = 0; ic < 2; ++ic) {
+          const int oc = 1 - ic;
+          for (int id = 0; id < 3; ++id) {
+            const double* c1 = CC(ic, i, j, k);
+            const double* wv = &w[9 * (2 * s + ic) + id];   /* w(id, :, ic): stride 3 */
+            const double* c3 = CC(oc, oi[ic][id], oj[ic][id], k);
+            const double* T[2] = {TX2(ic, i, j, k), TX2(oc, oi[ic][id], oj[ic][id], k)};
+            const double* src[2] = {c1, c3};
+            for (int e = 0; e < 2; ++e) {
+              double v[3], y[3];
+              for (int r = 0; r < 3; ++r) v[r] = src[e][r] - wv[3 * r];
+              for (int r = 0; r < 3; ++r) {
+                double acc = 0.0;
+                for (int q = 0; q < 3; ++q) acc = acc + T[e][r + 3 * q] * v[q];
+                y[r] = acc;
+              }
+              const double rmag = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+              for (int sp = 0; sp < 2; ++sp) m->ds_t[e + 2 * (id + 3 * (sp + 2 * (ic + 2 * s)))] = y[sp] / rmag;
+              m->ds_magt[e + 2 * (id + 3 * (ic + 2 * s))] = rmag;
+            }
+          }
+        }
+      }
+#undef P
+#undef CC
+#undef TX2
+  for (size_t n = 0; n < 6 * ns; ++n) m->ds_magn[n] = radius * m->ds_magn[n];
+  for (size_t n = 0; n < 12 * ns; ++n) m->ds_magt[n] = radius * m->ds_magt[n];
+}
+
+/* swm_dissipation_operator.F:212-443 dissipation: norder applications of the
+ * vector Laplacian (grad div + k x grad curl from edge divergences and
+ * curls) to the corner vectors, times -visc (norder even) or +visc;
+ * xin, xout (2,2,iim,jjm,10) */
+int orc_dissipation(const orc_model* m, int norder, double visc, const double* xin, double* xout) {
+  const int iim = m->iim, jjm = m->jjm;
+  const size_t ns = m->ns;
+  if (!m->ds_t) return 1;
+  double* y = zeros(4 * ns);
+  memcpy(xout, xin, 4 * ns * sizeof(double));
+#define X(sp, ic, sl) xout[(sp) + 2 * ((ic) + 2 * (sl))]
+#define T(a, id, sp, ic) m->ds_t[(a) + 2 * ((id) + 3 * ((sp) + 2 * ((ic) + 2 * s)))]
+#define MT(a, id, ic) m->ds_magt[(a) + 2 * ((id) + 3 * ((ic) + 2 * s))]
+  for (int it = 0; it < norder; ++it) {
+    halo(m->vrtx_dst, m->vrtx_src, m->nvrtx, xout, 2);
+    memset(y, 0, 4 * ns * sizeof(double));
+    for (int k = 1; k <= 10; ++k)
+      for (int j = 2; j <= jjm - 1; ++j)
+        for (int i = 2; i <= iim - 1; ++i) {
+          const size_t s = S(i, j, k);
+          const int oi[2][3] = {{i, i - 1, i}, {i, i, i + 1}}, oj[2][3] = {{j + 1, j, j}, {j, j - 1, j}};
+          for (int ic = 0; ic < 2; ++ic) {
+            const int oc = 1 - ic;
+            double gd[2] = {0.0, 0.0}, gr[2] = {0.0, 0.0};
+            for (int id = 0; id < 3; ++id) {
+              const size_t so = S(oi[ic][id], oj[ic][id], k);
+              const double msum = MT(0, id, ic) + MT(1, id, ic);
+              const double dv = (X(0, ic, s) * T(0, id, 0, ic) + X(1, ic, s) * T(0, id, 1, ic) +
+                                 X(0, oc, so) * T(1, id, 0, ic) + X(1, oc, so) * T(1, id, 1, ic)) / msum;
+              const double rl = (-X(0, ic, s) * T(0, id, 1, ic) + X(1, ic, s) * T(0, id, 0, ic) -
+                                 X(0, oc, so) * T(1, id, 1, ic) + X(1, oc, so) * T(1, id, 0, ic)) / msum;
+              const double mn = m->ds_magn[id + 3 * (ic + 2 * s)], ca = m->cai[ic + 2 * s];
+              for (int sp = 0; sp < 2; ++sp) {
+                gd[sp] = gd[sp] - dv * mn * T(0, id, sp, ic) * ca;
+                gr[sp] = gr[sp] - rl * mn * T(0, id, sp, ic) * ca;
+              }
+            }
+            y[0 + 2 * (ic + 2 * s)] = gd[0] + -gr[1];
+            y[1 + 2 * (ic + 2 * s)] = gd[1] + gr[0];
+          }
+        }
+    memcpy(xout, y, 4 * ns * sizeof(double));
+  }
+#undef X
+#undef T
+#undef MT
+  const double f = norder % 2 == 0 ? -visc : visc;
+  for (size_t n = 0; n < 4 * ns; ++n) xout[n] = f * xout[n];
+  free(y);
+  return 0;
+}
+
 /* --------------------------------------------------------------- model --- */
 orc_model* orc_create(const orc_grid* gr) {
   orc_model* m = (orc_model*)calloc(1, sizeof(orc_model));
@@ -1384,6 +1500,7 @@ orc_model* orc_create(const orc_grid* gr) {
 
 void orc_destroy(orc_model* m) {
   if (!m) return;
+  free(m->ds_t); free(m->ds_magt); free(m->ds_magn);
   if (m->rm_owned) {
     orc_remap* d = &m->rm;
     free((void*)d->c); free((void*)d->w); free((void*)d->trans_to_x1); free((void*)d->trans_x1_to_x4);

@@ -132,6 +132,13 @@ void orc_set_mass_fct(orc_model* m, const int* tend_index, const double* tend_we
                       double* mss_n4);
 void orc_set_tracer_fct(orc_model* m, const double* trc, const double* trc_high, const double* mss,
                         const double* mss_low, const double* mss_horz, const double* mflux, double* trc_fct);
+/* init_diss_op (swm_dissipation_operator.F:50-203) from the grid's cell
+ * centres p (3,slot), corners c (3,2,slot), edge midpoints w (3,3,2,slot)
+ * and corner frames trans_to_X2 (3,3,2,slot), and dissipation (:212-443)
+ * of corner vectors xin -> xout (2,2,iim,jjm,10); 1 without orc_set_diss. */
+void orc_set_diss(orc_model* m, const double* p, const double* c, const double* w, const double* trans_to_x2,
+                  double radius);
+int orc_dissipation(const orc_model* m, int norder, double visc, const double* xin, double* xout);
 /* One multigrid V-cycle and one full elliptic solve on (iim,jjm,nvar,10). */
 void orc_multigrid(orc_model* m, int nvar, const double* beta, double* alpha);
 int orc_elliptic(orc_model* m, int nvar, double* rhs, double* xout);

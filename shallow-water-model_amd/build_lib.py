@@ -22,9 +22,8 @@ LIB = os.path.join(OUT_DIR, "libswm_mi355x.so")
 FORTRAN_CHECK = os.path.join(OUT_DIR, "swm_fortran_check")
 DRIVER = os.path.join(OUT_DIR, "swm_driver")
 
-HOST_SOURCES = ["swm_setup.cpp", "swm_metrics.cpp", "swm_remap_setup.cpp", "swm_capi_grid.cpp", "swm_restart.cpp", "swm_qp.cpp",
-                "swm_ics.cpp",
-                "swm_namelist.cpp"]
+HOST_SOURCES = ["swm_setup.cpp", "swm_metrics.cpp", "swm_remap_setup.cpp", "swm_diss_setup.cpp", "swm_capi_grid.cpp",
+                "swm_restart.cpp", "swm_qp.cpp", "swm_ics.cpp", "swm_namelist.cpp"]
 DEVICE_SOURCES = ["swm_device.hip"]
 
 

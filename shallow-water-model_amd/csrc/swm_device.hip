@@ -19,6 +19,7 @@
 #include "swm_kernels.hip"
 #include "swm_tlad.hip"
 #include "swm_remap.hip"
+#include "swm_diss.hip"
 
 namespace {
 
@@ -190,6 +191,7 @@ struct swm_ctx {
   long long shadow_checks = 0, shadow_bad = 0;
   bool initialized = false;
   RemapDev rm;
+  double* diss = nullptr;      // dissipation operator geometry (swm_init_diss_op), 36 planes
   // elliptic solve settings (reference: mean(r^2) <= 1e-20, 12 cycles) and
   // the mean squared residuals / right-hand sides of the last solve
   double solver_tol = 1.0e-20;
@@ -1936,6 +1938,7 @@ void swm_destroy(swm_ctx* C) {
   if (C->marena.base) (void)hipFree(C->marena.base);
   if (C->host_sums) (void)hipHostFree(C->host_sums);
   if (C->rm.mem) (void)hipFree(C->rm.mem);
+  if (C->diss) (void)hipFree(C->diss);
   if (C->rm.flag_host) (void)hipHostFree(C->rm.flag_host);
   swm::tlad_free(C);
   if (C->stream) (void)hipStreamDestroy(C->stream);
@@ -3042,6 +3045,56 @@ swm_status swm_op_remap_mass_and_tracers(swm_ctx* C, int iim, int jjm, int nmass
         std::memcpy(&trace[((size_t)q + (size_t)ntrace * k) * pl], &planes[(size_t)q * ns + k * pl], pl * sizeof(double));
   }
   return SWM_OK;
+}
+
+swm_status swm_init_diss_op(swm_ctx* C, const swm_grid* grid) {
+  OP_BEGIN(C);
+  if (!grid) { swm::set_error("swm_init_diss_op: bad argument"); return SWM_ERR_ARG; }
+  const swm::Grid& Gh = swm::grid_of(grid);
+  if (Gh.iim != C->iim || Gh.jjm != C->jjm) {
+    swm::set_error("swm_init_diss_op: the grid is not the context's");
+    return SWM_ERR_DIMS;
+  }
+  swm::DissGeometry D;
+  swm::build_diss_geometry(Gh, D);
+  const size_t ns = (size_t)C->ns;
+  std::vector<double> planes((size_t)swm::dev::DS_PLANES * ns);
+  for (size_t n = 0; n < ns; ++n) {
+    for (int q = 0; q < 24; ++q) planes[(size_t)q * ns + n] = D.t[q + 24 * n];
+    for (int ic = 0; ic < 2; ++ic)
+      for (int id = 0; id < 3; ++id) {
+        const size_t m = 2 * (id + 3 * (ic + 2 * n));
+        planes[(size_t)(24 + id + 3 * ic) * ns + n] = D.magt[m] + D.magt[m + 1];
+        planes[(size_t)(30 + id + 3 * ic) * ns + n] = D.magn[id + 3 * (ic + 2 * n)];
+      }
+  }
+  if (!C->diss) HIP_TRY(hipMalloc(&C->diss, planes.size() * sizeof(double)));
+  HIP_TRY(hipMemcpy(C->diss, planes.data(), planes.size() * sizeof(double), hipMemcpyHostToDevice));
+  return SWM_OK;
+}
+
+swm_status swm_op_dissipation(swm_ctx* C, int norder, double visc, const double* xin, double* xout) {
+  OP_BEGIN(C);
+  OP_ARGS(xin, xout);
+  if (norder < 1) { swm::set_error("swm_op_dissipation: norder >= 1"); return SWM_ERR_ARG; }
+  if (!C->diss) { swm::set_error("swm_op_dissipation: swm_init_diss_op first"); return SWM_ERR_STATE; }
+  const size_t ns = (size_t)C->ns;
+  OpIO io{C, ns, {}, {}};
+  double* a = io.put(xin, 4, 4, kCornerVec);
+  double* b = io.zeros(4);
+  OP_BUF(a && b);
+  const unsigned nb = nblk_b(ns, C->blk_step);
+  hipStream_t S = C->stream;
+  // norder applications, each after the corner halo fill (the vertex wrap of
+  // :279); the last one times -visc (norder even) or +visc (:436-441)
+  for (int it = 0; it < norder; ++it) {
+    gather<double>(C, C->vrtx, {a, a + 2 * ns});
+    const double f = it + 1 < norder ? 1.0 : (norder % 2 == 0 ? -visc : visc);
+    hipLaunchKernelGGL(swm::dev::k_diss, dim3(nb), dim3(C->blk_step), 0, S, C->iim, C->jjm, (int)ns, C->diss, C->cai, f,
+                       a, b);
+    std::swap(a, b);
+  }
+  return io.get(xout, a, 4, kCornerVec);
 }
 
 swm_status swm_wrap(swm_ctx* C, int kind, int nspace, int nlayer, double* x) {

@@ -46,6 +46,16 @@ struct RemapGeometry {
   std::vector<double> garea_inv;        // (slot)
 };
 
+// Geometry of the dissipation operator (init_diss_op,
+// swm/source/swm_dissipation_operator.F:50-203) in the reference layout:
+// t (2, ndata, nspace, ncorners, slot) unit vectors in the corners' X2
+// frames, magt (2, ndata, ncorners, slot) their lengths and magn (ndata,
+// ncorners, slot) the centre-to-centre distances across the edges, in metres.
+struct DissGeometry {
+  std::vector<double> t, magt, magn;
+};
+void build_diss_geometry(const struct Grid& G, DissGeometry& D);
+
 // the four cells around edge ne (0-based) of cell (i, j): offsets of cell
 // nd = 0..3 (remap.F ishift_p / jshift_p)
 extern const int kRemapShiftI[3][4];

@@ -141,6 +141,9 @@ _sig("swm_create_members", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Confi
 _sig("swm_members", ctypes.c_int, [ctypes.c_void_p])
 _sig("swm_ctx_layout", ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p, _c_int_p])
 _sig("swm_set_remap_limiter", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int])
+_sig("swm_init_diss_op", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p])
+_sig("swm_op_dissipation", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                                          ctypes.c_void_p])
 _sig("swm_op_remap_mass_and_tracers", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 5 +
      [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p])
 _sig("swm_qp_name", ctypes.c_int, [ctypes.c_longlong, ctypes.c_int, ctypes.c_char_p, ctypes.c_int])
@@ -814,6 +817,19 @@ class Model:
         """Incremental remapping: the tracer limiter's l_limit_tracers_min /
         _max switches (swm_set_remap_limiter; both on by default)."""
         _check(lib.swm_set_remap_limiter(self._h, int(bool(limit_min)), int(bool(limit_max))))
+
+    def init_diss_op(self):
+        """init_diss_op (swm_dissipation_operator.F:50-203) for the model's
+        grid."""
+        _check(lib.swm_init_diss_op(self._h, self.grid.handle))
+
+    def dissipation(self, norder, visc, xin):
+        """dissipation (swm_dissipation_operator.F:212-443) of corner vectors
+        (2, 2, iim, jjm, 10)."""
+        x = _f64(xin, (2, 2, self.iim, self.jjm, 10))
+        out = np.zeros_like(x, order="F")
+        _check(lib.swm_op_dissipation(self._h, int(norder), float(visc), _ptr(x), _ptr(out)))
+        return out
 
     def remap_mass_and_tracers(self, mass, trace, dt, uc):
         """remap_mass_and_tracers (remap.F:1072-1604) on host arrays, mass

@@ -252,6 +252,21 @@ def test_oracle_remap_limiter_pinned():
     assert not np.array_equal(orc.field("tracer").reshape(-1, order="F"), ref["s2_tracer"])
 
 
+def test_oracle_dissipation_bit_exact():
+    """init_diss_op and dissipation (swm_dissipation_operator.F:50-443) of a
+    pseudo-random corner vector field, norder 1 and 2, against the compiled
+    reference on the shipped g = 4 grid, every slot."""
+    d = gold("diss_g4.npz")
+    grid = S.Grid(4, p_list=d["p_list"])
+    n = 4 * grid.iim * grid.jjm * 10
+    vin, o1, o2 = [d["raw"][k * n:(k + 1) * n].reshape((2, 2, grid.iim, grid.jjm, 10), order="F") for k in range(3)]
+    orc = Oracle(grid_arrays_from(grid), 1, 100.0, S.REF_GRAV)
+    orc.set_diss(grid)
+    assert np.array_equal(orc.dissipation(1, 2.5e12, vin), o1)
+    assert np.array_equal(orc.dissipation(2, 2.5e12, vin), o2)
+    orc.close()
+
+
 def test_oracle_budgets_against_reference():
     """compute_budgets of the reference (budgets.F:31-247) along 12 FCT steps at
     g = 4: its real*4 records (relative 1e-6) and the values it prints with 9
