@@ -18,6 +18,12 @@
 !        (qp_output/ of the working directory) with the initial state and
 !        the state after the steps
 !
+!   swm_fortran_check remap <input> <output>
+!
+! remap_mass_and_tracers and dissipation through their modules, then steps
+! of the l_remap model; input as the default mode's, output in the order
+! written by run_remap.
+!
 !   swm_fortran_check ops <input> <output>
 !
 ! calls every drop-in under the reference's names and argument lists: the 13
@@ -48,6 +54,12 @@ program swm_fortran_check
     call get_command_argument(2, fin)
     call get_command_argument(3, fout)
     call run_ops(fin, fout)
+    stop
+  end if
+  if (trim(fin) == 'remap') then
+    call get_command_argument(2, fin)
+    call get_command_argument(3, fout)
+    call run_remap(fin, fout)
     stop
   end if
   call get_command_argument(2, fout)
@@ -125,6 +137,59 @@ contains
       lon = 0.0_c_double
     end if
   end subroutine lonlat
+
+  ! incremental remapping and the dissipation operator through their
+  ! modules: remap_mass_and_tracers of the initial state (default limiter,
+  ! then with lin_limit_tracers = .false.), dissipation of the initial
+  ! velocity (norder 2), then nsteps steps of the l_remap model
+  subroutine run_remap(fin, fout)
+    use remap_mi355x
+    use swm_dissipation_operator_mi355x
+    character(len=*), intent(in) :: fin, fout
+    integer(c_int32_t) :: g, nt, ns
+    real(c_double) :: dt, grav, radius, omega
+    real(c_double), allocatable, target :: h(:,:,:), hs(:,:,:), u(:,:,:,:,:), tracer(:,:,:,:)
+    real(c_double), allocatable :: mass(:,:,:,:), trace(:,:,:,:,:), mass2(:,:,:,:), trace2(:,:,:,:,:), &
+        ef(:,:,:,:,:), visc(:,:,:,:,:)
+    integer :: n, k, fu
+    open(newunit=fu, file=trim(fin), access='stream', form='unformatted', status='old')
+    read(fu) g, nt, ns, dt, grav, radius, omega
+    n = 2 + 2**g
+    allocate(h(n,n,10), hs(n,n,10), u(2,2,n,n,10), tracer(n,n,nt,10))
+    read(fu) h, hs, u, tracer
+    close(fu)
+    call swm_gpu_initialize(int(g), int(nt), dt, grav, radius, omega, h, hs, u, tracer, l_remap=.true.)
+    call init_remap('swm')
+    ! the initialised fields, halos filled
+    call sync_prognostics_to_host(h, u, tracer)
+    allocate(mass(n,n,1,10), trace(n,n,nt,1,10), mass2(n,n,1,10), trace2(n,n,nt,1,10), ef(1,3,n,n,10), &
+             visc(2,2,n,n,10))
+    mass(:,:,1,:) = h
+    trace(:,:,:,1,:) = tracer
+    mass2 = mass
+    trace2 = trace
+    call remap_mass_and_tracers('swm', 1, int(nt), mass, trace, dt, u, edgeflux=ef)
+    call remap_mass_and_tracers('swm', 1, int(nt), mass2, trace2, dt, u, lin_limit_tracers=.false.)
+    call init_diss_op('swm')
+    call dissipation('swm', 2, 2.5e12_c_double, u, visc)
+    ! remapped fields are valid at the real cells: fill the halos before the
+    ! next remapping, as the reference's wraps do
+    call wrap('swm', face=mass2)
+    do k = 1, int(nt)
+      call wrap('swm', face_1L=trace2(:,:,k,1,:))
+    end do
+    call remap_mass_and_tracers('swm', 1, int(nt), mass2, trace2, dt, u, lin_limit_tracers_min=.true., &
+                                lin_limit_tracers_max=.true.)
+    do k = 1, ns
+      call step_dt()
+      call wrap_prognostics()
+    end do
+    call sync_prognostics_to_host(h, u, tracer)
+    open(newunit=fu, file=trim(fout), access='stream', form='unformatted', status='replace')
+    write(fu) mass, trace, ef, mass2, trace2, visc, h, tracer
+    close(fu)
+    call swm_gpu_finalize()
+  end subroutine run_remap
 
   subroutine run_ops(fin, fout)
     use mass_fct_mi355x

@@ -27,7 +27,7 @@ module swm_mi355x_c
   public
 
   integer(c_int), parameter :: SWM_OK = 0, SWM_FP64 = 0, SWM_MIXED = 1
-  integer(c_int), parameter :: SWM_ADV_FCT = 0, SWM_ADV_CENTERED = 1, SWM_ADV_UPWIND = 2
+  integer(c_int), parameter :: SWM_ADV_FCT = 0, SWM_ADV_CENTERED = 1, SWM_ADV_UPWIND = 2, SWM_ADV_REMAP = 3
 
   type, bind(C) :: swm_config
     integer(c_int) :: ntracer
@@ -147,6 +147,35 @@ module swm_mi355x_c
     function swm_read_restart(ctx, path) bind(C, name="swm_read_restart") result(st)
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, path
+      integer(c_int) :: st
+    end function
+    function swm_init_diss_op(ctx, grid) bind(C, name="swm_init_diss_op") result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, grid
+      integer(c_int) :: st
+    end function
+    function swm_op_dissipation(ctx, norder, visc, xin, xout) bind(C, name="swm_op_dissipation") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: norder
+      real(c_double), value :: visc
+      real(c_double) :: xin(*), xout(*)
+      integer(c_int) :: st
+    end function
+    function swm_set_remap_limiter(ctx, limit_min, limit_max) bind(C, name="swm_set_remap_limiter") result(st)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: limit_min, limit_max
+      integer(c_int) :: st
+    end function
+    function swm_op_remap_mass_and_tracers(ctx, iim, jjm, nmass, nsdm, ntrace, mass, trace, dt, uc, edgeflux) &
+        bind(C, name="swm_op_remap_mass_and_tracers") result(st)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: iim, jjm, nmass, nsdm, ntrace
+      real(c_double) :: mass(*), trace(*), uc(*)
+      real(c_double), value :: dt
+      type(c_ptr), value :: edgeflux
       integer(c_int) :: st
     end function
     function swm_qp_create_coords(grid, tag, nsecond, path, clat, clon, crlat, crlon, qp) &
@@ -395,13 +424,13 @@ contains
   ! initial fields h, hs (iim,jjm,10), u (2,2,iim,jjm,10), tracer
   ! (iim,jjm,ntracer,10).
   subroutine swm_gpu_initialize(g, ntracer, dt, grav, radius, omega, h, hs, u, tracer, p_list, device, &
-                                l_fct, l_centered, l_upwind)
+                                l_fct, l_centered, l_upwind, l_remap)
     integer, intent(in) :: g, ntracer
     real(c_double), intent(in) :: dt, grav, radius, omega
     real(c_double), intent(in), target :: h(:,:,:), hs(:,:,:), u(:,:,:,:,:), tracer(:,:,:,:)
     real(c_double), intent(in), optional, target :: p_list(:,:)
     integer, intent(in), optional :: device
-    logical, intent(in), optional :: l_fct, l_centered, l_upwind   ! the namelist flags (default l_fct)
+    logical, intent(in), optional :: l_fct, l_centered, l_upwind, l_remap   ! the namelist flags (default l_fct)
     type(swm_config) :: cfg
     type(c_ptr) :: pl
     integer(c_int) :: gg, mig
@@ -421,6 +450,9 @@ contains
     end if
     if (present(l_upwind)) then
       if (l_upwind) cfg%advection = SWM_ADV_UPWIND
+    end if
+    if (present(l_remap)) then
+      if (l_remap) cfg%advection = SWM_ADV_REMAP
     end if
     if (present(l_fct)) then
       if (l_fct) cfg%advection = SWM_ADV_FCT
@@ -757,6 +789,111 @@ contains
     call swm_gpu_check(swm_qp_write(qp, nsecond), 'write_swm_qp')
   end subroutine write_swm_qp
 end module swm_qp_module_mi355x
+
+!=======================================================================
+! swm/source/swm_dissipation_operator.F under its own names and arguments
+! (init_diss_op :50-203, dissipation :212-443) on the run's grid
+module swm_dissipation_operator_mi355x
+  use iso_c_binding
+  use swm_mi355x_c
+  use swm_mi355x, only: swm_gpu_check, swm_gpu_context, swm_gpu_grid
+  implicit none
+  private
+  public :: init_diss_op, dissipation
+contains
+  subroutine init_diss_op(component_name)
+    character(len=*), intent(in) :: component_name
+    call swm_gpu_check(swm_init_diss_op(swm_gpu_context(), swm_gpu_grid()), 'init_diss_op '//component_name)
+  end subroutine init_diss_op
+
+  subroutine dissipation(component_name, norder, visc, xin, xout)
+    character(len=*), intent(in) :: component_name
+    integer, intent(in) :: norder
+    real(c_double), intent(in) :: visc
+    real(c_double), intent(in) :: xin(:,:,:,:,:)
+    real(c_double), intent(out) :: xout(:,:,:,:,:)
+    call swm_gpu_check(swm_op_dissipation(swm_gpu_context(), int(norder, c_int), visc, xin, xout), &
+                       'dissipation '//component_name)
+  end subroutine dissipation
+end module swm_dissipation_operator_mi355x
+
+!=======================================================================
+! utilities/layer_2/remap.F under its own names and arguments: init_remap
+! (:176-1063; the quantities are built with the device model when
+! swm_gpu_initialize is given l_remap) and remap_mass_and_tracers
+! (:1072-1604).  Of the optional arguments the tracer limiter's
+! lin_limit_tracers, lin_limit_tracers_min and lin_limit_tracers_max are
+! taken (both switches on when absent, as l_limit_tracers = .true. intends);
+! the checks lin_check_* only print in the reference and are not made; the
+! others stop the program unless they hold get_remap_flux's values (no mask,
+! independent tracers, linear integrals, first-order departure points, mass
+! gradients unlimited, no positive-definite-only limits).
+module remap_mi355x
+  use iso_c_binding
+  use swm_mi355x_c
+  use swm_mi355x, only: swm_gpu_check, swm_gpu_context
+  implicit none
+  private
+  public :: init_remap, remap_mass_and_tracers
+contains
+  subroutine init_remap(component_name)
+    character(len=*), intent(in) :: component_name
+    if (.not. c_associated(swm_gpu_context())) call swm_gpu_check(7_c_int, 'init_remap '//component_name)
+  end subroutine init_remap
+
+  subroutine remap_mass_and_tracers(component_name, nmass, ntrace, mass, trace, dt, uc, pmask_in, depend_in, &
+                                    integral_order_in, lin_dp2ord, lin_check_conservation, &
+                                    lin_check_monotonicity, lin_check_divergence, lin_check_velocity, &
+                                    lin_mask_gradient, lin_limit_mass, lin_limit_mass_min, lin_limit_mass_max, &
+                                    lin_mass_pd, lin_limit_tracers, lin_limit_tracers_min, lin_limit_tracers_max, &
+                                    lin_tracers_pd, edgeflux)
+    character(len=*), intent(in) :: component_name
+    integer, intent(in) :: nmass, ntrace
+    real(c_double), intent(inout) :: mass(:,:,:,:), trace(:,:,:,:,:)
+    real(c_double), intent(in) :: dt
+    real(c_double), intent(in) :: uc(:,:,:,:,:)
+    real(c_double), intent(in), optional :: pmask_in(:,:,:)
+    integer, intent(in), optional :: depend_in(:), integral_order_in
+    logical, intent(in), optional :: lin_dp2ord, lin_check_conservation, lin_check_monotonicity, &
+        lin_check_divergence, lin_check_velocity, lin_mask_gradient, lin_limit_mass, lin_limit_mass_min, &
+        lin_limit_mass_max, lin_mass_pd, lin_limit_tracers, lin_limit_tracers_min, lin_limit_tracers_max, &
+        lin_tracers_pd
+    real(c_double), intent(out), optional, target :: edgeflux(:,:,:,:,:)
+    integer(c_int) :: lmin, lmax
+    type(c_ptr) :: ef
+    logical :: off
+    off = .false.
+    if (present(pmask_in)) off = off .or. any(pmask_in /= 1.0_c_double)
+    if (present(depend_in)) off = off .or. any(depend_in /= 0)
+    if (present(integral_order_in)) off = off .or. integral_order_in /= 1
+    if (present(lin_dp2ord)) off = off .or. lin_dp2ord
+    if (present(lin_mask_gradient)) off = off .or. lin_mask_gradient
+    if (present(lin_limit_mass)) off = off .or. lin_limit_mass
+    if (present(lin_mass_pd)) off = off .or. lin_mass_pd
+    if (present(lin_tracers_pd)) off = off .or. lin_tracers_pd
+    if (off) then
+      write(*, '(a)') 'swm_mi355x: remap_mass_and_tracers '//component_name// &
+          ': only the optional arguments get_remap_flux uses are supported'
+      stop 1
+    end if
+    lmin = 1
+    lmax = 1
+    if (present(lin_limit_tracers_min)) lmin = merge(1_c_int, 0_c_int, lin_limit_tracers_min)
+    if (present(lin_limit_tracers_max)) lmax = merge(1_c_int, 0_c_int, lin_limit_tracers_max)
+    if (present(lin_limit_tracers)) then
+      if (.not. lin_limit_tracers) then
+        lmin = 0
+        lmax = 0
+      end if
+    end if
+    call swm_gpu_check(swm_set_remap_limiter(swm_gpu_context(), lmin, lmax), 'remap_mass_and_tracers '//component_name)
+    ef = c_null_ptr
+    if (present(edgeflux)) ef = c_loc(edgeflux)
+    call swm_gpu_check(swm_op_remap_mass_and_tracers(swm_gpu_context(), int(size(mass, 1), c_int), &
+                       int(size(mass, 2), c_int), int(nmass, c_int), int(size(mass, 4), c_int), int(ntrace, c_int), &
+                       mass, trace, dt, uc, ef), 'remap_mass_and_tracers '//component_name)
+  end subroutine remap_mass_and_tracers
+end module remap_mi355x
 
 !=======================================================================
 ! global_reductions.F:77-98 global_real_sum under its own name and
