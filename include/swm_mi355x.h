@@ -363,11 +363,16 @@ swm_status swm_profile(swm_ctx* ctx, int enable);
  * (operators.F:270-392) of the mass flux, 1 curl_of_a_vector (operators.F:
  * 47-147) of the corner velocity, 2 laplacian_of_a_scalar (operators.F:
  * 402-518) of the thickness, 3 the face halo fill (wrap_data.F:1790-2607)
- * of one cell field.  Runs reps back-to-back launches of the kernel the
- * operator entry points run, between HIP events on the context's stream;
- * returns the mean microseconds per launch and the algorithmic bytes per
- * launch (40, 144 and 72 B per real cell; 16 B per ghost slot).  The state
- * is not changed.  No counterpart in the reference. */
+ * of one cell field; 4 and 5 the remapping's transport (locate_triangles and
+ * transport_integrals, remap.F:2903-3963) and field (construct_fields,
+ * remap.F:1803-2397) kernels of a remapping context after a step; 6 one
+ * application of the dissipation operator (after swm_init_diss_op) to the
+ * velocity.  Runs reps back-to-back launches of the kernel the operator
+ * entry points run, between HIP events on the context's stream; returns the
+ * mean microseconds per launch and the algorithmic bytes per launch (40,
+ * 144 and 72 B per real cell; 16 B per ghost slot; 1128 + 48 ntracer,
+ * 288 + 32 ntracer and 368 B per real cell).  The state is not changed.  No
+ * counterpart in the reference. */
 swm_status swm_operator_bench(swm_ctx* ctx, int op, int reps, double* us_per_launch, double* bytes_per_launch);
 
 /* Seeded synthetic initial state built on the device: harmonic sums of

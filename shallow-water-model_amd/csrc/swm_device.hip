@@ -2458,12 +2458,20 @@ swm_status swm_total_mass(swm_ctx* C, double* mass) {
 // runs, timed with HIP events on the context's stream.  The output goes to a
 // scratch plane of the step; the state is not changed.
 swm_status swm_operator_bench(swm_ctx* C, int op, int reps, double* us_per_launch, double* bytes_per_launch) {
-  if (!C || op < 0 || op > 3 || reps < 1 || !us_per_launch || !bytes_per_launch) {
-    swm::set_error("swm_operator_bench: bad argument (op 0..3, reps >= 1)");
+  if (!C || op < 0 || op > 6 || reps < 1 || !us_per_launch || !bytes_per_launch) {
+    swm::set_error("swm_operator_bench: bad argument (op 0..6, reps >= 1)");
     return SWM_ERR_ARG;
   }
   if (C->precision != SWM_FP64 || !C->initialized) {
     swm::set_error("swm_operator_bench: needs an initialised FP64 context");
+    return SWM_ERR_STATE;
+  }
+  if ((op == 4 || op == 5) && (C->advection != SWM_ADV_REMAP || C->step_count < 1)) {
+    swm::set_error("swm_operator_bench: the remapping kernels need a remapping context after a step");
+    return SWM_ERR_STATE;
+  }
+  if (op == 6 && !C->diss) {
+    swm::set_error("swm_operator_bench: the dissipation kernel needs swm_init_diss_op");
     return SWM_ERR_STATE;
   }
   HIP_TRY(hipSetDevice(C->device));
@@ -2475,8 +2483,25 @@ swm_status swm_operator_bench(swm_ctx* C, int op, int reps, double* us_per_launc
   double* out = V.flx1;
   if (op == 3)   // the halo fill works in place on a copy of the thickness
     HIP_TRY(hipMemcpyAsync(out, V.h[lev], C->ns * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
+  RemapDev& D = C->rm;
+  const swm::dev::RemapArgs RA{C->iim, C->jjm, C->ns, C->nt, D.cellp, D.edgep, D.cornp, C->radius, D.limit_min,
+                               D.limit_max, D.flag_dev};
+  const unsigned nbs = nblk_b(C->ns, C->blk_step);
+  double* const* tr3 = C->nt ? D.ptab + (size_t)(C->n3 - 1) * C->nt : nullptr;
   auto launch = [&]() {
     switch (op) {
+      case 4:
+        hipLaunchKernelGGL(swm::dev::k_rm_transport, zgrid(C, nbs), dim3(C->blk_step), 0, C->stream, RA, D.dp, D.mx,
+                           D.my, D.mp, D.tx, D.ty, D.tp, D.mflux, D.mtflux);
+        break;
+      case 5:
+        hipLaunchKernelGGL(swm::dev::k_rm_construct, zgrid(C, nbs), dim3(C->blk_step), 0, C->stream, RA,
+                           V.h[C->n3 - 1], (const double* const*)tr3, D.mx, D.my, D.mp, D.tx, D.ty, D.tp);
+        break;
+      case 6:
+        hipLaunchKernelGGL(swm::dev::k_diss, dim3(nbs), dim3(C->blk_step), 0, C->stream, C->iim, C->jjm, C->ns, C->diss,
+                           C->cai, 1.0, V.u[lev], V.flx1);
+        break;
       case 0: hipLaunchKernelGGL(swm::dev::k_op_div_edge, dim3(nb), dim3(BLK), 0, C->stream, G, V.mass_flux, out); break;
       case 1: hipLaunchKernelGGL(swm::dev::k_curl_div, dim3(nb), dim3(BLK), 0, C->stream, G, V.u[lev], out,
                                  (double*)nullptr); break;
@@ -2496,9 +2521,22 @@ swm_status swm_operator_bench(swm_ctx* C, int op, int reps, double* us_per_launc
   // + 3 neighbour edges, area_inv, result (40 B); curl of a corner vector
   // 12 v_weights, 4 velocity components, area_inv, result (144 B);
   // Laplacian 7 weights, centre value, result (72 B); halo fill 16 B per
-  // ghost slot (value read and written)
+  // ghost slot (value read and written).  Remapping transports: 43 edge
+  // planes of 3 edges (1032 B), the departure points of 2 corners (48 B),
+  // the mass fields mp, mx, my (24 B) and per tracer tp, tx, ty (24 B) read,
+  // the mass and per tracer the mass-tracer transports of 3 edges written
+  // (24 B each): 1128 + 48 nt B.  Remapping fields: thickness and tracers
+  // (8 + 8 nt B), 32 cell planes (256 B) read, mx, my, mp and per tracer
+  // tx, ty, tp written (24 + 24 nt B): 288 + 32 nt B.  Dissipation: 4
+  // corner-vector planes and 36 geometry planes and 2 corner-area planes
+  // read, 4 planes written (368 B).
   static const double per_cell[3] = {40.0, 144.0, 72.0};
-  *bytes_per_launch = op < 3 ? per_cell[op] * (double)C->max_ig : 16.0 * (double)C->face.n;
+  const double nt = (double)C->nt;
+  if (op < 3) *bytes_per_launch = per_cell[op] * (double)C->max_ig;
+  else if (op == 3) *bytes_per_launch = 16.0 * (double)C->face.n;
+  else if (op == 4) *bytes_per_launch = (1128.0 + 48.0 * nt) * (double)C->max_ig;
+  else if (op == 5) *bytes_per_launch = (288.0 + 32.0 * nt) * (double)C->max_ig;
+  else *bytes_per_launch = 368.0 * (double)C->max_ig;
   return SWM_OK;
 }
 

@@ -790,7 +790,8 @@ class Model:
         (False)."""
         _check(lib.swm_set_linearisation(self._h, 1 if exact else 0))
 
-    OPERATOR_BENCH = ("div_of_an_edge_flux", "curl_of_a_vector", "laplacian_of_a_scalar", "face_halo_fill")
+    OPERATOR_BENCH = ("div_of_an_edge_flux", "curl_of_a_vector", "laplacian_of_a_scalar", "face_halo_fill",
+                      "remap_transport", "remap_fields", "dissipation")
 
     def operator_bench(self, op, reps=200):
         """Microbench of one operator on the resident state (FP64): op is a

@@ -206,9 +206,26 @@ def test_operator_bench_leaves_the_state_and_reports_bytes():
     per = {"div_of_an_edge_flux": 40.0 * grid.max_ig, "curl_of_a_vector": 144.0 * grid.max_ig,
            "laplacian_of_a_scalar": 72.0 * grid.max_ig,
            "face_halo_fill": 16.0 * len(grid.array("face_dst"))}
-    for op in S.Model.OPERATOR_BENCH:
+    for op in S.Model.OPERATOR_BENCH[:4]:
         us, nbytes = m.operator_bench(op, reps=5)
         assert us > 0.0 and nbytes == per[op], (op, us, nbytes)
+    # the remapping and dissipation kernels need their contexts
+    for op in ("remap_transport", "remap_fields", "dissipation"):
+        with pytest.raises(S.SwmError):
+            m.operator_bench(op, reps=5)
+    rm = S.Model(grid, ntracer=2, dt=400.0, grav=S.REF_GRAV, advection="remap")
+    rm.init_synthetic(lmax=8)
+    rm.step(2)
+    rm.init_diss_op()
+    rb = {n: rm.field(n) for n in ("h", "u", "tracer")}
+    per_rm = {"remap_transport": (1128.0 + 48.0 * 2) * grid.max_ig, "remap_fields": (288.0 + 32.0 * 2) * grid.max_ig,
+              "dissipation": 368.0 * grid.max_ig}
+    for op, nb in per_rm.items():
+        us, nbytes = rm.operator_bench(op, reps=5)
+        assert us > 0.0 and nbytes == nb, (op, us, nbytes)
+    for n, x in rb.items():
+        assert np.array_equal(rm.field(n), x), n
+    rm.close()
     for n, x in before.items():
         assert np.array_equal(m.field(n), x), n
     mix = S.Model(grid, ntracer=1, dt=400.0, grav=S.REF_GRAV, precision=S.SWM_MIXED)
