@@ -186,6 +186,19 @@ def adjoint_bench(S, g=7, hours=6.0, every=16, ndir=8, advection="fct"):
     a = sum(float(np.sum(u * v)) for u, v in zip(tl, y))
     b = sum(float(np.sum(u * v)) for u, v in zip(dc, m.adj_run(*y, control=True)))
     below, edges = m.limiter_stats()
+    # the same window with the tracer carried (tracer FCT, frozen limiter)
+    ts = (grid.iim, grid.jjm, 1, 10)
+    t0 = time.perf_counter()
+    m.adj_run(lh, control=True, ltracer=np.zeros(ts))
+    t_adj_tr = (time.perf_counter() - t0) * 1e3
+    xt = 0.01 * rng.standard_normal(ts)
+    yt = 100.0 * rng.standard_normal(ts)
+    t0 = time.perf_counter()
+    tlt = m.tl_run(control=dc, dtracer=xt)
+    t_tl_tr = (time.perf_counter() - t0) * 1e3
+    gtr = m.adj_run(*y, control=True, ltracer=yt)
+    a_tr = sum(float(np.sum(u * v)) for u, v in zip(tlt, y + [yt]))
+    b_tr = sum(float(np.sum(u * v)) for u, v in zip(dc + [xt], gtr))
     # the gradient check on the smooth forward (3 V-cycles per solve)
     mf = model(3)
     mf.init_vorticity(*control)
@@ -204,6 +217,10 @@ def adjoint_bench(S, g=7, hours=6.0, every=16, ndir=8, advection="fct"):
             "vcycles_per_solve": ref.solver_stats()[1] / max(1, ref.solver_stats()[0]),
             "dot_product_rel": abs(a - b) / abs(a),
             "limiter_active_edge_steps": below, "edge_steps": edges,
+            "with_tracer": {"adjoint_ms": t_adj_tr, "tl_ms": t_tl_tr, "adj_over_fwd": t_adj_tr / t_fwd,
+                            "dot_product_rel": abs(a_tr - b_tr) / abs(a_tr),
+                            "note": "the window with its tracer carried through the linearised model "
+                                    "(swm_tl_run_tracer / swm_adj_run_tracer; tracer FCT with its limiter frozen)"},
             "fd_rel_errors": [r[2] for r in fd],
             "note": "forward at the reference criterion (mean(r^2) <= 1e-20); the adjoint is the exact transpose of "
                     "that forward's own cycles (recomputation included in adjoint_ms); finite differences: "
@@ -628,7 +645,9 @@ def main():
         adj = adjoint_bench(S)
         print(f"[bench] adjoint {adj['adjoint_ms']:.0f} ms vs forward {adj['forward_ms']:.0f} ms "
               f"(ratio {adj['adj_over_fwd']:.2f}), dot {adj['dot_product_rel']:.1e}, "
-              f"fd max {max(adj['fd_rel_errors']):.1e}", file=sys.stderr, flush=True)
+              f"fd max {max(adj['fd_rel_errors']):.1e}; with the tracer {adj['with_tracer']['adjoint_ms']:.0f} ms "
+              f"(ratio {adj['with_tracer']['adj_over_fwd']:.2f}), dot {adj['with_tracer']['dot_product_rel']:.1e}",
+              file=sys.stderr, flush=True)
 
     ensemble = None
     if not args.no_ensemble:

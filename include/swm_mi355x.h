@@ -415,7 +415,7 @@ swm_status swm_total_mass(swm_ctx* ctx, double* mass);
  * 114-212) with any of the three advection schemes: centred, upstream, or
  * flux-corrected transport (mass_fct.F) linearised with the limiter's
  * coefficients and the upstream directions frozen at the base trajectory.
- * Tracers are not part of it (they do not feed back).  Hand-written TL and
+ * Tracers are carried by the *_tracer entry points below.  Hand-written TL and
  * AD kernels cover every operator of the step; halo fills transpose to
  * accumulations into their sources; the elliptic solve's transpose reuses
  * the solver through the symmetry of area times Laplacian.  The trajectory
@@ -471,6 +471,36 @@ swm_status swm_set_linearisation(swm_ctx* ctx, int exact);
  * last swm_tl_run / swm_adj_run recomputed, and the number of edge-steps
  * (three edges per interior slot and step).  Both 0 for other schemes. */
 swm_status swm_adj_limiter_stats(swm_ctx* ctx, long long* below_one, long long* edges);
+
+/* The tracers in the linearised model.  Tracers do not feed back on the
+ * dynamics, but depend on it through the mass flux and the thickness; these
+ * entry points carry them through the window: the tracer transport of each
+ * scheme (step.F:611-678 with tracer_fct.F, its limiter's coefficients
+ * frozen at the base trajectory as for the mass; step.F:278-327 and the
+ * tracer fluxes of operators.F:1841-1991 for the centred and upstream
+ * schemes).  Tracer arrays are (iim,jjm,ntracer,10); the context must hold
+ * tracers.  The plain swm_tl_run / swm_adj_run leave the tracers out. */
+/* swm_init_vorticity with the initial tracers (both time levels) set to
+ * `tracer` (NULL: zero, as swm_init_vorticity). */
+swm_status swm_init_vorticity_tracer(swm_ctx* ctx, const double* zeta, const double* delta, const double* h,
+                                     const double* tracer);
+/* swm_tl_run with in_tr the tracers' perturbation (mode 0: the tracers read
+ * by the first step; mode 1: the initial tracers of the control) and out_tr
+ * the tracers' perturbation after the last step. */
+swm_status swm_tl_run_tracer(swm_ctx* ctx, int mode, const double* in0, const double* in1, const double* in2,
+                             const double* in3, const double* in_tr, double* out_h, double* out_eta,
+                             double* out_div, double* out_u, double* out_tr);
+/* swm_adj_run with ltr the adjoint of the tracers after the last step and
+ * out_tr the adjoint of the tracers read by the first step (mode 0) or the
+ * gradient with respect to the initial tracers (mode 1). */
+swm_status swm_adj_run_tracer(swm_ctx* ctx, int mode, const double* lh, const double* leta, const double* ldiv,
+                              const double* lu, const double* ltr, double* out0, double* out1, double* out2,
+                              double* out3, double* out_tr);
+/* FCT tracer runs: edges with a frozen tracer limiter coefficient below 1
+ * (tracer_fct.F:539-593 acting: there the frozen linearisation is not the
+ * derivative) over the base steps of the last tracer run, and the number
+ * of edge-steps (three per interior slot, tracer and step). */
+swm_status swm_adj_tracer_limiter_stats(swm_ctx* ctx, long long* below_one, long long* edges);
 
 /* ------------------------------------------------------------------------
  * Restart files: the direct-access format of utilities/layer_2/io_utilities.F

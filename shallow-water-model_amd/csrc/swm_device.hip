@@ -150,6 +150,7 @@ struct swm_ctx {
   long long epoch = 0;             // changes whenever captured launches would differ (initialisation, settings)
   double* track = nullptr;
   double* fct_rec = nullptr;       // FP64 FCT step: the limiter coefficients go here (linearised model's base steps)
+  double* tr_rec = nullptr;        // FP64 FCT step: the tracer limiter coefficients (3 planes per tracer, 6 planes apart)
   double* rmean = nullptr;         // mixed path: sums of area * rhs of the two fields (mean removal)
   double* part_rhs = nullptr;      // mixed path: their per-block partial sums, from the tendencies kernel         // per member: max mean(r^2) of the two fields, count of solves above the criterion
   // FP64 metrics (both paths: set-up, FP64 residual, sums)
@@ -1058,6 +1059,11 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
       hipLaunchKernelGGL(swm::dev::k_tr_limiter<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, t3, V.ml, V.tlow,
                          V.mss_low, V.f1, V.f2, V.anti, V.bdn, V.bup);
       gather<R>(C, C->face, {V.bdn, V.bup});
+      if constexpr (sizeof(R) == 8) {
+        if (C->tr_rec)
+          hipLaunchKernelGGL(swm::dev::k_tr_coef, zgrid(C, nb), dim3(C->blk_step), 0, St, G, V.bdn, V.bup, V.anti,
+                             C->tr_rec + (size_t)(6 * q + 3) * ns);
+      }
       hipLaunchKernelGGL(swm::dev::k_tr_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, V.bdn, V.bup, V.anti);
       gather<R>(C, C->edge, {V.anti});
       hipLaunchKernelGGL(swm::dev::k_tr_final<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, V.ml, V.anti, V.h[n4],
@@ -3221,18 +3227,20 @@ swm_status swm_multigrid(swm_ctx* C, int nvar, const double* beta, double* alpha
 }  // extern "C"
 
 // ===================================================================
-// Tangent-linear and adjoint model of a window of FP64 steps with centred
-// advection (the kernels are in swm_tlad.hip).
+// Tangent-linear and adjoint model of a window of FP64 steps with any of
+// the three advection schemes, and optionally the tracers (the kernels are
+// in swm_tlad.hip).
 //
 // swm_adj_forward runs the window from the initial state, keeping a
 // snapshot of the model state every `every` steps (and at the end).  The
 // TL and AD sweeps then recompute one segment of base states at a time
-// from its snapshot (h, eta and u at the start of each step: the
-// coefficients of the linearised step) and run the linear steps over it,
-// forward for TL, backward for AD.  Elliptic solves of the linear steps go
-// to a relative residual of 1e-13, so that the TL and AD of the solve are
-// the inverse Laplacian and its transpose to that accuracy; the transpose
-// of the solve uses the symmetry of area times the Laplacian, L^T = A L A^-1.
+// from its snapshot (h, eta and u at the start of each step, and what the
+// scheme and the tracers need: the coefficients of the linearised step) and
+// run the linear steps over it, forward for TL, backward for AD.  Each
+// step's elliptic solve is linearised as the base step's own V-cycles
+// (exact_lin), or as a converged inverse solved to a relative residual of
+// 1e-13, whose transpose uses the symmetry of area times the Laplacian,
+// L^T = A L A^-1.
 // ===================================================================
 struct TlAd {
   // TL or AD state: two time levels of h, eta, div, u (4 planes), three
@@ -3264,6 +3272,16 @@ struct TlAd {
   double *mg_a[24] = {}, *mg_b[24] = {}, *mg_t[24] = {};
   double* coarse_mt = nullptr;
   int coarse_n = 0;
+  // tracers (the *_tracer entry points): per tracer two levels, the AB3
+  // extrapolation's oldest level (FCT) and three tendency history slots
+  // (centred, upstream); scratch for the fluxes; the frozen tracer limiter's
+  // count as for the mass
+  bool tracers = false;
+  double* trmem = nullptr;
+  std::vector<double*> tr[2], tro, trf[3];
+  double *tf1 = nullptr, *tf2 = nullptr, *tan = nullptr, *tml = nullptr;
+  unsigned long long* tlim_count = nullptr;
+  long long tlim_steps = 0;
 };
 
 namespace swm {
@@ -3277,6 +3295,8 @@ void tlad_free(swm_ctx* C) {
   if (T->lim_count) (void)hipFree(T->lim_count);
   if (T->mg_mem) (void)hipFree(T->mg_mem);
   if (T->coarse_mt) (void)hipFree(T->coarse_mt);
+  if (T->trmem) (void)hipFree(T->trmem);
+  if (T->tlim_count) (void)hipFree(T->tlim_count);
   delete T;
   C->tl = nullptr;
 }
@@ -3312,6 +3332,23 @@ swm_status tlad_alloc(swm_ctx* C) {
   hipLaunchKernelGGL(swm::dev::k_mul_mask, dim3(nblk(ns)), dim3(BLK), 0, C->stream, (int)ns, C->mask, C->area_inv,
                      C->mask, T->am_inv);
   T->snap_words = C->d.state_words + 2 * ns;
+  if (C->nt > 0) {
+    const size_t tw = ns * (6 * (size_t)C->nt + 10);
+    HIP_TRY(hipMalloc(&T->trmem, tw * sizeof(double)));
+    HIP_TRY(hipMalloc(&T->tlim_count, sizeof(unsigned long long)));
+    HIP_TRY(hipMemsetAsync(T->tlim_count, 0, sizeof(unsigned long long), C->stream));
+    double* t = T->trmem;
+    for (int q = 0; q < C->nt; ++q) {
+      for (int l = 0; l < 2; ++l) { T->tr[l].push_back(t); t += ns; }
+      T->tro.push_back(t); t += ns;
+      for (int l = 0; l < 3; ++l) { T->trf[l].push_back(t); t += ns; }
+    }
+    T->tf1 = t; t += 3 * ns;
+    T->tf2 = t; t += 3 * ns;
+    T->tan = t; t += 3 * ns;
+    T->tml = t;
+    fill<double>(C, T->trmem, tw, 0.0);
+  }
   HIP_TRY(hipStreamSynchronize(C->stream));
   return SWM_OK;
 }
@@ -3320,6 +3357,7 @@ void clear_state(swm_ctx* C) {
   TlAd* T = C->tl;
   const size_t ns = C->ns;
   fill<double>(C, T->h[0], (size_t)ns * 2 * (1 + 1 + 1 + 4) + 18 * ns, 0.0);   // the state planes are contiguous
+  if (T->trmem) fill<double>(C, T->trmem, ns * (6 * (size_t)C->nt + 10), 0.0);
 }
 
 swm_status snapshot(swm_ctx* C, double* dst) {
@@ -3357,6 +3395,12 @@ double* seg_e(swm_ctx* C, int q) { return seg_h(C, q) + C->ns; }
 double* seg_u(swm_ctx* C, int q) { return seg_h(C, q) + 2 * (size_t)C->ns; }
 double* seg_c(swm_ctx* C, int q) { return seg_h(C, q) + 6 * (size_t)C->ns; }
 double* seg_F(swm_ctx* C, int q) { return seg_h(C, q) + 9 * (size_t)C->ns; }
+// tracer tq of base state q (tracer runs): the level the step reads (t3),
+// the older level (t4) and the AB3 extrapolation's oldest (told), then
+// (FCT) the step's three frozen limiter coefficient planes
+double* seg_tr(swm_ctx* C, int q, int tq) {
+  return seg_h(C, q) + (size_t)((C->advection == SWM_ADV_FCT ? 12 : 6) + 6 * tq) * C->ns;
+}
 
 swm_status record_base(swm_ctx* C, int q) {
   const int n3 = C->n4 - 1;   // the level the next step reads (after pointers of the last step: n4)
@@ -3364,6 +3408,15 @@ swm_status record_base(swm_ctx* C, int q) {
   HIP_TRY(hipMemcpyAsync(seg_h(C, q), C->d.h[n3], ns * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
   HIP_TRY(hipMemcpyAsync(seg_e(C, q), C->d.eta[n3], ns * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
   HIP_TRY(hipMemcpyAsync(seg_u(C, q), C->d.u[n3], 4 * ns * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
+  if (C->tl->tracers) {
+    const int n4 = C->n3 - 1;   // the other level: the next step's older one
+    for (int tq = 0; tq < C->nt; ++tq) {
+      double* b = seg_tr(C, q, tq);
+      HIP_TRY(hipMemcpyAsync(b, C->d.tr[n3][tq], ns * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
+      HIP_TRY(hipMemcpyAsync(b + ns, C->d.tr[n4][tq], ns * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
+      HIP_TRY(hipMemcpyAsync(b + 2 * ns, C->d.told[tq], ns * sizeof(double), hipMemcpyDeviceToDevice, C->stream));
+    }
+  }
   return SWM_OK;
 }
 
@@ -3379,9 +3432,19 @@ swm_status recompute_segment(swm_ctx* C, int sidx, int len) {
     // FCT: the step records its limiter coefficients (step_dt), then its
     // total mass flux is kept
     if (fct) C->fct_rec = seg_c(C, q);
+    if (fct && T->tracers) C->tr_rec = seg_tr(C, q, 0);
     st = base_step(C);
     C->fct_rec = nullptr;
+    C->tr_rec = nullptr;
     if (st) return st;
+    if (fct && T->tracers) {
+      for (int tq = 0; tq < C->nt; ++tq) {
+        const size_t n3 = 3 * (size_t)C->ns;
+        hipLaunchKernelGGL(swm::dev::k_count_below_one, dim3((unsigned)((n3 + BLK - 1) / BLK)), dim3(BLK), 0,
+                           C->stream, n3, (const double*)(seg_tr(C, q, tq) + 3 * (size_t)C->ns), T->tlim_count);
+      }
+      T->tlim_steps += 1;
+    }
     T->sched[q] = C->last_sched;
     if (fct) {
       HIP_TRY(hipMemcpyAsync(seg_F(C, q), C->d.mass_flux, 3 * (size_t)C->ns * sizeof(double), hipMemcpyDeviceToDevice,
@@ -3614,6 +3677,94 @@ void solve_sched_ad(swm_ctx* C, double* xb, double* bb, const swm_ctx::SolveSche
   for (int it = 0; it < sc.first; ++it) vcycle_ad(C, 2, xb, bb);
 }
 
+// ------------------------------------------------------- tracer transport
+// TL of the step's tracer transport (after the new thickness): FCT with the
+// base step's frozen tracer limiter (tracer_fct.F), or the centred /
+// upstream tracer fluxes and step_tracer (step.F:278-327, 495-602); bF is
+// the base mass flux the tracers take (FCT: the recorded total flux)
+void tl_tracers(swm_ctx* C, int q, const double* bF) {
+  TlAd* T = C->tl;
+  const size_t ns = C->ns;
+  const unsigned nb = nblk(ns);
+  hipStream_t S = C->stream;
+  const swm::dev::Geo& G = C->d.geo;
+  const int n3 = C->n3 - 1, n4 = C->n4 - 1;
+  const int a = C->n3f - 1, b = C->n2f - 1, c = C->n1f - 1;
+  const double dt = C->dt, w3 = C->w3, w2 = C->w2, w1 = C->w1;
+  for (int tq = 0; tq < C->nt; ++tq) {
+    const double* bt = seg_tr(C, q, tq);
+    const double* bt4n = seg_tr(C, q + 1, tq);
+    if (C->advection == SWM_ADV_FCT) {
+      hipLaunchKernelGGL(swm::dev::k_tl_tr_fluxes, dim3(nb), dim3(BLK), 0, S, G, w3, w2, w1, bF, bt, bt + ns,
+                         bt + 2 * ns, T->F, T->tr[n3][tq], T->tr[n4][tq], T->tro[tq], T->tf1, T->tf2);
+      gather<double>(C, C->edge, {T->tf1, T->tf2});
+      hipLaunchKernelGGL(swm::dev::k_tl_tr_low, dim3(nb), dim3(BLK), 0, S, G, dt, seg_h(C, q), bt, T->h[n3],
+                         T->tr[n3][tq], T->tf1, T->tml);
+      hipLaunchKernelGGL(swm::dev::k_tl_tr_anti, dim3(nblk(3 * ns)), dim3(BLK), 0, S, 3 * ns,
+                         (const double*)(bt + 3 * ns), T->tf1, T->tf2, T->tan);
+      gather<double>(C, C->edge, {T->tan});
+      hipLaunchKernelGGL(swm::dev::k_tl_tr_final, dim3(nb), dim3(BLK), 0, S, G, dt, seg_h(C, q + 1), bt4n, T->tml,
+                         T->tan, T->h[n4], T->tr[n4][tq], T->tro[tq]);
+    } else {
+      if (C->advection == SWM_ADV_CENTERED)
+        hipLaunchKernelGGL(swm::dev::k_tl_tr_flux<true>, dim3(nb), dim3(BLK), 0, S, G, bF, bt, T->F, T->tr[n3][tq],
+                           T->tf1);
+      else
+        hipLaunchKernelGGL(swm::dev::k_tl_tr_flux<false>, dim3(nb), dim3(BLK), 0, S, G, bF, bt, T->F, T->tr[n3][tq],
+                           T->tf1);
+      gather<double>(C, C->edge, {T->tf1});
+      hipLaunchKernelGGL(swm::dev::k_tl_tracer_update, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, T->tf1, bt,
+                         seg_h(C, q), seg_h(C, q + 1), bt4n, T->tr[n3][tq], T->h[n3], T->h[n4], T->trf[a][tq],
+                         T->trf[b][tq], T->trf[c][tq], T->tr[n4][tq]);
+    }
+  }
+}
+
+// transpose of tl_tracers: adds into the mass flux's, the thicknesses' and
+// the tracer levels' adjoints
+void ad_tracers(swm_ctx* C, int q, const double* bF) {
+  TlAd* T = C->tl;
+  const size_t ns = C->ns;
+  const unsigned nb = nblk(ns);
+  hipStream_t S = C->stream;
+  const swm::dev::Geo& G = C->d.geo;
+  const int n3 = C->n3 - 1, n4 = C->n4 - 1;
+  const int a = C->n3f - 1, b = C->n2f - 1, c = C->n1f - 1;
+  const double dt = C->dt, w3 = C->w3, w2 = C->w2, w1 = C->w1;
+  for (int tq = 0; tq < C->nt; ++tq) {
+    const double* bt = seg_tr(C, q, tq);
+    const double* bt4n = seg_tr(C, q + 1, tq);
+    if (C->advection == SWM_ADV_FCT) {
+      fill<double>(C, T->tml, ns, 0.0);
+      fill<double>(C, T->tan, 3 * ns, 0.0);
+      fill<double>(C, T->tf1, 3 * ns, 0.0);
+      fill<double>(C, T->tf2, 3 * ns, 0.0);
+      hipLaunchKernelGGL(swm::dev::k_ad_tr_final, dim3(nb), dim3(BLK), 0, S, G, dt, seg_h(C, q + 1), bt4n,
+                         T->tr[n4][tq], T->tro[tq], T->tml, T->tan, T->h[n4]);
+      gather_adj(C, C->edge, {T->tan});
+      hipLaunchKernelGGL(swm::dev::k_ad_tr_anti, dim3(nblk(3 * ns)), dim3(BLK), 0, S, 3 * ns,
+                         (const double*)(bt + 3 * ns), T->tan, T->tf1, T->tf2);
+      hipLaunchKernelGGL(swm::dev::k_ad_tr_low, dim3(nb), dim3(BLK), 0, S, G, dt, seg_h(C, q), bt, T->tml, T->h[n3],
+                         T->tr[n3][tq], T->tf1);
+      gather_adj(C, C->edge, {T->tf1, T->tf2});
+      hipLaunchKernelGGL(swm::dev::k_ad_tr_fluxes, dim3(nb), dim3(BLK), 0, S, G, w3, w2, w1, bF, bt, bt + ns,
+                         bt + 2 * ns, T->tf1, T->tf2, T->F, T->tr[n3][tq], T->tr[n4][tq], T->tro[tq]);
+    } else {
+      fill<double>(C, T->tf1, 3 * ns, 0.0);
+      hipLaunchKernelGGL(swm::dev::k_ad_tracer_update, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, bt, seg_h(C, q),
+                         seg_h(C, q + 1), bt4n, T->tr[n4][tq], T->trf[a][tq], T->trf[b][tq], T->trf[c][tq],
+                         T->tr[n3][tq], T->h[n3], T->h[n4], T->tf1);
+      gather_adj(C, C->edge, {T->tf1});
+      if (C->advection == SWM_ADV_CENTERED)
+        hipLaunchKernelGGL(swm::dev::k_ad_tr_flux<true>, dim3(nb), dim3(BLK), 0, S, G, bF, bt, T->tf1, T->F,
+                           T->tr[n3][tq]);
+      else
+        hipLaunchKernelGGL(swm::dev::k_ad_tr_flux<false>, dim3(nb), dim3(BLK), 0, S, G, bF, bt, T->tf1, T->F,
+                           T->tr[n3][tq]);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- TL step
 // window step with base state q -> q+1; pointers / weights of the model step
 swm_status tl_step(swm_ctx* C, int q) {
@@ -3646,6 +3797,7 @@ swm_status tl_step(swm_ctx* C, int q) {
     hipLaunchKernelGGL(swm::dev::k_mass_final<double>, dim3(nb), dim3(BLK), 0, S, G, dt, T->fl, T->an, T->ml, T->F,
                        T->h[n4], swm::dev::TrFluxArgs<double>{});
     gather<double>(C, C->face, {T->h[n4]});
+    if (T->tracers) tl_tracers(C, q, bF);
   } else {
     if (C->advection == SWM_ADV_CENTERED)
       hipLaunchKernelGGL(swm::dev::k_tl_mass_flux, dim3(nb), dim3(BLK), 0, S, G, seg_u(C, q), T->bhc, du3, T->hc, T->F);
@@ -3655,6 +3807,7 @@ swm_status tl_step(swm_ctx* C, int q) {
     gather<double>(C, C->edge, {T->F});
     hipLaunchKernelGGL(swm::dev::k_tl_height, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, T->F, dh3, T->hf[a],
                        T->hf[b], T->hf[c], seg_h(C, q + 1), T->h[n4]);
+    if (T->tracers) tl_tracers(C, q, bF);
   }
   hipLaunchKernelGGL(swm::dev::k_tl_vort_curl, dim3(nb), dim3(BLK), 0, S, G, bF, seg_h(C, q), seg_e(C, q), T->bhc,
                      seg_u(C, q), T->F, dh3, de3, T->hc, du3, T->vf, T->dc);
@@ -3674,6 +3827,11 @@ swm_status tl_step(swm_ctx* C, int q) {
   // wrap_prognostics
   gather<double>(C, C->face, {T->h[n4], T->eta[n4], T->div[n4]});
   gather<double>(C, C->vrtx, {T->u[0], T->u[0] + 2 * ns, T->u[1], T->u[1] + 2 * ns});
+  if (T->tracers) {
+    std::vector<double*> f;
+    for (int tq = 0; tq < C->nt; ++tq) f.push_back(T->tr[n4][tq]);
+    gather_list<double>(C, C->face, f);
+  }
   return SWM_OK;
 }
 
@@ -3690,6 +3848,8 @@ swm_status ad_step(swm_ctx* C, int q) {
   const double dt = C->dt, w3 = C->w3, w2 = C->w2, w1 = C->w1;
   const double* bF = base_coeffs(C, q);
   // wrap_prognostics
+  if (T->tracers)
+    for (int tq = 0; tq < C->nt; ++tq) gather_adj(C, C->face, {T->tr[n4][tq]});
   gather_adj(C, C->vrtx, {T->u[0], T->u[0] + 2 * ns, T->u[1], T->u[1] + 2 * ns});
   gather_adj(C, C->face, {T->h[n4], T->eta[n4], T->div[n4]});
   // u4 = W velocity(psi, chi)
@@ -3742,6 +3902,8 @@ swm_status ad_step(swm_ctx* C, int q) {
   fill<double>(C, T->hc, 2 * ns, 0.0);
   hipLaunchKernelGGL(swm::dev::k_ad_vort_curl, dim3(nb), dim3(BLK), 0, S, G, bF, seg_h(C, q), seg_e(C, q), T->bhc,
                      seg_u(C, q), T->vf, T->dc, T->F, T->h[n3], T->eta[n3], T->hc, T->u[n3]);
+  // the tracers read the new thickness and the mass flux
+  if (T->tracers) ad_tracers(C, q, bF);
   if (C->advection == SWM_ADV_FCT) {
     // transpose of the mass FCT with the frozen limiter (tl_step)
     gather_adj(C, C->face, {T->h[n4]});
@@ -3794,7 +3956,9 @@ void set_step(swm_ctx* C, int n) {
 // control (zeta, delta, h) -> state: (psi, chi) = L^-1 P (zeta, delta),
 // u = W velocity(psi, chi), the model's initialisation from h and u
 // (finish_init): both levels, eta = f + W curl(u), div = W div(u)
-swm_status init_tl(swm_ctx* C, const double* dz, const double* dd, const double* dh) {
+// dtr: the initial tracers' perturbation (device planes, tracer runs; both
+// levels, as finish_init copies them)
+swm_status init_tl(swm_ctx* C, const double* dz, const double* dd, const double* dh, const double* dtr) {
   TlAd* T = C->tl;
   const size_t ns = C->ns;
   const unsigned nb = nblk(ns);
@@ -3824,16 +3988,29 @@ swm_status init_tl(swm_ctx* C, const double* dz, const double* dd, const double*
   // initialisation: step_count 0)
   gather<double>(C, C->face, {T->h[1], T->eta[1], T->div[1]});
   gather<double>(C, C->vrtx, {T->u[0], T->u[0] + 2 * ns, T->u[1], T->u[1] + 2 * ns});
+  if (T->tracers && dtr)
+    for (int tq = 0; tq < C->nt; ++tq) {
+      for (int l = 0; l < 2; ++l)
+        HIP_TRY(hipMemcpyAsync(T->tr[l][tq], dtr + (size_t)tq * ns, ns * sizeof(double), hipMemcpyDeviceToDevice, S));
+      gather<double>(C, C->face, {T->tr[1][tq]});
+    }
   return SWM_OK;
 }
 
-// transpose of init_tl into gz, gd, gh (device planes)
-swm_status init_ad(swm_ctx* C, double* gz, double* gd, double* gh) {
+// transpose of init_tl into gz, gd, gh and (tracer runs) gtr (device planes)
+swm_status init_ad(swm_ctx* C, double* gz, double* gd, double* gh, double* gtr) {
   TlAd* T = C->tl;
   const size_t ns = C->ns;
   const unsigned nb = nblk(ns);
   hipStream_t S = C->stream;
   const swm::dev::Geo& G = C->d.geo;
+  if (T->tracers && gtr)
+    for (int tq = 0; tq < C->nt; ++tq) {
+      gather_adj(C, C->face, {T->tr[1][tq]});
+      double* g = gtr + (size_t)tq * ns;
+      HIP_TRY(hipMemcpyAsync(g, T->tr[0][tq], ns * sizeof(double), hipMemcpyDeviceToDevice, S));
+      hipLaunchKernelGGL(swm::dev::k_axpy, dim3(nb), dim3(BLK), 0, S, ns, 1.0, T->tr[1][tq], g);
+    }
   gather_adj(C, C->vrtx, {T->u[0], T->u[0] + 2 * ns, T->u[1], T->u[1] + 2 * ns});
   gather_adj(C, C->face, {T->h[1], T->eta[1], T->div[1]});
   hipLaunchKernelGGL(swm::dev::k_axpy, dim3(nb), dim3(BLK), 0, S, ns, 1.0, T->eta[1], T->eta[0]);
@@ -3870,12 +4047,16 @@ swm_status init_ad(swm_ctx* C, double* gz, double* gd, double* gh) {
   return SWM_OK;
 }
 
-// nonlinear initial state from the control (the base of init_tl)
-swm_status init_from_vorticity(swm_ctx* C, const double* z, const double* d, const double* h) {
+// nonlinear initial state from the control (the base of init_tl); tr the
+// initial tracers (device planes, or null: zero)
+swm_status init_from_vorticity(swm_ctx* C, const double* z, const double* d, const double* h, const double* tr) {
   TlAd* T = C->tl;
   const size_t ns = C->ns;
   hipStream_t S = C->stream;
   reset_state(C, true);
+  if (tr)
+    for (int tq = 0; tq < C->nt; ++tq)
+      HIP_TRY(hipMemcpyAsync(C->d.tr[0][tq], tr + (size_t)tq * ns, ns * sizeof(double), hipMemcpyDeviceToDevice, S));
   HIP_TRY(hipMemcpyAsync(T->rhs, z, ns * sizeof(double), hipMemcpyDeviceToDevice, S));
   HIP_TRY(hipMemcpyAsync(T->rhs + ns, d, ns * sizeof(double), hipMemcpyDeviceToDevice, S));
   fill<double>(C, T->xs, 2 * ns, 0.0);
@@ -3914,6 +4095,39 @@ swm_status state_out(swm_ctx* C, int lev, double* h, double* e, double* d, doubl
   return SWM_OK;
 }
 
+// tracers (iim, jjm, ntracer, 10) <-> planes (one per tracer)
+void tracer_to_planes(const swm_ctx* C, const double* ref, double* planes) {
+  const size_t pan = (size_t)C->iim * C->jjm, ns = C->ns;
+  for (int q = 0; q < C->nt; ++q)
+    for (int k = 0; k < 10; ++k)
+      std::memcpy(planes + q * ns + k * pan, ref + ((size_t)q + (size_t)C->nt * k) * pan, pan * sizeof(double));
+}
+void tracer_from_planes(const swm_ctx* C, const double* planes, double* ref) {
+  const size_t pan = (size_t)C->iim * C->jjm, ns = C->ns;
+  for (int q = 0; q < C->nt; ++q)
+    for (int k = 0; k < 10; ++k)
+      std::memcpy(ref + ((size_t)q + (size_t)C->nt * k) * pan, planes + q * ns + k * pan, pan * sizeof(double));
+}
+swm_status tracer_in(swm_ctx* C, int lev, const double* tr) {
+  TlAd* T = C->tl;
+  const size_t ns = C->ns;
+  std::vector<double> tp((size_t)C->nt * ns);
+  tracer_to_planes(C, tr, tp.data());
+  for (int tq = 0; tq < C->nt; ++tq)
+    HIP_TRY(hipMemcpy(T->tr[lev][tq], tp.data() + tq * ns, ns * sizeof(double), hipMemcpyHostToDevice));
+  return SWM_OK;
+}
+swm_status tracer_out(swm_ctx* C, int lev, double* tr) {
+  TlAd* T = C->tl;
+  const size_t ns = C->ns;
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  std::vector<double> tp((size_t)C->nt * ns);
+  for (int tq = 0; tq < C->nt; ++tq)
+    HIP_TRY(hipMemcpy(tp.data() + tq * ns, T->tr[lev][tq], ns * sizeof(double), hipMemcpyDeviceToHost));
+  tracer_from_planes(C, tp.data(), tr);
+  return SWM_OK;
+}
+
 swm_status window_check(swm_ctx* C, const char* who) {
   if (C->nmem != 1) {
     swm::set_error(std::string(who) + ": the linearised model runs on single-member contexts");
@@ -3939,21 +4153,35 @@ int last_level(int nsteps) { return (nsteps + 1) % 2 + 1 - 1; }
 
 extern "C" {
 
-swm_status swm_init_vorticity(swm_ctx* C, const double* zeta, const double* delta, const double* h) {
-  if (!C || !zeta || !delta || !h || C->nmem != 1) { swm::set_error("swm_init_vorticity: bad argument"); return SWM_ERR_ARG; }
+swm_status swm_init_vorticity_tracer(swm_ctx* C, const double* zeta, const double* delta, const double* h,
+                                     const double* tracer) {
+  if (!C || !zeta || !delta || !h || C->nmem != 1) {
+    swm::set_error("swm_init_vorticity: bad argument");
+    return SWM_ERR_ARG;
+  }
   HIP_TRY(hipSetDevice(C->device));
   swm_status st;
   if ((st = swm::tlad_alloc(C))) return st;
   const size_t ns = C->ns;
+  const size_t nt = tracer ? (size_t)C->nt : 0;
   double* buf = nullptr;
-  HIP_TRY(hipMalloc(&buf, 3 * ns * sizeof(double)));
+  HIP_TRY(hipMalloc(&buf, (3 + nt) * ns * sizeof(double)));
   HIP_TRY(hipMemcpy(buf, zeta, ns * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(buf + ns, delta, ns * sizeof(double), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(buf + 2 * ns, h, ns * sizeof(double), hipMemcpyHostToDevice));
-  st = swm::init_from_vorticity(C, buf, buf + ns, buf + 2 * ns);
+  if (nt) {
+    std::vector<double> tp(nt * ns);
+    swm::tracer_to_planes(C, tracer, tp.data());
+    HIP_TRY(hipMemcpy(buf + 3 * ns, tp.data(), nt * ns * sizeof(double), hipMemcpyHostToDevice));
+  }
+  st = swm::init_from_vorticity(C, buf, buf + ns, buf + 2 * ns, nt ? buf + 3 * ns : nullptr);
   (void)hipStreamSynchronize(C->stream);
   (void)hipFree(buf);
   return st;
+}
+
+swm_status swm_init_vorticity(swm_ctx* C, const double* zeta, const double* delta, const double* h) {
+  return swm_init_vorticity_tracer(C, zeta, delta, h, nullptr);
 }
 
 swm_status swm_adj_forward(swm_ctx* C, int nsteps, int every) {
@@ -3978,7 +4206,7 @@ swm_status swm_adj_forward(swm_ctx* C, int nsteps, int every) {
     HIP_TRY(hipMalloc(&p, T->snap_words * sizeof(double)));
     T->ckpt.push_back(p);
   }
-  T->seg_planes = C->advection == SWM_ADV_FCT ? 12 : 6;
+  T->seg_planes = (C->advection == SWM_ADV_FCT ? 12 : 6) + 6 * C->nt;   // tracer runs: 6 planes per tracer
   HIP_TRY(hipMalloc(&T->seg, (size_t)(every + 1) * T->seg_planes * C->ns * sizeof(double)));
   T->every = every;
   T->nsteps = nsteps;
@@ -4008,39 +4236,49 @@ swm_status swm_adj_forward(swm_ctx* C, int nsteps, int every) {
 // TL of the window: mode 0, inputs the state perturbation (h, eta, div, u)
 // read by the first step; mode 1, inputs the control perturbation (zeta,
 // delta, h; in3 unused) through the initial state map.  Outputs the state
-// perturbation after the last step.
-swm_status swm_tl_run(swm_ctx* C, int mode, const double* in0, const double* in1, const double* in2,
-                      const double* in3, double* out_h, double* out_eta, double* out_div, double* out_u) {
-  if (!C || mode < 0 || mode > 2 || !in0 || !in1 || !in2 || (mode == 0 && !in3) || !out_h || !out_eta ||
-      !out_div || !out_u) {
-    swm::set_error("swm_tl_run: bad argument");
-    return SWM_ERR_ARG;
-  }
+// perturbation after the last step.  With in_tr / out_tr (tracer runs) the
+// tracers' perturbations go in and out too.
+static swm_status tl_run_impl(swm_ctx* C, int mode, const double* in0, const double* in1, const double* in2,
+                              const double* in3, const double* in_tr, double* out_h, double* out_eta,
+                              double* out_div, double* out_u, double* out_tr) {
   if (!C->tl || !C->tl->have_window) { swm::set_error("swm_tl_run: no window (swm_adj_forward first)"); return SWM_ERR_STATE; }
   HIP_TRY(hipSetDevice(C->device));
   TlAd* T = C->tl;
   const size_t ns = C->ns;
   swm_status st;
+  T->tracers = in_tr != nullptr;
   swm::clear_state(C);
   HIP_TRY(hipMemsetAsync(T->lim_count, 0, sizeof(unsigned long long), C->stream));
   T->lim_steps = 0;
+  if (T->tlim_count) HIP_TRY(hipMemsetAsync(T->tlim_count, 0, sizeof(unsigned long long), C->stream));
+  T->tlim_steps = 0;
   fill<double>(C, T->xs, 2 * ns, 0.0);
   if (mode == 0) {
     if ((st = swm::state_in(C, swm::first_level(), in0, in1, in2, in3))) return st;
+    if (T->tracers && (st = swm::tracer_in(C, swm::first_level(), in_tr))) return st;
   } else {   // 1, or 2: the initial state map alone
+    const size_t nt = T->tracers ? (size_t)C->nt : 0;
     double* buf = nullptr;
-    HIP_TRY(hipMalloc(&buf, 3 * ns * sizeof(double)));
+    HIP_TRY(hipMalloc(&buf, (3 + nt) * ns * sizeof(double)));
     HIP_TRY(hipMemcpy(buf, in0, ns * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(buf + ns, in1, ns * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(buf + 2 * ns, in2, ns * sizeof(double), hipMemcpyHostToDevice));
-    st = swm::init_tl(C, buf, buf + ns, buf + 2 * ns);
+    if (nt) {
+      std::vector<double> tp(nt * ns);
+      swm::tracer_to_planes(C, in_tr, tp.data());
+      HIP_TRY(hipMemcpy(buf + 3 * ns, tp.data(), nt * ns * sizeof(double), hipMemcpyHostToDevice));
+    }
+    st = swm::init_tl(C, buf, buf + ns, buf + 2 * ns, nt ? buf + 3 * ns : nullptr);
     (void)hipStreamSynchronize(C->stream);
     (void)hipFree(buf);
     if (st) return st;
     // exact linearisation: xs, the initial solve's perturbation, is the
     // first step's warm start; else the solves do not depend on it
     if (!swm::exact_lin(C)) fill<double>(C, T->xs, 2 * ns, 0.0);
-    if (mode == 2) return swm::state_out(C, swm::first_level(), out_h, out_eta, out_div, out_u);
+    if (mode == 2) {
+      if ((st = swm::state_out(C, swm::first_level(), out_h, out_eta, out_div, out_u))) return st;
+      return T->tracers ? swm::tracer_out(C, swm::first_level(), out_tr) : SWM_OK;
+    }
   }
   const int nseg = (T->nsteps + T->every - 1) / T->every;
   for (int sg = 0; sg < nseg; ++sg) {
@@ -4053,6 +4291,7 @@ swm_status swm_tl_run(swm_ctx* C, int mode, const double* in0, const double* in1
   }
   const int lev = swm::last_level(T->nsteps);
   if ((st = swm::state_out(C, lev, out_h, out_eta, out_div, out_u))) return st;
+  if (T->tracers && (st = swm::tracer_out(C, lev, out_tr))) return st;
   // the model back at the window's end
   const int nck = (int)T->ckpt.size();
   if ((st = swm::restore(C, T->ckpt[nck - 1], T->ckpt_meta[2 * nck - 2], T->ckpt_meta[2 * nck - 1]))) return st;
@@ -4060,30 +4299,52 @@ swm_status swm_tl_run(swm_ctx* C, int mode, const double* in0, const double* in1
   return SWM_OK;
 }
 
+swm_status swm_tl_run(swm_ctx* C, int mode, const double* in0, const double* in1, const double* in2,
+                      const double* in3, double* out_h, double* out_eta, double* out_div, double* out_u) {
+  if (!C || mode < 0 || mode > 2 || !in0 || !in1 || !in2 || (mode == 0 && !in3) || !out_h || !out_eta ||
+      !out_div || !out_u) {
+    swm::set_error("swm_tl_run: bad argument");
+    return SWM_ERR_ARG;
+  }
+  return tl_run_impl(C, mode, in0, in1, in2, in3, nullptr, out_h, out_eta, out_div, out_u, nullptr);
+}
+
+swm_status swm_tl_run_tracer(swm_ctx* C, int mode, const double* in0, const double* in1, const double* in2,
+                             const double* in3, const double* in_tr, double* out_h, double* out_eta,
+                             double* out_div, double* out_u, double* out_tr) {
+  if (!C || mode < 0 || mode > 2 || !in0 || !in1 || !in2 || (mode == 0 && !in3) || !in_tr || !out_h ||
+      !out_eta || !out_div || !out_u || !out_tr || C->nt < 1) {
+    swm::set_error("swm_tl_run_tracer: bad argument (a context with tracers, every array given)");
+    return SWM_ERR_ARG;
+  }
+  return tl_run_impl(C, mode, in0, in1, in2, in3, in_tr, out_h, out_eta, out_div, out_u, out_tr);
+}
+
 // AD of the window: the adjoint of the state after the last step (lh, leta,
 // ldiv, lu) back to the first step (mode 0: out the adjoint state (h, eta,
 // div, u)) or to the control (mode 1: out0..2 the gradient with respect to
-// zeta, delta, h; out3 unused)
-swm_status swm_adj_run(swm_ctx* C, int mode, const double* lh, const double* leta, const double* ldiv,
-                       const double* lu, double* out0, double* out1, double* out2, double* out3) {
-  if (!C || mode < 0 || mode > 2 || !lh || !leta || !ldiv || !lu || !out0 || !out1 || !out2 ||
-      (mode == 0 && !out3)) {
-    swm::set_error("swm_adj_run: bad argument");
-    return SWM_ERR_ARG;
-  }
+// zeta, delta, h; out3 unused).  With ltr / out_tr (tracer runs) the tracers'
+// adjoints go in and out too.
+static swm_status adj_run_impl(swm_ctx* C, int mode, const double* lh, const double* leta, const double* ldiv,
+                               const double* lu, const double* ltr, double* out0, double* out1, double* out2,
+                               double* out3, double* out_tr) {
   if (!C->tl || !C->tl->have_window) { swm::set_error("swm_adj_run: no window (swm_adj_forward first)"); return SWM_ERR_STATE; }
   HIP_TRY(hipSetDevice(C->device));
   TlAd* T = C->tl;
   const size_t ns = C->ns;
   swm_status st;
   if (swm::exact_lin(C) && (st = swm::exact_prepare(C))) return st;
+  T->tracers = ltr != nullptr;
   swm::clear_state(C);
   HIP_TRY(hipMemsetAsync(T->lim_count, 0, sizeof(unsigned long long), C->stream));
   T->lim_steps = 0;
+  if (T->tlim_count) HIP_TRY(hipMemsetAsync(T->tlim_count, 0, sizeof(unsigned long long), C->stream));
+  T->tlim_steps = 0;
   fill<double>(C, T->xs, 2 * ns, 0.0);
   // mode 2: the initial state map alone, from the adjoint of its output
-  if ((st = swm::state_in(C, mode == 2 ? swm::first_level() : swm::last_level(T->nsteps), lh, leta, ldiv, lu)))
-    return st;
+  const int lev_in = mode == 2 ? swm::first_level() : swm::last_level(T->nsteps);
+  if ((st = swm::state_in(C, lev_in, lh, leta, ldiv, lu))) return st;
+  if (T->tracers && (st = swm::tracer_in(C, lev_in, ltr))) return st;
   const int nseg = mode == 2 ? 0 : (T->nsteps + T->every - 1) / T->every;
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int len = std::min(T->every, T->nsteps - sg * T->every);
@@ -4095,17 +4356,23 @@ swm_status swm_adj_run(swm_ctx* C, int mode, const double* lh, const double* let
   }
   if (mode == 0) {
     if ((st = swm::state_out(C, swm::first_level(), out0, out1, out2, out3))) return st;
+    if (T->tracers && (st = swm::tracer_out(C, swm::first_level(), out_tr))) return st;
   } else {
+    const size_t nt = T->tracers ? (size_t)C->nt : 0;
     double* buf = nullptr;
-    HIP_TRY(hipMalloc(&buf, 3 * ns * sizeof(double)));
-    st = swm::init_ad(C, buf, buf + ns, buf + 2 * ns);
+    HIP_TRY(hipMalloc(&buf, (3 + nt) * ns * sizeof(double)));
+    st = swm::init_ad(C, buf, buf + ns, buf + 2 * ns, nt ? buf + 3 * ns : nullptr);
     if (!st) {
       (void)hipStreamSynchronize(C->stream);
+      std::vector<double> tp(nt * ns);
       if (hipMemcpy(out0, buf, ns * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
           hipMemcpy(out1, buf + ns, ns * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
-          hipMemcpy(out2, buf + 2 * ns, ns * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+          hipMemcpy(out2, buf + 2 * ns, ns * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+          (nt && hipMemcpy(tp.data(), buf + 3 * ns, nt * ns * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)) {
         swm::set_error("swm_adj_run: copy of the gradient failed");
         st = SWM_ERR_HIP;
+      } else if (nt) {
+        swm::tracer_from_planes(C, tp.data(), out_tr);
       }
     }
     (void)hipFree(buf);
@@ -4115,6 +4382,27 @@ swm_status swm_adj_run(swm_ctx* C, int mode, const double* lh, const double* let
   if ((st = swm::restore(C, T->ckpt[nck - 1], T->ckpt_meta[2 * nck - 2], T->ckpt_meta[2 * nck - 1]))) return st;
   HIP_TRY(hipStreamSynchronize(C->stream));
   return SWM_OK;
+}
+
+swm_status swm_adj_run(swm_ctx* C, int mode, const double* lh, const double* leta, const double* ldiv,
+                       const double* lu, double* out0, double* out1, double* out2, double* out3) {
+  if (!C || mode < 0 || mode > 2 || !lh || !leta || !ldiv || !lu || !out0 || !out1 || !out2 ||
+      (mode == 0 && !out3)) {
+    swm::set_error("swm_adj_run: bad argument");
+    return SWM_ERR_ARG;
+  }
+  return adj_run_impl(C, mode, lh, leta, ldiv, lu, nullptr, out0, out1, out2, out3, nullptr);
+}
+
+swm_status swm_adj_run_tracer(swm_ctx* C, int mode, const double* lh, const double* leta, const double* ldiv,
+                              const double* lu, const double* ltr, double* out0, double* out1, double* out2,
+                              double* out3, double* out_tr) {
+  if (!C || mode < 0 || mode > 2 || !lh || !leta || !ldiv || !lu || !ltr || !out0 || !out1 || !out2 ||
+      (mode == 0 && !out3) || !out_tr || C->nt < 1) {
+    swm::set_error("swm_adj_run_tracer: bad argument (a context with tracers, every array given)");
+    return SWM_ERR_ARG;
+  }
+  return adj_run_impl(C, mode, lh, leta, ldiv, lu, ltr, out0, out1, out2, out3, out_tr);
 }
 
 swm_status swm_set_linearisation(swm_ctx* C, int exact) {
@@ -4164,6 +4452,22 @@ swm_status swm_adj_limiter_stats(swm_ctx* C, long long* below_one, long long* ed
   // three edges per interior slot (the poles, in halo slots, own none)
   const long long per = 3LL * (C->iim - 2) * (C->jjm - 2) * 10;
   *edges = per * C->tl->lim_steps;
+  return SWM_OK;
+}
+
+swm_status swm_adj_tracer_limiter_stats(swm_ctx* C, long long* below_one, long long* edges) {
+  if (!C || !below_one || !edges) { swm::set_error("swm_adj_tracer_limiter_stats: bad argument"); return SWM_ERR_ARG; }
+  if (!C->tl || !C->tl->have_window) {
+    swm::set_error("swm_adj_tracer_limiter_stats: no window (swm_adj_forward first)");
+    return SWM_ERR_STATE;
+  }
+  HIP_TRY(hipSetDevice(C->device));
+  HIP_TRY(hipStreamSynchronize(C->stream));
+  unsigned long long n = 0;
+  if (C->tl->tlim_count) HIP_TRY(hipMemcpy(&n, C->tl->tlim_count, sizeof(n), hipMemcpyDeviceToHost));
+  *below_one = (long long)n;
+  const long long per = 3LL * (C->iim - 2) * (C->jjm - 2) * 10 * C->nt;
+  *edges = per * C->tl->tlim_steps;
   return SWM_OK;
 }
 

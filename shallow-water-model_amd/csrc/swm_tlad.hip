@@ -1,6 +1,7 @@
 // swm_tlad.hip -- tangent-linear (TL) and adjoint (AD) kernels of the FP64
-// time step with centred advection (step.F:114-212 with get_centered_flux,
-// step.F:495-550; operators.F as named per kernel).
+// time step (step.F:114-212) with centred, upstream or flux-corrected
+// advection of the mass and the tracers (step.F:495-678, mass_fct.F,
+// tracer_fct.F; operators.F as named per kernel), and of the FP64 V-cycle.
 //
 // Every forward operation y = f(x; base) of the step gets a TL kernel
 // (dy = f'(x) dx, gather form, one thread per slot like the forward kernel)
@@ -616,6 +617,301 @@ __global__ void k_ad_fct_low(Geo G, double dt, double w3, double w2, double w1, 
   if (!real) return;
   aadd(&h3a[s], g);
   div_edge_ad(iim, jjm, ns, c, G.area_inv[s], -dt * g, flxa);
+}
+
+// ----------------------------------------------------- tracer transport --
+// Tracers do not feed back on the dynamics, but depend on it through the
+// total mass flux F and the old and new thickness h3, h4.  Base values (the
+// tracer levels t3, t4 and told of the AB3 extrapolation, F, h3, h4 and the
+// new tracer t4n) come from the recorded trajectory.
+
+// the frozen tracer limiter coefficient of every edge (base step, on the
+// raw antidiffusive flux, before k_tr_anti limits it): the factor k_tr_anti
+// applies, min(1, bdn(s), bup(up)) where the flux is >= 0, else
+// min(1, bup(s), bdn(up)); 1 on the other slots, which keep the unlimited
+// flux until the edge halo fill (tracer_fct.F:539-593)
+__global__ void k_tr_coef(Geo G, const double* __restrict__ bdn, const double* __restrict__ bup,
+                          const double* __restrict__ anti, double* __restrict__ coef) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) coef[e * ns + s] = 1.0;
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  for (int e = 0; e < 3; ++e)
+    coef[e * ns + s] = anti[e * ns + s] >= 0.0 ? rmin(rmin(1.0, bdn[s]), bup[up[e]])
+                                               : rmin(rmin(1.0, bup[s]), bdn[up[e]]);
+}
+
+// TL of the tracer FCT fluxes (tr_fluxes_at, tracer_fct.F:166-243): the
+// upstream flux f1 = t3(s) max(0, F) + t3(up) min(0, F) (the upstream cell
+// taken where F >= 0 for the flux's own perturbation) and the centred flux
+// of the AB3-extrapolated tracer, f2 = (TH(s) + TH(up)) F / 2; interior
+// slots (0 elsewhere)
+__global__ void k_tl_tr_fluxes(Geo G, double w3, double w2, double w1, const double* __restrict__ F,
+                               const double* __restrict__ t3, const double* __restrict__ t4,
+                               const double* __restrict__ to, const double* __restrict__ dF,
+                               const double* __restrict__ dt3, const double* __restrict__ dt4,
+                               const double* __restrict__ dto, double* __restrict__ df1, double* __restrict__ df2) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) { df1[e * ns + s] = 0.0; df2[e * ns + s] = 0.0; }
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  auto TH = [&](int sl) { return w3 * t3[sl] + w2 * t4[sl] + w1 * to[sl]; };
+  auto DTH = [&](int sl) { return w3 * dt3[sl] + w2 * dt4[sl] + w1 * dto[sl]; };
+  const double th0 = TH(s), dth0 = DTH(s);
+  for (int e = 0; e < 3; ++e) {
+    const double f = F[e * ns + s], df = dF[e * ns + s];
+    const int u = up[e];
+    df1[e * ns + s] = dt3[s] * dmax(0.0, f) + dt3[u] * dmin(0.0, f) + df * (f >= 0.0 ? t3[s] : t3[u]);
+    df2[e * ns + s] = 0.5 * ((dth0 + DTH(u)) * f + (th0 + TH(u)) * df);
+  }
+}
+
+// AD of k_tl_tr_fluxes: consumes the adjoints of f1 and f2 (cleared on every
+// slot), adds into F (own edges) and the three tracer levels
+__global__ void k_ad_tr_fluxes(Geo G, double w3, double w2, double w1, const double* __restrict__ F,
+                               const double* __restrict__ t3, const double* __restrict__ t4,
+                               const double* __restrict__ to, double* __restrict__ f1a, double* __restrict__ f2a,
+                               double* __restrict__ Fa, double* __restrict__ t3a, double* __restrict__ t4a,
+                               double* __restrict__ toa) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) { f1a[e * ns + s] = 0.0; f2a[e * ns + s] = 0.0; }
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  auto TH = [&](int sl) { return w3 * t3[sl] + w2 * t4[sl] + w1 * to[sl]; };
+  auto TH_AD = [&](int sl, double g) {
+    aadd(&t3a[sl], w3 * g);
+    aadd(&t4a[sl], w2 * g);
+    aadd(&toa[sl], w1 * g);
+  };
+  const double th0 = TH(s);
+  double g3 = 0.0, gth = 0.0;
+  for (int e = 0; e < 3; ++e) {
+    const double g1 = f1a[e * ns + s], g2 = f2a[e * ns + s];
+    f1a[e * ns + s] = 0.0;
+    f2a[e * ns + s] = 0.0;
+    const double f = F[e * ns + s];
+    const int u = up[e];
+    g3 += g1 * dmax(0.0, f);
+    aadd(&t3a[u], g1 * dmin(0.0, f));
+    const double gh = 0.5 * g2 * f;
+    gth += gh;
+    TH_AD(u, gh);
+    Fa[e * ns + s] += g1 * (f >= 0.0 ? t3[s] : t3[u]) + 0.5 * (th0 + TH(u)) * g2;
+  }
+  aadd(&t3a[s], g3);
+  TH_AD(s, gth);
+}
+
+// TL of k_tr_low's low-order tracer mass (tracer_fct.F:251-321): ml = h3 t3
+// - dt div(f1) at the real cells (edge_update's forms), 0 elsewhere
+__global__ void k_tl_tr_low(Geo G, double dt, const double* __restrict__ h3, const double* __restrict__ t3,
+                            const double* __restrict__ dh3, const double* __restrict__ dt3,
+                            const double* __restrict__ df1, double* __restrict__ dml) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  bool set;
+  dml[s] = edge_update<double>(iim, jjm, ns, c, dh3[s] * t3[s] + h3[s] * dt3[s], dt * G.area_inv[s], df1, set);
+}
+
+// AD of k_tl_tr_low: consumes the adjoint of ml, adds into h3, t3 and f1
+__global__ void k_ad_tr_low(Geo G, double dt, const double* __restrict__ h3, const double* __restrict__ t3,
+                            double* __restrict__ mla, double* __restrict__ h3a, double* __restrict__ t3a,
+                            double* __restrict__ f1a) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const double g = mla[s];
+  mla[s] = 0.0;
+  const bool real = interior(c, iim, jjm) || (c.i == 1 && c.j == jjm - 1) || (c.i == iim - 1 && c.j == 1);
+  if (!real) return;
+  aadd(&h3a[s], g * t3[s]);
+  aadd(&t3a[s], g * h3[s]);
+  div_edge_ad(iim, jjm, ns, c, G.area_inv[s], -dt * g, f1a);
+}
+
+// TL of the limited antidiffusive flux with the frozen coefficients:
+// anti = coef (f2 - f1) on every value (k_tr_limiter forms f2 - f1 on all
+// slots, k_tr_anti limits the interior ones)
+__global__ void k_tl_tr_anti(size_t n, const double* __restrict__ coef, const double* __restrict__ df1,
+                             const double* __restrict__ df2, double* __restrict__ dan) {
+  const size_t t = (size_t)blockIdx.x * bdim() + threadIdx.x;
+  if (t < n) dan[t] = coef[t] * (df2[t] - df1[t]);
+}
+// its AD: consumes the adjoint of anti, adds into f1 and f2
+__global__ void k_ad_tr_anti(size_t n, const double* __restrict__ coef, double* __restrict__ ana,
+                             double* __restrict__ f1a, double* __restrict__ f2a) {
+  const size_t t = (size_t)blockIdx.x * bdim() + threadIdx.x;
+  if (t >= n) return;
+  const double g = coef[t] * ana[t];
+  ana[t] = 0.0;
+  f2a[t] += g;
+  f1a[t] -= g;
+}
+
+// TL of k_tr_final (tracer_fct.F:601-665, step.F:661): mf = ml - dt
+// div(anti) at the real cells (0 elsewhere), the old level kept as told,
+// the new tracer mf / h4 (0 where h4 = 0): dt4 = (dmf - t4n dh4) / h4
+__global__ void k_tl_tr_final(Geo G, double dt, const double* __restrict__ h4, const double* __restrict__ t4n,
+                              const double* __restrict__ dml, const double* __restrict__ dan,
+                              const double* __restrict__ dh4, double* __restrict__ dt4, double* __restrict__ dto) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  bool set;
+  const double dmf = edge_update<double>(iim, jjm, ns, c, dml[s], dt * G.area_inv[s], dan, set);
+  dto[s] = dt4[s];
+  const double hh = h4[s];
+  dt4[s] = hh != 0.0 ? (dmf - t4n[s] * dh4[s]) / hh : 0.0;
+}
+
+// AD of k_tl_tr_final: the new level's adjoint (t4a) is consumed and
+// replaced by that of told (the copy's transpose: told held the old level),
+// told's is cleared; adds into ml, anti and h4
+__global__ void k_ad_tr_final(Geo G, double dt, const double* __restrict__ h4, const double* __restrict__ t4n,
+                              double* __restrict__ t4a, double* __restrict__ toa, double* __restrict__ mla,
+                              double* __restrict__ ana, double* __restrict__ h4a) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const double g = t4a[s];
+  t4a[s] = toa[s];
+  toa[s] = 0.0;
+  const double hh = h4[s];
+  if (hh == 0.0) return;
+  const double gm = g / hh;
+  aadd(&h4a[s], -t4n[s] * gm);
+  const bool real = interior(c, iim, jjm) || (c.i == 1 && c.j == jjm - 1) || (c.i == iim - 1 && c.j == 1);
+  if (!real) return;
+  aadd(&mla[s], gm);
+  div_edge_ad(iim, jjm, ns, c, G.area_inv[s], -dt * gm, ana);
+}
+
+// TL of the centred (operators.F:1923-1991) or upstream (:1841-1914) tracer
+// edge flux of k_tr_flux with the base mass flux F and tracer t; interior
+// slots (0 elsewhere)
+template <bool CENTERED>
+__global__ void k_tl_tr_flux(Geo G, const double* __restrict__ F, const double* __restrict__ t,
+                             const double* __restrict__ dF, const double* __restrict__ dtr, double* __restrict__ dout) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) dout[e * ns + s] = 0.0;
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  for (int e = 0; e < 3; ++e) {
+    const double f = F[e * ns + s], df = dF[e * ns + s];
+    const int u = up[e];
+    if (CENTERED)
+      dout[e * ns + s] = 0.5 * (df * (t[s] + t[u]) + f * (dtr[s] + dtr[u]));
+    else
+      dout[e * ns + s] = dmax(0.0, f) * dtr[s] + dmin(0.0, f) * dtr[u] + df * (f >= 0.0 ? t[s] : t[u]);
+  }
+}
+template <bool CENTERED>
+__global__ void k_ad_tr_flux(Geo G, const double* __restrict__ F, const double* __restrict__ t,
+                             double* __restrict__ outa, double* __restrict__ Fa, double* __restrict__ ta) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  if (!interior(c, iim, jjm)) {
+    for (int e = 0; e < 3; ++e) outa[e * ns + s] = 0.0;
+    return;
+  }
+  const int i = c.i, j = c.j, k = c.k;
+  const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  double g0 = 0.0;
+  for (int e = 0; e < 3; ++e) {
+    const double g = outa[e * ns + s];
+    outa[e * ns + s] = 0.0;
+    const double f = F[e * ns + s];
+    const int u = up[e];
+    if (CENTERED) {
+      Fa[e * ns + s] += 0.5 * g * (t[s] + t[u]);
+      g0 += 0.5 * f * g;
+      aadd(&ta[u], 0.5 * f * g);
+    } else {
+      Fa[e * ns + s] += g * (f >= 0.0 ? t[s] : t[u]);
+      g0 += dmax(0.0, f) * g;
+      aadd(&ta[u], dmin(0.0, f) * g);
+    }
+  }
+  aadd(&ta[s], g0);
+}
+
+// TL of k_tracer_update (step.F:278-327): tf = -div(tflux) into the AB3
+// history slot a, t = t3 h3 + dt (w3 tf_a + w2 tf_b + w1 tf_c), divided by
+// h4 where the base h4 exceeds 1e-10: dt4 = (dt - t4n dh4) / h4
+__global__ void k_tl_tracer_update(Geo G, double dt, double w3, double w2, double w1,
+                                   const double* __restrict__ dtflux, const double* __restrict__ t3,
+                                   const double* __restrict__ h3, const double* __restrict__ h4,
+                                   const double* __restrict__ t4n, const double* __restrict__ dt3,
+                                   const double* __restrict__ dh3, const double* __restrict__ dh4,
+                                   double* __restrict__ dtf_a, const double* __restrict__ dtf_b,
+                                   const double* __restrict__ dtf_c, double* __restrict__ dt4) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const double dtf = -div_edge_at<double>(iim, jjm, ns, c, dtflux, G.area_inv[s]);
+  dtf_a[s] = dtf;
+  const double d = dt3[s] * h3[s] + t3[s] * dh3[s] + dt * (w3 * dtf + w2 * dtf_b[s] + w1 * dtf_c[s]);
+  const double hn = h4[s];
+  dt4[s] = hn > 1.0e-10 ? (d - t4n[s] * dh4[s]) / hn : d;
+}
+// its AD: consumes the new level's adjoint and that of history slot a
+__global__ void k_ad_tracer_update(Geo G, double dt, double w3, double w2, double w1, const double* __restrict__ t3,
+                                   const double* __restrict__ h3, const double* __restrict__ h4,
+                                   const double* __restrict__ t4n, double* __restrict__ t4a,
+                                   double* __restrict__ tfa_a, double* __restrict__ tfb_a,
+                                   double* __restrict__ tfc_a, double* __restrict__ t3a, double* __restrict__ h3a,
+                                   double* __restrict__ h4a, double* __restrict__ tfluxa) {
+  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+  const int s = xcd_slot();
+  if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
+  const double g = t4a[s];
+  t4a[s] = 0.0;
+  const double hn = h4[s];
+  double gd = g;
+  if (hn > 1.0e-10) {
+    gd = g / hn;
+    aadd(&h4a[s], -t4n[s] * gd);
+  }
+  aadd(&t3a[s], gd * h3[s]);
+  aadd(&h3a[s], gd * t3[s]);
+  const double gtf = tfa_a[s] + dt * w3 * gd;
+  tfa_a[s] = 0.0;
+  tfb_a[s] += dt * w2 * gd;
+  tfc_a[s] += dt * w1 * gd;
+  div_edge_ad(iim, jjm, ns, c, G.area_inv[s], -gtf, tfluxa);
 }
 
 // ------------------------------------------------------- multigrid AD --

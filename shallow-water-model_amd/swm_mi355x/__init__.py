@@ -133,6 +133,10 @@ _sig("swm_halo_shadow_stats", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(
 _sig("swm_set_linearisation", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int])
 _sig("swm_operator_bench", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp])
 _sig("swm_adj_limiter_stats", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 2)
+_sig("swm_adj_tracer_limiter_stats", ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 2)
+_sig("swm_init_vorticity_tracer", ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4)
+_sig("swm_tl_run_tracer", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 10)
+_sig("swm_adj_run_tracer", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 10)
 _sig("swm_wrap", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p])
 _sig("swm_elliptic", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
 _sig("swm_multigrid", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
@@ -708,13 +712,18 @@ class Model:
         return a.value, b.value, c.value, n.value
 
     # ---------------------------------------------------------- TL / AD
-    def init_vorticity(self, zeta, delta, h):
+    def init_vorticity(self, zeta, delta, h, tracer=None):
         """Initial state from relative vorticity, divergence and thickness
         (iim,jjm,10): streamfunction and velocity potential solved, u from
-        them, the rest derived as by init_state."""
+        them, the rest derived as by init_state; tracers (iim,jjm,ntracer,10)
+        as given, or zero."""
         s = (self.iim, self.jjm, 10)
         z, d, hh = _f64(zeta, s), _f64(delta, s), _f64(h, s)
-        _check(lib.swm_init_vorticity(self._h, _ptr(z), _ptr(d), _ptr(hh)))
+        if tracer is None:
+            _check(lib.swm_init_vorticity(self._h, _ptr(z), _ptr(d), _ptr(hh)))
+        else:
+            t = _f64(tracer, (self.iim, self.jjm, self.ntracer, 10))
+            _check(lib.swm_init_vorticity_tracer(self._h, _ptr(z), _ptr(d), _ptr(hh), _ptr(t)))
 
     def adj_forward(self, nsteps, every=16):
         """The window of the linearised model: nsteps steps from the freshly
@@ -726,11 +735,13 @@ class Model:
         return [np.zeros(s, order="F"), np.zeros(s, order="F"), np.zeros(s, order="F"),
                 np.zeros((2, 2) + s, order="F")]
 
-    def tl_run(self, dh=None, deta=None, ddiv=None, du=None, control=None, init_only=False):
+    def tl_run(self, dh=None, deta=None, ddiv=None, du=None, control=None, init_only=False, dtracer=None):
         """Tangent-linear model of the window: the perturbation of the state
         read by the first step (dh, deta, ddiv, du) or of the control
         (control=(dzeta, ddelta, dh)); returns (h, eta, div, u) after the
-        last step."""
+        last step.  With dtracer (iim,jjm,ntracer,10), the perturbation of
+        the tracers read by the first step (or of the initial tracers), the
+        tracers are carried too and their perturbation is returned last."""
         s = (self.iim, self.jjm, 10)
         out = self._state_arrays()
         if control is not None:
@@ -739,24 +750,43 @@ class Model:
         else:
             ins = [_f64(dh, s), _f64(deta, s), _f64(ddiv, s), _f64(du, (2, 2) + s)]
             mode = 0
-        _check(lib.swm_tl_run(self._h, mode, *[_ptr(x) for x in ins], *[_ptr(x) for x in out]))
-        return tuple(out)
+        if dtracer is None:
+            _check(lib.swm_tl_run(self._h, mode, *[_ptr(x) for x in ins], *[_ptr(x) for x in out]))
+            return tuple(out)
+        ts = (self.iim, self.jjm, self.ntracer, 10)
+        tin = _f64(dtracer, ts)
+        tout = np.zeros(ts, order="F")
+        _check(lib.swm_tl_run_tracer(self._h, mode, *[_ptr(x) for x in ins], _ptr(tin), *[_ptr(x) for x in out],
+                                     _ptr(tout)))
+        return tuple(out) + (tout,)
 
-    def adj_run(self, lh, leta=None, ldiv=None, lu=None, control=False, init_only=False):
+    def adj_run(self, lh, leta=None, ldiv=None, lu=None, control=False, init_only=False, ltracer=None):
         """Adjoint of the window for the adjoint (gradient) of the state
         after the last step; returns the adjoint of the state read by the
         first step (h, eta, div, u), or with control=True the gradient with
-        respect to (zeta, delta, h)."""
+        respect to (zeta, delta, h).  With ltracer (iim,jjm,ntracer,10), the
+        adjoint of the tracers after the last step, the tracers are carried
+        too and their adjoint (the gradient with respect to the initial
+        tracers) is returned last."""
         s = (self.iim, self.jjm, 10)
         z = np.zeros(s, order="F")
         ins = [_f64(lh, s), _f64(z if leta is None else leta, s), _f64(z if ldiv is None else ldiv, s),
                _f64(np.zeros((2, 2) + s) if lu is None else lu, (2, 2) + s)]
         out = self._state_arrays()
         mode = 2 if init_only else (1 if control else 0)
-        _check(lib.swm_adj_run(self._h, mode, *[_ptr(x) for x in ins], *[_ptr(x) for x in out]))
-        if init_only:
-            return tuple(out[:3])
-        return tuple(out[:3]) if control else tuple(out)
+        if ltracer is None:
+            _check(lib.swm_adj_run(self._h, mode, *[_ptr(x) for x in ins], *[_ptr(x) for x in out]))
+            extra = ()
+        else:
+            ts = (self.iim, self.jjm, self.ntracer, 10)
+            tin = _f64(ltracer, ts)
+            tout = np.zeros(ts, order="F")
+            _check(lib.swm_adj_run_tracer(self._h, mode, *[_ptr(x) for x in ins], _ptr(tin),
+                                          *[_ptr(x) for x in out], _ptr(tout)))
+            extra = (tout,)
+        if init_only or control:
+            return tuple(out[:3]) + extra
+        return tuple(out) + extra
 
     def global_real_sum(self, x, mask=None):
         """global_real_sum (global_reductions.F:77-98): sum(x * mask) of host
@@ -807,6 +837,14 @@ class Model:
         edge-steps in all) over the base steps of the last TL or AD sweep."""
         a, b = ctypes.c_longlong(), ctypes.c_longlong()
         _check(lib.swm_adj_limiter_stats(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def tracer_limiter_stats(self):
+        """FCT window, tracer runs: (edge-steps with a frozen tracer limiter
+        coefficient below 1, edge-steps in all, over the tracers) of the last
+        TL or AD sweep with tracers."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib.swm_adj_tracer_limiter_stats(self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
     @property
