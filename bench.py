@@ -697,7 +697,7 @@ def main():
                                "weights)", "achieved": ach,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS if ach else None,
                      "traffic": traffic.get("k_relax_h<2>", {}).get("traffic_bytes"),
-                     "traffic_note": "HBM bytes per two-field sweep, rocprofv3 FETCH_SIZE x 1.998 + WRITE_SIZE "
+                     "traffic_note": "HBM bytes per two-field sweep, rocprofv3 FETCH_SIZE x 1.975 + WRITE_SIZE "
                                      "(scales from a plain copy kernel), profiles/traffic_g9_pmc.md",
                      "launches": tm["kernel_launches"], "mean_ms": tm["kernel_ms"],
                      "bytes_per_launch": tm["kernel_bytes"],
