@@ -325,16 +325,25 @@ swm_status swm_set_overlap(swm_ctx* ctx, int mode);
  * Refused for FP64 contexts, whose multigrid cycle is the reference's. */
 swm_status swm_set_mixed_coarse(swm_ctx* ctx, int direct);
 
-/* Kernel fusions of the multigrid cycle, both paths (default SWM_FUSE_ALL).
+/* Kernel fusions of the multigrid cycle and the step, both paths (default
+ * SWM_FUSE_ALL).
  * SWM_FUSE_INJECT_SWEEP: the restriction of the residual to a coarser level
  * that runs its own sweeps also performs that level's first smoothing sweep,
  * which starts from zero and so needs the injected right-hand side at its
  * own slot only; one launch per level and cycle fewer.  The results are the
  * same bit for bit with or without; the switch is there to show it.  No
  * counterpart in the reference (multigrid_solver.F:476-591 clears alpha,
- * sweeps, and injects, level by level). */
+ * sweeps, and injects, level by level).
+ * SWM_FUSE_HALO_PUSH (FCT step): the kernels that form a field which the
+ * step then fills the halo of also store its halo copies, from an inverted
+ * fill table (the slot whose value a halo slot holds writes it there), so
+ * that the step's halo fill launches are left out; the tracers' halo fill
+ * runs at the end of their transport.  The same values land in the same
+ * slots.  No counterpart in the reference (wrap_data.F:1790-2607 fills each
+ * halo after the routine that formed the field). */
 #define SWM_FUSE_INJECT_SWEEP 1
-#define SWM_FUSE_ALL 1
+#define SWM_FUSE_HALO_PUSH 2
+#define SWM_FUSE_ALL 3
 swm_status swm_set_fusion(swm_ctx* ctx, int flags);
 
 /* Mean squared residuals at the end of the last elliptic solve, and mean

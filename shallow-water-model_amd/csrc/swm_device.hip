@@ -49,6 +49,10 @@ struct DevTable {
   // for the halo shadow check
   std::vector<int> hdst, hsrc;
   size_t span = 0;
+  // the table inverted for producer-side halo writes (swm::dev::Push)
+  int4* hp = nullptr;
+  int* pe = nullptr;
+  swm::dev::Push push() const { return swm::dev::Push{hp, pe}; }
 };
 
 // one 256-byte aligned device arena for all buffers of a context
@@ -121,6 +125,7 @@ struct swm_ctx {
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;   // the tracer transport beside the dynamics (overlap)
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_norm = nullptr;    // the convergence test's norms are in host memory
   int overlap = 3;             // tracer transport on stream2: 0 off, 1 from the mass update, n >= 2 from level g - n + 1 of the first V-cycle
   // overlap: work issued by the first V-cycle of a step when it reaches
   // level hook_level on the way down (the tracer transport, started where
@@ -189,6 +194,12 @@ struct swm_ctx {
   struct SolveSched { int first = 0, extra = 0, bad_mask = 0, fixed = 0; } last_sched;
   int lin_exact = 1;           // TL/AD: the derivative of the forward's own cycles (1) or of converged solves (0)
   int halo_shadow = 0;         // every halo fill checked against the host's (swm_set_halo_shadow)
+  int fuse_push = 1;           // FCT step: producer kernels store the halo copies (SWM_FUSE_HALO_PUSH)
+  // the next step's corner thickness, queued while the host waits on the
+  // solve's convergence test (swm_step sets next_follows when another step
+  // of the same call follows; corners_ready: queued, the next step skips it)
+  bool next_follows = false, corners_ready = false;
+  swm_status layout_status = SWM_OK;   // halo table set-up
   long long shadow_checks = 0, shadow_bad = 0;
   bool initialized = false;
   RemapDev rm;
@@ -259,11 +270,56 @@ swm_status make_table(swm_ctx* C, const swm::HaloTable& t, int ns, DevTable& d) 
   d.span = (size_t)t.ncomp * ns;
   d.dst = C->arena.take<int>(dst.size() + 1);
   d.src = C->arena.take<int>(src.size() + 1);
+  // the push words and entries (layout in swm_kernels.h, Push): entries
+  // grouped by source slot, in table order within a slot
+  std::vector<int4> hp(ns, int4{0, 0, 0, 0});
+  std::vector<int> cnt(ns, 0);
+  int nover = 0;
+  for (size_t k = 0; k < dst.size(); ++k) {
+    const int g = dst[k] % ns, c = dst[k] / ns;
+    if (c > 2) { swm::set_error("halo table: more than three components"); return SWM_ERR_ARG; }
+    hp[g].x |= (src[k] >= 0 ? 1 : 2) << (4 + 2 * c);
+    if (src[k] >= 0) {
+      const int q = src[k] % ns;
+      if (dst[k] >= (1 << 28)) { swm::set_error("halo table: grid too large for the push entries"); return SWM_ERR_ARG; }
+      const int ent = (int)((unsigned)(src[k] / ns) << 28) | dst[k];
+      if (cnt[q] == 0) hp[q].y = ent;
+      else if (cnt[q] == 1) hp[q].z = ent;
+      else if (cnt[q] == 2) hp[q].w = ent;
+      else nover += 1;
+      cnt[q] += 1;
+      if (cnt[q] > 15) { swm::set_error("halo table: more than 15 copies of one slot"); return SWM_ERR_ARG; }
+    }
+  }
+  if (nover >= (1 << 22)) { swm::set_error("halo table: too many entries for the push words"); return SWM_ERR_ARG; }
+  std::vector<int> pe((size_t)nover + 1, 0), seen(ns, 0);
+  std::vector<int> start(ns, 0);
+  for (int q = 0, o = 0; q < ns; ++q) {
+    start[q] = o;
+    if (cnt[q] > 3) o += cnt[q] - 3;
+    hp[q].x |= cnt[q] | ((cnt[q] > 3 ? start[q] : 0) << 10);
+  }
+  for (size_t k = 0; k < dst.size(); ++k) {
+    if (src[k] < 0) continue;
+    const int q = src[k] % ns;
+    const int r = seen[q]++;
+    if (r >= 3) pe[(size_t)start[q] + r - 3] = (int)((unsigned)(src[k] / ns) << 28) | dst[k];
+  }
+  d.hp = C->arena.take<int4>(ns);
+  d.pe = C->arena.take<int>(pe.size());
   if (d.n && C->arena.base) {
     HIP_TRY(hipMemcpy(d.dst, dst.data(), dst.size() * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(d.src, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.hp, hp.data(), hp.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.pe, pe.data(), pe.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   return SWM_OK;
+}
+
+// producer-side halo writes in the step (SWM_FUSE_HALO_PUSH): FCT step only,
+// and not while every fill is checked against the host's
+inline bool halo_push(const swm_ctx* C) {
+  return C->fuse_push && !C->halo_shadow && C->advection == SWM_ADV_FCT;
 }
 
 // launch grid over the state of the nz members in play (z = member)
@@ -521,6 +577,29 @@ R* vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_ze
   return alpha;
 }
 
+// step.F:132-141 of the next step, queued before the host waits on a
+// convergence test of this step's solve, so that the GPU has work while the
+// host reads the norms: the corner thickness of the next step depends only
+// on this step's new thickness and its halo (FCT: both complete before the
+// solve), and nothing in the rest of this step reads or writes it.  The
+// next step then skips it.
+template <class R>
+void corners_next(swm_ctx* C, View<R>& V) {
+  const int n4 = C->n4 - 1;
+  const bool push = halo_push(C);
+  swm::dev::GeoT<R> Gp = V.geo;
+  if (push) Gp.pv = C->vrtx.push();
+  hipLaunchKernelGGL(swm::dev::k_avg_corners<R>, zgrid(C, nblk_b(C->ns, C->blk_step)), dim3(C->blk_step), 0, C->stream,
+                     Gp, V.geo.sw, V.h[n4], V.hc);
+  if (!push) gather<R>(C, C->vrtx, {V.hc});
+}
+void corners_for_next(swm_ctx* C) {
+  if (!C->next_follows || C->corners_ready || C->advection != SWM_ADV_FCT) return;
+  if (C->precision == SWM_MIXED) corners_next<float>(C, C->m);
+  else corners_next<double>(C, C->d);
+  C->corners_ready = true;
+}
+
 // sums of x0*w and x1*w into C->sums[0..1] (block partials, then a
 // compensated fixed-order sum)
 template <class R>
@@ -559,9 +638,12 @@ swm_status member_norms(swm_ctx* C, const double* part, int nb, const std::vecto
   std::vector<double> raw((size_t)nv * C->nz);
   if (C->nz == 1 && C->host_sums_dev) {
     // one member: the final sums go straight to pinned host memory (no copy
-    // launch before the host reads them)
+    // launch before the host reads them); the host waits for them, not for
+    // the next step's corners queued behind them
     sum_final(C, part, nb, C->host_sums_dev, nv);
-    HIP_TRY(hipStreamSynchronize(C->stream));
+    HIP_TRY(hipEventRecord(C->ev_norm, C->stream));
+    corners_for_next(C);
+    HIP_TRY(hipEventSynchronize(C->ev_norm));
     for (int b = 0; b < nv; ++b) raw[b] = C->host_sums[b];
   } else {
     sum_final(C, part, nb, C->sums, nv);
@@ -911,6 +993,9 @@ template <class R>
 void wrap_prognostics(swm_ctx* C, View<R>& V, bool after_step = false) {   // step.F:438-493
   const int n4 = C->n4 - 1;
   const int ns = C->ns;
+  // with producer-side halo writes the step's kernels have stored every
+  // halo copy this fill would make, and the tracer transport filled its own
+  if (after_step && halo_push(C)) return;
   // inv_h (step.F:448-452) is not formed: nothing on the path reads it
   std::vector<R*> f = {V.eta[n4], V.div[n4], V.relative, V.ke};
   if (!after_step || C->advection != SWM_ADV_FCT) f.push_back(V.h[n4]);
@@ -975,9 +1060,19 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   R* m1 = V.msh_f[C->n1f - 1];
   const bool fct = C->advection == SWM_ADV_FCT;
   const bool overlap = C->overlap && C->nt > 0 && fct;
+  // producer-side halo writes (FCT step): the kernels below that form a
+  // field whose halo the step fills store the halo copies themselves with
+  // Gp, and those fills are left out
+  const bool push = halo_push(C);
+  swm::dev::GeoT<R> Gp = G;
+  if (push) { Gp.pf = C->face.push(); Gp.pv = C->vrtx.push(); Gp.pe = C->edge.push(); }
 
-  hipLaunchKernelGGL(swm::dev::k_avg_corners<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, G.sw, V.h[n3], V.hc);
-  gather<R>(C, C->vrtx, {V.hc});
+  if (C->corners_ready) {
+    C->corners_ready = false;   // queued by the previous step (corners_next)
+  } else {
+    hipLaunchKernelGGL(swm::dev::k_avg_corners<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, G.sw, V.h[n3], V.hc);
+    if (!push) gather<R>(C, C->vrtx, {V.hc});
+  }
   if (C->advection == SWM_ADV_REMAP) {
     // get_remap_flux (step.F:687-771), FP64 contexts only
     if constexpr (sizeof(R) == 8) remap_step(C, V);
@@ -1007,24 +1102,24 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
                          V.tr[n3][q], V.h[n3], V.h[n4], V.tr_f[a][q], V.tr_f[b][q], V.tr_f[c][q], V.tr[n4][q]);
   } else {
   // mass FCT (mass_fct.F:23-539)
-  hipLaunchKernelGGL(swm::dev::k_mass_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.u[n3], V.h[n3], V.hc, V.flx1, m3);
-  gather<R>(C, C->edge, {V.flx1, m3});
-  hipLaunchKernelGGL(swm::dev::k_mass_low_beta<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, w3, w2, w1, V.flx1, m3, m2, m1,
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.u[n3], V.h[n3], V.hc, V.flx1, m3);
+  if (!push) gather<R>(C, C->edge, {V.flx1, m3});
+  hipLaunchKernelGGL(swm::dev::k_mass_low_beta<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, dt, w3, w2, w1, V.flx1, m3, m2, m1,
                      V.h[n3], V.anti, V.mss_low, V.beta);
-  gather<R>(C, C->face, {V.beta});
+  if (!push) gather<R>(C, C->face, {V.beta});
   if constexpr (sizeof(R) == 8) {
     if (C->fct_rec)
       hipLaunchKernelGGL(swm::dev::k_fct_coef, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.beta, V.anti, C->fct_rec);
   }
-  hipLaunchKernelGGL(swm::dev::k_mass_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.beta, V.anti);
-  gather<R>(C, C->edge, {V.anti});
+  hipLaunchKernelGGL(swm::dev::k_mass_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.beta, V.anti);
+  if (!push) gather<R>(C, C->edge, {V.anti});
   // with the first tracer's edge fluxes (k_tr_fluxes fused)
   swm::dev::TrFluxArgs<R> tf0{};
   if (C->nt > 0) tf0 = {V.tr[n3][0], V.tr[n4][0], V.told[0], V.f1, V.f2, w3, w2, w1};
-  hipLaunchKernelGGL(swm::dev::k_mass_final<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, V.flx1, V.anti, V.mss_low,
+  hipLaunchKernelGGL(swm::dev::k_mass_final<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, dt, V.flx1, V.anti, V.mss_low,
                      V.mass_flux, V.h[n4], tf0);
   // the tracer limiter fills the halo of the new mass in place (tracer_fct.F:344)
-  gather<R>(C, C->face, {V.h[n4]});
+  if (!push) gather<R>(C, C->face, {V.h[n4]});
   }
   // Tracer FCT (tracer_fct.F:22-670).  The tracers depend on the new mass
   // and the mass flux only, and nothing after them but the final halo fill
@@ -1052,7 +1147,7 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
       if (q > 0)
         hipLaunchKernelGGL(swm::dev::k_tr_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, w3, w2, w1, t3, t4,
                            V.told[q], V.mass_flux, V.f1, V.f2);
-      gather<R>(C, C->edge, {V.f1, V.f2});
+      if (q > 0 || !push) gather<R>(C, C->edge, {V.f1, V.f2});   // the first tracer's: k_mass_final's copies
       hipLaunchKernelGGL(swm::dev::k_tr_low<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, V.h[n3], t3, V.f1,
                          V.h[n4], V.ml, V.tlow);
       gather<R>(C, C->face, {V.ml, V.tlow});
@@ -1068,6 +1163,13 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
       gather<R>(C, C->edge, {V.anti});
       hipLaunchKernelGGL(swm::dev::k_tr_final<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, V.ml, V.anti, V.h[n4],
                          t4, V.told[q]);
+    }
+    // with producer-side halo writes the end-of-step fill (wrap_prognostics)
+    // has nothing else left: the tracers' halos are filled here instead
+    if (push) {
+      std::vector<R*> tf;
+      for (int q = 0; q < C->nt; ++q) tf.push_back(V.tr[n4][q]);
+      gather_list<R>(C, C->face, tf);
     }
     if (overlap) {
       const hipError_t he = hipEventRecord(C->ev_join, C->stream2);
@@ -1086,13 +1188,13 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
     tracers();
   }
   // vorticity and divergence (step.F:143-163, :336-429)
-  hipLaunchKernelGGL(swm::dev::k_vort_curl<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.mass_flux, V.h[n3], V.eta[n3], V.hc,
+  hipLaunchKernelGGL(swm::dev::k_vort_curl<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.mass_flux, V.h[n3], V.eta[n3], V.hc,
                      V.u[n3], V.vf, V.dcurl);
-  gather<R>(C, C->edge, {V.vf, V.dcurl});
+  if (!push) gather<R>(C, C->edge, {V.vf, V.dcurl});
   // mixed path: the kernel also forms the block sums of the right-hand
   // sides' area-weighted means, which the solve takes off on the fly
   constexpr bool mixed = sizeof(R) == 4;
-  hipLaunchKernelGGL((swm::dev::k_tendencies<R, mixed>), zgrid(C, nb), dim3(C->blk_step), 0, S, G, dt, grav, w3, w2, w1,
+  hipLaunchKernelGGL((swm::dev::k_tendencies<R, mixed>), zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, dt, grav, w3, w2, w1,
                      V.vf, V.dcurl, V.ke, V.h[n3], V.eta[n3], V.div[n3], V.eta_f[C->n3f - 1], V.eta_f[C->n2f - 1],
                      V.eta_f[C->n1f - 1], V.div_f[C->n3f - 1], V.div_f[C->n2f - 1], V.div_f[C->n1f - 1], V.eta[n4],
                      V.div[n4], V.relative, V.rhs, V.rhs + ns, mixed ? C->area_mask : nullptr,
@@ -1111,17 +1213,18 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   if (tr_st) return tr_st;
   if constexpr (mixed) {
     // the new vorticity (the right-hand side less its mean) with the velocity
-    hipLaunchKernelGGL(swm::dev::k_velocity<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, C->xout, C->xout + ns,
+    hipLaunchKernelGGL(swm::dev::k_velocity<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, C->xout, C->xout + ns,
                        V.u[n4], V.rhs, bsum, C->total_area, V.eta[n4]);
   } else {
-    hipLaunchKernelGGL(swm::dev::k_post_elliptic<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.rhs, V.eta[n4]);
-    hipLaunchKernelGGL(swm::dev::k_velocity<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, C->xout, C->xout + ns,
+    hipLaunchKernelGGL(swm::dev::k_post_elliptic<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.rhs, V.eta[n4]);
+    hipLaunchKernelGGL(swm::dev::k_velocity<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, C->xout, C->xout + ns,
                        V.u[n4], (const R*)nullptr, (const double*)nullptr, 0.0, (R*)nullptr);
   }
-  gather<R>(C, C->vrtx, {V.u[n4], V.u[n4] + 2 * (size_t)ns});
+  if (!push) gather<R>(C, C->vrtx, {V.u[n4], V.u[n4] + 2 * (size_t)ns});
   // the kinetic energy's halo is filled with the other fields' by
-  // wrap_prognostics, which follows the step directly
-  hipLaunchKernelGGL(swm::dev::k_center_ke<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.u[n4], V.ke);
+  // wrap_prognostics, which follows the step directly (with producer-side
+  // halo writes: by this kernel)
+  hipLaunchKernelGGL(swm::dev::k_center_ke<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.u[n4], V.ke);
   if (overlap) HIP_TRY(hipStreamWaitEvent(S, C->ev_join, 0));   // the tracers before the halo fill
   return SWM_OK;
 }
@@ -1488,9 +1591,8 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
   C->area = take(ns); C->area_inv = take(ns); C->cai = take(2 * ns); C->sub_area = take(6 * ns); C->dc = take(6 * ns);
   C->nrm = take(12 * ns); C->tng = take(12 * ns); C->vw = take(12 * ns); C->sw = take(6 * ns); C->lw = take(7 * ns);
   C->f = take(ns); C->hs = take(ns); C->mask = take(ns); C->area_mask = take(ns); C->pxyz = take(3 * ns);
-  make_table(C, Gh.face, C->ns, C->face);
-  make_table(C, Gh.vrtx, C->ns, C->vrtx);
-  make_table(C, Gh.edge, C->ns, C->edge);
+  for (auto tb : {std::make_pair(&Gh.face, &C->face), std::make_pair(&Gh.vrtx, &C->vrtx), std::make_pair(&Gh.edge, &C->edge)})
+    if (C->layout_status == SWM_OK) C->layout_status = make_table(C, *tb.first, C->ns, *tb.second);
   // the member block: solver iterates, partial sums, then the views' state
   auto mt = [&](size_t n) { return C->marena.take<double>(n); };
   C->xout = mt(2 * ns); C->r2 = mt(ns); C->xalt = mt(2 * ns); C->xprev = mt(2 * ns); C->xprev2 = mt(2 * ns); C->gauge = mt(2);
@@ -1684,6 +1786,7 @@ swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
   HIP_TRY(hipStreamCreateWithFlags(&C->stream2, hipStreamNonBlocking));
   HIP_TRY(hipEventCreateWithFlags(&C->ev_fork, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&C->ev_join, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&C->ev_norm, hipEventDisableTiming));
   HIP_TRY(hipEventCreate(&C->ev_a));
   HIP_TRY(hipEventCreate(&C->ev_b));
   // sizing pass, then the shared arena and the member blocks
@@ -1701,6 +1804,7 @@ swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
   C->marena.cap = member_bytes;
   C->face = C->vrtx = C->edge = DevTable{};
   layout_all(C, Gh);
+  if (C->layout_status != SWM_OK) return C->layout_status;
   if (C->arena.used > C->arena.cap || C->marena.used > C->marena.cap) {
     swm::set_error("swm_create: arena overflow");
     return SWM_ERR_HIP;
@@ -1937,6 +2041,7 @@ void swm_destroy(swm_ctx* C) {
   if (C->stream2) (void)hipStreamSynchronize(C->stream2);
   if (C->ev_fork) (void)hipEventDestroy(C->ev_fork);
   if (C->ev_join) (void)hipEventDestroy(C->ev_join);
+  if (C->ev_norm) (void)hipEventDestroy(C->ev_norm);
   if (C->stream2) (void)hipStreamDestroy(C->stream2);
   if (C->ev_a) (void)hipEventDestroy(C->ev_a);
   if (C->ev_b) (void)hipEventDestroy(C->ev_b);
@@ -2170,7 +2275,14 @@ swm_status swm_step(swm_ctx* C, int nsteps) {
     *C->rm.flag_host = 0;
     return SWM_ERR_STATE;
   };
+  // the corners queued ahead belong to this call's next step only
+  struct Ahead {
+    swm_ctx* C;
+    ~Ahead() { C->next_follows = false; C->corners_ready = false; }
+  } ahead{C};
+  C->corners_ready = false;
   for (int n = 0; n < nsteps; ++n) {
+    C->next_follows = n + 1 < nsteps;
     swm_status st = swm::step_once(C);
     if (st) return st;
     if (C->rm.flag_host && *(volatile int*)C->rm.flag_host) return remap_stop();
@@ -2283,6 +2395,7 @@ swm_status swm_set_mixed_coarse(swm_ctx* C, int direct) {
 swm_status swm_set_fusion(swm_ctx* C, int flags) {
   if (!C || (flags & ~SWM_FUSE_ALL)) { swm::set_error("swm_set_fusion: bad argument"); return SWM_ERR_ARG; }
   C->fuse_mg = (flags & SWM_FUSE_INJECT_SWEEP) ? 1 : 0;
+  C->fuse_push = (flags & SWM_FUSE_HALO_PUSH) ? 1 : 0;
   C->epoch += 1;
   return SWM_OK;
 }

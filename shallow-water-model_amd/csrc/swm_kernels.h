@@ -10,6 +10,23 @@
 namespace swm {
 namespace dev {
 
+// Producer-side halo writes of one halo table (face, vrtx or edge).  The
+// table copies element (c', s') to halo element (c, g); here it is inverted
+// so that the kernel that forms the value at slot s' also stores its copies.
+// hp holds four words per slot, read with one load:
+//   x   bits 0-3    number of entries whose source is this slot (at most 15)
+//       bits 4-9    two bits per component of the slot as a destination:
+//                   0 not one (the kernel stores its own value), 1 copied
+//                   from another slot (left to that slot's thread), 2 zero
+//       bits 10-31  where its entries beyond the third start in e
+//   y, z, w         its first three entries
+// An entry is (source component << 28) | destination plane index c * ns + g.
+// hp == nullptr: no producer-side writes (the halo fills run instead).
+struct Push {
+  const int4* hp;
+  const int* e;
+};
+
 // grid extents and metric planes of the model level.  Mixed path only: h0
 // is subtracted from every stored thickness-like field (h, the low-order
 // mass) and fo (the Coriolis parameter) from the stored vorticity, so that
@@ -33,6 +50,8 @@ struct GeoT {
   // sym(e) = area * l_weights(e+1) of the edge towards up_e (an edge field,
   // halos filled) and area_inv at real cells, 0 elsewhere
   const double *sym, *aim;
+  // producer-side halo writes of the step kernels (all null: none)
+  Push pf, pv, pe;
 };
 using Geo = GeoT<double>;
 

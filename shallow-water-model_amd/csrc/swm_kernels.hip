@@ -66,6 +66,79 @@ __device__ __forceinline__ void stg(T* __restrict__ p, int i, T x) {
 __device__ __forceinline__ bool interior(const IJK& c, int iim, int jjm) {
   return c.i >= 2 && c.i <= iim - 1 && c.j >= 2 && c.j <= jjm - 1;
 }
+// ------------------------------------------- producer-side halo writes ---
+// Every halo table entry copies a slot one cell inside a panel edge (or a
+// pole slot) to a halo slot, so only slots within one cell of the edge have
+// push words; the others read nothing.  The words (with the first three
+// entries) are one load at the start of a kernel, beside its other loads, so
+// that the copies at its end wait for no further load.
+struct PushW {
+  int4 w;
+};
+__device__ __forceinline__ PushW push_load(const Push& P, const IJK& c, int iim, int jjm, int s) {
+  PushW r;
+  const bool rim = c.i <= 2 || c.i >= iim - 1 || c.j <= 2 || c.j >= jjm - 1;
+  r.w = (P.hp != nullptr && rim) ? ldg(P.hp, s) : int4{0, 0, 0, 0};
+  return r;
+}
+// the copies of the slot's values v0, v1, v2 (components 0..NC-1 of field
+// x, planes ns apart) into the halo slots that hold them.  The values come
+// as scalars: a selected element of a local array would be read through a
+// selected address, which puts the array in memory.
+template <class R, int NC>
+__device__ __forceinline__ void push_one(R* __restrict__ x, int t, R v0, R v1, R v2) {
+  const int comp = (int)((unsigned)t >> 28);
+  R val = v0;
+  if constexpr (NC > 1) val = comp == 1 ? v1 : val;
+  if constexpr (NC > 2) val = comp == 2 ? v2 : val;
+  stg(x, t & 0x0fffffff, val);
+}
+template <class R, int NC>
+__device__ __forceinline__ void push(const Push& P, const PushW& W, R* __restrict__ x, R v0, R v1, R v2) {
+  const int n = W.w.x & 15;
+  if (n > 0) push_one<R, NC>(x, W.w.y, v0, v1, v2);
+  if (n > 1) push_one<R, NC>(x, W.w.z, v0, v1, v2);
+  if (n > 2) push_one<R, NC>(x, W.w.w, v0, v1, v2);
+  const int e0 = (int)((unsigned)W.w.x >> 10);
+  for (int q = 3; q < n; ++q) push_one<R, NC>(x, ldg(P.e, e0 + q - 3), v0, v1, v2);   // corner and pole slots only
+}
+// the slot's values of field x (components 0..NC-1, planes ns apart):
+// stored, and with a push word (w != 0) stored only where the slot is not a
+// halo copy (zero where the fill has no source) and copied into the halo
+// slots that hold them.  Slots without a word take the plain stores.
+template <class R, int NC>
+__device__ __forceinline__ void put_all(const Push& P, const PushW& W, R* __restrict__ x, int ns, int s, R v0,
+                                        R v1 = R(0), R v2 = R(0)) {
+  const int w = W.w.x;
+  if (w == 0) {
+    stg(x, s, v0);
+    if constexpr (NC > 1) stg(x + (size_t)ns, s, v1);
+    if constexpr (NC > 2) stg(x + (size_t)2 * ns, s, v2);
+    return;
+  }
+  const int m0 = (w >> 4) & 3, m1 = (w >> 6) & 3, m2 = (w >> 8) & 3;
+  if (m0 != 1) stg(x, s, m0 == 2 ? R(0) : v0);
+  if constexpr (NC > 1)
+    if (m1 != 1) stg(x + (size_t)ns, s, m1 == 2 ? R(0) : v1);
+  if constexpr (NC > 2)
+    if (m2 != 1) stg(x + (size_t)2 * ns, s, m2 == 2 ? R(0) : v2);
+  push<R, NC>(P, W, x, v0, v1, v2);
+}
+template <class R>
+__device__ __forceinline__ void put1(const Push& P, const PushW& W, R* __restrict__ x, int s, R v) {
+  put_all<R, 1>(P, W, x, 0, s, v);
+}
+// a kernel that leaves the slot's own values as they are: only the zeros of
+// the fill where it has no source
+template <class R, int NC>
+__device__ __forceinline__ void put_zeros(const PushW& W, R* __restrict__ x, int ns, int s) {
+  const int w = W.w.x;
+  if (w == 0) return;
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+    if (((w >> (4 + 2 * c)) & 3) == 2) stg(x + (size_t)c * ns, s, R(0));
+}
+
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 
@@ -146,18 +219,21 @@ __global__ void k_avg_corners(GeoT<R> G, const R* __restrict__ sw, const R* __re
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
+  const PushW w = push_load(G.pv, c, iim, jjm, s);
+  R v[2];
   if (i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1) {
     const int a = SL(i, j, k), b = SL(i - 1, j, k), d = SL(i - 1, j - 1, k), e = SL(i, j - 1, k);
     const R w10 = sw[0 * ns + s], w20 = sw[1 * ns + s], w30 = sw[2 * ns + s];
     const R w11 = sw[3 * ns + s], w21 = sw[4 * ns + s], w31 = sw[5 * ns + s];
     const R xa = habs(G, x1[a]), xb = habs(G, x1[b]), xd = habs(G, x1[d]), xe = habs(G, x1[e]);
-    o1[s] = +w10 * xa + w20 * xb + w30 * xd;
-    o1[ns + s] = +w11 * xa + w21 * xd + w31 * xe;
+    v[0] = +w10 * xa + w20 * xb + w30 * xd;
+    v[1] = +w11 * xa + w21 * xd + w31 * xe;
   } else {
     const R f1 = habs(G, x1[SL(2, 2, k)]);
-    o1[s] = f1;
-    o1[ns + s] = f1;
+    v[0] = f1;
+    v[1] = f1;
   }
+  put_all<R, 2>(G.pv, w, o1, ns, s, v[0], v[1]);
 }
 
 // half-edge fluxes of cell (i,j) as combined by centered_mass_flux
@@ -248,8 +324,10 @@ __global__ void k_mass_fluxes(GeoT<R> G, const R* __restrict__ u, const R* __res
   const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
+  const PushW w = push_load(G.pe, c, iim, jjm, s);
   if (!interior(c, iim, jjm)) {
-    for (int e = 0; e < 3; ++e) { flx1[e * ns + s] = R(0); mshf[e * ns + s] = R(0); }
+    put_all<R, 3>(G.pe, w, flx1, ns, s, R(0), R(0), R(0));
+    put_all<R, 3>(G.pe, w, mshf, ns, s, R(0), R(0), R(0));
     return;
   }
   const int i = c.i, j = c.j, k = c.k;
@@ -263,12 +341,16 @@ __global__ void k_mass_fluxes(GeoT<R> G, const R* __restrict__ u, const R* __res
   }
   const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
   const R hs0 = habs(G, h3[s]);
+  R vm[3], vf[3];
+#pragma unroll
   for (int e = 0; e < 3; ++e) {
-    mshf[e * ns + s] = -hc_.r[0][e] + hc_.r[1][e];
+    vm[e] = -hc_.r[0][e] + hc_.r[1][e];
     const R v = hu.r[1][e] - hu.r[0][e];
     const R mx = v > R(0) ? v : R(0), mn = v < R(0) ? v : R(0);
-    flx1[e * ns + s] = mx * hs0 + mn * habs(G, h3[up[e]]);
+    vf[e] = mx * hs0 + mn * habs(G, h3[up[e]]);
   }
+  put_all<R, 3>(G.pe, w, mshf, ns, s, vm[0], vm[1], vm[2]);
+  put_all<R, 3>(G.pe, w, flx1, ns, s, vf[0], vf[1], vf[2]);
 }
 
 // update of a cell field by the divergence of edge fluxes, pole forms
@@ -305,6 +387,7 @@ __global__ void k_mass_low_beta(GeoT<R> G, R dt, R w3, R w2, R w1, const R* __re
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
+  const PushW wb = push_load(G.pf, c, iim, jjm, s);
   // every value first (the loads overlap): the three edges at the slot and
   // the one edge each at (i+1, j), (i+1, j+1), (i, j+1), those clamped into
   // the array where they fall outside it (their values are then not used)
@@ -342,14 +425,14 @@ __global__ void k_mass_low_beta(GeoT<R> G, R dt, R w3, R w2, R w1, const R* __re
   else
     ml = R(0);
   stg(mss_low, s, ml);
+  R bt = R(0);
   if (i <= iim - 1 && j <= jjm - 1) {
     const R a_out = (rmax(R(0), ao[0]) + rmax(R(0), ao[1]) + rmax(R(0), ao[2])) -
                     (rmin(R(0), AB(hn3[0], hn2[0], hn1[0], fn[0])) + rmin(R(0), AB(hn3[1], hn2[1], hn1[1], fn[1])) +
                      rmin(R(0), AB(hn3[2], hn2[2], hn1[2], fn[2])));
-    stg(beta, s, ar * (habs(G, ml) - R(1.0e-6)) / (dt * (a_out + R(1.0e-18))));
-  } else {
-    stg(beta, s, R(0));
+    bt = ar * (habs(G, ml) - R(1.0e-6)) / (dt * (a_out + R(1.0e-18)));
   }
+  put1(G.pf, wb, beta, s, bt);
 }
 
 // mass_fct.F:410-470 set_corrected_mss_anti (in place; non-interior slots
@@ -361,14 +444,21 @@ __global__ void k_mass_anti(GeoT<R> G, const R* __restrict__ beta, R* __restrict
   const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
-  if (!interior(c, iim, jjm)) return;
+  const PushW w = push_load(G.pe, c, iim, jjm, s);
+  if (!interior(c, iim, jjm)) {
+    put_zeros<R, 3>(w, anti, ns, s);
+    return;
+  }
   const int i = c.i, j = c.j, k = c.k;
   const R b = rmin(R(1), beta[s]);
   const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+  R v[3];
+#pragma unroll
   for (int e = 0; e < 3; ++e) {
     const R a = anti[e * ns + s];
-    anti[e * ns + s] = b * rmax(R(0), a) + rmin(R(1), beta[up[e]]) * rmin(R(0), a);
+    v[e] = b * rmax(R(0), a) + rmin(R(1), beta[up[e]]) * rmin(R(0), a);
   }
+  put_all<R, 3>(G.pe, w, anti, ns, s, v[0], v[1], v[2]);
 }
 
 // tracer FCT, first- and high-order edge fluxes (tracer_fct.F:166-243) of
@@ -419,6 +509,8 @@ __global__ void k_mass_final(GeoT<R> G, R dt, const R* __restrict__ flx1, const 
   // (i, j+1) (edge_update), and the first tracer at the slot and at the
   // three upstream neighbours (tr_fluxes_at)
   const int en[3] = {cl(SL(i + 1, j, k)), cl(SL(i + 1, j + 1, k)), cl(SL(i, j + 1, k))};
+  const PushW wf = push_load(G.pf, c, iim, jjm, s);
+  const PushW we = T.t3 != nullptr ? push_load(G.pe, c, iim, jjm, s) : PushW{};
   R fl[3], an[6];
 #pragma unroll
   for (int e = 0; e < 3; ++e) {
@@ -464,24 +556,24 @@ __global__ void k_mass_final(GeoT<R> G, R dt, const R* __restrict__ flx1, const 
     hn = base - dt_ai * ((an[0] + an[1] + an[2]) - (an[3] + an[4] + an[5]));
   else
     hn = R(0);
-  stg(h4, s, hn);
+  put1(G.pf, wf, h4, s, hn);
   // the first tracer's edge fluxes take the total flux of the slot's own
   // edges only, which this slot has just formed (k_tr_fluxes fused;
   // tr_fluxes_at's arithmetic)
   if (tr) {
     const R w3 = T.w3, w2 = T.w2, w1 = T.w1;
-    if (!in) {
-#pragma unroll
-      for (int e = 0; e < 3; ++e) { stg(f1 + (size_t)e * ns, s, R(0)); stg(f2 + (size_t)e * ns, s, R(0)); }
-    } else {
+    R v1[3] = {R(0), R(0), R(0)}, v2[3] = {R(0), R(0), R(0)};
+    if (in) {
       auto TH = [&](int q) { return w3 * a3[q] + w2 * a4[q] + w1 * ao[q]; };
       const R ths = TH(0);
 #pragma unroll
       for (int e = 0; e < 3; ++e) {
-        stg(f1 + (size_t)e * ns, s, a3[0] * rmax(R(0), F[e]) + a3[1 + e] * rmin(R(0), F[e]));
-        stg(f2 + (size_t)e * ns, s, R(0.5) * (ths + TH(1 + e)) * F[e]);
+        v1[e] = a3[0] * rmax(R(0), F[e]) + a3[1 + e] * rmin(R(0), F[e]);
+        v2[e] = R(0.5) * (ths + TH(1 + e)) * F[e];
       }
     }
+    put_all<R, 3>(G.pe, we, f1, ns, s, v1[0], v1[1], v1[2]);
+    put_all<R, 3>(G.pe, we, f2, ns, s, v2[0], v2[1], v2[2]);
   }
 }
 
@@ -689,8 +781,10 @@ __global__ void k_vort_curl(GeoT<R> G, const R* __restrict__ mflux, const R* __r
   const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
+  const PushW w = push_load(G.pe, c, iim, jjm, s);
   if (!interior(c, iim, jjm)) {
-    for (int e = 0; e < 3; ++e) { vf[e * ns + s] = R(0); dcurl[e * ns + s] = R(0); }
+    put_all<R, 3>(G.pe, w, vf, ns, s, R(0), R(0), R(0));
+    put_all<R, 3>(G.pe, w, dcurl, ns, s, R(0), R(0), R(0));
     return;
   }
   const int i = c.i, j = c.j, k = c.k;
@@ -704,11 +798,15 @@ __global__ void k_vort_curl(GeoT<R> G, const R* __restrict__ mflux, const R* __r
   Half<R> ht;
   if constexpr (sizeof(R) == 4) ht = half_edges_pre<R>(iim, jjm, ns, i, j, k, G.tdc, hc, u);
   else ht = half_edges<R>(iim, jjm, ns, i, j, k, G.dc, G.tng, hc, u);
+  R v1[3], v2[3];
+#pragma unroll
   for (int e = 0; e < 3; ++e) {
     const R pvu = PV(up[e]);
-    vf[e * ns + s] = R(0.5) * (mflux[e * ns + s]) * (pv0 + pvu);
-    dcurl[e * ns + s] = (-ht.r[0][e] + ht.r[1][e]) * (pvu + pv0) * R(0.5);
+    v1[e] = R(0.5) * (mflux[e * ns + s]) * (pv0 + pvu);
+    v2[e] = (-ht.r[0][e] + ht.r[1][e]) * (pvu + pv0) * R(0.5);
   }
+  put_all<R, 3>(G.pe, w, vf, ns, s, v1[0], v1[1], v1[2]);
+  put_all<R, 3>(G.pe, w, dcurl, ns, s, v2[0], v2[1], v2[2]);
 }
 
 // operators.F:270-392 div_of_an_edge_flux at one slot (0 where not defined)
@@ -833,6 +931,7 @@ __device__ __forceinline__ void tendencies_at(const GeoT<R>& G, int s, R dt, R g
                                               R& rhs1, R& rhs2) {
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const IJK c = ijk_of(s, iim, jjm);
+  const PushW w = push_load(G.pf, c, iim, jjm, s);
   const R ai = G.area_inv[s];
   const R wv = div_edge_at<R>(iim, jjm, ns, c, vf, ai);
   const R ef = -wv;
@@ -848,11 +947,13 @@ __device__ __forceinline__ void tendencies_at(const GeoT<R>& G, int s, R dt, R g
   divf_a[s] = df;
   const R d4 = div3[s] + dt * (w3 * df + w2 * divf_b[s] + w1 * divf_c[s]);
   eta4[s] = e4;
-  div4[s] = d4;
   R rel;
   if constexpr (sizeof(R) == 4) rel = e4;
   else rel = e4 - G.f[s];
-  relative[s] = rel;
+  // the new divergence and relative vorticity with their halo copies (the
+  // new vorticity is formed again after the solve, which stores its copies)
+  put1(G.pf, w, div4, s, d4);
+  put1(G.pf, w, relative, s, rel);
   rhs1 = rel;
   rhs2 = d4;
 }
@@ -1653,21 +1754,30 @@ __global__ void k_velocity(GeoT<R> G, const double* __restrict__ psi, const doub
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
+  const IJK c = ijk_of(s, iim, jjm);
   if (eta4) {
     // mixed path: the new level's vorticity is the solve's right-hand side
     // less its mean (k_post_elliptic fused here)
     at_member(G.mb, rhs1, eta4);
     if (bsum) at_member(G.mb, bsum);
-    if constexpr (sizeof(R) == 4) eta4[s] = rhs_less_mean(rhs1[s], bsum, 0, total);
+    if constexpr (sizeof(R) == 4) {
+      const PushW wf = push_load(G.pf, c, iim, jjm, s);
+      const R e4 = rhs_less_mean(rhs1[s], bsum, 0, total);
+      put1(G.pf, wf, eta4, s, e4);
+    }
   }
-  const IJK c = ijk_of(s, iim, jjm);
+  const PushW w = push_load(G.pv, c, iim, jjm, s);
   if (!interior(c, iim, jjm)) {
-    for (int p = 0; p < 4; ++p) u[p * ns + s] = R(0);
+    // the corner table's components are the corners; each space component
+    // is one field of it
+    put_all<R, 2>(G.pv, w, u, ns, s, R(0), R(0));
+    put_all<R, 2>(G.pv, w, u + (size_t)2 * ns, ns, s, R(0), R(0));
     return;
   }
   const int i = c.i, j = c.j, k = c.k;
   const int nb[2][3] = {{s, SL(i - 1, j, k), SL(i - 1, j - 1, k)}, {s, SL(i - 1, j - 1, k), SL(i, j - 1, k)}};
 #define VW(kk, sp, ic) G.vw[((kk) + 3 * ((sp) + 2 * (ic))) * ns + s]
+  R uv[4];   // planes ic + 2 * space
   for (int ic = 0; ic < 2; ++ic) {
     if constexpr (sizeof(R) == 8) {
       const double cai = G.cai[ic * ns + s];
@@ -1676,8 +1786,8 @@ __global__ void k_velocity(GeoT<R> G, const double* __restrict__ psi, const doub
         gc[sp] = (+VW(0, sp, ic) * chi[nb[ic][0]] + VW(1, sp, ic) * chi[nb[ic][1]] + VW(2, sp, ic) * chi[nb[ic][2]]) * cai;
         gp[sp] = (+VW(0, sp, ic) * psi[nb[ic][0]] + VW(1, sp, ic) * psi[nb[ic][1]] + VW(2, sp, ic) * psi[nb[ic][2]]) * cai;
       }
-      u[(ic + 0) * ns + s] = gc[0] + (-gp[1]);
-      u[(ic + 2) * ns + s] = gc[1] + gp[0];
+      uv[ic + 0] = gc[0] + (-gp[1]);
+      uv[ic + 2] = gc[1] + gp[0];
     } else {
       const double cai = (double)G.cai[ic * ns + s];
       const double c1 = chi[nb[ic][1]] - chi[nb[ic][0]], c2 = chi[nb[ic][2]] - chi[nb[ic][0]];
@@ -1688,11 +1798,13 @@ __global__ void k_velocity(GeoT<R> G, const double* __restrict__ psi, const doub
         gc[sp] = (v1 * c1 + v2 * c2) * cai;
         gp[sp] = (v1 * p1 + v2 * p2) * cai;
       }
-      u[(ic + 0) * ns + s] = (R)(gc[0] - gp[1]);
-      u[(ic + 2) * ns + s] = (R)(gc[1] + gp[0]);
+      uv[ic + 0] = (R)(gc[0] - gp[1]);
+      uv[ic + 2] = (R)(gc[1] + gp[0]);
     }
   }
 #undef VW
+  put_all<R, 2>(G.pv, w, u, ns, s, uv[0], uv[1]);
+  put_all<R, 2>(G.pv, w, u + (size_t)2 * ns, ns, s, uv[2], uv[3]);
 }
 
 // operators.F:1316-1393 with the corner kinetic energy 0.5 |u|^2 (step.F:192-203)
@@ -1704,16 +1816,18 @@ __global__ void k_center_ke(GeoT<R> G, const R* __restrict__ u, R* __restrict__ 
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const int i = c.i, j = c.j, k = c.k;
-  if (i > iim - 1 || j > jjm - 1) { ke[s] = R(0); return; }
+  const PushW w = push_load(G.pf, c, iim, jjm, s);
+  if (i > iim - 1 || j > jjm - 1) { put1(G.pf, w, ke, s, R(0)); return; }
   auto KC = [&](int ic, int sl) {
     const R a = u[ic * ns + sl], b = u[(ic + 2) * ns + sl];
     return R(0.5) * (a * a + b * b);
   };
 #define SA(kk, ic, sl) G.sub_area[((kk) + 3 * (ic)) * ns + (sl)]
   const int se = SL(i + 1, j, k), sne = SL(i + 1, j + 1, k), sn = SL(i, j + 1, k);
-  ke[s] = (+SA(0, 0, s) * KC(0, s) + SA(0, 1, s) * KC(1, s) + SA(1, 0, se) * KC(0, se) + SA(1, 1, sne) * KC(1, sne) +
-           SA(2, 0, sne) * KC(0, sne) + SA(2, 1, sn) * KC(1, sn)) * G.area_inv[s];
+  const R v = (+SA(0, 0, s) * KC(0, s) + SA(0, 1, s) * KC(1, s) + SA(1, 0, se) * KC(0, se) +
+               SA(1, 1, sne) * KC(1, sne) + SA(2, 0, sne) * KC(0, sne) + SA(2, 1, sn) * KC(1, sn)) * G.area_inv[s];
 #undef SA
+  put1(G.pf, w, ke, s, v);
 }
 
 // operators.F:47-261 curl and divergence of a corner vector (initial state)
@@ -1751,8 +1865,11 @@ __global__ void k_post_elliptic(GeoT<R> G, const R* __restrict__ rhs1, R* __rest
   at_member(G.mb, rhs1, eta4);
   const int s = xcd_slot();
   if (s >= G.ns) return;
-  if constexpr (sizeof(R) == 4) eta4[s] = rhs1[s];
-  else eta4[s] = rhs1[s] + G.f[s];
+  R v;
+  if constexpr (sizeof(R) == 4) v = rhs1[s];
+  else v = rhs1[s] + G.f[s];
+  const PushW w = G.pf.hp ? push_load(G.pf, ijk_of(s, G.iim, G.jjm), G.iim, G.jjm, s) : PushW{};
+  put1(G.pf, w, eta4, s, v);
 }
 template <class R>
 __global__ void k_inv_h(GeoT<R> G, const R* __restrict__ h4, R* __restrict__ inv_h) {

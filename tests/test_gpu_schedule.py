@@ -7,6 +7,10 @@ change how the mixed solve converges:
   - the multigrid kernel fusions (swm_set_fusion) give the same fields bit
     for bit as the unfused cycle, FP64 and mixed, for two finest-level sweep
     counts, over levels 6..4 with their own sweeps;
+  - the producer-side halo writes of the step (SWM_FUSE_HALO_PUSH) give the
+    same fields, halo slots included, bit for bit as the step's halo fills,
+    FP64 and mixed, one and two tracers, with and without the tracer
+    stream, for every member of a two-member context;
   - the mixed solve's coarse correction (swm_set_mixed_coarse): the dense
     solve of level 3 and the reference's V-cycle over levels 3..1 both meet
     the criterion, with errors against the FP64 form within 3x of each
@@ -68,6 +72,29 @@ def test_fusion_gives_the_same_fields(precision, sweeps):
         assert np.array_equal(a.field(n), b.field(n)), n
     with pytest.raises(S.SwmError):
         b.set_fusion(S.FUSE_ALL + 8)
+
+
+@pytest.mark.parametrize("precision,ntracer,overlap", [(S.SWM_FP64, 2, 3), (S.SWM_MIXED, 2, 3), (S.SWM_MIXED, 1, 0),
+                                                     (S.SWM_FP64, 1, 1)])
+def test_halo_push_gives_the_same_fields(precision, ntracer, overlap):
+    print(f"case halo push precision={precision} ntracer={ntracer} overlap={overlap}", flush=True)
+    grid = S.Grid(6)
+    dt = 200.0 * 2 ** (5 - grid.g)
+    ms = []
+    for flags in (S.FUSE_INJECT_SWEEP, S.FUSE_ALL):
+        m = S.Model(grid, ntracer=ntracer, dt=dt, grav=9.81, precision=precision, members=2)
+        m.set_overlap(overlap)
+        m.set_fusion(flags)
+        for z in range(2):
+            m.select(z).init_synthetic_member(100 + z, lmax=12, seed_index=2)
+        m.step(6)
+        ms.append(m)
+    a, b = ms
+    for z in range(2):
+        a.select(z)
+        b.select(z)
+        for n in FIELDS + ("ke", "relative"):
+            assert np.array_equal(a.field(n), b.field(n)), (z, n)
 
 
 def test_mixed_coarse_solve_meets_the_criterion():
