@@ -316,13 +316,19 @@ swm_status swm_set_mixed_smoother(swm_ctx* ctx, int fine_sweeps, int half_weight
  * tracer mode 3 saves about 2 % of either step, mode 1 under 1 %. */
 swm_status swm_set_overlap(swm_ctx* ctx, int mode);
 
-/* Mixed path: the coarse correction below level 4 (default direct = 1) as
- * one dense FP32 solve of level 3, alpha = M beta with M the inverse of the
- * level's Laplacian bordered by ones, formed once in FP64 on the host (1.6
- * MB), instead of the reference's V-cycle over levels 3..1
- * (multigrid_solver.F:476-591), which direct = 0 restores.  It only
- * preconditions the FP64 defect correction, whose criterion is unchanged.
- * Refused for FP64 contexts, whose multigrid cycle is the reference's. */
+/* Mixed path: the coarse correction as one dense FP32 solve, alpha = M beta
+ * with M the inverse of the level's Laplacian bordered by ones, formed once
+ * per context in FP64, instead of the reference's V-cycle
+ * (multigrid_solver.F:476-591):
+ *   direct = 1 (default): of level 3 (642 cells, 1.6 MB; formed on the
+ *              host);
+ *   direct = 2: of level 4 (2562 cells, 26 MB; formed on the device at
+ *              set-up), where level 5 exists, else as direct = 1; at level
+ *              9 it takes the same cycles and time as direct = 1;
+ *   direct = 0: the reference's V-cycle below level 4.
+ * It only preconditions the FP64 defect correction, whose criterion is
+ * unchanged.  Refused for FP64 contexts, whose multigrid cycle is the
+ * reference's. */
 swm_status swm_set_mixed_coarse(swm_ctx* ctx, int direct);
 
 /* Kernel fusions of the multigrid cycle and the step, both paths (default

@@ -215,10 +215,14 @@ struct swm_ctx {
   int mixed_fine_sweeps = 2;   // mixed path: smoothing sweeps before / after the coarse correction on the finest level
   int mixed_half = 1;          // mixed path: half-precision weights in the smoother and restriction above the coarse levels
   int fuse_mg = 1;             // multigrid: an injection also performs the coarse level's first sweep (SWM_FUSE_INJECT_SWEEP)
-  int mixed_coarse_direct = 1; // mixed path: the three coarsest levels as one dense solve (k_coarse_direct)
+  int mixed_coarse_direct = 1; // mixed path: the coarsest levels as one dense solve of level 3 (1) or 4 (2, where level 5 exists), or the V-cycle (0)
   float* cd_m = nullptr;       // its FP32 matrix (cd_n x cd_n) and the real cells of level 3
   int* cd_cell = nullptr;
   int cd_n = 0;
+  float* cd_b = nullptr;       // the dense solve's right-hand side in its cell order (member block, 2 planes)
+  float* cd4_m = nullptr;      // level 4 (levels 5 and above exist): its matrix and real cells
+  int* cd4_cell = nullptr;
+  int cd4_n = 0;
   double last_resid[2] = {0.0, 0.0};
   double last_rhs_ms[2] = {0.0, 0.0};
   std::vector<double> rhs_ms;   // per member and field (relative criterion)
@@ -473,6 +477,14 @@ R* vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_ze
           bool pre_done = false, size_t pre_off = 0) {
   const int G = C->g;
   const int top = G < swm::dev::CO_MAXLEV ? G : swm::dev::CO_MAXLEV;   // levels 1..top in one workgroup
+  // mixed path: the level of the dense coarse solve (levels below it run no
+  // sweeps), else top
+  int dtop = top;
+  bool direct = false;
+  if constexpr (sizeof(R) == 4) {
+    if (C->mixed_coarse_direct == 2 && C->cd4_n && G > top + 1) { dtop = top + 1; direct = true; }
+    else if (C->mixed_coarse_direct && C->cd_n && G > top) direct = true;
+  }
   auto A = [&](int l) { return l == G ? alpha : V.lev[l - 1].alpha; };
   auto B = [&](int l) { return l == G ? const_cast<R*>(beta) : V.lev[l - 1].beta; };
   // finest level with another sweep count (mixed correction cycles): the
@@ -497,23 +509,28 @@ R* vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_ze
   auto inject = [&](int l, const R* xa, bool half_ok) {
     LevelT<R>& Fl = V.lev[l - 1];
     LevelT<R>& Cl = V.lev[l - 2];
-    R* z0 = fuse && l - 1 > top ? Cl.temp : nullptr;
+    R* z0 = fuse && l - 1 > dtop ? Cl.temp : nullptr;
+    // into the level of a dense coarse solve: its right-hand side in the
+    // solve's cell order as well
+    R* xc = nullptr;
+    if constexpr (sizeof(R) == 4)
+      if (direct && l - 1 == dtop) xc = C->cd_b;
     const dim3 grid = zgrid(C, nblk_b(Cl.ns, C->blk_mg)), blk(C->blk_mg);
     if constexpr (sizeof(R) == 4) {
       if (half_ok && Fl.L.hw && C->mixed_half && top_zero) {
         if (nvar == 2)
-          hipLaunchKernelGGL((swm::dev::k_inject_h<2>), grid, blk, 0, C->stream, Fl.L, Cl.L, xa, B(l), B(l - 1), z0);
+          hipLaunchKernelGGL((swm::dev::k_inject_h<2>), grid, blk, 0, C->stream, Fl.L, Cl.L, xa, B(l), B(l - 1), z0, xc);
         else
-          hipLaunchKernelGGL((swm::dev::k_inject_h<1>), grid, blk, 0, C->stream, Fl.L, Cl.L, xa, B(l), B(l - 1), z0);
+          hipLaunchKernelGGL((swm::dev::k_inject_h<1>), grid, blk, 0, C->stream, Fl.L, Cl.L, xa, B(l), B(l - 1), z0, xc);
         return;
       }
     }
     if (nvar == 2)
-      hipLaunchKernelGGL((swm::dev::k_inject<R, 2>), grid, blk, 0, C->stream, Fl.L, Cl.L, xa, B(l), B(l - 1), z0);
+      hipLaunchKernelGGL((swm::dev::k_inject<R, 2>), grid, blk, 0, C->stream, Fl.L, Cl.L, xa, B(l), B(l - 1), z0, xc);
     else
-      hipLaunchKernelGGL((swm::dev::k_inject<R, 1>), grid, blk, 0, C->stream, Fl.L, Cl.L, xa, B(l), B(l - 1), z0);
+      hipLaunchKernelGGL((swm::dev::k_inject<R, 1>), grid, blk, 0, C->stream, Fl.L, Cl.L, xa, B(l), B(l - 1), z0, xc);
   };
-  for (int l = G; l > top; --l) {
+  for (int l = G; l > dtop; --l) {
     if (C->hook && l == C->hook_level) {
       std::function<void()> h = std::move(C->hook);
       C->hook = nullptr;
@@ -536,18 +553,26 @@ R* vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_ze
       relax<R>(C, V, l, 2, A(l), B(l), nvar, l < G || top_zero);
     inject(l, A(l), true);
   }
-  bool direct = false;
   if constexpr (sizeof(R) == 4) {
-    if (C->cd_n && C->mixed_coarse_direct && top < G) {
+    if (direct && dtop == top + 1) {
+      // level 4: eight rows per workgroup of 512
+      const dim3 grid = zgrid(C, (unsigned)((C->cd4_n + 7) / 8)), blk(512);
+      const int n4 = V.lev[dtop - 1].ns;
+      if (nvar == 2)
+        hipLaunchKernelGGL((swm::dev::k_coarse_direct<2, swm::dev::CD_MAXN4>), grid, blk, 0, C->stream, C->cd4_m,
+                           C->cd4_cell, C->cd4_n, n4, C->cd_b, A(dtop), C->mb);
+      else
+        hipLaunchKernelGGL((swm::dev::k_coarse_direct<1, swm::dev::CD_MAXN4>), grid, blk, 0, C->stream, C->cd4_m,
+                           C->cd4_cell, C->cd4_n, n4, C->cd_b, A(dtop), C->mb);
+    } else if (direct) {
       const dim3 grid = zgrid(C, (unsigned)((C->cd_n + 3) / 4)), blk(256);
       const int n3 = V.lev[top - 1].ns;
       if (nvar == 2)
-        hipLaunchKernelGGL(swm::dev::k_coarse_direct<2>, grid, blk, 0, C->stream, C->cd_m, C->cd_cell, C->cd_n, n3, B(top),
-                           A(top), C->mb);
+        hipLaunchKernelGGL((swm::dev::k_coarse_direct<2, swm::dev::CD_MAXN>), grid, blk, 0, C->stream, C->cd_m, C->cd_cell,
+                           C->cd_n, n3, C->cd_b, A(top), C->mb);
       else
-        hipLaunchKernelGGL(swm::dev::k_coarse_direct<1>, grid, blk, 0, C->stream, C->cd_m, C->cd_cell, C->cd_n, n3, B(top),
-                           A(top), C->mb);
-      direct = true;
+        hipLaunchKernelGGL((swm::dev::k_coarse_direct<1, swm::dev::CD_MAXN>), grid, blk, 0, C->stream, C->cd_m, C->cd_cell,
+                           C->cd_n, n3, C->cd_b, A(top), C->mb);
     }
   }
   swm::dev::CoarseArgs<R> P{};
@@ -559,7 +584,7 @@ R* vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_ze
   P.zero = (top < G || top_zero) ? 1 : 0;
   P.mb = C->mb;
   if (!direct) hipLaunchKernelGGL(swm::dev::k_coarse_cycle<R>, zgrid(C, 1), dim3(swm::dev::CO_THREADS), 0, C->stream, P);
-  for (int l = top; l <= G - 1; ++l) {
+  for (int l = dtop; l <= G - 1; ++l) {
     const bool f = l + 1 == G && own;
     R* up = f ? fa : A(l + 1);
     if (nvar == 2)
@@ -577,12 +602,13 @@ R* vcycle(swm_ctx* C, View<R>& V, int nvar, R* alpha, const R* beta, bool top_ze
   return alpha;
 }
 
-// step.F:132-141 of the next step, queued before the host waits on a
-// convergence test of this step's solve, so that the GPU has work while the
-// host reads the norms: the corner thickness of the next step depends only
-// on this step's new thickness and its halo (FCT: both complete before the
-// solve), and nothing in the rest of this step reads or writes it.  The
-// next step then skips it.
+// step.F:132-141 of the next step, queued early: on the second stream with
+// the tracer transport, or else before the host waits on a convergence test
+// of this step's solve, so that the GPU has work while the host reads the
+// norms.  The corner thickness of the next step depends only on this step's
+// new thickness and its halo (FCT: both complete before the solve), and
+// nothing in the rest of this step reads or writes it.  The next step then
+// skips it.
 template <class R>
 void corners_next(swm_ctx* C, View<R>& V) {
   const int n4 = C->n4 - 1;
@@ -1128,7 +1154,8 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   // the first V-cycle when it reaches a level of latency-bound launches
   // (mode >= 2: level g - mode + 1, through C->hook).
   swm_status tr_st = SWM_OK;
-  auto tracers = [&]() {
+  // late: started by the V-cycle (after this step's k_vort_curl is queued)
+  auto tracers = [&](bool late) {
     const hipStream_t main_stream = C->stream;
     if (overlap) {
       hipError_t he = hipEventRecord(C->ev_fork, main_stream);
@@ -1171,6 +1198,10 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
       for (int q = 0; q < C->nt; ++q) tf.push_back(V.tr[n4][q]);
       gather_list<R>(C, C->face, tf);
     }
+    // on the second stream the next step's corner thickness joins the
+    // tracers when they start late (this step's last reader of it,
+    // k_vort_curl, is then before the fork; the next step waits for the join)
+    if (overlap && late) corners_for_next(C);
     if (overlap) {
       const hipError_t he = hipEventRecord(C->ev_join, C->stream2);
       if (he != hipSuccess) {
@@ -1182,10 +1213,10 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   };
   const int hook_level = C->g - C->overlap + 1;
   if (overlap && C->overlap >= 2 && hook_level > swm::dev::CO_MAXLEV) {
-    C->hook = tracers;
+    C->hook = [&]() { tracers(true); };
     C->hook_level = hook_level;
   } else {
-    tracers();
+    tracers(false);
   }
   // vorticity and divergence (step.F:143-163, :336-429)
   hipLaunchKernelGGL(swm::dev::k_vort_curl<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.mass_flux, V.h[n3], V.eta[n3], V.hc,
@@ -1475,12 +1506,11 @@ namespace swm {
 
 // The dense coarse solve of the mixed path (k_coarse_direct): the real cells
 // of level l (interior slots panel by panel, then the two poles) and the
-// FP32 n x n block of the inverse of the level's Laplacian (the smoother's
-// operator: l_weights(7) at the cell, l_weights(1..6) at its neighbours
-// through the level's halo map; the poles' forms leave one out, as k_relax),
-// bordered by ones so that it is regular.  Gauss-Jordan elimination with
-// partial pivoting in FP64 on the host, once per context.
-void coarse_direct_matrix(const std::vector<double>& lw, int l, std::vector<float>& M, std::vector<int>& cell) {
+// level's Laplacian (the smoother's operator: l_weights(7) at the cell,
+// l_weights(1..6) at its neighbours through the level's halo map; the
+// poles' forms leave one out, as k_relax), bordered by ones so that it is
+// regular: B = [[A, 1], [1^T, 0]], row-major m x m with m = n + 1.
+void coarse_bordered(const std::vector<double>& lw, int l, std::vector<double>& B, std::vector<int>& cell) {
   const int iim = 2 + (1 << l), jjm = iim, ns = iim * jjm * 10;
   auto SL = [&](int i, int j, int k) { return (i - 1) + iim * ((j - 1) + jjm * (k - 1)); };
   std::vector<int> src(ns);
@@ -1496,8 +1526,7 @@ void coarse_direct_matrix(const std::vector<double>& lw, int l, std::vector<floa
   const int n = (int)cell.size(), m = n + 1;
   std::vector<int> idx(ns, -1);
   for (int r = 0; r < n; ++r) idx[cell[r]] = r;
-  // bordered matrix [[A, 1], [1^T, 0]], row-major m x m, and its inverse
-  std::vector<double> B((size_t)m * m, 0.0), X((size_t)m * m, 0.0);
+  B.assign((size_t)m * m, 0.0);
   for (int r = 0; r < n; ++r) {
     const int s = cell[r];
     const int i = s % iim + 1, j = (s / iim) % jjm + 1, k = s / (iim * jjm) + 1;
@@ -1515,6 +1544,15 @@ void coarse_direct_matrix(const std::vector<double>& lw, int l, std::vector<floa
     B[(size_t)r * m + n] = 1.0;
     B[(size_t)n * m + r] = 1.0;
   }
+}
+
+// level 3: the FP32 n x n block of B's inverse by Gauss-Jordan elimination
+// with partial pivoting in FP64 on the host (0.1 s)
+void coarse_direct_matrix(const std::vector<double>& lw, int l, std::vector<float>& M, std::vector<int>& cell) {
+  std::vector<double> B;
+  coarse_bordered(lw, l, B, cell);
+  const int n = (int)cell.size(), m = n + 1;
+  std::vector<double> X((size_t)m * m, 0.0);
   for (int r = 0; r < m; ++r) X[(size_t)r * m + r] = 1.0;
   for (int c = 0; c < m; ++c) {
     int p = c;
@@ -1537,6 +1575,63 @@ void coarse_direct_matrix(const std::vector<double>& lw, int l, std::vector<floa
   M.assign((size_t)n * n, 0.0f);
   for (int r = 0; r < n; ++r)
     for (int c = 0; c < n; ++c) M[(size_t)r * n + c] = (float)X[(size_t)r * m + c];
+}
+
+// level 4 (2563 x 2563): the same inverse by in-place Gauss-Jordan on the
+// device (k_gj_pivot / k_gj_update, FP64, about 0.1 s), the row swaps undone
+// as column swaps on the host; its FP32 n x n block into Md
+swm_status coarse_direct_matrix_device(swm_ctx* C, const std::vector<double>& lw, int l, float* Md, int* celld,
+                                       int nexp) {
+  std::vector<double> B;
+  std::vector<int> cell;
+  coarse_bordered(lw, l, B, cell);
+  const int n = (int)cell.size(), m = n + 1;
+  if (n != nexp) { swm::set_error("coarse solve: level size"); return SWM_ERR_DIMS; }
+  double *A = nullptr, *rowb = nullptr, *colb = nullptr;
+  int* perm = nullptr;
+  swm_status st = SWM_OK;
+  auto chk = [&](hipError_t e) {
+    if (e != hipSuccess && st == SWM_OK) {
+      swm::set_error(std::string("coarse solve set-up: ") + hipGetErrorString(e));
+      st = SWM_ERR_HIP;
+    }
+    return st == SWM_OK;
+  };
+  if (chk(hipMalloc(&A, B.size() * sizeof(double))) && chk(hipMalloc(&rowb, m * sizeof(double))) &&
+      chk(hipMalloc(&colb, m * sizeof(double))) && chk(hipMalloc(&perm, m * sizeof(int))) &&
+      chk(hipMemcpyAsync(A, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice, C->stream))) {
+    const unsigned nbu = (unsigned)(((size_t)m * m + 255) / 256);
+    for (int c = 0; c < m; ++c) {
+      hipLaunchKernelGGL(swm::dev::k_gj_pivot, dim3(1), dim3(swm::dev::GJ_THREADS), 0, C->stream, A, m, c, rowb, colb, perm);
+      hipLaunchKernelGGL(swm::dev::k_gj_update, dim3(nbu), dim3(256), 0, C->stream, A, m, c, (const double*)rowb,
+                         (const double*)colb);
+    }
+    std::vector<int> pv(m);
+    if (chk(hipGetLastError()) &&
+        chk(hipMemcpyAsync(B.data(), A, B.size() * sizeof(double), hipMemcpyDeviceToHost, C->stream)) &&
+        chk(hipMemcpyAsync(pv.data(), perm, m * sizeof(int), hipMemcpyDeviceToHost, C->stream)) &&
+        chk(hipStreamSynchronize(C->stream))) {
+      for (int c = m - 1; c >= 0; --c)
+        if (pv[c] != c)
+          for (int r = 0; r < m; ++r) std::swap(B[(size_t)r * m + c], B[(size_t)r * m + pv[c]]);
+      std::vector<float> M((size_t)n * n);
+      bool finite = true;
+      for (int r = 0; r < n; ++r)
+        for (int c = 0; c < n; ++c) {
+          const double x = B[(size_t)r * m + c];
+          finite = finite && std::isfinite(x);
+          M[(size_t)r * n + c] = (float)x;
+        }
+      if (!finite && st == SWM_OK) { swm::set_error("coarse solve set-up: singular level-4 operator"); st = SWM_ERR_NAN; }
+      if (st == SWM_OK && chk(hipMemcpy(Md, M.data(), M.size() * sizeof(float), hipMemcpyHostToDevice)))
+        chk(hipMemcpy(celld, cell.data(), cell.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+  }
+  if (A) (void)hipFree(A);
+  if (rowb) (void)hipFree(rowb);
+  if (colb) (void)hipFree(colb);
+  if (perm) (void)hipFree(perm);
+  return st;
 }
 
 // buffers of a view: multigrid level weights and maps in the shared arena
@@ -1630,8 +1725,15 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
       C->cd_m = A.take<float>((size_t)C->cd_n * C->cd_n);
       C->cd_cell = A.take<int>((size_t)C->cd_n);
     }
+    if (C->g > swm::dev::CO_MAXLEV + 1) {
+      const int e = 2 << swm::dev::CO_MAXLEV;
+      C->cd4_n = 10 * e * e + 2;
+      C->cd4_m = A.take<float>((size_t)C->cd4_n * C->cd4_n);
+      C->cd4_cell = A.take<int>((size_t)C->cd4_n);
+    }
     C->mx_r = C->marena.take<float>(2 * ns);
     C->mx_e = C->marena.take<float>(2 * ns);
+    if (C->cd_n) C->cd_b = C->marena.take<float>(2 * (size_t)std::max(C->cd_n, C->cd4_n));
   }
 }
 
@@ -1936,6 +2038,12 @@ swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
       }
       HIP_TRY(hipMemcpy(C->cd_m, M.data(), M.size() * sizeof(float), hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(C->cd_cell, cell.data(), cell.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (C->cd4_n) {
+      if (C->cd4_n > swm::dev::CD_MAXN4) { swm::set_error("coarse solve: level size"); return SWM_ERR_DIMS; }
+      swm_status cst = coarse_direct_matrix_device(C, Gh.levels[swm::dev::CO_MAXLEV].lap_w, swm::dev::CO_MAXLEV + 1,
+                                                   C->cd4_m, C->cd4_cell, C->cd4_n);
+      if (cst) return cst;
     }
     (void)nb;
   }
@@ -2387,7 +2495,8 @@ swm_status swm_set_overlap(swm_ctx* C, int mode) {
 
 swm_status swm_set_mixed_coarse(swm_ctx* C, int direct) {
   if (!C || C->precision != SWM_MIXED) { swm::set_error("swm_set_mixed_coarse: needs a mixed-precision context"); return SWM_ERR_ARG; }
-  C->mixed_coarse_direct = direct ? 1 : 0;
+  if (direct < 0 || direct > 2) { swm::set_error("swm_set_mixed_coarse: direct is 0, 1 or 2"); return SWM_ERR_ARG; }
+  C->mixed_coarse_direct = direct;
   C->epoch += 1;
   return SWM_OK;
 }

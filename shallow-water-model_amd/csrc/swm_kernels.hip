@@ -1553,14 +1553,26 @@ __device__ __forceinline__ void coarse_zero_sweep_rest(const LevT<R>& C, const I
     z0[(size_t)v * C.ns + s1] = pole_form ? zero_sweep<R>(C.lwt[s1], x1[(size_t)v * C.ns + s1]) : R(0);
 }
 
+// the order of a level's real cells in the dense coarse solve (interior
+// slots panel by panel, then the north and the south pole) and their count
+__device__ __forceinline__ int dense_index(const IJK& c, int iim, int jjm, bool np, bool sp) {
+  const int e = iim - 2;
+  if (np) return 10 * e * e;
+  if (sp) return 10 * e * e + 1;
+  return (c.k - 1) * e * e + (c.j - 2) * e + (c.i - 2);
+}
+__device__ __forceinline__ int dense_count(int iim) { return 10 * (iim - 2) * (iim - 2) + 2; }
+
 // Mixed path, levels above the coarse three: k_inject with the half-precision weights in
 // difference form, r = beta - (1/S) sum_k hw_k (x_k - x0) at the coincident
-// fine points
+// fine points.  xc (the level of the dense coarse solve): the right-hand
+// side also in the dense solve's cell order, NV planes of dense_count
 template <int NV>
 __global__ void k_inject_h(LevT<float> F, LevT<float> C, const float* __restrict__ xa, const float* __restrict__ xb,
-                           float* __restrict__ x1, float* __restrict__ z0) {
+                           float* __restrict__ x1, float* __restrict__ z0, float* __restrict__ xc) {
   at_member(F.mb, xa, xb, x1);
   if (z0) at_member(F.mb, z0);
+  if (xc) at_member(F.mb, xc);
   int iim = C.iim, jjm = C.jjm;
   const int s1 = xcd_slot();
   if (s1 >= C.ns) return;
@@ -1611,6 +1623,7 @@ __global__ void k_inject_h(LevT<float> F, LevT<float> C, const float* __restrict
     const float r = ldg(xb + (size_t)v * ns, s) - F.his * acc;
     stg(x1 + (size_t)v * C.ns, s1, r);
     if (z0) stg(z0 + (size_t)v * C.ns, s1, zero_sweep<float>(ldg(C.lwt, s1), r));
+    if (xc) stg(xc + (size_t)v * dense_count(C.iim), dense_index(c, C.iim, C.jjm, np, sp), r);
   }
 }
 
@@ -1621,9 +1634,10 @@ __global__ void k_inject_h(LevT<float> F, LevT<float> C, const float* __restrict
 // interior themselves and are read without the halo map.
 template <class R, int NV>
 __global__ void k_inject(LevT<R> F, LevT<R> C, const R* __restrict__ xa, const R* __restrict__ xb, R* __restrict__ x1,
-                         R* __restrict__ z0) {
+                         R* __restrict__ z0, R* __restrict__ xc) {
   at_member(F.mb, xa, xb, x1);
   if (z0) at_member(F.mb, z0);
+  if (xc) at_member(F.mb, xc);
   int iim = C.iim, jjm = C.jjm;
   const int s1 = xcd_slot();
   if (s1 >= C.ns) return;
@@ -1681,6 +1695,7 @@ __global__ void k_inject(LevT<R> F, LevT<R> C, const R* __restrict__ xa, const R
                 w[5] * X(6));
     stg(x1 + (size_t)v * C.ns, s1, r);
     if (z0) stg(z0 + (size_t)v * C.ns, s1, zero_sweep<R>(ldg(C.lwt, s1), r));
+    if (xc) stg(xc + (size_t)v * dense_count(C.iim), dense_index(c, C.iim, C.jjm, np, sp), r);
   }
 }
 
@@ -2323,28 +2338,45 @@ __global__ void __launch_bounds__(CO_THREADS) k_coarse_cycle(CoarseArgs<R> P) {
   for (int t = threadIdx.x; t < n; t += CO_THREADS) p_alpha[t] = at[t];
 }
 
-constexpr int CD_MAXN = 10 * 64 + 2;   // real cells of level 3
-// Mixed path: the coarse correction of the three coarsest levels as one
-// dense solve.  alpha = M beta at the level's real cells (cell[0..n)), with M
+constexpr int CD_MAXN = 10 * 64 + 2;     // real cells of level 3
+constexpr int CD_MAXN4 = 10 * 256 + 2;   // real cells of level 4
+// Mixed path: the coarse correction of the coarsest levels as one dense
+// solve.  alpha = M beta at the real cells of the level (cell[0..n)), with M
 // the FP32 n x n block of the inverse of the level's Laplacian bordered by
 // ones (A alpha + lambda 1 = beta, sum alpha = 0: the solution without the
 // constant the singular operator leaves free, for beta less its component
-// along 1 outside A's range).  One wave per row, NV fields; the lanes take
-// the columns in a fixed order and a butterfly sums them, so the result does
-// not depend on the launch.  Halo slots of alpha are not written (the
-// interpolation reads the level through its map).
-template <int NV>
-__global__ void __launch_bounds__(256) k_coarse_direct(const float* __restrict__ M, const int* __restrict__ cell, int n,
-                                                       int ns, const float* __restrict__ beta,
+// along 1 outside A's range).  One wave per row, NV fields, at most MAXN
+// cells; the lanes take the columns in a fixed order and a butterfly sums
+// them, so the result does not depend on the launch.  Halo slots of alpha
+// are not written (the interpolation reads the level through its map).
+template <int NV, int MAXN>
+__global__ void __launch_bounds__(512) k_coarse_direct(const float* __restrict__ M, const int* __restrict__ cell, int n,
+                                                       int ns, const float* __restrict__ bc,
                                                        float* __restrict__ alpha, size_t mb) {
-  at_member(mb, beta, alpha);
-  // beta at the real cells, gathered once per workgroup into LDS (at most
-  // CD_MAXN cells per field), so the rows' dot products read M and LDS only
-  __shared__ float sb[NV][CD_MAXN];
-  for (int c = threadIdx.x; c < n; c += bdim()) {
-    const int sl = ldg(cell, c);
+  at_member(mb, bc, alpha);
+  // beta at the real cells in the cell order (bc, NV planes of n, written
+  // by the injection), staged once per workgroup in LDS, so the rows' dot
+  // products read M and LDS only
+  __shared__ float sb[NV][MAXN];
+  {
+    // all of a thread's loads issued together (workgroups of 256 or more)
+    const int nt = bdim();
+    constexpr int NL = (MAXN + 255) / 256;
+    float t[NV][NL];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) sb[v][c] = ldg(beta + (size_t)v * ns, sl);
+    for (int q = 0; q < NL; ++q) {
+      const int c = (int)threadIdx.x + nt * q;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) t[v][q] = c < n ? ldg(bc + (size_t)v * n, c) : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+      const int c = (int)threadIdx.x + nt * q;
+      if (c < n) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sb[v][c] = t[v][q];
+      }
+    }
   }
   __syncthreads();
   const int r = (int)(blockIdx.x * (bdim() >> 6) + (threadIdx.x >> 6));
@@ -2354,19 +2386,23 @@ __global__ void __launch_bounds__(256) k_coarse_direct(const float* __restrict__
   float acc[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = 0.0f;
-  // all of a lane's (at most 11) loads of the row issued together
-  float mr[(CD_MAXN + 63) / 64];
+  // a lane's loads of the row issued together
+  constexpr int NQ = (MAXN + 63) / 64, BQ = NQ;
 #pragma unroll
-  for (int q = 0; q < (CD_MAXN + 63) / 64; ++q) {
-    const int c = lane + 64 * q;
-    mr[q] = c < n ? row[c] : 0.0f;
-  }
+  for (int q0 = 0; q0 < NQ; q0 += BQ) {
+    float mr[BQ];
 #pragma unroll
-  for (int q = 0; q < (CD_MAXN + 63) / 64; ++q) {
-    const int c = lane + 64 * q;
-    if (c < n) {
+    for (int q = 0; q < BQ; ++q) {
+      const int c = lane + 64 * (q0 + q);
+      mr[q] = (q0 + q < NQ && c < n) ? row[c] : 0.0f;
+    }
 #pragma unroll
-      for (int v = 0; v < NV; ++v) acc[v] = acc[v] + mr[q] * sb[v][c];
+    for (int q = 0; q < BQ; ++q) {
+      const int c = lane + 64 * (q0 + q);
+      if (q0 + q < NQ && c < n) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = acc[v] + mr[q] * sb[v][c];
+      }
     }
   }
 #pragma unroll
@@ -2378,6 +2414,61 @@ __global__ void __launch_bounds__(256) k_coarse_direct(const float* __restrict__
 #pragma unroll
     for (int v = 0; v < NV; ++v) alpha[(size_t)v * ns + ldg(cell, r)] = acc[v];
   }
+}
+
+// Set-up of a dense coarse solve: Gauss-Jordan inversion in place of the
+// m x m row-major FP64 matrix A with partial pivoting, one pivot column c per
+// pair of launches.  k_gj_pivot (one workgroup of GJ_THREADS) picks the row
+// of the largest |A(r, c)|, r >= c (the first of equal ones), swaps it into
+// row c and copies row c and column c aside; k_gj_update then forms every
+// element from those copies.  The host undoes the row swaps as column swaps
+// in reverse order afterwards.
+constexpr int GJ_THREADS = 1024;
+__global__ void __launch_bounds__(GJ_THREADS) k_gj_pivot(double* __restrict__ A, int m, int c, double* __restrict__ rowb,
+                                                         double* __restrict__ colb, int* __restrict__ perm) {
+  __shared__ double bv[GJ_THREADS];
+  __shared__ int bi[GJ_THREADS];
+  const int t = threadIdx.x;
+  double best = -1.0;
+  int bidx = m;
+  for (int r = c + t; r < m; r += GJ_THREADS) {
+    const double v = fabs(A[(size_t)r * m + c]);
+    if (v > best) { best = v; bidx = r; }
+  }
+  bv[t] = best;
+  bi[t] = bidx;
+  __syncthreads();
+  for (int h = GJ_THREADS / 2; h > 0; h >>= 1) {
+    if (t < h) {
+      const double v = bv[t + h];
+      const int ix = bi[t + h];
+      if (v > bv[t] || (v == bv[t] && ix < bi[t])) { bv[t] = v; bi[t] = ix; }
+    }
+    __syncthreads();
+  }
+  const int p = bi[0];
+  if (p != c)
+    for (int q = t; q < m; q += GJ_THREADS) {
+      const double a = A[(size_t)p * m + q];
+      A[(size_t)p * m + q] = A[(size_t)c * m + q];
+      A[(size_t)c * m + q] = a;
+    }
+  __syncthreads();
+  for (int q = t; q < m; q += GJ_THREADS) {
+    rowb[q] = A[(size_t)c * m + q];
+    colb[q] = A[(size_t)q * m + c];
+  }
+  if (t == 0) perm[c] = p;
+}
+__global__ void k_gj_update(double* __restrict__ A, int m, int c, const double* __restrict__ rowb,
+                            const double* __restrict__ colb) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)m * m) return;
+  const int r = (int)(e / m), q = (int)(e % m);
+  const double pinv = 1.0 / rowb[c];
+  if (r == c) A[e] = q == c ? pinv : rowb[q] * pinv;
+  else if (q == c) A[e] = -colb[r] * pinv;
+  else A[e] = A[e] - colb[r] * (rowb[q] * pinv);
 }
 
 }  // namespace dev

@@ -506,10 +506,13 @@ class Model:
         first V-cycle at level g - n + 1."""
         _check(lib.swm_set_overlap(self._h, int(enable)))
 
-    def set_mixed_coarse(self, direct=True):
-        """Mixed path: the three coarsest levels as one dense solve (True)
-        or the reference's V-cycle over them (False)."""
-        _check(lib.swm_set_mixed_coarse(self._h, 1 if direct else 0))
+    def set_mixed_coarse(self, direct=True, level=3):
+        """Mixed path: the coarsest levels as one dense solve of level 3 or
+        of level 4 (where level 5 exists) (direct=True, level 3 or 4), or the
+        reference's V-cycle below level 4 (direct=False)."""
+        if level not in (3, 4):
+            raise ValueError("level is 3 or 4")
+        _check(lib.swm_set_mixed_coarse(self._h, (2 if level == 4 else 1) if direct else 0))
 
     def set_fusion(self, flags=None):
         """Kernel fusions of the multigrid cycle (FUSE_* flags; None: all).

@@ -12,9 +12,9 @@ change how the mixed solve converges:
     FP64 and mixed, one and two tracers, with and without the tracer
     stream, for every member of a two-member context;
   - the mixed solve's coarse correction (swm_set_mixed_coarse): the dense
-    solve of level 3 and the reference's V-cycle over levels 3..1 both meet
-    the criterion, with errors against the FP64 form within 3x of each
-    other plus 1e-7, and the dense solve takes no more cycles;
+    solves of level 4 and of level 3 and the reference's V-cycle below level
+    4 all meet the criterion, with errors against the FP64 form within 3x
+    of each other plus 1e-7, and the dense solves take no more cycles;
   - the mixed solve's finest-level sweep count (swm_set_mixed_smoother):
     every count meets the convergence criterion (the V-cycle only
     preconditions the FP64 defect correction), and its error against the
@@ -40,7 +40,7 @@ def _model(grid, precision, **kw):
     if "fusion" in kw:
         m.set_fusion(kw["fusion"])
     if "coarse_direct" in kw:
-        m.set_mixed_coarse(kw["coarse_direct"])
+        m.set_mixed_coarse(kw["coarse_direct"], kw.get("coarse_level", 4))
     m.init_synthetic(seed_index=2, lmax=12)
     return m
 
@@ -102,14 +102,16 @@ def test_mixed_coarse_solve_meets_the_criterion():
     grid = S.Grid(6)
     fp = _model(grid, S.SWM_FP64)
     vc = _model(grid, S.SWM_MIXED, coarse_direct=False)
-    de = _model(grid, S.SWM_MIXED, coarse_direct=True)
+    de = _model(grid, S.SWM_MIXED, coarse_direct=True, coarse_level=3)
+    d4 = _model(grid, S.SWM_MIXED, coarse_direct=True, coarse_level=4)
     for _ in range(8):
-        for m in (fp, vc, de):
+        for m in (fp, vc, de, d4):
             m.step(1)
-        for m in (vc, de):
+        for m in (vc, de, d4):
             r, _ = m.last_residual()
             assert max(r) <= 1e-20, r
     assert de.solver_stats()[1] <= vc.solver_stats()[1], (de.solver_stats(), vc.solver_stats())
+    assert d4.solver_stats()[1] <= vc.solver_stats()[1], (d4.solver_stats(), vc.solver_stats())
     mask = grid.array("grid_center_mask") == 1.0
     area = grid.array("area")[mask]
 
@@ -118,8 +120,9 @@ def test_mixed_coarse_solve_meets_the_criterion():
         return np.sqrt(np.sum(area * (y - x) ** 2) / np.sum(area * x * x))
 
     for n in ("h", "eta", "div"):
-        ev, ed = err(vc, n), err(de, n)
-        assert ed <= 3.0 * ev + 1e-7 and ev <= 3.0 * ed + 1e-7, (n, ev, ed)
+        ev = err(vc, n)
+        for ed in (err(de, n), err(d4, n)):
+            assert ed <= 3.0 * ev + 1e-7 and ev <= 3.0 * ed + 1e-7, (n, ev, ed)
     with pytest.raises(S.SwmError):
         fp.set_mixed_coarse(True)
 
