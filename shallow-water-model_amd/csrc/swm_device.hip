@@ -1108,7 +1108,7 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
     // then step_surface_height and step_tracer (step.F:221-327)
     const bool cen = C->advection == SWM_ADV_CENTERED;
     hipLaunchKernelGGL(swm::dev::k_mass_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.u[n3], V.h[n3], V.hc,
-                       cen ? V.flx1 : V.mass_flux, cen ? V.mass_flux : V.flx1);
+                       cen ? V.flx1 : V.mass_flux, cen ? V.mass_flux : V.flx1, (const R*)nullptr, (R*)nullptr);
     for (int q = 0; q < C->nt; ++q) {
       if (cen)
         hipLaunchKernelGGL((swm::dev::k_tr_flux<R, true>), zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.mass_flux, V.tr[n3][q],
@@ -1128,7 +1128,10 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
                          V.tr[n3][q], V.h[n3], V.h[n4], V.tr_f[a][q], V.tr_f[b][q], V.tr_f[c][q], V.tr[n4][q]);
   } else {
   // mass FCT (mass_fct.F:23-539)
-  hipLaunchKernelGGL(swm::dev::k_mass_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.u[n3], V.h[n3], V.hc, V.flx1, m3);
+  // with the divergence-curl flux of the old level (k_vort_curl's second
+  // half, fused: the same u, corner thickness, h and eta)
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.u[n3], V.h[n3], V.hc, V.flx1, m3,
+                     (const R*)V.eta[n3], V.dcurl);
   if (!push) gather<R>(C, C->edge, {V.flx1, m3});
   hipLaunchKernelGGL(swm::dev::k_mass_low_beta<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, dt, w3, w2, w1, V.flx1, m3, m2, m1,
                      V.h[n3], V.anti, V.mss_low, V.beta);
@@ -1142,8 +1145,11 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   // with the first tracer's edge fluxes (k_tr_fluxes fused)
   swm::dev::TrFluxArgs<R> tf0{};
   if (C->nt > 0) tf0 = {V.tr[n3][0], V.tr[n4][0], V.told[0], V.f1, V.f2, w3, w2, w1};
+  // with the vorticity edge flux of the new mass flux (k_vort_curl's first
+  // half, fused)
+  const swm::dev::VortFluxArgs<R> vfa{V.h[n3], V.eta[n3], V.vf};
   hipLaunchKernelGGL(swm::dev::k_mass_final<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, dt, V.flx1, V.anti, V.mss_low,
-                     V.mass_flux, V.h[n4], tf0);
+                     V.mass_flux, V.h[n4], tf0, vfa);
   // the tracer limiter fills the halo of the new mass in place (tracer_fct.F:344)
   if (!push) gather<R>(C, C->face, {V.h[n4]});
   }
@@ -1154,8 +1160,7 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   // the first V-cycle when it reaches a level of latency-bound launches
   // (mode >= 2: level g - mode + 1, through C->hook).
   swm_status tr_st = SWM_OK;
-  // late: started by the V-cycle (after this step's k_vort_curl is queued)
-  auto tracers = [&](bool late) {
+  auto tracers = [&]() {
     const hipStream_t main_stream = C->stream;
     if (overlap) {
       hipError_t he = hipEventRecord(C->ev_fork, main_stream);
@@ -1199,9 +1204,9 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
       gather_list<R>(C, C->face, tf);
     }
     // on the second stream the next step's corner thickness joins the
-    // tracers when they start late (this step's last reader of it,
-    // k_vort_curl, is then before the fork; the next step waits for the join)
-    if (overlap && late) corners_for_next(C);
+    // tracers (this step's last reader of it, k_mass_fluxes, is before the
+    // fork; the next step waits for the join)
+    if (overlap) corners_for_next(C);
     if (overlap) {
       const hipError_t he = hipEventRecord(C->ev_join, C->stream2);
       if (he != hipSuccess) {
@@ -1213,14 +1218,17 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   };
   const int hook_level = C->g - C->overlap + 1;
   if (overlap && C->overlap >= 2 && hook_level > swm::dev::CO_MAXLEV) {
-    C->hook = [&]() { tracers(true); };
+    C->hook = tracers;
     C->hook_level = hook_level;
   } else {
-    tracers(false);
+    tracers();
   }
   // vorticity and divergence (step.F:143-163, :336-429)
-  hipLaunchKernelGGL(swm::dev::k_vort_curl<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.mass_flux, V.h[n3], V.eta[n3], V.hc,
-                     V.u[n3], V.vf, V.dcurl);
+  // (FCT: both fluxes formed with the mass fluxes above)
+  if (!fct) {
+    hipLaunchKernelGGL(swm::dev::k_vort_curl<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.mass_flux, V.h[n3],
+                       V.eta[n3], V.hc, V.u[n3], V.vf, V.dcurl);
+  }
   if (!push) gather<R>(C, C->edge, {V.vf, V.dcurl});
   // mixed path: the kernel also forms the block sums of the right-hand
   // sides' area-weighted means, which the solve takes off on the fly
@@ -3039,7 +3047,8 @@ swm_status swm_op_centered_mass_flux(swm_ctx* C, int npad, int nx, int ny, int n
   double *mc = io.put(mass_c, 2, 2), *uu = io.put(u, 4, 4, kCornerVec), *m = io.put(mass, 1, 1), *up = io.dev(3),
          *y = io.dev(3);
   OP_BUF(G.dc && G.nrm && mc && uu && m && up && y);
-  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, G, uu, m, mc, up, y);
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, G, uu, m, mc, up, y, (const double*)nullptr,
+                     (double*)nullptr);
   return io.get(flux, y, 3);
 }
 
@@ -3054,7 +3063,8 @@ swm_status swm_op_upstream_mass_flux(swm_ctx* C, int npad, int nx, int ny, int n
   double *uu = io.put(u, 4, 4, kCornerVec), *m = io.put(mass, 1, 1), *mc = io.zeros(2), *y = io.dev(3),
          *cen = io.dev(3);
   OP_BUF(G.dc && G.nrm && uu && m && mc && y && cen);
-  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, G, uu, m, mc, y, cen);
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, G, uu, m, mc, y, cen, (const double*)nullptr,
+                     (double*)nullptr);
   return io.get(flux, y, 3);
 }
 
@@ -3158,7 +3168,8 @@ swm_status swm_op_set_mass_fct(swm_ctx* C, int iim, int jjm, int nlm, int nsdm, 
   hipStream_t S = C->stream;
   // mass_fct.F:98-133: upstream and centred fluxes, low order, beta down,
   // limited antidiffusive flux, flux-corrected mass and total flux
-  hipLaunchKernelGGL(swm::dev::k_mass_fluxes<double>, dim3(nb), dim3(BLK), 0, S, G, uu, h3, hc, flx1, ma);
+  hipLaunchKernelGGL(swm::dev::k_mass_fluxes<double>, dim3(nb), dim3(BLK), 0, S, G, uu, h3, hc, flx1, ma,
+                     (const double*)nullptr, (double*)nullptr);
   gather<double>(C, C->edge, {flx1, ma});
   hipLaunchKernelGGL(swm::dev::k_mass_low_beta<double>, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, flx1, ma, mb, mc,
                      h3, anti, low, beta);
@@ -3166,7 +3177,7 @@ swm_status swm_op_set_mass_fct(swm_ctx* C, int iim, int jjm, int nlm, int nsdm, 
   hipLaunchKernelGGL(swm::dev::k_mass_anti<double>, dim3(nb), dim3(BLK), 0, S, G, beta, anti);
   gather<double>(C, C->edge, {anti});
   hipLaunchKernelGGL(swm::dev::k_mass_final<double>, dim3(nb), dim3(BLK), 0, S, G, dt, flx1, anti, low, mflux, h4,
-                     swm::dev::TrFluxArgs<double>{});
+                     swm::dev::TrFluxArgs<double>{}, swm::dev::VortFluxArgs<double>{});
   swm_status st;
   std::vector<double> v((size_t)C->ns);
   if ((st = io.get(v.data(), h4, 1))) return st;
@@ -3709,7 +3720,7 @@ const double* base_coeffs(swm_ctx* C, int q) {
   // k_mass_fluxes: the upstream flux first, the centred one second
   const bool cen = C->advection == SWM_ADV_CENTERED;
   hipLaunchKernelGGL(swm::dev::k_mass_fluxes<double>, dim3(nb), dim3(BLK), 0, C->stream, G, seg_u(C, q), seg_h(C, q),
-                     T->bhc, cen ? T->vf : T->bF, cen ? T->bF : T->vf);
+                     T->bhc, cen ? T->vf : T->bF, cen ? T->bF : T->vf, (const double*)nullptr, (double*)nullptr);
   gather<double>(C, C->edge, {T->bF});
   return T->bF;
 }
@@ -4017,7 +4028,7 @@ swm_status tl_step(swm_ctx* C, int q) {
     hipLaunchKernelGGL(swm::dev::k_fct_scale, dim3(nblk(3 * ns)), dim3(BLK), 0, S, 3 * ns, seg_c(C, q), T->an);
     gather<double>(C, C->edge, {T->an});
     hipLaunchKernelGGL(swm::dev::k_mass_final<double>, dim3(nb), dim3(BLK), 0, S, G, dt, T->fl, T->an, T->ml, T->F,
-                       T->h[n4], swm::dev::TrFluxArgs<double>{});
+                       T->h[n4], swm::dev::TrFluxArgs<double>{}, swm::dev::VortFluxArgs<double>{});
     gather<double>(C, C->face, {T->h[n4]});
     if (T->tracers) tl_tracers(C, q, bF);
   } else {

@@ -300,6 +300,30 @@ __device__ __forceinline__ Half<R> half_edges_pre(int iim, int jjm, int ns, int 
   return h;
 }
 
+// the tangential half-edge fluxes (u . t) dc * mc of half_edges_pre from
+// the normal planes: the wall's unit tangent is its unit normal turned by a
+// right angle in the corner's east/north frame, t = (-n_N, n_E) (the metric
+// planes agree with it to one FP64 rounding on every slot), so
+// u . t dc = -u_E (n_N dc) + u_N (n_E dc)
+template <class R>
+__device__ __forceinline__ Half<R> half_edges_tan_pre(int iim, int jjm, int ns, int i, int j, int k,
+                                                      const R* __restrict__ nd, const R* __restrict__ mc,
+                                                      const R* __restrict__ u) {
+#define U(sp, ic, s_) u[((ic) + 2 * (sp)) * ns + (s_)]
+#define N(kk, a, ic, s_) nd[((kk) + 3 * ((a) + 2 * (ic))) * ns + (s_)]
+#define HT(s_, kk, ic) (U(0, ic, s_) * (-N(kk, 1, ic, s_)) + U(1, ic, s_) * N(kk, 0, ic, s_))
+  Half<R> h;
+  const int s0 = SL(i, j, k), sn = SL(i, j + 1, k), se = SL(i + 1, j, k);
+  h.r[0][0] = HT(sn, 1, 1); h.r[1][0] = HT(s0, 0, 0); h.r[0][1] = HT(s0, 2, 0);
+  h.r[1][1] = HT(s0, 0, 1); h.r[0][2] = HT(s0, 2, 1); h.r[1][2] = HT(se, 1, 0);
+  h.r[0][0] = h.r[0][0] * mc[ns + sn]; h.r[1][0] = h.r[1][0] * mc[s0]; h.r[0][1] = h.r[0][1] * mc[s0];
+  h.r[1][1] = h.r[1][1] * mc[ns + s0]; h.r[0][2] = h.r[0][2] * mc[ns + s0]; h.r[1][2] = h.r[1][2] * mc[se];
+#undef U
+#undef N
+#undef HT
+  return h;
+}
+
 // premultiplied metric planes of the mixed path: out(p) = nrm(p) * dc(kk, ic)
 // for the plane p = kk + 3 (a + 2 ic), formed in FP64
 __global__ void k_premul_dc(int ns, const double* __restrict__ nrm, const double* __restrict__ dc,
@@ -318,8 +342,11 @@ __global__ void k_premul_dc(int ns, const double* __restrict__ nrm, const double
 // (mass_fct.F:151-214 via operators.F:1626-1832)
 template <class R>
 __global__ void k_mass_fluxes(GeoT<R> G, const R* __restrict__ u, const R* __restrict__ h3, const R* __restrict__ hc,
-                              R* __restrict__ flx1, R* __restrict__ mshf) {
+                              R* __restrict__ flx1, R* __restrict__ mshf, const R* __restrict__ eta3,
+                              R* __restrict__ dcurl) {
   at_member(G.mb, u, h3, hc, flx1, mshf);
+  const bool dcr = dcurl != nullptr;
+  if (dcr) at_member(G.mb, eta3, dcurl);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
@@ -328,6 +355,7 @@ __global__ void k_mass_fluxes(GeoT<R> G, const R* __restrict__ u, const R* __res
   if (!interior(c, iim, jjm)) {
     put_all<R, 3>(G.pe, w, flx1, ns, s, R(0), R(0), R(0));
     put_all<R, 3>(G.pe, w, mshf, ns, s, R(0), R(0), R(0));
+    if (dcr) put_all<R, 3>(G.pe, w, dcurl, ns, s, R(0), R(0), R(0));
     return;
   }
   const int i = c.i, j = c.j, k = c.k;
@@ -351,6 +379,23 @@ __global__ void k_mass_fluxes(GeoT<R> G, const R* __restrict__ u, const R* __res
   }
   put_all<R, 3>(G.pe, w, mshf, ns, s, vm[0], vm[1], vm[2]);
   put_all<R, 3>(G.pe, w, flx1, ns, s, vf[0], vf[1], vf[2]);
+  if (dcr) {
+    // dcurl: centered_divergence_curl (operators.F:2083-2186, k_vort_curl's
+    // arithmetic) from the same old-level u, corner thickness, h and eta
+    Half<R> ht;
+    if constexpr (sizeof(R) == 4) ht = half_edges_tan_pre<R>(iim, jjm, ns, i, j, k, G.ndc, hc, u);
+    else ht = half_edges<R>(iim, jjm, ns, i, j, k, G.dc, G.tng, hc, u);
+    auto PV = [&](int sl) {
+      const R hh = habs(G, h3[sl]);
+      const R wi = hh > R(0) ? R(1) / hh : R(0);
+      return eabs(G, eta3, sl) * wi;
+    };
+    const R pv0 = PV(s);
+    R vd[3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) vd[e] = (-ht.r[0][e] + ht.r[1][e]) * (PV(up[e]) + pv0) * R(0.5);
+    put_all<R, 3>(G.pe, w, dcurl, ns, s, vd[0], vd[1], vd[2]);
+  }
 }
 
 // update of a cell field by the divergence of edge fluxes, pole forms
@@ -491,11 +536,20 @@ struct TrFluxArgs {
   R w3, w2, w1;
 };
 
-// flux-corrected mass and the total mass flux (mass_fct.F:121-128, :478-532)
+// the vorticity edge flux, fused into k_mass_final (vf == nullptr: none)
+template <class R>
+struct VortFluxArgs {
+  const R *h3, *eta3;
+  R* vf;
+};
+
+// flux-corrected mass and the total mass flux (mass_fct.F:121-128, :478-532);
+// with Vf also the vorticity edge flux of the new mass flux,
+// centered_vorticity_flux (operators.F:2000-2074, k_vort_curl's arithmetic)
 template <class R>
 __global__ void k_mass_final(GeoT<R> G, R dt, const R* __restrict__ flx1, const R* __restrict__ anti,
                              const R* __restrict__ mss_low, R* __restrict__ mflux, R* __restrict__ h4,
-                             TrFluxArgs<R> T) {
+                             TrFluxArgs<R> T, VortFluxArgs<R> Vf) {
   at_member(G.mb, flx1, anti, mss_low, mflux, h4);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
@@ -510,7 +564,8 @@ __global__ void k_mass_final(GeoT<R> G, R dt, const R* __restrict__ flx1, const 
   // three upstream neighbours (tr_fluxes_at)
   const int en[3] = {cl(SL(i + 1, j, k)), cl(SL(i + 1, j + 1, k)), cl(SL(i, j + 1, k))};
   const PushW wf = push_load(G.pf, c, iim, jjm, s);
-  const PushW we = T.t3 != nullptr ? push_load(G.pe, c, iim, jjm, s) : PushW{};
+  const bool vfl = Vf.vf != nullptr;
+  const PushW we = (T.t3 != nullptr || vfl) ? push_load(G.pe, c, iim, jjm, s) : PushW{};
   R fl[3], an[6];
 #pragma unroll
   for (int e = 0; e < 3; ++e) {
@@ -537,6 +592,21 @@ __global__ void k_mass_final(GeoT<R> G, R dt, const R* __restrict__ flx1, const 
       a3[q] = ldg(t3, sl);
       a4[q] = ldg(t4, sl);
       ao[q] = ldg(told, sl);
+    }
+  }
+  // the potential vorticity eta / h of the old level at the slot and its
+  // three upstream neighbours (k_vort_curl's PV)
+  R pv[4];
+  if (vfl) {
+    const R* h3 = Vf.h3;
+    const R* eta3 = Vf.eta3;
+    at_member(G.mb, h3, eta3);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int sl = q == 0 ? s : up[q - 1];
+      const R hh = habs(G, ldg(h3, sl));
+      const R wi = hh > R(0) ? R(1) / hh : R(0);
+      pv[q] = eabs(G, eta3, sl) * wi;
     }
   }
   R F[3];
@@ -574,6 +644,16 @@ __global__ void k_mass_final(GeoT<R> G, R dt, const R* __restrict__ flx1, const 
     }
     put_all<R, 3>(G.pe, we, f1, ns, s, v1[0], v1[1], v1[2]);
     put_all<R, 3>(G.pe, we, f2, ns, s, v2[0], v2[1], v2[2]);
+  }
+  if (vfl) {
+    R* vf = Vf.vf;
+    at_member(G.mb, vf);
+    R v[3] = {R(0), R(0), R(0)};
+    if (in) {
+#pragma unroll
+      for (int e = 0; e < 3; ++e) v[e] = R(0.5) * (F[e]) * (pv[0] + pv[1 + e]);
+    }
+    put_all<R, 3>(G.pe, we, vf, ns, s, v[0], v[1], v[2]);
   }
 }
 
