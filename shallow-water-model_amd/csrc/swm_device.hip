@@ -199,6 +199,9 @@ struct swm_ctx {
   // solve's convergence test (swm_step sets next_follows when another step
   // of the same call follows; corners_ready: queued, the next step skips it)
   bool next_follows = false, corners_ready = false;
+  // the last step left out the kinetic energy of its new velocity; the next
+  // step's mass-flux kernel forms it (producer-side halo writes only)
+  bool ke_deferred = false;
   swm_status layout_status = SWM_OK;   // halo table set-up
   long long shadow_checks = 0, shadow_bad = 0;
   bool initialized = false;
@@ -1108,7 +1111,7 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
     // then step_surface_height and step_tracer (step.F:221-327)
     const bool cen = C->advection == SWM_ADV_CENTERED;
     hipLaunchKernelGGL(swm::dev::k_mass_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.u[n3], V.h[n3], V.hc,
-                       cen ? V.flx1 : V.mass_flux, cen ? V.mass_flux : V.flx1, (const R*)nullptr, (R*)nullptr);
+                       cen ? V.flx1 : V.mass_flux, cen ? V.mass_flux : V.flx1, (const R*)nullptr, (R*)nullptr, (R*)nullptr);
     for (int q = 0; q < C->nt; ++q) {
       if (cen)
         hipLaunchKernelGGL((swm::dev::k_tr_flux<R, true>), zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.mass_flux, V.tr[n3][q],
@@ -1129,9 +1132,11 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   } else {
   // mass FCT (mass_fct.F:23-539)
   // with the divergence-curl flux of the old level (k_vort_curl's second
-  // half, fused: the same u, corner thickness, h and eta)
+  // half, fused: the same u, corner thickness, h and eta), and the old
+  // level's kinetic energy where the previous step of this call left it out
   hipLaunchKernelGGL(swm::dev::k_mass_fluxes<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.u[n3], V.h[n3], V.hc, V.flx1, m3,
-                     (const R*)V.eta[n3], V.dcurl);
+                     (const R*)V.eta[n3], V.dcurl, C->ke_deferred ? V.ke : (R*)nullptr);
+  C->ke_deferred = false;
   if (!push) gather<R>(C, C->edge, {V.flx1, m3});
   hipLaunchKernelGGL(swm::dev::k_mass_low_beta<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, dt, w3, w2, w1, V.flx1, m3, m2, m1,
                      V.h[n3], V.anti, V.mss_low, V.beta);
@@ -1262,8 +1267,10 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   if (!push) gather<R>(C, C->vrtx, {V.u[n4], V.u[n4] + 2 * (size_t)ns});
   // the kinetic energy's halo is filled with the other fields' by
   // wrap_prognostics, which follows the step directly (with producer-side
-  // halo writes: by this kernel)
-  hipLaunchKernelGGL(swm::dev::k_center_ke<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.u[n4], V.ke);
+  // halo writes: by this kernel).  With producer-side halo writes and a
+  // next step in this call, that step's mass-flux kernel forms it instead.
+  if (push && C->next_follows) C->ke_deferred = true;
+  else hipLaunchKernelGGL(swm::dev::k_center_ke<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.u[n4], V.ke);
   if (overlap) HIP_TRY(hipStreamWaitEvent(S, C->ev_join, 0));   // the tracers before the halo fill
   return SWM_OK;
 }
@@ -2394,9 +2401,10 @@ swm_status swm_step(swm_ctx* C, int nsteps) {
   // the corners queued ahead belong to this call's next step only
   struct Ahead {
     swm_ctx* C;
-    ~Ahead() { C->next_follows = false; C->corners_ready = false; }
+    ~Ahead() { C->next_follows = false; C->corners_ready = false; C->ke_deferred = false; }
   } ahead{C};
   C->corners_ready = false;
+  C->ke_deferred = false;
   for (int n = 0; n < nsteps; ++n) {
     C->next_follows = n + 1 < nsteps;
     swm_status st = swm::step_once(C);
@@ -3048,7 +3056,7 @@ swm_status swm_op_centered_mass_flux(swm_ctx* C, int npad, int nx, int ny, int n
          *y = io.dev(3);
   OP_BUF(G.dc && G.nrm && mc && uu && m && up && y);
   hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, G, uu, m, mc, up, y, (const double*)nullptr,
-                     (double*)nullptr);
+                     (double*)nullptr, (double*)nullptr);
   return io.get(flux, y, 3);
 }
 
@@ -3064,7 +3072,7 @@ swm_status swm_op_upstream_mass_flux(swm_ctx* C, int npad, int nx, int ny, int n
          *cen = io.dev(3);
   OP_BUF(G.dc && G.nrm && uu && m && mc && y && cen);
   hipLaunchKernelGGL(swm::dev::k_mass_fluxes, dim3(nb_), dim3(BLK), 0, C->stream, G, uu, m, mc, y, cen, (const double*)nullptr,
-                     (double*)nullptr);
+                     (double*)nullptr, (double*)nullptr);
   return io.get(flux, y, 3);
 }
 
@@ -3169,7 +3177,7 @@ swm_status swm_op_set_mass_fct(swm_ctx* C, int iim, int jjm, int nlm, int nsdm, 
   // mass_fct.F:98-133: upstream and centred fluxes, low order, beta down,
   // limited antidiffusive flux, flux-corrected mass and total flux
   hipLaunchKernelGGL(swm::dev::k_mass_fluxes<double>, dim3(nb), dim3(BLK), 0, S, G, uu, h3, hc, flx1, ma,
-                     (const double*)nullptr, (double*)nullptr);
+                     (const double*)nullptr, (double*)nullptr, (double*)nullptr);
   gather<double>(C, C->edge, {flx1, ma});
   hipLaunchKernelGGL(swm::dev::k_mass_low_beta<double>, dim3(nb), dim3(BLK), 0, S, G, dt, w3, w2, w1, flx1, ma, mb, mc,
                      h3, anti, low, beta);
@@ -3720,7 +3728,8 @@ const double* base_coeffs(swm_ctx* C, int q) {
   // k_mass_fluxes: the upstream flux first, the centred one second
   const bool cen = C->advection == SWM_ADV_CENTERED;
   hipLaunchKernelGGL(swm::dev::k_mass_fluxes<double>, dim3(nb), dim3(BLK), 0, C->stream, G, seg_u(C, q), seg_h(C, q),
-                     T->bhc, cen ? T->vf : T->bF, cen ? T->bF : T->vf, (const double*)nullptr, (double*)nullptr);
+                     T->bhc, cen ? T->vf : T->bF, cen ? T->bF : T->vf, (const double*)nullptr, (double*)nullptr,
+                     (double*)nullptr);
   gather<double>(C, C->edge, {T->bF});
   return T->bF;
 }

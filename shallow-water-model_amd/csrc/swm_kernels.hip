@@ -343,7 +343,7 @@ __global__ void k_premul_dc(int ns, const double* __restrict__ nrm, const double
 template <class R>
 __global__ void k_mass_fluxes(GeoT<R> G, const R* __restrict__ u, const R* __restrict__ h3, const R* __restrict__ hc,
                               R* __restrict__ flx1, R* __restrict__ mshf, const R* __restrict__ eta3,
-                              R* __restrict__ dcurl) {
+                              R* __restrict__ dcurl, R* __restrict__ ke) {
   at_member(G.mb, u, h3, hc, flx1, mshf);
   const bool dcr = dcurl != nullptr;
   if (dcr) at_member(G.mb, eta3, dcurl);
@@ -352,6 +352,27 @@ __global__ void k_mass_fluxes(GeoT<R> G, const R* __restrict__ u, const R* __res
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
   const PushW w = push_load(G.pe, c, iim, jjm, s);
+  if (ke) {
+    // the kinetic energy of this (old) level's velocity, k_center_ke's
+    // arithmetic (step.F:192-203, operators.F:1316-1393), from the velocity
+    // this kernel reads anyway; the previous step left it out
+    at_member(G.mb, ke);
+    const PushW wk = push_load(G.pf, c, iim, jjm, s);
+    const int i = c.i, j = c.j, k = c.k;
+    R kv = R(0);
+    if (i <= iim - 1 && j <= jjm - 1) {
+      auto KC = [&](int ic, int sl) {
+        const R a = u[ic * ns + sl], b = u[(ic + 2) * ns + sl];
+        return R(0.5) * (a * a + b * b);
+      };
+#define SA(kk, ic, sl) G.sub_area[((kk) + 3 * (ic)) * ns + (sl)]
+      const int se = SL(i + 1, j, k), sne = SL(i + 1, j + 1, k), sn = SL(i, j + 1, k);
+      kv = (+SA(0, 0, s) * KC(0, s) + SA(0, 1, s) * KC(1, s) + SA(1, 0, se) * KC(0, se) +
+            SA(1, 1, sne) * KC(1, sne) + SA(2, 0, sne) * KC(0, sne) + SA(2, 1, sn) * KC(1, sn)) * G.area_inv[s];
+#undef SA
+    }
+    put1(G.pf, wk, ke, s, kv);
+  }
   if (!interior(c, iim, jjm)) {
     put_all<R, 3>(G.pe, w, flx1, ns, s, R(0), R(0), R(0));
     put_all<R, 3>(G.pe, w, mshf, ns, s, R(0), R(0), R(0));
