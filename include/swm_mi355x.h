@@ -346,10 +346,17 @@ swm_status swm_set_mixed_coarse(swm_ctx* ctx, int direct);
  * that the step's halo fill launches are left out; the tracers' halo fill
  * runs at the end of their transport.  The same values land in the same
  * slots.  No counterpart in the reference (wrap_data.F:1790-2607 fills each
- * halo after the routine that formed the field). */
+ * halo after the routine that formed the field).
+ * SWM_FUSE_MASS_ANTI (FCT step): the final mass kernel corrects the
+ * antidiffusive fluxes it reads (set_corrected_mss_anti, mass_fct.F:410-470)
+ * instead of a pass of its own between the limiter and the final mass,
+ * where the set-up finds the fill tables copy each halo edge from the
+ * slot whose face value the halo holds (it does on the icosahedral grids).
+ * The same values land in the same slots. */
 #define SWM_FUSE_INJECT_SWEEP 1
 #define SWM_FUSE_HALO_PUSH 2
-#define SWM_FUSE_ALL 3
+#define SWM_FUSE_MASS_ANTI 4
+#define SWM_FUSE_ALL 7
 swm_status swm_set_fusion(swm_ctx* ctx, int flags);
 
 /* Mean squared residuals at the end of the last elliptic solve, and mean

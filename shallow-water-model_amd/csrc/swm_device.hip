@@ -195,6 +195,8 @@ struct swm_ctx {
   int lin_exact = 1;           // TL/AD: the derivative of the forward's own cycles (1) or of converged solves (0)
   int halo_shadow = 0;         // every halo fill checked against the host's (swm_set_halo_shadow)
   int fuse_push = 1;           // FCT step: producer kernels store the halo copies (SWM_FUSE_HALO_PUSH)
+  int fuse_anti = 1;           // FCT step: the final mass kernel corrects the antidiffusive fluxes (SWM_FUSE_MASS_ANTI)
+  bool anti_ok = false;        // the fill tables allow it (anti_fusion_check)
   // the next step's corner thickness, queued while the host waits on the
   // solve's convergence test (swm_step sets next_follows when another step
   // of the same call follows; corners_ready: queued, the next step skips it)
@@ -1145,8 +1147,13 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
     if (C->fct_rec)
       hipLaunchKernelGGL(swm::dev::k_fct_coef, zgrid(C, nb), dim3(C->blk_step), 0, S, G, V.beta, V.anti, C->fct_rec);
   }
-  hipLaunchKernelGGL(swm::dev::k_mass_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.beta, V.anti);
-  if (!push) gather<R>(C, C->edge, {V.anti});
+  // the correction of the antidiffusive fluxes, or (SWM_FUSE_MASS_ANTI)
+  // the final mass kernel corrects the values it reads itself
+  const bool anti_fused = C->fuse_anti && C->anti_ok;
+  if (!anti_fused) {
+    hipLaunchKernelGGL(swm::dev::k_mass_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.beta, V.anti);
+    if (!push) gather<R>(C, C->edge, {V.anti});
+  }
   // with the first tracer's edge fluxes (k_tr_fluxes fused)
   swm::dev::TrFluxArgs<R> tf0{};
   if (C->nt > 0) tf0 = {V.tr[n3][0], V.tr[n4][0], V.told[0], V.f1, V.f2, w3, w2, w1};
@@ -1154,7 +1161,7 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   // half, fused)
   const swm::dev::VortFluxArgs<R> vfa{V.h[n3], V.eta[n3], V.vf};
   hipLaunchKernelGGL(swm::dev::k_mass_final<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, dt, V.flx1, V.anti, V.mss_low,
-                     V.mass_flux, V.h[n4], tf0, vfa);
+                     V.mass_flux, V.h[n4], tf0, vfa, anti_fused ? (const R*)V.beta : (const R*)nullptr);
   // the tracer limiter fills the halo of the new mass in place (tracer_fct.F:344)
   if (!push) gather<R>(C, C->face, {V.h[n4]});
   }
@@ -1897,6 +1904,57 @@ swm_status remap_setup(swm_ctx* C, const swm::Grid& Gh) {
   return SWM_OK;
 }
 
+// Whether the final mass kernel can take over set_corrected_mss_anti
+// (SWM_FUSE_MASS_ANTI), from the host copies of the fill tables: every
+// forward edge of a real cell that lies in the halo (edge 0 at (i+1, j),
+// 1 at (i+1, j+1), 2 at (i, j+1)) is a fill destination that copies an edge
+// owned by the slot the face fill copies there (or is set to zero), so that
+// its corrected value follows from the halo copies of beta; and each real
+// pole's two own edges copy edges whose upstream neighbour is the pole.
+// The pole edges' source slots go to psrc.
+bool anti_fusion_check(const swm_ctx* C, int psrc[4]) {
+  const int iim = C->iim, jjm = C->jjm, ns = C->ns;
+  auto SLh = [&](int i, int j, int k) { return (i - 1) + iim * ((j - 1) + jjm * (k - 1)); };
+  std::vector<int> fsrc(ns, -1);
+  for (size_t q = 0; q < C->face.hdst.size(); ++q) fsrc[C->face.hdst[q] % ns] = C->face.hsrc[q];
+  std::vector<int> esrc((size_t)3 * ns, -2);   // -2: not a destination
+  for (size_t q = 0; q < C->edge.hdst.size(); ++q) esrc[C->edge.hdst[q]] = C->edge.hsrc[q];
+  auto inside = [&](int i, int j) { return i >= 2 && i <= iim - 1 && j >= 2 && j <= jjm - 1; };
+  auto fwd_ok = [&](int i, int j, int k) {
+    const int nbi[3] = {i + 1, i + 1, i}, nbj[3] = {j, j + 1, j + 1};
+    for (int e = 0; e < 3; ++e) {
+      if (inside(nbi[e], nbj[e])) continue;
+      const int n = SLh(nbi[e], nbj[e], k);
+      const int src = esrc[(size_t)e * ns + n];
+      if (src == -2) return false;
+      if (src >= 0 && src % ns != fsrc[n]) return false;
+    }
+    return true;
+  };
+  for (int k = 1; k <= 10; ++k)
+    for (int j = 2; j <= jjm - 1; ++j)
+      for (int i = 2; i <= iim - 1; ++i)
+        if ((i == iim - 1 || j == jjm - 1) && !fwd_ok(i, j, k)) return false;
+  const int poles[2][3] = {{1, jjm - 1, 1}, {iim - 1, 1, 10}};
+  const int own[2][2] = {{1, 2}, {0, 1}};
+  for (int p = 0; p < 2; ++p) {
+    const int i = poles[p][0], j = poles[p][1], k = poles[p][2];
+    const int sp = SLh(i, j, k);
+    if (!fwd_ok(i, j, k)) return false;
+    for (int q = 0; q < 2; ++q) {
+      const int src = esrc[(size_t)own[p][q] * ns + sp];
+      if (src < 0) return false;
+      const int s2 = src % ns, c2 = src / ns;
+      const int i2 = s2 % iim + 1, j2 = (s2 / iim) % jjm + 1, k2 = s2 / (iim * jjm) + 1;
+      if (!inside(i2, j2)) return false;
+      const int up = c2 == 0 ? SLh(i2 - 1, j2, k2) : c2 == 1 ? SLh(i2 - 1, j2 - 1, k2) : SLh(i2, j2 - 1, k2);
+      if (up != sp && fsrc[up] != sp) return false;
+      psrc[2 * p + q] = s2;
+    }
+  }
+  return true;
+}
+
 swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
   HIP_TRY(hipSetDevice(C->device));
   HIP_TRY(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
@@ -1928,6 +1986,11 @@ swm_status create_device(swm_ctx* C, const swm::Grid& Gh) {
   }
   C->d.geo.mb = C->mb;
   C->m.geo.mb = C->mb;
+  {
+    int psrc[4] = {0, 0, 0, 0};
+    C->anti_ok = anti_fusion_check(C, psrc);
+    for (int q = 0; q < 4; ++q) { C->d.geo.psrc[q] = psrc[q]; C->m.geo.psrc[q] = psrc[q]; }
+  }
   for (LevelT<double>& L : C->d.lev) L.L.mb = C->mb;
   for (LevelT<float>& L : C->m.lev) L.L.mb = C->mb;
   C->nz = C->nmem;
@@ -2521,6 +2584,7 @@ swm_status swm_set_fusion(swm_ctx* C, int flags) {
   if (!C || (flags & ~SWM_FUSE_ALL)) { swm::set_error("swm_set_fusion: bad argument"); return SWM_ERR_ARG; }
   C->fuse_mg = (flags & SWM_FUSE_INJECT_SWEEP) ? 1 : 0;
   C->fuse_push = (flags & SWM_FUSE_HALO_PUSH) ? 1 : 0;
+  C->fuse_anti = (flags & SWM_FUSE_MASS_ANTI) ? 1 : 0;
   C->epoch += 1;
   return SWM_OK;
 }
@@ -3185,7 +3249,8 @@ swm_status swm_op_set_mass_fct(swm_ctx* C, int iim, int jjm, int nlm, int nsdm, 
   hipLaunchKernelGGL(swm::dev::k_mass_anti<double>, dim3(nb), dim3(BLK), 0, S, G, beta, anti);
   gather<double>(C, C->edge, {anti});
   hipLaunchKernelGGL(swm::dev::k_mass_final<double>, dim3(nb), dim3(BLK), 0, S, G, dt, flx1, anti, low, mflux, h4,
-                     swm::dev::TrFluxArgs<double>{}, swm::dev::VortFluxArgs<double>{});
+                     swm::dev::TrFluxArgs<double>{}, swm::dev::VortFluxArgs<double>{},
+                     (const double*)nullptr);
   swm_status st;
   std::vector<double> v((size_t)C->ns);
   if ((st = io.get(v.data(), h4, 1))) return st;
@@ -4037,7 +4102,8 @@ swm_status tl_step(swm_ctx* C, int q) {
     hipLaunchKernelGGL(swm::dev::k_fct_scale, dim3(nblk(3 * ns)), dim3(BLK), 0, S, 3 * ns, seg_c(C, q), T->an);
     gather<double>(C, C->edge, {T->an});
     hipLaunchKernelGGL(swm::dev::k_mass_final<double>, dim3(nb), dim3(BLK), 0, S, G, dt, T->fl, T->an, T->ml, T->F,
-                       T->h[n4], swm::dev::TrFluxArgs<double>{}, swm::dev::VortFluxArgs<double>{});
+                       T->h[n4], swm::dev::TrFluxArgs<double>{}, swm::dev::VortFluxArgs<double>{},
+                     (const double*)nullptr);
     gather<double>(C, C->face, {T->h[n4]});
     if (T->tracers) tl_tracers(C, q, bF);
   } else {

@@ -52,6 +52,10 @@ struct GeoT {
   const double *sym, *aim;
   // producer-side halo writes of the step kernels (all null: none)
   Push pf, pv, pe;
+  // the slots whose edges the real poles' own edges copy (north pole edges
+  // 1 and 2, south pole edges 0 and 1): the final mass kernel corrects
+  // those edges itself when it takes over the antidiffusive correction
+  int psrc[4];
 };
 using Geo = GeoT<double>;
 

@@ -624,10 +624,20 @@ struct VortFluxArgs {
 // flux-corrected mass and the total mass flux (mass_fct.F:121-128, :478-532);
 // with Vf also the vorticity edge flux of the new mass flux,
 // centered_vorticity_flux (operators.F:2000-2074, k_vort_curl's arithmetic)
+// With beta (not nullptr) the kernel also performs set_corrected_mss_anti
+// (mass_fct.F:410-470, k_mass_anti's arithmetic) on the antidiffusive fluxes
+// it reads, which are then the uncorrected ones of k_mass_low_beta: a
+// slot's own edges with beta at the slot and its upstream neighbours, the
+// forward edge e at n_e = (i+1, j), (i+1, j+1), (i, j+1) with beta at n_e
+// and at the slot, whose upstream neighbour n_e's edge e reaches.  A forward
+// edge in the halo holds a copy of an edge that the slot n_e's image owns
+// (the set-up checks this on the fill tables), so the same form with the
+// halo copy of beta gives the corrected copy; the real poles' own edges copy
+// edges of the slots psrc, whose upstream neighbour is the pole.
 template <class R>
 __global__ void k_mass_final(GeoT<R> G, R dt, const R* __restrict__ flx1, const R* __restrict__ anti,
                              const R* __restrict__ mss_low, R* __restrict__ mflux, R* __restrict__ h4,
-                             TrFluxArgs<R> T, VortFluxArgs<R> Vf) {
+                             TrFluxArgs<R> T, VortFluxArgs<R> Vf, const R* __restrict__ beta) {
   at_member(G.mb, flx1, anti, mss_low, mflux, h4);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
@@ -653,6 +663,20 @@ __global__ void k_mass_final(GeoT<R> G, R dt, const R* __restrict__ flx1, const 
     an[3 + e] = ldg(anti + o, en[e]);
   }
   const R base = ldg(mss_low, s), ai = ldg(G.area_inv, s);
+  const bool corr = beta != nullptr;
+  const bool npole = i == 1 && j == jjm - 1 && k == 1, spole = i == iim - 1 && j == 1 && k == 10;
+  R bs = R(0), bup[3] = {R(0), R(0), R(0)}, bfw[3] = {R(0), R(0), R(0)}, bp[2] = {R(0), R(0)};
+  if (corr) {
+    at_member(G.mb, beta);
+    bs = rmin(R(1), ldg(beta, s));
+#pragma unroll
+    for (int e = 0; e < 3; ++e) bfw[e] = rmin(R(1), ldg(beta, en[e]));
+    const int upc[3] = {cl(SL(i - 1, j, k)), cl(SL(i - 1, j - 1, k)), cl(SL(i, j - 1, k))};
+#pragma unroll
+    for (int e = 0; e < 3; ++e) bup[e] = rmin(R(1), ldg(beta, upc[e]));
+    if (npole) { bp[0] = rmin(R(1), ldg(beta, G.psrc[0])); bp[1] = rmin(R(1), ldg(beta, G.psrc[1])); }
+    if (spole) { bp[0] = rmin(R(1), ldg(beta, G.psrc[2])); bp[1] = rmin(R(1), ldg(beta, G.psrc[3])); }
+  }
   const bool tr = T.t3 != nullptr;
   const R* t3 = T.t3;
   const R* t4 = T.t4;
@@ -686,6 +710,22 @@ __global__ void k_mass_final(GeoT<R> G, R dt, const R* __restrict__ flx1, const 
       const R wi = hh > R(0) ? R(1) / hh : R(0);
       pv[q] = eabs(G, eta3, sl) * wi;
     }
+  }
+  if (corr) {
+    // k_mass_anti's form: b_owner * max(0, a) + b_upstream * min(0, a)
+    auto cor = [](R bo, R bu, R a) { return bo * rmax(R(0), a) + bu * rmin(R(0), a); };
+    if (in) {
+#pragma unroll
+      for (int e = 0; e < 3; ++e) an[e] = cor(bs, bup[e], an[e]);
+    } else if (npole) {
+      an[1] = cor(bp[0], bs, an[1]);
+      an[2] = cor(bp[1], bs, an[2]);
+    } else if (spole) {
+      an[0] = cor(bp[0], bs, an[0]);
+      an[1] = cor(bp[1], bs, an[1]);
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e) an[3 + e] = cor(bfw[e], bs, an[3 + e]);
   }
   R F[3];
 #pragma unroll

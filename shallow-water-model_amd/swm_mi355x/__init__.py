@@ -28,7 +28,8 @@ SWM_MIXED = 1
 # swm_set_fusion flags
 FUSE_INJECT_SWEEP = 1
 FUSE_HALO_PUSH = 2
-FUSE_ALL = 3
+FUSE_MASS_ANTI = 4
+FUSE_ALL = 7
 # advection of step_dt: l_fct, l_centered, l_upwind, l_remap
 ADVECTION = {"fct": 0, "centered": 1, "upwind": 2, "remap": 3}
 
