@@ -1203,10 +1203,16 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
           hipLaunchKernelGGL(swm::dev::k_tr_coef, zgrid(C, nb), dim3(C->blk_step), 0, St, G, V.bdn, V.bup, V.anti,
                              C->tr_rec + (size_t)(6 * q + 3) * ns);
       }
-      hipLaunchKernelGGL(swm::dev::k_tr_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, V.bdn, V.bup, V.anti);
-      gather<R>(C, C->edge, {V.anti});
+      // the correction of the antidiffusive fluxes, or (SWM_FUSE_MASS_ANTI)
+      // the final tracer kernel corrects the values it reads itself
+      const bool tr_fused = C->fuse_anti && C->anti_ok;
+      if (!tr_fused) {
+        hipLaunchKernelGGL(swm::dev::k_tr_anti<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, V.bdn, V.bup, V.anti);
+        gather<R>(C, C->edge, {V.anti});
+      }
       hipLaunchKernelGGL(swm::dev::k_tr_final<R>, zgrid(C, nb), dim3(C->blk_step), 0, St, G, dt, V.ml, V.anti, V.h[n4],
-                         t4, V.told[q]);
+                         t4, V.told[q], tr_fused ? (const R*)V.bdn : (const R*)nullptr,
+                         tr_fused ? (const R*)V.bup : (const R*)nullptr);
     }
     // with producer-side halo writes the end-of-step fill (wrap_prognostics)
     // has nothing else left: the tracers' halos are filled here instead
@@ -3305,7 +3311,8 @@ swm_status swm_op_set_tracer_fct(swm_ctx* C, int iim, int jjm, int nlm, int nsdm
   gather<double>(C, C->face, {bdn, bup});
   hipLaunchKernelGGL(swm::dev::k_tr_anti<double>, dim3(nb), dim3(BLK), 0, S, G, bdn, bup, anti);
   gather<double>(C, C->edge, {anti});
-  hipLaunchKernelGGL(swm::dev::k_tr_final<double>, dim3(nb), dim3(BLK), 0, S, G, dt, ml, anti, h4, t4, told);
+  hipLaunchKernelGGL(swm::dev::k_tr_final<double>, dim3(nb), dim3(BLK), 0, S, G, dt, ml, anti, h4, t4, told,
+                     (const double*)nullptr, (const double*)nullptr);
   swm_status st;
   if ((st = io.get(trc_fct, t4, 1))) return st;
   if (msstrc_low_order && (st = io.get(msstrc_low_order, ml, 1))) return st;

@@ -887,20 +887,84 @@ __global__ void k_tr_anti(GeoT<R> G, const R* __restrict__ bdn, const R* __restr
 }
 
 // tracer_fct.F:601-665 set_msstrc_fct and the division by the new mass; the
-// previous tracer level is kept for the next AB3 extrapolation (step.F:661)
+// previous tracer level is kept for the next AB3 extrapolation (step.F:661).
+// With bdn (not nullptr) the kernel also performs set_corrected_trc_anti
+// (tracer_fct.F:539-593, k_tr_anti's arithmetic) on the antidiffusive
+// fluxes it reads (then k_tr_limiter's uncorrected ones), with the roles of
+// owner and upstream cell as k_mass_final takes them for the mass.
 template <class R>
 __global__ void k_tr_final(GeoT<R> G, R dt, const R* __restrict__ ml, const R* __restrict__ anti,
-                           const R* __restrict__ h4, R* __restrict__ t4, R* __restrict__ told) {
+                           const R* __restrict__ h4, R* __restrict__ t4, R* __restrict__ told,
+                           const R* __restrict__ bdn, const R* __restrict__ bup) {
   at_member(G.mb, ml, anti, h4, t4, told);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();
   if (s >= ns) return;
   const IJK c = ijk_of(s, iim, jjm);
-  bool set;
-  const R mf = edge_update<R>(iim, jjm, ns, c, ml[s], dt * G.area_inv[s], anti, set);
-  told[s] = t4[s];
-  const R hh = habs(G, h4[s]);
-  t4[s] = hh != R(0) ? mf / hh : R(0);
+  if (bdn == nullptr) {
+    bool set;
+    const R mf = edge_update<R>(iim, jjm, ns, c, ml[s], dt * G.area_inv[s], anti, set);
+    told[s] = t4[s];
+    const R hh = habs(G, h4[s]);
+    t4[s] = hh != R(0) ? mf / hh : R(0);
+    return;
+  }
+  at_member(G.mb, bdn, bup);
+  const int i = c.i, j = c.j, k = c.k;
+  auto cl = [&](int sl) { return min(max(sl, 0), ns - 1); };
+  const int en[3] = {cl(SL(i + 1, j, k)), cl(SL(i + 1, j + 1, k)), cl(SL(i, j + 1, k))};
+  const int upc[3] = {cl(SL(i - 1, j, k)), cl(SL(i - 1, j - 1, k)), cl(SL(i, j - 1, k))};
+  const bool in = interior(c, iim, jjm);
+  const bool np = i == 1 && j == jjm - 1, sp = i == iim - 1 && j == 1;
+  const bool npole = np && k == 1, spole = sp && k == 10;
+  // every value first: the slot's three edges and edge e at en[e], beta
+  // down / up at the slot, its upstream and forward neighbours (and the
+  // real poles' source slots)
+  R a[6];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    a[e] = ldg(anti + (size_t)e * ns, s);
+    a[3 + e] = ldg(anti + (size_t)e * ns, en[e]);
+  }
+  const R ds = ldg(bdn, s), us = ldg(bup, s);
+  R dup[3], uup[3], dfw[3], ufw[3];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    dup[e] = ldg(bdn, upc[e]); uup[e] = ldg(bup, upc[e]);
+    dfw[e] = ldg(bdn, en[e]); ufw[e] = ldg(bup, en[e]);
+  }
+  R dps[2] = {R(0), R(0)}, ups[2] = {R(0), R(0)};
+  if (npole || spole) {
+    const int o = npole ? 0 : 2;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) { dps[q] = ldg(bdn, G.psrc[o + q]); ups[q] = ldg(bup, G.psrc[o + q]); }
+  }
+  const R base = ldg(ml, s), dt_ai = dt * ldg(G.area_inv, s);
+  const R t4o = ldg(t4, s), hh = habs(G, ldg(h4, s));
+  // k_tr_anti's form: owner (down, up), upstream (down, up)
+  auto cor = [](R dn_o, R up_o, R dn_u, R up_u, R x) {
+    return rmin(rmin(R(1), dn_o), up_u) * rmax(R(0), x) + rmin(rmin(R(1), up_o), dn_u) * rmin(R(0), x);
+  };
+  if (in) {
+#pragma unroll
+    for (int e = 0; e < 3; ++e) a[e] = cor(ds, us, dup[e], uup[e], a[e]);
+  } else if (npole) {
+    a[1] = cor(dps[0], ups[0], ds, us, a[1]);
+    a[2] = cor(dps[1], ups[1], ds, us, a[2]);
+  } else if (spole) {
+    a[0] = cor(dps[0], ups[0], ds, us, a[0]);
+    a[1] = cor(dps[1], ups[1], ds, us, a[1]);
+  }
+#pragma unroll
+  for (int e = 0; e < 3; ++e) a[3 + e] = cor(dfw[e], ufw[e], ds, us, a[3 + e]);
+  // edge_update's forms (tracer_fct.F:601-665)
+  R mf;
+  if (np) mf = base + dt_ai * (a[1] + a[2] + a[3] + a[4] + a[5]);
+  else if (sp) mf = base + dt_ai * (a[0] + a[1] + a[3] + a[4] + a[5]);
+  else if (in) mf = base - dt_ai * ((a[0] + a[1] + a[2]) - (a[3] + a[4] + a[5]));
+  else mf = R(0);
+  stg(told, s, t4o);
+  stg(t4, s, hh != R(0) ? mf / hh : R(0));
 }
 
 // centred (operators.F:1923-1991) or upstream (:1841-1914) tracer edge flux
