@@ -344,136 +344,214 @@ template <class R>
 __global__ void k_mass_fluxes(GeoT<R> G, const R* __restrict__ u, const R* __restrict__ h3, const R* __restrict__ hc,
                               R* __restrict__ flx1, R* __restrict__ mshf, const R* __restrict__ eta3,
                               R* __restrict__ dcurl, R* __restrict__ ke) {
-  at_member(G.mb, u, h3, hc, flx1, mshf);
-  const bool dcr = dcurl != nullptr;
-  if (dcr) at_member(G.mb, eta3, dcurl);
-  if (ke) at_member(G.mb, ke);
-  const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = xcd_slot();
-  if (s >= ns) return;
-  const IJK c = ijk_of(s, iim, jjm);
-  const int i = c.i, j = c.j, k = c.k;
-  const PushW w = push_load(G.pe, c, iim, jjm, s);
-  const PushW wk = ke ? push_load(G.pf, c, iim, jjm, s) : PushW{};
-  const bool in = interior(c, iim, jjm);
-  const bool kreal = i <= iim - 1 && j <= jjm - 1;
-  // every value first, so that the loads overlap; neighbour indices are
-  // clamped into the array (values a slot's forms do not use are not used)
-  auto cl = [&](int sl) { return min(max(sl, 0), ns - 1); };
-  const int s0 = s, sn = cl(SL(i, j + 1, k)), se = cl(SL(i + 1, j, k)), sne = cl(SL(i + 1, j + 1, k));
-  const int up[3] = {cl(SL(i - 1, j, k)), cl(SL(i - 1, j - 1, k)), cl(SL(i, j - 1, k))};
-  // velocity (planes ic + 2 sp): both corners at s0, corner 1 at sn, corner
-  // 0 at se, and for the kinetic energy both at sne
-  auto U = [&](int sp, int ic, int sl) { return ldg(u + (size_t)(ic + 2 * sp) * ns, sl); };
-  const R u00 = U(0, 0, s0), u10 = U(1, 0, s0), u01 = U(0, 1, s0), u11 = U(1, 1, s0);
-  const R un0 = U(0, 1, sn), un1 = U(1, 1, sn), ue0 = U(0, 0, se), ue1 = U(1, 0, se);
-  R une[2][2] = {{R(0), R(0)}, {R(0), R(0)}};   // [ic][sp]
-  R sa[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
-  R ai = R(0);
-  if (ke) {
-#pragma unroll
-    for (int ic = 0; ic < 2; ++ic) { une[ic][0] = U(0, ic, sne); une[ic][1] = U(1, ic, sne); }
-    auto SA = [&](int kk, int ic, int sl) { return ldg(G.sub_area + (size_t)(kk + 3 * ic) * ns, sl); };
-    sa[0] = SA(0, 0, s0); sa[1] = SA(0, 1, s0); sa[2] = SA(1, 0, se);
-    sa[3] = SA(1, 1, sne); sa[4] = SA(2, 0, sne); sa[5] = SA(2, 1, sn);
-    ai = ldg(G.area_inv, s0);
-  }
-  // corner thickness: corner 1 at sn and s0, corner 0 at s0 and se
-  const R mcn = ldg(hc + (size_t)ns, sn), mc0 = ldg(hc, s0), mc1 = ldg(hc + (size_t)ns, s0), mce = ldg(hc, se);
-  const R hs = ldg(h3, s0);
-  R hup[3], es = R(0), eup[3] = {R(0), R(0), R(0)};
-#pragma unroll
-  for (int e = 0; e < 3; ++e) hup[e] = ldg(h3, up[e]);
-  if (dcr) {
-    es = eabs(G, eta3, s0);
-#pragma unroll
-    for (int e = 0; e < 3; ++e) eup[e] = eabs(G, eta3, up[e]);
-  }
-  // the metric values of the six half edges: (point, kk, ic) =
-  // (sn,1,1) (s0,0,0) (s0,2,0) (s0,0,1) (s0,2,1) (se,1,0)
-  const int hp[6] = {sn, s0, s0, s0, s0, se}, hk[6] = {1, 0, 2, 0, 2, 1}, hi[6] = {1, 0, 0, 1, 1, 0};
-  R nE[6], nN[6], tE[6], tN[6], dd[6];
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
+  if constexpr (sizeof(R) == 8) {
+    // FP64: the loads where the forms use them (measured: every load first
+    // is 40 us slower at g=9 for the FP64 planes)
+    at_member(G.mb, u, h3, hc, flx1, mshf);
+    const bool dcr = dcurl != nullptr;
+    if (dcr) at_member(G.mb, eta3, dcurl);
+    const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+    const int s = xcd_slot();
+    if (s >= ns) return;
+    const IJK c = ijk_of(s, iim, jjm);
+    const PushW w = push_load(G.pe, c, iim, jjm, s);
+    if (ke) {
+      // the kinetic energy of this (old) level's velocity, k_center_ke's
+      // arithmetic (step.F:192-203, operators.F:1316-1393), from the velocity
+      // this kernel reads anyway; the previous step left it out
+      at_member(G.mb, ke);
+      const PushW wk = push_load(G.pf, c, iim, jjm, s);
+      const int i = c.i, j = c.j, k = c.k;
+      R kv = R(0);
+      if (i <= iim - 1 && j <= jjm - 1) {
+        auto KC = [&](int ic, int sl) {
+          const R a = u[ic * ns + sl], b = u[(ic + 2) * ns + sl];
+          return R(0.5) * (a * a + b * b);
+        };
+  #define SA(kk, ic, sl) G.sub_area[((kk) + 3 * (ic)) * ns + (sl)]
+        const int se = SL(i + 1, j, k), sne = SL(i + 1, j + 1, k), sn = SL(i, j + 1, k);
+        kv = (+SA(0, 0, s) * KC(0, s) + SA(0, 1, s) * KC(1, s) + SA(1, 0, se) * KC(0, se) +
+              SA(1, 1, sne) * KC(1, sne) + SA(2, 0, sne) * KC(0, sne) + SA(2, 1, sn) * KC(1, sn)) * G.area_inv[s];
+  #undef SA
+      }
+      put1(G.pf, wk, ke, s, kv);
+    }
+    if (!interior(c, iim, jjm)) {
+      put_all<R, 3>(G.pe, w, flx1, ns, s, R(0), R(0), R(0));
+      put_all<R, 3>(G.pe, w, mshf, ns, s, R(0), R(0), R(0));
+      if (dcr) put_all<R, 3>(G.pe, w, dcurl, ns, s, R(0), R(0), R(0));
+      return;
+    }
+    const int i = c.i, j = c.j, k = c.k;
+    Half<R> hc_, hu;
     if constexpr (sizeof(R) == 4) {
-      // premultiplied n dc; the tangent turned from the normal
-      nE[q] = ldg(G.ndc + (size_t)(hk[q] + 3 * (0 + 2 * hi[q])) * ns, hp[q]);
-      nN[q] = ldg(G.ndc + (size_t)(hk[q] + 3 * (1 + 2 * hi[q])) * ns, hp[q]);
-      tE[q] = R(0); tN[q] = R(0); dd[q] = R(0);
+      hc_ = half_edges_pre<R>(iim, jjm, ns, i, j, k, G.ndc, hc, u);
+      hu = half_edges_pre<R>(iim, jjm, ns, i, j, k, G.ndc, nullptr, u);
     } else {
-      nE[q] = ldg(G.nrm + (size_t)(hk[q] + 3 * (0 + 2 * hi[q])) * ns, hp[q]);
-      nN[q] = ldg(G.nrm + (size_t)(hk[q] + 3 * (1 + 2 * hi[q])) * ns, hp[q]);
-      dd[q] = ldg(G.dc + (size_t)(hk[q] + 3 * hi[q]) * ns, hp[q]);
-      if (dcr) {
-        tE[q] = ldg(G.tng + (size_t)(hk[q] + 3 * (0 + 2 * hi[q])) * ns, hp[q]);
-        tN[q] = ldg(G.tng + (size_t)(hk[q] + 3 * (1 + 2 * hi[q])) * ns, hp[q]);
+      hc_ = half_edges<R>(iim, jjm, ns, i, j, k, G.dc, G.nrm, hc, u);
+      hu = half_edges<R>(iim, jjm, ns, i, j, k, G.dc, G.nrm, nullptr, u);
+    }
+    const int up[3] = {SL(i - 1, j, k), SL(i - 1, j - 1, k), SL(i, j - 1, k)};
+    const R hs0 = habs(G, h3[s]);
+    R vm[3], vf[3];
+  #pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      vm[e] = -hc_.r[0][e] + hc_.r[1][e];
+      const R v = hu.r[1][e] - hu.r[0][e];
+      const R mx = v > R(0) ? v : R(0), mn = v < R(0) ? v : R(0);
+      vf[e] = mx * hs0 + mn * habs(G, h3[up[e]]);
+    }
+    put_all<R, 3>(G.pe, w, mshf, ns, s, vm[0], vm[1], vm[2]);
+    put_all<R, 3>(G.pe, w, flx1, ns, s, vf[0], vf[1], vf[2]);
+    if (dcr) {
+      // dcurl: centered_divergence_curl (operators.F:2083-2186, k_vort_curl's
+      // arithmetic) from the same old-level u, corner thickness, h and eta
+      Half<R> ht;
+      if constexpr (sizeof(R) == 4) ht = half_edges_tan_pre<R>(iim, jjm, ns, i, j, k, G.ndc, hc, u);
+      else ht = half_edges<R>(iim, jjm, ns, i, j, k, G.dc, G.tng, hc, u);
+      auto PV = [&](int sl) {
+        const R hh = habs(G, h3[sl]);
+        const R wi = hh > R(0) ? R(1) / hh : R(0);
+        return eabs(G, eta3, sl) * wi;
+      };
+      const R pv0 = PV(s);
+      R vd[3];
+  #pragma unroll
+      for (int e = 0; e < 3; ++e) vd[e] = (-ht.r[0][e] + ht.r[1][e]) * (PV(up[e]) + pv0) * R(0.5);
+      put_all<R, 3>(G.pe, w, dcurl, ns, s, vd[0], vd[1], vd[2]);
+    }
+  } else {
+    at_member(G.mb, u, h3, hc, flx1, mshf);
+    const bool dcr = dcurl != nullptr;
+    if (dcr) at_member(G.mb, eta3, dcurl);
+    if (ke) at_member(G.mb, ke);
+    const int iim = G.iim, jjm = G.jjm, ns = G.ns;
+    const int s = xcd_slot();
+    if (s >= ns) return;
+    const IJK c = ijk_of(s, iim, jjm);
+    const int i = c.i, j = c.j, k = c.k;
+    const PushW w = push_load(G.pe, c, iim, jjm, s);
+    const PushW wk = ke ? push_load(G.pf, c, iim, jjm, s) : PushW{};
+    const bool in = interior(c, iim, jjm);
+    const bool kreal = i <= iim - 1 && j <= jjm - 1;
+    // every value first, so that the loads overlap; neighbour indices are
+    // clamped into the array (values a slot's forms do not use are not used)
+    auto cl = [&](int sl) { return min(max(sl, 0), ns - 1); };
+    const int s0 = s, sn = cl(SL(i, j + 1, k)), se = cl(SL(i + 1, j, k)), sne = cl(SL(i + 1, j + 1, k));
+    const int up[3] = {cl(SL(i - 1, j, k)), cl(SL(i - 1, j - 1, k)), cl(SL(i, j - 1, k))};
+    // velocity (planes ic + 2 sp): both corners at s0, corner 1 at sn, corner
+    // 0 at se, and for the kinetic energy both at sne
+    auto U = [&](int sp, int ic, int sl) { return ldg(u + (size_t)(ic + 2 * sp) * ns, sl); };
+    const R u00 = U(0, 0, s0), u10 = U(1, 0, s0), u01 = U(0, 1, s0), u11 = U(1, 1, s0);
+    const R un0 = U(0, 1, sn), un1 = U(1, 1, sn), ue0 = U(0, 0, se), ue1 = U(1, 0, se);
+    R une[2][2] = {{R(0), R(0)}, {R(0), R(0)}};   // [ic][sp]
+    R sa[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
+    R ai = R(0);
+    if (ke) {
+  #pragma unroll
+      for (int ic = 0; ic < 2; ++ic) { une[ic][0] = U(0, ic, sne); une[ic][1] = U(1, ic, sne); }
+      auto SA = [&](int kk, int ic, int sl) { return ldg(G.sub_area + (size_t)(kk + 3 * ic) * ns, sl); };
+      sa[0] = SA(0, 0, s0); sa[1] = SA(0, 1, s0); sa[2] = SA(1, 0, se);
+      sa[3] = SA(1, 1, sne); sa[4] = SA(2, 0, sne); sa[5] = SA(2, 1, sn);
+      ai = ldg(G.area_inv, s0);
+    }
+    // corner thickness: corner 1 at sn and s0, corner 0 at s0 and se
+    const R mcn = ldg(hc + (size_t)ns, sn), mc0 = ldg(hc, s0), mc1 = ldg(hc + (size_t)ns, s0), mce = ldg(hc, se);
+    const R hs = ldg(h3, s0);
+    R hup[3], es = R(0), eup[3] = {R(0), R(0), R(0)};
+  #pragma unroll
+    for (int e = 0; e < 3; ++e) hup[e] = ldg(h3, up[e]);
+    if (dcr) {
+      es = eabs(G, eta3, s0);
+  #pragma unroll
+      for (int e = 0; e < 3; ++e) eup[e] = eabs(G, eta3, up[e]);
+    }
+    // the metric values of the six half edges: (point, kk, ic) =
+    // (sn,1,1) (s0,0,0) (s0,2,0) (s0,0,1) (s0,2,1) (se,1,0)
+    const int hp[6] = {sn, s0, s0, s0, s0, se}, hk[6] = {1, 0, 2, 0, 2, 1}, hi[6] = {1, 0, 0, 1, 1, 0};
+    R nE[6], nN[6], tE[6], tN[6], dd[6];
+  #pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      if constexpr (sizeof(R) == 4) {
+        // premultiplied n dc; the tangent turned from the normal
+        nE[q] = ldg(G.ndc + (size_t)(hk[q] + 3 * (0 + 2 * hi[q])) * ns, hp[q]);
+        nN[q] = ldg(G.ndc + (size_t)(hk[q] + 3 * (1 + 2 * hi[q])) * ns, hp[q]);
+        tE[q] = R(0); tN[q] = R(0); dd[q] = R(0);
       } else {
-        tE[q] = R(0); tN[q] = R(0);
+        nE[q] = ldg(G.nrm + (size_t)(hk[q] + 3 * (0 + 2 * hi[q])) * ns, hp[q]);
+        nN[q] = ldg(G.nrm + (size_t)(hk[q] + 3 * (1 + 2 * hi[q])) * ns, hp[q]);
+        dd[q] = ldg(G.dc + (size_t)(hk[q] + 3 * hi[q]) * ns, hp[q]);
+        if (dcr) {
+          tE[q] = ldg(G.tng + (size_t)(hk[q] + 3 * (0 + 2 * hi[q])) * ns, hp[q]);
+          tN[q] = ldg(G.tng + (size_t)(hk[q] + 3 * (1 + 2 * hi[q])) * ns, hp[q]);
+        } else {
+          tE[q] = R(0); tN[q] = R(0);
+        }
       }
     }
-  }
-  // the velocity and corner thickness of each half edge
-  const R ua[6] = {un0, u00, u00, u01, u01, ue0}, ub[6] = {un1, u10, u10, u11, u11, ue1};
-  const R mcq[6] = {mcn, mc0, mc0, mc1, mc1, mce};
-  // half-edge normal fluxes with (hc) and without (hu) the corner thickness,
-  // and tangential ones with it (ht): r[0][e], r[1][e] = half edges 2e', as
-  // half_edges / half_edges_pre / half_edges_tan_pre order them
-  R hcq[6], huq[6], htq[6];
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    const R hn = ua[q] * nE[q] + ub[q] * nN[q];
-    if constexpr (sizeof(R) == 4) {
-      hcq[q] = hn * mcq[q];
-      huq[q] = hn;
-      htq[q] = (ua[q] * (-nN[q]) + ub[q] * nE[q]) * mcq[q];
-    } else {
-      hcq[q] = hn * mcq[q];
-      hcq[q] = hcq[q] * dd[q];
-      huq[q] = hn;
-      huq[q] = huq[q] * dd[q];
-      htq[q] = (ua[q] * tE[q] + ub[q] * tN[q]) * mcq[q];
-      htq[q] = htq[q] * dd[q];
+    // the velocity and corner thickness of each half edge
+    const R ua[6] = {un0, u00, u00, u01, u01, ue0}, ub[6] = {un1, u10, u10, u11, u11, ue1};
+    const R mcq[6] = {mcn, mc0, mc0, mc1, mc1, mce};
+    // half-edge normal fluxes with (hc) and without (hu) the corner thickness,
+    // and tangential ones with it (ht): r[0][e], r[1][e] = half edges 2e', as
+    // half_edges / half_edges_pre / half_edges_tan_pre order them
+    R hcq[6], huq[6], htq[6];
+  #pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const R hn = ua[q] * nE[q] + ub[q] * nN[q];
+      if constexpr (sizeof(R) == 4) {
+        hcq[q] = hn * mcq[q];
+        huq[q] = hn;
+        htq[q] = (ua[q] * (-nN[q]) + ub[q] * nE[q]) * mcq[q];
+      } else {
+        hcq[q] = hn * mcq[q];
+        hcq[q] = hcq[q] * dd[q];
+        huq[q] = hn;
+        huq[q] = huq[q] * dd[q];
+        htq[q] = (ua[q] * tE[q] + ub[q] * tN[q]) * mcq[q];
+        htq[q] = htq[q] * dd[q];
+      }
     }
-  }
-  // r[0][e] is half edge 2e (0: sn, 2: s0 k=2 ic=0, 4: s0 k=2 ic=1),
-  // r[1][e] half edge 2e+1 (1: s0 k=0 ic=0, 3: s0 k=0 ic=1, 5: se)
-  R vm[3] = {R(0), R(0), R(0)}, vf[3] = {R(0), R(0), R(0)}, vd[3] = {R(0), R(0), R(0)};
-  if (in) {
-    const R hs0 = habs(G, hs);
-#pragma unroll
-    for (int e = 0; e < 3; ++e) {
-      vm[e] = -hcq[2 * e] + hcq[2 * e + 1];
-      const R v = huq[2 * e + 1] - huq[2 * e];
-      const R mx = v > R(0) ? v : R(0), mn = v < R(0) ? v : R(0);
-      vf[e] = mx * hs0 + mn * habs(G, hup[e]);
+    // r[0][e] is half edge 2e (0: sn, 2: s0 k=2 ic=0, 4: s0 k=2 ic=1),
+    // r[1][e] half edge 2e+1 (1: s0 k=0 ic=0, 3: s0 k=0 ic=1, 5: se)
+    R vm[3] = {R(0), R(0), R(0)}, vf[3] = {R(0), R(0), R(0)}, vd[3] = {R(0), R(0), R(0)};
+    if (in) {
+      const R hs0 = habs(G, hs);
+  #pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        vm[e] = -hcq[2 * e] + hcq[2 * e + 1];
+        const R v = huq[2 * e + 1] - huq[2 * e];
+        const R mx = v > R(0) ? v : R(0), mn = v < R(0) ? v : R(0);
+        vf[e] = mx * hs0 + mn * habs(G, hup[e]);
+      }
+      if (dcr) {
+        // centered_divergence_curl (operators.F:2083-2186), k_vort_curl's PV
+        auto PV = [&](R hv, R ev) {
+          const R hh = habs(G, hv);
+          const R wi = hh > R(0) ? R(1) / hh : R(0);
+          return ev * wi;
+        };
+        const R pv0 = PV(hs, es);
+  #pragma unroll
+        for (int e = 0; e < 3; ++e) vd[e] = (-htq[2 * e] + htq[2 * e + 1]) * (PV(hup[e], eup[e]) + pv0) * R(0.5);
+      }
     }
-    if (dcr) {
-      // centered_divergence_curl (operators.F:2083-2186), k_vort_curl's PV
-      auto PV = [&](R hv, R ev) {
-        const R hh = habs(G, hv);
-        const R wi = hh > R(0) ? R(1) / hh : R(0);
-        return ev * wi;
-      };
-      const R pv0 = PV(hs, es);
-#pragma unroll
-      for (int e = 0; e < 3; ++e) vd[e] = (-htq[2 * e] + htq[2 * e + 1]) * (PV(hup[e], eup[e]) + pv0) * R(0.5);
+    if (ke) {
+      // the kinetic energy of this (old) level's velocity, k_center_ke's
+      // arithmetic (step.F:192-203, operators.F:1316-1393), from the velocity
+      // this kernel reads anyway; the previous step left it out
+      R kv = R(0);
+      if (kreal) {
+        auto KC = [&](R a, R b) { return R(0.5) * (a * a + b * b); };
+        kv = (+sa[0] * KC(u00, u10) + sa[1] * KC(u01, u11) + sa[2] * KC(ue0, ue1) + sa[3] * KC(une[1][0], une[1][1]) +
+              sa[4] * KC(une[0][0], une[0][1]) + sa[5] * KC(un0, un1)) * ai;
+      }
+      put1(G.pf, wk, ke, s, kv);
     }
+    put_all<R, 3>(G.pe, w, mshf, ns, s, vm[0], vm[1], vm[2]);
+    put_all<R, 3>(G.pe, w, flx1, ns, s, vf[0], vf[1], vf[2]);
+    if (dcr) put_all<R, 3>(G.pe, w, dcurl, ns, s, vd[0], vd[1], vd[2]);
   }
-  if (ke) {
-    // the kinetic energy of this (old) level's velocity, k_center_ke's
-    // arithmetic (step.F:192-203, operators.F:1316-1393), from the velocity
-    // this kernel reads anyway; the previous step left it out
-    R kv = R(0);
-    if (kreal) {
-      auto KC = [&](R a, R b) { return R(0.5) * (a * a + b * b); };
-      kv = (+sa[0] * KC(u00, u10) + sa[1] * KC(u01, u11) + sa[2] * KC(ue0, ue1) + sa[3] * KC(une[1][0], une[1][1]) +
-            sa[4] * KC(une[0][0], une[0][1]) + sa[5] * KC(un0, un1)) * ai;
-    }
-    put1(G.pf, wk, ke, s, kv);
-  }
-  put_all<R, 3>(G.pe, w, mshf, ns, s, vm[0], vm[1], vm[2]);
-  put_all<R, 3>(G.pe, w, flx1, ns, s, vf[0], vf[1], vf[2]);
-  if (dcr) put_all<R, 3>(G.pe, w, dcurl, ns, s, vd[0], vd[1], vd[2]);
 }
 
 // update of a cell field by the divergence of edge fluxes, pole forms
