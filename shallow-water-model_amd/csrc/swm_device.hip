@@ -828,6 +828,11 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
   float* r = C->mx_r;
   float* e = C->mx_e;
   const unsigned nb = nblk(ns);
+  // k_defect forms two slots per thread (ns is even: iim is): its launches
+  // have nbd blocks, and nbs is the block count of the partial sums of d^2
+  // that the last defect pass left
+  const unsigned nbd = nblk(ns / 2);
+  unsigned nbs = nbd;
   const size_t mb = C->mb;
   // The iterate's halos are consistent (every defect pass stores the whole
   // iterate, halos included) and the solve reads the right-hand side at real
@@ -877,12 +882,13 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
     float* e0 = last ? nullptr : first_e(v0, o);
     float* rr = last ? nullptr : r + o;
     if (nv == 2)
-      hipLaunchKernelGGL((swm::dev::k_defect<2, true>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, cur[0], e, oth[0],
+      hipLaunchKernelGGL((swm::dev::k_defect<2, true>), zgrid(C, nbd), dim3(BLK), 0, C->stream, G64, cur[0], e, oth[0],
                          rhs, rr, C->part_slots, gg, inv_area, bsum, total, fine_lwt, e0);
     else
-      hipLaunchKernelGGL((swm::dev::k_defect<1, true>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, cur[v0], e + o,
+      hipLaunchKernelGGL((swm::dev::k_defect<1, true>), zgrid(C, nbd), dim3(BLK), 0, C->stream, G64, cur[v0], e + o,
                          oth[v0], rhs + o, rr, C->part_slots, gg, inv_area, bsum ? bsum + v0 : nullptr, total,
                          fine_lwt, e0);
+    nbs = nbd;
     for (int v = v0; v < v0 + nv; ++v) {
       std::swap(cur[v], oth[v]);
       shift[v] = false;
@@ -891,12 +897,13 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
   auto redefect = [&](int nv, int v0) {
     const size_t o = (size_t)v0 * ns;
     if (nv == 2)
-      hipLaunchKernelGGL((swm::dev::k_defect<2, false>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, cur[0], e, cur[0],
+      hipLaunchKernelGGL((swm::dev::k_defect<2, false>), zgrid(C, nbd), dim3(BLK), 0, C->stream, G64, cur[0], e, cur[0],
                          rhs, r, C->part_slots, (const double*)nullptr, inv_area, bsum, total, fine_lwt, first_e(0, 0));
     else
-      hipLaunchKernelGGL((swm::dev::k_defect<1, false>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, cur[v0], e + o,
+      hipLaunchKernelGGL((swm::dev::k_defect<1, false>), zgrid(C, nbd), dim3(BLK), 0, C->stream, G64, cur[v0], e + o,
                          cur[v0], rhs + o, r + o, C->part_slots, (const double*)nullptr, inv_area,
                          bsum ? bsum + v0 : nullptr, total, fine_lwt, first_e(v0, o));
+    nbs = nbd;
   };
   // one FP32 correction cycle on fields v0.. (nv of them: both, or one);
   // the correction lands in e or, with another finest sweep count, in that
@@ -920,11 +927,15 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
                        C->xprev, blk[2], blk[1], C->part_slots, mb);
     if (late_mean)
       sum_final2(C, C->part_rhs, nb_rhs, C->rmean, 2, C->part_slots + 2 * (size_t)nb, (int)nb, C->gauge, 2);
-    hipLaunchKernelGGL((swm::dev::k_defect<2, false>), zgrid(C, nb), dim3(BLK), 0, C->stream, G64, blk[1], e, blk[1],
+    // (its d^2 sums, nbd blocks, go to columns 0 and 1 below the start's
+    // gauge sums in columns 2 and 3 of nb blocks each)
+    hipLaunchKernelGGL((swm::dev::k_defect<2, false>), zgrid(C, nbd), dim3(BLK), 0, C->stream, G64, blk[1], e, blk[1],
                        rhs, r, C->part_slots, (const double*)nullptr, inv_area, bsum, total, fine_lwt, first_e(0, 0));
+    nbs = nbd;
   } else {
     hipLaunchKernelGGL(swm::dev::k_defect_extrap<2>, zgrid(C, nb), dim3(BLK), 0, C->stream, G64, C->area_mask, blk[0],
                        extrap ? C->xprev : blk[0], blk[1], rhs, r, C->part_slots, bsum, total, fine_lwt, first_e(0, 0));
+    nbs = nb;
   }
   if (extrap) {
     if (!late_mean) sum_final(C, C->part_slots + 2 * (size_t)nb, (int)nb, C->gauge, 2);
@@ -938,7 +949,7 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
     // (the start-up cycles until the warm start has its full order)
     iter = C->nsol > C->mixed_extrapolate ? C->fixed_cycles : std::max(C->fixed_cycles, C->fixed_start);
     for (int it = 0; it < iter; ++it) cycle(2, 0, it == iter - 1);
-    sum_final(C, C->part_slots, (int)nb, C->sums, 2);
+    sum_final(C, C->part_slots, (int)nbs, C->sums, 2);
     hipLaunchKernelGGL(swm::dev::k_track_resid, zgrid(C, 1), dim3(64), 0, C->stream, C->sums, 1.0 / (double)C->max_ig,
                        C->solver_tol, C->track, mb);
   } else {
@@ -948,7 +959,7 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
     bool lazy = C->mixed_min_cycles > 0;
     const int M = C->nz;
     std::vector<double> ms(2 * (size_t)M);
-    swm_status st = member_norms(C, C->part_slots, (int)nb, {0, 1}, ms.data());
+    swm_status st = member_norms(C, C->part_slots, (int)nbs, {0, 1}, ms.data());
     if (st) return st;
     std::vector<int> bad;
     for (int v = 0; v < 2; ++v) {
@@ -971,7 +982,7 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
       if (lazy) redefect(nv, v0);
       cycle(nv, v0, true);
       lazy = true;
-      st = member_norms(C, C->part_slots, (int)nb, bad, ms.data());
+      st = member_norms(C, C->part_slots, (int)nbs, bad, ms.data());
       if (st) return st;
       any = false;
       for (int m = 0; m < M; ++m)

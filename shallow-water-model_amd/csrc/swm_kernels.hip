@@ -63,6 +63,37 @@ template <class T>
 __device__ __forceinline__ void stg(T* __restrict__ p, int i, T x) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + (uint32_t)i * (uint32_t)sizeof(T)) = x;
 }
+// two consecutive elements i, i + 1 in one 16-byte (FP64) or 8-byte (FP32)
+// access; the element index need only be even for the natural alignment
+// (odd ones give 8- or 4-byte aligned accesses, which global memory takes)
+struct __attribute__((aligned(8))) D2 {
+  double x, y;
+};
+struct __attribute__((aligned(4))) F2 {
+  float x, y;
+};
+__device__ __forceinline__ D2 ld2(const double* __restrict__ p, int i) {
+  return *reinterpret_cast<const D2*>(reinterpret_cast<const char*>(p) + (uint32_t)i * 8u);
+}
+__device__ __forceinline__ F2 ld2(const float* __restrict__ p, int i) {
+  return *reinterpret_cast<const F2*>(reinterpret_cast<const char*>(p) + (uint32_t)i * 4u);
+}
+__device__ __forceinline__ void st2(double* __restrict__ p, int i, double a, double b) {
+  *reinterpret_cast<D2*>(reinterpret_cast<char*>(p) + (uint32_t)i * 8u) = D2{a, b};
+}
+__device__ __forceinline__ void st2(float* __restrict__ p, int i, float a, float b) {
+  *reinterpret_cast<F2*>(reinterpret_cast<char*>(p) + (uint32_t)i * 4u) = F2{a, b};
+}
+// (p[i], p[i + 1]) of a plane of n elements, for -1 <= i <= n - 1: the
+// element outside [0, n) is a copy of the one inside
+template <class T>
+__device__ __forceinline__ auto ld2c(const T* __restrict__ p, int i, int n) {
+  const int b = i < 0 ? 0 : (i > n - 2 ? n - 2 : i);
+  auto t = ld2(p, b);
+  if (b > i) t.y = t.x;
+  if (b < i) t.x = t.y;
+  return t;
+}
 __device__ __forceinline__ bool interior(const IJK& c, int iim, int jjm) {
   return c.i >= 2 && c.i <= iim - 1 && c.j >= 2 && c.j <= jjm - 1;
 }
@@ -1583,14 +1614,25 @@ __device__ __forceinline__ float first_sweep(bool real, const float* __restrict_
   return zero_sweep<float>(ldg(lwt, s), d);
 }
 
-// Mixed path, FP64 defect of the Poisson problem for NV fields, one slot per
-// thread, the FP64 weights read once: d = rhs - L(x') at real cells (0
-// elsewhere), x' = x + e with CORR (the FP32 correction e, halos filled;
-// x' is stored to xn on every slot, halos included, so the iterate stays
-// halo-consistent) or x' = x.  d is stored in FP32 as the right-hand side of
-// the next correction cycle (r == nullptr: not stored, where no cycle is
-// expected to follow); per-block sums of d^2 for the convergence test
-// (elliptic_solver.F:203-239).
+// Mixed path, FP64 defect of the Poisson problem for NV fields: d = rhs -
+// L(x') at real cells (0 elsewhere), x' = x + e with CORR (the FP32
+// correction e, halos filled; x' is stored to xn on every slot, halos
+// included, so the iterate stays halo-consistent) or x' = x.  d is stored in
+// FP32 as the right-hand side of the next correction cycle (r == nullptr:
+// not stored, where no cycle is expected to follow); per-block sums of d^2
+// for the convergence test (elliptic_solver.F:203-239).
+//
+// Two slots per thread, s even and s + 1 (one row: iim is even).  Each row
+// of the stencil is read with 16-byte loads that carry the neighbours i - 1
+// and i + 2 as well (rows j - 1 and j + 1 in two loads each), so a value is
+// fetched through the vector L1 about twice instead of seven times: the
+// seven separate loads per point limited the one-slot form to 3.6 TB/s.
+// Every term is the one-slot form's, in lap_sym_apply's order, so the
+// results are the same bit for bit.  A real pole is always the even slot
+// (columns 1 and iim - 1 are odd): it takes lap_sym's pole form through the
+// same terms, with its own weights, its far neighbour for (i-1, j-1), the
+// neighbour the form leaves out read as the slot itself at weight 0, and
+// area factor 1.
 template <int NV, bool CORR>
 __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict__ x, const float* __restrict__ e,
                                                 double* __restrict__ xn, const float* __restrict__ b,
@@ -1603,52 +1645,105 @@ __global__ void __launch_bounds__(256) k_defect(Geo G, const double* __restrict_
   if (bsum) at_member(G.mb, bsum);
   if (e0) at_member(G.mb, e0);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
-  const int s = xcd_slot();
+  const int t = xcd_slot();
+  const bool act = 2 * t < ns;
+  // threads past the end work on the last pair and store nothing
+  const int s = act ? 2 * t : ns - 2;
+  const int su = s >= iim ? s - iim : s;
+  const int sd = s + iim < ns ? s + iim : s;
+  const int u1 = su + 1 < ns ? su + 1 : su, d2 = sd + 2 < ns ? sd + 2 : sd, s2 = s + 2 < ns ? s + 2 : s;
+  const double* s0p = G.sym;
+  const double* s1p = G.sym + ns;
+  const double* s2p = G.sym + 2 * (size_t)ns;
+  const IJK cA = ijk_of(s, iim, jjm);
+  const D2 ai = ld2(G.aim, s);
+  const bool npA = cA.i == 1 && cA.j == jjm - 1, spA = cA.i == iim - 1 && cA.j == 1;
+  const bool poleA = (npA || spA) && ai.x != 0.0;
+  // the weights: slot A sym(0..2) at s, sym0 at s + 1, sym1 at sd + 1,
+  // sym2 at sd; slot B the same one slot on
+  D2 S0 = ld2(s0p, s), S1 = ld2(s1p, s), S2 = ld2(s2p, s);
+  const double S0B3 = ldg(s0p, s2);
+  D2 S1d = ld2c(s1p, sd + 1, ns);
+  D2 S2d = ld2(s2p, sd);
+  const double S0B0 = S0.y;
+  double aiA = ai.x;
+  int far = s;
+  if (poleA) {
+    S0.x = npA ? 0.0 : ldg(G.lw, s);
+    S1.x = ldg(G.lw + ns, s);
+    S2.x = spA ? 0.0 : ldg(G.lw + 2 * (size_t)ns, s);
+    S0.y = ldg(G.lw + 3 * (size_t)ns, s);
+    S1d.x = ldg(G.lw + 4 * (size_t)ns, s);
+    S2d.x = ldg(G.lw + 5 * (size_t)ns, s);
+    aiA = 1.0;
+    far = npA ? SL(iim, 1, cA.k) : SL(1, jjm, cA.k);
+  }
+  double wA[NV], wB[NV], xA[NV], xB[NV];
+  F2 bq[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double* xv = x + (size_t)v * ns;
+    const float* ev = e + (size_t)v * ns;
+    // row j: (i-1, i), (i+1, i+2); row j-1: (i-1, i), i+1; row j+1: (i, i+1), i+2
+    const D2 r0 = ld2c(xv, s - 1, ns), r1 = ld2c(xv, s + 1, ns), u0 = ld2c(xv, su - 1, ns), d0 = ld2(xv, sd);
+    const double ux = ldg(xv, u1), dx = ldg(xv, d2);
+    F2 er0{0.f, 0.f}, er1{0.f, 0.f}, eu0{0.f, 0.f}, ed0{0.f, 0.f};
+    float eux = 0.f, edx = 0.f;
+    if (CORR) {
+      er0 = ld2c(ev, s - 1, ns); er1 = ld2c(ev, s + 1, ns); eu0 = ld2c(ev, su - 1, ns); ed0 = ld2(ev, sd);
+      eux = ldg(ev, u1); edx = ldg(ev, d2);
+    }
+    bq[v] = ld2(b + (size_t)v * ns, s);
+    auto X = [&](double xq, float eq) { return CORR ? xq + (double)eq : xq; };
+    const double c0 = X(r0.y, er0.y), c1 = X(r1.x, er1.x), p2 = X(r1.y, er1.y);
+    const double uc0 = X(u0.y, eu0.y), up1 = X(ux, eux);
+    const double dc0 = X(d0.x, ed0.x), dp1 = X(d0.y, ed0.y), dp2 = X(dx, edx);
+    double m1 = X(r0.x, er0.x), um1 = X(u0.x, eu0.x), uA = uc0;
+    if (poleA) {
+      um1 = X(ldg(xv, far), CORR ? ldg(ev, far) : 0.0f);
+      if (npA) m1 = c0;
+      else uA = c0;
+    }
+    // lap_sym_apply's terms in its order: i-1, (i-1, j-1), j-1, i+1,
+    // (i+1, j+1), j+1
+    double w = S0.x * (m1 - c0);
+    w = w + S1.x * (um1 - c0);
+    w = w + S2.x * (uA - c0);
+    w = w + S0.y * (c1 - c0);
+    w = w + S1d.x * (dp1 - c0);
+    w = w + S2d.x * (dc0 - c0);
+    wA[v] = w * aiA;
+    w = S0B0 * (c0 - c1);
+    w = w + S1.y * (uc0 - c1);
+    w = w + S2.y * (up1 - c1);
+    w = w + S0B3 * (p2 - c1);
+    w = w + S1d.y * (dp2 - c1);
+    w = w + S2d.y * (dp1 - c1);
+    wB[v] = w * ai.y;
+    xA[v] = c0;
+    xB[v] = c1;
+  }
+  // the first sweep's slots are k_relax's (interior and pole slots)
+  const IJK cB{cA.i + 1, cA.j, cA.k};
+  const bool rA = interior(cA, iim, jjm) || npA || spA;
+  const bool rB = interior(cB, iim, jjm);
+  const F2 wt = e0 ? ld2(lwt, s) : F2{0.0f, 0.0f};
   double acc[NV];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-  if (s < ns) {
-    const IJK c = ijk_of(s, iim, jjm);
-    const LapSym st = lap_sym(G, iim, jjm, ns, c);
-    const double m = st.real ? 1.0 : 0.0;
-    // the first sweep's slots are k_relax's (interior and pole slots)
-    const bool rslot = interior(c, iim, jjm) || (c.i == 1 && c.j == jjm - 1) || (c.i == iim - 1 && c.j == 1);
-    // every value of both fields loaded before any is used or stored, so
-    // that the loads of all seven points overlap
-    double xq[NV][7];
-    float eq[NV][7], bq[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      const double* xv = x + (size_t)v * ns;
-      const float* ev = e + (size_t)v * ns;
-#pragma unroll
-      for (int q = 0; q < 7; ++q) {
-        const int sl = q < 6 ? st.nb[q] : s;
-        xq[v][q] = ldg(xv, sl);
-        if (CORR) eq[v][q] = ldg(ev, sl);
+  for (int v = 0; v < NV; ++v) {
+    const double dA = ai.x != 0.0 ? (double)rhs_less_mean(bq[v].x, bsum, v, total) - wA[v] : 0.0;
+    const double dB = ai.y != 0.0 ? (double)rhs_less_mean(bq[v].y, bsum, v, total) - wB[v] : 0.0;
+    acc[v] = 0.0;
+    if (act) {
+      if (CORR) {
+        const double gs = gauge ? gauge[v] * inv_area : 0.0;
+        st2(xn + (size_t)v * ns, s, gauge ? xA[v] - gs : xA[v], gauge ? xB[v] - gs : xB[v]);
       }
-      bq[v] = ldg(b + (size_t)v * ns, s);
-    }
-    const float wt = e0 && rslot ? ldg(lwt, s) : 0.0f;
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      double X[7];
-#pragma unroll
-      for (int q = 0; q < 7; ++q) X[q] = CORR ? xq[v][q] + (double)eq[v][q] : xq[v][q];
-      // lap_sym_apply's terms in its order
-      double w = 0.0;
-      if (st.real) {
-        const double x0 = X[6];
-        w = st.S[0] * (X[0] - x0);
-#pragma unroll
-        for (int q = 1; q < 6; ++q) w = w + st.S[q] * (X[q] - x0);
-        w = w * st.ai;
-      }
-      const double d = m != 0.0 ? (double)rhs_less_mean(bq[v], bsum, v, total) - w : 0.0;
-      if (CORR) stg(xn + (size_t)v * ns, s, gauge ? X[6] - gauge[v] * inv_area : X[6]);
-      if (r) stg(r + (size_t)v * ns, s, (float)d);
-      if (e0) stg(e0 + (size_t)v * ns, s, rslot ? zero_sweep<float>(wt, (float)d) : 0.0f);
-      acc[v] = (d * d) * m;
+      if (r) st2(r + (size_t)v * ns, s, (float)dA, (float)dB);
+      if (e0)
+        st2(e0 + (size_t)v * ns, s, rA ? zero_sweep<float>(wt.x, (float)dA) : 0.0f,
+            rB ? zero_sweep<float>(wt.y, (float)dB) : 0.0f);
+      acc[v] = (ai.x != 0.0 ? dA * dA : 0.0) + (ai.y != 0.0 ? dB * dB : 0.0);
     }
   }
   block_sums<NV>(acc, part);
