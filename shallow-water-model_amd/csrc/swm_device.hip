@@ -132,6 +132,14 @@ struct swm_ctx {
   // the cycle's launches become latency-bound)
   std::function<void()> hook;
   int hook_level = 0;
+  // mixed step, one member, absolute criterion: the velocity kernel of the
+  // step, queued at a test of the solve before the host waits for the
+  // norms (post_solve, given the iterate's buffer); it runs only if the
+  // device's copy of the test (k_conv_go into go) passes.  post_done: the
+  // last test's launch stands (the solve ended there), so the step does not
+  // launch the kernel again
+  std::function<void(const double*)> post_solve;
+  bool post_done = false;
   Arena arena;             // shared: metrics, halo tables, multigrid weights and maps
   size_t arena_bytes = 0;
   // ensemble members (same grid and settings).  Member m's state and scratch
@@ -157,6 +165,7 @@ struct swm_ctx {
   double* fct_rec = nullptr;       // FP64 FCT step: the limiter coefficients go here (linearised model's base steps)
   double* tr_rec = nullptr;        // FP64 FCT step: the tracer limiter coefficients (3 planes per tracer, 6 planes apart)
   double* rmean = nullptr;         // mixed path: sums of area * rhs of the two fields (mean removal)
+  double* go = nullptr;            // mixed path: the device's copy of the solve's convergence test
   double* part_rhs = nullptr;      // mixed path: their per-block partial sums, from the tendencies kernel         // per member: max mean(r^2) of the two fields, count of solves above the criterion
   // FP64 metrics (both paths: set-up, FP64 residual, sums)
   double *area, *area_inv, *cai, *sub_area, *dc, *nrm, *tng, *vw, *sw, *lw, *f, *hs, *mask, *area_mask, *pxyz;
@@ -664,7 +673,10 @@ bool converged(const swm_ctx* C, int m, int v, double ms) {
 // the mean squared residuals of fields vars of every member in play (sums
 // of the per-block partials in part, nb blocks): ms[m * 2 + b] for vars[b];
 // last_resid takes the largest over the members
-swm_status member_norms(swm_ctx* C, const double* part, int nb, const std::vector<int>& vars, double* ms) {
+// before_wait (one member): work queued behind the sums, before the host
+// waits for them
+swm_status member_norms(swm_ctx* C, const double* part, int nb, const std::vector<int>& vars, double* ms,
+                        const std::function<void()>& before_wait = {}) {
   const int nv = (int)vars.size();
   std::vector<double> raw((size_t)nv * C->nz);
   if (C->nz == 1 && C->host_sums_dev) {
@@ -673,6 +685,7 @@ swm_status member_norms(swm_ctx* C, const double* part, int nb, const std::vecto
     // the next step's corners queued behind them
     sum_final(C, part, nb, C->host_sums_dev, nv);
     HIP_TRY(hipEventRecord(C->ev_norm, C->stream));
+    if (before_wait) before_wait();
     corners_for_next(C);
     HIP_TRY(hipEventSynchronize(C->ev_norm));
     for (int b = 0; b < nv; ++b) raw[b] = C->host_sums[b];
@@ -943,6 +956,7 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
   }
   int iter;
   bool any = false;
+  const double* post_at = nullptr;   // the iterate the last test's queued velocity kernel read
   if (fixed) {
     // the fixed schedule: no host-side test; the last defect's mean squares
     // go to the member's residual record on the device
@@ -959,7 +973,18 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
     bool lazy = C->mixed_min_cycles > 0;
     const int M = C->nz;
     std::vector<double> ms(2 * (size_t)M);
-    swm_status st = member_norms(C, C->part_slots, (int)nbs, {0, 1}, ms.data());
+    // the step's velocity kernel queued behind a test of both fields whose
+    // iterate is in its final buffer (field 1 next to field 0, no mean
+    // change left to take off): it runs if the device's copy of the test
+    // passes, so the GPU need not wait for the host's decision
+    auto post = [&]() {
+      if (!C->post_solve || M != 1 || C->solver_relative || cur[1] != cur[0] + ns || shift[0] || shift[1]) return;
+      hipLaunchKernelGGL(swm::dev::k_conv_go, dim3(1), dim3(64), 0, C->stream, C->host_sums_dev,
+                         (double)C->max_ig, C->solver_tol, C->go);
+      C->post_solve(cur[0]);
+      post_at = cur[0];
+    };
+    swm_status st = member_norms(C, C->part_slots, (int)nbs, {0, 1}, ms.data(), post);
     if (st) return st;
     std::vector<int> bad;
     for (int v = 0; v < 2; ++v) {
@@ -982,7 +1007,8 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
       if (lazy) redefect(nv, v0);
       cycle(nv, v0, true);
       lazy = true;
-      st = member_norms(C, C->part_slots, (int)nbs, bad, ms.data());
+      post_at = nullptr;
+      st = member_norms(C, C->part_slots, (int)nbs, bad, ms.data(), nv == 2 ? post : std::function<void()>{});
       if (st) return st;
       any = false;
       for (int m = 0; m < M; ++m)
@@ -1000,6 +1026,7 @@ swm_status elliptic_mixed(swm_ctx* C, float* rhs, double* xout, int nb_rhs, cons
   // which becomes xout; the previous solutions become xprev and xprev2
   if (cur[1] != cur[0] + ns) copy_state(C, cur[0] + ns, cur[1], (size_t)ns);
   double* fin = cur[0];
+  C->post_done = post_at != nullptr && !any && post_at == fin;
   C->xprev2 = C->xprev;
   C->xprev = blk[0];
   C->xout = fin;
@@ -1270,8 +1297,18 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   // streamfunction and velocity potential (step.F:170-184)
   swm_status st;
   const double* bsum = nullptr;
-  if constexpr (sizeof(R) == 8) st = elliptic(C, V.rhs, C->xout, 2);
-  else st = elliptic_mixed(C, V.rhs, C->xout, (int)nb, &bsum);
+  C->post_done = false;
+  if constexpr (sizeof(R) == 8) {
+    st = elliptic(C, V.rhs, C->xout, 2);
+  } else {
+    // the velocity kernel below, for the solve to queue before its tests
+    C->post_solve = [&](const double* x) {
+      hipLaunchKernelGGL(swm::dev::k_velocity<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, x, x + ns, V.u[n4], V.rhs,
+                         bsum, C->total_area, V.eta[n4], (const double*)C->go);
+    };
+    st = elliptic_mixed(C, V.rhs, C->xout, (int)nb, &bsum);
+    C->post_solve = nullptr;
+  }
   if (C->hook) {   // a solve that did not reach the hook's level
     std::function<void()> h = std::move(C->hook);
     C->hook = nullptr;
@@ -1281,12 +1318,14 @@ swm_status step_dt(swm_ctx* C, View<R>& V) {
   if (tr_st) return tr_st;
   if constexpr (mixed) {
     // the new vorticity (the right-hand side less its mean) with the velocity
-    hipLaunchKernelGGL(swm::dev::k_velocity<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, C->xout, C->xout + ns,
-                       V.u[n4], V.rhs, bsum, C->total_area, V.eta[n4]);
+    // (unless the solve's last test queued it and it stands)
+    if (!C->post_done)
+      hipLaunchKernelGGL(swm::dev::k_velocity<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, C->xout, C->xout + ns,
+                         V.u[n4], V.rhs, bsum, C->total_area, V.eta[n4], (const double*)nullptr);
   } else {
     hipLaunchKernelGGL(swm::dev::k_post_elliptic<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, V.rhs, V.eta[n4]);
     hipLaunchKernelGGL(swm::dev::k_velocity<R>, zgrid(C, nb), dim3(C->blk_step), 0, S, Gp, C->xout, C->xout + ns,
-                       V.u[n4], (const R*)nullptr, (const double*)nullptr, 0.0, (R*)nullptr);
+                       V.u[n4], (const R*)nullptr, (const double*)nullptr, 0.0, (R*)nullptr, (const double*)nullptr);
   }
   if (!push) gather<R>(C, C->vrtx, {V.u[n4], V.u[n4] + 2 * (size_t)ns});
   // the kinetic energy's halo is filled with the other fields' by
@@ -1735,6 +1774,7 @@ void layout_all(swm_ctx* C, const swm::Grid& Gh) {
   C->part_budget = mt((8 + swm::dev::BUDGET_MAXTR) * (size_t)C->npart);
   C->track = mt(4);
   C->rmean = mt(4);
+  C->go = mt(1);
   C->part_rhs = mt(2 * (size_t)nblk_b(ns, 64));
   layout_view<double>(C, C->d, nullptr);
   C->d.geo = swm::dev::Geo{C->iim, C->jjm, C->ns, C->area, C->area_inv, C->cai, C->sub_area, C->dc, C->nrm, C->tng,
@@ -1804,7 +1844,7 @@ void shift_members(swm_ctx* C, ptrdiff_t d) {
   shift_view(C->d, d);
   if (C->precision == SWM_MIXED) shift_view(C->m, d);
   for (double** p : {&C->xout, &C->r2, &C->xalt, &C->xprev, &C->xprev2, &C->gauge, &C->part_slots, &C->part, &C->sums,
-                     &C->part_budget, &C->track, &C->rmean, &C->part_rhs})
+                     &C->part_budget, &C->track, &C->rmean, &C->part_rhs, &C->go})
     shift_ptr(*p, d);
   shift_ptr(C->mx_r, d);
   shift_ptr(C->mx_e, d);
@@ -2397,7 +2437,7 @@ swm_status init_synthetic(swm_ctx* C, int seed_index, int lmax, double umax, dou
   const double rz = std::sqrt(sz / nreal), rd = std::sqrt(sd / nreal), rh = std::sqrt(sh / nreal);
   hipLaunchKernelGGL(swm::dev::k_synth_finish, dim3(nb), dim3(BLK), 0, S, C->d.geo, h0, h_rms / rh, hp, C->d.h[0],
                      0.1 * rz / rd, chi);
-  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, C->d.geo, psi, chi, C->d.u[0], (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr);
+  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, C->d.geo, psi, chi, C->d.u[0], (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr, (const double*)nullptr);
   hipLaunchKernelGGL(swm::dev::k_speedmax_partial, dim3(C->npart), dim3(BLK), 0, S, C->d.geo, C->d.u[0], C->part);
   std::vector<double> pm(C->npart);
   HIP_TRY(hipMemcpyAsync(pm.data(), C->part, C->npart * sizeof(double), hipMemcpyDeviceToHost, S));
@@ -3121,7 +3161,7 @@ swm_status swm_op_calculate_velocity(swm_ctx* C, int npad, int nx, int ny, int n
   G.vw = io.put(v_weights, 12, 12);
   double *p = io.put(psi, 1, 1), *c = io.put(chi, 1, 1), *y = io.dev(4);
   OP_BUF(G.cai && G.vw && p && c && y);
-  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb_), dim3(BLK), 0, C->stream, G, p, c, y, (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr);
+  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb_), dim3(BLK), 0, C->stream, G, p, c, y, (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr, (const double*)nullptr);
   return io.get(xout, y, 4, kCornerVec);
 }
 
@@ -4148,7 +4188,7 @@ swm_status tl_step(swm_ctx* C, int q) {
     if (st) return st;
   }
   hipLaunchKernelGGL(swm::dev::k_post_elliptic<double>, dim3(nb), dim3(BLK), 0, S, T->lin, T->rhs, T->eta[n4]);
-  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, G, T->xs, T->xs + ns, T->u[n4], (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr);
+  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, G, T->xs, T->xs + ns, T->u[n4], (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr, (const double*)nullptr);
   gather<double>(C, C->vrtx, {T->u[n4], T->u[n4] + 2 * ns});
   // wrap_prognostics
   gather<double>(C, C->face, {T->h[n4], T->eta[n4], T->div[n4]});
@@ -4296,7 +4336,7 @@ swm_status init_tl(swm_ctx* C, const double* dz, const double* dd, const double*
   swm_status st = lin_solve(C, T->rhs, T->xs);
   if (st) return st;
   clear_state(C);
-  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, G, T->xs, T->xs + ns, T->u[0], (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr);
+  hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nb), dim3(BLK), 0, S, G, T->xs, T->xs + ns, T->u[0], (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr, (const double*)nullptr);
   HIP_TRY(hipMemcpyAsync(T->h[0], dh, ns * sizeof(double), hipMemcpyDeviceToDevice, S));
   // finish_init: both levels, wrapped; curl / div of level 0 (wrapped again)
   HIP_TRY(hipMemcpyAsync(T->h[1], T->h[0], ns * sizeof(double), hipMemcpyDeviceToDevice, S));
@@ -4389,7 +4429,7 @@ swm_status init_from_vorticity(swm_ctx* C, const double* z, const double* d, con
   swm_status st = lin_solve(C, T->rhs, T->xs);
   if (st) return st;
   hipLaunchKernelGGL(swm::dev::k_velocity<double>, dim3(nblk(ns)), dim3(BLK), 0, S, C->d.geo, T->xs, T->xs + ns,
-                     C->d.u[0], (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr);
+                     C->d.u[0], (const double*)nullptr, (const double*)nullptr, 0.0, (double*)nullptr, (const double*)nullptr);
   HIP_TRY(hipMemcpyAsync(C->d.h[0], h, ns * sizeof(double), hipMemcpyDeviceToDevice, S));
   // the model's first solve starts from this streamfunction / velocity potential
   HIP_TRY(hipMemcpyAsync(C->xout, T->xs, 2 * ns * sizeof(double), hipMemcpyDeviceToDevice, S));

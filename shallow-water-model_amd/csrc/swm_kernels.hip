@@ -1822,6 +1822,15 @@ __global__ void __launch_bounds__(256) k_extrap_start(int ns, const double* __re
   block_sums<4>(acc, part);
 }
 
+// the convergence test of elliptic_solver.F:203-239 (absolute form) on the
+// device, for a velocity kernel queued before the host has read the norms:
+// go = 1 when both fields' mean squared residuals, sums / n, meet tol (the
+// host's test, same division and comparison, so the same decision; a NaN
+// fails it), else 0.  sums: the two final sums (one member).
+__global__ void k_conv_go(const double* __restrict__ sums, double n, double tol, double* __restrict__ go) {
+  if (threadIdx.x == 0) go[0] = (sums[0] / n <= tol && sums[1] / n <= tol) ? 1.0 : 0.0;
+}
+
 // fixed-schedule solves: per member (blockIdx.z), the largest mean squared
 // residual of the two fields so far (track[0], track[1]) and the number of
 // solves that ended above tol (track[2]); a NaN counts as above
@@ -2196,10 +2205,17 @@ __global__ void k_copy(double* __restrict__ y, const double* __restrict__ x, siz
 // the FP64 streamfunction and velocity potential.  Mixed path: the gradient
 // is formed on differences from the corner's first cell (its weights sum to
 // zero), in FP64 from the FP32 weights.
+//
+// go != nullptr: launched before the host has tested the solve; it runs only
+// where k_conv_go found the solve converged (go[0] == 1), else exits.
 template <class R>
 __global__ void k_velocity(GeoT<R> G, const double* __restrict__ psi, const double* __restrict__ chi,
                            R* __restrict__ u, const R* __restrict__ rhs1, const double* __restrict__ bsum,
-                           double total, R* __restrict__ eta4) {
+                           double total, R* __restrict__ eta4, const double* __restrict__ go) {
+  if (go) {
+    at_member(G.mb, go);
+    if (go[0] == 0.0) return;
+  }
   at_member(G.mb, psi, chi, u);
   const int iim = G.iim, jjm = G.jjm, ns = G.ns;
   const int s = xcd_slot();

@@ -11,6 +11,10 @@ change how the mixed solve converges:
     same fields, halo slots included, bit for bit as the step's halo fills,
     FP64 and mixed, one and two tracers, with and without the tracer
     stream, for every member of a two-member context;
+  - the velocity kernel that a one-member mixed step queues at each test
+    of its solve (run only where the device's copy of the test passes)
+    gives the same fields, and the same cycles, as a two-member context
+    with both members in the same state, which launches it after the test;
   - the mixed solve's coarse correction (swm_set_mixed_coarse): the dense
     solves of level 4 and of level 3 and the reference's V-cycle below level
     4 all meet the criterion, with errors against the FP64 form within 3x
@@ -95,6 +99,31 @@ def test_halo_push_gives_the_same_fields(precision, ntracer, overlap):
         b.select(z)
         for n in FIELDS + ("ke", "relative"):
             assert np.array_equal(a.field(n), b.field(n)), (z, n)
+
+
+@pytest.mark.parametrize("sweeps", [2, 1])
+def test_early_velocity_gives_the_same_fields(sweeps):
+    # one member: the step's velocity kernel is queued at each test of the
+    # solve and runs only where the device's copy of the test passes; two
+    # members (no early launch) with identical states take the same cycles
+    print(f"case early velocity sweeps={sweeps}", flush=True)
+    grid = S.Grid(6)
+    dt = 200.0 * 2 ** (5 - grid.g)
+    ms = []
+    for members in (1, 2):
+        m = S.Model(grid, ntracer=1, dt=dt, grav=9.81, precision=S.SWM_MIXED, members=members)
+        m.set_mixed_smoother(sweeps)
+        for z in range(members):
+            m.select(z).init_synthetic_member(7, lmax=12, seed_index=2)
+        m.step(8)
+        ms.append(m)
+    one, two = ms
+    cycles = one.solver_stats()
+    assert cycles == two.solver_stats(), (cycles, two.solver_stats())
+    for z in range(2):
+        two.select(z)
+        for n in FIELDS + ("ke", "relative"):
+            assert np.array_equal(one.field(n), two.field(n)), (z, n)
 
 
 def test_mixed_coarse_solve_meets_the_criterion():
