@@ -4437,9 +4437,13 @@ swm_status init_from_vorticity(swm_ctx* C, const double* z, const double* d, con
 }
 
 // host (reference layout) <-> device planes of the TL / AD state
+// The copies in are synchronous host copies, which do not wait for work on
+// the context's (non-blocking) stream: the stream is drained first, so that
+// clear_state's fills, queued just before, cannot land on the inputs.
 swm_status state_in(swm_ctx* C, int lev, const double* h, const double* e, const double* d, const double* u) {
   TlAd* T = C->tl;
   const size_t ns = C->ns;
+  HIP_TRY(hipStreamSynchronize(C->stream));
   std::vector<double> up(4 * ns);
   u_to_planes(u, up.data(), (int)ns);
   HIP_TRY(hipMemcpy(T->h[lev], h, ns * sizeof(double), hipMemcpyHostToDevice));
@@ -4474,9 +4478,10 @@ void tracer_from_planes(const swm_ctx* C, const double* planes, double* ref) {
     for (int k = 0; k < 10; ++k)
       std::memcpy(ref + ((size_t)q + (size_t)C->nt * k) * pan, planes + q * ns + k * pan, pan * sizeof(double));
 }
-swm_status tracer_in(swm_ctx* C, int lev, const double* tr) {
+swm_status tracer_in(swm_ctx* C, int lev, const double* tr) {   // (drains the stream as state_in)
   TlAd* T = C->tl;
   const size_t ns = C->ns;
+  HIP_TRY(hipStreamSynchronize(C->stream));
   std::vector<double> tp((size_t)C->nt * ns);
   tracer_to_planes(C, tr, tp.data());
   for (int tq = 0; tq < C->nt; ++tq)

@@ -10,6 +10,8 @@ swm_set_linearisation); the forward here solves to a relative residual of
 checked by a dot-product test of its own.
 
 Checks, as the linearised model's own references:
+  - repeated runs on the same input agree: TL bit for bit, AD to 1e-12 of
+    its largest value (its atomic adds change order);
   - dot-product tests <TL x, y> = <x, AD y> over every slot of every state
     field (h, eta, div, u), for the window alone, for the initial state map
     alone (vorticity, divergence, thickness -> state) and for both, to
@@ -94,6 +96,27 @@ def test_dot_product_window(window):
             aj = float(np.sum(tk[j] * y[j]))
             bj = float(np.sum(x[k] * m.adj_run(*yj)[k]))
             assert abs(aj - bj) <= 1e-10 * max(abs(aj), 1e-300), (k, j, aj, bj)
+
+
+def test_repeated_runs_agree(window):
+    # every run clears the linearised state on the device before it copies
+    # its inputs in; the same input gives the same TL output bit for bit, and
+    # the same AD output to round-off (its kernels scatter with atomic adds,
+    # whose order varies)
+    grid, m, _ = window
+    print(f"case TL/AD repeated runs, {m.scheme}", flush=True)
+    rng = np.random.default_rng(5)
+    x, y = _state_vectors(grid, rng)
+    for k in range(4):
+        xk = [v if i == k else np.zeros_like(v) for i, v in enumerate(x)]
+        yk = [v if i == k else np.zeros_like(v) for i, v in enumerate(y)]
+        t0, a0 = m.tl_run(*xk), m.adj_run(*yk)
+        for _ in range(3):
+            t1, a1 = m.tl_run(*xk), m.adj_run(*yk)
+            for j in range(4):
+                assert np.array_equal(t0[j], t1[j]), (k, j, "tl")
+                scale = max(float(np.max(np.abs(a0[j]))), 1e-300)
+                assert float(np.max(np.abs(a1[j] - a0[j]))) <= 1e-12 * scale, (k, j, "adj")
 
 
 def test_dot_product_initial_state_map(window):
